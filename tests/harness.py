@@ -1,0 +1,223 @@
+"""ctypes harness over the oracle (oracle/liboracle.so) and the reference build
+(oracle/_ref/libaprl_ref.so). Both export the same entry points with the
+prefixes ``orc_`` and ``ref_``; :class:`CpuFields` drives either one.
+
+This module is test infrastructure: the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+REF_LIB = os.path.join(ORACLE_DIR, "_ref", "libaprl_ref.so")
+REFERENCE_TREE = "/root/reference"
+
+C_SI = 299792458.0
+EPS0 = 8.8541878128e-12
+MU0 = 1.25663706212e-6
+
+_dp = C.POINTER(C.c_double)
+_libs: dict[str, C.CDLL] = {}
+
+
+def build_oracle() -> None:
+    """Compile liboracle.so (and _ref when the reference tree is present)."""
+    targets = ["liboracle.so"]
+    if os.path.isdir(REFERENCE_TREE):
+        targets.append("ref")
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR, *targets], check=True)
+
+
+def have_ref() -> bool:
+    return os.path.exists(REF_LIB)
+
+
+def load(prefix: str) -> C.CDLL:
+    if prefix in _libs:
+        return _libs[prefix]
+    path = ORACLE_LIB if prefix == "orc" else REF_LIB
+    if not os.path.exists(path):
+        build_oracle()
+    lib = C.CDLL(path)
+    P = prefix + "_"
+    sz = C.c_size_t
+    getattr(lib, P + "last_error").restype = C.c_char_p
+    getattr(lib, P + "cowan").argtypes = [C.c_double] * 3 + [_dp]
+    getattr(lib, P + "fields_new").argtypes = [sz, sz, sz, C.c_double, C.c_double, C.c_double, C.c_double,
+                                               C.c_int, C.POINTER(C.c_void_p)]
+    getattr(lib, P + "fields_free").argtypes = [C.c_void_p]
+    getattr(lib, P + "fields_free").restype = None
+    getattr(lib, P + "fields_component").argtypes = [C.c_void_p, C.c_int, C.POINTER(sz)]
+    getattr(lib, P + "fields_component").restype = _dp
+    getattr(lib, P + "fields_history").argtypes = [C.c_void_p, C.c_int, C.POINTER(sz)]
+    getattr(lib, P + "fields_history").restype = _dp
+    getattr(lib, P + "fields_install_cpml").argtypes = [C.c_void_p, C.POINTER(sz), C.c_double, C.c_double,
+                                                        C.c_double, C.c_double, sz, sz, C.c_int, C.c_int]
+    getattr(lib, P + "fields_install_window").argtypes = [C.c_void_p, C.POINTER(sz), C.c_double, C.c_double,
+                                                          C.c_double, C.c_double]
+    getattr(lib, P + "fields_advance_magnetic").argtypes = [C.c_void_p, C.POINTER(_dp)]
+    for name in ("fields_advance_electric", "fields_clear_current", "window_advance_magnetic",
+                 "window_undo_preview", "window_commit"):
+        getattr(lib, P + name).argtypes = [C.c_void_p]
+    getattr(lib, P + "window_preview").argtypes = [C.c_void_p, C.c_int]
+    getattr(lib, P + "window_begin").argtypes = [C.c_void_p, sz, C.c_ssize_t, C.POINTER(_dp)]
+    getattr(lib, P + "window_set_inlet_at").argtypes = [C.c_void_p, C.c_ssize_t, sz, _dp, _dp]
+    getattr(lib, P + "window_shift").argtypes = [C.c_void_p, sz, C.c_ssize_t, C.POINTER(_dp)]
+    for name in ("fields_energy_density",):
+        getattr(lib, P + name).argtypes = [C.c_void_p, sz, sz, sz]
+        getattr(lib, P + name).restype = C.c_double
+    getattr(lib, P + "fields_poynting").argtypes = [C.c_void_p, sz, sz, sz, C.c_uint]
+    getattr(lib, P + "fields_poynting").restype = C.c_double
+    _libs[prefix] = lib
+    return lib
+
+
+class CpuError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+def component_shape(nx, ny, nz, a):
+    """(nz, ny, nx) numpy shape of Yee component a (0..8: E, B, J)."""
+    electric = a < 3 or a >= 6
+    axis = a % 3
+    cells = [nx, ny, nz]
+    s = [cells[d] + ((d != axis) if electric else (d == axis)) for d in range(3)]
+    return (s[2], s[1], s[0])
+
+
+def _ptr(arr: np.ndarray):
+    assert arr.dtype == np.float64 and arr.flags["C_CONTIGUOUS"]
+    return arr.ctypes.data_as(_dp)
+
+
+def _ptr_array(arrays):
+    keep = [np.ascontiguousarray(a, dtype=np.float64) for a in arrays]
+    arr = (_dp * len(keep))(*[_ptr(a) for a in keep])
+    return arr, keep
+
+
+class CpuFields:
+    """One field grid in the oracle (``prefix='orc'``) or reference build."""
+
+    def __init__(self, prefix, nx, ny, nz, dx, dy, dz, dt, solver=1):
+        self.lib = load(prefix)
+        self.P = prefix + "_"
+        self.nx, self.ny, self.nz = nx, ny, nz
+        h = C.c_void_p()
+        self._check(self._fn("fields_new")(nx, ny, nz, dx, dy, dz, dt, solver, C.byref(h)))
+        self.h = h
+
+    def _fn(self, name):
+        return getattr(self.lib, self.P + name)
+
+    def _check(self, code):
+        if code:
+            raise CpuError(code, self._fn("last_error")().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._fn("fields_free")(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def view(self, a) -> np.ndarray:
+        n = C.c_size_t()
+        p = self._fn("fields_component")(self.h, a, C.byref(n))
+        arr = np.ctypeslib.as_array(p, shape=(n.value,))
+        return arr.reshape(component_shape(self.nx, self.ny, self.nz, a))
+
+    def history(self, index) -> np.ndarray:
+        n = C.c_size_t()
+        p = self._fn("fields_history")(self.h, index, C.byref(n))
+        if n.value == 0:
+            return np.zeros(0)
+        return np.ctypeslib.as_array(p, shape=(n.value,))
+
+    def install_cpml(self, cells, order=3.0, R=1e-8, kappa=8.0, alpha=0.0, global_nz=None, zoff=0,
+                     lower_z=True, upper_z=True):
+        c = (C.c_size_t * 3)(*cells)
+        self._check(self._fn("fields_install_cpml")(self.h, c, order, R, kappa, alpha,
+                                                    global_nz or self.nz, zoff, int(lower_z), int(upper_z)))
+
+    def install_window(self, cells, order=3.0, R=1e-8, kappa=8.0, alpha=0.0):
+        c = (C.c_size_t * 3)(*cells)
+        self._check(self._fn("fields_install_window")(self.h, c, order, R, kappa, alpha))
+
+    def advance_magnetic(self, halo=None):
+        if halo is None:
+            self._check(self._fn("fields_advance_magnetic")(self.h, None))
+        else:
+            planes = [np.zeros(1) if p is None else p for p in halo]
+            arr, keep = _ptr_array(planes)
+            for i, p in enumerate(halo):
+                if p is None:
+                    arr[i] = _dp()
+            self._check(self._fn("fields_advance_magnetic")(self.h, arr))
+
+    def advance_electric(self):
+        self._check(self._fn("fields_advance_electric")(self.h))
+
+    def clear_current(self):
+        self._check(self._fn("fields_clear_current")(self.h))
+
+    # Moving-window protocol.
+    def window_begin(self, step, offset, lower3):
+        arr, keep = _ptr_array(lower3)
+        self._check(self._fn("window_begin")(self.h, step, offset, arr))
+
+    def window_advance_magnetic(self):
+        self._check(self._fn("window_advance_magnetic")(self.h))
+
+    def window_set_inlet(self, offset, step1, ex, ey):
+        ex = np.ascontiguousarray(ex, dtype=np.float64)
+        ey = np.ascontiguousarray(ey, dtype=np.float64)
+        self._check(self._fn("window_set_inlet_at")(self.h, offset, step1, _ptr(ex), _ptr(ey)))
+
+    def window_shift(self, step, offset, planes6):
+        arr, keep = _ptr_array(planes6)
+        self._check(self._fn("window_shift")(self.h, step, offset, arr))
+
+    def window_preview(self, enabled):
+        self._check(self._fn("window_preview")(self.h, int(enabled)))
+
+    def window_undo_preview(self):
+        self._check(self._fn("window_undo_preview")(self.h))
+
+    def window_commit(self):
+        self._check(self._fn("window_commit")(self.h))
+
+    def energy_density(self, i, j, k):
+        return self._fn("fields_energy_density")(self.h, i, j, k)
+
+    def poynting(self, i, j, k, axis):
+        return self._fn("fields_poynting")(self.h, i, j, k, axis)
+
+
+def cowan(prefix, dx, dy, dz):
+    lib = load(prefix)
+    out = np.zeros(12)
+    code = getattr(lib, prefix + "_cowan")(dx, dy, dz, _ptr(out))
+    if code:
+        raise CpuError(code, getattr(lib, prefix + "_last_error")().decode())
+    return out
+
+
+def seeded_fields(fields, seed, components=range(6), scale=1.0):
+    """Deterministic smooth+noisy fill (threaded_fields.cpp:29-41 style)."""
+    rng = np.random.default_rng(seed)
+    for a in components:
+        v = fields.view(a)
+        n = np.arange(v.size, dtype=np.float64).reshape(v.shape)
+        v[...] = scale * (np.sin(0.245765 * n + 0.37 * a) + 0.25 * rng.standard_normal(v.shape))
+        if a >= 3 and a < 6:
+            v[...] /= C_SI
