@@ -1,0 +1,127 @@
+"""ctypes binding of ``libaprl_gpu.so`` (the C-ABI in include/aprl_gpu.h).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback anywhere in this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libaprl_gpu.so")
+
+_dp = C.POINTER(C.c_double)
+_sz = C.c_size_t
+
+
+class GridGeometry(C.Structure):
+    """aprl_grid_geometry (EBGridGeometry, eb_field.h:54-70)."""
+    _fields_ = [("nx", _sz), ("ny", _sz), ("nz", _sz), ("dx", C.c_double), ("dy", C.c_double),
+                ("dz", C.c_double), ("dt", C.c_double), ("solver", C.c_int)]
+
+
+class CpmlParameters(C.Structure):
+    """aprl_cpml_parameters (EBCPMLParameters, eb_cpml.h:12-24)."""
+    _fields_ = [("cells", _sz * 3), ("polynomial_order", C.c_double), ("target_reflection", C.c_double),
+                ("kappa_max", C.c_double), ("alpha_fraction", C.c_double)]
+
+
+class AprlError(RuntimeError):
+    """A failed C-ABI call; ``status`` is the aprl_status code."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(message)
+        self.status = status
+
+
+class InvalidArgument(AprlError, ValueError):
+    pass
+
+
+class OutOfRange(AprlError, IndexError):
+    pass
+
+
+class LogicError(AprlError):
+    pass
+
+
+class DeviceError(AprlError):
+    pass
+
+
+_STATUS = {1: InvalidArgument, 2: OutOfRange, 3: AprlError, 4: LogicError, 5: DeviceError}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m aprl_amd.build` "
+                          "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.aprl_last_error.restype = C.c_char_p
+    L.aprl_fields_create.argtypes = [C.POINTER(GridGeometry), C.c_int, C.POINTER(vp)]
+    L.aprl_fields_destroy.argtypes = [vp]
+    L.aprl_fields_destroy.restype = None
+    L.aprl_fields_upload.argtypes = [vp, C.c_int, _dp, _sz]
+    L.aprl_fields_download.argtypes = [vp, C.c_int, _dp, _sz]
+    L.aprl_fields_component_size.argtypes = [vp, C.c_int]
+    L.aprl_fields_component_size.restype = _sz
+    L.aprl_fields_set_cpml.argtypes = [vp, C.POINTER(CpmlParameters), _sz, _sz, C.c_int, C.c_int]
+    L.aprl_fields_history_size.argtypes = [vp, _sz]
+    L.aprl_fields_history_size.restype = _sz
+    L.aprl_fields_history_download.argtypes = [vp, _sz, _dp, _sz]
+    L.aprl_fields_history_upload.argtypes = [vp, _sz, _dp, _sz]
+    for name in ("aprl_fields_clear_current", "aprl_fields_advance_electric", "aprl_fields_synchronize",
+                 "aprl_profile_reset"):
+        getattr(L, name).argtypes = [vp]
+    L.aprl_fields_advance_magnetic.argtypes = [vp, C.POINTER(_dp)]
+    L.aprl_window_install.argtypes = [vp, C.POINTER(CpmlParameters)]
+    L.aprl_window_begin_step.argtypes = [vp, _sz, C.c_ssize_t, C.POINTER(_dp)]
+    L.aprl_window_set_inlet.argtypes = [vp, _dp, _dp]
+    L.aprl_window_shift.argtypes = [vp, _sz, C.c_ssize_t, C.POINTER(_dp)]
+    L.aprl_profile_enable.argtypes = [vp, C.c_int]
+    L.aprl_profile_query.argtypes = [vp, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]
+    _lib = L
+    return L
+
+
+def check(status: int) -> None:
+    if status:
+        msg = lib().aprl_last_error().decode()
+        raise _STATUS.get(status, AprlError)(status, msg)
+
+
+def dptr(a: np.ndarray):
+    if a is None:
+        return None
+    if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError("expected a C-contiguous float64 array")
+    return a.ctypes.data_as(_dp)
+
+
+def plane_array(planes):
+    """(double*)[] from a list of arrays/None; returns (array, keepalive)."""
+    if planes is None:
+        return None, None
+    keep = [None if p is None else np.ascontiguousarray(p, dtype=np.float64) for p in planes]
+    arr = (_dp * len(keep))(*[dptr(p) if p is not None else _dp() for p in keep])
+    return arr, keep
+
+
+EXPORTED = [
+    "aprl_last_error", "aprl_fields_create", "aprl_fields_destroy", "aprl_fields_upload", "aprl_fields_download",
+    "aprl_fields_component_size", "aprl_fields_set_cpml", "aprl_fields_history_size",
+    "aprl_fields_history_download", "aprl_fields_history_upload", "aprl_fields_clear_current",
+    "aprl_fields_advance_magnetic", "aprl_fields_advance_electric", "aprl_fields_synchronize",
+    "aprl_window_install", "aprl_window_begin_step", "aprl_window_set_inlet", "aprl_window_shift",
+    "aprl_profile_enable", "aprl_profile_query", "aprl_profile_reset",
+]

@@ -1,0 +1,165 @@
+// Shared host/device utilities of the gfx950 library: status mapping,
+// device buffers, per-kernel event timing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "aprl_gpu.h"
+
+namespace aprl_gpu {
+
+// SI constants of the reference (eb_field.cpp:15-19).
+constexpr double kC = 299792458.0;
+constexpr double kEps0 = 8.8541878128e-12;
+constexpr double kMu0 = 1.25663706212e-6;
+constexpr double kQe = 1.602176634e-19;
+constexpr double kMe = 9.1093837139e-31;
+
+struct DeviceError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void setLastError(const std::string& message);
+
+inline void hipCheck(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Run f and map the reference's exception types onto aprl_status.
+template <class F>
+aprl_status guarded(F&& f) {
+  try {
+    f();
+    return APRL_OK;
+  } catch (const DeviceError& e) {
+    setLastError(e.what());
+    return APRL_DEVICE_ERROR;
+  } catch (const std::invalid_argument& e) {
+    setLastError(e.what());
+    return APRL_INVALID_ARGUMENT;
+  } catch (const std::out_of_range& e) {
+    setLastError(e.what());
+    return APRL_OUT_OF_RANGE;
+  } catch (const std::logic_error& e) {
+    setLastError(e.what());
+    return APRL_LOGIC_ERROR;
+  } catch (const std::runtime_error& e) {
+    setLastError(e.what());
+    return APRL_RUNTIME_ERROR;
+  } catch (const std::exception& e) {
+    setLastError(e.what());
+    return APRL_RUNTIME_ERROR;
+  }
+}
+
+// Owning device allocation.
+template <class T>
+class DeviceArray {
+ public:
+  DeviceArray() = default;
+  explicit DeviceArray(std::size_t n) { resize(n); }
+  DeviceArray(const DeviceArray&) = delete;
+  DeviceArray& operator=(const DeviceArray&) = delete;
+  DeviceArray(DeviceArray&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  DeviceArray& operator=(DeviceArray&& o) noexcept {
+    if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+    return *this;
+  }
+  ~DeviceArray() { release(); }
+  void resize(std::size_t n) {
+    release();
+    if (n) hipCheck(hipMalloc(reinterpret_cast<void**>(&p_), n * sizeof(T)), "hipMalloc");
+    n_ = n;
+  }
+  void release() {
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  T* get() const { return p_; }
+  std::size_t size() const { return n_; }
+  void swap(DeviceArray& o) noexcept { std::swap(p_, o.p_); std::swap(n_, o.n_); }
+
+ private:
+  T* p_ = nullptr;
+  std::size_t n_ = 0;
+};
+
+// Per-kernel HIP-event timing on one stream. Events are recorded around each
+// launch only while enabled; durations are collected lazily on query.
+class Profiler {
+ public:
+  bool enabled = false;
+  struct Pending { std::string name; hipEvent_t a, b; };
+  struct Stat { uint64_t launches = 0; double ms = 0.0; };
+
+  void begin(hipStream_t s, const char* name) {
+    if (!enabled) return;
+    Pending p;
+    p.name = name;
+    hipCheck(hipEventCreate(&p.a), "hipEventCreate");
+    hipCheck(hipEventCreate(&p.b), "hipEventCreate");
+    hipCheck(hipEventRecord(p.a, s), "hipEventRecord");
+    open_.push_back(p);
+  }
+  void end(hipStream_t s) {
+    if (!enabled) return;
+    Pending& p = open_.back();
+    hipCheck(hipEventRecord(p.b, s), "hipEventRecord");
+    done_.push_back(p);
+    open_.pop_back();
+  }
+  void collect() {
+    for (auto& p : done_) {
+      hipCheck(hipEventSynchronize(p.b), "hipEventSynchronize");
+      float ms = 0.f;
+      hipCheck(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
+      Stat& s = stats_[p.name];
+      ++s.launches;
+      s.ms += ms;
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    done_.clear();
+  }
+  Stat query(const std::string& name) {
+    collect();
+    auto it = stats_.find(name);
+    return it == stats_.end() ? Stat() : it->second;
+  }
+  void reset() {
+    collect();
+    stats_.clear();
+  }
+  ~Profiler() {
+    for (auto& p : done_) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+  }
+
+ private:
+  std::vector<Pending> open_, done_;
+  std::map<std::string, Stat> stats_;
+};
+
+struct ScopedKernel {
+  Profiler& p;
+  hipStream_t s;
+  ScopedKernel(Profiler& prof, hipStream_t stream, const char* name) : p(prof), s(stream) { p.begin(s, name); }
+  ~ScopedKernel() noexcept(false) { p.end(s); }
+};
+
+inline void launchCheck(const char* name) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw DeviceError(std::string("launch of ") + name + ": " + hipGetErrorString(e));
+}
+
+inline unsigned ceilDiv(std::size_t a, std::size_t b) { return static_cast<unsigned>((a + b - 1) / b); }
+
+}  // namespace aprl_gpu

@@ -1,0 +1,613 @@
+// gfx950 field kernels: fused Cowan-z (or Yee) magnetic curl + CFS-CPML-B,
+// fused electric curl - dt/eps0 J + CFS-CPML-E + PEC + window inlet, and the
+// moving-window shift as a base-pointer slide.
+//
+// Arithmetic follows the reference's association order with contraction
+// disabled (-ffp-contract=off), so results are bitwise those of
+// eb_field.cpp / eb_cpml.cpp / moving_window_fields.cpp on the same inputs.
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+#include "fields.hpp"
+
+namespace aprl_gpu {
+
+// ---------------------------------------------------------------------------
+// Host-side set-up (restated from the reference's constructors).
+
+// EBCowanCoefficients::forZDispersion, eb_field.cpp:33-73.
+CowanCoef cowanForZDispersion(double dx, double dy, double dz) {
+  if (!(dx > 0.0) || !(dy > 0.0) || !(dz > 0.0) || !std::isfinite(dx) || !std::isfinite(dy) ||
+      !std::isfinite(dz))
+    throw std::invalid_argument("Cowan-z grid spacing must be positive and finite");
+  const double eps = std::numeric_limits<double>::epsilon();
+  const double tolerance = 64.0 * eps * std::max(dx, std::max(dy, dz));
+  if (dx + tolerance < dz || dy + tolerance < dz)
+    throw std::invalid_argument("Cowan-z requires dz to be the smallest grid spacing");
+  CowanCoef r{};
+  r.ratio[0] = (dz / dx) * (dz / dx);
+  r.ratio[1] = (dz / dy) * (dz / dy);
+  r.ratio[2] = 1.0;
+  const double rx = r.ratio[0], ry = r.ratio[1], rz = r.ratio[2];
+  const double den = rx * ry + ry * rz + rz * rx;
+  if (!(den > 0.0) || !std::isfinite(den)) throw std::invalid_argument("Invalid Cowan-z aspect ratios");
+  const double pf = rx * ry * rz / den;
+  r.beta[0] = rx * (1.0 - pf) / 8.0;
+  r.beta[1] = ry * (1.0 - pf) / 8.0;
+  r.beta[2] = rz * (1.0 - pf) / 8.0;
+  r.deltaXY = rx * ry * (1.0 / 16.0 - rx * ry / (8.0 * den));
+  r.deltaYZ = ry * rz * (1.0 / 16.0 - ry * rz / (8.0 * den));
+  r.deltaZX = rz * rx * (1.0 / 16.0 - rz * rx / (8.0 * den));
+  r.alpha[0] = 1.0 - 2.0 * r.beta[1] - 2.0 * r.beta[2] - 4.0 * r.deltaYZ;
+  r.alpha[1] = 1.0 - 2.0 * r.beta[2] - 2.0 * r.beta[0] - 4.0 * r.deltaZX;
+  r.alpha[2] = 1.0 - 2.0 * r.beta[0] - 2.0 * r.beta[1] - 4.0 * r.deltaXY;
+  return r;
+}
+
+// EBFieldGrid::validateGeometry, eb_field.cpp:878-943.
+void validateGeometry(const aprl_grid_geometry& g, const CowanCoef& cw) {
+  const double eps = std::numeric_limits<double>::epsilon();
+  if (g.nx < 2 || g.ny < 2 || g.nz < 2)
+    throw std::invalid_argument("E/B staggered grid needs at least two cells in every direction");
+  if (!(g.dx > 0.0) || !(g.dy > 0.0) || !(g.dz > 0.0) || !(g.dt > 0.0) || !std::isfinite(g.dx) ||
+      !std::isfinite(g.dy) || !std::isfinite(g.dz) || !std::isfinite(g.dt))
+    throw std::invalid_argument("E/B grid spacing and time step must be positive finite SI values");
+  if (g.nx + 1 > (1u << 30) || g.ny + 1 > (1u << 30) || g.nz + 1 > (1u << 30))
+    throw std::invalid_argument("E/B grid extent exceeds the device index range");
+  if (g.solver == 0) {
+    const double courant = kC * g.dt * std::sqrt(1.0 / (g.dx * g.dx) + 1.0 / (g.dy * g.dy) + 1.0 / (g.dz * g.dz));
+    if (courant > 1.0 + 16.0 * eps)
+      throw std::invalid_argument("E/B time step violates the three-dimensional Yee CFL limit");
+    return;
+  }
+  const double tol = 64.0 * eps * std::max(g.dx, std::max(g.dy, g.dz));
+  if (g.dx + tol < g.dz || g.dy + tol < g.dz)
+    throw std::invalid_argument("Cowan-z requires dz to be the smallest grid spacing");
+  if (std::abs(kC * g.dt / g.dz - 1.0) > 64.0 * eps)
+    throw std::invalid_argument("Cowan-z requires the dispersion-matched step c*dt=dz");
+  const double norm[3] = {cw.alpha[0] + 2.0 * cw.beta[1] + 2.0 * cw.beta[2] + 4.0 * cw.deltaYZ,
+                          cw.alpha[1] + 2.0 * cw.beta[2] + 2.0 * cw.beta[0] + 4.0 * cw.deltaZX,
+                          cw.alpha[2] + 2.0 * cw.beta[0] + 2.0 * cw.beta[1] + 4.0 * cw.deltaXY};
+  for (double v : norm)
+    if (!std::isfinite(v) || std::abs(v - 1.0) > 1.0e-12)
+      throw std::invalid_argument("Cowan-z smoothing coefficients are not normalized");
+  const double rx = cw.ratio[0], ry = cw.ratio[1], rz = cw.ratio[2];
+  const double fac[4] = {4.0 * ((cw.beta[2] + 2.0 * cw.deltaZX) / rz + (cw.beta[1] + 2.0 * cw.deltaXY) / ry),
+                         4.0 * ((cw.beta[0] + 2.0 * cw.deltaXY) / rx + (cw.beta[2] + 2.0 * cw.deltaYZ) / rz),
+                         4.0 * ((cw.beta[1] + 2.0 * cw.deltaYZ) / ry + (cw.beta[0] + 2.0 * cw.deltaZX) / rx),
+                         16.0 * (cw.deltaYZ / (ry * rz) + cw.deltaZX / (rz * rx) + cw.deltaXY / (rx * ry))};
+  for (double v : fac)
+    if (!std::isfinite(v) || std::abs(v - 1.0) > 1.0e-12)
+      throw std::invalid_argument("Cowan-z coefficients do not satisfy the stable dispersion factorization");
+}
+
+void allocateFields(FieldState& s, std::size_t slack) {
+  const std::size_t cells[3] = {s.g.nx, s.g.ny, s.g.nz};
+  for (int a = 0; a < 9; ++a) {
+    const bool electric = a < 3 || a >= 6;
+    const int axis = a % 3;
+    for (int d = 0; d < 3; ++d) s.dims[a][d] = cells[d] + (electric ? (d != axis) : (d == axis));
+  }
+  s.slack = slack;
+  s.zb = slack;
+  std::size_t largest = 0;
+  for (int a = 0; a < 9; ++a) {
+    const std::size_t planes = s.dims[a][2] + (s.slides(a) ? slack : 0);
+    s.comp[a].resize(planes * s.plane(a));
+    hipCheck(hipMemsetAsync(s.comp[a].get(), 0, s.comp[a].size() * sizeof(double), s.stream), "hipMemsetAsync");
+    largest = std::max(largest, s.compSize(a));
+  }
+  if (slack) s.scratch.resize(largest);
+}
+
+FieldPtrs FieldState::ptrs() const {
+  FieldPtrs f{};
+  for (int a = 0; a < 3; ++a) {
+    f.e[a] = base(a);
+    f.b[a] = base(3 + a);
+    f.j[a] = base(6 + a);
+    f.lo[a] = nullptr;
+    f.up[a] = nullptr;
+  }
+  for (int a = 0; a < 9; ++a)
+    for (int d = 0; d < 3; ++d) f.n[a][d] = static_cast<int>(dims[a][d]);
+  return f;
+}
+
+StepCoefs FieldState::coefs() const {
+  StepCoefs c{};
+  for (int a = 0; a < 3; ++a) { c.alpha[a] = cw.alpha[a]; c.beta[a] = cw.beta[a]; }
+  c.dYZ = cw.deltaYZ; c.dZX = cw.deltaZX; c.dXY = cw.deltaXY;
+  c.tx = g.dt / g.dx; c.ty = g.dt / g.dy; c.tz = g.dt / g.dz;
+  const double curlScale = g.dt / (kMu0 * kEps0);
+  c.cX = curlScale / g.dx; c.cY = curlScale / g.dy; c.cZ = curlScale / g.dz;
+  c.cs = g.dt / kEps0;
+  return c;
+}
+
+// Pair order of the checkpoint histories, eb_cpml.cpp:355-357 / 556-558.
+static const int kPairs[6][2] = {{0, 1}, {0, 2}, {1, 2}, {1, 0}, {2, 0}, {2, 1}};
+
+CpmlArgs FieldState::cpmlArgs() const {
+  CpmlArgs P{};
+  P.enabled = hasCpml ? 1 : 0;
+  P.lowerZ = lowerZ ? 1 : 0;
+  P.upperZ = upperZ ? 1 : 0;
+  P.zb = static_cast<int>(zb);
+  if (!hasCpml) return P;
+  for (int m = 0; m < 2; ++m)
+    for (int a = 0; a < 3; ++a) { P.prof[m][a] = prof[m][a].get(); P.ord[m][a] = ord[m][a].get(); }
+  for (int h = 0; h < 12; ++h) {
+    const int m = h < 6 ? 0 : 1, t = kPairs[h % 6][0], d = kPairs[h % 6][1];
+    P.hist[m][t][d] = hist[h].get();
+    P.row[m][t][d] = static_cast<int>(histRow[h]);
+    P.ostride[m][t][d] = static_cast<long long>(histSlides[h] ? (histLayers[h] + slack) * histRow[h]
+                                                               : histLayers[h] * histRow[h]);
+  }
+  const double curlScale = g.dt / (kMu0 * kEps0);
+  P.magScale[0] = g.dt / g.dx; P.magScale[1] = g.dt / g.dy; P.magScale[2] = g.dt / g.dz;
+  P.eleScale[0] = curlScale / g.dx; P.eleScale[1] = curlScale / g.dy; P.eleScale[2] = curlScale / g.dz;
+  return P;
+}
+
+// Offset of a history slot's first stored layer (checkpoint layer 0).
+std::size_t FieldState::historyDeviceOffset(std::size_t h, std::size_t slot) const {
+  if (!histSlides[h]) return slot * histLayers[h] * histRow[h];
+  return slot * (histLayers[h] + slack) * histRow[h] + zb * histRow[h];
+}
+
+// EBCPMLParameters::maximumConductivityRate, eb_cpml.cpp:263-281.
+static double maxRate(const aprl_cpml_parameters& p, int axis, const aprl_grid_geometry& g) {
+  if (p.cells[axis] == 0) return 0.0;
+  const double spacing = axis == 0 ? g.dx : (axis == 1 ? g.dy : g.dz);
+  if (!(spacing > 0.0) || !std::isfinite(spacing) || !(p.polynomial_order > 0.0) ||
+      !(p.target_reflection > 0.0 && p.target_reflection < 1.0))
+    throw std::invalid_argument("Cannot evaluate an invalid CPML conductivity profile");
+  const double thickness = static_cast<double>(p.cells[axis]) * spacing;
+  return -(p.polynomial_order + 1.0) * kC * std::log(p.target_reflection) / (2.0 * thickness);
+}
+
+// EBConvolutionalPML construction, eb_cpml.cpp:195-248 and 297-372; the
+// profiles are evaluated on the host with the same libm calls.
+void installCpml(FieldState& s, const aprl_cpml_parameters& p, std::size_t globalNz, std::size_t zOffset,
+                 bool lowerZ, bool upperZ) {
+  if (!(p.cells[0] > 0 || p.cells[1] > 0 || p.cells[2] > 0))
+    throw std::invalid_argument("CPML requires at least one nonzero boundary thickness");
+  if (!(p.polynomial_order > 0.0) || !(p.target_reflection > 0.0 && p.target_reflection < 1.0) ||
+      !(p.kappa_max >= 1.0) || !(p.alpha_fraction >= 0.0) || !std::isfinite(p.polynomial_order) ||
+      !std::isfinite(p.target_reflection) || !std::isfinite(p.kappa_max) || !std::isfinite(p.alpha_fraction))
+    throw std::invalid_argument("Invalid CPML grading parameters");
+  const aprl_grid_geometry& g = s.g;
+  const std::size_t gc[3] = {g.nx, g.ny, globalNz};
+  for (int a = 0; a < 3; ++a)
+    if (p.cells[a] > 0) {
+      if (p.cells[a] < 2) throw std::invalid_argument("Each enabled CPML direction needs at least two cells");
+      if (2 * p.cells[a] >= gc[a]) throw std::invalid_argument("Opposite CPML slabs must leave a non-PML interior");
+    }
+  if (p.cells[2] > 0) {
+    if (!lowerZ && zOffset < p.cells[2])
+      throw std::invalid_argument("The lower z CPML slab cannot span MPI ranks yet; reduce MPI ranks or CPML cells");
+    if (!upperZ && zOffset + g.nz > globalNz - p.cells[2])
+      throw std::invalid_argument("The upper z CPML slab cannot span MPI ranks yet; reduce MPI ranks or CPML cells");
+  }
+  if (s.slack && p.cells[2] > 0) throw std::invalid_argument("Native-z history transport requires transverse-only CPML");
+  std::size_t activeCount[2][3] = {};
+  for (int m = 0; m < 2; ++m) {
+    const bool electric = m == 1;
+    for (int axis = 0; axis < 3; ++axis) {
+      const std::size_t localCells = axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz);
+      const std::size_t globalCells = axis == 2 ? globalNz : localCells;
+      const std::size_t globalOffset = axis == 2 ? zOffset : 0;
+      const std::size_t count = localCells + (electric ? 1 : 0);
+      std::vector<CpmlPointDev> pts(count, CpmlPointDev{0.0, 0.0, 0.0});
+      std::vector<int> ordv(count, -1);
+      const std::size_t thicknessCells = p.cells[axis];
+      if (thicknessCells > 0) {
+        const double thickness = static_cast<double>(thicknessCells);
+        const double smax = maxRate(p, axis, g);
+        int ordinal = 0;
+        for (std::size_t li = 0; li < count; ++li) {
+          const double position = static_cast<double>(globalOffset + li) + (electric ? 0.0 : 0.5);
+          double depth = 0.0;
+          if (position < thickness) depth = (thickness - position) / thickness;
+          const double upperStart = static_cast<double>(globalCells - thicknessCells);
+          if (position > upperStart) depth = std::max(depth, (position - upperStart) / thickness);
+          if (!(depth > 0.0)) continue;
+          depth = std::min(1.0, depth);
+          const double graded = std::pow(depth, p.polynomial_order);
+          const double sigma = smax * graded;
+          const double kappa = 1.0 + (p.kappa_max - 1.0) * graded;
+          const double alpha = p.alpha_fraction * smax * (1.0 - depth);
+          CpmlPointDev& c = pts[li];
+          c.ikm1 = 1.0 / kappa - 1.0;
+          c.decay = std::exp(-(sigma / kappa + alpha) * g.dt);
+          const double den = sigma * kappa + alpha * kappa * kappa;
+          c.drive = den > 0.0 ? sigma * (c.decay - 1.0) / den : 0.0;
+          ordv[li] = ordinal++;
+        }
+      }
+      activeCount[m][axis] = static_cast<std::size_t>(std::count_if(ordv.begin(), ordv.end(), [](int o) { return o >= 0; }));
+      s.prof[m][axis].resize(count);
+      s.ord[m][axis].resize(count);
+      hipCheck(hipMemcpy(s.prof[m][axis].get(), pts.data(), count * sizeof(CpmlPointDev), hipMemcpyHostToDevice),
+               "hipMemcpy");
+      hipCheck(hipMemcpy(s.ord[m][axis].get(), ordv.data(), count * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+    }
+  }
+  for (int h = 0; h < 12; ++h) {
+    const int m = h < 6 ? 0 : 1, t = kPairs[h % 6][0], d = kPairs[h % 6][1];
+    const int a = m == 0 ? 3 + t : t;
+    const std::size_t* dim = s.dims[a];
+    s.histSlots[h] = activeCount[m][d];
+    if (d == 2) {
+      s.histSlides[h] = false;
+      s.histLayers[h] = dim[1];
+      s.histRow[h] = dim[0];
+    } else {
+      s.histSlides[h] = s.slack > 0;
+      s.histLayers[h] = dim[2];
+      s.histRow[h] = d == 0 ? dim[1] : dim[0];
+    }
+    const std::size_t layers = s.histLayers[h] + (s.histSlides[h] ? s.slack : 0);
+    s.hist[h].resize(s.histSlots[h] * layers * s.histRow[h]);
+    if (s.hist[h].size())
+      hipCheck(hipMemsetAsync(s.hist[h].get(), 0, s.hist[h].size() * sizeof(double), s.stream), "hipMemsetAsync");
+  }
+  s.cpml = p;
+  s.hasCpml = true;
+  s.lowerZ = lowerZ;
+  s.upperZ = upperZ;
+}
+
+// ---------------------------------------------------------------------------
+// Device code.
+
+// EBFieldGrid::electricValue, eb_field.cpp:350-417.
+template <int A>
+__device__ __forceinline__ double evalE(const FieldPtrs& F, int i, int j, int k) {
+  const int n0 = F.n[A][0], n1 = F.n[A][1], n2 = F.n[A][2];
+  double sign = 1.0;
+  if (i < 0) {
+    if (A == 0) i = 0;
+    else { i = -i; sign = -sign; }
+  } else if (i >= n0) {
+    if (A == 0) i = n0 - 1;
+    else { i = 2 * (n0 - 1) - i; sign = -sign; }
+  }
+  if (j < 0) {
+    if (A == 1) j = 0;
+    else { j = -j; sign = -sign; }
+  } else if (j >= n1) {
+    if (A == 1) j = n1 - 1;
+    else { j = 2 * (n1 - 1) - j; sign = -sign; }
+  }
+  if (k < 0 || k >= n2) {
+    const bool lower = k < 0;
+    const double* plane = lower ? F.lo[A] : F.up[A];
+    if (plane) return sign * plane[static_cast<size_t>(j) * n0 + i];
+    if (A == 2) k = lower ? 0 : n2 - 1;
+    else { k = lower ? -k : 2 * (n2 - 1) - k; sign = -sign; }
+  }
+  return sign * F.e[A][(static_cast<size_t>(k) * n1 + j) * n0 + i];
+}
+
+// EBFieldGrid::smoothedElectric, eb_field.cpp:419-486 (same accumulation
+// order on the interior fast path and the boundary path).
+template <int A, int D>
+__device__ __forceinline__ double smoothE(const FieldPtrs& F, const StepCoefs& c, int i, int j, int k) {
+  constexpr int TA = (D + 1) % 3, TB = (D + 2) % 3;
+  const double delta = D == 0 ? c.dYZ : (D == 1 ? c.dZX : c.dXY);
+  const int n0 = F.n[A][0], n1 = F.n[A][1], n2 = F.n[A][2];
+  if (i > 0 && i < n0 - 1 && j > 0 && j < n1 - 1 && k > 0 && k < n2 - 1) {
+    const long long st[3] = {1, n0, static_cast<long long>(n0) * n1};
+    const long long oA = st[TA], oB = st[TB];
+    const double* p = F.e[A] + (static_cast<size_t>(k) * n1 + j) * n0 + i;
+    double r = c.alpha[D] * p[0];
+    r += c.beta[TA] * p[-oA];
+    r += c.beta[TB] * p[-oB];
+    r += c.beta[TA] * p[oA];
+    r += c.beta[TB] * p[oB];
+    r += delta * p[-oA - oB];
+    r += delta * p[-oA + oB];
+    r += delta * p[oA - oB];
+    r += delta * p[oA + oB];
+    return r;
+  }
+  int b[3] = {i, j, k};
+  double r = c.alpha[D] * evalE<A>(F, i, j, k);
+  for (int dir = -1; dir <= 1; dir += 2) {
+    int pA[3] = {b[0], b[1], b[2]}, pB[3] = {b[0], b[1], b[2]};
+    pA[TA] += dir;
+    pB[TB] += dir;
+    r += c.beta[TA] * evalE<A>(F, pA[0], pA[1], pA[2]);
+    r += c.beta[TB] * evalE<A>(F, pB[0], pB[1], pB[2]);
+  }
+  for (int dA = -1; dA <= 1; dA += 2)
+    for (int dB = -1; dB <= 1; dB += 2) {
+      int p[3] = {b[0], b[1], b[2]};
+      p[TA] += dA;
+      p[TB] += dB;
+      r += delta * evalE<A>(F, p[0], p[1], p[2]);
+    }
+  return r;
+}
+
+template <int A>
+__device__ __forceinline__ double rawE(const FieldPtrs& F, int i, int j, int k) {
+  return F.e[A][(static_cast<size_t>(k) * F.n[A][1] + j) * F.n[A][0] + i];
+}
+
+// Index of a CPML history entry of target (M, T) and derivative D at node
+// (i, j, k), CPMLMemoryTerm::value (eb_cpml.cpp:78-89) with slid layers.
+template <int M, int T, int D>
+__device__ __forceinline__ double* histAt(const CpmlArgs& P, int o, int i, int j, int k, int dim0) {
+  double* h = P.hist[M][T][D] + static_cast<long long>(o) * P.ostride[M][T][D];
+  if (D == 0) return h + static_cast<long long>(P.zb + k) * P.row[M][T][D] + j;
+  if (D == 1) return h + static_cast<long long>(P.zb + k) * P.row[M][T][D] + i;
+  return h + static_cast<long long>(j) * dim0 + i;
+}
+
+// transformedCorrection, eb_cpml.cpp:131-138. WRITE selects whether the
+// advanced history is stored (preview keeps the old one).
+template <bool WRITE>
+__device__ __forceinline__ double cpmlCorrection(const CpmlPointDev& c, double raw, double* hist) {
+  const double h = c.decay * (*hist) + c.drive * raw;
+  if (WRITE) *hist = h;
+  return c.ikm1 * raw + h;
+}
+
+// Fused magnetic update over the union of the three B extents.
+// advanceMagneticCowan (eb_field.cpp:488-558) / advanceMagneticYee
+// (311-348) followed by EBConvolutionalPML::afterMagneticUpdate (406-468).
+template <bool COWAN>
+__global__ void __launch_bounds__(256) kMagnetic(FieldPtrs F, StepCoefs c, CpmlArgs P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
+  if (i > nx || j > ny || k > nz) return;
+  if (j < ny && k < nz) {  // Bx: (nx+1) x ny x nz
+    double dEzDy, dEyDz;
+    if (COWAN) {
+      dEzDy = smoothE<2, 1>(F, c, i, j + 1, k) - smoothE<2, 1>(F, c, i, j, k);
+      dEyDz = smoothE<1, 2>(F, c, i, j, k + 1) - smoothE<1, 2>(F, c, i, j, k);
+    } else {
+      dEzDy = rawE<2>(F, i, j + 1, k) - rawE<2>(F, i, j, k);
+      dEyDz = rawE<1>(F, i, j, k + 1) - rawE<1>(F, i, j, k);
+    }
+    const size_t idx = (static_cast<size_t>(k) * ny + j) * (nx + 1) + i;
+    double b = F.b[0][idx];
+    b -= c.ty * dEzDy - c.tz * dEyDz;
+    if (P.enabled) {
+      int o = P.ord[0][1][j];
+      if (o >= 0) b += (-1.0 * P.magScale[1]) * cpmlCorrection<true>(P.prof[0][1][j], dEzDy, histAt<0, 0, 1>(P, o, i, j, k, nx + 1));
+      o = P.ord[0][2][k];
+      if (o >= 0) b += (1.0 * P.magScale[2]) * cpmlCorrection<true>(P.prof[0][2][k], dEyDz, histAt<0, 0, 2>(P, o, i, j, k, nx + 1));
+    }
+    F.b[0][idx] = b;
+  }
+  if (i < nx && k < nz) {  // By: nx x (ny+1) x nz
+    double dExDz, dEzDx;
+    if (COWAN) {
+      dExDz = smoothE<0, 2>(F, c, i, j, k + 1) - smoothE<0, 2>(F, c, i, j, k);
+      dEzDx = smoothE<2, 0>(F, c, i + 1, j, k) - smoothE<2, 0>(F, c, i, j, k);
+    } else {
+      dExDz = rawE<0>(F, i, j, k + 1) - rawE<0>(F, i, j, k);
+      dEzDx = rawE<2>(F, i + 1, j, k) - rawE<2>(F, i, j, k);
+    }
+    const size_t idx = (static_cast<size_t>(k) * (ny + 1) + j) * nx + i;
+    double b = F.b[1][idx];
+    b -= c.tz * dExDz - c.tx * dEzDx;
+    if (P.enabled) {
+      int o = P.ord[0][2][k];
+      if (o >= 0) b += (-1.0 * P.magScale[2]) * cpmlCorrection<true>(P.prof[0][2][k], dExDz, histAt<0, 1, 2>(P, o, i, j, k, nx));
+      o = P.ord[0][0][i];
+      if (o >= 0) b += (1.0 * P.magScale[0]) * cpmlCorrection<true>(P.prof[0][0][i], dEzDx, histAt<0, 1, 0>(P, o, i, j, k, nx));
+    }
+    F.b[1][idx] = b;
+  }
+  if (i < nx && j < ny) {  // Bz: nx x ny x (nz+1)
+    double dEyDx, dExDy;
+    if (COWAN) {
+      dEyDx = smoothE<1, 0>(F, c, i + 1, j, k) - smoothE<1, 0>(F, c, i, j, k);
+      dExDy = smoothE<0, 1>(F, c, i, j + 1, k) - smoothE<0, 1>(F, c, i, j, k);
+    } else {
+      dEyDx = rawE<1>(F, i + 1, j, k) - rawE<1>(F, i, j, k);
+      dExDy = rawE<0>(F, i, j + 1, k) - rawE<0>(F, i, j, k);
+    }
+    const size_t idx = (static_cast<size_t>(k) * ny + j) * nx + i;
+    double b = F.b[2][idx];
+    b -= c.tx * dEyDx - c.ty * dExDy;
+    if (P.enabled) {
+      int o = P.ord[0][0][i];
+      if (o >= 0) b += (-1.0 * P.magScale[0]) * cpmlCorrection<true>(P.prof[0][0][i], dEyDx, histAt<0, 2, 0>(P, o, i, j, k, nx));
+      o = P.ord[0][1][j];
+      if (o >= 0) b += (1.0 * P.magScale[1]) * cpmlCorrection<true>(P.prof[0][1][j], dExDy, histAt<0, 2, 1>(P, o, i, j, k, nx));
+    }
+    F.b[2][idx] = b;
+  }
+}
+
+__device__ __forceinline__ double rd(const double* p, int nx, int ny, int i, int j, int k) {
+  return p[(static_cast<size_t>(k) * ny + j) * nx + i];
+}
+
+// Fused electric update: EBFieldGrid::advanceElectric interior curl
+// (eb_field.cpp:582-624), EBConvolutionalPML::afterElectricUpdate
+// (eb_cpml.cpp:470-539) incl. its PEC termination (eb_field.cpp:158-204),
+// and the moving-window inlet overwrite (moving_window_fields.cpp:276-280).
+// MODE 0 writes E in place and advances the histories; MODE 1 writes the
+// zero-current preview into `out` and leaves the histories untouched.
+template <int MODE>
+__global__ void __launch_bounds__(256) kElectric(FieldPtrs F, StepCoefs c, CpmlArgs P, double* ox, double* oy,
+                                                  double* oz, const double* inletX, const double* inletY) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
+  if (i > nx || j > ny || k > nz) return;
+  constexpr bool W = MODE == 0;
+  const double* bx = F.b[0];
+  const double* by = F.b[1];
+  const double* bz = F.b[2];
+  const int kb = P.lowerZ ? 1 : 0, ke = P.upperZ ? nz : nz + 1;
+  if (i < nx) {  // Ex: nx x (ny+1) x (nz+1)
+    const size_t idx = (static_cast<size_t>(k) * (ny + 1) + j) * nx + i;
+    double e = F.e[0][idx];
+    if (k >= 1 && k < nz && j >= 1 && j < ny)
+      e += c.cY * (rd(bz, nx, ny, i, j, k) - rd(bz, nx, ny, i, j - 1, k)) -
+           c.cZ * (rd(by, nx, ny + 1, i, j, k) - rd(by, nx, ny + 1, i, j, k - 1)) - c.cs * F.j[0][idx];
+    if (P.enabled && j >= 1 && j < ny && k >= kb && k < ke) {
+      int o = P.ord[1][1][j];
+      if (o >= 0)
+        e += (1.0 * P.eleScale[1]) * cpmlCorrection<W>(P.prof[1][1][j], rd(bz, nx, ny, i, j, k) - rd(bz, nx, ny, i, j - 1, k),
+                                                        histAt<1, 0, 1>(P, o, i, j, k, nx));
+      o = P.ord[1][2][k];
+      if (o >= 0)
+        e += (-1.0 * P.eleScale[2]) * cpmlCorrection<W>(P.prof[1][2][k], rd(by, nx, ny + 1, i, j, k) - rd(by, nx, ny + 1, i, j, k - 1),
+                                                         histAt<1, 0, 2>(P, o, i, j, k, nx));
+    }
+    if (j == 0 || j == ny || (P.lowerZ && k == 0) || (P.upperZ && k == nz)) e = 0.0;
+    if (inletX && k == 0) e = inletX[static_cast<size_t>(j) * nx + i];
+    (MODE == 0 ? F.e[0] : ox)[idx] = e;
+  }
+  if (j < ny) {  // Ey: (nx+1) x ny x (nz+1)
+    const size_t idx = (static_cast<size_t>(k) * ny + j) * (nx + 1) + i;
+    double e = F.e[1][idx];
+    if (k >= 1 && k < nz && i >= 1 && i < nx)
+      e += c.cZ * (rd(bx, nx + 1, ny, i, j, k) - rd(bx, nx + 1, ny, i, j, k - 1)) -
+           c.cX * (rd(bz, nx, ny, i, j, k) - rd(bz, nx, ny, i - 1, j, k)) - c.cs * F.j[1][idx];
+    if (P.enabled && i >= 1 && i < nx && k >= kb && k < ke) {
+      int o = P.ord[1][2][k];
+      if (o >= 0)
+        e += (1.0 * P.eleScale[2]) * cpmlCorrection<W>(P.prof[1][2][k], rd(bx, nx + 1, ny, i, j, k) - rd(bx, nx + 1, ny, i, j, k - 1),
+                                                        histAt<1, 1, 2>(P, o, i, j, k, nx + 1));
+      o = P.ord[1][0][i];
+      if (o >= 0)
+        e += (-1.0 * P.eleScale[0]) * cpmlCorrection<W>(P.prof[1][0][i], rd(bz, nx, ny, i, j, k) - rd(bz, nx, ny, i - 1, j, k),
+                                                         histAt<1, 1, 0>(P, o, i, j, k, nx + 1));
+    }
+    if (i == 0 || i == nx || (P.lowerZ && k == 0) || (P.upperZ && k == nz)) e = 0.0;
+    if (inletY && k == 0) e = inletY[static_cast<size_t>(j) * (nx + 1) + i];
+    (MODE == 0 ? F.e[1] : oy)[idx] = e;
+  }
+  if (k < nz) {  // Ez: (nx+1) x (ny+1) x nz
+    const size_t idx = (static_cast<size_t>(k) * (ny + 1) + j) * (nx + 1) + i;
+    double e = F.e[2][idx];
+    const bool in = i >= 1 && i < nx && j >= 1 && j < ny;
+    if (in)
+      e += c.cX * (rd(by, nx, ny + 1, i, j, k) - rd(by, nx, ny + 1, i - 1, j, k)) -
+           c.cY * (rd(bx, nx + 1, ny, i, j, k) - rd(bx, nx + 1, ny, i, j - 1, k)) - c.cs * F.j[2][idx];
+    if (P.enabled && in) {
+      int o = P.ord[1][0][i];
+      if (o >= 0)
+        e += (1.0 * P.eleScale[0]) * cpmlCorrection<W>(P.prof[1][0][i], rd(by, nx, ny + 1, i, j, k) - rd(by, nx, ny + 1, i - 1, j, k),
+                                                        histAt<1, 2, 0>(P, o, i, j, k, nx + 1));
+      o = P.ord[1][1][j];
+      if (o >= 0)
+        e += (-1.0 * P.eleScale[1]) * cpmlCorrection<W>(P.prof[1][1][j], rd(bx, nx + 1, ny, i, j, k) - rd(bx, nx + 1, ny, i, j - 1, k),
+                                                         histAt<1, 2, 1>(P, o, i, j, k, nx + 1));
+    }
+    if (i == 0 || i == nx || j == 0 || j == ny) e = 0.0;
+    (MODE == 0 ? F.e[2] : oz)[idx] = e;
+  }
+}
+
+__global__ void kFill(double* p, size_t n, double v) {
+  for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<size_t>(gridDim.x) * blockDim.x)
+    p[t] = v;
+}
+
+static void fill(hipStream_t s, double* p, size_t n, double v) {
+  if (!n) return;
+  const unsigned blocks = std::min<size_t>((n + 255) / 256, 4096);
+  kFill<<<blocks, 256, 0, s>>>(p, n, v);
+  launchCheck("fill");
+}
+
+void launchMagnetic(FieldState& s, const FieldPtrs& f) {
+  const StepCoefs c = s.coefs();
+  const CpmlArgs P = s.cpmlArgs();
+  dim3 block(64, 4, 1);
+  dim3 grid(ceilDiv(s.g.nx + 1, 64), ceilDiv(s.g.ny + 1, 4), static_cast<unsigned>(s.g.nz + 1));
+  ScopedKernel sk(s.profiler, s.stream, "magnetic");
+  if (s.g.solver == 1) kMagnetic<true><<<grid, block, 0, s.stream>>>(f, c, P);
+  else kMagnetic<false><<<grid, block, 0, s.stream>>>(f, c, P);
+  launchCheck("magnetic");
+}
+
+void launchElectric(FieldState& s, const FieldPtrs& f, int mode, double* const out[3], const double* inletX,
+                    const double* inletY) {
+  const StepCoefs c = s.coefs();
+  CpmlArgs P = s.cpmlArgs();
+  if (!s.hasCpml) {  // PerfectElectricConductorBoundary(lowerZ, upperZ)
+    P.lowerZ = s.lowerZ;
+    P.upperZ = s.upperZ;
+  }
+  dim3 block(64, 4, 1);
+  dim3 grid(ceilDiv(s.g.nx + 1, 64), ceilDiv(s.g.ny + 1, 4), static_cast<unsigned>(s.g.nz + 1));
+  if (mode == 0) {
+    ScopedKernel sk(s.profiler, s.stream, "electric");
+    kElectric<0><<<grid, block, 0, s.stream>>>(f, c, P, nullptr, nullptr, nullptr, inletX, inletY);
+  } else {
+    ScopedKernel sk(s.profiler, s.stream, "electric_preview");
+    kElectric<1><<<grid, block, 0, s.stream>>>(f, c, P, out[0], out[1], out[2], inletX, inletY);
+  }
+  launchCheck("electric");
+}
+
+void launchClearCurrent(FieldState& s) {
+  ScopedKernel sk(s.profiler, s.stream, "clear_current");
+  for (int a = 6; a < 9; ++a) fill(s.stream, s.comp[a].get(), s.compSize(a), 0.0);
+}
+
+// Recenter the slid arrays into their slack once the slack is used up.
+static void recenter(FieldState& s) {
+  for (int a = 0; a < 6; ++a) {
+    const size_t bytes = s.compSize(a) * sizeof(double);
+    hipCheck(hipMemcpyAsync(s.scratch.get(), s.base(a), bytes, hipMemcpyDeviceToDevice, s.stream), "hipMemcpyAsync");
+    hipCheck(hipMemcpyAsync(s.comp[a].get() + s.slack * s.plane(a), s.scratch.get(), bytes, hipMemcpyDeviceToDevice,
+                            s.stream),
+             "hipMemcpyAsync");
+  }
+  for (int h = 0; h < 12; ++h) {
+    if (!s.histSlides[h] || !s.hist[h].size()) continue;
+    const size_t block = s.histLayers[h] * s.histRow[h];
+    for (size_t slot = 0; slot < s.histSlots[h]; ++slot) {
+      double* slotBase = s.hist[h].get() + slot * (s.histLayers[h] + s.slack) * s.histRow[h];
+      hipCheck(hipMemcpyAsync(s.scratch.get(), slotBase + s.zb * s.histRow[h], block * sizeof(double),
+                              hipMemcpyDeviceToDevice, s.stream),
+               "hipMemcpyAsync");
+      hipCheck(hipMemcpyAsync(slotBase + s.slack * s.histRow[h], s.scratch.get(), block * sizeof(double),
+                              hipMemcpyDeviceToDevice, s.stream),
+               "hipMemcpyAsync");
+    }
+  }
+  s.zb = s.slack;
+}
+
+// MovingWindowFields::Impl::shift (moving_window_fields.cpp:439-518) with
+// resetCurrent: slide every E/B component and transverse history one plane,
+// insert the reference planes (zero histories) at k = 0, clear J.
+void windowShift(FieldState& s, const double* const planes[6]) {
+  ScopedKernel sk(s.profiler, s.stream, "shift");
+  if (s.zb == 0) recenter(s);
+  s.zb -= 1;
+  for (int a = 0; a < 6; ++a) {
+    if (planes && planes[a])
+      hipCheck(hipMemcpyAsync(s.base(a), planes[a], s.plane(a) * sizeof(double), hipMemcpyDeviceToDevice, s.stream),
+               "hipMemcpyAsync");
+    else
+      fill(s.stream, s.base(a), s.plane(a), 0.0);
+  }
+  for (int a = 6; a < 9; ++a) fill(s.stream, s.comp[a].get(), s.compSize(a), 0.0);
+  for (int h = 0; h < 12; ++h) {
+    if (!s.histSlides[h] || !s.hist[h].size()) continue;
+    for (size_t slot = 0; slot < s.histSlots[h]; ++slot)
+      fill(s.stream, s.hist[h].get() + s.historyDeviceOffset(h, slot), s.histRow[h], 0.0);
+  }
+}
+
+}  // namespace aprl_gpu

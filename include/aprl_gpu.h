@@ -1,0 +1,127 @@
+/* aprl_gpu.h — C-ABI of the MI355X (gfx950) implementation of APRL's
+ * per-timestep electromagnetic PIC loop.
+ *
+ * Plain pointers and sizes only. Every call is stream-ordered on the
+ * instance's own HIP stream and returns an aprl_status; aprl_last_error()
+ * gives the message of the last failure on the calling thread. Status codes
+ * map one-to-one onto the reference's exception types so that a C++ shim can
+ * rethrow the same type and text (SURVEY: eb_field.cpp:880-941,
+ * eb_field.cpp:976-990, simulation_source_coupling.cpp:54,
+ * moving_window_fields.cpp:208-209).
+ *
+ * Array layouts are the reference's: each Yee component is x-fastest,
+ * index (k*ny + j)*nx + i, with the staggered shapes of eb_field.cpp:217-225:
+ *   Ex nx x (ny+1) x (nz+1)   Ey (nx+1) x ny x (nz+1)   Ez (nx+1) x (ny+1) x nz
+ *   Bx (nx+1) x ny x nz       By nx x (ny+1) x nz       Bz nx x ny x (nz+1)
+ *   J  collocated with E.
+ * CPML histories use EBConvolutionalPML::checkpointHistory order and layout
+ * (eb_cpml.cpp:546-577).
+ */
+#ifndef APRL_GPU_H
+#define APRL_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum aprl_status {
+  APRL_OK = 0,
+  APRL_INVALID_ARGUMENT = 1, /* std::invalid_argument */
+  APRL_OUT_OF_RANGE = 2,     /* std::out_of_range */
+  APRL_RUNTIME_ERROR = 3,    /* std::runtime_error */
+  APRL_LOGIC_ERROR = 4,      /* std::logic_error */
+  APRL_DEVICE_ERROR = 5      /* HIP runtime failure */
+} aprl_status;
+
+typedef enum aprl_precision {
+  APRL_FP64 = 0,      /* plain form: everything in double */
+  APRL_FP32_SPLIT = 1 /* fields/currents/momenta float, positions cell+offset */
+} aprl_precision;
+
+/* EBGridGeometry, eb_field.h:54-70. solver: 0 = Yee, 1 = Cowan-z. */
+typedef struct aprl_grid_geometry {
+  size_t nx, ny, nz;
+  double dx, dy, dz, dt;
+  int solver;
+} aprl_grid_geometry;
+
+/* EBCPMLParameters, eb_cpml.h:12-24. */
+typedef struct aprl_cpml_parameters {
+  size_t cells[3];
+  double polynomial_order;
+  double target_reflection;
+  double kappa_max;
+  double alpha_fraction;
+} aprl_cpml_parameters;
+
+typedef struct aprl_fields aprl_fields;
+
+const char* aprl_last_error(void);
+
+/* EBFieldGrid::EBFieldGrid, eb_field.h:204 / eb_field.cpp:211-229: allocates
+ * the nine components on the current device, zeroed, with the default PEC
+ * boundary; validates the geometry (eb_field.cpp:878-943). */
+aprl_status aprl_fields_create(const aprl_grid_geometry* geometry, aprl_precision precision,
+                               aprl_fields** out);
+void aprl_fields_destroy(aprl_fields* fields);
+
+/* YeeComponent::data() copies (eb_field.h:129-131); component 0..8 =
+ * Ex Ey Ez Bx By Bz Jx Jy Jz; count must equal the component size. */
+aprl_status aprl_fields_upload(aprl_fields* fields, int component, const double* host, size_t count);
+aprl_status aprl_fields_download(aprl_fields* fields, int component, double* host, size_t count);
+size_t aprl_fields_component_size(const aprl_fields* fields, int component);
+
+/* EBConvolutionalPML::EBConvolutionalPML + EBFieldGrid::setBoundary
+ * (eb_cpml.h:33-39, eb_field.h:231). */
+aprl_status aprl_fields_set_cpml(aprl_fields* fields, const aprl_cpml_parameters* parameters,
+                                 size_t global_nz, size_t local_z_offset, int lower_z_physical,
+                                 int upper_z_physical);
+/* EBConvolutionalPML::checkpointHistory(index), eb_cpml.h:51-53. */
+size_t aprl_fields_history_size(const aprl_fields* fields, size_t index);
+aprl_status aprl_fields_history_download(aprl_fields* fields, size_t index, double* host, size_t count);
+aprl_status aprl_fields_history_upload(aprl_fields* fields, size_t index, const double* host, size_t count);
+
+/* EBFieldGrid::clearCurrent, eb_field.cpp:271-295. */
+aprl_status aprl_fields_clear_current(aprl_fields* fields);
+/* EBFieldGrid::advanceMagnetic(const EBElectricHaloView&), eb_field.cpp:302-309:
+ * the B curl (Cowan or Yee) fused with the boundary's afterMagneticUpdate.
+ * halo[0..2] = lower Ex,Ey,Ez planes, halo[3..5] = upper; NULL array or
+ * NULL entries select the PEC parity extension (eb_field.cpp:388-411). */
+aprl_status aprl_fields_advance_magnetic(aprl_fields* fields, const double* const* halo);
+/* EBFieldGrid::advanceElectric, eb_field.cpp:582-627: curl, -dt/eps0 J, and
+ * the boundary's afterElectricUpdate (CPML-E, PEC, window inlet). */
+aprl_status aprl_fields_advance_electric(aprl_fields* fields);
+aprl_status aprl_fields_synchronize(aprl_fields* fields);
+
+/* MovingWindowFields (moving_window_fields.h:72-134), single owner of the
+ * whole window. install: xy-only CPML, no z PEC (moving_window_fields.cpp:193). */
+aprl_status aprl_window_install(aprl_fields* fields, const aprl_cpml_parameters* parameters);
+/* beginStep + physicalElectricHalos (moving_window_fields.cpp:207-244):
+ * lower[a] = reference(a,i,j,offset-1,step) planes of Ex,Ey,Ez; NULL = zero
+ * reference. The upper halo is the sacrificial zero plane. */
+aprl_status aprl_window_begin_step(aprl_fields* fields, size_t step, ptrdiff_t offset,
+                                   const double* const* lower);
+/* Inlet overwrite values Ex/Ey(k=0) := reference(a,i,j,offset,step+1)
+ * (moving_window_fields.cpp:276-280); NULL = zero reference. */
+aprl_status aprl_window_set_inlet(aprl_fields* fields, const double* ex, const double* ey);
+/* shift (moving_window_fields.cpp:439-518, resetCurrent = true): every E/B
+ * component and every CPML history moves up one z plane; planes[a] =
+ * reference(a,i,j,new_offset,new_step) for a = 0..5 (NULL = zero). */
+aprl_status aprl_window_shift(aprl_fields* fields, size_t new_step, ptrdiff_t new_offset,
+                              const double* const* planes);
+
+/* Per-kernel timing with HIP events on the instance stream. enable=1 starts
+ * recording every launch; query returns launches and total milliseconds for
+ * a kernel name (0 launches if unknown). */
+aprl_status aprl_profile_enable(aprl_fields* fields, int enable);
+aprl_status aprl_profile_query(aprl_fields* fields, const char* kernel, uint64_t* launches, double* total_ms);
+aprl_status aprl_profile_reset(aprl_fields* fields);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* APRL_GPU_H */
