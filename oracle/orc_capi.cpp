@@ -141,3 +141,165 @@ double orc_fields_poynting(void* h, std::size_t i, std::size_t j, std::size_t k,
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Particle side.
+namespace {
+struct CouplingHandle {
+  orc::Coupling c;
+  std::vector<orc::Particle> particles;
+  std::size_t cpmlCells[3] = {0, 0, 0};
+};
+CouplingHandle* CH(void* h) { return static_cast<CouplingHandle*>(h); }
+orc::V3 vec(const double* p) { return orc::V3{{p[0], p[1], p[2]}}; }
+}  // namespace
+
+extern "C" {
+
+int orc_coupling_new(void* fields, int shape, int smoothX, void** out) {
+  return guarded([&] {
+    std::unique_ptr<CouplingHandle> h(new CouplingHandle);
+    h->c.fields = F(fields);
+    h->c.shape = shape;
+    h->c.smoothX = smoothX != 0;
+    if (smoothX && shape != orc::kBinomial3)
+      throw std::invalid_argument("Transverse three-point source requires longitudinal linear-binomial3");
+    if (F(fields)->cpml)
+      for (int a = 0; a < 3; ++a) h->cpmlCells[a] = F(fields)->cpml->p.cells[a];
+    *out = h.release();
+  });
+}
+void orc_coupling_free(void* h) { delete CH(h); }
+
+int orc_coupling_set_frame(void* h, double gamma, double labT0, double labZ0, double boxZ0) {
+  return guarded([&] { CH(h)->c.frame.setOriginsFromGamma(gamma, orc::kC, labT0, labZ0, boxZ0); });
+}
+
+// type: 0 planar undulator, 1 uniform dipole, 2 quadrupole.
+int orc_coupling_add_magnet(void* h, int type, const double* center, double length, double period, double peak,
+                            double angle, int gaussianFringe, double cutoff, double gradient, double endRamp,
+                            double phase) {
+  return guarded([&] {
+    orc::Magnet m;
+    m.type = type;
+    m.center = vec(center);
+    m.length = length; m.period = period; m.peak = peak; m.angle = angle;
+    m.gaussianFringe = gaussianFringe != 0; m.fringeCutoff = cutoff; m.gradient = gradient;
+    m.endRampPeriods = endRamp; m.phase = phase;
+    m.prepare();
+    CH(h)->c.sources.magnets.push_back(m);
+  });
+}
+
+int orc_coupling_set_particles(void* h, std::size_t n, const double* x3, const double* u3, const double* q,
+                               const double* m, const double* w, const std::uint64_t* id) {
+  return guarded([&] {
+    auto& P = CH(h)->particles;
+    P.assign(n, orc::Particle());
+    for (std::size_t i = 0; i < n; ++i) {
+      P[i].position = vec(x3 + 3 * i);
+      P[i].u = vec(u3 + 3 * i);
+      P[i].charge = q[i]; P[i].mass = m[i]; P[i].weight = w[i]; P[i].id = id[i];
+    }
+  });
+}
+
+std::size_t orc_coupling_count(void* h) { return CH(h)->particles.size(); }
+
+int orc_coupling_get_particles(void* h, double* x3, double* u3) {
+  return guarded([&] {
+    const auto& P = CH(h)->particles;
+    for (std::size_t i = 0; i < P.size(); ++i)
+      for (int a = 0; a < 3; ++a) { x3[3 * i + a] = P[i].position[a]; u3[3 * i + a] = P[i].u[a]; }
+  });
+}
+
+// One conservative-source step after beginStep/advanceMagnetic/inlet staging.
+// exitLabZ = +inf disables the lab exit plane.
+int orc_coupling_step(void* h, const double* origin, double time, unsigned substeps, double exitLabZ,
+                      unsigned* iterations, double* force, double* work) {
+  return guarded([&] {
+    CouplingHandle& H = *CH(h);
+    orc::Fields& f = *H.c.fields;
+    orc::ParticleBoundary pb;
+    pb.set(f.g, vec(origin), H.cpmlCells);
+    pb.window = std::isfinite(exitLabZ);
+    pb.exitLabZ = exitLabZ;
+    const orc::StepResult r = orc::sourceStep(f, H.c, H.particles, vec(origin), time, substeps, pb);
+    *iterations = r.iterations; *force = r.force; *work = r.work;
+  });
+}
+
+// Unit-level entry points: the visit list of longitudinal_shape::weights.
+int orc_shape_weights(int shape, int smoothX, const double* geometry7, const double* origin, const double* begin,
+                      const double* end, int* axis, long long* ijk, double* w, std::size_t capacity,
+                      std::size_t* count, unsigned* segments) {
+  return guarded([&] {
+    orc::Grid g;
+    g.nx = (std::size_t)geometry7[0]; g.ny = (std::size_t)geometry7[1]; g.nz = (std::size_t)geometry7[2];
+    g.dx = geometry7[3]; g.dy = geometry7[4]; g.dz = geometry7[5]; g.dt = geometry7[6];
+    std::size_t n = 0;
+    *segments = orc::weights(g, vec(origin), vec(begin), vec(end), [&](unsigned a, long long i, long long j, long long k, double v) {
+      if (n >= capacity) throw std::length_error("weights capacity");
+      axis[n] = (int)a; ijk[3 * n] = i; ijk[3 * n + 1] = j; ijk[3 * n + 2] = k; w[n] = v; ++n;
+    }, shape, smoothX != 0);
+    *count = n;
+  });
+}
+
+int orc_momentum(double* u, const double* e, const double* b, double q, double m, double h, double g0,
+                 unsigned* iterations) {
+  return guarded([&] {
+    orc::V3 uu = vec(u);
+    *iterations = orc::momentum(uu, vec(e), vec(b), q, m, h, g0);
+    for (int a = 0; a < 3; ++a) u[a] = uu[a];
+  });
+}
+
+// Copy the current E into the coupling's old-E and set the origin (begin).
+int orc_coupling_begin_fields(void* h, const double* origin) {
+  return guarded([&] { CH(h)->c.begin(*CH(h)->c.fields, vec(origin), CH(h)->particles); });
+}
+int orc_coupling_gather(void* h, const double* a, const double* b, double* e, double* bm) {
+  return guarded([&] {
+    const orc::V3 ee = CH(h)->c.electric(vec(a), vec(b));
+    const orc::V3 bb = CH(h)->c.magnetic(vec(a), vec(b));
+    for (int k = 0; k < 3; ++k) { e[k] = ee[k]; bm[k] = bb[k]; }
+  });
+}
+int orc_coupling_set_cpml_cells(void* h, const std::size_t* cells) {
+  return guarded([&] { for (int a = 0; a < 3; ++a) CH(h)->cpmlCells[a] = cells[a]; });
+}
+// Shape-level deposit into the grid's J (longitudinal_shape::Current, one rank).
+int orc_current_deposit(void* fields, int shape, int smoothX, const double* origin, const double* a, const double* b,
+                        double charge) {
+  return guarded([&] {
+    orc::Coupling c;
+    c.fields = F(fields);
+    c.shape = shape;
+    c.smoothX = smoothX != 0;
+    c.origin = vec(origin);
+    c.deposit(vec(a), vec(b), charge);
+  });
+}
+int orc_binomial_xz_magnetic(void* fields, const double* origin, const double* point, double* out) {
+  return guarded([&] {
+    const orc::V3 v = orc::binomialXZMagneticSample(*F(fields), vec(origin), vec(point));
+    for (int k = 0; k < 3; ++k) out[k] = v[k];
+  });
+}
+int orc_magnetic_sample_open(void* fields, const double* origin, const double* point, double* out) {
+  return guarded([&] {
+    const orc::V3 v = orc::magneticSampleOpen(*F(fields), vec(origin), vec(point));
+    for (int k = 0; k < 3; ++k) out[k] = v[k];
+  });
+}
+int orc_prescribed_box(void* h, const double* pos, double t, double* e, double* b) {
+  return guarded([&] {
+    orc::V3 ee = orc::v3(), bb = orc::v3();
+    CH(h)->c.sources.addPrescribedBox(vec(pos), t, CH(h)->c.frame, ee, bb);
+    for (int k = 0; k < 3; ++k) { e[k] = ee[k]; b[k] = bb[k]; }
+  });
+}
+
+}  // extern "C"

@@ -1,3 +1,858 @@
-// Oracle: particle side (TEST INFRASTRUCTURE ONLY). Filled in below.
+// Oracle: particle side of APRL's conservative-source PIC step
+// (TEST INFRASTRUCTURE ONLY; see the header of orc_fields.hpp).
+//
+// Restates, in the reference's association order:
+//   boostframe.cpp:74-87, 188-196, 248-265   frame transforms
+//   eb_sources.cpp:320-446, 539-582          prescribed lab magnets
+//   longitudinal_shape.h:17-117, 172-187     shapes, weights, deposit
+//   transverse_source_shape.h:8-10           x-binomial node weights
+//   source_coupling_math.h:105-167, 235-293  gathers, clipping, momentum
+//   source_coupling.cpp:31-51, 84-104, 105-159  coupling push/residual
+//   simulation_source_coupling.cpp:16-65     Picard orchestration
+//   simulation.cpp:4167-5605                 particle loop, active particles
+// Pinning: bitwise against oracle/_ref for weights, gathers, momentum,
+// deposit and whole coupled steps; plus charge continuity, gather/deposit
+// adjointness and the Lapenta-Markidis identities (tests/test_oracle_*.py).
 #pragma once
+
+#include <algorithm>
+#include <array>
+#include <cstdint>
+#include <unordered_map>
+
 #include "orc_fields.hpp"
+
+namespace orc {
+
+using V3 = std::array<double, 3>;
+inline V3 v3(double a = 0.0) { return V3{{a, a, a}}; }
+inline void pmv(V3& v, double s, const V3& x) { for (int i = 0; i < 3; ++i) v[i] += s * x[i]; }
+inline void addTo(V3& v, const V3& x) { for (int i = 0; i < 3; ++i) v[i] += x[i]; }
+inline void subFrom(V3& v, const V3& x) { for (int i = 0; i < 3; ++i) v[i] -= x[i]; }
+inline void scaleBy(V3& v, double s) { for (int i = 0; i < 3; ++i) v[i] *= s; }
+inline double norm2(const V3& v) { return v[0] * v[0] + v[1] * v[1] + v[2] * v[2]; }
+inline double dot(const V3& a, const V3& b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline V3 cross(const V3& l, const V3& r) {
+  V3 x = v3();
+  x[0] = l[1] * r[2] - l[2] * r[1];
+  x[1] = l[2] * r[0] - l[0] * r[2];
+  x[2] = l[0] * r[1] - l[1] * r[0];
+  return x;
+}
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+
+// BoostFrameTransform::gammaFromProperVelocity, boostframe.cpp:74-87.
+inline double gammaFromU(const V3& u) {
+  const double scale = std::max(1.0, std::max(std::abs(u[0]), std::max(std::abs(u[1]), std::abs(u[2]))));
+  const double inv = 1.0 / scale;
+  const double ux = u[0] * inv, uy = u[1] * inv, uz = u[2] * inv;
+  return scale * std::sqrt(inv * inv + ux * ux + uy * uy + uz * uz);
+}
+
+struct Boost {
+  double gamma = 1.0, beta = 0.0, gammaBeta = 0.0, c0 = 1.0, labT0 = 0.0, labZ0 = 0.0, boxZ0 = 0.0;
+  static double gammaBetaFromGamma(double g) {
+    if (!(g >= 1.0) || !std::isfinite(g)) throw std::invalid_argument("Lorentz gamma must be finite and >= 1");
+    return std::sqrt(g - 1.0) * std::sqrt(g + 1.0);
+  }
+  // setOriginsFromGamma, boostframe.cpp:42-60.
+  void setOriginsFromGamma(double g, double c, double tLab, double zLab, double zBox) {
+    const double b = gammaBetaFromGamma(g) / g;
+    if (!std::isfinite(b) || std::abs(b) >= 1.0) throw std::invalid_argument("Lorentz beta must be finite and |beta| < 1");
+    if (!(c > 0.0) || !std::isfinite(c)) throw std::invalid_argument("Transform light speed must be positive");
+    gamma = g; beta = b; gammaBeta = g * b; c0 = c;
+    gammaBeta = gammaBetaFromGamma(g);
+    if (!std::isfinite(tLab) || !std::isfinite(zLab) || !std::isfinite(zBox))
+      throw std::invalid_argument("Lorentz origins must be finite");
+    labT0 = tLab; labZ0 = zLab; boxZ0 = zBox;
+  }
+  // boxToLab, boostframe.cpp:188-196.
+  void boxToLab(double tBox, double zBox, double& tLab, double& zLab) const {
+    const double zShift = zBox - boxZ0;
+    zLab = labZ0 + std::fma(gammaBeta * c0, tBox, gamma * zShift);
+    tLab = labT0 + std::fma(gammaBeta / c0, zShift, gamma * tBox);
+  }
+  double labZFromBoxZT(double zBox, double tBox) const {
+    return labZ0 + std::fma(gammaBeta * c0, tBox, gamma * (zBox - boxZ0));
+  }
+  // fieldsLabToBox, boostframe.cpp:248-265.
+  void fieldsLabToBox(const V3& eL, const V3& bL, V3& eB, V3& bB) const {
+    eB[0] = std::fma(-gammaBeta * c0, bL[1], gamma * eL[0]);
+    eB[1] = std::fma(gammaBeta * c0, bL[0], gamma * eL[1]);
+    eB[2] = eL[2];
+    bB[0] = std::fma(gammaBeta / c0, eL[1], gamma * bL[0]);
+    bB[1] = std::fma(-gammaBeta / c0, eL[0], gamma * bL[1]);
+    bB[2] = bL[2];
+  }
+};
+
+enum MagnetType { kUndulator = 0, kDipole = 1, kQuadrupole = 2 };
+
+// SIMagneticElement, eb_sources.cpp:320-446.
+struct Magnet {
+  int type = kUndulator;
+  V3 center = v3();
+  double length = 0, period = 0, peak = 0, angle = 0;
+  bool gaussianFringe = true;
+  double fringeCutoff = 1.0e-9, gradient = 0, endRampPeriods = 0, phase = 0;
+  double waveNumber = 0, fringeExtent = 0, cosA = 1, sinA = 0;
+
+  void prepare() {
+    for (double c : center)
+      if (!std::isfinite(c)) throw std::invalid_argument("Invalid SI magnetic element");
+    if (!(length >= 0.0) || !std::isfinite(length) || !std::isfinite(peak) || !std::isfinite(angle))
+      throw std::invalid_argument("Invalid SI magnetic element");
+    if (type == kUndulator && (!(period > 0.0) || !std::isfinite(period)))
+      throw std::invalid_argument("Undulator period must be positive");
+    if (type == kUndulator && gaussianFringe &&
+        (!(fringeCutoff > 0.0) || !(fringeCutoff < 1.0) || !std::isfinite(fringeCutoff)))
+      throw std::invalid_argument("Undulator fringe_relative_cutoff must be in (0,1)");
+    if (type == kDipole && !(length > 0.0)) throw std::invalid_argument("Dipole length must be positive");
+    if (type == kQuadrupole && (!(length > 0.0) || !std::isfinite(gradient)))
+      throw std::invalid_argument("Quadrupole length/gradient is invalid");
+    if (!std::isfinite(endRampPeriods) || endRampPeriods < 0.0 ||
+        (endRampPeriods > 0.0 && (type != kUndulator || gaussianFringe || 2.0 * endRampPeriods * period > length)))
+      throw std::invalid_argument("Internal undulator end ramp requires non-Gaussian ends and fits inside length");
+    if (!std::isfinite(phase) || (phase != 0.0 && (type != kUndulator || endRampPeriods <= 0.0)))
+      throw std::invalid_argument("Undulator phase requires internal end ramps");
+    const double twoPi = 2.0 * kPi;
+    waveNumber = type == kUndulator ? twoPi / period : 0.0;
+    fringeExtent = type == kUndulator && gaussianFringe ? std::sqrt(-2.0 * std::log(fringeCutoff)) / waveNumber : 0.0;
+    cosA = std::cos(angle);
+    sinA = std::sin(angle);
+  }
+
+  void fieldsLab(const V3& pos, V3& e, V3& b) const {
+    e = v3();
+    b = v3();
+    V3 rel = pos;
+    subFrom(rel, center);
+    const double z = rel[2];
+    if (type == kQuadrupole) {
+      if (z >= 0.0 && z <= length) {
+        const double x = rel[0] * cosA + rel[1] * sinA;
+        const double y = -rel[0] * sinA + rel[1] * cosA;
+        const double bx = gradient * y, by = gradient * x;
+        b[0] = bx * cosA - by * sinA;
+        b[1] = bx * sinA + by * cosA;
+      }
+      return;
+    }
+    if (type == kDipole) {
+      if (z >= 0.0 && z <= length) {
+        b[0] = peak * cosA;
+        b[1] = peak * sinA;
+      }
+      return;
+    }
+    const double twoPi = 2.0 * kPi;
+    double ts = 0.0, ls = 0.0;
+    if (z >= 0.0 && z <= length) {
+      ts = std::sin(waveNumber * z);
+      ls = std::cos(waveNumber * z);
+      if (phase != 0.0) {
+        ts = std::sin(waveNumber * z + phase);
+        ls = std::cos(waveNumber * z + phase);
+      }
+      if (endRampPeriods > 0.0) {
+        const double ramp = endRampPeriods * period;
+        const double distance = std::min(z, length - z);
+        if (distance < ramp) {
+          const double ph = 0.5 * twoPi * distance / ramp;
+          const double envelope = 0.5 * (1.0 - std::cos(ph));
+          const double derivative = 0.25 * twoPi / ramp * std::sin(ph) * (z < ramp ? 1.0 : -1.0);
+          ts = envelope * ts - derivative / waveNumber * ls;
+          ls *= envelope;
+        }
+      }
+    } else if (gaussianFringe) {
+      const double fc = z < 0.0 ? z : z - length;
+      const double distance = std::abs(fc);
+      if (distance >= fringeExtent) return;
+      const double q = distance / fringeExtent;
+      const double q2 = q * q, q3 = q2 * q, q4 = q3 * q, q5 = q4 * q;
+      const double taper = 1.0 - 10.0 * q3 + 15.0 * q4 - 6.0 * q5;
+      const double taperD = (-30.0 * q2 + 60.0 * q3 - 30.0 * q4) / fringeExtent;
+      const double gaussian = std::exp(-0.5 * std::pow(waveNumber * distance, 2.0));
+      ls = gaussian * taper;
+      const double dd = gaussian * (taperD - waveNumber * waveNumber * distance * taper);
+      ts = z < 0.0 ? dd / waveNumber : -dd / waveNumber;
+    } else {
+      return;
+    }
+    const double tc = rel[0] * cosA + rel[1] * sinA;
+    const double tf = peak * std::cosh(waveNumber * tc) * ts;
+    b[0] = tf * cosA;
+    b[1] = tf * sinA;
+    b[2] = peak * std::sinh(waveNumber * tc) * ls;
+  }
+};
+
+// SIFieldSourceSet::addPrescribedBox with magnets only, eb_sources.cpp:539-582.
+struct Sources {
+  std::vector<Magnet> magnets;
+  void addPrescribedBox(const V3& posBox, double tBox, const Boost& frame, V3& eBox, V3& bBox) const {
+    if (magnets.empty()) return;
+    V3 posLab = posBox;
+    double tLab = 0.0;
+    frame.boxToLab(tBox, posBox[2], tLab, posLab[2]);
+    V3 eL = v3(), bL = v3();
+    for (const Magnet& m : magnets) {
+      V3 ce, cb;
+      m.fieldsLab(posLab, ce, cb);
+      addTo(eL, ce);
+      addTo(bL, cb);
+    }
+    V3 ex = v3(), bx = v3();
+    frame.fieldsLabToBox(eL, bL, ex, bx);
+    addTo(eBox, ex);
+    addTo(bBox, bx);
+  }
+};
+
+enum Shape { kQuintic = 0, kNotch = 1, kBinomial3 = 2 };
+
+// transverse_source_shape::nodeWeights, transverse_source_shape.h:8-10.
+inline std::array<double, 4> nodeWeights(double f) {
+  return {{0.25 * (1. - f), 0.5 - 0.25 * f, 0.25 + 0.25 * f, 0.25 * f}};
+}
+
+struct Basis { long long first; unsigned count; double v[12]; };
+
+// longitudinal_shape::basis, longitudinal_shape.h:25-55.
+inline Basis basis(double z, bool edge, int shape) {
+  Basis r;
+  std::fill(r.v, r.v + 12, 0.0);
+  if (shape == kBinomial3) {
+    const double cell = std::floor(z), f = z - cell;
+    r.first = static_cast<long long>(cell) - 1;
+    r.count = edge ? 3 : 4;
+    const double c[3] = {.25, .5, .25};
+    for (int k = -1; k <= 1; ++k) {
+      const double w = c[k + 1];
+      if (edge) r.v[k + 1] += w;
+      else { r.v[k + 1] += w * (1. - f); r.v[k + 2] += w * f; }
+    }
+    return r;
+  }
+  if (shape == kNotch) {
+    const double cell = std::floor(z), f = z - cell;
+    r.first = static_cast<long long>(cell) - 3;
+    r.count = edge ? 7 : 8;
+    const double c[7] = {1. / 64., -6. / 64., 15. / 64., 44. / 64., 15. / 64., -6. / 64., 1. / 64.};
+    for (int k = -3; k <= 3; ++k) {
+      const double w = c[k + 3];
+      if (edge) r.v[k + 3] += w;
+      else { r.v[k + 3] += w * (1. - f); r.v[k + 4] += w * f; }
+    }
+    return r;
+  }
+  const unsigned degree = edge ? 4 : 5;
+  const double cell = std::floor(z), f = z - cell;
+  double n[6] = {1., 0., 0., 0., 0., 0.};
+  for (unsigned j = 1; j <= degree; ++j) {
+    double saved = 0.;
+    for (unsigned k = 0; k < j; ++k) {
+      const double t = n[k] / j;
+      n[k] = saved + (k + 1 - f) * t;
+      saved = (f + j - k - 1) * t;
+    }
+    n[j] = saved;
+  }
+  const double p = 5., a = p / 6., b = 3 * p / 40. + p * (p - 1) / 72.,
+               d = 5 * p / 112. + p * (p - 1) / 80. + p * (p - 1) * (p - 2) / 1296.;
+  const double c[7] = {-d / 64., b / 16. + 3 * d / 32., -a / 4. - b / 4. - 15 * d / 64., 1 + a / 2. + 3 * b / 8. + 5 * d / 16.,
+                       -a / 4. - b / 4. - 15 * d / 64., b / 16. + 3 * d / 32., -d / 64.};
+  r.first = static_cast<long long>(cell) - 5;
+  r.count = degree + 7;
+  for (unsigned j = 0; j <= degree; ++j)
+    for (unsigned k = 0; k < 7; ++k) r.v[j + k] += n[j] * c[k];
+  return r;
+}
+
+// binomialXZSegment, longitudinal_shape.h:61-94.
+template <class Visit>
+inline void binomialXZSegment(const Grid& grid, const double start[3], const double delta[3], double first,
+                              double last, Visit& visit) {
+  const double width = last - first;
+  const double mid = (first + last) * 0.5;
+  const double x = start[0] + mid * delta[0], y = start[1] + mid * delta[1], z = start[2] + mid * delta[2];
+  long long i = static_cast<long long>(std::floor(x)), j = static_cast<long long>(std::floor(y));
+  const long long k = static_cast<long long>(std::floor(z));
+  if (i == static_cast<long long>(grid.nx)) --i;
+  if (j == static_cast<long long>(grid.ny)) --j;
+  const auto wx = nodeWeights(x - i);
+  const auto wz = nodeWeights(z - k);
+  const double wy[2] = {1. - (y - j), y - j};
+  const double edge[3] = {0.25, 0.5, 0.25};
+  const double slope[4] = {-0.25, -0.25, 0.25, 0.25};
+  const double sy[2] = {-1., 1.};
+  const double curvature = width * width / 12.;
+  const double xy = curvature * delta[0] * delta[1];
+  const double xz = curvature * delta[0] * delta[2];
+  const double yz = curvature * delta[1] * delta[2];
+  for (unsigned c = 0; c < 4; ++c) {
+    for (unsigned b = 0; b < 2; ++b) {
+      const double weight = width * (wy[b] * wz[c] + yz * sy[b] * slope[c]);
+      for (unsigned a = 0; a < 3; ++a) visit(0, i - 1 + a, j + b, k - 1 + c, weight * edge[a]);
+    }
+    for (unsigned a = 0; a < 4; ++a) visit(1, i - 1 + a, j, k - 1 + c, width * (wx[a] * wz[c] + xz * slope[a] * slope[c]));
+  }
+  for (unsigned b = 0; b < 2; ++b)
+    for (unsigned a = 0; a < 4; ++a) {
+      const double weight = width * (wx[a] * wy[b] + xy * slope[a] * sy[b]);
+      for (unsigned c = 0; c < 3; ++c) visit(2, i - 1 + a, j + b, k - 1 + c, weight * edge[c]);
+    }
+}
+
+// longitudinal_shape::weights, longitudinal_shape.h:97-117.
+template <class Visit>
+inline unsigned weights(const Grid& g, const V3& origin, const V3& begin, const V3& end, Visit visit, int shape,
+                        bool smoothX) {
+  const double h[3] = {g.dx, g.dy, g.dz};
+  double x[3], v[3], cuts[64] = {0., 1.};
+  unsigned count = 2;
+  for (unsigned a = 0; a < 3; ++a) {
+    x[a] = (begin[a] - origin[a]) / h[a];
+    v[a] = (end[a] - begin[a]) / h[a];
+    if (v[a] != 0.)
+      for (long long j = static_cast<long long>(std::floor(std::min(x[a], x[a] + v[a]))) + 1;
+           j < std::max(x[a], x[a] + v[a]); ++j) {
+        if (count == 64) throw std::runtime_error("Quintic orbit exceeds crossing budget");
+        cuts[count++] = (j - x[a]) / v[a];
+      }
+  }
+  std::sort(cuts, cuts + count);
+  const double q[4] = {-.8611363115940525752, -.3399810435848562648, .3399810435848562648, .8611363115940525752};
+  const double qw[4] = {.3478548451374538574, .6521451548625461426, .6521451548625461426, .3478548451374538574};
+  unsigned segments = 0;
+  for (unsigned s = 1; s < count; ++s) {
+    const double width = cuts[s] - cuts[s - 1];
+    if (width <= 0.) continue;
+    ++segments;
+    if (smoothX && shape == kBinomial3) {
+      binomialXZSegment(g, x, v, cuts[s - 1], cuts[s], visit);
+      continue;
+    }
+    const unsigned points = (shape == kNotch || shape == kBinomial3) ? 2 : 4;
+    for (unsigned t = 0; t < points; ++t) {
+      const double node = points == 2 ? (t ? 0.57735026918962576451 : -0.57735026918962576451) : q[t];
+      const double weight = points == 2 ? 1. : qw[t];
+      const double u = .5 * (cuts[s] + cuts[s - 1] + width * node), factor = .5 * width * weight;
+      const double xx = x[0] + u * v[0], yy = x[1] + u * v[1], zz = x[2] + u * v[2];
+      long long i = static_cast<long long>(std::floor(xx)), j = static_cast<long long>(std::floor(yy));
+      if (i == static_cast<long long>(g.nx)) --i;
+      if (j == static_cast<long long>(g.ny)) --j;
+      const double fx = xx - i, fy = yy - j;
+      const Basis nodes = basis(zz, false, shape), edge = basis(zz, true, shape);
+      for (unsigned k = 0; k < nodes.count; ++k) {
+        const double w = factor * nodes.v[k];
+        if (w == 0.) continue;
+        for (unsigned b = 0; b < 2; ++b) visit(0, i, j + b, nodes.first + k, w * (b ? fy : 1. - fy));
+        for (unsigned a = 0; a < 2; ++a) visit(1, i + a, j, nodes.first + k, w * (a ? fx : 1. - fx));
+      }
+      for (unsigned k = 0; k < edge.count; ++k) {
+        const double w = factor * edge.v[k];
+        if (w == 0.) continue;
+        for (unsigned b = 0; b < 2; ++b)
+          for (unsigned a = 0; a < 2; ++a) visit(2, i + a, j + b, edge.first + k, w * (a ? fx : 1. - fx) * (b ? fy : 1. - fy));
+      }
+    }
+  }
+  return segments;
+}
+
+// clipOpenOrbit for one rank, source_coupling_math.h:234-250.
+struct OpenOrbit { V3 begin, end; double first = 0., last = 1.; bool valid = true; };
+inline OpenOrbit clipOpenOrbit(const Grid& g, const V3& origin, const V3& begin, const V3& end) {
+  OpenOrbit r;
+  r.begin = begin;
+  r.end = end;
+  const double ds[3] = {g.dx, g.dy, g.dz};
+  const std::size_t size[3] = {g.nx, g.ny, g.nz};
+  for (unsigned a = 0; a < 3; ++a) {
+    const double x = (begin[a] - origin[a]) / ds[a], d = (end[a] - begin[a]) / ds[a];
+    if (!std::isfinite(x) || !std::isfinite(d)) throw std::runtime_error("Nonfinite open coupling orbit");
+    if (d == 0.) {
+      if (x < 0. || x > size[a]) r.valid = false;
+      continue;
+    }
+    {
+      const double t = -x / d;
+      if (d > 0.) r.first = std::max(r.first, t);
+      else r.last = std::min(r.last, t);
+    }
+    {
+      const double t = (double(size[a]) - x) / d;
+      if (d > 0.) r.last = std::min(r.last, t);
+      else r.first = std::max(r.first, t);
+    }
+  }
+  r.valid = r.valid && r.last >= r.first;
+  if (r.valid)
+    for (unsigned a = 0; a < 3; ++a) {
+      const double d = end[a] - begin[a];
+      if (r.first != 0.) r.begin[a] = begin[a] + r.first * d;
+      if (r.last != 1.) r.end[a] = begin[a] + r.last * d;
+    }
+  return r;
+}
+
+// NativeHalo::openMagneticValue on one rank: constant end continuation,
+// source_coupling_math.h:113-119.
+inline double openB(const Comp& v, long long i, long long j, long long k) {
+  i = std::max(0LL, std::min(i, static_cast<long long>(v.nx) - 1));
+  j = std::max(0LL, std::min(j, static_cast<long long>(v.ny) - 1));
+  k = std::max(0LL, std::min(k, static_cast<long long>(v.nz) - 1));
+  return v(i, j, k);
+}
+
+// magneticSample with openFaces and no bounds record, source_coupling_math.h:122-138.
+inline V3 magneticSampleOpen(const Fields& f, const V3& origin, const V3& position) {
+  V3 value = v3();
+  const Grid& g = f.g;
+  const double ds[3] = {g.dx, g.dy, g.dz};
+  int cell[2][3];
+  double fraction[2][3];
+  for (unsigned half = 0; half < 2; ++half)
+    for (unsigned d = 0; d < 3; ++d) {
+      const double x = (position[d] - origin[d]) / ds[d] - .5 * half;
+      cell[half][d] = static_cast<int>(std::floor(x));
+      fraction[half][d] = x - cell[half][d];
+    }
+  for (unsigned axis = 0; axis < 3; ++axis) {
+    const unsigned hx = axis == 0 ? 0 : 1, hy = axis == 1 ? 0 : 1, hz = axis == 2 ? 0 : 1;
+    for (unsigned k = 0; k < 2; ++k)
+      for (unsigned j = 0; j < 2; ++j)
+        for (unsigned i = 0; i < 2; ++i) {
+          const double x = i ? fraction[hx][0] : 1 - fraction[hx][0];
+          const double y = j ? fraction[hy][1] : 1 - fraction[hy][1];
+          const double z = k ? fraction[hz][2] : 1 - fraction[hz][2];
+          const double b = openB(f.f[3 + axis], cell[hx][0] + int(i), cell[hy][1] + int(j), cell[hz][2] + int(k));
+          value[axis] += x * y * z * b;
+        }
+  }
+  return value;
+}
+
+// binomialXZMagneticSample, source_coupling_math.h:142-167.
+inline V3 binomialXZMagneticSample(const Fields& f, const V3& origin, const V3& point) {
+  const Grid& g = f.g;
+  const double h[3] = {g.dx, g.dy, g.dz};
+  V3 value = v3();
+  for (unsigned a = 0; a < 3; ++a) {
+    long long cell[3];
+    double fraction[3];
+    for (unsigned d = 0; d < 3; ++d) {
+      const double pos = (point[d] - origin[d]) / h[d] - (a == d ? 0. : 0.5);
+      cell[d] = static_cast<long long>(std::floor(pos));
+      fraction[d] = pos - cell[d];
+    }
+    const auto wx = nodeWeights(fraction[0]);
+    const auto wz = nodeWeights(fraction[2]);
+    const double wy[2] = {1. - fraction[1], fraction[1]};
+    for (unsigned k = 0; k < 4; ++k)
+      for (unsigned j = 0; j < 2; ++j) {
+        double row = 0.;
+        for (unsigned i = 0; i < 4; ++i) row += wx[i] * openB(f.f[3 + a], cell[0] - 1 + i, cell[1] + j, cell[2] - 1 + k);
+        value[a] += wz[k] * wy[j] * row;
+      }
+  }
+  return value;
+}
+
+// source_coupling_math::momentum, source_coupling_math.h:272-293.
+inline unsigned momentum(V3& u, const V3& e, const V3& magnetic, double q, double m, double h, double knownGamma) {
+  const V3 before = u;
+  const double g0 = knownGamma > 0. ? knownGamma : gammaFromU(before);
+  V3 kick = e;
+  scaleBy(kick, q * h / (2 * m * kC));
+  V3 b = magnetic;
+  scaleBy(b, q * h / (2 * m));
+  V3 a = before;
+  addTo(a, kick);
+  const V3 axb = cross(a, b);
+  const double b2 = dot(b, b), ka = dot(kick, a), kb = dot(kick, b), ab = dot(a, b), kcross = dot(kick, axb);
+  const double first = dot(kick, before) / g0;
+  double G = g0 + first + (dot(kick, kick) + kcross / g0 - first * first) / g0;
+  G = std::max(G, .5 * (g0 + 1.));
+  unsigned iteration = 0;
+  for (; iteration < 12; ++iteration) {
+    const double den = G * G + b2, numerator = G * G * ka + G * kcross + ab * kb;
+    const double fv = G * (G - g0) - numerator / den;
+    const double derivative = 2 * G - g0 - ((2 * G * ka + kcross) * den - 2 * G * numerator) / (den * den);
+    const double correction = fv / derivative;
+    const double next = G - correction;
+    if (!std::isfinite(next) || next < .5 * (g0 + 1.) - 1e-13) throw std::runtime_error("Nonphysical local mean-gamma root");
+    G = std::max(next, .5 * (g0 + 1.));
+    if (std::abs(correction) <= 8 * kEpsilon * std::max(1., G)) break;
+  }
+  if (iteration == 12) throw std::runtime_error("Local mean-gamma root did not converge");
+  const double den = G * G + b2;
+  for (unsigned k = 0; k < 3; ++k) u[k] = 2 * (G * G * a[k] + ab * b[k] + G * axb[k]) / den - before[k];
+  return iteration + 1;
+}
+
+inline long double chargeWork(double q, const V3& e, const V3& x, const V3& y) {
+  long double s = 0;
+  for (unsigned a = 0; a < 3; ++a) s += (long double)q * e[a] * ((long double)y[a] - x[a]);
+  return s;
+}
+
+struct Particle {
+  V3 position = v3(), u = v3();
+  double charge = 0, mass = 0, weight = 1;
+  std::uint64_t id = 0;
+};
+
+// ConservativeSourceCoupling (source_coupling.cpp) for one rank, with the
+// longitudinal_shape::Current of depth 8 (production, compact=false).
+class Coupling {
+ public:
+  Fields* fields = nullptr;
+  int shape = kBinomial3;
+  bool smoothX = true;
+  V3 origin = v3();
+  Comp old[3];
+  Sources sources;
+  Boost frame;
+  bool replay = false;
+
+  struct Slot {
+    Particle first, last;
+    V3 guess = v3(), usedE = v3(), usedB = v3(), nextE = v3(), nextB = v3();
+    bool seen = false, initialized = false, replayed = false, nextReady = false;
+    unsigned pushes = 0;
+    double time = 0., dt = 0.;
+    std::vector<Particle> states;
+    std::vector<long double> gridWork;
+  };
+  std::unordered_map<std::uint64_t, std::size_t> index;
+  std::vector<Slot> slots;
+
+  bool wide() const { return true; }
+
+  // begin, source_coupling.cpp:114.
+  void begin(Fields& f, const V3& o, const std::vector<Particle>& p) {
+    fields = &f;
+    origin = o;
+    for (int a = 0; a < 3; ++a) old[a] = f.f[a];
+    index.clear();
+    slots.clear();
+    slots.reserve(p.size() + 32);
+    for (const auto& q : p) {
+      index.emplace(q.id, slots.size());
+      slots.emplace_back();
+    }
+    replay = false;
+  }
+  Slot& slot(const Particle& p) {
+    auto it = index.find(p.id);
+    if (it != index.end()) return slots[it->second];
+    index.emplace(p.id, slots.size());
+    slots.emplace_back();
+    return slots.back();
+  }
+  void beginTrial() { for (auto& q : slots) q.seen = false; }
+
+  // Impl::electric (wide branch), source_coupling.cpp:74-82.
+  V3 electric(const V3& a, const V3& b) {
+    const Grid& g = fields->g;
+    const OpenOrbit path = clipOpenOrbit(g, origin, a, b);
+    if (!path.valid) throw std::out_of_range("Quintic active orbit outside domain");
+    V3 result = v3();
+    weights(g, origin, path.begin, path.end, [&](unsigned d, long long i, long long j, long long k, double w) {
+      const Comp& v = fields->f[d];
+      if (i < 0 || j < 0 || i >= (long long)v.nx || j >= (long long)v.ny) return;
+      if (k < 0 || k >= (long long)v.nz) return;
+      result[d] += .5 * w * (v(i, j, k) + old[d](i, j, k));
+    }, shape, smoothX);
+    scaleBy(result, path.last - path.first);
+    return result;
+  }
+  // Impl::magnetic, source_coupling.cpp:83-94.
+  V3 magnetic(const V3& a, const V3& b) {
+    const Grid& g = fields->g;
+    if (shape == kQuintic) {
+      const OpenOrbit r = clipOpenOrbit(g, origin, a, b);
+      if (!r.valid) throw std::out_of_range("Active coupling magnetic orbit has no in-domain portion");
+      V3 middle = r.begin;
+      addTo(middle, r.end);
+      scaleBy(middle, .5);
+      return magneticSampleOpen(*fields, origin, middle);
+    }
+    const OpenOrbit path = clipOpenOrbit(g, origin, a, b);
+    if (!path.valid) throw std::out_of_range("Notch magnetic orbit outside domain");
+    V3 middle = path.begin;
+    addTo(middle, path.end);
+    scaleBy(middle, .5);
+    if (smoothX) return binomialXZMagneticSample(*fields, origin, middle);
+    V3 result = v3();
+    for (int k = -1; k <= 1; ++k) {
+      V3 point = middle;
+      point[2] += k * g.dz;
+      const V3 value = magneticSampleOpen(*fields, origin, point);
+      pmv(result, k == 0 ? .5 : .25, value);
+    }
+    return result;
+  }
+
+  // applyCurrentSource, source_coupling.cpp:127-132 (one rank).
+  void applyCurrentSource(double sign) {
+    const Grid& g = fields->g;
+    for (unsigned a = 0; a < 3; ++a) {
+      Comp& e = fields->f[a];
+      const Comp& cur = fields->f[6 + a];
+      for (std::size_t k = 0; k < e.nz; ++k)
+        for (std::size_t y = 0; y < e.ny; ++y)
+          for (std::size_t x = 0; x < e.nx; ++x) {
+            const bool applied = a == 0 ? (y > 0 && y < g.ny && k != 0 && k != g.nz)
+                                        : (a == 1 ? (x > 0 && x < g.nx && k != 0 && k != g.nz)
+                                                  : (x > 0 && x < g.nx && y > 0 && y < g.ny));
+            if (applied) {
+              const std::size_t n = (k * e.ny + y) * e.nx + x;
+              if (cur.v[n] != 0.) e.v[n] += sign * g.dt / kEps0 * cur.v[n];
+            }
+          }
+    }
+  }
+
+  void swapElectric() { for (int a = 0; a < 3; ++a) std::swap(fields->f[a].v, old[a].v); }
+
+  // push, source_coupling.cpp:133-147 (no observers or probes).
+  void push(Particle& p, double time, double dt, unsigned steps) {
+    Slot& q = slot(p);
+    if (replay) {
+      if (!q.seen || q.time != time || q.dt != dt || q.states.size() != steps + 1)
+        throw std::logic_error("Missing converged particle replay");
+      for (unsigned a = 0; a < 3; ++a)
+        if (q.first.position[a] != p.position[a] || q.first.u[a] != p.u[a])
+          throw std::logic_error("Release state changed during coupled replay");
+      q.replayed = true;
+      p = q.last;
+      return;
+    }
+    if (!q.initialized) {
+      q.guess = p.position;
+      const double gamma = gammaFromU(p.u);
+      pmv(q.guess, dt * kC / gamma, p.u);
+      q.initialized = true;
+    }
+    q.first = p;
+    q.time = time;
+    q.dt = dt;
+    q.seen = true;
+    ++q.pushes;
+    if (q.nextReady) {
+      q.usedE = q.nextE;
+      q.usedB = q.nextB;
+    } else {
+      q.usedE = electric(p.position, q.guess);
+      q.usedB = magnetic(p.position, q.guess);
+    }
+    q.states.clear();
+    q.gridWork.clear();
+    q.states.push_back(p);
+    const double h = dt / steps;
+    for (unsigned n = 0; n < steps; ++n) {
+      const Particle before = p;
+      const double g0 = gammaFromU(p.u);
+      V3 point = p.position;
+      pmv(point, .5 * h * kC / g0, p.u);
+      V3 ee = v3(), bb = v3();
+      sources.addPrescribedBox(point, time + (n + .5) * h, frame, ee, bb);
+      V3 e = q.usedE;
+      addTo(e, ee);
+      V3 b = q.usedB;
+      addTo(b, bb);
+      momentum(p.u, e, b, p.charge, p.mass, h, g0);
+      const double g1 = gammaFromU(p.u);
+      V3 mean = before.u;
+      addTo(mean, p.u);
+      pmv(p.position, h * kC / (g0 + g1), mean);
+      q.gridWork.push_back(chargeWork(p.charge, q.usedE, before.position, p.position));
+      q.states.push_back(p);
+    }
+    q.last = p;
+    q.guess = p.position;
+  }
+
+  // localResidual, source_coupling.cpp:148-202 (serial branch).
+  std::array<double, 4> localResidual() {
+    double force = 0., defect = 0., scale = 0., calls = 0.;
+    for (auto& s : slots) {
+      if (!s.seen) continue;
+      calls += 1;
+      s.nextE = electric(s.first.position, s.last.position);
+      s.nextB = magnetic(s.first.position, s.last.position);
+      s.nextReady = true;
+      V3 de = s.nextE, db = s.nextB;
+      subFrom(de, s.usedE);
+      subFrom(db, s.usedB);
+      force = std::max(force, std::abs(s.first.charge / s.first.mass) * s.dt / kC *
+                                  (std::sqrt(norm2(de)) + kC * std::sqrt(norm2(db))));
+      defect += std::abs(chargeWork(s.first.charge, de, s.first.position, s.last.position));
+      for (const auto& w : s.gridWork) scale += std::abs(w);
+      scale += std::abs(chargeWork(s.first.charge, s.nextE, s.first.position, s.last.position));
+    }
+    return {{force, defect, scale, calls}};
+  }
+
+  // Current::deposit for one rank and depth 8, longitudinal_shape.h:175-187.
+  unsigned deposit(const V3& begin, const V3& end, double charge) {
+    const Grid& g = fields->g;
+    double scale[3];
+    for (unsigned a = 0; a < 3; ++a) scale[a] = charge * (end[a] - begin[a]) / (g.dx * g.dy * g.dz * g.dt);
+    return weights(g, origin, begin, end, [&](unsigned a, long long i, long long j, long long k, double w) {
+      if (scale[a] == 0.) return;
+      Comp& v = fields->f[6 + a];
+      if (i < 0 || i >= (long long)v.nx) {
+        if (smoothX && shape == kBinomial3) return;
+        throw std::out_of_range("Transverse current support outside x grid");
+      }
+      if (j < 0 || j >= (long long)v.ny) throw std::out_of_range("Transverse current support outside y grid");
+      const double value = scale[a] * w;
+      if (k >= 0 && k < (long long)v.nz) v(i, j, k) += value;
+    }, shape, smoothX);
+  }
+
+  // ChargeConservingCurrentDepositor::validatePosition, eb_deposition.cpp:228-247.
+  void validatePosition(const V3& p) const {
+    const Grid& g = fields->g;
+    const double extent[3] = {static_cast<double>(g.nx) * g.dx, static_cast<double>(g.ny) * g.dy,
+                              static_cast<double>(g.nz) * g.dz};
+    for (unsigned a = 0; a < 3; ++a) {
+      const double rel = p[a] - origin[a];
+      const double tol = 64.0 * kEpsilon * std::max(1.0, std::max(std::abs(origin[a]), extent[a]));
+      if (!std::isfinite(p[a]) || rel < -tol || rel > extent[a] + tol)
+        throw std::out_of_range("Particle trajectory endpoint is outside the E/B domain");
+    }
+  }
+};
+
+// Particle-boundary geometry of one step: the transverse CPML inner faces
+// (ParticleCPMLRegion, particle_boundary.cpp:56-174), the outer faces
+// (ParticleOpenBoundary::firstExit, 274-352) and the moving-window lab exit
+// plane (Simulation::firstMovingWindowEntry, simulation_moving_window.cpp:380-395).
+struct ParticleBoundary {
+  double physLower[3], physUpper[3], outerLower[3], outerUpper[3], tol[3];
+  double exitLower[3], exitUpper[3], exitTol[3];  // ParticleOpenBoundary (fma upper face)
+  std::size_t cells[3];
+  double exitLabZ = std::numeric_limits<double>::infinity();
+  bool window = false;
+
+  void set(const Grid& g, const V3& origin, const std::size_t cpmlCells[3]) {
+    const double sp[3] = {g.dx, g.dy, g.dz};
+    const std::size_t n[3] = {g.nx, g.ny, g.nz};
+    for (unsigned a = 0; a < 3; ++a) {
+      cells[a] = cpmlCells[a];
+      outerLower[a] = origin[a];
+      outerUpper[a] = origin[a] + static_cast<double>(n[a]) * sp[a];
+      physLower[a] = outerLower[a] + static_cast<double>(cpmlCells[a]) * sp[a];
+      physUpper[a] = outerUpper[a] - static_cast<double>(cpmlCells[a]) * sp[a];
+      tol[a] = 64.0 * kEpsilon * std::max(1.0, std::max(std::abs(outerLower[a]), std::abs(outerUpper[a])));
+      exitLower[a] = origin[a];
+      exitUpper[a] = std::fma(static_cast<double>(n[a]), sp[a], origin[a]);
+      exitTol[a] = 64.0 * kEpsilon * std::max(1.0, std::max(std::abs(exitLower[a]), std::abs(exitUpper[a])));
+    }
+  }
+  // True when the segment triggers any boundary event (CPML entry, outer
+  // exit or the lab exit plane); throws like the reference on bad starts.
+  bool anyEvent(const V3& start, const V3& end, const Boost& frame, double tStart, double tEnd) const {
+    bool entered = false;
+    for (unsigned a = 0; a < 3; ++a) {
+      if (!std::isfinite(start[a]) || !std::isfinite(end[a]))
+        throw std::runtime_error("Particle CPML trajectory contains a non-finite coordinate");
+      if (start[a] < physLower[a] - tol[a] || start[a] > physUpper[a] + tol[a])
+        throw std::out_of_range("Active particle starts inside the non-physical CPML region");
+      if (cells[a] == 0) continue;
+      if (end[a] < physLower[a] || end[a] > physUpper[a]) entered = true;
+    }
+    for (unsigned a = 0; a < 3; ++a) {
+      if (start[a] < exitLower[a] - exitTol[a] || start[a] > exitUpper[a] + exitTol[a])
+        throw std::out_of_range("Particle trajectory starts outside the global E/B domain");
+      if (end[a] < exitLower[a] || end[a] > exitUpper[a]) entered = true;
+    }
+    if (window) {
+      const double za = frame.labZFromBoxZT(start[2], tStart) - exitLabZ;
+      const double zb = frame.labZFromBoxZT(end[2], tEnd) - exitLabZ;
+      if (za >= 0.) throw std::runtime_error("Active particle started beyond trusted moving-window exit");
+      if (zb >= 0. && zb > za) entered = true;
+    }
+    return entered;
+  }
+};
+
+// Simulation::solveConservativeSourceStep + commitConservativeElectric
+// (simulation_source_coupling.cpp:16-65) with the particle loop of
+// pushDepositAndMigrate (simulation.cpp:4167-5605) for active particles that
+// trigger no boundary event; an event raises runtime_error (unsupported).
+struct StepResult { unsigned iterations = 0; double force = 0, work = 0; };
+
+inline void pushDepositAll(Coupling& c, std::vector<Particle>& particles, double time, double dt, unsigned substeps,
+                           const ParticleBoundary& pb, const V3& windowOrigin) {
+  std::vector<Particle> prepared = particles;
+  for (auto& p : prepared) {
+    if (pb.window && p.position[2] < windowOrigin[2])
+      throw std::runtime_error("An active particle is upstream of the moving window; the inlet guard did not contain its release event");
+    c.push(p, time, dt, substeps);
+  }
+  for (std::size_t n = 0; n < particles.size(); ++n) {
+    const V3 start = particles[n].position;
+    const Particle& p = prepared[n];
+    if (pb.anyEvent(start, p.position, c.frame, time, time + dt))
+      throw std::runtime_error("Particle boundary event (CPML entry, domain exit or window exit) is not supported by this build");
+    c.validatePosition(start);
+    c.validatePosition(p.position);
+    if (!std::isfinite(p.charge)) throw std::invalid_argument("Deposited particle charge must be finite");
+    c.deposit(start, p.position, p.charge);
+  }
+  particles.swap(prepared);
+}
+
+inline StepResult sourceStep(Fields& f, Coupling& c, std::vector<Particle>& particles, const V3& origin, double time,
+                             unsigned substeps, const ParticleBoundary& pb) {
+  c.begin(f, origin, particles);
+  f.window->electricPreview = true;
+  f.advanceElectric();
+  f.window->electricPreview = false;
+  const std::vector<Particle> original = particles;
+  bool converged = false;
+  StepResult r;
+  const unsigned maximum = 64;
+  for (unsigned n = 0; n < maximum; ++n) {
+    if (n) c.applyCurrentSource(1.);
+    f.clearCurrent();
+    particles = original;
+    c.beginTrial();
+    pushDepositAll(c, particles, time, f.g.dt, substeps, pb, origin);
+    c.applyCurrentSource(-1.);
+    const auto res = c.localResidual();
+    r.force = res[0];
+    const double total[3] = {res[1], res[2], res[3]};
+    r.work = total[2] == 0. ? 0. : (total[1] > 0. ? total[0] / total[1] : total[0]);
+    r.iterations = n + 1;
+    if (r.iterations >= (total[2] > 0. ? 2u : 1u) && r.force < 1e-13 && r.work < 1e-8) {
+      converged = true;
+      break;
+    }
+  }
+  if (!converged) throw std::runtime_error("Conservative source coupling failed to converge");
+  c.applyCurrentSource(1.);
+  f.clearCurrent();
+  particles = original;
+  c.swapElectric();
+  c.replay = true;
+  pushDepositAll(c, particles, time, f.g.dt, substeps, pb, origin);
+  c.replay = false;
+  // commitConservativeElectric
+  c.swapElectric();
+  f.undoPreview();
+  c.applyCurrentSource(-1.);
+  if (f.window->electricPreview || f.window->previewPresent) throw std::logic_error("Electric preview not restored before commit");
+  f.afterElectricBoundary();
+  return r;
+}
+
+}  // namespace orc

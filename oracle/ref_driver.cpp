@@ -220,3 +220,263 @@ double ref_fields_poynting(void* h, std::size_t i, std::size_t j, std::size_t k,
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Particle side: the reference's ConservativeSourceCoupling, shapes and
+// sources driven by a restatement of simulation_source_coupling.cpp:16-65 and
+// the active-particle branch of simulation.cpp:4167-5605.
+#include <cstdlib>
+#include <unistd.h>
+
+#include "boostframe.h"
+#include "eb_deposition.h"
+#include "eb_particles.h"
+#include "eb_sources.h"
+#include "longitudinal_shape.h"
+#include "particle_boundary.h"
+#include "source_coupling.h"
+#include "source_coupling_math.h"
+
+namespace {
+struct RefCoupling {
+  RefFields* fields = nullptr;
+  std::unique_ptr<ConservativeSourceCoupling> c;
+  SIFieldSourceSet sources;
+  BoostFrameTransform frame;
+  std::vector<RelativisticParticleSI> particles;
+  std::string dir;
+  bool quintic = false, binomial3 = false, smoothX = false;
+  EBCPMLParameters cpml;
+  ~RefCoupling() {
+    c.reset();
+    if (!dir.empty()) {
+      std::string cmd = "rm -rf '" + dir + "'";
+      if (std::system(cmd.c_str())) {}
+    }
+  }
+};
+RefCoupling* RC(void* h) { return static_cast<RefCoupling*>(h); }
+FieldVector<Double> fv(const double* p) {
+  FieldVector<Double> v(0.0);
+  for (int a = 0; a < 3; ++a) v[a] = p[a];
+  return v;
+}
+}  // namespace
+
+extern "C" {
+
+int ref_coupling_new(void* fields, int shape, int smoothX, void** out) {
+  return guarded([&] {
+    std::unique_ptr<RefCoupling> h(new RefCoupling);
+    h->fields = R(fields);
+    h->quintic = shape == 0;
+    h->binomial3 = shape == 2;
+    h->smoothX = smoothX != 0;
+    char tmpl[] = "/tmp/aprl_ref_XXXXXX";
+    if (!mkdtemp(tmpl)) throw std::runtime_error("Cannot create a coupling log directory");
+    h->dir = tmpl;
+    h->c.reset(new ConservativeSourceCoupling(MPI_COMM_SELF, h->dir, h->quintic, shape == 1, h->binomial3, false, false,
+                                              h->smoothX));
+    *out = h.release();
+  });
+}
+void ref_coupling_free(void* h) { delete RC(h); }
+
+int ref_coupling_set_frame(void* h, double gamma, double labT0, double labZ0, double boxZ0) {
+  return guarded([&] { RC(h)->frame.setOriginsFromGamma(gamma, SI::c, labT0, labZ0, boxZ0); });
+}
+
+int ref_coupling_add_magnet(void* h, int type, const double* center, double length, double period, double peak,
+                            double angle, int gaussianFringe, double cutoff, double gradient, double endRamp,
+                            double phase) {
+  return guarded([&] {
+    SIMagneticElement m;
+    m.type = type == 0 ? SIMagnetType::PlanarUndulator : (type == 1 ? SIMagnetType::UniformDipole : SIMagnetType::Quadrupole);
+    m.center = fv(center);
+    m.length = length; m.period = period; m.peakMagneticField = peak; m.polarizationAngle = angle;
+    m.gaussianFringe = gaussianFringe != 0; m.fringeRelativeCutoff = cutoff; m.quadrupoleGradient = gradient;
+    m.endRampPeriods = endRamp; m.undulatorPhase = phase;
+    RC(h)->sources.addPrescribedLabMagnet(m);
+  });
+}
+
+int ref_coupling_set_cpml_cells(void* h, const std::size_t* cells) {
+  return guarded([&] { for (int a = 0; a < 3; ++a) RC(h)->cpml.cells[a] = cells[a]; });
+}
+
+int ref_coupling_set_particles(void* h, std::size_t n, const double* x3, const double* u3, const double* q,
+                               const double* m, const double* w, const std::uint64_t* id) {
+  return guarded([&] {
+    auto& P = RC(h)->particles;
+    P.assign(n, RelativisticParticleSI());
+    for (std::size_t i = 0; i < n; ++i) {
+      P[i].position = fv(x3 + 3 * i);
+      P[i].properVelocity = fv(u3 + 3 * i);
+      P[i].charge = q[i]; P[i].mass = m[i]; P[i].weight = w[i]; P[i].id = id[i];
+      P[i].frozen = false;
+    }
+  });
+}
+std::size_t ref_coupling_count(void* h) { return RC(h)->particles.size(); }
+int ref_coupling_get_particles(void* h, double* x3, double* u3) {
+  return guarded([&] {
+    const auto& P = RC(h)->particles;
+    for (std::size_t i = 0; i < P.size(); ++i)
+      for (int a = 0; a < 3; ++a) { x3[3 * i + a] = P[i].position[a]; u3[3 * i + a] = P[i].properVelocity[a]; }
+  });
+}
+
+int ref_coupling_step(void* h, const double* origin, double time, unsigned substeps, double exitLabZ,
+                      unsigned* iterations, double* force, double* work) {
+  return guarded([&] {
+    RefCoupling& H = *RC(h);
+    RefFields& rf = *H.fields;
+    EBFieldGrid& f = *rf.f;
+    const EBGridGeometry& g = f.geometry();
+    const FieldVector<Double> o = fv(origin);
+    ParticleCPMLRegion cpmlRegion(g, o, H.cpml);
+    ParticleOpenBoundary outer(g, o, g, o, 0);
+    const bool window = std::isfinite(exitLabZ);
+    auto pushDepositAll = [&]() {
+      ChargeConservingCurrentDepositor depositor(f, o, H.c->longitudinalCurrent());
+      std::vector<RelativisticParticleSI> prepared = H.particles;
+      for (auto& p : prepared) {
+        if (window && p.position[2] < o[2])
+          throw std::runtime_error("An active particle is upstream of the moving window; the inlet guard did not contain its release event");
+        H.c->push(p, H.sources, H.frame, time, g.dt, substeps, nullptr, nullptr, nullptr);
+      }
+      for (std::size_t n = 0; n < H.particles.size(); ++n) {
+        const FieldVector<Double> start(H.particles[n].position);
+        const RelativisticParticleSI& p = prepared[n];
+        ParticleBoundaryHit hit;
+        bool event = cpmlRegion.firstEntry(start, p.position, hit);
+        event = outer.firstExit(start, p.position, hit) || event;
+        if (window) {
+          const double a = H.frame.labZFromBoxZT(start[2], time) - exitLabZ;
+          const double b = H.frame.labZFromBoxZT(p.position[2], time + g.dt) - exitLabZ;
+          if (a >= 0.) throw std::runtime_error("Active particle started beyond trusted moving-window exit");
+          if (b >= 0. && b > a) event = true;
+        }
+        if (event)
+          throw std::runtime_error("Particle boundary event (CPML entry, domain exit or window exit) is not supported by this build");
+        depositor.depositSegment(start, p.position, p.charge);
+      }
+      H.particles.swap(prepared);
+    };
+    ConservativeSourceCoupling& coupling = *H.c;
+    coupling.begin(f, o, H.particles, H.cpml.cells[0], H.cpml.cells[1]);
+    rf.window->previewElectricBoundary(true);
+    f.advanceElectric();
+    rf.window->previewElectricBoundary(false);
+    const auto original = H.particles;
+    bool converged = false;
+    unsigned its = 0;
+    Double frc = 0., wrk = 0.;
+    for (unsigned n = 0; n < 64; ++n) {
+      if (n) coupling.applyCurrentSource(1.);
+      f.clearCurrent();
+      H.particles = original;
+      coupling.beginTrial();
+      coupling.exchangeCandidate();
+      pushDepositAll();
+      coupling.completeCurrent();
+      coupling.applyCurrentSource(-1.);
+      coupling.exchangeCandidate();
+      const auto r = coupling.localResidual();
+      frc = r[0];
+      const double total[3] = {r[1], r[2], r[3]};
+      wrk = total[2] == 0. ? 0. : (total[1] > 0. ? total[0] / total[1] : total[0]);
+      its = n + 1;
+      if (its >= (total[2] > 0. ? 2u : 1u) && frc < 1e-13 && wrk < 1e-8) { converged = true; break; }
+    }
+    if (!converged) throw std::runtime_error("Conservative source coupling failed to converge");
+    coupling.applyCurrentSource(1.);
+    f.clearCurrent();
+    H.particles = original;
+    coupling.swapElectric();
+    coupling.setReplay(true);
+    pushDepositAll();
+    coupling.completeCurrent();
+    coupling.setReplay(false);
+    coupling.finish(rf.step + 1, its, frc, wrk);
+    coupling.swapElectric();
+    rf.window->undoPreviewElectricBoundary(f);
+    coupling.applyCurrentSource(-1.);
+    rf.window->commitPreparedElectricBoundary(f);
+    *iterations = its; *force = frc; *work = wrk;
+  });
+}
+
+int ref_shape_weights(int shape, int smoothX, const double* geometry7, const double* origin, const double* begin,
+                      const double* end, int* axis, long long* ijk, double* w, std::size_t capacity,
+                      std::size_t* count, unsigned* segments) {
+  return guarded([&] {
+    EBGridGeometry g((std::size_t)geometry7[0], (std::size_t)geometry7[1], (std::size_t)geometry7[2], geometry7[3],
+                     geometry7[4], geometry7[5], geometry7[6], EBMaxwellSolver::CowanZ);
+    std::size_t n = 0;
+    const auto sh = shape == 0 ? longitudinal_shape::Shape::QuinticCompensated
+                               : (shape == 1 ? longitudinal_shape::Shape::LinearNotch : longitudinal_shape::Shape::LinearBinomial3);
+    *segments = longitudinal_shape::weights(g, fv(origin), fv(begin), fv(end), [&](unsigned a, long long i, long long j, long long k, double v) {
+      if (n >= capacity) throw std::length_error("weights capacity");
+      axis[n] = (int)a; ijk[3 * n] = i; ijk[3 * n + 1] = j; ijk[3 * n + 2] = k; w[n] = v; ++n;
+    }, sh, smoothX != 0);
+    *count = n;
+  });
+}
+
+int ref_momentum(double* u, const double* e, const double* b, double q, double m, double h, double g0,
+                 unsigned* iterations) {
+  return guarded([&] {
+    FieldVector<Double> uu = fv(u);
+    *iterations = source_coupling_math::momentum(uu, fv(e), fv(b), q, m, h, g0);
+    for (int a = 0; a < 3; ++a) u[a] = uu[a];
+  });
+}
+
+int ref_coupling_begin_fields(void* h, const double* origin) {
+  return guarded([&] {
+    RefCoupling& H = *RC(h);
+    H.c->begin(*H.fields->f, fv(origin), H.particles, H.cpml.cells[0], H.cpml.cells[1]);
+    H.c->beginTrial();
+    H.c->exchangeCandidate();
+  });
+}
+
+int ref_prescribed_box(void* h, const double* pos, double t, double* e, double* b) {
+  return guarded([&] {
+    FieldVector<Double> ee(0.0), bb(0.0);
+    RC(h)->sources.addPrescribedBox(fv(pos), t, RC(h)->frame, ee, bb);
+    for (int k = 0; k < 3; ++k) { e[k] = ee[k]; b[k] = bb[k]; }
+  });
+}
+
+// Direct shape-level deposit with longitudinal_shape::Current (one rank).
+int ref_current_deposit(void* fields, int shape, int smoothX, const double* origin, const double* a, const double* b,
+                        double charge) {
+  return guarded([&] {
+    EBFieldGrid& f = *R(fields)->f;
+    const auto sh = shape == 0 ? longitudinal_shape::Shape::QuinticCompensated
+                               : (shape == 1 ? longitudinal_shape::Shape::LinearNotch : longitudinal_shape::Shape::LinearBinomial3);
+    longitudinal_shape::Current cur(f, MPI_COMM_SELF, sh, 8, smoothX != 0);
+    cur.deposit(fv(origin), fv(a), fv(b), charge);
+  });
+}
+
+// Orbit gathers of source_coupling_math with an explicit old E (old = the
+// fields' E at the call) for the binomial3-x and quintic paths.
+int ref_binomial_xz_magnetic(void* fields, const double* origin, const double* point, double* out) {
+  return guarded([&] {
+    source_coupling_math::NativeHalo halo;
+    const auto v = source_coupling_math::binomialXZMagneticSample(*R(fields)->f, fv(origin), fv(point), halo);
+    for (int k = 0; k < 3; ++k) out[k] = v[k];
+  });
+}
+int ref_magnetic_sample_open(void* fields, const double* origin, const double* point, double* out) {
+  return guarded([&] {
+    source_coupling_math::NativeHalo halo;
+    const auto v = source_coupling_math::magneticSample(*R(fields)->f, fv(origin), fv(point), false, &halo, true).value;
+    for (int k = 0; k < 3; ++k) out[k] = v[k];
+  });
+}
+
+}  // extern "C"

@@ -221,3 +221,165 @@ def seeded_fields(fields, seed, components=range(6), scale=1.0):
         v[...] = scale * (np.sin(0.245765 * n + 0.37 * a) + 0.25 * rng.standard_normal(v.shape))
         if a >= 3 and a < 6:
             v[...] /= C_SI
+
+
+# ---------------------------------------------------------------------------
+# Particle side.
+
+_u64p = C.POINTER(C.c_uint64)
+
+
+def _bind_particles(lib, prefix):
+    P = prefix + "_"
+    vp, sz = C.c_void_p, C.c_size_t
+    f = lambda n: getattr(lib, P + n)  # noqa: E731
+    f("coupling_new").argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp)]
+    f("coupling_free").argtypes = [vp]
+    f("coupling_free").restype = None
+    f("coupling_set_frame").argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double]
+    f("coupling_add_magnet").argtypes = [vp, C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                         C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+    f("coupling_set_cpml_cells").argtypes = [vp, C.POINTER(sz)]
+    f("coupling_set_particles").argtypes = [vp, sz, _dp, _dp, _dp, _dp, _dp, _u64p]
+    f("coupling_count").argtypes = [vp]
+    f("coupling_count").restype = sz
+    f("coupling_get_particles").argtypes = [vp, _dp, _dp]
+    f("coupling_step").argtypes = [vp, _dp, C.c_double, C.c_uint, C.c_double, C.POINTER(C.c_uint), _dp, _dp]
+    f("coupling_begin_fields").argtypes = [vp, _dp]
+    f("shape_weights").argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.POINTER(C.c_int),
+                                   C.POINTER(C.c_longlong), _dp, sz, C.POINTER(sz), C.POINTER(C.c_uint)]
+    f("momentum").argtypes = [_dp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_uint)]
+    f("prescribed_box").argtypes = [vp, _dp, C.c_double, _dp, _dp]
+    f("current_deposit").argtypes = [vp, C.c_int, C.c_int, _dp, _dp, _dp, C.c_double]
+    f("binomial_xz_magnetic").argtypes = [vp, _dp, _dp, _dp]
+    f("magnetic_sample_open").argtypes = [vp, _dp, _dp, _dp]
+
+
+def load_particles(prefix):
+    lib = load(prefix)
+    if not getattr(lib, "_particles_bound", False):
+        _bind_particles(lib, prefix)
+        lib._particles_bound = True
+    return lib
+
+
+def _v(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def _call(prefix, name, *args):
+    lib = load_particles(prefix)
+    code = getattr(lib, prefix + "_" + name)(*args)
+    if code:
+        raise CpuError(code, getattr(lib, prefix + "_last_error")().decode())
+
+
+def shape_weights(prefix, shape, smooth_x, geometry7, origin, begin, end, capacity=20000):
+    axis = np.zeros(capacity, np.int32)
+    ijk = np.zeros(3 * capacity, np.int64)
+    w = np.zeros(capacity)
+    n = C.c_size_t()
+    seg = C.c_uint()
+    g, o, b, e = _v(geometry7), _v(origin), _v(begin), _v(end)
+    _call(prefix, "shape_weights", shape, int(smooth_x), _ptr(g), _ptr(o), _ptr(b), _ptr(e),
+          axis.ctypes.data_as(C.POINTER(C.c_int)), ijk.ctypes.data_as(C.POINTER(C.c_longlong)), _ptr(w),
+          capacity, C.byref(n), C.byref(seg))
+    k = n.value
+    return axis[:k].copy(), ijk[:3 * k].reshape(k, 3).copy(), w[:k].copy(), seg.value
+
+
+def momentum(prefix, u, e, b, q, m, h, g0=0.0):
+    uu = _v(u).copy()
+    it = C.c_uint()
+    _call(prefix, "momentum", _ptr(uu), _ptr(_v(e)), _ptr(_v(b)), q, m, h, g0, C.byref(it))
+    return uu, it.value
+
+
+class CpuCoupling:
+    """ConservativeSourceCoupling + particle loop in the oracle or reference."""
+
+    SHAPES = {"quintic": 0, "notch": 1, "binomial3": 2}
+
+    def __init__(self, fields: CpuFields, shape="binomial3", smooth_x=True, cpml_cells=(0, 0, 0)):
+        self.prefix = fields.P[:-1]
+        self.lib = load_particles(self.prefix)
+        self.fields = fields
+        h = C.c_void_p()
+        _call(self.prefix, "coupling_new", fields.h, self.SHAPES[shape], int(smooth_x), C.byref(h))
+        self.h = h
+        cells = (C.c_size_t * 3)(*cpml_cells)
+        _call(self.prefix, "coupling_set_cpml_cells", self.h, cells)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            getattr(self.lib, self.prefix + "_coupling_free")(self.h)
+            self.h = None
+
+    def set_frame(self, gamma, lab_t0=0.0, lab_z0=0.0, box_z0=0.0):
+        _call(self.prefix, "coupling_set_frame", self.h, gamma, lab_t0, lab_z0, box_z0)
+
+    def add_magnet(self, kind, center, length, period=0.0, peak=0.0, angle=0.0, gaussian_fringe=False,
+                   cutoff=1e-9, gradient=0.0, end_ramp=0.0, phase=0.0):
+        t = {"undulator": 0, "dipole": 1, "quadrupole": 2}[kind]
+        _call(self.prefix, "coupling_add_magnet", self.h, t, _ptr(_v(center)), length, period, peak, angle,
+              int(gaussian_fringe), cutoff, gradient, end_ramp, phase)
+
+    def set_particles(self, x, u, q, m, w, ids):
+        x, u = _v(x), _v(u)
+        q, m, w = _v(q), _v(m), _v(w)
+        ids = np.ascontiguousarray(ids, dtype=np.uint64)
+        _call(self.prefix, "coupling_set_particles", self.h, len(q), _ptr(x.ravel()), _ptr(u.ravel()), _ptr(q),
+              _ptr(m), _ptr(w), ids.ctypes.data_as(_u64p))
+
+    def particles(self):
+        n = getattr(self.lib, self.prefix + "_coupling_count")(self.h)
+        x = np.zeros((n, 3))
+        u = np.zeros((n, 3))
+        _call(self.prefix, "coupling_get_particles", self.h, _ptr(x), _ptr(u))
+        return x, u
+
+    def step(self, origin, time, substeps=1, exit_lab_z=float("inf")):
+        it = C.c_uint()
+        force = C.c_double()
+        work = C.c_double()
+        _call(self.prefix, "coupling_step", self.h, _ptr(_v(origin)), time, substeps, exit_lab_z, C.byref(it),
+              C.byref(force), C.byref(work))
+        return it.value, force.value, work.value
+
+    def prescribed_box(self, pos, t):
+        e = np.zeros(3)
+        b = np.zeros(3)
+        _call(self.prefix, "prescribed_box", self.h, _ptr(_v(pos)), t, _ptr(e), _ptr(b))
+        return e, b
+
+
+def current_deposit(fields: CpuFields, shape, smooth_x, origin, a, b, charge):
+    prefix = fields.P[:-1]
+    _call(prefix, "current_deposit", fields.h, CpuCoupling.SHAPES[shape], int(smooth_x), _ptr(_v(origin)),
+          _ptr(_v(a)), _ptr(_v(b)), charge)
+
+
+def binomial_xz_magnetic(fields: CpuFields, origin, point):
+    out = np.zeros(3)
+    _call(fields.P[:-1], "binomial_xz_magnetic", fields.h, _ptr(_v(origin)), _ptr(_v(point)), _ptr(out))
+    return out
+
+
+def magnetic_sample_open(fields: CpuFields, origin, point):
+    out = np.zeros(3)
+    _call(fields.P[:-1], "magnetic_sample_open", fields.h, _ptr(_v(origin)), _ptr(_v(point)), _ptr(out))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Seeded 6D Gaussian bunch (particle_initializer.cpp:24-107: splitmix64 +
+# Box-Muller per particle id), used for synthetic workloads.
+
+_M64 = (1 << 64) - 1
+
+
+def splitmix64(v):
+    v = (v + 0x9E3779B97F4A7C15) & _M64
+    v = ((v ^ (v >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    v = ((v ^ (v >> 27)) * 0x94D049BB133111EB) & _M64
+    return v ^ (v >> 31)
