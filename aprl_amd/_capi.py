@@ -29,6 +29,24 @@ class CpmlParameters(C.Structure):
                 ("kappa_max", C.c_double), ("alpha_fraction", C.c_double)]
 
 
+class Magnet(C.Structure):
+    """aprl_magnet (SIMagneticElement, eb_sources.h:107-144)."""
+    _fields_ = [("type", C.c_int), ("center", C.c_double * 3), ("length", C.c_double), ("period", C.c_double),
+                ("peak_field", C.c_double), ("polarization_angle", C.c_double), ("gaussian_fringe", C.c_int),
+                ("fringe_relative_cutoff", C.c_double), ("quadrupole_gradient", C.c_double),
+                ("end_ramp_periods", C.c_double), ("undulator_phase", C.c_double)]
+
+
+class Boost(C.Structure):
+    """aprl_boost (BoostFrameTransform::setOriginsFromGamma)."""
+    _fields_ = [("gamma", C.c_double), ("lab_time_origin", C.c_double), ("lab_z_origin", C.c_double),
+                ("box_z_origin", C.c_double)]
+
+
+class StepReport(C.Structure):
+    _fields_ = [("iterations", C.c_uint), ("force", C.c_double), ("work", C.c_double)]
+
+
 class AprlError(RuntimeError):
     """A failed C-ABI call; ``status`` is the aprl_status code."""
 
@@ -90,6 +108,21 @@ def lib() -> C.CDLL:
     L.aprl_window_shift.argtypes = [vp, _sz, C.c_ssize_t, C.POINTER(_dp)]
     L.aprl_profile_enable.argtypes = [vp, C.c_int]
     L.aprl_profile_query.argtypes = [vp, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]
+    u64p = C.POINTER(C.c_uint64)
+    L.aprl_particles_create.argtypes = [vp, _sz, C.POINTER(vp)]
+    L.aprl_particles_destroy.argtypes = [vp]
+    L.aprl_particles_destroy.restype = None
+    L.aprl_particles_upload.argtypes = [vp, _sz, _dp, _dp, _dp, _dp, _dp, u64p]
+    L.aprl_particles_download.argtypes = [vp, _dp, _dp, _dp, _dp, _dp, u64p]
+    L.aprl_particles_count.argtypes = [vp]
+    L.aprl_particles_count.restype = _sz
+    L.aprl_coupling_create.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp)]
+    L.aprl_coupling_destroy.argtypes = [vp]
+    L.aprl_coupling_destroy.restype = None
+    L.aprl_coupling_set_frame.argtypes = [vp, C.POINTER(Boost)]
+    L.aprl_coupling_add_magnet.argtypes = [vp, C.POINTER(Magnet)]
+    L.aprl_coupling_set_particle_boundary.argtypes = [vp, C.POINTER(_sz), C.c_double]
+    L.aprl_coupling_step.argtypes = [vp, vp, _dp, C.c_double, C.c_uint, C.POINTER(StepReport)]
     _lib = L
     return L
 
@@ -124,4 +157,7 @@ EXPORTED = [
     "aprl_fields_advance_magnetic", "aprl_fields_advance_electric", "aprl_fields_synchronize",
     "aprl_window_install", "aprl_window_begin_step", "aprl_window_set_inlet", "aprl_window_shift",
     "aprl_profile_enable", "aprl_profile_query", "aprl_profile_reset",
+    "aprl_particles_create", "aprl_particles_destroy", "aprl_particles_upload", "aprl_particles_download",
+    "aprl_particles_count", "aprl_coupling_create", "aprl_coupling_destroy", "aprl_coupling_set_frame",
+    "aprl_coupling_add_magnet", "aprl_coupling_set_particle_boundary", "aprl_coupling_step",
 ]

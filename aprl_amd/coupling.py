@@ -1,0 +1,115 @@
+"""Host-side mirror of the particle and coupling interfaces over the C-ABI.
+
+``ParticleStore`` holds RelativisticParticleSI records on the device
+(eb_particles.h:12-25); ``ConservativeSourceCoupling`` keeps the name of
+source_coupling.h:12-41 and runs one whole
+Simulation::solveConservativeSourceStep + commitConservativeElectric
+(simulation_source_coupling.cpp:16-65) per ``step`` call.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _capi
+from ._capi import check, dptr, lib
+from .fields import EBFieldGrid
+
+SHAPES = {"quintic-compensated": 0, "linear-binomial3": 2}
+MAGNETS = {"planar-undulator": 0, "uniform-dipole": 1, "quadrupole": 2}
+
+
+class ParticleStore:
+    def __init__(self, fields: EBFieldGrid, capacity: int = 1):
+        self.fields = fields
+        h = C.c_void_p()
+        check(lib().aprl_particles_create(fields.handle, capacity, C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().aprl_particles_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return lib().aprl_particles_count(self.handle)
+
+    def upload(self, position, u, charge, mass, weight=None, ids=None):
+        x = np.ascontiguousarray(position, dtype=np.float64).reshape(-1, 3)
+        n = x.shape[0]
+        uu = np.ascontiguousarray(u, dtype=np.float64).reshape(n, 3)
+        q = np.ascontiguousarray(np.broadcast_to(charge, (n,)), dtype=np.float64)
+        m = np.ascontiguousarray(np.broadcast_to(mass, (n,)), dtype=np.float64)
+        w = np.ascontiguousarray(np.ones(n) if weight is None else np.broadcast_to(weight, (n,)), dtype=np.float64)
+        i = np.ascontiguousarray(np.arange(1, n + 1) if ids is None else ids, dtype=np.uint64)
+        check(lib().aprl_particles_upload(self.handle, n, dptr(x.ravel()), dptr(uu.ravel()), dptr(q), dptr(m),
+                                          dptr(w), i.ctypes.data_as(C.POINTER(C.c_uint64))))
+
+    def download(self):
+        """(position (n,3), u (n,3), charge, mass, weight, ids) in device order."""
+        n = len(self)
+        x = np.empty((n, 3))
+        u = np.empty((n, 3))
+        q, m, w = np.empty(n), np.empty(n), np.empty(n)
+        ids = np.empty(n, dtype=np.uint64)
+        check(lib().aprl_particles_download(self.handle, dptr(x.ravel()) if n else None,
+                                            dptr(u.ravel()) if n else None, dptr(q) if n else None,
+                                            dptr(m) if n else None, dptr(w) if n else None,
+                                            ids.ctypes.data_as(C.POINTER(C.c_uint64)) if n else None))
+        return x, u, q, m, w, ids
+
+
+class ConservativeSourceCoupling:
+    def __init__(self, fields: EBFieldGrid, shape: str = "linear-binomial3", transverseXBinomial3: bool = True):
+        self.fields = fields
+        h = C.c_void_p()
+        check(lib().aprl_coupling_create(fields.handle, SHAPES[shape], int(transverseXBinomial3), C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().aprl_coupling_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def setFrame(self, gamma, labTimeOrigin=0.0, labZOrigin=0.0, boxZOrigin=0.0):
+        b = _capi.Boost(gamma, labTimeOrigin, labZOrigin, boxZOrigin)
+        check(lib().aprl_coupling_set_frame(self.handle, C.byref(b)))
+
+    def addPrescribedLabMagnet(self, kind, center, length, period=0.0, peak_field=0.0, polarization_angle=0.0,
+                               gaussian_fringe=False, fringe_relative_cutoff=1e-9, quadrupole_gradient=0.0,
+                               end_ramp_periods=0.0, undulator_phase=0.0):
+        m = _capi.Magnet()
+        m.type = MAGNETS[kind]
+        for a in range(3):
+            m.center[a] = center[a]
+        m.length, m.period, m.peak_field, m.polarization_angle = length, period, peak_field, polarization_angle
+        m.gaussian_fringe = int(gaussian_fringe)
+        m.fringe_relative_cutoff = fringe_relative_cutoff
+        m.quadrupole_gradient = quadrupole_gradient
+        m.end_ramp_periods = end_ramp_periods
+        m.undulator_phase = undulator_phase
+        check(lib().aprl_coupling_add_magnet(self.handle, C.byref(m)))
+
+    def setParticleBoundary(self, cpml_cells, exit_lab_z=math.inf):
+        cells = (C.c_size_t * 3)(*cpml_cells)
+        check(lib().aprl_coupling_set_particle_boundary(self.handle, cells, exit_lab_z))
+
+    def step(self, particles: ParticleStore, origin, time, substeps=1):
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        r = _capi.StepReport()
+        check(lib().aprl_coupling_step(self.handle, particles.handle, dptr(o), time, substeps, C.byref(r)))
+        return r.iterations, r.force, r.work

@@ -3,11 +3,7 @@
 #include <cstring>
 #include <memory>
 
-#include "fields.hpp"
-
-struct aprl_fields {
-  aprl_gpu::FieldState s;
-};
+#include "handles.hpp"
 
 namespace aprl_gpu {
 namespace {
@@ -85,6 +81,7 @@ aprl_status aprl_fields_upload(aprl_fields* fields, int component, const double*
     checkComponent(component);
     if (!host || count != s.compSize(component)) throw std::invalid_argument("Component upload size mismatch");
     upload(s.stream, s.base(component), host, count);
+    if (component >= 6) s.currentZero = false;
     hipCheck(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
   });
 }
@@ -220,6 +217,7 @@ aprl_status aprl_window_install(aprl_fields* fields, const aprl_cpml_parameters*
       hipCheck(hipMemcpyAsync(s.base(a), old[a].get(), s.compSize(a) * sizeof(double), hipMemcpyDeviceToDevice,
                               s.stream),
                "hipMemcpyAsync");
+    s.currentZero = false;
     hipCheck(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
     installCpml(s, *p, g.nz, 0, false, false);
     s.window = true;

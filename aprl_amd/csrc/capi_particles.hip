@@ -1,0 +1,197 @@
+// C-ABI entry points for the particle store and the conservative-source
+// coupling (declared in include/aprl_gpu.h).
+#include <cmath>
+#include <memory>
+
+#include "handles.hpp"
+
+namespace aprl_gpu {
+void reserveParticles(ParticleStore& P, std::size_t capacity);
+void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], double time, unsigned substeps,
+                  aprl_step_report* report);
+}  // namespace aprl_gpu
+
+using namespace aprl_gpu;
+
+namespace {
+// SIMagneticElement::prepare, eb_sources.cpp:320-344 (host, same libm).
+MagnetDev prepareMagnet(const aprl_magnet& in) {
+  MagnetDev m{};
+  m.type = in.type;
+  for (int a = 0; a < 3; ++a) m.center[a] = in.center[a];
+  m.length = in.length;
+  m.period = in.period;
+  m.peak = in.peak_field;
+  m.angle = in.polarization_angle;
+  m.gaussianFringe = in.gaussian_fringe;
+  m.cutoff = in.fringe_relative_cutoff;
+  m.gradient = in.quadrupole_gradient;
+  m.endRampPeriods = in.end_ramp_periods;
+  m.phase = in.undulator_phase;
+  if (m.type < 0 || m.type > 2) throw std::invalid_argument("Unknown magnetic element type");
+  for (double c : m.center)
+    if (!std::isfinite(c)) throw std::invalid_argument("Invalid SI magnetic element");
+  if (!(m.length >= 0.0) || !std::isfinite(m.length) || !std::isfinite(m.peak) || !std::isfinite(m.angle))
+    throw std::invalid_argument("Invalid SI magnetic element");
+  if (m.type == 0 && (!(m.period > 0.0) || !std::isfinite(m.period)))
+    throw std::invalid_argument("Undulator period must be positive");
+  if (m.type == 0 && m.gaussianFringe && (!(m.cutoff > 0.0) || !(m.cutoff < 1.0) || !std::isfinite(m.cutoff)))
+    throw std::invalid_argument("Undulator fringe_relative_cutoff must be in (0,1)");
+  if (m.type == 1 && !(m.length > 0.0)) throw std::invalid_argument("Dipole length must be positive");
+  if (m.type == 2 && (!(m.length > 0.0) || !std::isfinite(m.gradient)))
+    throw std::invalid_argument("Quadrupole length/gradient is invalid");
+  if (!std::isfinite(m.endRampPeriods) || m.endRampPeriods < 0.0 ||
+      (m.endRampPeriods > 0.0 && (m.type != 0 || m.gaussianFringe || 2.0 * m.endRampPeriods * m.period > m.length)))
+    throw std::invalid_argument("Internal undulator end ramp requires non-Gaussian ends and fits inside length");
+  if (!std::isfinite(m.phase) || (m.phase != 0.0 && (m.type != 0 || m.endRampPeriods <= 0.0)))
+    throw std::invalid_argument("Undulator phase requires internal end ramps");
+  const double twoPi = 2.0 * 3.141592653589793238462643383279502884;
+  m.waveNumber = m.type == 0 ? twoPi / m.period : 0.0;
+  m.fringeExtent = m.type == 0 && m.gaussianFringe ? std::sqrt(-2.0 * std::log(m.cutoff)) / m.waveNumber : 0.0;
+  m.cosA = std::cos(m.angle);
+  m.sinA = std::sin(m.angle);
+  return m;
+}
+}  // namespace
+
+extern "C" {
+
+aprl_status aprl_particles_create(aprl_fields* fields, size_t capacity, aprl_particles** out) {
+  return guarded([&] {
+    if (!fields || !out) throw std::invalid_argument("Null field grid or output handle");
+    std::unique_ptr<aprl_particles> p(new aprl_particles);
+    p->fields = fields;
+    reserveParticles(p->p, std::max<size_t>(capacity, 1));
+    *out = p.release();
+  });
+}
+
+void aprl_particles_destroy(aprl_particles* particles) {
+  if (!particles) return;
+  (void)hipStreamSynchronize(particles->fields->s.stream);
+  delete particles;
+}
+
+size_t aprl_particles_count(const aprl_particles* particles) { return particles ? particles->p.count : 0; }
+
+aprl_status aprl_particles_upload(aprl_particles* particles, size_t n, const double* x3, const double* u3,
+                                  const double* q, const double* m, const double* w, const uint64_t* id) {
+  return guarded([&] {
+    if (!particles) throw std::invalid_argument("Null particle store");
+    if (n && (!x3 || !u3 || !q || !m || !w || !id)) throw std::invalid_argument("Null particle array");
+    ParticleStore& P = particles->p;
+    hipStream_t st = particles->fields->s.stream;
+    reserveParticles(P, n);
+    std::vector<double> col(n);
+    for (int a = 0; a < 3; ++a) {
+      for (size_t i = 0; i < n; ++i) col[i] = x3[3 * i + a];
+      hipCheck(hipMemcpy(P.s[a].get(), col.data(), n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+      for (size_t i = 0; i < n; ++i) col[i] = u3[3 * i + a];
+      hipCheck(hipMemcpy(P.s[3 + a].get(), col.data(), n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+    }
+    const double* scalars[3] = {q, m, w};
+    for (int a = 0; a < 3; ++a)
+      hipCheck(hipMemcpy(P.s[6 + a].get(), scalars[a], n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+    hipCheck(hipMemcpy(P.id.get(), id, n * sizeof(uint64_t), hipMemcpyHostToDevice), "hipMemcpy");
+    (void)st;
+    P.count = n;
+  });
+}
+
+aprl_status aprl_particles_download(aprl_particles* particles, double* x3, double* u3, double* q, double* m,
+                                    double* w, uint64_t* id) {
+  return guarded([&] {
+    if (!particles) throw std::invalid_argument("Null particle store");
+    ParticleStore& P = particles->p;
+    hipCheck(hipStreamSynchronize(particles->fields->s.stream), "hipStreamSynchronize");
+    const size_t n = P.count;
+    std::vector<double> col(n);
+    for (int a = 0; a < 3; ++a) {
+      if (x3) {
+        hipCheck(hipMemcpy(col.data(), P.s[a].get(), n * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+        for (size_t i = 0; i < n; ++i) x3[3 * i + a] = col[i];
+      }
+      if (u3) {
+        hipCheck(hipMemcpy(col.data(), P.s[3 + a].get(), n * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+        for (size_t i = 0; i < n; ++i) u3[3 * i + a] = col[i];
+      }
+    }
+    double* scalars[3] = {q, m, w};
+    for (int a = 0; a < 3; ++a)
+      if (scalars[a])
+        hipCheck(hipMemcpy(scalars[a], P.s[6 + a].get(), n * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (id) hipCheck(hipMemcpy(id, P.id.get(), n * sizeof(uint64_t), hipMemcpyDeviceToHost), "hipMemcpy");
+  });
+}
+
+aprl_status aprl_coupling_create(aprl_fields* fields, aprl_shape shape, int smoothX, aprl_coupling** out) {
+  return guarded([&] {
+    if (!fields || !out) throw std::invalid_argument("Null field grid or output handle");
+    if (shape != APRL_SHAPE_LINEAR_BINOMIAL3)
+      throw std::invalid_argument("This build implements the linear-binomial3 source shape only");
+    if (!smoothX) throw std::invalid_argument("This build implements the production x-binomial3 smoothing only");
+    std::unique_ptr<aprl_coupling> c(new aprl_coupling);
+    c->c.fields = &fields->s;
+    c->c.shape = shape;
+    c->c.smoothX = smoothX != 0;
+    if (fields->s.hasCpml)
+      for (int a = 0; a < 3; ++a) c->c.cpmlCells[a] = fields->s.cpml.cells[a];
+    *out = c.release();
+  });
+}
+
+void aprl_coupling_destroy(aprl_coupling* coupling) {
+  if (!coupling) return;
+  (void)hipStreamSynchronize(coupling->c.fields->stream);
+  delete coupling;
+}
+
+aprl_status aprl_coupling_set_frame(aprl_coupling* coupling, const aprl_boost* f) {
+  return guarded([&] {
+    if (!coupling || !f) throw std::invalid_argument("Null coupling or frame");
+    // BoostFrameTransform::setOriginsFromGamma, boostframe.cpp:42-60.
+    const double g = f->gamma;
+    if (!(g >= 1.0) || !std::isfinite(g)) throw std::invalid_argument("Lorentz gamma must be finite and >= 1");
+    const double gb = std::sqrt(g - 1.0) * std::sqrt(g + 1.0);
+    if (!std::isfinite(f->lab_time_origin) || !std::isfinite(f->lab_z_origin) || !std::isfinite(f->box_z_origin))
+      throw std::invalid_argument("Lorentz origins must be finite");
+    BoostDev& B = coupling->c.frame;
+    B.gamma = g;
+    B.beta = gb / g;
+    B.gammaBeta = gb;
+    B.c0 = kC;
+    B.labT0 = f->lab_time_origin;
+    B.labZ0 = f->lab_z_origin;
+    B.boxZ0 = f->box_z_origin;
+  });
+}
+
+aprl_status aprl_coupling_add_magnet(aprl_coupling* coupling, const aprl_magnet* magnet) {
+  return guarded([&] {
+    if (!coupling || !magnet) throw std::invalid_argument("Null coupling or magnet");
+    if (coupling->c.magnets.size() >= static_cast<size_t>(kMaxMagnets))
+      throw std::invalid_argument("Too many prescribed magnets for one coupling");
+    coupling->c.magnets.push_back(prepareMagnet(*magnet));
+  });
+}
+
+aprl_status aprl_coupling_set_particle_boundary(aprl_coupling* coupling, const size_t* cells, double exitLabZ) {
+  return guarded([&] {
+    if (!coupling || !cells) throw std::invalid_argument("Null coupling or CPML cells");
+    for (int a = 0; a < 3; ++a) coupling->c.cpmlCells[a] = cells[a];
+    coupling->c.hasExit = std::isfinite(exitLabZ);
+    coupling->c.exitLabZ = exitLabZ;
+  });
+}
+
+aprl_status aprl_coupling_step(aprl_coupling* coupling, aprl_particles* particles, const double* origin,
+                               double time, unsigned substeps, aprl_step_report* report) {
+  return guarded([&] {
+    if (!coupling || !particles || !origin) throw std::invalid_argument("Null coupling, particles or origin");
+    if (&particles->fields->s != coupling->c.fields)
+      throw std::invalid_argument("Particles and coupling belong to different field grids");
+    couplingStep(coupling->c, particles->p, origin, time, substeps, report);
+  });
+}
+
+}  // extern "C"

@@ -429,6 +429,134 @@ __global__ void __launch_bounds__(256) kMagnetic(FieldPtrs F, StepCoefs c, CpmlA
   }
 }
 
+// z-marching LDS version of the Cowan magnetic update. A block owns a
+// (kMX x kMY) column of B nodes and kMZ consecutive z planes; E is staged in
+// a three-plane rolling buffer of electricValue() samples (reflection / halo
+// / PEC parity applied while loading), so every smoothed sample is the same
+// nine products in the same order as smoothedElectric().
+constexpr int kMX = 32, kMY = 8, kMZ = 16;
+constexpr int kLX = kMX + 2, kLY = kMY + 2;
+
+template <int A>
+__device__ __forceinline__ void stagePlane(const FieldPtrs& F, double* dst, int i0, int j0, int kk) {
+  const int n0 = F.n[A][0], n1 = F.n[A][1], n2 = F.n[A][2];
+  const bool zin = kk >= 0 && kk < n2;
+  for (int t = threadIdx.y * kMX + threadIdx.x; t < kLX * kLY; t += kMX * kMY) {
+    const int li = t % kLX, lj = t / kLX;
+    const int i = i0 - 1 + li, j = j0 - 1 + lj;
+    double v;
+    if (zin && i >= 0 && i < n0 && j >= 0 && j < n1) v = F.e[A][(static_cast<size_t>(kk) * n1 + j) * n0 + i];
+    else v = evalE<A>(F, i, j, kk);
+    dst[t] = v;
+  }
+}
+
+// Smoothed sample from the staged planes; (li, lj) local, s0/s1/s2 the
+// rolling slots of planes k-1, k, k+1. D is the derivative axis.
+template <int D>
+__device__ __forceinline__ double smoothL(const double* const pl[3], const StepCoefs& c, int li, int lj) {
+  // pl[0] = plane k-1, pl[1] = plane k, pl[2] = plane k+1
+  constexpr int TA = (D + 1) % 3, TB = (D + 2) % 3;
+  const double delta = D == 0 ? c.dYZ : (D == 1 ? c.dZX : c.dXY);
+  auto at = [&](int dx, int dy, int dz) { return pl[1 + dz][(lj + dy) * kLX + li + dx]; };
+  auto off = [&](int axis, int dir, int& dx, int& dy, int& dz) {
+    if (axis == 0) dx += dir; else if (axis == 1) dy += dir; else dz += dir;
+  };
+  double r = c.alpha[D] * at(0, 0, 0);
+  for (int dir = -1; dir <= 1; dir += 2) {
+    int ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
+    off(TA, dir, ax, ay, az);
+    off(TB, dir, bx, by, bz);
+    r += c.beta[TA] * at(ax, ay, az);
+    r += c.beta[TB] * at(bx, by, bz);
+  }
+  for (int dA = -1; dA <= 1; dA += 2)
+    for (int dB = -1; dB <= 1; dB += 2) {
+      int x = 0, y = 0, z = 0;
+      off(TA, dA, x, y, z);
+      off(TB, dB, x, y, z);
+      r += delta * at(x, y, z);
+    }
+  return r;
+}
+
+__global__ void __launch_bounds__(kMX * kMY) kMagneticCowanTiled(FieldPtrs F, StepCoefs c, CpmlArgs P) {
+  __shared__ double buf[3][3][kLX * kLY];  // [component][slot][plane]
+  const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
+  const int i0 = blockIdx.x * kMX, j0 = blockIdx.y * kMY;
+  const int k0 = blockIdx.z * kMZ, k1 = min(k0 + kMZ, nz + 1);
+  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+  const int li = threadIdx.x + 1, lj = threadIdx.y + 1;
+  auto slot = [](int kk) { return ((kk % 3) + 3) % 3; };
+  stagePlane<0>(F, buf[0][slot(k0 - 1)], i0, j0, k0 - 1);
+  stagePlane<1>(F, buf[1][slot(k0 - 1)], i0, j0, k0 - 1);
+  stagePlane<2>(F, buf[2][slot(k0 - 1)], i0, j0, k0 - 1);
+  stagePlane<0>(F, buf[0][slot(k0)], i0, j0, k0);
+  stagePlane<1>(F, buf[1][slot(k0)], i0, j0, k0);
+  stagePlane<2>(F, buf[2][slot(k0)], i0, j0, k0);
+  for (int k = k0; k < k1; ++k) {
+    stagePlane<0>(F, buf[0][slot(k + 1)], i0, j0, k + 1);
+    stagePlane<1>(F, buf[1][slot(k + 1)], i0, j0, k + 1);
+    stagePlane<2>(F, buf[2][slot(k + 1)], i0, j0, k + 1);
+    __syncthreads();
+    const double* ex[3] = {buf[0][slot(k - 1)], buf[0][slot(k)], buf[0][slot(k + 1)]};
+    const double* ey[3] = {buf[1][slot(k - 1)], buf[1][slot(k)], buf[1][slot(k + 1)]};
+    const double* ez[3] = {buf[2][slot(k - 1)], buf[2][slot(k)], buf[2][slot(k + 1)]};
+    const double* ey1[3] = {ey[1], ey[2], ey[2]};  // planes k, k+1 as "k-1, k" for z-shifted samples
+    const double* ex1[3] = {ex[1], ex[2], ex[2]};
+    if (i <= nx && j < ny && k < nz) {  // Bx
+      const double dEzDy = smoothL<1>(ez, c, li, lj + 1) - smoothL<1>(ez, c, li, lj);
+      // S(ey, d=z) is x-y smoothing: only its own plane is read.
+      const double* eyk[3] = {ey[1], ey[1], ey[1]};
+      const double* eyk1[3] = {ey[2], ey[2], ey[2]};
+      const double dEyDz = smoothL<2>(eyk1, c, li, lj) - smoothL<2>(eyk, c, li, lj);
+      const size_t idx = (static_cast<size_t>(k) * ny + j) * (nx + 1) + i;
+      double b = F.b[0][idx];
+      b -= c.ty * dEzDy - c.tz * dEyDz;
+      if (P.enabled) {
+        int o = P.ord[0][1][j];
+        if (o >= 0) b += (-1.0 * P.magScale[1]) * cpmlCorrection<true>(P.prof[0][1][j], dEzDy, histAt<0, 0, 1>(P, o, i, j, k, nx + 1));
+        o = P.ord[0][2][k];
+        if (o >= 0) b += (1.0 * P.magScale[2]) * cpmlCorrection<true>(P.prof[0][2][k], dEyDz, histAt<0, 0, 2>(P, o, i, j, k, nx + 1));
+      }
+      F.b[0][idx] = b;
+    }
+    if (i < nx && j <= ny && k < nz) {  // By
+      const double* exk[3] = {ex[1], ex[1], ex[1]};
+      const double* exk1[3] = {ex[2], ex[2], ex[2]};
+      const double dExDz = smoothL<2>(exk1, c, li, lj) - smoothL<2>(exk, c, li, lj);
+      const double dEzDx = smoothL<0>(ez, c, li + 1, lj) - smoothL<0>(ez, c, li, lj);
+      const size_t idx = (static_cast<size_t>(k) * (ny + 1) + j) * nx + i;
+      double b = F.b[1][idx];
+      b -= c.tz * dExDz - c.tx * dEzDx;
+      if (P.enabled) {
+        int o = P.ord[0][2][k];
+        if (o >= 0) b += (-1.0 * P.magScale[2]) * cpmlCorrection<true>(P.prof[0][2][k], dExDz, histAt<0, 1, 2>(P, o, i, j, k, nx));
+        o = P.ord[0][0][i];
+        if (o >= 0) b += (1.0 * P.magScale[0]) * cpmlCorrection<true>(P.prof[0][0][i], dEzDx, histAt<0, 1, 0>(P, o, i, j, k, nx));
+      }
+      F.b[1][idx] = b;
+    }
+    if (i < nx && j < ny) {  // Bz, k in [0, nz]
+      const double dEyDx = smoothL<0>(ey, c, li + 1, lj) - smoothL<0>(ey, c, li, lj);
+      const double dExDy = smoothL<1>(ex, c, li, lj + 1) - smoothL<1>(ex, c, li, lj);
+      const size_t idx = (static_cast<size_t>(k) * ny + j) * nx + i;
+      double b = F.b[2][idx];
+      b -= c.tx * dEyDx - c.ty * dExDy;
+      if (P.enabled) {
+        int o = P.ord[0][0][i];
+        if (o >= 0) b += (-1.0 * P.magScale[0]) * cpmlCorrection<true>(P.prof[0][0][i], dEyDx, histAt<0, 2, 0>(P, o, i, j, k, nx));
+        o = P.ord[0][1][j];
+        if (o >= 0) b += (1.0 * P.magScale[1]) * cpmlCorrection<true>(P.prof[0][1][j], dExDy, histAt<0, 2, 1>(P, o, i, j, k, nx));
+      }
+      F.b[2][idx] = b;
+    }
+    (void)ey1;
+    (void)ex1;
+    __syncthreads();
+  }
+}
+
 __device__ __forceinline__ double rd(const double* p, int nx, int ny, int i, int j, int k) {
   return p[(static_cast<size_t>(k) * ny + j) * nx + i];
 }
@@ -533,8 +661,13 @@ void launchMagnetic(FieldState& s, const FieldPtrs& f) {
   dim3 block(64, 4, 1);
   dim3 grid(ceilDiv(s.g.nx + 1, 64), ceilDiv(s.g.ny + 1, 4), static_cast<unsigned>(s.g.nz + 1));
   ScopedKernel sk(s.profiler, s.stream, "magnetic");
-  if (s.g.solver == 1) kMagnetic<true><<<grid, block, 0, s.stream>>>(f, c, P);
-  else kMagnetic<false><<<grid, block, 0, s.stream>>>(f, c, P);
+  if (s.g.solver == 1) {
+    dim3 tb(kMX, kMY, 1);
+    dim3 tg(ceilDiv(s.g.nx + 1, kMX), ceilDiv(s.g.ny + 1, kMY), ceilDiv(s.g.nz + 1, kMZ));
+    kMagneticCowanTiled<<<tg, tb, 0, s.stream>>>(f, c, P);
+  } else {
+    kMagnetic<false><<<grid, block, 0, s.stream>>>(f, c, P);
+  }
   launchCheck("magnetic");
 }
 
@@ -559,8 +692,10 @@ void launchElectric(FieldState& s, const FieldPtrs& f, int mode, double* const o
 }
 
 void launchClearCurrent(FieldState& s) {
+  if (s.currentZero) return;  // J already holds zeros
   ScopedKernel sk(s.profiler, s.stream, "clear_current");
   for (int a = 6; a < 9; ++a) fill(s.stream, s.comp[a].get(), s.compSize(a), 0.0);
+  s.currentZero = true;
 }
 
 // Recenter the slid arrays into their slack once the slack is used up.
@@ -588,6 +723,34 @@ static void recenter(FieldState& s) {
   s.zb = s.slack;
 }
 
+struct ShiftArgs {
+  double* dst[6];
+  const double* src[6];
+  int planeLen[6];
+  double* hist[12];
+  long long slotStride[12];
+  int row[12], slots[12];
+};
+
+// Insert the new k = 0 planes of the six E/B components and zero the new
+// layer of every transverse CPML history, in one launch.
+__global__ void kShiftInsert(ShiftArgs A) {
+  const int item = blockIdx.y;  // 0..5 component planes, 6.. history slots
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (item < 6) {
+    if (t < A.planeLen[item]) A.dst[item][t] = A.src[item] ? A.src[item][t] : 0.0;
+    return;
+  }
+  int rest = item - 6;
+  for (int h = 0; h < 12; ++h) {
+    if (rest < A.slots[h]) {
+      if (t < A.row[h]) A.hist[h][rest * A.slotStride[h] + t] = 0.0;
+      return;
+    }
+    rest -= A.slots[h];
+  }
+}
+
 // MovingWindowFields::Impl::shift (moving_window_fields.cpp:439-518) with
 // resetCurrent: slide every E/B component and transverse history one plane,
 // insert the reference planes (zero histories) at k = 0, clear J.
@@ -595,18 +758,29 @@ void windowShift(FieldState& s, const double* const planes[6]) {
   ScopedKernel sk(s.profiler, s.stream, "shift");
   if (s.zb == 0) recenter(s);
   s.zb -= 1;
+  ShiftArgs A{};
+  int maxLen = 0, items = 6;
   for (int a = 0; a < 6; ++a) {
-    if (planes && planes[a])
-      hipCheck(hipMemcpyAsync(s.base(a), planes[a], s.plane(a) * sizeof(double), hipMemcpyDeviceToDevice, s.stream),
-               "hipMemcpyAsync");
-    else
-      fill(s.stream, s.base(a), s.plane(a), 0.0);
+    A.dst[a] = s.base(a);
+    A.src[a] = planes ? planes[a] : nullptr;
+    A.planeLen[a] = static_cast<int>(s.plane(a));
+    maxLen = std::max(maxLen, A.planeLen[a]);
   }
-  for (int a = 6; a < 9; ++a) fill(s.stream, s.comp[a].get(), s.compSize(a), 0.0);
   for (int h = 0; h < 12; ++h) {
-    if (!s.histSlides[h] || !s.hist[h].size()) continue;
-    for (size_t slot = 0; slot < s.histSlots[h]; ++slot)
-      fill(s.stream, s.hist[h].get() + s.historyDeviceOffset(h, slot), s.histRow[h], 0.0);
+    const bool on = s.histSlides[h] && s.hist[h].size();
+    A.slots[h] = on ? static_cast<int>(s.histSlots[h]) : 0;
+    A.row[h] = static_cast<int>(s.histRow[h]);
+    A.hist[h] = on ? s.hist[h].get() + s.historyDeviceOffset(h, 0) : nullptr;
+    A.slotStride[h] = static_cast<long long>((s.histLayers[h] + s.slack) * s.histRow[h]);
+    items += A.slots[h];
+    maxLen = std::max(maxLen, A.row[h]);
+  }
+  dim3 grid(ceilDiv(maxLen, 256), static_cast<unsigned>(items));
+  kShiftInsert<<<grid, 256, 0, s.stream>>>(A);
+  launchCheck("shift insert");
+  if (!s.currentZero) {
+    for (int a = 6; a < 9; ++a) fill(s.stream, s.comp[a].get(), s.compSize(a), 0.0);
+    s.currentZero = true;
   }
 }
 

@@ -74,6 +74,10 @@ struct FieldState {
   bool inletSet = false;
   DeviceArray<double> inlet[2], shiftPlanes[6];
 
+  // True while J is known to hold zeros on the device (set by clears and
+  // shifts, dropped by any write of J), so repeated clears cost nothing.
+  bool currentZero = true;
+
   hipStream_t stream = nullptr;
   Profiler profiler;
 

@@ -1,0 +1,92 @@
+// Device particle store and the conservative-source coupling state.
+#pragma once
+
+#include "fields.hpp"
+
+namespace aprl_gpu {
+
+// Cell tile used for the per-step sort and the deposit. The extended node
+// range of a tile covers every binomial3-x tap of a chord that starts in the
+// tile and moves less than one cell per axis (always true when c*dt = dz and
+// dz <= dx, dy): x and z [t0-2, t0+T+2], y [t0-1, t0+T+1].
+constexpr int kTileX = 8, kTileY = 8, kTileZ = 4;
+constexpr int kExtX = kTileX + 5, kExtY = kTileY + 3, kExtZ = kTileZ + 5;
+constexpr int kExtNodes = kExtX * kExtY * kExtZ;
+// Sub-segments of one chord: at most one face crossing per axis per step.
+constexpr int kMaxSegments = 4;
+
+// Error bits raised by kernels; the host maps the lowest set bit onto the
+// reference's exception type and message.
+enum ErrorBit : unsigned {
+  kErrEndpointOutside = 1u << 0,   // eb_deposition.cpp:244-245 (out_of_range)
+  kErrCurrentY = 1u << 1,          // longitudinal_shape.h:185 (out_of_range)
+  kErrCurrentX = 1u << 2,          // longitudinal_shape.h:181-184 (out_of_range)
+  kErrOrbitOutside = 1u << 3,      // source_coupling.cpp:78 (out_of_range)
+  kErrMagneticOutside = 1u << 4,   // source_coupling.cpp:87 (out_of_range)
+  kErrRootNonphysical = 1u << 5,   // source_coupling_math.h:285 (runtime_error)
+  kErrRootNoConverge = 1u << 6,    // source_coupling_math.h:289 (runtime_error)
+  kErrNonfinite = 1u << 7,         // source_coupling_math.h:241 (runtime_error)
+  kErrBoundaryEvent = 1u << 8,     // boundary events are not supported yet
+  kErrStartInCpml = 1u << 9,       // particle_boundary.cpp:118-121 (out_of_range)
+  kErrStartOutside = 1u << 10,     // particle_boundary.cpp:325-330 (out_of_range)
+  kErrBeyondExit = 1u << 11,       // simulation_moving_window.cpp:387 (runtime_error)
+  kErrUpstream = 1u << 12,         // simulation.cpp:4926-4930 (runtime_error)
+  kErrCrossings = 1u << 13,        // orbit crossing budget (runtime_error)
+  kErrTileHalo = 1u << 14,         // chord longer than one cell per axis (runtime_error)
+};
+
+struct MagnetDev {
+  int type;  // 0 undulator, 1 dipole, 2 quadrupole
+  double center[3];
+  double length, period, peak, angle;
+  int gaussianFringe;
+  double cutoff, gradient, endRampPeriods, phase;
+  double waveNumber, fringeExtent, cosA, sinA;
+};
+constexpr int kMaxMagnets = 8;
+
+struct BoostDev {
+  double gamma, beta, gammaBeta, c0, labT0, labZ0, boxZ0;
+};
+
+struct ParticleStore {
+  std::size_t capacity = 0, count = 0;
+  // SoA state: x y z ux uy uz q m w, id
+  DeviceArray<double> s[9], t[9];
+  DeviceArray<uint64_t> id, idT;
+  // coupling slots (per step)
+  DeviceArray<double> slot[25];  // guess3 usedE3 usedB3 nextE3 nextB3 x1_3 u1_3 gridAbs pad
+  DeviceArray<int> ready;
+  // deposit chord table written by the push: per particle segment count,
+  // chord velocity (cells) and current scale; per segment the midpoint cell
+  // (int4) and (fx, fy, fz, width) (double4)
+  DeviceArray<int> segCount;
+  DeviceArray<double> segV;      // 6 per particle: v[3], scale[3]
+  DeviceArray<int4> segCell;     // kMaxSegments per particle
+  DeviceArray<double4> segFrac;  // kMaxSegments per particle
+  // sort
+  DeviceArray<unsigned> key, keySorted, perm, permSorted;
+  DeviceArray<unsigned char> sortTemp;
+  std::size_t sortTempBytes = 0;
+};
+
+struct CouplingState {
+  FieldState* fields = nullptr;
+  int shape = 2;
+  bool smoothX = true;
+  BoostDev frame{1.0, 0.0, 0.0, kC, 0.0, 0.0, 0.0};
+  std::vector<MagnetDev> magnets;
+  std::size_t cpmlCells[3] = {0, 0, 0};
+  double exitLabZ = 0.0;
+  bool hasExit = false;
+  DeviceArray<double> estar[3];
+  DeviceArray<int> tileBegin, tileEnd, tileSlot, occupied;
+  DeviceArray<double> partial;
+  std::size_t partialSlots = 0;
+  DeviceArray<double> reduceBlocks, reduceOut;
+  DeviceArray<unsigned> error, counter;
+  DeviceArray<MagnetDev> magnetsDev;
+  std::vector<double> hostReduce;
+};
+
+}  // namespace aprl_gpu

@@ -1,0 +1,120 @@
+"""Synthetic workloads of the benchmark configurations.
+
+``bending_radiation`` is the single-GPU headline case: 1e5 unit-weight
+electrons from a seeded 6D Gaussian (lab gamma 100, sigma_x = sigma_y =
+20 um, sigma_z = 5 um, sigma_gamma/gamma = 5e-4, normalized emittance 1 um)
+in the gamma_b = 5 boosted frame, a 64 x 64 x 512 Cowan-z moving window with
+8-cell transverse CFS-CPML, a 0.5 T lab dipole over 0.2 m, and the
+linear-binomial3 (+x) source shape. The bunch is an active box-frame
+snapshot (its lab sample is boosted with the light-front transform and its
+length stretched to the box frame), the inlet reference is the zero field,
+and no particle reaches a boundary during the timed steps.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import bunch
+
+C_SI = 299792458.0
+
+
+@dataclass
+class Workload:
+    name: str
+    dims: tuple
+    spacing: tuple
+    cpml: int
+    particles: int
+    gamma_lab: float
+    gamma_boost: float
+    sigma: tuple            # lab sigma_x, sigma_y, sigma_z [m]
+    sigma_gamma: float      # relative
+    emittance: float        # normalized [m]
+    charge_electrons: float  # represented electrons in total
+    dipole_tesla: float
+    dipole_length: float
+    seed: int
+    z_cells: float          # bunch centre, cells above the window origin
+
+    @property
+    def dt(self) -> float:
+        return self.spacing[2] / C_SI
+
+    @property
+    def cells(self) -> int:
+        return int(np.prod(self.dims))
+
+    def generate(self):
+        """Box-frame particles: position (n,3), u (n,3), charge, mass, ids."""
+        nx, ny, nz = self.dims
+        dx, dy, dz = self.spacing
+        u_lab = np.sqrt(self.gamma_lab ** 2 - 1.0)
+        gb = bunch.gamma_from_u(np.array([[0.0, 0.0, u_lab]]))  # lab gamma
+        ub = bunch.lab_to_box_u(np.array([[0.0, 0.0, u_lab]]), self.gamma_boost)
+        g_box = bunch.gamma_from_u(ub)[0]
+        stretch = gb[0] / g_box  # length of a moving bunch in the box frame
+        center = [0.5 * nx * dx, 0.5 * ny * dy, self.z_cells * dz]
+        sig_pos = [self.sigma[0], self.sigma[1], self.sigma[2] * stretch]
+        sig_u = [self.emittance / self.sigma[0], self.emittance / self.sigma[1], self.sigma_gamma * self.gamma_lab]
+        x, u, q, m, ids = bunch.seeded_gaussian(self.particles, self.seed, center, sig_pos, [0.0, 0.0, u_lab], sig_u,
+                                                self.charge_electrons)
+        u = bunch.lab_to_box_u(u, self.gamma_boost)
+        return x, u, q, m, ids
+
+
+BENDING = Workload(
+    name="bending-radiation 64x64x512 dipole, 1e5 e-",
+    dims=(64, 64, 512), spacing=(5.0e-6, 5.0e-6, 2.0e-6), cpml=8, particles=100_000,
+    gamma_lab=100.0, gamma_boost=5.0, sigma=(20e-6, 20e-6, 5e-6), sigma_gamma=5e-4, emittance=1e-6,
+    charge_electrons=100_000.0, dipole_tesla=0.5, dipole_length=0.2, seed=20241, z_cells=150.0)
+
+
+def setup_gpu(w: Workload):
+    """Field grid + window + coupling + particle store for workload w."""
+    import aprl_amd as A
+    g = A.EBGridGeometry(*w.dims, *w.spacing, w.dt)
+    fields = A.EBFieldGrid(g)
+    window = A.MovingWindowFields(fields, A.EBCPMLParameters((w.cpml, w.cpml, 0)))
+    coupling = A.ConservativeSourceCoupling(fields)
+    coupling.setFrame(w.gamma_boost)
+    # lab dipole B_y over z_lab in [-L/2, L/2]
+    coupling.addPrescribedLabMagnet("uniform-dipole", [0.0, 0.0, -0.5 * w.dipole_length], w.dipole_length,
+                                    peak_field=w.dipole_tesla, polarization_angle=0.5 * np.pi)
+    coupling.setParticleBoundary((w.cpml, w.cpml, 0))
+    x, u, q, m, ids = w.generate()
+    store = A.ParticleStore(fields, len(q))
+    store.upload(x, u, q, m, None, ids)
+    return fields, window, coupling, store
+
+
+class Stepper:
+    """Drives Simulation::solve's per-step order (simulation.cpp:518-583):
+    clear J, window begin, magnetic, coupled source step, shift."""
+
+    def __init__(self, w: Workload, fields, window, coupling, store, offset: int = 10_000):
+        self.w, self.fields, self.window, self.coupling, self.store = w, fields, window, coupling, store
+        self.step_index = 0
+        self.offset = offset
+        self.origin = np.zeros(3)
+        self.time = 0.0
+        self.trials = []
+
+    def step(self):
+        f, win = self.fields, self.window
+        n = self.step_index
+        f.clearCurrent()
+        win.beginStep(n, self.offset)
+        f.advanceMagnetic()
+        win.setInlet()
+        its, _, _ = self.coupling.step(self.store, self.origin, self.time)
+        self.trials.append(its)
+        self.offset -= 1
+        win.shift(n + 1, self.offset)
+        # Simulation::finishMovingWindowStep: origin = fma(offset, dz, ref)
+        # with the reference origin at offset 10000 (fma(a, b, 0) = a*b).
+        self.origin[2] = (self.offset - 10_000) * self.w.spacing[2]
+        self.time += self.w.dt
+        self.step_index += 1

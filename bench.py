@@ -1,0 +1,183 @@
+"""Benchmark of the MI355X conservative-source PIC step.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+One step = Simulation::solve's per-step order on the bending-radiation
+workload (64x64x512 Cowan-z moving window, 8-cell CFS-CPML, 1e5 electrons,
+0.5 T dipole, linear-binomial3 + x smoothing, FP64): clear J, magnetic
+update, conservative-source step (preview, Picard trials with gather / push /
+deposit / residual, commit), window shift. Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+WORD = 8  # FP64
+
+
+def kernel_bytes(w, name, launches_per_step, trials):
+    """Algorithmic HBM bytes of one launch (reads + writes of the state the
+    kernel must touch once): 9 words/cell magnetic, 12 words/cell electric,
+    14 words/particle push and residual gathers, 7 words/particle deposit."""
+    cells = w.cells
+    n = w.particles
+    cp = w.cpml
+    nx, ny, nz = w.dims
+    psi_b = 2 * cp * (2 * (ny * nz) + 2 * (nx * nz)) * 2  # x- and y-slab histories, ~2 terms per face
+    if name == "magnetic":
+        return WORD * (9 * cells + 2 * psi_b)
+    if name in ("electric", "electric_preview"):
+        return WORD * (12 * cells + 2 * psi_b)
+    if name in ("push", "residual"):
+        return WORD * 14 * n
+    if name == "deposit":
+        return WORD * 7 * n
+    return None
+
+
+def run_gpu(args):
+    import numpy as np
+    import aprl_amd as A
+    from aprl_amd import workloads
+    w = workloads.BENDING
+    fields, window, coupling, store = workloads.setup_gpu(w)
+    s = workloads.Stepper(w, fields, window, coupling, store)
+    for _ in range(args.warmup):
+        s.step()
+    fields.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.step()
+    fields.synchronize()
+    elapsed = time.perf_counter() - t0
+    trials = s.trials[args.warmup:]
+    # Per-kernel HIP-event durations (on the library's own stream) over a
+    # further profiled window of the same steps.
+    fields.profile(True)
+    prof_steps = max(5, min(args.steps, 20))
+    for _ in range(prof_steps):
+        s.step()
+    fields.synchronize()
+    kernels = {}
+    for name in ("magnetic", "electric_preview", "electric", "shift", "sort", "push", "deposit",
+                 "reduce_current", "residual", "clear_current"):
+        launches, ms = fields.profile_query(name)
+        if launches:
+            kernels[name] = {"launches": launches, "avg_us": 1e3 * ms / launches, "total_ms": ms}
+    fields.profile(False)
+    x, u, q, m, wt, ids = store.download()
+    return w, elapsed, trials, kernels, len(q)
+
+
+def run_cpu_baseline(w, seconds_budget=25.0):
+    """The reference's own compiled hot path (oracle/_ref, OpenMP on the
+    host cores) on the same workload for a bounded number of steps."""
+    import ctypes as C
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import harness as H
+    if not H.have_ref():
+        return None
+    lib = H.load_particles("ref")
+    threads = min(os.cpu_count() or 1, 16)
+    lib.ref_set_threads.argtypes = [C.c_int]
+    if lib.ref_set_threads(threads):
+        return None
+    nx, ny, nz = w.dims
+    dx, dy, dz = w.spacing
+    f = H.CpuFields("ref", nx, ny, nz, dx, dy, dz, w.dt, 1)
+    f.install_window((w.cpml, w.cpml, 0))
+    c = H.CpuCoupling(f, "binomial3", True, cpml_cells=(w.cpml, w.cpml, 0))
+    c.set_frame(w.gamma_boost)
+    c.add_magnet("dipole", [0.0, 0.0, -0.5 * w.dipole_length], w.dipole_length, peak=w.dipole_tesla,
+                 angle=0.5 * np.pi)
+    x, u, q, m, ids = w.generate()
+    c.set_particles(x, u, q, m, np.ones(len(q)), ids)
+    zero3 = [np.zeros(int(np.prod(H.component_shape(nx, ny, nz, a)[1:]))) for a in range(3)]
+    zero6 = [np.zeros(int(np.prod(H.component_shape(nx, ny, nz, a)[1:]))) for a in range(6)]
+    offset = 10_000
+    origin = np.zeros(3)
+    times = []
+    step = 0
+    while True:
+        t0 = time.perf_counter()
+        f.clear_current()
+        f.window_begin(step, offset, zero3)
+        f.window_advance_magnetic()
+        f.window_set_inlet(offset, step + 1, zero3[0], zero3[1])
+        c.step(origin, step * w.dt)
+        offset -= 1
+        f.window_shift(step + 1, offset, zero6)
+        origin[2] = (offset - 10_000) * dz
+        times.append(time.perf_counter() - t0)
+        step += 1
+        if step >= 2 and (sum(times[1:]) > seconds_budget or step >= 40):
+            break
+    timed = times[1:]  # the first step is warm-up
+    per_step = sum(timed) / len(timed)
+    return {"value": w.particles / per_step, "unit": "particle-steps/s", "cores": threads, "kind": "reference",
+            "cell_updates_per_s": w.cells / per_step,
+            "sample": f"{len(timed)} steps of the same workload after 1 warm-up step, reference TUs "
+                      f"(eb_field, eb_cpml, moving_window_fields, source_coupling, longitudinal_shape) "
+                      f"with OpenMP on {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (the whole window lives on one device); --gpus {args.gpus} "
+              "is not supported", file=sys.stderr)
+        sys.exit(2)
+    w, elapsed, trials, kernels, count = run_gpu(args)
+    ms = 1e3 * elapsed / args.steps
+    value = w.particles * args.steps / elapsed
+    dominant = max(kernels.items(), key=lambda kv: kv[1]["total_ms"])[0] if kernels else None
+    roofline = None
+    if dominant:
+        k = kernels[dominant]
+        steps_profiled = max(5, min(args.steps, 20))
+        b = kernel_bytes(w, dominant, k["launches"] / steps_profiled, sum(trials) / len(trials))
+        if b is not None:
+            achieved = b / (k["avg_us"] * 1e-6) / 1e9
+            roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "algorithmic_bytes": b, "avg_launch_us": round(k["avg_us"], 2)}
+    cpu = None if args.no_cpu_baseline else run_cpu_baseline(w)
+    line = {
+        "metric": "particle-steps/s + cell-updates/s, FP32-split vs FP64, % of HBM peak",
+        "value": value,
+        "unit": "particle-steps/s",
+        "cell_updates_per_s": w.cells * args.steps / elapsed,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": w.name, "grid": list(w.dims), "particles": w.particles, "form": "plain FP64",
+                   "shape": "linear-binomial3 + x-binomial3", "picard_trials_mean": sum(trials) / len(trials)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels_us": {k: round(v["avg_us"], 2) for k, v in kernels.items()},
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -113,6 +113,75 @@ aprl_status aprl_window_set_inlet(aprl_fields* fields, const double* ex, const d
 aprl_status aprl_window_shift(aprl_fields* fields, size_t new_step, ptrdiff_t new_offset,
                               const double* const* planes);
 
+/* ---------------------------------------------------------------------------
+ * Particles and the conservative-source coupling.
+ */
+typedef struct aprl_particles aprl_particles;
+typedef struct aprl_coupling aprl_coupling;
+
+/* Device store of RelativisticParticleSI (eb_particles.h:12-25) on the
+ * field grid's device and stream. Positions are absolute box coordinates in
+ * metres, u = gamma v / c. The store's order changes at every coupled step
+ * (stable sort by cell tile); ids travel with their particles. */
+aprl_status aprl_particles_create(aprl_fields* fields, size_t capacity, aprl_particles** out);
+void aprl_particles_destroy(aprl_particles* particles);
+/* position3/u3: interleaved (x,y,z) per particle; n <= capacity grows it. */
+aprl_status aprl_particles_upload(aprl_particles* particles, size_t n, const double* position3, const double* u3,
+                                  const double* charge, const double* mass, const double* weight,
+                                  const uint64_t* id);
+/* Any output pointer may be NULL. */
+aprl_status aprl_particles_download(aprl_particles* particles, double* position3, double* u3, double* charge,
+                                    double* mass, double* weight, uint64_t* id);
+size_t aprl_particles_count(const aprl_particles* particles);
+
+/* Longitudinal source shape (longitudinal_shape.h:16). */
+typedef enum aprl_shape {
+  APRL_SHAPE_QUINTIC_COMPENSATED = 0,
+  APRL_SHAPE_LINEAR_BINOMIAL3 = 2
+} aprl_shape;
+
+/* SIMagneticElement (eb_sources.h:107-144); type 0 planar undulator,
+ * 1 uniform dipole, 2 quadrupole. Lab frame, SI. */
+typedef struct aprl_magnet {
+  int type;
+  double center[3];
+  double length, period, peak_field, polarization_angle;
+  int gaussian_fringe;
+  double fringe_relative_cutoff, quadrupole_gradient, end_ramp_periods, undulator_phase;
+} aprl_magnet;
+
+/* BoostFrameTransform::setOriginsFromGamma (boostframe.h:25-28), c0 = c. */
+typedef struct aprl_boost {
+  double gamma, lab_time_origin, lab_z_origin, box_z_origin;
+} aprl_boost;
+
+typedef struct aprl_step_report {
+  unsigned iterations; /* Picard trials */
+  double force;        /* self-force defect of the last trial */
+  double work;         /* relative fixed-point work defect */
+} aprl_step_report;
+
+/* ConservativeSourceCoupling (source_coupling.h:12-41) bound to a field
+ * grid; transverse_x_binomial3 selects the production F3 x-smoothing. */
+aprl_status aprl_coupling_create(aprl_fields* fields, aprl_shape shape, int transverse_x_binomial3,
+                                 aprl_coupling** out);
+void aprl_coupling_destroy(aprl_coupling* coupling);
+aprl_status aprl_coupling_set_frame(aprl_coupling* coupling, const aprl_boost* frame);
+/* SIFieldSourceSet::addPrescribedLabMagnet (eb_sources.h:163). */
+aprl_status aprl_coupling_add_magnet(aprl_coupling* coupling, const aprl_magnet* magnet);
+/* ParticleCPMLRegion cells and the moving-window lab exit plane
+ * (particle_boundary.h:36-66; simulation_moving_window.cpp:380-395);
+ * a non-finite exit_lab_z disables the exit plane. */
+aprl_status aprl_coupling_set_particle_boundary(aprl_coupling* coupling, const size_t* cpml_cells,
+                                                double exit_lab_z);
+/* One Simulation::solveConservativeSourceStep + commitConservativeElectric
+ * (simulation_source_coupling.cpp:16-65) for every particle of the store,
+ * after clear_current / begin_step / advance_magnetic / set_inlet. origin =
+ * the grid's box origin (m); time = box time at the step start; substeps =
+ * particle substeps per field step. */
+aprl_status aprl_coupling_step(aprl_coupling* coupling, aprl_particles* particles, const double* origin,
+                               double time, unsigned substeps, aprl_step_report* report);
+
 /* Per-kernel timing with HIP events on the instance stream. enable=1 starts
  * recording every launch; query returns launches and total milliseconds for
  * a kernel name (0 launches if unknown). */

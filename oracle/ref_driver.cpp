@@ -6,6 +6,7 @@
 // Entry points mirror oracle/orc_capi.cpp with the prefix ref_. The moving
 // window's reference callback reads planes staged per (component, k, step),
 // so both builds consume identical inlet data.
+#include <exception>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -15,6 +16,7 @@
 
 #include <mpi.h>
 
+#include "compute_threads.h"
 #include "eb_cpml.h"
 #include "eb_field.h"
 #include "moving_window_fields.h"
@@ -103,6 +105,14 @@ EBCPMLParameters params(const std::size_t cells[3], double order, double R, doub
 extern "C" {
 
 const char* ref_last_error() { return lastError.c_str(); }
+
+// configureComputeThreads (compute_threads.cpp:50-91) for the CPU baseline.
+int ref_set_threads(int threads) {
+  return guarded([&] {
+    ensureMpi();
+    configureComputeThreads(MPI_THREAD_FUNNELED, threads);
+  });
+}
 
 int ref_cowan(double dx, double dy, double dz, double* out12) {
   return guarded([&] {
@@ -229,6 +239,7 @@ double ref_fields_poynting(void* h, std::size_t i, std::size_t j, std::size_t k,
 #include <unistd.h>
 
 #include "boostframe.h"
+#include "compute_threads.h"
 #include "eb_deposition.h"
 #include "eb_particles.h"
 #include "eb_sources.h"
@@ -340,10 +351,29 @@ int ref_coupling_step(void* h, const double* origin, double time, unsigned subst
     auto pushDepositAll = [&]() {
       ChargeConservingCurrentDepositor depositor(f, o, H.c->longitudinalCurrent());
       std::vector<RelativisticParticleSI> prepared = H.particles;
-      for (auto& p : prepared) {
+      for (auto& p : prepared)
         if (window && p.position[2] < o[2])
           throw std::runtime_error("An active particle is upstream of the moving window; the inlet guard did not contain its release event");
-        H.c->push(p, H.sources, H.frame, time, g.dt, substeps, nullptr, nullptr, nullptr);
+      // Preparation in batches of 2048 with OpenMP, as simulation.cpp:5041-5080.
+      const std::ptrdiff_t total = static_cast<std::ptrdiff_t>(prepared.size());
+      for (std::ptrdiff_t batch = 0; batch < total; batch += 2048) {
+        const std::ptrdiff_t count = std::min<std::ptrdiff_t>(2048, total - batch);
+        if (computeThreadCount() == 1 || count < 32) {
+          for (std::ptrdiff_t k = 0; k < count; ++k)
+            H.c->push(prepared[batch + k], H.sources, H.frame, time, g.dt, substeps, nullptr, nullptr, nullptr);
+        } else {
+          std::exception_ptr error;
+#pragma omp parallel for schedule(static) num_threads(computeThreadCount())
+          for (std::ptrdiff_t k = 0; k < count; ++k) {
+            try {
+              H.c->push(prepared[batch + k], H.sources, H.frame, time, g.dt, substeps, nullptr, nullptr, nullptr);
+            } catch (...) {
+#pragma omp critical(aprl_ref_prepare_error)
+              if (!error) error = std::current_exception();
+            }
+          }
+          if (error) std::rethrow_exception(error);
+        }
       }
       for (std::size_t n = 0; n < H.particles.size(); ++n) {
         const FieldVector<Double> start(H.particles[n].position);
