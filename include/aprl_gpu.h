@@ -5,9 +5,8 @@
  * instance's own HIP stream and returns an aprl_status; aprl_last_error()
  * gives the message of the last failure on the calling thread. Status codes
  * map one-to-one onto the reference's exception types so that a C++ shim can
- * rethrow the same type and text (SURVEY: eb_field.cpp:880-941,
- * eb_field.cpp:976-990, simulation_source_coupling.cpp:54,
- * moving_window_fields.cpp:208-209).
+ * rethrow the same type and text (eb_field.cpp:880-941, eb_field.cpp:976-990,
+ * simulation_source_coupling.cpp:54, moving_window_fields.cpp:208-209).
  *
  * Array layouts are the reference's: each Yee component is x-fastest,
  * index (k*ny + j)*nx + i, with the staggered shapes of eb_field.cpp:217-225:
