@@ -44,7 +44,8 @@ class Boost(C.Structure):
 
 
 class StepReport(C.Structure):
-    _fields_ = [("iterations", C.c_uint), ("force", C.c_double), ("work", C.c_double)]
+    _fields_ = [("iterations", C.c_uint), ("force", C.c_double), ("work", C.c_double), ("tiles", C.c_uint),
+                ("chunks", C.c_uint)]
 
 
 class AprlError(RuntimeError):

@@ -112,4 +112,5 @@ class ConservativeSourceCoupling:
         o = np.ascontiguousarray(origin, dtype=np.float64)
         r = _capi.StepReport()
         check(lib().aprl_coupling_step(self.handle, particles.handle, dptr(o), time, substeps, C.byref(r)))
+        self.last_tiles, self.last_chunks = r.tiles, r.chunks
         return r.iterations, r.force, r.work

@@ -15,12 +15,20 @@ static void upload(hipStream_t s, double* dst, const double* host, size_t n) {
   hipCheck(hipMemcpyAsync(dst, host, n * sizeof(double), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
 }
 
-static void uploadPlanes(FieldState& s, DeviceArray<double>* dst, const double* const* host, int first, int count) {
+// Upload caller planes; a NULL array or entry selects the zero plane (the
+// persistent zero plane of an E component, or NULL where the kernel itself
+// treats NULL as zero). out[a] receives the device plane to use.
+static void uploadPlanes(FieldState& s, DeviceArray<double>* dst, const double* const* host, int first, int count,
+                         const double** out) {
   for (int a = 0; a < count; ++a) {
     const int c = first + a;
-    if (dst[a].size() != s.plane(c)) dst[a].resize(s.plane(c));
-    if (host && host[a]) upload(s.stream, dst[a].get(), host[a], s.plane(c));
-    else hipCheck(hipMemsetAsync(dst[a].get(), 0, s.plane(c) * sizeof(double), s.stream), "hipMemsetAsync");
+    if (host && host[a]) {
+      if (dst[a].size() != s.plane(c)) dst[a].resize(s.plane(c));
+      upload(s.stream, dst[a].get(), host[a], s.plane(c));
+      out[a] = dst[a].get();
+    } else {
+      out[a] = c < 3 ? s.zeroPlane[c].get() : nullptr;
+    }
   }
 }
 
@@ -81,7 +89,10 @@ aprl_status aprl_fields_upload(aprl_fields* fields, int component, const double*
     checkComponent(component);
     if (!host || count != s.compSize(component)) throw std::invalid_argument("Component upload size mismatch");
     upload(s.stream, s.base(component), host, count);
-    if (component >= 6) s.currentZero = false;
+    if (component >= 6) {
+      s.currentZero = false;
+      s.footprint.valid = false;
+    }
     hipCheck(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
   });
 }
@@ -156,21 +167,15 @@ aprl_status aprl_fields_advance_magnetic(aprl_fields* fields, const double* cons
     if (s.window) {
       if (halo) throw std::invalid_argument("The moving window supplies its own electric halos");
       if (s.phase != 1) throw std::logic_error("Moving-window magnetic phase mismatch");
-      for (int a = 0; a < 3; ++a) { f.lo[a] = s.haloLo[a].get(); f.up[a] = s.zeroPlane[a].get(); }
+      for (int a = 0; a < 3; ++a) { f.lo[a] = s.loPlane[a]; f.up[a] = s.zeroPlane[a].get(); }
       launchMagnetic(s, f);
       s.phase = 2;
       return;
     }
     if (halo) {
       for (int a = 0; a < 3; ++a) {
-        if (halo[a]) {
-          uploadPlanes(s, &s.haloLo[a], &halo[a], a, 1);
-          f.lo[a] = s.haloLo[a].get();
-        }
-        if (halo[3 + a]) {
-          uploadPlanes(s, &s.haloUp[a], &halo[3 + a], a, 1);
-          f.up[a] = s.haloUp[a].get();
-        }
+        if (halo[a]) uploadPlanes(s, &s.haloLo[a], &halo[a], a, 1, &f.lo[a]);
+        if (halo[3 + a]) uploadPlanes(s, &s.haloUp[a], &halo[3 + a], a, 1, &f.up[a]);
       }
     }
     launchMagnetic(s, f);
@@ -185,7 +190,7 @@ aprl_status aprl_fields_advance_electric(aprl_fields* fields) {
     if (s.window) {
       if (s.phase != 2) throw std::logic_error("Moving-window electric phase mismatch");
       if (!s.inletSet) throw std::logic_error("Moving-window inlet planes were not supplied");
-      launchElectric(s, f, 0, none, s.inlet[0].get(), s.inlet[1].get());
+      launchElectric(s, f, 0, none, s.inletPlane[0], s.inletPlane[1]);
       s.phase = 3;
       return;
     }
@@ -212,18 +217,18 @@ aprl_status aprl_window_install(aprl_fields* fields, const aprl_cpml_parameters*
     // Re-home the existing E/B/J contents into arrays with z slack.
     DeviceArray<double> old[9];
     for (int a = 0; a < 9; ++a) old[a].swap(s.comp[a]);
-    allocateFields(s, 64);
+    allocateFields(s, std::max<std::size_t>(64, g.nz));
     for (int a = 0; a < 9; ++a)
       hipCheck(hipMemcpyAsync(s.base(a), old[a].get(), s.compSize(a) * sizeof(double), hipMemcpyDeviceToDevice,
                               s.stream),
                "hipMemcpyAsync");
     s.currentZero = false;
+    s.footprint.valid = false;
     hipCheck(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
     installCpml(s, *p, g.nz, 0, false, false);
     s.window = true;
     s.phase = 0;
     s.established = false;
-    for (int a = 0; a < 2; ++a) s.inlet[a].resize(s.plane(a));
     hipCheck(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
   });
 }
@@ -234,7 +239,7 @@ aprl_status aprl_window_begin_step(aprl_fields* fields, size_t step, ptrdiff_t o
     if (!s.window) throw std::logic_error("No moving window is installed");
     if (s.phase != 0 || (s.established && (step != s.step || offset != s.offset)))
       throw std::logic_error("Moving-window beginStep phase/origin mismatch");
-    uploadPlanes(s, s.haloLo, lower, 0, 3);
+    uploadPlanes(s, s.haloLo, lower, 0, 3, s.loPlane);
     s.step = step;
     s.offset = offset;
     s.established = true;
@@ -248,7 +253,7 @@ aprl_status aprl_window_set_inlet(aprl_fields* fields, const double* ex, const d
     FieldState& s = S(fields);
     if (!s.window) throw std::logic_error("No moving window is installed");
     const double* const host[2] = {ex, ey};
-    uploadPlanes(s, s.inlet, host, 0, 2);
+    uploadPlanes(s, s.inlet, host, 0, 2, s.inletPlane);
     s.inletSet = true;
   });
 }
@@ -259,9 +264,8 @@ aprl_status aprl_window_shift(aprl_fields* fields, size_t newStep, ptrdiff_t new
     if (!s.window) throw std::logic_error("No moving window is installed");
     if (s.phase != 3 || newStep != s.step + 1 || newOffset != s.offset - 1)
       throw std::logic_error("Moving-window requires one left shift after each complete step");
-    uploadPlanes(s, s.shiftPlanes, planes, 0, 6);
     const double* dev[6];
-    for (int a = 0; a < 6; ++a) dev[a] = s.shiftPlanes[a].get();
+    uploadPlanes(s, s.shiftPlanes, planes, 0, 6, dev);
     windowShift(s, dev);
     s.step = newStep;
     s.offset = newOffset;

@@ -93,6 +93,42 @@ class DeviceArray {
   std::size_t n_ = 0;
 };
 
+// Small page-locked host buffer for stream-ordered readbacks.
+class PinnedHost {
+ public:
+  PinnedHost() = default;
+  PinnedHost(const PinnedHost&) = delete;
+  PinnedHost& operator=(const PinnedHost&) = delete;
+  ~PinnedHost() {
+    if (p_) (void)hipHostFree(p_);
+  }
+  void* get(std::size_t bytes) {
+    if (!p_) hipCheck(hipHostMalloc(&p_, bytes < 256 ? 256 : bytes, hipHostMallocDefault), "hipHostMalloc");
+    return p_;
+  }
+
+ private:
+  void* p_ = nullptr;
+};
+
+// A timing-free HIP event, created on first use.
+class Event {
+ public:
+  Event() = default;
+  Event(const Event&) = delete;
+  Event& operator=(const Event&) = delete;
+  ~Event() {
+    if (e_) (void)hipEventDestroy(e_);
+  }
+  hipEvent_t get() {
+    if (!e_) hipCheck(hipEventCreateWithFlags(&e_, hipEventDisableTiming), "hipEventCreate");
+    return e_;
+  }
+
+ private:
+  hipEvent_t e_ = nullptr;
+};
+
 // Per-kernel HIP-event timing on one stream. Events are recorded around each
 // launch only while enabled; durations are collected lazily on query.
 class Profiler {

@@ -357,98 +357,149 @@ __device__ __forceinline__ double cpmlCorrection(const CpmlPointDev& c, double r
   return c.ikm1 * raw + h;
 }
 
-// Fused magnetic update over the union of the three B extents.
-// advanceMagneticCowan (eb_field.cpp:488-558) / advanceMagneticYee
-// (311-348) followed by EBConvolutionalPML::afterMagneticUpdate (406-468).
-template <bool COWAN>
-__global__ void __launch_bounds__(256) kMagnetic(FieldPtrs F, StepCoefs c, CpmlArgs P) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
-  const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
-  if (i > nx || j > ny || k > nz) return;
-  if (j < ny && k < nz) {  // Bx: (nx+1) x ny x nz
-    double dEzDy, dEyDz;
-    if (COWAN) {
-      dEzDy = smoothE<2, 1>(F, c, i, j + 1, k) - smoothE<2, 1>(F, c, i, j, k);
-      dEyDz = smoothE<1, 2>(F, c, i, j, k + 1) - smoothE<1, 2>(F, c, i, j, k);
-    } else {
-      dEzDy = rawE<2>(F, i, j + 1, k) - rawE<2>(F, i, j, k);
-      dEyDz = rawE<1>(F, i, j, k + 1) - rawE<1>(F, i, j, k);
-    }
-    const size_t idx = (static_cast<size_t>(k) * ny + j) * (nx + 1) + i;
-    double b = F.b[0][idx];
-    b -= c.ty * dEzDy - c.tz * dEyDz;
-    if (P.enabled) {
-      int o = P.ord[0][1][j];
-      if (o >= 0) b += (-1.0 * P.magScale[1]) * cpmlCorrection<true>(P.prof[0][1][j], dEzDy, histAt<0, 0, 1>(P, o, i, j, k, nx + 1));
-      o = P.ord[0][2][k];
-      if (o >= 0) b += (1.0 * P.magScale[2]) * cpmlCorrection<true>(P.prof[0][2][k], dEyDz, histAt<0, 0, 2>(P, o, i, j, k, nx + 1));
-    }
-    F.b[0][idx] = b;
+// Per-component magnetic updates at one node: advanceMagneticCowan
+// (eb_field.cpp:488-558) / advanceMagneticYee (311-348) followed by
+// EBConvolutionalPML::afterMagneticUpdate (406-468). dA/dB are the two curl
+// differences of the component; the CPML terms use the same differences.
+__device__ __forceinline__ size_t bIndex(const FieldPtrs& F, int a, int i, int j, int k) {
+  return (static_cast<size_t>(k) * F.n[3 + a][1] + j) * F.n[3 + a][0] + i;
+}
+__device__ __forceinline__ void finishBx(FieldPtrs& F, const StepCoefs& c, const CpmlArgs& P, int i, int j, int k,
+                                         double dEzDy, double dEyDz, const double* old = nullptr) {
+  const int nx = F.n[0][0], ny = F.n[1][1];
+  const size_t idx = (static_cast<size_t>(k) * ny + j) * (nx + 1) + i;
+  double b = old ? *old : F.b[0][idx];
+  b -= c.ty * dEzDy - c.tz * dEyDz;
+  if (P.enabled) {
+    int o = P.ord[0][1][j];
+    if (o >= 0) b += (-1.0 * P.magScale[1]) * cpmlCorrection<true>(P.prof[0][1][j], dEzDy, histAt<0, 0, 1>(P, o, i, j, k, nx + 1));
+    o = P.ord[0][2][k];
+    if (o >= 0) b += (1.0 * P.magScale[2]) * cpmlCorrection<true>(P.prof[0][2][k], dEyDz, histAt<0, 0, 2>(P, o, i, j, k, nx + 1));
   }
-  if (i < nx && k < nz) {  // By: nx x (ny+1) x nz
-    double dExDz, dEzDx;
-    if (COWAN) {
-      dExDz = smoothE<0, 2>(F, c, i, j, k + 1) - smoothE<0, 2>(F, c, i, j, k);
-      dEzDx = smoothE<2, 0>(F, c, i + 1, j, k) - smoothE<2, 0>(F, c, i, j, k);
-    } else {
-      dExDz = rawE<0>(F, i, j, k + 1) - rawE<0>(F, i, j, k);
-      dEzDx = rawE<2>(F, i + 1, j, k) - rawE<2>(F, i, j, k);
-    }
-    const size_t idx = (static_cast<size_t>(k) * (ny + 1) + j) * nx + i;
-    double b = F.b[1][idx];
-    b -= c.tz * dExDz - c.tx * dEzDx;
-    if (P.enabled) {
-      int o = P.ord[0][2][k];
-      if (o >= 0) b += (-1.0 * P.magScale[2]) * cpmlCorrection<true>(P.prof[0][2][k], dExDz, histAt<0, 1, 2>(P, o, i, j, k, nx));
-      o = P.ord[0][0][i];
-      if (o >= 0) b += (1.0 * P.magScale[0]) * cpmlCorrection<true>(P.prof[0][0][i], dEzDx, histAt<0, 1, 0>(P, o, i, j, k, nx));
-    }
-    F.b[1][idx] = b;
+  F.b[0][idx] = b;
+}
+__device__ __forceinline__ void finishBy(FieldPtrs& F, const StepCoefs& c, const CpmlArgs& P, int i, int j, int k,
+                                         double dExDz, double dEzDx, const double* old = nullptr) {
+  const int nx = F.n[0][0], ny = F.n[1][1];
+  const size_t idx = (static_cast<size_t>(k) * (ny + 1) + j) * nx + i;
+  double b = old ? *old : F.b[1][idx];
+  b -= c.tz * dExDz - c.tx * dEzDx;
+  if (P.enabled) {
+    int o = P.ord[0][2][k];
+    if (o >= 0) b += (-1.0 * P.magScale[2]) * cpmlCorrection<true>(P.prof[0][2][k], dExDz, histAt<0, 1, 2>(P, o, i, j, k, nx));
+    o = P.ord[0][0][i];
+    if (o >= 0) b += (1.0 * P.magScale[0]) * cpmlCorrection<true>(P.prof[0][0][i], dEzDx, histAt<0, 1, 0>(P, o, i, j, k, nx));
   }
-  if (i < nx && j < ny) {  // Bz: nx x ny x (nz+1)
-    double dEyDx, dExDy;
-    if (COWAN) {
-      dEyDx = smoothE<1, 0>(F, c, i + 1, j, k) - smoothE<1, 0>(F, c, i, j, k);
-      dExDy = smoothE<0, 1>(F, c, i, j + 1, k) - smoothE<0, 1>(F, c, i, j, k);
-    } else {
-      dEyDx = rawE<1>(F, i + 1, j, k) - rawE<1>(F, i, j, k);
-      dExDy = rawE<0>(F, i, j + 1, k) - rawE<0>(F, i, j, k);
-    }
-    const size_t idx = (static_cast<size_t>(k) * ny + j) * nx + i;
-    double b = F.b[2][idx];
-    b -= c.tx * dEyDx - c.ty * dExDy;
-    if (P.enabled) {
-      int o = P.ord[0][0][i];
-      if (o >= 0) b += (-1.0 * P.magScale[0]) * cpmlCorrection<true>(P.prof[0][0][i], dEyDx, histAt<0, 2, 0>(P, o, i, j, k, nx));
-      o = P.ord[0][1][j];
-      if (o >= 0) b += (1.0 * P.magScale[1]) * cpmlCorrection<true>(P.prof[0][1][j], dExDy, histAt<0, 2, 1>(P, o, i, j, k, nx));
-    }
-    F.b[2][idx] = b;
+  F.b[1][idx] = b;
+}
+__device__ __forceinline__ void finishBz(FieldPtrs& F, const StepCoefs& c, const CpmlArgs& P, int i, int j, int k,
+                                         double dEyDx, double dExDy, const double* old = nullptr) {
+  const int nx = F.n[0][0], ny = F.n[1][1];
+  const size_t idx = (static_cast<size_t>(k) * ny + j) * nx + i;
+  double b = old ? *old : F.b[2][idx];
+  b -= c.tx * dEyDx - c.ty * dExDy;
+  if (P.enabled) {
+    int o = P.ord[0][0][i];
+    if (o >= 0) b += (-1.0 * P.magScale[0]) * cpmlCorrection<true>(P.prof[0][0][i], dEyDx, histAt<0, 2, 0>(P, o, i, j, k, nx));
+    o = P.ord[0][1][j];
+    if (o >= 0) b += (1.0 * P.magScale[1]) * cpmlCorrection<true>(P.prof[0][1][j], dExDy, histAt<0, 2, 1>(P, o, i, j, k, nx));
   }
+  F.b[2][idx] = b;
 }
 
-// z-marching LDS version of the Cowan magnetic update. A block owns a
-// (kMX x kMY) column of B nodes and kMZ consecutive z planes; E is staged in
-// a three-plane rolling buffer of electricValue() samples (reflection / halo
-// / PEC parity applied while loading), so every smoothed sample is the same
-// nine products in the same order as smoothedElectric().
-constexpr int kMX = 32, kMY = 8, kMZ = 16;
+// One thread per node over the union of the three B extents (Yee; and the
+// Cowan edge nodes Bx(i = nx), By(j = ny) when EDGES).
+template <bool COWAN>
+__device__ __forceinline__ void magneticBx(FieldPtrs& F, const StepCoefs& c, const CpmlArgs& P, int i, int j, int k) {
+  double dEzDy, dEyDz;
+  if (COWAN) {
+    dEzDy = smoothE<2, 1>(F, c, i, j + 1, k) - smoothE<2, 1>(F, c, i, j, k);
+    dEyDz = smoothE<1, 2>(F, c, i, j, k + 1) - smoothE<1, 2>(F, c, i, j, k);
+  } else {
+    dEzDy = rawE<2>(F, i, j + 1, k) - rawE<2>(F, i, j, k);
+    dEyDz = rawE<1>(F, i, j, k + 1) - rawE<1>(F, i, j, k);
+  }
+  finishBx(F, c, P, i, j, k, dEzDy, dEyDz);
+}
+template <bool COWAN>
+__device__ __forceinline__ void magneticBy(FieldPtrs& F, const StepCoefs& c, const CpmlArgs& P, int i, int j, int k) {
+  double dExDz, dEzDx;
+  if (COWAN) {
+    dExDz = smoothE<0, 2>(F, c, i, j, k + 1) - smoothE<0, 2>(F, c, i, j, k);
+    dEzDx = smoothE<2, 0>(F, c, i + 1, j, k) - smoothE<2, 0>(F, c, i, j, k);
+  } else {
+    dExDz = rawE<0>(F, i, j, k + 1) - rawE<0>(F, i, j, k);
+    dEzDx = rawE<2>(F, i + 1, j, k) - rawE<2>(F, i, j, k);
+  }
+  finishBy(F, c, P, i, j, k, dExDz, dEzDx);
+}
+
+__global__ void __launch_bounds__(256) kMagneticYee(FieldPtrs F, StepCoefs c, CpmlArgs P) {
+  const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t % (nx + 1), j = t / (nx + 1);
+  const int k = blockIdx.y;
+  if (j > ny || k > nz) return;
+  if (j < ny && k < nz) magneticBx<false>(F, c, P, i, j, k);
+  if (i < nx && k < nz) magneticBy<false>(F, c, P, i, j, k);
+  if (i < nx && j < ny)
+    finishBz(F, c, P, i, j, k, rawE<1>(F, i + 1, j, k) - rawE<1>(F, i, j, k), rawE<0>(F, i, j + 1, k) - rawE<0>(F, i, j, k));
+}
+
+// Cowan edge nodes the tiles do not cover: Bx at i = nx and By at j = ny.
+__global__ void __launch_bounds__(256) kMagneticCowanEdges(FieldPtrs F, StepCoefs c, CpmlArgs P) {
+  const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ny * nz) {
+    magneticBx<true>(F, c, P, nx, t % ny, t / ny);
+    return;
+  }
+  const int u = t - ny * nz;
+  if (u < nx * nz) magneticBy<true>(F, c, P, u % nx, ny, u / nx);
+}
+
+// z-marching LDS version of the Cowan magnetic update for i < nx, j < ny. A
+// block owns a (kMX x kMY) column of nodes and kMZ consecutive z planes. E
+// is staged in a three-plane rolling buffer of electricValue() samples
+// (reflection / halo / PEC parity applied while loading); per plane the six
+// smoothed fields the curl needs are computed once into LDS (each is the
+// same nine products in the same order as smoothedElectric()), then every
+// thread finishes its Bx, By and Bz nodes.
+constexpr int kMX = 32, kMY = 8, kMZ = 8;
 constexpr int kLX = kMX + 2, kLY = kMY + 2;
 
+// electricValue() samples of one staged plane: lane t holds slots
+// t + r * blockDim for r < kPlaneRegs.
+constexpr int kPlaneRegs = (kLX * kLY + kMX * kMY - 1) / (kMX * kMY);
 template <int A>
-__device__ __forceinline__ void stagePlane(const FieldPtrs& F, double* dst, int i0, int j0, int kk) {
+__device__ __forceinline__ void loadPlane(const FieldPtrs& F, double reg[kPlaneRegs], int i0, int j0, int kk) {
   const int n0 = F.n[A][0], n1 = F.n[A][1], n2 = F.n[A][2];
   const bool zin = kk >= 0 && kk < n2;
-  for (int t = threadIdx.y * kMX + threadIdx.x; t < kLX * kLY; t += kMX * kMY) {
+#pragma unroll
+  for (int r = 0; r < kPlaneRegs; ++r) {
+    const int t = static_cast<int>(threadIdx.x) + r * kMX * kMY;
     const int li = t % kLX, lj = t / kLX;
     const int i = i0 - 1 + li, j = j0 - 1 + lj;
-    double v;
-    if (zin && i >= 0 && i < n0 && j >= 0 && j < n1) v = F.e[A][(static_cast<size_t>(kk) * n1 + j) * n0 + i];
-    else v = evalE<A>(F, i, j, kk);
-    dst[t] = v;
+    double v = 0.0;  // nodes more than one beyond the array feed no B node
+    if (t < kLX * kLY) {
+      if (zin && i >= 0 && i < n0 && j >= 0 && j < n1) v = F.e[A][(static_cast<size_t>(kk) * n1 + j) * n0 + i];
+      else if (i >= -1 && i <= n0 && j >= -1 && j <= n1) v = evalE<A>(F, i, j, kk);
+    }
+    reg[r] = v;
   }
+}
+__device__ __forceinline__ void storePlane(double* dst, const double reg[kPlaneRegs]) {
+#pragma unroll
+  for (int r = 0; r < kPlaneRegs; ++r) {
+    const int t = static_cast<int>(threadIdx.x) + r * kMX * kMY;
+    if (t < kLX * kLY) dst[t] = reg[r];
+  }
+}
+template <int A>
+__device__ __forceinline__ void stagePlane(const FieldPtrs& F, double* dst, int i0, int j0, int kk) {
+  double reg[kPlaneRegs];
+  loadPlane<A>(F, reg, i0, j0, kk);
+  storePlane(dst, reg);
 }
 
 // Smoothed sample from the staged planes; (li, lj) local, s0/s1/s2 the
@@ -480,80 +531,101 @@ __device__ __forceinline__ double smoothL(const double* const pl[3], const StepC
   return r;
 }
 
+// Smoothed fields of one plane k (nodes relative to the tile origin):
+// SyEz, SyEx: S_y at (a < kMX, b <= kMY); SxEz, SxEy: S_x at (a <= kMX,
+// b < kMY); SzEy, SzEx: S_z at plane k+1 (a < kMX, b < kMY), two rolling slots.
+constexpr int kSyN = kMX * (kMY + 1), kSxN = (kMX + 1) * kMY, kSzN = kMX * kMY;
+constexpr int kSmoothItems = 2 * kSyN + 2 * kSxN + 2 * kSzN;
+
 __global__ void __launch_bounds__(kMX * kMY) kMagneticCowanTiled(FieldPtrs F, StepCoefs c, CpmlArgs P) {
   __shared__ double buf[3][3][kLX * kLY];  // [component][slot][plane]
+  __shared__ double syEz[kSyN], syEx[kSyN], sxEz[kSxN], sxEy[kSxN];
+  __shared__ double szEy[2][kSzN], szEx[2][kSzN];
   const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
   const int i0 = blockIdx.x * kMX, j0 = blockIdx.y * kMY;
   const int k0 = blockIdx.z * kMZ, k1 = min(k0 + kMZ, nz + 1);
-  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
-  const int li = threadIdx.x + 1, lj = threadIdx.y + 1;
+  const int tx = static_cast<int>(threadIdx.x) % kMX, ty = static_cast<int>(threadIdx.x) / kMX;
+  const int i = i0 + tx, j = j0 + ty;
   auto slot = [](int kk) { return ((kk % 3) + 3) % 3; };
+  auto planes = [&](int comp, int kk, const double* out[3]) {
+    out[0] = buf[comp][slot(kk - 1)];
+    out[1] = buf[comp][slot(kk)];
+    out[2] = buf[comp][slot(kk + 1)];
+  };
   stagePlane<0>(F, buf[0][slot(k0 - 1)], i0, j0, k0 - 1);
   stagePlane<1>(F, buf[1][slot(k0 - 1)], i0, j0, k0 - 1);
   stagePlane<2>(F, buf[2][slot(k0 - 1)], i0, j0, k0 - 1);
   stagePlane<0>(F, buf[0][slot(k0)], i0, j0, k0);
   stagePlane<1>(F, buf[1][slot(k0)], i0, j0, k0);
   stagePlane<2>(F, buf[2][slot(k0)], i0, j0, k0);
+  __syncthreads();
+  {  // S_z of plane k0 into rolling slot k0 & 1
+    const double* px[3] = {buf[0][slot(k0)], buf[0][slot(k0)], buf[0][slot(k0)]};
+    const double* py[3] = {buf[1][slot(k0)], buf[1][slot(k0)], buf[1][slot(k0)]};
+    for (int t = threadIdx.x; t < 2 * kSzN; t += blockDim.x) {
+      const int u = t % kSzN, a = u % kMX, b = u / kMX;
+      if (t < kSzN) szEy[k0 & 1][u] = smoothL<2>(py, c, a + 1, b + 1);
+      else szEx[k0 & 1][u] = smoothL<2>(px, c, a + 1, b + 1);
+    }
+  }
+  // plane k+1 of E is loaded into registers one iteration ahead, so its
+  // global latency overlaps the smoothing and the B update of plane k
+  double rx[kPlaneRegs], ry[kPlaneRegs], rz[kPlaneRegs];
+  loadPlane<0>(F, rx, i0, j0, k0 + 1);
+  loadPlane<1>(F, ry, i0, j0, k0 + 1);
+  loadPlane<2>(F, rz, i0, j0, k0 + 1);
+  const bool mine = i < nx && j < ny;
   for (int k = k0; k < k1; ++k) {
-    stagePlane<0>(F, buf[0][slot(k + 1)], i0, j0, k + 1);
-    stagePlane<1>(F, buf[1][slot(k + 1)], i0, j0, k + 1);
-    stagePlane<2>(F, buf[2][slot(k + 1)], i0, j0, k + 1);
+    storePlane(buf[0][slot(k + 1)], rx);
+    storePlane(buf[1][slot(k + 1)], ry);
+    storePlane(buf[2][slot(k + 1)], rz);
+    if (k + 1 < k1) {
+      loadPlane<0>(F, rx, i0, j0, k + 2);
+      loadPlane<1>(F, ry, i0, j0, k + 2);
+      loadPlane<2>(F, rz, i0, j0, k + 2);
+    }
+    double bOld[3] = {0.0, 0.0, 0.0};
+    if (mine) {
+      if (k < nz) {
+        bOld[0] = F.b[0][bIndex(F, 0, i, j, k)];
+        bOld[1] = F.b[1][bIndex(F, 1, i, j, k)];
+      }
+      bOld[2] = F.b[2][bIndex(F, 2, i, j, k)];
+    }
     __syncthreads();
-    const double* ex[3] = {buf[0][slot(k - 1)], buf[0][slot(k)], buf[0][slot(k + 1)]};
-    const double* ey[3] = {buf[1][slot(k - 1)], buf[1][slot(k)], buf[1][slot(k + 1)]};
-    const double* ez[3] = {buf[2][slot(k - 1)], buf[2][slot(k)], buf[2][slot(k + 1)]};
-    const double* ey1[3] = {ey[1], ey[2], ey[2]};  // planes k, k+1 as "k-1, k" for z-shifted samples
-    const double* ex1[3] = {ex[1], ex[2], ex[2]};
-    if (i <= nx && j < ny && k < nz) {  // Bx
-      const double dEzDy = smoothL<1>(ez, c, li, lj + 1) - smoothL<1>(ez, c, li, lj);
-      // S(ey, d=z) is x-y smoothing: only its own plane is read.
-      const double* eyk[3] = {ey[1], ey[1], ey[1]};
-      const double* eyk1[3] = {ey[2], ey[2], ey[2]};
-      const double dEyDz = smoothL<2>(eyk1, c, li, lj) - smoothL<2>(eyk, c, li, lj);
-      const size_t idx = (static_cast<size_t>(k) * ny + j) * (nx + 1) + i;
-      double b = F.b[0][idx];
-      b -= c.ty * dEzDy - c.tz * dEyDz;
-      if (P.enabled) {
-        int o = P.ord[0][1][j];
-        if (o >= 0) b += (-1.0 * P.magScale[1]) * cpmlCorrection<true>(P.prof[0][1][j], dEzDy, histAt<0, 0, 1>(P, o, i, j, k, nx + 1));
-        o = P.ord[0][2][k];
-        if (o >= 0) b += (1.0 * P.magScale[2]) * cpmlCorrection<true>(P.prof[0][2][k], dEyDz, histAt<0, 0, 2>(P, o, i, j, k, nx + 1));
+    {
+      const double *ex[3], *ey[3], *ez[3];
+      planes(0, k, ex);
+      planes(1, k, ey);
+      planes(2, k, ez);
+      const double* ex1[3] = {ex[2], ex[2], ex[2]};
+      const double* ey1[3] = {ey[2], ey[2], ey[2]};
+      const int nk = (k + 1) & 1;
+      for (int t = threadIdx.x; t < kSmoothItems; t += blockDim.x) {
+        int u = t;
+        if (u < kSyN) { syEz[u] = smoothL<1>(ez, c, u % kMX + 1, u / kMX + 1); continue; }
+        u -= kSyN;
+        if (u < kSyN) { syEx[u] = smoothL<1>(ex, c, u % kMX + 1, u / kMX + 1); continue; }
+        u -= kSyN;
+        if (u < kSxN) { sxEz[u] = smoothL<0>(ez, c, u % (kMX + 1) + 1, u / (kMX + 1) + 1); continue; }
+        u -= kSxN;
+        if (u < kSxN) { sxEy[u] = smoothL<0>(ey, c, u % (kMX + 1) + 1, u / (kMX + 1) + 1); continue; }
+        u -= kSxN;
+        if (u < kSzN) { szEy[nk][u] = smoothL<2>(ey1, c, u % kMX + 1, u / kMX + 1); continue; }
+        u -= kSzN;
+        szEx[nk][u] = smoothL<2>(ex1, c, u % kMX + 1, u / kMX + 1);
       }
-      F.b[0][idx] = b;
     }
-    if (i < nx && j <= ny && k < nz) {  // By
-      const double* exk[3] = {ex[1], ex[1], ex[1]};
-      const double* exk1[3] = {ex[2], ex[2], ex[2]};
-      const double dExDz = smoothL<2>(exk1, c, li, lj) - smoothL<2>(exk, c, li, lj);
-      const double dEzDx = smoothL<0>(ez, c, li + 1, lj) - smoothL<0>(ez, c, li, lj);
-      const size_t idx = (static_cast<size_t>(k) * (ny + 1) + j) * nx + i;
-      double b = F.b[1][idx];
-      b -= c.tz * dExDz - c.tx * dEzDx;
-      if (P.enabled) {
-        int o = P.ord[0][2][k];
-        if (o >= 0) b += (-1.0 * P.magScale[2]) * cpmlCorrection<true>(P.prof[0][2][k], dExDz, histAt<0, 1, 2>(P, o, i, j, k, nx));
-        o = P.ord[0][0][i];
-        if (o >= 0) b += (1.0 * P.magScale[0]) * cpmlCorrection<true>(P.prof[0][0][i], dEzDx, histAt<0, 1, 0>(P, o, i, j, k, nx));
-      }
-      F.b[1][idx] = b;
-    }
-    if (i < nx && j < ny) {  // Bz, k in [0, nz]
-      const double dEyDx = smoothL<0>(ey, c, li + 1, lj) - smoothL<0>(ey, c, li, lj);
-      const double dExDy = smoothL<1>(ex, c, li, lj + 1) - smoothL<1>(ex, c, li, lj);
-      const size_t idx = (static_cast<size_t>(k) * ny + j) * nx + i;
-      double b = F.b[2][idx];
-      b -= c.tx * dEyDx - c.ty * dExDy;
-      if (P.enabled) {
-        int o = P.ord[0][0][i];
-        if (o >= 0) b += (-1.0 * P.magScale[0]) * cpmlCorrection<true>(P.prof[0][0][i], dEyDx, histAt<0, 2, 0>(P, o, i, j, k, nx));
-        o = P.ord[0][1][j];
-        if (o >= 0) b += (1.0 * P.magScale[1]) * cpmlCorrection<true>(P.prof[0][1][j], dExDy, histAt<0, 2, 1>(P, o, i, j, k, nx));
-      }
-      F.b[2][idx] = b;
-    }
-    (void)ey1;
-    (void)ex1;
     __syncthreads();
+    if (mine) {
+      const int ck = k & 1, nk = (k + 1) & 1;
+      const int sy = ty * kMX + tx, sx = ty * (kMX + 1) + tx, sz = ty * kMX + tx;
+      if (k < nz) {
+        finishBx(F, c, P, i, j, k, syEz[sy + kMX] - syEz[sy], szEy[nk][sz] - szEy[ck][sz], &bOld[0]);
+        finishBy(F, c, P, i, j, k, szEx[nk][sz] - szEx[ck][sz], sxEz[sx + 1] - sxEz[sx], &bOld[1]);
+      }
+      finishBz(F, c, P, i, j, k, sxEy[sx + 1] - sxEy[sx], syEx[sy + kMX] - syEx[sy], &bOld[2]);
+    }
   }
 }
 
@@ -570,11 +642,12 @@ __device__ __forceinline__ double rd(const double* p, int nx, int ny, int i, int
 template <int MODE>
 __global__ void __launch_bounds__(256) kElectric(FieldPtrs F, StepCoefs c, CpmlArgs P, double* ox, double* oy,
                                                   double* oz, const double* inletX, const double* inletY) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
+  // one thread per node of the (nx+1) x (ny+1) plane k = blockIdx.y
   const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
-  if (i > nx || j > ny || k > nz) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t % (nx + 1), j = t / (nx + 1);
+  const int k = blockIdx.y;
+  if (j > ny || k > nz) return;
   constexpr bool W = MODE == 0;
   const double* bx = F.b[0];
   const double* by = F.b[1];
@@ -658,15 +731,14 @@ static void fill(hipStream_t s, double* p, size_t n, double v) {
 void launchMagnetic(FieldState& s, const FieldPtrs& f) {
   const StepCoefs c = s.coefs();
   const CpmlArgs P = s.cpmlArgs();
-  dim3 block(64, 4, 1);
-  dim3 grid(ceilDiv(s.g.nx + 1, 64), ceilDiv(s.g.ny + 1, 4), static_cast<unsigned>(s.g.nz + 1));
   ScopedKernel sk(s.profiler, s.stream, "magnetic");
   if (s.g.solver == 1) {
-    dim3 tb(kMX, kMY, 1);
-    dim3 tg(ceilDiv(s.g.nx + 1, kMX), ceilDiv(s.g.ny + 1, kMY), ceilDiv(s.g.nz + 1, kMZ));
-    kMagneticCowanTiled<<<tg, tb, 0, s.stream>>>(f, c, P);
+    dim3 tg(ceilDiv(s.g.nx, kMX), ceilDiv(s.g.ny, kMY), ceilDiv(s.g.nz + 1, kMZ));
+    kMagneticCowanTiled<<<tg, kMX * kMY, 0, s.stream>>>(f, c, P);
+    kMagneticCowanEdges<<<ceilDiv((s.g.nx + s.g.ny) * s.g.nz, 256), 256, 0, s.stream>>>(f, c, P);
   } else {
-    kMagnetic<false><<<grid, block, 0, s.stream>>>(f, c, P);
+    dim3 grid(ceilDiv((s.g.nx + 1) * (s.g.ny + 1), 256), static_cast<unsigned>(s.g.nz + 1));
+    kMagneticYee<<<grid, 256, 0, s.stream>>>(f, c, P);
   }
   launchCheck("magnetic");
 }
@@ -679,8 +751,8 @@ void launchElectric(FieldState& s, const FieldPtrs& f, int mode, double* const o
     P.lowerZ = s.lowerZ;
     P.upperZ = s.upperZ;
   }
-  dim3 block(64, 4, 1);
-  dim3 grid(ceilDiv(s.g.nx + 1, 64), ceilDiv(s.g.ny + 1, 4), static_cast<unsigned>(s.g.nz + 1));
+  dim3 block(256, 1, 1);
+  dim3 grid(ceilDiv((s.g.nx + 1) * (s.g.ny + 1), 256), static_cast<unsigned>(s.g.nz + 1));
   if (mode == 0) {
     ScopedKernel sk(s.profiler, s.stream, "electric");
     kElectric<0><<<grid, block, 0, s.stream>>>(f, c, P, nullptr, nullptr, nullptr, inletX, inletY);
@@ -691,11 +763,44 @@ void launchElectric(FieldState& s, const FieldPtrs& f, int mode, double* const o
   launchCheck("electric");
 }
 
+// Zero the owned J nodes of every listed cell tile (the tiles with an
+// occupied neighbour tile: the only nodes a tile deposition can write).
+__global__ void __launch_bounds__(256) kClearCurrentTiles(double* jx, double* jy, double* jz, FieldPtrs F,
+                                                          FieldState::CurrentFootprint fp) {
+  const int tile = fp.active[blockIdx.x];
+  const int tx = tile % fp.ntx, ty = (tile / fp.ntx) % fp.nty, tz = tile / (fp.ntx * fp.nty);
+  for (int d = 0; d < 3; ++d) {
+    const int n0 = F.n[6 + d][0], n1 = F.n[6 + d][1], n2 = F.n[6 + d][2];
+    const int i0 = tx * fp.tx, i1 = tx == fp.ntx - 1 ? n0 : min(n0, i0 + fp.tx);
+    const int j0 = ty * fp.ty, j1 = ty == fp.nty - 1 ? n1 : min(n1, j0 + fp.ty);
+    const int k0 = tz * fp.tz, k1 = tz == fp.ntz - 1 ? n2 : min(n2, k0 + fp.tz);
+    const int wx = i1 - i0, wy = j1 - j0, wz = k1 - k0;
+    double* dst = d == 0 ? jx : (d == 1 ? jy : jz);
+    for (int t = threadIdx.x; t < wx * wy * wz; t += blockDim.x) {
+      const int i = i0 + t % wx, j = j0 + (t / wx) % wy, k = k0 + t / (wx * wy);
+      dst[(static_cast<size_t>(k) * n1 + j) * n0 + i] = 0.0;
+    }
+  }
+}
+
+static void clearCurrentNow(FieldState& s) {
+  if (s.footprint.valid) {
+    const FieldState::CurrentFootprint& fp = s.footprint;
+    if (fp.count > 0) {
+      kClearCurrentTiles<<<fp.count, 256, 0, s.stream>>>(s.base(6), s.base(7), s.base(8), s.ptrs(), fp);
+      launchCheck("clear current tiles");
+    }
+  } else {
+    for (int a = 6; a < 9; ++a) fill(s.stream, s.comp[a].get(), s.compSize(a), 0.0);
+  }
+  s.currentZero = true;
+  s.footprint.valid = false;
+}
+
 void launchClearCurrent(FieldState& s) {
   if (s.currentZero) return;  // J already holds zeros
   ScopedKernel sk(s.profiler, s.stream, "clear_current");
-  for (int a = 6; a < 9; ++a) fill(s.stream, s.comp[a].get(), s.compSize(a), 0.0);
-  s.currentZero = true;
+  clearCurrentNow(s);
 }
 
 // Recenter the slid arrays into their slack once the slack is used up.
@@ -778,10 +883,7 @@ void windowShift(FieldState& s, const double* const planes[6]) {
   dim3 grid(ceilDiv(maxLen, 256), static_cast<unsigned>(items));
   kShiftInsert<<<grid, 256, 0, s.stream>>>(A);
   launchCheck("shift insert");
-  if (!s.currentZero) {
-    for (int a = 6; a < 9; ++a) fill(s.stream, s.comp[a].get(), s.compSize(a), 0.0);
-    s.currentZero = true;
-  }
+  if (!s.currentZero) clearCurrentNow(s);
 }
 
 }  // namespace aprl_gpu

@@ -73,10 +73,20 @@ struct FieldState {
   int phase = 0;  // 0 Idle, 1 Magnetic, 2 Electric, 3 Complete
   bool inletSet = false;
   DeviceArray<double> inlet[2], shiftPlanes[6];
+  const double* loPlane[3] = {nullptr, nullptr, nullptr};  // lower E halo planes in use
+  const double* inletPlane[2] = {nullptr, nullptr};         // inlet Ex, Ey planes in use
 
   // True while J is known to hold zeros on the device (set by clears and
   // shifts, dropped by any write of J), so repeated clears cost nothing.
   bool currentZero = true;
+  // Tile footprint of the last deposit: J is zero outside the cell tiles
+  // with an occupied neighbour tile, so clearing it touches only those.
+  struct CurrentFootprint {
+    bool valid = false;
+    const int* active = nullptr;  // tiles with an occupied neighbour tile
+    int count = 0;
+    int ntx = 0, nty = 0, ntz = 0, tx = 0, ty = 0, tz = 0;
+  } footprint;
 
   hipStream_t stream = nullptr;
   Profiler profiler;

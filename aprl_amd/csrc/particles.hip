@@ -7,6 +7,7 @@
 // / longitudinal_shape.h in the reference's association order (contraction
 // off), so with identical field inputs a push is bitwise the reference's.
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -21,6 +22,7 @@ namespace aprl_gpu {
 // Kernel argument blocks.
 
 struct GArgs {
+  const double* esum[3];   // E_cand + E_old on the active tiles' nodes
   const double* estar[3];  // zero-current preview E*
   const double* eold[3];   // E^n
   const double* j[3];      // current of the trial
@@ -37,15 +39,11 @@ struct PArgs {
   double* u[3];
   const double* q;
   const double* m;
-  double* guess[3];
-  double* usedE[3];
-  double* usedB[3];
-  double* nextE[3];
-  double* nextB[3];
+  double* fin[6];   // fields the trial pushes with: Ex Ey Ez Bx By Bz
+  double* fout[6];  // next fields of the residual gather
   double* x1[3];
   double* u1[3];
   double* gridAbs;
-  int* ready;
   int* segCount;
   double* segV;
   int4* segCell;
@@ -294,10 +292,101 @@ __device__ __forceinline__ double eoldAt(const GArgs& G, int d, int i, int j, in
   return G.eold[d][(static_cast<size_t>(k) * G.ne[d][1] + j) * G.ne[d][0] + i];
 }
 
+// Extended node box of a cell tile: every binomial3-x tap of a chord that
+// starts in the tile and moves less than one cell per axis lies inside.
+struct TileBox {
+  int x0, y0, z0;
+  __device__ int local(int i, int j, int k, unsigned& err) const {
+    const unsigned li = static_cast<unsigned>(i - x0), lj = static_cast<unsigned>(j - y0),
+                   lk = static_cast<unsigned>(k - z0);
+    if (li >= static_cast<unsigned>(kExtX) || lj >= static_cast<unsigned>(kExtY) || lk >= static_cast<unsigned>(kExtZ)) {
+      err |= kErrTileHalo;
+      return -1;
+    }
+    return static_cast<int>((lk * kExtY + lj) * kExtX + li);
+  }
+};
+__device__ __forceinline__ TileBox tileBox(int tile, int ntx, int nty) {
+  const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+  return TileBox{tx * kTileX - 2, ty * kTileY - 1, tz * kTileZ - 2};
+}
+
+// E_cand + E_old at an in-range node, and B at a clamped in-range node,
+// read from LDS copies of a tile box.
+struct StagedFields {
+  const double* e;
+  const double* b;
+  TileBox box;
+  __device__ double esum(int d, int i, int j, int k, unsigned& err) const {
+    const int l = box.local(i, j, k, err);
+    return l < 0 ? 0.0 : e[d * kExtNodes + l];
+  }
+  __device__ double bval(int d, int i, int j, int k, unsigned& err) const {
+    const int l = box.local(i, j, k, err);
+    return l < 0 ? 0.0 : b[d * kExtNodes + l];
+  }
+};
+
+// Copy a tile box of E_cand + E_old and of B into LDS (zero outside the
+// component ranges; such nodes are never read). All loads of a lane are
+// issued before the first LDS store.
+constexpr int kStageThreads = 2 * kChunk;
+constexpr int kStageIters = (3 * kExtNodes + kStageThreads - 1) / kStageThreads;
+__device__ void stageTile(const GArgs& G, const TileBox& box, double* es, double* bs) {
+  double ev[kStageIters], bv[kStageIters];
+#pragma unroll
+  for (int r = 0; r < kStageIters; ++r) {
+    const int t = static_cast<int>(threadIdx.x) + r * kStageThreads;
+    ev[r] = 0.0;
+    bv[r] = 0.0;
+    if (t < 3 * kExtNodes) {
+      const int d = t / kExtNodes, q0 = t - d * kExtNodes;
+      const int li = q0 % kExtX, q = q0 / kExtX, lj = q % kExtY, lk = q / kExtY;
+      const int i = box.x0 + li, j = box.y0 + lj, k = box.z0 + lk;
+      if (i >= 0 && j >= 0 && k >= 0) {
+        if (i < G.ne[d][0] && j < G.ne[d][1] && k < G.ne[d][2])
+          ev[r] = G.esum[d][(static_cast<size_t>(k) * G.ne[d][1] + j) * G.ne[d][0] + i];
+        if (i < G.nb[d][0] && j < G.nb[d][1] && k < G.nb[d][2])
+          bv[r] = G.b[d][(static_cast<size_t>(k) * G.nb[d][1] + j) * G.nb[d][0] + i];
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kStageIters; ++r) {
+    const int t = static_cast<int>(threadIdx.x) + r * kStageThreads;
+    if (t < 3 * kExtNodes) {
+      es[t] = ev[r];
+      bs[t] = bv[r];
+    }
+  }
+}
+
+// E_cand + E_old on the owned nodes of every active tile (the union of the
+// staged boxes of the occupied tiles). E_cand is the preview E* with the
+// trial current applied (applyCurrentSource(-1)); G.j == nullptr means the
+// current is treated as zero.
+__global__ void __launch_bounds__(256) kEsum(GArgs G, const int* active, int ntx, int nty, int ntz) {
+  const int tile = active[blockIdx.x];
+  const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+  for (int d = 0; d < 3; ++d) {
+    const int n0 = G.ne[d][0], n1 = G.ne[d][1], n2 = G.ne[d][2];
+    const int i0 = tx * kTileX, i1 = tx == ntx - 1 ? n0 : min(n0, i0 + kTileX);
+    const int j0 = ty * kTileY, j1 = ty == nty - 1 ? n1 : min(n1, j0 + kTileY);
+    const int k0 = tz * kTileZ, k1 = tz == ntz - 1 ? n2 : min(n2, k0 + kTileZ);
+    const int wx = i1 - i0, wy = j1 - j0, wz = k1 - k0;
+    double* dst = const_cast<double*>(G.esum[d]);
+    for (int t = threadIdx.x; t < wx * wy * wz; t += blockDim.x) {
+      const int i = i0 + t % wx, j = j0 + (t / wx) % wy, k = k0 + t / (wx * wy);
+      dst[(static_cast<size_t>(k) * n1 + j) * n0 + i] = ecand(G, d, i, j, k) + eoldAt(G, d, i, j, k);
+    }
+  }
+}
+
 // Impl::electric wide branch for binomial3-x (source_coupling.cpp:74-82):
 // orbit average of 1/2 (E_cand + E_old) with the exact adjoint weights of
 // binomialXZSegment (longitudinal_shape.h:61-94).
-__device__ void gatherE(const GArgs& G, const double a[3], const double bpt[3], double res[3], unsigned& err) {
+__device__ void gatherE(const GArgs& G, const StagedFields& F, const double a[3], const double bpt[3], double res[3],
+                        unsigned& err) {
   res[0] = res[1] = res[2] = 0.0;
   const Clip path = clipOrbit(G, a, bpt, err);
   if (!path.valid) { err |= kErrOrbitOutside; return; }
@@ -328,10 +417,35 @@ __device__ void gatherE(const GArgs& G, const double a[3], const double bpt[3], 
     const double xy = curvature * v[0] * v[1];
     const double xz = curvature * v[0] * v[2];
     const double yz = curvature * v[1] * v[2];
+    // Interior segment: every tap of the three components is in range and
+    // inside the staged box, so the taps are fixed offsets from one base.
+    if (i >= 1 && i <= G.nx - 2 && j >= 0 && j <= G.ny - 1 && k >= 1 && k <= G.nz - 2 && i - 1 >= F.box.x0 &&
+        i + 2 < F.box.x0 + kExtX && j >= F.box.y0 && j + 1 < F.box.y0 + kExtY && k - 1 >= F.box.z0 &&
+        k + 2 < F.box.z0 + kExtZ) {
+      const int base = ((static_cast<int>(k) - 1 - F.box.z0) * kExtY + (static_cast<int>(j) - F.box.y0)) * kExtX +
+                       (static_cast<int>(i) - 1 - F.box.x0);
+      const double* e0 = F.e + base;
+      const double* e1 = F.e + kExtNodes + base;
+      const double* e2 = F.e + 2 * kExtNodes + base;
+      for (int c = 0; c < 4; ++c) {
+        for (int b = 0; b < 2; ++b) {
+          const double weight = width * (wy[b] * wz[c] + yz * sy[b] * slope[c]);
+          for (int aa = 0; aa < 3; ++aa) res[0] += .5 * (weight * edge[aa]) * e0[(c * kExtY + b) * kExtX + aa];
+        }
+        for (int aa = 0; aa < 4; ++aa)
+          res[1] += .5 * (width * (wx[aa] * wz[c] + xz * slope[aa] * slope[c])) * e1[c * kExtY * kExtX + aa];
+      }
+      for (int b = 0; b < 2; ++b)
+        for (int aa = 0; aa < 4; ++aa) {
+          const double weight = width * (wx[aa] * wy[b] + xy * slope[aa] * sy[b]);
+          for (int c = 0; c < 3; ++c) res[2] += .5 * (weight * edge[c]) * e2[(c * kExtY + b) * kExtX + aa];
+        }
+      continue;
+    }
     auto visit = [&](int d, long long ii, long long jj, long long kk, double w) {
       if (ii < 0 || jj < 0 || ii >= G.ne[d][0] || jj >= G.ne[d][1]) return;
       if (kk < 0 || kk >= G.ne[d][2]) return;
-      res[d] += .5 * w * (ecand(G, d, (int)ii, (int)jj, (int)kk) + eoldAt(G, d, (int)ii, (int)jj, (int)kk));
+      res[d] += .5 * w * F.esum(d, static_cast<int>(ii), static_cast<int>(jj), static_cast<int>(kk), err);
     };
     for (int c = 0; c < 4; ++c) {
       for (int b = 0; b < 2; ++b) {
@@ -352,17 +466,15 @@ __device__ void gatherE(const GArgs& G, const double a[3], const double bpt[3], 
   res[2] *= f;
 }
 
-__device__ __forceinline__ double openB(const GArgs& G, int a, long long i, long long j, long long k) {
-  const int* n = G.nb[a];
-  i = i < 0 ? 0 : (i > n[0] - 1 ? n[0] - 1 : i);
-  j = j < 0 ? 0 : (j > n[1] - 1 ? n[1] - 1 : j);
-  k = k < 0 ? 0 : (k > n[2] - 1 ? n[2] - 1 : k);
-  return G.b[a][(static_cast<size_t>(k) * n[1] + j) * n[0] + i];
+__device__ __forceinline__ int clampIndex(long long v, int n) {
+  return static_cast<int>(v < 0 ? 0 : (v > n - 1 ? n - 1 : v));
 }
 
 // Impl::magnetic for binomial3-x: binomialXZMagneticSample at the clipped
-// chord midpoint (source_coupling.cpp:83-90; source_coupling_math.h:142-167).
-__device__ void gatherB(const GArgs& G, const double a[3], const double bpt[3], double res[3], unsigned& err) {
+// chord midpoint with clamped ("open") indices (source_coupling.cpp:83-90;
+// source_coupling_math.h:142-167; NativeHalo::openMagneticValue).
+__device__ void gatherB(const GArgs& G, const StagedFields& F, const double a[3], const double bpt[3], double res[3],
+                        unsigned& err) {
   res[0] = res[1] = res[2] = 0.0;
   const Clip path = clipOrbit(G, a, bpt, err);
   if (!path.valid) { err |= kErrMagneticOutside; return; }
@@ -380,17 +492,35 @@ __device__ void gatherB(const GArgs& G, const double a[3], const double bpt[3], 
     nodeWeights(frac[0], wx);
     nodeWeights(frac[2], wz);
     const double wy[2] = {1. - frac[1], frac[1]};
+    const int* n = G.nb[comp];
+    if (cell[0] >= 1 && cell[0] + 2 <= n[0] - 1 && cell[1] >= 0 && cell[1] + 1 <= n[1] - 1 && cell[2] >= 1 &&
+        cell[2] + 2 <= n[2] - 1 && cell[0] - 1 >= F.box.x0 && cell[0] + 2 < F.box.x0 + kExtX && cell[1] >= F.box.y0 &&
+        cell[1] + 1 < F.box.y0 + kExtY && cell[2] - 1 >= F.box.z0 && cell[2] + 2 < F.box.z0 + kExtZ) {
+      // interior: no clamping, fixed offsets from one base
+      const double* bc = F.b + comp * kExtNodes +
+                         ((static_cast<int>(cell[2]) - 1 - F.box.z0) * kExtY + (static_cast<int>(cell[1]) - F.box.y0)) *
+                             kExtX +
+                         (static_cast<int>(cell[0]) - 1 - F.box.x0);
+      for (int k = 0; k < 4; ++k)
+        for (int j = 0; j < 2; ++j) {
+          double row = 0.;
+          for (int i = 0; i < 4; ++i) row += wx[i] * bc[(k * kExtY + j) * kExtX + i];
+          res[comp] += wz[k] * wy[j] * row;
+        }
+      continue;
+    }
     for (int k = 0; k < 4; ++k)
       for (int j = 0; j < 2; ++j) {
+        const int kk = clampIndex(cell[2] - 1 + k, n[2]), jj = clampIndex(cell[1] + j, n[1]);
         double row = 0.;
-        for (int i = 0; i < 4; ++i) row += wx[i] * openB(G, comp, cell[0] - 1 + i, cell[1] + j, cell[2] - 1 + k);
+        for (int i = 0; i < 4; ++i) row += wx[i] * F.bval(comp, clampIndex(cell[0] - 1 + i, n[0]), jj, kk, err);
         res[comp] += wz[k] * wy[j] * row;
       }
   }
 }
 
 // ---------------------------------------------------------------------------
-// Kernels.
+// Sort and chunk list.
 
 __device__ __forceinline__ int clampCell(double lattice, int n) {
   const int c = static_cast<int>(floor(lattice));
@@ -408,24 +538,81 @@ __global__ void kTileKeys(const double* x, const double* y, const double* z, int
   perm[p] = p;
 }
 
-template <class T>
-__global__ void kGather(const T* __restrict__ src, T* __restrict__ dst, const unsigned* perm, int n) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < n) dst[p] = src[perm[p]];
-}
+struct PermuteArgs {
+  const double* src[9];
+  double* dst[9];
+  const uint64_t* idSrc;
+  uint64_t* idDst;
+};
 
-__global__ void kTileRanges(const unsigned* key, int n, int* begin, int* end, int* occupied, unsigned* counter,
-                            int* slotOf) {
+__global__ void kPermute(PermuteArgs A, const unsigned* perm, int n) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  const unsigned k = key[p];
-  if (p == 0 || key[p - 1] != k) {
-    begin[k] = p;
-    const unsigned s = atomicAdd(counter, 1u);
-    occupied[s] = static_cast<int>(k);
-    slotOf[k] = static_cast<int>(s);
+  const unsigned q = perm[p];
+#pragma unroll
+  for (int a = 0; a < 9; ++a) A.dst[a][p] = A.src[a][q];
+  A.idDst[p] = A.idSrc[q];
+}
+
+// Particle range of every tile: binary search of the tile in the sorted keys.
+__global__ void kTileRanges(const unsigned* key, int n, int ntiles, int* begin, int* end, unsigned* occupied) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  auto lower = [&](unsigned v) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (key[mid] < v) lo = mid + 1;
+      else hi = mid;
+    }
+    return lo;
+  };
+  const int b = lower(static_cast<unsigned>(t)), e = lower(static_cast<unsigned>(t) + 1u);
+  begin[t] = b;
+  end[t] = e;
+  if (e > b) atomicAdd(occupied, 1u);
+}
+
+// Per tile: chunk count (low word) and whether any of its 27 neighbour
+// tiles is occupied (high word); entry ntiles is 0 so that the exclusive
+// scan ends with both totals.
+__global__ void kTileCounts(const int* begin, const int* end, int ntx, int nty, int ntz,
+                            unsigned long long* counts) {
+  const int ntiles = ntx * nty * ntz;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > ntiles) return;
+  if (t == ntiles) {
+    counts[t] = 0ull;
+    return;
   }
-  if (p == n - 1 || key[p + 1] != k) end[k] = p + 1;
+  const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
+  unsigned touched = 0;
+  for (int oz = -1; oz <= 1; ++oz)
+    for (int oy = -1; oy <= 1; ++oy)
+      for (int ox = -1; ox <= 1; ++ox) {
+        const int x = tx + ox, y = ty + oy, z = tz + oz;
+        if (x < 0 || x >= ntx || y < 0 || y >= nty || z < 0 || z >= ntz) continue;
+        const int s = (z * nty + y) * ntx + x;
+        if (end[s] > begin[s]) touched = 1;
+      }
+  const unsigned chunks = static_cast<unsigned>((end[t] - begin[t] + kChunk - 1) / kChunk);
+  counts[t] = (static_cast<unsigned long long>(touched) << 32) | chunks;
+}
+
+__device__ __forceinline__ int lowWord(unsigned long long v) { return static_cast<int>(v & 0xffffffffull); }
+__device__ __forceinline__ int highWord(unsigned long long v) { return static_cast<int>(v >> 32); }
+
+// Chunk list in tile order and the list of active tiles (tiles with an
+// occupied neighbour), both from the scanned counts.
+__global__ void kChunkList(const int* begin, const int* end, const unsigned long long* counts,
+                           const unsigned long long* scan, int ntiles, int4* chunks, int* active) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  const unsigned long long sc = scan[t];
+  if (counts[t] >> 32) active[highWord(sc)] = t;
+  const int e = end[t];
+  int c = lowWord(sc);
+  for (int p = begin[t]; p < e; p += kChunk, ++c) chunks[c] = make_int4(t, p, min(e, p + kChunk), 0);
 }
 
 __global__ void kFillInt(int* p, int n, int v) {
@@ -433,9 +620,113 @@ __global__ void kFillInt(int* p, int n, int v) {
   if (t < n) p[t] = v;
 }
 
-// Push of source_coupling.cpp:133-147 for one trial: first-trial gather at
-// (x0, ballistic guess), then the substeps with prescribed magnets, then the
-// particle-boundary and domain checks of the deposition loop.
+// ---------------------------------------------------------------------------
+// Gathers, push, deposit, residual.
+
+// Chunked orbit gathers from LDS-staged tile fields. A workgroup of
+// 2*kChunk lanes takes one chunk: the first kChunk lanes (whole waves)
+// gather E, the other kChunk lanes gather B, one lane per particle each.
+// MODE 0: the first trial's gather at (x0, ballistic guess) against the
+// zero-current preview (source_coupling.cpp:133-147); it writes the fields
+// the first push uses. MODE 1: the residual gather of localResidual
+// (source_coupling.cpp:148-202) at (x0, x1): next fields, and the chunk's
+// force max and defect / scale / call sums in a fixed tree order.
+template <int MODE>
+__global__ void __launch_bounds__(2 * kChunk) kGatherChunks(PArgs P, GArgs G, const int4* chunks, int ntx, int nty,
+                                                            double dt, double* chunkOut, unsigned* errOut) {
+  __shared__ double es[3 * kExtNodes];
+  __shared__ double bs[3 * kExtNodes];
+  const int4 ch = chunks[blockIdx.x];
+  const TileBox box = tileBox(ch.x, ntx, nty);
+  stageTile(G, box, es, bs);
+  __syncthreads();
+  const StagedFields F{es, bs, box};
+  const bool bWaves = threadIdx.x >= kChunk;
+  const int lane = static_cast<int>(threadIdx.x) - (bWaves ? kChunk : 0);
+  const int p = ch.y + lane;
+  const bool live = p < ch.z;
+  unsigned err = 0;
+  double a[3] = {0., 0., 0.}, b[3] = {0., 0., 0.}, g[3] = {0., 0., 0.};
+  if (live) {
+    for (int d = 0; d < 3; ++d) a[d] = P.x[d][p];
+    if (MODE == 0) {
+      const double u0[3] = {P.u[0][p], P.u[1][p], P.u[2][p]};
+      const double gamma = gammaFromU(u0[0], u0[1], u0[2]);
+      const double s = dt * kC / gamma;
+      for (int d = 0; d < 3; ++d) {
+        b[d] = a[d];
+        b[d] += s * u0[d];
+      }
+    } else {
+      for (int d = 0; d < 3; ++d) b[d] = P.x1[d][p];
+    }
+    if (bWaves) gatherB(G, F, a, b, g, err);
+    else gatherE(G, F, a, b, g, err);
+    double* const* dst = MODE == 0 ? P.fin : P.fout;
+    const int off = bWaves ? 3 : 0;
+    for (int d = 0; d < 3; ++d) dst[off + d][p] = g[d];
+  }
+  if (MODE == 1) {
+    // The staged box is no longer read: reuse it to pass B to the E lanes,
+    // then for the chunk reduction.
+    __syncthreads();
+    double* xb = es;  // 3 * kChunk
+    if (bWaves)
+      for (int d = 0; d < 3; ++d) xb[d * kChunk + lane] = g[d];
+    __syncthreads();
+    double force = 0., defect = 0., scale = 0., calls = 0.;
+    if (!bWaves && live) {
+      const double nb[3] = {xb[lane], xb[kChunk + lane], xb[2 * kChunk + lane]};
+      double de[3], db[3];
+      for (int d = 0; d < 3; ++d) {
+        de[d] = g[d] - P.fin[d][p];
+        db[d] = nb[d] - P.fin[3 + d][p];
+      }
+      const double q = P.q[p], m = P.m[p];
+      force = fabs(q / m) * dt / kC * (sqrt(de[0] * de[0] + de[1] * de[1] + de[2] * de[2]) +
+                                        kC * sqrt(db[0] * db[0] + db[1] * db[1] + db[2] * db[2]));
+      double w1 = 0., w2 = 0.;
+      for (int d = 0; d < 3; ++d) {
+        w1 += q * de[d] * (b[d] - a[d]);
+        w2 += q * g[d] * (b[d] - a[d]);
+      }
+      defect = fabs(w1);
+      scale = P.gridAbs[p] + fabs(w2);
+      calls = 1.;
+    }
+    double* sf = bs;
+    double* sd = bs + kChunk;
+    double* ss = bs + 2 * kChunk;
+    double* sc = bs + 3 * kChunk;
+    if (!bWaves) {
+      sf[lane] = force;
+      sd[lane] = defect;
+      ss[lane] = scale;
+      sc[lane] = calls;
+    }
+    __syncthreads();
+    for (int w = kChunk / 2; w > 0; w >>= 1) {
+      if (static_cast<int>(threadIdx.x) < w) {
+        sf[threadIdx.x] = fmax(sf[threadIdx.x], sf[threadIdx.x + w]);
+        sd[threadIdx.x] += sd[threadIdx.x + w];
+        ss[threadIdx.x] += ss[threadIdx.x + w];
+        sc[threadIdx.x] += sc[threadIdx.x + w];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      chunkOut[4 * blockIdx.x + 0] = sf[0];
+      chunkOut[4 * blockIdx.x + 1] = sd[0];
+      chunkOut[4 * blockIdx.x + 2] = ss[0];
+      chunkOut[4 * blockIdx.x + 3] = sc[0];
+    }
+  }
+  if (err) atomicOr(errOut, err);
+}
+
+// Push of source_coupling.cpp:133-147 for one trial with the gathered
+// fields: the substeps with prescribed magnets, then the particle-boundary
+// and domain checks of the deposition loop, then the chord table.
 __global__ void __launch_bounds__(128) kPush(PArgs P, GArgs G, StepArgs S, unsigned* errOut) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P.n) return;
@@ -444,17 +735,8 @@ __global__ void __launch_bounds__(128) kPush(PArgs P, GArgs G, StepArgs S, unsig
   const double u0[3] = {P.u[0][p], P.u[1][p], P.u[2][p]};
   const double q = P.q[p], m = P.m[p];
   if (S.window && x0[2] < G.o[2]) err |= kErrUpstream;
-  double usedE[3], usedB[3];
-  if (P.ready[p]) {
-    for (int a = 0; a < 3; ++a) { usedE[a] = P.nextE[a][p]; usedB[a] = P.nextB[a][p]; }
-  } else {
-    double guess[3] = {x0[0], x0[1], x0[2]};
-    const double gamma = gammaFromU(u0[0], u0[1], u0[2]);
-    const double s = S.dt * kC / gamma;
-    for (int a = 0; a < 3; ++a) guess[a] += s * u0[a];
-    gatherE(G, x0, guess, usedE, err);
-    gatherB(G, x0, guess, usedB, err);
-  }
+  const double usedE[3] = {P.fin[0][p], P.fin[1][p], P.fin[2][p]};
+  const double usedB[3] = {P.fin[3][p], P.fin[4][p], P.fin[5][p]};
   double x[3] = {x0[0], x0[1], x0[2]}, u[3] = {u0[0], u0[1], u0[2]};
   const double h = S.dt / S.substeps;
   double absGrid = 0.0;
@@ -498,8 +780,6 @@ __global__ void __launch_bounds__(128) kPush(PArgs P, GArgs G, StepArgs S, unsig
     if (zb >= 0. && zb > za) err |= kErrBoundaryEvent;
   }
   for (int a = 0; a < 3; ++a) {
-    P.usedE[a][p] = usedE[a];
-    P.usedB[a][p] = usedB[a];
     P.x1[a][p] = x[a];
     P.u1[a][p] = u[a];
   }
@@ -507,7 +787,6 @@ __global__ void __launch_bounds__(128) kPush(PArgs P, GArgs G, StepArgs S, unsig
   // Chord table of Current::deposit (longitudinal_shape.h:175-187 ->
   // weights 97-117 -> binomialXZSegment 61-94) for the tile deposition.
   {
-    const double q = P.q[p];
     double lx[3], v[3], cuts[kMaxCuts];
     for (int d = 0; d < 3; ++d) {
       P.segV[6 * static_cast<size_t>(p) + 3 + d] = q * (x[d] - x0[d]) / P.scaleDen;
@@ -540,45 +819,43 @@ __global__ void __launch_bounds__(128) kPush(PArgs P, GArgs G, StepArgs S, unsig
 
 // Deterministic tile deposition of Current::deposit (longitudinal_shape.h:
 // 175-187) with binomial3-x weights, from the chord table of the push. One
-// workgroup of W waves per occupied tile accumulates into one LDS tile of
-// 64-bit fixed-point integers whose binary exponent comes from a bound on
-// the tile's total current (2 * sum over its particles of max |scale|; each
-// chord's weights sum to at most 2 in absolute value per component).
-// Integer addition is associative, so the LDS atomics give a bitwise
-// reproducible tile sum in any order; each contribution is rounded once
-// (2^-63 of the tile bound) and the tile is converted back to double when it
-// is written to its partial buffer.
-template <int W>
-__global__ void __launch_bounds__(64 * W) kDeposit(PArgs P, GArgs G, const int* occupied, const int* tileBegin,
-                                                   const int* tileEnd, int ntx, int nty, double* partial,
-                                                   unsigned* errOut) {
+// workgroup per chunk, one lane per particle, accumulating into an LDS copy
+// of the tile box in 64-bit fixed point. The binary exponent comes from a
+// bound on the whole tile's current (2 * sum over the tile's particles of
+// max |scale|; each chord's weights sum to at most 2 in absolute value per
+// component), so every chunk of a tile uses the same scale and the chunk
+// partials of a tile add exactly. Integer addition is associative, so the
+// LDS atomics and the later chunk sums are reproducible in any order; each
+// tap is rounded once, to 2^-62 of the tile bound.
+__global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, const int4* chunks, const int* tileBegin,
+                                                   const int* tileEnd, const unsigned long long* tileScan, int ntx,
+                                                   int nty, long long* partial, int* tileShift, unsigned* errOut) {
   __shared__ unsigned long long acc[3 * kExtNodes];
-  __shared__ double bound[64 * W];
-  const int slot = blockIdx.x;
-  const int tile = occupied[slot];
-  const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
-  const int ix0 = tx * kTileX - 2, jy0 = ty * kTileY - 1, kz0 = tz * kTileZ - 2;
-  const int b0 = tileBegin[tile], b1 = tileEnd[tile];
+  __shared__ double bound[kChunk];
+  const int4 ch = chunks[blockIdx.x];
+  const int tile = ch.x;
+  const TileBox box = tileBox(tile, ntx, nty);
+  const int tb0 = tileBegin[tile], tb1 = tileEnd[tile];
   for (int t = threadIdx.x; t < 3 * kExtNodes; t += blockDim.x) acc[t] = 0ull;
   double bsum = 0.0;
-  for (int p = b0 + static_cast<int>(threadIdx.x); p < b1; p += blockDim.x) {
+  for (int p = tb0 + static_cast<int>(threadIdx.x); p < tb1; p += blockDim.x) {
     const double* sv = P.segV + 6 * static_cast<size_t>(p);
     bsum += fmax(fabs(sv[3]), fmax(fabs(sv[4]), fabs(sv[5])));
   }
   bound[threadIdx.x] = bsum;
   __syncthreads();
-  for (int w = 32 * W; w > 0; w >>= 1) {
+  for (int w = kChunk / 2; w > 0; w >>= 1) {
     if (static_cast<int>(threadIdx.x) < w) bound[threadIdx.x] += bound[threadIdx.x + w];
     __syncthreads();
   }
   int e2 = 0;
   (void)frexp(2.0 * bound[0], &e2);
   const int shift = 62 - e2;  // 2*bound * 2^shift < 2^62
+  if (threadIdx.x == 0 && static_cast<int>(blockIdx.x) == lowWord(tileScan[tile])) tileShift[tile] = shift;
   unsigned err = 0;
   const double edge[3] = {0.25, 0.5, 0.25};
   const double slope[4] = {-0.25, -0.25, 0.25, 0.25};
   const double sy[2] = {-1., 1.};
-  // One lane per particle; every tap of its chord becomes one integer add.
   auto add = [&](int comp, int ii, int jj, int kk, double value) {
     if (ii < 0 || ii >= G.ne[comp][0]) {
       if (!G.smoothX) err |= kErrCurrentX;
@@ -586,12 +863,13 @@ __global__ void __launch_bounds__(64 * W) kDeposit(PArgs P, GArgs G, const int* 
     }
     if (jj < 0 || jj >= G.ne[comp][1]) { err |= kErrCurrentY; return; }
     if (kk < 0 || kk >= G.ne[comp][2]) return;
-    const int li = ii - ix0, lj = jj - jy0, lk = kk - kz0;
-    if (li < 0 || li >= kExtX || lj < 0 || lj >= kExtY || lk < 0 || lk >= kExtZ) { err |= kErrTileHalo; return; }
+    const int l = box.local(ii, jj, kk, err);
+    if (l < 0) return;
     const long long fixed = llrint(ldexp(value, shift));
-    atomicAdd(&acc[comp * kExtNodes + (lk * kExtY + lj) * kExtX + li], static_cast<unsigned long long>(fixed));
+    atomicAdd(&acc[comp * kExtNodes + l], static_cast<unsigned long long>(fixed));
   };
-  for (int p = b0 + static_cast<int>(threadIdx.x); p < b1; p += blockDim.x) {
+  const int p = ch.y + static_cast<int>(threadIdx.x);
+  if (p < ch.z) {
     const int nseg = P.segCount[p];
     const double* sv = P.segV + 6 * static_cast<size_t>(p);
     const double v0 = sv[0], v1 = sv[1], v2 = sv[2];
@@ -607,6 +885,35 @@ __global__ void __launch_bounds__(64 * W) kDeposit(PArgs P, GArgs G, const int* 
       const double wy[2] = {1. - fr.y, fr.y};
       const double curvature = width * width / 12.;
       const double xy = curvature * v0 * v1, xz = curvature * v0 * v2, yz = curvature * v1 * v2;
+      if (i >= 1 && i <= G.nx - 2 && j >= 0 && j <= G.ny - 1 && k >= 1 && k <= G.nz - 2 && i - 1 >= box.x0 &&
+          i + 2 < box.x0 + kExtX && j >= box.y0 && j + 1 < box.y0 + kExtY && k - 1 >= box.z0 &&
+          k + 2 < box.z0 + kExtZ) {
+        // interior segment: all taps in range and in the box
+        const int base = ((k - 1 - box.z0) * kExtY + (j - box.y0)) * kExtX + (i - 1 - box.x0);
+        unsigned long long* a0 = acc + base;
+        unsigned long long* a1 = acc + kExtNodes + base;
+        unsigned long long* a2 = acc + 2 * kExtNodes + base;
+        auto put = [&](unsigned long long* at, double value) {
+          atomicAdd(at, static_cast<unsigned long long>(llrint(ldexp(value, shift))));
+        };
+        for (int c = 0; c < 4; ++c) {
+          if (scale[0] != 0.)
+            for (int b = 0; b < 2; ++b) {
+              const double weight = width * (wy[b] * wz[c] + yz * sy[b] * slope[c]);
+              for (int aa = 0; aa < 3; ++aa) put(a0 + (c * kExtY + b) * kExtX + aa, scale[0] * (weight * edge[aa]));
+            }
+          if (scale[1] != 0.)
+            for (int aa = 0; aa < 4; ++aa)
+              put(a1 + c * kExtY * kExtX + aa, scale[1] * (width * (wx[aa] * wz[c] + xz * slope[aa] * slope[c])));
+        }
+        if (scale[2] != 0.)
+          for (int b = 0; b < 2; ++b)
+            for (int aa = 0; aa < 4; ++aa) {
+              const double weight = width * (wx[aa] * wy[b] + xy * slope[aa] * sy[b]);
+              for (int c = 0; c < 3; ++c) put(a2 + (c * kExtY + b) * kExtX + aa, scale[2] * (weight * edge[c]));
+            }
+        continue;
+      }
       // binomialXZSegment visit order and weights, longitudinal_shape.h:81-93
       for (int c = 0; c < 4; ++c) {
         if (scale[0] != 0.)
@@ -627,32 +934,55 @@ __global__ void __launch_bounds__(64 * W) kDeposit(PArgs P, GArgs G, const int* 
     }
   }
   __syncthreads();
-  double* out = partial + static_cast<size_t>(slot) * 3 * kExtNodes;
-  for (int t = threadIdx.x; t < 3 * kExtNodes; t += blockDim.x)
-    out[t] = ldexp(static_cast<double>(static_cast<long long>(acc[t])), -shift);
+  long long* out = partial + static_cast<size_t>(blockIdx.x) * 3 * kExtNodes;
+  for (int t = threadIdx.x; t < 3 * kExtNodes; t += blockDim.x) out[t] = static_cast<long long>(acc[t]);
   if (err) atomicOr(errOut, err);
 }
 
-// Node-owner reduction of the tile partials into J, one workgroup per cell
-// tile: each owned node sums the covering occupied tiles in fixed
-// (tz, ty, tx) order. Tiles with no occupied neighbour exit at once; their
-// nodes keep the zeros J held at the start of the step.
-__global__ void __launch_bounds__(256) kReduceCurrent(double* jx, double* jy, double* jz, GArgs G, const int* slotOf,
-                                                      const double* partial, int ntx, int nty, int ntz) {
-  const int tile = blockIdx.x;
+// Exact integer total of a tile's chunk partials, converted with the
+// tile's exponent and stored as doubles over the tile's first chunk
+// partial. One workgroup per chunk; only a tile's first chunk works.
+__global__ void __launch_bounds__(256) kCombineTiles(const int4* chunks, const unsigned long long* tileScan,
+                                                     const int* tileShift, long long* partial) {
+  const int c = blockIdx.x;
+  const int tile = chunks[c].x;
+  const int first = lowWord(tileScan[tile]);
+  if (c != first) return;
+  const int count = lowWord(tileScan[tile + 1]) - first;
+  const int shift = tileShift[tile];
+  constexpr size_t chunkStride = 3 * static_cast<size_t>(kExtNodes);
+  long long* src = partial + static_cast<size_t>(c) * chunkStride;
+  double* dst = reinterpret_cast<double*>(src);
+  for (int t = threadIdx.x; t < 3 * kExtNodes; t += blockDim.x) {
+    long long v = src[t];
+    for (int k = 1; k < count; ++k) v += src[k * chunkStride + t];
+    dst[t] = ldexp(static_cast<double>(v), -shift);
+  }
+}
+
+// Node-owner reduction into J, one workgroup per active cell tile (a tile
+// with an occupied neighbour): each owned node sums the combined boxes of
+// the covering occupied tiles in fixed (tz, ty, tx) order. Nodes of other
+// tiles keep the zeros J held at the start of the step.
+__global__ void __launch_bounds__(256) kReduceCurrent(double* jx, double* jy, double* jz, GArgs G, const int* active,
+                                                      const unsigned long long* tileScan, const double* boxes,
+                                                      int ntx, int nty, int ntz) {
+  const int tile = active[blockIdx.x];
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
-  __shared__ int any;
-  if (threadIdx.x == 0) any = 0;
-  __syncthreads();
+  __shared__ int nbFirst[27];
   if (threadIdx.x < 27) {
-    const int ox = static_cast<int>(threadIdx.x) % 3 - 1, oy = (static_cast<int>(threadIdx.x) / 3) % 3 - 1,
-              oz = static_cast<int>(threadIdx.x) / 9 - 1;
-    const int x = tx + ox, y = ty + oy, z = tz + oz;
-    if (x >= 0 && x < ntx && y >= 0 && y < nty && z >= 0 && z < ntz && slotOf[(z * nty + y) * ntx + x] >= 0)
-      atomicOr(&any, 1);
+    const int x = tx + static_cast<int>(threadIdx.x) % 3 - 1, y = ty + (static_cast<int>(threadIdx.x) / 3) % 3 - 1,
+              z = tz + static_cast<int>(threadIdx.x) / 9 - 1;
+    int first = -1;
+    if (x >= 0 && x < ntx && y >= 0 && y < nty && z >= 0 && z < ntz) {
+      const int s = (z * nty + y) * ntx + x;
+      const int f = lowWord(tileScan[s]);
+      if (lowWord(tileScan[s + 1]) > f) first = f;
+    }
+    nbFirst[threadIdx.x] = first;
   }
   __syncthreads();
-  if (!any) return;
+  constexpr size_t chunkStride = 3 * static_cast<size_t>(kExtNodes);
   for (int d = 0; d < 3; ++d) {
     const int n0 = G.ne[d][0], n1 = G.ne[d][1], n2 = G.ne[d][2];
     // owned node box (the last tile of an axis also owns the extra node)
@@ -663,84 +993,25 @@ __global__ void __launch_bounds__(256) kReduceCurrent(double* jx, double* jy, do
     double* dst = d == 0 ? jx : (d == 1 ? jy : jz);
     for (int t = threadIdx.x; t < wx * wy * wz; t += blockDim.x) {
       const int i = i0 + t % wx, j = j0 + (t / wx) % wy, k = k0 + t / (wx * wy);
-      double sum = 0.0;
-      for (int sz = tz - 1; sz <= tz + 1; ++sz) {
-        if (sz < 0 || sz >= ntz) continue;
-        const int lk = k - (sz * kTileZ - 2);
-        if (lk < 0 || lk >= kExtZ) continue;
-        for (int sy = ty - 1; sy <= ty + 1; ++sy) {
-          if (sy < 0 || sy >= nty) continue;
-          const int lj = j - (sy * kTileY - 1);
-          if (lj < 0 || lj >= kExtY) continue;
-          for (int sx = tx - 1; sx <= tx + 1; ++sx) {
-            if (sx < 0 || sx >= ntx) continue;
-            const int li = i - (sx * kTileX - 2);
-            if (li < 0 || li >= kExtX) continue;
-            const int s = slotOf[(sz * nty + sy) * ntx + sx];
-            if (s < 0) continue;
-            sum += partial[(static_cast<size_t>(s) * 3 + d) * kExtNodes + (lk * kExtY + lj) * kExtX + li];
-          }
-        }
+      double v[27];
+      bool on[27];
+#pragma unroll
+      for (int nb = 0; nb < 27; ++nb) {
+        const int ox = nb % 3 - 1, oy = (nb / 3) % 3 - 1, oz = nb / 9 - 1;
+        const int li = i - ((tx + ox) * kTileX - 2), lj = j - ((ty + oy) * kTileY - 1),
+                  lk = k - ((tz + oz) * kTileZ - 2);
+        const int f = nbFirst[nb];
+        on[nb] = f >= 0 && li >= 0 && li < kExtX && lj >= 0 && lj < kExtY && lk >= 0 && lk < kExtZ;
+        v[nb] = on[nb] ? boxes[static_cast<size_t>(f) * chunkStride + d * kExtNodes + (lk * kExtY + lj) * kExtX + li]
+                       : 0.0;
       }
+      double sum = 0.0;
+#pragma unroll
+      for (int nb = 0; nb < 27; ++nb)
+        if (on[nb]) sum += v[nb];
       dst[(static_cast<size_t>(k) * n1 + j) * n0 + i] = sum;
     }
   }
-}
-
-// localResidual (source_coupling.cpp:148-202): residual gathers and the
-// per-block force max / defect, scale and call sums in fixed tree order.
-__global__ void __launch_bounds__(256) kResidual(PArgs P, GArgs G, double dt, double* blockOut, unsigned* errOut) {
-  __shared__ double sf[256], sd[256], ss[256], sc[256];
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  double force = 0., defect = 0., scale = 0., calls = 0.;
-  unsigned err = 0;
-  if (p < P.n) {
-    const double a[3] = {P.x[0][p], P.x[1][p], P.x[2][p]};
-    const double b[3] = {P.x1[0][p], P.x1[1][p], P.x1[2][p]};
-    double ne[3], nb[3];
-    gatherE(G, a, b, ne, err);
-    gatherB(G, a, b, nb, err);
-    double de[3], db[3];
-    for (int d = 0; d < 3; ++d) {
-      P.nextE[d][p] = ne[d];
-      P.nextB[d][p] = nb[d];
-      de[d] = ne[d] - P.usedE[d][p];
-      db[d] = nb[d] - P.usedB[d][p];
-    }
-    P.ready[p] = 1;
-    const double q = P.q[p], m = P.m[p];
-    force = fabs(q / m) * dt / kC * (sqrt(de[0] * de[0] + de[1] * de[1] + de[2] * de[2]) +
-                                      kC * sqrt(db[0] * db[0] + db[1] * db[1] + db[2] * db[2]));
-    double w1 = 0., w2 = 0.;
-    for (int d = 0; d < 3; ++d) {
-      w1 += q * de[d] * (b[d] - a[d]);
-      w2 += q * ne[d] * (b[d] - a[d]);
-    }
-    defect = fabs(w1);
-    scale = P.gridAbs[p] + fabs(w2);
-    calls = 1.;
-  }
-  sf[threadIdx.x] = force;
-  sd[threadIdx.x] = defect;
-  ss[threadIdx.x] = scale;
-  sc[threadIdx.x] = calls;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
-      sf[threadIdx.x] = fmax(sf[threadIdx.x], sf[threadIdx.x + w]);
-      sd[threadIdx.x] += sd[threadIdx.x + w];
-      ss[threadIdx.x] += ss[threadIdx.x + w];
-      sc[threadIdx.x] += sc[threadIdx.x + w];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    blockOut[4 * blockIdx.x + 0] = sf[0];
-    blockOut[4 * blockIdx.x + 1] = sd[0];
-    blockOut[4 * blockIdx.x + 2] = ss[0];
-    blockOut[4 * blockIdx.x + 3] = sc[0];
-  }
-  if (err) atomicOr(errOut, err);
 }
 
 __global__ void __launch_bounds__(256) kReduceBlocks(const double* blockIn, int blocks, double* out) {
@@ -766,18 +1037,13 @@ __global__ void __launch_bounds__(256) kReduceBlocks(const double* blockIn, int 
   if (threadIdx.x == 0) { out[0] = sf[0]; out[1] = sd[0]; out[2] = ss[0]; out[3] = sc[0]; }
 }
 
-__global__ void kCommitParticles(PArgs P) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P.n) return;
-  for (int a = 0; a < 3; ++a) { P.x[a][p] = P.x1[a][p]; P.u[a][p] = P.u1[a][p]; }
-}
-
 // ---------------------------------------------------------------------------
 // Host orchestration.
 
 static GArgs gatherArgs(const CouplingState& C, const FieldState& F, const double origin[3]) {
   GArgs G{};
   for (int a = 0; a < 3; ++a) {
+    G.esum[a] = C.esum[a].get();
     G.estar[a] = C.estar[a].get();
     G.eold[a] = F.base(a);
     G.j[a] = F.base(6 + a);
@@ -802,18 +1068,16 @@ static PArgs particleArgs(ParticleStore& P) {
   for (int a = 0; a < 3; ++a) {
     A.x[a] = P.s[a].get();
     A.u[a] = P.s[3 + a].get();
-    A.guess[a] = P.slot[a].get();
-    A.usedE[a] = P.slot[3 + a].get();
-    A.usedB[a] = P.slot[6 + a].get();
-    A.nextE[a] = P.slot[9 + a].get();
-    A.nextB[a] = P.slot[12 + a].get();
-    A.x1[a] = P.slot[15 + a].get();
-    A.u1[a] = P.slot[18 + a].get();
+    A.x1[a] = P.end[a].get();
+    A.u1[a] = P.end[3 + a].get();
+  }
+  for (int a = 0; a < 6; ++a) {
+    A.fin[a] = P.fields[0][a].get();
+    A.fout[a] = P.fields[1][a].get();
   }
   A.q = P.s[6].get();
   A.m = P.s[7].get();
-  A.gridAbs = P.slot[21].get();
-  A.ready = P.ready.get();
+  A.gridAbs = P.gridAbs.get();
   A.segCount = P.segCount.get();
   A.segV = P.segV.get();
   A.segCell = P.segCell.get();
@@ -827,8 +1091,12 @@ void reserveParticles(ParticleStore& P, std::size_t capacity) {
   for (int a = 0; a < 9; ++a) { P.s[a].resize(capacity); P.t[a].resize(capacity); }
   P.id.resize(capacity);
   P.idT.resize(capacity);
-  for (int a = 0; a < 22; ++a) P.slot[a].resize(capacity);
-  P.ready.resize(capacity);
+  for (int a = 0; a < 6; ++a) {
+    P.fields[0][a].resize(capacity);
+    P.fields[1][a].resize(capacity);
+    P.end[a].resize(capacity);
+  }
+  P.gridAbs.resize(capacity);
   P.segCount.resize(capacity);
   P.segV.resize(6 * capacity);
   P.segCell.resize(kMaxSegments * capacity);
@@ -898,27 +1166,62 @@ static StepArgs stepArgs(const CouplingState& C, const FieldState& F, const doub
   return S;
 }
 
-// Stable sort of the particle state by start tile.
-static void sortParticles(CouplingState& C, ParticleStore& P, const GArgs& G, int ntx, int nty) {
+static int bitsFor(unsigned v) {
+  int b = 1;
+  while (b < 32 && (v >> b) != 0u) ++b;
+  return b;
+}
+
+// Stable sort of the particle state by start tile, then the tile ranges and
+// the chunk list (tile order, kChunk particles per chunk) in device memory.
+static void sortAndChunk(CouplingState& C, ParticleStore& P, const GArgs& G, int ntx, int nty, int ntiles,
+                         unsigned* occupied) {
   FieldState& F = *C.fields;
+  hipStream_t st = F.stream;
   const int n = static_cast<int>(P.count);
-  if (!n) return;
-  ScopedKernel sk(F.profiler, F.stream, "sort");
+  ScopedKernel sk(F.profiler, st, "sort");
   const unsigned blocks = ceilDiv(n, 256);
-  kTileKeys<<<blocks, 256, 0, F.stream>>>(P.s[0].get(), P.s[1].get(), P.s[2].get(), n, G, ntx, nty, P.key.get(),
-                                          P.perm.get());
+  kTileKeys<<<blocks, 256, 0, st>>>(P.s[0].get(), P.s[1].get(), P.s[2].get(), n, G, ntx, nty, P.key.get(),
+                                    P.perm.get());
   launchCheck("tile keys");
-  std::size_t bytes = P.sortTempBytes;
-  hipCheck(rocprim::radix_sort_pairs(P.sortTemp.get(), bytes, P.key.get(), P.keySorted.get(), P.perm.get(),
-                                     P.permSorted.get(), static_cast<unsigned>(n), 0, 32, F.stream),
-           "radix_sort_pairs");
-  for (int a = 0; a < 9; ++a) {
-    kGather<double><<<blocks, 256, 0, F.stream>>>(P.s[a].get(), P.t[a].get(), P.permSorted.get(), n);
-    P.s[a].swap(P.t[a]);
+  const int bits = bitsFor(static_cast<unsigned>(ntiles - 1));
+  std::size_t bytes = 0;
+  hipCheck(rocprim::radix_sort_pairs(nullptr, bytes, P.key.get(), P.keySorted.get(), P.perm.get(),
+                                     P.permSorted.get(), static_cast<unsigned>(n), 0, bits, st),
+           "radix_sort_pairs size");
+  if (bytes > P.sortTempBytes) {
+    P.sortTemp.resize(bytes);
+    P.sortTempBytes = bytes;
   }
-  kGather<uint64_t><<<blocks, 256, 0, F.stream>>>(P.id.get(), P.idT.get(), P.permSorted.get(), n);
-  P.id.swap(P.idT);
+  bytes = P.sortTempBytes;
+  hipCheck(rocprim::radix_sort_pairs(P.sortTemp.get(), bytes, P.key.get(), P.keySorted.get(), P.perm.get(),
+                                     P.permSorted.get(), static_cast<unsigned>(n), 0, bits, st),
+           "radix_sort_pairs");
+  PermuteArgs A{};
+  for (int a = 0; a < 9; ++a) {
+    A.src[a] = P.s[a].get();
+    A.dst[a] = P.t[a].get();
+  }
+  A.idSrc = P.id.get();
+  A.idDst = P.idT.get();
+  kPermute<<<blocks, 256, 0, st>>>(A, P.permSorted.get(), n);
   launchCheck("permute");
+  for (int a = 0; a < 9; ++a) P.s[a].swap(P.t[a]);
+  P.id.swap(P.idT);
+  kTileRanges<<<ceilDiv(ntiles, 256), 256, 0, st>>>(P.keySorted.get(), n, ntiles, C.tileBegin.get(),
+                                                     C.tileEnd.get(), occupied);
+  launchCheck("tile ranges");
+  const int ntz = ntiles / (ntx * nty);
+  kTileCounts<<<ceilDiv(ntiles + 1, 256), 256, 0, st>>>(C.tileBegin.get(), C.tileEnd.get(), ntx, nty, ntz,
+                                                         C.tileCounts.get());
+  launchCheck("tile counts");
+  bytes = C.scanTempBytes;
+  hipCheck(rocprim::exclusive_scan(C.scanTemp.get(), bytes, C.tileCounts.get(), C.tileScan.get(), 0ull,
+                                   static_cast<std::size_t>(ntiles) + 1, rocprim::plus<unsigned long long>(), st),
+           "exclusive_scan");
+  kChunkList<<<ceilDiv(ntiles, 256), 256, 0, st>>>(C.tileBegin.get(), C.tileEnd.get(), C.tileCounts.get(),
+                                                    C.tileScan.get(), ntiles, C.chunks.get(), C.activeTiles.get());
+  launchCheck("chunk list");
 }
 
 void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], double time, unsigned substeps,
@@ -932,11 +1235,11 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
   // J is zero at the start of every step (simulation.cpp:518); the tile
   // reduction below rewrites only the neighbourhood of occupied tiles.
   launchClearCurrent(F);
-  // Zero-current preview E* (MovingWindowFields::previewElectricBoundary).
   for (int a = 0; a < 3; ++a)
-    if (C.estar[a].size() != F.compSize(a)) C.estar[a].resize(F.compSize(a));
-  double* const out[3] = {C.estar[0].get(), C.estar[1].get(), C.estar[2].get()};
-  launchElectric(F, F.ptrs(), 1, out, F.window ? F.inlet[0].get() : nullptr, F.window ? F.inlet[1].get() : nullptr);
+    if (C.estar[a].size() != F.compSize(a)) {
+      C.estar[a].resize(F.compSize(a));
+      C.esum[a].resize(F.compSize(a));
+    }
 
   const GArgs G = gatherArgs(C, F, origin);
   const int ntx = static_cast<int>((F.g.nx + kTileX - 1) / kTileX);
@@ -946,78 +1249,130 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
   if (C.tileBegin.size() != static_cast<std::size_t>(ntiles)) {
     C.tileBegin.resize(ntiles);
     C.tileEnd.resize(ntiles);
-    C.tileSlot.resize(ntiles);
-    C.occupied.resize(ntiles);
+    C.tileShift.resize(ntiles);
+    C.activeTiles.resize(ntiles);
+    C.tileCounts.resize(ntiles + 1);
+    C.tileScan.resize(ntiles + 1);
+    C.scanTempBytes = 0;
+    hipCheck(rocprim::exclusive_scan(nullptr, C.scanTempBytes, C.tileCounts.get(), C.tileScan.get(), 0ull,
+                                     static_cast<std::size_t>(ntiles) + 1, rocprim::plus<unsigned long long>(), st),
+             "exclusive_scan size");
+    C.scanTempBytes = std::max<std::size_t>(C.scanTempBytes, 256);
+    C.scanTemp.resize(C.scanTempBytes);
+    C.chunkCapacity = 0;
   }
-  if (!C.error.size()) { C.error.resize(1); C.counter.resize(1); C.reduceOut.resize(4); }
-  hipCheck(hipMemsetAsync(C.error.get(), 0, sizeof(unsigned), st), "hipMemsetAsync");
-  hipCheck(hipMemsetAsync(C.counter.get(), 0, sizeof(unsigned), st), "hipMemsetAsync");
-  kFillInt<<<ceilDiv(ntiles, 256), 256, 0, st>>>(C.tileSlot.get(), ntiles, -1);
-  kFillInt<<<ceilDiv(n ? n : 1, 256), 256, 0, st>>>(P.ready.get(), n ? n : 1, 0);
+  // Chunk list capacity: at most one partial chunk per occupied tile plus
+  // the full ones.
+  const std::size_t chunkBound = (static_cast<std::size_t>(n) + kChunk - 1) / kChunk +
+                                 std::min<std::size_t>(static_cast<std::size_t>(ntiles), static_cast<std::size_t>(n));
+  if (chunkBound > C.chunkCapacity) {
+    C.chunks.resize(std::max<std::size_t>(chunkBound, 1));
+    C.chunkCapacity = chunkBound;
+  }
+  if (!C.error.size()) { C.error.resize(2); C.reduceOut.resize(4); }
+  // pinned readback slots: [0..1] scan total (chunks, active tiles), [2]
+  // occupied tiles, [3] error bits; doubles from byte 16: force, defect,
+  // scale, calls
+  unsigned* pin = static_cast<unsigned*>(C.pinned.get(64));
+  double* pinD = reinterpret_cast<double*>(static_cast<unsigned char*>(C.pinned.get(64)) + 16);
+  kFillInt<<<1, 64, 0, st>>>(reinterpret_cast<int*>(C.error.get()), 2, 0);
   launchCheck("fill");
-  sortParticles(C, P, G, ntx, nty);
-  unsigned occupiedCount = 0;
   if (n) {
-    kTileRanges<<<ceilDiv(n, 256), 256, 0, st>>>(P.keySorted.get(), n, C.tileBegin.get(), C.tileEnd.get(),
-                                                 C.occupied.get(), C.counter.get(), C.tileSlot.get());
-    launchCheck("tile ranges");
-    hipCheck(hipMemcpyAsync(&occupiedCount, C.counter.get(), sizeof(unsigned), hipMemcpyDeviceToHost, st), "hipMemcpy");
-    hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+    sortAndChunk(C, P, G, ntx, nty, ntiles, C.error.get() + 1);
+    hipCheck(hipMemcpyAsync(pin, C.tileScan.get() + ntiles, sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
+             "hipMemcpyAsync");
+    hipCheck(hipMemcpyAsync(pin + 2, C.error.get() + 1, sizeof(unsigned), hipMemcpyDeviceToHost, st),
+             "hipMemcpyAsync");
+    hipCheck(hipEventRecord(C.sorted.get(), st), "hipEventRecord");
   }
-  if (occupiedCount > C.partialSlots) {
-    C.partial.resize(static_cast<std::size_t>(occupiedCount) * 3 * kExtNodes);
-    C.partialSlots = occupiedCount;
+  // Zero-current preview E* (MovingWindowFields::previewElectricBoundary);
+  // it runs while the host waits for the chunk count.
+  double* const out[3] = {C.estar[0].get(), C.estar[1].get(), C.estar[2].get()};
+  launchElectric(F, F.ptrs(), 1, out, F.window ? F.inletPlane[0] : nullptr, F.window ? F.inletPlane[1] : nullptr);
+  unsigned nChunks = 0, nActive = 0, occupiedTiles = 0;
+  if (n) {
+    hipCheck(hipEventSynchronize(C.sorted.get()), "hipEventSynchronize");
+    nChunks = pin[0];
+    nActive = pin[1];
+    occupiedTiles = pin[2];
+    if (nChunks > chunkBound) throw std::logic_error("Chunk list exceeds its bound");
   }
+  const std::size_t partialNeed = static_cast<std::size_t>(nChunks) * 3 * kExtNodes;
+  if (partialNeed > C.partial.size()) C.partial.resize(partialNeed);
+  if (C.reduceBlocks.size() < 4 * static_cast<std::size_t>(std::max(1u, nChunks)))
+    C.reduceBlocks.resize(4 * static_cast<std::size_t>(std::max(1u, nChunks)));
+
   const StepArgs S = stepArgs(C, F, origin, time, substeps);
   PArgs A = particleArgs(P);
   A.scaleDen = F.g.dx * F.g.dy * F.g.dz * F.g.dt;
-  const int resBlocks = std::max(1u, ceilDiv(n, 256));
-  if (C.reduceBlocks.size() < static_cast<std::size_t>(4 * resBlocks)) C.reduceBlocks.resize(4 * resBlocks);
-  double host[4];
-  unsigned hostErr = 0;
   bool converged = false;
   unsigned iterations = 0;
   double force = 0., work = 0.;
   for (unsigned trial = 0; trial < 64; ++trial) {
-    if (n) {
+    if (nChunks && trial == 0) {
       // The first trial pushes against the preview with J cleared
-      // (simulation_source_coupling.cpp:43); later trials reuse nextE/nextB.
-      GArgs Gp = G;
-      if (trial == 0)
-        for (int a = 0; a < 3; ++a) Gp.j[a] = nullptr;
+      // (simulation_source_coupling.cpp:43).
+      GArgs G0 = G;
+      for (int a = 0; a < 3; ++a) G0.j[a] = nullptr;
+      ScopedKernel sk(F.profiler, st, "gather");
+      kEsum<<<nActive, 256, 0, st>>>(G0, C.activeTiles.get(), ntx, nty, ntz);
+      kGatherChunks<0><<<nChunks, 2 * kChunk, 0, st>>>(A, G0, C.chunks.get(), ntx, nty, F.g.dt, nullptr,
+                                                       C.error.get());
+      launchCheck("gather");
+    }
+    if (n) {
       ScopedKernel sk(F.profiler, st, "push");
-      kPush<<<ceilDiv(n, 128), 128, 0, st>>>(A, Gp, S, C.error.get());
+      kPush<<<ceilDiv(n, 128), 128, 0, st>>>(A, G, S, C.error.get());
       launchCheck("push");
     }
-    if (occupiedCount) {
-      ScopedKernel sk(F.profiler, st, "deposit");
-      constexpr int W = 8;
-      kDeposit<W><<<occupiedCount, 64 * W, 0, st>>>(A, G, C.occupied.get(), C.tileBegin.get(), C.tileEnd.get(), ntx,
-                                                    nty, C.partial.get(), C.error.get());
-      launchCheck("deposit");
-    }
-    {
-      ScopedKernel sk(F.profiler, st, "reduce_current");
-      kReduceCurrent<<<ntiles, 256, 0, st>>>(F.base(6), F.base(7), F.base(8), G, C.tileSlot.get(), C.partial.get(), ntx,
-                                          nty, ntz);
-      launchCheck("reduce current");
+    if (nChunks) {
+      {
+        ScopedKernel sk(F.profiler, st, "deposit");
+        kDeposit<<<nChunks, kChunk, 0, st>>>(A, G, C.chunks.get(), C.tileBegin.get(), C.tileEnd.get(),
+                                             C.tileScan.get(), ntx, nty, C.partial.get(), C.tileShift.get(),
+                                             C.error.get());
+        launchCheck("deposit");
+      }
+      {
+        ScopedKernel sk(F.profiler, st, "reduce_current");
+        kCombineTiles<<<nChunks, 256, 0, st>>>(C.chunks.get(), C.tileScan.get(), C.tileShift.get(),
+                                               C.partial.get());
+        kReduceCurrent<<<nActive, 256, 0, st>>>(F.base(6), F.base(7), F.base(8), G, C.activeTiles.get(),
+                                                C.tileScan.get(), reinterpret_cast<const double*>(C.partial.get()),
+                                                ntx, nty, ntz);
+        launchCheck("reduce current");
+      }
       F.currentZero = false;
+      F.footprint.valid = true;
+      F.footprint.active = C.activeTiles.get();
+      F.footprint.count = static_cast<int>(nActive);
+      F.footprint.ntx = ntx;
+      F.footprint.nty = nty;
+      F.footprint.ntz = ntz;
+      F.footprint.tx = kTileX;
+      F.footprint.ty = kTileY;
+      F.footprint.tz = kTileZ;
     }
     {
       ScopedKernel sk(F.profiler, st, "residual");
-      if (n) kResidual<<<resBlocks, 256, 0, st>>>(A, G, F.g.dt, C.reduceBlocks.get(), C.error.get());
-      kReduceBlocks<<<1, 256, 0, st>>>(C.reduceBlocks.get(), n ? resBlocks : 0, C.reduceOut.get());
+      if (nChunks) kEsum<<<nActive, 256, 0, st>>>(G, C.activeTiles.get(), ntx, nty, ntz);
+      if (nChunks)
+        kGatherChunks<1><<<nChunks, 2 * kChunk, 0, st>>>(A, G, C.chunks.get(), ntx, nty, F.g.dt,
+                                                         C.reduceBlocks.get(), C.error.get());
+      kReduceBlocks<<<1, 256, 0, st>>>(C.reduceBlocks.get(), static_cast<int>(nChunks), C.reduceOut.get());
       launchCheck("residual");
     }
-    hipCheck(hipMemcpyAsync(host, C.reduceOut.get(), 4 * sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpy");
-    hipCheck(hipMemcpyAsync(&hostErr, C.error.get(), sizeof(unsigned), hipMemcpyDeviceToHost, st), "hipMemcpy");
+    hipCheck(hipMemcpyAsync(pinD, C.reduceOut.get(), 4 * sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpy");
+    hipCheck(hipMemcpyAsync(pin + 3, C.error.get(), sizeof(unsigned), hipMemcpyDeviceToHost, st), "hipMemcpy");
     hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
-    raiseErrors(hostErr);
+    raiseErrors(pin[3]);
     // simulation_source_coupling.cpp:46-51.
-    force = host[0];
-    work = host[3] == 0. ? 0. : (host[2] > 0. ? host[1] / host[2] : host[1]);
+    force = pinD[0];
+    work = pinD[3] == 0. ? 0. : (pinD[2] > 0. ? pinD[1] / pinD[2] : pinD[1]);
     iterations = trial + 1;
-    if (iterations >= (host[3] > 0. ? 2u : 1u) && force < 1e-13 && work < 1e-8) {
+    // The next trial pushes with the fields this residual gathered.
+    for (int a = 0; a < 6; ++a) std::swap(A.fin[a], A.fout[a]);
+    if (iterations >= (pinD[3] > 0. ? 2u : 1u) && force < 1e-13 && work < 1e-8) {
       converged = true;
       break;
     }
@@ -1030,16 +1385,16 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
   // commitConservativeElectric: E^n + curl B - dt/eps0 J, then the committed
   // CPML-E (history advance), PEC and inlet overwrite.
   double* const none[3] = {nullptr, nullptr, nullptr};
-  launchElectric(F, F.ptrs(), 0, none, F.window ? F.inlet[0].get() : nullptr, F.window ? F.inlet[1].get() : nullptr);
+  launchElectric(F, F.ptrs(), 0, none, F.window ? F.inletPlane[0] : nullptr, F.window ? F.inletPlane[1] : nullptr);
   if (F.window) F.phase = 3;
-  if (n) {
-    kCommitParticles<<<ceilDiv(n, 256), 256, 0, st>>>(A);
-    launchCheck("commit particles");
-  }
+  // The trial end state becomes the particle state.
+  for (int a = 0; a < 6; ++a) P.s[a].swap(P.end[a]);
   if (report) {
     report->iterations = iterations;
     report->force = force;
     report->work = work;
+    report->tiles = occupiedTiles;
+    report->chunks = nChunks;
   }
 }
 

@@ -54,12 +54,13 @@ struct ParticleStore {
   // SoA state: x y z ux uy uz q m w, id
   DeviceArray<double> s[9], t[9];
   DeviceArray<uint64_t> id, idT;
-  // coupling slots (per step)
-  DeviceArray<double> slot[25];  // guess3 usedE3 usedB3 nextE3 nextB3 x1_3 u1_3 gridAbs pad
-  DeviceArray<int> ready;
-  // deposit chord table written by the push: per particle segment count,
+  // Per-step slots: fields[0] and fields[1] alternate between the fields a
+  // trial pushes with (Ex Ey Ez Bx By Bz) and the residual gather's next
+  // fields; end[6] = trial end state x1, u1; gridAbs = per-substep work sum.
+  DeviceArray<double> fields[2][6], end[6], gridAbs;
+  // Deposit chord table written by the push: per particle the segment count,
   // chord velocity (cells) and current scale; per segment the midpoint cell
-  // (int4) and (fx, fy, fz, width) (double4)
+  // (int4) and (fx, fy, fz, width) (double4).
   DeviceArray<int> segCount;
   DeviceArray<double> segV;      // 6 per particle: v[3], scale[3]
   DeviceArray<int4> segCell;     // kMaxSegments per particle
@@ -69,6 +70,11 @@ struct ParticleStore {
   DeviceArray<unsigned char> sortTemp;
   std::size_t sortTempBytes = 0;
 };
+
+// Particles per work chunk: one chunk is a contiguous run of tile-sorted
+// particles of one tile. The deposit gives each chunk one workgroup of
+// kChunk lanes; the gathers give it 2*kChunk lanes (E waves and B waves).
+constexpr int kChunk = 128;
 
 struct CouplingState {
   FieldState* fields = nullptr;
@@ -80,13 +86,23 @@ struct CouplingState {
   double exitLabZ = 0.0;
   bool hasExit = false;
   DeviceArray<double> estar[3];
-  DeviceArray<int> tileBegin, tileEnd, tileSlot, occupied;
-  DeviceArray<double> partial;
-  std::size_t partialSlots = 0;
+  // E_cand + E_old of the trial on the nodes of the active tiles
+  DeviceArray<double> esum[3];
+  // tiles: particle range, chunk count (+1 entry for the scan total), first
+  // chunk, fixed-point exponent of the tile's current
+  DeviceArray<int> tileBegin, tileEnd, tileShift, activeTiles;
+  // per tile: chunk count (low 32 bits) and "has an occupied neighbour"
+  // (high 32 bits); the exclusive scan gives first chunk and active slot
+  DeviceArray<unsigned long long> tileCounts, tileScan;
+  DeviceArray<int4> chunks;  // (tile, first particle, end particle, 0)
+  std::size_t chunkCapacity = 0;
+  DeviceArray<unsigned char> scanTemp;
+  std::size_t scanTempBytes = 0;
+  DeviceArray<long long> partial;  // per chunk: 3 extended tile boxes of fixed-point current
   DeviceArray<double> reduceBlocks, reduceOut;
-  DeviceArray<unsigned> error, counter;
-  DeviceArray<MagnetDev> magnetsDev;
-  std::vector<double> hostReduce;
+  DeviceArray<unsigned> error;
+  PinnedHost pinned;
+  Event sorted;
 };
 
 }  // namespace aprl_gpu

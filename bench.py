@@ -23,23 +23,34 @@ HBM_PEAK_GBS = 8000.0
 WORD = 8  # FP64
 
 
-def kernel_bytes(w, name, launches_per_step, trials):
-    """Algorithmic HBM bytes of one launch (reads + writes of the state the
-    kernel must touch once): 9 words/cell magnetic, 12 words/cell electric,
-    14 words/particle push and residual gathers, 7 words/particle deposit."""
+def kernel_bytes(w, name, tiles, chunks):
+    """Algorithmic HBM bytes of one launch: what the kernel must read or
+    write at least once. Fields: 9 words/cell magnetic (read 3 E, read and
+    write 3 B), 12 words/cell electric (read 3 B, 3 J, read and write 3 E),
+    plus 2 words per active CPML history entry. Particles: push 14
+    words/particle (read x, u, q, m, 6 fields; write x, u); gathers 14
+    words/particle plus 9 words per cell of each occupied 8x8x4 tile (E*,
+    E_old, B); deposit 7 words/particle (x0, x1, q) plus 3 words per node of
+    each occupied tile's 13x11x9 extended box."""
     cells = w.cells
     n = w.particles
     cp = w.cpml
     nx, ny, nz = w.dims
-    psi_b = 2 * cp * (2 * (ny * nz) + 2 * (nx * nz)) * 2  # x- and y-slab histories, ~2 terms per face
+    psi = 2 * cp * (ny + nx) * nz * 2  # x and y slabs: two faces, two histories per field component pair
+    tile_cells = 8 * 8 * 4
+    ext_nodes = 13 * 11 * 9
     if name == "magnetic":
-        return WORD * (9 * cells + 2 * psi_b)
-    if name in ("electric", "electric_preview"):
-        return WORD * (12 * cells + 2 * psi_b)
-    if name in ("push", "residual"):
+        return WORD * (9 * cells + 2 * psi)
+    if name == "electric":
+        return WORD * (12 * cells + 2 * psi)
+    if name == "electric_preview":
+        return WORD * (9 * cells + psi)
+    if name == "push":
         return WORD * 14 * n
+    if name in ("gather", "residual"):
+        return WORD * (14 * n + 9 * tile_cells * tiles)
     if name == "deposit":
-        return WORD * 7 * n
+        return WORD * (7 * n + 3 * ext_nodes * tiles)
     return None
 
 
@@ -67,14 +78,14 @@ def run_gpu(args):
         s.step()
     fields.synchronize()
     kernels = {}
-    for name in ("magnetic", "electric_preview", "electric", "shift", "sort", "push", "deposit",
+    for name in ("magnetic", "electric_preview", "electric", "shift", "sort", "gather", "push", "deposit",
                  "reduce_current", "residual", "clear_current"):
         launches, ms = fields.profile_query(name)
         if launches:
             kernels[name] = {"launches": launches, "avg_us": 1e3 * ms / launches, "total_ms": ms}
     fields.profile(False)
     x, u, q, m, wt, ids = store.download()
-    return w, elapsed, trials, kernels, len(q)
+    return w, elapsed, trials, kernels, (coupling.last_tiles, coupling.last_chunks)
 
 
 def run_cpu_baseline(w, seconds_budget=25.0):
@@ -141,15 +152,15 @@ def main():
         print(f"bench.py measures one GPU only (the whole window lives on one device); --gpus {args.gpus} "
               "is not supported", file=sys.stderr)
         sys.exit(2)
-    w, elapsed, trials, kernels, count = run_gpu(args)
+    w, elapsed, trials, kernels, (tiles, chunks) = run_gpu(args)
     ms = 1e3 * elapsed / args.steps
     value = w.particles * args.steps / elapsed
-    dominant = max(kernels.items(), key=lambda kv: kv[1]["total_ms"])[0] if kernels else None
+    modeled = {k: v for k, v in kernels.items() if kernel_bytes(w, k, tiles, chunks) is not None}
+    dominant = max(modeled.items(), key=lambda kv: kv[1]["total_ms"])[0] if modeled else None
     roofline = None
     if dominant:
         k = kernels[dominant]
-        steps_profiled = max(5, min(args.steps, 20))
-        b = kernel_bytes(w, dominant, k["launches"] / steps_profiled, sum(trials) / len(trials))
+        b = kernel_bytes(w, dominant, tiles, chunks)
         if b is not None:
             achieved = b / (k["avg_us"] * 1e-6) / 1e9
             roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -171,10 +182,13 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": w.name, "grid": list(w.dims), "particles": w.particles, "form": "plain FP64",
-                   "shape": "linear-binomial3 + x-binomial3", "picard_trials_mean": sum(trials) / len(trials)},
+                   "shape": "linear-binomial3 + x-binomial3", "picard_trials_mean": sum(trials) / len(trials),
+                   "occupied_tiles": tiles, "chunks": chunks},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels_us": {k: round(v["avg_us"], 2) for k, v in kernels.items()},
+        "kernels_gbs": {k: round(kernel_bytes(w, k, tiles, chunks) / (v["avg_us"] * 1e-6) / 1e9, 1)
+                        for k, v in modeled.items()},
     }
     print(json.dumps(line))
 

@@ -158,6 +158,8 @@ typedef struct aprl_step_report {
   unsigned iterations; /* Picard trials */
   double force;        /* self-force defect of the last trial */
   double work;         /* relative fixed-point work defect */
+  unsigned tiles;      /* occupied cell tiles (8x8x4 cells) of the step */
+  unsigned chunks;     /* particle work chunks (at most 256 particles of one tile) */
 } aprl_step_report;
 
 /* ConservativeSourceCoupling (source_coupling.h:12-41) bound to a field
