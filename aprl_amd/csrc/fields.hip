@@ -458,55 +458,21 @@ __global__ void __launch_bounds__(256) kMagneticCowanEdges(FieldPtrs F, StepCoef
   if (u < nx * nz) magneticBy<true>(F, c, P, u % nx, ny, u / nx);
 }
 
-// z-marching LDS version of the Cowan magnetic update for i < nx, j < ny. A
-// block owns a (kMX x kMY) column of nodes and kMZ consecutive z planes. E
-// is staged in a three-plane rolling buffer of electricValue() samples
-// (reflection / halo / PEC parity applied while loading); per plane the six
-// smoothed fields the curl needs are computed once into LDS (each is the
-// same nine products in the same order as smoothedElectric()), then every
-// thread finishes its Bx, By and Bz nodes.
-constexpr int kMX = 32, kMY = 8, kMZ = 8;
-constexpr int kLX = kMX + 2, kLY = kMY + 2;
+// Tiled Cowan magnetic update for i < nx, j < ny, one workgroup per
+// (kMX x kMY) tile of one z plane k. E planes k-1, k, k+1 of the tile plus a
+// one-node halo are staged in LDS as electricValue() samples (reflection /
+// halo / PEC parity applied while loading; all loads of a lane are issued
+// before the first store); each smoothed sample is then the same nine
+// products in the same order as smoothedElectric().
+constexpr int kMX = 32, kMY = 8;
+constexpr int kLX = kMX + 2, kLY = kMY + 2, kLP = kLX * kLY;
+constexpr int kStageN = 9 * kLP;  // 3 components x 3 planes
+constexpr int kStageR = (kStageN + kMX * kMY - 1) / (kMX * kMY);
 
-// electricValue() samples of one staged plane: lane t holds slots
-// t + r * blockDim for r < kPlaneRegs.
-constexpr int kPlaneRegs = (kLX * kLY + kMX * kMY - 1) / (kMX * kMY);
-template <int A>
-__device__ __forceinline__ void loadPlane(const FieldPtrs& F, double reg[kPlaneRegs], int i0, int j0, int kk) {
-  const int n0 = F.n[A][0], n1 = F.n[A][1], n2 = F.n[A][2];
-  const bool zin = kk >= 0 && kk < n2;
-#pragma unroll
-  for (int r = 0; r < kPlaneRegs; ++r) {
-    const int t = static_cast<int>(threadIdx.x) + r * kMX * kMY;
-    const int li = t % kLX, lj = t / kLX;
-    const int i = i0 - 1 + li, j = j0 - 1 + lj;
-    double v = 0.0;  // nodes more than one beyond the array feed no B node
-    if (t < kLX * kLY) {
-      if (zin && i >= 0 && i < n0 && j >= 0 && j < n1) v = F.e[A][(static_cast<size_t>(kk) * n1 + j) * n0 + i];
-      else if (i >= -1 && i <= n0 && j >= -1 && j <= n1) v = evalE<A>(F, i, j, kk);
-    }
-    reg[r] = v;
-  }
-}
-__device__ __forceinline__ void storePlane(double* dst, const double reg[kPlaneRegs]) {
-#pragma unroll
-  for (int r = 0; r < kPlaneRegs; ++r) {
-    const int t = static_cast<int>(threadIdx.x) + r * kMX * kMY;
-    if (t < kLX * kLY) dst[t] = reg[r];
-  }
-}
-template <int A>
-__device__ __forceinline__ void stagePlane(const FieldPtrs& F, double* dst, int i0, int j0, int kk) {
-  double reg[kPlaneRegs];
-  loadPlane<A>(F, reg, i0, j0, kk);
-  storePlane(dst, reg);
-}
-
-// Smoothed sample from the staged planes; (li, lj) local, s0/s1/s2 the
-// rolling slots of planes k-1, k, k+1. D is the derivative axis.
+// Smoothed sample from the staged planes; (li, lj) local.
+// pl[0] = plane k-1, pl[1] = plane k, pl[2] = plane k+1; D is the derivative axis.
 template <int D>
 __device__ __forceinline__ double smoothL(const double* const pl[3], const StepCoefs& c, int li, int lj) {
-  // pl[0] = plane k-1, pl[1] = plane k, pl[2] = plane k+1
   constexpr int TA = (D + 1) % 3, TB = (D + 2) % 3;
   const double delta = D == 0 ? c.dYZ : (D == 1 ? c.dZX : c.dXY);
   auto at = [&](int dx, int dy, int dz) { return pl[1 + dz][(lj + dy) * kLX + li + dx]; };
@@ -531,102 +497,59 @@ __device__ __forceinline__ double smoothL(const double* const pl[3], const StepC
   return r;
 }
 
-// Smoothed fields of one plane k (nodes relative to the tile origin):
-// SyEz, SyEx: S_y at (a < kMX, b <= kMY); SxEz, SxEy: S_x at (a <= kMX,
-// b < kMY); SzEy, SzEx: S_z at plane k+1 (a < kMX, b < kMY), two rolling slots.
-constexpr int kSyN = kMX * (kMY + 1), kSxN = (kMX + 1) * kMY, kSzN = kMX * kMY;
-constexpr int kSmoothItems = 2 * kSyN + 2 * kSxN + 2 * kSzN;
+template <int A>
+__device__ __forceinline__ double stagedE(const FieldPtrs& F, int i, int j, int kk) {
+  const int n0 = F.n[A][0], n1 = F.n[A][1], n2 = F.n[A][2];
+  if (kk >= 0 && kk < n2 && i >= 0 && i < n0 && j >= 0 && j < n1)
+    return F.e[A][(static_cast<size_t>(kk) * n1 + j) * n0 + i];
+  if (i >= -1 && i <= n0 && j >= -1 && j <= n1) return evalE<A>(F, i, j, kk);
+  return 0.0;  // more than one node beyond the array: feeds no B node
+}
 
 __global__ void __launch_bounds__(kMX * kMY) kMagneticCowanTiled(FieldPtrs F, StepCoefs c, CpmlArgs P) {
-  __shared__ double buf[3][3][kLX * kLY];  // [component][slot][plane]
-  __shared__ double syEz[kSyN], syEx[kSyN], sxEz[kSxN], sxEy[kSxN];
-  __shared__ double szEy[2][kSzN], szEx[2][kSzN];
+  __shared__ double buf[kStageN];  // [component][plane k-1, k, k+1][kLY][kLX]
   const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
-  const int i0 = blockIdx.x * kMX, j0 = blockIdx.y * kMY;
-  const int k0 = blockIdx.z * kMZ, k1 = min(k0 + kMZ, nz + 1);
+  const int i0 = blockIdx.x * kMX, j0 = blockIdx.y * kMY, k = blockIdx.z;
   const int tx = static_cast<int>(threadIdx.x) % kMX, ty = static_cast<int>(threadIdx.x) / kMX;
   const int i = i0 + tx, j = j0 + ty;
-  auto slot = [](int kk) { return ((kk % 3) + 3) % 3; };
-  auto planes = [&](int comp, int kk, const double* out[3]) {
-    out[0] = buf[comp][slot(kk - 1)];
-    out[1] = buf[comp][slot(kk)];
-    out[2] = buf[comp][slot(kk + 1)];
-  };
-  stagePlane<0>(F, buf[0][slot(k0 - 1)], i0, j0, k0 - 1);
-  stagePlane<1>(F, buf[1][slot(k0 - 1)], i0, j0, k0 - 1);
-  stagePlane<2>(F, buf[2][slot(k0 - 1)], i0, j0, k0 - 1);
-  stagePlane<0>(F, buf[0][slot(k0)], i0, j0, k0);
-  stagePlane<1>(F, buf[1][slot(k0)], i0, j0, k0);
-  stagePlane<2>(F, buf[2][slot(k0)], i0, j0, k0);
+  {
+    double v[kStageR];
+#pragma unroll
+    for (int r = 0; r < kStageR; ++r) {
+      const int t = static_cast<int>(threadIdx.x) + r * kMX * kMY;
+      v[r] = 0.0;
+      if (t < kStageN) {
+        const int comp = t / (3 * kLP), rest = t - comp * 3 * kLP;
+        const int pz = rest / kLP, q = rest - pz * kLP;
+        const int gi = i0 - 1 + q % kLX, gj = j0 - 1 + q / kLX, gk = k - 1 + pz;
+        v[r] = comp == 0 ? stagedE<0>(F, gi, gj, gk) : (comp == 1 ? stagedE<1>(F, gi, gj, gk) : stagedE<2>(F, gi, gj, gk));
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kStageR; ++r) {
+      const int t = static_cast<int>(threadIdx.x) + r * kMX * kMY;
+      if (t < kStageN) buf[t] = v[r];
+    }
+  }
   __syncthreads();
-  {  // S_z of plane k0 into rolling slot k0 & 1
-    const double* px[3] = {buf[0][slot(k0)], buf[0][slot(k0)], buf[0][slot(k0)]};
-    const double* py[3] = {buf[1][slot(k0)], buf[1][slot(k0)], buf[1][slot(k0)]};
-    for (int t = threadIdx.x; t < 2 * kSzN; t += blockDim.x) {
-      const int u = t % kSzN, a = u % kMX, b = u / kMX;
-      if (t < kSzN) szEy[k0 & 1][u] = smoothL<2>(py, c, a + 1, b + 1);
-      else szEx[k0 & 1][u] = smoothL<2>(px, c, a + 1, b + 1);
-    }
+  if (i >= nx || j >= ny) return;
+  const double* ex[3] = {buf, buf + kLP, buf + 2 * kLP};
+  const double* ey[3] = {buf + 3 * kLP, buf + 4 * kLP, buf + 5 * kLP};
+  const double* ez[3] = {buf + 6 * kLP, buf + 7 * kLP, buf + 8 * kLP};
+  const int li = tx + 1, lj = ty + 1;
+  if (k < nz) {
+    // S_z is x-y smoothing: only its own plane is read
+    const double* eyk[3] = {ey[1], ey[1], ey[1]};
+    const double* eyk1[3] = {ey[2], ey[2], ey[2]};
+    const double* exk[3] = {ex[1], ex[1], ex[1]};
+    const double* exk1[3] = {ex[2], ex[2], ex[2]};
+    finishBx(F, c, P, i, j, k, smoothL<1>(ez, c, li, lj + 1) - smoothL<1>(ez, c, li, lj),
+             smoothL<2>(eyk1, c, li, lj) - smoothL<2>(eyk, c, li, lj));
+    finishBy(F, c, P, i, j, k, smoothL<2>(exk1, c, li, lj) - smoothL<2>(exk, c, li, lj),
+             smoothL<0>(ez, c, li + 1, lj) - smoothL<0>(ez, c, li, lj));
   }
-  // plane k+1 of E is loaded into registers one iteration ahead, so its
-  // global latency overlaps the smoothing and the B update of plane k
-  double rx[kPlaneRegs], ry[kPlaneRegs], rz[kPlaneRegs];
-  loadPlane<0>(F, rx, i0, j0, k0 + 1);
-  loadPlane<1>(F, ry, i0, j0, k0 + 1);
-  loadPlane<2>(F, rz, i0, j0, k0 + 1);
-  const bool mine = i < nx && j < ny;
-  for (int k = k0; k < k1; ++k) {
-    storePlane(buf[0][slot(k + 1)], rx);
-    storePlane(buf[1][slot(k + 1)], ry);
-    storePlane(buf[2][slot(k + 1)], rz);
-    if (k + 1 < k1) {
-      loadPlane<0>(F, rx, i0, j0, k + 2);
-      loadPlane<1>(F, ry, i0, j0, k + 2);
-      loadPlane<2>(F, rz, i0, j0, k + 2);
-    }
-    double bOld[3] = {0.0, 0.0, 0.0};
-    if (mine) {
-      if (k < nz) {
-        bOld[0] = F.b[0][bIndex(F, 0, i, j, k)];
-        bOld[1] = F.b[1][bIndex(F, 1, i, j, k)];
-      }
-      bOld[2] = F.b[2][bIndex(F, 2, i, j, k)];
-    }
-    __syncthreads();
-    {
-      const double *ex[3], *ey[3], *ez[3];
-      planes(0, k, ex);
-      planes(1, k, ey);
-      planes(2, k, ez);
-      const double* ex1[3] = {ex[2], ex[2], ex[2]};
-      const double* ey1[3] = {ey[2], ey[2], ey[2]};
-      const int nk = (k + 1) & 1;
-      for (int t = threadIdx.x; t < kSmoothItems; t += blockDim.x) {
-        int u = t;
-        if (u < kSyN) { syEz[u] = smoothL<1>(ez, c, u % kMX + 1, u / kMX + 1); continue; }
-        u -= kSyN;
-        if (u < kSyN) { syEx[u] = smoothL<1>(ex, c, u % kMX + 1, u / kMX + 1); continue; }
-        u -= kSyN;
-        if (u < kSxN) { sxEz[u] = smoothL<0>(ez, c, u % (kMX + 1) + 1, u / (kMX + 1) + 1); continue; }
-        u -= kSxN;
-        if (u < kSxN) { sxEy[u] = smoothL<0>(ey, c, u % (kMX + 1) + 1, u / (kMX + 1) + 1); continue; }
-        u -= kSxN;
-        if (u < kSzN) { szEy[nk][u] = smoothL<2>(ey1, c, u % kMX + 1, u / kMX + 1); continue; }
-        u -= kSzN;
-        szEx[nk][u] = smoothL<2>(ex1, c, u % kMX + 1, u / kMX + 1);
-      }
-    }
-    __syncthreads();
-    if (mine) {
-      const int ck = k & 1, nk = (k + 1) & 1;
-      const int sy = ty * kMX + tx, sx = ty * (kMX + 1) + tx, sz = ty * kMX + tx;
-      if (k < nz) {
-        finishBx(F, c, P, i, j, k, syEz[sy + kMX] - syEz[sy], szEy[nk][sz] - szEy[ck][sz], &bOld[0]);
-        finishBy(F, c, P, i, j, k, szEx[nk][sz] - szEx[ck][sz], sxEz[sx + 1] - sxEz[sx], &bOld[1]);
-      }
-      finishBz(F, c, P, i, j, k, sxEy[sx + 1] - sxEy[sx], syEx[sy + kMX] - syEx[sy], &bOld[2]);
-    }
-  }
+  finishBz(F, c, P, i, j, k, smoothL<0>(ey, c, li + 1, lj) - smoothL<0>(ey, c, li, lj),
+           smoothL<1>(ex, c, li, lj + 1) - smoothL<1>(ex, c, li, lj));
 }
 
 __device__ __forceinline__ double rd(const double* p, int nx, int ny, int i, int j, int k) {
@@ -641,7 +564,8 @@ __device__ __forceinline__ double rd(const double* p, int nx, int ny, int i, int
 // zero-current preview into `out` and leaves the histories untouched.
 template <int MODE>
 __global__ void __launch_bounds__(256) kElectric(FieldPtrs F, StepCoefs c, CpmlArgs P, double* ox, double* oy,
-                                                  double* oz, const double* inletX, const double* inletY) {
+                                                  double* oz, double* sx, double* sy, double* sz, const double* inletX,
+                                                  const double* inletY) {
   // one thread per node of the (nx+1) x (ny+1) plane k = blockIdx.y
   const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -655,7 +579,8 @@ __global__ void __launch_bounds__(256) kElectric(FieldPtrs F, StepCoefs c, CpmlA
   const int kb = P.lowerZ ? 1 : 0, ke = P.upperZ ? nz : nz + 1;
   if (i < nx) {  // Ex: nx x (ny+1) x (nz+1)
     const size_t idx = (static_cast<size_t>(k) * (ny + 1) + j) * nx + i;
-    double e = F.e[0][idx];
+    const double e0 = F.e[0][idx];
+    double e = e0;
     if (k >= 1 && k < nz && j >= 1 && j < ny)
       e += c.cY * (rd(bz, nx, ny, i, j, k) - rd(bz, nx, ny, i, j - 1, k)) -
            c.cZ * (rd(by, nx, ny + 1, i, j, k) - rd(by, nx, ny + 1, i, j, k - 1)) - c.cs * F.j[0][idx];
@@ -672,10 +597,12 @@ __global__ void __launch_bounds__(256) kElectric(FieldPtrs F, StepCoefs c, CpmlA
     if (j == 0 || j == ny || (P.lowerZ && k == 0) || (P.upperZ && k == nz)) e = 0.0;
     if (inletX && k == 0) e = inletX[static_cast<size_t>(j) * nx + i];
     (MODE == 0 ? F.e[0] : ox)[idx] = e;
+    if (MODE == 1 && sx) sx[idx] = e + e0;
   }
   if (j < ny) {  // Ey: (nx+1) x ny x (nz+1)
     const size_t idx = (static_cast<size_t>(k) * ny + j) * (nx + 1) + i;
-    double e = F.e[1][idx];
+    const double e0 = F.e[1][idx];
+    double e = e0;
     if (k >= 1 && k < nz && i >= 1 && i < nx)
       e += c.cZ * (rd(bx, nx + 1, ny, i, j, k) - rd(bx, nx + 1, ny, i, j, k - 1)) -
            c.cX * (rd(bz, nx, ny, i, j, k) - rd(bz, nx, ny, i - 1, j, k)) - c.cs * F.j[1][idx];
@@ -692,10 +619,12 @@ __global__ void __launch_bounds__(256) kElectric(FieldPtrs F, StepCoefs c, CpmlA
     if (i == 0 || i == nx || (P.lowerZ && k == 0) || (P.upperZ && k == nz)) e = 0.0;
     if (inletY && k == 0) e = inletY[static_cast<size_t>(j) * (nx + 1) + i];
     (MODE == 0 ? F.e[1] : oy)[idx] = e;
+    if (MODE == 1 && sy) sy[idx] = e + e0;
   }
   if (k < nz) {  // Ez: (nx+1) x (ny+1) x nz
     const size_t idx = (static_cast<size_t>(k) * (ny + 1) + j) * (nx + 1) + i;
-    double e = F.e[2][idx];
+    const double e0 = F.e[2][idx];
+    double e = e0;
     const bool in = i >= 1 && i < nx && j >= 1 && j < ny;
     if (in)
       e += c.cX * (rd(by, nx, ny + 1, i, j, k) - rd(by, nx, ny + 1, i - 1, j, k)) -
@@ -712,6 +641,7 @@ __global__ void __launch_bounds__(256) kElectric(FieldPtrs F, StepCoefs c, CpmlA
     }
     if (i == 0 || i == nx || j == 0 || j == ny) e = 0.0;
     (MODE == 0 ? F.e[2] : oz)[idx] = e;
+    if (MODE == 1 && sz) sz[idx] = e + e0;
   }
 }
 
@@ -733,7 +663,7 @@ void launchMagnetic(FieldState& s, const FieldPtrs& f) {
   const CpmlArgs P = s.cpmlArgs();
   ScopedKernel sk(s.profiler, s.stream, "magnetic");
   if (s.g.solver == 1) {
-    dim3 tg(ceilDiv(s.g.nx, kMX), ceilDiv(s.g.ny, kMY), ceilDiv(s.g.nz + 1, kMZ));
+    dim3 tg(ceilDiv(s.g.nx, kMX), ceilDiv(s.g.ny, kMY), static_cast<unsigned>(s.g.nz + 1));
     kMagneticCowanTiled<<<tg, kMX * kMY, 0, s.stream>>>(f, c, P);
     kMagneticCowanEdges<<<ceilDiv((s.g.nx + s.g.ny) * s.g.nz, 256), 256, 0, s.stream>>>(f, c, P);
   } else {
@@ -744,7 +674,7 @@ void launchMagnetic(FieldState& s, const FieldPtrs& f) {
 }
 
 void launchElectric(FieldState& s, const FieldPtrs& f, int mode, double* const out[3], const double* inletX,
-                    const double* inletY) {
+                    const double* inletY, double* const sum[3]) {
   const StepCoefs c = s.coefs();
   CpmlArgs P = s.cpmlArgs();
   if (!s.hasCpml) {  // PerfectElectricConductorBoundary(lowerZ, upperZ)
@@ -755,10 +685,12 @@ void launchElectric(FieldState& s, const FieldPtrs& f, int mode, double* const o
   dim3 grid(ceilDiv((s.g.nx + 1) * (s.g.ny + 1), 256), static_cast<unsigned>(s.g.nz + 1));
   if (mode == 0) {
     ScopedKernel sk(s.profiler, s.stream, "electric");
-    kElectric<0><<<grid, block, 0, s.stream>>>(f, c, P, nullptr, nullptr, nullptr, inletX, inletY);
+    kElectric<0><<<grid, block, 0, s.stream>>>(f, c, P, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, inletX,
+                                               inletY);
   } else {
     ScopedKernel sk(s.profiler, s.stream, "electric_preview");
-    kElectric<1><<<grid, block, 0, s.stream>>>(f, c, P, out[0], out[1], out[2], inletX, inletY);
+    kElectric<1><<<grid, block, 0, s.stream>>>(f, c, P, out[0], out[1], out[2], sum ? sum[0] : nullptr,
+                                               sum ? sum[1] : nullptr, sum ? sum[2] : nullptr, inletX, inletY);
   }
   launchCheck("electric");
 }

@@ -111,8 +111,9 @@ void installCpml(FieldState& s, const aprl_cpml_parameters& p, std::size_t globa
 void launchMagnetic(FieldState& s, const FieldPtrs& f);
 // mode 0: in place with history advance; mode 1: preview into `out` without
 // touching histories; the window inlet overwrite applies in both modes.
+// mode 1 with sum != nullptr also writes sum = preview + old E.
 void launchElectric(FieldState& s, const FieldPtrs& f, int mode, double* const out[3],
-                    const double* inletX, const double* inletY);
+                    const double* inletX, const double* inletY, double* const sum[3] = nullptr);
 void launchClearCurrent(FieldState& s);
 void windowShift(FieldState& s, const double* const planes[6]);
 

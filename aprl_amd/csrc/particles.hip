@@ -22,7 +22,7 @@ namespace aprl_gpu {
 // Kernel argument blocks.
 
 struct GArgs {
-  const double* esum[3];   // E_cand + E_old on the active tiles' nodes
+  double* esum[3];         // E_cand + E_old of the trial
   const double* estar[3];  // zero-current preview E*
   const double* eold[3];   // E^n
   const double* j[3];      // current of the trial
@@ -276,22 +276,6 @@ __device__ int orbitCuts(const double x[3], const double v[3], double cuts[kMaxC
   return count;
 }
 
-// Candidate E (applyCurrentSource(-1) applied to the preview) and old E.
-__device__ __forceinline__ double ecand(const GArgs& G, int d, int i, int j, int k) {
-  const size_t idx = (static_cast<size_t>(k) * G.ne[d][1] + j) * G.ne[d][0] + i;
-  double e = G.estar[d][idx];
-  const bool applied = d == 0 ? (j > 0 && j < G.ny && k != 0 && k != G.nz)
-                              : (d == 1 ? (i > 0 && i < G.nx && k != 0 && k != G.nz) : (i > 0 && i < G.nx && j > 0 && j < G.ny));
-  if (applied && G.j[d]) {
-    const double c = G.j[d][idx];
-    if (c != 0.) e += G.negS * c;
-  }
-  return e;
-}
-__device__ __forceinline__ double eoldAt(const GArgs& G, int d, int i, int j, int k) {
-  return G.eold[d][(static_cast<size_t>(k) * G.ne[d][1] + j) * G.ne[d][0] + i];
-}
-
 // Extended node box of a cell tile: every binomial3-x tap of a chord that
 // starts in the tile and moves less than one cell per axis lies inside.
 struct TileBox {
@@ -357,27 +341,6 @@ __device__ void stageTile(const GArgs& G, const TileBox& box, double* es, double
     if (t < 3 * kExtNodes) {
       es[t] = ev[r];
       bs[t] = bv[r];
-    }
-  }
-}
-
-// E_cand + E_old on the owned nodes of every active tile (the union of the
-// staged boxes of the occupied tiles). E_cand is the preview E* with the
-// trial current applied (applyCurrentSource(-1)); G.j == nullptr means the
-// current is treated as zero.
-__global__ void __launch_bounds__(256) kEsum(GArgs G, const int* active, int ntx, int nty, int ntz) {
-  const int tile = active[blockIdx.x];
-  const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
-  for (int d = 0; d < 3; ++d) {
-    const int n0 = G.ne[d][0], n1 = G.ne[d][1], n2 = G.ne[d][2];
-    const int i0 = tx * kTileX, i1 = tx == ntx - 1 ? n0 : min(n0, i0 + kTileX);
-    const int j0 = ty * kTileY, j1 = ty == nty - 1 ? n1 : min(n1, j0 + kTileY);
-    const int k0 = tz * kTileZ, k1 = tz == ntz - 1 ? n2 : min(n2, k0 + kTileZ);
-    const int wx = i1 - i0, wy = j1 - j0, wz = k1 - k0;
-    double* dst = const_cast<double*>(G.esum[d]);
-    for (int t = threadIdx.x; t < wx * wy * wz; t += blockDim.x) {
-      const int i = i0 + t % wx, j = j0 + (t / wx) % wy, k = k0 + t / (wx * wy);
-      dst[(static_cast<size_t>(k) * n1 + j) * n0 + i] = ecand(G, d, i, j, k) + eoldAt(G, d, i, j, k);
     }
   }
 }
@@ -934,52 +897,48 @@ __global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, const int4*
     }
   }
   __syncthreads();
-  long long* out = partial + static_cast<size_t>(blockIdx.x) * 3 * kExtNodes;
-  for (int t = threadIdx.x; t < 3 * kExtNodes; t += blockDim.x) out[t] = static_cast<long long>(acc[t]);
+  // The tile's partial sits at its first chunk's slot and holds zeros on
+  // entry (the reduction clears what it reads). A tile of one chunk stores
+  // its box; chunks of a larger tile add their nonzero nodes with integer
+  // atomics, whose sum is independent of their order.
+  const int first = lowWord(tileScan[tile]);
+  const bool alone = lowWord(tileScan[tile + 1]) - first == 1;
+  unsigned long long* out =
+      reinterpret_cast<unsigned long long*>(partial + static_cast<size_t>(first) * 3 * kExtNodes);
+  for (int t = threadIdx.x; t < 3 * kExtNodes; t += blockDim.x) {
+    const unsigned long long v = acc[t];
+    if (alone) out[t] = v;
+    else if (v) atomicAdd(out + t, v);
+  }
   if (err) atomicOr(errOut, err);
 }
 
-// Exact integer total of a tile's chunk partials, converted with the
-// tile's exponent and stored as doubles over the tile's first chunk
-// partial. One workgroup per chunk; only a tile's first chunk works.
-__global__ void __launch_bounds__(256) kCombineTiles(const int4* chunks, const unsigned long long* tileScan,
-                                                     const int* tileShift, long long* partial) {
-  const int c = blockIdx.x;
-  const int tile = chunks[c].x;
-  const int first = lowWord(tileScan[tile]);
-  if (c != first) return;
-  const int count = lowWord(tileScan[tile + 1]) - first;
-  const int shift = tileShift[tile];
-  constexpr size_t chunkStride = 3 * static_cast<size_t>(kExtNodes);
-  long long* src = partial + static_cast<size_t>(c) * chunkStride;
-  double* dst = reinterpret_cast<double*>(src);
-  for (int t = threadIdx.x; t < 3 * kExtNodes; t += blockDim.x) {
-    long long v = src[t];
-    for (int k = 1; k < count; ++k) v += src[k * chunkStride + t];
-    dst[t] = ldexp(static_cast<double>(v), -shift);
-  }
-}
-
 // Node-owner reduction into J, one workgroup per active cell tile (a tile
-// with an occupied neighbour): each owned node sums the combined boxes of
-// the covering occupied tiles in fixed (tz, ty, tx) order. Nodes of other
-// tiles keep the zeros J held at the start of the step.
+// with an occupied neighbour): each owned node sums, over the covering
+// occupied tiles in fixed (tz, ty, tx) order, the tile's exact integer
+// total converted with the tile's exponent, and clears what it read for
+// the next deposit. Nodes of other tiles keep the zeros J held at the
+// start of the step.
 __global__ void __launch_bounds__(256) kReduceCurrent(double* jx, double* jy, double* jz, GArgs G, const int* active,
-                                                      const unsigned long long* tileScan, const double* boxes,
-                                                      int ntx, int nty, int ntz) {
+                                                      const unsigned long long* tileScan, const int* tileShift,
+                                                      long long* partial, int ntx, int nty, int ntz) {
   const int tile = active[blockIdx.x];
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
-  __shared__ int nbFirst[27];
+  __shared__ int nbFirst[27], nbShift[27];
   if (threadIdx.x < 27) {
     const int x = tx + static_cast<int>(threadIdx.x) % 3 - 1, y = ty + (static_cast<int>(threadIdx.x) / 3) % 3 - 1,
               z = tz + static_cast<int>(threadIdx.x) / 9 - 1;
-    int first = -1;
+    int first = -1, sh = 0;
     if (x >= 0 && x < ntx && y >= 0 && y < nty && z >= 0 && z < ntz) {
       const int s = (z * nty + y) * ntx + x;
       const int f = lowWord(tileScan[s]);
-      if (lowWord(tileScan[s + 1]) > f) first = f;
+      if (lowWord(tileScan[s + 1]) > f) {
+        first = f;
+        sh = tileShift[s];
+      }
     }
     nbFirst[threadIdx.x] = first;
+    nbShift[threadIdx.x] = sh;
   }
   __syncthreads();
   constexpr size_t chunkStride = 3 * static_cast<size_t>(kExtNodes);
@@ -993,23 +952,36 @@ __global__ void __launch_bounds__(256) kReduceCurrent(double* jx, double* jy, do
     double* dst = d == 0 ? jx : (d == 1 ? jy : jz);
     for (int t = threadIdx.x; t < wx * wy * wz; t += blockDim.x) {
       const int i = i0 + t % wx, j = j0 + (t / wx) % wy, k = k0 + t / (wx * wy);
-      double v[27];
-      bool on[27];
+      long long v[27];
+      long long* at[27];
 #pragma unroll
       for (int nb = 0; nb < 27; ++nb) {
         const int ox = nb % 3 - 1, oy = (nb / 3) % 3 - 1, oz = nb / 9 - 1;
         const int li = i - ((tx + ox) * kTileX - 2), lj = j - ((ty + oy) * kTileY - 1),
                   lk = k - ((tz + oz) * kTileZ - 2);
         const int f = nbFirst[nb];
-        on[nb] = f >= 0 && li >= 0 && li < kExtX && lj >= 0 && lj < kExtY && lk >= 0 && lk < kExtZ;
-        v[nb] = on[nb] ? boxes[static_cast<size_t>(f) * chunkStride + d * kExtNodes + (lk * kExtY + lj) * kExtX + li]
-                       : 0.0;
+        const bool on = f >= 0 && li >= 0 && li < kExtX && lj >= 0 && lj < kExtY && lk >= 0 && lk < kExtZ;
+        at[nb] = on ? partial + static_cast<size_t>(f) * chunkStride + d * kExtNodes + (lk * kExtY + lj) * kExtX + li
+                    : nullptr;
+        v[nb] = on ? *at[nb] : 0;
       }
       double sum = 0.0;
 #pragma unroll
       for (int nb = 0; nb < 27; ++nb)
-        if (on[nb]) sum += v[nb];
-      dst[(static_cast<size_t>(k) * n1 + j) * n0 + i] = sum;
+        if (at[nb]) {
+          sum += ldexp(static_cast<double>(v[nb]), -nbShift[nb]);
+          *at[nb] = 0;
+        }
+      const size_t idx = (static_cast<size_t>(k) * n1 + j) * n0 + i;
+      dst[idx] = sum;
+      // E_cand + E_old of this trial (applyCurrentSource(-1) on the preview,
+      // source_coupling.cpp:84-89) for the next residual gather
+      const bool applied = d == 0 ? (j > 0 && j < G.ny && k != 0 && k != G.nz)
+                                  : (d == 1 ? (i > 0 && i < G.nx && k != 0 && k != G.nz)
+                                            : (i > 0 && i < G.nx && j > 0 && j < G.ny));
+      double e = G.estar[d][idx];
+      if (applied && sum != 0.) e += G.negS * sum;
+      G.esum[d][idx] = e + G.eold[d][idx];
     }
   }
 }
@@ -1288,7 +1260,9 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
   // Zero-current preview E* (MovingWindowFields::previewElectricBoundary);
   // it runs while the host waits for the chunk count.
   double* const out[3] = {C.estar[0].get(), C.estar[1].get(), C.estar[2].get()};
-  launchElectric(F, F.ptrs(), 1, out, F.window ? F.inletPlane[0] : nullptr, F.window ? F.inletPlane[1] : nullptr);
+  double* const sums[3] = {C.esum[0].get(), C.esum[1].get(), C.esum[2].get()};
+  launchElectric(F, F.ptrs(), 1, out, F.window ? F.inletPlane[0] : nullptr, F.window ? F.inletPlane[1] : nullptr,
+                 sums);
   unsigned nChunks = 0, nActive = 0, occupiedTiles = 0;
   if (n) {
     hipCheck(hipEventSynchronize(C.sorted.get()), "hipEventSynchronize");
@@ -1298,7 +1272,11 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
     if (nChunks > chunkBound) throw std::logic_error("Chunk list exceeds its bound");
   }
   const std::size_t partialNeed = static_cast<std::size_t>(nChunks) * 3 * kExtNodes;
-  if (partialNeed > C.partial.size()) C.partial.resize(partialNeed);
+  if (partialNeed > C.partial.size()) {
+    // tile partials must start at zero; the reduction keeps them so
+    C.partial.resize(partialNeed);
+    hipCheck(hipMemsetAsync(C.partial.get(), 0, partialNeed * sizeof(long long), st), "hipMemsetAsync");
+  }
   if (C.reduceBlocks.size() < 4 * static_cast<std::size_t>(std::max(1u, nChunks)))
     C.reduceBlocks.resize(4 * static_cast<std::size_t>(std::max(1u, nChunks)));
 
@@ -1315,7 +1293,6 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
       GArgs G0 = G;
       for (int a = 0; a < 3; ++a) G0.j[a] = nullptr;
       ScopedKernel sk(F.profiler, st, "gather");
-      kEsum<<<nActive, 256, 0, st>>>(G0, C.activeTiles.get(), ntx, nty, ntz);
       kGatherChunks<0><<<nChunks, 2 * kChunk, 0, st>>>(A, G0, C.chunks.get(), ntx, nty, F.g.dt, nullptr,
                                                        C.error.get());
       launchCheck("gather");
@@ -1335,11 +1312,8 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
       }
       {
         ScopedKernel sk(F.profiler, st, "reduce_current");
-        kCombineTiles<<<nChunks, 256, 0, st>>>(C.chunks.get(), C.tileScan.get(), C.tileShift.get(),
-                                               C.partial.get());
         kReduceCurrent<<<nActive, 256, 0, st>>>(F.base(6), F.base(7), F.base(8), G, C.activeTiles.get(),
-                                                C.tileScan.get(), reinterpret_cast<const double*>(C.partial.get()),
-                                                ntx, nty, ntz);
+                                                C.tileScan.get(), C.tileShift.get(), C.partial.get(), ntx, nty, ntz);
         launchCheck("reduce current");
       }
       F.currentZero = false;
@@ -1355,7 +1329,6 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
     }
     {
       ScopedKernel sk(F.profiler, st, "residual");
-      if (nChunks) kEsum<<<nActive, 256, 0, st>>>(G, C.activeTiles.get(), ntx, nty, ntz);
       if (nChunks)
         kGatherChunks<1><<<nChunks, 2 * kChunk, 0, st>>>(A, G, C.chunks.get(), ntx, nty, F.g.dt,
                                                          C.reduceBlocks.get(), C.error.get());

@@ -86,7 +86,8 @@ struct CouplingState {
   double exitLabZ = 0.0;
   bool hasExit = false;
   DeviceArray<double> estar[3];
-  // E_cand + E_old of the trial on the nodes of the active tiles
+  // E_cand + E_old of the current trial (the preview writes the zero-current
+  // sums on the whole grid; the current reduction rewrites the active tiles)
   DeviceArray<double> esum[3];
   // tiles: particle range, chunk count (+1 entry for the scan total), first
   // chunk, fixed-point exponent of the tile's current
