@@ -127,13 +127,16 @@ aprl_status aprl_particles_download(aprl_particles* particles, double* x3, doubl
 aprl_status aprl_coupling_create(aprl_fields* fields, aprl_shape shape, int smoothX, aprl_coupling** out) {
   return guarded([&] {
     if (!fields || !out) throw std::invalid_argument("Null field grid or output handle");
-    if (shape != APRL_SHAPE_LINEAR_BINOMIAL3)
-      throw std::invalid_argument("This build implements the linear-binomial3 source shape only");
-    if (!smoothX) throw std::invalid_argument("This build implements the production x-binomial3 smoothing only");
+    if (shape != APRL_SHAPE_LINEAR_BINOMIAL3 && shape != APRL_SHAPE_QUINTIC_COMPENSATED)
+      throw std::invalid_argument("This build implements the quintic-compensated and linear-binomial3 source shapes");
+    if (shape == APRL_SHAPE_LINEAR_BINOMIAL3 && !smoothX)
+      throw std::invalid_argument("This build implements linear-binomial3 with the production x-binomial3 smoothing only");
     std::unique_ptr<aprl_coupling> c(new aprl_coupling);
     c->c.fields = &fields->s;
     c->c.shape = shape;
-    c->c.smoothX = smoothX != 0;
+    // the x-binomial3 smoothing applies to the binomial3 shape only
+    // (longitudinal_shape.h:109)
+    c->c.smoothX = shape == APRL_SHAPE_LINEAR_BINOMIAL3 && smoothX != 0;
     if (fields->s.hasCpml)
       for (int a = 0; a < 3; ++a) c->c.cpmlCells[a] = fields->s.cpml.cells[a];
     *out = c.release();

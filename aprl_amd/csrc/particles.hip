@@ -1,7 +1,8 @@
 // gfx950 particle kernels of the conservative-source step (FP64 form):
 // tile sort, orbit gather + Lapenta-Markidis push with prescribed magnets,
-// deterministic tile deposition of the binomial3-x current, residual
-// gathers with fixed-order reductions, and the Picard orchestration.
+// deterministic tile deposition of the binomial3-x or compensated-quintic
+// current, residual gathers with fixed-order reductions, and the Picard
+// orchestration.
 //
 // Per-particle arithmetic follows source_coupling_math.h / source_coupling.cpp
 // / longitudinal_shape.h in the reference's association order (contraction
@@ -44,10 +45,6 @@ struct PArgs {
   double* x1[3];
   double* u1[3];
   double* gridAbs;
-  int* segCount;
-  double* segV;
-  int4* segCell;
-  double4* segFrac;
   double scaleDen;  // dx*dy*dz*dt
   int n;
 };
@@ -276,96 +273,163 @@ __device__ int orbitCuts(const double x[3], const double v[3], double cuts[kMaxC
   return count;
 }
 
-// Extended node box of a cell tile: every binomial3-x tap of a chord that
+// ---------------------------------------------------------------------------
+// Source shapes and tile boxes.
+
+enum : int { kShapeQuintic = 0, kShapeBinomial3X = 2 };
+
+// Extended node box of a cell tile per shape: every tap of a chord that
 // starts in the tile and moves less than one cell per axis lies inside.
+// (X0, Y0, Z0) is the box origin below the tile origin; RZ is how many
+// neighbour tiles the box reaches along z (one along x and y).
+template <int SHAPE>
+struct Box;
+template <>
+struct Box<kShapeBinomial3X> {
+  // binomialXZSegment taps: x [i-1, i+2], y [j, j+1], z [k-1, k+2]
+  static constexpr int X0 = 2, Y0 = 1, Z0 = 2;
+  static constexpr int NX = kTileX + 5, NY = kTileY + 3, NZ = kTileZ + 5;
+  static constexpr int RZ = 1;
+  // each chord's weights sum to at most 2 in absolute value per component
+  static constexpr double kWeightBound = 2.0;
+};
+template <>
+struct Box<kShapeQuintic> {
+  // quintic Gauss-point taps: x [i, i+1], y [j, j+1], z [k-5, k+6]
+  static constexpr int X0 = 1, Y0 = 1, Z0 = 6;
+  static constexpr int NX = kTileX + 3, NY = kTileY + 3, NZ = kTileZ + 13;
+  static constexpr int RZ = 2;
+  // the compensated quintic's coefficients sum to about 3.01 in absolute value
+  static constexpr double kWeightBound = 4.0;
+};
+template <int SHAPE>
+constexpr int boxNodes() {
+  return Box<SHAPE>::NX * Box<SHAPE>::NY * Box<SHAPE>::NZ;
+}
+
+template <int SHAPE>
 struct TileBox {
   int x0, y0, z0;
   __device__ int local(int i, int j, int k, unsigned& err) const {
     const unsigned li = static_cast<unsigned>(i - x0), lj = static_cast<unsigned>(j - y0),
                    lk = static_cast<unsigned>(k - z0);
-    if (li >= static_cast<unsigned>(kExtX) || lj >= static_cast<unsigned>(kExtY) || lk >= static_cast<unsigned>(kExtZ)) {
+    if (li >= static_cast<unsigned>(Box<SHAPE>::NX) || lj >= static_cast<unsigned>(Box<SHAPE>::NY) ||
+        lk >= static_cast<unsigned>(Box<SHAPE>::NZ)) {
       err |= kErrTileHalo;
       return -1;
     }
-    return static_cast<int>((lk * kExtY + lj) * kExtX + li);
+    return static_cast<int>((lk * Box<SHAPE>::NY + lj) * Box<SHAPE>::NX + li);
+  }
+  // box contains [i, i + wx) x [j, j + wy) x [k, k + wz)
+  __device__ bool holds(long long i, long long j, long long k, int wx, int wy, int wz) const {
+    return i >= x0 && i + wx <= x0 + Box<SHAPE>::NX && j >= y0 && j + wy <= y0 + Box<SHAPE>::NY && k >= z0 &&
+           k + wz <= z0 + Box<SHAPE>::NZ;
+  }
+  __device__ int base(long long i, long long j, long long k) const {
+    return (static_cast<int>(k - z0) * Box<SHAPE>::NY + static_cast<int>(j - y0)) * Box<SHAPE>::NX +
+           static_cast<int>(i - x0);
   }
 };
-__device__ __forceinline__ TileBox tileBox(int tile, int ntx, int nty) {
+template <int SHAPE>
+__device__ __forceinline__ TileBox<SHAPE> tileBox(int tile, int ntx, int nty) {
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
-  return TileBox{tx * kTileX - 2, ty * kTileY - 1, tz * kTileZ - 2};
+  return TileBox<SHAPE>{tx * kTileX - Box<SHAPE>::X0, ty * kTileY - Box<SHAPE>::Y0, tz * kTileZ - Box<SHAPE>::Z0};
 }
 
-// E_cand + E_old at an in-range node, and B at a clamped in-range node,
-// read from LDS copies of a tile box.
-struct StagedFields {
-  const double* e;
-  const double* b;
-  TileBox box;
-  __device__ double esum(int d, int i, int j, int k, unsigned& err) const {
-    const int l = box.local(i, j, k, err);
-    return l < 0 ? 0.0 : e[d * kExtNodes + l];
-  }
-  __device__ double bval(int d, int i, int j, int k, unsigned& err) const {
-    const int l = box.local(i, j, k, err);
-    return l < 0 ? 0.0 : b[d * kExtNodes + l];
-  }
-};
-
-// Copy a tile box of E_cand + E_old and of B into LDS (zero outside the
-// component ranges; such nodes are never read). All loads of a lane are
-// issued before the first LDS store.
-constexpr int kStageThreads = 2 * kChunk;
-constexpr int kStageIters = (3 * kExtNodes + kStageThreads - 1) / kStageThreads;
-__device__ void stageTile(const GArgs& G, const TileBox& box, double* es, double* bs) {
-  double ev[kStageIters], bv[kStageIters];
+// Copy a tile box of E_cand + E_old (and, when bs is given, of B) into LDS,
+// zero outside the component ranges (such nodes are never read). All loads
+// of a lane are issued before the first LDS store.
+template <int SHAPE, bool WITH_B>
+__device__ void stageTile(const GArgs& G, const TileBox<SHAPE>& box, double* es, double* bs) {
+  constexpr int N = boxNodes<SHAPE>();
+  constexpr int kIters = (3 * N + 2 * kChunk - 1) / (2 * kChunk);
+  double ev[kIters], bv[kIters];
 #pragma unroll
-  for (int r = 0; r < kStageIters; ++r) {
-    const int t = static_cast<int>(threadIdx.x) + r * kStageThreads;
+  for (int r = 0; r < kIters; ++r) {
+    const int t = static_cast<int>(threadIdx.x) + r * 2 * kChunk;
     ev[r] = 0.0;
     bv[r] = 0.0;
-    if (t < 3 * kExtNodes) {
-      const int d = t / kExtNodes, q0 = t - d * kExtNodes;
-      const int li = q0 % kExtX, q = q0 / kExtX, lj = q % kExtY, lk = q / kExtY;
+    if (t < 3 * N) {
+      const int d = t / N, q0 = t - d * N;
+      const int li = q0 % Box<SHAPE>::NX, q = q0 / Box<SHAPE>::NX, lj = q % Box<SHAPE>::NY, lk = q / Box<SHAPE>::NY;
       const int i = box.x0 + li, j = box.y0 + lj, k = box.z0 + lk;
       if (i >= 0 && j >= 0 && k >= 0) {
         if (i < G.ne[d][0] && j < G.ne[d][1] && k < G.ne[d][2])
           ev[r] = G.esum[d][(static_cast<size_t>(k) * G.ne[d][1] + j) * G.ne[d][0] + i];
-        if (i < G.nb[d][0] && j < G.nb[d][1] && k < G.nb[d][2])
+        if (WITH_B && i < G.nb[d][0] && j < G.nb[d][1] && k < G.nb[d][2])
           bv[r] = G.b[d][(static_cast<size_t>(k) * G.nb[d][1] + j) * G.nb[d][0] + i];
       }
     }
   }
 #pragma unroll
-  for (int r = 0; r < kStageIters; ++r) {
-    const int t = static_cast<int>(threadIdx.x) + r * kStageThreads;
-    if (t < 3 * kExtNodes) {
+  for (int r = 0; r < kIters; ++r) {
+    const int t = static_cast<int>(threadIdx.x) + r * 2 * kChunk;
+    if (t < 3 * N) {
       es[t] = ev[r];
-      bs[t] = bv[r];
+      if (WITH_B) bs[t] = bv[r];
     }
   }
 }
 
-// Impl::electric wide branch for binomial3-x (source_coupling.cpp:74-82):
-// orbit average of 1/2 (E_cand + E_old) with the exact adjoint weights of
-// binomialXZSegment (longitudinal_shape.h:61-94).
-__device__ void gatherE(const GArgs& G, const StagedFields& F, const double a[3], const double bpt[3], double res[3],
-                        unsigned& err) {
-  res[0] = res[1] = res[2] = 0.0;
-  const Clip path = clipOrbit(G, a, bpt, err);
-  if (!path.valid) { err |= kErrOrbitOutside; return; }
-  double x[3], v[3], cuts[kMaxCuts];
-  for (int d = 0; d < 3; ++d) {
-    x[d] = (path.b[d] - G.o[d]) / G.h[d];
-    v[d] = (path.e[d] - path.b[d]) / G.h[d];
+// Gauss-Legendre nodes and weights of longitudinal_shape::weights
+// (longitudinal_shape.h:105-106).
+__constant__ double kGaussNode[4] = {-.8611363115940525752, -.3399810435848562648, .3399810435848562648,
+                                     .8611363115940525752};
+__constant__ double kGaussWeight[4] = {.3478548451374538574, .6521451548625461426, .6521451548625461426,
+                                       .3478548451374538574};
+
+// longitudinal_shape::basis for the compensated quintic, longitudinal_shape.h:
+// 25-55: the degree-5 (nodes) or degree-4 (edges) B-spline at offset f,
+// convolved with the seven-point compensation.
+struct QBasis {
+  int first, count;
+  double v[12];
+};
+__device__ __forceinline__ void quinticBasis(double z, bool edge, QBasis& r) {
+#pragma unroll
+  for (int k = 0; k < 12; ++k) r.v[k] = 0.0;
+  const unsigned degree = edge ? 4u : 5u;
+  const double cell = floor(z), f = z - cell;
+  double n[6] = {1., 0., 0., 0., 0., 0.};
+#pragma unroll
+  for (unsigned j = 1; j <= 5; ++j) {
+    if (j > degree) break;
+    double saved = 0.;
+#pragma unroll
+    for (unsigned k = 0; k < j; ++k) {
+      const double t = n[k] / j;
+      n[k] = saved + (k + 1 - f) * t;
+      saved = (f + j - k - 1) * t;
+    }
+    n[j] = saved;
   }
-  const int count = orbitCuts(x, v, cuts, err);
-  const double edge[3] = {0.25, 0.5, 0.25};
-  const double slope[4] = {-0.25, -0.25, 0.25, 0.25};
-  const double sy[2] = {-1., 1.};
-  for (int s = 1; s < count; ++s) {
-    const double first = cuts[s - 1], last = cuts[s];
-    const double width = last - first;
-    if (width <= 0.) continue;
+  const double p = 5., a = p / 6., b = 3 * p / 40. + p * (p - 1) / 72.,
+               d = 5 * p / 112. + p * (p - 1) / 80. + p * (p - 1) * (p - 2) / 1296.;
+  const double c[7] = {-d / 64., b / 16. + 3 * d / 32., -a / 4. - b / 4. - 15 * d / 64.,
+                       1 + a / 2. + 3 * b / 8. + 5 * d / 16., -a / 4. - b / 4. - 15 * d / 64., b / 16. + 3 * d / 32.,
+                       -d / 64.};
+  r.first = static_cast<int>(cell) - 5;
+  r.count = static_cast<int>(degree) + 7;
+#pragma unroll
+  for (unsigned j = 0; j <= 5; ++j) {
+    if (j > degree) break;
+#pragma unroll
+    for (unsigned k = 0; k < 7; ++k) r.v[j + k] += n[j] * c[k];
+  }
+}
+
+// Visit every tap of one chord segment [first, last] (lattice start x,
+// velocity v) of longitudinal_shape::weights (longitudinal_shape.h:97-117)
+// in the reference's order: binomialXZSegment (61-94) for binomial3-x, four
+// Gauss points with the quintic basis otherwise. visit(d, i, j, k, w); fast
+// is called instead, with the segment's lowest tap corner, when the caller
+// reports (through interior) that every tap of the segment lies inside its
+// domain, so that taps are fixed offsets from one base.
+template <int SHAPE, class Interior, class Fast, class Visit>
+__device__ __forceinline__ void segmentTaps(const GArgs& G, const double x[3], const double v[3], double first,
+                                            double last, Interior interior, Fast fast, Visit visit) {
+  const double width = last - first;
+  if (SHAPE == kShapeBinomial3X) {
     const double mid = (first + last) * 0.5;
     const double px = x[0] + mid * v[0], py = x[1] + mid * v[1], pz = x[2] + mid * v[2];
     long long i = static_cast<long long>(floor(px)), j = static_cast<long long>(floor(py));
@@ -376,40 +440,30 @@ __device__ void gatherE(const GArgs& G, const StagedFields& F, const double a[3]
     nodeWeights(px - i, wx);
     nodeWeights(pz - k, wz);
     const double wy[2] = {1. - (py - j), py - j};
+    const double edge[3] = {0.25, 0.5, 0.25};
+    const double slope[4] = {-0.25, -0.25, 0.25, 0.25};
+    const double sy[2] = {-1., 1.};
     const double curvature = width * width / 12.;
     const double xy = curvature * v[0] * v[1];
     const double xz = curvature * v[0] * v[2];
     const double yz = curvature * v[1] * v[2];
-    // Interior segment: every tap of the three components is in range and
-    // inside the staged box, so the taps are fixed offsets from one base.
-    if (i >= 1 && i <= G.nx - 2 && j >= 0 && j <= G.ny - 1 && k >= 1 && k <= G.nz - 2 && i - 1 >= F.box.x0 &&
-        i + 2 < F.box.x0 + kExtX && j >= F.box.y0 && j + 1 < F.box.y0 + kExtY && k - 1 >= F.box.z0 &&
-        k + 2 < F.box.z0 + kExtZ) {
-      const int base = ((static_cast<int>(k) - 1 - F.box.z0) * kExtY + (static_cast<int>(j) - F.box.y0)) * kExtX +
-                       (static_cast<int>(i) - 1 - F.box.x0);
-      const double* e0 = F.e + base;
-      const double* e1 = F.e + kExtNodes + base;
-      const double* e2 = F.e + 2 * kExtNodes + base;
+    // binomialXZSegment visit order and weights, longitudinal_shape.h:81-93;
+    // offsets are relative to (i-1, j, k-1)
+    if (interior(i - 1, j, k - 1, 4, 2, 4)) {
       for (int c = 0; c < 4; ++c) {
         for (int b = 0; b < 2; ++b) {
           const double weight = width * (wy[b] * wz[c] + yz * sy[b] * slope[c]);
-          for (int aa = 0; aa < 3; ++aa) res[0] += .5 * (weight * edge[aa]) * e0[(c * kExtY + b) * kExtX + aa];
+          for (int aa = 0; aa < 3; ++aa) fast(0, aa, b, c, weight * edge[aa]);
         }
-        for (int aa = 0; aa < 4; ++aa)
-          res[1] += .5 * (width * (wx[aa] * wz[c] + xz * slope[aa] * slope[c])) * e1[c * kExtY * kExtX + aa];
+        for (int aa = 0; aa < 4; ++aa) fast(1, aa, 0, c, width * (wx[aa] * wz[c] + xz * slope[aa] * slope[c]));
       }
       for (int b = 0; b < 2; ++b)
         for (int aa = 0; aa < 4; ++aa) {
           const double weight = width * (wx[aa] * wy[b] + xy * slope[aa] * sy[b]);
-          for (int c = 0; c < 3; ++c) res[2] += .5 * (weight * edge[c]) * e2[(c * kExtY + b) * kExtX + aa];
+          for (int c = 0; c < 3; ++c) fast(2, aa, b, c, weight * edge[c]);
         }
-      continue;
+      return;
     }
-    auto visit = [&](int d, long long ii, long long jj, long long kk, double w) {
-      if (ii < 0 || jj < 0 || ii >= G.ne[d][0] || jj >= G.ne[d][1]) return;
-      if (kk < 0 || kk >= G.ne[d][2]) return;
-      res[d] += .5 * w * F.esum(d, static_cast<int>(ii), static_cast<int>(jj), static_cast<int>(kk), err);
-    };
     for (int c = 0; c < 4; ++c) {
       for (int b = 0; b < 2; ++b) {
         const double weight = width * (wy[b] * wz[c] + yz * sy[b] * slope[c]);
@@ -422,6 +476,93 @@ __device__ void gatherE(const GArgs& G, const StagedFields& F, const double a[3]
         const double weight = width * (wx[aa] * wy[b] + xy * slope[aa] * sy[b]);
         for (int c = 0; c < 3; ++c) visit(2, i - 1 + aa, j + b, k - 1 + c, weight * edge[c]);
       }
+  } else {
+    for (int t = 0; t < 4; ++t) {
+      const double node = kGaussNode[t], gw = kGaussWeight[t];
+      const double u = .5 * (last + first + width * node), factor = .5 * width * gw;
+      const double xx = x[0] + u * v[0], yy = x[1] + u * v[1], zz = x[2] + u * v[2];
+      long long i = static_cast<long long>(floor(xx)), j = static_cast<long long>(floor(yy));
+      if (i == G.nx) --i;
+      if (j == G.ny) --j;
+      const double fx = xx - i, fy = yy - j;
+      QBasis nodes, edge;
+      quinticBasis(zz, false, nodes);
+      quinticBasis(zz, true, edge);
+      // offsets are relative to (i, j, nodes.first); edge.first == nodes.first
+      if (interior(i, j, static_cast<long long>(nodes.first), 2, 2, 12)) {
+        for (int k = 0; k < 12; ++k) {
+          const double w = factor * nodes.v[k];
+          if (w == 0.) continue;
+          for (int b = 0; b < 2; ++b) fast(0, 0, b, k, w * (b ? fy : 1. - fy));
+          for (int a = 0; a < 2; ++a) fast(1, a, 0, k, w * (a ? fx : 1. - fx));
+        }
+        for (int k = 0; k < 11; ++k) {
+          const double w = factor * edge.v[k];
+          if (w == 0.) continue;
+          for (int b = 0; b < 2; ++b)
+            for (int a = 0; a < 2; ++a) fast(2, a, b, k, w * (a ? fx : 1. - fx) * (b ? fy : 1. - fy));
+        }
+        continue;
+      }
+      for (int k = 0; k < nodes.count; ++k) {
+        const double w = factor * nodes.v[k];
+        if (w == 0.) continue;
+        for (int b = 0; b < 2; ++b) visit(0, i, j + b, nodes.first + k, w * (b ? fy : 1. - fy));
+        for (int a = 0; a < 2; ++a) visit(1, i + a, j, nodes.first + k, w * (a ? fx : 1. - fx));
+      }
+      for (int k = 0; k < edge.count; ++k) {
+        const double w = factor * edge.v[k];
+        if (w == 0.) continue;
+        for (int b = 0; b < 2; ++b)
+          for (int a = 0; a < 2; ++a) visit(2, i + a, j + b, edge.first + k, w * (a ? fx : 1. - fx) * (b ? fy : 1. - fy));
+      }
+    }
+  }
+}
+
+// Impl::electric wide branch (source_coupling.cpp:74-82): orbit average of
+// 1/2 (E_cand + E_old) over the clipped chord with the exact adjoint weights
+// of the deposit, read from the staged box es.
+template <int SHAPE>
+__device__ void gatherE(const GArgs& G, const double* es, const TileBox<SHAPE>& box, const double a[3],
+                        const double bpt[3], double res[3], unsigned& err) {
+  constexpr int N = boxNodes<SHAPE>(), NX = Box<SHAPE>::NX, NY = Box<SHAPE>::NY;
+  res[0] = res[1] = res[2] = 0.0;
+  const Clip path = clipOrbit(G, a, bpt, err);
+  if (!path.valid) { err |= kErrOrbitOutside; return; }
+  double x[3], v[3], cuts[kMaxCuts];
+  for (int d = 0; d < 3; ++d) {
+    x[d] = (path.b[d] - G.o[d]) / G.h[d];
+    v[d] = (path.e[d] - path.b[d]) / G.h[d];
+  }
+  const int count = orbitCuts(x, v, cuts, err);
+  int base = 0;
+  // all taps in range of every component and inside the box
+  auto interior = [&](long long i, long long j, long long k, int wx, int wy, int wz) {
+    if (i < 0 || j < 0 || k < 0 || i + wx > G.nx + 1 || j + wy > G.ny + 1 || k + wz > G.nz + 1) return false;
+    if (SHAPE == kShapeBinomial3X) {
+      // Ex spans x [i, i+3) in nx, Ey y [j, j+1) in ny, Ez z [k, k+3) in nz
+      if (i + 3 > G.nx || j + 1 > G.ny || k + 3 > G.nz) return false;
+    } else {
+      // Ex spans x [i, i+1) in nx, Ey y [j, j+1) in ny, Ez z [k, k+11) in nz
+      if (i + 1 > G.nx || j + 1 > G.ny || k + 11 > G.nz) return false;
+    }
+    if (!box.holds(i, j, k, wx, wy, wz)) return false;
+    base = box.base(i, j, k);
+    return true;
+  };
+  auto fast = [&](int d, int oi, int oj, int ok, double w) {
+    res[d] += .5 * w * es[d * N + base + (ok * NY + oj) * NX + oi];
+  };
+  auto visit = [&](int d, long long ii, long long jj, long long kk, double w) {
+    if (ii < 0 || jj < 0 || ii >= G.ne[d][0] || jj >= G.ne[d][1]) return;
+    if (kk < 0 || kk >= G.ne[d][2]) return;
+    const int l = box.local(static_cast<int>(ii), static_cast<int>(jj), static_cast<int>(kk), err);
+    res[d] += .5 * w * (l < 0 ? 0.0 : es[d * N + l]);
+  };
+  for (int s = 1; s < count; ++s) {
+    if (cuts[s] - cuts[s - 1] <= 0.) continue;
+    segmentTaps<SHAPE>(G, x, v, cuts[s - 1], cuts[s], interior, fast, visit);
   }
   const double f = path.last - path.first;
   res[0] *= f;
@@ -435,9 +576,11 @@ __device__ __forceinline__ int clampIndex(long long v, int n) {
 
 // Impl::magnetic for binomial3-x: binomialXZMagneticSample at the clipped
 // chord midpoint with clamped ("open") indices (source_coupling.cpp:83-90;
-// source_coupling_math.h:142-167; NativeHalo::openMagneticValue).
-__device__ void gatherB(const GArgs& G, const StagedFields& F, const double a[3], const double bpt[3], double res[3],
-                        unsigned& err) {
+// source_coupling_math.h:142-167; NativeHalo::openMagneticValue), read from
+// the staged box bs.
+__device__ void gatherBBinomial(const GArgs& G, const double* bs, const TileBox<kShapeBinomial3X>& box,
+                                const double a[3], const double bpt[3], double res[3], unsigned& err) {
+  constexpr int N = boxNodes<kShapeBinomial3X>(), NX = Box<kShapeBinomial3X>::NX, NY = Box<kShapeBinomial3X>::NY;
   res[0] = res[1] = res[2] = 0.0;
   const Clip path = clipOrbit(G, a, bpt, err);
   if (!path.valid) { err |= kErrMagneticOutside; return; }
@@ -457,17 +600,13 @@ __device__ void gatherB(const GArgs& G, const StagedFields& F, const double a[3]
     const double wy[2] = {1. - frac[1], frac[1]};
     const int* n = G.nb[comp];
     if (cell[0] >= 1 && cell[0] + 2 <= n[0] - 1 && cell[1] >= 0 && cell[1] + 1 <= n[1] - 1 && cell[2] >= 1 &&
-        cell[2] + 2 <= n[2] - 1 && cell[0] - 1 >= F.box.x0 && cell[0] + 2 < F.box.x0 + kExtX && cell[1] >= F.box.y0 &&
-        cell[1] + 1 < F.box.y0 + kExtY && cell[2] - 1 >= F.box.z0 && cell[2] + 2 < F.box.z0 + kExtZ) {
+        cell[2] + 2 <= n[2] - 1 && box.holds(cell[0] - 1, cell[1], cell[2] - 1, 4, 2, 4)) {
       // interior: no clamping, fixed offsets from one base
-      const double* bc = F.b + comp * kExtNodes +
-                         ((static_cast<int>(cell[2]) - 1 - F.box.z0) * kExtY + (static_cast<int>(cell[1]) - F.box.y0)) *
-                             kExtX +
-                         (static_cast<int>(cell[0]) - 1 - F.box.x0);
+      const double* bc = bs + comp * N + box.base(cell[0] - 1, cell[1], cell[2] - 1);
       for (int k = 0; k < 4; ++k)
         for (int j = 0; j < 2; ++j) {
           double row = 0.;
-          for (int i = 0; i < 4; ++i) row += wx[i] * bc[(k * kExtY + j) * kExtX + i];
+          for (int i = 0; i < 4; ++i) row += wx[i] * bc[(k * NY + j) * NX + i];
           res[comp] += wz[k] * wy[j] * row;
         }
       continue;
@@ -476,9 +615,46 @@ __device__ void gatherB(const GArgs& G, const StagedFields& F, const double a[3]
       for (int j = 0; j < 2; ++j) {
         const int kk = clampIndex(cell[2] - 1 + k, n[2]), jj = clampIndex(cell[1] + j, n[1]);
         double row = 0.;
-        for (int i = 0; i < 4; ++i) row += wx[i] * F.bval(comp, clampIndex(cell[0] - 1 + i, n[0]), jj, kk, err);
+        for (int i = 0; i < 4; ++i) {
+          const int l = box.local(clampIndex(cell[0] - 1 + i, n[0]), jj, kk, err);
+          row += wx[i] * (l < 0 ? 0.0 : bs[comp * N + l]);
+        }
         res[comp] += wz[k] * wy[j] * row;
       }
+  }
+}
+
+// Impl::magnetic for the quintic shape: magneticSample at the clipped chord
+// midpoint with clamped indices (source_coupling.cpp:83-86;
+// source_coupling_math.h:122-138), read from HBM.
+__device__ void gatherBQuintic(const GArgs& G, const double a[3], const double bpt[3], double res[3], unsigned& err) {
+  res[0] = res[1] = res[2] = 0.0;
+  const Clip path = clipOrbit(G, a, bpt, err);
+  if (!path.valid) { err |= kErrMagneticNoDomain; return; }
+  double mid[3];
+  for (int d = 0; d < 3; ++d) { mid[d] = path.b[d]; mid[d] += path.e[d]; mid[d] *= .5; }
+  int cell[2][3];
+  double fraction[2][3];
+  for (int half = 0; half < 2; ++half)
+    for (int d = 0; d < 3; ++d) {
+      const double x = (mid[d] - G.o[d]) / G.h[d] - .5 * half;
+      cell[half][d] = static_cast<int>(floor(x));
+      fraction[half][d] = x - cell[half][d];
+    }
+  for (int axis = 0; axis < 3; ++axis) {
+    const int hx = axis == 0 ? 0 : 1, hy = axis == 1 ? 0 : 1, hz = axis == 2 ? 0 : 1;
+    const int* n = G.nb[axis];
+    const double* src = G.b[axis];
+    for (int k = 0; k < 2; ++k)
+      for (int j = 0; j < 2; ++j)
+        for (int i = 0; i < 2; ++i) {
+          const double xw = i ? fraction[hx][0] : 1 - fraction[hx][0];
+          const double yw = j ? fraction[hy][1] : 1 - fraction[hy][1];
+          const double zw = k ? fraction[hz][2] : 1 - fraction[hz][2];
+          const int ii = clampIndex(cell[hx][0] + i, n[0]), jj = clampIndex(cell[hy][1] + j, n[1]),
+                    kk = clampIndex(cell[hz][2] + k, n[2]);
+          res[axis] += xw * yw * zw * src[(static_cast<size_t>(kk) * n[1] + jj) * n[0] + ii];
+        }
   }
 }
 
@@ -536,10 +712,11 @@ __global__ void kTileRanges(const unsigned* key, int n, int ntiles, int* begin, 
   if (e > b) atomicAdd(occupied, 1u);
 }
 
-// Per tile: chunk count (low word) and whether any of its 27 neighbour
-// tiles is occupied (high word); entry ntiles is 0 so that the exclusive
-// scan ends with both totals.
-__global__ void kTileCounts(const int* begin, const int* end, int ntx, int nty, int ntz,
+// Per tile: chunk count (low word) and whether any tile within one tile in
+// x and y and rz tiles in z is occupied (high word: the tile's J nodes can
+// then receive current); entry ntiles is 0 so that the exclusive scan ends
+// with both totals.
+__global__ void kTileCounts(const int* begin, const int* end, int ntx, int nty, int ntz, int rz,
                             unsigned long long* counts) {
   const int ntiles = ntx * nty * ntz;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -550,7 +727,7 @@ __global__ void kTileCounts(const int* begin, const int* end, int ntx, int nty, 
   }
   const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
   unsigned touched = 0;
-  for (int oz = -1; oz <= 1; ++oz)
+  for (int oz = -rz; oz <= rz; ++oz)
     for (int oy = -1; oy <= 1; ++oy)
       for (int ox = -1; ox <= 1; ++ox) {
         const int x = tx + ox, y = ty + oy, z = tz + oz;
@@ -594,16 +771,16 @@ __global__ void kFillInt(int* p, int n, int v) {
 // the first push uses. MODE 1: the residual gather of localResidual
 // (source_coupling.cpp:148-202) at (x0, x1): next fields, and the chunk's
 // force max and defect / scale / call sums in a fixed tree order.
-template <int MODE>
+template <int MODE, int SHAPE>
 __global__ void __launch_bounds__(2 * kChunk) kGatherChunks(PArgs P, GArgs G, const int4* chunks, int ntx, int nty,
                                                             double dt, double* chunkOut, unsigned* errOut) {
-  __shared__ double es[3 * kExtNodes];
-  __shared__ double bs[3 * kExtNodes];
+  constexpr bool kStageB = SHAPE == kShapeBinomial3X;
+  __shared__ double es[3 * boxNodes<SHAPE>()];
+  __shared__ double bs[kStageB ? 3 * boxNodes<SHAPE>() : 4 * kChunk];
   const int4 ch = chunks[blockIdx.x];
-  const TileBox box = tileBox(ch.x, ntx, nty);
-  stageTile(G, box, es, bs);
+  const TileBox<SHAPE> box = tileBox<SHAPE>(ch.x, ntx, nty);
+  stageTile<SHAPE, kStageB>(G, box, es, bs);
   __syncthreads();
-  const StagedFields F{es, bs, box};
   const bool bWaves = threadIdx.x >= kChunk;
   const int lane = static_cast<int>(threadIdx.x) - (bWaves ? kChunk : 0);
   const int p = ch.y + lane;
@@ -623,8 +800,12 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks(PArgs P, GArgs G, co
     } else {
       for (int d = 0; d < 3; ++d) b[d] = P.x1[d][p];
     }
-    if (bWaves) gatherB(G, F, a, b, g, err);
-    else gatherE(G, F, a, b, g, err);
+    if (bWaves) {
+      if constexpr (SHAPE == kShapeBinomial3X) gatherBBinomial(G, bs, box, a, b, g, err);
+      else gatherBQuintic(G, a, b, g, err);
+    } else {
+      gatherE<SHAPE>(G, es, box, a, b, g, err);
+    }
     double* const* dst = MODE == 0 ? P.fin : P.fout;
     const int off = bWaves ? 3 : 0;
     for (int d = 0; d < 3; ++d) dst[off + d][p] = g[d];
@@ -687,9 +868,10 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks(PArgs P, GArgs G, co
   if (err) atomicOr(errOut, err);
 }
 
+
 // Push of source_coupling.cpp:133-147 for one trial with the gathered
 // fields: the substeps with prescribed magnets, then the particle-boundary
-// and domain checks of the deposition loop, then the chord table.
+// and domain checks of the deposition loop.
 __global__ void __launch_bounds__(128) kPush(PArgs P, GArgs G, StepArgs S, unsigned* errOut) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P.n) return;
@@ -747,63 +929,40 @@ __global__ void __launch_bounds__(128) kPush(PArgs P, GArgs G, StepArgs S, unsig
     P.u1[a][p] = u[a];
   }
   P.gridAbs[p] = absGrid;
-  // Chord table of Current::deposit (longitudinal_shape.h:175-187 ->
-  // weights 97-117 -> binomialXZSegment 61-94) for the tile deposition.
-  {
-    double lx[3], v[3], cuts[kMaxCuts];
-    for (int d = 0; d < 3; ++d) {
-      P.segV[6 * static_cast<size_t>(p) + 3 + d] = q * (x[d] - x0[d]) / P.scaleDen;
-      lx[d] = (x0[d] - G.o[d]) / G.h[d];
-      v[d] = (x[d] - x0[d]) / G.h[d];
-      P.segV[6 * static_cast<size_t>(p) + d] = v[d];
-    }
-    const int count = orbitCuts(lx, v, cuts, err);
-    int nseg = 0;
-    for (int s = 1; s < count; ++s) {
-      const double first = cuts[s - 1], last = cuts[s];
-      const double width = last - first;
-      if (width <= 0.) continue;
-      if (nseg == kMaxSegments) { err |= kErrTileHalo; break; }
-      const double mid = (first + last) * 0.5;
-      const double px = lx[0] + mid * v[0], py = lx[1] + mid * v[1], pz = lx[2] + mid * v[2];
-      long long i = static_cast<long long>(floor(px)), j = static_cast<long long>(floor(py));
-      const long long k = static_cast<long long>(floor(pz));
-      if (i == G.nx) --i;
-      if (j == G.ny) --j;
-      const size_t slot = static_cast<size_t>(p) * kMaxSegments + nseg;
-      P.segCell[slot] = make_int4(static_cast<int>(i), static_cast<int>(j), static_cast<int>(k), 0);
-      P.segFrac[slot] = make_double4(px - i, py - j, pz - k, width);
-      ++nseg;
-    }
-    P.segCount[p] = nseg;
-  }
   if (err) atomicOr(errOut, err);
 }
 
 // Deterministic tile deposition of Current::deposit (longitudinal_shape.h:
-// 175-187) with binomial3-x weights, from the chord table of the push. One
-// workgroup per chunk, one lane per particle, accumulating into an LDS copy
-// of the tile box in 64-bit fixed point. The binary exponent comes from a
-// bound on the whole tile's current (2 * sum over the tile's particles of
-// max |scale|; each chord's weights sum to at most 2 in absolute value per
-// component), so every chunk of a tile uses the same scale and the chunk
-// partials of a tile add exactly. Integer addition is associative, so the
-// LDS atomics and the later chunk sums are reproducible in any order; each
-// tap is rounded once, to 2^-62 of the tile bound.
+// 175-187): one workgroup per chunk, one lane per particle, every tap of
+// the chord x0 -> x1 accumulated into an LDS copy of the tile box in 64-bit
+// fixed point. The binary exponent comes from a bound on the whole tile's
+// current (the shape's weight bound times the sum over the tile's particles
+// of max |scale|), so every chunk of a tile uses the same scale. Integer
+// addition is associative, so the LDS atomics and the chunk sums are
+// reproducible in any order; each tap is rounded once, to 2^-62 of the
+// tile bound.
+template <int SHAPE>
 __global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, const int4* chunks, const int* tileBegin,
                                                    const int* tileEnd, const unsigned long long* tileScan, int ntx,
                                                    int nty, long long* partial, int* tileShift, unsigned* errOut) {
-  __shared__ unsigned long long acc[3 * kExtNodes];
+  constexpr int N = boxNodes<SHAPE>(), NX = Box<SHAPE>::NX, NY = Box<SHAPE>::NY;
+  __shared__ unsigned long long acc[3 * N];
   __shared__ double bound[kChunk];
   const int4 ch = chunks[blockIdx.x];
   const int tile = ch.x;
-  const TileBox box = tileBox(tile, ntx, nty);
+  const TileBox<SHAPE> box = tileBox<SHAPE>(tile, ntx, nty);
   const int tb0 = tileBegin[tile], tb1 = tileEnd[tile];
-  for (int t = threadIdx.x; t < 3 * kExtNodes; t += blockDim.x) acc[t] = 0ull;
+  for (int t = threadIdx.x; t < 3 * N; t += blockDim.x) acc[t] = 0ull;
+  // scale_a = q * (x1_a - x0_a) / (dx dy dz dt), eb_deposition.cpp:41-51
+  auto scaleOf = [&](int p, double s[3]) {
+    const double q = P.q[p];
+    for (int d = 0; d < 3; ++d) s[d] = q * (P.x1[d][p] - P.x[d][p]) / P.scaleDen;
+  };
   double bsum = 0.0;
   for (int p = tb0 + static_cast<int>(threadIdx.x); p < tb1; p += blockDim.x) {
-    const double* sv = P.segV + 6 * static_cast<size_t>(p);
-    bsum += fmax(fabs(sv[3]), fmax(fabs(sv[4]), fabs(sv[5])));
+    double s[3];
+    scaleOf(p, s);
+    bsum += fmax(fabs(s[0]), fmax(fabs(s[1]), fabs(s[2])));
   }
   bound[threadIdx.x] = bsum;
   __syncthreads();
@@ -812,88 +971,52 @@ __global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, const int4*
     __syncthreads();
   }
   int e2 = 0;
-  (void)frexp(2.0 * bound[0], &e2);
-  const int shift = 62 - e2;  // 2*bound * 2^shift < 2^62
+  (void)frexp(Box<SHAPE>::kWeightBound * bound[0], &e2);
+  const int shift = 62 - e2;  // bound * 2^shift < 2^62
   if (threadIdx.x == 0 && static_cast<int>(blockIdx.x) == lowWord(tileScan[tile])) tileShift[tile] = shift;
   unsigned err = 0;
-  const double edge[3] = {0.25, 0.5, 0.25};
-  const double slope[4] = {-0.25, -0.25, 0.25, 0.25};
-  const double sy[2] = {-1., 1.};
-  auto add = [&](int comp, int ii, int jj, int kk, double value) {
-    if (ii < 0 || ii >= G.ne[comp][0]) {
-      if (!G.smoothX) err |= kErrCurrentX;
-      return;
-    }
-    if (jj < 0 || jj >= G.ne[comp][1]) { err |= kErrCurrentY; return; }
-    if (kk < 0 || kk >= G.ne[comp][2]) return;
-    const int l = box.local(ii, jj, kk, err);
-    if (l < 0) return;
-    const long long fixed = llrint(ldexp(value, shift));
-    atomicAdd(&acc[comp * kExtNodes + l], static_cast<unsigned long long>(fixed));
-  };
   const int p = ch.y + static_cast<int>(threadIdx.x);
   if (p < ch.z) {
-    const int nseg = P.segCount[p];
-    const double* sv = P.segV + 6 * static_cast<size_t>(p);
-    const double v0 = sv[0], v1 = sv[1], v2 = sv[2];
-    const double scale[3] = {sv[3], sv[4], sv[5]};
-    for (int s = 0; s < nseg; ++s) {
-      const int4 cell = P.segCell[static_cast<size_t>(p) * kMaxSegments + s];
-      const double4 fr = P.segFrac[static_cast<size_t>(p) * kMaxSegments + s];
-      const int i = cell.x, j = cell.y, k = cell.z;
-      const double width = fr.w;
-      double wx[4], wz[4];
-      nodeWeights(fr.x, wx);
-      nodeWeights(fr.z, wz);
-      const double wy[2] = {1. - fr.y, fr.y};
-      const double curvature = width * width / 12.;
-      const double xy = curvature * v0 * v1, xz = curvature * v0 * v2, yz = curvature * v1 * v2;
-      if (i >= 1 && i <= G.nx - 2 && j >= 0 && j <= G.ny - 1 && k >= 1 && k <= G.nz - 2 && i - 1 >= box.x0 &&
-          i + 2 < box.x0 + kExtX && j >= box.y0 && j + 1 < box.y0 + kExtY && k - 1 >= box.z0 &&
-          k + 2 < box.z0 + kExtZ) {
-        // interior segment: all taps in range and in the box
-        const int base = ((k - 1 - box.z0) * kExtY + (j - box.y0)) * kExtX + (i - 1 - box.x0);
-        unsigned long long* a0 = acc + base;
-        unsigned long long* a1 = acc + kExtNodes + base;
-        unsigned long long* a2 = acc + 2 * kExtNodes + base;
-        auto put = [&](unsigned long long* at, double value) {
-          atomicAdd(at, static_cast<unsigned long long>(llrint(ldexp(value, shift))));
-        };
-        for (int c = 0; c < 4; ++c) {
-          if (scale[0] != 0.)
-            for (int b = 0; b < 2; ++b) {
-              const double weight = width * (wy[b] * wz[c] + yz * sy[b] * slope[c]);
-              for (int aa = 0; aa < 3; ++aa) put(a0 + (c * kExtY + b) * kExtX + aa, scale[0] * (weight * edge[aa]));
-            }
-          if (scale[1] != 0.)
-            for (int aa = 0; aa < 4; ++aa)
-              put(a1 + c * kExtY * kExtX + aa, scale[1] * (width * (wx[aa] * wz[c] + xz * slope[aa] * slope[c])));
-        }
-        if (scale[2] != 0.)
-          for (int b = 0; b < 2; ++b)
-            for (int aa = 0; aa < 4; ++aa) {
-              const double weight = width * (wx[aa] * wy[b] + xy * slope[aa] * sy[b]);
-              for (int c = 0; c < 3; ++c) put(a2 + (c * kExtY + b) * kExtX + aa, scale[2] * (weight * edge[c]));
-            }
-        continue;
+    double scale[3];
+    scaleOf(p, scale);
+    double x[3], v[3], cuts[kMaxCuts];
+    for (int d = 0; d < 3; ++d) {
+      x[d] = (P.x[d][p] - G.o[d]) / G.h[d];
+      v[d] = (P.x1[d][p] - P.x[d][p]) / G.h[d];
+    }
+    const int count = orbitCuts(x, v, cuts, err);
+    int base = 0;
+    auto interior = [&](long long i, long long j, long long k, int wx, int wy, int wz) {
+      if (i < 0 || j < 0 || k < 0 || i + wx > G.nx + 1 || j + wy > G.ny + 1 || k + wz > G.nz + 1) return false;
+      if (SHAPE == kShapeBinomial3X) {
+        if (i + 3 > G.nx || j + 1 > G.ny || k + 3 > G.nz) return false;
+      } else {
+        if (i + 1 > G.nx || j + 1 > G.ny || k + 11 > G.nz) return false;
       }
-      // binomialXZSegment visit order and weights, longitudinal_shape.h:81-93
-      for (int c = 0; c < 4; ++c) {
-        if (scale[0] != 0.)
-          for (int b = 0; b < 2; ++b) {
-            const double weight = width * (wy[b] * wz[c] + yz * sy[b] * slope[c]);
-            for (int aa = 0; aa < 3; ++aa) add(0, i - 1 + aa, j + b, k - 1 + c, scale[0] * (weight * edge[aa]));
-          }
-        if (scale[1] != 0.)
-          for (int aa = 0; aa < 4; ++aa)
-            add(1, i - 1 + aa, j, k - 1 + c, scale[1] * (width * (wx[aa] * wz[c] + xz * slope[aa] * slope[c])));
+      if (!box.holds(i, j, k, wx, wy, wz)) return false;
+      base = box.base(i, j, k);
+      return true;
+    };
+    auto fast = [&](int d, int oi, int oj, int ok, double w) {
+      if (scale[d] == 0.) return;
+      atomicAdd(&acc[d * N + base + (ok * NY + oj) * NX + oi],
+                static_cast<unsigned long long>(llrint(ldexp(scale[d] * w, shift))));
+    };
+    auto visit = [&](int d, long long ii, long long jj, long long kk, double w) {
+      if (scale[d] == 0.) return;
+      if (ii < 0 || ii >= G.ne[d][0]) {
+        if (SHAPE != kShapeBinomial3X || !G.smoothX) err |= kErrCurrentX;
+        return;
       }
-      if (scale[2] != 0.)
-        for (int b = 0; b < 2; ++b)
-          for (int aa = 0; aa < 4; ++aa) {
-            const double weight = width * (wx[aa] * wy[b] + xy * slope[aa] * sy[b]);
-            for (int c = 0; c < 3; ++c) add(2, i - 1 + aa, j + b, k - 1 + c, scale[2] * (weight * edge[c]));
-          }
+      if (jj < 0 || jj >= G.ne[d][1]) { err |= kErrCurrentY; return; }
+      if (kk < 0 || kk >= G.ne[d][2]) return;
+      const int l = box.local(static_cast<int>(ii), static_cast<int>(jj), static_cast<int>(kk), err);
+      if (l < 0) return;
+      atomicAdd(&acc[d * N + l], static_cast<unsigned long long>(llrint(ldexp(scale[d] * w, shift))));
+    };
+    for (int s = 1; s < count; ++s) {
+      if (cuts[s] - cuts[s - 1] <= 0.) continue;
+      segmentTaps<SHAPE>(G, x, v, cuts[s - 1], cuts[s], interior, fast, visit);
     }
   }
   __syncthreads();
@@ -903,31 +1026,35 @@ __global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, const int4*
   // atomics, whose sum is independent of their order.
   const int first = lowWord(tileScan[tile]);
   const bool alone = lowWord(tileScan[tile + 1]) - first == 1;
-  unsigned long long* out =
-      reinterpret_cast<unsigned long long*>(partial + static_cast<size_t>(first) * 3 * kExtNodes);
-  for (int t = threadIdx.x; t < 3 * kExtNodes; t += blockDim.x) {
-    const unsigned long long v = acc[t];
-    if (alone) out[t] = v;
-    else if (v) atomicAdd(out + t, v);
+  unsigned long long* out = reinterpret_cast<unsigned long long*>(partial + static_cast<size_t>(first) * 3 * N);
+  for (int t = threadIdx.x; t < 3 * N; t += blockDim.x) {
+    const unsigned long long val = acc[t];
+    if (alone) out[t] = val;
+    else if (val) atomicAdd(out + t, val);
   }
   if (err) atomicOr(errOut, err);
 }
 
 // Node-owner reduction into J, one workgroup per active cell tile (a tile
-// with an occupied neighbour): each owned node sums, over the covering
-// occupied tiles in fixed (tz, ty, tx) order, the tile's exact integer
-// total converted with the tile's exponent, and clears what it read for
-// the next deposit. Nodes of other tiles keep the zeros J held at the
+// with an occupied tile within its box reach): each owned node sums, over
+// the covering occupied tiles in fixed (tz, ty, tx) order, the tile's exact
+// integer total converted with the tile's exponent, and clears what it read
+// for the next deposit. It also writes E_cand + E_old of the trial
+// (applyCurrentSource(-1) on the preview, source_coupling.cpp:84-89) for the
+// next residual gather. Nodes of other tiles keep the zeros J held at the
 // start of the step.
+template <int SHAPE>
 __global__ void __launch_bounds__(256) kReduceCurrent(double* jx, double* jy, double* jz, GArgs G, const int* active,
                                                       const unsigned long long* tileScan, const int* tileShift,
                                                       long long* partial, int ntx, int nty, int ntz) {
+  constexpr int N = boxNodes<SHAPE>(), NX = Box<SHAPE>::NX, NY = Box<SHAPE>::NY, NZ = Box<SHAPE>::NZ;
+  constexpr int RZ = Box<SHAPE>::RZ, NB = 9 * (2 * RZ + 1);
   const int tile = active[blockIdx.x];
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
-  __shared__ int nbFirst[27], nbShift[27];
-  if (threadIdx.x < 27) {
-    const int x = tx + static_cast<int>(threadIdx.x) % 3 - 1, y = ty + (static_cast<int>(threadIdx.x) / 3) % 3 - 1,
-              z = tz + static_cast<int>(threadIdx.x) / 9 - 1;
+  __shared__ int nbFirst[NB], nbShift[NB];
+  if (threadIdx.x < NB) {
+    const int nb = static_cast<int>(threadIdx.x);
+    const int x = tx + nb % 3 - 1, y = ty + (nb / 3) % 3 - 1, z = tz + nb / 9 - RZ;
     int first = -1, sh = 0;
     if (x >= 0 && x < ntx && y >= 0 && y < nty && z >= 0 && z < ntz) {
       const int s = (z * nty + y) * ntx + x;
@@ -937,11 +1064,11 @@ __global__ void __launch_bounds__(256) kReduceCurrent(double* jx, double* jy, do
         sh = tileShift[s];
       }
     }
-    nbFirst[threadIdx.x] = first;
-    nbShift[threadIdx.x] = sh;
+    nbFirst[nb] = first;
+    nbShift[nb] = sh;
   }
   __syncthreads();
-  constexpr size_t chunkStride = 3 * static_cast<size_t>(kExtNodes);
+  constexpr size_t chunkStride = 3 * static_cast<size_t>(N);
   for (int d = 0; d < 3; ++d) {
     const int n0 = G.ne[d][0], n1 = G.ne[d][1], n2 = G.ne[d][2];
     // owned node box (the last tile of an axis also owns the extra node)
@@ -952,30 +1079,27 @@ __global__ void __launch_bounds__(256) kReduceCurrent(double* jx, double* jy, do
     double* dst = d == 0 ? jx : (d == 1 ? jy : jz);
     for (int t = threadIdx.x; t < wx * wy * wz; t += blockDim.x) {
       const int i = i0 + t % wx, j = j0 + (t / wx) % wy, k = k0 + t / (wx * wy);
-      long long v[27];
-      long long* at[27];
+      long long v[NB];
+      long long* at[NB];
 #pragma unroll
-      for (int nb = 0; nb < 27; ++nb) {
-        const int ox = nb % 3 - 1, oy = (nb / 3) % 3 - 1, oz = nb / 9 - 1;
-        const int li = i - ((tx + ox) * kTileX - 2), lj = j - ((ty + oy) * kTileY - 1),
-                  lk = k - ((tz + oz) * kTileZ - 2);
+      for (int nb = 0; nb < NB; ++nb) {
+        const int ox = nb % 3 - 1, oy = (nb / 3) % 3 - 1, oz = nb / 9 - RZ;
+        const int li = i - ((tx + ox) * kTileX - Box<SHAPE>::X0), lj = j - ((ty + oy) * kTileY - Box<SHAPE>::Y0),
+                  lk = k - ((tz + oz) * kTileZ - Box<SHAPE>::Z0);
         const int f = nbFirst[nb];
-        const bool on = f >= 0 && li >= 0 && li < kExtX && lj >= 0 && lj < kExtY && lk >= 0 && lk < kExtZ;
-        at[nb] = on ? partial + static_cast<size_t>(f) * chunkStride + d * kExtNodes + (lk * kExtY + lj) * kExtX + li
-                    : nullptr;
+        const bool on = f >= 0 && li >= 0 && li < NX && lj >= 0 && lj < NY && lk >= 0 && lk < NZ;
+        at[nb] = on ? partial + static_cast<size_t>(f) * chunkStride + d * N + (lk * NY + lj) * NX + li : nullptr;
         v[nb] = on ? *at[nb] : 0;
       }
       double sum = 0.0;
 #pragma unroll
-      for (int nb = 0; nb < 27; ++nb)
+      for (int nb = 0; nb < NB; ++nb)
         if (at[nb]) {
           sum += ldexp(static_cast<double>(v[nb]), -nbShift[nb]);
           *at[nb] = 0;
         }
       const size_t idx = (static_cast<size_t>(k) * n1 + j) * n0 + i;
       dst[idx] = sum;
-      // E_cand + E_old of this trial (applyCurrentSource(-1) on the preview,
-      // source_coupling.cpp:84-89) for the next residual gather
       const bool applied = d == 0 ? (j > 0 && j < G.ny && k != 0 && k != G.nz)
                                   : (d == 1 ? (i > 0 && i < G.nx && k != 0 && k != G.nz)
                                             : (i > 0 && i < G.nx && j > 0 && j < G.ny));
@@ -1050,10 +1174,6 @@ static PArgs particleArgs(ParticleStore& P) {
   A.q = P.s[6].get();
   A.m = P.s[7].get();
   A.gridAbs = P.gridAbs.get();
-  A.segCount = P.segCount.get();
-  A.segV = P.segV.get();
-  A.segCell = P.segCell.get();
-  A.segFrac = P.segFrac.get();
   A.n = static_cast<int>(P.count);
   return A;
 }
@@ -1069,10 +1189,6 @@ void reserveParticles(ParticleStore& P, std::size_t capacity) {
     P.end[a].resize(capacity);
   }
   P.gridAbs.resize(capacity);
-  P.segCount.resize(capacity);
-  P.segV.resize(6 * capacity);
-  P.segCell.resize(kMaxSegments * capacity);
-  P.segFrac.resize(kMaxSegments * capacity);
   P.key.resize(capacity);
   P.keySorted.resize(capacity);
   P.perm.resize(capacity);
@@ -1099,6 +1215,7 @@ static void raiseErrors(unsigned e) {
   if (e & kErrEndpointOutside) throw std::out_of_range("Particle trajectory endpoint is outside the E/B domain");
   if (e & kErrOrbitOutside) throw std::out_of_range("Quintic active orbit outside domain");
   if (e & kErrMagneticOutside) throw std::out_of_range("Notch magnetic orbit outside domain");
+  if (e & kErrMagneticNoDomain) throw std::out_of_range("Active coupling magnetic orbit has no in-domain portion");
   if (e & kErrRootNonphysical) throw std::runtime_error("Nonphysical local mean-gamma root");
   if (e & kErrRootNoConverge) throw std::runtime_error("Local mean-gamma root did not converge");
   if (e & kErrCurrentY) throw std::out_of_range("Transverse current support outside y grid");
@@ -1184,7 +1301,8 @@ static void sortAndChunk(CouplingState& C, ParticleStore& P, const GArgs& G, int
                                                      C.tileEnd.get(), occupied);
   launchCheck("tile ranges");
   const int ntz = ntiles / (ntx * nty);
-  kTileCounts<<<ceilDiv(ntiles + 1, 256), 256, 0, st>>>(C.tileBegin.get(), C.tileEnd.get(), ntx, nty, ntz,
+  const int rz = C.shape == kShapeQuintic ? Box<kShapeQuintic>::RZ : Box<kShapeBinomial3X>::RZ;
+  kTileCounts<<<ceilDiv(ntiles + 1, 256), 256, 0, st>>>(C.tileBegin.get(), C.tileEnd.get(), ntx, nty, ntz, rz,
                                                          C.tileCounts.get());
   launchCheck("tile counts");
   bytes = C.scanTempBytes;
@@ -1271,7 +1389,9 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
     occupiedTiles = pin[2];
     if (nChunks > chunkBound) throw std::logic_error("Chunk list exceeds its bound");
   }
-  const std::size_t partialNeed = static_cast<std::size_t>(nChunks) * 3 * kExtNodes;
+  const bool quintic = C.shape == kShapeQuintic;
+  const int boxN = quintic ? boxNodes<kShapeQuintic>() : boxNodes<kShapeBinomial3X>();
+  const std::size_t partialNeed = static_cast<std::size_t>(nChunks) * 3 * boxN;
   if (partialNeed > C.partial.size()) {
     // tile partials must start at zero; the reduction keeps them so
     C.partial.resize(partialNeed);
@@ -1293,8 +1413,12 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
       GArgs G0 = G;
       for (int a = 0; a < 3; ++a) G0.j[a] = nullptr;
       ScopedKernel sk(F.profiler, st, "gather");
-      kGatherChunks<0><<<nChunks, 2 * kChunk, 0, st>>>(A, G0, C.chunks.get(), ntx, nty, F.g.dt, nullptr,
-                                                       C.error.get());
+      if (quintic)
+        kGatherChunks<0, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G0, C.chunks.get(), ntx, nty, F.g.dt,
+                                                                         nullptr, C.error.get());
+      else
+        kGatherChunks<0, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G0, C.chunks.get(), ntx, nty, F.g.dt,
+                                                                            nullptr, C.error.get());
       launchCheck("gather");
     }
     if (n) {
@@ -1305,15 +1429,26 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
     if (nChunks) {
       {
         ScopedKernel sk(F.profiler, st, "deposit");
-        kDeposit<<<nChunks, kChunk, 0, st>>>(A, G, C.chunks.get(), C.tileBegin.get(), C.tileEnd.get(),
-                                             C.tileScan.get(), ntx, nty, C.partial.get(), C.tileShift.get(),
-                                             C.error.get());
+        if (quintic)
+          kDeposit<kShapeQuintic><<<nChunks, kChunk, 0, st>>>(A, G, C.chunks.get(), C.tileBegin.get(),
+                                                              C.tileEnd.get(), C.tileScan.get(), ntx, nty,
+                                                              C.partial.get(), C.tileShift.get(), C.error.get());
+        else
+          kDeposit<kShapeBinomial3X><<<nChunks, kChunk, 0, st>>>(A, G, C.chunks.get(), C.tileBegin.get(),
+                                                                 C.tileEnd.get(), C.tileScan.get(), ntx, nty,
+                                                                 C.partial.get(), C.tileShift.get(), C.error.get());
         launchCheck("deposit");
       }
       {
         ScopedKernel sk(F.profiler, st, "reduce_current");
-        kReduceCurrent<<<nActive, 256, 0, st>>>(F.base(6), F.base(7), F.base(8), G, C.activeTiles.get(),
-                                                C.tileScan.get(), C.tileShift.get(), C.partial.get(), ntx, nty, ntz);
+        if (quintic)
+          kReduceCurrent<kShapeQuintic><<<nActive, 256, 0, st>>>(F.base(6), F.base(7), F.base(8), G,
+                                                                 C.activeTiles.get(), C.tileScan.get(),
+                                                                 C.tileShift.get(), C.partial.get(), ntx, nty, ntz);
+        else
+          kReduceCurrent<kShapeBinomial3X><<<nActive, 256, 0, st>>>(F.base(6), F.base(7), F.base(8), G,
+                                                                    C.activeTiles.get(), C.tileScan.get(),
+                                                                    C.tileShift.get(), C.partial.get(), ntx, nty, ntz);
         launchCheck("reduce current");
       }
       F.currentZero = false;
@@ -1329,9 +1464,12 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
     }
     {
       ScopedKernel sk(F.profiler, st, "residual");
-      if (nChunks)
-        kGatherChunks<1><<<nChunks, 2 * kChunk, 0, st>>>(A, G, C.chunks.get(), ntx, nty, F.g.dt,
-                                                         C.reduceBlocks.get(), C.error.get());
+      if (nChunks && quintic)
+        kGatherChunks<1, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G, C.chunks.get(), ntx, nty, F.g.dt,
+                                                                         C.reduceBlocks.get(), C.error.get());
+      else if (nChunks)
+        kGatherChunks<1, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G, C.chunks.get(), ntx, nty, F.g.dt,
+                                                                            C.reduceBlocks.get(), C.error.get());
       kReduceBlocks<<<1, 256, 0, st>>>(C.reduceBlocks.get(), static_cast<int>(nChunks), C.reduceOut.get());
       launchCheck("residual");
     }

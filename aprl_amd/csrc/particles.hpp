@@ -5,15 +5,11 @@
 
 namespace aprl_gpu {
 
-// Cell tile used for the per-step sort and the deposit. The extended node
-// range of a tile covers every binomial3-x tap of a chord that starts in the
-// tile and moves less than one cell per axis (always true when c*dt = dz and
-// dz <= dx, dy): x and z [t0-2, t0+T+2], y [t0-1, t0+T+1].
+// Cell tile used for the per-step sort, the gathers and the deposit.
+// Chords move less than one cell per axis per step (c*dt = dz <= dx, dy),
+// so each shape's taps of a chord that starts in a tile lie in a fixed
+// extended node box around the tile.
 constexpr int kTileX = 8, kTileY = 8, kTileZ = 4;
-constexpr int kExtX = kTileX + 5, kExtY = kTileY + 3, kExtZ = kTileZ + 5;
-constexpr int kExtNodes = kExtX * kExtY * kExtZ;
-// Sub-segments of one chord: at most one face crossing per axis per step.
-constexpr int kMaxSegments = 4;
 
 // Error bits raised by kernels; the host maps the lowest set bit onto the
 // reference's exception type and message.
@@ -33,6 +29,7 @@ enum ErrorBit : unsigned {
   kErrUpstream = 1u << 12,         // simulation.cpp:4926-4930 (runtime_error)
   kErrCrossings = 1u << 13,        // orbit crossing budget (runtime_error)
   kErrTileHalo = 1u << 14,         // chord longer than one cell per axis (runtime_error)
+  kErrMagneticNoDomain = 1u << 15, // source_coupling.cpp:84 (out_of_range)
 };
 
 struct MagnetDev {
@@ -58,13 +55,6 @@ struct ParticleStore {
   // trial pushes with (Ex Ey Ez Bx By Bz) and the residual gather's next
   // fields; end[6] = trial end state x1, u1; gridAbs = per-substep work sum.
   DeviceArray<double> fields[2][6], end[6], gridAbs;
-  // Deposit chord table written by the push: per particle the segment count,
-  // chord velocity (cells) and current scale; per segment the midpoint cell
-  // (int4) and (fx, fy, fz, width) (double4).
-  DeviceArray<int> segCount;
-  DeviceArray<double> segV;      // 6 per particle: v[3], scale[3]
-  DeviceArray<int4> segCell;     // kMaxSegments per particle
-  DeviceArray<double4> segFrac;  // kMaxSegments per particle
   // sort
   DeviceArray<unsigned> key, keySorted, perm, permSorted;
   DeviceArray<unsigned char> sortTemp;

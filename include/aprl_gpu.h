@@ -163,7 +163,10 @@ typedef struct aprl_step_report {
 } aprl_step_report;
 
 /* ConservativeSourceCoupling (source_coupling.h:12-41) bound to a field
- * grid; transverse_x_binomial3 selects the production F3 x-smoothing. */
+ * grid. Shapes: APRL_SHAPE_QUINTIC_COMPENSATED, or
+ * APRL_SHAPE_LINEAR_BINOMIAL3 with transverse_x_binomial3 = 1 (the
+ * production F3 x-smoothing); transverse_x_binomial3 is ignored for the
+ * quintic shape. */
 aprl_status aprl_coupling_create(aprl_fields* fields, aprl_shape shape, int transverse_x_binomial3,
                                  aprl_coupling** out);
 void aprl_coupling_destroy(aprl_coupling* coupling);

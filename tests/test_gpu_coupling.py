@@ -22,13 +22,17 @@ def rel_l2(a, b):
     return np.linalg.norm(a - b) / (den if den > 0 else 1.0)
 
 
-def case(dims, cells, n_particles, seed, gamma_u=10.0, spread=(1.0, 1.0, 3.0)):
+SHAPES = {"binomial3": ("binomial3", True, "linear-binomial3"), "quintic": ("quintic", False, "quintic-compensated")}
+
+
+def case(dims, cells, n_particles, seed, gamma_u=10.0, spread=(1.0, 1.0, 3.0), shape="binomial3"):
     import aprl_amd as A
+    oshape, smooth, gshape = SHAPES[shape]
     dx = 2.5 * DZ
     orc = H.CpuFields("orc", *dims, dx, dx, DZ, DZ / H.C_SI, 1)
     orc.install_window(cells)
     H.seeded_fields(orc, seed, scale=1e5)
-    co = H.CpuCoupling(orc, "binomial3", True, cpml_cells=cells)
+    co = H.CpuCoupling(orc, oshape, smooth, cpml_cells=cells)
     co.set_frame(5.0)
     co.add_magnet("dipole", [0, 0, -1.0], 2.0, peak=0.5)
     g = A.EBGridGeometry(*dims, dx, dx, DZ, DZ / H.C_SI)
@@ -36,7 +40,7 @@ def case(dims, cells, n_particles, seed, gamma_u=10.0, spread=(1.0, 1.0, 3.0)):
     win = A.MovingWindowFields(gpu, A.EBCPMLParameters(cells))
     for a in range(9):
         gpu.set_component(a, orc.view(a))
-    cg = A.ConservativeSourceCoupling(gpu)
+    cg = A.ConservativeSourceCoupling(gpu, gshape)
     cg.setFrame(5.0)
     cg.addPrescribedLabMagnet("uniform-dipole", [0, 0, -1.0], 2.0, peak_field=0.5)
     cg.setParticleBoundary(cells)
@@ -93,17 +97,19 @@ def compare(orc, co, gpu, store, tol=TOL):
     assert rel_l2(ug[order], uo) <= tol
 
 
-def test_coupled_steps_match_oracle():
+@pytest.mark.parametrize("shape", ["binomial3", "quintic"])
+def test_coupled_steps_match_oracle(shape):
     dims = (20, 20, 40)
-    args = case(dims, (4, 4, 0), 200, 17)
+    args = case(dims, (4, 4, 0), 200, 17, shape=shape)
     logs = run_both(*args, dims, 4, 4, check_each=True)
     assert all(a == b for a, b in logs), logs
     assert all(a >= 2 for a, _ in logs)
 
 
-def test_dense_bunch_many_particles_per_tile():
+@pytest.mark.parametrize("shape", ["binomial3", "quintic"])
+def test_dense_bunch_many_particles_per_tile(shape):
     dims = (24, 24, 48)
-    args = case(dims, (4, 4, 0), 4000, 23, spread=(0.8, 0.8, 2.0))
+    args = case(dims, (4, 4, 0), 4000, 23, spread=(0.8, 0.8, 2.0), shape=shape)
     logs = run_both(*args, dims, 3, 9)
     assert all(a == b for a, b in logs), logs
     compare(*args[:3], args[5])
