@@ -124,6 +124,10 @@ def lib() -> C.CDLL:
     L.aprl_coupling_add_magnet.argtypes = [vp, C.POINTER(Magnet)]
     L.aprl_coupling_set_particle_boundary.argtypes = [vp, C.POINTER(_sz), C.c_double]
     L.aprl_coupling_step.argtypes = [vp, vp, _dp, C.c_double, C.c_uint, C.POINTER(StepReport)]
+    L.aprl_fields_interior_energy.argtypes = [vp, C.POINTER(_sz), C.POINTER(_sz), _dp]
+    L.aprl_fields_face_power.argtypes = [vp, C.POINTER(_sz), C.POINTER(_sz), _dp]
+    L.aprl_coupling_continuity.argtypes = [vp, vp, _dp]
+    L.aprl_coupling_gauss.argtypes = [vp, vp, _dp, C.POINTER(_sz), C.POINTER(_sz), _dp]
     _lib = L
     return L
 
@@ -140,6 +144,10 @@ def dptr(a: np.ndarray):
     if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
         raise TypeError("expected a C-contiguous float64 array")
     return a.ctypes.data_as(_dp)
+
+
+def size3(v):
+    return (_sz * 3)(*[int(x) for x in v])
 
 
 def plane_array(planes):
@@ -161,4 +169,5 @@ EXPORTED = [
     "aprl_particles_create", "aprl_particles_destroy", "aprl_particles_upload", "aprl_particles_download",
     "aprl_particles_count", "aprl_coupling_create", "aprl_coupling_destroy", "aprl_coupling_set_frame",
     "aprl_coupling_add_magnet", "aprl_coupling_set_particle_boundary", "aprl_coupling_step",
+    "aprl_fields_interior_energy", "aprl_fields_face_power", "aprl_coupling_continuity", "aprl_coupling_gauss",
 ]

@@ -114,3 +114,21 @@ class ConservativeSourceCoupling:
         check(lib().aprl_coupling_step(self.handle, particles.handle, dptr(o), time, substeps, C.byref(r)))
         self.last_tiles, self.last_chunks = r.tiles, r.chunks
         return r.iterations, r.force, r.work
+
+    def continuity(self, particles: ParticleStore):
+        """(max |(rho1 - rho0)/dt + div J|, max |(rho1 - rho0)/dt|) of the
+        last step (ChargeConservingCurrentDepositor::maxContinuityResidual,
+        eb_deposition.cpp:195-226); call before the window shift."""
+        out = np.zeros(2)
+        check(lib().aprl_coupling_continuity(self.handle, particles.handle, dptr(out)))
+        return out[0], out[1]
+
+    def gaussResidual(self, particles: ParticleStore, origin, lo, hi):
+        """(sum err^2, sum src^2, max |err|, max |src|) of div E - rho/eps0
+        over the vertices strictly inside [lo, hi] (eb_initial_field.cpp:
+        774-799)."""
+        out = np.zeros(4)
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        check(lib().aprl_coupling_gauss(self.handle, particles.handle, dptr(o), _capi.size3(lo), _capi.size3(hi),
+                                        dptr(out)))
+        return tuple(out)

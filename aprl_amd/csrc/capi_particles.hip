@@ -95,6 +95,7 @@ aprl_status aprl_particles_upload(aprl_particles* particles, size_t n, const dou
     hipCheck(hipMemcpy(P.id.get(), id, n * sizeof(uint64_t), hipMemcpyHostToDevice), "hipMemcpy");
     (void)st;
     P.count = n;
+    ++P.generation;
   });
 }
 

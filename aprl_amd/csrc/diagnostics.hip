@@ -1,0 +1,475 @@
+// FP64 diagnostics of the field grid and the coupled step on the device:
+// interior field energy, outward Poynting power through the faces of a
+// cell box (the energy-ledger channels), the charge-continuity residual of
+// the last coupled step and the Gauss-law residual. Every sum is a
+// compensated, fixed-order reduction, so results do not depend on timing.
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+#include "diagnostics.hpp"
+
+namespace aprl_gpu {
+
+namespace {
+
+constexpr int kRedBlocks = 512, kRedThreads = 256;
+
+// Neumaier two-term accumulator.
+struct Acc {
+  double s = 0.0, c = 0.0;
+  __device__ void add(double x) {
+    const double t = s + x;
+    if (fabs(s) >= fabs(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+  }
+  __device__ void add(const Acc& o) {
+    add(o.s);
+    add(o.c);
+  }
+};
+
+__device__ __forceinline__ double at(const double* p, const int* n, int i, int j, int k) {
+  return p[(static_cast<size_t>(k) * n[1] + j) * n[0] + i];
+}
+
+// EBFieldGrid::fieldEnergyDensityCell, eb_field.cpp:715-738.
+__device__ double energyDensityCell(const FieldPtrs& F, int i, int j, int k) {
+  const double e0 = 0.25 * (at(F.e[0], F.n[0], i, j, k) + at(F.e[0], F.n[0], i, j + 1, k) +
+                            at(F.e[0], F.n[0], i, j, k + 1) + at(F.e[0], F.n[0], i, j + 1, k + 1));
+  const double e1 = 0.25 * (at(F.e[1], F.n[1], i, j, k) + at(F.e[1], F.n[1], i + 1, j, k) +
+                            at(F.e[1], F.n[1], i, j, k + 1) + at(F.e[1], F.n[1], i + 1, j, k + 1));
+  const double e2 = 0.25 * (at(F.e[2], F.n[2], i, j, k) + at(F.e[2], F.n[2], i + 1, j, k) +
+                            at(F.e[2], F.n[2], i, j + 1, k) + at(F.e[2], F.n[2], i + 1, j + 1, k));
+  const double b0 = 0.5 * (at(F.b[0], F.n[3], i, j, k) + at(F.b[0], F.n[3], i + 1, j, k));
+  const double b1 = 0.5 * (at(F.b[1], F.n[4], i, j, k) + at(F.b[1], F.n[4], i, j + 1, k));
+  const double b2 = 0.5 * (at(F.b[2], F.n[5], i, j, k) + at(F.b[2], F.n[5], i, j, k + 1));
+  return 0.5 * (kEps0 * (e0 * e0 + e1 * e1 + e2 * e2) + (b0 * b0 + b1 * b1 + b2 * b2) / kMu0);
+}
+
+// EBFieldGrid::poyntingComponentCell, eb_field.cpp:740-782.
+__device__ double poyntingCell(const FieldPtrs& F, int i, int j, int k, int axis) {
+  auto cEx = [&]() {
+    return 0.25 * (at(F.e[0], F.n[0], i, j, k) + at(F.e[0], F.n[0], i, j + 1, k) + at(F.e[0], F.n[0], i, j, k + 1) +
+                   at(F.e[0], F.n[0], i, j + 1, k + 1));
+  };
+  auto cEy = [&]() {
+    return 0.25 * (at(F.e[1], F.n[1], i, j, k) + at(F.e[1], F.n[1], i + 1, j, k) + at(F.e[1], F.n[1], i, j, k + 1) +
+                   at(F.e[1], F.n[1], i + 1, j, k + 1));
+  };
+  auto cEz = [&]() {
+    return 0.25 * (at(F.e[2], F.n[2], i, j, k) + at(F.e[2], F.n[2], i + 1, j, k) + at(F.e[2], F.n[2], i, j + 1, k) +
+                   at(F.e[2], F.n[2], i + 1, j + 1, k));
+  };
+  auto cBx = [&]() { return 0.5 * (at(F.b[0], F.n[3], i, j, k) + at(F.b[0], F.n[3], i + 1, j, k)); };
+  auto cBy = [&]() { return 0.5 * (at(F.b[1], F.n[4], i, j, k) + at(F.b[1], F.n[4], i, j + 1, k)); };
+  auto cBz = [&]() { return 0.5 * (at(F.b[2], F.n[5], i, j, k) + at(F.b[2], F.n[5], i, j, k + 1)); };
+  double eA, eB, bA, bB;
+  if (axis == 0) { eA = cEy(); eB = cEz(); bA = cBy(); bB = cBz(); }
+  else if (axis == 1) { eA = cEz(); eB = cEx(); bA = cBz(); bB = cBx(); }
+  else { eA = cEx(); eB = cEy(); bA = cBx(); bB = cBy(); }
+  double s = eA * bB - eB * bA;
+  s /= kMu0;
+  return s;
+}
+
+// Block reduction of one accumulator in a fixed tree order; thread 0
+// returns the block total.
+__device__ Acc blockReduce(Acc a, double* ss, double* sc) {
+  ss[threadIdx.x] = a.s;
+  sc[threadIdx.x] = a.c;
+  __syncthreads();
+  for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) {
+      Acc x{ss[threadIdx.x], sc[threadIdx.x]};
+      x.add(Acc{ss[threadIdx.x + w], sc[threadIdx.x + w]});
+      ss[threadIdx.x] = x.s;
+      sc[threadIdx.x] = x.c;
+    }
+    __syncthreads();
+  }
+  return Acc{ss[0], sc[0]};
+}
+
+// Sum of energyDensityCell * volume over the cells of [lo, hi).
+__global__ void __launch_bounds__(kRedThreads) kEnergyBox(FieldPtrs F, int3 lo, int3 hi, double volume,
+                                                          double* partial) {
+  __shared__ double ss[kRedThreads], sc[kRedThreads];
+  const long long wx = hi.x - lo.x, wy = hi.y - lo.y, wz = hi.z - lo.z, total = wx * wy * wz;
+  Acc a;
+  for (long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; t < total;
+       t += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const int i = lo.x + static_cast<int>(t % wx), j = lo.y + static_cast<int>((t / wx) % wy),
+              k = lo.z + static_cast<int>(t / (wx * wy));
+    a.add(energyDensityCell(F, i, j, k) * volume);
+  }
+  const Acc b = blockReduce(a, ss, sc);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = b.s;
+    partial[2 * blockIdx.x + 1] = b.c;
+  }
+}
+
+// Outward power through the six faces of the cell box [lo, hi):
+// face 2a (lower, sign -) and 2a+1 (upper, sign +) sum the axis-a Poynting
+// component over the face's cells times the face area. blockIdx.y = face.
+__global__ void __launch_bounds__(kRedThreads) kFacePower(FieldPtrs F, int3 lo, int3 hi, double3 area,
+                                                          double* partial) {
+  __shared__ double ss[kRedThreads], sc[kRedThreads];
+  const int face = blockIdx.y, axis = face / 2;
+  const bool upper = face & 1;
+  const int l[3] = {lo.x, lo.y, lo.z}, h[3] = {hi.x, hi.y, hi.z};
+  const int ua = (axis + 1) % 3, va = (axis + 2) % 3;
+  // loop order of the reference: the in-face axes run x fastest
+  const int fa = ua < va ? ua : va, sa = ua < va ? va : ua;
+  const long long wf = h[fa] - l[fa], ws = h[sa] - l[sa], total = wf * ws;
+  const double ar = axis == 0 ? area.x : (axis == 1 ? area.y : area.z);
+  Acc a;
+  for (long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; t < total;
+       t += static_cast<long long>(gridDim.x) * blockDim.x) {
+    int c[3];
+    c[axis] = upper ? h[axis] - 1 : l[axis];
+    c[fa] = l[fa] + static_cast<int>(t % wf);
+    c[sa] = l[sa] + static_cast<int>(t / wf);
+    const double s = poyntingCell(F, c[0], c[1], c[2], axis) * ar;
+    a.add(upper ? s : -s);
+  }
+  const Acc b = blockReduce(a, ss, sc);
+  if (threadIdx.x == 0) {
+    partial[2 * (face * gridDim.x + blockIdx.x)] = b.s;
+    partial[2 * (face * gridDim.x + blockIdx.x) + 1] = b.c;
+  }
+}
+
+// Final compensated sum of `blocks` partial accumulators per output.
+__global__ void __launch_bounds__(kRedThreads) kFinish(const double* partial, int blocks, double* out) {
+  __shared__ double ss[kRedThreads], sc[kRedThreads];
+  const double* p = partial + 2 * static_cast<size_t>(blockIdx.x) * blocks;
+  Acc a;
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x) a.add(Acc{p[2 * b], p[2 * b + 1]});
+  const Acc r = blockReduce(a, ss, sc);
+  if (threadIdx.x == 0) out[blockIdx.x] = r.s + r.c;
+}
+
+// ---------------------------------------------------------------------------
+// Charge density, continuity and Gauss residuals.
+
+struct RhoArgs {
+  const double* x[3];
+  const double* q;
+  int n;
+  double o[3], h[3];
+  int nx, ny, nz;
+  int shape;  // 0 compensated quintic, 2 binomial3 with x-binomial3
+  int shift;
+  double invVolume;
+  unsigned long long* acc;  // (nx+1)(ny+1)(nz+1) vertices
+};
+
+// longitudinal_shape::basis (longitudinal_shape.h:25-55) for the binomial3
+// and compensated quintic nodes.
+__device__ int chargeBasis(int shape, double z, double v[12]) {
+  const double cell = floor(z), f = z - cell;
+  for (int k = 0; k < 12; ++k) v[k] = 0.0;
+  if (shape == 2) {
+    const double c[3] = {.25, .5, .25};
+    for (int k = -1; k <= 1; ++k) {
+      v[k + 1] += c[k + 1] * (1. - f);
+      v[k + 2] += c[k + 1] * f;
+    }
+    return static_cast<int>(cell) - 1;
+  }
+  double n[6] = {1., 0., 0., 0., 0., 0.};
+  for (unsigned j = 1; j <= 5; ++j) {
+    double saved = 0.;
+    for (unsigned k = 0; k < j; ++k) {
+      const double t = n[k] / j;
+      n[k] = saved + (k + 1 - f) * t;
+      saved = (f + j - k - 1) * t;
+    }
+    n[j] = saved;
+  }
+  const double p = 5., a = p / 6., b = 3 * p / 40. + p * (p - 1) / 72.,
+               d = 5 * p / 112. + p * (p - 1) / 80. + p * (p - 1) * (p - 2) / 1296.;
+  const double c[7] = {-d / 64., b / 16. + 3 * d / 32., -a / 4. - b / 4. - 15 * d / 64.,
+                       1 + a / 2. + 3 * b / 8. + 5 * d / 16., -a / 4. - b / 4. - 15 * d / 64., b / 16. + 3 * d / 32.,
+                       -d / 64.};
+  for (unsigned j = 0; j <= 5; ++j)
+    for (unsigned k = 0; k < 7; ++k) v[j + k] += n[j] * c[k];
+  return static_cast<int>(cell) - 5;
+}
+
+// Shaped charge of every particle on the vertices, in 64-bit fixed point:
+// binomial3-x: x nodeWeights (4) x linear y (2) x binomial3 z basis (4),
+// the charge whose change the binomial3-x current conserves
+// (verification/transverse_binomial_checks.cpp:33-40); quintic: linear x, y
+// (2 x 2) x quintic z basis (12). Vertices outside the grid are dropped.
+__global__ void kChargeDensity(RhoArgs R) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= R.n) return;
+  const double lx = (R.x[0][p] - R.o[0]) / R.h[0], ly = (R.x[1][p] - R.o[1]) / R.h[1],
+               lz = (R.x[2][p] - R.o[2]) / R.h[2];
+  const int i = static_cast<int>(floor(lx)), j = static_cast<int>(floor(ly));
+  const double fx = lx - i, fy = ly - j;
+  double wz[12];
+  const int k0 = chargeBasis(R.shape, lz, wz);
+  const int nz = R.shape == 2 ? 4 : 12;
+  double wx[4];
+  int nxw, i0;
+  if (R.shape == 2) {
+    wx[0] = 0.25 * (1. - fx); wx[1] = 0.5 - 0.25 * fx; wx[2] = 0.25 + 0.25 * fx; wx[3] = 0.25 * fx;
+    nxw = 4;
+    i0 = i - 1;
+  } else {
+    wx[0] = 1. - fx; wx[1] = fx;
+    nxw = 2;
+    i0 = i;
+  }
+  const double wy[2] = {1. - fy, fy};
+  const double qv = R.q[p] * R.invVolume;
+  const int vx = R.nx + 1, vy = R.ny + 1, vz = R.nz + 1;
+  for (int c = 0; c < nz; ++c) {
+    const int kk = k0 + c;
+    if (kk < 0 || kk >= vz) continue;
+    for (int b = 0; b < 2; ++b) {
+      const int jj = j + b;
+      if (jj < 0 || jj >= vy) continue;
+      for (int a = 0; a < nxw; ++a) {
+        const int ii = i0 + a;
+        if (ii < 0 || ii >= vx) continue;
+        const double w = qv * wx[a] * wy[b] * wz[c];
+        atomicAdd(R.acc + (static_cast<size_t>(kk) * vy + jj) * vx + ii,
+                  static_cast<unsigned long long>(llrint(ldexp(w, R.shift))));
+      }
+    }
+  }
+}
+
+__global__ void kFixedToDouble(unsigned long long* acc, double* out, size_t n, int shift) {
+  for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    out[t] = ldexp(static_cast<double>(static_cast<long long>(acc[t])), -shift);
+    acc[t] = 0ull;
+  }
+}
+
+__global__ void kSumAbs(const double* q, int n, double* partial) {
+  __shared__ double ss[kRedThreads], sc[kRedThreads];
+  Acc a;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) a.add(fabs(q[p]));
+  const Acc b = blockReduce(a, ss, sc);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = b.s;
+    partial[2 * blockIdx.x + 1] = b.c;
+  }
+}
+
+__device__ __forceinline__ void atomicMaxAbs(unsigned long long* slot, double v) {
+  // non-negative doubles order like their bit patterns
+  atomicMax(slot, static_cast<unsigned long long>(__double_as_longlong(fabs(v))));
+}
+
+// ChargeConservingCurrentDepositor::maxContinuityResidual,
+// eb_deposition.cpp:195-226: max over vertices of
+// |(rho1 - rho0)/dt + div J|, and of |(rho1 - rho0)/dt| as its scale.
+__global__ void kContinuity(FieldPtrs F, const double* rho0, const double* rho1, double dt, double3 h,
+                            unsigned long long* out) {
+  const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
+  const size_t total = static_cast<size_t>(nx + 1) * (ny + 1) * (nz + 1);
+  for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < total;
+       t += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const int i = static_cast<int>(t % (nx + 1)), j = static_cast<int>((t / (nx + 1)) % (ny + 1)),
+              k = static_cast<int>(t / (static_cast<size_t>(nx + 1) * (ny + 1)));
+    const double rightX = i < nx ? at(F.j[0], F.n[6], i, j, k) : 0.0;
+    const double leftX = i > 0 ? at(F.j[0], F.n[6], i - 1, j, k) : 0.0;
+    const double rightY = j < ny ? at(F.j[1], F.n[7], i, j, k) : 0.0;
+    const double leftY = j > 0 ? at(F.j[1], F.n[7], i, j - 1, k) : 0.0;
+    const double rightZ = k < nz ? at(F.j[2], F.n[8], i, j, k) : 0.0;
+    const double leftZ = k > 0 ? at(F.j[2], F.n[8], i, j, k - 1) : 0.0;
+    const double rate = (rho1[t] - rho0[t]) / dt;
+    const double residual = rate + (rightX - leftX) / h.x + (rightY - leftY) / h.y + (rightZ - leftZ) / h.z;
+    atomicMaxAbs(out, residual);
+    atomicMaxAbs(out + 1, rate);
+  }
+}
+
+// Gauss residual of eb_initial_field.cpp:774-799 over the vertices strictly
+// inside [lo, hi]: error = div E - rho / eps0; per block the compensated
+// sums of error^2 and source^2, and the max |error| and |source|.
+__global__ void __launch_bounds__(kRedThreads) kGauss(FieldPtrs F, const double* rho, int3 lo, int3 hi, double3 h,
+                                                      double* partial, unsigned long long* maxima) {
+  __shared__ double ss[kRedThreads], sc[kRedThreads];
+  const int nx = F.n[0][0], ny = F.n[1][1];
+  const long long wx = hi.x - lo.x - 1, wy = hi.y - lo.y - 1, wz = hi.z - lo.z - 1;
+  const long long total = wx > 0 && wy > 0 && wz > 0 ? wx * wy * wz : 0;
+  Acc e2, s2;
+  for (long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; t < total;
+       t += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const int i = lo.x + 1 + static_cast<int>(t % wx), j = lo.y + 1 + static_cast<int>((t / wx) % wy),
+              k = lo.z + 1 + static_cast<int>(t / (wx * wy));
+    const double divergence = (at(F.e[0], F.n[0], i, j, k) - at(F.e[0], F.n[0], i - 1, j, k)) / h.x +
+                              (at(F.e[1], F.n[1], i, j, k) - at(F.e[1], F.n[1], i, j - 1, k)) / h.y +
+                              (at(F.e[2], F.n[2], i, j, k) - at(F.e[2], F.n[2], i, j, k - 1)) / h.z;
+    const double source = rho[(static_cast<size_t>(k) * (ny + 1) + j) * (nx + 1) + i] / kEps0;
+    const double error = divergence - source;
+    e2.add(error * error);
+    s2.add(source * source);
+    atomicMaxAbs(maxima, error);
+    atomicMaxAbs(maxima + 1, source);
+  }
+  const Acc a = blockReduce(e2, ss, sc);
+  __syncthreads();
+  const Acc b = blockReduce(s2, ss, sc);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = a.s;
+    partial[2 * blockIdx.x + 1] = a.c;
+    partial[2 * (kRedBlocks + blockIdx.x)] = b.s;
+    partial[2 * (kRedBlocks + blockIdx.x) + 1] = b.c;
+  }
+}
+
+__global__ void kZeroU64(unsigned long long* p, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0ull;
+}
+
+template <class T>
+void ensure(DeviceArray<T>& a, std::size_t n) {
+  if (a.size() < n) a.resize(n);
+}
+
+int3 toInt3(const std::size_t v[3]) {
+  return make_int3(static_cast<int>(v[0]), static_cast<int>(v[1]), static_cast<int>(v[2]));
+}
+
+void checkBox(const FieldState& s, const std::size_t lo[3], const std::size_t hi[3], bool cells) {
+  const std::size_t n[3] = {s.g.nx, s.g.ny, s.g.nz};
+  for (int a = 0; a < 3; ++a)
+    if (lo[a] > hi[a] || hi[a] > n[a] + (cells ? 0 : 1))
+      throw std::out_of_range("Diagnostic box is outside the E/B grid");
+}
+
+}  // namespace
+
+double interiorEnergy(FieldState& s, const std::size_t lo[3], const std::size_t hi[3]) {
+  checkBox(s, lo, hi, true);
+  DiagnosticScratch& D = s.diag;
+  ensure(D.partial, 2 * 6 * kRedBlocks);
+  ensure(D.out, 8);
+  const double volume = s.g.dx * s.g.dy * s.g.dz;
+  kEnergyBox<<<kRedBlocks, kRedThreads, 0, s.stream>>>(s.ptrs(), toInt3(lo), toInt3(hi), volume, D.partial.get());
+  kFinish<<<1, kRedThreads, 0, s.stream>>>(D.partial.get(), kRedBlocks, D.out.get());
+  launchCheck("interior energy");
+  double e = 0.0;
+  hipCheck(hipMemcpyAsync(&e, D.out.get(), sizeof(double), hipMemcpyDeviceToHost, s.stream), "hipMemcpyAsync");
+  hipCheck(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
+  return e;
+}
+
+void facePower(FieldState& s, const std::size_t lo[3], const std::size_t hi[3], double power[6]) {
+  checkBox(s, lo, hi, true);
+  for (int a = 0; a < 3; ++a)
+    if (lo[a] >= hi[a]) throw std::out_of_range("Diagnostic face box is empty");
+  DiagnosticScratch& D = s.diag;
+  ensure(D.partial, 2 * 6 * kRedBlocks);
+  ensure(D.out, 8);
+  const double3 area = make_double3(s.g.dy * s.g.dz, s.g.dx * s.g.dz, s.g.dx * s.g.dy);
+  constexpr int kFaceBlocks = 64;
+  kFacePower<<<dim3(kFaceBlocks, 6), kRedThreads, 0, s.stream>>>(s.ptrs(), toInt3(lo), toInt3(hi), area,
+                                                                   D.partial.get());
+  kFinish<<<6, kRedThreads, 0, s.stream>>>(D.partial.get(), kFaceBlocks, D.out.get());
+  launchCheck("face power");
+  hipCheck(hipMemcpyAsync(power, D.out.get(), 6 * sizeof(double), hipMemcpyDeviceToHost, s.stream),
+           "hipMemcpyAsync");
+  hipCheck(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
+}
+
+// Shaped charge density of a particle set (positions x[3], charges q) on
+// the vertices into `rho`.
+void chargeDensity(FieldState& s, int shape, const double* const x[3], const double* q, int n, const double origin[3],
+                   DeviceArray<double>& rho) {
+  DiagnosticScratch& D = s.diag;
+  const std::size_t nv = (s.g.nx + 1) * (s.g.ny + 1) * (s.g.nz + 1);
+  if (D.acc.size() != nv) {
+    D.acc.resize(nv);
+    hipCheck(hipMemsetAsync(D.acc.get(), 0, nv * sizeof(unsigned long long), s.stream), "hipMemsetAsync");
+  }
+  if (rho.size() != nv) rho.resize(nv);
+  ensure(D.partial, 2 * 6 * kRedBlocks);
+  ensure(D.out, 8);
+  // global fixed-point exponent from the total |charge| density bound (each
+  // shape's weights sum to at most 4 in absolute value)
+  double total = 0.0;
+  if (n > 0) {
+    kSumAbs<<<kRedBlocks, kRedThreads, 0, s.stream>>>(q, n, D.partial.get());
+    kFinish<<<1, kRedThreads, 0, s.stream>>>(D.partial.get(), kRedBlocks, D.out.get());
+    launchCheck("charge bound");
+    hipCheck(hipMemcpyAsync(&total, D.out.get(), sizeof(double), hipMemcpyDeviceToHost, s.stream), "hipMemcpyAsync");
+    hipCheck(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
+  }
+  const double invVolume = 1.0 / (s.g.dx * s.g.dy * s.g.dz);
+  int e2 = 0;
+  (void)std::frexp(4.0 * total * invVolume, &e2);
+  const int shift = total > 0.0 ? 62 - e2 : 0;
+  RhoArgs R{};
+  for (int a = 0; a < 3; ++a) {
+    R.x[a] = x[a];
+    R.o[a] = origin[a];
+  }
+  R.h[0] = s.g.dx; R.h[1] = s.g.dy; R.h[2] = s.g.dz;
+  R.q = q;
+  R.n = n;
+  R.nx = static_cast<int>(s.g.nx);
+  R.ny = static_cast<int>(s.g.ny);
+  R.nz = static_cast<int>(s.g.nz);
+  R.shape = shape;
+  R.shift = shift;
+  R.invVolume = invVolume;
+  R.acc = D.acc.get();
+  if (n > 0) kChargeDensity<<<ceilDiv(n, 256), 256, 0, s.stream>>>(R);
+  kFixedToDouble<<<2048, 256, 0, s.stream>>>(D.acc.get(), rho.get(), nv, shift);
+  launchCheck("charge density");
+}
+
+void continuityResidual(FieldState& s, const DeviceArray<double>& rho0, const DeviceArray<double>& rho1,
+                        double out[2]) {
+  DiagnosticScratch& D = s.diag;
+  ensure(D.maxima, 4);
+  kZeroU64<<<1, 64, 0, s.stream>>>(D.maxima.get(), 4);
+  kContinuity<<<2048, 256, 0, s.stream>>>(s.ptrs(), rho0.get(), rho1.get(), s.g.dt,
+                                          make_double3(s.g.dx, s.g.dy, s.g.dz), D.maxima.get());
+  launchCheck("continuity");
+  unsigned long long m[2];
+  hipCheck(hipMemcpyAsync(m, D.maxima.get(), sizeof(m), hipMemcpyDeviceToHost, s.stream), "hipMemcpyAsync");
+  hipCheck(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
+  std::memcpy(&out[0], &m[0], sizeof(double));
+  std::memcpy(&out[1], &m[1], sizeof(double));
+}
+
+void gaussResidual(FieldState& s, const DeviceArray<double>& rho, const std::size_t lo[3], const std::size_t hi[3],
+                   double out[4]) {
+  // strictly interior vertices i in (lo, hi) read Ex(i), Ey(j), Ez(k): hi <= n
+  checkBox(s, lo, hi, true);
+  DiagnosticScratch& D = s.diag;
+  ensure(D.partial, 2 * 6 * kRedBlocks);
+  ensure(D.out, 8);
+  ensure(D.maxima, 4);
+  kZeroU64<<<1, 64, 0, s.stream>>>(D.maxima.get(), 4);
+  kGauss<<<kRedBlocks, kRedThreads, 0, s.stream>>>(s.ptrs(), rho.get(), toInt3(lo), toInt3(hi),
+                                                   make_double3(s.g.dx, s.g.dy, s.g.dz), D.partial.get(),
+                                                   D.maxima.get());
+  kFinish<<<2, kRedThreads, 0, s.stream>>>(D.partial.get(), kRedBlocks, D.out.get());
+  launchCheck("gauss");
+  double sums[2];
+  unsigned long long m[2];
+  hipCheck(hipMemcpyAsync(sums, D.out.get(), sizeof(sums), hipMemcpyDeviceToHost, s.stream), "hipMemcpyAsync");
+  hipCheck(hipMemcpyAsync(m, D.maxima.get(), sizeof(m), hipMemcpyDeviceToHost, s.stream), "hipMemcpyAsync");
+  hipCheck(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
+  out[0] = sums[0];
+  out[1] = sums[1];
+  std::memcpy(&out[2], &m[0], sizeof(double));
+  std::memcpy(&out[3], &m[1], sizeof(double));
+}
+
+}  // namespace aprl_gpu

@@ -46,6 +46,12 @@ struct CpmlArgs {
   double magScale[3], eleScale[3];
 };
 
+// Device scratch of the diagnostics.
+struct DiagnosticScratch {
+  DeviceArray<double> partial, out;
+  DeviceArray<unsigned long long> acc, maxima;
+};
+
 struct FieldState {
   aprl_grid_geometry g{};
   aprl_precision precision = APRL_FP64;
@@ -90,6 +96,7 @@ struct FieldState {
 
   hipStream_t stream = nullptr;
   Profiler profiler;
+  DiagnosticScratch diag;
 
   std::size_t compSize(int a) const { return dims[a][0] * dims[a][1] * dims[a][2]; }
   std::size_t plane(int a) const { return dims[a][0] * dims[a][1]; }

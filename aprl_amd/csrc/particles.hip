@@ -1359,13 +1359,15 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
     C.chunks.resize(std::max<std::size_t>(chunkBound, 1));
     C.chunkCapacity = chunkBound;
   }
-  if (!C.error.size()) { C.error.resize(2); C.reduceOut.resize(4); }
+  // error words: [0] step level, [1] occupied-tile counter, [2 + t] the
+  // kernels of Picard trial t (checked in trial order)
+  if (!C.error.size()) { C.error.resize(2 + 64); C.reduceOut.resize(4); }
   // pinned readback slots: [0..1] scan total (chunks, active tiles), [2]
-  // occupied tiles, [3] error bits; doubles from byte 16: force, defect,
-  // scale, calls
-  unsigned* pin = static_cast<unsigned*>(C.pinned.get(64));
-  double* pinD = reinterpret_cast<double*>(static_cast<unsigned char*>(C.pinned.get(64)) + 16);
-  kFillInt<<<1, 64, 0, st>>>(reinterpret_cast<int*>(C.error.get()), 2, 0);
+  // occupied tiles, [4..68) trial error words; doubles from byte 512:
+  // force, defect, scale, calls
+  unsigned* pin = static_cast<unsigned*>(C.pinned.get(1024));
+  double* pinD = reinterpret_cast<double*>(static_cast<unsigned char*>(C.pinned.get(1024)) + 512);
+  kFillInt<<<1, 128, 0, st>>>(reinterpret_cast<int*>(C.error.get()), 2 + 64, 0);
   launchCheck("fill");
   if (n) {
     sortAndChunk(C, P, G, ntx, nty, ntiles, C.error.get() + 1);
@@ -1406,7 +1408,9 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
   bool converged = false;
   unsigned iterations = 0;
   double force = 0., work = 0.;
+  unsigned checkedTrials = 0;
   for (unsigned trial = 0; trial < 64; ++trial) {
+    unsigned* errT = C.error.get() + 2 + trial;
     if (nChunks && trial == 0) {
       // The first trial pushes against the preview with J cleared
       // (simulation_source_coupling.cpp:43).
@@ -1415,15 +1419,15 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
       ScopedKernel sk(F.profiler, st, "gather");
       if (quintic)
         kGatherChunks<0, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G0, C.chunks.get(), ntx, nty, F.g.dt,
-                                                                         nullptr, C.error.get());
+                                                                         nullptr, errT);
       else
         kGatherChunks<0, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G0, C.chunks.get(), ntx, nty, F.g.dt,
-                                                                            nullptr, C.error.get());
+                                                                            nullptr, errT);
       launchCheck("gather");
     }
     if (n) {
       ScopedKernel sk(F.profiler, st, "push");
-      kPush<<<ceilDiv(n, 128), 128, 0, st>>>(A, G, S, C.error.get());
+      kPush<<<ceilDiv(n, 128), 128, 0, st>>>(A, G, S, errT);
       launchCheck("push");
     }
     if (nChunks) {
@@ -1432,11 +1436,11 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
         if (quintic)
           kDeposit<kShapeQuintic><<<nChunks, kChunk, 0, st>>>(A, G, C.chunks.get(), C.tileBegin.get(),
                                                               C.tileEnd.get(), C.tileScan.get(), ntx, nty,
-                                                              C.partial.get(), C.tileShift.get(), C.error.get());
+                                                              C.partial.get(), C.tileShift.get(), errT);
         else
           kDeposit<kShapeBinomial3X><<<nChunks, kChunk, 0, st>>>(A, G, C.chunks.get(), C.tileBegin.get(),
                                                                  C.tileEnd.get(), C.tileScan.get(), ntx, nty,
-                                                                 C.partial.get(), C.tileShift.get(), C.error.get());
+                                                                 C.partial.get(), C.tileShift.get(), errT);
         launchCheck("deposit");
       }
       {
@@ -1466,23 +1470,30 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
       ScopedKernel sk(F.profiler, st, "residual");
       if (nChunks && quintic)
         kGatherChunks<1, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G, C.chunks.get(), ntx, nty, F.g.dt,
-                                                                         C.reduceBlocks.get(), C.error.get());
+                                                                         C.reduceBlocks.get(), errT);
       else if (nChunks)
         kGatherChunks<1, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G, C.chunks.get(), ntx, nty, F.g.dt,
-                                                                            C.reduceBlocks.get(), C.error.get());
+                                                                            C.reduceBlocks.get(), errT);
       kReduceBlocks<<<1, 256, 0, st>>>(C.reduceBlocks.get(), static_cast<int>(nChunks), C.reduceOut.get());
       launchCheck("residual");
     }
-    hipCheck(hipMemcpyAsync(pinD, C.reduceOut.get(), 4 * sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpy");
-    hipCheck(hipMemcpyAsync(pin + 3, C.error.get(), sizeof(unsigned), hipMemcpyDeviceToHost, st), "hipMemcpy");
-    hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
-    raiseErrors(pin[3]);
-    // simulation_source_coupling.cpp:46-51.
-    force = pinD[0];
-    work = pinD[3] == 0. ? 0. : (pinD[2] > 0. ? pinD[1] / pinD[2] : pinD[1]);
     iterations = trial + 1;
     // The next trial pushes with the fields this residual gathered.
     for (int a = 0; a < 6; ++a) std::swap(A.fin[a], A.fout[a]);
+    // With active particles the convergence test needs at least two trials
+    // (simulation_source_coupling.cpp:50), so trial 0 is never read back;
+    // its error word is checked, first, after trial 1.
+    if (n && trial == 0) continue;
+    hipCheck(hipMemcpyAsync(pinD, C.reduceOut.get(), 4 * sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpy");
+    hipCheck(hipMemcpyAsync(pin + 4, C.error.get() + 2 + checkedTrials, (trial + 1 - checkedTrials) * sizeof(unsigned),
+                            hipMemcpyDeviceToHost, st),
+             "hipMemcpy");
+    hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+    for (unsigned t = checkedTrials; t <= trial; ++t) raiseErrors(pin[4 + t - checkedTrials]);
+    checkedTrials = trial + 1;
+    // simulation_source_coupling.cpp:46-51.
+    force = pinD[0];
+    work = pinD[3] == 0. ? 0. : (pinD[2] > 0. ? pinD[1] / pinD[2] : pinD[1]);
     if (iterations >= (pinD[3] > 0. ? 2u : 1u) && force < 1e-13 && work < 1e-8) {
       converged = true;
       break;
@@ -1498,8 +1509,13 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
   double* const none[3] = {nullptr, nullptr, nullptr};
   launchElectric(F, F.ptrs(), 0, none, F.window ? F.inletPlane[0] : nullptr, F.window ? F.inletPlane[1] : nullptr);
   if (F.window) F.phase = 3;
-  // The trial end state becomes the particle state.
+  // The trial end state becomes the particle state; the start positions
+  // stay in the end arrays for the step diagnostics.
   for (int a = 0; a < 6; ++a) P.s[a].swap(P.end[a]);
+  C.lastStep.valid = true;
+  C.lastStep.store = &P;
+  C.lastStep.generation = P.generation;
+  for (int a = 0; a < 3; ++a) C.lastStep.origin[a] = origin[a];
   if (report) {
     report->iterations = iterations;
     report->force = force;

@@ -48,6 +48,7 @@ struct BoostDev {
 
 struct ParticleStore {
   std::size_t capacity = 0, count = 0;
+  std::uint64_t generation = 0;  // bumped by every host upload
   // SoA state: x y z ux uy uz q m w, id
   DeviceArray<double> s[9], t[9];
   DeviceArray<uint64_t> id, idT;
@@ -94,6 +95,15 @@ struct CouplingState {
   DeviceArray<unsigned> error;
   PinnedHost pinned;
   Event sorted;
+  // the last coupled step: its store and origin (its start positions stay
+  // in the store's end arrays until the next step)
+  struct {
+    bool valid = false;
+    const ParticleStore* store = nullptr;
+    std::uint64_t generation = 0;
+    double origin[3] = {0.0, 0.0, 0.0};
+  } lastStep;
+  DeviceArray<double> rho[2];  // diagnostic charge densities
 };
 
 }  // namespace aprl_gpu

@@ -148,6 +148,21 @@ class EBFieldGrid:
         check(lib().aprl_fields_synchronize(self.handle))
 
     # -- timing -----------------------------------------------------------
+    def interiorFieldEnergy(self, lo, hi):
+        """Sum of fieldEnergyDensityCell * dV over the cells [lo, hi)
+        (Simulation::localInteriorFieldEnergy, simulation.cpp:3308-3342)."""
+        out = C.c_double()
+        check(lib().aprl_fields_interior_energy(self.handle, _capi.size3(lo), _capi.size3(hi), C.byref(out)))
+        return out.value
+
+    def facePower(self, lo, hi):
+        """Outward Poynting power through the six faces of the cell box
+        [lo, hi) (Simulation::localInteriorBoundaryPower,
+        simulation.cpp:3344-3432): [-x, +x, -y, +y, -z, +z]."""
+        out = np.zeros(6)
+        check(lib().aprl_fields_face_power(self.handle, _capi.size3(lo), _capi.size3(hi), dptr(out)))
+        return out
+
     def profile(self, enable: bool) -> None:
         check(lib().aprl_profile_enable(self.handle, int(enable)))
 

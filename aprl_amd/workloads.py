@@ -72,13 +72,15 @@ BENDING = Workload(
     charge_electrons=100_000.0, dipole_tesla=0.5, dipole_length=0.2, seed=20241, z_cells=150.0)
 
 
-def setup_gpu(w: Workload):
-    """Field grid + window + coupling + particle store for workload w."""
+def setup_gpu(w: Workload, shape: str = "linear-binomial3"):
+    """Field grid + window + coupling + particle store for workload w;
+    shape is "linear-binomial3" (with x-binomial3 smoothing) or
+    "quintic-compensated"."""
     import aprl_amd as A
     g = A.EBGridGeometry(*w.dims, *w.spacing, w.dt)
     fields = A.EBFieldGrid(g)
     window = A.MovingWindowFields(fields, A.EBCPMLParameters((w.cpml, w.cpml, 0)))
-    coupling = A.ConservativeSourceCoupling(fields)
+    coupling = A.ConservativeSourceCoupling(fields, shape)
     coupling.setFrame(w.gamma_boost)
     # lab dipole B_y over z_lab in [-L/2, L/2]
     coupling.addPrescribedLabMagnet("uniform-dipole", [0.0, 0.0, -0.5 * w.dipole_length], w.dipole_length,

@@ -186,6 +186,37 @@ aprl_status aprl_coupling_set_particle_boundary(aprl_coupling* coupling, const s
 aprl_status aprl_coupling_step(aprl_coupling* coupling, aprl_particles* particles, const double* origin,
                                double time, unsigned substeps, aprl_step_report* report);
 
+/* ---------------------------------------------------------------------------
+ * FP64 diagnostics (compensated, fixed-order device reductions).
+ */
+
+/* Field energy: sum of EBFieldGrid::fieldEnergyDensityCell (eb_field.cpp:
+ * 715-738) times dx dy dz over the cells [lo, hi) -- Simulation::
+ * localInteriorFieldEnergy (simulation.cpp:3308-3342) with lo = CPML cells,
+ * hi = n - CPML cells; MovingWindowFields planeEnergy (moving_window_fields.
+ * cpp:387-397) with a one-plane box. */
+aprl_status aprl_fields_interior_energy(aprl_fields* fields, const size_t* lo, const size_t* hi, double* energy);
+/* Outward power through the faces of the cell box [lo, hi) from
+ * poyntingComponentCell (eb_field.cpp:740-782): power[2a] = -sum over the
+ * lower face of axis a, power[2a+1] = +sum over the upper face, times the
+ * face area -- Simulation::localInteriorBoundaryPower (simulation.cpp:
+ * 3344-3432) and MovingWindowFields boundaryPower (moving_window_fields.cpp:
+ * 398-419). */
+aprl_status aprl_fields_face_power(aprl_fields* fields, const size_t* lo, const size_t* hi, double* power);
+/* ChargeConservingCurrentDepositor::maxContinuityResidual (eb_deposition.cpp:
+ * 195-226) of the last aprl_coupling_step of this store, before the window
+ * shift clears J: out[0] = max over vertices of |(rho1 - rho0)/dt + div J|,
+ * out[1] = max |(rho1 - rho0)/dt|, with rho the charge of the coupling's
+ * shape at the step's start and end positions (binomial3-x: the shaped
+ * charge of verification/transverse_binomial_checks.cpp:33-40). */
+aprl_status aprl_coupling_continuity(aprl_coupling* coupling, aprl_particles* particles, double* out);
+/* Gauss residual of eb_initial_field.cpp:774-799 over the vertices strictly
+ * inside [lo, hi] (hi <= (nx, ny, nz)) with the shaped charge of the particles at their current
+ * positions (grid origin `origin`): out = {sum error^2, sum source^2,
+ * max |error|, max |source|}, error = div E - rho / eps0. */
+aprl_status aprl_coupling_gauss(aprl_coupling* coupling, aprl_particles* particles, const double* origin,
+                                const size_t* lo, const size_t* hi, double* out);
+
 /* Per-kernel timing with HIP events on the instance stream. enable=1 starts
  * recording every launch; query returns launches and total milliseconds for
  * a kernel name (0 launches if unknown). */

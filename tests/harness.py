@@ -383,3 +383,68 @@ def splitmix64(v):
     v = ((v ^ (v >> 30)) * 0xBF58476D1CE4E5B9) & _M64
     v = ((v ^ (v >> 27)) * 0x94D049BB133111EB) & _M64
     return v ^ (v >> 31)
+
+
+# Shaped vertex charge of a source shape (the charge whose change the
+# shape's current conserves): binomial3-x -- x nodeWeights x linear y x
+# binomial3 z basis; quintic -- linear x, y x the compensated quintic z basis
+# (longitudinal_shape.h:25-55). Positions relative to the grid origin.
+def quintic_basis(z):
+    cell = np.floor(z)
+    f = z - cell
+    n = [1.0, 0.0, 0.0, 0.0, 0.0, 0.0]
+    for j in range(1, 6):
+        saved = 0.0
+        for k in range(j):
+            t = n[k] / j
+            n[k] = saved + (k + 1 - f) * t
+            saved = (f + j - k - 1) * t
+        n[j] = saved
+    p = 5.0
+    a = p / 6.0
+    b = 3 * p / 40.0 + p * (p - 1) / 72.0
+    d = 5 * p / 112.0 + p * (p - 1) / 80.0 + p * (p - 1) * (p - 2) / 1296.0
+    c = [-d / 64.0, b / 16.0 + 3 * d / 32.0, -a / 4.0 - b / 4.0 - 15 * d / 64.0, 1 + a / 2.0 + 3 * b / 8.0 + 5 * d / 16.0,
+         -a / 4.0 - b / 4.0 - 15 * d / 64.0, b / 16.0 + 3 * d / 32.0, -d / 64.0]
+    v = np.zeros(12)
+    for j in range(6):
+        for k in range(7):
+            v[j + k] += n[j] * c[k]
+    return int(cell) - 5, v
+
+
+def shaped_charge(shape, x, q, dims, h):
+    nx, ny, nz = dims
+    rho = np.zeros((nz + 1, ny + 1, nx + 1))
+    inv = 1.0 / (h[0] * h[1] * h[2])
+    for p in range(len(q)):
+        lx, ly, lz = x[p, 0] / h[0], x[p, 1] / h[1], x[p, 2] / h[2]
+        i, j = int(np.floor(lx)), int(np.floor(ly))
+        fx, fy = lx - i, ly - j
+        if shape == "binomial3":
+            wx, i0 = [0.25 * (1 - fx), 0.5 - 0.25 * fx, 0.25 + 0.25 * fx, 0.25 * fx], i - 1
+            cz = np.floor(lz)
+            fz = lz - cz
+            wz, k0 = [0.25 * (1 - fz), 0.5 * (1 - fz) + 0.25 * fz, 0.25 * (1 - fz) + 0.5 * fz, 0.25 * fz], int(cz) - 1
+        else:
+            wx, i0 = [1 - fx, fx], i
+            k0, wz = quintic_basis(lz)
+        for c, w3 in enumerate(wz):
+            for b, w2 in enumerate([1 - fy, fy]):
+                for a, w1 in enumerate(wx):
+                    kk, jj, ii = k0 + c, j + b, i0 + a
+                    if 0 <= kk <= nz and 0 <= jj <= ny and 0 <= ii <= nx:
+                        rho[kk, jj, ii] += q[p] * inv * w1 * w2 * w3
+    return rho
+
+
+def continuity(rho0, rho1, jx, jy, jz, dt, h):
+    rate = (rho1 - rho0) / dt
+    div = np.zeros_like(rho0)
+    div[:, :, :-1] += jx / h[0]
+    div[:, :, 1:] -= jx / h[0]
+    div[:, :-1, :] += jy / h[1]
+    div[:, 1:, :] -= jy / h[1]
+    div[:-1, :, :] += jz / h[2]
+    div[1:, :, :] -= jz / h[2]
+    return np.abs(rate + div).max(), np.abs(rate).max()
