@@ -7,6 +7,7 @@ using namespace aprl_gpu;
 namespace {
 FieldState& fieldsOf(aprl_fields* f) {
   if (!f) throw std::invalid_argument("Null field grid handle");
+  if (f->fp32) throw std::invalid_argument("This diagnostic entry point takes an APRL_FP64 grid");
   return f->s;
 }
 void checkCoupled(aprl_coupling* c, aprl_particles* p) {

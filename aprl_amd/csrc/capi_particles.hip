@@ -59,6 +59,7 @@ extern "C" {
 aprl_status aprl_particles_create(aprl_fields* fields, size_t capacity, aprl_particles** out) {
   return guarded([&] {
     if (!fields || !out) throw std::invalid_argument("Null field grid or output handle");
+    if (fields->fp32) throw std::invalid_argument("Use aprl_split_particles_create for an APRL_FP32_SPLIT grid");
     std::unique_ptr<aprl_particles> p(new aprl_particles);
     p->fields = fields;
     reserveParticles(p->p, std::max<size_t>(capacity, 1));
@@ -128,6 +129,7 @@ aprl_status aprl_particles_download(aprl_particles* particles, double* x3, doubl
 aprl_status aprl_coupling_create(aprl_fields* fields, aprl_shape shape, int smoothX, aprl_coupling** out) {
   return guarded([&] {
     if (!fields || !out) throw std::invalid_argument("Null field grid or output handle");
+    if (fields->fp32) throw std::invalid_argument("Use aprl_split_coupling_create for an APRL_FP32_SPLIT grid");
     if (shape != APRL_SHAPE_LINEAR_BINOMIAL3 && shape != APRL_SHAPE_QUINTIC_COMPENSATED)
       throw std::invalid_argument("This build implements the quintic-compensated and linear-binomial3 source shapes");
     if (shape == APRL_SHAPE_LINEAR_BINOMIAL3 && !smoothX)

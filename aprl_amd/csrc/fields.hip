@@ -82,7 +82,8 @@ void validateGeometry(const aprl_grid_geometry& g, const CowanCoef& cw) {
       throw std::invalid_argument("Cowan-z coefficients do not satisfy the stable dispersion factorization");
 }
 
-void allocateFields(FieldState& s, std::size_t slack) {
+template <class R>
+void allocateFields(FieldStateT<R>& s, std::size_t slack) {
   const std::size_t cells[3] = {s.g.nx, s.g.ny, s.g.nz};
   for (int a = 0; a < 9; ++a) {
     const bool electric = a < 3 || a >= 6;
@@ -95,14 +96,15 @@ void allocateFields(FieldState& s, std::size_t slack) {
   for (int a = 0; a < 9; ++a) {
     const std::size_t planes = s.dims[a][2] + (s.slides(a) ? slack : 0);
     s.comp[a].resize(planes * s.plane(a));
-    hipCheck(hipMemsetAsync(s.comp[a].get(), 0, s.comp[a].size() * sizeof(double), s.stream), "hipMemsetAsync");
+    hipCheck(hipMemsetAsync(s.comp[a].get(), 0, s.comp[a].size() * sizeof(R), s.stream), "hipMemsetAsync");
     largest = std::max(largest, s.compSize(a));
   }
   if (slack) s.scratch.resize(largest);
 }
 
-FieldPtrs FieldState::ptrs() const {
-  FieldPtrs f{};
+template <class R>
+FieldPtrsT<R> FieldStateT<R>::ptrs() const {
+  FieldPtrsT<R> f{};
   for (int a = 0; a < 3; ++a) {
     f.e[a] = base(a);
     f.b[a] = base(3 + a);
@@ -115,22 +117,26 @@ FieldPtrs FieldState::ptrs() const {
   return f;
 }
 
-StepCoefs FieldState::coefs() const {
-  StepCoefs c{};
-  for (int a = 0; a < 3; ++a) { c.alpha[a] = cw.alpha[a]; c.beta[a] = cw.beta[a]; }
-  c.dYZ = cw.deltaYZ; c.dZX = cw.deltaZX; c.dXY = cw.deltaXY;
-  c.tx = g.dt / g.dx; c.ty = g.dt / g.dy; c.tz = g.dt / g.dz;
+// Coefficients are evaluated in double and rounded once to R.
+template <class R>
+StepCoefsT<R> FieldStateT<R>::coefs() const {
+  StepCoefsT<R> c{};
+  for (int a = 0; a < 3; ++a) { c.alpha[a] = static_cast<R>(cw.alpha[a]); c.beta[a] = static_cast<R>(cw.beta[a]); }
+  c.dYZ = static_cast<R>(cw.deltaYZ); c.dZX = static_cast<R>(cw.deltaZX); c.dXY = static_cast<R>(cw.deltaXY);
+  c.tx = static_cast<R>(g.dt / g.dx); c.ty = static_cast<R>(g.dt / g.dy); c.tz = static_cast<R>(g.dt / g.dz);
   const double curlScale = g.dt / (kMu0 * kEps0);
-  c.cX = curlScale / g.dx; c.cY = curlScale / g.dy; c.cZ = curlScale / g.dz;
-  c.cs = g.dt / kEps0;
+  c.cX = static_cast<R>(curlScale / g.dx); c.cY = static_cast<R>(curlScale / g.dy);
+  c.cZ = static_cast<R>(curlScale / g.dz);
+  c.cs = static_cast<R>(g.dt / kEps0);
   return c;
 }
 
 // Pair order of the checkpoint histories, eb_cpml.cpp:355-357 / 556-558.
 static const int kPairs[6][2] = {{0, 1}, {0, 2}, {1, 2}, {1, 0}, {2, 0}, {2, 1}};
 
-CpmlArgs FieldState::cpmlArgs() const {
-  CpmlArgs P{};
+template <class R>
+CpmlArgsT<R> FieldStateT<R>::cpmlArgs() const {
+  CpmlArgsT<R> P{};
   P.enabled = hasCpml ? 1 : 0;
   P.lowerZ = lowerZ ? 1 : 0;
   P.upperZ = upperZ ? 1 : 0;
@@ -146,13 +152,16 @@ CpmlArgs FieldState::cpmlArgs() const {
                                                                : histLayers[h] * histRow[h]);
   }
   const double curlScale = g.dt / (kMu0 * kEps0);
-  P.magScale[0] = g.dt / g.dx; P.magScale[1] = g.dt / g.dy; P.magScale[2] = g.dt / g.dz;
-  P.eleScale[0] = curlScale / g.dx; P.eleScale[1] = curlScale / g.dy; P.eleScale[2] = curlScale / g.dz;
+  P.magScale[0] = static_cast<R>(g.dt / g.dx); P.magScale[1] = static_cast<R>(g.dt / g.dy);
+  P.magScale[2] = static_cast<R>(g.dt / g.dz);
+  P.eleScale[0] = static_cast<R>(curlScale / g.dx); P.eleScale[1] = static_cast<R>(curlScale / g.dy);
+  P.eleScale[2] = static_cast<R>(curlScale / g.dz);
   return P;
 }
 
 // Offset of a history slot's first stored layer (checkpoint layer 0).
-std::size_t FieldState::historyDeviceOffset(std::size_t h, std::size_t slot) const {
+template <class R>
+std::size_t FieldStateT<R>::historyDeviceOffset(std::size_t h, std::size_t slot) const {
   if (!histSlides[h]) return slot * histLayers[h] * histRow[h];
   return slot * (histLayers[h] + slack) * histRow[h] + zb * histRow[h];
 }
@@ -170,7 +179,8 @@ static double maxRate(const aprl_cpml_parameters& p, int axis, const aprl_grid_g
 
 // EBConvolutionalPML construction, eb_cpml.cpp:195-248 and 297-372; the
 // profiles are evaluated on the host with the same libm calls.
-void installCpml(FieldState& s, const aprl_cpml_parameters& p, std::size_t globalNz, std::size_t zOffset,
+template <class R>
+void installCpml(FieldStateT<R>& s, const aprl_cpml_parameters& p, std::size_t globalNz, std::size_t zOffset,
                  bool lowerZ, bool upperZ) {
   if (!(p.cells[0] > 0 || p.cells[1] > 0 || p.cells[2] > 0))
     throw std::invalid_argument("CPML requires at least one nonzero boundary thickness");
@@ -228,9 +238,13 @@ void installCpml(FieldState& s, const aprl_cpml_parameters& p, std::size_t globa
         }
       }
       activeCount[m][axis] = static_cast<std::size_t>(std::count_if(ordv.begin(), ordv.end(), [](int o) { return o >= 0; }));
+      std::vector<CpmlPointT<R>> ptsR(count);
+      for (std::size_t li = 0; li < count; ++li)
+        ptsR[li] = CpmlPointT<R>{static_cast<R>(pts[li].ikm1), static_cast<R>(pts[li].decay),
+                                 static_cast<R>(pts[li].drive)};
       s.prof[m][axis].resize(count);
       s.ord[m][axis].resize(count);
-      hipCheck(hipMemcpy(s.prof[m][axis].get(), pts.data(), count * sizeof(CpmlPointDev), hipMemcpyHostToDevice),
+      hipCheck(hipMemcpy(s.prof[m][axis].get(), ptsR.data(), count * sizeof(CpmlPointT<R>), hipMemcpyHostToDevice),
                "hipMemcpy");
       hipCheck(hipMemcpy(s.ord[m][axis].get(), ordv.data(), count * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
     }
@@ -252,7 +266,7 @@ void installCpml(FieldState& s, const aprl_cpml_parameters& p, std::size_t globa
     const std::size_t layers = s.histLayers[h] + (s.histSlides[h] ? s.slack : 0);
     s.hist[h].resize(s.histSlots[h] * layers * s.histRow[h]);
     if (s.hist[h].size())
-      hipCheck(hipMemsetAsync(s.hist[h].get(), 0, s.hist[h].size() * sizeof(double), s.stream), "hipMemsetAsync");
+      hipCheck(hipMemsetAsync(s.hist[h].get(), 0, s.hist[h].size() * sizeof(R), s.stream), "hipMemsetAsync");
   }
   s.cpml = p;
   s.hasCpml = true;
@@ -264,10 +278,10 @@ void installCpml(FieldState& s, const aprl_cpml_parameters& p, std::size_t globa
 // Device code.
 
 // EBFieldGrid::electricValue, eb_field.cpp:350-417.
-template <int A>
-__device__ __forceinline__ double evalE(const FieldPtrs& F, int i, int j, int k) {
+template <int A, class R>
+__device__ __forceinline__ R evalE(const FieldPtrsT<R>& F, int i, int j, int k) {
   const int n0 = F.n[A][0], n1 = F.n[A][1], n2 = F.n[A][2];
-  double sign = 1.0;
+  R sign = R(1);
   if (i < 0) {
     if (A == 0) i = 0;
     else { i = -i; sign = -sign; }
@@ -284,7 +298,7 @@ __device__ __forceinline__ double evalE(const FieldPtrs& F, int i, int j, int k)
   }
   if (k < 0 || k >= n2) {
     const bool lower = k < 0;
-    const double* plane = lower ? F.lo[A] : F.up[A];
+    const R* plane = lower ? F.lo[A] : F.up[A];
     if (plane) return sign * plane[static_cast<size_t>(j) * n0 + i];
     if (A == 2) k = lower ? 0 : n2 - 1;
     else { k = lower ? -k : 2 * (n2 - 1) - k; sign = -sign; }
@@ -294,16 +308,16 @@ __device__ __forceinline__ double evalE(const FieldPtrs& F, int i, int j, int k)
 
 // EBFieldGrid::smoothedElectric, eb_field.cpp:419-486 (same accumulation
 // order on the interior fast path and the boundary path).
-template <int A, int D>
-__device__ __forceinline__ double smoothE(const FieldPtrs& F, const StepCoefs& c, int i, int j, int k) {
+template <int A, int D, class R>
+__device__ __forceinline__ R smoothE(const FieldPtrsT<R>& F, const StepCoefsT<R>& c, int i, int j, int k) {
   constexpr int TA = (D + 1) % 3, TB = (D + 2) % 3;
-  const double delta = D == 0 ? c.dYZ : (D == 1 ? c.dZX : c.dXY);
+  const R delta = D == 0 ? c.dYZ : (D == 1 ? c.dZX : c.dXY);
   const int n0 = F.n[A][0], n1 = F.n[A][1], n2 = F.n[A][2];
   if (i > 0 && i < n0 - 1 && j > 0 && j < n1 - 1 && k > 0 && k < n2 - 1) {
     const long long st[3] = {1, n0, static_cast<long long>(n0) * n1};
     const long long oA = st[TA], oB = st[TB];
-    const double* p = F.e[A] + (static_cast<size_t>(k) * n1 + j) * n0 + i;
-    double r = c.alpha[D] * p[0];
+    const R* p = F.e[A] + (static_cast<size_t>(k) * n1 + j) * n0 + i;
+    R r = c.alpha[D] * p[0];
     r += c.beta[TA] * p[-oA];
     r += c.beta[TB] * p[-oB];
     r += c.beta[TA] * p[oA];
@@ -315,7 +329,7 @@ __device__ __forceinline__ double smoothE(const FieldPtrs& F, const StepCoefs& c
     return r;
   }
   int b[3] = {i, j, k};
-  double r = c.alpha[D] * evalE<A>(F, i, j, k);
+  R r = c.alpha[D] * evalE<A>(F, i, j, k);
   for (int dir = -1; dir <= 1; dir += 2) {
     int pA[3] = {b[0], b[1], b[2]}, pB[3] = {b[0], b[1], b[2]};
     pA[TA] += dir;
@@ -333,16 +347,16 @@ __device__ __forceinline__ double smoothE(const FieldPtrs& F, const StepCoefs& c
   return r;
 }
 
-template <int A>
-__device__ __forceinline__ double rawE(const FieldPtrs& F, int i, int j, int k) {
+template <int A, class R>
+__device__ __forceinline__ R rawE(const FieldPtrsT<R>& F, int i, int j, int k) {
   return F.e[A][(static_cast<size_t>(k) * F.n[A][1] + j) * F.n[A][0] + i];
 }
 
 // Index of a CPML history entry of target (M, T) and derivative D at node
 // (i, j, k), CPMLMemoryTerm::value (eb_cpml.cpp:78-89) with slid layers.
-template <int M, int T, int D>
-__device__ __forceinline__ double* histAt(const CpmlArgs& P, int o, int i, int j, int k, int dim0) {
-  double* h = P.hist[M][T][D] + static_cast<long long>(o) * P.ostride[M][T][D];
+template <int M, int T, int D, class R>
+__device__ __forceinline__ R* histAt(const CpmlArgsT<R>& P, int o, int i, int j, int k, int dim0) {
+  R* h = P.hist[M][T][D] + static_cast<long long>(o) * P.ostride[M][T][D];
   if (D == 0) return h + static_cast<long long>(P.zb + k) * P.row[M][T][D] + j;
   if (D == 1) return h + static_cast<long long>(P.zb + k) * P.row[M][T][D] + i;
   return h + static_cast<long long>(j) * dim0 + i;
@@ -350,9 +364,9 @@ __device__ __forceinline__ double* histAt(const CpmlArgs& P, int o, int i, int j
 
 // transformedCorrection, eb_cpml.cpp:131-138. WRITE selects whether the
 // advanced history is stored (preview keeps the old one).
-template <bool WRITE>
-__device__ __forceinline__ double cpmlCorrection(const CpmlPointDev& c, double raw, double* hist) {
-  const double h = c.decay * (*hist) + c.drive * raw;
+template <bool WRITE, class R>
+__device__ __forceinline__ R cpmlCorrection(const CpmlPointT<R>& c, R raw, R* hist) {
+  const R h = c.decay * (*hist) + c.drive * raw;
   if (WRITE) *hist = h;
   return c.ikm1 * raw + h;
 }
@@ -361,57 +375,61 @@ __device__ __forceinline__ double cpmlCorrection(const CpmlPointDev& c, double r
 // (eb_field.cpp:488-558) / advanceMagneticYee (311-348) followed by
 // EBConvolutionalPML::afterMagneticUpdate (406-468). dA/dB are the two curl
 // differences of the component; the CPML terms use the same differences.
-__device__ __forceinline__ size_t bIndex(const FieldPtrs& F, int a, int i, int j, int k) {
+template <class R>
+__device__ __forceinline__ size_t bIndex(const FieldPtrsT<R>& F, int a, int i, int j, int k) {
   return (static_cast<size_t>(k) * F.n[3 + a][1] + j) * F.n[3 + a][0] + i;
 }
-__device__ __forceinline__ void finishBx(FieldPtrs& F, const StepCoefs& c, const CpmlArgs& P, int i, int j, int k,
-                                         double dEzDy, double dEyDz, const double* old = nullptr) {
+template <class R>
+__device__ __forceinline__ void finishBx(FieldPtrsT<R>& F, const StepCoefsT<R>& c, const CpmlArgsT<R>& P, int i, int j, int k,
+                                         R dEzDy, R dEyDz, const R* old = nullptr) {
   const int nx = F.n[0][0], ny = F.n[1][1];
   const size_t idx = (static_cast<size_t>(k) * ny + j) * (nx + 1) + i;
-  double b = old ? *old : F.b[0][idx];
+  R b = old ? *old : F.b[0][idx];
   b -= c.ty * dEzDy - c.tz * dEyDz;
   if (P.enabled) {
     int o = P.ord[0][1][j];
-    if (o >= 0) b += (-1.0 * P.magScale[1]) * cpmlCorrection<true>(P.prof[0][1][j], dEzDy, histAt<0, 0, 1>(P, o, i, j, k, nx + 1));
+    if (o >= 0) b += (R(-1) * P.magScale[1]) * cpmlCorrection<true>(P.prof[0][1][j], dEzDy, histAt<0, 0, 1>(P, o, i, j, k, nx + 1));
     o = P.ord[0][2][k];
-    if (o >= 0) b += (1.0 * P.magScale[2]) * cpmlCorrection<true>(P.prof[0][2][k], dEyDz, histAt<0, 0, 2>(P, o, i, j, k, nx + 1));
+    if (o >= 0) b += (R(1) * P.magScale[2]) * cpmlCorrection<true>(P.prof[0][2][k], dEyDz, histAt<0, 0, 2>(P, o, i, j, k, nx + 1));
   }
   F.b[0][idx] = b;
 }
-__device__ __forceinline__ void finishBy(FieldPtrs& F, const StepCoefs& c, const CpmlArgs& P, int i, int j, int k,
-                                         double dExDz, double dEzDx, const double* old = nullptr) {
+template <class R>
+__device__ __forceinline__ void finishBy(FieldPtrsT<R>& F, const StepCoefsT<R>& c, const CpmlArgsT<R>& P, int i, int j, int k,
+                                         R dExDz, R dEzDx, const R* old = nullptr) {
   const int nx = F.n[0][0], ny = F.n[1][1];
   const size_t idx = (static_cast<size_t>(k) * (ny + 1) + j) * nx + i;
-  double b = old ? *old : F.b[1][idx];
+  R b = old ? *old : F.b[1][idx];
   b -= c.tz * dExDz - c.tx * dEzDx;
   if (P.enabled) {
     int o = P.ord[0][2][k];
-    if (o >= 0) b += (-1.0 * P.magScale[2]) * cpmlCorrection<true>(P.prof[0][2][k], dExDz, histAt<0, 1, 2>(P, o, i, j, k, nx));
+    if (o >= 0) b += (R(-1) * P.magScale[2]) * cpmlCorrection<true>(P.prof[0][2][k], dExDz, histAt<0, 1, 2>(P, o, i, j, k, nx));
     o = P.ord[0][0][i];
-    if (o >= 0) b += (1.0 * P.magScale[0]) * cpmlCorrection<true>(P.prof[0][0][i], dEzDx, histAt<0, 1, 0>(P, o, i, j, k, nx));
+    if (o >= 0) b += (R(1) * P.magScale[0]) * cpmlCorrection<true>(P.prof[0][0][i], dEzDx, histAt<0, 1, 0>(P, o, i, j, k, nx));
   }
   F.b[1][idx] = b;
 }
-__device__ __forceinline__ void finishBz(FieldPtrs& F, const StepCoefs& c, const CpmlArgs& P, int i, int j, int k,
-                                         double dEyDx, double dExDy, const double* old = nullptr) {
+template <class R>
+__device__ __forceinline__ void finishBz(FieldPtrsT<R>& F, const StepCoefsT<R>& c, const CpmlArgsT<R>& P, int i, int j, int k,
+                                         R dEyDx, R dExDy, const R* old = nullptr) {
   const int nx = F.n[0][0], ny = F.n[1][1];
   const size_t idx = (static_cast<size_t>(k) * ny + j) * nx + i;
-  double b = old ? *old : F.b[2][idx];
+  R b = old ? *old : F.b[2][idx];
   b -= c.tx * dEyDx - c.ty * dExDy;
   if (P.enabled) {
     int o = P.ord[0][0][i];
-    if (o >= 0) b += (-1.0 * P.magScale[0]) * cpmlCorrection<true>(P.prof[0][0][i], dEyDx, histAt<0, 2, 0>(P, o, i, j, k, nx));
+    if (o >= 0) b += (R(-1) * P.magScale[0]) * cpmlCorrection<true>(P.prof[0][0][i], dEyDx, histAt<0, 2, 0>(P, o, i, j, k, nx));
     o = P.ord[0][1][j];
-    if (o >= 0) b += (1.0 * P.magScale[1]) * cpmlCorrection<true>(P.prof[0][1][j], dExDy, histAt<0, 2, 1>(P, o, i, j, k, nx));
+    if (o >= 0) b += (R(1) * P.magScale[1]) * cpmlCorrection<true>(P.prof[0][1][j], dExDy, histAt<0, 2, 1>(P, o, i, j, k, nx));
   }
   F.b[2][idx] = b;
 }
 
 // One thread per node over the union of the three B extents (Yee; and the
 // Cowan edge nodes Bx(i = nx), By(j = ny) when EDGES).
-template <bool COWAN>
-__device__ __forceinline__ void magneticBx(FieldPtrs& F, const StepCoefs& c, const CpmlArgs& P, int i, int j, int k) {
-  double dEzDy, dEyDz;
+template <bool COWAN, class R>
+__device__ __forceinline__ void magneticBx(FieldPtrsT<R>& F, const StepCoefsT<R>& c, const CpmlArgsT<R>& P, int i, int j, int k) {
+  R dEzDy, dEyDz;
   if (COWAN) {
     dEzDy = smoothE<2, 1>(F, c, i, j + 1, k) - smoothE<2, 1>(F, c, i, j, k);
     dEyDz = smoothE<1, 2>(F, c, i, j, k + 1) - smoothE<1, 2>(F, c, i, j, k);
@@ -421,9 +439,9 @@ __device__ __forceinline__ void magneticBx(FieldPtrs& F, const StepCoefs& c, con
   }
   finishBx(F, c, P, i, j, k, dEzDy, dEyDz);
 }
-template <bool COWAN>
-__device__ __forceinline__ void magneticBy(FieldPtrs& F, const StepCoefs& c, const CpmlArgs& P, int i, int j, int k) {
-  double dExDz, dEzDx;
+template <bool COWAN, class R>
+__device__ __forceinline__ void magneticBy(FieldPtrsT<R>& F, const StepCoefsT<R>& c, const CpmlArgsT<R>& P, int i, int j, int k) {
+  R dExDz, dEzDx;
   if (COWAN) {
     dExDz = smoothE<0, 2>(F, c, i, j, k + 1) - smoothE<0, 2>(F, c, i, j, k);
     dEzDx = smoothE<2, 0>(F, c, i + 1, j, k) - smoothE<2, 0>(F, c, i, j, k);
@@ -434,7 +452,8 @@ __device__ __forceinline__ void magneticBy(FieldPtrs& F, const StepCoefs& c, con
   finishBy(F, c, P, i, j, k, dExDz, dEzDx);
 }
 
-__global__ void __launch_bounds__(256) kMagneticYee(FieldPtrs F, StepCoefs c, CpmlArgs P) {
+template <class R>
+__global__ void __launch_bounds__(256) kMagneticYee(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P) {
   const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = t % (nx + 1), j = t / (nx + 1);
@@ -447,7 +466,8 @@ __global__ void __launch_bounds__(256) kMagneticYee(FieldPtrs F, StepCoefs c, Cp
 }
 
 // Cowan edge nodes the tiles do not cover: Bx at i = nx and By at j = ny.
-__global__ void __launch_bounds__(256) kMagneticCowanEdges(FieldPtrs F, StepCoefs c, CpmlArgs P) {
+template <class R>
+__global__ void __launch_bounds__(256) kMagneticCowanEdges(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P) {
   const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < ny * nz) {
@@ -471,15 +491,15 @@ constexpr int kStageR = (kStageN + kMX * kMY - 1) / (kMX * kMY);
 
 // Smoothed sample from the staged planes; (li, lj) local.
 // pl[0] = plane k-1, pl[1] = plane k, pl[2] = plane k+1; D is the derivative axis.
-template <int D>
-__device__ __forceinline__ double smoothL(const double* const pl[3], const StepCoefs& c, int li, int lj) {
+template <int D, class R>
+__device__ __forceinline__ R smoothL(const R* const pl[3], const StepCoefsT<R>& c, int li, int lj) {
   constexpr int TA = (D + 1) % 3, TB = (D + 2) % 3;
-  const double delta = D == 0 ? c.dYZ : (D == 1 ? c.dZX : c.dXY);
+  const R delta = D == 0 ? c.dYZ : (D == 1 ? c.dZX : c.dXY);
   auto at = [&](int dx, int dy, int dz) { return pl[1 + dz][(lj + dy) * kLX + li + dx]; };
   auto off = [&](int axis, int dir, int& dx, int& dy, int& dz) {
     if (axis == 0) dx += dir; else if (axis == 1) dy += dir; else dz += dir;
   };
-  double r = c.alpha[D] * at(0, 0, 0);
+  R r = c.alpha[D] * at(0, 0, 0);
   for (int dir = -1; dir <= 1; dir += 2) {
     int ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
     off(TA, dir, ax, ay, az);
@@ -497,8 +517,8 @@ __device__ __forceinline__ double smoothL(const double* const pl[3], const StepC
   return r;
 }
 
-template <int A>
-__device__ __forceinline__ double stagedE(const FieldPtrs& F, int i, int j, int kk) {
+template <int A, class R>
+__device__ __forceinline__ R stagedE(const FieldPtrsT<R>& F, int i, int j, int kk) {
   const int n0 = F.n[A][0], n1 = F.n[A][1], n2 = F.n[A][2];
   if (kk >= 0 && kk < n2 && i >= 0 && i < n0 && j >= 0 && j < n1)
     return F.e[A][(static_cast<size_t>(kk) * n1 + j) * n0 + i];
@@ -506,14 +526,15 @@ __device__ __forceinline__ double stagedE(const FieldPtrs& F, int i, int j, int 
   return 0.0;  // more than one node beyond the array: feeds no B node
 }
 
-__global__ void __launch_bounds__(kMX * kMY) kMagneticCowanTiled(FieldPtrs F, StepCoefs c, CpmlArgs P) {
-  __shared__ double buf[kStageN];  // [component][plane k-1, k, k+1][kLY][kLX]
+template <class R>
+__global__ void __launch_bounds__(kMX * kMY) kMagneticCowanTiled(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P) {
+  __shared__ R buf[kStageN];  // [component][plane k-1, k, k+1][kLY][kLX]
   const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
   const int i0 = blockIdx.x * kMX, j0 = blockIdx.y * kMY, k = blockIdx.z;
   const int tx = static_cast<int>(threadIdx.x) % kMX, ty = static_cast<int>(threadIdx.x) / kMX;
   const int i = i0 + tx, j = j0 + ty;
   {
-    double v[kStageR];
+    R v[kStageR];
 #pragma unroll
     for (int r = 0; r < kStageR; ++r) {
       const int t = static_cast<int>(threadIdx.x) + r * kMX * kMY;
@@ -533,16 +554,16 @@ __global__ void __launch_bounds__(kMX * kMY) kMagneticCowanTiled(FieldPtrs F, St
   }
   __syncthreads();
   if (i >= nx || j >= ny) return;
-  const double* ex[3] = {buf, buf + kLP, buf + 2 * kLP};
-  const double* ey[3] = {buf + 3 * kLP, buf + 4 * kLP, buf + 5 * kLP};
-  const double* ez[3] = {buf + 6 * kLP, buf + 7 * kLP, buf + 8 * kLP};
+  const R* ex[3] = {buf, buf + kLP, buf + 2 * kLP};
+  const R* ey[3] = {buf + 3 * kLP, buf + 4 * kLP, buf + 5 * kLP};
+  const R* ez[3] = {buf + 6 * kLP, buf + 7 * kLP, buf + 8 * kLP};
   const int li = tx + 1, lj = ty + 1;
   if (k < nz) {
     // S_z is x-y smoothing: only its own plane is read
-    const double* eyk[3] = {ey[1], ey[1], ey[1]};
-    const double* eyk1[3] = {ey[2], ey[2], ey[2]};
-    const double* exk[3] = {ex[1], ex[1], ex[1]};
-    const double* exk1[3] = {ex[2], ex[2], ex[2]};
+    const R* eyk[3] = {ey[1], ey[1], ey[1]};
+    const R* eyk1[3] = {ey[2], ey[2], ey[2]};
+    const R* exk[3] = {ex[1], ex[1], ex[1]};
+    const R* exk1[3] = {ex[2], ex[2], ex[2]};
     finishBx(F, c, P, i, j, k, smoothL<1>(ez, c, li, lj + 1) - smoothL<1>(ez, c, li, lj),
              smoothL<2>(eyk1, c, li, lj) - smoothL<2>(eyk, c, li, lj));
     finishBy(F, c, P, i, j, k, smoothL<2>(exk1, c, li, lj) - smoothL<2>(exk, c, li, lj),
@@ -552,7 +573,8 @@ __global__ void __launch_bounds__(kMX * kMY) kMagneticCowanTiled(FieldPtrs F, St
            smoothL<1>(ex, c, li, lj + 1) - smoothL<1>(ex, c, li, lj));
 }
 
-__device__ __forceinline__ double rd(const double* p, int nx, int ny, int i, int j, int k) {
+template <class R>
+__device__ __forceinline__ R rd(const R* p, int nx, int ny, int i, int j, int k) {
   return p[(static_cast<size_t>(k) * ny + j) * nx + i];
 }
 
@@ -562,10 +584,10 @@ __device__ __forceinline__ double rd(const double* p, int nx, int ny, int i, int
 // and the moving-window inlet overwrite (moving_window_fields.cpp:276-280).
 // MODE 0 writes E in place and advances the histories; MODE 1 writes the
 // zero-current preview into `out` and leaves the histories untouched.
-template <int MODE>
-__global__ void __launch_bounds__(256) kElectric(FieldPtrs F, StepCoefs c, CpmlArgs P, double* ox, double* oy,
-                                                  double* oz, double* sx, double* sy, double* sz, const double* inletX,
-                                                  const double* inletY) {
+template <int MODE, class R>
+__global__ void __launch_bounds__(256) kElectric(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P, R* ox, R* oy,
+                                                  R* oz, R* sx, R* sy, R* sz, const R* inletX,
+                                                  const R* inletY) {
   // one thread per node of the (nx+1) x (ny+1) plane k = blockIdx.y
   const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -573,25 +595,25 @@ __global__ void __launch_bounds__(256) kElectric(FieldPtrs F, StepCoefs c, CpmlA
   const int k = blockIdx.y;
   if (j > ny || k > nz) return;
   constexpr bool W = MODE == 0;
-  const double* bx = F.b[0];
-  const double* by = F.b[1];
-  const double* bz = F.b[2];
+  const R* bx = F.b[0];
+  const R* by = F.b[1];
+  const R* bz = F.b[2];
   const int kb = P.lowerZ ? 1 : 0, ke = P.upperZ ? nz : nz + 1;
   if (i < nx) {  // Ex: nx x (ny+1) x (nz+1)
     const size_t idx = (static_cast<size_t>(k) * (ny + 1) + j) * nx + i;
-    const double e0 = F.e[0][idx];
-    double e = e0;
+    const R e0 = F.e[0][idx];
+    R e = e0;
     if (k >= 1 && k < nz && j >= 1 && j < ny)
       e += c.cY * (rd(bz, nx, ny, i, j, k) - rd(bz, nx, ny, i, j - 1, k)) -
            c.cZ * (rd(by, nx, ny + 1, i, j, k) - rd(by, nx, ny + 1, i, j, k - 1)) - c.cs * F.j[0][idx];
     if (P.enabled && j >= 1 && j < ny && k >= kb && k < ke) {
       int o = P.ord[1][1][j];
       if (o >= 0)
-        e += (1.0 * P.eleScale[1]) * cpmlCorrection<W>(P.prof[1][1][j], rd(bz, nx, ny, i, j, k) - rd(bz, nx, ny, i, j - 1, k),
+        e += (R(1) * P.eleScale[1]) * cpmlCorrection<W>(P.prof[1][1][j], rd(bz, nx, ny, i, j, k) - rd(bz, nx, ny, i, j - 1, k),
                                                         histAt<1, 0, 1>(P, o, i, j, k, nx));
       o = P.ord[1][2][k];
       if (o >= 0)
-        e += (-1.0 * P.eleScale[2]) * cpmlCorrection<W>(P.prof[1][2][k], rd(by, nx, ny + 1, i, j, k) - rd(by, nx, ny + 1, i, j, k - 1),
+        e += (R(-1) * P.eleScale[2]) * cpmlCorrection<W>(P.prof[1][2][k], rd(by, nx, ny + 1, i, j, k) - rd(by, nx, ny + 1, i, j, k - 1),
                                                          histAt<1, 0, 2>(P, o, i, j, k, nx));
     }
     if (j == 0 || j == ny || (P.lowerZ && k == 0) || (P.upperZ && k == nz)) e = 0.0;
@@ -601,19 +623,19 @@ __global__ void __launch_bounds__(256) kElectric(FieldPtrs F, StepCoefs c, CpmlA
   }
   if (j < ny) {  // Ey: (nx+1) x ny x (nz+1)
     const size_t idx = (static_cast<size_t>(k) * ny + j) * (nx + 1) + i;
-    const double e0 = F.e[1][idx];
-    double e = e0;
+    const R e0 = F.e[1][idx];
+    R e = e0;
     if (k >= 1 && k < nz && i >= 1 && i < nx)
       e += c.cZ * (rd(bx, nx + 1, ny, i, j, k) - rd(bx, nx + 1, ny, i, j, k - 1)) -
            c.cX * (rd(bz, nx, ny, i, j, k) - rd(bz, nx, ny, i - 1, j, k)) - c.cs * F.j[1][idx];
     if (P.enabled && i >= 1 && i < nx && k >= kb && k < ke) {
       int o = P.ord[1][2][k];
       if (o >= 0)
-        e += (1.0 * P.eleScale[2]) * cpmlCorrection<W>(P.prof[1][2][k], rd(bx, nx + 1, ny, i, j, k) - rd(bx, nx + 1, ny, i, j, k - 1),
+        e += (R(1) * P.eleScale[2]) * cpmlCorrection<W>(P.prof[1][2][k], rd(bx, nx + 1, ny, i, j, k) - rd(bx, nx + 1, ny, i, j, k - 1),
                                                         histAt<1, 1, 2>(P, o, i, j, k, nx + 1));
       o = P.ord[1][0][i];
       if (o >= 0)
-        e += (-1.0 * P.eleScale[0]) * cpmlCorrection<W>(P.prof[1][0][i], rd(bz, nx, ny, i, j, k) - rd(bz, nx, ny, i - 1, j, k),
+        e += (R(-1) * P.eleScale[0]) * cpmlCorrection<W>(P.prof[1][0][i], rd(bz, nx, ny, i, j, k) - rd(bz, nx, ny, i - 1, j, k),
                                                          histAt<1, 1, 0>(P, o, i, j, k, nx + 1));
     }
     if (i == 0 || i == nx || (P.lowerZ && k == 0) || (P.upperZ && k == nz)) e = 0.0;
@@ -623,8 +645,8 @@ __global__ void __launch_bounds__(256) kElectric(FieldPtrs F, StepCoefs c, CpmlA
   }
   if (k < nz) {  // Ez: (nx+1) x (ny+1) x nz
     const size_t idx = (static_cast<size_t>(k) * (ny + 1) + j) * (nx + 1) + i;
-    const double e0 = F.e[2][idx];
-    double e = e0;
+    const R e0 = F.e[2][idx];
+    R e = e0;
     const bool in = i >= 1 && i < nx && j >= 1 && j < ny;
     if (in)
       e += c.cX * (rd(by, nx, ny + 1, i, j, k) - rd(by, nx, ny + 1, i - 1, j, k)) -
@@ -632,11 +654,11 @@ __global__ void __launch_bounds__(256) kElectric(FieldPtrs F, StepCoefs c, CpmlA
     if (P.enabled && in) {
       int o = P.ord[1][0][i];
       if (o >= 0)
-        e += (1.0 * P.eleScale[0]) * cpmlCorrection<W>(P.prof[1][0][i], rd(by, nx, ny + 1, i, j, k) - rd(by, nx, ny + 1, i - 1, j, k),
+        e += (R(1) * P.eleScale[0]) * cpmlCorrection<W>(P.prof[1][0][i], rd(by, nx, ny + 1, i, j, k) - rd(by, nx, ny + 1, i - 1, j, k),
                                                         histAt<1, 2, 0>(P, o, i, j, k, nx + 1));
       o = P.ord[1][1][j];
       if (o >= 0)
-        e += (-1.0 * P.eleScale[1]) * cpmlCorrection<W>(P.prof[1][1][j], rd(bx, nx + 1, ny, i, j, k) - rd(bx, nx + 1, ny, i, j - 1, k),
+        e += (R(-1) * P.eleScale[1]) * cpmlCorrection<W>(P.prof[1][1][j], rd(bx, nx + 1, ny, i, j, k) - rd(bx, nx + 1, ny, i, j - 1, k),
                                                          histAt<1, 2, 1>(P, o, i, j, k, nx + 1));
     }
     if (i == 0 || i == nx || j == 0 || j == ny) e = 0.0;
@@ -645,38 +667,42 @@ __global__ void __launch_bounds__(256) kElectric(FieldPtrs F, StepCoefs c, CpmlA
   }
 }
 
-__global__ void kFill(double* p, size_t n, double v) {
+template <class R>
+__global__ void kFill(R* p, size_t n, R v) {
   for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < n;
        t += static_cast<size_t>(gridDim.x) * blockDim.x)
     p[t] = v;
 }
 
-static void fill(hipStream_t s, double* p, size_t n, double v) {
+template <class R>
+static void fill(hipStream_t s, R* p, size_t n, R v) {
   if (!n) return;
   const unsigned blocks = std::min<size_t>((n + 255) / 256, 4096);
-  kFill<<<blocks, 256, 0, s>>>(p, n, v);
+  kFill<R><<<blocks, 256, 0, s>>>(p, n, v);
   launchCheck("fill");
 }
 
-void launchMagnetic(FieldState& s, const FieldPtrs& f) {
-  const StepCoefs c = s.coefs();
-  const CpmlArgs P = s.cpmlArgs();
+template <class R>
+void launchMagnetic(FieldStateT<R>& s, const FieldPtrsT<R>& f) {
+  const StepCoefsT<R> c = s.coefs();
+  const CpmlArgsT<R> P = s.cpmlArgs();
   ScopedKernel sk(s.profiler, s.stream, "magnetic");
   if (s.g.solver == 1) {
     dim3 tg(ceilDiv(s.g.nx, kMX), ceilDiv(s.g.ny, kMY), static_cast<unsigned>(s.g.nz + 1));
-    kMagneticCowanTiled<<<tg, kMX * kMY, 0, s.stream>>>(f, c, P);
-    kMagneticCowanEdges<<<ceilDiv((s.g.nx + s.g.ny) * s.g.nz, 256), 256, 0, s.stream>>>(f, c, P);
+    kMagneticCowanTiled<R><<<tg, kMX * kMY, 0, s.stream>>>(f, c, P);
+    kMagneticCowanEdges<R><<<ceilDiv((s.g.nx + s.g.ny) * s.g.nz, 256), 256, 0, s.stream>>>(f, c, P);
   } else {
     dim3 grid(ceilDiv((s.g.nx + 1) * (s.g.ny + 1), 256), static_cast<unsigned>(s.g.nz + 1));
-    kMagneticYee<<<grid, 256, 0, s.stream>>>(f, c, P);
+    kMagneticYee<R><<<grid, 256, 0, s.stream>>>(f, c, P);
   }
   launchCheck("magnetic");
 }
 
-void launchElectric(FieldState& s, const FieldPtrs& f, int mode, double* const out[3], const double* inletX,
-                    const double* inletY, double* const sum[3]) {
-  const StepCoefs c = s.coefs();
-  CpmlArgs P = s.cpmlArgs();
+template <class R>
+void launchElectric(FieldStateT<R>& s, const FieldPtrsT<R>& f, int mode, R* const out[3], const R* inletX,
+                    const R* inletY, R* const sum[3]) {
+  const StepCoefsT<R> c = s.coefs();
+  CpmlArgsT<R> P = s.cpmlArgs();
   if (!s.hasCpml) {  // PerfectElectricConductorBoundary(lowerZ, upperZ)
     P.lowerZ = s.lowerZ;
     P.upperZ = s.upperZ;
@@ -685,11 +711,11 @@ void launchElectric(FieldState& s, const FieldPtrs& f, int mode, double* const o
   dim3 grid(ceilDiv((s.g.nx + 1) * (s.g.ny + 1), 256), static_cast<unsigned>(s.g.nz + 1));
   if (mode == 0) {
     ScopedKernel sk(s.profiler, s.stream, "electric");
-    kElectric<0><<<grid, block, 0, s.stream>>>(f, c, P, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, inletX,
+    kElectric<0, R><<<grid, block, 0, s.stream>>>(f, c, P, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, inletX,
                                                inletY);
   } else {
     ScopedKernel sk(s.profiler, s.stream, "electric_preview");
-    kElectric<1><<<grid, block, 0, s.stream>>>(f, c, P, out[0], out[1], out[2], sum ? sum[0] : nullptr,
+    kElectric<1, R><<<grid, block, 0, s.stream>>>(f, c, P, out[0], out[1], out[2], sum ? sum[0] : nullptr,
                                                sum ? sum[1] : nullptr, sum ? sum[2] : nullptr, inletX, inletY);
   }
   launchCheck("electric");
@@ -697,8 +723,9 @@ void launchElectric(FieldState& s, const FieldPtrs& f, int mode, double* const o
 
 // Zero the owned J nodes of every listed cell tile (the tiles with an
 // occupied neighbour tile: the only nodes a tile deposition can write).
-__global__ void __launch_bounds__(256) kClearCurrentTiles(double* jx, double* jy, double* jz, FieldPtrs F,
-                                                          FieldState::CurrentFootprint fp) {
+template <class R>
+__global__ void __launch_bounds__(256) kClearCurrentTiles(R* jx, R* jy, R* jz, FieldPtrsT<R> F,
+                                                          CurrentFootprint fp) {
   const int tile = fp.active[blockIdx.x];
   const int tx = tile % fp.ntx, ty = (tile / fp.ntx) % fp.nty, tz = tile / (fp.ntx * fp.nty);
   for (int d = 0; d < 3; ++d) {
@@ -707,7 +734,7 @@ __global__ void __launch_bounds__(256) kClearCurrentTiles(double* jx, double* jy
     const int j0 = ty * fp.ty, j1 = ty == fp.nty - 1 ? n1 : min(n1, j0 + fp.ty);
     const int k0 = tz * fp.tz, k1 = tz == fp.ntz - 1 ? n2 : min(n2, k0 + fp.tz);
     const int wx = i1 - i0, wy = j1 - j0, wz = k1 - k0;
-    double* dst = d == 0 ? jx : (d == 1 ? jy : jz);
+    R* dst = d == 0 ? jx : (d == 1 ? jy : jz);
     for (int t = threadIdx.x; t < wx * wy * wz; t += blockDim.x) {
       const int i = i0 + t % wx, j = j0 + (t / wx) % wy, k = k0 + t / (wx * wy);
       dst[(static_cast<size_t>(k) * n1 + j) * n0 + i] = 0.0;
@@ -715,30 +742,33 @@ __global__ void __launch_bounds__(256) kClearCurrentTiles(double* jx, double* jy
   }
 }
 
-static void clearCurrentNow(FieldState& s) {
+template <class R>
+static void clearCurrentNow(FieldStateT<R>& s) {
   if (s.footprint.valid) {
-    const FieldState::CurrentFootprint& fp = s.footprint;
+    const CurrentFootprint& fp = s.footprint;
     if (fp.count > 0) {
-      kClearCurrentTiles<<<fp.count, 256, 0, s.stream>>>(s.base(6), s.base(7), s.base(8), s.ptrs(), fp);
+      kClearCurrentTiles<R><<<fp.count, 256, 0, s.stream>>>(s.base(6), s.base(7), s.base(8), s.ptrs(), fp);
       launchCheck("clear current tiles");
     }
   } else {
-    for (int a = 6; a < 9; ++a) fill(s.stream, s.comp[a].get(), s.compSize(a), 0.0);
+    for (int a = 6; a < 9; ++a) fill(s.stream, s.comp[a].get(), s.compSize(a), R(0));
   }
   s.currentZero = true;
   s.footprint.valid = false;
 }
 
-void launchClearCurrent(FieldState& s) {
+template <class R>
+void launchClearCurrent(FieldStateT<R>& s) {
   if (s.currentZero) return;  // J already holds zeros
   ScopedKernel sk(s.profiler, s.stream, "clear_current");
   clearCurrentNow(s);
 }
 
 // Recenter the slid arrays into their slack once the slack is used up.
-static void recenter(FieldState& s) {
+template <class R>
+static void recenter(FieldStateT<R>& s) {
   for (int a = 0; a < 6; ++a) {
-    const size_t bytes = s.compSize(a) * sizeof(double);
+    const size_t bytes = s.compSize(a) * sizeof(R);
     hipCheck(hipMemcpyAsync(s.scratch.get(), s.base(a), bytes, hipMemcpyDeviceToDevice, s.stream), "hipMemcpyAsync");
     hipCheck(hipMemcpyAsync(s.comp[a].get() + s.slack * s.plane(a), s.scratch.get(), bytes, hipMemcpyDeviceToDevice,
                             s.stream),
@@ -748,11 +778,11 @@ static void recenter(FieldState& s) {
     if (!s.histSlides[h] || !s.hist[h].size()) continue;
     const size_t block = s.histLayers[h] * s.histRow[h];
     for (size_t slot = 0; slot < s.histSlots[h]; ++slot) {
-      double* slotBase = s.hist[h].get() + slot * (s.histLayers[h] + s.slack) * s.histRow[h];
-      hipCheck(hipMemcpyAsync(s.scratch.get(), slotBase + s.zb * s.histRow[h], block * sizeof(double),
+      R* slotBase = s.hist[h].get() + slot * (s.histLayers[h] + s.slack) * s.histRow[h];
+      hipCheck(hipMemcpyAsync(s.scratch.get(), slotBase + s.zb * s.histRow[h], block * sizeof(R),
                               hipMemcpyDeviceToDevice, s.stream),
                "hipMemcpyAsync");
-      hipCheck(hipMemcpyAsync(slotBase + s.slack * s.histRow[h], s.scratch.get(), block * sizeof(double),
+      hipCheck(hipMemcpyAsync(slotBase + s.slack * s.histRow[h], s.scratch.get(), block * sizeof(R),
                               hipMemcpyDeviceToDevice, s.stream),
                "hipMemcpyAsync");
     }
@@ -760,18 +790,20 @@ static void recenter(FieldState& s) {
   s.zb = s.slack;
 }
 
+template <class R>
 struct ShiftArgs {
-  double* dst[6];
-  const double* src[6];
+  R* dst[6];
+  const R* src[6];
   int planeLen[6];
-  double* hist[12];
+  R* hist[12];
   long long slotStride[12];
   int row[12], slots[12];
 };
 
 // Insert the new k = 0 planes of the six E/B components and zero the new
 // layer of every transverse CPML history, in one launch.
-__global__ void kShiftInsert(ShiftArgs A) {
+template <class R>
+__global__ void kShiftInsert(ShiftArgs<R> A) {
   const int item = blockIdx.y;  // 0..5 component planes, 6.. history slots
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (item < 6) {
@@ -791,11 +823,12 @@ __global__ void kShiftInsert(ShiftArgs A) {
 // MovingWindowFields::Impl::shift (moving_window_fields.cpp:439-518) with
 // resetCurrent: slide every E/B component and transverse history one plane,
 // insert the reference planes (zero histories) at k = 0, clear J.
-void windowShift(FieldState& s, const double* const planes[6]) {
+template <class R>
+void windowShift(FieldStateT<R>& s, const R* const planes[6]) {
   ScopedKernel sk(s.profiler, s.stream, "shift");
   if (s.zb == 0) recenter(s);
   s.zb -= 1;
-  ShiftArgs A{};
+  ShiftArgs<R> A{};
   int maxLen = 0, items = 6;
   for (int a = 0; a < 6; ++a) {
     A.dst[a] = s.base(a);
@@ -813,9 +846,23 @@ void windowShift(FieldState& s, const double* const planes[6]) {
     maxLen = std::max(maxLen, A.row[h]);
   }
   dim3 grid(ceilDiv(maxLen, 256), static_cast<unsigned>(items));
-  kShiftInsert<<<grid, 256, 0, s.stream>>>(A);
+  kShiftInsert<R><<<grid, 256, 0, s.stream>>>(A);
   launchCheck("shift insert");
   if (!s.currentZero) clearCurrentNow(s);
 }
+
+#define APRL_FIELDS_INSTANTIATE(R)                                                                            \
+  template void allocateFields<R>(FieldStateT<R>&, std::size_t);                                              \
+  template void installCpml<R>(FieldStateT<R>&, const aprl_cpml_parameters&, std::size_t, std::size_t, bool, \
+                               bool);                                                                         \
+  template struct FieldStateT<R>;                                                                            \
+  template void launchMagnetic<R>(FieldStateT<R>&, const FieldPtrsT<R>&);                                     \
+  template void launchElectric<R>(FieldStateT<R>&, const FieldPtrsT<R>&, int, R* const*, const R*, const R*,  \
+                                  R* const*);                                                                 \
+  template void launchClearCurrent<R>(FieldStateT<R>&);                                                       \
+  template void windowShift<R>(FieldStateT<R>&, const R* const*);
+APRL_FIELDS_INSTANTIATE(double)
+APRL_FIELDS_INSTANTIATE(float)
+#undef APRL_FIELDS_INSTANTIATE
 
 }  // namespace aprl_gpu

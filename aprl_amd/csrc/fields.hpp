@@ -1,4 +1,5 @@
-// Device-resident Yee field state and its host-side bookkeeping.
+// Device-resident Yee field state and its host-side bookkeeping, for the
+// plain FP64 form (R = double) and the FP32 form (R = float).
 //
 // E and B components carry `slack` spare z planes in front of the window so
 // that the moving-window shift (moving_window_fields.cpp:482-498) becomes a
@@ -16,34 +17,48 @@ struct CowanCoef {
 };
 CowanCoef cowanForZDispersion(double dx, double dy, double dz);
 
-struct CpmlPointDev { double ikm1, decay, drive; };
+template <class R>
+struct CpmlPointT { R ikm1, decay, drive; };
 
 // Kernel-side view of the field arrays (base pointers already slid).
-struct FieldPtrs {
-  double* e[3];
-  double* b[3];
-  double* j[3];
-  int n[9][3];          // (nx, ny, nz) per component
-  const double* lo[3];  // lower E halo planes (null => PEC parity)
-  const double* up[3];  // upper E halo planes
+template <class R>
+struct FieldPtrsT {
+  R* e[3];
+  R* b[3];
+  R* j[3];
+  int n[9][3];     // (nx, ny, nz) per component
+  const R* lo[3];  // lower E halo planes (null => PEC parity)
+  const R* up[3];  // upper E halo planes
 };
 
-struct StepCoefs {
-  double alpha[3], beta[3], dYZ, dZX, dXY;
-  double tx, ty, tz;        // dt/dx, dt/dy, dt/dz
-  double cX, cY, cZ, cs;    // dt/(mu0 eps0 d*), dt/eps0
+template <class R>
+struct StepCoefsT {
+  R alpha[3], beta[3], dYZ, dZX, dXY;
+  R tx, ty, tz;      // dt/dx, dt/dy, dt/dz
+  R cX, cY, cZ, cs;  // dt/(mu0 eps0 d*), dt/eps0
 };
 
-struct CpmlArgs {
+template <class R>
+struct CpmlArgsT {
   int enabled;
   int lowerZ, upperZ;
-  int zb;                                   // slid layer base of transverse histories
-  const CpmlPointDev* prof[2][3];           // [magnetic/electric][axis]
-  const int* ord[2][3];                     // ordinal of an active index, -1 if inactive
-  double* hist[2][3][3];                    // [mag/ele][target comp][derivative]
-  long long ostride[2][3][3];               // stride between ordinals
-  int row[2][3][3];                         // row length of a layer
-  double magScale[3], eleScale[3];
+  int zb;                               // slid layer base of transverse histories
+  const CpmlPointT<R>* prof[2][3];      // [magnetic/electric][axis]
+  const int* ord[2][3];                 // ordinal of an active index, -1 if inactive
+  R* hist[2][3][3];                     // [mag/ele][target comp][derivative]
+  long long ostride[2][3][3];           // stride between ordinals
+  int row[2][3][3];                     // row length of a layer
+  R magScale[3], eleScale[3];
+};
+
+// Tile footprint of the last deposit: J is zero outside the listed cell
+// tiles (those with an occupied tile within a deposit's reach), so
+// clearing it touches only those.
+struct CurrentFootprint {
+  bool valid = false;
+  const int* active = nullptr;
+  int count = 0;
+  int ntx = 0, nty = 0, ntz = 0, tx = 0, ty = 0, tz = 0;
 };
 
 // Device scratch of the diagnostics.
@@ -52,47 +67,41 @@ struct DiagnosticScratch {
   DeviceArray<unsigned long long> acc, maxima;
 };
 
-struct FieldState {
+template <class R>
+struct FieldStateT {
   aprl_grid_geometry g{};
   aprl_precision precision = APRL_FP64;
   CowanCoef cw{};
   std::size_t dims[9][3]{};
   std::size_t slack = 0, zb = 0;
-  DeviceArray<double> comp[9];
-  DeviceArray<double> scratch;
+  DeviceArray<R> comp[9];
+  DeviceArray<R> scratch;
 
   // boundary
   bool hasCpml = false, lowerZ = true, upperZ = true;
   aprl_cpml_parameters cpml{};
-  DeviceArray<CpmlPointDev> prof[2][3];
+  DeviceArray<CpmlPointT<R>> prof[2][3];
   DeviceArray<int> ord[2][3];
-  DeviceArray<double> hist[12];
+  DeviceArray<R> hist[12];
   std::size_t histSlots[12]{}, histLayers[12]{}, histRow[12]{};
   bool histSlides[12]{};
 
   // halos and moving window
-  DeviceArray<double> haloLo[3], haloUp[3], zeroPlane[3];
+  DeviceArray<R> haloLo[3], haloUp[3], zeroPlane[3];
   bool window = false;
   std::size_t step = 0;
   std::ptrdiff_t offset = 0;
   bool established = false;
   int phase = 0;  // 0 Idle, 1 Magnetic, 2 Electric, 3 Complete
   bool inletSet = false;
-  DeviceArray<double> inlet[2], shiftPlanes[6];
-  const double* loPlane[3] = {nullptr, nullptr, nullptr};  // lower E halo planes in use
-  const double* inletPlane[2] = {nullptr, nullptr};         // inlet Ex, Ey planes in use
+  DeviceArray<R> inlet[2], shiftPlanes[6];
+  const R* loPlane[3] = {nullptr, nullptr, nullptr};  // lower E halo planes in use
+  const R* inletPlane[2] = {nullptr, nullptr};         // inlet Ex, Ey planes in use
 
   // True while J is known to hold zeros on the device (set by clears and
   // shifts, dropped by any write of J), so repeated clears cost nothing.
   bool currentZero = true;
-  // Tile footprint of the last deposit: J is zero outside the cell tiles
-  // with an occupied neighbour tile, so clearing it touches only those.
-  struct CurrentFootprint {
-    bool valid = false;
-    const int* active = nullptr;  // tiles with an occupied neighbour tile
-    int count = 0;
-    int ntx = 0, nty = 0, ntz = 0, tx = 0, ty = 0, tz = 0;
-  } footprint;
+  CurrentFootprint footprint;
 
   hipStream_t stream = nullptr;
   Profiler profiler;
@@ -101,27 +110,40 @@ struct FieldState {
   std::size_t compSize(int a) const { return dims[a][0] * dims[a][1] * dims[a][2]; }
   std::size_t plane(int a) const { return dims[a][0] * dims[a][1]; }
   bool slides(int a) const { return a < 6; }
-  double* base(int a) const { return comp[a].get() + (slides(a) ? zb * plane(a) : 0); }
+  R* base(int a) const { return comp[a].get() + (slides(a) ? zb * plane(a) : 0); }
 
-  FieldPtrs ptrs() const;
-  StepCoefs coefs() const;
-  CpmlArgs cpmlArgs() const;
+  FieldPtrsT<R> ptrs() const;
+  StepCoefsT<R> coefs() const;
+  CpmlArgsT<R> cpmlArgs() const;
   std::size_t historyDeviceOffset(std::size_t h, std::size_t slot) const;
 };
 
+using CpmlPointDev = CpmlPointT<double>;
+using FieldPtrs = FieldPtrsT<double>;
+using StepCoefs = StepCoefsT<double>;
+using CpmlArgs = CpmlArgsT<double>;
+using FieldState = FieldStateT<double>;
+using FieldState32 = FieldStateT<float>;
+
 void validateGeometry(const aprl_grid_geometry& g, const CowanCoef& cw);
-void allocateFields(FieldState& s, std::size_t slack);
-void installCpml(FieldState& s, const aprl_cpml_parameters& p, std::size_t globalNz, std::size_t zOffset,
+template <class R>
+void allocateFields(FieldStateT<R>& s, std::size_t slack);
+template <class R>
+void installCpml(FieldStateT<R>& s, const aprl_cpml_parameters& p, std::size_t globalNz, std::size_t zOffset,
                  bool lowerZ, bool upperZ);
 
 // Stream-ordered field operations.
-void launchMagnetic(FieldState& s, const FieldPtrs& f);
+template <class R>
+void launchMagnetic(FieldStateT<R>& s, const FieldPtrsT<R>& f);
 // mode 0: in place with history advance; mode 1: preview into `out` without
-// touching histories; the window inlet overwrite applies in both modes.
-// mode 1 with sum != nullptr also writes sum = preview + old E.
-void launchElectric(FieldState& s, const FieldPtrs& f, int mode, double* const out[3],
-                    const double* inletX, const double* inletY, double* const sum[3] = nullptr);
-void launchClearCurrent(FieldState& s);
-void windowShift(FieldState& s, const double* const planes[6]);
+// touching histories (and, when sum != nullptr, sum = preview + old E); the
+// window inlet overwrite applies in both modes.
+template <class R>
+void launchElectric(FieldStateT<R>& s, const FieldPtrsT<R>& f, int mode, R* const out[3], const R* inletX,
+                    const R* inletY, R* const sum[3] = nullptr);
+template <class R>
+void launchClearCurrent(FieldStateT<R>& s);
+template <class R>
+void windowShift(FieldStateT<R>& s, const R* const planes[6]);
 
 }  // namespace aprl_gpu

@@ -4,7 +4,9 @@
 #include "particles.hpp"
 
 struct aprl_fields {
-  aprl_gpu::FieldState s;
+  bool fp32 = false;
+  aprl_gpu::FieldState s;      // plain FP64 form
+  aprl_gpu::FieldState32 s32;  // FP32 form
 };
 struct aprl_particles {
   aprl_fields* fields;

@@ -67,14 +67,22 @@ def component_shape(g: EBGridGeometry, a: int) -> tuple:
     return (s[2], s[1], s[0])
 
 
-class EBFieldGrid:
-    """Device-resident Yee grid (EBFieldGrid, eb_field.h:201-357)."""
+PRECISIONS = {"fp64": 0, "fp32-split": 1}
 
-    def __init__(self, geometry: EBGridGeometry):
+
+class EBFieldGrid:
+    """Device-resident Yee grid (EBFieldGrid, eb_field.h:201-357).
+
+    precision "fp64" is the plain form; "fp32-split" keeps fields, currents
+    and CPML histories in FP32 (host arrays stay float64 and are rounded on
+    upload)."""
+
+    def __init__(self, geometry: EBGridGeometry, precision: str = "fp64"):
         self.geometry = geometry
+        self.precision = precision
         self._g = geometry.c_struct()
         h = C.c_void_p()
-        check(lib().aprl_fields_create(C.byref(self._g), 0, C.byref(h)))
+        check(lib().aprl_fields_create(C.byref(self._g), PRECISIONS[precision], C.byref(h)))
         self.handle = h
 
     def close(self):
