@@ -15,7 +15,7 @@
 #include <cstring>
 #include <limits>
 
-#include "particles.hpp"
+#include "particle_device.hpp"
 
 namespace aprl_gpu {
 
@@ -49,118 +49,8 @@ struct PArgs {
   int n;
 };
 
-struct StepArgs {
-  double time, dt;
-  int substeps;
-  BoostDev frame;
-  int nMagnets;
-  MagnetDev mag[kMaxMagnets];
-  // particle boundary geometry (ParticleCPMLRegion / ParticleOpenBoundary)
-  double physLo[3], physHi[3], cpmlTol[3];
-  double outLo[3], outHi[3], outTol[3];
-  int cpmlCells[3];
-  int hasExit, window;
-  double exitLabZ;
-  double extent[3], posTol[3];  // validatePosition
-};
-
 // ---------------------------------------------------------------------------
 // Device math restated from the reference.
-
-__device__ __forceinline__ double gammaFromU(double ux, double uy, double uz) {
-  const double scale = fmax(1.0, fmax(fabs(ux), fmax(fabs(uy), fabs(uz))));
-  const double inv = 1.0 / scale;
-  const double a = ux * inv, b = uy * inv, c = uz * inv;
-  return scale * sqrt(inv * inv + a * a + b * b + c * c);
-}
-
-// SIMagneticElement::fieldsLab, eb_sources.cpp:346-446 (E is zero).
-__device__ void magnetLab(const MagnetDev& M, const double pos[3], double b[3]) {
-  b[0] = b[1] = b[2] = 0.0;
-  const double rel0 = pos[0] - M.center[0], rel1 = pos[1] - M.center[1], z = pos[2] - M.center[2];
-  if (M.type == 2) {
-    if (z >= 0.0 && z <= M.length) {
-      const double x = rel0 * M.cosA + rel1 * M.sinA;
-      const double y = -rel0 * M.sinA + rel1 * M.cosA;
-      const double bx = M.gradient * y, by = M.gradient * x;
-      b[0] = bx * M.cosA - by * M.sinA;
-      b[1] = bx * M.sinA + by * M.cosA;
-    }
-    return;
-  }
-  if (M.type == 1) {
-    if (z >= 0.0 && z <= M.length) {
-      b[0] = M.peak * M.cosA;
-      b[1] = M.peak * M.sinA;
-    }
-    return;
-  }
-  const double twoPi = 2.0 * 3.141592653589793238462643383279502884;
-  double ts = 0.0, ls = 0.0;
-  if (z >= 0.0 && z <= M.length) {
-    ts = sin(M.waveNumber * z);
-    ls = cos(M.waveNumber * z);
-    if (M.phase != 0.0) {
-      ts = sin(M.waveNumber * z + M.phase);
-      ls = cos(M.waveNumber * z + M.phase);
-    }
-    if (M.endRampPeriods > 0.0) {
-      const double ramp = M.endRampPeriods * M.period;
-      const double distance = fmin(z, M.length - z);
-      if (distance < ramp) {
-        const double ph = 0.5 * twoPi * distance / ramp;
-        const double envelope = 0.5 * (1.0 - cos(ph));
-        const double derivative = 0.25 * twoPi / ramp * sin(ph) * (z < ramp ? 1.0 : -1.0);
-        ts = envelope * ts - derivative / M.waveNumber * ls;
-        ls *= envelope;
-      }
-    }
-  } else if (M.gaussianFringe) {
-    const double fc = z < 0.0 ? z : z - M.length;
-    const double distance = fabs(fc);
-    if (distance >= M.fringeExtent) return;
-    const double q = distance / M.fringeExtent;
-    const double q2 = q * q, q3 = q2 * q, q4 = q3 * q, q5 = q4 * q;
-    const double taper = 1.0 - 10.0 * q3 + 15.0 * q4 - 6.0 * q5;
-    const double taperD = (-30.0 * q2 + 60.0 * q3 - 30.0 * q4) / M.fringeExtent;
-    const double gaussian = exp(-0.5 * pow(M.waveNumber * distance, 2.0));
-    ls = gaussian * taper;
-    const double dd = gaussian * (taperD - M.waveNumber * M.waveNumber * distance * taper);
-    ts = z < 0.0 ? dd / M.waveNumber : -dd / M.waveNumber;
-  } else {
-    return;
-  }
-  const double tc = rel0 * M.cosA + rel1 * M.sinA;
-  const double tf = M.peak * cosh(M.waveNumber * tc) * ts;
-  b[0] = tf * M.cosA;
-  b[1] = tf * M.sinA;
-  b[2] = M.peak * sinh(M.waveNumber * tc) * ls;
-}
-
-// SIFieldSourceSet::addPrescribedBox (eb_sources.cpp:571-582) with the
-// boxToLab / fieldsLabToBox transforms (boostframe.cpp:188-196, 248-265).
-__device__ void prescribedBox(const StepArgs& S, const double posBox[3], double tBox, double e[3], double b[3]) {
-  if (S.nMagnets == 0) return;
-  const BoostDev& F = S.frame;
-  double posLab[3] = {posBox[0], posBox[1], posBox[2]};
-  const double zShift = posBox[2] - F.boxZ0;
-  posLab[2] = F.labZ0 + fma(F.gammaBeta * F.c0, tBox, F.gamma * zShift);
-  double eL[3] = {0.0, 0.0, 0.0}, bL[3] = {0.0, 0.0, 0.0};
-  for (int n = 0; n < S.nMagnets; ++n) {
-    double cb[3];
-    magnetLab(S.mag[n], posLab, cb);
-    bL[0] += cb[0]; bL[1] += cb[1]; bL[2] += cb[2];
-    eL[0] += 0.0; eL[1] += 0.0; eL[2] += 0.0;
-  }
-  double eb[3], bb[3];
-  eb[0] = fma(-F.gammaBeta * F.c0, bL[1], F.gamma * eL[0]);
-  eb[1] = fma(F.gammaBeta * F.c0, bL[0], F.gamma * eL[1]);
-  eb[2] = eL[2];
-  bb[0] = fma(F.gammaBeta / F.c0, eL[1], F.gamma * bL[0]);
-  bb[1] = fma(-F.gammaBeta / F.c0, eL[0], F.gamma * bL[1]);
-  bb[2] = bL[2];
-  for (int a = 0; a < 3; ++a) { e[a] += eb[a]; b[a] += bb[a]; }
-}
 
 // source_coupling_math::momentum, source_coupling_math.h:272-293.
 __device__ void momentum(double u[3], const double e[3], const double mag[3], double q, double m, double h,
@@ -1202,7 +1092,7 @@ void reserveParticles(ParticleStore& P, std::size_t capacity) {
   P.sortTemp.resize(P.sortTempBytes);
 }
 
-static void raiseErrors(unsigned e) {
+void raiseParticleErrors(unsigned e) {
   if (!e) return;
   if (e & kErrNonfinite) throw std::runtime_error("Nonfinite open coupling orbit");
   if (e & kErrUpstream)
@@ -1225,17 +1115,17 @@ static void raiseErrors(unsigned e) {
   throw std::runtime_error("Unknown device error");
 }
 
-static StepArgs stepArgs(const CouplingState& C, const FieldState& F, const double origin[3], double time,
-                         unsigned substeps) {
+StepArgs stepArgsFor(const CouplingState& C, const aprl_grid_geometry& g, bool window, const double origin[3],
+                     double time, unsigned substeps) {
   StepArgs S{};
   S.time = time;
-  S.dt = F.g.dt;
+  S.dt = g.dt;
   S.substeps = static_cast<int>(substeps);
   S.frame = C.frame;
   S.nMagnets = static_cast<int>(C.magnets.size());
   for (int n = 0; n < S.nMagnets; ++n) S.mag[n] = C.magnets[n];
-  const double sp[3] = {F.g.dx, F.g.dy, F.g.dz};
-  const std::size_t cells[3] = {F.g.nx, F.g.ny, F.g.nz};
+  const double sp[3] = {g.dx, g.dy, g.dz};
+  const std::size_t cells[3] = {g.nx, g.ny, g.nz};
   const double eps = std::numeric_limits<double>::epsilon();
   for (int a = 0; a < 3; ++a) {
     const double lo = origin[a], hi = origin[a] + static_cast<double>(cells[a]) * sp[a];
@@ -1250,7 +1140,7 @@ static StepArgs stepArgs(const CouplingState& C, const FieldState& F, const doub
     S.posTol[a] = 64.0 * eps * std::max(1.0, std::max(std::abs(origin[a]), S.extent[a]));
   }
   S.hasExit = C.hasExit ? 1 : 0;
-  S.window = F.window ? 1 : 0;
+  S.window = window ? 1 : 0;
   S.exitLabZ = C.exitLabZ;
   return S;
 }
@@ -1297,21 +1187,68 @@ static void sortAndChunk(CouplingState& C, ParticleStore& P, const GArgs& G, int
   launchCheck("permute");
   for (int a = 0; a < 9; ++a) P.s[a].swap(P.t[a]);
   P.id.swap(P.idT);
-  kTileRanges<<<ceilDiv(ntiles, 256), 256, 0, st>>>(P.keySorted.get(), n, ntiles, C.tileBegin.get(),
-                                                     C.tileEnd.get(), occupied);
+  const int rz = C.shape == kShapeQuintic ? Box<kShapeQuintic>::RZ : Box<kShapeBinomial3X>::RZ;
+  buildChunkLists(C, st, P.keySorted.get(), n, ntx, nty, ntiles, rz, occupied);
+}
+
+// Tile ranges of the sorted keys, chunk counts and the active flags, their
+// scan, the chunk list and the active-tile list (all in tile order).
+void buildChunkLists(CouplingState& C, hipStream_t st, const unsigned* keySorted, int n, int ntx, int nty,
+                     int ntiles, int rz, unsigned* occupied) {
+  kTileRanges<<<ceilDiv(ntiles, 256), 256, 0, st>>>(keySorted, n, ntiles, C.tileBegin.get(), C.tileEnd.get(),
+                                                     occupied);
   launchCheck("tile ranges");
   const int ntz = ntiles / (ntx * nty);
-  const int rz = C.shape == kShapeQuintic ? Box<kShapeQuintic>::RZ : Box<kShapeBinomial3X>::RZ;
   kTileCounts<<<ceilDiv(ntiles + 1, 256), 256, 0, st>>>(C.tileBegin.get(), C.tileEnd.get(), ntx, nty, ntz, rz,
                                                          C.tileCounts.get());
   launchCheck("tile counts");
-  bytes = C.scanTempBytes;
+  std::size_t bytes = C.scanTempBytes;
   hipCheck(rocprim::exclusive_scan(C.scanTemp.get(), bytes, C.tileCounts.get(), C.tileScan.get(), 0ull,
                                    static_cast<std::size_t>(ntiles) + 1, rocprim::plus<unsigned long long>(), st),
            "exclusive_scan");
   kChunkList<<<ceilDiv(ntiles, 256), 256, 0, st>>>(C.tileBegin.get(), C.tileEnd.get(), C.tileCounts.get(),
                                                     C.tileScan.get(), ntiles, C.chunks.get(), C.activeTiles.get());
   launchCheck("chunk list");
+}
+
+// Tile arrays sized for ntiles and a chunk list for n particles.
+void reserveTileLists(CouplingState& C, hipStream_t st, int ntiles, int n) {
+  if (C.tileBegin.size() != static_cast<std::size_t>(ntiles)) {
+    C.tileBegin.resize(ntiles);
+    C.tileEnd.resize(ntiles);
+    C.tileShift.resize(ntiles);
+    C.activeTiles.resize(ntiles);
+    C.tileCounts.resize(ntiles + 1);
+    C.tileScan.resize(ntiles + 1);
+    C.scanTempBytes = 0;
+    hipCheck(rocprim::exclusive_scan(nullptr, C.scanTempBytes, C.tileCounts.get(), C.tileScan.get(), 0ull,
+                                     static_cast<std::size_t>(ntiles) + 1, rocprim::plus<unsigned long long>(), st),
+             "exclusive_scan size");
+    C.scanTempBytes = std::max<std::size_t>(C.scanTempBytes, 256);
+    C.scanTemp.resize(C.scanTempBytes);
+    C.chunkCapacity = 0;
+  }
+  const std::size_t chunkBound = chunkListBound(ntiles, n);
+  if (chunkBound > C.chunkCapacity) {
+    C.chunks.resize(std::max<std::size_t>(chunkBound, 1));
+    C.chunkCapacity = chunkBound;
+  }
+}
+
+// At most one partial chunk per occupied tile plus the full ones.
+std::size_t chunkListBound(int ntiles, int n) {
+  return (static_cast<std::size_t>(n) + kChunk - 1) / kChunk +
+         std::min<std::size_t>(static_cast<std::size_t>(ntiles), static_cast<std::size_t>(n));
+}
+
+void fillInts(hipStream_t st, int* p, int n, int v) {
+  kFillInt<<<ceilDiv(n ? n : 1, 256), 256, 0, st>>>(p, n, v);
+  launchCheck("fill");
+}
+
+void reduceChunkStats(hipStream_t st, const double* in, int chunks, double* out) {
+  kReduceBlocks<<<1, 256, 0, st>>>(in, chunks, out);
+  launchCheck("chunk stats");
 }
 
 void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], double time, unsigned substeps,
@@ -1336,29 +1273,8 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
   const int nty = static_cast<int>((F.g.ny + kTileY - 1) / kTileY);
   const int ntz = static_cast<int>((F.g.nz + kTileZ - 1) / kTileZ);
   const int ntiles = ntx * nty * ntz;
-  if (C.tileBegin.size() != static_cast<std::size_t>(ntiles)) {
-    C.tileBegin.resize(ntiles);
-    C.tileEnd.resize(ntiles);
-    C.tileShift.resize(ntiles);
-    C.activeTiles.resize(ntiles);
-    C.tileCounts.resize(ntiles + 1);
-    C.tileScan.resize(ntiles + 1);
-    C.scanTempBytes = 0;
-    hipCheck(rocprim::exclusive_scan(nullptr, C.scanTempBytes, C.tileCounts.get(), C.tileScan.get(), 0ull,
-                                     static_cast<std::size_t>(ntiles) + 1, rocprim::plus<unsigned long long>(), st),
-             "exclusive_scan size");
-    C.scanTempBytes = std::max<std::size_t>(C.scanTempBytes, 256);
-    C.scanTemp.resize(C.scanTempBytes);
-    C.chunkCapacity = 0;
-  }
-  // Chunk list capacity: at most one partial chunk per occupied tile plus
-  // the full ones.
-  const std::size_t chunkBound = (static_cast<std::size_t>(n) + kChunk - 1) / kChunk +
-                                 std::min<std::size_t>(static_cast<std::size_t>(ntiles), static_cast<std::size_t>(n));
-  if (chunkBound > C.chunkCapacity) {
-    C.chunks.resize(std::max<std::size_t>(chunkBound, 1));
-    C.chunkCapacity = chunkBound;
-  }
+  reserveTileLists(C, st, ntiles, n);
+  const std::size_t chunkBound = chunkListBound(ntiles, n);
   // error words: [0] step level, [1] occupied-tile counter, [2 + t] the
   // kernels of Picard trial t (checked in trial order)
   if (!C.error.size()) { C.error.resize(2 + 64); C.reduceOut.resize(4); }
@@ -1402,7 +1318,7 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
   if (C.reduceBlocks.size() < 4 * static_cast<std::size_t>(std::max(1u, nChunks)))
     C.reduceBlocks.resize(4 * static_cast<std::size_t>(std::max(1u, nChunks)));
 
-  const StepArgs S = stepArgs(C, F, origin, time, substeps);
+  const StepArgs S = stepArgsFor(C, F.g, F.window, origin, time, substeps);
   PArgs A = particleArgs(P);
   A.scaleDen = F.g.dx * F.g.dy * F.g.dz * F.g.dt;
   bool converged = false;
@@ -1489,7 +1405,7 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
                             hipMemcpyDeviceToHost, st),
              "hipMemcpy");
     hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
-    for (unsigned t = checkedTrials; t <= trial; ++t) raiseErrors(pin[4 + t - checkedTrials]);
+    for (unsigned t = checkedTrials; t <= trial; ++t) raiseParticleErrors(pin[4 + t - checkedTrials]);
     checkedTrials = trial + 1;
     // simulation_source_coupling.cpp:46-51.
     force = pinD[0];

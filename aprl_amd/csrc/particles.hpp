@@ -106,4 +106,13 @@ struct CouplingState {
   DeviceArray<double> rho[2];  // diagnostic charge densities
 };
 
+// Host helpers shared by the FP64 and FP32-split coupling steps.
+void buildChunkLists(CouplingState& C, hipStream_t st, const unsigned* keySorted, int n, int ntx, int nty,
+                     int ntiles, int rz, unsigned* occupied);
+void reserveTileLists(CouplingState& C, hipStream_t st, int ntiles, int n);
+std::size_t chunkListBound(int ntiles, int n);
+void fillInts(hipStream_t st, int* p, int n, int v);
+void reduceChunkStats(hipStream_t st, const double* in, int chunks, double* out);
+void raiseParticleErrors(unsigned e);
+
 }  // namespace aprl_gpu
