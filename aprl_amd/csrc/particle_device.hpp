@@ -122,4 +122,99 @@ __device__ void prescribedBox(const StepArgs& S, const double posBox[3], double 
   for (int a = 0; a < 3; ++a) { e[a] += eb[a]; b[a] += bb[a]; }
 }
 
+// source_coupling_math::momentum, source_coupling_math.h:272-293.
+__device__ void momentum(double u[3], const double e[3], const double mag[3], double q, double m, double h,
+                         double g0, unsigned& err) {
+  const double before[3] = {u[0], u[1], u[2]};
+  const double ks = q * h / (2 * m * kC);
+  const double kick[3] = {e[0] * ks, e[1] * ks, e[2] * ks};
+  const double bs = q * h / (2 * m);
+  const double b[3] = {mag[0] * bs, mag[1] * bs, mag[2] * bs};
+  const double a[3] = {before[0] + kick[0], before[1] + kick[1], before[2] + kick[2]};
+  const double axb[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+  const double b2 = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+  const double ka = kick[0] * a[0] + kick[1] * a[1] + kick[2] * a[2];
+  const double kb = kick[0] * b[0] + kick[1] * b[1] + kick[2] * b[2];
+  const double ab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+  const double kcross = kick[0] * axb[0] + kick[1] * axb[1] + kick[2] * axb[2];
+  const double first = (kick[0] * before[0] + kick[1] * before[1] + kick[2] * before[2]) / g0;
+  const double kk = kick[0] * kick[0] + kick[1] * kick[1] + kick[2] * kick[2];
+  double G = g0 + first + (kk + kcross / g0 - first * first) / g0;
+  G = fmax(G, .5 * (g0 + 1.));
+  const double eps = 2.220446049250313e-16;
+  int iteration = 0;
+  for (; iteration < 12; ++iteration) {
+    const double den = G * G + b2, numerator = G * G * ka + G * kcross + ab * kb;
+    const double fv = G * (G - g0) - numerator / den;
+    const double derivative = 2 * G - g0 - ((2 * G * ka + kcross) * den - 2 * G * numerator) / (den * den);
+    const double correction = fv / derivative;
+    const double next = G - correction;
+    if (!isfinite(next) || next < .5 * (g0 + 1.) - 1e-13) {
+      err |= kErrRootNonphysical;
+      return;
+    }
+    G = fmax(next, .5 * (g0 + 1.));
+    if (fabs(correction) <= 8 * eps * fmax(1., G)) break;
+  }
+  if (iteration == 12) {
+    err |= kErrRootNoConverge;
+    return;
+  }
+  const double den = G * G + b2;
+  for (int k = 0; k < 3; ++k) u[k] = 2 * (G * G * a[k] + ab * b[k] + G * axb[k]) / den - before[k];
+}
+
+// Push of source_coupling.cpp:90-104 for one particle over one step with the
+// gathered fields usedE / usedB (fixed over the step): the substeps with
+// prescribed magnets at the explicit midpoint, then the particle-boundary
+// and domain checks of the deposition loop against the grid origin o.
+// Positions are absolute box coordinates in metres.
+__device__ void pushChord(const StepArgs& S, const double o[3], const double x0[3], const double u0[3], double q,
+                          double m, const double usedE[3], const double usedB[3], double x[3], double u[3],
+                          double& absGrid, unsigned& err) {
+  if (S.window && x0[2] < o[2]) err |= kErrUpstream;
+  for (int a = 0; a < 3; ++a) { x[a] = x0[a]; u[a] = u0[a]; }
+  const double h = S.dt / S.substeps;
+  absGrid = 0.0;
+  for (int n = 0; n < S.substeps; ++n) {
+    const double bx[3] = {x[0], x[1], x[2]}, bu[3] = {u[0], u[1], u[2]};
+    const double g0 = gammaFromU(u[0], u[1], u[2]);
+    double point[3] = {x[0], x[1], x[2]};
+    const double ps = .5 * h * kC / g0;
+    for (int a = 0; a < 3; ++a) point[a] += ps * u[a];
+    double ee[3] = {0., 0., 0.}, bb[3] = {0., 0., 0.};
+    prescribedBox(S, point, S.time + (n + .5) * h, ee, bb);
+    double e[3] = {usedE[0], usedE[1], usedE[2]}, b[3] = {usedB[0], usedB[1], usedB[2]};
+    for (int a = 0; a < 3; ++a) { e[a] += ee[a]; b[a] += bb[a]; }
+    momentum(u, e, b, q, m, h, g0, err);
+    const double g1 = gammaFromU(u[0], u[1], u[2]);
+    double mean[3] = {bu[0], bu[1], bu[2]};
+    for (int a = 0; a < 3; ++a) mean[a] += u[a];
+    const double xs = h * kC / (g0 + g1);
+    for (int a = 0; a < 3; ++a) x[a] += xs * mean[a];
+    double w = 0.0;
+    for (int a = 0; a < 3; ++a) w += q * usedE[a] * (x[a] - bx[a]);
+    absGrid += fabs(w);
+  }
+  // Boundary events of the deposition loop (simulation.cpp:5229-5245).
+  for (int a = 0; a < 3; ++a) {
+    if (!isfinite(x0[a]) || !isfinite(x[a])) err |= kErrNonfinite;
+    if (x0[a] < S.physLo[a] - S.cpmlTol[a] || x0[a] > S.physHi[a] + S.cpmlTol[a]) err |= kErrStartInCpml;
+    if (S.cpmlCells[a] && (x[a] < S.physLo[a] || x[a] > S.physHi[a])) err |= kErrBoundaryEvent;
+    if (x0[a] < S.outLo[a] - S.outTol[a] || x0[a] > S.outHi[a] + S.outTol[a]) err |= kErrStartOutside;
+    if (x[a] < S.outLo[a] || x[a] > S.outHi[a]) err |= kErrBoundaryEvent;
+    const double rel0 = x0[a] - o[a], rel1 = x[a] - o[a];
+    if (rel0 < -S.posTol[a] || rel0 > S.extent[a] + S.posTol[a] || rel1 < -S.posTol[a] ||
+        rel1 > S.extent[a] + S.posTol[a])
+      err |= kErrEndpointOutside;
+  }
+  if (S.hasExit) {
+    const BoostDev& F = S.frame;
+    const double za = F.labZ0 + fma(F.gammaBeta * F.c0, S.time, F.gamma * (x0[2] - F.boxZ0)) - S.exitLabZ;
+    const double zb = F.labZ0 + fma(F.gammaBeta * F.c0, S.time + S.dt, F.gamma * (x[2] - F.boxZ0)) - S.exitLabZ;
+    if (za >= 0.) err |= kErrBeyondExit;
+    if (zb >= 0. && zb > za) err |= kErrBoundaryEvent;
+  }
+}
+
 }  // namespace aprl_gpu

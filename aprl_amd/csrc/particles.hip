@@ -16,6 +16,7 @@
 #include <limits>
 
 #include "particle_device.hpp"
+#include "shapes.hpp"
 
 namespace aprl_gpu {
 
@@ -51,55 +52,6 @@ struct PArgs {
 
 // ---------------------------------------------------------------------------
 // Device math restated from the reference.
-
-// source_coupling_math::momentum, source_coupling_math.h:272-293.
-__device__ void momentum(double u[3], const double e[3], const double mag[3], double q, double m, double h,
-                         double g0, unsigned& err) {
-  const double before[3] = {u[0], u[1], u[2]};
-  const double ks = q * h / (2 * m * kC);
-  const double kick[3] = {e[0] * ks, e[1] * ks, e[2] * ks};
-  const double bs = q * h / (2 * m);
-  const double b[3] = {mag[0] * bs, mag[1] * bs, mag[2] * bs};
-  const double a[3] = {before[0] + kick[0], before[1] + kick[1], before[2] + kick[2]};
-  const double axb[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
-  const double b2 = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
-  const double ka = kick[0] * a[0] + kick[1] * a[1] + kick[2] * a[2];
-  const double kb = kick[0] * b[0] + kick[1] * b[1] + kick[2] * b[2];
-  const double ab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
-  const double kcross = kick[0] * axb[0] + kick[1] * axb[1] + kick[2] * axb[2];
-  const double first = (kick[0] * before[0] + kick[1] * before[1] + kick[2] * before[2]) / g0;
-  const double kk = kick[0] * kick[0] + kick[1] * kick[1] + kick[2] * kick[2];
-  double G = g0 + first + (kk + kcross / g0 - first * first) / g0;
-  G = fmax(G, .5 * (g0 + 1.));
-  const double eps = 2.220446049250313e-16;
-  int iteration = 0;
-  for (; iteration < 12; ++iteration) {
-    const double den = G * G + b2, numerator = G * G * ka + G * kcross + ab * kb;
-    const double fv = G * (G - g0) - numerator / den;
-    const double derivative = 2 * G - g0 - ((2 * G * ka + kcross) * den - 2 * G * numerator) / (den * den);
-    const double correction = fv / derivative;
-    const double next = G - correction;
-    if (!isfinite(next) || next < .5 * (g0 + 1.) - 1e-13) {
-      err |= kErrRootNonphysical;
-      return;
-    }
-    G = fmax(next, .5 * (g0 + 1.));
-    if (fabs(correction) <= 8 * eps * fmax(1., G)) break;
-  }
-  if (iteration == 12) {
-    err |= kErrRootNoConverge;
-    return;
-  }
-  const double den = G * G + b2;
-  for (int k = 0; k < 3; ++k) u[k] = 2 * (G * G * a[k] + ab * b[k] + G * axb[k]) / den - before[k];
-}
-
-__device__ __forceinline__ void nodeWeights(double f, double w[4]) {
-  w[0] = 0.25 * (1. - f);
-  w[1] = 0.5 - 0.25 * f;
-  w[2] = 0.25 + 0.25 * f;
-  w[3] = 0.25 * f;
-}
 
 // clipOpenOrbit for one owner of the whole domain, source_coupling_math.h:235-250.
 struct Clip { double b[3], e[3], first, last; bool valid; };
@@ -139,93 +91,6 @@ __device__ Clip clipOrbit(const GArgs& G, const double begin[3], const double en
   return r;
 }
 
-// Crossing cuts of longitudinal_shape::weights (longitudinal_shape.h:100-103).
-constexpr int kMaxCuts = 10;
-__device__ int orbitCuts(const double x[3], const double v[3], double cuts[kMaxCuts], unsigned& err) {
-  int count = 2;
-  cuts[0] = 0.;
-  cuts[1] = 1.;
-  for (int a = 0; a < 3; ++a) {
-    if (v[a] != 0.) {
-      const double lo = fmin(x[a], x[a] + v[a]), hi = fmax(x[a], x[a] + v[a]);
-      for (long long j = static_cast<long long>(floor(lo)) + 1; j < hi; ++j) {
-        if (count == kMaxCuts) { err |= kErrCrossings; return count; }
-        cuts[count++] = (j - x[a]) / v[a];
-      }
-    }
-  }
-  for (int s = 1; s < count; ++s) {  // insertion sort, same order as std::sort
-    const double key = cuts[s];
-    int t = s - 1;
-    while (t >= 0 && cuts[t] > key) { cuts[t + 1] = cuts[t]; --t; }
-    cuts[t + 1] = key;
-  }
-  return count;
-}
-
-// ---------------------------------------------------------------------------
-// Source shapes and tile boxes.
-
-enum : int { kShapeQuintic = 0, kShapeBinomial3X = 2 };
-
-// Extended node box of a cell tile per shape: every tap of a chord that
-// starts in the tile and moves less than one cell per axis lies inside.
-// (X0, Y0, Z0) is the box origin below the tile origin; RZ is how many
-// neighbour tiles the box reaches along z (one along x and y).
-template <int SHAPE>
-struct Box;
-template <>
-struct Box<kShapeBinomial3X> {
-  // binomialXZSegment taps: x [i-1, i+2], y [j, j+1], z [k-1, k+2]
-  static constexpr int X0 = 2, Y0 = 1, Z0 = 2;
-  static constexpr int NX = kTileX + 5, NY = kTileY + 3, NZ = kTileZ + 5;
-  static constexpr int RZ = 1;
-  // each chord's weights sum to at most 2 in absolute value per component
-  static constexpr double kWeightBound = 2.0;
-};
-template <>
-struct Box<kShapeQuintic> {
-  // quintic Gauss-point taps: x [i, i+1], y [j, j+1], z [k-5, k+6]
-  static constexpr int X0 = 1, Y0 = 1, Z0 = 6;
-  static constexpr int NX = kTileX + 3, NY = kTileY + 3, NZ = kTileZ + 13;
-  static constexpr int RZ = 2;
-  // the compensated quintic's coefficients sum to about 3.01 in absolute value
-  static constexpr double kWeightBound = 4.0;
-};
-template <int SHAPE>
-constexpr int boxNodes() {
-  return Box<SHAPE>::NX * Box<SHAPE>::NY * Box<SHAPE>::NZ;
-}
-
-template <int SHAPE>
-struct TileBox {
-  int x0, y0, z0;
-  __device__ int local(int i, int j, int k, unsigned& err) const {
-    const unsigned li = static_cast<unsigned>(i - x0), lj = static_cast<unsigned>(j - y0),
-                   lk = static_cast<unsigned>(k - z0);
-    if (li >= static_cast<unsigned>(Box<SHAPE>::NX) || lj >= static_cast<unsigned>(Box<SHAPE>::NY) ||
-        lk >= static_cast<unsigned>(Box<SHAPE>::NZ)) {
-      err |= kErrTileHalo;
-      return -1;
-    }
-    return static_cast<int>((lk * Box<SHAPE>::NY + lj) * Box<SHAPE>::NX + li);
-  }
-  // box contains [i, i + wx) x [j, j + wy) x [k, k + wz)
-  __device__ bool holds(long long i, long long j, long long k, int wx, int wy, int wz) const {
-    return i >= x0 && i + wx <= x0 + Box<SHAPE>::NX && j >= y0 && j + wy <= y0 + Box<SHAPE>::NY && k >= z0 &&
-           k + wz <= z0 + Box<SHAPE>::NZ;
-  }
-  __device__ int base(long long i, long long j, long long k) const {
-    return (static_cast<int>(k - z0) * Box<SHAPE>::NY + static_cast<int>(j - y0)) * Box<SHAPE>::NX +
-           static_cast<int>(i - x0);
-  }
-};
-template <int SHAPE>
-__device__ __forceinline__ TileBox<SHAPE> tileBox(int tile, int ntx, int nty) {
-  const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
-  return TileBox<SHAPE>{tx * kTileX - Box<SHAPE>::X0, ty * kTileY - Box<SHAPE>::Y0, tz * kTileZ - Box<SHAPE>::Z0};
-}
-
 // Copy a tile box of E_cand + E_old (and, when bs is given, of B) into LDS,
 // zero outside the component ranges (such nodes are never read). All loads
 // of a lane are issued before the first LDS store.
@@ -261,155 +126,6 @@ __device__ void stageTile(const GArgs& G, const TileBox<SHAPE>& box, double* es,
   }
 }
 
-// Gauss-Legendre nodes and weights of longitudinal_shape::weights
-// (longitudinal_shape.h:105-106).
-__constant__ double kGaussNode[4] = {-.8611363115940525752, -.3399810435848562648, .3399810435848562648,
-                                     .8611363115940525752};
-__constant__ double kGaussWeight[4] = {.3478548451374538574, .6521451548625461426, .6521451548625461426,
-                                       .3478548451374538574};
-
-// longitudinal_shape::basis for the compensated quintic, longitudinal_shape.h:
-// 25-55: the degree-5 (nodes) or degree-4 (edges) B-spline at offset f,
-// convolved with the seven-point compensation.
-struct QBasis {
-  int first, count;
-  double v[12];
-};
-__device__ __forceinline__ void quinticBasis(double z, bool edge, QBasis& r) {
-#pragma unroll
-  for (int k = 0; k < 12; ++k) r.v[k] = 0.0;
-  const unsigned degree = edge ? 4u : 5u;
-  const double cell = floor(z), f = z - cell;
-  double n[6] = {1., 0., 0., 0., 0., 0.};
-#pragma unroll
-  for (unsigned j = 1; j <= 5; ++j) {
-    if (j > degree) break;
-    double saved = 0.;
-#pragma unroll
-    for (unsigned k = 0; k < j; ++k) {
-      const double t = n[k] / j;
-      n[k] = saved + (k + 1 - f) * t;
-      saved = (f + j - k - 1) * t;
-    }
-    n[j] = saved;
-  }
-  const double p = 5., a = p / 6., b = 3 * p / 40. + p * (p - 1) / 72.,
-               d = 5 * p / 112. + p * (p - 1) / 80. + p * (p - 1) * (p - 2) / 1296.;
-  const double c[7] = {-d / 64., b / 16. + 3 * d / 32., -a / 4. - b / 4. - 15 * d / 64.,
-                       1 + a / 2. + 3 * b / 8. + 5 * d / 16., -a / 4. - b / 4. - 15 * d / 64., b / 16. + 3 * d / 32.,
-                       -d / 64.};
-  r.first = static_cast<int>(cell) - 5;
-  r.count = static_cast<int>(degree) + 7;
-#pragma unroll
-  for (unsigned j = 0; j <= 5; ++j) {
-    if (j > degree) break;
-#pragma unroll
-    for (unsigned k = 0; k < 7; ++k) r.v[j + k] += n[j] * c[k];
-  }
-}
-
-// Visit every tap of one chord segment [first, last] (lattice start x,
-// velocity v) of longitudinal_shape::weights (longitudinal_shape.h:97-117)
-// in the reference's order: binomialXZSegment (61-94) for binomial3-x, four
-// Gauss points with the quintic basis otherwise. visit(d, i, j, k, w); fast
-// is called instead, with the segment's lowest tap corner, when the caller
-// reports (through interior) that every tap of the segment lies inside its
-// domain, so that taps are fixed offsets from one base.
-template <int SHAPE, class Interior, class Fast, class Visit>
-__device__ __forceinline__ void segmentTaps(const GArgs& G, const double x[3], const double v[3], double first,
-                                            double last, Interior interior, Fast fast, Visit visit) {
-  const double width = last - first;
-  if (SHAPE == kShapeBinomial3X) {
-    const double mid = (first + last) * 0.5;
-    const double px = x[0] + mid * v[0], py = x[1] + mid * v[1], pz = x[2] + mid * v[2];
-    long long i = static_cast<long long>(floor(px)), j = static_cast<long long>(floor(py));
-    const long long k = static_cast<long long>(floor(pz));
-    if (i == G.nx) --i;
-    if (j == G.ny) --j;
-    double wx[4], wz[4];
-    nodeWeights(px - i, wx);
-    nodeWeights(pz - k, wz);
-    const double wy[2] = {1. - (py - j), py - j};
-    const double edge[3] = {0.25, 0.5, 0.25};
-    const double slope[4] = {-0.25, -0.25, 0.25, 0.25};
-    const double sy[2] = {-1., 1.};
-    const double curvature = width * width / 12.;
-    const double xy = curvature * v[0] * v[1];
-    const double xz = curvature * v[0] * v[2];
-    const double yz = curvature * v[1] * v[2];
-    // binomialXZSegment visit order and weights, longitudinal_shape.h:81-93;
-    // offsets are relative to (i-1, j, k-1)
-    if (interior(i - 1, j, k - 1, 4, 2, 4)) {
-      for (int c = 0; c < 4; ++c) {
-        for (int b = 0; b < 2; ++b) {
-          const double weight = width * (wy[b] * wz[c] + yz * sy[b] * slope[c]);
-          for (int aa = 0; aa < 3; ++aa) fast(0, aa, b, c, weight * edge[aa]);
-        }
-        for (int aa = 0; aa < 4; ++aa) fast(1, aa, 0, c, width * (wx[aa] * wz[c] + xz * slope[aa] * slope[c]));
-      }
-      for (int b = 0; b < 2; ++b)
-        for (int aa = 0; aa < 4; ++aa) {
-          const double weight = width * (wx[aa] * wy[b] + xy * slope[aa] * sy[b]);
-          for (int c = 0; c < 3; ++c) fast(2, aa, b, c, weight * edge[c]);
-        }
-      return;
-    }
-    for (int c = 0; c < 4; ++c) {
-      for (int b = 0; b < 2; ++b) {
-        const double weight = width * (wy[b] * wz[c] + yz * sy[b] * slope[c]);
-        for (int aa = 0; aa < 3; ++aa) visit(0, i - 1 + aa, j + b, k - 1 + c, weight * edge[aa]);
-      }
-      for (int aa = 0; aa < 4; ++aa) visit(1, i - 1 + aa, j, k - 1 + c, width * (wx[aa] * wz[c] + xz * slope[aa] * slope[c]));
-    }
-    for (int b = 0; b < 2; ++b)
-      for (int aa = 0; aa < 4; ++aa) {
-        const double weight = width * (wx[aa] * wy[b] + xy * slope[aa] * sy[b]);
-        for (int c = 0; c < 3; ++c) visit(2, i - 1 + aa, j + b, k - 1 + c, weight * edge[c]);
-      }
-  } else {
-    for (int t = 0; t < 4; ++t) {
-      const double node = kGaussNode[t], gw = kGaussWeight[t];
-      const double u = .5 * (last + first + width * node), factor = .5 * width * gw;
-      const double xx = x[0] + u * v[0], yy = x[1] + u * v[1], zz = x[2] + u * v[2];
-      long long i = static_cast<long long>(floor(xx)), j = static_cast<long long>(floor(yy));
-      if (i == G.nx) --i;
-      if (j == G.ny) --j;
-      const double fx = xx - i, fy = yy - j;
-      QBasis nodes, edge;
-      quinticBasis(zz, false, nodes);
-      quinticBasis(zz, true, edge);
-      // offsets are relative to (i, j, nodes.first); edge.first == nodes.first
-      if (interior(i, j, static_cast<long long>(nodes.first), 2, 2, 12)) {
-        for (int k = 0; k < 12; ++k) {
-          const double w = factor * nodes.v[k];
-          if (w == 0.) continue;
-          for (int b = 0; b < 2; ++b) fast(0, 0, b, k, w * (b ? fy : 1. - fy));
-          for (int a = 0; a < 2; ++a) fast(1, a, 0, k, w * (a ? fx : 1. - fx));
-        }
-        for (int k = 0; k < 11; ++k) {
-          const double w = factor * edge.v[k];
-          if (w == 0.) continue;
-          for (int b = 0; b < 2; ++b)
-            for (int a = 0; a < 2; ++a) fast(2, a, b, k, w * (a ? fx : 1. - fx) * (b ? fy : 1. - fy));
-        }
-        continue;
-      }
-      for (int k = 0; k < nodes.count; ++k) {
-        const double w = factor * nodes.v[k];
-        if (w == 0.) continue;
-        for (int b = 0; b < 2; ++b) visit(0, i, j + b, nodes.first + k, w * (b ? fy : 1. - fy));
-        for (int a = 0; a < 2; ++a) visit(1, i + a, j, nodes.first + k, w * (a ? fx : 1. - fx));
-      }
-      for (int k = 0; k < edge.count; ++k) {
-        const double w = factor * edge.v[k];
-        if (w == 0.) continue;
-        for (int b = 0; b < 2; ++b)
-          for (int a = 0; a < 2; ++a) visit(2, i + a, j + b, edge.first + k, w * (a ? fx : 1. - fx) * (b ? fy : 1. - fy));
-      }
-    }
-  }
-}
-
 // Impl::electric wide branch (source_coupling.cpp:74-82): orbit average of
 // 1/2 (E_cand + E_old) over the clipped chord with the exact adjoint weights
 // of the deposit, read from the staged box es.
@@ -426,6 +142,7 @@ __device__ void gatherE(const GArgs& G, const double* es, const TileBox<SHAPE>& 
     v[d] = (path.e[d] - path.b[d]) / G.h[d];
   }
   const int count = orbitCuts(x, v, cuts, err);
+  const int org[3] = {0, 0, 0};  // x is grid-relative
   int base = 0;
   // all taps in range of every component and inside the box
   auto interior = [&](long long i, long long j, long long k, int wx, int wy, int wz) {
@@ -452,7 +169,7 @@ __device__ void gatherE(const GArgs& G, const double* es, const TileBox<SHAPE>& 
   };
   for (int s = 1; s < count; ++s) {
     if (cuts[s] - cuts[s - 1] <= 0.) continue;
-    segmentTaps<SHAPE>(G, x, v, cuts[s - 1], cuts[s], interior, fast, visit);
+    segmentTaps<SHAPE, double>(G.nx, G.ny, org, x, v, cuts[s - 1], cuts[s], interior, fast, visit);
   }
   const double f = path.last - path.first;
   res[0] *= f;
@@ -768,52 +485,10 @@ __global__ void __launch_bounds__(128) kPush(PArgs P, GArgs G, StepArgs S, unsig
   unsigned err = 0;
   const double x0[3] = {P.x[0][p], P.x[1][p], P.x[2][p]};
   const double u0[3] = {P.u[0][p], P.u[1][p], P.u[2][p]};
-  const double q = P.q[p], m = P.m[p];
-  if (S.window && x0[2] < G.o[2]) err |= kErrUpstream;
   const double usedE[3] = {P.fin[0][p], P.fin[1][p], P.fin[2][p]};
   const double usedB[3] = {P.fin[3][p], P.fin[4][p], P.fin[5][p]};
-  double x[3] = {x0[0], x0[1], x0[2]}, u[3] = {u0[0], u0[1], u0[2]};
-  const double h = S.dt / S.substeps;
-  double absGrid = 0.0;
-  for (int n = 0; n < S.substeps; ++n) {
-    const double bx[3] = {x[0], x[1], x[2]}, bu[3] = {u[0], u[1], u[2]};
-    const double g0 = gammaFromU(u[0], u[1], u[2]);
-    double point[3] = {x[0], x[1], x[2]};
-    const double ps = .5 * h * kC / g0;
-    for (int a = 0; a < 3; ++a) point[a] += ps * u[a];
-    double ee[3] = {0., 0., 0.}, bb[3] = {0., 0., 0.};
-    prescribedBox(S, point, S.time + (n + .5) * h, ee, bb);
-    double e[3] = {usedE[0], usedE[1], usedE[2]}, b[3] = {usedB[0], usedB[1], usedB[2]};
-    for (int a = 0; a < 3; ++a) { e[a] += ee[a]; b[a] += bb[a]; }
-    momentum(u, e, b, q, m, h, g0, err);
-    const double g1 = gammaFromU(u[0], u[1], u[2]);
-    double mean[3] = {bu[0], bu[1], bu[2]};
-    for (int a = 0; a < 3; ++a) mean[a] += u[a];
-    const double xs = h * kC / (g0 + g1);
-    for (int a = 0; a < 3; ++a) x[a] += xs * mean[a];
-    double w = 0.0;
-    for (int a = 0; a < 3; ++a) w += q * usedE[a] * (x[a] - bx[a]);
-    absGrid += fabs(w);
-  }
-  // Boundary events of the deposition loop (simulation.cpp:5229-5245).
-  for (int a = 0; a < 3; ++a) {
-    if (!isfinite(x0[a]) || !isfinite(x[a])) err |= kErrNonfinite;
-    if (x0[a] < S.physLo[a] - S.cpmlTol[a] || x0[a] > S.physHi[a] + S.cpmlTol[a]) err |= kErrStartInCpml;
-    if (S.cpmlCells[a] && (x[a] < S.physLo[a] || x[a] > S.physHi[a])) err |= kErrBoundaryEvent;
-    if (x0[a] < S.outLo[a] - S.outTol[a] || x0[a] > S.outHi[a] + S.outTol[a]) err |= kErrStartOutside;
-    if (x[a] < S.outLo[a] || x[a] > S.outHi[a]) err |= kErrBoundaryEvent;
-    const double rel0 = x0[a] - G.o[a], rel1 = x[a] - G.o[a];
-    if (rel0 < -S.posTol[a] || rel0 > S.extent[a] + S.posTol[a] || rel1 < -S.posTol[a] ||
-        rel1 > S.extent[a] + S.posTol[a])
-      err |= kErrEndpointOutside;
-  }
-  if (S.hasExit) {
-    const BoostDev& F = S.frame;
-    const double za = F.labZ0 + fma(F.gammaBeta * F.c0, S.time, F.gamma * (x0[2] - F.boxZ0)) - S.exitLabZ;
-    const double zb = F.labZ0 + fma(F.gammaBeta * F.c0, S.time + S.dt, F.gamma * (x[2] - F.boxZ0)) - S.exitLabZ;
-    if (za >= 0.) err |= kErrBeyondExit;
-    if (zb >= 0. && zb > za) err |= kErrBoundaryEvent;
-  }
+  double x[3], u[3], absGrid;
+  pushChord(S, G.o, x0, u0, P.q[p], P.m[p], usedE, usedB, x, u, absGrid, err);
   for (int a = 0; a < 3; ++a) {
     P.x1[a][p] = x[a];
     P.u1[a][p] = u[a];
@@ -875,6 +550,7 @@ __global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, const int4*
       v[d] = (P.x1[d][p] - P.x[d][p]) / G.h[d];
     }
     const int count = orbitCuts(x, v, cuts, err);
+  const int org[3] = {0, 0, 0};  // x is grid-relative
     int base = 0;
     auto interior = [&](long long i, long long j, long long k, int wx, int wy, int wz) {
       if (i < 0 || j < 0 || k < 0 || i + wx > G.nx + 1 || j + wy > G.ny + 1 || k + wz > G.nz + 1) return false;
@@ -906,7 +582,7 @@ __global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, const int4*
     };
     for (int s = 1; s < count; ++s) {
       if (cuts[s] - cuts[s - 1] <= 0.) continue;
-      segmentTaps<SHAPE>(G, x, v, cuts[s - 1], cuts[s], interior, fast, visit);
+      segmentTaps<SHAPE, double>(G.nx, G.ny, org, x, v, cuts[s - 1], cuts[s], interior, fast, visit);
     }
   }
   __syncthreads();
