@@ -124,6 +124,7 @@ def lib() -> C.CDLL:
     L.aprl_coupling_add_magnet.argtypes = [vp, C.POINTER(Magnet)]
     L.aprl_coupling_set_particle_boundary.argtypes = [vp, C.POINTER(_sz), C.c_double]
     L.aprl_coupling_step.argtypes = [vp, vp, _dp, C.c_double, C.c_uint, C.POINTER(StepReport)]
+    L.aprl_coupling_set_split_trials.argtypes = [vp, C.c_uint, C.c_int]
     L.aprl_fields_interior_energy.argtypes = [vp, C.POINTER(_sz), C.POINTER(_sz), _dp]
     L.aprl_fields_face_power.argtypes = [vp, C.POINTER(_sz), C.POINTER(_sz), _dp]
     L.aprl_coupling_continuity.argtypes = [vp, vp, _dp]
@@ -169,5 +170,6 @@ EXPORTED = [
     "aprl_particles_create", "aprl_particles_destroy", "aprl_particles_upload", "aprl_particles_download",
     "aprl_particles_count", "aprl_coupling_create", "aprl_coupling_destroy", "aprl_coupling_set_frame",
     "aprl_coupling_add_magnet", "aprl_coupling_set_particle_boundary", "aprl_coupling_step",
+    "aprl_coupling_set_split_trials",
     "aprl_fields_interior_energy", "aprl_fields_face_power", "aprl_coupling_continuity", "aprl_coupling_gauss",
 ]

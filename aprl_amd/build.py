@@ -21,12 +21,18 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # Bitwise parity with the reference's non-FMA x86-64 arithmetic; the
-    # FP32-split kernels opt back in to contraction locally.
+    # FP32-split particle kernels opt back in to contraction (FILE_FLAGS).
     "-ffp-contract=off",
     # Keep the compiler from forming packed FP32 ops (v_pk_*_f32).
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", INCLUDE, "-I", CSRC, "-Wno-unused-command-line-argument",
 ]
+
+
+# Per-file additions (a later flag overrides COMMON's).
+FILE_FLAGS = {
+    "particles32.hip": ["-ffp-contract=fast"],
+}
 
 
 def sources() -> list[str]:
@@ -35,7 +41,7 @@ def sources() -> list[str]:
 
 def _compile(src: str) -> str:
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *COMMON, "-c", src, "-o", obj]
+    cmd = [HIPCC, *COMMON, *FILE_FLAGS.get(os.path.basename(src), []), "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")

@@ -108,6 +108,12 @@ class ConservativeSourceCoupling:
         cells = (C.c_size_t * 3)(*cpml_cells)
         check(lib().aprl_coupling_set_particle_boundary(self.handle, cells, exit_lab_z))
 
+    def setSplitTrials(self, trials=2, verify=False):
+        """FP32-split grids: run exactly ``trials`` Picard trials per step in
+        place of the FP64 convergence test; ``verify`` adds a residual gather
+        after the last trial whose force and work the step reports."""
+        check(lib().aprl_coupling_set_split_trials(self.handle, int(trials), 1 if verify else 0))
+
     def step(self, particles: ParticleStore, origin, time, substeps=1):
         o = np.ascontiguousarray(origin, dtype=np.float64)
         r = _capi.StepReport()

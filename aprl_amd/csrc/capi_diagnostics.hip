@@ -1,23 +1,42 @@
-// C-ABI entry points of the FP64 diagnostics (declared in include/aprl_gpu.h).
+// C-ABI entry points of the FP64 diagnostics (declared in include/aprl_gpu.h),
+// for both forms of the grid: the FP32-split form's fields, currents and
+// positions are widened and every sum runs in FP64.
+#include <cmath>
+
 #include "diagnostics.hpp"
 #include "handles.hpp"
 
 using namespace aprl_gpu;
 
 namespace {
-FieldState& fieldsOf(aprl_fields* f) {
+void checkFields(aprl_fields* f) {
   if (!f) throw std::invalid_argument("Null field grid handle");
-  if (f->fp32) throw std::invalid_argument("This diagnostic entry point takes an APRL_FP64 grid");
-  return f->s;
 }
 void checkCoupled(aprl_coupling* c, aprl_particles* p) {
   if (!c || !p) throw std::invalid_argument("Null coupling or particle store");
-  if (p->fields == nullptr || &p->fields->s != c->c.fields)
+  if (p->fields == nullptr || p->fields != c->fields)
     throw std::invalid_argument("Particle store and coupling belong to different field grids");
 }
 const double* const* positions(ParticleStore& P, bool start, const double* out[3]) {
   for (int a = 0; a < 3; ++a) out[a] = start ? P.end[a].get() : P.s[a].get();
   return out;
+}
+// Split-form positions: start of the last step (end arrays) or current.
+void splitPositions(SplitStore& P, bool start, const int* cell[3], const float* off[3]) {
+  for (int a = 0; a < 3; ++a) {
+    cell[a] = start ? P.endCell[a].get() : P.cell[a].get();
+    off[a] = start ? P.end[a].get() : P.s[a].get();
+  }
+}
+// Lattice cell of a grid origin of the split form.
+void latticeOrigin(const FieldState32& F, const double origin[3], int oc[3]) {
+  const double h[3] = {F.g.dx, F.g.dy, F.g.dz};
+  for (int a = 0; a < 3; ++a) {
+    const double r = origin[a] / h[a];
+    if (!std::isfinite(r) || std::fabs(r) >= 1.0e9 || std::fabs(r - std::nearbyint(r)) > 1e-6)
+      throw std::invalid_argument("An FP32-split grid origin must lie on the cell lattice (origin = integer * spacing)");
+    oc[a] = static_cast<int>(std::nearbyint(r));
+  }
 }
 }  // namespace
 
@@ -26,14 +45,17 @@ extern "C" {
 aprl_status aprl_fields_interior_energy(aprl_fields* fields, const size_t* lo, const size_t* hi, double* energy) {
   return guarded([&] {
     if (!lo || !hi || !energy) throw std::invalid_argument("Null diagnostic argument");
-    *energy = interiorEnergy(fieldsOf(fields), lo, hi);
+    checkFields(fields);
+    *energy = fields->fp32 ? interiorEnergy(fields->s32, lo, hi) : interiorEnergy(fields->s, lo, hi);
   });
 }
 
 aprl_status aprl_fields_face_power(aprl_fields* fields, const size_t* lo, const size_t* hi, double* power) {
   return guarded([&] {
     if (!lo || !hi || !power) throw std::invalid_argument("Null diagnostic argument");
-    facePower(fieldsOf(fields), lo, hi, power);
+    checkFields(fields);
+    if (fields->fp32) facePower(fields->s32, lo, hi, power);
+    else facePower(fields->s, lo, hi, power);
   });
 }
 
@@ -42,6 +64,23 @@ aprl_status aprl_coupling_continuity(aprl_coupling* coupling, aprl_particles* pa
     checkCoupled(coupling, particles);
     if (!out) throw std::invalid_argument("Null diagnostic argument");
     CouplingState& C = coupling->c;
+    if (coupling->fields->fp32) {
+      SplitStore& P = particles->sp;
+      if (!C.lastStep.valid || C.lastStep.store != &P || C.lastStep.generation != P.generation)
+        throw std::logic_error("Continuity needs the particle store of the last coupled step");
+      FieldState32& F = *C.fields32;
+      int oc[3];
+      latticeOrigin(F, C.lastStep.origin, oc);
+      const int* cell[3];
+      const float* off[3];
+      const int n = static_cast<int>(P.count);
+      splitPositions(P, true, cell, off);
+      chargeDensitySplit(F, C.shape, cell, off, P.s[6].get(), n, oc, C.rho[0]);
+      splitPositions(P, false, cell, off);
+      chargeDensitySplit(F, C.shape, cell, off, P.s[6].get(), n, oc, C.rho[1]);
+      continuityResidual(F, C.rho[0], C.rho[1], out);
+      return;
+    }
     ParticleStore& P = particles->p;
     if (!C.lastStep.valid || C.lastStep.store != &P || C.lastStep.generation != P.generation)
       throw std::logic_error("Continuity needs the particle store of the last coupled step");
@@ -60,6 +99,18 @@ aprl_status aprl_coupling_gauss(aprl_coupling* coupling, aprl_particles* particl
     checkCoupled(coupling, particles);
     if (!origin || !lo || !hi || !out) throw std::invalid_argument("Null diagnostic argument");
     CouplingState& C = coupling->c;
+    if (coupling->fields->fp32) {
+      SplitStore& P = particles->sp;
+      FieldState32& F = *C.fields32;
+      int oc[3];
+      latticeOrigin(F, origin, oc);
+      const int* cell[3];
+      const float* off[3];
+      splitPositions(P, false, cell, off);
+      chargeDensitySplit(F, C.shape, cell, off, P.s[6].get(), static_cast<int>(P.count), oc, C.rho[0]);
+      gaussResidual(F, C.rho[0], lo, hi, out);
+      return;
+    }
     ParticleStore& P = particles->p;
     const double* x[3];
     chargeDensity(*C.fields, C.shape, positions(P, false, x), P.s[6].get(), static_cast<int>(P.count), origin,

@@ -6,7 +6,6 @@
 #include "handles.hpp"
 
 namespace aprl_gpu {
-void reserveParticles(ParticleStore& P, std::size_t capacity);
 void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], double time, unsigned substeps,
                   aprl_step_report* report);
 }  // namespace aprl_gpu
@@ -59,29 +58,37 @@ extern "C" {
 aprl_status aprl_particles_create(aprl_fields* fields, size_t capacity, aprl_particles** out) {
   return guarded([&] {
     if (!fields || !out) throw std::invalid_argument("Null field grid or output handle");
-    if (fields->fp32) throw std::invalid_argument("Use aprl_split_particles_create for an APRL_FP32_SPLIT grid");
     std::unique_ptr<aprl_particles> p(new aprl_particles);
     p->fields = fields;
-    reserveParticles(p->p, std::max<size_t>(capacity, 1));
+    if (fields->fp32) reserveSplitParticles(p->sp, std::max<size_t>(capacity, 1));
+    else reserveParticles(p->p, std::max<size_t>(capacity, 1));
     *out = p.release();
   });
 }
 
 void aprl_particles_destroy(aprl_particles* particles) {
   if (!particles) return;
-  (void)hipStreamSynchronize(particles->fields->s.stream);
+  (void)hipStreamSynchronize(particles->stream());
   delete particles;
 }
 
-size_t aprl_particles_count(const aprl_particles* particles) { return particles ? particles->p.count : 0; }
+size_t aprl_particles_count(const aprl_particles* particles) {
+  if (!particles) return 0;
+  return particles->fields->fp32 ? particles->sp.count : particles->p.count;
+}
 
 aprl_status aprl_particles_upload(aprl_particles* particles, size_t n, const double* x3, const double* u3,
                                   const double* q, const double* m, const double* w, const uint64_t* id) {
   return guarded([&] {
     if (!particles) throw std::invalid_argument("Null particle store");
     if (n && (!x3 || !u3 || !q || !m || !w || !id)) throw std::invalid_argument("Null particle array");
+    if (particles->fields->fp32) {
+      hipCheck(hipStreamSynchronize(particles->stream()), "hipStreamSynchronize");
+      uploadSplitParticles(particles->sp, particles->fields->s32, n, x3, u3, q, m, w, id);
+      return;
+    }
     ParticleStore& P = particles->p;
-    hipStream_t st = particles->fields->s.stream;
+    hipCheck(hipStreamSynchronize(particles->stream()), "hipStreamSynchronize");
     reserveParticles(P, n);
     std::vector<double> col(n);
     for (int a = 0; a < 3; ++a) {
@@ -94,7 +101,6 @@ aprl_status aprl_particles_upload(aprl_particles* particles, size_t n, const dou
     for (int a = 0; a < 3; ++a)
       hipCheck(hipMemcpy(P.s[6 + a].get(), scalars[a], n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
     hipCheck(hipMemcpy(P.id.get(), id, n * sizeof(uint64_t), hipMemcpyHostToDevice), "hipMemcpy");
-    (void)st;
     P.count = n;
     ++P.generation;
   });
@@ -104,8 +110,12 @@ aprl_status aprl_particles_download(aprl_particles* particles, double* x3, doubl
                                     double* w, uint64_t* id) {
   return guarded([&] {
     if (!particles) throw std::invalid_argument("Null particle store");
+    if (particles->fields->fp32) {
+      downloadSplitParticles(particles->sp, particles->fields->s32, x3, u3, q, m, w, id);
+      return;
+    }
     ParticleStore& P = particles->p;
-    hipCheck(hipStreamSynchronize(particles->fields->s.stream), "hipStreamSynchronize");
+    hipCheck(hipStreamSynchronize(particles->stream()), "hipStreamSynchronize");
     const size_t n = P.count;
     std::vector<double> col(n);
     for (int a = 0; a < 3; ++a) {
@@ -129,26 +139,29 @@ aprl_status aprl_particles_download(aprl_particles* particles, double* x3, doubl
 aprl_status aprl_coupling_create(aprl_fields* fields, aprl_shape shape, int smoothX, aprl_coupling** out) {
   return guarded([&] {
     if (!fields || !out) throw std::invalid_argument("Null field grid or output handle");
-    if (fields->fp32) throw std::invalid_argument("Use aprl_split_coupling_create for an APRL_FP32_SPLIT grid");
     if (shape != APRL_SHAPE_LINEAR_BINOMIAL3 && shape != APRL_SHAPE_QUINTIC_COMPENSATED)
       throw std::invalid_argument("This build implements the quintic-compensated and linear-binomial3 source shapes");
     if (shape == APRL_SHAPE_LINEAR_BINOMIAL3 && !smoothX)
       throw std::invalid_argument("This build implements linear-binomial3 with the production x-binomial3 smoothing only");
     std::unique_ptr<aprl_coupling> c(new aprl_coupling);
-    c->c.fields = &fields->s;
+    c->fields = fields;
+    if (fields->fp32) c->c.fields32 = &fields->s32;
+    else c->c.fields = &fields->s;
     c->c.shape = shape;
     // the x-binomial3 smoothing applies to the binomial3 shape only
     // (longitudinal_shape.h:109)
     c->c.smoothX = shape == APRL_SHAPE_LINEAR_BINOMIAL3 && smoothX != 0;
-    if (fields->s.hasCpml)
-      for (int a = 0; a < 3; ++a) c->c.cpmlCells[a] = fields->s.cpml.cells[a];
+    const bool hasCpml = fields->fp32 ? fields->s32.hasCpml : fields->s.hasCpml;
+    const aprl_cpml_parameters& cpml = fields->fp32 ? fields->s32.cpml : fields->s.cpml;
+    if (hasCpml)
+      for (int a = 0; a < 3; ++a) c->c.cpmlCells[a] = cpml.cells[a];
     *out = c.release();
   });
 }
 
 void aprl_coupling_destroy(aprl_coupling* coupling) {
   if (!coupling) return;
-  (void)hipStreamSynchronize(coupling->c.fields->stream);
+  (void)hipStreamSynchronize(coupling->fields->fp32 ? coupling->fields->s32.stream : coupling->fields->s.stream);
   delete coupling;
 }
 
@@ -194,9 +207,20 @@ aprl_status aprl_coupling_step(aprl_coupling* coupling, aprl_particles* particle
                                double time, unsigned substeps, aprl_step_report* report) {
   return guarded([&] {
     if (!coupling || !particles || !origin) throw std::invalid_argument("Null coupling, particles or origin");
-    if (&particles->fields->s != coupling->c.fields)
+    if (particles->fields != coupling->fields)
       throw std::invalid_argument("Particles and coupling belong to different field grids");
-    couplingStep(coupling->c, particles->p, origin, time, substeps, report);
+    if (coupling->fields->fp32) couplingStepSplit(coupling->c, particles->sp, origin, time, substeps, report);
+    else couplingStep(coupling->c, particles->p, origin, time, substeps, report);
+  });
+}
+
+aprl_status aprl_coupling_set_split_trials(aprl_coupling* coupling, unsigned trials, int verify) {
+  return guarded([&] {
+    if (!coupling) throw std::invalid_argument("Null coupling");
+    if (!coupling->fields->fp32) throw std::invalid_argument("Fixed Picard trials apply to FP32-split grids only");
+    if (trials < 2 || trials > 64) throw std::invalid_argument("FP32-split Picard trials must be in [2, 64]");
+    coupling->c.splitTrials = trials;
+    coupling->c.splitVerify = verify != 0;
   });
 }
 

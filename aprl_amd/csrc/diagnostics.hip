@@ -30,12 +30,14 @@ struct Acc {
   }
 };
 
-__device__ __forceinline__ double at(const double* p, const int* n, int i, int j, int k) {
-  return p[(static_cast<size_t>(k) * n[1] + j) * n[0] + i];
+template <class R>
+__device__ __forceinline__ double at(const R* p, const int* n, int i, int j, int k) {
+  return static_cast<double>(p[(static_cast<size_t>(k) * n[1] + j) * n[0] + i]);
 }
 
 // EBFieldGrid::fieldEnergyDensityCell, eb_field.cpp:715-738.
-__device__ double energyDensityCell(const FieldPtrs& F, int i, int j, int k) {
+template <class R>
+__device__ double energyDensityCell(const FieldPtrsT<R>& F, int i, int j, int k) {
   const double e0 = 0.25 * (at(F.e[0], F.n[0], i, j, k) + at(F.e[0], F.n[0], i, j + 1, k) +
                             at(F.e[0], F.n[0], i, j, k + 1) + at(F.e[0], F.n[0], i, j + 1, k + 1));
   const double e1 = 0.25 * (at(F.e[1], F.n[1], i, j, k) + at(F.e[1], F.n[1], i + 1, j, k) +
@@ -49,7 +51,8 @@ __device__ double energyDensityCell(const FieldPtrs& F, int i, int j, int k) {
 }
 
 // EBFieldGrid::poyntingComponentCell, eb_field.cpp:740-782.
-__device__ double poyntingCell(const FieldPtrs& F, int i, int j, int k, int axis) {
+template <class R>
+__device__ double poyntingCell(const FieldPtrsT<R>& F, int i, int j, int k, int axis) {
   auto cEx = [&]() {
     return 0.25 * (at(F.e[0], F.n[0], i, j, k) + at(F.e[0], F.n[0], i, j + 1, k) + at(F.e[0], F.n[0], i, j, k + 1) +
                    at(F.e[0], F.n[0], i, j + 1, k + 1));
@@ -93,7 +96,8 @@ __device__ Acc blockReduce(Acc a, double* ss, double* sc) {
 }
 
 // Sum of energyDensityCell * volume over the cells of [lo, hi).
-__global__ void __launch_bounds__(kRedThreads) kEnergyBox(FieldPtrs F, int3 lo, int3 hi, double volume,
+template <class R>
+__global__ void __launch_bounds__(kRedThreads) kEnergyBox(FieldPtrsT<R> F, int3 lo, int3 hi, double volume,
                                                           double* partial) {
   __shared__ double ss[kRedThreads], sc[kRedThreads];
   const long long wx = hi.x - lo.x, wy = hi.y - lo.y, wz = hi.z - lo.z, total = wx * wy * wz;
@@ -114,7 +118,8 @@ __global__ void __launch_bounds__(kRedThreads) kEnergyBox(FieldPtrs F, int3 lo, 
 // Outward power through the six faces of the cell box [lo, hi):
 // face 2a (lower, sign -) and 2a+1 (upper, sign +) sum the axis-a Poynting
 // component over the face's cells times the face area. blockIdx.y = face.
-__global__ void __launch_bounds__(kRedThreads) kFacePower(FieldPtrs F, int3 lo, int3 hi, double3 area,
+template <class R>
+__global__ void __launch_bounds__(kRedThreads) kFacePower(FieldPtrsT<R> F, int3 lo, int3 hi, double3 area,
                                                           double* partial) {
   __shared__ double ss[kRedThreads], sc[kRedThreads];
   const int face = blockIdx.y, axis = face / 2;
@@ -156,8 +161,12 @@ __global__ void __launch_bounds__(kRedThreads) kFinish(const double* partial, in
 // Charge density, continuity and Gauss residuals.
 
 struct RhoArgs {
-  const double* x[3];
+  const double* x[3];  // absolute positions (plain form), or
+  const int* cell[3];  // lattice cell and in-cell offset (FP32-split form)
+  const float* off[3];
+  int oc[3];           // lattice cell of the grid origin (split form)
   const double* q;
+  const float* qf;     // charges of the split form
   int n;
   double o[3], h[3];
   int nx, ny, nz;
@@ -208,8 +217,11 @@ __device__ int chargeBasis(int shape, double z, double v[12]) {
 __global__ void kChargeDensity(RhoArgs R) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= R.n) return;
-  const double lx = (R.x[0][p] - R.o[0]) / R.h[0], ly = (R.x[1][p] - R.o[1]) / R.h[1],
-               lz = (R.x[2][p] - R.o[2]) / R.h[2];
+  double l[3];
+  for (int a = 0; a < 3; ++a)
+    l[a] = R.x[0] ? (R.x[a][p] - R.o[a]) / R.h[a]
+                  : static_cast<double>(R.cell[a][p] - R.oc[a]) + static_cast<double>(R.off[a][p]);
+  const double lx = l[0], ly = l[1], lz = l[2];
   const int i = static_cast<int>(floor(lx)), j = static_cast<int>(floor(ly));
   const double fx = lx - i, fy = ly - j;
   double wz[12];
@@ -227,7 +239,7 @@ __global__ void kChargeDensity(RhoArgs R) {
     i0 = i;
   }
   const double wy[2] = {1. - fy, fy};
-  const double qv = R.q[p] * R.invVolume;
+  const double qv = (R.q ? R.q[p] : static_cast<double>(R.qf[p])) * R.invVolume;
   const int vx = R.nx + 1, vy = R.ny + 1, vz = R.nz + 1;
   for (int c = 0; c < nz; ++c) {
     const int kk = k0 + c;
@@ -254,10 +266,11 @@ __global__ void kFixedToDouble(unsigned long long* acc, double* out, size_t n, i
   }
 }
 
-__global__ void kSumAbs(const double* q, int n, double* partial) {
+template <class T>
+__global__ void kSumAbs(const T* q, int n, double* partial) {
   __shared__ double ss[kRedThreads], sc[kRedThreads];
   Acc a;
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) a.add(fabs(q[p]));
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) a.add(fabs(static_cast<double>(q[p])));
   const Acc b = blockReduce(a, ss, sc);
   if (threadIdx.x == 0) {
     partial[2 * blockIdx.x] = b.s;
@@ -273,7 +286,8 @@ __device__ __forceinline__ void atomicMaxAbs(unsigned long long* slot, double v)
 // ChargeConservingCurrentDepositor::maxContinuityResidual,
 // eb_deposition.cpp:195-226: max over vertices of
 // |(rho1 - rho0)/dt + div J|, and of |(rho1 - rho0)/dt| as its scale.
-__global__ void kContinuity(FieldPtrs F, const double* rho0, const double* rho1, double dt, double3 h,
+template <class R>
+__global__ void kContinuity(FieldPtrsT<R> F, const double* rho0, const double* rho1, double dt, double3 h,
                             unsigned long long* out) {
   const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
   const size_t total = static_cast<size_t>(nx + 1) * (ny + 1) * (nz + 1);
@@ -297,7 +311,8 @@ __global__ void kContinuity(FieldPtrs F, const double* rho0, const double* rho1,
 // Gauss residual of eb_initial_field.cpp:774-799 over the vertices strictly
 // inside [lo, hi]: error = div E - rho / eps0; per block the compensated
 // sums of error^2 and source^2, and the max |error| and |source|.
-__global__ void __launch_bounds__(kRedThreads) kGauss(FieldPtrs F, const double* rho, int3 lo, int3 hi, double3 h,
+template <class R>
+__global__ void __launch_bounds__(kRedThreads) kGauss(FieldPtrsT<R> F, const double* rho, int3 lo, int3 hi, double3 h,
                                                       double* partial, unsigned long long* maxima) {
   __shared__ double ss[kRedThreads], sc[kRedThreads];
   const int nx = F.n[0][0], ny = F.n[1][1];
@@ -343,7 +358,8 @@ int3 toInt3(const std::size_t v[3]) {
   return make_int3(static_cast<int>(v[0]), static_cast<int>(v[1]), static_cast<int>(v[2]));
 }
 
-void checkBox(const FieldState& s, const std::size_t lo[3], const std::size_t hi[3], bool cells) {
+template <class R>
+void checkBox(const FieldStateT<R>& s, const std::size_t lo[3], const std::size_t hi[3], bool cells) {
   const std::size_t n[3] = {s.g.nx, s.g.ny, s.g.nz};
   for (int a = 0; a < 3; ++a)
     if (lo[a] > hi[a] || hi[a] > n[a] + (cells ? 0 : 1))
@@ -352,7 +368,8 @@ void checkBox(const FieldState& s, const std::size_t lo[3], const std::size_t hi
 
 }  // namespace
 
-double interiorEnergy(FieldState& s, const std::size_t lo[3], const std::size_t hi[3]) {
+template <class R>
+double interiorEnergy(FieldStateT<R>& s, const std::size_t lo[3], const std::size_t hi[3]) {
   checkBox(s, lo, hi, true);
   DiagnosticScratch& D = s.diag;
   ensure(D.partial, 2 * 6 * kRedBlocks);
@@ -367,7 +384,8 @@ double interiorEnergy(FieldState& s, const std::size_t lo[3], const std::size_t 
   return e;
 }
 
-void facePower(FieldState& s, const std::size_t lo[3], const std::size_t hi[3], double power[6]) {
+template <class R>
+void facePower(FieldStateT<R>& s, const std::size_t lo[3], const std::size_t hi[3], double power[6]) {
   checkBox(s, lo, hi, true);
   for (int a = 0; a < 3; ++a)
     if (lo[a] >= hi[a]) throw std::out_of_range("Diagnostic face box is empty");
@@ -385,10 +403,10 @@ void facePower(FieldState& s, const std::size_t lo[3], const std::size_t hi[3], 
   hipCheck(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
 }
 
-// Shaped charge density of a particle set (positions x[3], charges q) on
-// the vertices into `rho`.
-void chargeDensity(FieldState& s, int shape, const double* const x[3], const double* q, int n, const double origin[3],
-                   DeviceArray<double>& rho) {
+// Shaped charge density of a particle set on the vertices into `rho`; R
+// carries the positions and charges, q the charges for the bound.
+template <class Real, class Q>
+void depositCharge(FieldStateT<Real>& s, RhoArgs R, const Q* q, DeviceArray<double>& rho) {
   DiagnosticScratch& D = s.diag;
   const std::size_t nv = (s.g.nx + 1) * (s.g.ny + 1) * (s.g.nz + 1);
   if (D.acc.size() != nv) {
@@ -400,6 +418,7 @@ void chargeDensity(FieldState& s, int shape, const double* const x[3], const dou
   ensure(D.out, 8);
   // global fixed-point exponent from the total |charge| density bound (each
   // shape's weights sum to at most 4 in absolute value)
+  const int n = R.n;
   double total = 0.0;
   if (n > 0) {
     kSumAbs<<<kRedBlocks, kRedThreads, 0, s.stream>>>(q, n, D.partial.get());
@@ -412,18 +431,10 @@ void chargeDensity(FieldState& s, int shape, const double* const x[3], const dou
   int e2 = 0;
   (void)std::frexp(4.0 * total * invVolume, &e2);
   const int shift = total > 0.0 ? 62 - e2 : 0;
-  RhoArgs R{};
-  for (int a = 0; a < 3; ++a) {
-    R.x[a] = x[a];
-    R.o[a] = origin[a];
-  }
   R.h[0] = s.g.dx; R.h[1] = s.g.dy; R.h[2] = s.g.dz;
-  R.q = q;
-  R.n = n;
   R.nx = static_cast<int>(s.g.nx);
   R.ny = static_cast<int>(s.g.ny);
   R.nz = static_cast<int>(s.g.nz);
-  R.shape = shape;
   R.shift = shift;
   R.invVolume = invVolume;
   R.acc = D.acc.get();
@@ -432,7 +443,35 @@ void chargeDensity(FieldState& s, int shape, const double* const x[3], const dou
   launchCheck("charge density");
 }
 
-void continuityResidual(FieldState& s, const DeviceArray<double>& rho0, const DeviceArray<double>& rho1,
+void chargeDensity(FieldState& s, int shape, const double* const x[3], const double* q, int n, const double origin[3],
+                   DeviceArray<double>& rho) {
+  RhoArgs R{};
+  for (int a = 0; a < 3; ++a) {
+    R.x[a] = x[a];
+    R.o[a] = origin[a];
+  }
+  R.q = q;
+  R.n = n;
+  R.shape = shape;
+  depositCharge(s, R, q, rho);
+}
+
+void chargeDensitySplit(FieldState32& s, int shape, const int* const cell[3], const float* const off[3],
+                        const float* q, int n, const int oc[3], DeviceArray<double>& rho) {
+  RhoArgs R{};
+  for (int a = 0; a < 3; ++a) {
+    R.cell[a] = cell[a];
+    R.off[a] = off[a];
+    R.oc[a] = oc[a];
+  }
+  R.qf = q;
+  R.n = n;
+  R.shape = shape;
+  depositCharge(s, R, q, rho);
+}
+
+template <class R>
+void continuityResidual(FieldStateT<R>& s, const DeviceArray<double>& rho0, const DeviceArray<double>& rho1,
                         double out[2]) {
   DiagnosticScratch& D = s.diag;
   ensure(D.maxima, 4);
@@ -447,7 +486,8 @@ void continuityResidual(FieldState& s, const DeviceArray<double>& rho0, const De
   std::memcpy(&out[1], &m[1], sizeof(double));
 }
 
-void gaussResidual(FieldState& s, const DeviceArray<double>& rho, const std::size_t lo[3], const std::size_t hi[3],
+template <class R>
+void gaussResidual(FieldStateT<R>& s, const DeviceArray<double>& rho, const std::size_t lo[3], const std::size_t hi[3],
                    double out[4]) {
   // strictly interior vertices i in (lo, hi) read Ex(i), Ey(j), Ez(k): hi <= n
   checkBox(s, lo, hi, true);
@@ -471,5 +511,16 @@ void gaussResidual(FieldState& s, const DeviceArray<double>& rho, const std::siz
   std::memcpy(&out[2], &m[0], sizeof(double));
   std::memcpy(&out[3], &m[1], sizeof(double));
 }
+
+#define APRL_DIAG_INSTANTIATE(R)                                                                              \
+  template double interiorEnergy(FieldStateT<R>&, const std::size_t*, const std::size_t*);                    \
+  template void facePower(FieldStateT<R>&, const std::size_t*, const std::size_t*, double*);                  \
+  template void continuityResidual(FieldStateT<R>&, const DeviceArray<double>&, const DeviceArray<double>&,   \
+                                   double*);                                                                  \
+  template void gaussResidual(FieldStateT<R>&, const DeviceArray<double>&, const std::size_t*,                \
+                              const std::size_t*, double*);
+APRL_DIAG_INSTANTIATE(double)
+APRL_DIAG_INSTANTIATE(float)
+#undef APRL_DIAG_INSTANTIATE
 
 }  // namespace aprl_gpu

@@ -5,20 +5,30 @@
 
 namespace aprl_gpu {
 
+// All sums run in FP64 on either form of the grid (R = double or float).
+
 // Sum of fieldEnergyDensityCell * dx dy dz over the cells of [lo, hi).
-double interiorEnergy(FieldState& s, const std::size_t lo[3], const std::size_t hi[3]);
+template <class R>
+double interiorEnergy(FieldStateT<R>& s, const std::size_t lo[3], const std::size_t hi[3]);
 // Outward Poynting power through the six faces of the cell box [lo, hi):
 // power[2a] = -sum over the lower face, power[2a+1] = +sum over the upper.
-void facePower(FieldState& s, const std::size_t lo[3], const std::size_t hi[3], double power[6]);
-// Shaped vertex charge density (shape 0 quintic, 2 binomial3-x).
+template <class R>
+void facePower(FieldStateT<R>& s, const std::size_t lo[3], const std::size_t hi[3], double power[6]);
+// Shaped vertex charge density (shape 0 quintic, 2 binomial3-x) of
+// particles at absolute positions x, or (split form) at lattice cell +
+// offset with the grid origin at lattice cell oc.
 void chargeDensity(FieldState& s, int shape, const double* const x[3], const double* q, int n, const double origin[3],
                    DeviceArray<double>& rho);
+void chargeDensitySplit(FieldState32& s, int shape, const int* const cell[3], const float* const off[3],
+                        const float* q, int n, const int oc[3], DeviceArray<double>& rho);
 // out = {max |(rho1 - rho0)/dt + div J|, max |(rho1 - rho0)/dt|}.
-void continuityResidual(FieldState& s, const DeviceArray<double>& rho0, const DeviceArray<double>& rho1,
+template <class R>
+void continuityResidual(FieldStateT<R>& s, const DeviceArray<double>& rho0, const DeviceArray<double>& rho1,
                         double out[2]);
 // out = {sum error^2, sum source^2, max |error|, max |source|} over the
 // vertices strictly inside [lo, hi], error = div E - rho / eps0.
-void gaussResidual(FieldState& s, const DeviceArray<double>& rho, const std::size_t lo[3], const std::size_t hi[3],
+template <class R>
+void gaussResidual(FieldStateT<R>& s, const DeviceArray<double>& rho, const std::size_t lo[3], const std::size_t hi[3],
                    double out[4]);
 
 }  // namespace aprl_gpu

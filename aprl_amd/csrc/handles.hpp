@@ -10,8 +10,11 @@ struct aprl_fields {
 };
 struct aprl_particles {
   aprl_fields* fields;
-  aprl_gpu::ParticleStore p;
+  aprl_gpu::ParticleStore p;  // plain FP64 form
+  aprl_gpu::SplitStore sp;    // FP32-split form
+  hipStream_t stream() const { return fields->fp32 ? fields->s32.stream : fields->s.stream; }
 };
 struct aprl_coupling {
   aprl_gpu::CouplingState c;
+  aprl_fields* fields = nullptr;
 };

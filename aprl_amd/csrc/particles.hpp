@@ -62,13 +62,33 @@ struct ParticleStore {
   std::size_t sortTempBytes = 0;
 };
 
+// FP32-split particle store: per axis the absolute lattice cell (int32,
+// floor(x / h) with the lattice anchored at x = 0) and the FP32 offset in
+// [0, 1) inside it, so that neither long runs nor the moving window lose
+// position bits; momenta, charge, mass and weight in FP32.
+struct SplitStore {
+  std::size_t capacity = 0, count = 0;
+  std::uint64_t generation = 0;  // bumped by every host upload
+  DeviceArray<int> cell[3], cellT[3];
+  DeviceArray<float> s[9], t[9];  // offset x y z, u x y z, q m w
+  DeviceArray<uint64_t> id, idT;
+  // per-step slots as in ParticleStore: alternating gathered fields, the
+  // trial end state (cell1, offset1, u1) and the per-step work sum
+  DeviceArray<float> fields[2][6], end[6], gridAbs;
+  DeviceArray<int> endCell[3];
+  DeviceArray<unsigned> key, keySorted, perm, permSorted;
+  DeviceArray<unsigned char> sortTemp;
+  std::size_t sortTempBytes = 0;
+};
+
 // Particles per work chunk: one chunk is a contiguous run of tile-sorted
 // particles of one tile. The deposit gives each chunk one workgroup of
 // kChunk lanes; the gathers give it 2*kChunk lanes (E waves and B waves).
 constexpr int kChunk = 128;
 
 struct CouplingState {
-  FieldState* fields = nullptr;
+  FieldState* fields = nullptr;      // plain FP64 grid, or
+  FieldState32* fields32 = nullptr;  // FP32-split grid
   int shape = 2;
   bool smoothX = true;
   BoostDev frame{1.0, 0.0, 0.0, kC, 0.0, 0.0, 0.0};
@@ -80,6 +100,11 @@ struct CouplingState {
   // E_cand + E_old of the current trial (the preview writes the zero-current
   // sums on the whole grid; the current reduction rewrites the active tiles)
   DeviceArray<double> esum[3];
+  DeviceArray<float> estar32[3], esum32[3];  // the same for an FP32-split grid
+  // FP32-split form: Picard trials per step (no host round trip inside a
+  // step) and whether a last residual gather reports force and work
+  unsigned splitTrials = 2;
+  bool splitVerify = false;
   // tiles: particle range, chunk count (+1 entry for the scan total), first
   // chunk, fixed-point exponent of the tile's current
   DeviceArray<int> tileBegin, tileEnd, tileShift, activeTiles;
@@ -99,7 +124,7 @@ struct CouplingState {
   // in the store's end arrays until the next step)
   struct {
     bool valid = false;
-    const ParticleStore* store = nullptr;
+    const void* store = nullptr;  // ParticleStore or SplitStore
     std::uint64_t generation = 0;
     double origin[3] = {0.0, 0.0, 0.0};
   } lastStep;
@@ -114,5 +139,15 @@ std::size_t chunkListBound(int ntiles, int n);
 void fillInts(hipStream_t st, int* p, int n, int v);
 void reduceChunkStats(hipStream_t st, const double* in, int chunks, double* out);
 void raiseParticleErrors(unsigned e);
+void reserveParticles(ParticleStore& P, std::size_t capacity);
+
+// FP32-split form (particles32.hip).
+void reserveSplitParticles(SplitStore& P, std::size_t capacity);
+void uploadSplitParticles(SplitStore& P, const FieldState32& F, std::size_t n, const double* x3, const double* u3,
+                          const double* q, const double* m, const double* w, const uint64_t* id);
+void downloadSplitParticles(const SplitStore& P, const FieldState32& F, double* x3, double* u3, double* q, double* m,
+                            double* w, uint64_t* id);
+void couplingStepSplit(CouplingState& C, SplitStore& P, const double origin[3], double time, unsigned substeps,
+                       aprl_step_report* report);
 
 }  // namespace aprl_gpu

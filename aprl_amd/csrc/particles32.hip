@@ -1,0 +1,1013 @@
+// gfx950 particle kernels of the conservative-source step in the FP32-split
+// form: fields, currents, gathered fields and momenta in FP32; positions as
+// an absolute int32 lattice cell plus an FP32 in-cell offset.
+//
+// The step is the FP64 form's (tile sort, orbit gathers from LDS-staged
+// tile boxes, Lapenta-Markidis push, fixed-point tile deposition, owner
+// reduction) with these differences:
+//  * shape weights, chord cuts and gathers run in FP32 on coordinates taken
+//    relative to the particle's start cell, so the in-cell offset keeps its
+//    24 bits whatever the absolute position;
+//  * the push converts its inputs to FP64 and reuses the FP64 orbit push
+//    (prescribed magnets, Newton root, boundary checks) on absolute box
+//    coordinates, then stores the end point as cell + FP32 offset again;
+//  * the Picard loop runs a fixed number of trials (at least the two the
+//    reference requires with active particles, simulation_source_coupling.cpp:
+//    50): its convergence tolerances (force < 1e-13) lie below FP32
+//    resolution, and a fixed count keeps the host out of the step.
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+
+#include "particle_device.hpp"
+#include "shapes.hpp"
+
+namespace aprl_gpu {
+
+// ---------------------------------------------------------------------------
+// Kernel argument blocks.
+
+struct GArgs32 {
+  float* esum[3];         // E_cand + E_old of the trial
+  const float* estar[3];  // zero-current preview E*
+  const float* eold[3];   // E^n
+  const float* b[3];      // B^(n+1/2)
+  int ne[3][3], nb[3][3];
+  int nx, ny, nz;
+  int oc[3];       // absolute lattice cell of the grid origin
+  float ch[3];     // c dt / h per axis (ballistic guess, cells per unit u/gamma)
+  float negS;      // (-1*dt)/eps0
+  int smoothX;
+};
+
+struct PArgs32 {
+  const int* cell[3];
+  const float* off[3];
+  const float* u[3];
+  const float* q;
+  const float* m;
+  float* fin[6];
+  float* fout[6];
+  int* cell1[3];
+  float* off1[3];
+  float* u1[3];
+  float* gridAbs;
+  float kq[3];  // h_a / (dx dy dz dt): current density per unit charge and cell of motion
+  double h[3];
+  int n;
+};
+
+// ---------------------------------------------------------------------------
+// Chord of one particle relative to its start cell.
+
+struct Chord32 {
+  int org[3];        // grid index of the start cell
+  float x[3], v[3];  // start offset and displacement, in cells
+};
+
+__device__ __forceinline__ Chord32 chordOf(const GArgs32& G, int c0x, int c0y, int c0z, const float off0[3],
+                                           const float v[3]) {
+  Chord32 r;
+  r.org[0] = c0x - G.oc[0];
+  r.org[1] = c0y - G.oc[1];
+  r.org[2] = c0z - G.oc[2];
+  for (int a = 0; a < 3; ++a) {
+    r.x[a] = off0[a];
+    r.v[a] = v[a];
+  }
+  return r;
+}
+
+// clipOpenOrbit (source_coupling_math.h:235-250) in start-cell coordinates:
+// the grid spans [-org, n - org) per axis. On success the chord is replaced
+// by its clipped part and the kept fraction is returned in kept.
+__device__ bool clipChord(const GArgs32& G, Chord32& c, float& kept, unsigned& err) {
+  float first = 0.f, last = 1.f;
+  const int size[3] = {G.nx, G.ny, G.nz};
+  bool valid = true;
+  for (int a = 0; a < 3; ++a) {
+    const float x = c.x[a], d = c.v[a];
+    if (!isfinite(x) || !isfinite(d)) { err |= kErrNonfinite; return false; }
+    const float lo = static_cast<float>(-c.org[a]) - x, hi = static_cast<float>(size[a] - c.org[a]) - x;
+    if (d == 0.f) {
+      if (lo > 0.f || hi < 0.f) valid = false;
+      continue;
+    }
+    const float tlo = lo / d, thi = hi / d;
+    if (d > 0.f) { first = fmaxf(first, tlo); last = fminf(last, thi); }
+    else { last = fminf(last, tlo); first = fmaxf(first, thi); }
+  }
+  if (!valid || !(last >= first)) return false;
+  for (int a = 0; a < 3; ++a) {
+    const float d = c.v[a];
+    if (first != 0.f) c.x[a] += first * d;
+    c.v[a] = (last - first) * d;
+  }
+  kept = last - first;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Gathers from the staged tile box.
+
+template <int SHAPE, bool WITH_B>
+__device__ void stageTile32(const GArgs32& G, const TileBox<SHAPE>& box, float* es, float* bs) {
+  constexpr int N = boxNodes<SHAPE>();
+  constexpr int kIters = (3 * N + 2 * kChunk - 1) / (2 * kChunk);
+  float ev[kIters], bv[kIters];
+#pragma unroll
+  for (int r = 0; r < kIters; ++r) {
+    const int t = static_cast<int>(threadIdx.x) + r * 2 * kChunk;
+    ev[r] = 0.f;
+    bv[r] = 0.f;
+    if (t < 3 * N) {
+      const int d = t / N, q0 = t - d * N;
+      const int li = q0 % Box<SHAPE>::NX, q = q0 / Box<SHAPE>::NX, lj = q % Box<SHAPE>::NY, lk = q / Box<SHAPE>::NY;
+      const int i = box.x0 + li, j = box.y0 + lj, k = box.z0 + lk;
+      if (i >= 0 && j >= 0 && k >= 0) {
+        if (i < G.ne[d][0] && j < G.ne[d][1] && k < G.ne[d][2])
+          ev[r] = G.esum[d][(static_cast<size_t>(k) * G.ne[d][1] + j) * G.ne[d][0] + i];
+        if (WITH_B && i < G.nb[d][0] && j < G.nb[d][1] && k < G.nb[d][2])
+          bv[r] = G.b[d][(static_cast<size_t>(k) * G.nb[d][1] + j) * G.nb[d][0] + i];
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kIters; ++r) {
+    const int t = static_cast<int>(threadIdx.x) + r * 2 * kChunk;
+    if (t < 3 * N) {
+      es[t] = ev[r];
+      if (WITH_B) bs[t] = bv[r];
+    }
+  }
+}
+
+// Impl::electric (source_coupling.cpp:31-39): orbit average of
+// 1/2 (E_cand + E_old) over the clipped chord with the deposit's weights.
+template <int SHAPE>
+__device__ void gatherE32(const GArgs32& G, const float* es, const TileBox<SHAPE>& box, Chord32 c, float res[3],
+                          unsigned& err) {
+  constexpr int N = boxNodes<SHAPE>(), NX = Box<SHAPE>::NX, NY = Box<SHAPE>::NY;
+  res[0] = res[1] = res[2] = 0.f;
+  float kept = 1.f;
+  if (!clipChord(G, c, kept, err)) { err |= kErrOrbitOutside; return; }
+  float cuts[kMaxCuts];
+  const int count = orbitCuts(c.x, c.v, cuts, err);
+  int base = 0;
+  auto interior = [&](long long i, long long j, long long k, int wx, int wy, int wz) {
+    if (i < 0 || j < 0 || k < 0 || i + wx > G.nx + 1 || j + wy > G.ny + 1 || k + wz > G.nz + 1) return false;
+    if (SHAPE == kShapeBinomial3X) {
+      if (i + 3 > G.nx || j + 1 > G.ny || k + 3 > G.nz) return false;
+    } else {
+      if (i + 1 > G.nx || j + 1 > G.ny || k + 11 > G.nz) return false;
+    }
+    if (!box.holds(i, j, k, wx, wy, wz)) return false;
+    base = box.base(i, j, k);
+    return true;
+  };
+  auto fast = [&](int d, int oi, int oj, int ok, float w) {
+    res[d] += w * es[d * N + base + (ok * NY + oj) * NX + oi];
+  };
+  auto visit = [&](int d, long long ii, long long jj, long long kk, float w) {
+    if (ii < 0 || jj < 0 || ii >= G.ne[d][0] || jj >= G.ne[d][1]) return;
+    if (kk < 0 || kk >= G.ne[d][2]) return;
+    const int l = box.local(static_cast<int>(ii), static_cast<int>(jj), static_cast<int>(kk), err);
+    res[d] += w * (l < 0 ? 0.f : es[d * N + l]);
+  };
+  for (int s = 1; s < count; ++s) {
+    if (cuts[s] - cuts[s - 1] <= 0.f) continue;
+    segmentTaps<SHAPE, float>(G.nx, G.ny, c.org, c.x, c.v, cuts[s - 1], cuts[s], interior, fast, visit);
+  }
+  const float f = 0.5f * kept;
+  res[0] *= f;
+  res[1] *= f;
+  res[2] *= f;
+}
+
+__device__ __forceinline__ int clampIndex32(int v, int n) { return v < 0 ? 0 : (v > n - 1 ? n - 1 : v); }
+
+// Impl::magnetic for binomial3-x (source_coupling_math.h:142-167) at the
+// clipped chord midpoint, clamped indices, from the staged box.
+__device__ void gatherBBinomial32(const GArgs32& G, const float* bs, const TileBox<kShapeBinomial3X>& box, Chord32 c,
+                                  float res[3], unsigned& err) {
+  constexpr int N = boxNodes<kShapeBinomial3X>(), NX = Box<kShapeBinomial3X>::NX, NY = Box<kShapeBinomial3X>::NY;
+  res[0] = res[1] = res[2] = 0.f;
+  float kept;
+  if (!clipChord(G, c, kept, err)) { err |= kErrMagneticOutside; return; }
+  float mid[3];
+  for (int d = 0; d < 3; ++d) mid[d] = c.x[d] + 0.5f * c.v[d];
+  for (int comp = 0; comp < 3; ++comp) {
+    int cell[3];
+    float frac[3];
+    for (int d = 0; d < 3; ++d) {
+      const float pos = mid[d] - (comp == d ? 0.f : 0.5f);
+      const float fl = floorf(pos);
+      cell[d] = static_cast<int>(fl) + c.org[d];
+      frac[d] = pos - fl;
+    }
+    float wx[4], wz[4];
+    nodeWeights(frac[0], wx);
+    nodeWeights(frac[2], wz);
+    const float wy[2] = {1.f - frac[1], frac[1]};
+    const int* n = G.nb[comp];
+    if (cell[0] >= 1 && cell[0] + 2 <= n[0] - 1 && cell[1] >= 0 && cell[1] + 1 <= n[1] - 1 && cell[2] >= 1 &&
+        cell[2] + 2 <= n[2] - 1 && box.holds(cell[0] - 1, cell[1], cell[2] - 1, 4, 2, 4)) {
+      const float* bc = bs + comp * N + box.base(cell[0] - 1, cell[1], cell[2] - 1);
+      for (int k = 0; k < 4; ++k)
+        for (int j = 0; j < 2; ++j) {
+          float row = 0.f;
+          for (int i = 0; i < 4; ++i) row += wx[i] * bc[(k * NY + j) * NX + i];
+          res[comp] += wz[k] * wy[j] * row;
+        }
+      continue;
+    }
+    for (int k = 0; k < 4; ++k)
+      for (int j = 0; j < 2; ++j) {
+        const int kk = clampIndex32(cell[2] - 1 + k, n[2]), jj = clampIndex32(cell[1] + j, n[1]);
+        float row = 0.f;
+        for (int i = 0; i < 4; ++i) {
+          const int l = box.local(clampIndex32(cell[0] - 1 + i, n[0]), jj, kk, err);
+          row += wx[i] * (l < 0 ? 0.f : bs[comp * N + l]);
+        }
+        res[comp] += wz[k] * wy[j] * row;
+      }
+  }
+}
+
+// Impl::magnetic for the quintic shape (source_coupling_math.h:122-138):
+// trilinear sample at the clipped chord midpoint, clamped indices, from HBM.
+__device__ void gatherBQuintic32(const GArgs32& G, Chord32 c, float res[3], unsigned& err) {
+  res[0] = res[1] = res[2] = 0.f;
+  float kept;
+  if (!clipChord(G, c, kept, err)) { err |= kErrMagneticNoDomain; return; }
+  int cell[2][3];
+  float fraction[2][3];
+  for (int half = 0; half < 2; ++half)
+    for (int d = 0; d < 3; ++d) {
+      const float x = c.x[d] + 0.5f * c.v[d] - 0.5f * half;
+      const float fl = floorf(x);
+      cell[half][d] = static_cast<int>(fl) + c.org[d];
+      fraction[half][d] = x - fl;
+    }
+  for (int axis = 0; axis < 3; ++axis) {
+    const int hx = axis == 0 ? 0 : 1, hy = axis == 1 ? 0 : 1, hz = axis == 2 ? 0 : 1;
+    const int* n = G.nb[axis];
+    const float* src = G.b[axis];
+    for (int k = 0; k < 2; ++k)
+      for (int j = 0; j < 2; ++j)
+        for (int i = 0; i < 2; ++i) {
+          const float xw = i ? fraction[hx][0] : 1.f - fraction[hx][0];
+          const float yw = j ? fraction[hy][1] : 1.f - fraction[hy][1];
+          const float zw = k ? fraction[hz][2] : 1.f - fraction[hz][2];
+          const int ii = clampIndex32(cell[hx][0] + i, n[0]), jj = clampIndex32(cell[hy][1] + j, n[1]),
+                    kk = clampIndex32(cell[hz][2] + k, n[2]);
+          res[axis] += xw * yw * zw * src[(static_cast<size_t>(kk) * n[1] + jj) * n[0] + ii];
+        }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sort.
+
+__global__ void kTileKeys32(const int* cx, const int* cy, const int* cz, int n, GArgs32 G, int ntx, int nty,
+                            unsigned* key, unsigned* perm) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int x = clampIndex32(cx[p] - G.oc[0], G.nx);
+  const int y = clampIndex32(cy[p] - G.oc[1], G.ny);
+  const int z = clampIndex32(cz[p] - G.oc[2], G.nz);
+  key[p] = (static_cast<unsigned>(z / kTileZ) * nty + y / kTileY) * ntx + x / kTileX;
+  perm[p] = p;
+}
+
+struct PermuteArgs32 {
+  const int* cellSrc[3];
+  int* cellDst[3];
+  const float* src[9];
+  float* dst[9];
+  const uint64_t* idSrc;
+  uint64_t* idDst;
+};
+
+__global__ void kPermute32(PermuteArgs32 A, const unsigned* perm, int n) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const unsigned q = perm[p];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) A.cellDst[a][p] = A.cellSrc[a][q];
+#pragma unroll
+  for (int a = 0; a < 9; ++a) A.dst[a][p] = A.src[a][q];
+  A.idDst[p] = A.idSrc[q];
+}
+
+// ---------------------------------------------------------------------------
+// Gathers, push, deposit, reduction.
+
+__device__ __forceinline__ void chordAt(const PArgs32& P, int p, bool guess, const GArgs32& G, int c0[3],
+                                        float off0[3], float v[3]) {
+  for (int d = 0; d < 3; ++d) {
+    c0[d] = P.cell[d][p];
+    off0[d] = P.off[d][p];
+  }
+  if (guess) {
+    // ballistic end point x0 + c dt u0 / gamma0 (source_coupling.cpp:133-147)
+    const float ux = P.u[0][p], uy = P.u[1][p], uz = P.u[2][p];
+    const float ig = rsqrtf(1.f + ux * ux + uy * uy + uz * uz);
+    v[0] = G.ch[0] * ux * ig;
+    v[1] = G.ch[1] * uy * ig;
+    v[2] = G.ch[2] * uz * ig;
+  } else {
+    for (int d = 0; d < 3; ++d)
+      v[d] = static_cast<float>(P.cell1[d][p] - c0[d]) + (P.off1[d][p] - off0[d]);
+  }
+}
+
+// Chunked orbit gathers (one workgroup of 2*kChunk lanes per chunk: E
+// lanes, then B lanes). MODE 0 gathers at (x0, ballistic guess) against the
+// zero-current preview into fin; MODE 1 is the residual gather at (x0, x1)
+// into fout, with the chunk's force max and defect / scale / call sums when
+// chunkOut is given.
+template <int MODE, int SHAPE>
+__global__ void __launch_bounds__(2 * kChunk) kGatherChunks32(PArgs32 P, GArgs32 G, const int4* chunks, int ntx,
+                                                              int nty, double dt, double* chunkOut, unsigned* errOut) {
+  constexpr bool kStageB = SHAPE == kShapeBinomial3X;
+  __shared__ float es[3 * boxNodes<SHAPE>()];
+  __shared__ float bs[kStageB ? 3 * boxNodes<SHAPE>() : 1];
+  __shared__ double red[MODE == 1 ? 4 * kChunk : 1];
+  const int4 ch = chunks[blockIdx.x];
+  const TileBox<SHAPE> box = tileBox<SHAPE>(ch.x, ntx, nty);
+  stageTile32<SHAPE, kStageB>(G, box, es, bs);
+  __syncthreads();
+  const bool bWaves = threadIdx.x >= kChunk;
+  const int lane = static_cast<int>(threadIdx.x) - (bWaves ? kChunk : 0);
+  const int p = ch.y + lane;
+  const bool live = p < ch.z;
+  unsigned err = 0;
+  float g[3] = {0.f, 0.f, 0.f};
+  int c0[3] = {0, 0, 0};
+  float off0[3] = {0.f, 0.f, 0.f}, v[3] = {0.f, 0.f, 0.f};
+  if (live) {
+    chordAt(P, p, MODE == 0, G, c0, off0, v);
+    const Chord32 c = chordOf(G, c0[0], c0[1], c0[2], off0, v);
+    if (bWaves) {
+      if constexpr (SHAPE == kShapeBinomial3X) gatherBBinomial32(G, bs, box, c, g, err);
+      else gatherBQuintic32(G, c, g, err);
+    } else {
+      gatherE32<SHAPE>(G, es, box, c, g, err);
+    }
+    float* const* dst = MODE == 0 ? P.fin : P.fout;
+    const int off = bWaves ? 3 : 0;
+    for (int d = 0; d < 3; ++d) dst[off + d][p] = g[d];
+  }
+  if (MODE == 1 && chunkOut) {
+    // localResidual (source_coupling.cpp:105-159) per chunk, fixed tree order
+    __syncthreads();
+    float* xb = es;  // B of the B lanes, for the E lanes
+    if (bWaves)
+      for (int d = 0; d < 3; ++d) xb[d * kChunk + lane] = g[d];
+    __syncthreads();
+    double force = 0., defect = 0., scale = 0., calls = 0.;
+    if (!bWaves && live) {
+      double de[3], db[3];
+      for (int d = 0; d < 3; ++d) {
+        de[d] = static_cast<double>(g[d]) - P.fin[d][p];
+        db[d] = static_cast<double>(xb[d * kChunk + lane]) - P.fin[3 + d][p];
+      }
+      const double q = P.q[p], m = P.m[p];
+      force = fabs(q / m) * dt / kC * (sqrt(de[0] * de[0] + de[1] * de[1] + de[2] * de[2]) +
+                                        kC * sqrt(db[0] * db[0] + db[1] * db[1] + db[2] * db[2]));
+      double w1 = 0., w2 = 0.;
+      for (int d = 0; d < 3; ++d) {
+        const double dx = static_cast<double>(v[d]) * P.h[d];
+        w1 += q * de[d] * dx;
+        w2 += q * g[d] * dx;
+      }
+      defect = fabs(w1);
+      scale = static_cast<double>(P.gridAbs[p]) + fabs(w2);
+      calls = 1.;
+    }
+    double* sf = red;
+    double* sd = red + kChunk;
+    double* ss = red + 2 * kChunk;
+    double* sc = red + 3 * kChunk;
+    if (!bWaves) {
+      sf[lane] = force;
+      sd[lane] = defect;
+      ss[lane] = scale;
+      sc[lane] = calls;
+    }
+    __syncthreads();
+    for (int w = kChunk / 2; w > 0; w >>= 1) {
+      if (static_cast<int>(threadIdx.x) < w) {
+        sf[threadIdx.x] = fmax(sf[threadIdx.x], sf[threadIdx.x + w]);
+        sd[threadIdx.x] += sd[threadIdx.x + w];
+        ss[threadIdx.x] += ss[threadIdx.x + w];
+        sc[threadIdx.x] += sc[threadIdx.x + w];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      chunkOut[4 * blockIdx.x + 0] = sf[0];
+      chunkOut[4 * blockIdx.x + 1] = sd[0];
+      chunkOut[4 * blockIdx.x + 2] = ss[0];
+      chunkOut[4 * blockIdx.x + 3] = sc[0];
+    }
+  }
+  if (err) atomicOr(errOut, err);
+}
+
+// One trial's push: FP32 state and fields widened to FP64, the FP64 orbit
+// push on absolute box coordinates (origin o = the grid origin cell times
+// h), and the end point split again into cell + FP32 offset.
+__global__ void __launch_bounds__(128) kPushSplit(PArgs32 P, StepArgs S, double o0, double o1, double o2,
+                                                  unsigned* errOut) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P.n) return;
+  unsigned err = 0;
+  const double o[3] = {o0, o1, o2};
+  int c0[3];
+  double x0[3], u0[3], usedE[3], usedB[3];
+  for (int a = 0; a < 3; ++a) {
+    c0[a] = P.cell[a][p];
+    x0[a] = (static_cast<double>(c0[a]) + static_cast<double>(P.off[a][p])) * P.h[a];
+    u0[a] = P.u[a][p];
+    usedE[a] = P.fin[a][p];
+    usedB[a] = P.fin[3 + a][p];
+  }
+  double x[3], u[3], absGrid;
+  pushChord(S, o, x0, u0, P.q[p], P.m[p], usedE, usedB, x, u, absGrid, err);
+  for (int a = 0; a < 3; ++a) {
+    const double rel = static_cast<double>(P.off[a][p]) + (x[a] - x0[a]) / P.h[a];
+    double fl = floor(rel);
+    float off = static_cast<float>(rel - fl);
+    if (off >= 1.f) {  // rel - fl rounded up to 1
+      off = 0.f;
+      fl += 1.;
+    }
+    int cell = c0[a];
+    if (isfinite(fl) && fabs(fl) < 1e9) cell += static_cast<int>(fl);
+    else err |= kErrNonfinite;
+    P.cell1[a][p] = cell;
+    P.off1[a][p] = off;
+    P.u1[a][p] = static_cast<float>(u[a]);
+  }
+  P.gridAbs[p] = static_cast<float>(absGrid);
+  if (err) atomicOr(errOut, err);
+}
+
+// Deterministic tile deposition (Current::deposit, longitudinal_shape.h:
+// 175-187) of the FP32 chords into 64-bit fixed-point LDS tile boxes; the
+// scheme is the FP64 form's: one exponent per tile from the tile's current
+// bound, integer sums independent of order.
+template <int SHAPE>
+__global__ void __launch_bounds__(kChunk) kDepositSplit(PArgs32 P, GArgs32 G, const int4* chunks, const int* tileBegin,
+                                                        const int* tileEnd, const unsigned long long* tileScan,
+                                                        int ntx, int nty, long long* partial, int* tileShift,
+                                                        unsigned* errOut) {
+  constexpr int N = boxNodes<SHAPE>(), NX = Box<SHAPE>::NX, NY = Box<SHAPE>::NY;
+  __shared__ unsigned long long acc[3 * N];
+  __shared__ double bound[kChunk];
+  const int4 ch = chunks[blockIdx.x];
+  const int tile = ch.x;
+  const TileBox<SHAPE> box = tileBox<SHAPE>(tile, ntx, nty);
+  const int tb0 = tileBegin[tile], tb1 = tileEnd[tile];
+  for (int t = threadIdx.x; t < 3 * N; t += blockDim.x) acc[t] = 0ull;
+  // scale_a = q * dx_a / (dx dy dz dt), eb_deposition.cpp:41-51
+  auto scaleOf = [&](int p, const float v[3], float s[3]) {
+    const float q = P.q[p];
+    for (int d = 0; d < 3; ++d) s[d] = q * P.kq[d] * v[d];
+  };
+  double bsum = 0.0;
+  for (int p = tb0 + static_cast<int>(threadIdx.x); p < tb1; p += blockDim.x) {
+    int c0[3];
+    float off0[3], v[3], s[3];
+    chordAt(P, p, false, G, c0, off0, v);
+    scaleOf(p, v, s);
+    bsum += fmax(fabs(static_cast<double>(s[0])), fmax(fabs(static_cast<double>(s[1])), fabs(static_cast<double>(s[2]))));
+  }
+  bound[threadIdx.x] = bsum;
+  __syncthreads();
+  for (int w = kChunk / 2; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) bound[threadIdx.x] += bound[threadIdx.x + w];
+    __syncthreads();
+  }
+  int e2 = 0;
+  (void)frexp(Box<SHAPE>::kWeightBound * bound[0], &e2);
+  const int shift = 62 - e2;  // bound * 2^shift < 2^62
+  if (threadIdx.x == 0 && static_cast<int>(blockIdx.x) == static_cast<int>(tileScan[tile] & 0xffffffffull))
+    tileShift[tile] = shift;
+  unsigned err = 0;
+  const int p = ch.y + static_cast<int>(threadIdx.x);
+  if (p < ch.z) {
+    int c0[3];
+    float off0[3], v[3], scale[3];
+    chordAt(P, p, false, G, c0, off0, v);
+    scaleOf(p, v, scale);
+    const Chord32 c = chordOf(G, c0[0], c0[1], c0[2], off0, v);
+    float cuts[kMaxCuts];
+    const int count = orbitCuts(c.x, c.v, cuts, err);
+    int base = 0;
+    auto interior = [&](long long i, long long j, long long k, int wx, int wy, int wz) {
+      if (i < 0 || j < 0 || k < 0 || i + wx > G.nx + 1 || j + wy > G.ny + 1 || k + wz > G.nz + 1) return false;
+      if (SHAPE == kShapeBinomial3X) {
+        if (i + 3 > G.nx || j + 1 > G.ny || k + 3 > G.nz) return false;
+      } else {
+        if (i + 1 > G.nx || j + 1 > G.ny || k + 11 > G.nz) return false;
+      }
+      if (!box.holds(i, j, k, wx, wy, wz)) return false;
+      base = box.base(i, j, k);
+      return true;
+    };
+    auto fixedOf = [&](float value) {
+      return static_cast<unsigned long long>(llrint(ldexp(static_cast<double>(value), shift)));
+    };
+    auto fast = [&](int d, int oi, int oj, int ok, float w) {
+      if (scale[d] == 0.f) return;
+      atomicAdd(&acc[d * N + base + (ok * NY + oj) * NX + oi], fixedOf(scale[d] * w));
+    };
+    auto visit = [&](int d, long long ii, long long jj, long long kk, float w) {
+      if (scale[d] == 0.f) return;
+      if (ii < 0 || ii >= G.ne[d][0]) {
+        if (SHAPE != kShapeBinomial3X || !G.smoothX) err |= kErrCurrentX;
+        return;
+      }
+      if (jj < 0 || jj >= G.ne[d][1]) { err |= kErrCurrentY; return; }
+      if (kk < 0 || kk >= G.ne[d][2]) return;
+      const int l = box.local(static_cast<int>(ii), static_cast<int>(jj), static_cast<int>(kk), err);
+      if (l < 0) return;
+      atomicAdd(&acc[d * N + l], fixedOf(scale[d] * w));
+    };
+    for (int s = 1; s < count; ++s) {
+      if (cuts[s] - cuts[s - 1] <= 0.f) continue;
+      segmentTaps<SHAPE, float>(G.nx, G.ny, c.org, c.x, c.v, cuts[s - 1], cuts[s], interior, fast, visit);
+    }
+  }
+  __syncthreads();
+  const int first = static_cast<int>(tileScan[tile] & 0xffffffffull);
+  const bool alone = static_cast<int>(tileScan[tile + 1] & 0xffffffffull) - first == 1;
+  unsigned long long* out = reinterpret_cast<unsigned long long*>(partial + static_cast<size_t>(first) * 3 * N);
+  for (int t = threadIdx.x; t < 3 * N; t += blockDim.x) {
+    const unsigned long long val = acc[t];
+    if (alone) out[t] = val;
+    else if (val) atomicAdd(out + t, val);
+  }
+  if (err) atomicOr(errOut, err);
+}
+
+// Node-owner reduction of the tile partials into FP32 J (sums in FP64, one
+// rounding), clearing what it read; with WRITE_ESUM it also writes
+// E_cand + E_old (applyCurrentSource(-1) on the preview,
+// source_coupling.cpp:84-89) for the next residual gather.
+template <int SHAPE, bool WRITE_ESUM>
+__global__ void __launch_bounds__(256) kReduceCurrentSplit(float* jx, float* jy, float* jz, GArgs32 G,
+                                                           const int* active, const unsigned long long* tileScan,
+                                                           const int* tileShift, long long* partial, int ntx,
+                                                           int nty, int ntz) {
+  constexpr int N = boxNodes<SHAPE>(), NX = Box<SHAPE>::NX, NY = Box<SHAPE>::NY, NZ = Box<SHAPE>::NZ;
+  constexpr int RZ = Box<SHAPE>::RZ, NB = 9 * (2 * RZ + 1);
+  const int tile = active[blockIdx.x];
+  const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+  __shared__ int nbFirst[NB], nbShift[NB];
+  if (threadIdx.x < NB) {
+    const int nb = static_cast<int>(threadIdx.x);
+    const int x = tx + nb % 3 - 1, y = ty + (nb / 3) % 3 - 1, z = tz + nb / 9 - RZ;
+    int first = -1, sh = 0;
+    if (x >= 0 && x < ntx && y >= 0 && y < nty && z >= 0 && z < ntz) {
+      const int s = (z * nty + y) * ntx + x;
+      const int f = static_cast<int>(tileScan[s] & 0xffffffffull);
+      if (static_cast<int>(tileScan[s + 1] & 0xffffffffull) > f) {
+        first = f;
+        sh = tileShift[s];
+      }
+    }
+    nbFirst[nb] = first;
+    nbShift[nb] = sh;
+  }
+  __syncthreads();
+  constexpr size_t chunkStride = 3 * static_cast<size_t>(N);
+  for (int d = 0; d < 3; ++d) {
+    const int n0 = G.ne[d][0], n1 = G.ne[d][1], n2 = G.ne[d][2];
+    const int i0 = tx * kTileX, i1 = tx == ntx - 1 ? n0 : min(n0, i0 + kTileX);
+    const int j0 = ty * kTileY, j1 = ty == nty - 1 ? n1 : min(n1, j0 + kTileY);
+    const int k0 = tz * kTileZ, k1 = tz == ntz - 1 ? n2 : min(n2, k0 + kTileZ);
+    const int wx = i1 - i0, wy = j1 - j0, wz = k1 - k0;
+    float* dst = d == 0 ? jx : (d == 1 ? jy : jz);
+    for (int t = threadIdx.x; t < wx * wy * wz; t += blockDim.x) {
+      const int i = i0 + t % wx, j = j0 + (t / wx) % wy, k = k0 + t / (wx * wy);
+      long long v[NB];
+      long long* at[NB];
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const int ox = nb % 3 - 1, oy = (nb / 3) % 3 - 1, oz = nb / 9 - RZ;
+        const int li = i - ((tx + ox) * kTileX - Box<SHAPE>::X0), lj = j - ((ty + oy) * kTileY - Box<SHAPE>::Y0),
+                  lk = k - ((tz + oz) * kTileZ - Box<SHAPE>::Z0);
+        const int f = nbFirst[nb];
+        const bool on = f >= 0 && li >= 0 && li < NX && lj >= 0 && lj < NY && lk >= 0 && lk < NZ;
+        at[nb] = on ? partial + static_cast<size_t>(f) * chunkStride + d * N + (lk * NY + lj) * NX + li : nullptr;
+        v[nb] = on ? *at[nb] : 0;
+      }
+      double sum = 0.0;
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb)
+        if (at[nb]) {
+          sum += ldexp(static_cast<double>(v[nb]), -nbShift[nb]);
+          *at[nb] = 0;
+        }
+      const size_t idx = (static_cast<size_t>(k) * n1 + j) * n0 + i;
+      const float jv = static_cast<float>(sum);
+      dst[idx] = jv;
+      if (WRITE_ESUM) {
+        const bool applied = d == 0 ? (j > 0 && j < G.ny && k != 0 && k != G.nz)
+                                    : (d == 1 ? (i > 0 && i < G.nx && k != 0 && k != G.nz)
+                                              : (i > 0 && i < G.nx && j > 0 && j < G.ny));
+        float e = G.estar[d][idx];
+        if (applied && jv != 0.f) e += G.negS * jv;
+        G.esum[d][idx] = e + G.eold[d][idx];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+void reserveSplitParticles(SplitStore& P, std::size_t capacity) {
+  if (capacity <= P.capacity) return;
+  for (int a = 0; a < 3; ++a) {
+    P.cell[a].resize(capacity);
+    P.cellT[a].resize(capacity);
+    P.endCell[a].resize(capacity);
+  }
+  for (int a = 0; a < 9; ++a) {
+    P.s[a].resize(capacity);
+    P.t[a].resize(capacity);
+  }
+  P.id.resize(capacity);
+  P.idT.resize(capacity);
+  for (int a = 0; a < 6; ++a) {
+    P.fields[0][a].resize(capacity);
+    P.fields[1][a].resize(capacity);
+    P.end[a].resize(capacity);
+  }
+  P.gridAbs.resize(capacity);
+  P.key.resize(capacity);
+  P.keySorted.resize(capacity);
+  P.perm.resize(capacity);
+  P.permSorted.resize(capacity);
+  P.capacity = capacity;
+  P.sortTempBytes = 0;
+  hipCheck(rocprim::radix_sort_pairs(nullptr, P.sortTempBytes, P.key.get(), P.keySorted.get(), P.perm.get(),
+                                     P.permSorted.get(), static_cast<unsigned>(capacity), 0, 32),
+           "radix_sort_pairs size");
+  P.sortTempBytes = std::max<std::size_t>(P.sortTempBytes, 256);
+  P.sortTemp.resize(P.sortTempBytes);
+}
+
+namespace {
+// Split absolute coordinate x (metres) on the lattice of spacing h.
+inline void splitCoordinate(double x, double h, int& cell, float& off) {
+  const double r = x / h;
+  if (!std::isfinite(r) || std::fabs(r) >= 2.0e9)
+    throw std::out_of_range("Particle position is outside the FP32-split lattice range");
+  double fl = std::floor(r);
+  float f = static_cast<float>(r - fl);
+  if (f >= 1.0f) {
+    f = 0.0f;
+    fl += 1.0;
+  }
+  cell = static_cast<int>(fl);
+  off = f;
+}
+}  // namespace
+
+void uploadSplitParticles(SplitStore& P, const FieldState32& F, std::size_t n, const double* x3, const double* u3,
+                          const double* q, const double* m, const double* w, const uint64_t* id) {
+  reserveSplitParticles(P, n);
+  const double h[3] = {F.g.dx, F.g.dy, F.g.dz};
+  std::vector<int> cells(n);
+  std::vector<float> col(n);
+  for (int a = 0; a < 3; ++a) {
+    for (std::size_t i = 0; i < n; ++i) splitCoordinate(x3[3 * i + a], h[a], cells[i], col[i]);
+    hipCheck(hipMemcpy(P.cell[a].get(), cells.data(), n * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+    hipCheck(hipMemcpy(P.s[a].get(), col.data(), n * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+    for (std::size_t i = 0; i < n; ++i) col[i] = static_cast<float>(u3[3 * i + a]);
+    hipCheck(hipMemcpy(P.s[3 + a].get(), col.data(), n * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+  }
+  const double* scalars[3] = {q, m, w};
+  for (int a = 0; a < 3; ++a) {
+    for (std::size_t i = 0; i < n; ++i) col[i] = static_cast<float>(scalars[a][i]);
+    hipCheck(hipMemcpy(P.s[6 + a].get(), col.data(), n * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+  }
+  hipCheck(hipMemcpy(P.id.get(), id, n * sizeof(uint64_t), hipMemcpyHostToDevice), "hipMemcpy");
+  P.count = n;
+  ++P.generation;
+}
+
+void downloadSplitParticles(const SplitStore& P, const FieldState32& F, double* x3, double* u3, double* q, double* m,
+                            double* w, uint64_t* id) {
+  hipCheck(hipStreamSynchronize(F.stream), "hipStreamSynchronize");
+  const std::size_t n = P.count;
+  const double h[3] = {F.g.dx, F.g.dy, F.g.dz};
+  std::vector<int> cells(n);
+  std::vector<float> col(n);
+  for (int a = 0; a < 3; ++a) {
+    if (x3) {
+      hipCheck(hipMemcpy(cells.data(), P.cell[a].get(), n * sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy");
+      hipCheck(hipMemcpy(col.data(), P.s[a].get(), n * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+      for (std::size_t i = 0; i < n; ++i)
+        x3[3 * i + a] = (static_cast<double>(cells[i]) + static_cast<double>(col[i])) * h[a];
+    }
+    if (u3) {
+      hipCheck(hipMemcpy(col.data(), P.s[3 + a].get(), n * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+      for (std::size_t i = 0; i < n; ++i) u3[3 * i + a] = col[i];
+    }
+  }
+  double* scalars[3] = {q, m, w};
+  for (int a = 0; a < 3; ++a)
+    if (scalars[a]) {
+      hipCheck(hipMemcpy(col.data(), P.s[6 + a].get(), n * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+      for (std::size_t i = 0; i < n; ++i) scalars[a][i] = col[i];
+    }
+  if (id) hipCheck(hipMemcpy(id, P.id.get(), n * sizeof(uint64_t), hipMemcpyDeviceToHost), "hipMemcpy");
+}
+
+namespace {
+
+GArgs32 gatherArgs32(CouplingState& C, const FieldState32& F, const int oc[3]) {
+  GArgs32 G{};
+  for (int a = 0; a < 3; ++a) {
+    G.esum[a] = C.esum32[a].get();
+    G.estar[a] = C.estar32[a].get();
+    G.eold[a] = F.base(a);
+    G.b[a] = F.base(3 + a);
+    for (int d = 0; d < 3; ++d) {
+      G.ne[a][d] = static_cast<int>(F.dims[a][d]);
+      G.nb[a][d] = static_cast<int>(F.dims[3 + a][d]);
+    }
+    G.oc[a] = oc[a];
+  }
+  G.nx = static_cast<int>(F.g.nx);
+  G.ny = static_cast<int>(F.g.ny);
+  G.nz = static_cast<int>(F.g.nz);
+  G.ch[0] = static_cast<float>(F.g.dt * kC / F.g.dx);
+  G.ch[1] = static_cast<float>(F.g.dt * kC / F.g.dy);
+  G.ch[2] = static_cast<float>(F.g.dt * kC / F.g.dz);
+  G.negS = static_cast<float>(-1.0 * F.g.dt / kEps0);
+  G.smoothX = C.smoothX ? 1 : 0;
+  return G;
+}
+
+PArgs32 particleArgs32(SplitStore& P, const FieldState32& F) {
+  PArgs32 A{};
+  for (int a = 0; a < 3; ++a) {
+    A.cell[a] = P.cell[a].get();
+    A.off[a] = P.s[a].get();
+    A.u[a] = P.s[3 + a].get();
+    A.cell1[a] = P.endCell[a].get();
+    A.off1[a] = P.end[a].get();
+    A.u1[a] = P.end[3 + a].get();
+  }
+  for (int a = 0; a < 6; ++a) {
+    A.fin[a] = P.fields[0][a].get();
+    A.fout[a] = P.fields[1][a].get();
+  }
+  A.q = P.s[6].get();
+  A.m = P.s[7].get();
+  A.gridAbs = P.gridAbs.get();
+  const double h[3] = {F.g.dx, F.g.dy, F.g.dz};
+  const double den = F.g.dx * F.g.dy * F.g.dz * F.g.dt;
+  for (int a = 0; a < 3; ++a) {
+    A.h[a] = h[a];
+    A.kq[a] = static_cast<float>(h[a] / den);
+  }
+  A.n = static_cast<int>(P.count);
+  return A;
+}
+
+int bitsFor32(unsigned v) {
+  int b = 1;
+  while (b < 32 && (v >> b) != 0u) ++b;
+  return b;
+}
+
+void sortSplit(CouplingState& C, SplitStore& P, const GArgs32& G, int ntx, int nty, int ntiles, unsigned* occupied) {
+  FieldState32& F = *C.fields32;
+  hipStream_t st = F.stream;
+  const int n = static_cast<int>(P.count);
+  ScopedKernel sk(F.profiler, st, "sort");
+  const unsigned blocks = ceilDiv(n, 256);
+  kTileKeys32<<<blocks, 256, 0, st>>>(P.cell[0].get(), P.cell[1].get(), P.cell[2].get(), n, G, ntx, nty,
+                                      P.key.get(), P.perm.get());
+  launchCheck("tile keys");
+  const int bits = bitsFor32(static_cast<unsigned>(ntiles - 1));
+  std::size_t bytes = 0;
+  hipCheck(rocprim::radix_sort_pairs(nullptr, bytes, P.key.get(), P.keySorted.get(), P.perm.get(),
+                                     P.permSorted.get(), static_cast<unsigned>(n), 0, bits, st),
+           "radix_sort_pairs size");
+  if (bytes > P.sortTempBytes) {
+    P.sortTemp.resize(bytes);
+    P.sortTempBytes = bytes;
+  }
+  bytes = P.sortTempBytes;
+  hipCheck(rocprim::radix_sort_pairs(P.sortTemp.get(), bytes, P.key.get(), P.keySorted.get(), P.perm.get(),
+                                     P.permSorted.get(), static_cast<unsigned>(n), 0, bits, st),
+           "radix_sort_pairs");
+  PermuteArgs32 A{};
+  for (int a = 0; a < 3; ++a) {
+    A.cellSrc[a] = P.cell[a].get();
+    A.cellDst[a] = P.cellT[a].get();
+  }
+  for (int a = 0; a < 9; ++a) {
+    A.src[a] = P.s[a].get();
+    A.dst[a] = P.t[a].get();
+  }
+  A.idSrc = P.id.get();
+  A.idDst = P.idT.get();
+  kPermute32<<<blocks, 256, 0, st>>>(A, P.permSorted.get(), n);
+  launchCheck("permute");
+  for (int a = 0; a < 3; ++a) P.cell[a].swap(P.cellT[a]);
+  for (int a = 0; a < 9; ++a) P.s[a].swap(P.t[a]);
+  P.id.swap(P.idT);
+  const int rz = C.shape == kShapeQuintic ? Box<kShapeQuintic>::RZ : Box<kShapeBinomial3X>::RZ;
+  buildChunkLists(C, st, P.keySorted.get(), n, ntx, nty, ntiles, rz, occupied);
+}
+
+}  // namespace
+
+void couplingStepSplit(CouplingState& C, SplitStore& P, const double origin[3], double time, unsigned substeps,
+                       aprl_step_report* report) {
+  FieldState32& F = *C.fields32;
+  if (F.window && F.phase != 2) throw std::logic_error("Moving-window electric phase mismatch");
+  if (F.window && !F.inletSet) throw std::logic_error("Moving-window inlet planes were not supplied");
+  if (substeps == 0) throw std::invalid_argument("Particle substeps must be positive");
+  // The split positions live on the lattice x = cell * h; the grid origin
+  // must be one of its nodes.
+  const double h[3] = {F.g.dx, F.g.dy, F.g.dz};
+  int oc[3];
+  double o[3];
+  for (int a = 0; a < 3; ++a) {
+    const double r = origin[a] / h[a];
+    if (!std::isfinite(r) || std::fabs(r) >= 1.0e9 || std::fabs(r - std::nearbyint(r)) > 1e-6)
+      throw std::invalid_argument("An FP32-split grid origin must lie on the cell lattice (origin = integer * spacing)");
+    oc[a] = static_cast<int>(std::nearbyint(r));
+    o[a] = static_cast<double>(oc[a]) * h[a];
+  }
+  const int n = static_cast<int>(P.count);
+  hipStream_t st = F.stream;
+  launchClearCurrent(F);
+  for (int a = 0; a < 3; ++a)
+    if (C.estar32[a].size() != F.compSize(a)) {
+      C.estar32[a].resize(F.compSize(a));
+      C.esum32[a].resize(F.compSize(a));
+    }
+  const GArgs32 G = gatherArgs32(C, F, oc);
+  const int ntx = static_cast<int>((F.g.nx + kTileX - 1) / kTileX);
+  const int nty = static_cast<int>((F.g.ny + kTileY - 1) / kTileY);
+  const int ntz = static_cast<int>((F.g.nz + kTileZ - 1) / kTileZ);
+  const int ntiles = ntx * nty * ntz;
+  reserveTileLists(C, st, ntiles, n);
+  const std::size_t chunkBound = chunkListBound(ntiles, n);
+  const unsigned trials = std::max(2u, C.splitTrials);
+  if (C.error.size() < 2 + 64) { C.error.resize(2 + 64); C.reduceOut.resize(4); }
+  if (C.reduceOut.size() < 4) C.reduceOut.resize(4);
+  unsigned* pin = static_cast<unsigned*>(C.pinned.get(1024));
+  double* pinD = reinterpret_cast<double*>(static_cast<unsigned char*>(C.pinned.get(1024)) + 512);
+  fillInts(st, reinterpret_cast<int*>(C.error.get()), 2 + 64, 0);
+  if (n) {
+    sortSplit(C, P, G, ntx, nty, ntiles, C.error.get() + 1);
+    hipCheck(hipMemcpyAsync(pin, C.tileScan.get() + ntiles, sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
+             "hipMemcpyAsync");
+    hipCheck(hipMemcpyAsync(pin + 2, C.error.get() + 1, sizeof(unsigned), hipMemcpyDeviceToHost, st),
+             "hipMemcpyAsync");
+    hipCheck(hipEventRecord(C.sorted.get(), st), "hipEventRecord");
+  }
+  float* const out[3] = {C.estar32[0].get(), C.estar32[1].get(), C.estar32[2].get()};
+  float* const sums[3] = {C.esum32[0].get(), C.esum32[1].get(), C.esum32[2].get()};
+  launchElectric(F, F.ptrs(), 1, out, F.window ? F.inletPlane[0] : nullptr, F.window ? F.inletPlane[1] : nullptr,
+                 sums);
+  unsigned nChunks = 0, nActive = 0, occupiedTiles = 0;
+  if (n) {
+    hipCheck(hipEventSynchronize(C.sorted.get()), "hipEventSynchronize");
+    nChunks = pin[0];
+    nActive = pin[1];
+    occupiedTiles = pin[2];
+    if (nChunks > chunkBound) throw std::logic_error("Chunk list exceeds its bound");
+  }
+  const bool quintic = C.shape == kShapeQuintic;
+  const int boxN = quintic ? boxNodes<kShapeQuintic>() : boxNodes<kShapeBinomial3X>();
+  const std::size_t partialNeed = static_cast<std::size_t>(nChunks) * 3 * boxN;
+  if (partialNeed > C.partial.size()) {
+    C.partial.resize(partialNeed);
+    hipCheck(hipMemsetAsync(C.partial.get(), 0, partialNeed * sizeof(long long), st), "hipMemsetAsync");
+  }
+  if (C.reduceBlocks.size() < 4 * static_cast<std::size_t>(std::max(1u, nChunks)))
+    C.reduceBlocks.resize(4 * static_cast<std::size_t>(std::max(1u, nChunks)));
+
+  const StepArgs S = stepArgsFor(C, F.g, F.window, o, time, substeps);
+  PArgs32 A = particleArgs32(P, F);
+  const bool verify = C.splitVerify;
+  for (unsigned trial = 0; trial < trials; ++trial) {
+    unsigned* errT = C.error.get() + 2 + trial;
+    const bool last = trial + 1 == trials;
+    if (nChunks && trial == 0) {
+      ScopedKernel sk(F.profiler, st, "gather");
+      if (quintic)
+        kGatherChunks32<0, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G, C.chunks.get(), ntx, nty, F.g.dt,
+                                                                           nullptr, errT);
+      else
+        kGatherChunks32<0, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G, C.chunks.get(), ntx, nty,
+                                                                              F.g.dt, nullptr, errT);
+      launchCheck("gather");
+    }
+    if (n) {
+      ScopedKernel sk(F.profiler, st, "push");
+      kPushSplit<<<ceilDiv(n, 128), 128, 0, st>>>(A, S, o[0], o[1], o[2], errT);
+      launchCheck("push");
+    }
+    if (!nChunks) continue;
+    {
+      ScopedKernel sk(F.profiler, st, "deposit");
+      if (quintic)
+        kDepositSplit<kShapeQuintic><<<nChunks, kChunk, 0, st>>>(A, G, C.chunks.get(), C.tileBegin.get(),
+                                                                 C.tileEnd.get(), C.tileScan.get(), ntx, nty,
+                                                                 C.partial.get(), C.tileShift.get(), errT);
+      else
+        kDepositSplit<kShapeBinomial3X><<<nChunks, kChunk, 0, st>>>(A, G, C.chunks.get(), C.tileBegin.get(),
+                                                                    C.tileEnd.get(), C.tileScan.get(), ntx, nty,
+                                                                    C.partial.get(), C.tileShift.get(), errT);
+      launchCheck("deposit");
+    }
+    // the last trial's E_cand + E_old is read only by a verifying residual
+    const bool esum = !last || verify;
+    {
+      ScopedKernel sk(F.profiler, st, "reduce_current");
+      float* j[3] = {F.base(6), F.base(7), F.base(8)};
+#define APRL_REDUCE32(SH, W)                                                                                      \
+  kReduceCurrentSplit<SH, W><<<nActive, 256, 0, st>>>(j[0], j[1], j[2], G, C.activeTiles.get(), C.tileScan.get(), \
+                                                       C.tileShift.get(), C.partial.get(), ntx, nty, ntz)
+      if (quintic && esum) APRL_REDUCE32(kShapeQuintic, true);
+      else if (quintic) APRL_REDUCE32(kShapeQuintic, false);
+      else if (esum) APRL_REDUCE32(kShapeBinomial3X, true);
+      else APRL_REDUCE32(kShapeBinomial3X, false);
+#undef APRL_REDUCE32
+      launchCheck("reduce current");
+    }
+    F.currentZero = false;
+    F.footprint.valid = true;
+    F.footprint.active = C.activeTiles.get();
+    F.footprint.count = static_cast<int>(nActive);
+    F.footprint.ntx = ntx;
+    F.footprint.nty = nty;
+    F.footprint.ntz = ntz;
+    F.footprint.tx = kTileX;
+    F.footprint.ty = kTileY;
+    F.footprint.tz = kTileZ;
+    if (last && !verify) break;
+    {
+      ScopedKernel sk(F.profiler, st, "residual");
+      double* stats = last ? C.reduceBlocks.get() : nullptr;
+      if (quintic)
+        kGatherChunks32<1, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G, C.chunks.get(), ntx, nty, F.g.dt,
+                                                                           stats, errT);
+      else
+        kGatherChunks32<1, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G, C.chunks.get(), ntx, nty,
+                                                                              F.g.dt, stats, errT);
+      launchCheck("residual");
+      if (last) reduceChunkStats(st, C.reduceBlocks.get(), static_cast<int>(nChunks), C.reduceOut.get());
+    }
+    for (int a = 0; a < 6; ++a) std::swap(A.fin[a], A.fout[a]);
+  }
+  // commitConservativeElectric with the last trial's current
+  float* const none[3] = {nullptr, nullptr, nullptr};
+  launchElectric(F, F.ptrs(), 0, none, F.window ? F.inletPlane[0] : nullptr, F.window ? F.inletPlane[1] : nullptr);
+  if (F.window) F.phase = 3;
+  // one readback per step: the trial error words (and the verifying residual)
+  hipCheck(hipMemcpyAsync(pin + 4, C.error.get() + 2, trials * sizeof(unsigned), hipMemcpyDeviceToHost, st),
+           "hipMemcpyAsync");
+  if (verify && nChunks)
+    hipCheck(hipMemcpyAsync(pinD, C.reduceOut.get(), 4 * sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+  hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+  for (unsigned t = 0; t < trials; ++t) raiseParticleErrors(pin[4 + t]);
+  double force = 0., work = 0.;
+  if (verify && nChunks) {
+    force = pinD[0];
+    work = pinD[3] == 0. ? 0. : (pinD[2] > 0. ? pinD[1] / pinD[2] : pinD[1]);
+  }
+  for (int a = 0; a < 3; ++a) P.cell[a].swap(P.endCell[a]);
+  for (int a = 0; a < 6; ++a) P.s[a].swap(P.end[a]);
+  C.lastStep.valid = true;
+  C.lastStep.store = &P;
+  C.lastStep.generation = P.generation;
+  for (int a = 0; a < 3; ++a) C.lastStep.origin[a] = o[a];
+  if (report) {
+    report->iterations = trials;
+    report->force = force;
+    report->work = work;
+    report->tiles = occupiedTiles;
+    report->chunks = nChunks;
+  }
+}
+
+}  // namespace aprl_gpu

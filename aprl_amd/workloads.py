@@ -72,13 +72,13 @@ BENDING = Workload(
     charge_electrons=100_000.0, dipole_tesla=0.5, dipole_length=0.2, seed=20241, z_cells=150.0)
 
 
-def setup_gpu(w: Workload, shape: str = "linear-binomial3"):
+def setup_gpu(w: Workload, shape: str = "linear-binomial3", precision: str = "fp64"):
     """Field grid + window + coupling + particle store for workload w;
     shape is "linear-binomial3" (with x-binomial3 smoothing) or
-    "quintic-compensated"."""
+    "quintic-compensated", precision "fp64" or "fp32-split"."""
     import aprl_amd as A
     g = A.EBGridGeometry(*w.dims, *w.spacing, w.dt)
-    fields = A.EBFieldGrid(g)
+    fields = A.EBFieldGrid(g, precision=precision)
     window = A.MovingWindowFields(fields, A.EBCPMLParameters((w.cpml, w.cpml, 0)))
     coupling = A.ConservativeSourceCoupling(fields, shape)
     coupling.setFrame(w.gamma_boost)

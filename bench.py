@@ -54,12 +54,10 @@ def kernel_bytes(w, name, tiles, chunks):
     return None
 
 
-def run_gpu(args, shape="linear-binomial3", profile=True):
-    import numpy as np
-    import aprl_amd as A
+def run_gpu(args, shape="linear-binomial3", profile=True, precision="fp64"):
     from aprl_amd import workloads
     w = workloads.BENDING
-    fields, window, coupling, store = workloads.setup_gpu(w, shape)
+    fields, window, coupling, store = workloads.setup_gpu(w, shape, precision)
     s = workloads.Stepper(w, fields, window, coupling, store)
     for _ in range(args.warmup):
         s.step()
@@ -88,6 +86,33 @@ def run_gpu(args, shape="linear-binomial3", profile=True):
     fields.profile(False)
     x, u, q, m, wt, ids = store.download()
     return w, elapsed, trials, kernels, (coupling.last_tiles, coupling.last_chunks)
+
+
+def split_error(steps=100, shape="linear-binomial3"):
+    """Relative L2 error of the FP32-split form against the plain FP64 form
+    (itself within 1e-12 of the oracle per step) after `steps` steps of the
+    same workload from the same state: max over E and B components, and u."""
+    import numpy as np
+    from aprl_amd import workloads
+    w = workloads.BENDING
+    runs = []
+    for precision in ("fp64", "fp32-split"):
+        fields, window, coupling, store = workloads.setup_gpu(w, shape, precision)
+        s = workloads.Stepper(w, fields, window, coupling, store)
+        for _ in range(steps):
+            s.step()
+        x, u, _, _, _, ids = store.download()
+        order = np.argsort(ids)
+        runs.append(([fields.component(a) for a in range(6)], u[order], x[order]))
+        del s, fields, window, coupling, store
+
+    def rel(a, b):
+        d = np.linalg.norm(b)
+        return float(np.linalg.norm(a - b) / (d if d > 0 else 1.0))
+    (f64, u64, x64), (f32, u32, x32) = runs
+    return {"steps": steps, "e": max(rel(f32[a], f64[a]) for a in range(3)),
+            "b": max(rel(f32[a], f64[a]) for a in range(3, 6)), "u": rel(u32, u64),
+            "x_cells": float(np.abs((x32 - x64) / np.array(w.spacing)).max())}
 
 
 def run_cpu_baseline(w, seconds_budget=25.0):
@@ -157,6 +182,10 @@ def main():
     w, elapsed, trials, kernels, (tiles, chunks) = run_gpu(args)
     # the other production shape of the same workload, timed the same way
     _, q_elapsed, q_trials, _, _ = run_gpu(args, "quintic-compensated", profile=False)
+    # the FP32-split form of both shapes
+    _, s_elapsed, s_trials, s_kernels, _ = run_gpu(args, precision="fp32-split")
+    _, sq_elapsed, _, _, _ = run_gpu(args, "quintic-compensated", profile=False, precision="fp32-split")
+    s_error = split_error()
     ms = 1e3 * elapsed / args.steps
     value = w.particles * args.steps / elapsed
     modeled = {k: v for k, v in kernels.items() if kernel_bytes(w, k, tiles, chunks) is not None}
@@ -190,6 +219,12 @@ def main():
                    "occupied_tiles": tiles, "chunks": chunks},
         "quintic": {"value": w.particles * args.steps / q_elapsed, "ms_per_step": 1e3 * q_elapsed / args.steps,
                     "picard_trials_mean": sum(q_trials) / len(q_trials)},
+        "fp32_split": {"value": w.particles * args.steps / s_elapsed, "ms_per_step": 1e3 * s_elapsed / args.steps,
+                       "vs_fp64": round(elapsed / s_elapsed, 3), "picard_trials": s_trials[0],
+                       "quintic_value": w.particles * args.steps / sq_elapsed,
+                       "quintic_vs_fp64": round(q_elapsed / sq_elapsed, 3),
+                       "error_vs_fp64": s_error,
+                       "kernels_us": {k: round(v["avg_us"], 2) for k, v in s_kernels.items()}},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels_us": {k: round(v["avg_us"], 2) for k, v in kernels.items()},

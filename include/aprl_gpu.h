@@ -121,7 +121,10 @@ typedef struct aprl_coupling aprl_coupling;
 /* Device store of RelativisticParticleSI (eb_particles.h:12-25) on the
  * field grid's device and stream. Positions are absolute box coordinates in
  * metres, u = gamma v / c. The store's order changes at every coupled step
- * (stable sort by cell tile); ids travel with their particles. */
+ * (stable sort by cell tile); ids travel with their particles.
+ * On an APRL_FP32_SPLIT grid the store keeps per axis the lattice cell
+ * floor(x / h) (int32, lattice anchored at x = 0) and the FP32 offset inside
+ * it, and u, charge, mass and weight in FP32; upload and download convert. */
 aprl_status aprl_particles_create(aprl_fields* fields, size_t capacity, aprl_particles** out);
 void aprl_particles_destroy(aprl_particles* particles);
 /* position3/u3: interleaved (x,y,z) per particle; n <= capacity grows it. */
@@ -185,6 +188,14 @@ aprl_status aprl_coupling_set_particle_boundary(aprl_coupling* coupling, const s
  * particle substeps per field step. */
 aprl_status aprl_coupling_step(aprl_coupling* coupling, aprl_particles* particles, const double* origin,
                                double time, unsigned substeps, aprl_step_report* report);
+/* FP32-split grids only (no counterpart in the reference, which has no FP32
+ * form): the Picard loop of aprl_coupling_step runs exactly `trials` trials
+ * (2..64, default 2, the reference's minimum with active particles) in
+ * place of the FP64 convergence test, whose tolerances (force < 1e-13) lie
+ * below FP32 resolution. verify = 1 adds one residual gather after the last
+ * trial so that the step report carries its force and work (otherwise 0).
+ * On such a grid the step's origin must lie on the cell lattice. */
+aprl_status aprl_coupling_set_split_trials(aprl_coupling* coupling, unsigned trials, int verify);
 
 /* ---------------------------------------------------------------------------
  * FP64 diagnostics (compensated, fixed-order device reductions).

@@ -56,11 +56,11 @@ def test_energy_and_face_power_match_long_double_sums():
     assert np.all(np.abs(got - power.astype(float)) <= 1e-13 * scale), (got, power)
 
 
-def coupled_case(shape, seed, dims=(20, 20, 40), cells=(4, 4, 0), n=300):
+def coupled_case(shape, seed, dims=(20, 20, 40), cells=(4, 4, 0), n=300, precision="fp64"):
     import aprl_amd as A
     _, _, gshape = SHAPES[shape]
     dx = 2.5 * DZ
-    gpu = A.EBFieldGrid(A.EBGridGeometry(*dims, dx, dx, DZ, DZ / H.C_SI))
+    gpu = A.EBFieldGrid(A.EBGridGeometry(*dims, dx, dx, DZ, DZ / H.C_SI), precision=precision)
     win = A.MovingWindowFields(gpu, A.EBCPMLParameters(cells))
     cg = A.ConservativeSourceCoupling(gpu, gshape)
     cg.setFrame(5.0)

@@ -1,0 +1,227 @@
+"""FP32-split coupled step vs the FP64 oracle.
+
+The FP32-split form keeps fields, currents, gathered fields and momenta in
+FP32 and positions as an int32 lattice cell plus an FP32 in-cell offset, and
+runs a fixed number of Picard trials. It cannot match the FP64 oracle to
+round-off; the bars below are relative L2 errors after a few window steps
+with white-noise fields (the least favourable case for the FP32 field
+update), and exactness where the form promises it: the position split loses
+at most one FP32 rounding of the in-cell offset, ids survive the sort, and
+the trial count is the one requested.
+"""
+import numpy as np
+import pytest
+
+import harness as H
+from aprl_amd import bunch
+
+pytestmark = pytest.mark.gpu
+
+DZ = 1.0e-6
+DX = 2.5 * DZ
+SHAPES = {"binomial3": ("binomial3", True, "linear-binomial3"), "quintic": ("quintic", False, "quintic-compensated")}
+# FP32 fields after a few steps of white-noise window planes (the FP32 field
+# form alone reaches 2.3e-5 on such planes); momenta and chords accumulate
+# only FP32 roundings of the gathered fields.
+FIELD_TOL = 1e-4
+U_TOL = 1e-5
+X_TOL = 1e-6
+
+
+def rel_l2(a, b):
+    den = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (den if den > 0 else 1.0)
+
+
+def split_case(dims, cells, n_particles, seed, shape="binomial3", spread=(1.0, 1.0, 3.0)):
+    import aprl_amd as A
+    oshape, smooth, gshape = SHAPES[shape]
+    orc = H.CpuFields("orc", *dims, DX, DX, DZ, DZ / H.C_SI, 1)
+    orc.install_window(cells)
+    H.seeded_fields(orc, seed, scale=1e5)
+    co = H.CpuCoupling(orc, oshape, smooth, cpml_cells=cells)
+    co.set_frame(5.0)
+    co.add_magnet("dipole", [0, 0, -1.0], 2.0, peak=0.5)
+    gpu = A.EBFieldGrid(A.EBGridGeometry(*dims, DX, DX, DZ, DZ / H.C_SI), precision="fp32-split")
+    win = A.MovingWindowFields(gpu, A.EBCPMLParameters(cells))
+    for a in range(9):
+        gpu.set_component(a, orc.view(a))
+    cg = A.ConservativeSourceCoupling(gpu, gshape)
+    cg.setFrame(5.0)
+    cg.addPrescribedLabMagnet("uniform-dipole", [0, 0, -1.0], 2.0, peak_field=0.5)
+    cg.setParticleBoundary(cells)
+    nx, ny, nz = dims
+    center = [0.5 * nx * DX, 0.5 * ny * DX, 0.45 * nz * DZ]
+    x, u, q, m, ids = bunch.seeded_gaussian(n_particles, seed, center,
+                                            [spread[0] * DX, spread[1] * DX, spread[2] * DZ],
+                                            [0.0, 0.0, 10.0], [0.02, 0.02, 0.05], 3.0e5)
+    co.set_particles(x, u, q, m, np.ones(n_particles), ids)
+    store = A.ParticleStore(gpu, n_particles)
+    store.upload(x, u, q, m, np.ones(n_particles), ids)
+    return orc, co, gpu, win, cg, store
+
+
+def planes_for(rng, dims, comps):
+    return [rng.standard_normal(int(np.prod(H.component_shape(*dims, a)[1:]))) * (10.0 if a < 3 else 10.0 / H.C_SI)
+            for a in comps]
+
+
+def run_split(orc, co, gpu, win, cg, store, dims, steps, seed):
+    rng = np.random.default_rng(seed)
+    offset = 50
+    origin = np.zeros(3)
+    trials = []
+    for n in range(steps):
+        lower, inlet, shift = planes_for(rng, dims, range(3)), planes_for(rng, dims, range(2)), planes_for(rng, dims, range(6))
+        t = n * 1e-15
+        orc.clear_current(); gpu.clearCurrent()
+        orc.window_begin(n, offset, lower); win.beginStep(n, offset, lower)
+        orc.window_advance_magnetic(); gpu.advanceMagnetic()
+        orc.window_set_inlet(offset, n + 1, *inlet); win.setInlet(*inlet)
+        ro = co.step(origin, t)
+        rg = cg.step(store, origin, t)
+        trials.append((ro, rg))
+        if n + 1 < steps:
+            offset -= 1
+            orc.window_shift(n + 1, offset, shift); win.shift(n + 1, offset, shift)
+    return trials
+
+
+def errors(orc, co, gpu, store):
+    out = {f"c{a}": rel_l2(gpu.component(a), orc.view(a)) for a in range(9)}
+    xo, uo = co.particles()
+    xg, ug, _, _, _, ids = store.download()
+    order = np.argsort(ids)
+    assert np.array_equal(np.sort(ids), np.arange(1, len(ids) + 1, dtype=np.uint64))
+    out["x"] = rel_l2(xg[order] - xo.mean(0), xo - xo.mean(0))
+    out["u"] = rel_l2(ug[order], uo)
+    return out
+
+
+@pytest.mark.parametrize("shape", ["binomial3", "quintic"])
+def test_split_coupled_steps_track_fp64_oracle(shape):
+    dims = (20, 20, 40)
+    args = split_case(dims, (4, 4, 0), 400, 17, shape=shape)
+    trials = run_split(*args, dims, 4, 4)
+    assert all(g[0] == 2 for _, g in trials), trials
+    assert all(o[0] == 2 for o, _ in trials), trials  # the oracle converges in two trials here too
+    err = errors(args[0], args[1], args[2], args[5])
+    print(shape, {k: f"{v:.2e}" for k, v in err.items()})
+    for a in range(9):
+        assert err[f"c{a}"] <= FIELD_TOL, (a, err)
+    assert err["u"] <= U_TOL, err
+    assert err["x"] <= X_TOL, err
+    assert err["c6"] > 0 or err["c8"] > 0  # the FP32 form really rounds the current
+
+
+def test_split_dense_bunch_and_verifying_residual():
+    dims = (24, 24, 48)
+    orc, co, gpu, win, cg, store = split_case(dims, (4, 4, 0), 4000, 23, spread=(0.8, 0.8, 2.0))
+    cg.setSplitTrials(3, verify=True)
+    trials = run_split(orc, co, gpu, win, cg, store, dims, 3, 9)
+    assert all(g[0] == 3 for _, g in trials), trials
+    # the verifying residual of the last trial sits at FP32 resolution
+    assert all(0 < g[1] < 1e-6 for _, g in trials), trials
+    err = errors(orc, co, gpu, store)
+    print({k: f"{v:.2e}" for k, v in err.items()})
+    for a in range(9):
+        assert err[f"c{a}"] <= FIELD_TOL, (a, err)
+    assert err["u"] <= U_TOL, err
+
+
+def test_split_positions_round_trip_within_one_offset_rounding():
+    import aprl_amd as A
+    dims = (16, 16, 32)
+    gpu = A.EBFieldGrid(A.EBGridGeometry(*dims, DX, DX, DZ, DZ / H.C_SI), precision="fp32-split")
+    rng = np.random.default_rng(3)
+    n = 1000
+    # absolute positions far from the lattice origin, as after a long window run
+    x = rng.uniform(0, 1, (n, 3)) * np.array(dims) * [DX, DX, DZ] + [0.0, 0.0, 20000 * DZ]
+    u = rng.standard_normal((n, 3))
+    store = A.ParticleStore(gpu, n)
+    store.upload(x, u, -bunch.QE, bunch.ME, np.ones(n), np.arange(1, n + 1))
+    xg, ug, q, m, w, ids = store.download()
+    h = np.array([DX, DX, DZ])
+    # offset x/h - cell rounded once to FP32: error <= 2^-24 of a cell
+    assert np.all(np.abs(xg / h - x / h) <= 2.0 ** -24 * 1.0001), np.abs(xg / h - x / h).max()
+    assert np.array_equal(ug, u.astype(np.float32).astype(np.float64))
+    assert np.array_equal(ids, np.arange(1, n + 1, dtype=np.uint64))
+
+
+def test_split_origin_must_lie_on_the_lattice():
+    import aprl_amd as A
+    dims = (16, 16, 32)
+    cells = (4, 4, 0)
+    gpu = A.EBFieldGrid(A.EBGridGeometry(*dims, DX, DX, DZ, DZ / H.C_SI), precision="fp32-split")
+    win = A.MovingWindowFields(gpu, A.EBCPMLParameters(cells))
+    cg = A.ConservativeSourceCoupling(gpu, "linear-binomial3")
+    cg.setParticleBoundary(cells)
+    store = A.ParticleStore(gpu, 4)
+    x = np.array([[8 * DX, 8 * DX, 16 * DZ]] * 4)
+    store.upload(x, np.zeros((4, 3)), -bunch.QE, bunch.ME)
+    gpu.clearCurrent()
+    win.beginStep(0, 50)
+    gpu.advanceMagnetic()
+    win.setInlet()
+    with pytest.raises(A.AprlError) as e:
+        cg.step(store, np.array([0.0, 0.0, 0.3 * DZ]), 0.0)
+    assert "lattice" in str(e.value)
+    with pytest.raises(A.AprlError):
+        cg.setSplitTrials(1)
+
+
+def test_split_energy_ledger_sums_in_fp64():
+    """The ledger channels of an FP32 grid are FP64 sums of the widened FP32
+    fields: equal to long-double sums of the same rounded fields to 1e-13."""
+    import aprl_amd as A
+    dims = (14, 12, 18)
+    dx, dy = 2 * DZ, 1.5 * DZ
+    orc = H.CpuFields("orc", *dims, dx, dy, DZ, DZ / H.C_SI, 1)
+    H.seeded_fields(orc, 8, scale=1e4)
+    for a in range(9):  # the oracle holds the FP32-rounded fields
+        v = orc.view(a)
+        v[...] = v.astype(np.float32).astype(np.float64)
+    gpu = A.EBFieldGrid(A.EBGridGeometry(*dims, dx, dy, DZ, DZ / H.C_SI), precision="fp32-split")
+    for a in range(9):
+        gpu.set_component(a, orc.view(a))
+    lo, hi = (3, 2, 4), (11, 10, 15)
+    vol = np.longdouble(dx) * np.longdouble(dy) * np.longdouble(DZ)
+    ref = np.longdouble(0)
+    for k in range(lo[2], hi[2]):
+        for j in range(lo[1], hi[1]):
+            for i in range(lo[0], hi[0]):
+                ref += np.longdouble(orc.energy_density(i, j, k)) * vol
+    got = gpu.interiorFieldEnergy(lo, hi)
+    assert abs(got - float(ref)) <= 1e-13 * abs(float(ref)), (got, float(ref))
+    power = gpu.facePower(lo, hi)
+    assert np.all(np.isfinite(power)) and np.abs(power).max() > 0
+
+
+@pytest.mark.parametrize("shape", ["binomial3", "quintic"])
+def test_split_continuity_and_gauss_at_fp32_round_off(shape):
+    """Charge conservation of the split step: (rho1 - rho0)/dt + div J with
+    rho from the split positions (FP64) and J in FP32 stays at FP32 round-off
+    of the charge rate; a Gauss-consistent start keeps div E - rho/eps0 at
+    FP32 round-off of the source."""
+    from test_gpu_diagnostics import coupled_case, one_step
+    dims = (20, 20, 40)
+    gpu, win, cg, store, (x, q), dx = coupled_case(shape, 43, precision="fp32-split")
+    h = (dx, dx, DZ)
+    rho = H.shaped_charge(shape, x, q, dims, h)
+    ez = np.zeros((dims[2], dims[1] + 1, dims[0] + 1))
+    acc = np.zeros((dims[1] + 1, dims[0] + 1))
+    for k in range(dims[2]):
+        acc = acc + rho[k] * DZ / H.EPS0
+        ez[k] = acc
+    gpu.set_component(2, ez)
+    lo, hi = (6, 6, 4), (14, 14, 36)
+    before = cg.gaussResidual(store, np.zeros(3), lo, hi)
+    rng = np.random.default_rng(5)
+    one_step(gpu, win, cg, store, rng, dims)
+    residual, rate = cg.continuity(store)
+    after = cg.gaussResidual(store, np.zeros(3), lo, hi)
+    print(shape, "continuity", residual / rate, "gauss before", before[2] / before[3], "after", after[2] / after[3])
+    assert rate > 0
+    assert residual <= 1e-5 * rate, (residual, rate)
+    assert before[2] <= 1e-5 * before[3], before
+    assert after[2] <= 1e-5 * after[3], after
