@@ -92,6 +92,7 @@ def lib() -> C.CDLL:
     L.aprl_fields_destroy.restype = None
     L.aprl_fields_upload.argtypes = [vp, C.c_int, _dp, _sz]
     L.aprl_fields_download.argtypes = [vp, C.c_int, _dp, _sz]
+    L.aprl_fields_download_plane.argtypes = [vp, C.c_int, _sz, _dp, _sz]
     L.aprl_fields_component_size.argtypes = [vp, C.c_int]
     L.aprl_fields_component_size.restype = _sz
     L.aprl_fields_set_cpml.argtypes = [vp, C.POINTER(CpmlParameters), _sz, _sz, C.c_int, C.c_int]
@@ -162,6 +163,7 @@ def plane_array(planes):
 
 EXPORTED = [
     "aprl_last_error", "aprl_fields_create", "aprl_fields_destroy", "aprl_fields_upload", "aprl_fields_download",
+    "aprl_fields_download_plane",
     "aprl_fields_component_size", "aprl_fields_set_cpml", "aprl_fields_history_size",
     "aprl_fields_history_download", "aprl_fields_history_upload", "aprl_fields_clear_current",
     "aprl_fields_advance_magnetic", "aprl_fields_advance_electric", "aprl_fields_synchronize",

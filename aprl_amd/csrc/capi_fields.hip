@@ -164,6 +164,17 @@ aprl_status aprl_fields_download(aprl_fields* fields, int component, double* hos
   });
 }
 
+aprl_status aprl_fields_download_plane(aprl_fields* fields, int component, size_t k, double* host, size_t count) {
+  return guarded([&] {
+    withState(fields, [&](auto& s) {
+      checkComponent(component);
+      if (k >= s.dims[component][2]) throw std::out_of_range("Plane index is outside the component");
+      if (!host || count != s.plane(component)) throw std::invalid_argument("Plane download size mismatch");
+      toHost(s.stream, host, s.base(component) + k * s.plane(component), count);
+    });
+  });
+}
+
 aprl_status aprl_fields_set_cpml(aprl_fields* fields, const aprl_cpml_parameters* p, size_t globalNz,
                                  size_t zOffset, int lowerZ, int upperZ) {
   return guarded([&] {

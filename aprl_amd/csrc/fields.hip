@@ -584,16 +584,13 @@ __device__ __forceinline__ R rd(const R* p, int nx, int ny, int i, int j, int k)
 // and the moving-window inlet overwrite (moving_window_fields.cpp:276-280).
 // MODE 0 writes E in place and advances the histories; MODE 1 writes the
 // zero-current preview into `out` and leaves the histories untouched.
+// E update at lattice node (i, j, k) of the (nx+1) x (ny+1) x (nz+1) node
+// box: Ex (i < nx), Ey (j < ny) and Ez (k < nz) of that node.
 template <int MODE, class R>
-__global__ void __launch_bounds__(256) kElectric(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P, R* ox, R* oy,
-                                                  R* oz, R* sx, R* sy, R* sz, const R* inletX,
-                                                  const R* inletY) {
-  // one thread per node of the (nx+1) x (ny+1) plane k = blockIdx.y
+__device__ __forceinline__ void electricNode(const FieldPtrsT<R>& F, const StepCoefsT<R>& c, const CpmlArgsT<R>& P,
+                                             R* ox, R* oy, R* oz, R* sx, R* sy, R* sz, const R* inletX,
+                                             const R* inletY, int i, int j, int k) {
   const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = t % (nx + 1), j = t / (nx + 1);
-  const int k = blockIdx.y;
-  if (j > ny || k > nz) return;
   constexpr bool W = MODE == 0;
   const R* bx = F.b[0];
   const R* by = F.b[1];
@@ -667,6 +664,44 @@ __global__ void __launch_bounds__(256) kElectric(FieldPtrsT<R> F, StepCoefsT<R> 
   }
 }
 
+template <int MODE, class R>
+__global__ void __launch_bounds__(256) kElectric(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P, R* ox, R* oy,
+                                                  R* oz, R* sx, R* sy, R* sz, const R* inletX,
+                                                  const R* inletY) {
+  // one thread per node of the (nx+1) x (ny+1) plane k = blockIdx.y
+  const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t % (nx + 1), j = t / (nx + 1);
+  const int k = blockIdx.y;
+  if (j > ny || k > nz) return;
+  electricNode<MODE>(F, c, P, ox, oy, oz, sx, sy, sz, inletX, inletY, i, j, k);
+}
+
+// The zero-current preview (MODE 1 of kElectric) on the nodes owned by the
+// listed cell tiles only: count = high word of *total, the tiles in list
+// order, a workgroup per tile (looping when the grid is smaller). Every
+// node value is kElectric<1>'s.
+template <class R>
+__global__ void __launch_bounds__(256) kElectricPreviewTiles(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P,
+                                                             R* ox, R* oy, R* oz, R* sx, R* sy, R* sz,
+                                                             const R* inletX, const R* inletY, const int* tiles,
+                                                             const unsigned long long* total, TileGrid tg) {
+  const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
+  const int count = static_cast<int>(*total >> 32);
+  for (int b = blockIdx.x; b < count; b += gridDim.x) {
+    const int tile = tiles[b];
+    const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
+    const int i0 = tx * tg.tx, i1 = tx == tg.ntx - 1 ? nx + 1 : min(nx + 1, i0 + tg.tx);
+    const int j0 = ty * tg.ty, j1 = ty == tg.nty - 1 ? ny + 1 : min(ny + 1, j0 + tg.ty);
+    const int k0 = tz * tg.tz, k1 = tz == tg.ntz - 1 ? nz + 1 : min(nz + 1, k0 + tg.tz);
+    const int wx = i1 - i0, wy = j1 - j0, wz = k1 - k0;
+    for (int t = threadIdx.x; t < wx * wy * wz; t += blockDim.x) {
+      const int i = i0 + t % wx, j = j0 + (t / wx) % wy, k = k0 + t / (wx * wy);
+      electricNode<1>(F, c, P, ox, oy, oz, sx, sy, sz, inletX, inletY, i, j, k);
+    }
+  }
+}
+
 template <class R>
 __global__ void kFill(R* p, size_t n, R v) {
   for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < n;
@@ -719,6 +754,24 @@ void launchElectric(FieldStateT<R>& s, const FieldPtrsT<R>& f, int mode, R* cons
                                                sum ? sum[1] : nullptr, sum ? sum[2] : nullptr, inletX, inletY);
   }
   launchCheck("electric");
+}
+
+template <class R>
+void launchElectricPreviewTiles(FieldStateT<R>& s, const FieldPtrsT<R>& f, R* const out[3], R* const sum[3],
+                                const R* inletX, const R* inletY, const int* tiles, const unsigned long long* total,
+                                const TileGrid& tg) {
+  const StepCoefsT<R> c = s.coefs();
+  CpmlArgsT<R> P = s.cpmlArgs();
+  if (!s.hasCpml) {
+    P.lowerZ = s.lowerZ;
+    P.upperZ = s.upperZ;
+  }
+  const unsigned blocks = static_cast<unsigned>(std::min(tg.ntx * tg.nty * tg.ntz, 4096));
+  ScopedKernel sk(s.profiler, s.stream, "electric_preview");
+  kElectricPreviewTiles<R><<<std::max(1u, blocks), 256, 0, s.stream>>>(f, c, P, out[0], out[1], out[2], sum[0],
+                                                                        sum[1], sum[2], inletX, inletY, tiles,
+                                                                        total, tg);
+  launchCheck("electric preview tiles");
 }
 
 // Zero the owned J nodes of every listed cell tile (the tiles with an
@@ -860,6 +913,9 @@ void windowShift(FieldStateT<R>& s, const R* const planes[6]) {
   template void launchElectric<R>(FieldStateT<R>&, const FieldPtrsT<R>&, int, R* const*, const R*, const R*,  \
                                   R* const*);                                                                 \
   template void launchClearCurrent<R>(FieldStateT<R>&);                                                       \
+  template void launchElectricPreviewTiles<R>(FieldStateT<R>&, const FieldPtrsT<R>&, R* const*, R* const*,     \
+                                              const R*, const R*, const int*, const unsigned long long*,       \
+                                              const TileGrid&);                                                \
   template void windowShift<R>(FieldStateT<R>&, const R* const*);
 APRL_FIELDS_INSTANTIATE(double)
 APRL_FIELDS_INSTANTIATE(float)

@@ -61,6 +61,13 @@ struct CurrentFootprint {
   int ntx = 0, nty = 0, ntz = 0, tx = 0, ty = 0, tz = 0;
 };
 
+// Cell-tile decomposition of the grid: ntx x nty x ntz tiles of
+// tx x ty x tz cells; a tile owns the lattice nodes of its cells' lower
+// corners, and the last tile of an axis also the closing node.
+struct TileGrid {
+  int ntx, nty, ntz, tx, ty, tz;
+};
+
 // Device scratch of the diagnostics.
 struct DiagnosticScratch {
   DeviceArray<double> partial, out;
@@ -141,6 +148,13 @@ void launchMagnetic(FieldStateT<R>& s, const FieldPtrsT<R>& f);
 template <class R>
 void launchElectric(FieldStateT<R>& s, const FieldPtrsT<R>& f, int mode, R* const out[3], const R* inletX,
                     const R* inletY, R* const sum[3] = nullptr);
+// The MODE 1 preview of launchElectric (with sums) restricted to the nodes
+// owned by the first count tiles of `tiles`, count = high word of *total
+// (read on the device).
+template <class R>
+void launchElectricPreviewTiles(FieldStateT<R>& s, const FieldPtrsT<R>& f, R* const out[3], R* const sum[3],
+                                const R* inletX, const R* inletY, const int* tiles, const unsigned long long* total,
+                                const TileGrid& tg);
 template <class R>
 void launchClearCurrent(FieldStateT<R>& s);
 template <class R>

@@ -613,7 +613,6 @@ template <int SHAPE>
 __global__ void __launch_bounds__(256) kReduceCurrent(double* jx, double* jy, double* jz, GArgs G, const int* active,
                                                       const unsigned long long* tileScan, const int* tileShift,
                                                       long long* partial, int ntx, int nty, int ntz) {
-  constexpr int N = boxNodes<SHAPE>(), NX = Box<SHAPE>::NX, NY = Box<SHAPE>::NY, NZ = Box<SHAPE>::NZ;
   constexpr int RZ = Box<SHAPE>::RZ, NB = 9 * (2 * RZ + 1);
   const int tile = active[blockIdx.x];
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
@@ -634,7 +633,6 @@ __global__ void __launch_bounds__(256) kReduceCurrent(double* jx, double* jy, do
     nbShift[nb] = sh;
   }
   __syncthreads();
-  constexpr size_t chunkStride = 3 * static_cast<size_t>(N);
   for (int d = 0; d < 3; ++d) {
     const int n0 = G.ne[d][0], n1 = G.ne[d][1], n2 = G.ne[d][2];
     // owned node box (the last tile of an axis also owns the extra node)
@@ -645,25 +643,7 @@ __global__ void __launch_bounds__(256) kReduceCurrent(double* jx, double* jy, do
     double* dst = d == 0 ? jx : (d == 1 ? jy : jz);
     for (int t = threadIdx.x; t < wx * wy * wz; t += blockDim.x) {
       const int i = i0 + t % wx, j = j0 + (t / wx) % wy, k = k0 + t / (wx * wy);
-      long long v[NB];
-      long long* at[NB];
-#pragma unroll
-      for (int nb = 0; nb < NB; ++nb) {
-        const int ox = nb % 3 - 1, oy = (nb / 3) % 3 - 1, oz = nb / 9 - RZ;
-        const int li = i - ((tx + ox) * kTileX - Box<SHAPE>::X0), lj = j - ((ty + oy) * kTileY - Box<SHAPE>::Y0),
-                  lk = k - ((tz + oz) * kTileZ - Box<SHAPE>::Z0);
-        const int f = nbFirst[nb];
-        const bool on = f >= 0 && li >= 0 && li < NX && lj >= 0 && lj < NY && lk >= 0 && lk < NZ;
-        at[nb] = on ? partial + static_cast<size_t>(f) * chunkStride + d * N + (lk * NY + lj) * NX + li : nullptr;
-        v[nb] = on ? *at[nb] : 0;
-      }
-      double sum = 0.0;
-#pragma unroll
-      for (int nb = 0; nb < NB; ++nb)
-        if (at[nb]) {
-          sum += ldexp(static_cast<double>(v[nb]), -nbShift[nb]);
-          *at[nb] = 0;
-        }
+      const double sum = ownedNodeSum<SHAPE>(partial, nbFirst, nbShift, tx, ty, tz, d, i, j, k);
       const size_t idx = (static_cast<size_t>(k) * n1 + j) * n0 + i;
       dst[idx] = sum;
       const bool applied = d == 0 ? (j > 0 && j < G.ny && k != 0 && k != G.nz)
@@ -761,11 +741,21 @@ void reserveParticles(ParticleStore& P, std::size_t capacity) {
   P.permSorted.resize(capacity);
   P.capacity = capacity;
   P.sortTempBytes = 0;
-  hipCheck(rocprim::radix_sort_pairs(nullptr, P.sortTempBytes, P.key.get(), P.keySorted.get(), P.perm.get(),
-                                     P.permSorted.get(), static_cast<unsigned>(capacity), 0, 32),
-           "radix_sort_pairs size");
+  tileSort(nullptr, P.sortTempBytes, P.key.get(), P.keySorted.get(), P.perm.get(), P.permSorted.get(), static_cast<unsigned>(capacity), 32, nullptr);
   P.sortTempBytes = std::max<std::size_t>(P.sortTempBytes, 256);
   P.sortTemp.resize(P.sortTempBytes);
+}
+
+// Onesweep radix sort at every size above one block: rocprim's merge-sort
+// path for mid-size inputs costs one kernel per merge pass.
+using TileSortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                                  rocprim::default_config, 0>;
+
+void tileSort(void* temp, std::size_t& bytes, unsigned* key, unsigned* keySorted, unsigned* perm,
+              unsigned* permSorted, unsigned n, int bits, hipStream_t st) {
+  hipCheck(rocprim::radix_sort_pairs<TileSortConfig>(temp, bytes, key, keySorted, perm, permSorted, n, 0,
+                                                     static_cast<unsigned>(bits), st),
+           temp ? "radix_sort_pairs" : "radix_sort_pairs size");
 }
 
 void raiseParticleErrors(unsigned e) {
@@ -841,17 +831,13 @@ static void sortAndChunk(CouplingState& C, ParticleStore& P, const GArgs& G, int
   launchCheck("tile keys");
   const int bits = bitsFor(static_cast<unsigned>(ntiles - 1));
   std::size_t bytes = 0;
-  hipCheck(rocprim::radix_sort_pairs(nullptr, bytes, P.key.get(), P.keySorted.get(), P.perm.get(),
-                                     P.permSorted.get(), static_cast<unsigned>(n), 0, bits, st),
-           "radix_sort_pairs size");
+  tileSort(nullptr, bytes, P.key.get(), P.keySorted.get(), P.perm.get(), P.permSorted.get(), static_cast<unsigned>(n), bits, st);
   if (bytes > P.sortTempBytes) {
     P.sortTemp.resize(bytes);
     P.sortTempBytes = bytes;
   }
   bytes = P.sortTempBytes;
-  hipCheck(rocprim::radix_sort_pairs(P.sortTemp.get(), bytes, P.key.get(), P.keySorted.get(), P.perm.get(),
-                                     P.permSorted.get(), static_cast<unsigned>(n), 0, bits, st),
-           "radix_sort_pairs");
+  tileSort(P.sortTemp.get(), bytes, P.key.get(), P.keySorted.get(), P.perm.get(), P.permSorted.get(), static_cast<unsigned>(n), bits, st);
   PermuteArgs A{};
   for (int a = 0; a < 9; ++a) {
     A.src[a] = P.s[a].get();
@@ -971,10 +957,14 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
   }
   // Zero-current preview E* (MovingWindowFields::previewElectricBoundary);
   // it runs while the host waits for the chunk count.
+  // E* and E* + E^n are read only on the nodes of active tiles (gathers
+  // stage the boxes of occupied tiles; the reduction rewrites active tiles).
   double* const out[3] = {C.estar[0].get(), C.estar[1].get(), C.estar[2].get()};
   double* const sums[3] = {C.esum[0].get(), C.esum[1].get(), C.esum[2].get()};
-  launchElectric(F, F.ptrs(), 1, out, F.window ? F.inletPlane[0] : nullptr, F.window ? F.inletPlane[1] : nullptr,
-                 sums);
+  if (n)
+    launchElectricPreviewTiles(F, F.ptrs(), out, sums, F.window ? F.inletPlane[0] : nullptr,
+                               F.window ? F.inletPlane[1] : nullptr, C.activeTiles.get(), C.tileScan.get() + ntiles,
+                               TileGrid{ntx, nty, ntz, kTileX, kTileY, kTileZ});
   unsigned nChunks = 0, nActive = 0, occupiedTiles = 0;
   if (n) {
     hipCheck(hipEventSynchronize(C.sorted.get()), "hipEventSynchronize");

@@ -140,6 +140,10 @@ void fillInts(hipStream_t st, int* p, int n, int v);
 void reduceChunkStats(hipStream_t st, const double* in, int chunks, double* out);
 void raiseParticleErrors(unsigned e);
 void reserveParticles(ParticleStore& P, std::size_t capacity);
+// Stable sort of (tile key, particle index) pairs on the low `bits` key
+// bits; temp == nullptr returns the scratch size in bytes.
+void tileSort(void* temp, std::size_t& bytes, unsigned* key, unsigned* keySorted, unsigned* perm,
+              unsigned* permSorted, unsigned n, int bits, hipStream_t st);
 
 // FP32-split form (particles32.hip).
 void reserveSplitParticles(SplitStore& P, std::size_t capacity);

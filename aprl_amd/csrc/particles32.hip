@@ -565,7 +565,6 @@ __global__ void __launch_bounds__(256) kReduceCurrentSplit(float* jx, float* jy,
                                                            const int* active, const unsigned long long* tileScan,
                                                            const int* tileShift, long long* partial, int ntx,
                                                            int nty, int ntz) {
-  constexpr int N = boxNodes<SHAPE>(), NX = Box<SHAPE>::NX, NY = Box<SHAPE>::NY, NZ = Box<SHAPE>::NZ;
   constexpr int RZ = Box<SHAPE>::RZ, NB = 9 * (2 * RZ + 1);
   const int tile = active[blockIdx.x];
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
@@ -586,7 +585,6 @@ __global__ void __launch_bounds__(256) kReduceCurrentSplit(float* jx, float* jy,
     nbShift[nb] = sh;
   }
   __syncthreads();
-  constexpr size_t chunkStride = 3 * static_cast<size_t>(N);
   for (int d = 0; d < 3; ++d) {
     const int n0 = G.ne[d][0], n1 = G.ne[d][1], n2 = G.ne[d][2];
     const int i0 = tx * kTileX, i1 = tx == ntx - 1 ? n0 : min(n0, i0 + kTileX);
@@ -596,25 +594,7 @@ __global__ void __launch_bounds__(256) kReduceCurrentSplit(float* jx, float* jy,
     float* dst = d == 0 ? jx : (d == 1 ? jy : jz);
     for (int t = threadIdx.x; t < wx * wy * wz; t += blockDim.x) {
       const int i = i0 + t % wx, j = j0 + (t / wx) % wy, k = k0 + t / (wx * wy);
-      long long v[NB];
-      long long* at[NB];
-#pragma unroll
-      for (int nb = 0; nb < NB; ++nb) {
-        const int ox = nb % 3 - 1, oy = (nb / 3) % 3 - 1, oz = nb / 9 - RZ;
-        const int li = i - ((tx + ox) * kTileX - Box<SHAPE>::X0), lj = j - ((ty + oy) * kTileY - Box<SHAPE>::Y0),
-                  lk = k - ((tz + oz) * kTileZ - Box<SHAPE>::Z0);
-        const int f = nbFirst[nb];
-        const bool on = f >= 0 && li >= 0 && li < NX && lj >= 0 && lj < NY && lk >= 0 && lk < NZ;
-        at[nb] = on ? partial + static_cast<size_t>(f) * chunkStride + d * N + (lk * NY + lj) * NX + li : nullptr;
-        v[nb] = on ? *at[nb] : 0;
-      }
-      double sum = 0.0;
-#pragma unroll
-      for (int nb = 0; nb < NB; ++nb)
-        if (at[nb]) {
-          sum += ldexp(static_cast<double>(v[nb]), -nbShift[nb]);
-          *at[nb] = 0;
-        }
+      const double sum = ownedNodeSum<SHAPE>(partial, nbFirst, nbShift, tx, ty, tz, d, i, j, k);
       const size_t idx = (static_cast<size_t>(k) * n1 + j) * n0 + i;
       const float jv = static_cast<float>(sum);
       dst[idx] = jv;
@@ -658,9 +638,7 @@ void reserveSplitParticles(SplitStore& P, std::size_t capacity) {
   P.permSorted.resize(capacity);
   P.capacity = capacity;
   P.sortTempBytes = 0;
-  hipCheck(rocprim::radix_sort_pairs(nullptr, P.sortTempBytes, P.key.get(), P.keySorted.get(), P.perm.get(),
-                                     P.permSorted.get(), static_cast<unsigned>(capacity), 0, 32),
-           "radix_sort_pairs size");
+  tileSort(nullptr, P.sortTempBytes, P.key.get(), P.keySorted.get(), P.perm.get(), P.permSorted.get(), static_cast<unsigned>(capacity), 32, nullptr);
   P.sortTempBytes = std::max<std::size_t>(P.sortTempBytes, 256);
   P.sortTemp.resize(P.sortTempBytes);
 }
@@ -803,17 +781,13 @@ void sortSplit(CouplingState& C, SplitStore& P, const GArgs32& G, int ntx, int n
   launchCheck("tile keys");
   const int bits = bitsFor32(static_cast<unsigned>(ntiles - 1));
   std::size_t bytes = 0;
-  hipCheck(rocprim::radix_sort_pairs(nullptr, bytes, P.key.get(), P.keySorted.get(), P.perm.get(),
-                                     P.permSorted.get(), static_cast<unsigned>(n), 0, bits, st),
-           "radix_sort_pairs size");
+  tileSort(nullptr, bytes, P.key.get(), P.keySorted.get(), P.perm.get(), P.permSorted.get(), static_cast<unsigned>(n), bits, st);
   if (bytes > P.sortTempBytes) {
     P.sortTemp.resize(bytes);
     P.sortTempBytes = bytes;
   }
   bytes = P.sortTempBytes;
-  hipCheck(rocprim::radix_sort_pairs(P.sortTemp.get(), bytes, P.key.get(), P.keySorted.get(), P.perm.get(),
-                                     P.permSorted.get(), static_cast<unsigned>(n), 0, bits, st),
-           "radix_sort_pairs");
+  tileSort(P.sortTemp.get(), bytes, P.key.get(), P.keySorted.get(), P.perm.get(), P.permSorted.get(), static_cast<unsigned>(n), bits, st);
   PermuteArgs32 A{};
   for (int a = 0; a < 3; ++a) {
     A.cellSrc[a] = P.cell[a].get();
@@ -883,10 +857,13 @@ void couplingStepSplit(CouplingState& C, SplitStore& P, const double origin[3], 
              "hipMemcpyAsync");
     hipCheck(hipEventRecord(C.sorted.get(), st), "hipEventRecord");
   }
+  // E* and E* + E^n on the nodes of active tiles only (their only readers)
   float* const out[3] = {C.estar32[0].get(), C.estar32[1].get(), C.estar32[2].get()};
   float* const sums[3] = {C.esum32[0].get(), C.esum32[1].get(), C.esum32[2].get()};
-  launchElectric(F, F.ptrs(), 1, out, F.window ? F.inletPlane[0] : nullptr, F.window ? F.inletPlane[1] : nullptr,
-                 sums);
+  if (n)
+    launchElectricPreviewTiles(F, F.ptrs(), out, sums, F.window ? F.inletPlane[0] : nullptr,
+                               F.window ? F.inletPlane[1] : nullptr, C.activeTiles.get(), C.tileScan.get() + ntiles,
+                               TileGrid{ntx, nty, ntz, kTileX, kTileY, kTileZ});
   unsigned nChunks = 0, nActive = 0, occupiedTiles = 0;
   if (n) {
     hipCheck(hipEventSynchronize(C.sorted.get()), "hipEventSynchronize");

@@ -269,4 +269,43 @@ __device__ __forceinline__ void segmentTaps(int nx, int ny, const int org[3], co
   }
 }
 
+// Fixed-point tile partials (deposit side): the partial box of an occupied
+// tile sits at its first chunk's slot, 3 components of boxNodes<SHAPE>()
+// 64-bit integers scaled by 2^shift (shift per tile).
+//
+// Owner-side sum of lattice node (i, j, k) of component d of the tile
+// (tx, ty, tz): over the neighbour tiles whose extended box covers the node,
+// in ascending (z, y, x) tile order, the partial converted with that tile's
+// exponent; each entry read is cleared for the next deposit. nbFirst /
+// nbShift hold, per neighbour offset (oz + RZ, oy + 1, ox + 1), the first
+// chunk slot (-1: unoccupied) and exponent.
+template <int SHAPE>
+__device__ __forceinline__ double ownedNodeSum(long long* partial, const int* nbFirst, const int* nbShift, int tx,
+                                               int ty, int tz, int d, int i, int j, int k) {
+  constexpr int N = boxNodes<SHAPE>(), NX = Box<SHAPE>::NX, NY = Box<SHAPE>::NY, NZ = Box<SHAPE>::NZ;
+  constexpr int RZ = Box<SHAPE>::RZ;
+  // tile t' covers node i when t'*T - X0 <= i < t'*T - X0 + NX
+  auto lowTile = [](int v, int t0, int n, int tile) {
+    const int a = v + t0 - n + tile;  // floor((v + t0 - n) / tile) + 1 for v + t0 - n >= -tile
+    return a >= 0 ? a / tile : -((-a + tile - 1) / tile);
+  };
+  const int xLo = max(tx - 1, lowTile(i, Box<SHAPE>::X0, NX, kTileX)), xHi = min(tx + 1, (i + Box<SHAPE>::X0) / kTileX);
+  const int yLo = max(ty - 1, lowTile(j, Box<SHAPE>::Y0, NY, kTileY)), yHi = min(ty + 1, (j + Box<SHAPE>::Y0) / kTileY);
+  const int zLo = max(tz - RZ, lowTile(k, Box<SHAPE>::Z0, NZ, kTileZ)), zHi = min(tz + RZ, (k + Box<SHAPE>::Z0) / kTileZ);
+  double sum = 0.0;
+  for (int z = zLo; z <= zHi; ++z)
+    for (int y = yLo; y <= yHi; ++y)
+      for (int x = xLo; x <= xHi; ++x) {
+        const int nb = ((z - tz + RZ) * 3 + (y - ty + 1)) * 3 + (x - tx + 1);
+        const int f = nbFirst[nb];
+        if (f < 0) continue;
+        const int li = i - (x * kTileX - Box<SHAPE>::X0), lj = j - (y * kTileY - Box<SHAPE>::Y0),
+                  lk = k - (z * kTileZ - Box<SHAPE>::Z0);
+        long long* at = partial + static_cast<size_t>(f) * 3 * N + d * N + (lk * NY + lj) * NX + li;
+        sum += ldexp(static_cast<double>(*at), -nbShift[nb]);
+        *at = 0;
+      }
+  return sum;
+}
+
 }  // namespace aprl_gpu

@@ -102,6 +102,13 @@ class EBFieldGrid:
         check(lib().aprl_fields_download(self.handle, a, dptr(out), out.size))
         return out
 
+    def component_plane(self, a: int, k: int) -> np.ndarray:
+        """z plane k of component a, shape (ny_a, nx_a)."""
+        shape = component_shape(self.geometry, a)[1:]
+        out = np.empty(shape, dtype=np.float64)
+        check(lib().aprl_fields_download_plane(self.handle, a, k, dptr(out), out.size))
+        return out
+
     def set_component(self, a: int, values) -> None:
         arr = np.ascontiguousarray(values, dtype=np.float64)
         if arr.size != int(np.prod(component_shape(self.geometry, a))):

@@ -1,12 +1,15 @@
 """Benchmark of the MI355X conservative-source PIC step.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--error-study]
 
 One step = Simulation::solve's per-step order on the bending-radiation
 workload (64x64x512 Cowan-z moving window, 8-cell CFS-CPML, 1e5 electrons,
 0.5 T dipole, linear-binomial3 + x smoothing, FP64): clear J, magnetic
 update, conservative-source step (preview, Picard trials with gather / push /
-deposit / residual, commit), window shift. Prints one JSON line.
+deposit / residual, commit), window shift. Prints one JSON line: the plain
+FP64 form's throughput, with the quintic shape and the FP32-split form of
+both shapes timed the same way beside it. --error-study prints the
+FP32-split form's errors against FP64 at 100, 1000 and 10000 steps instead.
 """
 from __future__ import annotations
 
@@ -115,6 +118,50 @@ def split_error(steps=100, shape="linear-binomial3"):
             "x_cells": float(np.abs((x32 - x64) / np.array(w.spacing)).max())}
 
 
+def error_study(horizons=(100, 1000, 10000), shape="linear-binomial3"):
+    """FP32-split against the plain FP64 form on the co-moving drift bunch
+    (stays inside the window for 10^4 steps) at each horizon: relative L2
+    error of E and B (max over components), of u, the largest position
+    difference in cells, and of the power spectrum of Ex on the grid plane
+    through the bunch (sampled every step, |rfft over time|^2 summed over
+    the plane)."""
+    import numpy as np
+    from aprl_amd import workloads
+    w = workloads.COMOVING
+    forms = ("fp64", "fp32-split")
+    setups = {p: workloads.setup_gpu(w, shape, p) for p in forms}
+    steppers = {p: workloads.Stepper(w, *setups[p]) for p in forms}
+    k0 = w.dims[2] // 2
+    planes = {p: [] for p in forms}
+
+    def rel(a, b):
+        d = np.linalg.norm(b)
+        return float(np.linalg.norm(a - b) / (d if d > 0 else 1.0))
+    rows = []
+    n = 0
+    t0 = time.perf_counter()
+    for horizon in horizons:
+        while n < horizon:
+            for p in forms:
+                steppers[p].step()
+                planes[p].append(setups[p][0].component_plane(0, k0))
+            n += 1
+        f64, f32 = setups["fp64"][0], setups["fp32-split"][0]
+        us = []
+        for p in forms:
+            x, u, _, _, _, ids = setups[p][3].download()
+            order = np.argsort(ids)
+            us.append((x[order], u[order]))
+        spectra = [np.sum(np.abs(np.fft.rfft(np.array(planes[p]), axis=0)) ** 2, axis=(1, 2)) for p in forms]
+        rows.append({"steps": n, "e": max(rel(f32.component(a), f64.component(a)) for a in range(3)),
+                     "b": max(rel(f32.component(a), f64.component(a)) for a in range(3, 6)),
+                     "u": rel(us[1][1], us[0][1]),
+                     "x_cells": float(np.abs((us[1][0] - us[0][0]) / np.array(w.spacing)).max()),
+                     "plane_spectrum": rel(spectra[1], spectra[0])})
+        print(f"error study: {n} steps ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    return {"workload": w.name, "shape": shape, "plane_k": k0, "rows": rows}
+
+
 def run_cpu_baseline(w, seconds_budget=25.0):
     """The reference's own compiled hot path (oracle/_ref, OpenMP on the
     host cores) on the same workload for a bounded number of steps."""
@@ -174,11 +221,16 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--error-study", action="store_true",
+                    help="print the FP32-split vs FP64 errors at 100, 1000 and 10000 steps and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the whole window lives on one device); --gpus {args.gpus} "
               "is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.error_study:
+        print(json.dumps({"error_study": error_study()}))
+        return
     w, elapsed, trials, kernels, (tiles, chunks) = run_gpu(args)
     # the other production shape of the same workload, timed the same way
     _, q_elapsed, q_trials, _, _ = run_gpu(args, "quintic-compensated", profile=False)

@@ -71,6 +71,10 @@ void aprl_fields_destroy(aprl_fields* fields);
  * Ex Ey Ez Bx By Bz Jx Jy Jz; count must equal the component size. */
 aprl_status aprl_fields_upload(aprl_fields* fields, int component, const double* host, size_t count);
 aprl_status aprl_fields_download(aprl_fields* fields, int component, double* host, size_t count);
+/* One z plane k of a component (count = nx_a * ny_a values, x fastest): the
+ * plane read of LabDetectorManager::sampleFieldPlanes (lab_detectors.cpp:
+ * 2426-2487) without a whole-component copy. */
+aprl_status aprl_fields_download_plane(aprl_fields* fields, int component, size_t k, double* host, size_t count);
 size_t aprl_fields_component_size(const aprl_fields* fields, int component);
 
 /* EBConvolutionalPML::EBConvolutionalPML + EBFieldGrid::setBoundary
