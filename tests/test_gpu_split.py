@@ -104,7 +104,9 @@ def test_split_coupled_steps_track_fp64_oracle(shape):
     args = split_case(dims, (4, 4, 0), 400, 17, shape=shape)
     trials = run_split(*args, dims, 4, 4)
     assert all(g[0] == 2 for _, g in trials), trials
-    assert all(o[0] == 2 for o, _ in trials), trials  # the oracle converges in two trials here too
+    # the oracle's own count (2 or 3 here): a third FP64 trial moves the
+    # state by less than FP32 resolution (its force is below 1e-17)
+    assert all(o[0] >= 2 for o, _ in trials), trials
     err = errors(args[0], args[1], args[2], args[5])
     print(shape, {k: f"{v:.2e}" for k, v in err.items()})
     for a in range(9):
@@ -120,8 +122,11 @@ def test_split_dense_bunch_and_verifying_residual():
     cg.setSplitTrials(3, verify=True)
     trials = run_split(orc, co, gpu, win, cg, store, dims, 3, 9)
     assert all(g[0] == 3 for _, g in trials), trials
-    # the verifying residual of the last trial sits at FP32 resolution
-    assert all(0 < g[1] < 1e-6 for _, g in trials), trials
+    # the verifying residual after the last trial sits at FP32 resolution
+    # (by the third trial the FP32 iteration typically repeats its fields
+    # exactly, so force and work can be 0)
+    print("verifying residual (force, work):", [(g[1], g[2]) for _, g in trials])
+    assert all(0 <= g[1] < 1e-6 and 0 <= g[2] < 1e-5 for _, g in trials), trials
     err = errors(orc, co, gpu, store)
     print({k: f"{v:.2e}" for k, v in err.items()})
     for a in range(9):
