@@ -117,17 +117,19 @@ FieldPtrsT<R> FieldStateT<R>::ptrs() const {
   return f;
 }
 
-// Coefficients are evaluated in double and rounded once to R.
+// Coefficients stay in double for both storage types: the FP32 form stores
+// its fields in FP32 but evaluates every update in FP64 and rounds the
+// result once, so no coefficient rounding biases its dispersion.
 template <class R>
 StepCoefsT<R> FieldStateT<R>::coefs() const {
   StepCoefsT<R> c{};
-  for (int a = 0; a < 3; ++a) { c.alpha[a] = static_cast<R>(cw.alpha[a]); c.beta[a] = static_cast<R>(cw.beta[a]); }
-  c.dYZ = static_cast<R>(cw.deltaYZ); c.dZX = static_cast<R>(cw.deltaZX); c.dXY = static_cast<R>(cw.deltaXY);
-  c.tx = static_cast<R>(g.dt / g.dx); c.ty = static_cast<R>(g.dt / g.dy); c.tz = static_cast<R>(g.dt / g.dz);
+  for (int a = 0; a < 3; ++a) { c.alpha[a] = cw.alpha[a]; c.beta[a] = cw.beta[a]; }
+  c.dYZ = cw.deltaYZ; c.dZX = cw.deltaZX; c.dXY = cw.deltaXY;
+  c.tx = g.dt / g.dx; c.ty = g.dt / g.dy; c.tz = g.dt / g.dz;
   const double curlScale = g.dt / (kMu0 * kEps0);
-  c.cX = static_cast<R>(curlScale / g.dx); c.cY = static_cast<R>(curlScale / g.dy);
-  c.cZ = static_cast<R>(curlScale / g.dz);
-  c.cs = static_cast<R>(g.dt / kEps0);
+  c.cX = curlScale / g.dx; c.cY = curlScale / g.dy;
+  c.cZ = curlScale / g.dz;
+  c.cs = g.dt / kEps0;
   return c;
 }
 
@@ -152,10 +154,10 @@ CpmlArgsT<R> FieldStateT<R>::cpmlArgs() const {
                                                                : histLayers[h] * histRow[h]);
   }
   const double curlScale = g.dt / (kMu0 * kEps0);
-  P.magScale[0] = static_cast<R>(g.dt / g.dx); P.magScale[1] = static_cast<R>(g.dt / g.dy);
-  P.magScale[2] = static_cast<R>(g.dt / g.dz);
-  P.eleScale[0] = static_cast<R>(curlScale / g.dx); P.eleScale[1] = static_cast<R>(curlScale / g.dy);
-  P.eleScale[2] = static_cast<R>(curlScale / g.dz);
+  P.magScale[0] = g.dt / g.dx; P.magScale[1] = g.dt / g.dy;
+  P.magScale[2] = g.dt / g.dz;
+  P.eleScale[0] = curlScale / g.dx; P.eleScale[1] = curlScale / g.dy;
+  P.eleScale[2] = curlScale / g.dz;
   return P;
 }
 
@@ -240,8 +242,7 @@ void installCpml(FieldStateT<R>& s, const aprl_cpml_parameters& p, std::size_t g
       activeCount[m][axis] = static_cast<std::size_t>(std::count_if(ordv.begin(), ordv.end(), [](int o) { return o >= 0; }));
       std::vector<CpmlPointT<R>> ptsR(count);
       for (std::size_t li = 0; li < count; ++li)
-        ptsR[li] = CpmlPointT<R>{static_cast<R>(pts[li].ikm1), static_cast<R>(pts[li].decay),
-                                 static_cast<R>(pts[li].drive)};
+        ptsR[li] = CpmlPointT<R>{pts[li].ikm1, pts[li].decay, pts[li].drive};
       s.prof[m][axis].resize(count);
       s.ord[m][axis].resize(count);
       hipCheck(hipMemcpy(s.prof[m][axis].get(), ptsR.data(), count * sizeof(CpmlPointT<R>), hipMemcpyHostToDevice),
@@ -279,9 +280,9 @@ void installCpml(FieldStateT<R>& s, const aprl_cpml_parameters& p, std::size_t g
 
 // EBFieldGrid::electricValue, eb_field.cpp:350-417.
 template <int A, class R>
-__device__ __forceinline__ R evalE(const FieldPtrsT<R>& F, int i, int j, int k) {
+__device__ __forceinline__ double evalE(const FieldPtrsT<R>& F, int i, int j, int k) {
   const int n0 = F.n[A][0], n1 = F.n[A][1], n2 = F.n[A][2];
-  R sign = R(1);
+  double sign = 1.0;
   if (i < 0) {
     if (A == 0) i = 0;
     else { i = -i; sign = -sign; }
@@ -299,37 +300,37 @@ __device__ __forceinline__ R evalE(const FieldPtrsT<R>& F, int i, int j, int k) 
   if (k < 0 || k >= n2) {
     const bool lower = k < 0;
     const R* plane = lower ? F.lo[A] : F.up[A];
-    if (plane) return sign * plane[static_cast<size_t>(j) * n0 + i];
+    if (plane) return sign * static_cast<double>(plane[static_cast<size_t>(j) * n0 + i]);
     if (A == 2) k = lower ? 0 : n2 - 1;
     else { k = lower ? -k : 2 * (n2 - 1) - k; sign = -sign; }
   }
-  return sign * F.e[A][(static_cast<size_t>(k) * n1 + j) * n0 + i];
+  return sign * static_cast<double>(F.e[A][(static_cast<size_t>(k) * n1 + j) * n0 + i]);
 }
 
 // EBFieldGrid::smoothedElectric, eb_field.cpp:419-486 (same accumulation
 // order on the interior fast path and the boundary path).
 template <int A, int D, class R>
-__device__ __forceinline__ R smoothE(const FieldPtrsT<R>& F, const StepCoefsT<R>& c, int i, int j, int k) {
+__device__ __forceinline__ double smoothE(const FieldPtrsT<R>& F, const StepCoefsT<R>& c, int i, int j, int k) {
   constexpr int TA = (D + 1) % 3, TB = (D + 2) % 3;
-  const R delta = D == 0 ? c.dYZ : (D == 1 ? c.dZX : c.dXY);
+  const double delta = D == 0 ? c.dYZ : (D == 1 ? c.dZX : c.dXY);
   const int n0 = F.n[A][0], n1 = F.n[A][1], n2 = F.n[A][2];
   if (i > 0 && i < n0 - 1 && j > 0 && j < n1 - 1 && k > 0 && k < n2 - 1) {
     const long long st[3] = {1, n0, static_cast<long long>(n0) * n1};
     const long long oA = st[TA], oB = st[TB];
     const R* p = F.e[A] + (static_cast<size_t>(k) * n1 + j) * n0 + i;
-    R r = c.alpha[D] * p[0];
-    r += c.beta[TA] * p[-oA];
-    r += c.beta[TB] * p[-oB];
-    r += c.beta[TA] * p[oA];
-    r += c.beta[TB] * p[oB];
-    r += delta * p[-oA - oB];
-    r += delta * p[-oA + oB];
-    r += delta * p[oA - oB];
-    r += delta * p[oA + oB];
+    double r = c.alpha[D] * static_cast<double>(p[0]);
+    r += c.beta[TA] * static_cast<double>(p[-oA]);
+    r += c.beta[TB] * static_cast<double>(p[-oB]);
+    r += c.beta[TA] * static_cast<double>(p[oA]);
+    r += c.beta[TB] * static_cast<double>(p[oB]);
+    r += delta * static_cast<double>(p[-oA - oB]);
+    r += delta * static_cast<double>(p[-oA + oB]);
+    r += delta * static_cast<double>(p[oA - oB]);
+    r += delta * static_cast<double>(p[oA + oB]);
     return r;
   }
   int b[3] = {i, j, k};
-  R r = c.alpha[D] * evalE<A>(F, i, j, k);
+  double r = c.alpha[D] * evalE<A>(F, i, j, k);
   for (int dir = -1; dir <= 1; dir += 2) {
     int pA[3] = {b[0], b[1], b[2]}, pB[3] = {b[0], b[1], b[2]};
     pA[TA] += dir;
@@ -348,8 +349,8 @@ __device__ __forceinline__ R smoothE(const FieldPtrsT<R>& F, const StepCoefsT<R>
 }
 
 template <int A, class R>
-__device__ __forceinline__ R rawE(const FieldPtrsT<R>& F, int i, int j, int k) {
-  return F.e[A][(static_cast<size_t>(k) * F.n[A][1] + j) * F.n[A][0] + i];
+__device__ __forceinline__ double rawE(const FieldPtrsT<R>& F, int i, int j, int k) {
+  return static_cast<double>(F.e[A][(static_cast<size_t>(k) * F.n[A][1] + j) * F.n[A][0] + i]);
 }
 
 // Index of a CPML history entry of target (M, T) and derivative D at node
@@ -365,9 +366,9 @@ __device__ __forceinline__ R* histAt(const CpmlArgsT<R>& P, int o, int i, int j,
 // transformedCorrection, eb_cpml.cpp:131-138. WRITE selects whether the
 // advanced history is stored (preview keeps the old one).
 template <bool WRITE, class R>
-__device__ __forceinline__ R cpmlCorrection(const CpmlPointT<R>& c, R raw, R* hist) {
-  const R h = c.decay * (*hist) + c.drive * raw;
-  if (WRITE) *hist = h;
+__device__ __forceinline__ double cpmlCorrection(const CpmlPointT<R>& c, double raw, R* hist) {
+  const double h = c.decay * static_cast<double>(*hist) + c.drive * raw;
+  if (WRITE) *hist = static_cast<R>(h);
   return c.ikm1 * raw + h;
 }
 
@@ -381,55 +382,55 @@ __device__ __forceinline__ size_t bIndex(const FieldPtrsT<R>& F, int a, int i, i
 }
 template <class R>
 __device__ __forceinline__ void finishBx(FieldPtrsT<R>& F, const StepCoefsT<R>& c, const CpmlArgsT<R>& P, int i, int j, int k,
-                                         R dEzDy, R dEyDz, const R* old = nullptr) {
+                                         double dEzDy, double dEyDz, const R* old = nullptr) {
   const int nx = F.n[0][0], ny = F.n[1][1];
   const size_t idx = (static_cast<size_t>(k) * ny + j) * (nx + 1) + i;
-  R b = old ? *old : F.b[0][idx];
+  double b = old ? static_cast<double>(*old) : static_cast<double>(F.b[0][idx]);
   b -= c.ty * dEzDy - c.tz * dEyDz;
   if (P.enabled) {
     int o = P.ord[0][1][j];
-    if (o >= 0) b += (R(-1) * P.magScale[1]) * cpmlCorrection<true>(P.prof[0][1][j], dEzDy, histAt<0, 0, 1>(P, o, i, j, k, nx + 1));
+    if (o >= 0) b += (-1.0 * P.magScale[1]) * cpmlCorrection<true>(P.prof[0][1][j], dEzDy, histAt<0, 0, 1>(P, o, i, j, k, nx + 1));
     o = P.ord[0][2][k];
-    if (o >= 0) b += (R(1) * P.magScale[2]) * cpmlCorrection<true>(P.prof[0][2][k], dEyDz, histAt<0, 0, 2>(P, o, i, j, k, nx + 1));
+    if (o >= 0) b += (1.0 * P.magScale[2]) * cpmlCorrection<true>(P.prof[0][2][k], dEyDz, histAt<0, 0, 2>(P, o, i, j, k, nx + 1));
   }
-  F.b[0][idx] = b;
+  F.b[0][idx] = static_cast<R>(b);
 }
 template <class R>
 __device__ __forceinline__ void finishBy(FieldPtrsT<R>& F, const StepCoefsT<R>& c, const CpmlArgsT<R>& P, int i, int j, int k,
-                                         R dExDz, R dEzDx, const R* old = nullptr) {
+                                         double dExDz, double dEzDx, const R* old = nullptr) {
   const int nx = F.n[0][0], ny = F.n[1][1];
   const size_t idx = (static_cast<size_t>(k) * (ny + 1) + j) * nx + i;
-  R b = old ? *old : F.b[1][idx];
+  double b = old ? static_cast<double>(*old) : static_cast<double>(F.b[1][idx]);
   b -= c.tz * dExDz - c.tx * dEzDx;
   if (P.enabled) {
     int o = P.ord[0][2][k];
-    if (o >= 0) b += (R(-1) * P.magScale[2]) * cpmlCorrection<true>(P.prof[0][2][k], dExDz, histAt<0, 1, 2>(P, o, i, j, k, nx));
+    if (o >= 0) b += (-1.0 * P.magScale[2]) * cpmlCorrection<true>(P.prof[0][2][k], dExDz, histAt<0, 1, 2>(P, o, i, j, k, nx));
     o = P.ord[0][0][i];
-    if (o >= 0) b += (R(1) * P.magScale[0]) * cpmlCorrection<true>(P.prof[0][0][i], dEzDx, histAt<0, 1, 0>(P, o, i, j, k, nx));
+    if (o >= 0) b += (1.0 * P.magScale[0]) * cpmlCorrection<true>(P.prof[0][0][i], dEzDx, histAt<0, 1, 0>(P, o, i, j, k, nx));
   }
-  F.b[1][idx] = b;
+  F.b[1][idx] = static_cast<R>(b);
 }
 template <class R>
 __device__ __forceinline__ void finishBz(FieldPtrsT<R>& F, const StepCoefsT<R>& c, const CpmlArgsT<R>& P, int i, int j, int k,
-                                         R dEyDx, R dExDy, const R* old = nullptr) {
+                                         double dEyDx, double dExDy, const R* old = nullptr) {
   const int nx = F.n[0][0], ny = F.n[1][1];
   const size_t idx = (static_cast<size_t>(k) * ny + j) * nx + i;
-  R b = old ? *old : F.b[2][idx];
+  double b = old ? static_cast<double>(*old) : static_cast<double>(F.b[2][idx]);
   b -= c.tx * dEyDx - c.ty * dExDy;
   if (P.enabled) {
     int o = P.ord[0][0][i];
-    if (o >= 0) b += (R(-1) * P.magScale[0]) * cpmlCorrection<true>(P.prof[0][0][i], dEyDx, histAt<0, 2, 0>(P, o, i, j, k, nx));
+    if (o >= 0) b += (-1.0 * P.magScale[0]) * cpmlCorrection<true>(P.prof[0][0][i], dEyDx, histAt<0, 2, 0>(P, o, i, j, k, nx));
     o = P.ord[0][1][j];
-    if (o >= 0) b += (R(1) * P.magScale[1]) * cpmlCorrection<true>(P.prof[0][1][j], dExDy, histAt<0, 2, 1>(P, o, i, j, k, nx));
+    if (o >= 0) b += (1.0 * P.magScale[1]) * cpmlCorrection<true>(P.prof[0][1][j], dExDy, histAt<0, 2, 1>(P, o, i, j, k, nx));
   }
-  F.b[2][idx] = b;
+  F.b[2][idx] = static_cast<R>(b);
 }
 
 // One thread per node over the union of the three B extents (Yee; and the
 // Cowan edge nodes Bx(i = nx), By(j = ny) when EDGES).
 template <bool COWAN, class R>
 __device__ __forceinline__ void magneticBx(FieldPtrsT<R>& F, const StepCoefsT<R>& c, const CpmlArgsT<R>& P, int i, int j, int k) {
-  R dEzDy, dEyDz;
+  double dEzDy, dEyDz;
   if (COWAN) {
     dEzDy = smoothE<2, 1>(F, c, i, j + 1, k) - smoothE<2, 1>(F, c, i, j, k);
     dEyDz = smoothE<1, 2>(F, c, i, j, k + 1) - smoothE<1, 2>(F, c, i, j, k);
@@ -441,7 +442,7 @@ __device__ __forceinline__ void magneticBx(FieldPtrsT<R>& F, const StepCoefsT<R>
 }
 template <bool COWAN, class R>
 __device__ __forceinline__ void magneticBy(FieldPtrsT<R>& F, const StepCoefsT<R>& c, const CpmlArgsT<R>& P, int i, int j, int k) {
-  R dExDz, dEzDx;
+  double dExDz, dEzDx;
   if (COWAN) {
     dExDz = smoothE<0, 2>(F, c, i, j, k + 1) - smoothE<0, 2>(F, c, i, j, k);
     dEzDx = smoothE<2, 0>(F, c, i + 1, j, k) - smoothE<2, 0>(F, c, i, j, k);
@@ -492,14 +493,14 @@ constexpr int kStageR = (kStageN + kMX * kMY - 1) / (kMX * kMY);
 // Smoothed sample from the staged planes; (li, lj) local.
 // pl[0] = plane k-1, pl[1] = plane k, pl[2] = plane k+1; D is the derivative axis.
 template <int D, class R>
-__device__ __forceinline__ R smoothL(const R* const pl[3], const StepCoefsT<R>& c, int li, int lj) {
+__device__ __forceinline__ double smoothL(const R* const pl[3], const StepCoefsT<R>& c, int li, int lj) {
   constexpr int TA = (D + 1) % 3, TB = (D + 2) % 3;
-  const R delta = D == 0 ? c.dYZ : (D == 1 ? c.dZX : c.dXY);
-  auto at = [&](int dx, int dy, int dz) { return pl[1 + dz][(lj + dy) * kLX + li + dx]; };
+  const double delta = D == 0 ? c.dYZ : (D == 1 ? c.dZX : c.dXY);
+  auto at = [&](int dx, int dy, int dz) { return static_cast<double>(pl[1 + dz][(lj + dy) * kLX + li + dx]); };
   auto off = [&](int axis, int dir, int& dx, int& dy, int& dz) {
     if (axis == 0) dx += dir; else if (axis == 1) dy += dir; else dz += dir;
   };
-  R r = c.alpha[D] * at(0, 0, 0);
+  double r = c.alpha[D] * at(0, 0, 0);
   for (int dir = -1; dir <= 1; dir += 2) {
     int ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
     off(TA, dir, ax, ay, az);
@@ -522,8 +523,8 @@ __device__ __forceinline__ R stagedE(const FieldPtrsT<R>& F, int i, int j, int k
   const int n0 = F.n[A][0], n1 = F.n[A][1], n2 = F.n[A][2];
   if (kk >= 0 && kk < n2 && i >= 0 && i < n0 && j >= 0 && j < n1)
     return F.e[A][(static_cast<size_t>(kk) * n1 + j) * n0 + i];
-  if (i >= -1 && i <= n0 && j >= -1 && j <= n1) return evalE<A>(F, i, j, kk);
-  return 0.0;  // more than one node beyond the array: feeds no B node
+  if (i >= -1 && i <= n0 && j >= -1 && j <= n1) return static_cast<R>(evalE<A>(F, i, j, kk));
+  return R(0);  // more than one node beyond the array: feeds no B node
 }
 
 template <class R>
@@ -574,8 +575,8 @@ __global__ void __launch_bounds__(kMX * kMY) kMagneticCowanTiled(FieldPtrsT<R> F
 }
 
 template <class R>
-__device__ __forceinline__ R rd(const R* p, int nx, int ny, int i, int j, int k) {
-  return p[(static_cast<size_t>(k) * ny + j) * nx + i];
+__device__ __forceinline__ double rd(const R* p, int nx, int ny, int i, int j, int k) {
+  return static_cast<double>(p[(static_cast<size_t>(k) * ny + j) * nx + i]);
 }
 
 // Fused electric update: EBFieldGrid::advanceElectric interior curl
@@ -598,52 +599,52 @@ __device__ __forceinline__ void electricNode(const FieldPtrsT<R>& F, const StepC
   const int kb = P.lowerZ ? 1 : 0, ke = P.upperZ ? nz : nz + 1;
   if (i < nx) {  // Ex: nx x (ny+1) x (nz+1)
     const size_t idx = (static_cast<size_t>(k) * (ny + 1) + j) * nx + i;
-    const R e0 = F.e[0][idx];
-    R e = e0;
+    const double e0 = static_cast<double>(F.e[0][idx]);
+    double e = e0;
     if (k >= 1 && k < nz && j >= 1 && j < ny)
       e += c.cY * (rd(bz, nx, ny, i, j, k) - rd(bz, nx, ny, i, j - 1, k)) -
            c.cZ * (rd(by, nx, ny + 1, i, j, k) - rd(by, nx, ny + 1, i, j, k - 1)) - c.cs * F.j[0][idx];
     if (P.enabled && j >= 1 && j < ny && k >= kb && k < ke) {
       int o = P.ord[1][1][j];
       if (o >= 0)
-        e += (R(1) * P.eleScale[1]) * cpmlCorrection<W>(P.prof[1][1][j], rd(bz, nx, ny, i, j, k) - rd(bz, nx, ny, i, j - 1, k),
+        e += (1.0 * P.eleScale[1]) * cpmlCorrection<W>(P.prof[1][1][j], rd(bz, nx, ny, i, j, k) - rd(bz, nx, ny, i, j - 1, k),
                                                         histAt<1, 0, 1>(P, o, i, j, k, nx));
       o = P.ord[1][2][k];
       if (o >= 0)
-        e += (R(-1) * P.eleScale[2]) * cpmlCorrection<W>(P.prof[1][2][k], rd(by, nx, ny + 1, i, j, k) - rd(by, nx, ny + 1, i, j, k - 1),
+        e += (-1.0 * P.eleScale[2]) * cpmlCorrection<W>(P.prof[1][2][k], rd(by, nx, ny + 1, i, j, k) - rd(by, nx, ny + 1, i, j, k - 1),
                                                          histAt<1, 0, 2>(P, o, i, j, k, nx));
     }
     if (j == 0 || j == ny || (P.lowerZ && k == 0) || (P.upperZ && k == nz)) e = 0.0;
     if (inletX && k == 0) e = inletX[static_cast<size_t>(j) * nx + i];
-    (MODE == 0 ? F.e[0] : ox)[idx] = e;
-    if (MODE == 1 && sx) sx[idx] = e + e0;
+    (MODE == 0 ? F.e[0] : ox)[idx] = static_cast<R>(e);
+    if (MODE == 1 && sx) sx[idx] = static_cast<R>(e + e0);
   }
   if (j < ny) {  // Ey: (nx+1) x ny x (nz+1)
     const size_t idx = (static_cast<size_t>(k) * ny + j) * (nx + 1) + i;
-    const R e0 = F.e[1][idx];
-    R e = e0;
+    const double e0 = static_cast<double>(F.e[1][idx]);
+    double e = e0;
     if (k >= 1 && k < nz && i >= 1 && i < nx)
       e += c.cZ * (rd(bx, nx + 1, ny, i, j, k) - rd(bx, nx + 1, ny, i, j, k - 1)) -
            c.cX * (rd(bz, nx, ny, i, j, k) - rd(bz, nx, ny, i - 1, j, k)) - c.cs * F.j[1][idx];
     if (P.enabled && i >= 1 && i < nx && k >= kb && k < ke) {
       int o = P.ord[1][2][k];
       if (o >= 0)
-        e += (R(1) * P.eleScale[2]) * cpmlCorrection<W>(P.prof[1][2][k], rd(bx, nx + 1, ny, i, j, k) - rd(bx, nx + 1, ny, i, j, k - 1),
+        e += (1.0 * P.eleScale[2]) * cpmlCorrection<W>(P.prof[1][2][k], rd(bx, nx + 1, ny, i, j, k) - rd(bx, nx + 1, ny, i, j, k - 1),
                                                         histAt<1, 1, 2>(P, o, i, j, k, nx + 1));
       o = P.ord[1][0][i];
       if (o >= 0)
-        e += (R(-1) * P.eleScale[0]) * cpmlCorrection<W>(P.prof[1][0][i], rd(bz, nx, ny, i, j, k) - rd(bz, nx, ny, i - 1, j, k),
+        e += (-1.0 * P.eleScale[0]) * cpmlCorrection<W>(P.prof[1][0][i], rd(bz, nx, ny, i, j, k) - rd(bz, nx, ny, i - 1, j, k),
                                                          histAt<1, 1, 0>(P, o, i, j, k, nx + 1));
     }
     if (i == 0 || i == nx || (P.lowerZ && k == 0) || (P.upperZ && k == nz)) e = 0.0;
     if (inletY && k == 0) e = inletY[static_cast<size_t>(j) * (nx + 1) + i];
-    (MODE == 0 ? F.e[1] : oy)[idx] = e;
-    if (MODE == 1 && sy) sy[idx] = e + e0;
+    (MODE == 0 ? F.e[1] : oy)[idx] = static_cast<R>(e);
+    if (MODE == 1 && sy) sy[idx] = static_cast<R>(e + e0);
   }
   if (k < nz) {  // Ez: (nx+1) x (ny+1) x nz
     const size_t idx = (static_cast<size_t>(k) * (ny + 1) + j) * (nx + 1) + i;
-    const R e0 = F.e[2][idx];
-    R e = e0;
+    const double e0 = static_cast<double>(F.e[2][idx]);
+    double e = e0;
     const bool in = i >= 1 && i < nx && j >= 1 && j < ny;
     if (in)
       e += c.cX * (rd(by, nx, ny + 1, i, j, k) - rd(by, nx, ny + 1, i - 1, j, k)) -
@@ -651,16 +652,16 @@ __device__ __forceinline__ void electricNode(const FieldPtrsT<R>& F, const StepC
     if (P.enabled && in) {
       int o = P.ord[1][0][i];
       if (o >= 0)
-        e += (R(1) * P.eleScale[0]) * cpmlCorrection<W>(P.prof[1][0][i], rd(by, nx, ny + 1, i, j, k) - rd(by, nx, ny + 1, i - 1, j, k),
+        e += (1.0 * P.eleScale[0]) * cpmlCorrection<W>(P.prof[1][0][i], rd(by, nx, ny + 1, i, j, k) - rd(by, nx, ny + 1, i - 1, j, k),
                                                         histAt<1, 2, 0>(P, o, i, j, k, nx + 1));
       o = P.ord[1][1][j];
       if (o >= 0)
-        e += (R(-1) * P.eleScale[1]) * cpmlCorrection<W>(P.prof[1][1][j], rd(bx, nx + 1, ny, i, j, k) - rd(bx, nx + 1, ny, i, j - 1, k),
+        e += (-1.0 * P.eleScale[1]) * cpmlCorrection<W>(P.prof[1][1][j], rd(bx, nx + 1, ny, i, j, k) - rd(bx, nx + 1, ny, i, j - 1, k),
                                                          histAt<1, 2, 1>(P, o, i, j, k, nx + 1));
     }
     if (i == 0 || i == nx || j == 0 || j == ny) e = 0.0;
-    (MODE == 0 ? F.e[2] : oz)[idx] = e;
-    if (MODE == 1 && sz) sz[idx] = e + e0;
+    (MODE == 0 ? F.e[2] : oz)[idx] = static_cast<R>(e);
+    if (MODE == 1 && sz) sz[idx] = static_cast<R>(e + e0);
   }
 }
 

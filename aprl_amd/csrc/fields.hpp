@@ -17,8 +17,10 @@ struct CowanCoef {
 };
 CowanCoef cowanForZDispersion(double dx, double dy, double dz);
 
+// Profiles and coefficients are FP64 for both storage types R: updates
+// of the FP32 form are evaluated in FP64 and rounded once on store.
 template <class R>
-struct CpmlPointT { R ikm1, decay, drive; };
+struct CpmlPointT { double ikm1, decay, drive; };
 
 // Kernel-side view of the field arrays (base pointers already slid).
 template <class R>
@@ -33,9 +35,9 @@ struct FieldPtrsT {
 
 template <class R>
 struct StepCoefsT {
-  R alpha[3], beta[3], dYZ, dZX, dXY;
-  R tx, ty, tz;      // dt/dx, dt/dy, dt/dz
-  R cX, cY, cZ, cs;  // dt/(mu0 eps0 d*), dt/eps0
+  double alpha[3], beta[3], dYZ, dZX, dXY;
+  double tx, ty, tz;      // dt/dx, dt/dy, dt/dz
+  double cX, cY, cZ, cs;  // dt/(mu0 eps0 d*), dt/eps0
 };
 
 template <class R>
@@ -48,7 +50,7 @@ struct CpmlArgsT {
   R* hist[2][3][3];                     // [mag/ele][target comp][derivative]
   long long ostride[2][3][3];           // stride between ordinals
   int row[2][3][3];                     // row length of a layer
-  R magScale[3], eleScale[3];
+  double magScale[3], eleScale[3];
 };
 
 // Tile footprint of the last deposit: J is zero outside the listed cell

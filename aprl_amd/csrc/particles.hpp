@@ -76,6 +76,7 @@ struct SplitStore {
   // trial end state (cell1, offset1, u1) and the per-step work sum
   DeviceArray<float> fields[2][6], end[6], gridAbs;
   DeviceArray<int> endCell[3];
+  DeviceArray<float> chord[3];  // trial displacement x1 - x0 in cells
   DeviceArray<unsigned> key, keySorted, perm, permSorted;
   DeviceArray<unsigned char> sortTemp;
   std::size_t sortTempBytes = 0;

@@ -52,6 +52,7 @@ struct PArgs32 {
   float* fout[6];
   int* cell1[3];
   float* off1[3];
+  float* chord[3];  // trial chord x1 - x0 in cells, rounded once from FP64
   float* u1[3];
   float* gridAbs;
   float kq[3];  // h_a / (dx dy dz dt): current density per unit charge and cell of motion
@@ -319,8 +320,9 @@ __device__ __forceinline__ void chordAt(const PArgs32& P, int p, bool guess, con
     v[1] = G.ch[1] * uy * ig;
     v[2] = G.ch[2] * uz * ig;
   } else {
-    for (int d = 0; d < 3; ++d)
-      v[d] = static_cast<float>(P.cell1[d][p] - c0[d]) + (P.off1[d][p] - off0[d]);
+    // the push's displacement rounded once: keeps its relative precision
+    // where x1 - x0 is far below the offsets' resolution (transverse motion)
+    for (int d = 0; d < 3; ++d) v[d] = P.chord[d][p];
   }
 }
 
@@ -451,6 +453,7 @@ __global__ void __launch_bounds__(128) kPushSplit(PArgs32 P, StepArgs S, double 
     else err |= kErrNonfinite;
     P.cell1[a][p] = cell;
     P.off1[a][p] = off;
+    P.chord[a][p] = static_cast<float>((x[a] - x0[a]) / P.h[a]);
     P.u1[a][p] = static_cast<float>(u[a]);
   }
   P.gridAbs[p] = static_cast<float>(absGrid);
@@ -619,6 +622,7 @@ void reserveSplitParticles(SplitStore& P, std::size_t capacity) {
     P.cell[a].resize(capacity);
     P.cellT[a].resize(capacity);
     P.endCell[a].resize(capacity);
+    P.chord[a].resize(capacity);
   }
   for (int a = 0; a < 9; ++a) {
     P.s[a].resize(capacity);
@@ -745,6 +749,7 @@ PArgs32 particleArgs32(SplitStore& P, const FieldState32& F) {
     A.u[a] = P.s[3 + a].get();
     A.cell1[a] = P.endCell[a].get();
     A.off1[a] = P.end[a].get();
+    A.chord[a] = P.chord[a].get();
     A.u1[a] = P.end[3 + a].get();
   }
   for (int a = 0; a < 6; ++a) {
