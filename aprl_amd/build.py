@@ -32,6 +32,7 @@ COMMON = [
 # Per-file additions (a later flag overrides COMMON's).
 FILE_FLAGS = {
     "particles32.hip": ["-ffp-contract=fast"],
+    "fields32.hip": ["-ffp-contract=fast"],
 }
 
 

@@ -1,0 +1,881 @@
+// Templates of the field kernels and their host launchers for a storage
+// type R (double: the plain form; float: the FP32 form), included by the
+// translation units that instantiate them (fields.hip for R = double,
+// fields32.hip for R = float). Updates are evaluated in FP64 for both and
+// rounded once to R on store.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+#include "fields.hpp"
+
+namespace aprl_gpu {
+
+template <class R>
+void allocateFields(FieldStateT<R>& s, std::size_t slack) {
+  const std::size_t cells[3] = {s.g.nx, s.g.ny, s.g.nz};
+  for (int a = 0; a < 9; ++a) {
+    const bool electric = a < 3 || a >= 6;
+    const int axis = a % 3;
+    for (int d = 0; d < 3; ++d) s.dims[a][d] = cells[d] + (electric ? (d != axis) : (d == axis));
+  }
+  s.slack = slack;
+  s.zb = slack;
+  std::size_t largest = 0;
+  for (int a = 0; a < 9; ++a) {
+    const std::size_t planes = s.dims[a][2] + (s.slides(a) ? slack : 0);
+    s.comp[a].resize(planes * s.plane(a));
+    hipCheck(hipMemsetAsync(s.comp[a].get(), 0, s.comp[a].size() * sizeof(R), s.stream), "hipMemsetAsync");
+    largest = std::max(largest, s.compSize(a));
+  }
+  if (slack) s.scratch.resize(largest);
+}
+
+template <class R>
+FieldPtrsT<R> FieldStateT<R>::ptrs() const {
+  FieldPtrsT<R> f{};
+  for (int a = 0; a < 3; ++a) {
+    f.e[a] = base(a);
+    f.b[a] = base(3 + a);
+    f.j[a] = base(6 + a);
+    f.lo[a] = nullptr;
+    f.up[a] = nullptr;
+  }
+  for (int a = 0; a < 9; ++a)
+    for (int d = 0; d < 3; ++d) f.n[a][d] = static_cast<int>(dims[a][d]);
+  return f;
+}
+
+// Coefficients stay in double for both storage types: the FP32 form stores
+// its fields in FP32 but evaluates every update in FP64 and rounds the
+// result once, so no coefficient rounding biases its dispersion.
+template <class R>
+StepCoefsT<R> FieldStateT<R>::coefs() const {
+  StepCoefsT<R> c{};
+  for (int a = 0; a < 3; ++a) { c.alpha[a] = cw.alpha[a]; c.beta[a] = cw.beta[a]; }
+  c.dYZ = cw.deltaYZ; c.dZX = cw.deltaZX; c.dXY = cw.deltaXY;
+  c.tx = g.dt / g.dx; c.ty = g.dt / g.dy; c.tz = g.dt / g.dz;
+  const double curlScale = g.dt / (kMu0 * kEps0);
+  c.cX = curlScale / g.dx; c.cY = curlScale / g.dy;
+  c.cZ = curlScale / g.dz;
+  c.cs = g.dt / kEps0;
+  return c;
+}
+
+// Pair order of the checkpoint histories, eb_cpml.cpp:355-357 / 556-558.
+static const int kPairs[6][2] = {{0, 1}, {0, 2}, {1, 2}, {1, 0}, {2, 0}, {2, 1}};
+
+template <class R>
+CpmlArgsT<R> FieldStateT<R>::cpmlArgs() const {
+  CpmlArgsT<R> P{};
+  P.enabled = hasCpml ? 1 : 0;
+  P.lowerZ = lowerZ ? 1 : 0;
+  P.upperZ = upperZ ? 1 : 0;
+  P.zb = static_cast<int>(zb);
+  if (!hasCpml) return P;
+  for (int m = 0; m < 2; ++m)
+    for (int a = 0; a < 3; ++a) { P.prof[m][a] = prof[m][a].get(); P.ord[m][a] = ord[m][a].get(); }
+  for (int h = 0; h < 12; ++h) {
+    const int m = h < 6 ? 0 : 1, t = kPairs[h % 6][0], d = kPairs[h % 6][1];
+    P.hist[m][t][d] = hist[h].get();
+    P.row[m][t][d] = static_cast<int>(histRow[h]);
+    P.ostride[m][t][d] = static_cast<long long>(histSlides[h] ? (histLayers[h] + slack) * histRow[h]
+                                                               : histLayers[h] * histRow[h]);
+  }
+  const double curlScale = g.dt / (kMu0 * kEps0);
+  P.magScale[0] = g.dt / g.dx; P.magScale[1] = g.dt / g.dy;
+  P.magScale[2] = g.dt / g.dz;
+  P.eleScale[0] = curlScale / g.dx; P.eleScale[1] = curlScale / g.dy;
+  P.eleScale[2] = curlScale / g.dz;
+  return P;
+}
+
+// Offset of a history slot's first stored layer (checkpoint layer 0).
+template <class R>
+std::size_t FieldStateT<R>::historyDeviceOffset(std::size_t h, std::size_t slot) const {
+  if (!histSlides[h]) return slot * histLayers[h] * histRow[h];
+  return slot * (histLayers[h] + slack) * histRow[h] + zb * histRow[h];
+}
+
+// EBCPMLParameters::maximumConductivityRate, eb_cpml.cpp:263-281.
+static double maxRate(const aprl_cpml_parameters& p, int axis, const aprl_grid_geometry& g) {
+  if (p.cells[axis] == 0) return 0.0;
+  const double spacing = axis == 0 ? g.dx : (axis == 1 ? g.dy : g.dz);
+  if (!(spacing > 0.0) || !std::isfinite(spacing) || !(p.polynomial_order > 0.0) ||
+      !(p.target_reflection > 0.0 && p.target_reflection < 1.0))
+    throw std::invalid_argument("Cannot evaluate an invalid CPML conductivity profile");
+  const double thickness = static_cast<double>(p.cells[axis]) * spacing;
+  return -(p.polynomial_order + 1.0) * kC * std::log(p.target_reflection) / (2.0 * thickness);
+}
+
+// EBConvolutionalPML construction, eb_cpml.cpp:195-248 and 297-372; the
+// profiles are evaluated on the host with the same libm calls.
+template <class R>
+void installCpml(FieldStateT<R>& s, const aprl_cpml_parameters& p, std::size_t globalNz, std::size_t zOffset,
+                 bool lowerZ, bool upperZ) {
+  if (!(p.cells[0] > 0 || p.cells[1] > 0 || p.cells[2] > 0))
+    throw std::invalid_argument("CPML requires at least one nonzero boundary thickness");
+  if (!(p.polynomial_order > 0.0) || !(p.target_reflection > 0.0 && p.target_reflection < 1.0) ||
+      !(p.kappa_max >= 1.0) || !(p.alpha_fraction >= 0.0) || !std::isfinite(p.polynomial_order) ||
+      !std::isfinite(p.target_reflection) || !std::isfinite(p.kappa_max) || !std::isfinite(p.alpha_fraction))
+    throw std::invalid_argument("Invalid CPML grading parameters");
+  const aprl_grid_geometry& g = s.g;
+  const std::size_t gc[3] = {g.nx, g.ny, globalNz};
+  for (int a = 0; a < 3; ++a)
+    if (p.cells[a] > 0) {
+      if (p.cells[a] < 2) throw std::invalid_argument("Each enabled CPML direction needs at least two cells");
+      if (2 * p.cells[a] >= gc[a]) throw std::invalid_argument("Opposite CPML slabs must leave a non-PML interior");
+    }
+  if (p.cells[2] > 0) {
+    if (!lowerZ && zOffset < p.cells[2])
+      throw std::invalid_argument("The lower z CPML slab cannot span MPI ranks yet; reduce MPI ranks or CPML cells");
+    if (!upperZ && zOffset + g.nz > globalNz - p.cells[2])
+      throw std::invalid_argument("The upper z CPML slab cannot span MPI ranks yet; reduce MPI ranks or CPML cells");
+  }
+  if (s.slack && p.cells[2] > 0) throw std::invalid_argument("Native-z history transport requires transverse-only CPML");
+  std::size_t activeCount[2][3] = {};
+  for (int m = 0; m < 2; ++m) {
+    const bool electric = m == 1;
+    for (int axis = 0; axis < 3; ++axis) {
+      const std::size_t localCells = axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz);
+      const std::size_t globalCells = axis == 2 ? globalNz : localCells;
+      const std::size_t globalOffset = axis == 2 ? zOffset : 0;
+      const std::size_t count = localCells + (electric ? 1 : 0);
+      std::vector<CpmlPointDev> pts(count, CpmlPointDev{0.0, 0.0, 0.0});
+      std::vector<int> ordv(count, -1);
+      const std::size_t thicknessCells = p.cells[axis];
+      if (thicknessCells > 0) {
+        const double thickness = static_cast<double>(thicknessCells);
+        const double smax = maxRate(p, axis, g);
+        int ordinal = 0;
+        for (std::size_t li = 0; li < count; ++li) {
+          const double position = static_cast<double>(globalOffset + li) + (electric ? 0.0 : 0.5);
+          double depth = 0.0;
+          if (position < thickness) depth = (thickness - position) / thickness;
+          const double upperStart = static_cast<double>(globalCells - thicknessCells);
+          if (position > upperStart) depth = std::max(depth, (position - upperStart) / thickness);
+          if (!(depth > 0.0)) continue;
+          depth = std::min(1.0, depth);
+          const double graded = std::pow(depth, p.polynomial_order);
+          const double sigma = smax * graded;
+          const double kappa = 1.0 + (p.kappa_max - 1.0) * graded;
+          const double alpha = p.alpha_fraction * smax * (1.0 - depth);
+          CpmlPointDev& c = pts[li];
+          c.ikm1 = 1.0 / kappa - 1.0;
+          c.decay = std::exp(-(sigma / kappa + alpha) * g.dt);
+          const double den = sigma * kappa + alpha * kappa * kappa;
+          c.drive = den > 0.0 ? sigma * (c.decay - 1.0) / den : 0.0;
+          ordv[li] = ordinal++;
+        }
+      }
+      activeCount[m][axis] = static_cast<std::size_t>(std::count_if(ordv.begin(), ordv.end(), [](int o) { return o >= 0; }));
+      std::vector<CpmlPointT<R>> ptsR(count);
+      for (std::size_t li = 0; li < count; ++li)
+        ptsR[li] = CpmlPointT<R>{pts[li].ikm1, pts[li].decay, pts[li].drive};
+      s.prof[m][axis].resize(count);
+      s.ord[m][axis].resize(count);
+      hipCheck(hipMemcpy(s.prof[m][axis].get(), ptsR.data(), count * sizeof(CpmlPointT<R>), hipMemcpyHostToDevice),
+               "hipMemcpy");
+      hipCheck(hipMemcpy(s.ord[m][axis].get(), ordv.data(), count * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+    }
+  }
+  for (int h = 0; h < 12; ++h) {
+    const int m = h < 6 ? 0 : 1, t = kPairs[h % 6][0], d = kPairs[h % 6][1];
+    const int a = m == 0 ? 3 + t : t;
+    const std::size_t* dim = s.dims[a];
+    s.histSlots[h] = activeCount[m][d];
+    if (d == 2) {
+      s.histSlides[h] = false;
+      s.histLayers[h] = dim[1];
+      s.histRow[h] = dim[0];
+    } else {
+      s.histSlides[h] = s.slack > 0;
+      s.histLayers[h] = dim[2];
+      s.histRow[h] = d == 0 ? dim[1] : dim[0];
+    }
+    const std::size_t layers = s.histLayers[h] + (s.histSlides[h] ? s.slack : 0);
+    s.hist[h].resize(s.histSlots[h] * layers * s.histRow[h]);
+    if (s.hist[h].size())
+      hipCheck(hipMemsetAsync(s.hist[h].get(), 0, s.hist[h].size() * sizeof(R), s.stream), "hipMemsetAsync");
+  }
+  s.cpml = p;
+  s.hasCpml = true;
+  s.lowerZ = lowerZ;
+  s.upperZ = upperZ;
+}
+
+// ---------------------------------------------------------------------------
+// Device code.
+
+// EBFieldGrid::electricValue, eb_field.cpp:350-417.
+template <int A, class R>
+__device__ __forceinline__ double evalE(const FieldPtrsT<R>& F, int i, int j, int k) {
+  const int n0 = F.n[A][0], n1 = F.n[A][1], n2 = F.n[A][2];
+  double sign = 1.0;
+  if (i < 0) {
+    if (A == 0) i = 0;
+    else { i = -i; sign = -sign; }
+  } else if (i >= n0) {
+    if (A == 0) i = n0 - 1;
+    else { i = 2 * (n0 - 1) - i; sign = -sign; }
+  }
+  if (j < 0) {
+    if (A == 1) j = 0;
+    else { j = -j; sign = -sign; }
+  } else if (j >= n1) {
+    if (A == 1) j = n1 - 1;
+    else { j = 2 * (n1 - 1) - j; sign = -sign; }
+  }
+  if (k < 0 || k >= n2) {
+    const bool lower = k < 0;
+    const R* plane = lower ? F.lo[A] : F.up[A];
+    if (plane) return sign * static_cast<double>(plane[static_cast<size_t>(j) * n0 + i]);
+    if (A == 2) k = lower ? 0 : n2 - 1;
+    else { k = lower ? -k : 2 * (n2 - 1) - k; sign = -sign; }
+  }
+  return sign * static_cast<double>(F.e[A][(static_cast<size_t>(k) * n1 + j) * n0 + i]);
+}
+
+// EBFieldGrid::smoothedElectric, eb_field.cpp:419-486 (same accumulation
+// order on the interior fast path and the boundary path).
+template <int A, int D, class R>
+__device__ __forceinline__ double smoothE(const FieldPtrsT<R>& F, const StepCoefsT<R>& c, int i, int j, int k) {
+  constexpr int TA = (D + 1) % 3, TB = (D + 2) % 3;
+  const double delta = D == 0 ? c.dYZ : (D == 1 ? c.dZX : c.dXY);
+  const int n0 = F.n[A][0], n1 = F.n[A][1], n2 = F.n[A][2];
+  if (i > 0 && i < n0 - 1 && j > 0 && j < n1 - 1 && k > 0 && k < n2 - 1) {
+    const long long st[3] = {1, n0, static_cast<long long>(n0) * n1};
+    const long long oA = st[TA], oB = st[TB];
+    const R* p = F.e[A] + (static_cast<size_t>(k) * n1 + j) * n0 + i;
+    double r = c.alpha[D] * static_cast<double>(p[0]);
+    r += c.beta[TA] * static_cast<double>(p[-oA]);
+    r += c.beta[TB] * static_cast<double>(p[-oB]);
+    r += c.beta[TA] * static_cast<double>(p[oA]);
+    r += c.beta[TB] * static_cast<double>(p[oB]);
+    r += delta * static_cast<double>(p[-oA - oB]);
+    r += delta * static_cast<double>(p[-oA + oB]);
+    r += delta * static_cast<double>(p[oA - oB]);
+    r += delta * static_cast<double>(p[oA + oB]);
+    return r;
+  }
+  int b[3] = {i, j, k};
+  double r = c.alpha[D] * evalE<A>(F, i, j, k);
+  for (int dir = -1; dir <= 1; dir += 2) {
+    int pA[3] = {b[0], b[1], b[2]}, pB[3] = {b[0], b[1], b[2]};
+    pA[TA] += dir;
+    pB[TB] += dir;
+    r += c.beta[TA] * evalE<A>(F, pA[0], pA[1], pA[2]);
+    r += c.beta[TB] * evalE<A>(F, pB[0], pB[1], pB[2]);
+  }
+  for (int dA = -1; dA <= 1; dA += 2)
+    for (int dB = -1; dB <= 1; dB += 2) {
+      int p[3] = {b[0], b[1], b[2]};
+      p[TA] += dA;
+      p[TB] += dB;
+      r += delta * evalE<A>(F, p[0], p[1], p[2]);
+    }
+  return r;
+}
+
+template <int A, class R>
+__device__ __forceinline__ double rawE(const FieldPtrsT<R>& F, int i, int j, int k) {
+  return static_cast<double>(F.e[A][(static_cast<size_t>(k) * F.n[A][1] + j) * F.n[A][0] + i]);
+}
+
+// Index of a CPML history entry of target (M, T) and derivative D at node
+// (i, j, k), CPMLMemoryTerm::value (eb_cpml.cpp:78-89) with slid layers.
+template <int M, int T, int D, class R>
+__device__ __forceinline__ R* histAt(const CpmlArgsT<R>& P, int o, int i, int j, int k, int dim0) {
+  R* h = P.hist[M][T][D] + static_cast<long long>(o) * P.ostride[M][T][D];
+  if (D == 0) return h + static_cast<long long>(P.zb + k) * P.row[M][T][D] + j;
+  if (D == 1) return h + static_cast<long long>(P.zb + k) * P.row[M][T][D] + i;
+  return h + static_cast<long long>(j) * dim0 + i;
+}
+
+// transformedCorrection, eb_cpml.cpp:131-138. WRITE selects whether the
+// advanced history is stored (preview keeps the old one).
+template <bool WRITE, class R>
+__device__ __forceinline__ double cpmlCorrection(const CpmlPointT<R>& c, double raw, R* hist) {
+  const double h = c.decay * static_cast<double>(*hist) + c.drive * raw;
+  if (WRITE) *hist = static_cast<R>(h);
+  return c.ikm1 * raw + h;
+}
+
+// Per-component magnetic updates at one node: advanceMagneticCowan
+// (eb_field.cpp:488-558) / advanceMagneticYee (311-348) followed by
+// EBConvolutionalPML::afterMagneticUpdate (406-468). dA/dB are the two curl
+// differences of the component; the CPML terms use the same differences.
+template <class R>
+__device__ __forceinline__ size_t bIndex(const FieldPtrsT<R>& F, int a, int i, int j, int k) {
+  return (static_cast<size_t>(k) * F.n[3 + a][1] + j) * F.n[3 + a][0] + i;
+}
+template <class R>
+__device__ __forceinline__ void finishBx(FieldPtrsT<R>& F, const StepCoefsT<R>& c, const CpmlArgsT<R>& P, int i, int j, int k,
+                                         double dEzDy, double dEyDz, const R* old = nullptr) {
+  const int nx = F.n[0][0], ny = F.n[1][1];
+  const size_t idx = (static_cast<size_t>(k) * ny + j) * (nx + 1) + i;
+  double b = old ? static_cast<double>(*old) : static_cast<double>(F.b[0][idx]);
+  b -= c.ty * dEzDy - c.tz * dEyDz;
+  if (P.enabled) {
+    int o = P.ord[0][1][j];
+    if (o >= 0) b += (-1.0 * P.magScale[1]) * cpmlCorrection<true>(P.prof[0][1][j], dEzDy, histAt<0, 0, 1>(P, o, i, j, k, nx + 1));
+    o = P.ord[0][2][k];
+    if (o >= 0) b += (1.0 * P.magScale[2]) * cpmlCorrection<true>(P.prof[0][2][k], dEyDz, histAt<0, 0, 2>(P, o, i, j, k, nx + 1));
+  }
+  F.b[0][idx] = static_cast<R>(b);
+}
+template <class R>
+__device__ __forceinline__ void finishBy(FieldPtrsT<R>& F, const StepCoefsT<R>& c, const CpmlArgsT<R>& P, int i, int j, int k,
+                                         double dExDz, double dEzDx, const R* old = nullptr) {
+  const int nx = F.n[0][0], ny = F.n[1][1];
+  const size_t idx = (static_cast<size_t>(k) * (ny + 1) + j) * nx + i;
+  double b = old ? static_cast<double>(*old) : static_cast<double>(F.b[1][idx]);
+  b -= c.tz * dExDz - c.tx * dEzDx;
+  if (P.enabled) {
+    int o = P.ord[0][2][k];
+    if (o >= 0) b += (-1.0 * P.magScale[2]) * cpmlCorrection<true>(P.prof[0][2][k], dExDz, histAt<0, 1, 2>(P, o, i, j, k, nx));
+    o = P.ord[0][0][i];
+    if (o >= 0) b += (1.0 * P.magScale[0]) * cpmlCorrection<true>(P.prof[0][0][i], dEzDx, histAt<0, 1, 0>(P, o, i, j, k, nx));
+  }
+  F.b[1][idx] = static_cast<R>(b);
+}
+template <class R>
+__device__ __forceinline__ void finishBz(FieldPtrsT<R>& F, const StepCoefsT<R>& c, const CpmlArgsT<R>& P, int i, int j, int k,
+                                         double dEyDx, double dExDy, const R* old = nullptr) {
+  const int nx = F.n[0][0], ny = F.n[1][1];
+  const size_t idx = (static_cast<size_t>(k) * ny + j) * nx + i;
+  double b = old ? static_cast<double>(*old) : static_cast<double>(F.b[2][idx]);
+  b -= c.tx * dEyDx - c.ty * dExDy;
+  if (P.enabled) {
+    int o = P.ord[0][0][i];
+    if (o >= 0) b += (-1.0 * P.magScale[0]) * cpmlCorrection<true>(P.prof[0][0][i], dEyDx, histAt<0, 2, 0>(P, o, i, j, k, nx));
+    o = P.ord[0][1][j];
+    if (o >= 0) b += (1.0 * P.magScale[1]) * cpmlCorrection<true>(P.prof[0][1][j], dExDy, histAt<0, 2, 1>(P, o, i, j, k, nx));
+  }
+  F.b[2][idx] = static_cast<R>(b);
+}
+
+// One thread per node over the union of the three B extents (Yee; and the
+// Cowan edge nodes Bx(i = nx), By(j = ny) when EDGES).
+template <bool COWAN, class R>
+__device__ __forceinline__ void magneticBx(FieldPtrsT<R>& F, const StepCoefsT<R>& c, const CpmlArgsT<R>& P, int i, int j, int k) {
+  double dEzDy, dEyDz;
+  if (COWAN) {
+    dEzDy = smoothE<2, 1>(F, c, i, j + 1, k) - smoothE<2, 1>(F, c, i, j, k);
+    dEyDz = smoothE<1, 2>(F, c, i, j, k + 1) - smoothE<1, 2>(F, c, i, j, k);
+  } else {
+    dEzDy = rawE<2>(F, i, j + 1, k) - rawE<2>(F, i, j, k);
+    dEyDz = rawE<1>(F, i, j, k + 1) - rawE<1>(F, i, j, k);
+  }
+  finishBx(F, c, P, i, j, k, dEzDy, dEyDz);
+}
+template <bool COWAN, class R>
+__device__ __forceinline__ void magneticBy(FieldPtrsT<R>& F, const StepCoefsT<R>& c, const CpmlArgsT<R>& P, int i, int j, int k) {
+  double dExDz, dEzDx;
+  if (COWAN) {
+    dExDz = smoothE<0, 2>(F, c, i, j, k + 1) - smoothE<0, 2>(F, c, i, j, k);
+    dEzDx = smoothE<2, 0>(F, c, i + 1, j, k) - smoothE<2, 0>(F, c, i, j, k);
+  } else {
+    dExDz = rawE<0>(F, i, j, k + 1) - rawE<0>(F, i, j, k);
+    dEzDx = rawE<2>(F, i + 1, j, k) - rawE<2>(F, i, j, k);
+  }
+  finishBy(F, c, P, i, j, k, dExDz, dEzDx);
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) kMagneticYee(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P) {
+  const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t % (nx + 1), j = t / (nx + 1);
+  const int k = blockIdx.y;
+  if (j > ny || k > nz) return;
+  if (j < ny && k < nz) magneticBx<false>(F, c, P, i, j, k);
+  if (i < nx && k < nz) magneticBy<false>(F, c, P, i, j, k);
+  if (i < nx && j < ny)
+    finishBz(F, c, P, i, j, k, rawE<1>(F, i + 1, j, k) - rawE<1>(F, i, j, k), rawE<0>(F, i, j + 1, k) - rawE<0>(F, i, j, k));
+}
+
+// Cowan edge nodes the tiles do not cover: Bx at i = nx and By at j = ny.
+template <class R>
+__global__ void __launch_bounds__(256) kMagneticCowanEdges(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P) {
+  const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ny * nz) {
+    magneticBx<true>(F, c, P, nx, t % ny, t / ny);
+    return;
+  }
+  const int u = t - ny * nz;
+  if (u < nx * nz) magneticBy<true>(F, c, P, u % nx, ny, u / nx);
+}
+
+// Tiled Cowan magnetic update for i < nx, j < ny, one workgroup per
+// (kMX x kMY) tile of one z plane k. E planes k-1, k, k+1 of the tile plus a
+// one-node halo are staged in LDS as electricValue() samples (reflection /
+// halo / PEC parity applied while loading; all loads of a lane are issued
+// before the first store); each smoothed sample is then the same nine
+// products in the same order as smoothedElectric().
+constexpr int kMX = 32, kMY = 8;
+constexpr int kLX = kMX + 2, kLY = kMY + 2, kLP = kLX * kLY;
+constexpr int kStageN = 9 * kLP;  // 3 components x 3 planes
+constexpr int kStageR = (kStageN + kMX * kMY - 1) / (kMX * kMY);
+
+// Smoothed sample from the staged planes; (li, lj) local.
+// pl[0] = plane k-1, pl[1] = plane k, pl[2] = plane k+1; D is the derivative axis.
+template <int D, class R>
+__device__ __forceinline__ double smoothL(const R* const pl[3], const StepCoefsT<R>& c, int li, int lj) {
+  constexpr int TA = (D + 1) % 3, TB = (D + 2) % 3;
+  const double delta = D == 0 ? c.dYZ : (D == 1 ? c.dZX : c.dXY);
+  auto at = [&](int dx, int dy, int dz) { return static_cast<double>(pl[1 + dz][(lj + dy) * kLX + li + dx]); };
+  auto off = [&](int axis, int dir, int& dx, int& dy, int& dz) {
+    if (axis == 0) dx += dir; else if (axis == 1) dy += dir; else dz += dir;
+  };
+  double r = c.alpha[D] * at(0, 0, 0);
+  for (int dir = -1; dir <= 1; dir += 2) {
+    int ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
+    off(TA, dir, ax, ay, az);
+    off(TB, dir, bx, by, bz);
+    r += c.beta[TA] * at(ax, ay, az);
+    r += c.beta[TB] * at(bx, by, bz);
+  }
+  for (int dA = -1; dA <= 1; dA += 2)
+    for (int dB = -1; dB <= 1; dB += 2) {
+      int x = 0, y = 0, z = 0;
+      off(TA, dA, x, y, z);
+      off(TB, dB, x, y, z);
+      r += delta * at(x, y, z);
+    }
+  return r;
+}
+
+template <int A, class R>
+__device__ __forceinline__ R stagedE(const FieldPtrsT<R>& F, int i, int j, int kk) {
+  const int n0 = F.n[A][0], n1 = F.n[A][1], n2 = F.n[A][2];
+  if (kk >= 0 && kk < n2 && i >= 0 && i < n0 && j >= 0 && j < n1)
+    return F.e[A][(static_cast<size_t>(kk) * n1 + j) * n0 + i];
+  if (i >= -1 && i <= n0 && j >= -1 && j <= n1) return static_cast<R>(evalE<A>(F, i, j, kk));
+  return R(0);  // more than one node beyond the array: feeds no B node
+}
+
+template <class R>
+__global__ void __launch_bounds__(kMX * kMY) kMagneticCowanTiled(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P) {
+  __shared__ R buf[kStageN];  // [component][plane k-1, k, k+1][kLY][kLX]
+  const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
+  const int i0 = blockIdx.x * kMX, j0 = blockIdx.y * kMY, k = blockIdx.z;
+  const int tx = static_cast<int>(threadIdx.x) % kMX, ty = static_cast<int>(threadIdx.x) / kMX;
+  const int i = i0 + tx, j = j0 + ty;
+  if (i0 >= 1 && i0 + kMX + 1 <= nx && j0 >= 1 && j0 + kMY + 1 <= ny && k >= 1 && k + 2 <= nz) {
+    // Interior tile: every staged node lies inside all three components, so
+    // electricValue() is a plain load. The node (li, lj) of the region is
+    // the same for the nine (component, plane) pairs.
+    constexpr int kRegionR = (kLP + kMX * kMY - 1) / (kMX * kMY);
+    R v[kRegionR][9];
+#pragma unroll
+    for (int r = 0; r < kRegionR; ++r) {
+      const int q = static_cast<int>(threadIdx.x) + r * kMX * kMY;
+      const int qc = q < kLP ? q : 0;
+      const int gi = i0 - 1 + qc % kLX, gj = j0 - 1 + qc / kLX;
+#pragma unroll
+      for (int comp = 0; comp < 3; ++comp) {
+        const int n0 = F.n[comp][0], n1 = F.n[comp][1];
+        const R* base = F.e[comp] + (static_cast<size_t>(k - 1) * n1 + gj) * n0 + gi;
+        const size_t plane = static_cast<size_t>(n0) * n1;
+#pragma unroll
+        for (int pz = 0; pz < 3; ++pz) v[r][comp * 3 + pz] = base[pz * plane];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kRegionR; ++r) {
+      const int q = static_cast<int>(threadIdx.x) + r * kMX * kMY;
+      if (q < kLP)
+#pragma unroll
+        for (int cp = 0; cp < 9; ++cp) buf[cp * kLP + q] = v[r][cp];
+    }
+  } else {
+    R v[kStageR];
+#pragma unroll
+    for (int r = 0; r < kStageR; ++r) {
+      const int t = static_cast<int>(threadIdx.x) + r * kMX * kMY;
+      v[r] = 0.0;
+      if (t < kStageN) {
+        const int comp = t / (3 * kLP), rest = t - comp * 3 * kLP;
+        const int pz = rest / kLP, q = rest - pz * kLP;
+        const int gi = i0 - 1 + q % kLX, gj = j0 - 1 + q / kLX, gk = k - 1 + pz;
+        v[r] = comp == 0 ? stagedE<0>(F, gi, gj, gk) : (comp == 1 ? stagedE<1>(F, gi, gj, gk) : stagedE<2>(F, gi, gj, gk));
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kStageR; ++r) {
+      const int t = static_cast<int>(threadIdx.x) + r * kMX * kMY;
+      if (t < kStageN) buf[t] = v[r];
+    }
+  }
+  __syncthreads();
+  if (i >= nx || j >= ny) return;
+  const R* ex[3] = {buf, buf + kLP, buf + 2 * kLP};
+  const R* ey[3] = {buf + 3 * kLP, buf + 4 * kLP, buf + 5 * kLP};
+  const R* ez[3] = {buf + 6 * kLP, buf + 7 * kLP, buf + 8 * kLP};
+  const int li = tx + 1, lj = ty + 1;
+  if (k < nz) {
+    // S_z is x-y smoothing: only its own plane is read
+    const R* eyk[3] = {ey[1], ey[1], ey[1]};
+    const R* eyk1[3] = {ey[2], ey[2], ey[2]};
+    const R* exk[3] = {ex[1], ex[1], ex[1]};
+    const R* exk1[3] = {ex[2], ex[2], ex[2]};
+    finishBx(F, c, P, i, j, k, smoothL<1>(ez, c, li, lj + 1) - smoothL<1>(ez, c, li, lj),
+             smoothL<2>(eyk1, c, li, lj) - smoothL<2>(eyk, c, li, lj));
+    finishBy(F, c, P, i, j, k, smoothL<2>(exk1, c, li, lj) - smoothL<2>(exk, c, li, lj),
+             smoothL<0>(ez, c, li + 1, lj) - smoothL<0>(ez, c, li, lj));
+  }
+  finishBz(F, c, P, i, j, k, smoothL<0>(ey, c, li + 1, lj) - smoothL<0>(ey, c, li, lj),
+           smoothL<1>(ex, c, li, lj + 1) - smoothL<1>(ex, c, li, lj));
+}
+
+template <class R>
+__device__ __forceinline__ double rd(const R* p, int nx, int ny, int i, int j, int k) {
+  return static_cast<double>(p[(static_cast<size_t>(k) * ny + j) * nx + i]);
+}
+
+// Fused electric update: EBFieldGrid::advanceElectric interior curl
+// (eb_field.cpp:582-624), EBConvolutionalPML::afterElectricUpdate
+// (eb_cpml.cpp:470-539) incl. its PEC termination (eb_field.cpp:158-204),
+// and the moving-window inlet overwrite (moving_window_fields.cpp:276-280).
+// MODE 0 writes E in place and advances the histories; MODE 1 writes the
+// zero-current preview into `out` and leaves the histories untouched.
+// E update at lattice node (i, j, k) of the (nx+1) x (ny+1) x (nz+1) node
+// box: Ex (i < nx), Ey (j < ny) and Ez (k < nz) of that node.
+template <int MODE, class R>
+__device__ __forceinline__ void electricNode(const FieldPtrsT<R>& F, const StepCoefsT<R>& c, const CpmlArgsT<R>& P,
+                                             R* ox, R* oy, R* oz, R* sx, R* sy, R* sz, const R* inletX,
+                                             const R* inletY, int i, int j, int k) {
+  const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
+  constexpr bool W = MODE == 0;
+  const R* bx = F.b[0];
+  const R* by = F.b[1];
+  const R* bz = F.b[2];
+  const int kb = P.lowerZ ? 1 : 0, ke = P.upperZ ? nz : nz + 1;
+  if (i < nx) {  // Ex: nx x (ny+1) x (nz+1)
+    const size_t idx = (static_cast<size_t>(k) * (ny + 1) + j) * nx + i;
+    const double e0 = static_cast<double>(F.e[0][idx]);
+    double e = e0;
+    if (k >= 1 && k < nz && j >= 1 && j < ny)
+      e += c.cY * (rd(bz, nx, ny, i, j, k) - rd(bz, nx, ny, i, j - 1, k)) -
+           c.cZ * (rd(by, nx, ny + 1, i, j, k) - rd(by, nx, ny + 1, i, j, k - 1)) - c.cs * F.j[0][idx];
+    if (P.enabled && j >= 1 && j < ny && k >= kb && k < ke) {
+      int o = P.ord[1][1][j];
+      if (o >= 0)
+        e += (1.0 * P.eleScale[1]) * cpmlCorrection<W>(P.prof[1][1][j], rd(bz, nx, ny, i, j, k) - rd(bz, nx, ny, i, j - 1, k),
+                                                        histAt<1, 0, 1>(P, o, i, j, k, nx));
+      o = P.ord[1][2][k];
+      if (o >= 0)
+        e += (-1.0 * P.eleScale[2]) * cpmlCorrection<W>(P.prof[1][2][k], rd(by, nx, ny + 1, i, j, k) - rd(by, nx, ny + 1, i, j, k - 1),
+                                                         histAt<1, 0, 2>(P, o, i, j, k, nx));
+    }
+    if (j == 0 || j == ny || (P.lowerZ && k == 0) || (P.upperZ && k == nz)) e = 0.0;
+    if (inletX && k == 0) e = inletX[static_cast<size_t>(j) * nx + i];
+    (MODE == 0 ? F.e[0] : ox)[idx] = static_cast<R>(e);
+    if (MODE == 1 && sx) sx[idx] = static_cast<R>(e + e0);
+  }
+  if (j < ny) {  // Ey: (nx+1) x ny x (nz+1)
+    const size_t idx = (static_cast<size_t>(k) * ny + j) * (nx + 1) + i;
+    const double e0 = static_cast<double>(F.e[1][idx]);
+    double e = e0;
+    if (k >= 1 && k < nz && i >= 1 && i < nx)
+      e += c.cZ * (rd(bx, nx + 1, ny, i, j, k) - rd(bx, nx + 1, ny, i, j, k - 1)) -
+           c.cX * (rd(bz, nx, ny, i, j, k) - rd(bz, nx, ny, i - 1, j, k)) - c.cs * F.j[1][idx];
+    if (P.enabled && i >= 1 && i < nx && k >= kb && k < ke) {
+      int o = P.ord[1][2][k];
+      if (o >= 0)
+        e += (1.0 * P.eleScale[2]) * cpmlCorrection<W>(P.prof[1][2][k], rd(bx, nx + 1, ny, i, j, k) - rd(bx, nx + 1, ny, i, j, k - 1),
+                                                        histAt<1, 1, 2>(P, o, i, j, k, nx + 1));
+      o = P.ord[1][0][i];
+      if (o >= 0)
+        e += (-1.0 * P.eleScale[0]) * cpmlCorrection<W>(P.prof[1][0][i], rd(bz, nx, ny, i, j, k) - rd(bz, nx, ny, i - 1, j, k),
+                                                         histAt<1, 1, 0>(P, o, i, j, k, nx + 1));
+    }
+    if (i == 0 || i == nx || (P.lowerZ && k == 0) || (P.upperZ && k == nz)) e = 0.0;
+    if (inletY && k == 0) e = inletY[static_cast<size_t>(j) * (nx + 1) + i];
+    (MODE == 0 ? F.e[1] : oy)[idx] = static_cast<R>(e);
+    if (MODE == 1 && sy) sy[idx] = static_cast<R>(e + e0);
+  }
+  if (k < nz) {  // Ez: (nx+1) x (ny+1) x nz
+    const size_t idx = (static_cast<size_t>(k) * (ny + 1) + j) * (nx + 1) + i;
+    const double e0 = static_cast<double>(F.e[2][idx]);
+    double e = e0;
+    const bool in = i >= 1 && i < nx && j >= 1 && j < ny;
+    if (in)
+      e += c.cX * (rd(by, nx, ny + 1, i, j, k) - rd(by, nx, ny + 1, i - 1, j, k)) -
+           c.cY * (rd(bx, nx + 1, ny, i, j, k) - rd(bx, nx + 1, ny, i, j - 1, k)) - c.cs * F.j[2][idx];
+    if (P.enabled && in) {
+      int o = P.ord[1][0][i];
+      if (o >= 0)
+        e += (1.0 * P.eleScale[0]) * cpmlCorrection<W>(P.prof[1][0][i], rd(by, nx, ny + 1, i, j, k) - rd(by, nx, ny + 1, i - 1, j, k),
+                                                        histAt<1, 2, 0>(P, o, i, j, k, nx + 1));
+      o = P.ord[1][1][j];
+      if (o >= 0)
+        e += (-1.0 * P.eleScale[1]) * cpmlCorrection<W>(P.prof[1][1][j], rd(bx, nx + 1, ny, i, j, k) - rd(bx, nx + 1, ny, i, j - 1, k),
+                                                         histAt<1, 2, 1>(P, o, i, j, k, nx + 1));
+    }
+    if (i == 0 || i == nx || j == 0 || j == ny) e = 0.0;
+    (MODE == 0 ? F.e[2] : oz)[idx] = static_cast<R>(e);
+    if (MODE == 1 && sz) sz[idx] = static_cast<R>(e + e0);
+  }
+}
+
+template <int MODE, class R>
+__global__ void __launch_bounds__(256) kElectric(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P, R* ox, R* oy,
+                                                  R* oz, R* sx, R* sy, R* sz, const R* inletX,
+                                                  const R* inletY) {
+  // one thread per node of the (nx+1) x (ny+1) plane k = blockIdx.y
+  const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t % (nx + 1), j = t / (nx + 1);
+  const int k = blockIdx.y;
+  if (j > ny || k > nz) return;
+  electricNode<MODE>(F, c, P, ox, oy, oz, sx, sy, sz, inletX, inletY, i, j, k);
+}
+
+// The zero-current preview (MODE 1 of kElectric) on the nodes owned by the
+// listed cell tiles only: count = high word of *total, the tiles in list
+// order, a workgroup per tile (looping when the grid is smaller). Every
+// node value is kElectric<1>'s.
+template <class R>
+__global__ void __launch_bounds__(256) kElectricPreviewTiles(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P,
+                                                             R* ox, R* oy, R* oz, R* sx, R* sy, R* sz,
+                                                             const R* inletX, const R* inletY, const int* tiles,
+                                                             const unsigned long long* total, TileGrid tg) {
+  const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
+  const int count = static_cast<int>(*total >> 32);
+  for (int b = blockIdx.x; b < count; b += gridDim.x) {
+    const int tile = tiles[b];
+    const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
+    const int i0 = tx * tg.tx, i1 = tx == tg.ntx - 1 ? nx + 1 : min(nx + 1, i0 + tg.tx);
+    const int j0 = ty * tg.ty, j1 = ty == tg.nty - 1 ? ny + 1 : min(ny + 1, j0 + tg.ty);
+    const int k0 = tz * tg.tz, k1 = tz == tg.ntz - 1 ? nz + 1 : min(nz + 1, k0 + tg.tz);
+    const int wx = i1 - i0, wy = j1 - j0, wz = k1 - k0;
+    for (int t = threadIdx.x; t < wx * wy * wz; t += blockDim.x) {
+      const int i = i0 + t % wx, j = j0 + (t / wx) % wy, k = k0 + t / (wx * wy);
+      electricNode<1>(F, c, P, ox, oy, oz, sx, sy, sz, inletX, inletY, i, j, k);
+    }
+  }
+}
+
+template <class R>
+__global__ void kFill(R* p, size_t n, R v) {
+  for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<size_t>(gridDim.x) * blockDim.x)
+    p[t] = v;
+}
+
+template <class R>
+static void fill(hipStream_t s, R* p, size_t n, R v) {
+  if (!n) return;
+  const unsigned blocks = std::min<size_t>((n + 255) / 256, 4096);
+  kFill<R><<<blocks, 256, 0, s>>>(p, n, v);
+  launchCheck("fill");
+}
+
+template <class R>
+void launchMagnetic(FieldStateT<R>& s, const FieldPtrsT<R>& f) {
+  const StepCoefsT<R> c = s.coefs();
+  const CpmlArgsT<R> P = s.cpmlArgs();
+  ScopedKernel sk(s.profiler, s.stream, "magnetic");
+  if (s.g.solver == 1) {
+    dim3 tg(ceilDiv(s.g.nx, kMX), ceilDiv(s.g.ny, kMY), static_cast<unsigned>(s.g.nz + 1));
+    kMagneticCowanTiled<R><<<tg, kMX * kMY, 0, s.stream>>>(f, c, P);
+    kMagneticCowanEdges<R><<<ceilDiv((s.g.nx + s.g.ny) * s.g.nz, 256), 256, 0, s.stream>>>(f, c, P);
+  } else {
+    dim3 grid(ceilDiv((s.g.nx + 1) * (s.g.ny + 1), 256), static_cast<unsigned>(s.g.nz + 1));
+    kMagneticYee<R><<<grid, 256, 0, s.stream>>>(f, c, P);
+  }
+  launchCheck("magnetic");
+}
+
+template <class R>
+void launchElectric(FieldStateT<R>& s, const FieldPtrsT<R>& f, int mode, R* const out[3], const R* inletX,
+                    const R* inletY, R* const sum[3]) {
+  const StepCoefsT<R> c = s.coefs();
+  CpmlArgsT<R> P = s.cpmlArgs();
+  if (!s.hasCpml) {  // PerfectElectricConductorBoundary(lowerZ, upperZ)
+    P.lowerZ = s.lowerZ;
+    P.upperZ = s.upperZ;
+  }
+  dim3 block(256, 1, 1);
+  dim3 grid(ceilDiv((s.g.nx + 1) * (s.g.ny + 1), 256), static_cast<unsigned>(s.g.nz + 1));
+  if (mode == 0) {
+    ScopedKernel sk(s.profiler, s.stream, "electric");
+    kElectric<0, R><<<grid, block, 0, s.stream>>>(f, c, P, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, inletX,
+                                               inletY);
+  } else {
+    ScopedKernel sk(s.profiler, s.stream, "electric_preview");
+    kElectric<1, R><<<grid, block, 0, s.stream>>>(f, c, P, out[0], out[1], out[2], sum ? sum[0] : nullptr,
+                                               sum ? sum[1] : nullptr, sum ? sum[2] : nullptr, inletX, inletY);
+  }
+  launchCheck("electric");
+}
+
+template <class R>
+void launchElectricPreviewTiles(FieldStateT<R>& s, const FieldPtrsT<R>& f, R* const out[3], R* const sum[3],
+                                const R* inletX, const R* inletY, const int* tiles, const unsigned long long* total,
+                                const TileGrid& tg) {
+  const StepCoefsT<R> c = s.coefs();
+  CpmlArgsT<R> P = s.cpmlArgs();
+  if (!s.hasCpml) {
+    P.lowerZ = s.lowerZ;
+    P.upperZ = s.upperZ;
+  }
+  const unsigned blocks = static_cast<unsigned>(std::min(tg.ntx * tg.nty * tg.ntz, 4096));
+  ScopedKernel sk(s.profiler, s.stream, "electric_preview");
+  kElectricPreviewTiles<R><<<std::max(1u, blocks), 256, 0, s.stream>>>(f, c, P, out[0], out[1], out[2], sum[0],
+                                                                        sum[1], sum[2], inletX, inletY, tiles,
+                                                                        total, tg);
+  launchCheck("electric preview tiles");
+}
+
+// Zero the owned J nodes of every listed cell tile (the tiles with an
+// occupied neighbour tile: the only nodes a tile deposition can write).
+template <class R>
+__global__ void __launch_bounds__(256) kClearCurrentTiles(R* jx, R* jy, R* jz, FieldPtrsT<R> F,
+                                                          CurrentFootprint fp) {
+  const int tile = fp.active[blockIdx.x];
+  const int tx = tile % fp.ntx, ty = (tile / fp.ntx) % fp.nty, tz = tile / (fp.ntx * fp.nty);
+  for (int d = 0; d < 3; ++d) {
+    const int n0 = F.n[6 + d][0], n1 = F.n[6 + d][1], n2 = F.n[6 + d][2];
+    const int i0 = tx * fp.tx, i1 = tx == fp.ntx - 1 ? n0 : min(n0, i0 + fp.tx);
+    const int j0 = ty * fp.ty, j1 = ty == fp.nty - 1 ? n1 : min(n1, j0 + fp.ty);
+    const int k0 = tz * fp.tz, k1 = tz == fp.ntz - 1 ? n2 : min(n2, k0 + fp.tz);
+    const int wx = i1 - i0, wy = j1 - j0, wz = k1 - k0;
+    R* dst = d == 0 ? jx : (d == 1 ? jy : jz);
+    for (int t = threadIdx.x; t < wx * wy * wz; t += blockDim.x) {
+      const int i = i0 + t % wx, j = j0 + (t / wx) % wy, k = k0 + t / (wx * wy);
+      dst[(static_cast<size_t>(k) * n1 + j) * n0 + i] = 0.0;
+    }
+  }
+}
+
+template <class R>
+static void clearCurrentNow(FieldStateT<R>& s) {
+  if (s.footprint.valid) {
+    const CurrentFootprint& fp = s.footprint;
+    if (fp.count > 0) {
+      kClearCurrentTiles<R><<<fp.count, 256, 0, s.stream>>>(s.base(6), s.base(7), s.base(8), s.ptrs(), fp);
+      launchCheck("clear current tiles");
+    }
+  } else {
+    for (int a = 6; a < 9; ++a) fill(s.stream, s.comp[a].get(), s.compSize(a), R(0));
+  }
+  s.currentZero = true;
+  s.footprint.valid = false;
+}
+
+template <class R>
+void launchClearCurrent(FieldStateT<R>& s) {
+  if (s.currentZero) return;  // J already holds zeros
+  ScopedKernel sk(s.profiler, s.stream, "clear_current");
+  clearCurrentNow(s);
+}
+
+// Recenter the slid arrays into their slack once the slack is used up.
+template <class R>
+static void recenter(FieldStateT<R>& s) {
+  for (int a = 0; a < 6; ++a) {
+    const size_t bytes = s.compSize(a) * sizeof(R);
+    hipCheck(hipMemcpyAsync(s.scratch.get(), s.base(a), bytes, hipMemcpyDeviceToDevice, s.stream), "hipMemcpyAsync");
+    hipCheck(hipMemcpyAsync(s.comp[a].get() + s.slack * s.plane(a), s.scratch.get(), bytes, hipMemcpyDeviceToDevice,
+                            s.stream),
+             "hipMemcpyAsync");
+  }
+  for (int h = 0; h < 12; ++h) {
+    if (!s.histSlides[h] || !s.hist[h].size()) continue;
+    const size_t block = s.histLayers[h] * s.histRow[h];
+    for (size_t slot = 0; slot < s.histSlots[h]; ++slot) {
+      R* slotBase = s.hist[h].get() + slot * (s.histLayers[h] + s.slack) * s.histRow[h];
+      hipCheck(hipMemcpyAsync(s.scratch.get(), slotBase + s.zb * s.histRow[h], block * sizeof(R),
+                              hipMemcpyDeviceToDevice, s.stream),
+               "hipMemcpyAsync");
+      hipCheck(hipMemcpyAsync(slotBase + s.slack * s.histRow[h], s.scratch.get(), block * sizeof(R),
+                              hipMemcpyDeviceToDevice, s.stream),
+               "hipMemcpyAsync");
+    }
+  }
+  s.zb = s.slack;
+}
+
+template <class R>
+struct ShiftArgs {
+  R* dst[6];
+  const R* src[6];
+  int planeLen[6];
+  R* hist[12];
+  long long slotStride[12];
+  int row[12], slots[12];
+};
+
+// Insert the new k = 0 planes of the six E/B components and zero the new
+// layer of every transverse CPML history, in one launch.
+template <class R>
+__global__ void kShiftInsert(ShiftArgs<R> A) {
+  const int item = blockIdx.y;  // 0..5 component planes, 6.. history slots
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (item < 6) {
+    if (t < A.planeLen[item]) A.dst[item][t] = A.src[item] ? A.src[item][t] : 0.0;
+    return;
+  }
+  int rest = item - 6;
+  for (int h = 0; h < 12; ++h) {
+    if (rest < A.slots[h]) {
+      if (t < A.row[h]) A.hist[h][rest * A.slotStride[h] + t] = 0.0;
+      return;
+    }
+    rest -= A.slots[h];
+  }
+}
+
+// MovingWindowFields::Impl::shift (moving_window_fields.cpp:439-518) with
+// resetCurrent: slide every E/B component and transverse history one plane,
+// insert the reference planes (zero histories) at k = 0, clear J.
+template <class R>
+void windowShift(FieldStateT<R>& s, const R* const planes[6]) {
+  ScopedKernel sk(s.profiler, s.stream, "shift");
+  if (s.zb == 0) recenter(s);
+  s.zb -= 1;
+  ShiftArgs<R> A{};
+  int maxLen = 0, items = 6;
+  for (int a = 0; a < 6; ++a) {
+    A.dst[a] = s.base(a);
+    A.src[a] = planes ? planes[a] : nullptr;
+    A.planeLen[a] = static_cast<int>(s.plane(a));
+    maxLen = std::max(maxLen, A.planeLen[a]);
+  }
+  for (int h = 0; h < 12; ++h) {
+    const bool on = s.histSlides[h] && s.hist[h].size();
+    A.slots[h] = on ? static_cast<int>(s.histSlots[h]) : 0;
+    A.row[h] = static_cast<int>(s.histRow[h]);
+    A.hist[h] = on ? s.hist[h].get() + s.historyDeviceOffset(h, 0) : nullptr;
+    A.slotStride[h] = static_cast<long long>((s.histLayers[h] + s.slack) * s.histRow[h]);
+    items += A.slots[h];
+    maxLen = std::max(maxLen, A.row[h]);
+  }
+  dim3 grid(ceilDiv(maxLen, 256), static_cast<unsigned>(items));
+  kShiftInsert<R><<<grid, 256, 0, s.stream>>>(A);
+  launchCheck("shift insert");
+  if (!s.currentZero) clearCurrentNow(s);
+}
+
+#define APRL_FIELDS_INSTANTIATE(R)                                                                            \
+  template void allocateFields<R>(FieldStateT<R>&, std::size_t);                                              \
+  template void installCpml<R>(FieldStateT<R>&, const aprl_cpml_parameters&, std::size_t, std::size_t, bool, \
+                               bool);                                                                         \
+  template struct FieldStateT<R>;                                                                            \
+  template void launchMagnetic<R>(FieldStateT<R>&, const FieldPtrsT<R>&);                                     \
+  template void launchElectric<R>(FieldStateT<R>&, const FieldPtrsT<R>&, int, R* const*, const R*, const R*,  \
+                                  R* const*);                                                                 \
+  template void launchClearCurrent<R>(FieldStateT<R>&);                                                       \
+  template void launchElectricPreviewTiles<R>(FieldStateT<R>&, const FieldPtrsT<R>&, R* const*, R* const*,     \
+                                              const R*, const R*, const int*, const unsigned long long*,       \
+                                              const TileGrid&);                                                \
+  template void windowShift<R>(FieldStateT<R>&, const R* const*);
+
+}  // namespace aprl_gpu

@@ -274,8 +274,9 @@ __device__ __forceinline__ int clampCell(double lattice, int n) {
 }
 
 __global__ void kTileKeys(const double* x, const double* y, const double* z, int n, GArgs G, int ntx, int nty,
-                          unsigned* key, unsigned* perm) {
+                          unsigned* key, unsigned* perm, TileRangeArgs R) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  clearTileRanges(R, p);
   if (p >= n) return;
   const int cx = clampCell((x[p] - G.o[0]) / G.h[0], G.nx);
   const int cy = clampCell((y[p] - G.o[1]) / G.h[1], G.ny);
@@ -300,23 +301,18 @@ __global__ void kPermute(PermuteArgs A, const unsigned* perm, int n) {
   A.idDst[p] = A.idSrc[q];
 }
 
-// Particle range of every tile: binary search of the tile in the sorted keys.
-__global__ void kTileRanges(const unsigned* key, int n, int ntiles, int* begin, int* end, unsigned* occupied) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntiles) return;
-  auto lower = [&](unsigned v) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (key[mid] < v) lo = mid + 1;
-      else hi = mid;
-    }
-    return lo;
-  };
-  const int b = lower(static_cast<unsigned>(t)), e = lower(static_cast<unsigned>(t) + 1u);
-  begin[t] = b;
-  end[t] = e;
-  if (e > b) atomicAdd(occupied, 1u);
+// Particle range of every occupied tile from the run boundaries of the
+// sorted keys (the ranges were cleared to empty before the sort), and the
+// occupied-tile count.
+__global__ void kTileBounds(const unsigned* key, int n, int* begin, int* end, unsigned* occupied) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const unsigned k = key[p];
+  if (p == 0 || key[p - 1] != k) {
+    begin[k] = p;
+    atomicAdd(occupied, 1u);
+  }
+  if (p == n - 1 || key[p + 1] != k) end[k] = p + 1;
 }
 
 // Per tile: chunk count (low word) and whether any tile within one tile in
@@ -746,18 +742,6 @@ void reserveParticles(ParticleStore& P, std::size_t capacity) {
   P.sortTemp.resize(P.sortTempBytes);
 }
 
-// Onesweep radix sort at every size above one block: rocprim's merge-sort
-// path for mid-size inputs costs one kernel per merge pass.
-using TileSortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
-                                                  rocprim::default_config, 0>;
-
-void tileSort(void* temp, std::size_t& bytes, unsigned* key, unsigned* keySorted, unsigned* perm,
-              unsigned* permSorted, unsigned n, int bits, hipStream_t st) {
-  hipCheck(rocprim::radix_sort_pairs<TileSortConfig>(temp, bytes, key, keySorted, perm, permSorted, n, 0,
-                                                     static_cast<unsigned>(bits), st),
-           temp ? "radix_sort_pairs" : "radix_sort_pairs size");
-}
-
 void raiseParticleErrors(unsigned e) {
   if (!e) return;
   if (e & kErrNonfinite) throw std::runtime_error("Nonfinite open coupling orbit");
@@ -827,7 +811,7 @@ static void sortAndChunk(CouplingState& C, ParticleStore& P, const GArgs& G, int
   ScopedKernel sk(F.profiler, st, "sort");
   const unsigned blocks = ceilDiv(n, 256);
   kTileKeys<<<blocks, 256, 0, st>>>(P.s[0].get(), P.s[1].get(), P.s[2].get(), n, G, ntx, nty, P.key.get(),
-                                    P.perm.get());
+                                    P.perm.get(), TileRangeArgs{C.tileBegin.get(), C.tileEnd.get(), ntiles});
   launchCheck("tile keys");
   const int bits = bitsFor(static_cast<unsigned>(ntiles - 1));
   std::size_t bytes = 0;
@@ -857,9 +841,8 @@ static void sortAndChunk(CouplingState& C, ParticleStore& P, const GArgs& G, int
 // scan, the chunk list and the active-tile list (all in tile order).
 void buildChunkLists(CouplingState& C, hipStream_t st, const unsigned* keySorted, int n, int ntx, int nty,
                      int ntiles, int rz, unsigned* occupied) {
-  kTileRanges<<<ceilDiv(ntiles, 256), 256, 0, st>>>(keySorted, n, ntiles, C.tileBegin.get(), C.tileEnd.get(),
-                                                     occupied);
-  launchCheck("tile ranges");
+  kTileBounds<<<ceilDiv(n, 256), 256, 0, st>>>(keySorted, n, C.tileBegin.get(), C.tileEnd.get(), occupied);
+  launchCheck("tile bounds");
   const int ntz = ntiles / (ntx * nty);
   kTileCounts<<<ceilDiv(ntiles + 1, 256), 256, 0, st>>>(C.tileBegin.get(), C.tileEnd.get(), ntx, nty, ntz, rz,
                                                          C.tileCounts.get());

@@ -132,6 +132,21 @@ struct CouplingState {
   DeviceArray<double> rho[2];  // diagnostic charge densities
 };
 
+// Tile ranges cleared to empty by the key kernels (grid-stride over the
+// tiles), so that the boundary pass after the sort only writes occupied
+// tiles.
+struct TileRangeArgs {
+  int* begin;
+  int* end;
+  int ntiles;
+};
+__device__ __forceinline__ void clearTileRanges(const TileRangeArgs& R, int gid) {
+  for (int t = gid; t < R.ntiles; t += static_cast<int>(gridDim.x * blockDim.x)) {
+    R.begin[t] = 0;
+    R.end[t] = 0;
+  }
+}
+
 // Host helpers shared by the FP64 and FP32-split coupling steps.
 void buildChunkLists(CouplingState& C, hipStream_t st, const unsigned* keySorted, int n, int ntx, int nty,
                      int ntiles, int rz, unsigned* occupied);

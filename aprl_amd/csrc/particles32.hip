@@ -273,8 +273,9 @@ __device__ void gatherBQuintic32(const GArgs32& G, Chord32 c, float res[3], unsi
 // Sort.
 
 __global__ void kTileKeys32(const int* cx, const int* cy, const int* cz, int n, GArgs32 G, int ntx, int nty,
-                            unsigned* key, unsigned* perm) {
+                            unsigned* key, unsigned* perm, TileRangeArgs R) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  clearTileRanges(R, p);
   if (p >= n) return;
   const int x = clampIndex32(cx[p] - G.oc[0], G.nx);
   const int y = clampIndex32(cy[p] - G.oc[1], G.ny);
@@ -782,7 +783,8 @@ void sortSplit(CouplingState& C, SplitStore& P, const GArgs32& G, int ntx, int n
   ScopedKernel sk(F.profiler, st, "sort");
   const unsigned blocks = ceilDiv(n, 256);
   kTileKeys32<<<blocks, 256, 0, st>>>(P.cell[0].get(), P.cell[1].get(), P.cell[2].get(), n, G, ntx, nty,
-                                      P.key.get(), P.perm.get());
+                                      P.key.get(), P.perm.get(),
+                                      TileRangeArgs{C.tileBegin.get(), C.tileEnd.get(), ntiles});
   launchCheck("tile keys");
   const int bits = bitsFor32(static_cast<unsigned>(ntiles - 1));
   std::size_t bytes = 0;
