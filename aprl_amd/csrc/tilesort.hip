@@ -15,8 +15,6 @@
 
 namespace aprl_gpu {
 
-namespace {
-
 constexpr int kSortThreads = 256;  // one item per thread and round
 constexpr int kSortWaves = kSortThreads / 64;
 
@@ -117,7 +115,7 @@ __global__ void __launch_bounds__(kSortThreads) kDigitScatter(const unsigned* ke
 // kCountingSortLimit items the rounds per block grow too long and rocprim's
 // onesweep radix sort takes over.
 constexpr unsigned kCountingSortLimit = 1u << 18;
-int itemsPerBlock(unsigned n) {
+static int itemsPerBlock(unsigned n) {
   int items = kSortThreads;
   while ((static_cast<unsigned long long>(n) + items - 1) / items > 128ull) items *= 2;
   return items;
@@ -125,8 +123,6 @@ int itemsPerBlock(unsigned n) {
 
 using WideSortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
                                                   rocprim::default_config, 0>;
-
-}  // namespace
 
 // Scratch layout for the counting sort: digit table, then one key and one
 // value buffer for the intermediate pass.

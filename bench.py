@@ -25,6 +25,23 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 WORD = 8  # FP64
 
+# Measured HBM bytes per launch of the plain FP64 form on this workload:
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, one counter per run, calibrated
+# on a plain 256 MiB device copy (bytes = 2.0 x FETCH_SIZE + 1.0 x
+# WRITE_SIZE, in KiB). Source and commands:
+# profiles/bending_64x64x512_fp64_binomial3x_kernels_traffic.md. Traffic
+# above the algorithmic bytes is re-reading (staged halos, per-chunk boxes).
+MEASURED_TRAFFIC_FP64 = {
+    "magnetic": (2 * 89686 + 69040 + 2 * 10118 + 1792) * 1024,
+    "electric": (2 * 119936 + 70141) * 1024,
+    "electric_preview": (2 * 56547 + 27084) * 1024,
+    "gather": (2 * 37738 + 4862) * 1024,
+    "residual": (2 * 42060 + 4901) * 1024,
+    "deposit": (2 * 18253 + 26176) * 1024,
+    "push": (2 * 5562 + 5469) * 1024,
+    "reduce_current": (2 * 59630 + 57898) * 1024,
+}
+
 
 def kernel_bytes(w, name, tiles, chunks):
     """Algorithmic HBM bytes of one launch: what the kernel must read or
@@ -226,6 +243,8 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--leg", choices=("all", "fp64", "fp32-split"), default="all",
+                    help="time only this form's binomial3-x leg (for profiling runs)")
     ap.add_argument("--error-study", action="store_true",
                     help="print the FP32-split vs FP64 errors at 100, 1000 and 10000 steps and exit")
     args = ap.parse_args()
@@ -235,6 +254,12 @@ def main():
         sys.exit(2)
     if args.error_study:
         print(json.dumps({"error_study": error_study()}))
+        return
+    if args.leg != "all":
+        w, elapsed, trials, _, (tiles, chunks) = run_gpu(args, profile=False, precision=args.leg)
+        print(json.dumps({"leg": args.leg, "value": w.particles * args.steps / elapsed, "unit": "particle-steps/s",
+                          "ms_per_step": 1e3 * elapsed / args.steps, "steps": args.steps, "warmup": args.warmup,
+                          "occupied_tiles": tiles, "chunks": chunks}))
         return
     w, elapsed, trials, kernels, (tiles, chunks) = run_gpu(args)
     # the other production shape of the same workload, timed the same way
@@ -254,8 +279,9 @@ def main():
         if b is not None:
             achieved = b / (k["avg_us"] * 1e-6) / 1e9
             roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                        "algorithmic_bytes": b, "avg_launch_us": round(k["avg_us"], 2)}
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": MEASURED_TRAFFIC_FP64.get(dominant), "algorithmic_bytes": b,
+                        "avg_launch_us": round(k["avg_us"], 2)}
     cpu = None if args.no_cpu_baseline else run_cpu_baseline(w)
     line = {
         "metric": "particle-steps/s + cell-updates/s, FP32-split vs FP64, % of HBM peak",
@@ -287,6 +313,8 @@ def main():
         "kernels_us": {k: round(v["avg_us"], 2) for k, v in kernels.items()},
         "kernels_gbs": {k: round(kernel_bytes(w, k, tiles, chunks) / (v["avg_us"] * 1e-6) / 1e9, 1)
                         for k, v in modeled.items()},
+        "kernels_traffic_gbs": {k: round(MEASURED_TRAFFIC_FP64[k] / (v["avg_us"] * 1e-6) / 1e9, 1)
+                                for k, v in kernels.items() if k in MEASURED_TRAFFIC_FP64},
     }
     print(json.dumps(line))
 
