@@ -48,6 +48,13 @@ class StepReport(C.Structure):
                 ("chunks", C.c_uint)]
 
 
+class InstanceStep(C.Structure):
+    """aprl_instance_step: one instance of a batched window step."""
+    _fields_ = [("fields", C.c_void_p), ("coupling", C.c_void_p), ("particles", C.c_void_p), ("step", C.c_size_t),
+                ("offset", C.c_ssize_t), ("origin", C.c_double * 3), ("time", C.c_double), ("substeps", C.c_uint),
+                ("report", StepReport)]
+
+
 class AprlError(RuntimeError):
     """A failed C-ABI call; ``status`` is the aprl_status code."""
 
@@ -126,6 +133,7 @@ def lib() -> C.CDLL:
     L.aprl_coupling_set_particle_boundary.argtypes = [vp, C.POINTER(_sz), C.c_double]
     L.aprl_coupling_step.argtypes = [vp, vp, _dp, C.c_double, C.c_uint, C.POINTER(StepReport)]
     L.aprl_coupling_set_split_trials.argtypes = [vp, C.c_uint, C.c_int]
+    L.aprl_window_step_batch.argtypes = [C.POINTER(InstanceStep), _sz]
     L.aprl_fields_interior_energy.argtypes = [vp, C.POINTER(_sz), C.POINTER(_sz), _dp]
     L.aprl_fields_face_power.argtypes = [vp, C.POINTER(_sz), C.POINTER(_sz), _dp]
     L.aprl_coupling_continuity.argtypes = [vp, vp, _dp]
@@ -172,6 +180,6 @@ EXPORTED = [
     "aprl_particles_create", "aprl_particles_destroy", "aprl_particles_upload", "aprl_particles_download",
     "aprl_particles_count", "aprl_coupling_create", "aprl_coupling_destroy", "aprl_coupling_set_frame",
     "aprl_coupling_add_magnet", "aprl_coupling_set_particle_boundary", "aprl_coupling_step",
-    "aprl_coupling_set_split_trials",
+    "aprl_coupling_set_split_trials", "aprl_window_step_batch",
     "aprl_fields_interior_energy", "aprl_fields_face_power", "aprl_coupling_continuity", "aprl_coupling_gauss",
 ]

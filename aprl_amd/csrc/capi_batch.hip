@@ -1,0 +1,65 @@
+// C-ABI entry point that steps many independent window instances together
+// (declared in include/aprl_gpu.h).
+#include "handles.hpp"
+
+namespace aprl_gpu {
+void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], double time, unsigned substeps,
+                  aprl_step_report* report);
+}  // namespace aprl_gpu
+
+using namespace aprl_gpu;
+
+namespace {
+// Rethrow a nested C-ABI failure with the exception type its status maps to.
+void rethrow(aprl_status s) {
+  if (s == APRL_OK) return;
+  const std::string m = aprl_last_error();
+  switch (s) {
+    case APRL_INVALID_ARGUMENT: throw std::invalid_argument(m);
+    case APRL_OUT_OF_RANGE: throw std::out_of_range(m);
+    case APRL_LOGIC_ERROR: throw std::logic_error(m);
+    case APRL_DEVICE_ERROR: throw DeviceError(m);
+    default: throw std::runtime_error(m);
+  }
+}
+}  // namespace
+
+extern "C" {
+
+aprl_status aprl_window_step_batch(aprl_instance_step* steps, size_t count) {
+  return guarded([&] {
+    if (count && !steps) throw std::invalid_argument("Null instance step array");
+    for (size_t i = 0; i < count; ++i) {
+      const aprl_instance_step& s = steps[i];
+      if (!s.fields || !s.coupling || !s.particles) throw std::invalid_argument("Null instance handle");
+      if (s.coupling->fields != s.fields || s.particles->fields != s.fields)
+        throw std::invalid_argument("Instance coupling, particles and field grid do not belong together");
+      for (size_t j = 0; j < i; ++j)
+        if (steps[j].fields == s.fields) throw std::invalid_argument("An instance appears twice in one batch");
+    }
+    // Phase 1: field half step and, for the FP32-split form, the sort and
+    // the preview, for every instance before any instance waits.
+    for (size_t i = 0; i < count; ++i) {
+      aprl_instance_step& s = steps[i];
+      rethrow(aprl_fields_clear_current(s.fields));
+      rethrow(aprl_window_begin_step(s.fields, s.step, s.offset, nullptr));
+      rethrow(aprl_fields_advance_magnetic(s.fields, nullptr));
+      rethrow(aprl_window_set_inlet(s.fields, nullptr, nullptr));
+      if (s.fields->fp32) splitStepBegin(s.coupling->c, s.particles->sp, s.origin, s.time, s.substeps);
+    }
+    // Phase 2: the trials, the commit and the window shift.
+    for (size_t i = 0; i < count; ++i) {
+      aprl_instance_step& s = steps[i];
+      if (s.fields->fp32) splitStepMiddle(s.coupling->c, s.particles->sp);
+      else couplingStep(s.coupling->c, s.particles->p, s.origin, s.time, s.substeps, &s.report);
+      rethrow(aprl_window_shift(s.fields, s.step + 1, s.offset - 1, nullptr));
+    }
+    // Phase 3: wait for every instance and check its errors.
+    for (size_t i = 0; i < count; ++i) {
+      aprl_instance_step& s = steps[i];
+      if (s.fields->fp32) splitStepEnd(s.coupling->c, s.particles->sp, &s.report);
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,6 +1,8 @@
 // Device particle store and the conservative-source coupling state.
 #pragma once
 
+#include <memory>
+
 #include "fields.hpp"
 
 namespace aprl_gpu {
@@ -82,6 +84,8 @@ struct SplitStore {
   std::size_t sortTempBytes = 0;
 };
 
+struct SplitStepContext;  // per-step state between the phases of a split step
+
 // Particles per work chunk: one chunk is a contiguous run of tile-sorted
 // particles of one tile. The deposit gives each chunk one workgroup of
 // kChunk lanes; the gathers give it 2*kChunk lanes (E waves and B waves).
@@ -120,7 +124,8 @@ struct CouplingState {
   DeviceArray<double> reduceBlocks, reduceOut;
   DeviceArray<unsigned> error;
   PinnedHost pinned;
-  Event sorted;
+  Event sorted, stepDone;
+  std::shared_ptr<SplitStepContext> split;
   // the last coupled step: its store and origin (its start positions stay
   // in the store's end arrays until the next step)
   struct {
@@ -169,5 +174,11 @@ void downloadSplitParticles(const SplitStore& P, const FieldState32& F, double* 
                             double* w, uint64_t* id);
 void couplingStepSplit(CouplingState& C, SplitStore& P, const double origin[3], double time, unsigned substeps,
                        aprl_step_report* report);
+// The same in three phases, so that steps of independent instances can be
+// interleaved: begin (sort, preview), middle (trials, commit, async
+// readback), end (wait, errors, report).
+void splitStepBegin(CouplingState& C, SplitStore& P, const double origin[3], double time, unsigned substeps);
+void splitStepMiddle(CouplingState& C, SplitStore& P);
+void splitStepEnd(CouplingState& C, SplitStore& P, aprl_step_report* report);
 
 }  // namespace aprl_gpu

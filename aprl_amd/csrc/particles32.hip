@@ -817,9 +817,23 @@ void sortSplit(CouplingState& C, SplitStore& P, const GArgs32& G, int ntx, int n
 
 }  // namespace
 
-void couplingStepSplit(CouplingState& C, SplitStore& P, const double origin[3], double time, unsigned substeps,
-                       aprl_step_report* report) {
+struct SplitStepContext {
+  bool pending = false, middleDone = false, verify = false;
+  int n = 0, ntx = 0, nty = 0, ntz = 0, ntiles = 0;
+  std::size_t chunkBound = 0;
+  unsigned trials = 2, nChunks = 0, occupied = 0;
+  GArgs32 G{};
+  StepArgs S{};
+  double o[3] = {0.0, 0.0, 0.0};
+};
+
+// Phase 1 of a split step: checks, J clear, sort and chunk lists (their
+// totals copied back asynchronously), and the zero-current preview.
+void splitStepBegin(CouplingState& C, SplitStore& P, const double origin[3], double time, unsigned substeps) {
   FieldState32& F = *C.fields32;
+  if (!C.split) C.split = std::make_shared<SplitStepContext>();
+  SplitStepContext& X = *C.split;
+  if (X.pending) throw std::logic_error("The previous FP32-split coupling step was not finished");
   if (F.window && F.phase != 2) throw std::logic_error("Moving-window electric phase mismatch");
   if (F.window && !F.inletSet) throw std::logic_error("Moving-window inlet planes were not supplied");
   if (substeps == 0) throw std::invalid_argument("Particle substeps must be positive");
@@ -871,6 +885,36 @@ void couplingStepSplit(CouplingState& C, SplitStore& P, const double origin[3], 
     launchElectricPreviewTiles(F, F.ptrs(), out, sums, F.window ? F.inletPlane[0] : nullptr,
                                F.window ? F.inletPlane[1] : nullptr, C.activeTiles.get(), C.tileScan.get() + ntiles,
                                TileGrid{ntx, nty, ntz, kTileX, kTileY, kTileZ});
+  X.pending = true;
+  X.middleDone = false;
+  X.n = n;
+  X.ntx = ntx;
+  X.nty = nty;
+  X.ntz = ntz;
+  X.ntiles = ntiles;
+  X.chunkBound = chunkBound;
+  X.trials = trials;
+  X.G = G;
+  for (int a = 0; a < 3; ++a) X.o[a] = o[a];
+  X.S = stepArgsFor(C, F.g, F.window, o, time, substeps);
+}
+
+// Phase 2: wait for the chunk totals, enqueue the Picard trials and the
+// commit, and the one asynchronous readback of the step; the store's
+// state arrays are swapped (stream order keeps later work consistent).
+void splitStepMiddle(CouplingState& C, SplitStore& P) {
+  FieldState32& F = *C.fields32;
+  if (!C.split || !C.split->pending || C.split->middleDone) throw std::logic_error("FP32-split step phase mismatch");
+  SplitStepContext& X = *C.split;
+  const int n = X.n, ntx = X.ntx, nty = X.nty, ntz = X.ntz, ntiles = X.ntiles;
+  const std::size_t chunkBound = X.chunkBound;
+  const unsigned trials = X.trials;
+  const GArgs32& G = X.G;
+  const double* o = X.o;
+  hipStream_t st = F.stream;
+  unsigned* pin = static_cast<unsigned*>(C.pinned.get(1024));
+  double* pinD = reinterpret_cast<double*>(static_cast<unsigned char*>(C.pinned.get(1024)) + 512);
+  (void)ntiles;
   unsigned nChunks = 0, nActive = 0, occupiedTiles = 0;
   if (n) {
     hipCheck(hipEventSynchronize(C.sorted.get()), "hipEventSynchronize");
@@ -889,7 +933,7 @@ void couplingStepSplit(CouplingState& C, SplitStore& P, const double origin[3], 
   if (C.reduceBlocks.size() < 4 * static_cast<std::size_t>(std::max(1u, nChunks)))
     C.reduceBlocks.resize(4 * static_cast<std::size_t>(std::max(1u, nChunks)));
 
-  const StepArgs S = stepArgsFor(C, F.g, F.window, o, time, substeps);
+  const StepArgs& S = X.S;
   PArgs32 A = particleArgs32(P, F);
   const bool verify = C.splitVerify;
   for (unsigned trial = 0; trial < trials; ++trial) {
@@ -972,26 +1016,52 @@ void couplingStepSplit(CouplingState& C, SplitStore& P, const double origin[3], 
            "hipMemcpyAsync");
   if (verify && nChunks)
     hipCheck(hipMemcpyAsync(pinD, C.reduceOut.get(), 4 * sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
-  hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
-  for (unsigned t = 0; t < trials; ++t) raiseParticleErrors(pin[4 + t]);
-  double force = 0., work = 0.;
-  if (verify && nChunks) {
-    force = pinD[0];
-    work = pinD[3] == 0. ? 0. : (pinD[2] > 0. ? pinD[1] / pinD[2] : pinD[1]);
-  }
+  hipCheck(hipEventRecord(C.stepDone.get(), st), "hipEventRecord");
+  X.verify = verify;
+  X.nChunks = nChunks;
+  X.occupied = occupiedTiles;
+  X.middleDone = true;
   for (int a = 0; a < 3; ++a) P.cell[a].swap(P.endCell[a]);
   for (int a = 0; a < 6; ++a) P.s[a].swap(P.end[a]);
   C.lastStep.valid = true;
   C.lastStep.store = &P;
   C.lastStep.generation = P.generation;
   for (int a = 0; a < 3; ++a) C.lastStep.origin[a] = o[a];
+}
+
+// Phase 3: wait for the step, raise the first device error in trial order,
+// fill the report.
+void splitStepEnd(CouplingState& C, SplitStore& P, aprl_step_report* report) {
+  (void)P;
+  if (!C.split || !C.split->pending || !C.split->middleDone) throw std::logic_error("FP32-split step phase mismatch");
+  SplitStepContext& X = *C.split;
+  X.pending = false;
+  const unsigned trials = X.trials;
+  const bool verify = X.verify;
+  const unsigned nChunks = X.nChunks;
+  unsigned* pin = static_cast<unsigned*>(C.pinned.get(1024));
+  double* pinD = reinterpret_cast<double*>(static_cast<unsigned char*>(C.pinned.get(1024)) + 512);
+  hipCheck(hipEventSynchronize(C.stepDone.get()), "hipEventSynchronize");
+  for (unsigned t = 0; t < trials; ++t) raiseParticleErrors(pin[4 + t]);
+  double force = 0., work = 0.;
+  if (verify && nChunks) {
+    force = pinD[0];
+    work = pinD[3] == 0. ? 0. : (pinD[2] > 0. ? pinD[1] / pinD[2] : pinD[1]);
+  }
   if (report) {
     report->iterations = trials;
     report->force = force;
     report->work = work;
-    report->tiles = occupiedTiles;
+    report->tiles = X.occupied;
     report->chunks = nChunks;
   }
+}
+
+void couplingStepSplit(CouplingState& C, SplitStore& P, const double origin[3], double time, unsigned substeps,
+                       aprl_step_report* report) {
+  splitStepBegin(C, P, origin, time, substeps);
+  splitStepMiddle(C, P);
+  splitStepEnd(C, P, report);
 }
 
 }  // namespace aprl_gpu

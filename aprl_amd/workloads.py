@@ -89,10 +89,11 @@ COMOVING = Workload(
     charge_electrons=6.0e5, dipole_tesla=0.0, dipole_length=0.0, seed=777, z_cells=128.0, box_uz=-30.0)
 
 
-def setup_gpu(w: Workload, shape: str = "linear-binomial3", precision: str = "fp64"):
+def setup_gpu(w: Workload, shape: str = "linear-binomial3", precision: str = "fp64", data=None):
     """Field grid + window + coupling + particle store for workload w;
     shape is "linear-binomial3" (with x-binomial3 smoothing) or
-    "quintic-compensated", precision "fp64" or "fp32-split"."""
+    "quintic-compensated", precision "fp64" or "fp32-split"; data = the
+    output of w.generate() when already generated."""
     import aprl_amd as A
     g = A.EBGridGeometry(*w.dims, *w.spacing, w.dt)
     fields = A.EBFieldGrid(g, precision=precision)
@@ -104,7 +105,7 @@ def setup_gpu(w: Workload, shape: str = "linear-binomial3", precision: str = "fp
         coupling.addPrescribedLabMagnet("uniform-dipole", [0.0, 0.0, -0.5 * w.dipole_length], w.dipole_length,
                                         peak_field=w.dipole_tesla, polarization_angle=0.5 * np.pi)
     coupling.setParticleBoundary((w.cpml, w.cpml, 0))
-    x, u, q, m, ids = w.generate()
+    x, u, q, m, ids = w.generate() if data is None else data
     store = A.ParticleStore(fields, len(q))
     store.upload(x, u, q, m, None, ids)
     return fields, window, coupling, store
@@ -138,3 +139,57 @@ class Stepper:
         self.origin[2] = (self.offset - 10_000) * self.w.spacing[2]
         self.time += self.w.dt
         self.step_index += 1
+
+
+class BatchStepper:
+    """Steps several independent instances (each a workload with its own
+    grid, window, coupling and store) with aprl_window_step_batch: the
+    per-instance bookkeeping of Stepper, one C-ABI call per step."""
+
+    def __init__(self, instances, offset: int = 10_000):
+        import ctypes as C
+        from . import _capi
+        self.instances = instances  # (workload, fields, window, coupling, store)
+        self.steps = (_capi.InstanceStep * len(instances))()
+        for s, (w, f, win, cg, st) in zip(self.steps, instances):
+            s.fields, s.coupling, s.particles = f.handle, cg.handle, st.handle
+            s.step, s.offset, s.time, s.substeps = 0, offset, 0.0, 1
+            for a in range(3):
+                s.origin[a] = 0.0
+        self._C = C
+        self.trials = []
+
+    def step(self):
+        from ._capi import check, lib
+        check(lib().aprl_window_step_batch(self.steps, len(self.instances)))
+        self.trials.append([s.report.iterations for s in self.steps])
+        for s, (w, *_rest) in zip(self.steps, self.instances):
+            s.step += 1
+            s.offset -= 1
+            # Simulation::finishMovingWindowStep: origin = fma(offset, dz, ref)
+            s.origin[2] = (s.offset - 10_000) * w.spacing[2]
+            s.time += w.dt
+
+
+def batch_workloads(count: int, seed: int = 4242):
+    """Independent instances of the batched scan: particle counts
+    log-uniform in [2e4, 1e6], grids drawn from 32x32x256, 48x48x384 and
+    64x64x512 (8-cell transverse CPML), lab gamma uniform in [100, 1000], a
+    lab dipole uniform in [0.1, 1] T; transverse sizes scaled to each grid's
+    physical aperture so that no particle starts in the CPML."""
+    rng = np.random.default_rng(seed)
+    grids = [(32, 32, 256), (48, 48, 384), (64, 64, 512)]
+    out = []
+    for i in range(count):
+        dims = grids[int(rng.integers(0, 3))]
+        n = int(round(np.exp(rng.uniform(np.log(2e4), np.log(1e6)))))
+        gamma = float(rng.uniform(100.0, 1000.0))
+        tesla = float(rng.uniform(0.1, 1.0))
+        aperture = (dims[0] - 2 * 8) * 5.0e-6
+        sig = aperture / 12.0
+        out.append(Workload(
+            name=f"batch instance {i}", dims=dims, spacing=(5.0e-6, 5.0e-6, 2.0e-6), cpml=8, particles=n,
+            gamma_lab=gamma, gamma_boost=5.0, sigma=(sig, sig, 2e-6), sigma_gamma=5e-4, emittance=1e-6 * sig / 20e-6,
+            charge_electrons=float(n), dipole_tesla=tesla, dipole_length=0.2, seed=seed + 17 * i,
+            z_cells=60.0))
+    return out

@@ -108,6 +108,35 @@ def run_gpu(args, shape="linear-binomial3", profile=True, precision="fp64"):
     return w, elapsed, trials, kernels, (coupling.last_tiles, coupling.last_chunks)
 
 
+def run_batch(count=48, steps=20, warmup=2):
+    """The batched scan: `count` independent instances (workloads.
+    batch_workloads) advanced together by aprl_window_step_batch, in both
+    forms from the same generated bunches; whole-batch particle-steps/s."""
+    from aprl_amd import workloads
+    ws = workloads.batch_workloads(count)
+    data = [w.generate() for w in ws]
+    out = {"instances": count, "particles": int(sum(w.particles for w in ws)), "steps": steps}
+    for precision in ("fp32-split", "fp64"):
+        inst = [(w, *workloads.setup_gpu(w, "linear-binomial3", precision, d)) for w, d in zip(ws, data)]
+        bs = workloads.BatchStepper(inst)
+        for _ in range(warmup):
+            bs.step()
+        for _, f, *_r in inst:
+            f.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            bs.step()
+        for _, f, *_r in inst:
+            f.synchronize()
+        elapsed = time.perf_counter() - t0
+        out[precision.replace("-", "_")] = {"value": out["particles"] * steps / elapsed,
+                                            "ms_per_step": 1e3 * elapsed / steps}
+        del bs, inst
+        print(f"batch {precision}: {1e3 * elapsed / steps:.2f} ms per step", file=sys.stderr, flush=True)
+    out["fp32_split_vs_fp64"] = round(out["fp32_split"]["value"] / out["fp64"]["value"], 3)
+    return out
+
+
 def split_error(steps=100, shape="linear-binomial3"):
     """Relative L2 error of the FP32-split form against the plain FP64 form
     (itself within 1e-12 of the oracle per step) after `steps` steps of the
@@ -268,6 +297,7 @@ def main():
     _, s_elapsed, s_trials, s_kernels, _ = run_gpu(args, precision="fp32-split")
     _, sq_elapsed, _, _, _ = run_gpu(args, "quintic-compensated", profile=False, precision="fp32-split")
     s_error = split_error()
+    batch = run_batch()
     ms = 1e3 * elapsed / args.steps
     value = w.particles * args.steps / elapsed
     modeled = {k: v for k, v in kernels.items() if kernel_bytes(w, k, tiles, chunks) is not None}
@@ -308,6 +338,7 @@ def main():
                        "quintic_vs_fp64": round(q_elapsed / sq_elapsed, 3),
                        "error_vs_fp64": s_error,
                        "kernels_us": {k: round(v["avg_us"], 2) for k, v in s_kernels.items()}},
+        "batch": batch,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels_us": {k: round(v["avg_us"], 2) for k, v in kernels.items()},

@@ -202,6 +202,37 @@ aprl_status aprl_coupling_step(aprl_coupling* coupling, aprl_particles* particle
 aprl_status aprl_coupling_set_split_trials(aprl_coupling* coupling, unsigned trials, int verify);
 
 /* ---------------------------------------------------------------------------
+ * Batched steps of independent instances.
+ */
+
+/* One instance of a batched window step. step/offset/origin/time are the
+ * instance's window step index, window offset, grid box origin (m) and box
+ * time at the start of the step (the values the single-instance calls take);
+ * report is written. */
+typedef struct aprl_instance_step {
+  aprl_fields* fields;
+  aprl_coupling* coupling;
+  aprl_particles* particles;
+  size_t step;
+  ptrdiff_t offset;
+  double origin[3];
+  double time;
+  unsigned substeps;
+  aprl_step_report report;
+} aprl_instance_step;
+
+/* No counterpart in the reference, which advances one bunch per process:
+ * one whole window step of Simulation::solve (simulation.cpp:518-583) --
+ * clear J, begin_step, advance_magnetic, set_inlet, the coupled source step,
+ * shift -- for each of `count` independent instances with a zero inlet
+ * reference, enqueued phase by phase across the instances so that their
+ * kernels overlap on the device (every field grid owns a stream). The
+ * FP32-split form runs its step in three phases (sort and preview; trials,
+ * commit and shift; wait); the FP64 form runs its adaptive Picard loop in
+ * the middle phase. Errors are the single-instance calls' errors. */
+aprl_status aprl_window_step_batch(aprl_instance_step* steps, size_t count);
+
+/* ---------------------------------------------------------------------------
  * FP64 diagnostics (compensated, fixed-order device reductions).
  */
 
