@@ -148,7 +148,17 @@ namespace {
 struct CouplingHandle {
   orc::Coupling c;
   std::vector<orc::Particle> particles;
+  std::vector<orc::Carrier> carriers;
+  orc::BoundaryCounters counters;
   std::size_t cpmlCells[3] = {0, 0, 0};
+  // Particle-boundary geometry: the grid's CPML grading with the particle
+  // cells above.
+  orc::CpmlParams boundaryParams() const {
+    orc::CpmlParams p;
+    if (c.fields->cpml) p = c.fields->cpml->p;
+    for (int a = 0; a < 3; ++a) p.cells[a] = cpmlCells[a];
+    return p;
+  }
 };
 CouplingHandle* CH(void* h) { return static_cast<CouplingHandle*>(h); }
 orc::V3 vec(const double* p) { return orc::V3{{p[0], p[1], p[2]}}; }
@@ -195,6 +205,7 @@ int orc_coupling_set_particles(void* h, std::size_t n, const double* x3, const d
                                const double* m, const double* w, const std::uint64_t* id) {
   return guarded([&] {
     auto& P = CH(h)->particles;
+    CH(h)->carriers.clear();
     P.assign(n, orc::Particle());
     for (std::size_t i = 0; i < n; ++i) {
       P[i].position = vec(x3 + 3 * i);
@@ -222,11 +233,86 @@ int orc_coupling_step(void* h, const double* origin, double time, unsigned subst
     CouplingHandle& H = *CH(h);
     orc::Fields& f = *H.c.fields;
     orc::ParticleBoundary pb;
-    pb.set(f.g, vec(origin), H.cpmlCells);
+    pb.set(f.g, vec(origin), H.boundaryParams());
     pb.window = std::isfinite(exitLabZ);
     pb.exitLabZ = exitLabZ;
-    const orc::StepResult r = orc::sourceStep(f, H.c, H.particles, vec(origin), time, substeps, pb);
+    const orc::StepResult r =
+        orc::sourceStep(f, H.c, H.particles, H.carriers, H.counters, vec(origin), time, substeps, pb);
     *iterations = r.iterations; *force = r.force; *work = r.work;
+  });
+}
+
+int orc_coupling_get_ids(void* h, std::uint64_t* id) {
+  return guarded([&] {
+    const auto& P = CH(h)->particles;
+    for (std::size_t i = 0; i < P.size(); ++i) id[i] = P[i].id;
+  });
+}
+
+std::size_t orc_coupling_carrier_count(void* h) { return CH(h)->carriers.size(); }
+
+// Carriers in list order: position, velocity, charge, current weight and the
+// id of the particle each came from.
+int orc_coupling_get_carriers(void* h, double* x3, double* v3, double* q, double* w, std::uint64_t* id) {
+  return guarded([&] {
+    const auto& C = CH(h)->carriers;
+    for (std::size_t i = 0; i < C.size(); ++i) {
+      for (int a = 0; a < 3; ++a) {
+        x3[3 * i + a] = C[i].position[a];
+        v3[3 * i + a] = C[i].velocity[a];
+      }
+      q[i] = C[i].charge;
+      w[i] = C[i].weight;
+      id[i] = C[i].id;
+    }
+  });
+}
+
+// Cumulative boundary counters: counts[20] and charges[20] hold the CPML
+// entries per face (0-5), outer exits of particles (6-11) and of carriers
+// (12-17), moving-window exits (18) and cropped carriers (19).
+int orc_coupling_counters(void* h, std::uint64_t* counts, double* charges) {
+  return guarded([&] {
+    const auto& k = CH(h)->counters;
+    for (int f = 0; f < 6; ++f) {
+      counts[f] = k.cpml[f]; charges[f] = k.cpmlQ[f];
+      counts[6 + f] = k.direct[f]; charges[6 + f] = k.directQ[f];
+      counts[12 + f] = k.carrier[f]; charges[12 + f] = k.carrierQ[f];
+    }
+    counts[18] = k.exits; charges[18] = k.exitQ;
+    counts[19] = k.cropped; charges[19] = k.croppedQ;
+  });
+}
+
+// The particle part of finishMovingWindowStep after a shift to new_origin.
+int orc_coupling_finish_window(void* h, const double* newOrigin) {
+  return guarded([&] {
+    CouplingHandle& H = *CH(h);
+    orc::finishWindowStep(H.c.fields->g, vec(newOrigin), H.particles, H.carriers, H.counters);
+  });
+}
+
+// Unit-level boundary geometry of one grid origin: out6 = {hit, fraction,
+// face, position[3]} of ParticleCPMLRegion::firstEntry (kind 0),
+// ParticleOpenBoundary::firstExit (kind 1); kind 2 returns the damping
+// factor currentDampingFactor(a, b, duration) in out6[0].
+int orc_boundary_query(void* h, const double* origin, int kind, const double* a, const double* b, double duration,
+                       double* out6) {
+  return guarded([&] {
+    CouplingHandle& H = *CH(h);
+    orc::ParticleBoundary pb;
+    pb.set(H.c.fields->g, vec(origin), H.boundaryParams());
+    orc::Hit hit;
+    bool present = false;
+    if (kind == 2) {
+      out6[0] = pb.damping(vec(a), vec(b), duration, H.c.fields->g);
+      return;
+    }
+    present = kind == 0 ? pb.firstEntry(vec(a), vec(b), hit) : pb.firstExit(vec(a), vec(b), hit);
+    out6[0] = present ? 1.0 : 0.0;
+    out6[1] = hit.fraction;
+    out6[2] = hit.face;
+    for (int k = 0; k < 3; ++k) out6[3 + k] = hit.position[k];
   });
 }
 

@@ -732,92 +732,379 @@ class Coupling {
 };
 
 // Particle-boundary geometry of one step: the transverse CPML inner faces
-// (ParticleCPMLRegion, particle_boundary.cpp:56-174), the outer faces
-// (ParticleOpenBoundary::firstExit, 274-352) and the moving-window lab exit
+// (ParticleCPMLRegion, particle_boundary.cpp:56-241), the outer faces
+// (ParticleOpenBoundary::firstExit, 274-373) and the moving-window lab exit
 // plane (Simulation::firstMovingWindowEntry, simulation_moving_window.cpp:380-395).
+// Faces are numbered x-, x+, y-, y+, z-, z+ (ParticleBoundaryFace).
+struct Hit {
+  V3 position = v3();
+  double fraction = 1.0;
+  int face = 0;
+};
+
 struct ParticleBoundary {
   double physLower[3], physUpper[3], outerLower[3], outerUpper[3], tol[3];
   double exitLower[3], exitUpper[3], exitTol[3];  // ParticleOpenBoundary (fma upper face)
+  double maxRate[3] = {0, 0, 0}, order = 3.0;
   std::size_t cells[3];
   double exitLabZ = std::numeric_limits<double>::infinity();
   bool window = false;
 
-  void set(const Grid& g, const V3& origin, const std::size_t cpmlCells[3]) {
+  // ParticleCPMLRegion constructor (particle_boundary.cpp:56-84) and
+  // ParticleOpenBoundary constructor (254-287); params carries the CPML
+  // grading (polynomial order, target reflection) and the particle cells.
+  void set(const Grid& g, const V3& origin, const CpmlParams& params) {
     const double sp[3] = {g.dx, g.dy, g.dz};
     const std::size_t n[3] = {g.nx, g.ny, g.nz};
+    order = params.order;
     for (unsigned a = 0; a < 3; ++a) {
-      cells[a] = cpmlCells[a];
+      cells[a] = params.cells[a];
+      if (2 * cells[a] >= n[a] && cells[a] > 0)
+        throw std::invalid_argument("Particle CPML region must leave a physical interior");
       outerLower[a] = origin[a];
       outerUpper[a] = origin[a] + static_cast<double>(n[a]) * sp[a];
-      physLower[a] = outerLower[a] + static_cast<double>(cpmlCells[a]) * sp[a];
-      physUpper[a] = outerUpper[a] - static_cast<double>(cpmlCells[a]) * sp[a];
+      physLower[a] = outerLower[a] + static_cast<double>(cells[a]) * sp[a];
+      physUpper[a] = outerUpper[a] - static_cast<double>(cells[a]) * sp[a];
       tol[a] = 64.0 * kEpsilon * std::max(1.0, std::max(std::abs(outerLower[a]), std::abs(outerUpper[a])));
+      maxRate[a] = params.maxRate(a, g);
       exitLower[a] = origin[a];
       exitUpper[a] = std::fma(static_cast<double>(n[a]), sp[a], origin[a]);
       exitTol[a] = 64.0 * kEpsilon * std::max(1.0, std::max(std::abs(exitLower[a]), std::abs(exitUpper[a])));
     }
   }
-  // True when the segment triggers any boundary event (CPML entry, outer
-  // exit or the lab exit plane); throws like the reference on bad starts.
-  bool anyEvent(const V3& start, const V3& end, const Boost& frame, double tStart, double tEnd) const {
+  bool cpmlEnabled() const { return cells[0] > 0 || cells[1] > 0 || cells[2] > 0; }
+
+  // ParticleCPMLRegion::firstEntry, particle_boundary.cpp:104-174.
+  bool firstEntry(const V3& s, const V3& e, Hit& hit) const {
+    if (!cpmlEnabled()) return false;
+    double best = std::numeric_limits<double>::infinity();
+    int bestFace = 0;
     bool entered = false;
     for (unsigned a = 0; a < 3; ++a) {
-      if (!std::isfinite(start[a]) || !std::isfinite(end[a]))
+      if (!std::isfinite(s[a]) || !std::isfinite(e[a]))
         throw std::runtime_error("Particle CPML trajectory contains a non-finite coordinate");
-      if (start[a] < physLower[a] - tol[a] || start[a] > physUpper[a] + tol[a])
+      if (s[a] < physLower[a] - tol[a] || s[a] > physUpper[a] + tol[a])
         throw std::out_of_range("Active particle starts inside the non-physical CPML region");
       if (cells[a] == 0) continue;
-      if (end[a] < physLower[a] || end[a] > physUpper[a]) entered = true;
+      double candidate = 0.0;
+      int face = 0;
+      bool axisEntered = false;
+      if (e[a] < physLower[a]) {
+        candidate = (physLower[a] - s[a]) / (e[a] - s[a]);
+        face = static_cast<int>(2 * a);
+        axisEntered = true;
+      } else if (e[a] > physUpper[a]) {
+        candidate = (physUpper[a] - s[a]) / (e[a] - s[a]);
+        face = static_cast<int>(2 * a + 1);
+        axisEntered = true;
+      }
+      if (axisEntered && candidate < best) {
+        best = candidate;
+        bestFace = face;
+        entered = true;
+      }
     }
+    if (!entered) return false;
+    const double ft = 64.0 * kEpsilon;
+    if (!std::isfinite(best) || best < -ft || best > 1.0 + ft)
+      throw std::runtime_error("Cannot locate the first particle entry into CPML");
+    hit.fraction = std::max(0.0, std::min(1.0, best));
+    hit.face = bestFace;
     for (unsigned a = 0; a < 3; ++a) {
-      if (start[a] < exitLower[a] - exitTol[a] || start[a] > exitUpper[a] + exitTol[a])
+      hit.position[a] = s[a] + hit.fraction * (e[a] - s[a]);
+      hit.position[a] = std::max(outerLower[a], std::min(outerUpper[a], hit.position[a]));
+    }
+    const unsigned normal = static_cast<unsigned>(hit.face / 2);
+    hit.position[normal] = (hit.face % 2) != 0 ? physUpper[normal] : physLower[normal];
+    return true;
+  }
+
+  // Simulation::firstMovingWindowEntry, simulation_moving_window.cpp:380-395.
+  bool windowEntry(const V3& s, const V3& e, double tStart, double tEnd, const Boost& frame, Hit& hit) const {
+    bool present = firstEntry(s, e, hit);
+    if (!window) return present;
+    const double a = frame.labZFromBoxZT(s[2], tStart) - exitLabZ;
+    const double b = frame.labZFromBoxZT(e[2], tEnd) - exitLabZ;
+    if (a >= 0.) throw std::runtime_error("Active particle started beyond trusted moving-window exit");
+    if (b >= 0. && b > a) {
+      const double fraction = -a / (b - a);
+      if (!present || fraction < hit.fraction) {
+        for (unsigned d = 0; d < 3; ++d) hit.position[d] = s[d] + fraction * (e[d] - s[d]);
+        hit.fraction = fraction;
+        hit.face = 5;
+        present = true;
+      }
+    }
+    return present;
+  }
+
+  // ParticleOpenBoundary::firstExit, particle_boundary.cpp:303-373.
+  bool firstExit(const V3& s, const V3& e, Hit& hit) const {
+    double best = std::numeric_limits<double>::infinity();
+    int bestFace = 0;
+    bool escaped = false;
+    for (unsigned a = 0; a < 3; ++a) {
+      if (!std::isfinite(s[a]) || !std::isfinite(e[a]))
+        throw std::runtime_error("Particle trajectory contains a non-finite coordinate");
+      if (s[a] < exitLower[a] - exitTol[a] || s[a] > exitUpper[a] + exitTol[a])
         throw std::out_of_range("Particle trajectory starts outside the global E/B domain");
-      if (end[a] < exitLower[a] || end[a] > exitUpper[a]) entered = true;
+      double candidate = 0.0;
+      int face = 0;
+      bool axisEscaped = false;
+      if (e[a] < exitLower[a]) {
+        candidate = (exitLower[a] - s[a]) / (e[a] - s[a]);
+        face = static_cast<int>(2 * a);
+        axisEscaped = true;
+      } else if (e[a] > exitUpper[a]) {
+        candidate = (exitUpper[a] - s[a]) / (e[a] - s[a]);
+        face = static_cast<int>(2 * a + 1);
+        axisEscaped = true;
+      }
+      if (axisEscaped && candidate < best) {
+        best = candidate;
+        bestFace = face;
+        escaped = true;
+      }
     }
-    if (window) {
-      const double za = frame.labZFromBoxZT(start[2], tStart) - exitLabZ;
-      const double zb = frame.labZFromBoxZT(end[2], tEnd) - exitLabZ;
-      if (za >= 0.) throw std::runtime_error("Active particle started beyond trusted moving-window exit");
-      if (zb >= 0. && zb > za) entered = true;
+    if (!escaped) return false;
+    const double ft = 64.0 * kEpsilon;
+    if (best < -ft || best > 1.0 + ft || !std::isfinite(best))
+      throw std::runtime_error("Cannot locate the first particle-domain exit");
+    hit.fraction = std::max(0.0, std::min(1.0, best));
+    hit.face = bestFace;
+    for (unsigned a = 0; a < 3; ++a) {
+      hit.position[a] = s[a] + hit.fraction * (e[a] - s[a]);
+      hit.position[a] = std::max(exitLower[a], std::min(exitUpper[a], hit.position[a]));
     }
-    return entered;
+    const unsigned normal = static_cast<unsigned>(hit.face / 2);
+    hit.position[normal] = (hit.face % 2) != 0 ? exitUpper[normal] : exitLower[normal];
+    return true;
+  }
+
+  // ParticleCPMLRegion::conductivityRate, particle_boundary.cpp:216-241.
+  double conductivityRate(const V3& p, const Grid& g) const {
+    const double sp[3] = {g.dx, g.dy, g.dz};
+    double rate = 0.0;
+    for (unsigned a = 0; a < 3; ++a) {
+      if (cells[a] == 0) continue;
+      const double thickness = static_cast<double>(cells[a]) * sp[a];
+      double depth = 0.0;
+      if (p[a] < physLower[a]) depth = (physLower[a] - p[a]) / thickness;
+      else if (p[a] > physUpper[a]) depth = (p[a] - physUpper[a]) / thickness;
+      depth = std::max(0.0, std::min(1.0, depth));
+      rate += maxRate[a] * std::pow(depth, order);
+    }
+    return rate;
+  }
+
+  // ParticleCPMLRegion::currentDampingFactor, particle_boundary.cpp:176-205.
+  double damping(const V3& s, const V3& e, double duration, const Grid& g) const {
+    if (!(duration >= 0.0) || !std::isfinite(duration))
+      throw std::invalid_argument("CPML carrier segment duration must be finite and nonnegative");
+    if (duration == 0.0 || !cpmlEnabled()) return 1.0;
+    const double abscissa = std::sqrt(3.0 / 5.0);
+    const double node[3] = {0.5 * (1.0 - abscissa), 0.5, 0.5 * (1.0 + abscissa)};
+    const double weight[3] = {5.0 / 18.0, 4.0 / 9.0, 5.0 / 18.0};
+    double average = 0.0;
+    for (unsigned k = 0; k < 3; ++k) {
+      V3 p = v3();
+      for (unsigned a = 0; a < 3; ++a) p[a] = s[a] + node[k] * (e[a] - s[a]);
+      average += weight[k] * conductivityRate(p, g);
+    }
+    const double exponent = -duration * average;
+    return exponent < std::log(std::numeric_limits<double>::min()) ? 0.0 : std::exp(exponent);
   }
 };
 
-// Simulation::solveConservativeSourceStep + commitConservativeElectric
-// (simulation_source_coupling.cpp:16-65) with the particle loop of
-// pushDepositAndMigrate (simulation.cpp:4167-5605) for active particles that
-// trigger no boundary event; an event raises runtime_error (unsupported).
+// ParticleCPMLCarrier (particle_boundary.h:76-84): a particle that entered
+// the CPML (or passed the moving-window exit plane) moves on straight and
+// deposits damped current. id and u are bookkeeping only: the particle the
+// carrier came from and its proper velocity, from which velocity = u c/gamma.
+struct Carrier {
+  V3 position = v3(), velocity = v3(), u = v3();
+  double charge = 0, weight = 1;
+  std::uint64_t id = 0;
+};
+
+// Cumulative particle-boundary counters (simulation.h cpmlEntryCount_,
+// directOuterCount_, carrierOuterCount_, MovingWindowState exits and
+// croppedCarriers), with their charges.
+struct BoundaryCounters {
+  std::uint64_t cpml[6] = {0, 0, 0, 0, 0, 0}, direct[6] = {0, 0, 0, 0, 0, 0}, carrier[6] = {0, 0, 0, 0, 0, 0};
+  std::uint64_t exits = 0, cropped = 0;
+  double cpmlQ[6] = {0, 0, 0, 0, 0, 0}, directQ[6] = {0, 0, 0, 0, 0, 0}, carrierQ[6] = {0, 0, 0, 0, 0, 0};
+  double exitQ = 0, croppedQ = 0;
+};
+
 struct StepResult { unsigned iterations = 0; double force = 0, work = 0; };
 
-inline void pushDepositAll(Coupling& c, std::vector<Particle>& particles, double time, double dt, unsigned substeps,
+// Segment of one CPML carrier, processCarrierSegment (simulation.cpp:4310-
+// 4407) for one rank: the carrier moves from `start` to its end position,
+// is cut at the first outer-face exit, deposits its current with the
+// damping at the segment midpoint and keeps the end damping.
+inline void carrierSegment(Coupling& c, const ParticleBoundary& pb, Carrier carrier, const V3& start,
+                           double startFraction, std::vector<Carrier>& retained, BoundaryCounters& k) {
+  const Grid& g = c.fields->g;
+  if (!(startFraction >= 0.0 && startFraction <= 1.0) || !std::isfinite(startFraction))
+    throw std::runtime_error("Invalid CPML-carrier segment time fraction");
+  if (!(carrier.weight >= 0.0) || !std::isfinite(carrier.weight))
+    throw std::runtime_error("Invalid CPML carrier current weight");
+  const double cutoff = 64.0 * kEpsilon;
+  if (carrier.weight <= cutoff) return;
+  Hit outer;
+  const bool exits = pb.firstExit(start, carrier.position, outer);
+  double localEnd = 1.0;
+  V3 end = carrier.position;
+  if (exits) {
+    localEnd = outer.fraction;
+    end = outer.position;
+  }
+  const double globalEnd = startFraction + localEnd * (1.0 - startFraction);
+  const double duration = (globalEnd - startFraction) * g.dt;
+  V3 mid = v3();
+  for (unsigned a = 0; a < 3; ++a) mid[a] = 0.5 * (start[a] + end[a]);
+  const double midDamping = pb.damping(start, mid, 0.5 * duration, g);
+  const double endDamping = pb.damping(start, end, duration, g);
+  const double deposited = carrier.weight * midDamping;
+  const double endWeight = carrier.weight * endDamping;
+  c.validatePosition(start);
+  c.validatePosition(end);
+  c.deposit(start, end, carrier.charge * deposited);
+  if (exits) {
+    ++k.carrier[outer.face];
+    k.carrierQ[outer.face] += carrier.charge * endWeight;
+    return;
+  }
+  if (endWeight <= cutoff) return;
+  carrier.weight = endWeight;
+  carrier.position = end;
+  retained.push_back(carrier);
+}
+
+// pushDepositAndMigrate (simulation.cpp:4167-5605) for one rank: the carriers
+// first (4407-4425), then every active particle pushed and its segment ended
+// at the first boundary event (processActiveSegment, 4555-4917): CPML entry
+// or the moving-window exit plane (the particle becomes a carrier from the
+// hit), or an outer-face exit (the particle is removed). Retirement planes,
+// held particles and ballistic transverse motion are not part of this path.
+inline void pushDepositAll(Coupling& c, std::vector<Particle>& particles, std::vector<Carrier>& carriers,
+                           BoundaryCounters& k, double time, double dt, unsigned substeps,
                            const ParticleBoundary& pb, const V3& windowOrigin) {
+  std::vector<Carrier> retainedCarriers;
+  retainedCarriers.reserve(carriers.size());
+  for (const Carrier& original : carriers) {
+    Carrier carrier = original;
+    const V3 start = carrier.position;
+    pmv(carrier.position, dt, carrier.velocity);
+    carrierSegment(c, pb, carrier, start, 0.0, retainedCarriers, k);
+  }
   std::vector<Particle> prepared = particles;
   for (auto& p : prepared) {
     if (pb.window && p.position[2] < windowOrigin[2])
       throw std::runtime_error("An active particle is upstream of the moving window; the inlet guard did not contain its release event");
     c.push(p, time, dt, substeps);
   }
+  const double tolerance = 64.0 * kEpsilon;
+  std::vector<Particle> retained;
+  retained.reserve(prepared.size());
   for (std::size_t n = 0; n < particles.size(); ++n) {
     const V3 start = particles[n].position;
     const Particle& p = prepared[n];
-    if (pb.anyEvent(start, p.position, c.frame, time, time + dt))
-      throw std::runtime_error("Particle boundary event (CPML entry, domain exit or window exit) is not supported by this build");
+    Hit cpmlHit, outerHit;
+    const bool entersCpml = pb.window ? pb.windowEntry(start, p.position, time, time + dt, c.frame, cpmlHit)
+                                      : pb.firstEntry(start, p.position, cpmlHit);
+    const bool exitsOuter = pb.firstExit(start, p.position, outerHit);
+    enum { Retain, EnterCpml, ExitOuter } action = Retain;
+    double localEnd = 1.0;
+    V3 end = p.position;
+    Hit selected;
+    if (entersCpml && (!exitsOuter || cpmlHit.fraction <= outerHit.fraction + tolerance)) {
+      action = EnterCpml;
+      localEnd = cpmlHit.fraction;
+      end = cpmlHit.position;
+      selected = cpmlHit;
+    } else if (exitsOuter) {
+      action = ExitOuter;
+      localEnd = outerHit.fraction;
+      end = outerHit.position;
+      selected = outerHit;
+    }
     c.validatePosition(start);
-    c.validatePosition(p.position);
+    c.validatePosition(end);
     if (!std::isfinite(p.charge)) throw std::invalid_argument("Deposited particle charge must be finite");
-    c.deposit(start, p.position, p.charge);
+    c.deposit(start, end, p.charge);
+    if (action == EnterCpml) {
+      if (pb.window && selected.face == 5) {
+        ++k.exits;
+        k.exitQ += p.charge;
+      } else {
+        ++k.cpml[selected.face];
+        k.cpmlQ[selected.face] += p.charge;
+      }
+      Carrier carrier;
+      carrier.position = p.position;
+      const double gamma = gammaFromU(p.u);
+      carrier.velocity = p.u;
+      scaleBy(carrier.velocity, kC / gamma);
+      carrier.u = p.u;
+      carrier.charge = p.charge;
+      carrier.weight = 1.0;
+      carrier.id = p.id;
+      carrierSegment(c, pb, carrier, selected.position, localEnd, retainedCarriers, k);
+    } else if (action == ExitOuter) {
+      ++k.direct[selected.face];
+      k.directQ[selected.face] += p.charge;
+    } else {
+      retained.push_back(p);
+    }
   }
-  particles.swap(prepared);
+  particles.swap(retained);
+  carriers.swap(retainedCarriers);
 }
 
-inline StepResult sourceStep(Fields& f, Coupling& c, std::vector<Particle>& particles, const V3& origin, double time,
-                             unsigned substeps, const ParticleBoundary& pb) {
+// The carrier part of Simulation::finishMovingWindowStep after the shift
+// (simulation_moving_window.cpp:353-367): active particles must not lie
+// upstream of the new origin; carriers at or beyond the new upper face are
+// cropped and counted.
+inline void finishWindowStep(const Grid& g, const V3& newOrigin, std::vector<Particle>& particles,
+                             std::vector<Carrier>& carriers, BoundaryCounters& k) {
+  for (const auto& p : particles) {
+    if (p.position[2] < newOrigin[2]) throw std::runtime_error("Active particle fell behind moving inlet");
+    // migrateWindowObjects with ownerRank (simulation.cpp:4068-4083) on one
+    // rank: a particle beyond the upper face has no owner.
+    const double coordinate = (p.position[2] - newOrigin[2]) / g.dz;
+    const double tolerance = 64.0 * kEpsilon * std::max(1.0, std::abs(coordinate));
+    if (coordinate > static_cast<double>(g.nz) + tolerance)
+      throw std::runtime_error("Moving-window remap particle crossed a non-adjacent slab or outer boundary");
+  }
+  const double upper = std::fma(static_cast<double>(g.nz), g.dz, newOrigin[2]);
+  std::size_t retained = 0;
+  for (const auto& q : carriers) {
+    if (q.position[2] >= upper) {
+      ++k.cropped;
+      k.croppedQ += q.charge * q.weight;
+    } else if (q.position[2] < newOrigin[2]) {
+      throw std::runtime_error("Current carrier fell behind moving inlet");
+    } else {
+      carriers[retained++] = q;
+    }
+  }
+  carriers.resize(retained);
+}
+
+// Simulation::solveConservativeSourceStep + commitConservativeElectric
+// (simulation_source_coupling.cpp:16-65); the particles, the carriers and the
+// boundary counters are restored before every trial (restore(), 24-31).
+inline StepResult sourceStep(Fields& f, Coupling& c, std::vector<Particle>& particles, std::vector<Carrier>& carriers,
+                             BoundaryCounters& counters, const V3& origin, double time, unsigned substeps,
+                             const ParticleBoundary& pb) {
   c.begin(f, origin, particles);
   f.window->electricPreview = true;
   f.advanceElectric();
   f.window->electricPreview = false;
   const std::vector<Particle> original = particles;
+  const std::vector<Carrier> originalCarriers = carriers;
+  const BoundaryCounters originalCounters = counters;
   bool converged = false;
   StepResult r;
   const unsigned maximum = 64;
@@ -825,8 +1112,10 @@ inline StepResult sourceStep(Fields& f, Coupling& c, std::vector<Particle>& part
     if (n) c.applyCurrentSource(1.);
     f.clearCurrent();
     particles = original;
+    carriers = originalCarriers;
+    counters = originalCounters;
     c.beginTrial();
-    pushDepositAll(c, particles, time, f.g.dt, substeps, pb, origin);
+    pushDepositAll(c, particles, carriers, counters, time, f.g.dt, substeps, pb, origin);
     c.applyCurrentSource(-1.);
     const auto res = c.localResidual();
     r.force = res[0];
@@ -842,9 +1131,11 @@ inline StepResult sourceStep(Fields& f, Coupling& c, std::vector<Particle>& part
   c.applyCurrentSource(1.);
   f.clearCurrent();
   particles = original;
+  carriers = originalCarriers;
+  counters = originalCounters;
   c.swapElectric();
   c.replay = true;
-  pushDepositAll(c, particles, time, f.g.dt, substeps, pb, origin);
+  pushDepositAll(c, particles, carriers, counters, time, f.g.dt, substeps, pb, origin);
   c.replay = false;
   // commitConservativeElectric
   c.swapElectric();

@@ -75,6 +75,7 @@ struct RefFields {
   std::map<std::tuple<unsigned, std::ptrdiff_t, std::size_t>, std::vector<double>> staged;
   std::size_t planeX[6] = {0, 0, 0, 0, 0, 0};
   EBConvolutionalPML* cpml = nullptr;
+  EBCPMLParameters installed;  // grading of the installed CPML (particle damping)
   std::size_t step = 0;
 
   static Double reference(void* context, unsigned a, std::size_t i, std::size_t j, std::ptrdiff_t k, std::size_t n) {
@@ -149,6 +150,7 @@ int ref_fields_install_cpml(void* h, const std::size_t* cells, double order, dou
                                                                  params(cells, order, Rf, kappa, alpha), lowerZ != 0,
                                                                  upperZ != 0));
     R(h)->cpml = p.get();
+    R(h)->installed = params(cells, order, Rf, kappa, alpha);
     R(h)->f->setBoundary(std::move(p));
   });
 }
@@ -163,6 +165,7 @@ int ref_fields_install_window(void* h, const std::size_t* cells, double order, d
     r.window.reset(new MovingWindowFields(g, g.nz, 0, MPI_COMM_SELF, params(cells, order, Rf, kappa, alpha), ref));
     r.window->installBoundary(*r.f);
     r.cpml = dynamic_cast<EBConvolutionalPML*>(&r.f->boundary());
+    r.installed = params(cells, order, Rf, kappa, alpha);
   });
 }
 
@@ -255,6 +258,12 @@ struct RefCoupling {
   SIFieldSourceSet sources;
   BoostFrameTransform frame;
   std::vector<RelativisticParticleSI> particles;
+  std::vector<ParticleCPMLCarrier> carriers;
+  std::vector<std::uint64_t> carrierIds;  // bookkeeping: the particle each carrier came from
+  // cumulative counters: CPML entries (0-5), particle outer exits (6-11),
+  // carrier outer exits (12-17), window exits (18), cropped carriers (19)
+  unsigned long long counts[20] = {};
+  double charges[20] = {};
   std::string dir;
   bool quintic = false, binomial3 = false, smoothX = false;
   EBCPMLParameters cpml;
@@ -326,6 +335,8 @@ int ref_coupling_set_particles(void* h, std::size_t n, const double* x3, const d
       P[i].charge = q[i]; P[i].mass = m[i]; P[i].weight = w[i]; P[i].id = id[i];
       P[i].frozen = false;
     }
+    RC(h)->carriers.clear();
+    RC(h)->carrierIds.clear();
   });
 }
 std::size_t ref_coupling_count(void* h) { return RC(h)->particles.size(); }
@@ -345,7 +356,9 @@ int ref_coupling_step(void* h, const double* origin, double time, unsigned subst
     EBFieldGrid& f = *rf.f;
     const EBGridGeometry& g = f.geometry();
     const FieldVector<Double> o = fv(origin);
-    ParticleCPMLRegion cpmlRegion(g, o, H.cpml);
+    EBCPMLParameters boundaryParams = rf.installed;
+    for (int a = 0; a < 3; ++a) boundaryParams.cells[a] = H.cpml.cells[a];
+    ParticleCPMLRegion cpmlRegion(g, o, boundaryParams);
     ParticleOpenBoundary outer(g, o, g, o, 0);
     const bool window = std::isfinite(exitLabZ);
     auto pushDepositAll = [&]() {
@@ -375,23 +388,116 @@ int ref_coupling_step(void* h, const double* origin, double time, unsigned subst
           if (error) std::rethrow_exception(error);
         }
       }
+      // processCarrierSegment (simulation.cpp:4310-4407), one rank.
+      const Double cutoff = 64.0 * std::numeric_limits<Double>::epsilon();
+      std::vector<ParticleCPMLCarrier> keptCarriers;
+      std::vector<std::uint64_t> keptIds;
+      auto carrierSegment = [&](ParticleCPMLCarrier carrier, std::uint64_t id, const FieldVector<Double>& segStart,
+                                Double startFraction) {
+        if (carrier.currentWeight <= cutoff) return;
+        ParticleBoundaryHit outerHit;
+        const bool exits = outer.firstExit(segStart, carrier.position, outerHit);
+        Double localEnd = 1.0;
+        FieldVector<Double> segEnd(carrier.position);
+        if (exits) {
+          localEnd = outerHit.fraction;
+          segEnd = outerHit.position;
+        }
+        const Double globalEnd = startFraction + localEnd * (1.0 - startFraction);
+        const Double duration = (globalEnd - startFraction) * g.dt;
+        FieldVector<Double> mid(0.0);
+        for (unsigned a = 0; a < 3; ++a) mid[a] = 0.5 * (segStart[a] + segEnd[a]);
+        const Double midDamping = cpmlRegion.currentDampingFactor(segStart, mid, 0.5 * duration);
+        const Double endDamping = cpmlRegion.currentDampingFactor(segStart, segEnd, duration);
+        const Double endWeight = carrier.currentWeight * endDamping;
+        depositor.depositSegment(segStart, segEnd, carrier.charge * (carrier.currentWeight * midDamping));
+        if (exits) {
+          outer.depositOutgoingFlux(outerHit, carrier.charge * endWeight);
+          const std::size_t face = static_cast<std::size_t>(outerHit.face);
+          ++H.counts[12 + face];
+          H.charges[12 + face] += carrier.charge * endWeight;
+          return;
+        }
+        if (endWeight <= cutoff) return;
+        carrier.currentWeight = endWeight;
+        carrier.position = segEnd;
+        keptCarriers.push_back(carrier);
+        keptIds.push_back(id);
+      };
+      for (std::size_t n = 0; n < H.carriers.size(); ++n) {
+        ParticleCPMLCarrier carrier = H.carriers[n];
+        const FieldVector<Double> start(carrier.position);
+        carrier.position.pmv(g.dt, carrier.velocity);
+        carrierSegment(carrier, H.carrierIds[n], start, 0.0);
+      }
+      // processActiveSegment (simulation.cpp:4555-4917) for active particles.
+      const Double tolerance = 64.0 * std::numeric_limits<Double>::epsilon();
+      std::vector<RelativisticParticleSI> retained;
       for (std::size_t n = 0; n < H.particles.size(); ++n) {
         const FieldVector<Double> start(H.particles[n].position);
         const RelativisticParticleSI& p = prepared[n];
-        ParticleBoundaryHit hit;
-        bool event = cpmlRegion.firstEntry(start, p.position, hit);
-        event = outer.firstExit(start, p.position, hit) || event;
+        ParticleBoundaryHit cpmlHit, outerHit;
+        // firstMovingWindowEntry (simulation_moving_window.cpp:380-395)
+        bool entersCpml = cpmlRegion.firstEntry(start, p.position, cpmlHit);
         if (window) {
           const double a = H.frame.labZFromBoxZT(start[2], time) - exitLabZ;
           const double b = H.frame.labZFromBoxZT(p.position[2], time + g.dt) - exitLabZ;
           if (a >= 0.) throw std::runtime_error("Active particle started beyond trusted moving-window exit");
-          if (b >= 0. && b > a) event = true;
+          if (b >= 0. && b > a) {
+            const double fraction = -a / (b - a);
+            if (!entersCpml || fraction < cpmlHit.fraction) {
+              for (unsigned d = 0; d < 3; ++d) cpmlHit.position[d] = start[d] + fraction * (p.position[d] - start[d]);
+              cpmlHit.fraction = fraction;
+              cpmlHit.face = ParticleBoundaryFace::UpperZ;
+              entersCpml = true;
+            }
+          }
         }
-        if (event)
-          throw std::runtime_error("Particle boundary event (CPML entry, domain exit or window exit) is not supported by this build");
-        depositor.depositSegment(start, p.position, p.charge);
+        const bool exitsOuter = outer.firstExit(start, p.position, outerHit);
+        int action = 0;  // retain / enter CPML / exit outer
+        Double localEnd = 1.0;
+        FieldVector<Double> segEnd(p.position);
+        ParticleBoundaryHit selected;
+        if (entersCpml && (!exitsOuter || cpmlHit.fraction <= outerHit.fraction + tolerance)) {
+          action = 1;
+          localEnd = cpmlHit.fraction;
+          segEnd = cpmlHit.position;
+          selected = cpmlHit;
+        } else if (exitsOuter) {
+          action = 2;
+          localEnd = outerHit.fraction;
+          segEnd = outerHit.position;
+          selected = outerHit;
+        }
+        depositor.depositSegment(start, segEnd, p.charge);
+        const std::size_t face = static_cast<std::size_t>(selected.face);
+        if (action == 1) {
+          if (window && selected.face == ParticleBoundaryFace::UpperZ) {
+            ++H.counts[18];
+            H.charges[18] += p.charge;
+          } else {
+            ++H.counts[face];
+            H.charges[face] += p.charge;
+          }
+          ParticleCPMLCarrier carrier;
+          carrier.position = p.position;
+          const Double gamma = BoostFrameTransform::gammaFromProperVelocity(p.properVelocity);
+          carrier.velocity = p.properVelocity;
+          carrier.velocity *= SI::c / gamma;
+          carrier.charge = p.charge;
+          carrier.currentWeight = 1.0;
+          carrierSegment(carrier, p.id, selected.position, localEnd);
+        } else if (action == 2) {
+          outer.depositOutgoingFlux(selected, p.charge);
+          ++H.counts[6 + face];
+          H.charges[6 + face] += p.charge;
+        } else {
+          retained.push_back(p);
+        }
       }
-      H.particles.swap(prepared);
+      H.particles.swap(retained);
+      H.carriers.swap(keptCarriers);
+      H.carrierIds.swap(keptIds);
     };
     ConservativeSourceCoupling& coupling = *H.c;
     coupling.begin(f, o, H.particles, H.cpml.cells[0], H.cpml.cells[1]);
@@ -399,13 +505,26 @@ int ref_coupling_step(void* h, const double* origin, double time, unsigned subst
     f.advanceElectric();
     rf.window->previewElectricBoundary(false);
     const auto original = H.particles;
+    const auto originalCarriers = H.carriers;
+    const auto originalIds = H.carrierIds;
+    unsigned long long counts0[20];
+    double charges0[20];
+    std::copy(H.counts, H.counts + 20, counts0);
+    std::copy(H.charges, H.charges + 20, charges0);
+    auto restore = [&]() {
+      H.particles = original;
+      H.carriers = originalCarriers;
+      H.carrierIds = originalIds;
+      std::copy(counts0, counts0 + 20, H.counts);
+      std::copy(charges0, charges0 + 20, H.charges);
+    };
     bool converged = false;
     unsigned its = 0;
     Double frc = 0., wrk = 0.;
     for (unsigned n = 0; n < 64; ++n) {
       if (n) coupling.applyCurrentSource(1.);
       f.clearCurrent();
-      H.particles = original;
+      restore();
       coupling.beginTrial();
       coupling.exchangeCandidate();
       pushDepositAll();
@@ -422,7 +541,7 @@ int ref_coupling_step(void* h, const double* origin, double time, unsigned subst
     if (!converged) throw std::runtime_error("Conservative source coupling failed to converge");
     coupling.applyCurrentSource(1.);
     f.clearCurrent();
-    H.particles = original;
+    restore();
     coupling.swapElectric();
     coupling.setReplay(true);
     pushDepositAll();
@@ -434,6 +553,86 @@ int ref_coupling_step(void* h, const double* origin, double time, unsigned subst
     coupling.applyCurrentSource(-1.);
     rf.window->commitPreparedElectricBoundary(f);
     *iterations = its; *force = frc; *work = wrk;
+  });
+}
+
+int ref_coupling_get_ids(void* h, std::uint64_t* id) {
+  return guarded([&] {
+    const auto& P = RC(h)->particles;
+    for (std::size_t i = 0; i < P.size(); ++i) id[i] = P[i].id;
+  });
+}
+std::size_t ref_coupling_carrier_count(void* h) { return RC(h)->carriers.size(); }
+int ref_coupling_get_carriers(void* h, double* x3, double* v3, double* q, double* w, std::uint64_t* id) {
+  return guarded([&] {
+    const auto& C = RC(h)->carriers;
+    for (std::size_t i = 0; i < C.size(); ++i) {
+      for (int a = 0; a < 3; ++a) { x3[3 * i + a] = C[i].position[a]; v3[3 * i + a] = C[i].velocity[a]; }
+      q[i] = C[i].charge;
+      w[i] = C[i].currentWeight;
+      id[i] = RC(h)->carrierIds[i];
+    }
+  });
+}
+int ref_coupling_counters(void* h, unsigned long long* counts, double* charges) {
+  return guarded([&] {
+    std::copy(RC(h)->counts, RC(h)->counts + 20, counts);
+    std::copy(RC(h)->charges, RC(h)->charges + 20, charges);
+  });
+}
+// The particle part of finishMovingWindowStep (simulation_moving_window.cpp:353-367).
+int ref_coupling_finish_window(void* h, const double* newOrigin) {
+  return guarded([&] {
+    RefCoupling& H = *RC(h);
+    const EBGridGeometry& g = H.fields->f->geometry();
+    for (const auto& p : H.particles) {
+      if (p.position[2] < newOrigin[2] && !p.frozen) throw std::runtime_error("Active particle fell behind moving inlet");
+      // ownerRank (simulation.cpp:4068-4083) with one rank
+      const Double coordinate = (p.position[2] - newOrigin[2]) / g.dz;
+      const Double tolerance = 64.0 * std::numeric_limits<Double>::epsilon() * std::max(1.0, std::abs(coordinate));
+      if (coordinate > static_cast<Double>(g.nz) + tolerance)
+        throw std::runtime_error("Moving-window remap particle crossed a non-adjacent slab or outer boundary");
+    }
+    const double upper = std::fma(static_cast<double>(g.nz), g.dz, newOrigin[2]);
+    std::size_t retained = 0;
+    for (std::size_t n = 0; n < H.carriers.size(); ++n) {
+      const auto& p = H.carriers[n];
+      if (p.position[2] >= upper) {
+        ++H.counts[19];
+        H.charges[19] += p.charge * p.currentWeight;
+      } else if (p.position[2] < newOrigin[2]) {
+        throw std::runtime_error("Current carrier fell behind moving inlet");
+      } else {
+        H.carrierIds[retained] = H.carrierIds[n];
+        H.carriers[retained++] = p;
+      }
+    }
+    H.carriers.resize(retained);
+    H.carrierIds.resize(retained);
+  });
+}
+// ParticleCPMLRegion::firstEntry (kind 0), ParticleOpenBoundary::firstExit
+// (kind 1) and currentDampingFactor (kind 2) at one grid origin.
+int ref_boundary_query(void* h, const double* origin, int kind, const double* a, const double* b, double duration,
+                       double* out6) {
+  return guarded([&] {
+    RefCoupling& H = *RC(h);
+    const EBGridGeometry& g = H.fields->f->geometry();
+    const FieldVector<Double> o = fv(origin);
+    EBCPMLParameters p = H.fields->installed;
+    for (int k = 0; k < 3; ++k) p.cells[k] = H.cpml.cells[k];
+    ParticleCPMLRegion region(g, o, p);
+    ParticleOpenBoundary outer(g, o, g, o, 0);
+    if (kind == 2) {
+      out6[0] = region.currentDampingFactor(fv(a), fv(b), duration);
+      return;
+    }
+    ParticleBoundaryHit hit;
+    const bool present = kind == 0 ? region.firstEntry(fv(a), fv(b), hit) : outer.firstExit(fv(a), fv(b), hit);
+    out6[0] = present ? 1.0 : 0.0;
+    out6[1] = hit.fraction;
+    out6[2] = static_cast<double>(static_cast<int>(hit.face));
+    for (int k = 0; k < 3; ++k) out6[3 + k] = hit.position[k];
   });
 }
 

@@ -1,0 +1,99 @@
+"""Moving-window cases whose particles trigger particle-boundary events.
+
+Shared by the oracle tests and the GPU parity tests. A 24 x 24 x 40 window
+(dz = 1 um, dx = dy = 2.5 um, c dt = dz) with four CPML cells on x and y
+holds four groups of electrons: one driven into the x- CPML, one into the
+y+ CPML, one through the upper z face (or, with an exit plane, across the
+moving-window lab exit plane first) and one in the interior. The window
+origin follows the reference's fma(offset, dz, reference origin) and moves
+by -dz per step, so everything drifts towards the upper face; the interior
+group is slow enough to stay inside for the steps the tests run.
+"""
+import numpy as np
+
+from aprl_amd import bunch
+
+C_SI = 299792458.0
+DIMS = (24, 24, 40)
+DZ = 1.0e-6
+DX = 2.5 * DZ
+CELLS = (4, 4, 0)
+GAMMA_BOOST = 5.0
+OFFSET0 = 50
+REF_ORIGIN = np.array([0.0, 0.0, -OFFSET0 * DZ])
+
+
+def _fma(a, b, c):
+    """fma(a, b, c) for doubles, exact through integer arithmetic of the
+    binary significands (numpy has no fused multiply-add)."""
+    from fractions import Fraction
+    v = Fraction(a) * Fraction(b) + Fraction(c)
+    return float(v)
+
+
+def window_origin(offset):
+    o = REF_ORIGIN.copy()
+    o[2] = _fma(float(offset), DZ, float(REF_ORIGIN[2]))
+    return o
+
+
+def particles(seed=11, per_group=16, exit_plane=False):
+    """Positions, proper velocities, charges, masses, ids of the four groups.
+    With exit_plane the upper group starts below the plane of exit_lab_z()."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = DIMS
+    o = window_origin(OFFSET0)
+    lo = o + np.array([CELLS[0] * DX, CELLS[1] * DX, 0.0])
+    hi = o + np.array([(nx - CELLS[0]) * DX, (ny - CELLS[1]) * DX, nz * DZ])
+    mid = 0.5 * (lo + hi)
+    groups = []
+    # x- CPML entry
+    x = np.column_stack([rng.uniform(lo[0] + 0.05 * DX, lo[0] + 0.4 * DX, per_group),
+                         mid[1] + rng.uniform(-2, 2, per_group) * DX,
+                         mid[2] + rng.uniform(-5, 5, per_group) * DZ])
+    u = np.column_stack([rng.uniform(-7, -5, per_group), rng.uniform(-.5, .5, per_group),
+                         rng.uniform(1, 3, per_group)])
+    groups.append((x, u))
+    # y+ CPML entry
+    x = np.column_stack([mid[0] + rng.uniform(-2, 2, per_group) * DX,
+                         rng.uniform(hi[1] - 0.4 * DX, hi[1] - 0.05 * DX, per_group),
+                         mid[2] + rng.uniform(-5, 5, per_group) * DZ])
+    u = np.column_stack([rng.uniform(-.5, .5, per_group), rng.uniform(5, 7, per_group),
+                         rng.uniform(1, 3, per_group)])
+    groups.append((x, u))
+    # upper z face
+    x = np.column_stack([mid[0] + rng.uniform(-3, 3, per_group) * DX,
+                         mid[1] + rng.uniform(-3, 3, per_group) * DX,
+                         rng.uniform(hi[2] - (3.5 if exit_plane else 0.6) * DZ,
+                                     hi[2] - (2.2 if exit_plane else 0.1) * DZ, per_group)])
+    u = np.column_stack([rng.uniform(-.3, .3, per_group), rng.uniform(-.3, .3, per_group),
+                         rng.uniform(7, 9, per_group)])
+    groups.append((x, u))
+    # interior
+    x = np.column_stack([mid[0] + rng.uniform(-3, 3, per_group) * DX,
+                         mid[1] + rng.uniform(-3, 3, per_group) * DX,
+                         mid[2] + rng.uniform(-8, 8, per_group) * DZ])
+    u = np.column_stack([rng.normal(0, .05, per_group), rng.normal(0, .05, per_group),
+                         rng.uniform(0.5, 2, per_group)])
+    groups.append((x, u))
+    x = np.vstack([g[0] for g in groups])
+    u = np.vstack([g[1] for g in groups])
+    n = len(x)
+    macro = 2.0e5 / n
+    q = np.full(n, -bunch.QE * macro)
+    m = np.full(n, bunch.ME * macro)
+    ids = np.arange(1, n + 1, dtype=np.uint64)
+    return x, u, q, m, ids
+
+
+def exit_lab_z(boost_gamma=GAMMA_BOOST):
+    """Lab z of the box plane 2 dz below the upper face at t = 0 (frame
+    origins zero): boxToLab gives z_lab = gamma (z_box + beta c t)."""
+    z = window_origin(OFFSET0)[2] + DIMS[2] * DZ - 2.0 * DZ
+    return boost_gamma * z
+
+
+def planes_for(rng, comps):
+    from harness import component_shape
+    return [rng.standard_normal(int(np.prod(component_shape(*DIMS, a)[1:]))) * (10.0 if a < 3 else 10.0 / C_SI)
+            for a in comps]

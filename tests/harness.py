@@ -253,6 +253,13 @@ def _bind_particles(lib, prefix):
     f("current_deposit").argtypes = [vp, C.c_int, C.c_int, _dp, _dp, _dp, C.c_double]
     f("binomial_xz_magnetic").argtypes = [vp, _dp, _dp, _dp]
     f("magnetic_sample_open").argtypes = [vp, _dp, _dp, _dp]
+    f("coupling_get_ids").argtypes = [vp, _u64p]
+    f("coupling_carrier_count").argtypes = [vp]
+    f("coupling_carrier_count").restype = sz
+    f("coupling_get_carriers").argtypes = [vp, _dp, _dp, _dp, _dp, _u64p]
+    f("coupling_counters").argtypes = [vp, _u64p, _dp]
+    f("coupling_finish_window").argtypes = [vp, _dp]
+    f("boundary_query").argtypes = [vp, _dp, C.c_int, _dp, _dp, C.c_double, _dp]
 
 
 def load_particles(prefix):
@@ -337,6 +344,40 @@ class CpuCoupling:
         u = np.zeros((n, 3))
         _call(self.prefix, "coupling_get_particles", self.h, _ptr(x), _ptr(u))
         return x, u
+
+    def ids(self):
+        n = getattr(self.lib, self.prefix + "_coupling_count")(self.h)
+        ids = np.zeros(n, np.uint64)
+        _call(self.prefix, "coupling_get_ids", self.h, ids.ctypes.data_as(_u64p))
+        return ids
+
+    def carriers(self):
+        """(x, v, q, weight, id) of the CPML carriers in list order."""
+        n = getattr(self.lib, self.prefix + "_coupling_carrier_count")(self.h)
+        x, v = np.zeros((n, 3)), np.zeros((n, 3))
+        q, w = np.zeros(n), np.zeros(n)
+        ids = np.zeros(n, np.uint64)
+        _call(self.prefix, "coupling_get_carriers", self.h, _ptr(x), _ptr(v), _ptr(q), _ptr(w),
+              ids.ctypes.data_as(_u64p))
+        return x, v, q, w, ids
+
+    def counters(self):
+        """Cumulative counts[20] and charges[20]: CPML entries per face (0-5),
+        particle outer exits (6-11), carrier outer exits (12-17), window
+        exits (18) and cropped carriers (19)."""
+        counts = np.zeros(20, np.uint64)
+        charges = np.zeros(20)
+        _call(self.prefix, "coupling_counters", self.h, counts.ctypes.data_as(_u64p), _ptr(charges))
+        return counts, charges
+
+    def finish_window(self, new_origin):
+        _call(self.prefix, "coupling_finish_window", self.h, _ptr(_v(new_origin)))
+
+    def boundary_query(self, origin, kind, a, b, duration=0.0):
+        out = np.zeros(6)
+        _call(self.prefix, "boundary_query", self.h, _ptr(_v(origin)), kind, _ptr(_v(a)), _ptr(_v(b)), duration,
+              _ptr(out))
+        return out
 
     def step(self, origin, time, substeps=1, exit_lab_z=float("inf")):
         it = C.c_uint()

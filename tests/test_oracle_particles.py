@@ -276,15 +276,3 @@ def test_coupled_window_steps_bitwise(shape, smooth):
     xr, ur = cr.particles()
     assert np.array_equal(xo, xr) and np.array_equal(uo, ur)
     assert all(t[0] >= 2 for t in lo)
-
-
-def test_unsupported_boundary_event_raises():
-    dims = (20, 20, 40)
-    f, c = make_window_case("orc", dims, 1e-6, (4, 4, 0), 8, 3)
-    x, u = c.particles()
-    x[0, 0] = 4.0 * 2.5e-6 + 1e-9  # just inside the physical x face
-    u[0, 0] = -5.0                 # moving into the CPML
-    c.set_particles(x, u, np.full(8, -1e-15), np.full(8, 9e-31), np.ones(8), np.arange(1, 9))
-    with pytest.raises(H.CpuError) as e:
-        run_coupled(f, c, dims, 1, 1)
-    assert e.value.code == 3 and "not supported" in str(e.value)
