@@ -125,6 +125,8 @@ def lib() -> C.CDLL:
     L.aprl_particles_download.argtypes = [vp, _dp, _dp, _dp, _dp, _dp, u64p]
     L.aprl_particles_count.argtypes = [vp]
     L.aprl_particles_count.restype = _sz
+    L.aprl_particles_download_states.argtypes = [vp, C.POINTER(C.c_int)]
+    L.aprl_coupling_boundary_counters.argtypes = [vp, u64p, _dp]
     L.aprl_coupling_create.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp)]
     L.aprl_coupling_destroy.argtypes = [vp]
     L.aprl_coupling_destroy.restype = None
@@ -178,7 +180,8 @@ EXPORTED = [
     "aprl_window_install", "aprl_window_begin_step", "aprl_window_set_inlet", "aprl_window_shift",
     "aprl_profile_enable", "aprl_profile_query", "aprl_profile_reset",
     "aprl_particles_create", "aprl_particles_destroy", "aprl_particles_upload", "aprl_particles_download",
-    "aprl_particles_count", "aprl_coupling_create", "aprl_coupling_destroy", "aprl_coupling_set_frame",
+    "aprl_particles_count", "aprl_particles_download_states", "aprl_coupling_boundary_counters",
+    "aprl_coupling_create", "aprl_coupling_destroy", "aprl_coupling_set_frame",
     "aprl_coupling_add_magnet", "aprl_coupling_set_particle_boundary", "aprl_coupling_step",
     "aprl_coupling_set_split_trials", "aprl_window_step_batch",
     "aprl_fields_interior_energy", "aprl_fields_face_power", "aprl_coupling_continuity", "aprl_coupling_gauss",

@@ -66,6 +66,15 @@ class ParticleStore:
                                             ids.ctypes.data_as(C.POINTER(C.c_uint64)) if n else None))
         return x, u, q, m, w, ids
 
+    def states(self):
+        """Entry state per live entry in download order: 0 particle, 1 CPML
+        carrier (ParticleCPMLCarrier, particle_boundary.h:76-84)."""
+        n = len(self)
+        st = np.zeros(n, dtype=np.int32)
+        if n:
+            check(lib().aprl_particles_download_states(self.handle, st.ctypes.data_as(C.POINTER(C.c_int))))
+        return st
+
 
 class ConservativeSourceCoupling:
     def __init__(self, fields: EBFieldGrid, shape: str = "linear-binomial3", transverseXBinomial3: bool = True):
@@ -107,6 +116,16 @@ class ConservativeSourceCoupling:
     def setParticleBoundary(self, cpml_cells, exit_lab_z=math.inf):
         cells = (C.c_size_t * 3)(*cpml_cells)
         check(lib().aprl_coupling_set_particle_boundary(self.handle, cells, exit_lab_z))
+
+    def boundaryCounters(self):
+        """Cumulative (counts[20], charges[20]): CPML entries per face (0-5),
+        outer exits of particles (6-11) and of carriers (12-17), window exits
+        (18) and cropped carriers (19)."""
+        counts = np.zeros(20, dtype=np.uint64)
+        charges = np.zeros(20)
+        check(lib().aprl_coupling_boundary_counters(self.handle, counts.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                                    dptr(charges)))
+        return counts, charges
 
     def setSplitTrials(self, trials=2, verify=False):
         """FP32-split grids: run exactly ``trials`` Picard trials per step in

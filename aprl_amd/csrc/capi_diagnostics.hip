@@ -28,6 +28,19 @@ void splitPositions(SplitStore& P, bool start, const int* cell[3], const float* 
     off[a] = start ? P.end[a].get() : P.s[a].get();
   }
 }
+// Charge factors of a store at the start (mode 2) or end (mode 1) of its
+// last step: removed entries carry no charge and carriers their weight.
+template <class Store, class W>
+ChargeWeights weightsOf(Store& P, int mode, const W* w, const W* wStart) {
+  ChargeWeights c;
+  c.mode = mode;
+  c.state = P.state.get();
+  c.evt = P.evt.get();
+  const W* src = mode == 2 ? wStart : w;
+  if constexpr (sizeof(W) == sizeof(double)) c.wd = reinterpret_cast<const double*>(src);
+  else c.wf = reinterpret_cast<const float*>(src);
+  return c;
+}
 // Lattice cell of a grid origin of the split form.
 void latticeOrigin(const FieldState32& F, const double origin[3], int oc[3]) {
   const double h[3] = {F.g.dx, F.g.dy, F.g.dz};
@@ -75,9 +88,11 @@ aprl_status aprl_coupling_continuity(aprl_coupling* coupling, aprl_particles* pa
       const float* off[3];
       const int n = static_cast<int>(P.count);
       splitPositions(P, true, cell, off);
-      chargeDensitySplit(F, C.shape, cell, off, P.s[6].get(), n, oc, C.rho[0]);
+      chargeDensitySplit(F, C.shape, cell, off, P.s[6].get(), n, oc, weightsOf(P, 2, P.s[8].get(), P.wend.get()),
+                         C.rho[0]);
       splitPositions(P, false, cell, off);
-      chargeDensitySplit(F, C.shape, cell, off, P.s[6].get(), n, oc, C.rho[1]);
+      chargeDensitySplit(F, C.shape, cell, off, P.s[6].get(), n, oc, weightsOf(P, 1, P.s[8].get(), P.wend.get()),
+                         C.rho[1]);
       continuityResidual(F, C.rho[0], C.rho[1], out);
       return;
     }
@@ -87,8 +102,10 @@ aprl_status aprl_coupling_continuity(aprl_coupling* coupling, aprl_particles* pa
     FieldState& F = *C.fields;
     const double* x[3];
     const int n = static_cast<int>(P.count);
-    chargeDensity(F, C.shape, positions(P, true, x), P.s[6].get(), n, C.lastStep.origin, C.rho[0]);
-    chargeDensity(F, C.shape, positions(P, false, x), P.s[6].get(), n, C.lastStep.origin, C.rho[1]);
+    chargeDensity(F, C.shape, positions(P, true, x), P.s[6].get(), n, C.lastStep.origin,
+                  weightsOf(P, 2, P.s[8].get(), P.wend.get()), C.rho[0]);
+    chargeDensity(F, C.shape, positions(P, false, x), P.s[6].get(), n, C.lastStep.origin,
+                  weightsOf(P, 1, P.s[8].get(), P.wend.get()), C.rho[1]);
     continuityResidual(F, C.rho[0], C.rho[1], out);
   });
 }
@@ -107,14 +124,15 @@ aprl_status aprl_coupling_gauss(aprl_coupling* coupling, aprl_particles* particl
       const int* cell[3];
       const float* off[3];
       splitPositions(P, false, cell, off);
-      chargeDensitySplit(F, C.shape, cell, off, P.s[6].get(), static_cast<int>(P.count), oc, C.rho[0]);
+      chargeDensitySplit(F, C.shape, cell, off, P.s[6].get(), static_cast<int>(P.count), oc,
+                         weightsOf(P, 1, P.s[8].get(), P.wend.get()), C.rho[0]);
       gaussResidual(F, C.rho[0], lo, hi, out);
       return;
     }
     ParticleStore& P = particles->p;
     const double* x[3];
     chargeDensity(*C.fields, C.shape, positions(P, false, x), P.s[6].get(), static_cast<int>(P.count), origin,
-                  C.rho[0]);
+                  weightsOf(P, 1, P.s[8].get(), P.wend.get()), C.rho[0]);
     gaussResidual(*C.fields, C.rho[0], lo, hi, out);
   });
 }

@@ -74,7 +74,12 @@ void aprl_particles_destroy(aprl_particles* particles) {
 
 size_t aprl_particles_count(const aprl_particles* particles) {
   if (!particles) return 0;
-  return particles->fields->fp32 ? particles->sp.count : particles->p.count;
+  try {
+    return particles->fields->fp32 ? liveEntries(particles->sp.count, particles->sp.removed, particles->stream())
+                                   : liveEntries(particles->p.count, particles->p.removed, particles->stream());
+  } catch (...) {
+    return 0;
+  }
 }
 
 aprl_status aprl_particles_upload(aprl_particles* particles, size_t n, const double* x3, const double* u3,
@@ -82,57 +87,129 @@ aprl_status aprl_particles_upload(aprl_particles* particles, size_t n, const dou
   return guarded([&] {
     if (!particles) throw std::invalid_argument("Null particle store");
     if (n && (!x3 || !u3 || !q || !m || !w || !id)) throw std::invalid_argument("Null particle array");
-    if (particles->fields->fp32) {
-      hipCheck(hipStreamSynchronize(particles->stream()), "hipStreamSynchronize");
-      uploadSplitParticles(particles->sp, particles->fields->s32, n, x3, u3, q, m, w, id);
-      return;
-    }
-    ParticleStore& P = particles->p;
     hipCheck(hipStreamSynchronize(particles->stream()), "hipStreamSynchronize");
-    reserveParticles(P, n);
-    std::vector<double> col(n);
-    for (int a = 0; a < 3; ++a) {
-      for (size_t i = 0; i < n; ++i) col[i] = x3[3 * i + a];
-      hipCheck(hipMemcpy(P.s[a].get(), col.data(), n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
-      for (size_t i = 0; i < n; ++i) col[i] = u3[3 * i + a];
-      hipCheck(hipMemcpy(P.s[3 + a].get(), col.data(), n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+    int* state = nullptr;
+    unsigned* removed = nullptr;
+    if (particles->fields->fp32) {
+      uploadSplitParticles(particles->sp, particles->fields->s32, n, x3, u3, q, m, w, id);
+      state = particles->sp.state.get();
+      removed = particles->sp.removed.get();
+    } else {
+      ParticleStore& P = particles->p;
+      reserveParticles(P, n);
+      std::vector<double> col(n);
+      for (int a = 0; a < 3; ++a) {
+        for (size_t i = 0; i < n; ++i) col[i] = x3[3 * i + a];
+        hipCheck(hipMemcpy(P.s[a].get(), col.data(), n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+        for (size_t i = 0; i < n; ++i) col[i] = u3[3 * i + a];
+        hipCheck(hipMemcpy(P.s[3 + a].get(), col.data(), n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+      }
+      const double* scalars[3] = {q, m, w};
+      for (int a = 0; a < 3; ++a)
+        hipCheck(hipMemcpy(P.s[6 + a].get(), scalars[a], n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+      hipCheck(hipMemcpy(P.id.get(), id, n * sizeof(uint64_t), hipMemcpyHostToDevice), "hipMemcpy");
+      P.count = n;
+      ++P.generation;
+      state = P.state.get();
+      removed = P.removed.get();
     }
-    const double* scalars[3] = {q, m, w};
-    for (int a = 0; a < 3; ++a)
-      hipCheck(hipMemcpy(P.s[6 + a].get(), scalars[a], n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
-    hipCheck(hipMemcpy(P.id.get(), id, n * sizeof(uint64_t), hipMemcpyHostToDevice), "hipMemcpy");
-    P.count = n;
-    ++P.generation;
+    // every uploaded entry is an active particle
+    if (n) hipCheck(hipMemset(state, 0, n * sizeof(int)), "hipMemset");
+    hipCheck(hipMemset(removed, 0, sizeof(unsigned)), "hipMemset");
   });
 }
+
+namespace {
+// Entry states of the whole store (live and not yet compacted entries).
+std::vector<int> storeStates(const aprl_particles* particles, std::size_t n) {
+  std::vector<int> st(n);
+  const int* src = particles->fields->fp32 ? particles->sp.state.get() : particles->p.state.get();
+  if (n) hipCheck(hipMemcpy(st.data(), src, n * sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy");
+  return st;
+}
+
+void downloadAll(aprl_particles* particles, double* x3, double* u3, double* q, double* m, double* w, uint64_t* id) {
+  if (particles->fields->fp32) {
+    downloadSplitParticles(particles->sp, particles->fields->s32, x3, u3, q, m, w, id);
+    return;
+  }
+  ParticleStore& P = particles->p;
+  const size_t n = P.count;
+  std::vector<double> col(n);
+  for (int a = 0; a < 3; ++a) {
+    if (x3) {
+      hipCheck(hipMemcpy(col.data(), P.s[a].get(), n * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+      for (size_t i = 0; i < n; ++i) x3[3 * i + a] = col[i];
+    }
+    if (u3) {
+      hipCheck(hipMemcpy(col.data(), P.s[3 + a].get(), n * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+      for (size_t i = 0; i < n; ++i) u3[3 * i + a] = col[i];
+    }
+  }
+  double* scalars[3] = {q, m, w};
+  for (int a = 0; a < 3; ++a)
+    if (scalars[a])
+      hipCheck(hipMemcpy(scalars[a], P.s[6 + a].get(), n * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+  if (id) hipCheck(hipMemcpy(id, P.id.get(), n * sizeof(uint64_t), hipMemcpyDeviceToHost), "hipMemcpy");
+}
+}  // namespace
 
 aprl_status aprl_particles_download(aprl_particles* particles, double* x3, double* u3, double* q, double* m,
                                     double* w, uint64_t* id) {
   return guarded([&] {
     if (!particles) throw std::invalid_argument("Null particle store");
-    if (particles->fields->fp32) {
-      downloadSplitParticles(particles->sp, particles->fields->s32, x3, u3, q, m, w, id);
+    hipCheck(hipStreamSynchronize(particles->stream()), "hipStreamSynchronize");
+    const size_t n = particles->fields->fp32 ? particles->sp.count : particles->p.count;
+    const size_t live = aprl_particles_count(particles);
+    if (live == n) {
+      downloadAll(particles, x3, u3, q, m, w, id);
       return;
     }
-    ParticleStore& P = particles->p;
-    hipCheck(hipStreamSynchronize(particles->stream()), "hipStreamSynchronize");
-    const size_t n = P.count;
-    std::vector<double> col(n);
-    for (int a = 0; a < 3; ++a) {
-      if (x3) {
-        hipCheck(hipMemcpy(col.data(), P.s[a].get(), n * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
-        for (size_t i = 0; i < n; ++i) x3[3 * i + a] = col[i];
+    // entries removed by the last step are skipped (the next sort drops them)
+    std::vector<double> X(x3 ? 3 * n : 0), U(u3 ? 3 * n : 0), Q(q ? n : 0), M(m ? n : 0), W(w ? n : 0);
+    std::vector<uint64_t> I(id ? n : 0);
+    downloadAll(particles, x3 ? X.data() : nullptr, u3 ? U.data() : nullptr, q ? Q.data() : nullptr,
+                m ? M.data() : nullptr, w ? W.data() : nullptr, id ? I.data() : nullptr);
+    const std::vector<int> st = storeStates(particles, n);
+    size_t k = 0;
+    for (size_t i = 0; i < n; ++i) {
+      if (st[i] < 0) continue;
+      for (int a = 0; a < 3; ++a) {
+        if (x3) x3[3 * k + a] = X[3 * i + a];
+        if (u3) u3[3 * k + a] = U[3 * i + a];
       }
-      if (u3) {
-        hipCheck(hipMemcpy(col.data(), P.s[3 + a].get(), n * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
-        for (size_t i = 0; i < n; ++i) u3[3 * i + a] = col[i];
-      }
+      if (q) q[k] = Q[i];
+      if (m) m[k] = M[i];
+      if (w) w[k] = W[i];
+      if (id) id[k] = I[i];
+      ++k;
     }
-    double* scalars[3] = {q, m, w};
-    for (int a = 0; a < 3; ++a)
-      if (scalars[a])
-        hipCheck(hipMemcpy(scalars[a], P.s[6 + a].get(), n * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
-    if (id) hipCheck(hipMemcpy(id, P.id.get(), n * sizeof(uint64_t), hipMemcpyDeviceToHost), "hipMemcpy");
+  });
+}
+
+aprl_status aprl_particles_download_states(aprl_particles* particles, int* state) {
+  return guarded([&] {
+    if (!particles || !state) throw std::invalid_argument("Null particle store or state array");
+    hipCheck(hipStreamSynchronize(particles->stream()), "hipStreamSynchronize");
+    const size_t n = particles->fields->fp32 ? particles->sp.count : particles->p.count;
+    const std::vector<int> st = storeStates(particles, n);
+    size_t k = 0;
+    for (size_t i = 0; i < n; ++i)
+      if (st[i] >= 0) state[k++] = st[i];
+  });
+}
+
+aprl_status aprl_coupling_boundary_counters(aprl_coupling* coupling, uint64_t* counts, double* charges) {
+  return guarded([&] {
+    if (!coupling || !counts || !charges) throw std::invalid_argument("Null coupling or counter array");
+    CouplingState& C = coupling->c;
+    hipStream_t st = coupling->fields->fp32 ? coupling->fields->s32.stream : coupling->fields->s.stream;
+    reserveCouplingWords(C, st);
+    hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+    unsigned long long n[kCounterSlots];
+    hipCheck(hipMemcpy(n, C.counterN.get(), sizeof(n), hipMemcpyDeviceToHost), "hipMemcpy");
+    hipCheck(hipMemcpy(charges, C.counterQ.get(), kCounterSlots * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+    for (int k = 0; k < kCounterSlots; ++k) counts[k] = n[k];
   });
 }
 

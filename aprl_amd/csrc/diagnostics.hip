@@ -8,6 +8,7 @@
 #include <limits>
 
 #include "diagnostics.hpp"
+#include "particles.hpp"
 
 namespace aprl_gpu {
 
@@ -174,6 +175,7 @@ struct RhoArgs {
   int shift;
   double invVolume;
   unsigned long long* acc;  // (nx+1)(ny+1)(nz+1) vertices
+  ChargeWeights cw;
 };
 
 // longitudinal_shape::basis (longitudinal_shape.h:25-55) for the binomial3
@@ -217,6 +219,14 @@ __device__ int chargeBasis(int shape, double z, double v[12]) {
 __global__ void kChargeDensity(RhoArgs R) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= R.n) return;
+  double factor = 1.0;
+  if (R.cw.mode == 1) {
+    const int st = R.cw.state[p];
+    if (st < 0) return;
+    if (st == kStateCarrier) factor = R.cw.wd ? R.cw.wd[p] : static_cast<double>(R.cw.wf[p]);
+  } else if (R.cw.mode == 2 && R.cw.evt[p]) {
+    factor = R.cw.wd ? R.cw.wd[p] : static_cast<double>(R.cw.wf[p]);
+  }
   double l[3];
   for (int a = 0; a < 3; ++a)
     l[a] = R.x[0] ? (R.x[a][p] - R.o[a]) / R.h[a]
@@ -239,7 +249,7 @@ __global__ void kChargeDensity(RhoArgs R) {
     i0 = i;
   }
   const double wy[2] = {1. - fy, fy};
-  const double qv = (R.q ? R.q[p] : static_cast<double>(R.qf[p])) * R.invVolume;
+  const double qv = (R.q ? R.q[p] : static_cast<double>(R.qf[p])) * factor * R.invVolume;
   const int vx = R.nx + 1, vy = R.ny + 1, vz = R.nz + 1;
   for (int c = 0; c < nz; ++c) {
     const int kk = k0 + c;
@@ -444,8 +454,9 @@ void depositCharge(FieldStateT<Real>& s, RhoArgs R, const Q* q, DeviceArray<doub
 }
 
 void chargeDensity(FieldState& s, int shape, const double* const x[3], const double* q, int n, const double origin[3],
-                   DeviceArray<double>& rho) {
+                   const ChargeWeights& cw, DeviceArray<double>& rho) {
   RhoArgs R{};
+  R.cw = cw;
   for (int a = 0; a < 3; ++a) {
     R.x[a] = x[a];
     R.o[a] = origin[a];
@@ -457,8 +468,9 @@ void chargeDensity(FieldState& s, int shape, const double* const x[3], const dou
 }
 
 void chargeDensitySplit(FieldState32& s, int shape, const int* const cell[3], const float* const off[3],
-                        const float* q, int n, const int oc[3], DeviceArray<double>& rho) {
+                        const float* q, int n, const int oc[3], const ChargeWeights& cw, DeviceArray<double>& rho) {
   RhoArgs R{};
+  R.cw = cw;
   for (int a = 0; a < 3; ++a) {
     R.cell[a] = cell[a];
     R.off[a] = off[a];

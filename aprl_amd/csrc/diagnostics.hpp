@@ -14,13 +14,24 @@ double interiorEnergy(FieldStateT<R>& s, const std::size_t lo[3], const std::siz
 // power[2a] = -sum over the lower face, power[2a+1] = +sum over the upper.
 template <class R>
 void facePower(FieldStateT<R>& s, const std::size_t lo[3], const std::size_t hi[3], double power[6]);
+// Charge factor of the store entries: mode 0 all 1; mode 1 the current
+// entries (removed ones 0, carriers their current weight w); mode 2 the
+// entries at the start of the last step (event code evt: 0 means an
+// unchanged particle, otherwise w holds the start weight).
+struct ChargeWeights {
+  int mode = 0;
+  const int* state = nullptr;
+  const int* evt = nullptr;
+  const double* wd = nullptr;  // weights of the plain form, or
+  const float* wf = nullptr;   // of the split form
+};
 // Shaped vertex charge density (shape 0 quintic, 2 binomial3-x) of
 // particles at absolute positions x, or (split form) at lattice cell +
 // offset with the grid origin at lattice cell oc.
 void chargeDensity(FieldState& s, int shape, const double* const x[3], const double* q, int n, const double origin[3],
-                   DeviceArray<double>& rho);
+                   const ChargeWeights& cw, DeviceArray<double>& rho);
 void chargeDensitySplit(FieldState32& s, int shape, const int* const cell[3], const float* const off[3],
-                        const float* q, int n, const int oc[3], DeviceArray<double>& rho);
+                        const float* q, int n, const int oc[3], const ChargeWeights& cw, DeviceArray<double>& rho);
 // out = {max |(rho1 - rho0)/dt + div J|, max |(rho1 - rho0)/dt|}.
 template <class R>
 void continuityResidual(FieldStateT<R>& s, const DeviceArray<double>& rho0, const DeviceArray<double>& rho1,

@@ -16,16 +16,19 @@ struct StepArgs {
   MagnetDev mag[kMaxMagnets];
   // particle boundary geometry (ParticleCPMLRegion / ParticleOpenBoundary)
   double physLo[3], physHi[3], cpmlTol[3];
-  double outLo[3], outHi[3], outTol[3];
-  int cpmlCells[3];
+  double cpmlOuterLo[3], cpmlOuterHi[3];  // origin and origin + n h (CPML region)
+  double outLo[3], outHi[3], outTol[3];   // origin and fma(n, h, origin) (open boundary)
+  int cpmlCells[3], cpmlEnabled;
+  double cpmlRate[3], cpmlThick[3], cpmlOrder;  // conductivity grading of the carriers' damping
   int hasExit, window;
   double exitLabZ;
   double extent[3], posTol[3];  // validatePosition
 };
 
-// Per-step arguments of the push (time, magnets, boundary geometry).
-StepArgs stepArgsFor(const CouplingState& C, const aprl_grid_geometry& g, bool window, const double origin[3],
-                     double time, unsigned substeps);
+// Per-step arguments of the push (time, magnets, boundary geometry); cp is
+// the grid's CPML grading (null without a CPML).
+StepArgs stepArgsFor(const CouplingState& C, const aprl_grid_geometry& g, const aprl_cpml_parameters* cp,
+                     bool window, const double origin[3], double time, unsigned substeps);
 
 __device__ __forceinline__ double gammaFromU(double ux, double uy, double uz) {
   const double scale = fmax(1.0, fmax(fabs(ux), fmax(fabs(uy), fabs(uz))));
@@ -196,24 +199,387 @@ __device__ void pushChord(const StepArgs& S, const double o[3], const double x0[
     for (int a = 0; a < 3; ++a) w += q * usedE[a] * (x[a] - bx[a]);
     absGrid += fabs(w);
   }
-  // Boundary events of the deposition loop (simulation.cpp:5229-5245).
+  // Start checks of the deposition loop's boundary queries
+  // (particle_boundary.cpp:118-121, 325-330) and of validatePosition.
   for (int a = 0; a < 3; ++a) {
     if (!isfinite(x0[a]) || !isfinite(x[a])) err |= kErrNonfinite;
     if (x0[a] < S.physLo[a] - S.cpmlTol[a] || x0[a] > S.physHi[a] + S.cpmlTol[a]) err |= kErrStartInCpml;
-    if (S.cpmlCells[a] && (x[a] < S.physLo[a] || x[a] > S.physHi[a])) err |= kErrBoundaryEvent;
     if (x0[a] < S.outLo[a] - S.outTol[a] || x0[a] > S.outHi[a] + S.outTol[a]) err |= kErrStartOutside;
-    if (x[a] < S.outLo[a] || x[a] > S.outHi[a]) err |= kErrBoundaryEvent;
-    const double rel0 = x0[a] - o[a], rel1 = x[a] - o[a];
-    if (rel0 < -S.posTol[a] || rel0 > S.extent[a] + S.posTol[a] || rel1 < -S.posTol[a] ||
-        rel1 > S.extent[a] + S.posTol[a])
-      err |= kErrEndpointOutside;
+    const double rel0 = x0[a] - o[a];
+    if (rel0 < -S.posTol[a] || rel0 > S.extent[a] + S.posTol[a]) err |= kErrEndpointOutside;
   }
-  if (S.hasExit) {
-    const BoostDev& F = S.frame;
-    const double za = F.labZ0 + fma(F.gammaBeta * F.c0, S.time, F.gamma * (x0[2] - F.boxZ0)) - S.exitLabZ;
-    const double zb = F.labZ0 + fma(F.gammaBeta * F.c0, S.time + S.dt, F.gamma * (x[2] - F.boxZ0)) - S.exitLabZ;
-    if (za >= 0.) err |= kErrBeyondExit;
-    if (zb >= 0. && zb > za) err |= kErrBoundaryEvent;
+}
+
+// ---------------------------------------------------------------------------
+// Particle-boundary events (the deposit loop of pushDepositAndMigrate,
+// simulation.cpp:4310-4917, for one rank).
+
+// Event code of one store entry for one trial (0: an active particle whose
+// segment ends at its push end point).
+constexpr int kEvtKindMask = 3;           // active segment end: 1 outer exit, 2 CPML entry, 3 window exit
+constexpr int kEvtFaceShift = 2;          // face of that hit (3 bits)
+constexpr int kEvtCarrier = 1 << 5;       // a carrier segment follows (always set for a carrier entry)
+constexpr int kEvtCarrierExit = 1 << 6;   // the carrier segment ends at an outer face
+constexpr int kEvtCarrierFaceShift = 7;   // face of that exit (3 bits)
+constexpr int kEvtCarrierDrop = 1 << 10;  // the carrier's end weight is at or below the cutoff
+
+struct BoundaryHit {
+  double pos[3];
+  double fraction;
+  int face;
+};
+
+__device__ __forceinline__ double labZOf(const StepArgs& S, double z, double t) {
+  const BoostDev& F = S.frame;
+  return F.labZ0 + fma(F.gammaBeta * F.c0, t, F.gamma * (z - F.boxZ0));
+}
+
+// ChargeConservingCurrentDepositor::validatePosition, eb_deposition.cpp:228-247.
+__device__ __forceinline__ void validateEnd(const StepArgs& S, const double o[3], const double x[3], unsigned& err) {
+  for (int a = 0; a < 3; ++a) {
+    const double rel = x[a] - o[a];
+    if (!isfinite(x[a]) || rel < -S.posTol[a] || rel > S.extent[a] + S.posTol[a]) err |= kErrEndpointOutside;
+  }
+}
+
+// ParticleCPMLRegion::firstEntry, particle_boundary.cpp:104-174 (start
+// checks in pushChord).
+__device__ bool cpmlFirstEntry(const StepArgs& S, const double s[3], const double e[3], BoundaryHit& h,
+                               unsigned& err) {
+  if (!S.cpmlEnabled) return false;
+  double best = INFINITY;
+  int bestFace = 0;
+  bool entered = false;
+  for (int a = 0; a < 3; ++a) {
+    if (!S.cpmlCells[a]) continue;
+    double candidate = 0.0;
+    int face = 0;
+    bool axisEntered = false;
+    if (e[a] < S.physLo[a]) {
+      candidate = (S.physLo[a] - s[a]) / (e[a] - s[a]);
+      face = 2 * a;
+      axisEntered = true;
+    } else if (e[a] > S.physHi[a]) {
+      candidate = (S.physHi[a] - s[a]) / (e[a] - s[a]);
+      face = 2 * a + 1;
+      axisEntered = true;
+    }
+    if (axisEntered && candidate < best) {
+      best = candidate;
+      bestFace = face;
+      entered = true;
+    }
+  }
+  if (!entered) return false;
+  const double ft = 64.0 * 2.220446049250313e-16;
+  if (!isfinite(best) || best < -ft || best > 1.0 + ft) {
+    err |= kErrCpmlLocate;
+    return false;
+  }
+  h.fraction = fmax(0.0, fmin(1.0, best));
+  h.face = bestFace;
+  for (int a = 0; a < 3; ++a) {
+    h.pos[a] = s[a] + h.fraction * (e[a] - s[a]);
+    h.pos[a] = fmax(S.cpmlOuterLo[a], fmin(S.cpmlOuterHi[a], h.pos[a]));
+  }
+  const int normal = bestFace / 2;
+  h.pos[normal] = (bestFace % 2) ? S.physHi[normal] : S.physLo[normal];
+  return true;
+}
+
+// Simulation::firstMovingWindowEntry, simulation_moving_window.cpp:380-395
+// (the lab exit plane only when one is set).
+__device__ bool windowFirstEntry(const StepArgs& S, const double s[3], const double e[3], BoundaryHit& h,
+                                 unsigned& err) {
+  bool present = cpmlFirstEntry(S, s, e, h, err);
+  if (!S.hasExit) return present;
+  const double a = labZOf(S, s[2], S.time) - S.exitLabZ;
+  const double b = labZOf(S, e[2], S.time + S.dt) - S.exitLabZ;
+  if (a >= 0.) err |= kErrBeyondExit;
+  if (b >= 0. && b > a) {
+    const double fraction = -a / (b - a);
+    if (!present || fraction < h.fraction) {
+      for (int d = 0; d < 3; ++d) h.pos[d] = s[d] + fraction * (e[d] - s[d]);
+      h.fraction = fraction;
+      h.face = 5;
+      present = true;
+    }
+  }
+  return present;
+}
+
+// ParticleOpenBoundary::firstExit, particle_boundary.cpp:303-373.
+__device__ bool outerFirstExit(const StepArgs& S, const double s[3], const double e[3], BoundaryHit& h,
+                               unsigned& err) {
+  double best = INFINITY;
+  int bestFace = 0;
+  bool escaped = false;
+  for (int a = 0; a < 3; ++a) {
+    if (!isfinite(s[a]) || !isfinite(e[a])) err |= kErrNonfinite;
+    if (s[a] < S.outLo[a] - S.outTol[a] || s[a] > S.outHi[a] + S.outTol[a]) err |= kErrStartOutside;
+    double candidate = 0.0;
+    int face = 0;
+    bool axisEscaped = false;
+    if (e[a] < S.outLo[a]) {
+      candidate = (S.outLo[a] - s[a]) / (e[a] - s[a]);
+      face = 2 * a;
+      axisEscaped = true;
+    } else if (e[a] > S.outHi[a]) {
+      candidate = (S.outHi[a] - s[a]) / (e[a] - s[a]);
+      face = 2 * a + 1;
+      axisEscaped = true;
+    }
+    if (axisEscaped && candidate < best) {
+      best = candidate;
+      bestFace = face;
+      escaped = true;
+    }
+  }
+  if (!escaped) return false;
+  const double ft = 64.0 * 2.220446049250313e-16;
+  if (best < -ft || best > 1.0 + ft || !isfinite(best)) {
+    err |= kErrExitLocate;
+    return false;
+  }
+  h.fraction = fmax(0.0, fmin(1.0, best));
+  h.face = bestFace;
+  for (int a = 0; a < 3; ++a) {
+    h.pos[a] = s[a] + h.fraction * (e[a] - s[a]);
+    h.pos[a] = fmax(S.outLo[a], fmin(S.outHi[a], h.pos[a]));
+  }
+  const int normal = bestFace / 2;
+  h.pos[normal] = (bestFace % 2) ? S.outHi[normal] : S.outLo[normal];
+  return true;
+}
+
+// ParticleCPMLRegion::conductivityRate, particle_boundary.cpp:216-241.
+__device__ double conductivityRate(const StepArgs& S, const double p[3]) {
+  double rate = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    if (!S.cpmlCells[a]) continue;
+    double depth = 0.0;
+    if (p[a] < S.physLo[a]) depth = (S.physLo[a] - p[a]) / S.cpmlThick[a];
+    else if (p[a] > S.physHi[a]) depth = (p[a] - S.physHi[a]) / S.cpmlThick[a];
+    depth = fmax(0.0, fmin(1.0, depth));
+    rate += S.cpmlRate[a] * pow(depth, S.cpmlOrder);
+  }
+  return rate;
+}
+
+// ParticleCPMLRegion::currentDampingFactor, particle_boundary.cpp:176-205:
+// exp(-duration * mean rate) with the three-point Gauss-Legendre mean.
+__device__ double dampingFactor(const StepArgs& S, const double s[3], const double e[3], double duration) {
+  if (duration == 0.0 || !S.cpmlEnabled) return 1.0;
+  const double abscissa = sqrt(3.0 / 5.0);
+  const double node[3] = {0.5 * (1.0 - abscissa), 0.5, 0.5 * (1.0 + abscissa)};
+  const double weight[3] = {5.0 / 18.0, 4.0 / 9.0, 5.0 / 18.0};
+  double average = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    double p[3];
+    for (int a = 0; a < 3; ++a) p[a] = s[a] + node[k] * (e[a] - s[a]);
+    average += weight[k] * conductivityRate(S, p);
+  }
+  const double exponent = -duration * average;
+  return exponent < log(2.2250738585072014e-308) ? 0.0 : exp(exponent);
+}
+
+// processCarrierSegment, simulation.cpp:4310-4407: the carrier moves from
+// start (time fraction startFraction) to pos, is cut at its first outer
+// exit, deposits with the midpoint damping and keeps the end damping.
+// Returns the code bits of the carrier part; tEnd is its end as a fraction
+// of the step.
+__device__ int carrierSegment(const StepArgs& S, const double o[3], const double start[3], double startFraction,
+                              const double pos[3], double weight, double& tEnd, double& wDeposit, double& wEnd,
+                              unsigned& err) {
+  BoundaryHit oh;
+  const bool exits = outerFirstExit(S, start, pos, oh, err);
+  double localEnd = 1.0;
+  double end[3] = {pos[0], pos[1], pos[2]};
+  if (exits) {
+    localEnd = oh.fraction;
+    for (int a = 0; a < 3; ++a) end[a] = oh.pos[a];
+  }
+  const double globalEnd = startFraction + localEnd * (1.0 - startFraction);
+  const double duration = (globalEnd - startFraction) * S.dt;
+  double mid[3];
+  for (int a = 0; a < 3; ++a) mid[a] = 0.5 * (start[a] + end[a]);
+  wDeposit = weight * dampingFactor(S, start, mid, 0.5 * duration);
+  wEnd = weight * dampingFactor(S, start, end, duration);
+  validateEnd(S, o, start, err);
+  validateEnd(S, o, end, err);
+  tEnd = globalEnd;
+  int code = kEvtCarrier;
+  if (exits) code |= kEvtCarrierExit | (oh.face << kEvtCarrierFaceShift);
+  else if (wEnd <= 64.0 * 2.220446049250313e-16) code |= kEvtCarrierDrop;
+  return code;
+}
+
+// End action of an active particle's segment x0 -> x1 (processActiveSegment,
+// simulation.cpp:4787-4836 without retirement planes or rank interfaces):
+// a CPML entry or window exit at or before the first outer exit (within
+// 64 eps) hands the rest of the chord to a new carrier of weight 1; an outer
+// exit removes the particle. t1 / t2: chord fractions ending the unit-weight
+// part and the carrier part.
+__device__ int activeSegmentEvents(const StepArgs& S, const double o[3], const double x0[3], const double x1[3],
+                                   double& t1, double& t2, double& wDeposit, double& wEnd, unsigned& err) {
+  BoundaryHit ch, oh;
+  const bool entersCpml = S.window ? windowFirstEntry(S, x0, x1, ch, err) : cpmlFirstEntry(S, x0, x1, ch, err);
+  const bool exitsOuter = outerFirstExit(S, x0, x1, oh, err);
+  const double tolerance = 64.0 * 2.220446049250313e-16;
+  t1 = t2 = 1.0;
+  wDeposit = wEnd = 0.0;
+  if (entersCpml && (!exitsOuter || ch.fraction <= oh.fraction + tolerance)) {
+    validateEnd(S, o, ch.pos, err);
+    t1 = ch.fraction;
+    const int kind = S.window && ch.face == 5 ? 3 : 2;
+    return kind | (ch.face << kEvtFaceShift) | carrierSegment(S, o, ch.pos, t1, x1, 1.0, t2, wDeposit, wEnd, err);
+  }
+  if (exitsOuter) {
+    validateEnd(S, o, oh.pos, err);
+    t1 = t2 = oh.fraction;
+    return 1 | (oh.face << kEvtFaceShift);
+  }
+  validateEnd(S, o, x1, err);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Entry classification of the per-step sort: removed entries and, in a
+// moving window, the particle part of finishMovingWindowStep applied to the
+// new origin (simulation_moving_window.cpp:353-367): carriers at or beyond
+// the upper face are cropped, an active particle beyond it or a carrier
+// below the origin is an error. Returns false for an entry keyed behind the
+// live ones.
+struct SortArgs {
+  int window, nz;
+  double lowZ, upZ, dz;    // window origin z, fma(nz, dz, origin z)
+  unsigned* words;         // coupling error words: [0] errors, [2] keyed behind, [3] cropped
+};
+
+__device__ __forceinline__ bool liveEntry(const SortArgs& K, int* state, int p, double z) {
+  const int st = state[p];
+  if (st < 0) {
+    atomicAdd(K.words + 2, 1u);
+    return false;
+  }
+  if (!K.window) return true;
+  if (st == kStateCarrier) {
+    if (z >= K.upZ) {
+      state[p] = kStateCropped;
+      atomicAdd(K.words + 2, 1u);
+      atomicAdd(K.words + 3, 1u);
+      return false;
+    }
+    if (z < K.lowZ) atomicOr(K.words, static_cast<unsigned>(kErrCarrierBehind));
+    return true;
+  }
+  if (z < K.lowZ) atomicOr(K.words, static_cast<unsigned>(kErrBehindInlet));
+  // ownerRank (simulation.cpp:4068-4083) of one rank
+  const double coordinate = (z - K.lowZ) / K.dz;
+  const double tolerance = 64.0 * 2.220446049250313e-16 * fmax(1.0, fabs(coordinate));
+  if (coordinate > static_cast<double>(K.nz) + tolerance) atomicOr(K.words, static_cast<unsigned>(kErrRemapOutside));
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Commit of the converged trial's events and the counter reductions. Each
+// block writes its per-slot charge sums and counts in entry order; the
+// totals kernel adds the blocks in block order, so the sums are
+// reproducible.
+
+constexpr int kEventBlock = 256;
+
+template <typename R>
+__global__ void __launch_bounds__(kEventBlock) kCommitEvents(int n, const int* evt, int* state, R* w, const R* q,
+                                                             R* wend, unsigned* removed, const unsigned* trialWord,
+                                                             double* blockQ, unsigned* blockN) {
+  __shared__ double sq[2][kEventBlock];
+  __shared__ int ss[2][kEventBlock];
+  if (!(*trialWord & kEvtAny)) return;
+  const int t = static_cast<int>(threadIdx.x);
+  const int p = blockIdx.x * kEventBlock + t;
+  int s0 = -1, s1 = -1;
+  double q0 = 0.0, q1 = 0.0;
+  if (p < n) {
+    const int e = evt[p];
+    if (e) {
+      const int kind = e & kEvtKindMask, face = (e >> kEvtFaceShift) & 7;
+      const double charge = static_cast<double>(q[p]);
+      bool dead = false;
+      if (kind == 1) {
+        s0 = 6 + face;
+        q0 = charge;
+        dead = true;
+      } else if (kind == 2) {
+        s0 = face;
+        q0 = charge;
+      } else if (kind == 3) {
+        s0 = 18;
+        q0 = charge;
+      }
+      const R startWeight = w[p];
+      if (e & kEvtCarrier) {
+        const R we = wend[p];
+        if (e & kEvtCarrierExit) {
+          s1 = 12 + ((e >> kEvtCarrierFaceShift) & 7);
+          q1 = charge * static_cast<double>(we);
+          dead = true;
+        } else if (e & kEvtCarrierDrop) {
+          dead = true;
+        } else {
+          state[p] = kStateCarrier;
+          w[p] = we;
+        }
+      }
+      // keep the weight the entry started the step with (1 for a particle)
+      wend[p] = kind ? R(1) : startWeight;
+      if (dead) {
+        state[p] = kStateRemoved;
+        atomicAdd(removed, 1u);
+      }
+    }
+  }
+  sq[0][t] = q0;
+  ss[0][t] = s0;
+  sq[1][t] = q1;
+  ss[1][t] = s1;
+  __syncthreads();
+  if (t < kCounterSlots) {
+    double acc = 0.0;
+    unsigned cnt = 0;
+    const int k = t >= 12 && t < 18 ? 1 : 0;
+    for (int i = 0; i < kEventBlock; ++i)
+      if (ss[k][i] == t) {
+        acc += sq[k][i];
+        ++cnt;
+      }
+    blockQ[blockIdx.x * kCounterSlots + t] = acc;
+    blockN[blockIdx.x * kCounterSlots + t] = cnt;
+  }
+}
+
+// Cropped carriers (state kStateCropped) of the sorted tail [lo, n).
+template <typename R>
+__global__ void __launch_bounds__(kEventBlock) kCropCounters(int lo, int n, const int* state, const R* q, const R* w,
+                                                             double* blockQ, unsigned* blockN) {
+  __shared__ double sq[kEventBlock];
+  __shared__ int ss[kEventBlock];
+  const int t = static_cast<int>(threadIdx.x);
+  const int p = lo + blockIdx.x * kEventBlock + t;
+  const bool cropped = p < n && state[p] == kStateCropped;
+  sq[t] = cropped ? static_cast<double>(q[p]) * static_cast<double>(w[p]) : 0.0;
+  ss[t] = cropped ? 19 : -1;
+  __syncthreads();
+  if (t < kCounterSlots) {
+    double acc = 0.0;
+    unsigned cnt = 0;
+    for (int i = 0; i < kEventBlock; ++i)
+      if (ss[i] == t) {
+        acc += sq[i];
+        ++cnt;
+      }
+    blockQ[blockIdx.x * kCounterSlots + t] = acc;
+    blockN[blockIdx.x * kCounterSlots + t] = cnt;
   }
 }
 

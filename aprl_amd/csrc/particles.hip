@@ -46,7 +46,12 @@ struct PArgs {
   double* x1[3];
   double* u1[3];
   double* gridAbs;
-  double scaleDen;  // dx*dy*dz*dt
+  const int* state;  // entry states (kStateActive / kStateCarrier for live entries)
+  int* evt;          // boundary event code of the trial
+  double* cut[3];    // t1, t2, carrier-part deposit weight
+  double* wend;      // carrier end weight
+  const double* w;   // weight slot (a carrier's current weight)
+  double scaleDen;   // dx*dy*dz*dt
   int n;
 };
 
@@ -273,16 +278,20 @@ __device__ __forceinline__ int clampCell(double lattice, int n) {
   return c < 0 ? 0 : (c > n - 1 ? n - 1 : c);
 }
 
-__global__ void kTileKeys(const double* x, const double* y, const double* z, int n, GArgs G, int ntx, int nty,
-                          unsigned* key, unsigned* perm, TileRangeArgs R) {
+__global__ void kTileKeys(const double* x, const double* y, const double* z, int* state, int n, GArgs G, int ntx,
+                          int nty, unsigned* key, unsigned* perm, TileRangeArgs R, SortArgs K) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   clearTileRanges(R, p);
   if (p >= n) return;
+  perm[p] = p;
+  if (!liveEntry(K, state, p, z[p])) {
+    key[p] = static_cast<unsigned>(R.ntiles);
+    return;
+  }
   const int cx = clampCell((x[p] - G.o[0]) / G.h[0], G.nx);
   const int cy = clampCell((y[p] - G.o[1]) / G.h[1], G.ny);
   const int cz = clampCell((z[p] - G.o[2]) / G.h[2], G.nz);
   key[p] = (static_cast<unsigned>(cz / kTileZ) * nty + cy / kTileY) * ntx + cx / kTileX;
-  perm[p] = p;
 }
 
 struct PermuteArgs {
@@ -290,6 +299,8 @@ struct PermuteArgs {
   double* dst[9];
   const uint64_t* idSrc;
   uint64_t* idDst;
+  const int* stateSrc;
+  int* stateDst;
 };
 
 __global__ void kPermute(PermuteArgs A, const unsigned* perm, int n) {
@@ -299,15 +310,17 @@ __global__ void kPermute(PermuteArgs A, const unsigned* perm, int n) {
 #pragma unroll
   for (int a = 0; a < 9; ++a) A.dst[a][p] = A.src[a][q];
   A.idDst[p] = A.idSrc[q];
+  A.stateDst[p] = A.stateSrc[q];
 }
 
 // Particle range of every occupied tile from the run boundaries of the
 // sorted keys (the ranges were cleared to empty before the sort), and the
 // occupied-tile count.
-__global__ void kTileBounds(const unsigned* key, int n, int* begin, int* end, unsigned* occupied) {
+__global__ void kTileBounds(const unsigned* key, int n, int ntiles, int* begin, int* end, unsigned* occupied) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const unsigned k = key[p];
+  if (k >= static_cast<unsigned>(ntiles)) return;  // removed or cropped entries sort last
   if (p == 0 || key[p - 1] != k) {
     begin[k] = p;
     atomicAdd(occupied, 1u);
@@ -387,7 +400,8 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks(PArgs P, GArgs G, co
   const bool bWaves = threadIdx.x >= kChunk;
   const int lane = static_cast<int>(threadIdx.x) - (bWaves ? kChunk : 0);
   const int p = ch.y + lane;
-  const bool live = p < ch.z;
+  // carriers move without fields and take no part in the residual
+  const bool live = p < ch.z && P.state[p] == kStateActive;
   unsigned err = 0;
   double a[3] = {0., 0., 0.}, b[3] = {0., 0., 0.}, g[3] = {0., 0., 0.};
   if (live) {
@@ -473,23 +487,45 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks(PArgs P, GArgs G, co
 
 
 // Push of source_coupling.cpp:133-147 for one trial with the gathered
-// fields: the substeps with prescribed magnets, then the particle-boundary
-// and domain checks of the deposition loop.
+// fields (the substeps with prescribed magnets), or the straight move of a
+// CPML carrier (position += dt u c / gamma, simulation.cpp:4407-4413), then
+// the segment's boundary events.
 __global__ void __launch_bounds__(128) kPush(PArgs P, GArgs G, StepArgs S, unsigned* errOut) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P.n) return;
   unsigned err = 0;
   const double x0[3] = {P.x[0][p], P.x[1][p], P.x[2][p]};
   const double u0[3] = {P.u[0][p], P.u[1][p], P.u[2][p]};
-  const double usedE[3] = {P.fin[0][p], P.fin[1][p], P.fin[2][p]};
-  const double usedB[3] = {P.fin[3][p], P.fin[4][p], P.fin[5][p]};
-  double x[3], u[3], absGrid;
-  pushChord(S, G.o, x0, u0, P.q[p], P.m[p], usedE, usedB, x, u, absGrid, err);
+  double x[3], u[3], absGrid = 0.0, t1 = 0.0, t2 = 1.0, wDeposit = 0.0, wEnd = 0.0;
+  int code;
+  if (P.state[p] == kStateCarrier) {
+    const double scale = kC / gammaFromU(u0[0], u0[1], u0[2]);
+    for (int a = 0; a < 3; ++a) {
+      const double v = u0[a] * scale;
+      x[a] = x0[a];
+      x[a] += S.dt * v;
+      u[a] = u0[a];
+    }
+    code = carrierSegment(S, G.o, x0, 0.0, x, P.w[p], t2, wDeposit, wEnd, err);
+  } else {
+    const double usedE[3] = {P.fin[0][p], P.fin[1][p], P.fin[2][p]};
+    const double usedB[3] = {P.fin[3][p], P.fin[4][p], P.fin[5][p]};
+    pushChord(S, G.o, x0, u0, P.q[p], P.m[p], usedE, usedB, x, u, absGrid, err);
+    code = activeSegmentEvents(S, G.o, x0, x, t1, t2, wDeposit, wEnd, err);
+  }
   for (int a = 0; a < 3; ++a) {
     P.x1[a][p] = x[a];
     P.u1[a][p] = u[a];
   }
   P.gridAbs[p] = absGrid;
+  P.evt[p] = code;
+  if (code) {
+    P.cut[0][p] = t1;
+    P.cut[1][p] = t2;
+    P.cut[2][p] = wDeposit;
+    P.wend[p] = wEnd;
+    err |= kEvtAny;
+  }
   if (err) atomicOr(errOut, err);
 }
 
@@ -559,10 +595,11 @@ __global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, const int4*
       base = box.base(i, j, k);
       return true;
     };
+    double mult = 1.0;  // weight of the chord part being deposited
     auto fast = [&](int d, int oi, int oj, int ok, double w) {
       if (scale[d] == 0.) return;
       atomicAdd(&acc[d * N + base + (ok * NY + oj) * NX + oi],
-                static_cast<unsigned long long>(llrint(ldexp(scale[d] * w, shift))));
+                static_cast<unsigned long long>(llrint(ldexp(scale[d] * w * mult, shift))));
     };
     auto visit = [&](int d, long long ii, long long jj, long long kk, double w) {
       if (scale[d] == 0.) return;
@@ -574,11 +611,29 @@ __global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, const int4*
       if (kk < 0 || kk >= G.ne[d][2]) return;
       const int l = box.local(static_cast<int>(ii), static_cast<int>(jj), static_cast<int>(kk), err);
       if (l < 0) return;
-      atomicAdd(&acc[d * N + l], static_cast<unsigned long long>(llrint(ldexp(scale[d] * w, shift))));
+      atomicAdd(&acc[d * N + l], static_cast<unsigned long long>(llrint(ldexp(scale[d] * w * mult, shift))));
     };
+    // A boundary event splits the chord: [0, t1] at unit weight (the active
+    // part) and [t1, t2] at the carrier part's deposit weight.
+    const int e = P.evt[p];
+    const double t1 = e ? P.cut[0][p] : 1.0, t2 = e ? P.cut[1][p] : 1.0, wCarrier = e ? P.cut[2][p] : 0.0;
     for (int s = 1; s < count; ++s) {
-      if (cuts[s] - cuts[s - 1] <= 0.) continue;
-      segmentTaps<SHAPE, double>(G.nx, G.ny, org, x, v, cuts[s - 1], cuts[s], interior, fast, visit);
+      const double c0 = cuts[s - 1], c1 = cuts[s];
+      if (c1 - c0 <= 0.) continue;
+      if (!e) {
+        segmentTaps<SHAPE, double>(G.nx, G.ny, org, x, v, c0, c1, interior, fast, visit);
+        continue;
+      }
+      const double a1 = c0, b1 = fmin(c1, t1);
+      if (b1 > a1) {
+        mult = 1.0;
+        segmentTaps<SHAPE, double>(G.nx, G.ny, org, x, v, a1, b1, interior, fast, visit);
+      }
+      const double a2 = fmax(c0, t1), b2 = fmin(c1, t2);
+      if (b2 > a2) {
+        mult = wCarrier;
+        segmentTaps<SHAPE, double>(G.nx, G.ny, org, x, v, a2, b2, interior, fast, visit);
+      }
     }
   }
   __syncthreads();
@@ -715,7 +770,12 @@ static PArgs particleArgs(ParticleStore& P) {
   }
   A.q = P.s[6].get();
   A.m = P.s[7].get();
+  A.w = P.s[8].get();
   A.gridAbs = P.gridAbs.get();
+  A.state = P.state.get();
+  A.evt = P.evt.get();
+  for (int a = 0; a < 3; ++a) A.cut[a] = P.cut[a].get();
+  A.wend = P.wend.get();
   A.n = static_cast<int>(P.count);
   return A;
 }
@@ -725,6 +785,12 @@ void reserveParticles(ParticleStore& P, std::size_t capacity) {
   for (int a = 0; a < 9; ++a) { P.s[a].resize(capacity); P.t[a].resize(capacity); }
   P.id.resize(capacity);
   P.idT.resize(capacity);
+  P.state.resize(capacity);
+  P.stateT.resize(capacity);
+  P.evt.resize(capacity);
+  for (int a = 0; a < 3; ++a) P.cut[a].resize(capacity);
+  P.wend.resize(capacity);
+  if (!P.removed.size()) P.removed.resize(1);
   for (int a = 0; a < 6; ++a) {
     P.fields[0][a].resize(capacity);
     P.fields[1][a].resize(capacity);
@@ -743,15 +809,20 @@ void reserveParticles(ParticleStore& P, std::size_t capacity) {
 }
 
 void raiseParticleErrors(unsigned e) {
+  e &= ~kEvtAny;
   if (!e) return;
   if (e & kErrNonfinite) throw std::runtime_error("Nonfinite open coupling orbit");
+  if (e & kErrBehindInlet) throw std::runtime_error("Active particle fell behind moving inlet");
+  if (e & kErrCarrierBehind) throw std::runtime_error("Current carrier fell behind moving inlet");
+  if (e & kErrRemapOutside)
+    throw std::runtime_error("Moving-window remap particle crossed a non-adjacent slab or outer boundary");
   if (e & kErrUpstream)
     throw std::runtime_error("An active particle is upstream of the moving window; the inlet guard did not contain its release event");
   if (e & kErrStartInCpml) throw std::out_of_range("Active particle starts inside the non-physical CPML region");
   if (e & kErrStartOutside) throw std::out_of_range("Particle trajectory starts outside the global E/B domain");
   if (e & kErrBeyondExit) throw std::runtime_error("Active particle started beyond trusted moving-window exit");
-  if (e & kErrBoundaryEvent)
-    throw std::runtime_error("Particle boundary event (CPML entry, domain exit or window exit) is not supported by this build");
+  if (e & kErrCpmlLocate) throw std::runtime_error("Cannot locate the first particle entry into CPML");
+  if (e & kErrExitLocate) throw std::runtime_error("Cannot locate the first particle-domain exit");
   if (e & kErrEndpointOutside) throw std::out_of_range("Particle trajectory endpoint is outside the E/B domain");
   if (e & kErrOrbitOutside) throw std::out_of_range("Quintic active orbit outside domain");
   if (e & kErrMagneticOutside) throw std::out_of_range("Notch magnetic orbit outside domain");
@@ -765,8 +836,8 @@ void raiseParticleErrors(unsigned e) {
   throw std::runtime_error("Unknown device error");
 }
 
-StepArgs stepArgsFor(const CouplingState& C, const aprl_grid_geometry& g, bool window, const double origin[3],
-                     double time, unsigned substeps) {
+StepArgs stepArgsFor(const CouplingState& C, const aprl_grid_geometry& g, const aprl_cpml_parameters* cp,
+                     bool window, const double origin[3], double time, unsigned substeps) {
   StepArgs S{};
   S.time = time;
   S.dt = g.dt;
@@ -777,12 +848,28 @@ StepArgs stepArgsFor(const CouplingState& C, const aprl_grid_geometry& g, bool w
   const double sp[3] = {g.dx, g.dy, g.dz};
   const std::size_t cells[3] = {g.nx, g.ny, g.nz};
   const double eps = std::numeric_limits<double>::epsilon();
+  S.cpmlEnabled = 0;
+  S.cpmlOrder = cp ? cp->polynomial_order : 0.0;
   for (int a = 0; a < 3; ++a) {
     const double lo = origin[a], hi = origin[a] + static_cast<double>(cells[a]) * sp[a];
+    if (2 * C.cpmlCells[a] >= cells[a] && C.cpmlCells[a] > 0)
+      throw std::invalid_argument("Particle CPML region must leave a physical interior");
     S.physLo[a] = lo + static_cast<double>(C.cpmlCells[a]) * sp[a];
     S.physHi[a] = hi - static_cast<double>(C.cpmlCells[a]) * sp[a];
     S.cpmlTol[a] = 64.0 * eps * std::max(1.0, std::max(std::abs(lo), std::abs(hi)));
+    S.cpmlOuterLo[a] = lo;
+    S.cpmlOuterHi[a] = hi;
     S.cpmlCells[a] = static_cast<int>(C.cpmlCells[a]);
+    S.cpmlRate[a] = 0.0;
+    S.cpmlThick[a] = static_cast<double>(C.cpmlCells[a]) * sp[a];
+    if (C.cpmlCells[a]) {
+      // EBCPMLParameters::maximumConductivityRate, eb_cpml.cpp:263-281
+      S.cpmlEnabled = 1;
+      const double order = cp ? cp->polynomial_order : 0.0, reflection = cp ? cp->target_reflection : 0.0;
+      if (!(order > 0.0) || !(reflection > 0.0 && reflection < 1.0))
+        throw std::invalid_argument("Cannot evaluate an invalid CPML conductivity profile");
+      S.cpmlRate[a] = -(order + 1.0) * kC * std::log(reflection) / (2.0 * S.cpmlThick[a]);
+    }
     S.outLo[a] = origin[a];
     S.outHi[a] = std::fma(static_cast<double>(cells[a]), sp[a], origin[a]);
     S.outTol[a] = 64.0 * eps * std::max(1.0, std::max(std::abs(S.outLo[a]), std::abs(S.outHi[a])));
@@ -801,19 +888,21 @@ static int bitsFor(unsigned v) {
   return b;
 }
 
-// Stable sort of the particle state by start tile, then the tile ranges and
-// the chunk list (tile order, kChunk particles per chunk) in device memory.
-static void sortAndChunk(CouplingState& C, ParticleStore& P, const GArgs& G, int ntx, int nty, int ntiles,
-                         unsigned* occupied) {
+// Stable sort of the particle state by start tile (removed and cropped
+// entries last), then the tile ranges and the chunk list (tile order,
+// kChunk particles per chunk) in device memory.
+static void sortAndChunk(CouplingState& C, ParticleStore& P, const GArgs& G, const SortArgs& K, int ntx, int nty,
+                         int ntiles, unsigned* occupied) {
   FieldState& F = *C.fields;
   hipStream_t st = F.stream;
   const int n = static_cast<int>(P.count);
   ScopedKernel sk(F.profiler, st, "sort");
   const unsigned blocks = ceilDiv(n, 256);
-  kTileKeys<<<blocks, 256, 0, st>>>(P.s[0].get(), P.s[1].get(), P.s[2].get(), n, G, ntx, nty, P.key.get(),
-                                    P.perm.get(), TileRangeArgs{C.tileBegin.get(), C.tileEnd.get(), ntiles});
+  kTileKeys<<<blocks, 256, 0, st>>>(P.s[0].get(), P.s[1].get(), P.s[2].get(), P.state.get(), n, G, ntx, nty,
+                                    P.key.get(), P.perm.get(), TileRangeArgs{C.tileBegin.get(), C.tileEnd.get(), ntiles},
+                                    K);
   launchCheck("tile keys");
-  const int bits = bitsFor(static_cast<unsigned>(ntiles - 1));
+  const int bits = bitsFor(static_cast<unsigned>(ntiles));  // keys 0..ntiles (ntiles: behind the live entries)
   std::size_t bytes = 0;
   tileSort(nullptr, bytes, P.key.get(), P.keySorted.get(), P.perm.get(), P.permSorted.get(), static_cast<unsigned>(n), bits, st);
   if (bytes > P.sortTempBytes) {
@@ -829,10 +918,13 @@ static void sortAndChunk(CouplingState& C, ParticleStore& P, const GArgs& G, int
   }
   A.idSrc = P.id.get();
   A.idDst = P.idT.get();
+  A.stateSrc = P.state.get();
+  A.stateDst = P.stateT.get();
   kPermute<<<blocks, 256, 0, st>>>(A, P.permSorted.get(), n);
   launchCheck("permute");
   for (int a = 0; a < 9; ++a) P.s[a].swap(P.t[a]);
   P.id.swap(P.idT);
+  P.state.swap(P.stateT);
   const int rz = C.shape == kShapeQuintic ? Box<kShapeQuintic>::RZ : Box<kShapeBinomial3X>::RZ;
   buildChunkLists(C, st, P.keySorted.get(), n, ntx, nty, ntiles, rz, occupied);
 }
@@ -841,7 +933,7 @@ static void sortAndChunk(CouplingState& C, ParticleStore& P, const GArgs& G, int
 // scan, the chunk list and the active-tile list (all in tile order).
 void buildChunkLists(CouplingState& C, hipStream_t st, const unsigned* keySorted, int n, int ntx, int nty,
                      int ntiles, int rz, unsigned* occupied) {
-  kTileBounds<<<ceilDiv(n, 256), 256, 0, st>>>(keySorted, n, C.tileBegin.get(), C.tileEnd.get(), occupied);
+  kTileBounds<<<ceilDiv(n, 256), 256, 0, st>>>(keySorted, n, ntiles, C.tileBegin.get(), C.tileEnd.get(), occupied);
   launchCheck("tile bounds");
   const int ntz = ntiles / (ntx * nty);
   kTileCounts<<<ceilDiv(ntiles + 1, 256), 256, 0, st>>>(C.tileBegin.get(), C.tileEnd.get(), ntx, nty, ntz, rz,
@@ -896,13 +988,94 @@ void reduceChunkStats(hipStream_t st, const double* in, int chunks, double* out)
   launchCheck("chunk stats");
 }
 
+void reserveCouplingWords(CouplingState& C, hipStream_t st) {
+  if (C.error.size() < static_cast<std::size_t>(kErrorWords)) C.error.resize(kErrorWords);
+  if (C.reduceOut.size() < 4) C.reduceOut.resize(4);
+  if (!C.counterQ.size()) {
+    C.counterQ.resize(kCounterSlots);
+    C.counterN.resize(kCounterSlots);
+    hipCheck(hipMemsetAsync(C.counterQ.get(), 0, kCounterSlots * sizeof(double), st), "hipMemsetAsync");
+    hipCheck(hipMemsetAsync(C.counterN.get(), 0, kCounterSlots * sizeof(unsigned long long), st), "hipMemsetAsync");
+  }
+}
+
+namespace {
+// Block partials of the counters, added in block order to the cumulative
+// counters; a null flag always adds, otherwise only when it carries kEvtAny.
+__global__ void __launch_bounds__(64) kCounterTotals(const double* blockQ, const unsigned* blockN, int blocks,
+                                                     const unsigned* flag, double* q, unsigned long long* count) {
+  if (flag && !(*flag & kEvtAny)) return;
+  const int t = static_cast<int>(threadIdx.x);
+  if (t >= kCounterSlots) return;
+  double acc = 0.0;
+  unsigned long long cnt = 0;
+  for (int b = 0; b < blocks; ++b) {
+    acc += blockQ[b * kCounterSlots + t];
+    cnt += blockN[b * kCounterSlots + t];
+  }
+  q[t] += acc;
+  count[t] += cnt;
+}
+
+void reserveBlockCounters(CouplingState& C, int blocks) {
+  const std::size_t need = static_cast<std::size_t>(std::max(blocks, 1)) * kCounterSlots;
+  if (C.blockQ.size() < need) {
+    C.blockQ.resize(need);
+    C.blockN.resize(need);
+  }
+}
+}  // namespace
+
+void commitEvents(CouplingState& C, hipStream_t st, int n, const int* evt, int* state, void* w, const void* q,
+                  void* wend, unsigned* removed, const unsigned* trialWord, bool fp32) {
+  if (n <= 0) return;
+  const int blocks = static_cast<int>(ceilDiv(n, kEventBlock));
+  reserveBlockCounters(C, blocks);
+  if (fp32)
+    kCommitEvents<float><<<blocks, kEventBlock, 0, st>>>(n, evt, state, static_cast<float*>(w),
+                                                          static_cast<const float*>(q), static_cast<float*>(wend),
+                                                          removed, trialWord, C.blockQ.get(), C.blockN.get());
+  else
+    kCommitEvents<double><<<blocks, kEventBlock, 0, st>>>(n, evt, state, static_cast<double*>(w),
+                                                           static_cast<const double*>(q), static_cast<double*>(wend),
+                                                           removed, trialWord, C.blockQ.get(), C.blockN.get());
+  kCounterTotals<<<1, 64, 0, st>>>(C.blockQ.get(), C.blockN.get(), blocks, trialWord, C.counterQ.get(),
+                                   C.counterN.get());
+  launchCheck("commit events");
+}
+
+void cropCounters(CouplingState& C, hipStream_t st, int lo, int n, const int* state, const void* q, const void* w,
+                  bool fp32) {
+  if (n <= lo) return;
+  const int blocks = static_cast<int>(ceilDiv(n - lo, kEventBlock));
+  reserveBlockCounters(C, blocks);
+  if (fp32)
+    kCropCounters<float><<<blocks, kEventBlock, 0, st>>>(lo, n, state, static_cast<const float*>(q),
+                                                          static_cast<const float*>(w), C.blockQ.get(), C.blockN.get());
+  else
+    kCropCounters<double><<<blocks, kEventBlock, 0, st>>>(lo, n, state, static_cast<const double*>(q),
+                                                           static_cast<const double*>(w), C.blockQ.get(),
+                                                           C.blockN.get());
+  kCounterTotals<<<1, 64, 0, st>>>(C.blockQ.get(), C.blockN.get(), blocks, nullptr, C.counterQ.get(),
+                                   C.counterN.get());
+  launchCheck("crop counters");
+}
+
+std::size_t liveEntries(std::size_t count, const DeviceArray<unsigned>& removed, hipStream_t st) {
+  if (!count || !removed.size()) return count;
+  unsigned r = 0;
+  hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+  hipCheck(hipMemcpy(&r, removed.get(), sizeof(unsigned), hipMemcpyDeviceToHost), "hipMemcpy");
+  return count - std::min<std::size_t>(count, r);
+}
+
 void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], double time, unsigned substeps,
                   aprl_step_report* report) {
   FieldState& F = *C.fields;
   if (F.window && F.phase != 2) throw std::logic_error("Moving-window electric phase mismatch");
   if (F.window && !F.inletSet) throw std::logic_error("Moving-window inlet planes were not supplied");
   if (substeps == 0) throw std::invalid_argument("Particle substeps must be positive");
-  const int n = static_cast<int>(P.count);
+  int n = static_cast<int>(P.count);
   hipStream_t st = F.stream;
   // J is zero at the start of every step (simulation.cpp:518); the tile
   // reduction below rewrites only the neighbourhood of occupied tiles.
@@ -920,21 +1093,27 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
   const int ntiles = ntx * nty * ntz;
   reserveTileLists(C, st, ntiles, n);
   const std::size_t chunkBound = chunkListBound(ntiles, n);
-  // error words: [0] step level, [1] occupied-tile counter, [2 + t] the
-  // kernels of Picard trial t (checked in trial order)
-  if (!C.error.size()) { C.error.resize(2 + 64); C.reduceOut.resize(4); }
-  // pinned readback slots: [0..1] scan total (chunks, active tiles), [2]
-  // occupied tiles, [4..68) trial error words; doubles from byte 512:
-  // force, defect, scale, calls
+  reserveCouplingWords(C, st);
+  // pinned readback slots: [0..1] scan total (chunks, active tiles),
+  // [4..68) trial error words, [120..123] the sort-phase words; doubles from
+  // byte 512: force, defect, scale, calls
   unsigned* pin = static_cast<unsigned*>(C.pinned.get(1024));
   double* pinD = reinterpret_cast<double*>(static_cast<unsigned char*>(C.pinned.get(1024)) + 512);
-  kFillInt<<<1, 128, 0, st>>>(reinterpret_cast<int*>(C.error.get()), 2 + 64, 0);
+  kFillInt<<<1, 128, 0, st>>>(reinterpret_cast<int*>(C.error.get()), kErrorWords, 0);
   launchCheck("fill");
   if (n) {
-    sortAndChunk(C, P, G, ntx, nty, ntiles, C.error.get() + 1);
+    SortArgs K{};
+    K.window = F.window ? 1 : 0;
+    K.nz = static_cast<int>(F.g.nz);
+    K.lowZ = origin[2];
+    K.upZ = std::fma(static_cast<double>(F.g.nz), F.g.dz, origin[2]);
+    K.dz = F.g.dz;
+    K.words = C.error.get();
+    hipCheck(hipMemsetAsync(P.removed.get(), 0, sizeof(unsigned), st), "hipMemsetAsync");
+    sortAndChunk(C, P, G, K, ntx, nty, ntiles, C.error.get() + 1);
     hipCheck(hipMemcpyAsync(pin, C.tileScan.get() + ntiles, sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
              "hipMemcpyAsync");
-    hipCheck(hipMemcpyAsync(pin + 2, C.error.get() + 1, sizeof(unsigned), hipMemcpyDeviceToHost, st),
+    hipCheck(hipMemcpyAsync(pin + 120, C.error.get(), 4 * sizeof(unsigned), hipMemcpyDeviceToHost, st),
              "hipMemcpyAsync");
     hipCheck(hipEventRecord(C.sorted.get(), st), "hipEventRecord");
   }
@@ -953,7 +1132,13 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
     hipCheck(hipEventSynchronize(C.sorted.get()), "hipEventSynchronize");
     nChunks = pin[0];
     nActive = pin[1];
-    occupiedTiles = pin[2];
+    occupiedTiles = pin[121];
+    // finishMovingWindowStep's checks of the entries, applied to this origin
+    raiseParticleErrors(pin[120]);
+    const int behind = static_cast<int>(pin[122]);
+    if (pin[123]) cropCounters(C, st, n - behind, n, P.state.get(), P.s[6].get(), P.s[8].get(), false);
+    n -= behind;  // removed and cropped entries now trail the live ones
+    P.count = static_cast<std::size_t>(n);
     if (nChunks > chunkBound) throw std::logic_error("Chunk list exceeds its bound");
   }
   const bool quintic = C.shape == kShapeQuintic;
@@ -967,15 +1152,15 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
   if (C.reduceBlocks.size() < 4 * static_cast<std::size_t>(std::max(1u, nChunks)))
     C.reduceBlocks.resize(4 * static_cast<std::size_t>(std::max(1u, nChunks)));
 
-  const StepArgs S = stepArgsFor(C, F.g, F.window, origin, time, substeps);
+  const StepArgs S = stepArgsFor(C, F.g, F.hasCpml ? &F.cpml : nullptr, F.window, origin, time, substeps);
   PArgs A = particleArgs(P);
   A.scaleDen = F.g.dx * F.g.dy * F.g.dz * F.g.dt;
   bool converged = false;
   unsigned iterations = 0;
   double force = 0., work = 0.;
-  unsigned checkedTrials = 0;
+  unsigned checkedTrials = 0, lastWord = 0;
   for (unsigned trial = 0; trial < 64; ++trial) {
-    unsigned* errT = C.error.get() + 2 + trial;
+    unsigned* errT = C.error.get() + kTrialWord + trial;
     if (nChunks && trial == 0) {
       // The first trial pushes against the preview with J cleared
       // (simulation_source_coupling.cpp:43).
@@ -1050,11 +1235,12 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
     // its error word is checked, first, after trial 1.
     if (n && trial == 0) continue;
     hipCheck(hipMemcpyAsync(pinD, C.reduceOut.get(), 4 * sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpy");
-    hipCheck(hipMemcpyAsync(pin + 4, C.error.get() + 2 + checkedTrials, (trial + 1 - checkedTrials) * sizeof(unsigned),
-                            hipMemcpyDeviceToHost, st),
+    hipCheck(hipMemcpyAsync(pin + 4, C.error.get() + kTrialWord + checkedTrials,
+                            (trial + 1 - checkedTrials) * sizeof(unsigned), hipMemcpyDeviceToHost, st),
              "hipMemcpy");
     hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
     for (unsigned t = checkedTrials; t <= trial; ++t) raiseParticleErrors(pin[4 + t - checkedTrials]);
+    lastWord = pin[4 + trial - checkedTrials];
     checkedTrials = trial + 1;
     // simulation_source_coupling.cpp:46-51.
     force = pinD[0];
@@ -1074,6 +1260,10 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
   double* const none[3] = {nullptr, nullptr, nullptr};
   launchElectric(F, F.ptrs(), 0, none, F.window ? F.inletPlane[0] : nullptr, F.window ? F.inletPlane[1] : nullptr);
   if (F.window) F.phase = 3;
+  // Boundary events of the converged trial: carriers, removals, counters.
+  if (lastWord & kEvtAny)
+    commitEvents(C, st, n, P.evt.get(), P.state.get(), P.s[8].get(), P.s[6].get(), P.wend.get(), P.removed.get(),
+                 C.error.get() + kTrialWord + (iterations - 1), false);
   // The trial end state becomes the particle state; the start positions
   // stay in the end arrays for the step diagnostics.
   for (int a = 0; a < 6; ++a) P.s[a].swap(P.end[a]);

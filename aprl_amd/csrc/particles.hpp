@@ -24,7 +24,7 @@ enum ErrorBit : unsigned {
   kErrRootNonphysical = 1u << 5,   // source_coupling_math.h:285 (runtime_error)
   kErrRootNoConverge = 1u << 6,    // source_coupling_math.h:289 (runtime_error)
   kErrNonfinite = 1u << 7,         // source_coupling_math.h:241 (runtime_error)
-  kErrBoundaryEvent = 1u << 8,     // boundary events are not supported yet
+  kErrRemapOutside = 1u << 8,      // simulation_moving_window.cpp:42, 358-360 (runtime_error)
   kErrStartInCpml = 1u << 9,       // particle_boundary.cpp:118-121 (out_of_range)
   kErrStartOutside = 1u << 10,     // particle_boundary.cpp:325-330 (out_of_range)
   kErrBeyondExit = 1u << 11,       // simulation_moving_window.cpp:387 (runtime_error)
@@ -32,7 +32,33 @@ enum ErrorBit : unsigned {
   kErrCrossings = 1u << 13,        // orbit crossing budget (runtime_error)
   kErrTileHalo = 1u << 14,         // chord longer than one cell per axis (runtime_error)
   kErrMagneticNoDomain = 1u << 15, // source_coupling.cpp:84 (out_of_range)
+  kErrCarrierBehind = 1u << 16,    // simulation_moving_window.cpp:365 (runtime_error)
+  kErrCpmlLocate = 1u << 17,       // particle_boundary.cpp:153-157 (runtime_error)
+  kErrExitLocate = 1u << 18,       // particle_boundary.cpp:356-360 (runtime_error)
+  kErrBehindInlet = 1u << 19,      // simulation_moving_window.cpp:359 (runtime_error)
+  // not an error: some entry ended a trial's segment at a boundary event
+  kEvtAny = 1u << 31,
 };
+
+// State of a particle-store entry. A CPML carrier (ParticleCPMLCarrier,
+// particle_boundary.h:76-84) keeps its proper velocity (velocity = u c /
+// gamma) and holds its current weight in the weight slot. Removed and
+// cropped entries stay in the arrays until the next step's sort moves them
+// behind the live ones.
+enum EntryState : int { kStateActive = 0, kStateCarrier = 1, kStateRemoved = -1, kStateCropped = -2 };
+
+// Boundary counter slots (cumulative per coupling): CPML entries per face
+// (0-5, faces x- x+ y- y+ z- z+), outer exits of particles (6-11) and of
+// carriers (12-17), moving-window exits (18) and cropped carriers (19);
+// simulation.h cpmlEntryCount_ / directOuterCount_ / carrierOuterCount_ and
+// the window state's exits / croppedCarriers, with their charges.
+constexpr int kCounterSlots = 20;
+
+// Error words of a coupling step: [0] sort-phase errors, [1] occupied
+// tiles, [2] entries keyed behind the live ones (removed or cropped),
+// [3] cropped carriers, [kTrialWord + t] the kernels of Picard trial t.
+constexpr int kTrialWord = 4;
+constexpr int kErrorWords = kTrialWord + 64;
 
 struct MagnetDev {
   int type;  // 0 undulator, 1 dipole, 2 quadrupole
@@ -51,13 +77,22 @@ struct BoostDev {
 struct ParticleStore {
   std::size_t capacity = 0, count = 0;
   std::uint64_t generation = 0;  // bumped by every host upload
-  // SoA state: x y z ux uy uz q m w, id
+  // SoA state: x y z ux uy uz q m w, id, entry state
   DeviceArray<double> s[9], t[9];
   DeviceArray<uint64_t> id, idT;
+  DeviceArray<int> state, stateT;
   // Per-step slots: fields[0] and fields[1] alternate between the fields a
   // trial pushes with (Ex Ey Ez Bx By Bz) and the residual gather's next
   // fields; end[6] = trial end state x1, u1; gridAbs = per-substep work sum.
   DeviceArray<double> fields[2][6], end[6], gridAbs;
+  // per-step boundary events (rewritten by every trial's push): event
+  // code, chord fractions t1 (end of the unit-weight part) and t2 (end of
+  // the carrier part), the carrier part's deposit weight, and the carrier
+  // end weight (after the commit: the entry's weight at the step start)
+  DeviceArray<int> evt;
+  DeviceArray<double> cut[3], wend;
+  // entries removed by the last commit (read lazily between steps)
+  DeviceArray<unsigned> removed;
   // sort
   DeviceArray<unsigned> key, keySorted, perm, permSorted;
   DeviceArray<unsigned char> sortTemp;
@@ -74,11 +109,15 @@ struct SplitStore {
   DeviceArray<int> cell[3], cellT[3];
   DeviceArray<float> s[9], t[9];  // offset x y z, u x y z, q m w
   DeviceArray<uint64_t> id, idT;
+  DeviceArray<int> state, stateT;
   // per-step slots as in ParticleStore: alternating gathered fields, the
   // trial end state (cell1, offset1, u1) and the per-step work sum
   DeviceArray<float> fields[2][6], end[6], gridAbs;
   DeviceArray<int> endCell[3];
   DeviceArray<float> chord[3];  // trial displacement x1 - x0 in cells
+  DeviceArray<int> evt;            // per-step boundary events, as in ParticleStore
+  DeviceArray<float> cut[3], wend;
+  DeviceArray<unsigned> removed;
   DeviceArray<unsigned> key, keySorted, perm, permSorted;
   DeviceArray<unsigned char> sortTemp;
   std::size_t sortTempBytes = 0;
@@ -123,6 +162,10 @@ struct CouplingState {
   DeviceArray<long long> partial;  // per chunk: 3 extended tile boxes of fixed-point current
   DeviceArray<double> reduceBlocks, reduceOut;
   DeviceArray<unsigned> error;
+  // cumulative boundary counters and their per-block partials
+  DeviceArray<double> counterQ, blockQ;
+  DeviceArray<unsigned long long> counterN;
+  DeviceArray<unsigned> blockN;
   PinnedHost pinned;
   Event sorted, stepDone;
   std::shared_ptr<SplitStepContext> split;
@@ -161,6 +204,19 @@ void fillInts(hipStream_t st, int* p, int n, int v);
 void reduceChunkStats(hipStream_t st, const double* in, int chunks, double* out);
 void raiseParticleErrors(unsigned e);
 void reserveParticles(ParticleStore& P, std::size_t capacity);
+// Error words and boundary counters of a coupling (allocated and cleared
+// once).
+void reserveCouplingWords(CouplingState& C, hipStream_t st);
+// Boundary-event commit after the converged trial: entry states and
+// weights, removed-entry count and the counters, skipped on the device when
+// the trial's error word carries no kEvtAny. R is the store's real type.
+void commitEvents(CouplingState& C, hipStream_t st, int n, const int* evt, int* state, void* w, const void* q,
+                  void* wend, unsigned* removed, const unsigned* trialWord, bool fp32);
+// Cropped-carrier counters of the sorted tail [lo, n).
+void cropCounters(CouplingState& C, hipStream_t st, int lo, int n, const int* state, const void* q, const void* w,
+                  bool fp32);
+// Live entries of a store whose last commit may have removed some.
+std::size_t liveEntries(std::size_t count, const DeviceArray<unsigned>& removed, hipStream_t st);
 // Stable sort of (tile key, particle index) pairs on the low `bits` key
 // bits; temp == nullptr returns the scratch size in bytes.
 void tileSort(void* temp, std::size_t& bytes, unsigned* key, unsigned* keySorted, unsigned* perm,

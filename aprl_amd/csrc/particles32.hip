@@ -55,6 +55,11 @@ struct PArgs32 {
   float* chord[3];  // trial chord x1 - x0 in cells, rounded once from FP64
   float* u1[3];
   float* gridAbs;
+  const int* state;  // entry states (kStateActive / kStateCarrier for live entries)
+  int* evt;          // boundary event code of the trial
+  float* cut[3];     // t1, t2, carrier-part deposit weight
+  float* wend;       // carrier end weight
+  const float* w;    // weight slot (a carrier's current weight)
   float kq[3];  // h_a / (dx dy dz dt): current density per unit charge and cell of motion
   double h[3];
   int n;
@@ -272,16 +277,22 @@ __device__ void gatherBQuintic32(const GArgs32& G, Chord32 c, float res[3], unsi
 // ---------------------------------------------------------------------------
 // Sort.
 
-__global__ void kTileKeys32(const int* cx, const int* cy, const int* cz, int n, GArgs32 G, int ntx, int nty,
-                            unsigned* key, unsigned* perm, TileRangeArgs R) {
+__global__ void kTileKeys32(const int* cx, const int* cy, const int* cz, const float* oz, int* state, int n,
+                            GArgs32 G, int ntx, int nty, unsigned* key, unsigned* perm, TileRangeArgs R, SortArgs K,
+                            double hz) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   clearTileRanges(R, p);
   if (p >= n) return;
+  perm[p] = p;
+  const double z = (static_cast<double>(cz[p]) + static_cast<double>(oz[p])) * hz;
+  if (!liveEntry(K, state, p, z)) {
+    key[p] = static_cast<unsigned>(R.ntiles);
+    return;
+  }
   const int x = clampIndex32(cx[p] - G.oc[0], G.nx);
   const int y = clampIndex32(cy[p] - G.oc[1], G.ny);
-  const int z = clampIndex32(cz[p] - G.oc[2], G.nz);
-  key[p] = (static_cast<unsigned>(z / kTileZ) * nty + y / kTileY) * ntx + x / kTileX;
-  perm[p] = p;
+  const int zc = clampIndex32(cz[p] - G.oc[2], G.nz);
+  key[p] = (static_cast<unsigned>(zc / kTileZ) * nty + y / kTileY) * ntx + x / kTileX;
 }
 
 struct PermuteArgs32 {
@@ -291,6 +302,8 @@ struct PermuteArgs32 {
   float* dst[9];
   const uint64_t* idSrc;
   uint64_t* idDst;
+  const int* stateSrc;
+  int* stateDst;
 };
 
 __global__ void kPermute32(PermuteArgs32 A, const unsigned* perm, int n) {
@@ -302,6 +315,7 @@ __global__ void kPermute32(PermuteArgs32 A, const unsigned* perm, int n) {
 #pragma unroll
   for (int a = 0; a < 9; ++a) A.dst[a][p] = A.src[a][q];
   A.idDst[p] = A.idSrc[q];
+  A.stateDst[p] = A.stateSrc[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -346,7 +360,8 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks32(PArgs32 P, GArgs32
   const bool bWaves = threadIdx.x >= kChunk;
   const int lane = static_cast<int>(threadIdx.x) - (bWaves ? kChunk : 0);
   const int p = ch.y + lane;
-  const bool live = p < ch.z;
+  // carriers move without fields and take no part in the residual
+  const bool live = p < ch.z && P.state[p] == kStateActive;
   unsigned err = 0;
   float g[3] = {0.f, 0.f, 0.f};
   int c0[3] = {0, 0, 0};
@@ -423,7 +438,9 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks32(PArgs32 P, GArgs32
 
 // One trial's push: FP32 state and fields widened to FP64, the FP64 orbit
 // push on absolute box coordinates (origin o = the grid origin cell times
-// h), and the end point split again into cell + FP32 offset.
+// h) or a carrier's straight move (position += dt u c / gamma), the
+// segment's boundary events, and the end point split again into cell +
+// FP32 offset.
 __global__ void __launch_bounds__(128) kPushSplit(PArgs32 P, StepArgs S, double o0, double o1, double o2,
                                                   unsigned* errOut) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -431,16 +448,30 @@ __global__ void __launch_bounds__(128) kPushSplit(PArgs32 P, StepArgs S, double 
   unsigned err = 0;
   const double o[3] = {o0, o1, o2};
   int c0[3];
-  double x0[3], u0[3], usedE[3], usedB[3];
+  double x0[3], u0[3];
   for (int a = 0; a < 3; ++a) {
     c0[a] = P.cell[a][p];
     x0[a] = (static_cast<double>(c0[a]) + static_cast<double>(P.off[a][p])) * P.h[a];
     u0[a] = P.u[a][p];
-    usedE[a] = P.fin[a][p];
-    usedB[a] = P.fin[3 + a][p];
   }
-  double x[3], u[3], absGrid;
-  pushChord(S, o, x0, u0, P.q[p], P.m[p], usedE, usedB, x, u, absGrid, err);
+  double x[3], u[3], absGrid = 0.0, t1 = 0.0, t2 = 1.0, wDeposit = 0.0, wEnd = 0.0;
+  int code;
+  if (P.state[p] == kStateCarrier) {
+    const double scale = kC / gammaFromU(u0[0], u0[1], u0[2]);
+    for (int a = 0; a < 3; ++a) {
+      x[a] = x0[a] + S.dt * (u0[a] * scale);
+      u[a] = u0[a];
+    }
+    code = carrierSegment(S, o, x0, 0.0, x, static_cast<double>(P.w[p]), t2, wDeposit, wEnd, err);
+  } else {
+    double usedE[3], usedB[3];
+    for (int a = 0; a < 3; ++a) {
+      usedE[a] = P.fin[a][p];
+      usedB[a] = P.fin[3 + a][p];
+    }
+    pushChord(S, o, x0, u0, P.q[p], P.m[p], usedE, usedB, x, u, absGrid, err);
+    code = activeSegmentEvents(S, o, x0, x, t1, t2, wDeposit, wEnd, err);
+  }
   for (int a = 0; a < 3; ++a) {
     const double rel = static_cast<double>(P.off[a][p]) + (x[a] - x0[a]) / P.h[a];
     double fl = floor(rel);
@@ -458,6 +489,14 @@ __global__ void __launch_bounds__(128) kPushSplit(PArgs32 P, StepArgs S, double 
     P.u1[a][p] = static_cast<float>(u[a]);
   }
   P.gridAbs[p] = static_cast<float>(absGrid);
+  P.evt[p] = code;
+  if (code) {
+    P.cut[0][p] = static_cast<float>(t1);
+    P.cut[1][p] = static_cast<float>(t2);
+    P.cut[2][p] = static_cast<float>(wDeposit);
+    P.wend[p] = static_cast<float>(wEnd);
+    err |= kEvtAny;
+  }
   if (err) atomicOr(errOut, err);
 }
 
@@ -527,9 +566,10 @@ __global__ void __launch_bounds__(kChunk) kDepositSplit(PArgs32 P, GArgs32 G, co
     auto fixedOf = [&](float value) {
       return static_cast<unsigned long long>(llrint(ldexp(static_cast<double>(value), shift)));
     };
+    float mult = 1.f;  // weight of the chord part being deposited
     auto fast = [&](int d, int oi, int oj, int ok, float w) {
       if (scale[d] == 0.f) return;
-      atomicAdd(&acc[d * N + base + (ok * NY + oj) * NX + oi], fixedOf(scale[d] * w));
+      atomicAdd(&acc[d * N + base + (ok * NY + oj) * NX + oi], fixedOf(scale[d] * w * mult));
     };
     auto visit = [&](int d, long long ii, long long jj, long long kk, float w) {
       if (scale[d] == 0.f) return;
@@ -541,11 +581,29 @@ __global__ void __launch_bounds__(kChunk) kDepositSplit(PArgs32 P, GArgs32 G, co
       if (kk < 0 || kk >= G.ne[d][2]) return;
       const int l = box.local(static_cast<int>(ii), static_cast<int>(jj), static_cast<int>(kk), err);
       if (l < 0) return;
-      atomicAdd(&acc[d * N + l], fixedOf(scale[d] * w));
+      atomicAdd(&acc[d * N + l], fixedOf(scale[d] * w * mult));
     };
+    // A boundary event splits the chord: [0, t1] at unit weight (the active
+    // part) and [t1, t2] at the carrier part's deposit weight.
+    const int e = P.evt[p];
+    const float t1 = e ? P.cut[0][p] : 1.f, t2 = e ? P.cut[1][p] : 1.f, wCarrier = e ? P.cut[2][p] : 0.f;
     for (int s = 1; s < count; ++s) {
-      if (cuts[s] - cuts[s - 1] <= 0.f) continue;
-      segmentTaps<SHAPE, float>(G.nx, G.ny, c.org, c.x, c.v, cuts[s - 1], cuts[s], interior, fast, visit);
+      const float k0 = cuts[s - 1], k1 = cuts[s];
+      if (k1 - k0 <= 0.f) continue;
+      if (!e) {
+        segmentTaps<SHAPE, float>(G.nx, G.ny, c.org, c.x, c.v, k0, k1, interior, fast, visit);
+        continue;
+      }
+      const float a1 = k0, b1 = fminf(k1, t1);
+      if (b1 > a1) {
+        mult = 1.f;
+        segmentTaps<SHAPE, float>(G.nx, G.ny, c.org, c.x, c.v, a1, b1, interior, fast, visit);
+      }
+      const float a2 = fmaxf(k0, t1), b2 = fminf(k1, t2);
+      if (b2 > a2) {
+        mult = wCarrier;
+        segmentTaps<SHAPE, float>(G.nx, G.ny, c.org, c.x, c.v, a2, b2, interior, fast, visit);
+      }
     }
   }
   __syncthreads();
@@ -631,6 +689,12 @@ void reserveSplitParticles(SplitStore& P, std::size_t capacity) {
   }
   P.id.resize(capacity);
   P.idT.resize(capacity);
+  P.state.resize(capacity);
+  P.stateT.resize(capacity);
+  P.evt.resize(capacity);
+  for (int a = 0; a < 3; ++a) P.cut[a].resize(capacity);
+  P.wend.resize(capacity);
+  if (!P.removed.size()) P.removed.resize(1);
   for (int a = 0; a < 6; ++a) {
     P.fields[0][a].resize(capacity);
     P.fields[1][a].resize(capacity);
@@ -759,7 +823,12 @@ PArgs32 particleArgs32(SplitStore& P, const FieldState32& F) {
   }
   A.q = P.s[6].get();
   A.m = P.s[7].get();
+  A.w = P.s[8].get();
   A.gridAbs = P.gridAbs.get();
+  A.state = P.state.get();
+  A.evt = P.evt.get();
+  for (int a = 0; a < 3; ++a) A.cut[a] = P.cut[a].get();
+  A.wend = P.wend.get();
   const double h[3] = {F.g.dx, F.g.dy, F.g.dz};
   const double den = F.g.dx * F.g.dy * F.g.dz * F.g.dt;
   for (int a = 0; a < 3; ++a) {
@@ -776,17 +845,18 @@ int bitsFor32(unsigned v) {
   return b;
 }
 
-void sortSplit(CouplingState& C, SplitStore& P, const GArgs32& G, int ntx, int nty, int ntiles, unsigned* occupied) {
+void sortSplit(CouplingState& C, SplitStore& P, const GArgs32& G, const SortArgs& K, int ntx, int nty, int ntiles,
+               unsigned* occupied) {
   FieldState32& F = *C.fields32;
   hipStream_t st = F.stream;
   const int n = static_cast<int>(P.count);
   ScopedKernel sk(F.profiler, st, "sort");
   const unsigned blocks = ceilDiv(n, 256);
-  kTileKeys32<<<blocks, 256, 0, st>>>(P.cell[0].get(), P.cell[1].get(), P.cell[2].get(), n, G, ntx, nty,
-                                      P.key.get(), P.perm.get(),
-                                      TileRangeArgs{C.tileBegin.get(), C.tileEnd.get(), ntiles});
+  kTileKeys32<<<blocks, 256, 0, st>>>(P.cell[0].get(), P.cell[1].get(), P.cell[2].get(), P.s[2].get(),
+                                      P.state.get(), n, G, ntx, nty, P.key.get(), P.perm.get(),
+                                      TileRangeArgs{C.tileBegin.get(), C.tileEnd.get(), ntiles}, K, F.g.dz);
   launchCheck("tile keys");
-  const int bits = bitsFor32(static_cast<unsigned>(ntiles - 1));
+  const int bits = bitsFor32(static_cast<unsigned>(ntiles));  // keys 0..ntiles (ntiles: behind the live entries)
   std::size_t bytes = 0;
   tileSort(nullptr, bytes, P.key.get(), P.keySorted.get(), P.perm.get(), P.permSorted.get(), static_cast<unsigned>(n), bits, st);
   if (bytes > P.sortTempBytes) {
@@ -806,11 +876,14 @@ void sortSplit(CouplingState& C, SplitStore& P, const GArgs32& G, int ntx, int n
   }
   A.idSrc = P.id.get();
   A.idDst = P.idT.get();
+  A.stateSrc = P.state.get();
+  A.stateDst = P.stateT.get();
   kPermute32<<<blocks, 256, 0, st>>>(A, P.permSorted.get(), n);
   launchCheck("permute");
   for (int a = 0; a < 3; ++a) P.cell[a].swap(P.cellT[a]);
   for (int a = 0; a < 9; ++a) P.s[a].swap(P.t[a]);
   P.id.swap(P.idT);
+  P.state.swap(P.stateT);
   const int rz = C.shape == kShapeQuintic ? Box<kShapeQuintic>::RZ : Box<kShapeBinomial3X>::RZ;
   buildChunkLists(C, st, P.keySorted.get(), n, ntx, nty, ntiles, rz, occupied);
 }
@@ -865,16 +938,22 @@ void splitStepBegin(CouplingState& C, SplitStore& P, const double origin[3], dou
   reserveTileLists(C, st, ntiles, n);
   const std::size_t chunkBound = chunkListBound(ntiles, n);
   const unsigned trials = std::max(2u, C.splitTrials);
-  if (C.error.size() < 2 + 64) { C.error.resize(2 + 64); C.reduceOut.resize(4); }
-  if (C.reduceOut.size() < 4) C.reduceOut.resize(4);
+  reserveCouplingWords(C, st);
   unsigned* pin = static_cast<unsigned*>(C.pinned.get(1024));
-  double* pinD = reinterpret_cast<double*>(static_cast<unsigned char*>(C.pinned.get(1024)) + 512);
-  fillInts(st, reinterpret_cast<int*>(C.error.get()), 2 + 64, 0);
+  fillInts(st, reinterpret_cast<int*>(C.error.get()), kErrorWords, 0);
   if (n) {
-    sortSplit(C, P, G, ntx, nty, ntiles, C.error.get() + 1);
+    SortArgs K{};
+    K.window = F.window ? 1 : 0;
+    K.nz = static_cast<int>(F.g.nz);
+    K.lowZ = o[2];
+    K.upZ = std::fma(static_cast<double>(F.g.nz), F.g.dz, o[2]);
+    K.dz = F.g.dz;
+    K.words = C.error.get();
+    hipCheck(hipMemsetAsync(P.removed.get(), 0, sizeof(unsigned), st), "hipMemsetAsync");
+    sortSplit(C, P, G, K, ntx, nty, ntiles, C.error.get() + 1);
     hipCheck(hipMemcpyAsync(pin, C.tileScan.get() + ntiles, sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
              "hipMemcpyAsync");
-    hipCheck(hipMemcpyAsync(pin + 2, C.error.get() + 1, sizeof(unsigned), hipMemcpyDeviceToHost, st),
+    hipCheck(hipMemcpyAsync(pin + 120, C.error.get(), 4 * sizeof(unsigned), hipMemcpyDeviceToHost, st),
              "hipMemcpyAsync");
     hipCheck(hipEventRecord(C.sorted.get(), st), "hipEventRecord");
   }
@@ -896,7 +975,7 @@ void splitStepBegin(CouplingState& C, SplitStore& P, const double origin[3], dou
   X.trials = trials;
   X.G = G;
   for (int a = 0; a < 3; ++a) X.o[a] = o[a];
-  X.S = stepArgsFor(C, F.g, F.window, o, time, substeps);
+  X.S = stepArgsFor(C, F.g, F.hasCpml ? &F.cpml : nullptr, F.window, o, time, substeps);
 }
 
 // Phase 2: wait for the chunk totals, enqueue the Picard trials and the
@@ -906,7 +985,8 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
   FieldState32& F = *C.fields32;
   if (!C.split || !C.split->pending || C.split->middleDone) throw std::logic_error("FP32-split step phase mismatch");
   SplitStepContext& X = *C.split;
-  const int n = X.n, ntx = X.ntx, nty = X.nty, ntz = X.ntz, ntiles = X.ntiles;
+  int n = X.n;
+  const int ntx = X.ntx, nty = X.nty, ntz = X.ntz, ntiles = X.ntiles;
   const std::size_t chunkBound = X.chunkBound;
   const unsigned trials = X.trials;
   const GArgs32& G = X.G;
@@ -920,7 +1000,14 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
     hipCheck(hipEventSynchronize(C.sorted.get()), "hipEventSynchronize");
     nChunks = pin[0];
     nActive = pin[1];
-    occupiedTiles = pin[2];
+    occupiedTiles = pin[121];
+    // finishMovingWindowStep's checks of the entries, applied to this origin
+    raiseParticleErrors(pin[120]);
+    const int behind = static_cast<int>(pin[122]);
+    if (pin[123]) cropCounters(C, st, n - behind, n, P.state.get(), P.s[6].get(), P.s[8].get(), true);
+    n -= behind;  // removed and cropped entries now trail the live ones
+    P.count = static_cast<std::size_t>(n);
+    X.n = n;
     if (nChunks > chunkBound) throw std::logic_error("Chunk list exceeds its bound");
   }
   const bool quintic = C.shape == kShapeQuintic;
@@ -937,7 +1024,7 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
   PArgs32 A = particleArgs32(P, F);
   const bool verify = C.splitVerify;
   for (unsigned trial = 0; trial < trials; ++trial) {
-    unsigned* errT = C.error.get() + 2 + trial;
+    unsigned* errT = C.error.get() + kTrialWord + trial;
     const bool last = trial + 1 == trials;
     if (nChunks && trial == 0) {
       ScopedKernel sk(F.profiler, st, "gather");
@@ -1007,12 +1094,17 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
     }
     for (int a = 0; a < 6; ++a) std::swap(A.fin[a], A.fout[a]);
   }
+  // Boundary events of the last trial (the kernels return at once when its
+  // error word carries no event).
+  if (n)
+    commitEvents(C, st, n, P.evt.get(), P.state.get(), P.s[8].get(), P.s[6].get(), P.wend.get(), P.removed.get(),
+                 C.error.get() + kTrialWord + (trials - 1), true);
   // commitConservativeElectric with the last trial's current
   float* const none[3] = {nullptr, nullptr, nullptr};
   launchElectric(F, F.ptrs(), 0, none, F.window ? F.inletPlane[0] : nullptr, F.window ? F.inletPlane[1] : nullptr);
   if (F.window) F.phase = 3;
   // one readback per step: the trial error words (and the verifying residual)
-  hipCheck(hipMemcpyAsync(pin + 4, C.error.get() + 2, trials * sizeof(unsigned), hipMemcpyDeviceToHost, st),
+  hipCheck(hipMemcpyAsync(pin + 4, C.error.get() + kTrialWord, trials * sizeof(unsigned), hipMemcpyDeviceToHost, st),
            "hipMemcpyAsync");
   if (verify && nChunks)
     hipCheck(hipMemcpyAsync(pinD, C.reduceOut.get(), 4 * sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");

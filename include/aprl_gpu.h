@@ -122,10 +122,17 @@ aprl_status aprl_window_shift(aprl_fields* fields, size_t new_step, ptrdiff_t ne
 typedef struct aprl_particles aprl_particles;
 typedef struct aprl_coupling aprl_coupling;
 
-/* Device store of RelativisticParticleSI (eb_particles.h:12-25) on the
+/* Device store of RelativisticParticleSI (eb_particles.h:12-25) and of the
+ * CPML carriers (ParticleCPMLCarrier, particle_boundary.h:76-84) on the
  * field grid's device and stream. Positions are absolute box coordinates in
  * metres, u = gamma v / c. The store's order changes at every coupled step
- * (stable sort by cell tile); ids travel with their particles.
+ * (stable sort by cell tile); ids travel with their particles. A particle
+ * that enters the CPML or passes the moving-window exit plane becomes a
+ * carrier in place: it keeps its id, position and u (its straight-line
+ * velocity is u c / gamma) and its weight slot holds the carrier's current
+ * weight from then on. Particles and carriers that leave the domain are
+ * removed. Count and downloads cover the live entries (particles and
+ * carriers) in store order.
  * On an APRL_FP32_SPLIT grid the store keeps per axis the lattice cell
  * floor(x / h) (int32, lattice anchored at x = 0) and the FP32 offset inside
  * it, and u, charge, mass and weight in FP32; upload and download convert. */
@@ -139,6 +146,9 @@ aprl_status aprl_particles_upload(aprl_particles* particles, size_t n, const dou
 aprl_status aprl_particles_download(aprl_particles* particles, double* position3, double* u3, double* charge,
                                     double* mass, double* weight, uint64_t* id);
 size_t aprl_particles_count(const aprl_particles* particles);
+/* Entry state per live entry, in download order: 0 particle, 1 CPML carrier
+ * (no counterpart in the reference, which keeps the two in separate lists). */
+aprl_status aprl_particles_download_states(aprl_particles* particles, int* state);
 
 /* Longitudinal source shape (longitudinal_shape.h:16). */
 typedef enum aprl_shape {
@@ -182,14 +192,28 @@ aprl_status aprl_coupling_set_frame(aprl_coupling* coupling, const aprl_boost* f
 aprl_status aprl_coupling_add_magnet(aprl_coupling* coupling, const aprl_magnet* magnet);
 /* ParticleCPMLRegion cells and the moving-window lab exit plane
  * (particle_boundary.h:36-66; simulation_moving_window.cpp:380-395);
- * a non-finite exit_lab_z disables the exit plane. */
+ * a non-finite exit_lab_z disables the exit plane. The carriers' current
+ * damping uses the grid's installed CPML grading (polynomial order, target
+ * reflection; particle_boundary.cpp:176-241). */
 aprl_status aprl_coupling_set_particle_boundary(aprl_coupling* coupling, const size_t* cpml_cells,
                                                 double exit_lab_z);
+/* Cumulative particle-boundary counters since the coupling was created:
+ * counts[20] / charges[20] (C) hold the CPML entries per face (0-5, faces
+ * x- x+ y- y+ z- z+), the outer-face exits of particles (6-11) and of
+ * carriers (12-17, charge times end weight), the moving-window exits (18)
+ * and the cropped carriers (19) (simulation.h cpmlEntryCount_,
+ * directOuterCount_, carrierOuterCount_; simulation_moving_window.cpp:
+ * 363-367, 396-398). The carriers beyond a shifted window's upper face are
+ * cropped by the next coupled step, which checks the entries against its
+ * origin as finishMovingWindowStep does after the shift. */
+aprl_status aprl_coupling_boundary_counters(aprl_coupling* coupling, uint64_t* counts, double* charges);
 /* One Simulation::solveConservativeSourceStep + commitConservativeElectric
- * (simulation_source_coupling.cpp:16-65) for every particle of the store,
+ * (simulation_source_coupling.cpp:16-65) for every entry of the store,
  * after clear_current / begin_step / advance_magnetic / set_inlet. origin =
  * the grid's box origin (m); time = box time at the step start; substeps =
- * particle substeps per field step. */
+ * particle substeps per field step. Particle segments end at their first
+ * boundary event and carriers move and deposit damped current as in
+ * pushDepositAndMigrate (simulation.cpp:4310-4917). */
 aprl_status aprl_coupling_step(aprl_coupling* coupling, aprl_particles* particles, const double* origin,
                                double time, unsigned substeps, aprl_step_report* report);
 /* FP32-split grids only (no counterpart in the reference, which has no FP32

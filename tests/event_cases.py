@@ -97,3 +97,16 @@ def planes_for(rng, comps):
     from harness import component_shape
     return [rng.standard_normal(int(np.prod(component_shape(*DIMS, a)[1:]))) * (10.0 if a < 3 else 10.0 / C_SI)
             for a in comps]
+
+
+def oracle_case(prefix, shape="binomial3", smooth=True, exit_plane=False, seed=11):
+    """CPU fields (oracle or reference build) and coupling of the case."""
+    import harness as H
+    f = H.CpuFields(prefix, *DIMS, DX, DX, DZ, DZ / C_SI, 1)
+    f.install_window(CELLS)
+    H.seeded_fields(f, 5, scale=1e5)
+    c = H.CpuCoupling(f, shape, smooth, cpml_cells=CELLS)
+    c.set_frame(GAMMA_BOOST, 0.0, 0.0, 0.0)
+    x, u, q, m, ids = particles(seed, exit_plane=exit_plane)
+    c.set_particles(x, u, q, m, np.ones(len(q)), ids)
+    return f, c

@@ -144,13 +144,14 @@ def test_gpu_step_is_deterministic_and_sort_is_stable():
         assert np.array_equal(results[0][2][a], results[1][2][a])
 
 
-def test_boundary_event_raises_runtime_error():
+def test_particle_starting_in_cpml_raises_out_of_range():
+    """ParticleCPMLRegion::firstEntry's start check (particle_boundary.cpp:
+    118-121) surfaces as the reference's out_of_range message."""
     import aprl_amd as A
     dims = (20, 20, 40)
     orc, co, gpu, win, cg, store = case(dims, (4, 4, 0), 16, 3)
     x, u, q, m, w, ids = store.download()
-    x[0, 0] = 4.0 * 2.5 * DZ + 1e-9
-    u[0, 0] = -5.0
+    x[0, 0] = 2.0 * 2.5 * DZ  # inside the x- CPML layer
     store.upload(x, u, q, m, w, ids)
     gpu.clearCurrent()
     win.beginStep(0, 50)
@@ -158,4 +159,4 @@ def test_boundary_event_raises_runtime_error():
     win.setInlet()
     with pytest.raises(A.AprlError) as e:
         cg.step(store, np.zeros(3), 0.0)
-    assert e.value.status == 3 and "not supported" in str(e.value)
+    assert "starts inside the non-physical CPML region" in str(e.value)

@@ -21,15 +21,7 @@ import harness as H
 needs_ref = pytest.mark.skipif(not H.have_ref(), reason="reference build oracle/_ref not present")
 
 
-def make_case(prefix, shape="binomial3", smooth=True, exit_plane=False, seed=11):
-    f = H.CpuFields(prefix, *E.DIMS, E.DX, E.DX, E.DZ, E.DZ / H.C_SI, 1)
-    f.install_window(E.CELLS)
-    H.seeded_fields(f, 5, scale=1e5)
-    c = H.CpuCoupling(f, shape, smooth, cpml_cells=E.CELLS)
-    c.set_frame(E.GAMMA_BOOST, 0.0, 0.0, 0.0)
-    x, u, q, m, ids = E.particles(seed, exit_plane=exit_plane)
-    c.set_particles(x, u, q, m, np.ones(len(q)), ids)
-    return f, c
+make_case = E.oracle_case
 
 
 def run_window(f, c, steps, seed=4, exit_plane=False):
