@@ -250,6 +250,7 @@ __device__ bool cpmlFirstEntry(const StepArgs& S, const double s[3], const doubl
   double best = INFINITY;
   int bestFace = 0;
   bool entered = false;
+#pragma unroll
   for (int a = 0; a < 3; ++a) {
     if (!S.cpmlCells[a]) continue;
     double candidate = 0.0;
@@ -282,8 +283,11 @@ __device__ bool cpmlFirstEntry(const StepArgs& S, const double s[3], const doubl
     h.pos[a] = s[a] + h.fraction * (e[a] - s[a]);
     h.pos[a] = fmax(S.cpmlOuterLo[a], fmin(S.cpmlOuterHi[a], h.pos[a]));
   }
-  const int normal = bestFace / 2;
-  h.pos[normal] = (bestFace % 2) ? S.physHi[normal] : S.physLo[normal];
+  // the normal coordinate sits on the face (static indices keep the hit in
+  // registers)
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    if (a == bestFace / 2) h.pos[a] = (bestFace % 2) ? S.physHi[a] : S.physLo[a];
   return true;
 }
 
@@ -314,6 +318,7 @@ __device__ bool outerFirstExit(const StepArgs& S, const double s[3], const doubl
   double best = INFINITY;
   int bestFace = 0;
   bool escaped = false;
+#pragma unroll
   for (int a = 0; a < 3; ++a) {
     if (!isfinite(s[a]) || !isfinite(e[a])) err |= kErrNonfinite;
     if (s[a] < S.outLo[a] - S.outTol[a] || s[a] > S.outHi[a] + S.outTol[a]) err |= kErrStartOutside;
@@ -347,8 +352,9 @@ __device__ bool outerFirstExit(const StepArgs& S, const double s[3], const doubl
     h.pos[a] = s[a] + h.fraction * (e[a] - s[a]);
     h.pos[a] = fmax(S.outLo[a], fmin(S.outHi[a], h.pos[a]));
   }
-  const int normal = bestFace / 2;
-  h.pos[normal] = (bestFace % 2) ? S.outHi[normal] : S.outLo[normal];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    if (a == bestFace / 2) h.pos[a] = (bestFace % 2) ? S.outHi[a] : S.outLo[a];
   return true;
 }
 
@@ -454,6 +460,7 @@ struct SortArgs {
   int window, nz;
   double lowZ, upZ, dz;    // window origin z, fma(nz, dz, origin z)
   unsigned* words;         // coupling error words: [0] errors, [2] keyed behind, [3] cropped
+  unsigned* removed;       // the store's removed-entry count, cleared by the sort
 };
 
 __device__ __forceinline__ bool liveEntry(const SortArgs& K, int* state, int p, double z) {

@@ -282,6 +282,7 @@ __global__ void kTileKeys(const double* x, const double* y, const double* z, int
                           int nty, unsigned* key, unsigned* perm, TileRangeArgs R, SortArgs K) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   clearTileRanges(R, p);
+  if (p == 0) *K.removed = 0u;
   if (p >= n) return;
   perm[p] = p;
   if (!liveEntry(K, state, p, z[p])) {
@@ -620,19 +621,14 @@ __global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, const int4*
     for (int s = 1; s < count; ++s) {
       const double c0 = cuts[s - 1], c1 = cuts[s];
       if (c1 - c0 <= 0.) continue;
-      if (!e) {
-        segmentTaps<SHAPE, double>(G.nx, G.ny, org, x, v, c0, c1, interior, fast, visit);
-        continue;
-      }
-      const double a1 = c0, b1 = fmin(c1, t1);
-      if (b1 > a1) {
-        mult = 1.0;
-        segmentTaps<SHAPE, double>(G.nx, G.ny, org, x, v, a1, b1, interior, fast, visit);
-      }
-      const double a2 = fmax(c0, t1), b2 = fmin(c1, t2);
-      if (b2 > a2) {
-        mult = wCarrier;
-        segmentTaps<SHAPE, double>(G.nx, G.ny, org, x, v, a2, b2, interior, fast, visit);
+      // without an event: one part [c0, c1] (t1 = t2 = 1)
+#pragma unroll 1
+      for (int part = 0; part < 2; ++part) {
+        const double a = part ? fmax(c0, t1) : c0;
+        const double b = part ? fmin(c1, t2) : fmin(c1, t1);
+        if (!(b > a)) continue;
+        mult = part ? wCarrier : 1.0;
+        segmentTaps<SHAPE, double>(G.nx, G.ny, org, x, v, a, b, interior, fast, visit);
       }
     }
   }
@@ -1109,7 +1105,7 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
     K.upZ = std::fma(static_cast<double>(F.g.nz), F.g.dz, origin[2]);
     K.dz = F.g.dz;
     K.words = C.error.get();
-    hipCheck(hipMemsetAsync(P.removed.get(), 0, sizeof(unsigned), st), "hipMemsetAsync");
+    K.removed = P.removed.get();
     sortAndChunk(C, P, G, K, ntx, nty, ntiles, C.error.get() + 1);
     hipCheck(hipMemcpyAsync(pin, C.tileScan.get() + ntiles, sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
              "hipMemcpyAsync");

@@ -282,6 +282,7 @@ __global__ void kTileKeys32(const int* cx, const int* cy, const int* cz, const f
                             double hz) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   clearTileRanges(R, p);
+  if (p == 0) *K.removed = 0u;
   if (p >= n) return;
   perm[p] = p;
   const double z = (static_cast<double>(cz[p]) + static_cast<double>(oz[p])) * hz;
@@ -590,19 +591,14 @@ __global__ void __launch_bounds__(kChunk) kDepositSplit(PArgs32 P, GArgs32 G, co
     for (int s = 1; s < count; ++s) {
       const float k0 = cuts[s - 1], k1 = cuts[s];
       if (k1 - k0 <= 0.f) continue;
-      if (!e) {
-        segmentTaps<SHAPE, float>(G.nx, G.ny, c.org, c.x, c.v, k0, k1, interior, fast, visit);
-        continue;
-      }
-      const float a1 = k0, b1 = fminf(k1, t1);
-      if (b1 > a1) {
-        mult = 1.f;
-        segmentTaps<SHAPE, float>(G.nx, G.ny, c.org, c.x, c.v, a1, b1, interior, fast, visit);
-      }
-      const float a2 = fmaxf(k0, t1), b2 = fminf(k1, t2);
-      if (b2 > a2) {
-        mult = wCarrier;
-        segmentTaps<SHAPE, float>(G.nx, G.ny, c.org, c.x, c.v, a2, b2, interior, fast, visit);
+      // without an event: one part [k0, k1] (t1 = t2 = 1)
+#pragma unroll 1
+      for (int part = 0; part < 2; ++part) {
+        const float a = part ? fmaxf(k0, t1) : k0;
+        const float b = part ? fminf(k1, t2) : fminf(k1, t1);
+        if (!(b > a)) continue;
+        mult = part ? wCarrier : 1.f;
+        segmentTaps<SHAPE, float>(G.nx, G.ny, c.org, c.x, c.v, a, b, interior, fast, visit);
       }
     }
   }
@@ -949,7 +945,7 @@ void splitStepBegin(CouplingState& C, SplitStore& P, const double origin[3], dou
     K.upZ = std::fma(static_cast<double>(F.g.nz), F.g.dz, o[2]);
     K.dz = F.g.dz;
     K.words = C.error.get();
-    hipCheck(hipMemsetAsync(P.removed.get(), 0, sizeof(unsigned), st), "hipMemsetAsync");
+    K.removed = P.removed.get();
     sortSplit(C, P, G, K, ntx, nty, ntiles, C.error.get() + 1);
     hipCheck(hipMemcpyAsync(pin, C.tileScan.get() + ntiles, sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
              "hipMemcpyAsync");
@@ -1094,11 +1090,6 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
     }
     for (int a = 0; a < 6; ++a) std::swap(A.fin[a], A.fout[a]);
   }
-  // Boundary events of the last trial (the kernels return at once when its
-  // error word carries no event).
-  if (n)
-    commitEvents(C, st, n, P.evt.get(), P.state.get(), P.s[8].get(), P.s[6].get(), P.wend.get(), P.removed.get(),
-                 C.error.get() + kTrialWord + (trials - 1), true);
   // commitConservativeElectric with the last trial's current
   float* const none[3] = {nullptr, nullptr, nullptr};
   launchElectric(F, F.ptrs(), 0, none, F.window ? F.inletPlane[0] : nullptr, F.window ? F.inletPlane[1] : nullptr);
@@ -1124,7 +1115,6 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
 // Phase 3: wait for the step, raise the first device error in trial order,
 // fill the report.
 void splitStepEnd(CouplingState& C, SplitStore& P, aprl_step_report* report) {
-  (void)P;
   if (!C.split || !C.split->pending || !C.split->middleDone) throw std::logic_error("FP32-split step phase mismatch");
   SplitStepContext& X = *C.split;
   X.pending = false;
@@ -1135,6 +1125,12 @@ void splitStepEnd(CouplingState& C, SplitStore& P, aprl_step_report* report) {
   double* pinD = reinterpret_cast<double*>(static_cast<unsigned char*>(C.pinned.get(1024)) + 512);
   hipCheck(hipEventSynchronize(C.stepDone.get()), "hipEventSynchronize");
   for (unsigned t = 0; t < trials; ++t) raiseParticleErrors(pin[4 + t]);
+  // Boundary events of the last trial: carriers, removals, counters. The
+  // commit only touches entry states, weights and counters, which nothing
+  // reads before the next step or query.
+  if (X.n && (pin[4 + trials - 1] & kEvtAny))
+    commitEvents(C, C.fields32->stream, X.n, P.evt.get(), P.state.get(), P.s[8].get(), P.s[6].get(), P.wend.get(),
+                 P.removed.get(), C.error.get() + kTrialWord + (trials - 1), true);
   double force = 0., work = 0.;
   if (verify && nChunks) {
     force = pinD[0];
