@@ -8,6 +8,7 @@
 #include <type_traits>
 
 #include "handles.hpp"
+#include "inlet.hpp"
 
 namespace aprl_gpu {
 namespace {
@@ -299,9 +300,48 @@ aprl_status aprl_window_begin_step(aprl_fields* fields, size_t step, ptrdiff_t o
   return guarded([&] {
     withState(fields, [&](auto& s) {
       if (!s.window) throw std::logic_error("No moving window is installed");
+      if (s.chargeMatch) throw std::logic_error("A charge-matched inlet begins its steps with the centered reference");
       if (s.phase != 0 || (s.established && (step != s.step || offset != s.offset)))
         throw std::logic_error("Moving-window beginStep phase/origin mismatch");
       uploadPlanes(s, s.haloLo, lower, 0, 3, s.loPlane);
+      s.step = step;
+      s.offset = offset;
+      s.established = true;
+      s.inletSet = false;
+      s.phase = 1;
+    });
+  });
+}
+
+aprl_status aprl_window_set_charge_match(aprl_fields* fields, int enable) {
+  return guarded([&] {
+    withState(fields, [&](auto& s) {
+      if (!s.window) throw std::logic_error("No moving window is installed");
+      s.chargeMatch = enable != 0;
+    });
+  });
+}
+
+aprl_status aprl_window_begin_step_centered(aprl_fields* fields, size_t step, ptrdiff_t offset,
+                                            const double* const* lower, const double* ez_at_offset) {
+  return guarded([&] {
+    withState(fields, [&](auto& s) {
+      using R = RealOf<std::remove_reference_t<decltype(s)>>;
+      if (!s.window) throw std::logic_error("No moving window is installed");
+      if (!s.chargeMatch) throw std::logic_error("The window has no charge-matched inlet");
+      if (s.phase != 0 || (s.established && (step != s.step || offset != s.offset)))
+        throw std::logic_error("Moving-window beginStep phase/origin mismatch");
+      // the halo planes are private copies: the Ez plane is corrected in place
+      for (int a = 0; a < 3; ++a) {
+        if (s.haloLo[a].size() != s.plane(a)) s.haloLo[a].resize(s.plane(a));
+        if (lower && lower[a]) toDevice(s.stream, s.haloLo[a].get(), lower[a], s.plane(a));
+        else hipCheck(hipMemsetAsync(s.haloLo[a].get(), 0, s.plane(a) * sizeof(R), s.stream), "hipMemsetAsync");
+        s.loPlane[a] = s.haloLo[a].get();
+      }
+      if (s.refEz.size() != s.plane(2)) s.refEz.resize(s.plane(2));
+      if (ez_at_offset) toDevice(s.stream, s.refEz.get(), ez_at_offset, s.plane(2));
+      else hipCheck(hipMemsetAsync(s.refEz.get(), 0, s.plane(2) * sizeof(R), s.stream), "hipMemsetAsync");
+      centerInletHalo(s, s.haloLo[2].get(), s.refEz.get());
       s.step = step;
       s.offset = offset;
       s.established = true;

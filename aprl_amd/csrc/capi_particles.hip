@@ -4,6 +4,7 @@
 #include <memory>
 
 #include "handles.hpp"
+#include "inlet.hpp"
 
 namespace aprl_gpu {
 void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], double time, unsigned substeps,
@@ -196,6 +197,19 @@ aprl_status aprl_particles_download_states(aprl_particles* particles, int* state
     size_t k = 0;
     for (size_t i = 0; i < n; ++i)
       if (st[i] >= 0) state[k++] = st[i];
+  });
+}
+
+aprl_status aprl_window_complete_inlet(aprl_fields* fields, aprl_coupling* coupling, aprl_particles* particles,
+                                       const double* origin) {
+  return guarded([&] {
+    if (!fields || !coupling || !particles || !origin) throw std::invalid_argument("Null inlet argument");
+    if (coupling->fields != fields || particles->fields != fields)
+      throw std::invalid_argument("Inlet, coupling and particles belong to different field grids");
+    const int shape = coupling->c.shape;
+    const bool xBinomial = coupling->c.smoothX;
+    if (fields->fp32) completeInletSplit(fields->s32, particles->sp, origin, shape, xBinomial);
+    else completeInletPlain(fields->s, particles->p, origin, shape, xBinomial);
   });
 }
 

@@ -106,6 +106,11 @@ struct FieldStateT {
   DeviceArray<R> inlet[2], shiftPlanes[6];
   const R* loPlane[3] = {nullptr, nullptr, nullptr};  // lower E halo planes in use
   const R* inletPlane[2] = {nullptr, nullptr};         // inlet Ex, Ey planes in use
+  // charge-matched inlet with the centered normal reference
+  bool chargeMatch = false;
+  DeviceArray<R> refEz;                      // reference Ez at the step's offset
+  DeviceArray<unsigned long long> inletAcc;  // fixed-point inlet charge, (nx+1)(ny+1)
+  DeviceArray<unsigned long long> inletWords;
 
   // True while J is known to hold zeros on the device (set by clears and
   // shifts, dropped by any write of J), so repeated clears cost nothing.

@@ -205,6 +205,21 @@ class MovingWindowFields:
         arr, keep = plane_array(lower)
         check(lib().aprl_window_begin_step(self.fields.handle, step, offset, arr))
 
+    def setChargeMatch(self, enable: bool = True) -> None:
+        """inlet_charge_match: ChargeMatchedInlet as the new-layer completion
+        with the centered normal reference (charge_matched_inlet.h:16-72)."""
+        check(lib().aprl_window_set_charge_match(self.fields.handle, int(enable)))
+
+    def beginStepCentered(self, step: int, offset: int, lower=None, ez_at_offset=None) -> None:
+        arr, keep = plane_array(lower)
+        ez = None if ez_at_offset is None else np.ascontiguousarray(ez_at_offset, dtype=np.float64)
+        check(lib().aprl_window_begin_step_centered(self.fields.handle, step, offset, arr, dptr(ez)))
+
+    def completeInlet(self, coupling, particles, origin) -> None:
+        """completeMovingInletLayer after a shift to the window origin."""
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        check(lib().aprl_window_complete_inlet(self.fields.handle, coupling.handle, particles.handle, dptr(o)))
+
     def setInlet(self, ex=None, ey=None) -> None:
         ex = None if ex is None else np.ascontiguousarray(ex, dtype=np.float64)
         ey = None if ey is None else np.ascontiguousarray(ey, dtype=np.float64)

@@ -107,6 +107,19 @@ aprl_status aprl_window_install(aprl_fields* fields, const aprl_cpml_parameters*
  * reference. The upper halo is the sacrificial zero plane. */
 aprl_status aprl_window_begin_step(aprl_fields* fields, size_t step, ptrdiff_t offset,
                                    const double* const* lower);
+/* Charge-matched inlet (inlet_charge_match: ChargeMatchedInlet,
+ * charge_matched_inlet.h:16-72, installed by setNewLayerCompletion with a
+ * centered normal reference, moving_window_fields.cpp:551-555). When
+ * enabled, steps begin with aprl_window_begin_step_centered and every
+ * shift is followed by aprl_window_complete_inlet. Not available to
+ * aprl_window_step_batch. */
+aprl_status aprl_window_set_charge_match(aprl_fields* fields, int enable);
+/* beginStep with the centered normal reference (moving_window_fields.cpp:
+ * 225-234): as aprl_window_begin_step, then on the interior aperture the
+ * lower Ez halo -= Ez(i,j,0) - ez_at_offset, where ez_at_offset =
+ * reference(2,i,j,offset,step) (NULL = zero reference). */
+aprl_status aprl_window_begin_step_centered(aprl_fields* fields, size_t step, ptrdiff_t offset,
+                                            const double* const* lower, const double* ez_at_offset);
 /* Inlet overwrite values Ex/Ey(k=0) := reference(a,i,j,offset,step+1)
  * (moving_window_fields.cpp:276-280); NULL = zero reference. */
 aprl_status aprl_window_set_inlet(aprl_fields* fields, const double* ex, const double* ey);
@@ -197,6 +210,13 @@ aprl_status aprl_coupling_add_magnet(aprl_coupling* coupling, const aprl_magnet*
  * reflection; particle_boundary.cpp:176-241). */
 aprl_status aprl_coupling_set_particle_boundary(aprl_coupling* coupling, const size_t* cpml_cells,
                                                 double exit_lab_z);
+/* Simulation::completeMovingInletLayer (simulation_moving_window.cpp:
+ * 312-324) after aprl_window_shift to the window origin `origin`: the new
+ * layer's Ez on the interior aperture := Ez(k=1) + dz (div_xy E(k=1) -
+ * rho / eps0), rho being the shaped inlet charge of the store's particles
+ * (carriers excluded) whose longitudinal shape reaches node k = 1. */
+aprl_status aprl_window_complete_inlet(aprl_fields* fields, aprl_coupling* coupling, aprl_particles* particles,
+                                       const double* origin);
 /* Cumulative particle-boundary counters since the coupling was created:
  * counts[20] / charges[20] (C) hold the CPML entries per face (0-5, faces
  * x- x+ y- y+ z- z+), the outer-face exits of particles (6-11) and of

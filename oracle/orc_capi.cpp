@@ -102,6 +102,18 @@ int orc_fields_clear_current(void* h) { return guarded([&] { F(h)->clearCurrent(
 int orc_window_begin(void* h, std::size_t step, std::ptrdiff_t offset, const double* const* lower3) {
   return guarded([&] { F(h)->beginWindowStep(step, offset, lower3); });
 }
+// Window with a charge-matched inlet (setNewLayerCompletion with a
+// centered normal reference).
+int orc_window_set_charge_match(void* h, int enable) {
+  return guarded([&] {
+    if (!F(h)->window) throw std::logic_error("No moving window is installed");
+    F(h)->window->centered = enable != 0;
+  });
+}
+int orc_window_begin_centered(void* h, std::size_t step, std::ptrdiff_t offset, const double* const* lower3,
+                              const double* ezAtOffset) {
+  return guarded([&] { F(h)->beginWindowStep(step, offset, lower3, ezAtOffset); });
+}
 int orc_window_advance_magnetic(void* h) {
   return guarded([&] {
     orc::Window& w = *F(h)->window;
@@ -281,6 +293,19 @@ int orc_coupling_counters(void* h, std::uint64_t* counts, double* charges) {
     }
     counts[18] = k.exits; charges[18] = k.exitQ;
     counts[19] = k.cropped; charges[19] = k.croppedQ;
+  });
+}
+
+// The inlet completion of a shift to new_origin (the reference runs it
+// inside shift, after the new layer is inserted) over the coupling's
+// particles.
+int orc_coupling_complete_inlet(void* h, const double* newOrigin) {
+  return guarded([&] {
+    CouplingHandle& H = *CH(h);
+    orc::Fields& f = *H.c.fields;
+    if (!f.window || !f.window->centered) throw std::logic_error("The window has no charge-matched inlet");
+    orc::chargeMatchedInlet(f, H.particles, vec(newOrigin), H.c.shape, H.c.smoothX, f.cpml->p.cells[0],
+                            f.cpml->p.cells[1]);
   });
 }
 

@@ -548,7 +548,8 @@ class Fields {
   void advanceElectric();
   void afterElectricBoundary();
   void installWindow(const CpmlParams& p);
-  Halo beginWindowStep(std::size_t n, std::ptrdiff_t k, const double* const lowerPlanes[3]);
+  Halo beginWindowStep(std::size_t n, std::ptrdiff_t k, const double* const lowerPlanes[3],
+                       const double* ezAtOffset = nullptr);
   void setWindowInlet(const double* ex, const double* ey);
   void undoPreview();
   void shiftWindow(std::size_t n, std::ptrdiff_t k, const double* const planes[6]);
@@ -603,6 +604,10 @@ struct Window {
   std::vector<Delta> delta;
   std::vector<double> inlet[2];  // Ex/Ey(k=0) := reference(a,i,j,offset,step+1)
   bool inletSet = false;
+  // setNewLayerCompletion(..., centerNormalReference = true)
+  // (moving_window_fields.cpp:551-555): the inlet Ez is completed from the
+  // charge and its departure from the reference is reflected oddly.
+  bool centered = false;
 };
 
 inline Fields::Fields(const Grid& grid) : g(grid) {
@@ -692,15 +697,26 @@ inline void Fields::installWindow(const CpmlParams& p) {
   for (unsigned a = 0; a < 2; ++a) window->inlet[a].assign(f[a].nx * f[a].ny, 0.0);
 }
 
-// Boundary::begin + halos, moving_window_fields.cpp:207-244 (no centered
-// normal reference). lowerPlanes[a] = reference(a,i,j,offset-1,step).
-inline Halo Fields::beginWindowStep(std::size_t n, std::ptrdiff_t k, const double* const lowerPlanes[3]) {
+// Boundary::begin + halos, moving_window_fields.cpp:207-244.
+// lowerPlanes[a] = reference(a,i,j,offset-1,step); with a centered normal
+// reference, ezAtOffset = reference(2,i,j,offset,step) and the Ez halo of
+// the aperture nodes takes minus the inlet Ez's departure from it (225-234).
+inline Halo Fields::beginWindowStep(std::size_t n, std::ptrdiff_t k, const double* const lowerPlanes[3],
+                                    const double* ezAtOffset) {
   Window& w = *window;
   if (w.phase != 0 || (w.established && (n != w.step || k != w.offset)))
     throw std::logic_error("Moving-window beginStep phase/origin mismatch");
+  if (w.centered && !ezAtOffset) throw std::logic_error("A centered inlet needs the reference Ez at the offset");
   w.step = n; w.offset = k; w.established = true;
   for (unsigned a = 0; a < 3; ++a)
     std::memcpy(w.lower[a].data(), lowerPlanes[a], w.lower[a].size() * sizeof(double));
+  if (w.centered) {
+    const Comp& ez = f[2];
+    const std::size_t cx = cpml->p.cells[0], cy = cpml->p.cells[1];
+    for (std::size_t j = cy + 1; j < g.ny - cy; ++j)
+      for (std::size_t i = cx + 1; i < g.nx - cx; ++i)
+        w.lower[2][j * ez.nx + i] -= ez(i, j, 0) - ezAtOffset[j * ez.nx + i];
+  }
   w.phase = 1;
   w.inletSet = false;
   Halo h;

@@ -1092,6 +1092,55 @@ inline void finishWindowStep(const Grid& g, const V3& newOrigin, std::vector<Par
   carriers.resize(retained);
 }
 
+// ChargeMatchedInlet (charge_matched_inlet.h:16-72) for one rank: after a
+// shift, the new inlet layer's Ez on the interior aperture is set so that
+// div E at the inlet vertices matches the shaped charge of the particles
+// near the inlet: Ez(k=0) = Ez(k=1) + dz (div_xy E(k=1) - rho / eps0), in
+// long double like the reference. origin = the shifted window's origin.
+inline void chargeMatchedInlet(Fields& f, const std::vector<Particle>& particles, const V3& origin, int shape,
+                               bool xBinomial, std::size_t gx, std::size_t gy) {
+  const Grid& g = f.g;
+  if (g.nz < 2 || gx + 1 >= g.nx - gx || gy + 1 >= g.ny - gy)
+    throw std::invalid_argument("Charge-matched inlet needs an interior physical aperture");
+  std::vector<double> rho((g.nx + 1) * (g.ny + 1), 0.0);
+  for (const Particle& p : particles) {
+    const double z = (p.position[2] - origin[2]) / g.dz;
+    const Basis b = basis(z, false, shape);
+    const double wz = 1 < b.first || 1 >= b.first + static_cast<long long>(b.count) ? 0. : b.v[1 - b.first];
+    if (wz == 0.) continue;
+    const double x = (p.position[0] - origin[0]) / g.dx, y = (p.position[1] - origin[1]) / g.dy;
+    if (!std::isfinite(x) || !std::isfinite(y) || !std::isfinite(z) || !std::isfinite(p.charge))
+      throw std::runtime_error("Nonfinite charge-matched inlet source");
+    if (y < 0. || y > static_cast<double>(g.ny)) continue;
+    if (xBinomial ? (x <= -2. || x >= static_cast<double>(g.nx) + 2.) : (x < 0. || x > static_cast<double>(g.nx)))
+      continue;
+    const long long ix = static_cast<long long>(std::floor(x));
+    const std::size_t j = static_cast<std::size_t>(std::floor(y));
+    const double fx = x - ix, fy = y - j, scale = p.charge * (1. / (g.dx * g.dy * g.dz));
+    const std::array<double, 4> wx = xBinomial ? nodeWeights(fx) : std::array<double, 4>{{1. - fx, fx, 0., 0.}};
+    const unsigned nX = xBinomial ? 4 : 2;
+    const long long first = ix - (xBinomial ? 1 : 0);
+    for (unsigned bb = 0; bb < 2; ++bb)
+      for (unsigned a = 0; a < nX; ++a) {
+        const long long i = first + a;
+        if (i >= 0 && i <= static_cast<long long>(g.nx) && j + bb <= g.ny)
+          rho[(j + bb) * (g.nx + 1) + static_cast<std::size_t>(i)] += scale * wx[a] * (bb ? fy : 1. - fy) * wz;
+      }
+  }
+  Comp& ex = f.f[0];
+  Comp& ey = f.f[1];
+  Comp& ez = f.f[2];
+  for (std::size_t j = gy + 1; j < g.ny - gy; ++j)
+    for (std::size_t i = gx + 1; i < g.nx - gx; ++i) {
+      const long double divxy = (static_cast<long double>(ex(i, j, 1)) - ex(i - 1, j, 1)) / g.dx +
+                                (static_cast<long double>(ey(i, j, 1)) - ey(i, j - 1, 1)) / g.dy;
+      const long double r = rho[j * (g.nx + 1) + i], retained = ez(i, j, 1);
+      const double value = static_cast<double>(retained + g.dz * (divxy - r / kEps0));
+      if (!std::isfinite(value)) throw std::runtime_error("Nonfinite charge-matched inlet field");
+      ez(i, j, 0) = value;
+    }
+}
+
 // Simulation::solveConservativeSourceStep + commitConservativeElectric
 // (simulation_source_coupling.cpp:16-65); the particles, the carriers and the
 // boundary counters are restored before every trial (restore(), 24-31).

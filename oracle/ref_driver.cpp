@@ -250,6 +250,7 @@ double ref_fields_poynting(void* h, std::size_t i, std::size_t j, std::size_t k,
 #include "particle_boundary.h"
 #include "source_coupling.h"
 #include "source_coupling_math.h"
+#include "charge_matched_inlet.h"
 
 namespace {
 struct RefCoupling {
@@ -553,6 +554,50 @@ int ref_coupling_step(void* h, const double* origin, double time, unsigned subst
     coupling.applyCurrentSource(-1.);
     rf.window->commitPreparedElectricBoundary(f);
     *iterations = its; *force = frc; *work = wrk;
+  });
+}
+
+// Simulation::completeMovingInletLayer (simulation_moving_window.cpp:312-324)
+// as the window's new-layer completion, over the coupling's particles.
+namespace {
+struct InletContext {
+  RefCoupling* coupling = nullptr;
+  FieldVector<Double> referenceOrigin{0.0};
+  std::unique_ptr<ChargeMatchedInlet> inlet;
+};
+std::map<void*, std::unique_ptr<InletContext>> inletContexts;
+void completeInlet(void* context, EBFieldGrid& f, std::size_t, std::ptrdiff_t offset) {
+  InletContext& c = *static_cast<InletContext*>(context);
+  FieldVector<Double> origin = c.referenceOrigin;
+  origin[2] = std::fma(static_cast<double>(offset), f.geometry().dz, c.referenceOrigin[2]);
+  c.inlet->begin(origin);
+  for (const auto& p : c.coupling->particles) c.inlet->particle(p);
+  c.inlet->complete(f, false);
+}
+}  // namespace
+
+int ref_window_set_charge_match(void* fields, void* coupling, const double* referenceOrigin) {
+  return guarded([&] {
+    RefFields& r = *R(fields);
+    RefCoupling& H = *RC(coupling);
+    std::unique_ptr<InletContext> c(new InletContext);
+    c->coupling = &H;
+    c->referenceOrigin = fv(referenceOrigin);
+    c->inlet.reset(new ChargeMatchedInlet(r.f->geometry(), r.installed.cells[0], r.installed.cells[1], H.quintic,
+                                          false, H.binomial3, H.smoothX));
+    r.window->setNewLayerCompletion(&completeInlet, c.get(), true);
+    inletContexts[fields] = std::move(c);
+  });
+}
+int ref_window_begin_centered(void* h, std::size_t step, std::ptrdiff_t offset, const double* const* lower3,
+                              const double* ezAtOffset) {
+  return guarded([&] {
+    RefFields& r = *R(h);
+    r.staged.clear();
+    for (unsigned a = 0; a < 3; ++a) r.stage(a, offset - 1, step, lower3[a]);
+    r.stage(2, offset, step, ezAtOffset);
+    r.step = step;
+    r.window->beginStep(*r.f, step, offset);
   });
 }
 

@@ -110,3 +110,34 @@ def oracle_case(prefix, shape="binomial3", smooth=True, exit_plane=False, seed=1
     x, u, q, m, ids = particles(seed, exit_plane=exit_plane)
     c.set_particles(x, u, q, m, np.ones(len(q)), ids)
     return f, c
+
+
+def inlet_particles(seed=21, n=48):
+    """Electrons near the inlet that drift up slowly against the window
+    (u_z = -2 in the box, so they stay within the inlet's shape reach for a
+    few steps), inside the transverse aperture."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = DIMS
+    o = window_origin(OFFSET0)
+    x = np.column_stack([o[0] + rng.uniform(CELLS[0] + 2, nx - CELLS[0] - 2, n) * DX,
+                         o[1] + rng.uniform(CELLS[1] + 2, ny - CELLS[1] - 2, n) * DX,
+                         o[2] + rng.uniform(0.3, 4.0, n) * DZ])
+    u = np.column_stack([rng.normal(0, .1, n), rng.normal(0, .1, n), rng.uniform(-2.2, -1.8, n)])
+    macro = 2.0e5 / n
+    q = np.full(n, -bunch.QE * macro)
+    m = np.full(n, bunch.ME * macro)
+    ids = np.arange(1, n + 1, dtype=np.uint64)
+    return x, u, q, m, ids
+
+
+def inlet_oracle_case(prefix, shape="binomial3", smooth=True, seed=21):
+    import harness as H
+    f = H.CpuFields(prefix, *DIMS, DX, DX, DZ, DZ / C_SI, 1)
+    f.install_window(CELLS)
+    H.seeded_fields(f, 7, scale=1e5)
+    c = H.CpuCoupling(f, shape, smooth, cpml_cells=CELLS)
+    c.set_frame(GAMMA_BOOST, 0.0, 0.0, 0.0)
+    x, u, q, m, ids = inlet_particles(seed)
+    c.set_particles(x, u, q, m, np.ones(len(q)), ids)
+    c.enable_charge_match(REF_ORIGIN)
+    return f, c

@@ -178,6 +178,11 @@ class CpuFields:
     def window_advance_magnetic(self):
         self._check(self._fn("window_advance_magnetic")(self.h))
 
+    def window_begin_centered(self, step, offset, lower3, ez_at_offset):
+        arr, keep = _ptr_array(lower3)
+        ez = np.ascontiguousarray(ez_at_offset, dtype=np.float64)
+        self._check(self._fn("window_begin_centered")(self.h, step, offset, arr, _ptr(ez)))
+
     def window_set_inlet(self, offset, step1, ex, ey):
         ex = np.ascontiguousarray(ex, dtype=np.float64)
         ey = np.ascontiguousarray(ey, dtype=np.float64)
@@ -259,6 +264,12 @@ def _bind_particles(lib, prefix):
     f("coupling_get_carriers").argtypes = [vp, _dp, _dp, _dp, _dp, _u64p]
     f("coupling_counters").argtypes = [vp, _u64p, _dp]
     f("coupling_finish_window").argtypes = [vp, _dp]
+    if prefix == "orc":
+        f("window_set_charge_match").argtypes = [vp, C.c_int]
+        f("coupling_complete_inlet").argtypes = [vp, _dp]
+    else:
+        f("window_set_charge_match").argtypes = [vp, vp, _dp]
+    f("window_begin_centered").argtypes = [vp, sz, C.c_ssize_t, C.POINTER(_dp), _dp]
     f("boundary_query").argtypes = [vp, _dp, C.c_int, _dp, _dp, C.c_double, _dp]
 
 
@@ -370,6 +381,19 @@ class CpuCoupling:
         _call(self.prefix, "coupling_counters", self.h, counts.ctypes.data_as(_u64p), _ptr(charges))
         return counts, charges
 
+    def enable_charge_match(self, reference_origin):
+        """Charge-matched inlet completion with a centered normal reference."""
+        if self.prefix == "orc":
+            _call(self.prefix, "window_set_charge_match", self.fields.h, 1)
+        else:
+            _call(self.prefix, "window_set_charge_match", self.fields.h, self.h, _ptr(_v(reference_origin)))
+
+    def complete_inlet(self, new_origin):
+        """The oracle completes the inlet after a shift; the reference build
+        does it inside its shift."""
+        if self.prefix == "orc":
+            _call(self.prefix, "coupling_complete_inlet", self.h, _ptr(_v(new_origin)))
+
     def finish_window(self, new_origin):
         _call(self.prefix, "coupling_finish_window", self.h, _ptr(_v(new_origin)))
 
@@ -452,6 +476,19 @@ def quintic_basis(z):
         for k in range(7):
             v[j + k] += n[j] * c[k]
     return int(cell) - 5, v
+
+
+def shaped_node_weight(shape, z, node):
+    """longitudinal_shape::nodeWeight (longitudinal_shape.h:56) for the
+    linear-binomial3 and compensated quintic shapes."""
+    if shape == "binomial3":
+        cz = np.floor(z)
+        fz = z - cz
+        k0 = int(cz) - 1
+        w = [0.25 * (1 - fz), 0.5 * (1 - fz) + 0.25 * fz, 0.25 * (1 - fz) + 0.5 * fz, 0.25 * fz]
+    else:
+        k0, w = quintic_basis(z)
+    return w[node - k0] if 0 <= node - k0 < len(w) else 0.0
 
 
 def shaped_charge(shape, x, q, dims, h):
