@@ -129,6 +129,7 @@ def lib() -> C.CDLL:
     L.aprl_window_set_charge_match.argtypes = [vp, C.c_int]
     L.aprl_window_begin_step_centered.argtypes = [vp, _sz, C.c_ssize_t, C.POINTER(_dp), _dp]
     L.aprl_window_complete_inlet.argtypes = [vp, vp, vp, _dp]
+    L.aprl_coupling_set_reference_current.argtypes = [vp, _dp, _sz, _sz, C.POINTER(_sz), C.c_double]
     L.aprl_coupling_boundary_counters.argtypes = [vp, u64p, _dp]
     L.aprl_coupling_create.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp)]
     L.aprl_coupling_destroy.argtypes = [vp]
@@ -182,6 +183,7 @@ EXPORTED = [
     "aprl_fields_advance_magnetic", "aprl_fields_advance_electric", "aprl_fields_synchronize",
     "aprl_window_install", "aprl_window_begin_step", "aprl_window_set_inlet", "aprl_window_shift",
     "aprl_window_set_charge_match", "aprl_window_begin_step_centered", "aprl_window_complete_inlet",
+    "aprl_coupling_set_reference_current",
     "aprl_profile_enable", "aprl_profile_query", "aprl_profile_reset",
     "aprl_particles_create", "aprl_particles_destroy", "aprl_particles_upload", "aprl_particles_download",
     "aprl_particles_count", "aprl_particles_download_states", "aprl_coupling_boundary_counters",

@@ -117,6 +117,17 @@ class ConservativeSourceCoupling:
         cells = (C.c_size_t * 3)(*cpml_cells)
         check(lib().aprl_coupling_set_particle_boundary(self.handle, cells, exit_lab_z))
 
+    def setReferenceCurrent(self, table=None, guards=(0, 0), translation_beta=0.0):
+        """TransverseReferenceCurrent with its prepared table (rows, points);
+        None disables it."""
+        if table is None:
+            check(lib().aprl_coupling_set_reference_current(self.handle, None, 0, 0, None, 0.0))
+            return
+        t = np.ascontiguousarray(table, dtype=np.float64)
+        g = (C.c_size_t * 2)(*[int(v) for v in guards])
+        check(lib().aprl_coupling_set_reference_current(self.handle, dptr(t.ravel()), t.shape[0], t.shape[1], g,
+                                                        float(translation_beta)))
+
     def boundaryCounters(self):
         """Cumulative (counts[20], charges[20]): CPML entries per face (0-5),
         outer exits of particles (6-11) and of carriers (12-17), window exits

@@ -200,6 +200,43 @@ aprl_status aprl_particles_download_states(aprl_particles* particles, int* state
   });
 }
 
+aprl_status aprl_coupling_set_reference_current(aprl_coupling* coupling, const double* table, size_t rows,
+                                               size_t count, const size_t* guards, double translation_beta) {
+  return guarded([&] {
+    if (!coupling) throw std::invalid_argument("Null coupling");
+    CouplingState& C = coupling->c;
+    if (!table) {
+      C.refCurrent = false;
+      return;
+    }
+    if (!guards) throw std::invalid_argument("Null reference-current guards");
+    const aprl_grid_geometry& g = coupling->fields->fp32 ? coupling->fields->s32.g : coupling->fields->s.g;
+    if (!guards[0] || !guards[1]) throw std::invalid_argument("Reference surface current requires transverse-only CPML");
+    if (2 * guards[0] >= g.nx || 2 * guards[1] >= g.ny) throw std::invalid_argument("Reference-current guards leave no aperture");
+    // TransverseReferenceCurrent points() (transverse_reference_current.cpp:18-26)
+    std::vector<int> ij;
+    const size_t x0 = guards[0], x1 = g.nx - guards[0], y0 = guards[1], y1 = g.ny - guards[1];
+    for (size_t i = x0; i <= x1; ++i)
+      for (size_t j : {y0, y1}) { ij.push_back(static_cast<int>(i)); ij.push_back(static_cast<int>(j)); }
+    for (size_t j = y0 + 1; j < y1; ++j)
+      for (size_t i : {x0, x1}) { ij.push_back(static_cast<int>(i)); ij.push_back(static_cast<int>(j)); }
+    const size_t points = ij.size() / 2;
+    if (count != points) throw std::invalid_argument("Reference surface-current table does not match the interface points");
+    if (rows < 1) throw std::invalid_argument("Reference surface-current table has no rows");
+    for (size_t t = 0; t < rows * points; ++t)
+      if (!std::isfinite(table[t])) throw std::runtime_error("Nonfinite equivalent boundary current");
+    if (!std::isfinite(translation_beta)) throw std::invalid_argument("Nonfinite translation beta");
+    C.refTable.resize(rows * points);
+    C.refPoints.resize(ij.size());
+    hipCheck(hipMemcpy(C.refTable.get(), table, rows * points * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+    hipCheck(hipMemcpy(C.refPoints.get(), ij.data(), ij.size() * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+    C.refRows = rows;
+    C.refCount = points;
+    C.refBeta = translation_beta;
+    C.refCurrent = true;
+  });
+}
+
 aprl_status aprl_window_complete_inlet(aprl_fields* fields, aprl_coupling* coupling, aprl_particles* particles,
                                        const double* origin) {
   return guarded([&] {

@@ -1212,6 +1212,9 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
       F.footprint.ty = kTileY;
       F.footprint.tz = kTileZ;
     }
+    // TransverseReferenceCurrent::apply after the trial's deposit
+    // (simulation_source_coupling.cpp:44)
+    referenceCurrentTrial(C, F, C.esum[2].get(), C.estar[2].get(), true, nChunks > 0, ntx, nty, ntz);
     {
       ScopedKernel sk(F.profiler, st, "residual");
       if (nChunks && quintic)

@@ -169,6 +169,13 @@ struct CouplingState {
   PinnedHost pinned;
   Event sorted, stepDone;
   std::shared_ptr<SplitStepContext> split;
+  // reference surface current (TransverseReferenceCurrent): rows x points
+  // table, target (i, j) pairs, translation beta
+  bool refCurrent = false;
+  DeviceArray<double> refTable;
+  DeviceArray<int> refPoints;
+  std::size_t refRows = 0, refCount = 0;
+  double refBeta = 0.0;
   // the last coupled step: its store and origin (its start positions stay
   // in the store's end arrays until the next step)
   struct {
@@ -215,6 +222,13 @@ void commitEvents(CouplingState& C, hipStream_t st, int n, const int* evt, int* 
 // Cropped-carrier counters of the sorted tail [lo, n).
 void cropCounters(CouplingState& C, hipStream_t st, int lo, int n, const int* state, const void* q, const void* w,
                   bool fp32);
+// Reference surface current of one trial on the Jz ring nodes: Jz = (the
+// trial's particle current on active tiles, else 0) + the interpolated
+// table value; with writeEsum the E_cand + E_old sums of the active ring
+// nodes follow. Checks the table range and leaves J for a full clear.
+template <class R>
+void referenceCurrentTrial(CouplingState& C, FieldStateT<R>& F, R* esumZ, const R* estarZ, bool writeEsum,
+                           bool anyActive, int ntx, int nty, int ntz);
 // Live entries of a store whose last commit may have removed some.
 std::size_t liveEntries(std::size_t count, const DeviceArray<unsigned>& removed, hipStream_t st);
 // Stable sort of (tile key, particle index) pairs on the low `bits` key

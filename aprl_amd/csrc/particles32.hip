@@ -1037,7 +1037,10 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
       kPushSplit<<<ceilDiv(n, 128), 128, 0, st>>>(A, S, o[0], o[1], o[2], errT);
       launchCheck("push");
     }
-    if (!nChunks) continue;
+    if (!nChunks) {
+      referenceCurrentTrial(C, F, C.esum32[2].get(), C.estar32[2].get(), false, false, ntx, nty, ntz);
+      continue;
+    }
     {
       ScopedKernel sk(F.profiler, st, "deposit");
       if (quintic)
@@ -1075,6 +1078,8 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
     F.footprint.tx = kTileX;
     F.footprint.ty = kTileY;
     F.footprint.tz = kTileZ;
+    // TransverseReferenceCurrent::apply after the trial's deposit
+    referenceCurrentTrial(C, F, C.esum32[2].get(), C.estar32[2].get(), esum, true, ntx, nty, ntz);
     if (last && !verify) break;
     {
       ScopedKernel sk(F.profiler, st, "residual");

@@ -210,6 +210,17 @@ aprl_status aprl_coupling_add_magnet(aprl_coupling* coupling, const aprl_magnet*
  * reflection; particle_boundary.cpp:176-241). */
 aprl_status aprl_coupling_set_particle_boundary(aprl_coupling* coupling, const size_t* cpml_cells,
                                                 double exit_lab_z);
+/* reference_surface_current: TransverseReferenceCurrent
+ * (transverse_reference_current.cpp:18-26, 104-120) with its prepared table
+ * in memory instead of a file: table[row * count + p] is the equivalent
+ * surface current at interface point p for reference z cell `row`, points
+ * in the reference's order (for i = g0..nx-g0: (i, g1), (i, ny-g1); then for
+ * j = g1+1..ny-g1-1: (g0, j), (nx-g0, j); count = 2 (nx - 2 g0 + 1) +
+ * 2 (ny - 2 g1 - 1)). Every Picard trial and the commit add to Jz the table
+ * interpolated at z = offset + k - translation_beta (step + 1), offset and
+ * step being the window's current ones. table = NULL disables it. */
+aprl_status aprl_coupling_set_reference_current(aprl_coupling* coupling, const double* table, size_t rows,
+                                               size_t count, const size_t* guards, double translation_beta);
 /* Simulation::completeMovingInletLayer (simulation_moving_window.cpp:
  * 312-324) after aprl_window_shift to the window origin `origin`: the new
  * layer's Ez on the interior aperture := Ez(k=1) + dz (div_xy E(k=1) -

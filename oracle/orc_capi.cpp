@@ -162,6 +162,7 @@ struct CouplingHandle {
   std::vector<orc::Particle> particles;
   std::vector<orc::Carrier> carriers;
   orc::BoundaryCounters counters;
+  orc::RefCurrent ref;
   std::size_t cpmlCells[3] = {0, 0, 0};
   // Particle-boundary geometry: the grid's CPML grading with the particle
   // cells above.
@@ -249,7 +250,7 @@ int orc_coupling_step(void* h, const double* origin, double time, unsigned subst
     pb.window = std::isfinite(exitLabZ);
     pb.exitLabZ = exitLabZ;
     const orc::StepResult r =
-        orc::sourceStep(f, H.c, H.particles, H.carriers, H.counters, vec(origin), time, substeps, pb);
+        orc::sourceStep(f, H.c, H.particles, H.carriers, H.counters, vec(origin), time, substeps, pb, &H.ref);
     *iterations = r.iterations; *force = r.force; *work = r.work;
   });
 }
@@ -307,6 +308,31 @@ int orc_coupling_complete_inlet(void* h, const double* newOrigin) {
     orc::chargeMatchedInlet(f, H.particles, vec(newOrigin), H.c.shape, H.c.smoothX, f.cpml->p.cells[0],
                             f.cpml->p.cells[1]);
   });
+}
+
+// Reference surface current: table of rows x points (the point order of
+// TransverseReferenceCurrent's points() for guards gx, gy), translation beta.
+int orc_coupling_set_reference_current(void* h, const double* table, std::size_t rows, std::size_t gx,
+                                       std::size_t gy, double beta) {
+  return guarded([&] {
+    CouplingHandle& H = *CH(h);
+    H.ref.setPoints(H.c.fields->g, gx, gy);
+    H.ref.rows = rows;
+    H.ref.table.assign(table, table + rows * H.ref.points.size());
+    H.ref.beta = beta;
+    H.ref.enabled = true;
+  });
+}
+std::size_t orc_reference_current_points(void* h, std::size_t gx, std::size_t gy, std::size_t* ij) {
+  orc::RefCurrent r;
+  try {
+    r.setPoints(CH(h)->c.fields->g, gx, gy);
+  } catch (...) {
+    return 0;
+  }
+  if (ij)
+    for (std::size_t p = 0; p < r.points.size(); ++p) { ij[2 * p] = r.points[p].i; ij[2 * p + 1] = r.points[p].j; }
+  return r.points.size();
 }
 
 // The particle part of finishMovingWindowStep after a shift to new_origin.

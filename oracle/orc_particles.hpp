@@ -1092,6 +1092,48 @@ inline void finishWindowStep(const Grid& g, const V3& newOrigin, std::vector<Par
   carriers.resize(retained);
 }
 
+// TransverseReferenceCurrent (transverse_reference_current.cpp:18-26,
+// 104-120) without its file cache: the prepared surface current of the
+// reference field at the four transverse interfaces of the aperture, one
+// table row per reference z cell, added to Jz every trial and at the
+// commit, interpolated linearly at z = offset + k - beta (step + 1).
+struct RefCurrent {
+  struct Point { std::size_t i, j; };
+  std::vector<Point> points;
+  std::vector<double> table;  // rows x points
+  std::size_t rows = 0;
+  double beta = 0.0;
+  bool enabled = false;
+
+  // points(), transverse_reference_current.cpp:18-26
+  void setPoints(const Grid& g, std::size_t gx, std::size_t gy) {
+    if (!gx || !gy) throw std::invalid_argument("Reference surface current requires transverse-only CPML");
+    points.clear();
+    const std::size_t x0 = gx, x1 = g.nx - gx, y0 = gy, y1 = g.ny - gy;
+    for (std::size_t i = x0; i <= x1; ++i)
+      for (std::size_t j : {y0, y1}) points.push_back({i, j});
+    for (std::size_t j = y0 + 1; j < y1; ++j)
+      for (std::size_t i : {x0, x1}) points.push_back({i, j});
+  }
+  // apply, transverse_reference_current.cpp:104-120 (no work audit)
+  void apply(Fields& f, std::ptrdiff_t offset, std::size_t step) const {
+    const std::size_t nz = f.g.nz, count = points.size();
+    Comp& jz = f.f[8];
+    for (std::size_t k = 0; k < nz; ++k) {
+      const double z = static_cast<double>(offset) + k - beta * static_cast<double>(step + 1);
+      if (!std::isfinite(z) || z < 0. || z > rows - 1) throw std::out_of_range("Reference surface-current query outside domain");
+      const auto lo = static_cast<std::size_t>(std::floor(z)), hi = std::min<std::size_t>(lo + 1, rows - 1);
+      const double fraction = z - lo;
+      const double* a = table.data() + lo * count;
+      const double* b = table.data() + hi * count;
+      for (std::size_t p = 0; p < count; ++p) {
+        const double value = fraction == 0. ? a[p] : (1. - fraction) * a[p] + fraction * b[p];
+        jz(points[p].i, points[p].j, k) += value;
+      }
+    }
+  }
+};
+
 // ChargeMatchedInlet (charge_matched_inlet.h:16-72) for one rank: after a
 // shift, the new inlet layer's Ez on the interior aperture is set so that
 // div E at the inlet vertices matches the shaped charge of the particles
@@ -1146,7 +1188,7 @@ inline void chargeMatchedInlet(Fields& f, const std::vector<Particle>& particles
 // boundary counters are restored before every trial (restore(), 24-31).
 inline StepResult sourceStep(Fields& f, Coupling& c, std::vector<Particle>& particles, std::vector<Carrier>& carriers,
                              BoundaryCounters& counters, const V3& origin, double time, unsigned substeps,
-                             const ParticleBoundary& pb) {
+                             const ParticleBoundary& pb, const RefCurrent* ref = nullptr) {
   c.begin(f, origin, particles);
   f.window->electricPreview = true;
   f.advanceElectric();
@@ -1165,6 +1207,8 @@ inline StepResult sourceStep(Fields& f, Coupling& c, std::vector<Particle>& part
     counters = originalCounters;
     c.beginTrial();
     pushDepositAll(c, particles, carriers, counters, time, f.g.dt, substeps, pb, origin);
+    // applyMovingTransverseReferenceCurrent (simulation_source_coupling.cpp:44)
+    if (ref && ref->enabled) ref->apply(f, f.window->offset, f.window->step);
     c.applyCurrentSource(-1.);
     const auto res = c.localResidual();
     r.force = res[0];
@@ -1186,6 +1230,8 @@ inline StepResult sourceStep(Fields& f, Coupling& c, std::vector<Particle>& part
   c.replay = true;
   pushDepositAll(c, particles, carriers, counters, time, f.g.dt, substeps, pb, origin);
   c.replay = false;
+  // the solve loop's reference current before the commit (simulation.cpp:548-549)
+  if (ref && ref->enabled) ref->apply(f, f.window->offset, f.window->step);
   // commitConservativeElectric
   c.swapElectric();
   f.undoPreview();

@@ -267,6 +267,9 @@ def _bind_particles(lib, prefix):
     if prefix == "orc":
         f("window_set_charge_match").argtypes = [vp, C.c_int]
         f("coupling_complete_inlet").argtypes = [vp, _dp]
+        f("coupling_set_reference_current").argtypes = [vp, _dp, sz, sz, sz, C.c_double]
+        f("reference_current_points").argtypes = [vp, sz, sz, C.POINTER(sz)]
+        f("reference_current_points").restype = sz
     else:
         f("window_set_charge_match").argtypes = [vp, vp, _dp]
     f("window_begin_centered").argtypes = [vp, sz, C.c_ssize_t, C.POINTER(_dp), _dp]
@@ -393,6 +396,19 @@ class CpuCoupling:
         does it inside its shift."""
         if self.prefix == "orc":
             _call(self.prefix, "coupling_complete_inlet", self.h, _ptr(_v(new_origin)))
+
+    def set_reference_current(self, table, guards, beta):
+        t = _v(table)
+        _call(self.prefix, "coupling_set_reference_current", self.h, _ptr(t.ravel()), t.shape[0], guards[0],
+              guards[1], beta)
+
+    def reference_points(self, guards):
+        lib = load_particles(self.prefix)
+        n = getattr(lib, self.prefix + "_reference_current_points")(self.h, guards[0], guards[1], None)
+        ij = np.zeros(2 * n, dtype=np.uint64)
+        getattr(lib, self.prefix + "_reference_current_points")(self.h, guards[0], guards[1],
+                                                               ij.ctypes.data_as(C.POINTER(C.c_size_t)))
+        return ij.reshape(n, 2).astype(np.int64)
 
     def finish_window(self, new_origin):
         _call(self.prefix, "coupling_finish_window", self.h, _ptr(_v(new_origin)))
