@@ -163,6 +163,7 @@ struct CouplingHandle {
   std::vector<orc::Carrier> carriers;
   orc::BoundaryCounters counters;
   orc::RefCurrent ref;
+  orc::HeldRelease held;
   std::size_t cpmlCells[3] = {0, 0, 0};
   // Particle-boundary geometry: the grid's CPML grading with the particle
   // cells above.
@@ -247,12 +248,33 @@ int orc_coupling_step(void* h, const double* origin, double time, unsigned subst
     orc::Fields& f = *H.c.fields;
     orc::ParticleBoundary pb;
     pb.set(f.g, vec(origin), H.boundaryParams());
-    pb.window = std::isfinite(exitLabZ);
+    pb.window = static_cast<bool>(f.window);
     pb.exitLabZ = exitLabZ;
     const orc::StepResult r =
-        orc::sourceStep(f, H.c, H.particles, H.carriers, H.counters, vec(origin), time, substeps, pb, &H.ref);
+        orc::sourceStep(f, H.c, H.particles, H.carriers, H.counters, vec(origin), time, substeps, pb, &H.ref, H.held);
     *iterations = r.iterations; *force = r.force; *work = r.work;
   });
+}
+
+// Held (frozen) state of the particle list in set_particles order, the
+// activation plane (lab z) of config reference.activation_plane_z.
+int orc_coupling_set_held(void* h, const int* frozen, const double* beta) {
+  return guarded([&] {
+    auto& P = CH(h)->particles;
+    for (std::size_t i = 0; i < P.size(); ++i) {
+      P[i].frozen = frozen[i] != 0;
+      P[i].betaHeld = beta[i];
+    }
+  });
+}
+int orc_coupling_get_frozen(void* h, int* frozen) {
+  return guarded([&] {
+    const auto& P = CH(h)->particles;
+    for (std::size_t i = 0; i < P.size(); ++i) frozen[i] = P[i].frozen ? 1 : 0;
+  });
+}
+int orc_coupling_set_activation(void* h, double labZ) {
+  return guarded([&] { CH(h)->held.activationLabZ = labZ; });
 }
 
 int orc_coupling_get_ids(void* h, std::uint64_t* id) {

@@ -504,6 +504,9 @@ struct Particle {
   V3 position = v3(), u = v3();
   double charge = 0, mass = 0, weight = 1;
   std::uint64_t id = 0;
+  // upstream lab-plane source state held until activation (eb_particles.h:21-22)
+  bool frozen = false;
+  double betaHeld = 0.0;
 };
 
 // ConservativeSourceCoupling (source_coupling.cpp) for one rank, with the
@@ -822,7 +825,7 @@ struct ParticleBoundary {
   // Simulation::firstMovingWindowEntry, simulation_moving_window.cpp:380-395.
   bool windowEntry(const V3& s, const V3& e, double tStart, double tEnd, const Boost& frame, Hit& hit) const {
     bool present = firstEntry(s, e, hit);
-    if (!window) return present;
+    if (!window || !std::isfinite(exitLabZ)) return present;
     const double a = frame.labZFromBoxZT(s[2], tStart) - exitLabZ;
     const double b = frame.labZFromBoxZT(e[2], tEnd) - exitLabZ;
     if (a >= 0.) throw std::runtime_error("Active particle started beyond trusted moving-window exit");
@@ -982,14 +985,22 @@ inline void carrierSegment(Coupling& c, const ParticleBoundary& pb, Carrier carr
 }
 
 // pushDepositAndMigrate (simulation.cpp:4167-5605) for one rank: the carriers
-// first (4407-4425), then every active particle pushed and its segment ended
-// at the first boundary event (processActiveSegment, 4555-4917): CPML entry
-// or the moving-window exit plane (the particle becomes a carrier from the
-// hit), or an outer-face exit (the particle is removed). Retirement planes,
-// held particles and ballistic transverse motion are not part of this path.
+// first (4407-4425), then every particle prepared (prepareParticle,
+// 4919-5035: held particles move z += c dt beta_held with their transverse
+// motion held and are released where their lab z reaches the activation
+// plane, the rest of the step pushed from the release point) and its
+// segment ended at the first boundary event (processActiveSegment,
+// 4555-4917): CPML entry or the moving-window exit plane (the particle
+// becomes a carrier from the hit), or an outer-face exit (the particle is
+// removed). Held particles upstream of the window deposit only their shaped
+// support (depositFrozenSourceSupport) until they cross the inlet
+// (5121-5155). Retirement planes, ballistic transverse motion and the
+// multi-rank transfers are not part of this path.
+struct HeldRelease { double activationLabZ = std::numeric_limits<double>::infinity(); };
+
 inline void pushDepositAll(Coupling& c, std::vector<Particle>& particles, std::vector<Carrier>& carriers,
                            BoundaryCounters& k, double time, double dt, unsigned substeps,
-                           const ParticleBoundary& pb, const V3& windowOrigin) {
+                           const ParticleBoundary& pb, const V3& windowOrigin, const HeldRelease& held = HeldRelease()) {
   std::vector<Carrier> retainedCarriers;
   retainedCarriers.reserve(carriers.size());
   for (const Carrier& original : carriers) {
@@ -998,20 +1009,81 @@ inline void pushDepositAll(Coupling& c, std::vector<Particle>& particles, std::v
     pmv(carrier.position, dt, carrier.velocity);
     carrierSegment(c, pb, carrier, start, 0.0, retainedCarriers, k);
   }
-  std::vector<Particle> prepared = particles;
-  for (auto& p : prepared) {
-    if (pb.window && p.position[2] < windowOrigin[2])
+  struct Prepared { Particle particle; bool hasActive = false; double activeStartTime = 0; Particle activeStart; };
+  std::vector<Prepared> prepared(particles.size());
+  for (std::size_t n = 0; n < particles.size(); ++n) {
+    Prepared& r = prepared[n];
+    r.particle = particles[n];
+    Particle& p = r.particle;
+    const V3 start = p.position;
+    if (pb.window && start[2] < windowOrigin[2] && !p.frozen)
       throw std::runtime_error("An active particle is upstream of the moving window; the inlet guard did not contain its release event");
-    c.push(p, time, dt, substeps);
+    r.activeStart = p;
+    r.activeStartTime = time;
+    r.hasActive = !p.frozen;
+    if (p.frozen) {
+      const double startLabZ = c.frame.labZFromBoxZT(start[2], time);
+      p.position[2] += kC * dt * p.betaHeld;
+      const double endLabZ = c.frame.labZFromBoxZT(p.position[2], time + dt);
+      if (endLabZ >= held.activationLabZ) {
+        if (pb.window && start[2] < windowOrigin[2])
+          throw std::runtime_error("A pending upstream held particle would release before a complete in-window step; check the moving inlet guard");
+        const double f = startLabZ >= held.activationLabZ
+                             ? 0.0
+                             : std::max(0.0, std::min(1.0, (held.activationLabZ - startLabZ) / (endLabZ - startLabZ)));
+        p.position = start;
+        p.position[2] += kC * f * dt * p.betaHeld;
+        p.frozen = false;
+        r.hasActive = true;
+        r.activeStart = p;
+        r.activeStartTime = time + f * dt;
+        const double remaining = (1.0 - f) * dt;
+        if (remaining > 0.0) c.push(p, r.activeStartTime, remaining, substeps);
+      }
+    } else {
+      c.push(p, time, dt, substeps);
+    }
   }
   const double tolerance = 64.0 * kEpsilon;
   std::vector<Particle> retained;
   retained.reserve(prepared.size());
   for (std::size_t n = 0; n < particles.size(); ++n) {
-    const V3 start = particles[n].position;
-    const Particle& p = prepared[n];
+    V3 start = particles[n].position;
+    double startFraction = 0.0;
+    const Particle& p = prepared[n].particle;
+    if (pb.window && start[2] < windowOrigin[2]) {
+      if (!particles[n].frozen || !p.frozen || prepared[n].hasActive)
+        throw std::runtime_error("Moving-window upstream admission requires an unreleased held particle on rank zero");
+      if (p.position[2] < windowOrigin[2]) {
+        c.deposit(start, p.position, p.charge);  // depositFrozenSourceSupport
+        retained.push_back(p);
+        continue;
+      }
+      const double fraction = (windowOrigin[2] - start[2]) / (p.position[2] - start[2]);
+      if (!(fraction >= 0.0 && fraction <= 1.0) || !std::isfinite(fraction))
+        throw std::runtime_error("Invalid moving-window admission fraction");
+      for (unsigned a = 0; a < 3; ++a) start[a] += fraction * (p.position[a] - start[a]);
+      start[2] = windowOrigin[2];
+      startFraction = fraction;
+      c.deposit(particles[n].position, start, p.charge);  // the support's prefix
+    }
+    const bool kink = particles[n].frozen && prepared[n].hasActive && prepared[n].activeStartTime > time &&
+                      prepared[n].activeStartTime < time + dt;
+    if (kink) {
+      // processActiveSegment's frozen leg (4676-4746)
+      const V3& knot = prepared[n].activeStart.position;
+      const double kinkFraction = (prepared[n].activeStartTime - time) / dt;
+      Hit h1, h2;
+      if (pb.firstEntry(start, knot, h1) || pb.firstExit(start, knot, h2))
+        throw std::runtime_error("Held-transverse activation crossed a particle boundary during the frozen part of a field step; enable mesh.ballistic_outside_transverse for boundary injection");
+      c.validatePosition(start);
+      c.validatePosition(knot);
+      c.deposit(start, knot, p.charge);
+      start = knot;
+      startFraction = kinkFraction;
+    }
     Hit cpmlHit, outerHit;
-    const bool entersCpml = pb.window ? pb.windowEntry(start, p.position, time, time + dt, c.frame, cpmlHit)
+    const bool entersCpml = pb.window ? pb.windowEntry(start, p.position, time + startFraction * dt, time + dt, c.frame, cpmlHit)
                                       : pb.firstEntry(start, p.position, cpmlHit);
     const bool exitsOuter = pb.firstExit(start, p.position, outerHit);
     enum { Retain, EnterCpml, ExitOuter } action = Retain;
@@ -1033,6 +1105,7 @@ inline void pushDepositAll(Coupling& c, std::vector<Particle>& particles, std::v
     c.validatePosition(end);
     if (!std::isfinite(p.charge)) throw std::invalid_argument("Deposited particle charge must be finite");
     c.deposit(start, end, p.charge);
+    const double globalEnd = startFraction + localEnd * (1.0 - startFraction);
     if (action == EnterCpml) {
       if (pb.window && selected.face == 5) {
         ++k.exits;
@@ -1050,7 +1123,7 @@ inline void pushDepositAll(Coupling& c, std::vector<Particle>& particles, std::v
       carrier.charge = p.charge;
       carrier.weight = 1.0;
       carrier.id = p.id;
-      carrierSegment(c, pb, carrier, selected.position, localEnd, retainedCarriers, k);
+      carrierSegment(c, pb, carrier, selected.position, globalEnd, retainedCarriers, k);
     } else if (action == ExitOuter) {
       ++k.direct[selected.face];
       k.directQ[selected.face] += p.charge;
@@ -1069,7 +1142,7 @@ inline void pushDepositAll(Coupling& c, std::vector<Particle>& particles, std::v
 inline void finishWindowStep(const Grid& g, const V3& newOrigin, std::vector<Particle>& particles,
                              std::vector<Carrier>& carriers, BoundaryCounters& k) {
   for (const auto& p : particles) {
-    if (p.position[2] < newOrigin[2]) throw std::runtime_error("Active particle fell behind moving inlet");
+    if (p.position[2] < newOrigin[2] && !p.frozen) throw std::runtime_error("Active particle fell behind moving inlet");
     // migrateWindowObjects with ownerRank (simulation.cpp:4068-4083) on one
     // rank: a particle beyond the upper face has no owner.
     const double coordinate = (p.position[2] - newOrigin[2]) / g.dz;
@@ -1188,7 +1261,8 @@ inline void chargeMatchedInlet(Fields& f, const std::vector<Particle>& particles
 // boundary counters are restored before every trial (restore(), 24-31).
 inline StepResult sourceStep(Fields& f, Coupling& c, std::vector<Particle>& particles, std::vector<Carrier>& carriers,
                              BoundaryCounters& counters, const V3& origin, double time, unsigned substeps,
-                             const ParticleBoundary& pb, const RefCurrent* ref = nullptr) {
+                             const ParticleBoundary& pb, const RefCurrent* ref = nullptr,
+                             const HeldRelease& held = HeldRelease()) {
   c.begin(f, origin, particles);
   f.window->electricPreview = true;
   f.advanceElectric();
@@ -1206,7 +1280,7 @@ inline StepResult sourceStep(Fields& f, Coupling& c, std::vector<Particle>& part
     carriers = originalCarriers;
     counters = originalCounters;
     c.beginTrial();
-    pushDepositAll(c, particles, carriers, counters, time, f.g.dt, substeps, pb, origin);
+    pushDepositAll(c, particles, carriers, counters, time, f.g.dt, substeps, pb, origin, held);
     // applyMovingTransverseReferenceCurrent (simulation_source_coupling.cpp:44)
     if (ref && ref->enabled) ref->apply(f, f.window->offset, f.window->step);
     c.applyCurrentSource(-1.);
@@ -1228,7 +1302,7 @@ inline StepResult sourceStep(Fields& f, Coupling& c, std::vector<Particle>& part
   counters = originalCounters;
   c.swapElectric();
   c.replay = true;
-  pushDepositAll(c, particles, carriers, counters, time, f.g.dt, substeps, pb, origin);
+  pushDepositAll(c, particles, carriers, counters, time, f.g.dt, substeps, pb, origin, held);
   c.replay = false;
   // the solve loop's reference current before the commit (simulation.cpp:548-549)
   if (ref && ref->enabled) ref->apply(f, f.window->offset, f.window->step);

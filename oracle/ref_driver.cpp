@@ -268,6 +268,7 @@ struct RefCoupling {
   std::string dir;
   bool quintic = false, binomial3 = false, smoothX = false;
   EBCPMLParameters cpml;
+  Double activationLabZ = std::numeric_limits<Double>::infinity();
   ~RefCoupling() {
     c.reset();
     if (!dir.empty()) {
@@ -361,26 +362,57 @@ int ref_coupling_step(void* h, const double* origin, double time, unsigned subst
     for (int a = 0; a < 3; ++a) boundaryParams.cells[a] = H.cpml.cells[a];
     ParticleCPMLRegion cpmlRegion(g, o, boundaryParams);
     ParticleOpenBoundary outer(g, o, g, o, 0);
-    const bool window = std::isfinite(exitLabZ);
+    const bool window = static_cast<bool>(rf.window);
     auto pushDepositAll = [&]() {
       ChargeConservingCurrentDepositor depositor(f, o, H.c->longitudinalCurrent());
       std::vector<RelativisticParticleSI> prepared = H.particles;
-      for (auto& p : prepared)
-        if (window && p.position[2] < o[2])
+      // prepareParticle (simulation.cpp:4919-5035), hold-transverse release
+      std::vector<char> hasActive(prepared.size(), 0);
+      std::vector<Double> activeTime(prepared.size(), time);
+      std::vector<FieldVector<Double>> activePosition(prepared.size(), FieldVector<Double>(0.0));
+      auto prepare = [&](std::size_t n) {
+        RelativisticParticleSI& particle = prepared[n];
+        const FieldVector<Double> start(particle.position);
+        if (window && start[2] < o[2] && !particle.frozen)
           throw std::runtime_error("An active particle is upstream of the moving window; the inlet guard did not contain its release event");
+        hasActive[n] = !particle.frozen;
+        activePosition[n] = particle.position;
+        if (particle.frozen) {
+          const Double betaHeld = particle.frozenBetaZ;
+          const Double startLabZ = H.frame.labZFromBoxZT(start[2], time);
+          particle.position[2] += SI::c * g.dt * betaHeld;
+          const Double endLabZ = H.frame.labZFromBoxZT(particle.position[2], time + g.dt);
+          if (endLabZ >= H.activationLabZ) {
+            if (window && start[2] < o[2])
+              throw std::runtime_error("A pending upstream held particle would release before a complete in-window step; check the moving inlet guard");
+            const Double crossingFraction = startLabZ >= H.activationLabZ ? 0.0 :
+              std::max(0.0, std::min(1.0, (H.activationLabZ - startLabZ) / (endLabZ - startLabZ)));
+            particle.position = start;
+            particle.position[2] += SI::c * crossingFraction * g.dt * betaHeld;
+            particle.frozen = false;
+            hasActive[n] = 1;
+            activePosition[n] = particle.position;
+            activeTime[n] = time + crossingFraction * g.dt;
+            const Double remaining = (1.0 - crossingFraction) * g.dt;
+            if (remaining > 0.0)
+              H.c->push(particle, H.sources, H.frame, activeTime[n], remaining, substeps, nullptr, nullptr, nullptr);
+          }
+        } else {
+          H.c->push(particle, H.sources, H.frame, time, g.dt, substeps, nullptr, nullptr, nullptr);
+        }
+      };
       // Preparation in batches of 2048 with OpenMP, as simulation.cpp:5041-5080.
       const std::ptrdiff_t total = static_cast<std::ptrdiff_t>(prepared.size());
       for (std::ptrdiff_t batch = 0; batch < total; batch += 2048) {
         const std::ptrdiff_t count = std::min<std::ptrdiff_t>(2048, total - batch);
         if (computeThreadCount() == 1 || count < 32) {
-          for (std::ptrdiff_t k = 0; k < count; ++k)
-            H.c->push(prepared[batch + k], H.sources, H.frame, time, g.dt, substeps, nullptr, nullptr, nullptr);
+          for (std::ptrdiff_t k = 0; k < count; ++k) prepare(static_cast<std::size_t>(batch + k));
         } else {
           std::exception_ptr error;
 #pragma omp parallel for schedule(static) num_threads(computeThreadCount())
           for (std::ptrdiff_t k = 0; k < count; ++k) {
             try {
-              H.c->push(prepared[batch + k], H.sources, H.frame, time, g.dt, substeps, nullptr, nullptr, nullptr);
+              prepare(static_cast<std::size_t>(batch + k));
             } catch (...) {
 #pragma omp critical(aprl_ref_prepare_error)
               if (!error) error = std::current_exception();
@@ -435,13 +467,41 @@ int ref_coupling_step(void* h, const double* origin, double time, unsigned subst
       const Double tolerance = 64.0 * std::numeric_limits<Double>::epsilon();
       std::vector<RelativisticParticleSI> retained;
       for (std::size_t n = 0; n < H.particles.size(); ++n) {
-        const FieldVector<Double> start(H.particles[n].position);
+        FieldVector<Double> start(H.particles[n].position);
+        Double startFraction = 0.0;
         const RelativisticParticleSI& p = prepared[n];
+        // moving-window upstream admission (simulation.cpp:5121-5155)
+        if (window && start[2] < o[2]) {
+          if (!H.particles[n].frozen || !p.frozen || hasActive[n])
+            throw std::runtime_error("Moving-window upstream admission requires an unreleased held particle on rank zero");
+          if (p.position[2] < o[2]) {
+            H.c->depositFrozenSourceSupport(start, p.position, p.charge, false);
+            retained.push_back(p);
+            continue;
+          }
+          const Double fraction = (o[2] - start[2]) / (p.position[2] - start[2]);
+          if (!(fraction >= 0.0 && fraction <= 1.0) || !std::isfinite(fraction))
+            throw std::runtime_error("Invalid moving-window admission fraction");
+          for (unsigned axis = 0; axis < 3; ++axis) start[axis] += fraction * (p.position[axis] - start[axis]);
+          start[2] = o[2];
+          startFraction = fraction;
+          H.c->depositFrozenSourceSupport(H.particles[n].position, start, p.charge, false);
+        }
+        const bool kink = H.particles[n].frozen && hasActive[n] && activeTime[n] > time && activeTime[n] < time + g.dt;
+        if (kink) {
+          // processActiveSegment's frozen leg (simulation.cpp:4676-4746)
+          ParticleBoundaryHit h1, h2;
+          if (cpmlRegion.firstEntry(start, activePosition[n], h1) || outer.firstExit(start, activePosition[n], h2))
+            throw std::runtime_error("Held-transverse activation crossed a particle boundary during the frozen part of a field step; enable mesh.ballistic_outside_transverse for boundary injection");
+          depositor.depositSegment(start, activePosition[n], p.charge);
+          start = activePosition[n];
+          startFraction = (activeTime[n] - time) / g.dt;
+        }
         ParticleBoundaryHit cpmlHit, outerHit;
         // firstMovingWindowEntry (simulation_moving_window.cpp:380-395)
         bool entersCpml = cpmlRegion.firstEntry(start, p.position, cpmlHit);
-        if (window) {
-          const double a = H.frame.labZFromBoxZT(start[2], time) - exitLabZ;
+        if (window && std::isfinite(exitLabZ)) {
+          const double a = H.frame.labZFromBoxZT(start[2], time + startFraction * g.dt) - exitLabZ;
           const double b = H.frame.labZFromBoxZT(p.position[2], time + g.dt) - exitLabZ;
           if (a >= 0.) throw std::runtime_error("Active particle started beyond trusted moving-window exit");
           if (b >= 0. && b > a) {
@@ -487,7 +547,7 @@ int ref_coupling_step(void* h, const double* origin, double time, unsigned subst
           carrier.velocity *= SI::c / gamma;
           carrier.charge = p.charge;
           carrier.currentWeight = 1.0;
-          carrierSegment(carrier, p.id, selected.position, localEnd);
+          carrierSegment(carrier, p.id, selected.position, startFraction + localEnd * (1.0 - startFraction));
         } else if (action == 2) {
           outer.depositOutgoingFlux(selected, p.charge);
           ++H.counts[6 + face];
@@ -599,6 +659,25 @@ int ref_window_begin_centered(void* h, std::size_t step, std::ptrdiff_t offset, 
     r.step = step;
     r.window->beginStep(*r.f, step, offset);
   });
+}
+
+int ref_coupling_set_held(void* h, const int* frozen, const double* beta) {
+  return guarded([&] {
+    auto& P = RC(h)->particles;
+    for (std::size_t i = 0; i < P.size(); ++i) {
+      P[i].frozen = frozen[i] != 0;
+      P[i].frozenBetaZ = beta[i];
+    }
+  });
+}
+int ref_coupling_get_frozen(void* h, int* frozen) {
+  return guarded([&] {
+    const auto& P = RC(h)->particles;
+    for (std::size_t i = 0; i < P.size(); ++i) frozen[i] = P[i].frozen ? 1 : 0;
+  });
+}
+int ref_coupling_set_activation(void* h, double labZ) {
+  return guarded([&] { RC(h)->activationLabZ = labZ; });
 }
 
 int ref_coupling_get_ids(void* h, std::uint64_t* id) {

@@ -141,3 +141,52 @@ def inlet_oracle_case(prefix, shape="binomial3", smooth=True, seed=21):
     c.set_particles(x, u, q, m, np.ones(len(q)), ids)
     c.enable_charge_match(REF_ORIGIN)
     return f, c
+
+
+def held_particles(seed=31, per_group=16):
+    """Held (frozen) electrons with beta_held = 0.99: one group inside the
+    window, one upstream of the inlet that crosses it within two steps, plus
+    one group of active electrons. Returns the particle arrays, the frozen
+    flags and the held speeds."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = DIMS
+    o = window_origin(OFFSET0)
+    def transverse(n):
+        return (o[0] + rng.uniform(CELLS[0] + 3, nx - CELLS[0] - 3, n) * DX,
+                o[1] + rng.uniform(CELLS[1] + 3, ny - CELLS[1] - 3, n) * DX)
+    xs, ys = transverse(3 * per_group)
+    z = np.concatenate([o[2] + rng.uniform(4.0, 14.0, per_group) * DZ,      # held in the window
+                        o[2] - rng.uniform(0.5, 3.0, per_group) * DZ,       # held upstream
+                        o[2] + rng.uniform(8.0, 16.0, per_group) * DZ])     # active
+    x = np.column_stack([xs, ys, z])
+    u = np.column_stack([rng.normal(0, .2, 3 * per_group), rng.normal(0, .2, 3 * per_group),
+                         rng.uniform(0.5, 2.0, 3 * per_group)])
+    frozen = np.concatenate([np.ones(2 * per_group, np.int32), np.zeros(per_group, np.int32)])
+    beta = np.where(frozen == 1, 0.99, 0.0)
+    n = len(x)
+    macro = 2.0e5 / n
+    q = np.full(n, -bunch.QE * macro)
+    m = np.full(n, bunch.ME * macro)
+    ids = np.arange(1, n + 1, dtype=np.uint64)
+    return x, u, q, m, ids, frozen, beta
+
+
+def activation_lab_z(boost_gamma=GAMMA_BOOST):
+    """Activation plane: the lab z of the box plane 12 cells above the
+    first origin at t = 0, so that held particles release during the run
+    (boxToLab with zero frame origins: z_lab = gamma (z_box + beta c t))."""
+    return boost_gamma * (window_origin(OFFSET0)[2] + 12.0 * DZ)
+
+
+def held_oracle_case(prefix, shape="binomial3", smooth=True, seed=31):
+    import harness as H
+    f = H.CpuFields(prefix, *DIMS, DX, DX, DZ, DZ / C_SI, 1)
+    f.install_window(CELLS)
+    H.seeded_fields(f, 9, scale=1e5)
+    c = H.CpuCoupling(f, shape, smooth, cpml_cells=CELLS)
+    c.set_frame(GAMMA_BOOST, 0.0, 0.0, 0.0)
+    x, u, q, m, ids, frozen, beta = held_particles(seed)
+    c.set_particles(x, u, q, m, np.ones(len(q)), ids)
+    c.set_held(frozen, beta)
+    c.set_activation(activation_lab_z())
+    return f, c

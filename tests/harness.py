@@ -264,6 +264,9 @@ def _bind_particles(lib, prefix):
     f("coupling_get_carriers").argtypes = [vp, _dp, _dp, _dp, _dp, _u64p]
     f("coupling_counters").argtypes = [vp, _u64p, _dp]
     f("coupling_finish_window").argtypes = [vp, _dp]
+    f("coupling_set_held").argtypes = [vp, C.POINTER(C.c_int), _dp]
+    f("coupling_get_frozen").argtypes = [vp, C.POINTER(C.c_int)]
+    f("coupling_set_activation").argtypes = [vp, C.c_double]
     if prefix == "orc":
         f("window_set_charge_match").argtypes = [vp, C.c_int]
         f("coupling_complete_inlet").argtypes = [vp, _dp]
@@ -409,6 +412,19 @@ class CpuCoupling:
         getattr(lib, self.prefix + "_reference_current_points")(self.h, guards[0], guards[1],
                                                                ij.ctypes.data_as(C.POINTER(C.c_size_t)))
         return ij.reshape(n, 2).astype(np.int64)
+
+    def set_held(self, frozen, beta):
+        fr = np.ascontiguousarray(frozen, dtype=np.int32)
+        _call(self.prefix, "coupling_set_held", self.h, fr.ctypes.data_as(C.POINTER(C.c_int)), _ptr(_v(beta)))
+
+    def frozen(self):
+        n = getattr(self.lib, self.prefix + "_coupling_count")(self.h)
+        fr = np.zeros(n, dtype=np.int32)
+        _call(self.prefix, "coupling_get_frozen", self.h, fr.ctypes.data_as(C.POINTER(C.c_int)))
+        return fr
+
+    def set_activation(self, lab_z):
+        _call(self.prefix, "coupling_set_activation", self.h, lab_z)
 
     def finish_window(self, new_origin):
         _call(self.prefix, "coupling_finish_window", self.h, _ptr(_v(new_origin)))
