@@ -126,6 +126,8 @@ def lib() -> C.CDLL:
     L.aprl_particles_count.argtypes = [vp]
     L.aprl_particles_count.restype = _sz
     L.aprl_particles_download_states.argtypes = [vp, C.POINTER(C.c_int)]
+    L.aprl_particles_set_held.argtypes = [vp, C.POINTER(C.c_int), _dp]
+    L.aprl_coupling_set_activation.argtypes = [vp, C.c_double]
     L.aprl_window_set_charge_match.argtypes = [vp, C.c_int]
     L.aprl_window_begin_step_centered.argtypes = [vp, _sz, C.c_ssize_t, C.POINTER(_dp), _dp]
     L.aprl_window_complete_inlet.argtypes = [vp, vp, vp, _dp]
@@ -187,6 +189,7 @@ EXPORTED = [
     "aprl_profile_enable", "aprl_profile_query", "aprl_profile_reset",
     "aprl_particles_create", "aprl_particles_destroy", "aprl_particles_upload", "aprl_particles_download",
     "aprl_particles_count", "aprl_particles_download_states", "aprl_coupling_boundary_counters",
+    "aprl_particles_set_held", "aprl_coupling_set_activation",
     "aprl_coupling_create", "aprl_coupling_destroy", "aprl_coupling_set_frame",
     "aprl_coupling_add_magnet", "aprl_coupling_set_particle_boundary", "aprl_coupling_step",
     "aprl_coupling_set_split_trials", "aprl_window_step_batch",

@@ -66,6 +66,13 @@ class ParticleStore:
                                             ids.ctypes.data_as(C.POINTER(C.c_uint64)) if n else None))
         return x, u, q, m, w, ids
 
+    def setHeld(self, frozen, beta):
+        """RelativisticParticleSI::frozen / frozenBetaZ for the entries in
+        upload order (before the first step)."""
+        fr = np.ascontiguousarray(frozen, dtype=np.int32)
+        b = np.ascontiguousarray(beta, dtype=np.float64)
+        check(lib().aprl_particles_set_held(self.handle, fr.ctypes.data_as(C.POINTER(C.c_int)), dptr(b)))
+
     def states(self):
         """Entry state per live entry in download order: 0 particle, 1 CPML
         carrier (ParticleCPMLCarrier, particle_boundary.h:76-84)."""
@@ -127,6 +134,10 @@ class ConservativeSourceCoupling:
         g = (C.c_size_t * 2)(*[int(v) for v in guards])
         check(lib().aprl_coupling_set_reference_current(self.handle, dptr(t.ravel()), t.shape[0], t.shape[1], g,
                                                         float(translation_beta)))
+
+    def setActivationPlane(self, lab_z):
+        """Lab z of the held particles' release (NaN disables it)."""
+        check(lib().aprl_coupling_set_activation(self.handle, float(lab_z)))
 
     def boundaryCounters(self):
         """Cumulative (counts[20], charges[20]): CPML entries per face (0-5),

@@ -115,7 +115,11 @@ aprl_status aprl_particles_upload(aprl_particles* particles, size_t n, const dou
       removed = P.removed.get();
     }
     // every uploaded entry is an active particle
-    if (n) hipCheck(hipMemset(state, 0, n * sizeof(int)), "hipMemset");
+    if (n) {
+      hipCheck(hipMemset(state, 0, n * sizeof(int)), "hipMemset");
+      if (particles->fields->fp32) hipCheck(hipMemset(particles->sp.beta.get(), 0, n * sizeof(float)), "hipMemset");
+      else hipCheck(hipMemset(particles->p.beta.get(), 0, n * sizeof(double)), "hipMemset");
+    }
     hipCheck(hipMemset(removed, 0, sizeof(unsigned)), "hipMemset");
   });
 }
@@ -185,6 +189,43 @@ aprl_status aprl_particles_download(aprl_particles* particles, double* x3, doubl
       if (id) id[k] = I[i];
       ++k;
     }
+  });
+}
+
+aprl_status aprl_particles_set_held(aprl_particles* particles, const int* frozen, const double* beta) {
+  return guarded([&] {
+    if (!particles || !frozen || !beta) throw std::invalid_argument("Null particle store or held arrays");
+    hipCheck(hipStreamSynchronize(particles->stream()), "hipStreamSynchronize");
+    const size_t n = particles->fields->fp32 ? particles->sp.count : particles->p.count;
+    if (aprl_particles_count(particles) != n)
+      throw std::logic_error("Held states are set on a store without removed entries (after an upload)");
+    std::vector<int> st = storeStates(particles, n);
+    for (size_t i = 0; i < n; ++i) {
+      if (st[i] == kStateCarrier) {
+        if (frozen[i]) throw std::invalid_argument("A CPML carrier cannot be held");
+        continue;
+      }
+      if (frozen[i] && !(std::isfinite(beta[i]) && beta[i] >= 0.0 && beta[i] < 1.0))
+        throw std::invalid_argument("Held beta must be finite in [0, 1)");
+      st[i] = frozen[i] ? kStateHeld : kStateActive;
+    }
+    int* dst = particles->fields->fp32 ? particles->sp.state.get() : particles->p.state.get();
+    if (n) hipCheck(hipMemcpy(dst, st.data(), n * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+    if (particles->fields->fp32) {
+      std::vector<float> b(n);
+      for (size_t i = 0; i < n; ++i) b[i] = static_cast<float>(beta[i]);
+      if (n) hipCheck(hipMemcpy(particles->sp.beta.get(), b.data(), n * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+    } else if (n) {
+      hipCheck(hipMemcpy(particles->p.beta.get(), beta, n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+    }
+  });
+}
+
+aprl_status aprl_coupling_set_activation(aprl_coupling* coupling, double activation_lab_z) {
+  return guarded([&] {
+    if (!coupling) throw std::invalid_argument("Null coupling");
+    coupling->c.hasActivation = !std::isnan(activation_lab_z);
+    coupling->c.activationLabZ = activation_lab_z;
   });
 }
 

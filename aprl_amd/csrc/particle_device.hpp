@@ -22,6 +22,8 @@ struct StepArgs {
   double cpmlRate[3], cpmlThick[3], cpmlOrder;  // conductivity grading of the carriers' damping
   int hasExit, window;
   double exitLabZ;
+  int hasActivation;
+  double activationLabZ;
   double extent[3], posTol[3];  // validatePosition
 };
 
@@ -38,7 +40,7 @@ __device__ __forceinline__ double gammaFromU(double ux, double uy, double uz) {
 }
 
 // SIMagneticElement::fieldsLab, eb_sources.cpp:346-446 (E is zero).
-__device__ void magnetLab(const MagnetDev& M, const double pos[3], double b[3]) {
+__device__ __forceinline__ void magnetLab(const MagnetDev& M, const double pos[3], double b[3]) {
   b[0] = b[1] = b[2] = 0.0;
   const double rel0 = pos[0] - M.center[0], rel1 = pos[1] - M.center[1], z = pos[2] - M.center[2];
   if (M.type == 2) {
@@ -102,7 +104,7 @@ __device__ void magnetLab(const MagnetDev& M, const double pos[3], double b[3]) 
 
 // SIFieldSourceSet::addPrescribedBox (eb_sources.cpp:571-582) with the
 // boxToLab / fieldsLabToBox transforms (boostframe.cpp:188-196, 248-265).
-__device__ void prescribedBox(const StepArgs& S, const double posBox[3], double tBox, double e[3], double b[3]) {
+__device__ __forceinline__ void prescribedBox(const StepArgs& S, const double posBox[3], double tBox, double e[3], double b[3]) {
   if (S.nMagnets == 0) return;
   const BoostDev& F = S.frame;
   double posLab[3] = {posBox[0], posBox[1], posBox[2]};
@@ -126,7 +128,7 @@ __device__ void prescribedBox(const StepArgs& S, const double posBox[3], double 
 }
 
 // source_coupling_math::momentum, source_coupling_math.h:272-293.
-__device__ void momentum(double u[3], const double e[3], const double mag[3], double q, double m, double h,
+__device__ __forceinline__ void momentum(double u[3], const double e[3], const double mag[3], double q, double m, double h,
                          double g0, unsigned& err) {
   const double before[3] = {u[0], u[1], u[2]};
   const double ks = q * h / (2 * m * kC);
@@ -172,12 +174,12 @@ __device__ void momentum(double u[3], const double e[3], const double mag[3], do
 // prescribed magnets at the explicit midpoint, then the particle-boundary
 // and domain checks of the deposition loop against the grid origin o.
 // Positions are absolute box coordinates in metres.
-__device__ void pushChord(const StepArgs& S, const double o[3], const double x0[3], const double u0[3], double q,
-                          double m, const double usedE[3], const double usedB[3], double x[3], double u[3],
-                          double& absGrid, unsigned& err) {
+__device__ __forceinline__ void pushChordAt(const StepArgs& S, double time, double dt, const double o[3], const double x0[3],
+                            const double u0[3], double q, double m, const double usedE[3], const double usedB[3],
+                            double x[3], double u[3], double& absGrid, unsigned& err) {
   if (S.window && x0[2] < o[2]) err |= kErrUpstream;
   for (int a = 0; a < 3; ++a) { x[a] = x0[a]; u[a] = u0[a]; }
-  const double h = S.dt / S.substeps;
+  const double h = dt / S.substeps;
   absGrid = 0.0;
   for (int n = 0; n < S.substeps; ++n) {
     const double bx[3] = {x[0], x[1], x[2]}, bu[3] = {u[0], u[1], u[2]};
@@ -186,7 +188,7 @@ __device__ void pushChord(const StepArgs& S, const double o[3], const double x0[
     const double ps = .5 * h * kC / g0;
     for (int a = 0; a < 3; ++a) point[a] += ps * u[a];
     double ee[3] = {0., 0., 0.}, bb[3] = {0., 0., 0.};
-    prescribedBox(S, point, S.time + (n + .5) * h, ee, bb);
+    prescribedBox(S, point, time + (n + .5) * h, ee, bb);
     double e[3] = {usedE[0], usedE[1], usedE[2]}, b[3] = {usedB[0], usedB[1], usedB[2]};
     for (int a = 0; a < 3; ++a) { e[a] += ee[a]; b[a] += bb[a]; }
     momentum(u, e, b, q, m, h, g0, err);
@@ -210,6 +212,12 @@ __device__ void pushChord(const StepArgs& S, const double o[3], const double x0[
   }
 }
 
+__device__ __forceinline__ void pushChord(const StepArgs& S, const double o[3], const double x0[3], const double u0[3],
+                                          double q, double m, const double usedE[3], const double usedB[3],
+                                          double x[3], double u[3], double& absGrid, unsigned& err) {
+  pushChordAt(S, S.time, S.dt, o, x0, u0, q, m, usedE, usedB, x, u, absGrid, err);
+}
+
 // ---------------------------------------------------------------------------
 // Particle-boundary events (the deposit loop of pushDepositAndMigrate,
 // simulation.cpp:4310-4917, for one rank).
@@ -222,6 +230,7 @@ constexpr int kEvtCarrier = 1 << 5;       // a carrier segment follows (always s
 constexpr int kEvtCarrierExit = 1 << 6;   // the carrier segment ends at an outer face
 constexpr int kEvtCarrierFaceShift = 7;   // face of that exit (3 bits)
 constexpr int kEvtCarrierDrop = 1 << 10;  // the carrier's end weight is at or below the cutoff
+constexpr int kEvtReleased = 1 << 11;     // a held particle was released during the step
 
 struct BoundaryHit {
   double pos[3];
@@ -244,7 +253,7 @@ __device__ __forceinline__ void validateEnd(const StepArgs& S, const double o[3]
 
 // ParticleCPMLRegion::firstEntry, particle_boundary.cpp:104-174 (start
 // checks in pushChord).
-__device__ bool cpmlFirstEntry(const StepArgs& S, const double s[3], const double e[3], BoundaryHit& h,
+__device__ __forceinline__ bool cpmlFirstEntry(const StepArgs& S, const double s[3], const double e[3], BoundaryHit& h,
                                unsigned& err) {
   if (!S.cpmlEnabled) return false;
   double best = INFINITY;
@@ -293,11 +302,11 @@ __device__ bool cpmlFirstEntry(const StepArgs& S, const double s[3], const doubl
 
 // Simulation::firstMovingWindowEntry, simulation_moving_window.cpp:380-395
 // (the lab exit plane only when one is set).
-__device__ bool windowFirstEntry(const StepArgs& S, const double s[3], const double e[3], BoundaryHit& h,
-                                 unsigned& err) {
+__device__ __forceinline__ bool windowFirstEntry(const StepArgs& S, const double s[3], const double e[3], double startFraction,
+                                 BoundaryHit& h, unsigned& err) {
   bool present = cpmlFirstEntry(S, s, e, h, err);
   if (!S.hasExit) return present;
-  const double a = labZOf(S, s[2], S.time) - S.exitLabZ;
+  const double a = labZOf(S, s[2], S.time + startFraction * S.dt) - S.exitLabZ;
   const double b = labZOf(S, e[2], S.time + S.dt) - S.exitLabZ;
   if (a >= 0.) err |= kErrBeyondExit;
   if (b >= 0. && b > a) {
@@ -313,7 +322,7 @@ __device__ bool windowFirstEntry(const StepArgs& S, const double s[3], const dou
 }
 
 // ParticleOpenBoundary::firstExit, particle_boundary.cpp:303-373.
-__device__ bool outerFirstExit(const StepArgs& S, const double s[3], const double e[3], BoundaryHit& h,
+__device__ __forceinline__ bool outerFirstExit(const StepArgs& S, const double s[3], const double e[3], BoundaryHit& h,
                                unsigned& err) {
   double best = INFINITY;
   int bestFace = 0;
@@ -359,7 +368,7 @@ __device__ bool outerFirstExit(const StepArgs& S, const double s[3], const doubl
 }
 
 // ParticleCPMLRegion::conductivityRate, particle_boundary.cpp:216-241.
-__device__ double conductivityRate(const StepArgs& S, const double p[3]) {
+__device__ __forceinline__ double conductivityRate(const StepArgs& S, const double p[3]) {
   double rate = 0.0;
   for (int a = 0; a < 3; ++a) {
     if (!S.cpmlCells[a]) continue;
@@ -374,7 +383,7 @@ __device__ double conductivityRate(const StepArgs& S, const double p[3]) {
 
 // ParticleCPMLRegion::currentDampingFactor, particle_boundary.cpp:176-205:
 // exp(-duration * mean rate) with the three-point Gauss-Legendre mean.
-__device__ double dampingFactor(const StepArgs& S, const double s[3], const double e[3], double duration) {
+__device__ __forceinline__ double dampingFactor(const StepArgs& S, const double s[3], const double e[3], double duration) {
   if (duration == 0.0 || !S.cpmlEnabled) return 1.0;
   const double abscissa = sqrt(3.0 / 5.0);
   const double node[3] = {0.5 * (1.0 - abscissa), 0.5, 0.5 * (1.0 + abscissa)};
@@ -392,14 +401,14 @@ __device__ double dampingFactor(const StepArgs& S, const double s[3], const doub
 // processCarrierSegment, simulation.cpp:4310-4407: the carrier moves from
 // start (time fraction startFraction) to pos, is cut at its first outer
 // exit, deposits with the midpoint damping and keeps the end damping.
-// Returns the code bits of the carrier part; tEnd is its end as a fraction
-// of the step.
-__device__ int carrierSegment(const StepArgs& S, const double o[3], const double start[3], double startFraction,
-                              const double pos[3], double weight, double& tEnd, double& wDeposit, double& wEnd,
+// Returns the code bits of the carrier part; localEnd is its end as a
+// fraction of start -> pos.
+__device__ __forceinline__ int carrierSegment(const StepArgs& S, const double o[3], const double start[3], double startFraction,
+                              const double pos[3], double weight, double& localEnd, double& wDeposit, double& wEnd,
                               unsigned& err) {
   BoundaryHit oh;
   const bool exits = outerFirstExit(S, start, pos, oh, err);
-  double localEnd = 1.0;
+  localEnd = 1.0;
   double end[3] = {pos[0], pos[1], pos[2]};
   if (exits) {
     localEnd = oh.fraction;
@@ -413,23 +422,24 @@ __device__ int carrierSegment(const StepArgs& S, const double o[3], const double
   wEnd = weight * dampingFactor(S, start, end, duration);
   validateEnd(S, o, start, err);
   validateEnd(S, o, end, err);
-  tEnd = globalEnd;
   int code = kEvtCarrier;
   if (exits) code |= kEvtCarrierExit | (oh.face << kEvtCarrierFaceShift);
   else if (wEnd <= 64.0 * 2.220446049250313e-16) code |= kEvtCarrierDrop;
   return code;
 }
 
-// End action of an active particle's segment x0 -> x1 (processActiveSegment,
-// simulation.cpp:4787-4836 without retirement planes or rank interfaces):
-// a CPML entry or window exit at or before the first outer exit (within
-// 64 eps) hands the rest of the chord to a new carrier of weight 1; an outer
-// exit removes the particle. t1 / t2: chord fractions ending the unit-weight
-// part and the carrier part.
-__device__ int activeSegmentEvents(const StepArgs& S, const double o[3], const double x0[3], const double x1[3],
-                                   double& t1, double& t2, double& wDeposit, double& wEnd, unsigned& err) {
+// End action of an active particle's segment x0 -> x1 starting at the
+// step's time fraction sf (processActiveSegment, simulation.cpp:4787-4836
+// without retirement planes or rank interfaces): a CPML entry or window exit
+// at or before the first outer exit (within 64 eps) hands the rest of the
+// segment to a new carrier of weight 1; an outer exit removes the particle.
+// t1 / t2: segment fractions ending the unit-weight part and the carrier
+// part.
+__device__ __forceinline__ int activeSegmentEvents(const StepArgs& S, const double o[3], const double x0[3], const double x1[3],
+                                   double sf, double& t1, double& t2, double& wDeposit, double& wEnd,
+                                   unsigned& err) {
   BoundaryHit ch, oh;
-  const bool entersCpml = S.window ? windowFirstEntry(S, x0, x1, ch, err) : cpmlFirstEntry(S, x0, x1, ch, err);
+  const bool entersCpml = S.window ? windowFirstEntry(S, x0, x1, sf, ch, err) : cpmlFirstEntry(S, x0, x1, ch, err);
   const bool exitsOuter = outerFirstExit(S, x0, x1, oh, err);
   const double tolerance = 64.0 * 2.220446049250313e-16;
   t1 = t2 = 1.0;
@@ -438,7 +448,11 @@ __device__ int activeSegmentEvents(const StepArgs& S, const double o[3], const d
     validateEnd(S, o, ch.pos, err);
     t1 = ch.fraction;
     const int kind = S.window && ch.face == 5 ? 3 : 2;
-    return kind | (ch.face << kEvtFaceShift) | carrierSegment(S, o, ch.pos, t1, x1, 1.0, t2, wDeposit, wEnd, err);
+    double local = 1.0;
+    const int code = kind | (ch.face << kEvtFaceShift) |
+                     carrierSegment(S, o, ch.pos, sf + t1 * (1.0 - sf), x1, 1.0, local, wDeposit, wEnd, err);
+    t2 = t1 + local * (1.0 - t1);
+    return code;
   }
   if (exitsOuter) {
     validateEnd(S, o, oh.pos, err);
@@ -447,6 +461,128 @@ __device__ int activeSegmentEvents(const StepArgs& S, const double o[3], const d
   }
   validateEnd(S, o, x1, err);
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Held particles (prepareParticle's hold-transverse branch,
+// simulation.cpp:4951-5000).
+
+struct HeldMove {
+  bool release;     // the lab z reaches the activation plane this step
+  bool kink;        // released strictly inside the step
+  double f;         // release time fraction
+  double z1;        // held end z (no release) or release z
+  double remaining; // push time after the release
+};
+
+__device__ __forceinline__ HeldMove heldMove(const StepArgs& S, const double x0[3], double beta) {
+  HeldMove h;
+  const double startLabZ = labZOf(S, x0[2], S.time);
+  double z = x0[2];
+  z += kC * S.dt * beta;
+  const double endLabZ = labZOf(S, z, S.time + S.dt);
+  h.release = S.hasActivation && endLabZ >= S.activationLabZ;
+  h.f = 0.0;
+  h.kink = false;
+  h.remaining = 0.0;
+  h.z1 = z;
+  if (h.release) {
+    h.f = startLabZ >= S.activationLabZ
+              ? 0.0
+              : fmax(0.0, fmin(1.0, (S.activationLabZ - startLabZ) / (endLabZ - startLabZ)));
+    double zk = x0[2];
+    zk += kC * h.f * S.dt * beta;
+    h.z1 = zk;
+    const double activeStart = S.time + h.f * S.dt;
+    h.kink = activeStart > S.time && activeStart < S.time + S.dt;
+    h.remaining = (1.0 - h.f) * S.dt;
+  }
+  return h;
+}
+
+// One trial's move of a store entry (shared by both forms): a carrier's
+// straight move; a held entry's held move, or its release (the frozen leg's
+// checks, then the push of the rest of the step from the release point,
+// processActiveSegment, simulation.cpp:4676-4746) and, upstream of the
+// window, its admission (5121-5155); or the orbit push. Then the events of
+// the active part (one call site, which keeps the kernel in registers).
+// Returns the event code; t1 / t2 are fractions of the chord x0 -> x, or of
+// kink -> x after a release kink.
+__device__ __forceinline__ int entryStep(const StepArgs& S, const double o[3], int state, const double x0[3],
+                                         const double u0[3], double q, double m, double weight, double beta,
+                                         const double usedE[3], const double usedB[3], double x[3], double u[3],
+                                         double& absGrid, double& t1, double& t2, double& wDeposit, double& wEnd,
+                                         unsigned& err) {
+  absGrid = 0.0;
+  t1 = 0.0;
+  t2 = 1.0;
+  wDeposit = wEnd = 0.0;
+  if (state == kStateCarrier) {
+    const double scale = kC / gammaFromU(u0[0], u0[1], u0[2]);
+    for (int a = 0; a < 3; ++a) {
+      const double v = u0[a] * scale;
+      x[a] = x0[a];
+      x[a] += S.dt * v;
+      u[a] = u0[a];
+    }
+    return carrierSegment(S, o, x0, 0.0, x, weight, t2, wDeposit, wEnd, err);
+  }
+  const bool held = state == kStateHeld;
+  HeldMove h{};
+  double xs[3] = {x0[0], x0[1], x0[2]};
+  double t0 = S.time, dtp = S.dt;
+  bool push = true;
+  if (held) {
+    h = heldMove(S, x0, beta);
+    xs[2] = h.z1;
+    push = h.release && h.remaining > 0.0;
+    t0 = S.time + h.f * S.dt;
+    dtp = h.remaining;
+  }
+  if (push) {
+    pushChordAt(S, t0, dtp, o, xs, u0, q, m, usedE, usedB, x, u, absGrid, err);
+  } else {
+    for (int a = 0; a < 3; ++a) {
+      x[a] = xs[a];
+      u[a] = u0[a];
+    }
+  }
+  // start of the event segment, its time fraction, and the map of its
+  // fractions onto the deposited chord (t = ta + local tb)
+  double sp[3] = {x0[0], x0[1], x0[2]};
+  double sf = 0.0, ta = 0.0, tb = 1.0;
+  int code = 0;
+  if (held && h.release) {
+    code = kEvtReleased;
+    if (S.window && x0[2] < o[2]) err |= kErrHeldRelease;
+    if (h.kink) {
+      sp[2] = h.z1;
+      BoundaryHit b1, b2;
+      if (cpmlFirstEntry(S, x0, sp, b1, err) || outerFirstExit(S, x0, sp, b2, err)) err |= kErrHeldFrozenLeg;
+      validateEnd(S, o, x0, err);
+      validateEnd(S, o, sp, err);
+      sf = h.f;
+    }
+  } else if (held && S.window && x0[2] < o[2]) {
+    // upstream: only the shaped support is deposited until the inlet
+    t1 = t2 = 1.0;
+    if (x[2] < o[2]) return 0;
+    const double fraction = (o[2] - x0[2]) / (x[2] - x0[2]);
+    if (!(fraction >= 0.0 && fraction <= 1.0) || !isfinite(fraction)) {
+      err |= kErrAdmission;
+      return 0;
+    }
+    for (int a = 0; a < 3; ++a) sp[a] = x0[a] + fraction * (x[a] - x0[a]);
+    sp[2] = o[2];
+    sf = fraction;
+    ta = fraction;
+    tb = 1.0 - fraction;
+  }
+  double l1, l2;
+  code |= activeSegmentEvents(S, o, sp, x, sf, l1, l2, wDeposit, wEnd, err);
+  t1 = ta + l1 * tb;
+  t2 = ta + l2 * tb;
+  return code;
 }
 
 // ---------------------------------------------------------------------------
@@ -480,7 +616,7 @@ __device__ __forceinline__ bool liveEntry(const SortArgs& K, int* state, int p, 
     if (z < K.lowZ) atomicOr(K.words, static_cast<unsigned>(kErrCarrierBehind));
     return true;
   }
-  if (z < K.lowZ) atomicOr(K.words, static_cast<unsigned>(kErrBehindInlet));
+  if (z < K.lowZ && st != kStateHeld) atomicOr(K.words, static_cast<unsigned>(kErrBehindInlet));
   // ownerRank (simulation.cpp:4068-4083) of one rank
   const double coordinate = (z - K.lowZ) / K.dz;
   const double tolerance = 64.0 * 2.220446049250313e-16 * fmax(1.0, fabs(coordinate));
@@ -525,6 +661,7 @@ __global__ void __launch_bounds__(kEventBlock) kCommitEvents(int n, const int* e
         q0 = charge;
       }
       const R startWeight = w[p];
+      if (e & kEvtReleased) state[p] = kStateActive;
       if (e & kEvtCarrier) {
         const R we = wend[p];
         if (e & kEvtCarrierExit) {
@@ -539,7 +676,8 @@ __global__ void __launch_bounds__(kEventBlock) kCommitEvents(int n, const int* e
         }
       }
       // keep the weight the entry started the step with (1 for a particle)
-      wend[p] = kind ? R(1) : startWeight;
+      const bool wasCarrier = (e & kEvtCarrier) && !kind && !(e & kEvtReleased);
+      wend[p] = wasCarrier ? startWeight : R(1);
       if (dead) {
         state[p] = kStateRemoved;
         atomicAdd(removed, 1u);

@@ -51,6 +51,7 @@ struct PArgs {
   double* cut[3];    // t1, t2, carrier-part deposit weight
   double* wend;      // carrier end weight
   const double* w;   // weight slot (a carrier's current weight)
+  const double* beta;  // held speed of held entries
   double scaleDen;   // dx*dy*dz*dt
   int n;
 };
@@ -302,6 +303,8 @@ struct PermuteArgs {
   uint64_t* idDst;
   const int* stateSrc;
   int* stateDst;
+  const double* betaSrc;
+  double* betaDst;
 };
 
 __global__ void kPermute(PermuteArgs A, const unsigned* perm, int n) {
@@ -312,6 +315,7 @@ __global__ void kPermute(PermuteArgs A, const unsigned* perm, int n) {
   for (int a = 0; a < 9; ++a) A.dst[a][p] = A.src[a][q];
   A.idDst[p] = A.idSrc[q];
   A.stateDst[p] = A.stateSrc[q];
+  A.betaDst[p] = A.betaSrc[q];
 }
 
 // Particle range of every occupied tile from the run boundaries of the
@@ -389,8 +393,8 @@ __global__ void kFillInt(int* p, int n, int v) {
 // (source_coupling.cpp:148-202) at (x0, x1): next fields, and the chunk's
 // force max and defect / scale / call sums in a fixed tree order.
 template <int MODE, int SHAPE>
-__global__ void __launch_bounds__(2 * kChunk) kGatherChunks(PArgs P, GArgs G, const int4* chunks, int ntx, int nty,
-                                                            double dt, double* chunkOut, unsigned* errOut) {
+__global__ void __launch_bounds__(2 * kChunk) kGatherChunks(PArgs P, GArgs G, StepArgs S, const int4* chunks, int ntx,
+                                                            int nty, double dt, double* chunkOut, unsigned* errOut) {
   constexpr bool kStageB = SHAPE == kShapeBinomial3X;
   __shared__ double es[3 * boxNodes<SHAPE>()];
   __shared__ double bs[kStageB ? 3 * boxNodes<SHAPE>() : 4 * kChunk];
@@ -401,16 +405,28 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks(PArgs P, GArgs G, co
   const bool bWaves = threadIdx.x >= kChunk;
   const int lane = static_cast<int>(threadIdx.x) - (bWaves ? kChunk : 0);
   const int p = ch.y + lane;
-  // carriers move without fields and take no part in the residual
-  const bool live = p < ch.z && P.state[p] == kStateActive;
+  // carriers move without fields and take no part in the residual; a held
+  // entry takes part from its release point with the rest of the step
+  const int st = p < ch.z ? P.state[p] : kStateRemoved;
+  bool live = st == kStateActive;
+  double dtp = dt;
   unsigned err = 0;
   double a[3] = {0., 0., 0.}, b[3] = {0., 0., 0.}, g[3] = {0., 0., 0.};
-  if (live) {
+  if (st == kStateActive || st == kStateHeld)
     for (int d = 0; d < 3; ++d) a[d] = P.x[d][p];
+  if (st == kStateHeld) {
+    const HeldMove h = heldMove(S, a, P.beta[p]);
+    if (h.release && h.remaining > 0.0) {
+      live = true;
+      a[2] = h.z1;
+      dtp = h.remaining;
+    }
+  }
+  if (live) {
     if (MODE == 0) {
       const double u0[3] = {P.u[0][p], P.u[1][p], P.u[2][p]};
       const double gamma = gammaFromU(u0[0], u0[1], u0[2]);
-      const double s = dt * kC / gamma;
+      const double s = dtp * kC / gamma;
       for (int d = 0; d < 3; ++d) {
         b[d] = a[d];
         b[d] += s * u0[d];
@@ -445,7 +461,7 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks(PArgs P, GArgs G, co
         db[d] = nb[d] - P.fin[3 + d][p];
       }
       const double q = P.q[p], m = P.m[p];
-      force = fabs(q / m) * dt / kC * (sqrt(de[0] * de[0] + de[1] * de[1] + de[2] * de[2]) +
+      force = fabs(q / m) * dtp / kC * (sqrt(de[0] * de[0] + de[1] * de[1] + de[2] * de[2]) +
                                         kC * sqrt(db[0] * db[0] + db[1] * db[1] + db[2] * db[2]));
       double w1 = 0., w2 = 0.;
       for (int d = 0; d < 3; ++d) {
@@ -497,23 +513,11 @@ __global__ void __launch_bounds__(128) kPush(PArgs P, GArgs G, StepArgs S, unsig
   unsigned err = 0;
   const double x0[3] = {P.x[0][p], P.x[1][p], P.x[2][p]};
   const double u0[3] = {P.u[0][p], P.u[1][p], P.u[2][p]};
-  double x[3], u[3], absGrid = 0.0, t1 = 0.0, t2 = 1.0, wDeposit = 0.0, wEnd = 0.0;
-  int code;
-  if (P.state[p] == kStateCarrier) {
-    const double scale = kC / gammaFromU(u0[0], u0[1], u0[2]);
-    for (int a = 0; a < 3; ++a) {
-      const double v = u0[a] * scale;
-      x[a] = x0[a];
-      x[a] += S.dt * v;
-      u[a] = u0[a];
-    }
-    code = carrierSegment(S, G.o, x0, 0.0, x, P.w[p], t2, wDeposit, wEnd, err);
-  } else {
-    const double usedE[3] = {P.fin[0][p], P.fin[1][p], P.fin[2][p]};
-    const double usedB[3] = {P.fin[3][p], P.fin[4][p], P.fin[5][p]};
-    pushChord(S, G.o, x0, u0, P.q[p], P.m[p], usedE, usedB, x, u, absGrid, err);
-    code = activeSegmentEvents(S, G.o, x0, x, t1, t2, wDeposit, wEnd, err);
-  }
+  double x[3], u[3], absGrid, t1, t2, wDeposit, wEnd;
+  const double usedE[3] = {P.fin[0][p], P.fin[1][p], P.fin[2][p]};
+  const double usedB[3] = {P.fin[3][p], P.fin[4][p], P.fin[5][p]};
+  const int code = entryStep(S, G.o, P.state[p], x0, u0, P.q[p], P.m[p], P.w[p], P.beta[p], usedE, usedB, x, u,
+                             absGrid, t1, t2, wDeposit, wEnd, err);
   for (int a = 0; a < 3; ++a) {
     P.x1[a][p] = x[a];
     P.u1[a][p] = u[a];
@@ -540,9 +544,10 @@ __global__ void __launch_bounds__(128) kPush(PArgs P, GArgs G, StepArgs S, unsig
 // reproducible in any order; each tap is rounded once, to 2^-62 of the
 // tile bound.
 template <int SHAPE>
-__global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, const int4* chunks, const int* tileBegin,
-                                                   const int* tileEnd, const unsigned long long* tileScan, int ntx,
-                                                   int nty, long long* partial, int* tileShift, unsigned* errOut) {
+__global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, StepArgs S, const int4* chunks,
+                                                   const int* tileBegin, const int* tileEnd,
+                                                   const unsigned long long* tileScan, int ntx, int nty,
+                                                   long long* partial, int* tileShift, unsigned* errOut) {
   constexpr int N = boxNodes<SHAPE>(), NX = Box<SHAPE>::NX, NY = Box<SHAPE>::NY;
   __shared__ unsigned long long acc[3 * N];
   __shared__ double bound[kChunk];
@@ -561,6 +566,9 @@ __global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, const int4*
     double s[3];
     scaleOf(p, s);
     bsum += fmax(fabs(s[0]), fmax(fabs(s[1]), fabs(s[2])));
+    // a held entry's release kink splits its path: both legs stay below
+    // the chord plus the held leg
+    if (P.state[p] == kStateHeld) bsum += fabs(P.q[p]) * (kC * S.dt * fabs(P.beta[p])) / P.scaleDen;
   }
   bound[threadIdx.x] = bsum;
   __syncthreads();
@@ -576,14 +584,8 @@ __global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, const int4*
   const int p = ch.y + static_cast<int>(threadIdx.x);
   if (p < ch.z) {
     double scale[3];
-    scaleOf(p, scale);
     double x[3], v[3], cuts[kMaxCuts];
-    for (int d = 0; d < 3; ++d) {
-      x[d] = (P.x[d][p] - G.o[d]) / G.h[d];
-      v[d] = (P.x1[d][p] - P.x[d][p]) / G.h[d];
-    }
-    const int count = orbitCuts(x, v, cuts, err);
-  const int org[3] = {0, 0, 0};  // x is grid-relative
+    const int org[3] = {0, 0, 0};  // x is grid-relative
     int base = 0;
     auto interior = [&](long long i, long long j, long long k, int wx, int wy, int wz) {
       if (i < 0 || j < 0 || k < 0 || i + wx > G.nx + 1 || j + wy > G.ny + 1 || k + wz > G.nz + 1) return false;
@@ -614,21 +616,50 @@ __global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, const int4*
       if (l < 0) return;
       atomicAdd(&acc[d * N + l], static_cast<unsigned long long>(llrint(ldexp(scale[d] * w * mult, shift))));
     };
-    // A boundary event splits the chord: [0, t1] at unit weight (the active
-    // part) and [t1, t2] at the carrier part's deposit weight.
+    // A boundary event splits the last leg: [0, t1] at unit weight (the
+    // active part) and [t1, t2] at the carrier part's deposit weight. A held
+    // entry released inside the step deposits its held leg x0 -> kink first.
     const int e = P.evt[p];
     const double t1 = e ? P.cut[0][p] : 1.0, t2 = e ? P.cut[1][p] : 1.0, wCarrier = e ? P.cut[2][p] : 0.0;
-    for (int s = 1; s < count; ++s) {
-      const double c0 = cuts[s - 1], c1 = cuts[s];
-      if (c1 - c0 <= 0.) continue;
-      // without an event: one part [c0, c1] (t1 = t2 = 1)
+    double xa[3], xb[3], xk[3];
+    for (int d = 0; d < 3; ++d) {
+      xa[d] = P.x[d][p];
+      xb[d] = P.x1[d][p];
+      xk[d] = xa[d];
+    }
+    int legs = 1;
+    if (P.state[p] == kStateHeld) {
+      const HeldMove h = heldMove(S, xa, P.beta[p]);
+      if (h.release && h.kink) {
+        legs = 2;
+        xk[2] = h.z1;
+      }
+    }
+    const double q = P.q[p];
 #pragma unroll 1
-      for (int part = 0; part < 2; ++part) {
-        const double a = part ? fmax(c0, t1) : c0;
-        const double b = part ? fmin(c1, t2) : fmin(c1, t1);
-        if (!(b > a)) continue;
-        mult = part ? wCarrier : 1.0;
-        segmentTaps<SHAPE, double>(G.nx, G.ny, org, x, v, a, b, interior, fast, visit);
+    for (int leg = 0; leg < legs; ++leg) {
+      const bool last = leg + 1 == legs;
+      for (int d = 0; d < 3; ++d) {
+        const double s0 = leg ? xk[d] : xa[d], s1 = last ? xb[d] : xk[d];
+        // scale_a = q * (x1_a - x0_a) / (dx dy dz dt), eb_deposition.cpp:41-51
+        scale[d] = q * (s1 - s0) / P.scaleDen;
+        x[d] = (s0 - G.o[d]) / G.h[d];
+        v[d] = (s1 - s0) / G.h[d];
+      }
+      const int count = orbitCuts(x, v, cuts, err);
+      const double f1 = last ? t1 : 1.0, f2 = last ? t2 : 1.0, fw = last ? wCarrier : 0.0;
+      for (int sgm = 1; sgm < count; ++sgm) {
+        const double c0 = cuts[sgm - 1], c1 = cuts[sgm];
+        if (c1 - c0 <= 0.) continue;
+        // without an event: one part [c0, c1] (t1 = t2 = 1)
+#pragma unroll 1
+        for (int part = 0; part < 2; ++part) {
+          const double a = part ? fmax(c0, f1) : c0;
+          const double b = part ? fmin(c1, f2) : fmin(c1, f1);
+          if (!(b > a)) continue;
+          mult = part ? fw : 1.0;
+          segmentTaps<SHAPE, double>(G.nx, G.ny, org, x, v, a, b, interior, fast, visit);
+        }
       }
     }
   }
@@ -767,6 +798,7 @@ static PArgs particleArgs(ParticleStore& P) {
   A.q = P.s[6].get();
   A.m = P.s[7].get();
   A.w = P.s[8].get();
+  A.beta = P.beta.get();
   A.gridAbs = P.gridAbs.get();
   A.state = P.state.get();
   A.evt = P.evt.get();
@@ -783,6 +815,8 @@ void reserveParticles(ParticleStore& P, std::size_t capacity) {
   P.idT.resize(capacity);
   P.state.resize(capacity);
   P.stateT.resize(capacity);
+  P.beta.resize(capacity);
+  P.betaT.resize(capacity);
   P.evt.resize(capacity);
   for (int a = 0; a < 3; ++a) P.cut[a].resize(capacity);
   P.wend.resize(capacity);
@@ -875,6 +909,8 @@ StepArgs stepArgsFor(const CouplingState& C, const aprl_grid_geometry& g, const 
   S.hasExit = C.hasExit ? 1 : 0;
   S.window = window ? 1 : 0;
   S.exitLabZ = C.exitLabZ;
+  S.hasActivation = C.hasActivation ? 1 : 0;
+  S.activationLabZ = C.activationLabZ;
   return S;
 }
 
@@ -916,11 +952,14 @@ static void sortAndChunk(CouplingState& C, ParticleStore& P, const GArgs& G, con
   A.idDst = P.idT.get();
   A.stateSrc = P.state.get();
   A.stateDst = P.stateT.get();
+  A.betaSrc = P.beta.get();
+  A.betaDst = P.betaT.get();
   kPermute<<<blocks, 256, 0, st>>>(A, P.permSorted.get(), n);
   launchCheck("permute");
   for (int a = 0; a < 9; ++a) P.s[a].swap(P.t[a]);
   P.id.swap(P.idT);
   P.state.swap(P.stateT);
+  P.beta.swap(P.betaT);
   const int rz = C.shape == kShapeQuintic ? Box<kShapeQuintic>::RZ : Box<kShapeBinomial3X>::RZ;
   buildChunkLists(C, st, P.keySorted.get(), n, ntx, nty, ntiles, rz, occupied);
 }
@@ -1164,10 +1203,10 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
       for (int a = 0; a < 3; ++a) G0.j[a] = nullptr;
       ScopedKernel sk(F.profiler, st, "gather");
       if (quintic)
-        kGatherChunks<0, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G0, C.chunks.get(), ntx, nty, F.g.dt,
+        kGatherChunks<0, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G0, S, C.chunks.get(), ntx, nty, F.g.dt,
                                                                          nullptr, errT);
       else
-        kGatherChunks<0, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G0, C.chunks.get(), ntx, nty, F.g.dt,
+        kGatherChunks<0, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G0, S, C.chunks.get(), ntx, nty, F.g.dt,
                                                                             nullptr, errT);
       launchCheck("gather");
     }
@@ -1180,11 +1219,11 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
       {
         ScopedKernel sk(F.profiler, st, "deposit");
         if (quintic)
-          kDeposit<kShapeQuintic><<<nChunks, kChunk, 0, st>>>(A, G, C.chunks.get(), C.tileBegin.get(),
+          kDeposit<kShapeQuintic><<<nChunks, kChunk, 0, st>>>(A, G, S, C.chunks.get(), C.tileBegin.get(),
                                                               C.tileEnd.get(), C.tileScan.get(), ntx, nty,
                                                               C.partial.get(), C.tileShift.get(), errT);
         else
-          kDeposit<kShapeBinomial3X><<<nChunks, kChunk, 0, st>>>(A, G, C.chunks.get(), C.tileBegin.get(),
+          kDeposit<kShapeBinomial3X><<<nChunks, kChunk, 0, st>>>(A, G, S, C.chunks.get(), C.tileBegin.get(),
                                                                  C.tileEnd.get(), C.tileScan.get(), ntx, nty,
                                                                  C.partial.get(), C.tileShift.get(), errT);
         launchCheck("deposit");
@@ -1218,10 +1257,10 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
     {
       ScopedKernel sk(F.profiler, st, "residual");
       if (nChunks && quintic)
-        kGatherChunks<1, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G, C.chunks.get(), ntx, nty, F.g.dt,
+        kGatherChunks<1, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G, S, C.chunks.get(), ntx, nty, F.g.dt,
                                                                          C.reduceBlocks.get(), errT);
       else if (nChunks)
-        kGatherChunks<1, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G, C.chunks.get(), ntx, nty, F.g.dt,
+        kGatherChunks<1, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G, S, C.chunks.get(), ntx, nty, F.g.dt,
                                                                             C.reduceBlocks.get(), errT);
       kReduceBlocks<<<1, 256, 0, st>>>(C.reduceBlocks.get(), static_cast<int>(nChunks), C.reduceOut.get());
       launchCheck("residual");

@@ -36,6 +36,9 @@ enum ErrorBit : unsigned {
   kErrCpmlLocate = 1u << 17,       // particle_boundary.cpp:153-157 (runtime_error)
   kErrExitLocate = 1u << 18,       // particle_boundary.cpp:356-360 (runtime_error)
   kErrBehindInlet = 1u << 19,      // simulation_moving_window.cpp:359 (runtime_error)
+  kErrHeldRelease = 1u << 20,      // simulation.cpp:4966-4970 (runtime_error)
+  kErrHeldFrozenLeg = 1u << 21,    // simulation.cpp:4687-4691 (runtime_error)
+  kErrAdmission = 1u << 22,        // simulation.cpp:5141-5143 (runtime_error)
   // not an error: some entry ended a trial's segment at a boundary event
   kEvtAny = 1u << 31,
 };
@@ -45,7 +48,10 @@ enum ErrorBit : unsigned {
 // gamma) and holds its current weight in the weight slot. Removed and
 // cropped entries stay in the arrays until the next step's sort moves them
 // behind the live ones.
-enum EntryState : int { kStateActive = 0, kStateCarrier = 1, kStateRemoved = -1, kStateCropped = -2 };
+// A held entry (RelativisticParticleSI::frozen) moves z += c dt beta_held
+// with its transverse motion held until its lab z reaches the activation
+// plane; beta_held is kept per entry.
+enum EntryState : int { kStateActive = 0, kStateCarrier = 1, kStateHeld = 2, kStateRemoved = -1, kStateCropped = -2 };
 
 // Boundary counter slots (cumulative per coupling): CPML entries per face
 // (0-5, faces x- x+ y- y+ z- z+), outer exits of particles (6-11) and of
@@ -77,10 +83,11 @@ struct BoostDev {
 struct ParticleStore {
   std::size_t capacity = 0, count = 0;
   std::uint64_t generation = 0;  // bumped by every host upload
-  // SoA state: x y z ux uy uz q m w, id, entry state
+  // SoA state: x y z ux uy uz q m w, id, entry state, held speed
   DeviceArray<double> s[9], t[9];
   DeviceArray<uint64_t> id, idT;
   DeviceArray<int> state, stateT;
+  DeviceArray<double> beta, betaT;
   // Per-step slots: fields[0] and fields[1] alternate between the fields a
   // trial pushes with (Ex Ey Ez Bx By Bz) and the residual gather's next
   // fields; end[6] = trial end state x1, u1; gridAbs = per-substep work sum.
@@ -110,6 +117,7 @@ struct SplitStore {
   DeviceArray<float> s[9], t[9];  // offset x y z, u x y z, q m w
   DeviceArray<uint64_t> id, idT;
   DeviceArray<int> state, stateT;
+  DeviceArray<float> beta, betaT;  // held speed of held entries
   // per-step slots as in ParticleStore: alternating gathered fields, the
   // trial end state (cell1, offset1, u1) and the per-step work sum
   DeviceArray<float> fields[2][6], end[6], gridAbs;
@@ -140,6 +148,9 @@ struct CouplingState {
   std::size_t cpmlCells[3] = {0, 0, 0};
   double exitLabZ = 0.0;
   bool hasExit = false;
+  // lab activation plane of held particles (config reference.activation_plane_z)
+  double activationLabZ = 0.0;
+  bool hasActivation = false;
   DeviceArray<double> estar[3];
   // E_cand + E_old of the current trial (the preview writes the zero-current
   // sums on the whole grid; the current reduction rewrites the active tiles)

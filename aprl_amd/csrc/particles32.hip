@@ -60,6 +60,7 @@ struct PArgs32 {
   float* cut[3];     // t1, t2, carrier-part deposit weight
   float* wend;       // carrier end weight
   const float* w;    // weight slot (a carrier's current weight)
+  const float* beta; // held speed of held entries
   float kq[3];  // h_a / (dx dy dz dt): current density per unit charge and cell of motion
   double h[3];
   int n;
@@ -305,6 +306,8 @@ struct PermuteArgs32 {
   uint64_t* idDst;
   const int* stateSrc;
   int* stateDst;
+  const float* betaSrc;
+  float* betaDst;
 };
 
 __global__ void kPermute32(PermuteArgs32 A, const unsigned* perm, int n) {
@@ -317,6 +320,7 @@ __global__ void kPermute32(PermuteArgs32 A, const unsigned* perm, int n) {
   for (int a = 0; a < 9; ++a) A.dst[a][p] = A.src[a][q];
   A.idDst[p] = A.idSrc[q];
   A.stateDst[p] = A.stateSrc[q];
+  A.betaDst[p] = A.betaSrc[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -348,8 +352,9 @@ __device__ __forceinline__ void chordAt(const PArgs32& P, int p, bool guess, con
 // into fout, with the chunk's force max and defect / scale / call sums when
 // chunkOut is given.
 template <int MODE, int SHAPE>
-__global__ void __launch_bounds__(2 * kChunk) kGatherChunks32(PArgs32 P, GArgs32 G, const int4* chunks, int ntx,
-                                                              int nty, double dt, double* chunkOut, unsigned* errOut) {
+__global__ void __launch_bounds__(2 * kChunk) kGatherChunks32(PArgs32 P, GArgs32 G, StepArgs S, const int4* chunks,
+                                                              int ntx, int nty, double dt, double* chunkOut,
+                                                              unsigned* errOut) {
   constexpr bool kStageB = SHAPE == kShapeBinomial3X;
   __shared__ float es[3 * boxNodes<SHAPE>()];
   __shared__ float bs[kStageB ? 3 * boxNodes<SHAPE>() : 1];
@@ -361,14 +366,38 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks32(PArgs32 P, GArgs32
   const bool bWaves = threadIdx.x >= kChunk;
   const int lane = static_cast<int>(threadIdx.x) - (bWaves ? kChunk : 0);
   const int p = ch.y + lane;
-  // carriers move without fields and take no part in the residual
-  const bool live = p < ch.z && P.state[p] == kStateActive;
+  // carriers move without fields and take no part in the residual; a held
+  // entry takes part from its release point with the rest of the step
+  const int st = p < ch.z ? P.state[p] : kStateRemoved;
+  bool live = st == kStateActive, released = false;
+  double dtp = dt;
+  float dzk = 0.f;  // held leg to the release point, in cells
+  if (st == kStateHeld) {
+    double x0[3];
+    for (int d = 0; d < 3; ++d)
+      x0[d] = (static_cast<double>(P.cell[d][p]) + static_cast<double>(P.off[d][p])) * P.h[d];
+    const HeldMove h = heldMove(S, x0, static_cast<double>(P.beta[p]));
+    if (h.release && h.remaining > 0.0) {
+      live = released = true;
+      dtp = h.remaining;
+      dzk = static_cast<float>((h.z1 - x0[2]) / P.h[2]);
+    }
+  }
   unsigned err = 0;
   float g[3] = {0.f, 0.f, 0.f};
   int c0[3] = {0, 0, 0};
   float off0[3] = {0.f, 0.f, 0.f}, v[3] = {0.f, 0.f, 0.f};
   if (live) {
     chordAt(P, p, MODE == 0, G, c0, off0, v);
+    if (released) {
+      off0[2] += dzk;
+      if (MODE == 0) {
+        const float r = static_cast<float>(dtp / dt);
+        for (int d = 0; d < 3; ++d) v[d] *= r;
+      } else {
+        v[2] -= dzk;
+      }
+    }
     const Chord32 c = chordOf(G, c0[0], c0[1], c0[2], off0, v);
     if (bWaves) {
       if constexpr (SHAPE == kShapeBinomial3X) gatherBBinomial32(G, bs, box, c, g, err);
@@ -395,7 +424,7 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks32(PArgs32 P, GArgs32
         db[d] = static_cast<double>(xb[d * kChunk + lane]) - P.fin[3 + d][p];
       }
       const double q = P.q[p], m = P.m[p];
-      force = fabs(q / m) * dt / kC * (sqrt(de[0] * de[0] + de[1] * de[1] + de[2] * de[2]) +
+      force = fabs(q / m) * dtp / kC * (sqrt(de[0] * de[0] + de[1] * de[1] + de[2] * de[2]) +
                                         kC * sqrt(db[0] * db[0] + db[1] * db[1] + db[2] * db[2]));
       double w1 = 0., w2 = 0.;
       for (int d = 0; d < 3; ++d) {
@@ -455,24 +484,14 @@ __global__ void __launch_bounds__(128) kPushSplit(PArgs32 P, StepArgs S, double 
     x0[a] = (static_cast<double>(c0[a]) + static_cast<double>(P.off[a][p])) * P.h[a];
     u0[a] = P.u[a][p];
   }
-  double x[3], u[3], absGrid = 0.0, t1 = 0.0, t2 = 1.0, wDeposit = 0.0, wEnd = 0.0;
-  int code;
-  if (P.state[p] == kStateCarrier) {
-    const double scale = kC / gammaFromU(u0[0], u0[1], u0[2]);
-    for (int a = 0; a < 3; ++a) {
-      x[a] = x0[a] + S.dt * (u0[a] * scale);
-      u[a] = u0[a];
-    }
-    code = carrierSegment(S, o, x0, 0.0, x, static_cast<double>(P.w[p]), t2, wDeposit, wEnd, err);
-  } else {
-    double usedE[3], usedB[3];
-    for (int a = 0; a < 3; ++a) {
-      usedE[a] = P.fin[a][p];
-      usedB[a] = P.fin[3 + a][p];
-    }
-    pushChord(S, o, x0, u0, P.q[p], P.m[p], usedE, usedB, x, u, absGrid, err);
-    code = activeSegmentEvents(S, o, x0, x, t1, t2, wDeposit, wEnd, err);
+  double x[3], u[3], absGrid, t1, t2, wDeposit, wEnd, usedE[3], usedB[3];
+  for (int a = 0; a < 3; ++a) {
+    usedE[a] = P.fin[a][p];
+    usedB[a] = P.fin[3 + a][p];
   }
+  const int code = entryStep(S, o, P.state[p], x0, u0, P.q[p], P.m[p], static_cast<double>(P.w[p]),
+                             static_cast<double>(P.beta[p]), usedE, usedB, x, u, absGrid, t1, t2, wDeposit, wEnd,
+                             err);
   for (int a = 0; a < 3; ++a) {
     const double rel = static_cast<double>(P.off[a][p]) + (x[a] - x0[a]) / P.h[a];
     double fl = floor(rel);
@@ -506,7 +525,8 @@ __global__ void __launch_bounds__(128) kPushSplit(PArgs32 P, StepArgs S, double 
 // scheme is the FP64 form's: one exponent per tile from the tile's current
 // bound, integer sums independent of order.
 template <int SHAPE>
-__global__ void __launch_bounds__(kChunk) kDepositSplit(PArgs32 P, GArgs32 G, const int4* chunks, const int* tileBegin,
+__global__ void __launch_bounds__(kChunk) kDepositSplit(PArgs32 P, GArgs32 G, StepArgs S, const int4* chunks,
+                                                        const int* tileBegin,
                                                         const int* tileEnd, const unsigned long long* tileScan,
                                                         int ntx, int nty, long long* partial, int* tileShift,
                                                         unsigned* errOut) {
@@ -530,6 +550,11 @@ __global__ void __launch_bounds__(kChunk) kDepositSplit(PArgs32 P, GArgs32 G, co
     chordAt(P, p, false, G, c0, off0, v);
     scaleOf(p, v, s);
     bsum += fmax(fabs(static_cast<double>(s[0])), fmax(fabs(static_cast<double>(s[1])), fabs(static_cast<double>(s[2]))));
+    // a held entry's release kink splits its path: both legs stay below
+    // the chord plus the held leg
+    if (P.state[p] == kStateHeld)
+      bsum += fabs(static_cast<double>(P.q[p])) * static_cast<double>(P.kq[2]) *
+              (kC * S.dt * fabs(static_cast<double>(P.beta[p])) / P.h[2]);
   }
   bound[threadIdx.x] = bsum;
   __syncthreads();
@@ -548,10 +573,21 @@ __global__ void __launch_bounds__(kChunk) kDepositSplit(PArgs32 P, GArgs32 G, co
     int c0[3];
     float off0[3], v[3], scale[3];
     chordAt(P, p, false, G, c0, off0, v);
-    scaleOf(p, v, scale);
-    const Chord32 c = chordOf(G, c0[0], c0[1], c0[2], off0, v);
+    // A held entry released inside the step deposits its held leg to the
+    // release point first, then the rest of the chord.
+    float dzk = 0.f;
+    int legs = 1;
+    if (P.state[p] == kStateHeld) {
+      double x0[3];
+      for (int d = 0; d < 3; ++d) x0[d] = (static_cast<double>(c0[d]) + static_cast<double>(off0[d])) * P.h[d];
+      const HeldMove h = heldMove(S, x0, static_cast<double>(P.beta[p]));
+      if (h.release && h.kink) {
+        legs = 2;
+        dzk = static_cast<float>((h.z1 - x0[2]) / P.h[2]);
+      }
+    }
+    Chord32 c = chordOf(G, c0[0], c0[1], c0[2], off0, v);
     float cuts[kMaxCuts];
-    const int count = orbitCuts(c.x, c.v, cuts, err);
     int base = 0;
     auto interior = [&](long long i, long long j, long long k, int wx, int wy, int wz) {
       if (i < 0 || j < 0 || k < 0 || i + wx > G.nx + 1 || j + wy > G.ny + 1 || k + wz > G.nz + 1) return false;
@@ -584,21 +620,35 @@ __global__ void __launch_bounds__(kChunk) kDepositSplit(PArgs32 P, GArgs32 G, co
       if (l < 0) return;
       atomicAdd(&acc[d * N + l], fixedOf(scale[d] * w * mult));
     };
-    // A boundary event splits the chord: [0, t1] at unit weight (the active
-    // part) and [t1, t2] at the carrier part's deposit weight.
+    // A boundary event splits the last leg: [0, t1] at unit weight (the
+    // active part) and [t1, t2] at the carrier part's deposit weight.
     const int e = P.evt[p];
     const float t1 = e ? P.cut[0][p] : 1.f, t2 = e ? P.cut[1][p] : 1.f, wCarrier = e ? P.cut[2][p] : 0.f;
-    for (int s = 1; s < count; ++s) {
-      const float k0 = cuts[s - 1], k1 = cuts[s];
-      if (k1 - k0 <= 0.f) continue;
-      // without an event: one part [k0, k1] (t1 = t2 = 1)
 #pragma unroll 1
-      for (int part = 0; part < 2; ++part) {
-        const float a = part ? fmaxf(k0, t1) : k0;
-        const float b = part ? fminf(k1, t2) : fminf(k1, t1);
-        if (!(b > a)) continue;
-        mult = part ? wCarrier : 1.f;
-        segmentTaps<SHAPE, float>(G.nx, G.ny, c.org, c.x, c.v, a, b, interior, fast, visit);
+    for (int leg = 0; leg < legs; ++leg) {
+      const bool last = leg + 1 == legs;
+      if (legs == 2) {
+        // leg 0: (0, 0, dzk) from the start; leg 1: the rest from the kink
+        c.x[2] = off0[2] + (leg ? dzk : 0.f);
+        c.v[0] = leg ? v[0] : 0.f;
+        c.v[1] = leg ? v[1] : 0.f;
+        c.v[2] = leg ? v[2] - dzk : dzk;
+      }
+      scaleOf(p, c.v, scale);
+      const int count = orbitCuts(c.x, c.v, cuts, err);
+      const float f1 = last ? t1 : 1.f, f2 = last ? t2 : 1.f, fw = last ? wCarrier : 0.f;
+      for (int sgm = 1; sgm < count; ++sgm) {
+        const float k0 = cuts[sgm - 1], k1 = cuts[sgm];
+        if (k1 - k0 <= 0.f) continue;
+        // without an event: one part [k0, k1] (t1 = t2 = 1)
+#pragma unroll 1
+        for (int part = 0; part < 2; ++part) {
+          const float a = part ? fmaxf(k0, f1) : k0;
+          const float b = part ? fminf(k1, f2) : fminf(k1, f1);
+          if (!(b > a)) continue;
+          mult = part ? fw : 1.f;
+          segmentTaps<SHAPE, float>(G.nx, G.ny, c.org, c.x, c.v, a, b, interior, fast, visit);
+        }
       }
     }
   }
@@ -687,6 +737,8 @@ void reserveSplitParticles(SplitStore& P, std::size_t capacity) {
   P.idT.resize(capacity);
   P.state.resize(capacity);
   P.stateT.resize(capacity);
+  P.beta.resize(capacity);
+  P.betaT.resize(capacity);
   P.evt.resize(capacity);
   for (int a = 0; a < 3; ++a) P.cut[a].resize(capacity);
   P.wend.resize(capacity);
@@ -820,6 +872,7 @@ PArgs32 particleArgs32(SplitStore& P, const FieldState32& F) {
   A.q = P.s[6].get();
   A.m = P.s[7].get();
   A.w = P.s[8].get();
+  A.beta = P.beta.get();
   A.gridAbs = P.gridAbs.get();
   A.state = P.state.get();
   A.evt = P.evt.get();
@@ -874,12 +927,15 @@ void sortSplit(CouplingState& C, SplitStore& P, const GArgs32& G, const SortArgs
   A.idDst = P.idT.get();
   A.stateSrc = P.state.get();
   A.stateDst = P.stateT.get();
+  A.betaSrc = P.beta.get();
+  A.betaDst = P.betaT.get();
   kPermute32<<<blocks, 256, 0, st>>>(A, P.permSorted.get(), n);
   launchCheck("permute");
   for (int a = 0; a < 3; ++a) P.cell[a].swap(P.cellT[a]);
   for (int a = 0; a < 9; ++a) P.s[a].swap(P.t[a]);
   P.id.swap(P.idT);
   P.state.swap(P.stateT);
+  P.beta.swap(P.betaT);
   const int rz = C.shape == kShapeQuintic ? Box<kShapeQuintic>::RZ : Box<kShapeBinomial3X>::RZ;
   buildChunkLists(C, st, P.keySorted.get(), n, ntx, nty, ntiles, rz, occupied);
 }
@@ -1025,10 +1081,10 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
     if (nChunks && trial == 0) {
       ScopedKernel sk(F.profiler, st, "gather");
       if (quintic)
-        kGatherChunks32<0, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G, C.chunks.get(), ntx, nty, F.g.dt,
+        kGatherChunks32<0, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G, S, C.chunks.get(), ntx, nty, F.g.dt,
                                                                            nullptr, errT);
       else
-        kGatherChunks32<0, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G, C.chunks.get(), ntx, nty,
+        kGatherChunks32<0, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G, S, C.chunks.get(), ntx, nty,
                                                                               F.g.dt, nullptr, errT);
       launchCheck("gather");
     }
@@ -1044,11 +1100,11 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
     {
       ScopedKernel sk(F.profiler, st, "deposit");
       if (quintic)
-        kDepositSplit<kShapeQuintic><<<nChunks, kChunk, 0, st>>>(A, G, C.chunks.get(), C.tileBegin.get(),
+        kDepositSplit<kShapeQuintic><<<nChunks, kChunk, 0, st>>>(A, G, S, C.chunks.get(), C.tileBegin.get(),
                                                                  C.tileEnd.get(), C.tileScan.get(), ntx, nty,
                                                                  C.partial.get(), C.tileShift.get(), errT);
       else
-        kDepositSplit<kShapeBinomial3X><<<nChunks, kChunk, 0, st>>>(A, G, C.chunks.get(), C.tileBegin.get(),
+        kDepositSplit<kShapeBinomial3X><<<nChunks, kChunk, 0, st>>>(A, G, S, C.chunks.get(), C.tileBegin.get(),
                                                                     C.tileEnd.get(), C.tileScan.get(), ntx, nty,
                                                                     C.partial.get(), C.tileShift.get(), errT);
       launchCheck("deposit");
@@ -1085,10 +1141,10 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
       ScopedKernel sk(F.profiler, st, "residual");
       double* stats = last ? C.reduceBlocks.get() : nullptr;
       if (quintic)
-        kGatherChunks32<1, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G, C.chunks.get(), ntx, nty, F.g.dt,
+        kGatherChunks32<1, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G, S, C.chunks.get(), ntx, nty, F.g.dt,
                                                                            stats, errT);
       else
-        kGatherChunks32<1, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G, C.chunks.get(), ntx, nty,
+        kGatherChunks32<1, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G, S, C.chunks.get(), ntx, nty,
                                                                               F.g.dt, stats, errT);
       launchCheck("residual");
       if (last) reduceChunkStats(st, C.reduceBlocks.get(), static_cast<int>(nChunks), C.reduceOut.get());

@@ -159,9 +159,17 @@ aprl_status aprl_particles_upload(aprl_particles* particles, size_t n, const dou
 aprl_status aprl_particles_download(aprl_particles* particles, double* position3, double* u3, double* charge,
                                     double* mass, double* weight, uint64_t* id);
 size_t aprl_particles_count(const aprl_particles* particles);
-/* Entry state per live entry, in download order: 0 particle, 1 CPML carrier
- * (no counterpart in the reference, which keeps the two in separate lists). */
+/* Entry state per live entry, in download order: 0 particle, 1 CPML carrier,
+ * 2 held particle (no counterpart in the reference, which keeps carriers in
+ * a separate list and flags held particles in RelativisticParticleSI). */
 aprl_status aprl_particles_download_states(aprl_particles* particles, int* state);
+/* RelativisticParticleSI::frozen / frozenBetaZ (eb_particles.h:21-22) for
+ * the entries in upload order, before the store's first step: a held
+ * particle moves z += c dt beta with its transverse motion and momentum held
+ * until its lab z reaches the activation plane (prepareParticle's
+ * hold-transverse branch, simulation.cpp:4951-5000); upstream of a moving
+ * window it deposits only its shaped support until it crosses the inlet. */
+aprl_status aprl_particles_set_held(aprl_particles* particles, const int* frozen, const double* beta);
 
 /* Longitudinal source shape (longitudinal_shape.h:16). */
 typedef enum aprl_shape {
@@ -228,6 +236,9 @@ aprl_status aprl_coupling_set_reference_current(aprl_coupling* coupling, const d
  * (carriers excluded) whose longitudinal shape reaches node k = 1. */
 aprl_status aprl_window_complete_inlet(aprl_fields* fields, aprl_coupling* coupling, aprl_particles* particles,
                                        const double* origin);
+/* Lab z of the held particles' activation plane (config
+ * reference.activation_plane_z); NaN disables the release. */
+aprl_status aprl_coupling_set_activation(aprl_coupling* coupling, double activation_lab_z);
 /* Cumulative particle-boundary counters since the coupling was created:
  * counts[20] / charges[20] (C) hold the CPML entries per face (0-5, faces
  * x- x+ y- y+ z- z+), the outer-face exits of particles (6-11) and of
