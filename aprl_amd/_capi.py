@@ -55,6 +55,12 @@ class InstanceStep(C.Structure):
                 ("report", StepReport)]
 
 
+class PendingGeometry(C.Structure):
+    """aprl_pending_geometry (PendingWindowGeometry, pending_particle_queue.h:16-23)."""
+    _fields_ = [("reference_origin_z", C.c_double), ("initial_offset", C.c_int64), ("dz", C.c_double),
+                ("held_distance_per_beta", C.c_double), ("maximum_step", C.c_uint64)]
+
+
 class AprlError(RuntimeError):
     """A failed C-ABI call; ``status`` is the aprl_status code."""
 
@@ -128,6 +134,16 @@ def lib() -> C.CDLL:
     L.aprl_particles_download_states.argtypes = [vp, C.POINTER(C.c_int)]
     L.aprl_particles_set_held.argtypes = [vp, C.POINTER(C.c_int), _dp]
     L.aprl_coupling_set_activation.argtypes = [vp, C.c_double]
+    L.aprl_repeat_held_addition.argtypes = [C.c_double, C.c_double, C.c_uint64, _dp]
+    L.aprl_pending_create.argtypes = [C.POINTER(PendingGeometry), C.c_uint64, C.POINTER(vp)]
+    L.aprl_pending_destroy.argtypes = [vp]
+    L.aprl_pending_destroy.restype = None
+    L.aprl_pending_extract.argtypes = [vp, _sz, _dp, _dp, _dp, _dp, _dp, u64p, C.POINTER(C.c_int), _dp,
+                                       C.POINTER(_sz)]
+    L.aprl_pending_admit.argtypes = [vp, C.c_uint64, C.c_uint64, vp, C.POINTER(_sz)]
+    L.aprl_pending_view.argtypes = [vp, C.c_uint64, C.c_uint64, _sz, _dp, u64p, C.POINTER(_sz)]
+    L.aprl_pending_state.argtypes = [vp, u64p, u64p, u64p]
+    L.aprl_pending_origin.argtypes = [vp, C.c_uint64, _dp]
     L.aprl_window_set_charge_match.argtypes = [vp, C.c_int]
     L.aprl_window_begin_step_centered.argtypes = [vp, _sz, C.c_ssize_t, C.POINTER(_dp), _dp]
     L.aprl_window_complete_inlet.argtypes = [vp, vp, vp, _dp]
@@ -190,6 +206,8 @@ EXPORTED = [
     "aprl_particles_create", "aprl_particles_destroy", "aprl_particles_upload", "aprl_particles_download",
     "aprl_particles_count", "aprl_particles_download_states", "aprl_coupling_boundary_counters",
     "aprl_particles_set_held", "aprl_coupling_set_activation",
+    "aprl_repeat_held_addition", "aprl_pending_create", "aprl_pending_destroy", "aprl_pending_extract",
+    "aprl_pending_admit", "aprl_pending_view", "aprl_pending_state", "aprl_pending_origin",
     "aprl_coupling_create", "aprl_coupling_destroy", "aprl_coupling_set_frame",
     "aprl_coupling_add_magnet", "aprl_coupling_set_particle_boundary", "aprl_coupling_step",
     "aprl_coupling_set_split_trials", "aprl_window_step_batch",

@@ -38,7 +38,9 @@ struct InletArgs {
 // Node-1 weight and transverse position of one particle, or false when it
 // does not reach the inlet (ChargeMatchedInlet::particle's filters).
 __device__ bool inletWeights(const InletArgs& A, int p, double& wz, double& x, double& y, double& q) {
-  if (A.state[p] != kStateActive) return false;
+  // particles_ with the held ones (pending particles near the inlet are
+  // held entries of the store here); carriers and removed entries do not count
+  if (A.state[p] != kStateActive && A.state[p] != kStateHeld) return false;
   double l[3];
   for (int a = 0; a < 3; ++a)
     l[a] = A.x[0] ? (A.x[a][p] - A.o[a]) / A.h[a]

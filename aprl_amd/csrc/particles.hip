@@ -838,6 +838,50 @@ void reserveParticles(ParticleStore& P, std::size_t capacity) {
   P.sortTemp.resize(P.sortTempBytes);
 }
 
+// Grows the store to `capacity` entries keeping the persistent state of
+// its first `count` entries (SoA, ids, entry states, held speeds); the
+// per-step arrays are rewritten by every step.
+void growParticles(ParticleStore& P, std::size_t capacity) {
+  if (capacity <= P.capacity) return;
+  DeviceArray<double> s[9], beta;
+  DeviceArray<uint64_t> id;
+  DeviceArray<int> state;
+  for (int a = 0; a < 9; ++a) s[a].swap(P.s[a]);
+  id.swap(P.id);
+  state.swap(P.state);
+  beta.swap(P.beta);
+  reserveParticles(P, capacity);
+  const std::size_t n = P.count;
+  if (!n) return;
+  for (int a = 0; a < 9; ++a)
+    hipCheck(hipMemcpy(P.s[a].get(), s[a].get(), n * sizeof(double), hipMemcpyDeviceToDevice), "hipMemcpy");
+  hipCheck(hipMemcpy(P.id.get(), id.get(), n * sizeof(uint64_t), hipMemcpyDeviceToDevice), "hipMemcpy");
+  hipCheck(hipMemcpy(P.state.get(), state.get(), n * sizeof(int), hipMemcpyDeviceToDevice), "hipMemcpy");
+  hipCheck(hipMemcpy(P.beta.get(), beta.get(), n * sizeof(double), hipMemcpyDeviceToDevice), "hipMemcpy");
+}
+
+void appendParticles(ParticleStore& P, std::size_t k, const double* x3, const double* u3, const double* q,
+                     const double* m, const double* w, const uint64_t* id, const int* state, const double* beta) {
+  if (!k) return;
+  const std::size_t n = P.count;
+  if (n + k > P.capacity) growParticles(P, std::max(n + k, P.capacity + P.capacity / 4 + 1024));
+  std::vector<double> col(k);
+  for (int a = 0; a < 3; ++a) {
+    for (std::size_t i = 0; i < k; ++i) col[i] = x3[3 * i + a];
+    hipCheck(hipMemcpy(P.s[a].get() + n, col.data(), k * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+    for (std::size_t i = 0; i < k; ++i) col[i] = u3[3 * i + a];
+    hipCheck(hipMemcpy(P.s[3 + a].get() + n, col.data(), k * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+  }
+  const double* scalars[3] = {q, m, w};
+  for (int a = 0; a < 3; ++a)
+    hipCheck(hipMemcpy(P.s[6 + a].get() + n, scalars[a], k * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+  hipCheck(hipMemcpy(P.id.get() + n, id, k * sizeof(uint64_t), hipMemcpyHostToDevice), "hipMemcpy");
+  hipCheck(hipMemcpy(P.state.get() + n, state, k * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+  hipCheck(hipMemcpy(P.beta.get() + n, beta, k * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+  P.count = n + k;
+  ++P.generation;
+}
+
 void raiseParticleErrors(unsigned e) {
   e &= ~kEvtAny;
   if (!e) return;

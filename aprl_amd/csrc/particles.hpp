@@ -222,6 +222,11 @@ void fillInts(hipStream_t st, int* p, int n, int v);
 void reduceChunkStats(hipStream_t st, const double* in, int chunks, double* out);
 void raiseParticleErrors(unsigned e);
 void reserveParticles(ParticleStore& P, std::size_t capacity);
+// Appends k host entries after the store's count (growing it and keeping
+// what it holds): positions and momenta as xyz triples, per-entry entry
+// state and held speed.
+void appendParticles(ParticleStore& P, std::size_t k, const double* x3, const double* u3, const double* q,
+                     const double* m, const double* w, const uint64_t* id, const int* state, const double* beta);
 // Error words and boundary counters of a coupling (allocated and cleared
 // once).
 void reserveCouplingWords(CouplingState& C, hipStream_t st);
@@ -249,6 +254,9 @@ void tileSort(void* temp, std::size_t& bytes, unsigned* key, unsigned* keySorted
 
 // FP32-split form (particles32.hip).
 void reserveSplitParticles(SplitStore& P, std::size_t capacity);
+void appendSplitParticles(SplitStore& P, const FieldState32& F, std::size_t k, const double* x3, const double* u3,
+                          const double* q, const double* m, const double* w, const uint64_t* id, const int* state,
+                          const double* beta);
 void uploadSplitParticles(SplitStore& P, const FieldState32& F, std::size_t n, const double* x3, const double* u3,
                           const double* q, const double* m, const double* w, const uint64_t* id);
 void downloadSplitParticles(const SplitStore& P, const FieldState32& F, double* x3, double* u3, double* q, double* m,

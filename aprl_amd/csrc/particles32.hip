@@ -800,6 +800,64 @@ void uploadSplitParticles(SplitStore& P, const FieldState32& F, std::size_t n, c
   ++P.generation;
 }
 
+// Grows the store keeping the persistent state of its first `count`
+// entries (cells, offsets, momenta, charges, ids, entry states, held
+// speeds); the per-step arrays are rewritten by every step.
+void growSplitParticles(SplitStore& P, std::size_t capacity) {
+  if (capacity <= P.capacity) return;
+  DeviceArray<int> cell[3], state;
+  DeviceArray<float> s[9], beta;
+  DeviceArray<uint64_t> id;
+  for (int a = 0; a < 3; ++a) cell[a].swap(P.cell[a]);
+  for (int a = 0; a < 9; ++a) s[a].swap(P.s[a]);
+  id.swap(P.id);
+  state.swap(P.state);
+  beta.swap(P.beta);
+  reserveSplitParticles(P, capacity);
+  const std::size_t n = P.count;
+  if (!n) return;
+  for (int a = 0; a < 3; ++a)
+    hipCheck(hipMemcpy(P.cell[a].get(), cell[a].get(), n * sizeof(int), hipMemcpyDeviceToDevice), "hipMemcpy");
+  for (int a = 0; a < 9; ++a)
+    hipCheck(hipMemcpy(P.s[a].get(), s[a].get(), n * sizeof(float), hipMemcpyDeviceToDevice), "hipMemcpy");
+  hipCheck(hipMemcpy(P.id.get(), id.get(), n * sizeof(uint64_t), hipMemcpyDeviceToDevice), "hipMemcpy");
+  hipCheck(hipMemcpy(P.state.get(), state.get(), n * sizeof(int), hipMemcpyDeviceToDevice), "hipMemcpy");
+  hipCheck(hipMemcpy(P.beta.get(), beta.get(), n * sizeof(float), hipMemcpyDeviceToDevice), "hipMemcpy");
+}
+
+void appendSplitParticles(SplitStore& P, const FieldState32& F, std::size_t k, const double* x3, const double* u3,
+                          const double* q, const double* m, const double* w, const uint64_t* id, const int* state,
+                          const double* beta) {
+  if (!k) return;
+  const std::size_t n = P.count;
+  const double h[3] = {F.g.dx, F.g.dy, F.g.dz};
+  std::vector<float> col(k);
+  // convert before growing, so that a rejected position leaves the store as it was
+  std::vector<int> allCells(3 * k);
+  std::vector<float> offsets(3 * k);
+  for (int a = 0; a < 3; ++a)
+    for (std::size_t i = 0; i < k; ++i) splitCoordinate(x3[3 * i + a], h[a], allCells[a * k + i], offsets[a * k + i]);
+  if (n + k > P.capacity) growSplitParticles(P, std::max(n + k, P.capacity + P.capacity / 4 + 1024));
+  for (int a = 0; a < 3; ++a) {
+    hipCheck(hipMemcpy(P.cell[a].get() + n, allCells.data() + a * k, k * sizeof(int), hipMemcpyHostToDevice),
+             "hipMemcpy");
+    hipCheck(hipMemcpy(P.s[a].get() + n, offsets.data() + a * k, k * sizeof(float), hipMemcpyHostToDevice),
+             "hipMemcpy");
+    for (std::size_t i = 0; i < k; ++i) col[i] = static_cast<float>(u3[3 * i + a]);
+    hipCheck(hipMemcpy(P.s[3 + a].get() + n, col.data(), k * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+  }
+  const double* scalars[4] = {q, m, w, beta};
+  DeviceArray<float>* dst[4] = {&P.s[6], &P.s[7], &P.s[8], &P.beta};
+  for (int a = 0; a < 4; ++a) {
+    for (std::size_t i = 0; i < k; ++i) col[i] = static_cast<float>(scalars[a][i]);
+    hipCheck(hipMemcpy(dst[a]->get() + n, col.data(), k * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+  }
+  hipCheck(hipMemcpy(P.id.get() + n, id, k * sizeof(uint64_t), hipMemcpyHostToDevice), "hipMemcpy");
+  hipCheck(hipMemcpy(P.state.get() + n, state, k * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+  P.count = n + k;
+  ++P.generation;
+}
+
 void downloadSplitParticles(const SplitStore& P, const FieldState32& F, double* x3, double* u3, double* q, double* m,
                             double* w, uint64_t* id) {
   hipCheck(hipStreamSynchronize(F.stream), "hipStreamSynchronize");

@@ -239,6 +239,56 @@ aprl_status aprl_window_complete_inlet(aprl_fields* fields, aprl_coupling* coupl
 /* Lab z of the held particles' activation plane (config
  * reference.activation_plane_z); NaN disables the release. */
 aprl_status aprl_coupling_set_activation(aprl_coupling* coupling, double activation_lab_z);
+
+/* Pending admission queue of the moving window (PendingParticleQueue,
+ * pending_particle_queue.h:25-76): held particles upstream of the inlet
+ * wait in host memory, each with the step in which its held path first
+ * reaches the inlet, and enter a device store as held entries. */
+typedef struct aprl_pending aprl_pending;
+/* PendingWindowGeometry (pending_particle_queue.h:16-23): the inlet of step
+ * s is fma(initial_offset - s, dz, reference_origin_z); a held particle
+ * moves held_distance_per_beta * beta per step (pass c * dt). */
+typedef struct aprl_pending_geometry {
+  double reference_origin_z;
+  int64_t initial_offset;
+  double dz;
+  double held_distance_per_beta;
+  uint64_t maximum_step; /* last step of the run */
+} aprl_pending_geometry;
+/* repeatHeldAddition (pending_particle_queue.h:11-14): the value of
+ * `count` separately rounded binary64 additions x += increment. */
+aprl_status aprl_repeat_held_addition(double initial, double increment, uint64_t count, double* out);
+/* PendingParticleQueue(geometry, initialStep) (pending_particle_queue.h:
+ * 30-31); invalid geometry or horizon -> APRL_INVALID_ARGUMENT. */
+aprl_status aprl_pending_create(const aprl_pending_geometry* geometry, uint64_t initial_step, aprl_pending** out);
+void aprl_pending_destroy(aprl_pending* queue);
+/* extract (pending_particle_queue.h:33-36, .cpp:148-175) over host arrays:
+ * held particles upstream of the initial inlet with beta >= 0 that do not
+ * take part in the initial step move to the queue; the arrays are compacted
+ * in place, order kept, and *retained is the number left. Once per queue. */
+aprl_status aprl_pending_extract(aprl_pending* queue, size_t n, double* x3, double* u3, double* q, double* m,
+                                 double* w, uint64_t* id, int* frozen, double* beta, size_t* retained);
+/* admit (pending_particle_queue.h:38-43, .cpp:184-206) into a device store,
+ * before the step: appends the waiting particles due at or before
+ * min(step + look_ahead, maximum_step) as held entries, in due order, with
+ * z at the start of `step`. look_ahead = 0 is the reference's admit(step).
+ * look_ahead = 8 also holds in the store the particles whose shaped support
+ * the reference deposits through visitNear(step, 8) (simulation.cpp:
+ * 4170-4172) and counts in the inlet charge (simulation_moving_window.cpp:
+ * 317): an upstream held entry deposits exactly that support. Steps must
+ * come in order and skip no due step (APRL_LOGIC_ERROR). */
+aprl_status aprl_pending_admit(aprl_pending* queue, uint64_t step, uint64_t look_ahead, aprl_particles* particles,
+                               size_t* admitted);
+/* visitNear (pending_particle_queue.h:48-49) as arrays: the first
+ * `capacity` waiting particles due by step + look_ahead, z at the start of
+ * `step`; *n = how many there are. */
+aprl_status aprl_pending_view(const aprl_pending* queue, uint64_t step, uint64_t look_ahead, size_t capacity,
+                              double* x3, uint64_t* id, size_t* n);
+/* size, nextDueStep (maximum_step + 1: none due within the horizon),
+ * admittedCount (pending_particle_queue.h:51-62); NULL outputs skipped. */
+aprl_status aprl_pending_state(const aprl_pending* queue, uint64_t* waiting, uint64_t* next_due, uint64_t* admitted);
+/* origin (pending_particle_queue.h:56): the inlet z of a step. */
+aprl_status aprl_pending_origin(const aprl_pending* queue, uint64_t step, double* z);
 /* Cumulative particle-boundary counters since the coupling was created:
  * counts[20] / charges[20] (C) hold the CPML entries per face (0-5, faces
  * x- x+ y- y+ z- z+), the outer-face exits of particles (6-11) and of

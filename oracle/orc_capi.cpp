@@ -2,10 +2,12 @@
 // into the product). Entry points mirror oracle/ref_driver.cpp (prefix ref_)
 // so the same Python test drives the restatement and the reference build.
 #include <cstring>
+#include <memory>
 #include <string>
 
 #include "orc_fields.hpp"
 #include "orc_particles.hpp"
+#include "orc_pending.hpp"
 
 namespace {
 thread_local std::string lastError;
@@ -164,6 +166,10 @@ struct CouplingHandle {
   orc::BoundaryCounters counters;
   orc::RefCurrent ref;
   orc::HeldRelease held;
+  // pending admission queue (rank 0 of the reference) and the step of the
+  // last admission
+  std::unique_ptr<orc::PendingQueue> pending;
+  std::uint64_t pendingStep = 0;
   std::size_t cpmlCells[3] = {0, 0, 0};
   // Particle-boundary geometry: the grid's CPML grading with the particle
   // cells above.
@@ -250,8 +256,10 @@ int orc_coupling_step(void* h, const double* origin, double time, unsigned subst
     pb.set(f.g, vec(origin), H.boundaryParams());
     pb.window = static_cast<bool>(f.window);
     pb.exitLabZ = exitLabZ;
-    const orc::StepResult r =
-        orc::sourceStep(f, H.c, H.particles, H.carriers, H.counters, vec(origin), time, substeps, pb, &H.ref, H.held);
+    std::vector<orc::Particle> waiting;
+    if (H.pending) waiting = H.pending->near(H.pendingStep, 8);
+    const orc::StepResult r = orc::sourceStep(f, H.c, H.particles, H.carriers, H.counters, vec(origin), time,
+                                              substeps, pb, &H.ref, H.held, H.pending ? &waiting : nullptr);
     *iterations = r.iterations; *force = r.force; *work = r.work;
   });
 }
@@ -327,9 +335,68 @@ int orc_coupling_complete_inlet(void* h, const double* newOrigin) {
     CouplingHandle& H = *CH(h);
     orc::Fields& f = *H.c.fields;
     if (!f.window || !f.window->centered) throw std::logic_error("The window has no charge-matched inlet");
-    orc::chargeMatchedInlet(f, H.particles, vec(newOrigin), H.c.shape, H.c.smoothX, f.cpml->p.cells[0],
+    std::vector<orc::Particle> sources = H.particles;
+    // with the queue: its particles near the inlet at the new step
+    // (simulation_moving_window.cpp:317, wide shapes)
+    if (H.pending)
+      for (const orc::Particle& p : H.pending->near(f.window->step, 8)) sources.push_back(p);
+    orc::chargeMatchedInlet(f, sources, vec(newOrigin), H.c.shape, H.c.smoothX, f.cpml->p.cells[0],
                             f.cpml->p.cells[1]);
   });
+}
+
+// PendingParticleQueue over the coupling's particles (set_particles and
+// set_held first): the queue of initial step `initialStep` extracts its
+// particles; admit(step) appends the due ones before the step.
+int orc_coupling_set_pending(void* h, double referenceOriginZ, std::int64_t initialOffset, double dz,
+                             double heldDistancePerBeta, std::uint64_t maximumStep, std::uint64_t initialStep) {
+  return guarded([&] {
+    CouplingHandle& H = *CH(h);
+    orc::PendingGeometry g;
+    g.referenceOriginZ = referenceOriginZ;
+    g.initialOffset = initialOffset;
+    g.dz = dz;
+    g.heldDistancePerBeta = heldDistancePerBeta;
+    g.maximumStep = maximumStep;
+    H.pending.reset(new orc::PendingQueue(g, initialStep));
+    H.pending->extract(H.particles);
+    H.pendingStep = initialStep;
+  });
+}
+int orc_coupling_admit(void* h, std::uint64_t step, std::size_t* admitted) {
+  return guarded([&] {
+    CouplingHandle& H = *CH(h);
+    if (!H.pending) throw std::logic_error("No pending queue");
+    *admitted = H.pending->admit(step, H.particles);
+    H.pendingStep = step;
+  });
+}
+// waiting, next due step, admitted
+int orc_coupling_pending_state(void* h, std::uint64_t* out3) {
+  return guarded([&] {
+    CouplingHandle& H = *CH(h);
+    if (!H.pending) throw std::logic_error("No pending queue");
+    out3[0] = H.pending->size();
+    out3[1] = H.pending->nextDueStep();
+    out3[2] = H.pending->admitted();
+  });
+}
+// visitNear(step, lookAhead): z and ids of up to `capacity` waiting particles
+int orc_coupling_pending_near(void* h, std::uint64_t step, std::uint64_t lookAhead, std::size_t capacity, double* z,
+                              std::uint64_t* id, std::size_t* n) {
+  return guarded([&] {
+    CouplingHandle& H = *CH(h);
+    if (!H.pending) throw std::logic_error("No pending queue");
+    const std::vector<orc::Particle> near = H.pending->near(step, lookAhead);
+    *n = near.size();
+    for (std::size_t i = 0; i < near.size() && i < capacity; ++i) {
+      z[i] = near[i].position[2];
+      id[i] = near[i].id;
+    }
+  });
+}
+int orc_repeat_held_addition(double initial, double increment, std::uint64_t count, double* out) {
+  return guarded([&] { *out = orc::repeatHeldAddition(initial, increment, count); });
 }
 
 // Reference surface current: table of rows x points (the point order of

@@ -1000,7 +1000,16 @@ struct HeldRelease { double activationLabZ = std::numeric_limits<double>::infini
 
 inline void pushDepositAll(Coupling& c, std::vector<Particle>& particles, std::vector<Carrier>& carriers,
                            BoundaryCounters& k, double time, double dt, unsigned substeps,
-                           const ParticleBoundary& pb, const V3& windowOrigin, const HeldRelease& held = HeldRelease()) {
+                           const ParticleBoundary& pb, const V3& windowOrigin, const HeldRelease& held = HeldRelease(),
+                           const std::vector<Particle>* waiting = nullptr) {
+  // the pending queue's particles near the inlet deposit their held
+  // support first (visitNear(step, 8), simulation.cpp:4170-4172)
+  if (waiting)
+    for (const Particle& p : *waiting) {
+      V3 end = p.position;
+      end[2] += kC * dt * p.betaHeld;
+      c.deposit(p.position, end, p.charge);
+    }
   std::vector<Carrier> retainedCarriers;
   retainedCarriers.reserve(carriers.size());
   for (const Carrier& original : carriers) {
@@ -1262,7 +1271,7 @@ inline void chargeMatchedInlet(Fields& f, const std::vector<Particle>& particles
 inline StepResult sourceStep(Fields& f, Coupling& c, std::vector<Particle>& particles, std::vector<Carrier>& carriers,
                              BoundaryCounters& counters, const V3& origin, double time, unsigned substeps,
                              const ParticleBoundary& pb, const RefCurrent* ref = nullptr,
-                             const HeldRelease& held = HeldRelease()) {
+                             const HeldRelease& held = HeldRelease(), const std::vector<Particle>* waiting = nullptr) {
   c.begin(f, origin, particles);
   f.window->electricPreview = true;
   f.advanceElectric();
@@ -1280,7 +1289,7 @@ inline StepResult sourceStep(Fields& f, Coupling& c, std::vector<Particle>& part
     carriers = originalCarriers;
     counters = originalCounters;
     c.beginTrial();
-    pushDepositAll(c, particles, carriers, counters, time, f.g.dt, substeps, pb, origin, held);
+    pushDepositAll(c, particles, carriers, counters, time, f.g.dt, substeps, pb, origin, held, waiting);
     // applyMovingTransverseReferenceCurrent (simulation_source_coupling.cpp:44)
     if (ref && ref->enabled) ref->apply(f, f.window->offset, f.window->step);
     c.applyCurrentSource(-1.);
@@ -1302,7 +1311,7 @@ inline StepResult sourceStep(Fields& f, Coupling& c, std::vector<Particle>& part
   counters = originalCounters;
   c.swapElectric();
   c.replay = true;
-  pushDepositAll(c, particles, carriers, counters, time, f.g.dt, substeps, pb, origin, held);
+  pushDepositAll(c, particles, carriers, counters, time, f.g.dt, substeps, pb, origin, held, waiting);
   c.replay = false;
   // the solve loop's reference current before the commit (simulation.cpp:548-549)
   if (ref && ref->enabled) ref->apply(f, f.window->offset, f.window->step);

@@ -251,6 +251,7 @@ double ref_fields_poynting(void* h, std::size_t i, std::size_t j, std::size_t k,
 #include "source_coupling.h"
 #include "source_coupling_math.h"
 #include "charge_matched_inlet.h"
+#include "pending_particle_queue.h"
 
 namespace {
 struct RefCoupling {
@@ -269,6 +270,9 @@ struct RefCoupling {
   bool quintic = false, binomial3 = false, smoothX = false;
   EBCPMLParameters cpml;
   Double activationLabZ = std::numeric_limits<Double>::infinity();
+  // rank 0's pending admission queue and the step of its last admission
+  std::unique_ptr<PendingParticleQueue> pending;
+  std::size_t pendingStep = 0;
   ~RefCoupling() {
     c.reset();
     if (!dir.empty()) {
@@ -365,6 +369,13 @@ int ref_coupling_step(void* h, const double* origin, double time, unsigned subst
     const bool window = static_cast<bool>(rf.window);
     auto pushDepositAll = [&]() {
       ChargeConservingCurrentDepositor depositor(f, o, H.c->longitudinalCurrent());
+      // pushDepositAndMigrate's pending support (simulation.cpp:4170-4172)
+      if ((H.quintic || H.binomial3) && H.pending)
+        H.pending->visitNear(H.pendingStep, 8, [&](const RelativisticParticleSI& p) {
+          auto end = p.position;
+          end[2] += SI::c * g.dt * p.frozenBetaZ;
+          H.c->depositFrozenSourceSupport(p.position, end, p.charge, false);
+        });
       std::vector<RelativisticParticleSI> prepared = H.particles;
       // prepareParticle (simulation.cpp:4919-5035), hold-transverse release
       std::vector<char> hasActive(prepared.size(), 0);
@@ -626,12 +637,18 @@ struct InletContext {
   std::unique_ptr<ChargeMatchedInlet> inlet;
 };
 std::map<void*, std::unique_ptr<InletContext>> inletContexts;
-void completeInlet(void* context, EBFieldGrid& f, std::size_t, std::ptrdiff_t offset) {
+void completeInlet(void* context, EBFieldGrid& f, std::size_t step, std::ptrdiff_t offset) {
   InletContext& c = *static_cast<InletContext*>(context);
   FieldVector<Double> origin = c.referenceOrigin;
   origin[2] = std::fma(static_cast<double>(offset), f.geometry().dz, c.referenceOrigin[2]);
   c.inlet->begin(origin);
   for (const auto& p : c.coupling->particles) c.inlet->particle(p);
+  // Simulation::completeMovingInletLayer's pending visit (simulation_moving_window.cpp:317)
+  if (c.coupling->pending) {
+    const auto visit = [&](const RelativisticParticleSI& p) { c.inlet->particle(p); };
+    if (c.coupling->quintic || c.coupling->binomial3) c.coupling->pending->visitNear(step, 8, visit);
+    else c.coupling->pending->visitDue(step, visit);
+  }
   c.inlet->complete(f, false);
 }
 }  // namespace
@@ -678,6 +695,62 @@ int ref_coupling_get_frozen(void* h, int* frozen) {
 }
 int ref_coupling_set_activation(void* h, double labZ) {
   return guarded([&] { RC(h)->activationLabZ = labZ; });
+}
+
+// The reference's PendingParticleQueue over the coupling's particles, as
+// Simulation sets it up (simulation_moving_window.cpp:231-238).
+int ref_coupling_set_pending(void* h, double referenceOriginZ, std::int64_t initialOffset, double dz,
+                             double heldDistancePerBeta, std::uint64_t maximumStep, std::uint64_t initialStep) {
+  return guarded([&] {
+    RefCoupling& H = *RC(h);
+    PendingWindowGeometry g;
+    g.referenceOriginZ = referenceOriginZ;
+    g.initialOffset = static_cast<std::ptrdiff_t>(initialOffset);
+    g.dz = dz;
+    g.heldDistancePerBeta = heldDistancePerBeta;
+    g.maximumStep = static_cast<std::size_t>(maximumStep);
+    H.pending.reset(new PendingParticleQueue(g, static_cast<std::size_t>(initialStep)));
+    H.pending->extract(H.particles);
+    H.pendingStep = static_cast<std::size_t>(initialStep);
+  });
+}
+// Simulation::admitMovingWindowParticles (simulation_moving_window.cpp:275-278)
+int ref_coupling_admit(void* h, std::uint64_t step, std::size_t* admitted) {
+  return guarded([&] {
+    RefCoupling& H = *RC(h);
+    if (!H.pending) throw std::logic_error("No pending queue");
+    *admitted = H.pending->admit(static_cast<std::size_t>(step), H.particles);
+    H.pendingStep = static_cast<std::size_t>(step);
+  });
+}
+int ref_coupling_pending_state(void* h, std::uint64_t* out3) {
+  return guarded([&] {
+    RefCoupling& H = *RC(h);
+    if (!H.pending) throw std::logic_error("No pending queue");
+    out3[0] = H.pending->size();
+    out3[1] = H.pending->nextDueStep();
+    out3[2] = H.pending->admittedCount();
+  });
+}
+int ref_coupling_pending_near(void* h, std::uint64_t step, std::uint64_t lookAhead, std::size_t capacity, double* z,
+                              std::uint64_t* id, std::size_t* n) {
+  return guarded([&] {
+    RefCoupling& H = *RC(h);
+    if (!H.pending) throw std::logic_error("No pending queue");
+    std::size_t count = 0;
+    H.pending->visitNear(static_cast<std::size_t>(step), static_cast<std::size_t>(lookAhead),
+                         [&](const RelativisticParticleSI& p) {
+                           if (count < capacity) {
+                             z[count] = p.position[2];
+                             id[count] = p.id;
+                           }
+                           ++count;
+                         });
+    *n = count;
+  });
+}
+int ref_repeat_held_addition(double initial, double increment, std::uint64_t count, double* out) {
+  return guarded([&] { *out = repeatHeldAddition(initial, increment, static_cast<std::size_t>(count)); });
 }
 
 int ref_coupling_get_ids(void* h, std::uint64_t* id) {

@@ -190,3 +190,55 @@ def held_oracle_case(prefix, shape="binomial3", smooth=True, seed=31):
     c.set_held(frozen, beta)
     c.set_activation(activation_lab_z())
     return f, c
+
+
+PENDING_MAXIMUM_STEP = 40
+
+
+def pending_particles(seed=41, per_group=16, upstream=40):
+    """Held electrons (beta_held = 0.99) inside the window and spread from
+    0.5 to 30 cells upstream of the inlet, so that the pending queue holds
+    particles due within the first steps, within its 8-step look-ahead and
+    beyond it; plus active electrons. Same return as held_particles()."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = DIMS
+    o = window_origin(OFFSET0)
+    n = 2 * per_group + upstream
+    xs = o[0] + rng.uniform(CELLS[0] + 3, nx - CELLS[0] - 3, n) * DX
+    ys = o[1] + rng.uniform(CELLS[1] + 3, ny - CELLS[1] - 3, n) * DX
+    z = np.concatenate([o[2] + rng.uniform(4.0, 14.0, per_group) * DZ,      # held in the window
+                        o[2] - rng.uniform(0.5, 30.0, upstream) * DZ,       # held upstream
+                        o[2] + rng.uniform(8.0, 16.0, per_group) * DZ])     # active
+    x = np.column_stack([xs, ys, z])
+    u = np.column_stack([rng.normal(0, .2, n), rng.normal(0, .2, n), rng.uniform(0.5, 2.0, n)])
+    frozen = np.concatenate([np.ones(per_group + upstream, np.int32), np.zeros(per_group, np.int32)])
+    beta = np.where(frozen == 1, 0.99, 0.0)
+    macro = 2.0e5 / n
+    q = np.full(n, -bunch.QE * macro)
+    m = np.full(n, bunch.ME * macro)
+    ids = np.arange(1, n + 1, dtype=np.uint64)
+    return x, u, q, m, ids, frozen, beta
+
+
+def pending_geometry():
+    """(reference origin z, initial offset, dz, c dt, maximum step) of the
+    window of these cases (PendingWindowGeometry)."""
+    return float(REF_ORIGIN[2]), OFFSET0, DZ, C_SI * (DZ / C_SI), PENDING_MAXIMUM_STEP
+
+
+def pending_oracle_case(prefix, shape="binomial3", smooth=True, seed=41, queue=True):
+    """CPU fields and coupling of the pending case; with queue the upstream
+    held particles wait in the coupling's PendingParticleQueue."""
+    import harness as H
+    f = H.CpuFields(prefix, *DIMS, DX, DX, DZ, DZ / C_SI, 1)
+    f.install_window(CELLS)
+    H.seeded_fields(f, 13, scale=1e5)
+    c = H.CpuCoupling(f, shape, smooth, cpml_cells=CELLS)
+    c.set_frame(GAMMA_BOOST, 0.0, 0.0, 0.0)
+    x, u, q, m, ids, frozen, beta = pending_particles(seed)
+    c.set_particles(x, u, q, m, np.ones(len(q)), ids)
+    c.set_held(frozen, beta)
+    c.set_activation(activation_lab_z())
+    if queue:
+        c.set_pending(*pending_geometry())
+    return f, c

@@ -267,6 +267,12 @@ def _bind_particles(lib, prefix):
     f("coupling_set_held").argtypes = [vp, C.POINTER(C.c_int), _dp]
     f("coupling_get_frozen").argtypes = [vp, C.POINTER(C.c_int)]
     f("coupling_set_activation").argtypes = [vp, C.c_double]
+    f("coupling_set_pending").argtypes = [vp, C.c_double, C.c_int64, C.c_double, C.c_double, C.c_uint64,
+                                          C.c_uint64]
+    f("coupling_admit").argtypes = [vp, C.c_uint64, C.POINTER(sz)]
+    f("coupling_pending_state").argtypes = [vp, _u64p]
+    f("coupling_pending_near").argtypes = [vp, C.c_uint64, C.c_uint64, sz, _dp, _u64p, C.POINTER(sz)]
+    f("repeat_held_addition").argtypes = [C.c_double, C.c_double, C.c_uint64, _dp]
     if prefix == "orc":
         f("window_set_charge_match").argtypes = [vp, C.c_int]
         f("coupling_complete_inlet").argtypes = [vp, _dp]
@@ -289,6 +295,13 @@ def load_particles(prefix):
 
 def _v(x):
     return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def repeat_held_addition(prefix, initial, increment, count):
+    """repeatHeldAddition of the oracle (literal additions) or the reference build."""
+    out = C.c_double()
+    _call(prefix, "repeat_held_addition", float(initial), float(increment), int(count), C.byref(out))
+    return out.value
 
 
 def _call(prefix, name, *args):
@@ -425,6 +438,34 @@ class CpuCoupling:
 
     def set_activation(self, lab_z):
         _call(self.prefix, "coupling_set_activation", self.h, lab_z)
+
+    def set_pending(self, reference_origin_z, initial_offset, dz, held_distance_per_beta, maximum_step,
+                    initial_step=0):
+        """PendingParticleQueue over the particles (after set_held)."""
+        _call(self.prefix, "coupling_set_pending", self.h, reference_origin_z, initial_offset, dz,
+              held_distance_per_beta, maximum_step, initial_step)
+
+    def admit(self, step):
+        n = C.c_size_t()
+        _call(self.prefix, "coupling_admit", self.h, step, C.byref(n))
+        return n.value
+
+    def pending_state(self):
+        """(waiting, next due step, admitted)."""
+        out = np.zeros(3, np.uint64)
+        _call(self.prefix, "coupling_pending_state", self.h, out.ctypes.data_as(_u64p))
+        return tuple(int(v) for v in out)
+
+    def pending_near(self, step, look_ahead):
+        """(z, ids) of the waiting particles due by step + look_ahead."""
+        n = C.c_size_t()
+        _call(self.prefix, "coupling_pending_near", self.h, step, look_ahead, 0, None, None, C.byref(n))
+        z = np.zeros(n.value)
+        ids = np.zeros(n.value, np.uint64)
+        if n.value:
+            _call(self.prefix, "coupling_pending_near", self.h, step, look_ahead, n.value, _ptr(z),
+                  ids.ctypes.data_as(_u64p), C.byref(n))
+        return z, ids
 
     def finish_window(self, new_origin):
         _call(self.prefix, "coupling_finish_window", self.h, _ptr(_v(new_origin)))
