@@ -420,7 +420,6 @@ __global__ void __launch_bounds__(256) kMagneticCowanEdges(FieldPtrsT<R> F, Step
 constexpr int kMX = 32, kMY = 8;
 constexpr int kLX = kMX + 2, kLY = kMY + 2, kLP = kLX * kLY;
 constexpr int kStageN = 9 * kLP;  // 3 components x 3 planes
-constexpr int kStageR = (kStageN + kMX * kMY - 1) / (kMX * kMY);
 
 // Smoothed sample from the staged planes; (li, lj) local.
 // pl[0] = plane k-1, pl[1] = plane k, pl[2] = plane k+1; D is the derivative axis.
@@ -450,13 +449,46 @@ __device__ __forceinline__ double smoothL(const R* const pl[3], const StepCoefsT
   return r;
 }
 
+// evalE() split into its in-plane part and its z part, so that staging can
+// compute addresses without loads and then issue every load at once.
+// In-plane: the index j * n0 + i of the node evalE reads for (i, j) (PEC
+// parity clamp on the component's own axis, odd reflection across the
+// others) and its sign; sign 0 for nodes more than one node beyond the
+// array, which feed no B node.
 template <int A, class R>
-__device__ __forceinline__ R stagedE(const FieldPtrsT<R>& F, int i, int j, int kk) {
+__device__ __forceinline__ int planeIndexE(const FieldPtrsT<R>& F, int i, int j, int& sign) {
+  const int n0 = F.n[A][0], n1 = F.n[A][1];
+  sign = (i < -1 || i > n0 || j < -1 || j > n1) ? 0 : 1;
+  if (i < 0) {
+    if (A == 0) i = 0;
+    else { i = -i; sign = -sign; }
+  } else if (i >= n0) {
+    if (A == 0) i = n0 - 1;
+    else { i = 2 * (n0 - 1) - i; sign = -sign; }
+  }
+  if (j < 0) {
+    if (A == 1) j = 0;
+    else { j = -j; sign = -sign; }
+  } else if (j >= n1) {
+    if (A == 1) j = n1 - 1;
+    else { j = 2 * (n1 - 1) - j; sign = -sign; }
+  }
+  return sign == 0 ? 0 : j * n0 + i;
+}
+
+// z part: the plane evalE reads for plane k (the array's own plane, the halo
+// plane, or the PEC parity image) and its sign.
+template <int A, class R>
+__device__ __forceinline__ const R* planeBaseE(const FieldPtrsT<R>& F, int k, int& sign) {
   const int n0 = F.n[A][0], n1 = F.n[A][1], n2 = F.n[A][2];
-  if (kk >= 0 && kk < n2 && i >= 0 && i < n0 && j >= 0 && j < n1)
-    return F.e[A][(static_cast<size_t>(kk) * n1 + j) * n0 + i];
-  if (i >= -1 && i <= n0 && j >= -1 && j <= n1) return static_cast<R>(evalE<A>(F, i, j, kk));
-  return R(0);  // more than one node beyond the array: feeds no B node
+  sign = 1;
+  if (k >= 0 && k < n2) return F.e[A] + static_cast<size_t>(k) * n1 * n0;
+  const bool lower = k < 0;
+  const R* plane = lower ? F.lo[A] : F.up[A];
+  if (plane) return plane;
+  if (A == 2) k = lower ? 0 : n2 - 1;
+  else { k = lower ? -k : 2 * (n2 - 1) - k; sign = -1; }
+  return F.e[A] + static_cast<size_t>(k) * n1 * n0;
 }
 
 template <class R>
@@ -466,51 +498,42 @@ __global__ void __launch_bounds__(kMX * kMY) kMagneticCowanTiled(FieldPtrsT<R> F
   const int i0 = blockIdx.x * kMX, j0 = blockIdx.y * kMY, k = blockIdx.z;
   const int tx = static_cast<int>(threadIdx.x) % kMX, ty = static_cast<int>(threadIdx.x) / kMX;
   const int i = i0 + tx, j = j0 + ty;
-  if (i0 >= 1 && i0 + kMX + 1 <= nx && j0 >= 1 && j0 + kMY + 1 <= ny && k >= 1 && k + 2 <= nz) {
-    // Interior tile: every staged node lies inside all three components, so
-    // electricValue() is a plain load. The node (li, lj) of the region is
-    // the same for the nine (component, plane) pairs.
-    constexpr int kRegionR = (kLP + kMX * kMY - 1) / (kMX * kMY);
-    R v[kRegionR][9];
+  // Staging as electricValue() samples: the region's node (gi, gj) is the
+  // same for the nine (component, plane) pairs; each component maps it to
+  // the node evalE() reads (in-plane index and sign per lane, the plane
+  // base per block), and every load is issued before the first store.
+  constexpr int kRegionR = (kLP + kMX * kMY - 1) / (kMX * kMY);
+  const R* zbase[9];
+  int zsign[9];
 #pragma unroll
-    for (int r = 0; r < kRegionR; ++r) {
-      const int q = static_cast<int>(threadIdx.x) + r * kMX * kMY;
-      const int qc = q < kLP ? q : 0;
-      const int gi = i0 - 1 + qc % kLX, gj = j0 - 1 + qc / kLX;
+  for (int pz = 0; pz < 3; ++pz) {
+    zbase[pz] = planeBaseE<0>(F, k - 1 + pz, zsign[pz]);
+    zbase[3 + pz] = planeBaseE<1>(F, k - 1 + pz, zsign[3 + pz]);
+    zbase[6 + pz] = planeBaseE<2>(F, k - 1 + pz, zsign[6 + pz]);
+  }
+  R v[kRegionR][9];
+  int sxy[kRegionR][3];
 #pragma unroll
-      for (int comp = 0; comp < 3; ++comp) {
-        const int n0 = F.n[comp][0], n1 = F.n[comp][1];
-        const R* base = F.e[comp] + (static_cast<size_t>(k - 1) * n1 + gj) * n0 + gi;
-        const size_t plane = static_cast<size_t>(n0) * n1;
+  for (int r = 0; r < kRegionR; ++r) {
+    const int q = static_cast<int>(threadIdx.x) + r * kMX * kMY;
+    const int qc = q < kLP ? q : 0;
+    const int gi = i0 - 1 + qc % kLX, gj = j0 - 1 + qc / kLX;
+    int ix[3];
+    ix[0] = planeIndexE<0>(F, gi, gj, sxy[r][0]);
+    ix[1] = planeIndexE<1>(F, gi, gj, sxy[r][1]);
+    ix[2] = planeIndexE<2>(F, gi, gj, sxy[r][2]);
 #pragma unroll
-        for (int pz = 0; pz < 3; ++pz) v[r][comp * 3 + pz] = base[pz * plane];
+    for (int cp = 0; cp < 9; ++cp) v[r][cp] = zbase[cp][ix[cp / 3]];
+  }
+#pragma unroll
+  for (int r = 0; r < kRegionR; ++r) {
+    const int q = static_cast<int>(threadIdx.x) + r * kMX * kMY;
+    if (q < kLP)
+#pragma unroll
+      for (int cp = 0; cp < 9; ++cp) {
+        const int sg = sxy[r][cp / 3] * zsign[cp];
+        buf[cp * kLP + q] = sg == 0 ? R(0) : (sg > 0 ? v[r][cp] : -v[r][cp]);
       }
-    }
-#pragma unroll
-    for (int r = 0; r < kRegionR; ++r) {
-      const int q = static_cast<int>(threadIdx.x) + r * kMX * kMY;
-      if (q < kLP)
-#pragma unroll
-        for (int cp = 0; cp < 9; ++cp) buf[cp * kLP + q] = v[r][cp];
-    }
-  } else {
-    R v[kStageR];
-#pragma unroll
-    for (int r = 0; r < kStageR; ++r) {
-      const int t = static_cast<int>(threadIdx.x) + r * kMX * kMY;
-      v[r] = 0.0;
-      if (t < kStageN) {
-        const int comp = t / (3 * kLP), rest = t - comp * 3 * kLP;
-        const int pz = rest / kLP, q = rest - pz * kLP;
-        const int gi = i0 - 1 + q % kLX, gj = j0 - 1 + q / kLX, gk = k - 1 + pz;
-        v[r] = comp == 0 ? stagedE<0>(F, gi, gj, gk) : (comp == 1 ? stagedE<1>(F, gi, gj, gk) : stagedE<2>(F, gi, gj, gk));
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < kStageR; ++r) {
-      const int t = static_cast<int>(threadIdx.x) + r * kMX * kMY;
-      if (t < kStageN) buf[t] = v[r];
-    }
   }
   __syncthreads();
   if (i >= nx || j >= ny) return;
