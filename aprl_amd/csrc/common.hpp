@@ -4,8 +4,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <chrono>
 #include <cstddef>
 #include <cstdint>
+#include <cstring>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -109,6 +112,61 @@ class PinnedHost {
 
  private:
   void* p_ = nullptr;
+};
+
+// Readback slots that kernels write straight into coherent page-locked host
+// memory: a kernel stores its values, fences at system scope and then stores
+// a sequence number in one of the flag words; the host spins on that word
+// (a stream-ordered copy and a stream synchronization cost a copy kernel and
+// a wake-up each). Layout in 32-bit words: [0, 8) flags, [8, 512) values;
+// doubles from byte 2048.
+class HostMailbox {
+ public:
+  static constexpr std::size_t kBytes = 4096;
+  HostMailbox() = default;
+  HostMailbox(const HostMailbox&) = delete;
+  HostMailbox& operator=(const HostMailbox&) = delete;
+  ~HostMailbox() {
+    if (host_) (void)hipHostFree(host_);
+  }
+  unsigned* host() {
+    if (!host_) {
+      void* p = nullptr;
+      hipCheck(hipHostMalloc(&p, kBytes, hipHostMallocMapped | hipHostMallocCoherent), "hipHostMalloc");
+      std::memset(p, 0, kBytes);
+      void* d = nullptr;
+      hipCheck(hipHostGetDevicePointer(&d, p, 0), "hipHostGetDevicePointer");
+      host_ = static_cast<unsigned*>(p);
+      dev_ = static_cast<unsigned*>(d);
+    }
+    return host_;
+  }
+  unsigned* device() {
+    host();
+    return dev_;
+  }
+  double* hostDoubles() { return reinterpret_cast<double*>(reinterpret_cast<unsigned char*>(host()) + 2048); }
+  // the next sequence number of flag word f
+  unsigned next(int f) { return ++seq_[f]; }
+  // Wait until flag word f holds seq; after a long wait the stream is
+  // synchronized (reporting a failed kernel) and the flag checked again.
+  void wait(int f, unsigned seq, hipStream_t st) {
+    volatile unsigned* flag = host() + f;
+    const auto start = std::chrono::steady_clock::now();
+    for (unsigned spins = 0; *flag != seq; ++spins) {
+      if ((spins & 1023u) == 1023u && std::chrono::steady_clock::now() - start > std::chrono::milliseconds(200)) {
+        hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
+        if (*flag != seq) throw DeviceError("A kernel readback did not arrive in host memory");
+        break;
+      }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+  }
+
+ private:
+  unsigned* host_ = nullptr;
+  unsigned* dev_ = nullptr;
+  unsigned seq_[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 // A timing-free HIP event, created on first use.

@@ -365,9 +365,22 @@ __device__ __forceinline__ int highWord(unsigned long long v) { return static_ca
 
 // Chunk list in tile order and the list of active tiles (tiles with an
 // occupied neighbour), both from the scanned counts.
+// With a mailbox, lane 0 also posts the scan total (chunks, active tiles)
+// and the four sort-phase words to host memory (values at words 8..13,
+// sequence number in flag word 0).
 __global__ void kChunkList(const int* begin, const int* end, const unsigned long long* counts,
-                           const unsigned long long* scan, int ntiles, int4* chunks, int* active) {
+                           const unsigned long long* scan, int ntiles, int4* chunks, int* active, unsigned* box,
+                           unsigned seq, const unsigned* words) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (box && t == 0) {
+    const unsigned long long total = scan[ntiles];
+    volatile unsigned* v = box;
+    v[8] = static_cast<unsigned>(total & 0xffffffffull);
+    v[9] = static_cast<unsigned>(total >> 32);
+    for (int w = 0; w < 4; ++w) v[10 + w] = words[w];
+    __threadfence_system();
+    v[0] = seq;
+  }
   if (t >= ntiles) return;
   const unsigned long long sc = scan[t];
   if (counts[t] >> 32) active[highWord(sc)] = t;
@@ -734,7 +747,11 @@ __global__ void __launch_bounds__(256) kReduceCurrent(double* jx, double* jy, do
   }
 }
 
-__global__ void __launch_bounds__(256) kReduceBlocks(const double* blockIn, int blocks, double* out) {
+// With a mailbox, lane 0 also posts the four sums (doubles at byte 2048)
+// and nWords trial words (words 16..) to host memory, sequence number in
+// flag word 1.
+__global__ void __launch_bounds__(256) kReduceBlocks(const double* blockIn, int blocks, double* out, unsigned* box,
+                                                     unsigned seq, const unsigned* trialWords, int nWords) {
   __shared__ double sf[256], sd[256], ss[256], sc[256];
   double f = 0., d = 0., s = 0., c = 0.;
   for (int b = threadIdx.x; b < blocks; b += 256) {
@@ -754,7 +771,17 @@ __global__ void __launch_bounds__(256) kReduceBlocks(const double* blockIn, int 
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) { out[0] = sf[0]; out[1] = sd[0]; out[2] = ss[0]; out[3] = sc[0]; }
+  if (threadIdx.x == 0) {
+    out[0] = sf[0]; out[1] = sd[0]; out[2] = ss[0]; out[3] = sc[0];
+    if (box) {
+      volatile double* vd = reinterpret_cast<volatile double*>(reinterpret_cast<unsigned char*>(box) + 2048);
+      vd[0] = sf[0]; vd[1] = sd[0]; vd[2] = ss[0]; vd[3] = sc[0];
+      volatile unsigned* v = box;
+      for (int w = 0; w < nWords; ++w) v[16 + w] = trialWords[w];
+      __threadfence_system();
+      v[1] = seq;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -968,7 +995,7 @@ static int bitsFor(unsigned v) {
 // entries last), then the tile ranges and the chunk list (tile order,
 // kChunk particles per chunk) in device memory.
 static void sortAndChunk(CouplingState& C, ParticleStore& P, const GArgs& G, const SortArgs& K, int ntx, int nty,
-                         int ntiles, unsigned* occupied) {
+                         int ntiles, unsigned* occupied, unsigned seq) {
   FieldState& F = *C.fields;
   hipStream_t st = F.stream;
   const int n = static_cast<int>(P.count);
@@ -1005,13 +1032,13 @@ static void sortAndChunk(CouplingState& C, ParticleStore& P, const GArgs& G, con
   P.state.swap(P.stateT);
   P.beta.swap(P.betaT);
   const int rz = C.shape == kShapeQuintic ? Box<kShapeQuintic>::RZ : Box<kShapeBinomial3X>::RZ;
-  buildChunkLists(C, st, P.keySorted.get(), n, ntx, nty, ntiles, rz, occupied);
+  buildChunkLists(C, st, P.keySorted.get(), n, ntx, nty, ntiles, rz, occupied, C.mailbox.device(), seq);
 }
 
 // Tile ranges of the sorted keys, chunk counts and the active flags, their
 // scan, the chunk list and the active-tile list (all in tile order).
 void buildChunkLists(CouplingState& C, hipStream_t st, const unsigned* keySorted, int n, int ntx, int nty,
-                     int ntiles, int rz, unsigned* occupied) {
+                     int ntiles, int rz, unsigned* occupied, unsigned* box, unsigned seq) {
   kTileBounds<<<ceilDiv(n, 256), 256, 0, st>>>(keySorted, n, ntiles, C.tileBegin.get(), C.tileEnd.get(), occupied);
   launchCheck("tile bounds");
   const int ntz = ntiles / (ntx * nty);
@@ -1023,7 +1050,8 @@ void buildChunkLists(CouplingState& C, hipStream_t st, const unsigned* keySorted
                                    static_cast<std::size_t>(ntiles) + 1, rocprim::plus<unsigned long long>(), st),
            "exclusive_scan");
   kChunkList<<<ceilDiv(ntiles, 256), 256, 0, st>>>(C.tileBegin.get(), C.tileEnd.get(), C.tileCounts.get(),
-                                                    C.tileScan.get(), ntiles, C.chunks.get(), C.activeTiles.get());
+                                                    C.tileScan.get(), ntiles, C.chunks.get(), C.activeTiles.get(),
+                                                    box, seq, C.error.get());
   launchCheck("chunk list");
 }
 
@@ -1063,7 +1091,7 @@ void fillInts(hipStream_t st, int* p, int n, int v) {
 }
 
 void reduceChunkStats(hipStream_t st, const double* in, int chunks, double* out) {
-  kReduceBlocks<<<1, 256, 0, st>>>(in, chunks, out);
+  kReduceBlocks<<<1, 256, 0, st>>>(in, chunks, out, nullptr, 0u, nullptr, 0);
   launchCheck("chunk stats");
 }
 
@@ -1173,11 +1201,11 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
   reserveTileLists(C, st, ntiles, n);
   const std::size_t chunkBound = chunkListBound(ntiles, n);
   reserveCouplingWords(C, st);
-  // pinned readback slots: [0..1] scan total (chunks, active tiles),
-  // [4..68) trial error words, [120..123] the sort-phase words; doubles from
-  // byte 512: force, defect, scale, calls
-  unsigned* pin = static_cast<unsigned*>(C.pinned.get(1024));
-  double* pinD = reinterpret_cast<double*>(static_cast<unsigned char*>(C.pinned.get(1024)) + 512);
+  // Readbacks arrive in the mailbox: the scan total (chunks, active tiles)
+  // and the sort-phase words at words 8..13 (flag 0); a trial's residual
+  // sums (force, defect, scale, calls; doubles) and its error words from
+  // word 16 (flag 1).
+  unsigned sortSeq = 0;
   kFillInt<<<1, 128, 0, st>>>(reinterpret_cast<int*>(C.error.get()), kErrorWords, 0);
   launchCheck("fill");
   if (n) {
@@ -1189,12 +1217,8 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
     K.dz = F.g.dz;
     K.words = C.error.get();
     K.removed = P.removed.get();
-    sortAndChunk(C, P, G, K, ntx, nty, ntiles, C.error.get() + 1);
-    hipCheck(hipMemcpyAsync(pin, C.tileScan.get() + ntiles, sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
-             "hipMemcpyAsync");
-    hipCheck(hipMemcpyAsync(pin + 120, C.error.get(), 4 * sizeof(unsigned), hipMemcpyDeviceToHost, st),
-             "hipMemcpyAsync");
-    hipCheck(hipEventRecord(C.sorted.get(), st), "hipEventRecord");
+    sortSeq = C.mailbox.next(0);
+    sortAndChunk(C, P, G, K, ntx, nty, ntiles, C.error.get() + 1, sortSeq);
   }
   // Zero-current preview E* (MovingWindowFields::previewElectricBoundary);
   // it runs while the host waits for the chunk count.
@@ -1208,14 +1232,15 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
                                TileGrid{ntx, nty, ntz, kTileX, kTileY, kTileZ});
   unsigned nChunks = 0, nActive = 0, occupiedTiles = 0;
   if (n) {
-    hipCheck(hipEventSynchronize(C.sorted.get()), "hipEventSynchronize");
-    nChunks = pin[0];
-    nActive = pin[1];
-    occupiedTiles = pin[121];
+    C.mailbox.wait(0, sortSeq, st);
+    const unsigned* box = C.mailbox.host();
+    nChunks = box[8];
+    nActive = box[9];
+    occupiedTiles = box[11];
     // finishMovingWindowStep's checks of the entries, applied to this origin
-    raiseParticleErrors(pin[120]);
-    const int behind = static_cast<int>(pin[122]);
-    if (pin[123]) cropCounters(C, st, n - behind, n, P.state.get(), P.s[6].get(), P.s[8].get(), false);
+    raiseParticleErrors(box[10]);
+    const int behind = static_cast<int>(box[12]);
+    if (box[13]) cropCounters(C, st, n - behind, n, P.state.get(), P.s[6].get(), P.s[8].get(), false);
     n -= behind;  // removed and cropped entries now trail the live ones
     P.count = static_cast<std::size_t>(n);
     if (nChunks > chunkBound) throw std::logic_error("Chunk list exceeds its bound");
@@ -1298,6 +1323,11 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
     // TransverseReferenceCurrent::apply after the trial's deposit
     // (simulation_source_coupling.cpp:44)
     referenceCurrentTrial(C, F, C.esum[2].get(), C.estar[2].get(), true, nChunks > 0, ntx, nty, ntz);
+    // With active particles the convergence test needs at least two trials
+    // (simulation_source_coupling.cpp:50), so trial 0 is never read back;
+    // its error word is checked, first, after trial 1.
+    const bool readback = !(n && trial == 0);
+    const unsigned trialSeq = readback ? C.mailbox.next(1) : 0u;
     {
       ScopedKernel sk(F.profiler, st, "residual");
       if (nChunks && quintic)
@@ -1306,28 +1336,26 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
       else if (nChunks)
         kGatherChunks<1, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G, S, C.chunks.get(), ntx, nty, F.g.dt,
                                                                             C.reduceBlocks.get(), errT);
-      kReduceBlocks<<<1, 256, 0, st>>>(C.reduceBlocks.get(), static_cast<int>(nChunks), C.reduceOut.get());
+      kReduceBlocks<<<1, 256, 0, st>>>(C.reduceBlocks.get(), static_cast<int>(nChunks), C.reduceOut.get(),
+                                       readback ? C.mailbox.device() : nullptr, trialSeq,
+                                       C.error.get() + kTrialWord + checkedTrials,
+                                       static_cast<int>(trial + 1 - checkedTrials));
       launchCheck("residual");
     }
     iterations = trial + 1;
     // The next trial pushes with the fields this residual gathered.
     for (int a = 0; a < 6; ++a) std::swap(A.fin[a], A.fout[a]);
-    // With active particles the convergence test needs at least two trials
-    // (simulation_source_coupling.cpp:50), so trial 0 is never read back;
-    // its error word is checked, first, after trial 1.
-    if (n && trial == 0) continue;
-    hipCheck(hipMemcpyAsync(pinD, C.reduceOut.get(), 4 * sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpy");
-    hipCheck(hipMemcpyAsync(pin + 4, C.error.get() + kTrialWord + checkedTrials,
-                            (trial + 1 - checkedTrials) * sizeof(unsigned), hipMemcpyDeviceToHost, st),
-             "hipMemcpy");
-    hipCheck(hipStreamSynchronize(st), "hipStreamSynchronize");
-    for (unsigned t = checkedTrials; t <= trial; ++t) raiseParticleErrors(pin[4 + t - checkedTrials]);
-    lastWord = pin[4 + trial - checkedTrials];
+    if (!readback) continue;
+    C.mailbox.wait(1, trialSeq, st);
+    const unsigned* words = C.mailbox.host() + 16;
+    const double* sums = C.mailbox.hostDoubles();
+    for (unsigned t = checkedTrials; t <= trial; ++t) raiseParticleErrors(words[t - checkedTrials]);
+    lastWord = words[trial - checkedTrials];
     checkedTrials = trial + 1;
     // simulation_source_coupling.cpp:46-51.
-    force = pinD[0];
-    work = pinD[3] == 0. ? 0. : (pinD[2] > 0. ? pinD[1] / pinD[2] : pinD[1]);
-    if (iterations >= (pinD[3] > 0. ? 2u : 1u) && force < 1e-13 && work < 1e-8) {
+    force = sums[0];
+    work = sums[3] == 0. ? 0. : (sums[2] > 0. ? sums[1] / sums[2] : sums[1]);
+    if (iterations >= (sums[3] > 0. ? 2u : 1u) && force < 1e-13 && work < 1e-8) {
       converged = true;
       break;
     }

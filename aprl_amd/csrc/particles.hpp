@@ -178,6 +178,7 @@ struct CouplingState {
   DeviceArray<unsigned long long> counterN;
   DeviceArray<unsigned> blockN;
   PinnedHost pinned;
+  HostMailbox mailbox;  // kernel-written readbacks of the plain form
   Event sorted, stepDone;
   std::shared_ptr<SplitStepContext> split;
   // reference surface current (TransverseReferenceCurrent): rows x points
@@ -214,8 +215,10 @@ __device__ __forceinline__ void clearTileRanges(const TileRangeArgs& R, int gid)
 }
 
 // Host helpers shared by the FP64 and FP32-split coupling steps.
+// box / seq: the mailbox (device address) that receives the scan total and
+// the sort-phase words, or nullptr.
 void buildChunkLists(CouplingState& C, hipStream_t st, const unsigned* keySorted, int n, int ntx, int nty,
-                     int ntiles, int rz, unsigned* occupied);
+                     int ntiles, int rz, unsigned* occupied, unsigned* box = nullptr, unsigned seq = 0);
 void reserveTileLists(CouplingState& C, hipStream_t st, int ntiles, int n);
 std::size_t chunkListBound(int ntiles, int n);
 void fillInts(hipStream_t st, int* p, int n, int v);
