@@ -103,32 +103,36 @@ __device__ Clip clipOrbit(const GArgs& G, const double begin[3], const double en
 template <int SHAPE, bool WITH_B>
 __device__ void stageTile(const GArgs& G, const TileBox<SHAPE>& box, double* es, double* bs) {
   constexpr int N = boxNodes<SHAPE>();
-  constexpr int kIters = (3 * N + 2 * kChunk - 1) / (2 * kChunk);
-  double ev[kIters], bv[kIters];
+  constexpr int kPer = (N + 2 * kChunk - 1) / (2 * kChunk);
+  // The component loop is unrolled so that its extents and base pointers
+  // are kernel-argument constants (a runtime component index would turn
+  // each of them into a dependent memory load).
+  double ev[kPer][3], bv[kPer][3];
 #pragma unroll
-  for (int r = 0; r < kIters; ++r) {
-    const int t = static_cast<int>(threadIdx.x) + r * 2 * kChunk;
-    ev[r] = 0.0;
-    bv[r] = 0.0;
-    if (t < 3 * N) {
-      const int d = t / N, q0 = t - d * N;
-      const int li = q0 % Box<SHAPE>::NX, q = q0 / Box<SHAPE>::NX, lj = q % Box<SHAPE>::NY, lk = q / Box<SHAPE>::NY;
-      const int i = box.x0 + li, j = box.y0 + lj, k = box.z0 + lk;
-      if (i >= 0 && j >= 0 && k >= 0) {
-        if (i < G.ne[d][0] && j < G.ne[d][1] && k < G.ne[d][2])
-          ev[r] = G.esum[d][(static_cast<size_t>(k) * G.ne[d][1] + j) * G.ne[d][0] + i];
-        if (WITH_B && i < G.nb[d][0] && j < G.nb[d][1] && k < G.nb[d][2])
-          bv[r] = G.b[d][(static_cast<size_t>(k) * G.nb[d][1] + j) * G.nb[d][0] + i];
-      }
+  for (int r = 0; r < kPer; ++r) {
+    const int q0 = static_cast<int>(threadIdx.x) + r * 2 * kChunk;
+    const int li = q0 % Box<SHAPE>::NX, q = q0 / Box<SHAPE>::NX, lj = q % Box<SHAPE>::NY, lk = q / Box<SHAPE>::NY;
+    const int i = box.x0 + li, j = box.y0 + lj, k = box.z0 + lk;
+    const bool inBox = q0 < N && i >= 0 && j >= 0 && k >= 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      ev[r][d] = double(0);
+      bv[r][d] = double(0);
+      if (inBox && i < G.ne[d][0] && j < G.ne[d][1] && k < G.ne[d][2])
+        ev[r][d] = G.esum[d][(static_cast<size_t>(k) * G.ne[d][1] + j) * G.ne[d][0] + i];
+      if (WITH_B && inBox && i < G.nb[d][0] && j < G.nb[d][1] && k < G.nb[d][2])
+        bv[r][d] = G.b[d][(static_cast<size_t>(k) * G.nb[d][1] + j) * G.nb[d][0] + i];
     }
   }
 #pragma unroll
-  for (int r = 0; r < kIters; ++r) {
-    const int t = static_cast<int>(threadIdx.x) + r * 2 * kChunk;
-    if (t < 3 * N) {
-      es[t] = ev[r];
-      if (WITH_B) bs[t] = bv[r];
-    }
+  for (int r = 0; r < kPer; ++r) {
+    const int q0 = static_cast<int>(threadIdx.x) + r * 2 * kChunk;
+    if (q0 < N)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        es[d * N + q0] = ev[r][d];
+        if (WITH_B) bs[d * N + q0] = bv[r][d];
+      }
   }
 }
 
@@ -254,6 +258,7 @@ __device__ void gatherBQuintic(const GArgs& G, const double a[3], const double b
       cell[half][d] = static_cast<int>(floor(x));
       fraction[half][d] = x - cell[half][d];
     }
+#pragma unroll
   for (int axis = 0; axis < 3; ++axis) {
     const int hx = axis == 0 ? 0 : 1, hy = axis == 1 ? 0 : 1, hz = axis == 2 ? 0 : 1;
     const int* n = G.nb[axis];
@@ -413,22 +418,39 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks(PArgs P, GArgs G, St
   __shared__ double bs[kStageB ? 3 * boxNodes<SHAPE>() : 4 * kChunk];
   const int4 ch = chunks[blockIdx.x];
   const TileBox<SHAPE> box = tileBox<SHAPE>(ch.x, ntx, nty);
-  stageTile<SHAPE, kStageB>(G, box, es, bs);
-  __syncthreads();
   const bool bWaves = threadIdx.x >= kChunk;
   const int lane = static_cast<int>(threadIdx.x) - (bWaves ? kChunk : 0);
   const int p = ch.y + lane;
+  // This lane's particle data is loaded before the tile is staged, so that
+  // its latency overlaps the staging loads' (lanes past the chunk's end read
+  // its first entry and ignore it).
+  const int pv = p < ch.z ? p : ch.y;
+  const int stored = P.state[pv];
+  const double betaP = P.beta[pv];
+  double x0[3], uv[3], fin[6] = {0., 0., 0., 0., 0., 0.}, qm[3] = {0., 0., 0.};
+  for (int d = 0; d < 3; ++d) {
+    x0[d] = P.x[d][pv];
+    uv[d] = MODE == 0 ? P.u[d][pv] : P.x1[d][pv];  // u (first gather) or the trial end x1
+  }
+  if (MODE == 1 && !bWaves) {
+    for (int d = 0; d < 6; ++d) fin[d] = P.fin[d][pv];
+    qm[0] = P.q[pv];
+    qm[1] = P.m[pv];
+    qm[2] = P.gridAbs[pv];
+  }
+  stageTile<SHAPE, kStageB>(G, box, es, bs);
+  __syncthreads();
   // carriers move without fields and take no part in the residual; a held
   // entry takes part from its release point with the rest of the step
-  const int st = p < ch.z ? P.state[p] : kStateRemoved;
+  const int st = p < ch.z ? stored : kStateRemoved;
   bool live = st == kStateActive;
   double dtp = dt;
   unsigned err = 0;
   double a[3] = {0., 0., 0.}, b[3] = {0., 0., 0.}, g[3] = {0., 0., 0.};
   if (st == kStateActive || st == kStateHeld)
-    for (int d = 0; d < 3; ++d) a[d] = P.x[d][p];
+    for (int d = 0; d < 3; ++d) a[d] = x0[d];
   if (st == kStateHeld) {
-    const HeldMove h = heldMove(S, a, P.beta[p]);
+    const HeldMove h = heldMove(S, a, betaP);
     if (h.release && h.remaining > 0.0) {
       live = true;
       a[2] = h.z1;
@@ -437,15 +459,14 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks(PArgs P, GArgs G, St
   }
   if (live) {
     if (MODE == 0) {
-      const double u0[3] = {P.u[0][p], P.u[1][p], P.u[2][p]};
-      const double gamma = gammaFromU(u0[0], u0[1], u0[2]);
+      const double gamma = gammaFromU(uv[0], uv[1], uv[2]);
       const double s = dtp * kC / gamma;
       for (int d = 0; d < 3; ++d) {
         b[d] = a[d];
-        b[d] += s * u0[d];
+        b[d] += s * uv[d];
       }
     } else {
-      for (int d = 0; d < 3; ++d) b[d] = P.x1[d][p];
+      for (int d = 0; d < 3; ++d) b[d] = uv[d];
     }
     if (bWaves) {
       if constexpr (SHAPE == kShapeBinomial3X) gatherBBinomial(G, bs, box, a, b, g, err);
@@ -470,10 +491,10 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks(PArgs P, GArgs G, St
       const double nb[3] = {xb[lane], xb[kChunk + lane], xb[2 * kChunk + lane]};
       double de[3], db[3];
       for (int d = 0; d < 3; ++d) {
-        de[d] = g[d] - P.fin[d][p];
-        db[d] = nb[d] - P.fin[3 + d][p];
+        de[d] = g[d] - fin[d];
+        db[d] = nb[d] - fin[3 + d];
       }
-      const double q = P.q[p], m = P.m[p];
+      const double q = qm[0], m = qm[1];
       force = fabs(q / m) * dtp / kC * (sqrt(de[0] * de[0] + de[1] * de[1] + de[2] * de[2]) +
                                         kC * sqrt(db[0] * db[0] + db[1] * db[1] + db[2] * db[2]));
       double w1 = 0., w2 = 0.;
@@ -482,7 +503,7 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks(PArgs P, GArgs G, St
         w2 += q * g[d] * (b[d] - a[d]);
       }
       defect = fabs(w1);
-      scale = P.gridAbs[p] + fabs(w2);
+      scale = qm[2] + fabs(w2);
       calls = 1.;
     }
     double* sf = bs;
@@ -724,6 +745,7 @@ __global__ void __launch_bounds__(256) kReduceCurrent(double* jx, double* jy, do
     nbShift[nb] = sh;
   }
   __syncthreads();
+#pragma unroll
   for (int d = 0; d < 3; ++d) {
     const int n0 = G.ne[d][0], n1 = G.ne[d][1], n2 = G.ne[d][2];
     // owned node box (the last tile of an axis also owns the extra node)

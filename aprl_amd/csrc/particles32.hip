@@ -122,32 +122,36 @@ __device__ bool clipChord(const GArgs32& G, Chord32& c, float& kept, unsigned& e
 template <int SHAPE, bool WITH_B>
 __device__ void stageTile32(const GArgs32& G, const TileBox<SHAPE>& box, float* es, float* bs) {
   constexpr int N = boxNodes<SHAPE>();
-  constexpr int kIters = (3 * N + 2 * kChunk - 1) / (2 * kChunk);
-  float ev[kIters], bv[kIters];
+  constexpr int kPer = (N + 2 * kChunk - 1) / (2 * kChunk);
+  // The component loop is unrolled so that its extents and base pointers
+  // are kernel-argument constants (a runtime component index would turn
+  // each of them into a dependent memory load).
+  float ev[kPer][3], bv[kPer][3];
 #pragma unroll
-  for (int r = 0; r < kIters; ++r) {
-    const int t = static_cast<int>(threadIdx.x) + r * 2 * kChunk;
-    ev[r] = 0.f;
-    bv[r] = 0.f;
-    if (t < 3 * N) {
-      const int d = t / N, q0 = t - d * N;
-      const int li = q0 % Box<SHAPE>::NX, q = q0 / Box<SHAPE>::NX, lj = q % Box<SHAPE>::NY, lk = q / Box<SHAPE>::NY;
-      const int i = box.x0 + li, j = box.y0 + lj, k = box.z0 + lk;
-      if (i >= 0 && j >= 0 && k >= 0) {
-        if (i < G.ne[d][0] && j < G.ne[d][1] && k < G.ne[d][2])
-          ev[r] = G.esum[d][(static_cast<size_t>(k) * G.ne[d][1] + j) * G.ne[d][0] + i];
-        if (WITH_B && i < G.nb[d][0] && j < G.nb[d][1] && k < G.nb[d][2])
-          bv[r] = G.b[d][(static_cast<size_t>(k) * G.nb[d][1] + j) * G.nb[d][0] + i];
-      }
+  for (int r = 0; r < kPer; ++r) {
+    const int q0 = static_cast<int>(threadIdx.x) + r * 2 * kChunk;
+    const int li = q0 % Box<SHAPE>::NX, q = q0 / Box<SHAPE>::NX, lj = q % Box<SHAPE>::NY, lk = q / Box<SHAPE>::NY;
+    const int i = box.x0 + li, j = box.y0 + lj, k = box.z0 + lk;
+    const bool inBox = q0 < N && i >= 0 && j >= 0 && k >= 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      ev[r][d] = float(0);
+      bv[r][d] = float(0);
+      if (inBox && i < G.ne[d][0] && j < G.ne[d][1] && k < G.ne[d][2])
+        ev[r][d] = G.esum[d][(static_cast<size_t>(k) * G.ne[d][1] + j) * G.ne[d][0] + i];
+      if (WITH_B && inBox && i < G.nb[d][0] && j < G.nb[d][1] && k < G.nb[d][2])
+        bv[r][d] = G.b[d][(static_cast<size_t>(k) * G.nb[d][1] + j) * G.nb[d][0] + i];
     }
   }
 #pragma unroll
-  for (int r = 0; r < kIters; ++r) {
-    const int t = static_cast<int>(threadIdx.x) + r * 2 * kChunk;
-    if (t < 3 * N) {
-      es[t] = ev[r];
-      if (WITH_B) bs[t] = bv[r];
-    }
+  for (int r = 0; r < kPer; ++r) {
+    const int q0 = static_cast<int>(threadIdx.x) + r * 2 * kChunk;
+    if (q0 < N)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        es[d * N + q0] = ev[r][d];
+        if (WITH_B) bs[d * N + q0] = bv[r][d];
+      }
   }
 }
 
@@ -258,6 +262,7 @@ __device__ void gatherBQuintic32(const GArgs32& G, Chord32 c, float res[3], unsi
       cell[half][d] = static_cast<int>(fl) + c.org[d];
       fraction[half][d] = x - fl;
     }
+#pragma unroll
   for (int axis = 0; axis < 3; ++axis) {
     const int hx = axis == 0 ? 0 : 1, hy = axis == 1 ? 0 : 1, hz = axis == 2 ? 0 : 1;
     const int* n = G.nb[axis];
@@ -693,6 +698,7 @@ __global__ void __launch_bounds__(256) kReduceCurrentSplit(float* jx, float* jy,
     nbShift[nb] = sh;
   }
   __syncthreads();
+#pragma unroll
   for (int d = 0; d < 3; ++d) {
     const int n0 = G.ne[d][0], n1 = G.ne[d][1], n2 = G.ne[d][2];
     const int i0 = tx * kTileX, i1 = tx == ntx - 1 ? n0 : min(n0, i0 + kTileX);
