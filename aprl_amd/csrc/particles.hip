@@ -1351,13 +1351,15 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
     const bool readback = !(n && trial == 0);
     const unsigned trialSeq = readback ? C.mailbox.next(1) : 0u;
     {
-      ScopedKernel sk(F.profiler, st, "residual");
-      if (nChunks && quintic)
-        kGatherChunks<1, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G, S, C.chunks.get(), ntx, nty, F.g.dt,
-                                                                         C.reduceBlocks.get(), errT);
-      else if (nChunks)
-        kGatherChunks<1, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G, S, C.chunks.get(), ntx, nty, F.g.dt,
-                                                                            C.reduceBlocks.get(), errT);
+      {
+        ScopedKernel sk(F.profiler, st, "residual");
+        if (nChunks && quintic)
+          kGatherChunks<1, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G, S, C.chunks.get(), ntx, nty, F.g.dt,
+                                                                           C.reduceBlocks.get(), errT);
+        else if (nChunks)
+          kGatherChunks<1, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G, S, C.chunks.get(), ntx, nty,
+                                                                              F.g.dt, C.reduceBlocks.get(), errT);
+      }
       kReduceBlocks<<<1, 256, 0, st>>>(C.reduceBlocks.get(), static_cast<int>(nChunks), C.reduceOut.get(),
                                        readback ? C.mailbox.device() : nullptr, trialSeq,
                                        C.error.get() + kTrialWord + checkedTrials,

@@ -1211,8 +1211,8 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
         kGatherChunks32<1, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G, S, C.chunks.get(), ntx, nty,
                                                                               F.g.dt, stats, errT);
       launchCheck("residual");
-      if (last) reduceChunkStats(st, C.reduceBlocks.get(), static_cast<int>(nChunks), C.reduceOut.get());
     }
+    if (last) reduceChunkStats(st, C.reduceBlocks.get(), static_cast<int>(nChunks), C.reduceOut.get());
     for (int a = 0; a < 6; ++a) std::swap(A.fin[a], A.fout[a]);
   }
   // commitConservativeElectric with the last trial's current

@@ -32,14 +32,14 @@ WORD = 8  # FP64
 # profiles/bending_64x64x512_fp64_binomial3x_kernels_traffic.md. Traffic
 # above the algorithmic bytes is re-reading (staged halos, per-chunk boxes).
 MEASURED_TRAFFIC_FP64 = {
-    "magnetic": (2 * 89686 + 69040 + 2 * 10118 + 1792) * 1024,
-    "electric": (2 * 119936 + 70141) * 1024,
-    "electric_preview": (2 * 56547 + 27084) * 1024,
-    "gather": (2 * 37738 + 4862) * 1024,
-    "residual": (2 * 42060 + 4901) * 1024,
-    "deposit": (2 * 18253 + 26176) * 1024,
-    "push": (2 * 5562 + 5469) * 1024,
-    "reduce_current": (2 * 59630 + 57898) * 1024,
+    "magnetic": (2 * 89556 + 69040 + 2 * 10115 + 1792) * 1024,
+    "electric": (2 * 119920 + 70137) * 1024,
+    "electric_preview": (2 * 56589 + 27084) * 1024,
+    "gather": (2 * 38786 + 4868) * 1024,
+    "residual": (2 * 42991 + 4904) * 1024,
+    "deposit": (2 * 19937 + 26176) * 1024,
+    "push": (2 * 5845 + 5859) * 1024,
+    "reduce_current": (2 * 59655 + 57903) * 1024,
 }
 
 
