@@ -27,11 +27,12 @@ _libs: dict[str, C.CDLL] = {}
 
 
 def build_oracle() -> None:
-    """Compile liboracle.so (and _ref when the reference tree is present)."""
+    """Compile liboracle.so (and _ref when the reference tree is present),
+    from scratch on every call."""
     targets = ["liboracle.so"]
     if os.path.isdir(REFERENCE_TREE):
         targets.append("ref")
-    subprocess.run(["make", "-s", "-C", ORACLE_DIR, *targets], check=True)
+    subprocess.run(["make", "-s", "-B", "-C", ORACLE_DIR, *targets], check=True)
 
 
 def have_ref() -> bool:
