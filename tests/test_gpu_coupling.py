@@ -25,7 +25,7 @@ def rel_l2(a, b):
 SHAPES = {"binomial3": ("binomial3", True, "linear-binomial3"), "quintic": ("quintic", False, "quintic-compensated")}
 
 
-def case(dims, cells, n_particles, seed, gamma_u=10.0, spread=(1.0, 1.0, 3.0), shape="binomial3"):
+def case(dims, cells, n_particles, seed, gamma_u=10.0, spread=(1.0, 1.0, 3.0), shape="binomial3", magnet="dipole"):
     import aprl_amd as A
     oshape, smooth, gshape = SHAPES[shape]
     dx = 2.5 * DZ
@@ -34,7 +34,14 @@ def case(dims, cells, n_particles, seed, gamma_u=10.0, spread=(1.0, 1.0, 3.0), s
     H.seeded_fields(orc, seed, scale=1e5)
     co = H.CpuCoupling(orc, oshape, smooth, cpml_cells=cells)
     co.set_frame(5.0)
-    co.add_magnet("dipole", [0, 0, -1.0], 2.0, peak=0.5)
+    # the lab element: a 0.5 T dipole, or a planar undulator / quadrupole
+    # over the bunch's lab z (about 1e-4 m at t = 0)
+    if magnet == "dipole":
+        co.add_magnet("dipole", [0, 0, -1.0], 2.0, peak=0.5)
+    elif magnet == "undulator":
+        co.add_magnet("undulator", [0, 0, 0.0], 1.0, period=0.02, peak=1.2, angle=0.3, end_ramp=1.0, phase=0.4)
+    else:
+        co.add_magnet("quadrupole", [0, 0, 0.0], 1.0, gradient=40.0)
     g = A.EBGridGeometry(*dims, dx, dx, DZ, DZ / H.C_SI)
     gpu = A.EBFieldGrid(g)
     win = A.MovingWindowFields(gpu, A.EBCPMLParameters(cells))
@@ -42,7 +49,13 @@ def case(dims, cells, n_particles, seed, gamma_u=10.0, spread=(1.0, 1.0, 3.0), s
         gpu.set_component(a, orc.view(a))
     cg = A.ConservativeSourceCoupling(gpu, gshape)
     cg.setFrame(5.0)
-    cg.addPrescribedLabMagnet("uniform-dipole", [0, 0, -1.0], 2.0, peak_field=0.5)
+    if magnet == "dipole":
+        cg.addPrescribedLabMagnet("uniform-dipole", [0, 0, -1.0], 2.0, peak_field=0.5)
+    elif magnet == "undulator":
+        cg.addPrescribedLabMagnet("planar-undulator", [0, 0, 0.0], 1.0, period=0.02, peak_field=1.2,
+                                  polarization_angle=0.3, end_ramp_periods=1.0, undulator_phase=0.4)
+    else:
+        cg.addPrescribedLabMagnet("quadrupole", [0, 0, 0.0], 1.0, quadrupole_gradient=40.0)
     cg.setParticleBoundary(cells)
     nx, ny, nz = dims
     center = [0.5 * nx * dx, 0.5 * ny * dx, 0.45 * nz * DZ]
@@ -60,7 +73,7 @@ def planes_for(rng, dims, comps):
             for a in comps]
 
 
-def run_both(orc, co, gpu, win, cg, store, dims, steps, seed, check_each=False):
+def run_both(orc, co, gpu, win, cg, store, dims, steps, seed, check_each=False, substeps=1):
     rng = np.random.default_rng(seed)
     offset = 50
     origin = np.zeros(3)
@@ -72,8 +85,8 @@ def run_both(orc, co, gpu, win, cg, store, dims, steps, seed, check_each=False):
         orc.window_begin(n, offset, lower); win.beginStep(n, offset, lower)
         orc.window_advance_magnetic(); gpu.advanceMagnetic()
         orc.window_set_inlet(offset, n + 1, *inlet); win.setInlet(*inlet)
-        ro = co.step(origin, t)
-        rg = cg.step(store, origin, t)
+        ro = co.step(origin, t, substeps)
+        rg = cg.step(store, origin, t, substeps)
         logs.append((ro[0], rg[0]))
         if check_each:
             compare(orc, co, gpu, store)
@@ -104,6 +117,22 @@ def test_coupled_steps_match_oracle(shape):
     logs = run_both(*args, dims, 4, 4, check_each=True)
     assert all(a == b for a, b in logs), logs
     assert all(a >= 2 for a, _ in logs)
+
+
+@pytest.mark.parametrize("magnet,substeps", [("undulator", 4), ("quadrupole", 1)])
+def test_prescribed_undulator_and_quadrupole_match_oracle(magnet, substeps):
+    """SIMagneticElement::fieldsLab of the planar undulator (with its end
+    ramps, phase and polarization; several push substeps per step, as the
+    reference uses with an undulator) and of the quadrupole, boosted into the
+    box frame, through whole coupled steps."""
+    dims = (20, 20, 40)
+    args = case(dims, (4, 4, 0), 200, 29, magnet=magnet)
+    logs = run_both(*args, dims, 3, 6, check_each=True, substeps=substeps)
+    assert all(a == b for a, b in logs), logs
+    # the element's field acts on the bunch (box-frame B at a particle)
+    xo, _ = args[1].particles()
+    _, b = args[1].prescribed_box(xo[0], 0.0)
+    assert np.linalg.norm(b) > 1e-3
 
 
 @pytest.mark.parametrize("shape", ["binomial3", "quintic"])
