@@ -139,6 +139,12 @@ class ConservativeSourceCoupling:
         """Lab z of the held particles' release (NaN disables it)."""
         check(lib().aprl_coupling_set_activation(self.handle, float(lab_z)))
 
+    def setHoldTransverse(self, hold: bool):
+        """config reference.hold_transverse_until_activation: held particles
+        move z += c dt beta (True, the default here) or ballistically along
+        their proper velocity (False, the reference's default)."""
+        check(lib().aprl_coupling_set_hold_transverse(self.handle, int(bool(hold))))
+
     def boundaryCounters(self):
         """Cumulative (counts[20], charges[20]): CPML entries per face (0-5),
         outer exits of particles (6-11) and of carriers (12-17), window exits

@@ -229,6 +229,13 @@ aprl_status aprl_coupling_set_activation(aprl_coupling* coupling, double activat
   });
 }
 
+aprl_status aprl_coupling_set_hold_transverse(aprl_coupling* coupling, int hold) {
+  return guarded([&] {
+    if (!coupling) throw std::invalid_argument("Null coupling");
+    coupling->c.holdTransverse = hold != 0;
+  });
+}
+
 aprl_status aprl_particles_download_states(aprl_particles* particles, int* state) {
   return guarded([&] {
     if (!particles || !state) throw std::invalid_argument("Null particle store or state array");

@@ -24,6 +24,7 @@ struct StepArgs {
   double exitLabZ;
   int hasActivation;
   double activationLabZ;
+  int holdTransverse;  // held motion: z += c dt beta (1) or ballistic along u (0)
   double extent[3], posTol[3];  // validatePosition
 };
 
@@ -464,35 +465,48 @@ __device__ __forceinline__ int activeSegmentEvents(const StepArgs& S, const doub
 }
 
 // ---------------------------------------------------------------------------
-// Held particles (prepareParticle's hold-transverse branch,
-// simulation.cpp:4951-5000).
+// Held particles (prepareParticle's frozen branch, simulation.cpp:
+// 4951-5000): with hold_transverse_until_activation z += c dt beta_held and
+// the transverse position stays; otherwise the held particle moves
+// ballistically, x += (c dt / gamma) u (RelativisticBorisPusher::
+// pushPosition, eb_particles.cpp:131-139). Released where its lab z reaches
+// the activation plane, at the fraction f of the step, from the point the
+// same motion reaches in f dt.
 
 struct HeldMove {
   bool release;     // the lab z reaches the activation plane this step
   bool kink;        // released strictly inside the step
   double f;         // release time fraction
-  double z1;        // held end z (no release) or release z
+  double x1[3];     // held end (no release) or release point
   double remaining; // push time after the release
 };
 
-__device__ __forceinline__ HeldMove heldMove(const StepArgs& S, const double x0[3], double beta) {
+__device__ __forceinline__ void heldAdvance(const StepArgs& S, const double x0[3], const double u[3], double beta,
+                                            double timeStep, double frac, double out[3]) {
+  for (int a = 0; a < 3; ++a) out[a] = x0[a];
+  if (S.holdTransverse) {
+    // c dt beta, or c f dt beta at the release, in the reference's order
+    out[2] += (frac < 0.0 ? kC * S.dt : kC * frac * S.dt) * beta;
+  } else {
+    const double s = kC * timeStep / gammaFromU(u[0], u[1], u[2]);
+    for (int a = 0; a < 3; ++a) out[a] += s * u[a];
+  }
+}
+
+__device__ __forceinline__ HeldMove heldMove(const StepArgs& S, const double x0[3], const double u[3], double beta) {
   HeldMove h;
   const double startLabZ = labZOf(S, x0[2], S.time);
-  double z = x0[2];
-  z += kC * S.dt * beta;
-  const double endLabZ = labZOf(S, z, S.time + S.dt);
+  heldAdvance(S, x0, u, beta, S.dt, -1.0, h.x1);
+  const double endLabZ = labZOf(S, h.x1[2], S.time + S.dt);
   h.release = S.hasActivation && endLabZ >= S.activationLabZ;
   h.f = 0.0;
   h.kink = false;
   h.remaining = 0.0;
-  h.z1 = z;
   if (h.release) {
     h.f = startLabZ >= S.activationLabZ
               ? 0.0
               : fmax(0.0, fmin(1.0, (S.activationLabZ - startLabZ) / (endLabZ - startLabZ)));
-    double zk = x0[2];
-    zk += kC * h.f * S.dt * beta;
-    h.z1 = zk;
+    heldAdvance(S, x0, u, beta, h.f * S.dt, h.f, h.x1);
     const double activeStart = S.time + h.f * S.dt;
     h.kink = activeStart > S.time && activeStart < S.time + S.dt;
     h.remaining = (1.0 - h.f) * S.dt;
@@ -533,8 +547,8 @@ __device__ __forceinline__ int entryStep(const StepArgs& S, const double o[3], i
   double t0 = S.time, dtp = S.dt;
   bool push = true;
   if (held) {
-    h = heldMove(S, x0, beta);
-    xs[2] = h.z1;
+    h = heldMove(S, x0, u0, beta);
+    for (int a = 0; a < 3; ++a) xs[a] = h.x1[a];
     push = h.release && h.remaining > 0.0;
     t0 = S.time + h.f * S.dt;
     dtp = h.remaining;
@@ -556,7 +570,7 @@ __device__ __forceinline__ int entryStep(const StepArgs& S, const double o[3], i
     code = kEvtReleased;
     if (S.window && x0[2] < o[2]) err |= kErrHeldRelease;
     if (h.kink) {
-      sp[2] = h.z1;
+      for (int a = 0; a < 3; ++a) sp[a] = h.x1[a];
       BoundaryHit b1, b2;
       if (cpmlFirstEntry(S, x0, sp, b1, err) || outerFirstExit(S, x0, sp, b2, err)) err |= kErrHeldFrozenLeg;
       validateEnd(S, o, x0, err);

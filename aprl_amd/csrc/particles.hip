@@ -450,10 +450,11 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks(PArgs P, GArgs G, St
   if (st == kStateActive || st == kStateHeld)
     for (int d = 0; d < 3; ++d) a[d] = x0[d];
   if (st == kStateHeld) {
-    const HeldMove h = heldMove(S, a, betaP);
+    const double uh[3] = {P.u[0][p], P.u[1][p], P.u[2][p]};
+    const HeldMove h = heldMove(S, a, uh, betaP);
     if (h.release && h.remaining > 0.0) {
       live = true;
-      a[2] = h.z1;
+      for (int d = 0; d < 3; ++d) a[d] = h.x1[d];
       dtp = h.remaining;
     }
   }
@@ -602,7 +603,7 @@ __global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, StepArgs S,
     bsum += fmax(fabs(s[0]), fmax(fabs(s[1]), fabs(s[2])));
     // a held entry's release kink splits its path: both legs stay below
     // the chord plus the held leg
-    if (P.state[p] == kStateHeld) bsum += fabs(P.q[p]) * (kC * S.dt * fabs(P.beta[p])) / P.scaleDen;
+    if (P.state[p] == kStateHeld) bsum += fabs(P.q[p]) * (kC * S.dt * (S.holdTransverse ? fabs(P.beta[p]) : 1.0)) / P.scaleDen;
   }
   bound[threadIdx.x] = bsum;
   __syncthreads();
@@ -663,10 +664,11 @@ __global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, StepArgs S,
     }
     int legs = 1;
     if (P.state[p] == kStateHeld) {
-      const HeldMove h = heldMove(S, xa, P.beta[p]);
+      const double uh[3] = {P.u[0][p], P.u[1][p], P.u[2][p]};
+      const HeldMove h = heldMove(S, xa, uh, P.beta[p]);
       if (h.release && h.kink) {
         legs = 2;
-        xk[2] = h.z1;
+        for (int d = 0; d < 3; ++d) xk[d] = h.x1[d];
       }
     }
     const double q = P.q[p];
@@ -1004,6 +1006,7 @@ StepArgs stepArgsFor(const CouplingState& C, const aprl_grid_geometry& g, const 
   S.exitLabZ = C.exitLabZ;
   S.hasActivation = C.hasActivation ? 1 : 0;
   S.activationLabZ = C.activationLabZ;
+  S.holdTransverse = C.holdTransverse ? 1 : 0;
   return S;
 }
 

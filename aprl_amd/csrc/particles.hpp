@@ -151,6 +151,7 @@ struct CouplingState {
   // lab activation plane of held particles (config reference.activation_plane_z)
   double activationLabZ = 0.0;
   bool hasActivation = false;
+  bool holdTransverse = true;  // reference.hold_transverse_until_activation
   DeviceArray<double> estar[3];
   // E_cand + E_old of the current trial (the preview writes the zero-current
   // sums on the whole grid; the current reduction rewrites the active tiles)

@@ -376,16 +376,18 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks32(PArgs32 P, GArgs32
   const int st = p < ch.z ? P.state[p] : kStateRemoved;
   bool live = st == kStateActive, released = false;
   double dtp = dt;
-  float dzk = 0.f;  // held leg to the release point, in cells
+  float dk[3] = {0.f, 0.f, 0.f};  // held leg to the release point, in cells
   if (st == kStateHeld) {
-    double x0[3];
-    for (int d = 0; d < 3; ++d)
+    double x0[3], uh[3];
+    for (int d = 0; d < 3; ++d) {
       x0[d] = (static_cast<double>(P.cell[d][p]) + static_cast<double>(P.off[d][p])) * P.h[d];
-    const HeldMove h = heldMove(S, x0, static_cast<double>(P.beta[p]));
+      uh[d] = P.u[d][p];
+    }
+    const HeldMove h = heldMove(S, x0, uh, static_cast<double>(P.beta[p]));
     if (h.release && h.remaining > 0.0) {
       live = released = true;
       dtp = h.remaining;
-      dzk = static_cast<float>((h.z1 - x0[2]) / P.h[2]);
+      for (int d = 0; d < 3; ++d) dk[d] = static_cast<float>((h.x1[d] - x0[d]) / P.h[d]);
     }
   }
   unsigned err = 0;
@@ -395,12 +397,12 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks32(PArgs32 P, GArgs32
   if (live) {
     chordAt(P, p, MODE == 0, G, c0, off0, v);
     if (released) {
-      off0[2] += dzk;
+      for (int d = 0; d < 3; ++d) off0[d] += dk[d];
       if (MODE == 0) {
         const float r = static_cast<float>(dtp / dt);
         for (int d = 0; d < 3; ++d) v[d] *= r;
       } else {
-        v[2] -= dzk;
+        for (int d = 0; d < 3; ++d) v[d] -= dk[d];
       }
     }
     const Chord32 c = chordOf(G, c0[0], c0[1], c0[2], off0, v);
@@ -557,9 +559,12 @@ __global__ void __launch_bounds__(kChunk) kDepositSplit(PArgs32 P, GArgs32 G, St
     bsum += fmax(fabs(static_cast<double>(s[0])), fmax(fabs(static_cast<double>(s[1])), fabs(static_cast<double>(s[2]))));
     // a held entry's release kink splits its path: both legs stay below
     // the chord plus the held leg
-    if (P.state[p] == kStateHeld)
-      bsum += fabs(static_cast<double>(P.q[p])) * static_cast<double>(P.kq[2]) *
-              (kC * S.dt * fabs(static_cast<double>(P.beta[p])) / P.h[2]);
+    if (P.state[p] == kStateHeld) {
+      double leg = static_cast<double>(P.kq[2]) * (kC * S.dt * fabs(static_cast<double>(P.beta[p])) / P.h[2]);
+      if (!S.holdTransverse)  // ballistic: at most c dt along any axis
+        for (int a = 0; a < 3; ++a) leg = fmax(leg, static_cast<double>(P.kq[a]) * (kC * S.dt / P.h[a]));
+      bsum += fabs(static_cast<double>(P.q[p])) * leg;
+    }
   }
   bound[threadIdx.x] = bsum;
   __syncthreads();
@@ -580,15 +585,18 @@ __global__ void __launch_bounds__(kChunk) kDepositSplit(PArgs32 P, GArgs32 G, St
     chordAt(P, p, false, G, c0, off0, v);
     // A held entry released inside the step deposits its held leg to the
     // release point first, then the rest of the chord.
-    float dzk = 0.f;
+    float dk[3] = {0.f, 0.f, 0.f};
     int legs = 1;
     if (P.state[p] == kStateHeld) {
-      double x0[3];
-      for (int d = 0; d < 3; ++d) x0[d] = (static_cast<double>(c0[d]) + static_cast<double>(off0[d])) * P.h[d];
-      const HeldMove h = heldMove(S, x0, static_cast<double>(P.beta[p]));
+      double x0[3], uh[3];
+      for (int d = 0; d < 3; ++d) {
+        x0[d] = (static_cast<double>(c0[d]) + static_cast<double>(off0[d])) * P.h[d];
+        uh[d] = P.u[d][p];
+      }
+      const HeldMove h = heldMove(S, x0, uh, static_cast<double>(P.beta[p]));
       if (h.release && h.kink) {
         legs = 2;
-        dzk = static_cast<float>((h.z1 - x0[2]) / P.h[2]);
+        for (int d = 0; d < 3; ++d) dk[d] = static_cast<float>((h.x1[d] - x0[d]) / P.h[d]);
       }
     }
     Chord32 c = chordOf(G, c0[0], c0[1], c0[2], off0, v);
@@ -633,11 +641,11 @@ __global__ void __launch_bounds__(kChunk) kDepositSplit(PArgs32 P, GArgs32 G, St
     for (int leg = 0; leg < legs; ++leg) {
       const bool last = leg + 1 == legs;
       if (legs == 2) {
-        // leg 0: (0, 0, dzk) from the start; leg 1: the rest from the kink
-        c.x[2] = off0[2] + (leg ? dzk : 0.f);
-        c.v[0] = leg ? v[0] : 0.f;
-        c.v[1] = leg ? v[1] : 0.f;
-        c.v[2] = leg ? v[2] - dzk : dzk;
+        // leg 0: dk from the start; leg 1: the rest from the kink
+        for (int d = 0; d < 3; ++d) {
+          c.x[d] = off0[d] + (leg ? dk[d] : 0.f);
+          c.v[d] = leg ? v[d] - dk[d] : dk[d];
+        }
       }
       scaleOf(p, c.v, scale);
       const int count = orbitCuts(c.x, c.v, cuts, err);

@@ -165,8 +165,9 @@ size_t aprl_particles_count(const aprl_particles* particles);
 aprl_status aprl_particles_download_states(aprl_particles* particles, int* state);
 /* RelativisticParticleSI::frozen / frozenBetaZ (eb_particles.h:21-22) for
  * the entries in upload order, before the store's first step: a held
- * particle moves z += c dt beta with its transverse motion and momentum held
- * until its lab z reaches the activation plane (prepareParticle's
+ * particle keeps its momentum and moves without fields (z += c dt beta, or
+ * ballistically, aprl_coupling_set_hold_transverse) until its lab z reaches
+ * the activation plane (prepareParticle's
  * hold-transverse branch, simulation.cpp:4951-5000); upstream of a moving
  * window it deposits only its shaped support until it crosses the inlet. */
 aprl_status aprl_particles_set_held(aprl_particles* particles, const int* frozen, const double* beta);
@@ -239,6 +240,12 @@ aprl_status aprl_window_complete_inlet(aprl_fields* fields, aprl_coupling* coupl
 /* Lab z of the held particles' activation plane (config
  * reference.activation_plane_z); NaN disables the release. */
 aprl_status aprl_coupling_set_activation(aprl_coupling* coupling, double activation_lab_z);
+/* config reference.hold_transverse_until_activation (prepareParticle,
+ * simulation.cpp:4951-4965): hold = 1 moves a held particle z += c dt beta
+ * with its transverse position held (the library's default); hold = 0 moves
+ * it ballistically, x += (c dt / gamma) u (RelativisticBorisPusher::
+ * pushPosition, eb_particles.cpp:131-139), the reference's default. */
+aprl_status aprl_coupling_set_hold_transverse(aprl_coupling* coupling, int hold);
 
 /* Pending admission queue of the moving window (PendingParticleQueue,
  * pending_particle_queue.h:25-76): held particles upstream of the inlet

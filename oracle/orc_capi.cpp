@@ -284,6 +284,9 @@ int orc_coupling_get_frozen(void* h, int* frozen) {
 int orc_coupling_set_activation(void* h, double labZ) {
   return guarded([&] { CH(h)->held.activationLabZ = labZ; });
 }
+int orc_coupling_set_hold_transverse(void* h, int hold) {
+  return guarded([&] { CH(h)->held.holdTransverse = hold != 0; });
+}
 
 int orc_coupling_get_ids(void* h, std::uint64_t* id) {
   return guarded([&] {

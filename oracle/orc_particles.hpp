@@ -996,7 +996,18 @@ inline void carrierSegment(Coupling& c, const ParticleBoundary& pb, Carrier carr
 // support (depositFrozenSourceSupport) until they cross the inlet
 // (5121-5155). Retirement planes, ballistic transverse motion and the
 // multi-rank transfers are not part of this path.
-struct HeldRelease { double activationLabZ = std::numeric_limits<double>::infinity(); };
+struct HeldRelease {
+  double activationLabZ = std::numeric_limits<double>::infinity();
+  // reference.hold_transverse_until_activation: z += c dt beta (true) or the
+  // ballistic RelativisticBorisPusher::pushPosition (eb_particles.cpp:131-139)
+  bool holdTransverse = true;
+};
+
+// pushPosition: x += (c dt / gamma) u
+inline void heldBallistic(V3& x, const V3& u, double timeStep) {
+  const double s = kC * timeStep / gammaFromU(u);
+  for (int a = 0; a < 3; ++a) x[a] += s * u[a];
+}
 
 inline void pushDepositAll(Coupling& c, std::vector<Particle>& particles, std::vector<Carrier>& carriers,
                            BoundaryCounters& k, double time, double dt, unsigned substeps,
@@ -1032,7 +1043,8 @@ inline void pushDepositAll(Coupling& c, std::vector<Particle>& particles, std::v
     r.hasActive = !p.frozen;
     if (p.frozen) {
       const double startLabZ = c.frame.labZFromBoxZT(start[2], time);
-      p.position[2] += kC * dt * p.betaHeld;
+      if (held.holdTransverse) p.position[2] += kC * dt * p.betaHeld;
+      else heldBallistic(p.position, p.u, dt);
       const double endLabZ = c.frame.labZFromBoxZT(p.position[2], time + dt);
       if (endLabZ >= held.activationLabZ) {
         if (pb.window && start[2] < windowOrigin[2])
@@ -1041,7 +1053,8 @@ inline void pushDepositAll(Coupling& c, std::vector<Particle>& particles, std::v
                              ? 0.0
                              : std::max(0.0, std::min(1.0, (held.activationLabZ - startLabZ) / (endLabZ - startLabZ)));
         p.position = start;
-        p.position[2] += kC * f * dt * p.betaHeld;
+        if (held.holdTransverse) p.position[2] += kC * f * dt * p.betaHeld;
+        else heldBallistic(p.position, p.u, f * dt);
         p.frozen = false;
         r.hasActive = true;
         r.activeStart = p;

@@ -270,6 +270,7 @@ struct RefCoupling {
   bool quintic = false, binomial3 = false, smoothX = false;
   EBCPMLParameters cpml;
   Double activationLabZ = std::numeric_limits<Double>::infinity();
+  bool holdTransverse = true;  // reference.hold_transverse_until_activation
   // rank 0's pending admission queue and the step of its last admission
   std::unique_ptr<PendingParticleQueue> pending;
   std::size_t pendingStep = 0;
@@ -389,9 +390,12 @@ int ref_coupling_step(void* h, const double* origin, double time, unsigned subst
         hasActive[n] = !particle.frozen;
         activePosition[n] = particle.position;
         if (particle.frozen) {
-          const Double betaHeld = particle.frozenBetaZ;
+          const Double betaHeld = H.holdTransverse ? particle.frozenBetaZ : 0.0;
           const Double startLabZ = H.frame.labZFromBoxZT(start[2], time);
-          particle.position[2] += SI::c * g.dt * betaHeld;
+          if (H.holdTransverse)
+            particle.position[2] += SI::c * g.dt * betaHeld;
+          else
+            RelativisticBorisPusher::pushPosition(particle.position, particle.properVelocity, g.dt);
           const Double endLabZ = H.frame.labZFromBoxZT(particle.position[2], time + g.dt);
           if (endLabZ >= H.activationLabZ) {
             if (window && start[2] < o[2])
@@ -399,7 +403,11 @@ int ref_coupling_step(void* h, const double* origin, double time, unsigned subst
             const Double crossingFraction = startLabZ >= H.activationLabZ ? 0.0 :
               std::max(0.0, std::min(1.0, (H.activationLabZ - startLabZ) / (endLabZ - startLabZ)));
             particle.position = start;
-            particle.position[2] += SI::c * crossingFraction * g.dt * betaHeld;
+            if (H.holdTransverse)
+              particle.position[2] += SI::c * crossingFraction * g.dt * betaHeld;
+            else
+              RelativisticBorisPusher::pushPosition(particle.position, particle.properVelocity,
+                                                    crossingFraction * g.dt);
             particle.frozen = false;
             hasActive[n] = 1;
             activePosition[n] = particle.position;
@@ -695,6 +703,9 @@ int ref_coupling_get_frozen(void* h, int* frozen) {
 }
 int ref_coupling_set_activation(void* h, double labZ) {
   return guarded([&] { RC(h)->activationLabZ = labZ; });
+}
+int ref_coupling_set_hold_transverse(void* h, int hold) {
+  return guarded([&] { RC(h)->holdTransverse = hold != 0; });
 }
 
 // The reference's PendingParticleQueue over the coupling's particles, as

@@ -178,7 +178,7 @@ def activation_lab_z(boost_gamma=GAMMA_BOOST):
     return boost_gamma * (window_origin(OFFSET0)[2] + 12.0 * DZ)
 
 
-def held_oracle_case(prefix, shape="binomial3", smooth=True, seed=31):
+def held_oracle_case(prefix, shape="binomial3", smooth=True, seed=31, hold=True):
     import harness as H
     f = H.CpuFields(prefix, *DIMS, DX, DX, DZ, DZ / C_SI, 1)
     f.install_window(CELLS)
@@ -189,6 +189,7 @@ def held_oracle_case(prefix, shape="binomial3", smooth=True, seed=31):
     c.set_particles(x, u, q, m, np.ones(len(q)), ids)
     c.set_held(frozen, beta)
     c.set_activation(activation_lab_z())
+    c.set_hold_transverse(hold)
     return f, c
 
 

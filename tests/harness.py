@@ -268,6 +268,7 @@ def _bind_particles(lib, prefix):
     f("coupling_set_held").argtypes = [vp, C.POINTER(C.c_int), _dp]
     f("coupling_get_frozen").argtypes = [vp, C.POINTER(C.c_int)]
     f("coupling_set_activation").argtypes = [vp, C.c_double]
+    f("coupling_set_hold_transverse").argtypes = [vp, C.c_int]
     f("coupling_set_pending").argtypes = [vp, C.c_double, C.c_int64, C.c_double, C.c_double, C.c_uint64,
                                           C.c_uint64]
     f("coupling_admit").argtypes = [vp, C.c_uint64, C.POINTER(sz)]
@@ -439,6 +440,9 @@ class CpuCoupling:
 
     def set_activation(self, lab_z):
         _call(self.prefix, "coupling_set_activation", self.h, lab_z)
+
+    def set_hold_transverse(self, hold):
+        _call(self.prefix, "coupling_set_hold_transverse", self.h, int(bool(hold)))
 
     def set_pending(self, reference_origin_z, initial_offset, dz, held_distance_per_beta, maximum_step,
                     initial_step=0):

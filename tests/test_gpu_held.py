@@ -1,7 +1,8 @@
 """GPU held (frozen) particles vs the oracle.
 
-Held electrons inside the window and upstream of it (beta_held = 0.99)
-with an activation plane that releases them during the run, next to active
+Held electrons inside the window and upstream of it (beta_held = 0.99,
+or ballistic along their momentum without the transverse hold) with an
+activation plane that releases them during the run, next to active
 electrons: the held motion, the upstream shaped support, the admission at
 the inlet, the release kink with the push of the rest of the step, and the
 residual of released particles. FP64 to 1e-12 with identical Picard trial
@@ -23,10 +24,10 @@ def rel_l2(a, b):
     return np.linalg.norm(a - b) / (den if den > 0 else 1.0)
 
 
-def run(shape, precision, steps=6, seed=12):
+def run(shape, precision, steps=6, seed=12, hold=True):
     import aprl_amd as A
     oshape, smooth, gshape = SHAPES[shape]
-    orc, co = E.held_oracle_case("orc", oshape, smooth)
+    orc, co = E.held_oracle_case("orc", oshape, smooth, hold=hold)
     gpu = A.EBFieldGrid(A.EBGridGeometry(*E.DIMS, E.DX, E.DX, E.DZ, E.DZ / H.C_SI), precision=precision)
     win = A.MovingWindowFields(gpu, A.EBCPMLParameters(E.CELLS))
     for a in range(9):
@@ -35,6 +36,7 @@ def run(shape, precision, steps=6, seed=12):
     cg.setFrame(E.GAMMA_BOOST)
     cg.setParticleBoundary(E.CELLS)
     cg.setActivationPlane(E.activation_lab_z())
+    cg.setHoldTransverse(hold)
     x, u, q, m, ids, frozen, beta = E.held_particles()
     store = A.ParticleStore(gpu, len(q))
     store.upload(x, u, q, m, np.ones(len(q)), ids)
@@ -79,15 +81,17 @@ def compare(orc, co, gpu, store, tol):
     return fo
 
 
+@pytest.mark.parametrize("hold", [True, False])
 @pytest.mark.parametrize("shape", ["binomial3", "quintic"])
-def test_held_fp64_matches_oracle(shape):
-    orc, co, gpu, store, logs = run(shape, "fp64")
+def test_held_fp64_matches_oracle(shape, hold):
+    orc, co, gpu, store, logs = run(shape, "fp64", hold=hold)
     assert all(a == b for a, b in logs), logs
     fo = compare(orc, co, gpu, store, 1e-12)
     x0, *_ , frozen0, beta = E.held_particles()
     assert fo.sum() > 0 and fo.sum() < frozen0.sum()  # some released, some still held
 
 
-def test_held_fp32_split_tracks_oracle():
-    orc, co, gpu, store, logs = run("binomial3", "fp32-split")
+@pytest.mark.parametrize("hold", [True, False])
+def test_held_fp32_split_tracks_oracle(hold):
+    orc, co, gpu, store, logs = run("binomial3", "fp32-split", hold=hold)
     compare(orc, co, gpu, store, 2e-5)

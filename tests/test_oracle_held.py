@@ -36,10 +36,13 @@ def run_held(f, c, steps, seed=12):
 
 
 @needs_ref
+@pytest.mark.parametrize("hold", [True, False])
 @pytest.mark.parametrize("shape,smooth", [("binomial3", True), ("quintic", False)])
-def test_held_release_and_admission_bitwise(shape, smooth):
-    fo, co = E.held_oracle_case("orc", shape, smooth)
-    fr, cr = E.held_oracle_case("ref", shape, smooth)
+def test_held_release_and_admission_bitwise(shape, smooth, hold):
+    """Both held motions: z += c dt beta with the transverse position held,
+    and the reference's default ballistic pushPosition."""
+    fo, co = E.held_oracle_case("orc", shape, smooth, hold=hold)
+    fr, cr = E.held_oracle_case("ref", shape, smooth, hold=hold)
     lo = run_held(fo, co, 6)
     lr = run_held(fr, cr, 6)
     assert lo == lr
@@ -50,6 +53,32 @@ def test_held_release_and_admission_bitwise(shape, smooth):
     assert np.array_equal(xo, xr) and np.array_equal(uo, ur)
     assert np.array_equal(co.frozen(), cr.frozen())
     assert np.array_equal(co.ids(), cr.ids())
+    fr0 = E.held_particles()[5]
+    assert 0 < co.frozen().sum() < fr0.sum()  # some released, some still held
+
+
+def test_ballistic_held_particles_move_along_their_momentum():
+    """Without the transverse hold a held particle moves x += (c dt / gamma) u
+    each step with its momentum unchanged until its release."""
+    f, c = E.held_oracle_case("orc", hold=False)
+    x0, u0, q, m, ids, frozen0, beta = E.held_particles()
+    steps = 3
+    run_held(f, c, steps)
+    x, u = c.particles()
+    fr = c.frozen()
+    checked = 0
+    for k, pid in enumerate(c.ids()):
+        i = int(pid) - 1
+        if not (frozen0[i] and fr[k]):
+            continue
+        g = np.sqrt(1.0 + np.dot(u0[i], u0[i]))
+        expect = x0[i].copy()
+        for _ in range(steps):
+            expect = expect + (H.C_SI * (E.DZ / H.C_SI) / g) * u0[i]
+        assert np.allclose(x[k], expect, rtol=0, atol=1e-15)
+        assert np.array_equal(u[k], u0[i])
+        checked += 1
+    assert checked > 0
 
 
 def test_held_particles_move_by_repeated_addition_until_release():
