@@ -276,7 +276,20 @@ def main():
                     help="time only this form's binomial3-x leg (for profiling runs)")
     ap.add_argument("--error-study", action="store_true",
                     help="print the FP32-split vs FP64 errors at 100, 1000 and 10000 steps and exit")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="three plain 256 MiB device-to-device copies (the byte reference of the HBM "
+                         "traffic counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 32 * 1024 * 1024  # doubles: 256 MiB
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 8, "copies": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the whole window lives on one device); --gpus {args.gpus} "
               "is not supported", file=sys.stderr)
