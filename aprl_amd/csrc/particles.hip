@@ -312,7 +312,8 @@ struct PermuteArgs {
   double* betaDst;
 };
 
-__global__ void kPermute(PermuteArgs A, const unsigned* perm, int n) {
+// State scatter into sorted order, and the tile ranges of the sorted keys.
+__global__ void kPermute(PermuteArgs A, const unsigned* perm, int n, TileBoundsArgs T) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const unsigned q = perm[p];
@@ -321,21 +322,15 @@ __global__ void kPermute(PermuteArgs A, const unsigned* perm, int n) {
   A.idDst[p] = A.idSrc[q];
   A.stateDst[p] = A.stateSrc[q];
   A.betaDst[p] = A.betaSrc[q];
+  tileBoundsAt(T, p);
 }
 
 // Particle range of every occupied tile from the run boundaries of the
 // sorted keys (the ranges were cleared to empty before the sort), and the
 // occupied-tile count.
-__global__ void kTileBounds(const unsigned* key, int n, int ntiles, int* begin, int* end, unsigned* occupied) {
+__global__ void kTileBounds(TileBoundsArgs T) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const unsigned k = key[p];
-  if (k >= static_cast<unsigned>(ntiles)) return;  // removed or cropped entries sort last
-  if (p == 0 || key[p - 1] != k) {
-    begin[k] = p;
-    atomicAdd(occupied, 1u);
-  }
-  if (p == n - 1 || key[p + 1] != k) end[k] = p + 1;
+  if (p < T.n) tileBoundsAt(T, p);
 }
 
 // Per tile: chunk count (low word) and whether any tile within one tile in
@@ -1050,22 +1045,26 @@ static void sortAndChunk(CouplingState& C, ParticleStore& P, const GArgs& G, con
   A.stateDst = P.stateT.get();
   A.betaSrc = P.beta.get();
   A.betaDst = P.betaT.get();
-  kPermute<<<blocks, 256, 0, st>>>(A, P.permSorted.get(), n);
+  const TileBoundsArgs T{P.keySorted.get(), n, ntiles, C.tileBegin.get(), C.tileEnd.get(), occupied};
+  kPermute<<<blocks, 256, 0, st>>>(A, P.permSorted.get(), n, T);
   launchCheck("permute");
   for (int a = 0; a < 9; ++a) P.s[a].swap(P.t[a]);
   P.id.swap(P.idT);
   P.state.swap(P.stateT);
   P.beta.swap(P.betaT);
   const int rz = C.shape == kShapeQuintic ? Box<kShapeQuintic>::RZ : Box<kShapeBinomial3X>::RZ;
-  buildChunkLists(C, st, P.keySorted.get(), n, ntx, nty, ntiles, rz, occupied, C.mailbox.device(), seq);
+  buildChunkLists(C, st, P.keySorted.get(), n, ntx, nty, ntiles, rz, occupied, C.mailbox.device(), seq, true);
 }
 
 // Tile ranges of the sorted keys, chunk counts and the active flags, their
 // scan, the chunk list and the active-tile list (all in tile order).
 void buildChunkLists(CouplingState& C, hipStream_t st, const unsigned* keySorted, int n, int ntx, int nty,
-                     int ntiles, int rz, unsigned* occupied, unsigned* box, unsigned seq) {
-  kTileBounds<<<ceilDiv(n, 256), 256, 0, st>>>(keySorted, n, ntiles, C.tileBegin.get(), C.tileEnd.get(), occupied);
-  launchCheck("tile bounds");
+                     int ntiles, int rz, unsigned* occupied, unsigned* box, unsigned seq, bool boundsDone) {
+  if (!boundsDone) {
+    kTileBounds<<<ceilDiv(n, 256), 256, 0, st>>>(
+        TileBoundsArgs{keySorted, n, ntiles, C.tileBegin.get(), C.tileEnd.get(), occupied});
+    launchCheck("tile bounds");
+  }
   const int ntz = ntiles / (ntx * nty);
   kTileCounts<<<ceilDiv(ntiles + 1, 256), 256, 0, st>>>(C.tileBegin.get(), C.tileEnd.get(), ntx, nty, ntz, rz,
                                                          C.tileCounts.get());

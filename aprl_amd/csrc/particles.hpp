@@ -200,6 +200,26 @@ struct CouplingState {
   DeviceArray<double> rho[2];  // diagnostic charge densities
 };
 
+// Run boundaries of the sorted tile keys, folded into the permute kernels:
+// entry p opens its tile's range when it differs from its predecessor and
+// closes it when it differs from its successor.
+struct TileBoundsArgs {
+  const unsigned* key;  // sorted keys
+  int n, ntiles;
+  int *begin, *end;
+  unsigned* occupied;
+};
+
+__device__ __forceinline__ void tileBoundsAt(const TileBoundsArgs& T, int p) {
+  const unsigned k = T.key[p];
+  if (k >= static_cast<unsigned>(T.ntiles)) return;  // removed or cropped entries sort last
+  if (p == 0 || T.key[p - 1] != k) {
+    T.begin[k] = p;
+    atomicAdd(T.occupied, 1u);
+  }
+  if (p == T.n - 1 || T.key[p + 1] != k) T.end[k] = p + 1;
+}
+
 // Tile ranges cleared to empty by the key kernels (grid-stride over the
 // tiles), so that the boundary pass after the sort only writes occupied
 // tiles.
@@ -219,7 +239,8 @@ __device__ __forceinline__ void clearTileRanges(const TileRangeArgs& R, int gid)
 // box / seq: the mailbox (device address) that receives the scan total and
 // the sort-phase words, or nullptr.
 void buildChunkLists(CouplingState& C, hipStream_t st, const unsigned* keySorted, int n, int ntx, int nty,
-                     int ntiles, int rz, unsigned* occupied, unsigned* box = nullptr, unsigned seq = 0);
+                     int ntiles, int rz, unsigned* occupied, unsigned* box = nullptr, unsigned seq = 0,
+                     bool boundsDone = false);
 void reserveTileLists(CouplingState& C, hipStream_t st, int ntiles, int n);
 std::size_t chunkListBound(int ntiles, int n);
 void fillInts(hipStream_t st, int* p, int n, int v);

@@ -315,7 +315,8 @@ struct PermuteArgs32 {
   float* betaDst;
 };
 
-__global__ void kPermute32(PermuteArgs32 A, const unsigned* perm, int n) {
+// State scatter into sorted order, and the tile ranges of the sorted keys.
+__global__ void kPermute32(PermuteArgs32 A, const unsigned* perm, int n, TileBoundsArgs T) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const unsigned q = perm[p];
@@ -326,6 +327,7 @@ __global__ void kPermute32(PermuteArgs32 A, const unsigned* perm, int n) {
   A.idDst[p] = A.idSrc[q];
   A.stateDst[p] = A.stateSrc[q];
   A.betaDst[p] = A.betaSrc[q];
+  tileBoundsAt(T, p);
 }
 
 // ---------------------------------------------------------------------------
@@ -1001,7 +1003,8 @@ void sortSplit(CouplingState& C, SplitStore& P, const GArgs32& G, const SortArgs
   A.stateDst = P.stateT.get();
   A.betaSrc = P.beta.get();
   A.betaDst = P.betaT.get();
-  kPermute32<<<blocks, 256, 0, st>>>(A, P.permSorted.get(), n);
+  const TileBoundsArgs T{P.keySorted.get(), n, ntiles, C.tileBegin.get(), C.tileEnd.get(), occupied};
+  kPermute32<<<blocks, 256, 0, st>>>(A, P.permSorted.get(), n, T);
   launchCheck("permute");
   for (int a = 0; a < 3; ++a) P.cell[a].swap(P.cellT[a]);
   for (int a = 0; a < 9; ++a) P.s[a].swap(P.t[a]);
@@ -1009,7 +1012,7 @@ void sortSplit(CouplingState& C, SplitStore& P, const GArgs32& G, const SortArgs
   P.state.swap(P.stateT);
   P.beta.swap(P.betaT);
   const int rz = C.shape == kShapeQuintic ? Box<kShapeQuintic>::RZ : Box<kShapeBinomial3X>::RZ;
-  buildChunkLists(C, st, P.keySorted.get(), n, ntx, nty, ntiles, rz, occupied);
+  buildChunkLists(C, st, P.keySorted.get(), n, ntx, nty, ntiles, rz, occupied, nullptr, 0u, true);
 }
 
 }  // namespace
