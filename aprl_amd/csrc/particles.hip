@@ -411,7 +411,8 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks(PArgs P, GArgs G, St
   constexpr bool kStageB = SHAPE == kShapeBinomial3X;
   __shared__ double es[3 * boxNodes<SHAPE>()];
   __shared__ double bs[kStageB ? 3 * boxNodes<SHAPE>() : 4 * kChunk];
-  const int4 ch = chunks[blockIdx.x];
+  const int cb = xcdContiguous(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+  const int4 ch = chunks[cb];
   const TileBox<SHAPE> box = tileBox<SHAPE>(ch.x, ntx, nty);
   const bool bWaves = threadIdx.x >= kChunk;
   const int lane = static_cast<int>(threadIdx.x) - (bWaves ? kChunk : 0);
@@ -523,10 +524,10 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks(PArgs P, GArgs G, St
       __syncthreads();
     }
     if (threadIdx.x == 0) {
-      chunkOut[4 * blockIdx.x + 0] = sf[0];
-      chunkOut[4 * blockIdx.x + 1] = sd[0];
-      chunkOut[4 * blockIdx.x + 2] = ss[0];
-      chunkOut[4 * blockIdx.x + 3] = sc[0];
+      chunkOut[4 * cb + 0] = sf[0];
+      chunkOut[4 * cb + 1] = sd[0];
+      chunkOut[4 * cb + 2] = ss[0];
+      chunkOut[4 * cb + 3] = sc[0];
     }
   }
   if (err) atomicOr(errOut, err);

@@ -366,7 +366,8 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks32(PArgs32 P, GArgs32
   __shared__ float es[3 * boxNodes<SHAPE>()];
   __shared__ float bs[kStageB ? 3 * boxNodes<SHAPE>() : 1];
   __shared__ double red[MODE == 1 ? 4 * kChunk : 1];
-  const int4 ch = chunks[blockIdx.x];
+  const int cb = xcdContiguous(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+  const int4 ch = chunks[cb];
   const TileBox<SHAPE> box = tileBox<SHAPE>(ch.x, ntx, nty);
   stageTile32<SHAPE, kStageB>(G, box, es, bs);
   __syncthreads();
@@ -466,10 +467,10 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks32(PArgs32 P, GArgs32
       __syncthreads();
     }
     if (threadIdx.x == 0) {
-      chunkOut[4 * blockIdx.x + 0] = sf[0];
-      chunkOut[4 * blockIdx.x + 1] = sd[0];
-      chunkOut[4 * blockIdx.x + 2] = ss[0];
-      chunkOut[4 * blockIdx.x + 3] = sc[0];
+      chunkOut[4 * cb + 0] = sf[0];
+      chunkOut[4 * cb + 1] = sd[0];
+      chunkOut[4 * cb + 2] = ss[0];
+      chunkOut[4 * cb + 3] = sc[0];
     }
   }
   if (err) atomicOr(errOut, err);
