@@ -724,7 +724,7 @@ __global__ void __launch_bounds__(256) kReduceCurrent(double* jx, double* jy, do
                                                       const unsigned long long* tileScan, const int* tileShift,
                                                       long long* partial, int ntx, int nty, int ntz) {
   constexpr int RZ = Box<SHAPE>::RZ, NB = 9 * (2 * RZ + 1);
-  const int tile = active[blockIdx.x];
+  const int tile = active[xcdContiguous(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x))];
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
   __shared__ int nbFirst[NB], nbShift[NB];
   if (threadIdx.x < NB) {
