@@ -33,13 +33,13 @@ WORD = 8  # FP64
 # above the algorithmic bytes is re-reading (staged halos, per-chunk boxes).
 MEASURED_TRAFFIC_FP64 = {
     "magnetic": (2 * 89556 + 69040 + 2 * 10115 + 1792) * 1024,
-    "electric": (2 * 119920 + 70137) * 1024,
-    "electric_preview": (2 * 56589 + 27084) * 1024,
-    "gather": (2 * 38786 + 4868) * 1024,
-    "residual": (2 * 42991 + 4904) * 1024,
-    "deposit": (2 * 19937 + 26176) * 1024,
-    "push": (2 * 5845 + 5859) * 1024,
-    "reduce_current": (2 * 59655 + 57903) * 1024,
+    "electric": (2 * 119401 + 70101) * 1024,
+    "electric_preview": (2 * 56568 + 27084) * 1024,
+    "gather": (2 * 13734 + 4690) * 1024,
+    "residual": (2 * 17274 + 4729) * 1024,
+    "deposit": (2 * 19942 + 26176) * 1024,
+    "push": (2 * 5841 + 5859) * 1024,
+    "reduce_current": (2 * 24287 + 46902) * 1024,
 }
 
 
