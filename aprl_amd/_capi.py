@@ -55,6 +55,13 @@ class InstanceStep(C.Structure):
                 ("report", StepReport)]
 
 
+class GaussianBeam(C.Structure):
+    """aprl_gaussian_beam (generated-gaussian beam, particle_initializer.cpp:60-100)."""
+    _fields_ = [("center", C.c_double * 3), ("sigma_position", C.c_double * 3), ("mean_u", C.c_double * 3),
+                ("sigma_u", C.c_double * 3), ("electrons", C.c_double), ("seed", C.c_uint64), ("first", C.c_uint64),
+                ("total", C.c_uint64), ("boost_gamma", C.c_double)]
+
+
 class PendingGeometry(C.Structure):
     """aprl_pending_geometry (PendingWindowGeometry, pending_particle_queue.h:16-23)."""
     _fields_ = [("reference_origin_z", C.c_double), ("initial_offset", C.c_int64), ("dz", C.c_double),
@@ -128,6 +135,7 @@ def lib() -> C.CDLL:
     L.aprl_particles_destroy.argtypes = [vp]
     L.aprl_particles_destroy.restype = None
     L.aprl_particles_upload.argtypes = [vp, _sz, _dp, _dp, _dp, _dp, _dp, u64p]
+    L.aprl_particles_generate_gaussian.argtypes = [vp, C.POINTER(GaussianBeam), _sz]
     L.aprl_particles_download.argtypes = [vp, _dp, _dp, _dp, _dp, _dp, u64p]
     L.aprl_particles_count.argtypes = [vp]
     L.aprl_particles_count.restype = _sz
@@ -205,6 +213,7 @@ EXPORTED = [
     "aprl_coupling_set_reference_current",
     "aprl_profile_enable", "aprl_profile_query", "aprl_profile_reset",
     "aprl_particles_create", "aprl_particles_destroy", "aprl_particles_upload", "aprl_particles_download",
+    "aprl_particles_generate_gaussian",
     "aprl_particles_count", "aprl_particles_download_states", "aprl_coupling_boundary_counters",
     "aprl_particles_set_held", "aprl_coupling_set_activation", "aprl_coupling_set_hold_transverse",
     "aprl_repeat_held_addition", "aprl_pending_create", "aprl_pending_destroy", "aprl_pending_extract",

@@ -53,6 +53,20 @@ class ParticleStore:
         check(lib().aprl_particles_upload(self.handle, n, dptr(x.ravel()), dptr(uu.ravel()), dptr(q), dptr(m),
                                           dptr(w), i.ctypes.data_as(C.POINTER(C.c_uint64))))
 
+    def generateGaussian(self, count, center, sigma_position, mean_u, sigma_u, electrons, seed, first=0, total=None,
+                         boost_gamma=1.0):
+        """ParticleInitializer's generated-gaussian beam drawn on the device
+        (aprl_particles_generate_gaussian), replacing the store's contents."""
+        from ._capi import GaussianBeam
+        b = GaussianBeam()
+        for a in range(3):
+            b.center[a], b.sigma_position[a] = center[a], sigma_position[a]
+            b.mean_u[a], b.sigma_u[a] = mean_u[a], sigma_u[a]
+        b.electrons, b.seed, b.first = float(electrons), int(seed), int(first)
+        b.total = int(count if total is None else total)
+        b.boost_gamma = float(boost_gamma)
+        check(lib().aprl_particles_generate_gaussian(self.handle, C.byref(b), int(count)))
+
     def download(self):
         """(position (n,3), u (n,3), charge, mass, weight, ids) in device order."""
         n = len(self)

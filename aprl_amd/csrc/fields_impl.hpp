@@ -11,6 +11,7 @@
 #include <limits>
 
 #include "fields.hpp"
+#include "tilecounts.hpp"
 
 namespace aprl_gpu {
 
@@ -661,7 +662,7 @@ __global__ void __launch_bounds__(256) kElectric(FieldPtrsT<R> F, StepCoefsT<R> 
 }
 
 // The zero-current preview (MODE 1 of kElectric) on the nodes owned by the
-// listed cell tiles only: count = high word of *total, the tiles in list
+// listed cell tiles only: count = the active field of *total, the tiles in list
 // order, a workgroup per tile (looping when the grid is smaller). Every
 // node value is kElectric<1>'s.
 template <class R>
@@ -670,7 +671,7 @@ __global__ void __launch_bounds__(256) kElectricPreviewTiles(FieldPtrsT<R> F, St
                                                              const R* inletX, const R* inletY, const int* tiles,
                                                              const unsigned long long* total, TileGrid tg) {
   const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
-  const int count = static_cast<int>(*total >> 32);
+  const int count = activeField(*total);
   for (int b = blockIdx.x; b < count; b += gridDim.x) {
     const int tile = tiles[b];
     const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);

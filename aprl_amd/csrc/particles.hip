@@ -7,9 +7,6 @@
 // Per-particle arithmetic follows source_coupling_math.h / source_coupling.cpp
 // / longitudinal_shape.h in the reference's association order (contraction
 // off), so with identical field inputs a push is bitwise the reference's.
-#include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_scan.hpp>
-
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -52,6 +49,8 @@ struct PArgs {
   double* wend;      // carrier end weight
   const double* w;   // weight slot (a carrier's current weight)
   const double* beta;  // held speed of held entries
+  const unsigned* tileOf;         // sorted tile key of each entry
+  unsigned long long* tileBound;  // per tile: largest particle current bound
   double scaleDen;   // dx*dy*dz*dt
   int n;
 };
@@ -325,18 +324,10 @@ __global__ void kPermute(PermuteArgs A, const unsigned* perm, int n, TileBoundsA
   tileBoundsAt(T, p);
 }
 
-// Particle range of every occupied tile from the run boundaries of the
-// sorted keys (the ranges were cleared to empty before the sort), and the
-// occupied-tile count.
-__global__ void kTileBounds(TileBoundsArgs T) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < T.n) tileBoundsAt(T, p);
-}
-
-// Per tile: chunk count (low word) and whether any tile within one tile in
-// x and y and rz tiles in z is occupied (high word: the tile's J nodes can
-// then receive current); entry ntiles is 0 so that the exclusive scan ends
-// with both totals.
+// Per tile, packed (packTileCounts): chunk count, whether it is occupied,
+// and whether any tile within one tile in x and y and rz tiles in z is
+// occupied (its J nodes can then receive current); entry ntiles is 0 so
+// that the exclusive scan ends with the totals.
 __global__ void kTileCounts(const int* begin, const int* end, int ntx, int nty, int ntz, int rz,
                             unsigned long long* counts) {
   const int ntiles = ntx * nty * ntz;
@@ -357,11 +348,8 @@ __global__ void kTileCounts(const int* begin, const int* end, int ntx, int nty, 
         if (end[s] > begin[s]) touched = 1;
       }
   const unsigned chunks = static_cast<unsigned>((end[t] - begin[t] + kChunk - 1) / kChunk);
-  counts[t] = (static_cast<unsigned long long>(touched) << 32) | chunks;
+  counts[t] = packTileCounts(chunks, chunks ? 1u : 0u, touched);
 }
-
-__device__ __forceinline__ int lowWord(unsigned long long v) { return static_cast<int>(v & 0xffffffffull); }
-__device__ __forceinline__ int highWord(unsigned long long v) { return static_cast<int>(v >> 32); }
 
 // Chunk list in tile order and the list of active tiles (tiles with an
 // occupied neighbour), both from the scanned counts.
@@ -375,17 +363,17 @@ __global__ void kChunkList(const int* begin, const int* end, const unsigned long
   if (box && t == 0) {
     const unsigned long long total = scan[ntiles];
     volatile unsigned* v = box;
-    v[8] = static_cast<unsigned>(total & 0xffffffffull);
-    v[9] = static_cast<unsigned>(total >> 32);
+    v[8] = static_cast<unsigned>(chunkField(total));
+    v[9] = static_cast<unsigned>(activeField(total));
     for (int w = 0; w < 4; ++w) v[10 + w] = words[w];
     __threadfence_system();
     v[0] = seq;
   }
   if (t >= ntiles) return;
   const unsigned long long sc = scan[t];
-  if (counts[t] >> 32) active[highWord(sc)] = t;
+  if (activeField(counts[t])) active[activeField(sc)] = t;
   const int e = end[t];
-  int c = lowWord(sc);
+  int c = chunkField(sc);
   for (int p = begin[t]; p < e; p += kChunk, ++c) chunks[c] = make_int4(t, p, min(e, p + kChunk), 0);
 }
 
@@ -538,42 +526,55 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks(PArgs P, GArgs G, St
 // fields (the substeps with prescribed magnets), or the straight move of a
 // CPML carrier (position += dt u c / gamma, simulation.cpp:4407-4413), then
 // the segment's boundary events.
+// It also raises the entry's tile current bound: the largest |scale_a| of
+// its chord (scale_a = q (x1_a - x0_a) / (dx dy dz dt), eb_deposition.cpp:
+// 41-51), plus, for a held entry whose release kink splits the path, the
+// held leg's largest scale (both legs stay below the chord plus the leg).
 __global__ void __launch_bounds__(128) kPush(PArgs P, GArgs G, StepArgs S, unsigned* errOut) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P.n) return;
-  unsigned err = 0;
-  const double x0[3] = {P.x[0][p], P.x[1][p], P.x[2][p]};
-  const double u0[3] = {P.u[0][p], P.u[1][p], P.u[2][p]};
-  double x[3], u[3], absGrid, t1, t2, wDeposit, wEnd;
-  const double usedE[3] = {P.fin[0][p], P.fin[1][p], P.fin[2][p]};
-  const double usedB[3] = {P.fin[3][p], P.fin[4][p], P.fin[5][p]};
-  const int code = entryStep(S, G.o, P.state[p], x0, u0, P.q[p], P.m[p], P.w[p], P.beta[p], usedE, usedB, x, u,
-                             absGrid, t1, t2, wDeposit, wEnd, err);
-  for (int a = 0; a < 3; ++a) {
-    P.x1[a][p] = x[a];
-    P.u1[a][p] = u[a];
+  unsigned tile = ~0u;
+  double bound = 0.0;
+  if (p < P.n) {
+    unsigned err = 0;
+    const double x0[3] = {P.x[0][p], P.x[1][p], P.x[2][p]};
+    const double u0[3] = {P.u[0][p], P.u[1][p], P.u[2][p]};
+    double x[3], u[3], absGrid, t1, t2, wDeposit, wEnd;
+    const double usedE[3] = {P.fin[0][p], P.fin[1][p], P.fin[2][p]};
+    const double usedB[3] = {P.fin[3][p], P.fin[4][p], P.fin[5][p]};
+    const int state = P.state[p];
+    const double q = P.q[p], beta = P.beta[p];
+    const int code = entryStep(S, G.o, state, x0, u0, q, P.m[p], P.w[p], beta, usedE, usedB, x, u, absGrid, t1, t2,
+                               wDeposit, wEnd, err);
+    for (int a = 0; a < 3; ++a) {
+      P.x1[a][p] = x[a];
+      P.u1[a][p] = u[a];
+      bound = fmax(bound, fabs(q * (x[a] - x0[a]) / P.scaleDen));
+    }
+    if (state == kStateHeld) bound += fabs(q) * (kC * S.dt * (S.holdTransverse ? fabs(beta) : 1.0)) / P.scaleDen;
+    tile = P.tileOf[p];
+    P.gridAbs[p] = absGrid;
+    P.evt[p] = code;
+    if (code) {
+      P.cut[0][p] = t1;
+      P.cut[1][p] = t2;
+      P.cut[2][p] = wDeposit;
+      P.wend[p] = wEnd;
+      err |= kEvtAny;
+    }
+    if (err) atomicOr(errOut, err);
   }
-  P.gridAbs[p] = absGrid;
-  P.evt[p] = code;
-  if (code) {
-    P.cut[0][p] = t1;
-    P.cut[1][p] = t2;
-    P.cut[2][p] = wDeposit;
-    P.wend[p] = wEnd;
-    err |= kEvtAny;
-  }
-  if (err) atomicOr(errOut, err);
+  raiseTileBound(P.tileBound, tile, bound);
 }
 
 // Deterministic tile deposition of Current::deposit (longitudinal_shape.h:
 // 175-187): one workgroup per chunk, one lane per particle, every tap of
 // the chord x0 -> x1 accumulated into an LDS copy of the tile box in 64-bit
 // fixed point. The binary exponent comes from a bound on the whole tile's
-// current (the shape's weight bound times the sum over the tile's particles
-// of max |scale|), so every chunk of a tile uses the same scale. Integer
-// addition is associative, so the LDS atomics and the chunk sums are
-// reproducible in any order; each tap is rounded once, to 2^-62 of the
-// tile bound.
+// current (the shape's weight bound times the tile's particle count times
+// the largest per-particle bound the push recorded), so every chunk of a
+// tile uses the same scale. Integer addition is associative, so the LDS
+// atomics and the chunk sums are reproducible in any order; each tap is
+// rounded once, to 2^-62 of the tile bound.
 template <int SHAPE>
 __global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, StepArgs S, const int4* chunks,
                                                    const int* tileBegin, const int* tileEnd,
@@ -581,36 +582,14 @@ __global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, StepArgs S,
                                                    long long* partial, int* tileShift, unsigned* errOut) {
   constexpr int N = boxNodes<SHAPE>(), NX = Box<SHAPE>::NX, NY = Box<SHAPE>::NY;
   __shared__ unsigned long long acc[3 * N];
-  __shared__ double bound[kChunk];
   const int4 ch = chunks[blockIdx.x];
   const int tile = ch.x;
   const TileBox<SHAPE> box = tileBox<SHAPE>(tile, ntx, nty);
-  const int tb0 = tileBegin[tile], tb1 = tileEnd[tile];
   for (int t = threadIdx.x; t < 3 * N; t += blockDim.x) acc[t] = 0ull;
-  // scale_a = q * (x1_a - x0_a) / (dx dy dz dt), eb_deposition.cpp:41-51
-  auto scaleOf = [&](int p, double s[3]) {
-    const double q = P.q[p];
-    for (int d = 0; d < 3; ++d) s[d] = q * (P.x1[d][p] - P.x[d][p]) / P.scaleDen;
-  };
-  double bsum = 0.0;
-  for (int p = tb0 + static_cast<int>(threadIdx.x); p < tb1; p += blockDim.x) {
-    double s[3];
-    scaleOf(p, s);
-    bsum += fmax(fabs(s[0]), fmax(fabs(s[1]), fabs(s[2])));
-    // a held entry's release kink splits its path: both legs stay below
-    // the chord plus the held leg
-    if (P.state[p] == kStateHeld) bsum += fabs(P.q[p]) * (kC * S.dt * (S.holdTransverse ? fabs(P.beta[p]) : 1.0)) / P.scaleDen;
-  }
-  bound[threadIdx.x] = bsum;
+  const unsigned long long tsc = tileScan[tile];
+  const int shift = tileFixedShift(Box<SHAPE>::kWeightBound, P.tileBound[tile], tileEnd[tile] - tileBegin[tile]);
+  if (threadIdx.x == 0 && static_cast<int>(blockIdx.x) == chunkField(tsc)) tileShift[tile] = shift;
   __syncthreads();
-  for (int w = kChunk / 2; w > 0; w >>= 1) {
-    if (static_cast<int>(threadIdx.x) < w) bound[threadIdx.x] += bound[threadIdx.x + w];
-    __syncthreads();
-  }
-  int e2 = 0;
-  (void)frexp(Box<SHAPE>::kWeightBound * bound[0], &e2);
-  const int shift = 62 - e2;  // bound * 2^shift < 2^62
-  if (threadIdx.x == 0 && static_cast<int>(blockIdx.x) == lowWord(tileScan[tile])) tileShift[tile] = shift;
   unsigned err = 0;
   const int p = ch.y + static_cast<int>(threadIdx.x);
   if (p < ch.z) {
@@ -696,13 +675,13 @@ __global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, StepArgs S,
     }
   }
   __syncthreads();
-  // The tile's partial sits at its first chunk's slot and holds zeros on
+  // The tile's partial sits at its occupied-tile slot and holds zeros on
   // entry (the reduction clears what it reads). A tile of one chunk stores
   // its box; chunks of a larger tile add their nonzero nodes with integer
   // atomics, whose sum is independent of their order.
-  const int first = lowWord(tileScan[tile]);
-  const bool alone = lowWord(tileScan[tile + 1]) - first == 1;
-  unsigned long long* out = reinterpret_cast<unsigned long long*>(partial + static_cast<size_t>(first) * 3 * N);
+  const bool alone = chunkField(tileScan[tile + 1]) - chunkField(tsc) == 1;
+  unsigned long long* out =
+      reinterpret_cast<unsigned long long*>(partial + static_cast<size_t>(occupiedField(tsc)) * 3 * N);
   for (int t = threadIdx.x; t < 3 * N; t += blockDim.x) {
     const unsigned long long val = acc[t];
     if (alone) out[t] = val;
@@ -722,20 +701,23 @@ __global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, StepArgs S,
 template <int SHAPE>
 __global__ void __launch_bounds__(256) kReduceCurrent(double* jx, double* jy, double* jz, GArgs G, const int* active,
                                                       const unsigned long long* tileScan, const int* tileShift,
-                                                      long long* partial, int ntx, int nty, int ntz) {
+                                                      long long* partial, unsigned long long* tileBound, int ntx,
+                                                      int nty, int ntz) {
   constexpr int RZ = Box<SHAPE>::RZ, NB = 9 * (2 * RZ + 1);
   const int tile = active[xcdContiguous(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x))];
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
   __shared__ int nbFirst[NB], nbShift[NB];
+  // the next trial's push raises the bound from zero
+  if (threadIdx.x == 0) tileBound[tile] = 0ull;
   if (threadIdx.x < NB) {
     const int nb = static_cast<int>(threadIdx.x);
     const int x = tx + nb % 3 - 1, y = ty + (nb / 3) % 3 - 1, z = tz + nb / 9 - RZ;
     int first = -1, sh = 0;
     if (x >= 0 && x < ntx && y >= 0 && y < nty && z >= 0 && z < ntz) {
       const int s = (z * nty + y) * ntx + x;
-      const int f = lowWord(tileScan[s]);
-      if (lowWord(tileScan[s + 1]) > f) {
-        first = f;
+      const unsigned long long sc = tileScan[s];
+      if (chunkField(tileScan[s + 1]) > chunkField(sc)) {
+        first = occupiedField(sc);
         sh = tileShift[s];
       }
     }
@@ -1022,9 +1004,9 @@ static void sortAndChunk(CouplingState& C, ParticleStore& P, const GArgs& G, con
   const int n = static_cast<int>(P.count);
   ScopedKernel sk(F.profiler, st, "sort");
   const unsigned blocks = ceilDiv(n, 256);
-  kTileKeys<<<blocks, 256, 0, st>>>(P.s[0].get(), P.s[1].get(), P.s[2].get(), P.state.get(), n, G, ntx, nty,
-                                    P.key.get(), P.perm.get(), TileRangeArgs{C.tileBegin.get(), C.tileEnd.get(), ntiles},
-                                    K);
+  kTileKeys<<<blocks, 256, 0, st>>>(
+      P.s[0].get(), P.s[1].get(), P.s[2].get(), P.state.get(), n, G, ntx, nty, P.key.get(), P.perm.get(),
+      TileRangeArgs{C.tileBegin.get(), C.tileEnd.get(), ntiles, C.tileBound.get()}, K);
   launchCheck("tile keys");
   const int bits = bitsFor(static_cast<unsigned>(ntiles));  // keys 0..ntiles (ntiles: behind the live entries)
   std::size_t bytes = 0;
@@ -1054,26 +1036,20 @@ static void sortAndChunk(CouplingState& C, ParticleStore& P, const GArgs& G, con
   P.state.swap(P.stateT);
   P.beta.swap(P.betaT);
   const int rz = C.shape == kShapeQuintic ? Box<kShapeQuintic>::RZ : Box<kShapeBinomial3X>::RZ;
-  buildChunkLists(C, st, P.keySorted.get(), n, ntx, nty, ntiles, rz, occupied, C.mailbox.device(), seq, true);
+  buildChunkLists(C, st, ntx, nty, ntiles, rz, C.mailbox.device(), seq);
 }
 
-// Tile ranges of the sorted keys, chunk counts and the active flags, their
-// scan, the chunk list and the active-tile list (all in tile order).
-void buildChunkLists(CouplingState& C, hipStream_t st, const unsigned* keySorted, int n, int ntx, int nty,
-                     int ntiles, int rz, unsigned* occupied, unsigned* box, unsigned seq, bool boundsDone) {
-  if (!boundsDone) {
-    kTileBounds<<<ceilDiv(n, 256), 256, 0, st>>>(
-        TileBoundsArgs{keySorted, n, ntiles, C.tileBegin.get(), C.tileEnd.get(), occupied});
-    launchCheck("tile bounds");
-  }
+// From the tile ranges of the sorted keys (written by the permute): packed
+// chunk, occupied and active counts, their scan, the chunk list and the
+// active-tile list (all in tile order).
+void buildChunkLists(CouplingState& C, hipStream_t st, int ntx, int nty, int ntiles, int rz, unsigned* box,
+                     unsigned seq) {
   const int ntz = ntiles / (ntx * nty);
   kTileCounts<<<ceilDiv(ntiles + 1, 256), 256, 0, st>>>(C.tileBegin.get(), C.tileEnd.get(), ntx, nty, ntz, rz,
                                                          C.tileCounts.get());
   launchCheck("tile counts");
-  std::size_t bytes = C.scanTempBytes;
-  hipCheck(rocprim::exclusive_scan(C.scanTemp.get(), bytes, C.tileCounts.get(), C.tileScan.get(), 0ull,
-                                   static_cast<std::size_t>(ntiles) + 1, rocprim::plus<unsigned long long>(), st),
-           "exclusive_scan");
+  exclusiveScan64(C.tileCounts.get(), C.tileScan.get(), static_cast<std::size_t>(ntiles) + 1, C.scanTemp.get(),
+                  C.scanTempBytes, st);
   kChunkList<<<ceilDiv(ntiles, 256), 256, 0, st>>>(C.tileBegin.get(), C.tileEnd.get(), C.tileCounts.get(),
                                                     C.tileScan.get(), ntiles, C.chunks.get(), C.activeTiles.get(),
                                                     box, seq, C.error.get());
@@ -1082,22 +1058,25 @@ void buildChunkLists(CouplingState& C, hipStream_t st, const unsigned* keySorted
 
 // Tile arrays sized for ntiles and a chunk list for n particles.
 void reserveTileLists(CouplingState& C, hipStream_t st, int ntiles, int n) {
+  if (static_cast<unsigned long long>(ntiles) > kMaxTiles)
+    throw std::invalid_argument("The grid has more than " + std::to_string(kMaxTiles) + " cell tiles of 8x8x4");
   if (C.tileBegin.size() != static_cast<std::size_t>(ntiles)) {
     C.tileBegin.resize(ntiles);
     C.tileEnd.resize(ntiles);
     C.tileShift.resize(ntiles);
     C.activeTiles.resize(ntiles);
+    C.tileBound.resize(ntiles);
+    hipCheck(hipMemsetAsync(C.tileBound.get(), 0, static_cast<std::size_t>(ntiles) * sizeof(unsigned long long), st),
+             "hipMemsetAsync");
     C.tileCounts.resize(ntiles + 1);
     C.tileScan.resize(ntiles + 1);
-    C.scanTempBytes = 0;
-    hipCheck(rocprim::exclusive_scan(nullptr, C.scanTempBytes, C.tileCounts.get(), C.tileScan.get(), 0ull,
-                                     static_cast<std::size_t>(ntiles) + 1, rocprim::plus<unsigned long long>(), st),
-             "exclusive_scan size");
-    C.scanTempBytes = std::max<std::size_t>(C.scanTempBytes, 256);
+    C.scanTempBytes = std::max<std::size_t>(scanTempBytes64(static_cast<std::size_t>(ntiles) + 1), 256);
     C.scanTemp.resize(C.scanTempBytes);
     C.chunkCapacity = 0;
   }
   const std::size_t chunkBound = chunkListBound(ntiles, n);
+  if (chunkBound > kMaxChunks)
+    throw std::invalid_argument("More than " + std::to_string(kMaxChunks) + " particle chunks in one step");
   if (chunkBound > C.chunkCapacity) {
     C.chunks.resize(std::max<std::size_t>(chunkBound, 1));
     C.chunkCapacity = chunkBound;
@@ -1272,7 +1251,7 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
   }
   const bool quintic = C.shape == kShapeQuintic;
   const int boxN = quintic ? boxNodes<kShapeQuintic>() : boxNodes<kShapeBinomial3X>();
-  const std::size_t partialNeed = static_cast<std::size_t>(nChunks) * 3 * boxN;
+  const std::size_t partialNeed = static_cast<std::size_t>(occupiedTiles) * 3 * boxN;
   if (partialNeed > C.partial.size()) {
     // tile partials must start at zero; the reduction keeps them so
     C.partial.resize(partialNeed);
@@ -1284,6 +1263,8 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
   const StepArgs S = stepArgsFor(C, F.g, F.hasCpml ? &F.cpml : nullptr, F.window, origin, time, substeps);
   PArgs A = particleArgs(P);
   A.scaleDen = F.g.dx * F.g.dy * F.g.dz * F.g.dt;
+  A.tileOf = P.keySorted.get();
+  A.tileBound = C.tileBound.get();
   bool converged = false;
   unsigned iterations = 0;
   double force = 0., work = 0.;
@@ -1325,13 +1306,13 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
       {
         ScopedKernel sk(F.profiler, st, "reduce_current");
         if (quintic)
-          kReduceCurrent<kShapeQuintic><<<nActive, 256, 0, st>>>(F.base(6), F.base(7), F.base(8), G,
-                                                                 C.activeTiles.get(), C.tileScan.get(),
-                                                                 C.tileShift.get(), C.partial.get(), ntx, nty, ntz);
+          kReduceCurrent<kShapeQuintic><<<nActive, 256, 0, st>>>(
+              F.base(6), F.base(7), F.base(8), G, C.activeTiles.get(), C.tileScan.get(), C.tileShift.get(),
+              C.partial.get(), C.tileBound.get(), ntx, nty, ntz);
         else
-          kReduceCurrent<kShapeBinomial3X><<<nActive, 256, 0, st>>>(F.base(6), F.base(7), F.base(8), G,
-                                                                    C.activeTiles.get(), C.tileScan.get(),
-                                                                    C.tileShift.get(), C.partial.get(), ntx, nty, ntz);
+          kReduceCurrent<kShapeBinomial3X><<<nActive, 256, 0, st>>>(
+              F.base(6), F.base(7), F.base(8), G, C.activeTiles.get(), C.tileScan.get(), C.tileShift.get(),
+              C.partial.get(), C.tileBound.get(), ntx, nty, ntz);
         launchCheck("reduce current");
       }
       F.currentZero = false;

@@ -4,6 +4,7 @@
 #include <memory>
 
 #include "fields.hpp"
+#include "tilecounts.hpp"
 
 namespace aprl_gpu {
 
@@ -161,17 +162,24 @@ struct CouplingState {
   // step) and whether a last residual gather reports force and work
   unsigned splitTrials = 2;
   bool splitVerify = false;
-  // tiles: particle range, chunk count (+1 entry for the scan total), first
-  // chunk, fixed-point exponent of the tile's current
+  // tiles: particle range, fixed-point exponent of the tile's current,
+  // active-tile list
   DeviceArray<int> tileBegin, tileEnd, tileShift, activeTiles;
-  // per tile: chunk count (low 32 bits) and "has an occupied neighbour"
-  // (high 32 bits); the exclusive scan gives first chunk and active slot
+  // per tile: the largest current bound of one of its particles in the
+  // trial (bit pattern of a non-negative double; written by the push,
+  // read by the deposit, cleared by the current reduction and the sort)
+  DeviceArray<unsigned long long> tileBound;
+  // per tile (+1 entry for the scan totals): packed counts (chunks,
+  // occupied, has an occupied neighbour; packTileCounts); the exclusive
+  // scan gives the first chunk, the partial slot and the active slot
   DeviceArray<unsigned long long> tileCounts, tileScan;
   DeviceArray<int4> chunks;  // (tile, first particle, end particle, 0)
   std::size_t chunkCapacity = 0;
   DeviceArray<unsigned char> scanTemp;
   std::size_t scanTempBytes = 0;
-  DeviceArray<long long> partial;  // per chunk: 3 extended tile boxes of fixed-point current
+  // per occupied tile (in tile order): 3 extended tile boxes of
+  // fixed-point current
+  DeviceArray<long long> partial;
   DeviceArray<double> reduceBlocks, reduceOut;
   DeviceArray<unsigned> error;
   // cumulative boundary counters and their per-block partials
@@ -200,6 +208,40 @@ struct CouplingState {
   DeviceArray<double> rho[2];  // diagnostic charge densities
 };
 
+// Raise a tile's current bound to b (b >= 0): the lanes of a wave that
+// hold the same tile (sorted particles: usually the whole wave) combine
+// first, so that one atomic per tile and wave reaches memory. Every lane of
+// the wave must call it; lanes without a particle pass tile = ~0u.
+__device__ __forceinline__ void raiseTileBound(unsigned long long* tileBound, unsigned tile, double b) {
+  const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(b));
+  bool pending = tile != ~0u;
+  const int lane = static_cast<int>(threadIdx.x & 63u);
+  while (true) {
+    const unsigned long long mask = __ballot(pending);
+    if (!mask) break;
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    const unsigned lt = __shfl(tile, leader, 64);
+    unsigned long long v = pending && tile == lt ? bits : 0ull;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(v, off, 64);
+      v = o > v ? o : v;
+    }
+    if (lane == leader) atomicMax(tileBound + lt, v);
+    if (tile == lt) pending = false;
+  }
+}
+
+// Fixed-point exponent of a tile: every tap of the tile's particles is at
+// most weightBound x (largest per-particle bound), so the tile's integer
+// sums stay below 2^62 with count particles.
+__device__ __forceinline__ int tileFixedShift(double weightBound, unsigned long long boundBits, int count) {
+  const double b = __longlong_as_double(static_cast<long long>(boundBits));
+  int e2 = 0;
+  (void)frexp(weightBound * b * static_cast<double>(count), &e2);
+  return 62 - e2;
+}
+
 // Run boundaries of the sorted tile keys, folded into the permute kernels:
 // entry p opens its tile's range when it differs from its predecessor and
 // closes it when it differs from its successor.
@@ -227,20 +269,21 @@ struct TileRangeArgs {
   int* begin;
   int* end;
   int ntiles;
+  unsigned long long* bound;
 };
 __device__ __forceinline__ void clearTileRanges(const TileRangeArgs& R, int gid) {
   for (int t = gid; t < R.ntiles; t += static_cast<int>(gridDim.x * blockDim.x)) {
     R.begin[t] = 0;
     R.end[t] = 0;
+    R.bound[t] = 0ull;
   }
 }
 
 // Host helpers shared by the FP64 and FP32-split coupling steps.
 // box / seq: the mailbox (device address) that receives the scan total and
 // the sort-phase words, or nullptr.
-void buildChunkLists(CouplingState& C, hipStream_t st, const unsigned* keySorted, int n, int ntx, int nty,
-                     int ntiles, int rz, unsigned* occupied, unsigned* box = nullptr, unsigned seq = 0,
-                     bool boundsDone = false);
+void buildChunkLists(CouplingState& C, hipStream_t st, int ntx, int nty, int ntiles, int rz, unsigned* box = nullptr,
+                     unsigned seq = 0);
 void reserveTileLists(CouplingState& C, hipStream_t st, int ntiles, int n);
 std::size_t chunkListBound(int ntiles, int n);
 void fillInts(hipStream_t st, int* p, int n, int v);
@@ -276,6 +319,14 @@ std::size_t liveEntries(std::size_t count, const DeviceArray<unsigned>& removed,
 // bits; temp == nullptr returns the scratch size in bytes.
 void tileSort(void* temp, std::size_t& bytes, unsigned* key, unsigned* keySorted, unsigned* perm,
               unsigned* permSorted, unsigned n, int bits, hipStream_t st);
+// Device-wide exclusive scans (out may equal in); the scratch sizes are in
+// bytes (0: no scratch needed).
+std::size_t scanTempBytes32(std::size_t n);
+std::size_t scanTempBytes64(std::size_t n);
+void exclusiveScan32(const unsigned* in, unsigned* out, std::size_t n, void* temp, std::size_t bytes,
+                     hipStream_t st);
+void exclusiveScan64(const unsigned long long* in, unsigned long long* out, std::size_t n, void* temp,
+                     std::size_t bytes, hipStream_t st);
 
 // FP32-split form (particles32.hip).
 void reserveSplitParticles(SplitStore& P, std::size_t capacity);

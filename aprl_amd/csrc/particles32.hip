@@ -15,7 +15,6 @@
 //    reference requires with active particles, simulation_source_coupling.cpp:
 //    50): its convergence tolerances (force < 1e-13) lie below FP32
 //    resolution, and a fixed count keeps the host out of the step.
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -61,6 +60,8 @@ struct PArgs32 {
   float* wend;       // carrier end weight
   const float* w;    // weight slot (a carrier's current weight)
   const float* beta; // held speed of held entries
+  const unsigned* tileOf;         // sorted tile key of each entry
+  unsigned long long* tileBound;  // per tile: largest particle current bound
   float kq[3];  // h_a / (dx dy dz dt): current density per unit charge and cell of motion
   double h[3];
   int n;
@@ -476,15 +477,12 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks32(PArgs32 P, GArgs32
   if (err) atomicOr(errOut, err);
 }
 
-// One trial's push: FP32 state and fields widened to FP64, the FP64 orbit
-// push on absolute box coordinates (origin o = the grid origin cell times
-// h) or a carrier's straight move (position += dt u c / gamma), the
-// segment's boundary events, and the end point split again into cell +
-// FP32 offset.
-__global__ void __launch_bounds__(128) kPushSplit(PArgs32 P, StepArgs S, double o0, double o1, double o2,
-                                                  unsigned* errOut) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P.n) return;
+// One entry of kPushSplit; returns the entry's current bound: the largest
+// |scale_a| of its FP32 chord as the deposit forms it (q kq_a v_a), plus,
+// for a held entry whose release kink splits the path, the held leg's
+// largest scale.
+__device__ __forceinline__ double pushSplitEntry(const PArgs32& P, const StepArgs& S, double o0, double o1, double o2,
+                                                 int p, unsigned* errOut) {
   unsigned err = 0;
   const double o[3] = {o0, o1, o2};
   int c0[3];
@@ -528,6 +526,36 @@ __global__ void __launch_bounds__(128) kPushSplit(PArgs32 P, StepArgs S, double 
     err |= kEvtAny;
   }
   if (err) atomicOr(errOut, err);
+  const float q = P.q[p];
+  double bound = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const float v = static_cast<float>((x[a] - x0[a]) / P.h[a]);
+    bound = fmax(bound, fabs(static_cast<double>(q * P.kq[a] * v)));
+  }
+  if (P.state[p] == kStateHeld) {
+    double leg = static_cast<double>(P.kq[2]) * (kC * S.dt * fabs(static_cast<double>(P.beta[p])) / P.h[2]);
+    if (!S.holdTransverse)  // ballistic: at most c dt along any axis
+      for (int a = 0; a < 3; ++a) leg = fmax(leg, static_cast<double>(P.kq[a]) * (kC * S.dt / P.h[a]));
+    bound += fabs(static_cast<double>(q)) * leg;
+  }
+  return bound;
+}
+
+// One trial's push: FP32 state and fields widened to FP64, the FP64 orbit
+// push on absolute box coordinates (origin o = the grid origin cell times
+// h) or a carrier's straight move (position += dt u c / gamma), the
+// segment's boundary events, and the end point split again into cell +
+// FP32 offset; then the entries' tile current bounds.
+__global__ void __launch_bounds__(128) kPushSplit(PArgs32 P, StepArgs S, double o0, double o1, double o2,
+                                                  unsigned* errOut) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned tile = ~0u;
+  double bound = 0.0;
+  if (p < P.n) {
+    tile = P.tileOf[p];
+    bound = pushSplitEntry(P, S, o0, o1, o2, p, errOut);
+  }
+  raiseTileBound(P.tileBound, tile, bound);
 }
 
 // Deterministic tile deposition (Current::deposit, longitudinal_shape.h:
@@ -542,44 +570,19 @@ __global__ void __launch_bounds__(kChunk) kDepositSplit(PArgs32 P, GArgs32 G, St
                                                         unsigned* errOut) {
   constexpr int N = boxNodes<SHAPE>(), NX = Box<SHAPE>::NX, NY = Box<SHAPE>::NY;
   __shared__ unsigned long long acc[3 * N];
-  __shared__ double bound[kChunk];
   const int4 ch = chunks[blockIdx.x];
   const int tile = ch.x;
   const TileBox<SHAPE> box = tileBox<SHAPE>(tile, ntx, nty);
-  const int tb0 = tileBegin[tile], tb1 = tileEnd[tile];
   for (int t = threadIdx.x; t < 3 * N; t += blockDim.x) acc[t] = 0ull;
   // scale_a = q * dx_a / (dx dy dz dt), eb_deposition.cpp:41-51
   auto scaleOf = [&](int p, const float v[3], float s[3]) {
     const float q = P.q[p];
     for (int d = 0; d < 3; ++d) s[d] = q * P.kq[d] * v[d];
   };
-  double bsum = 0.0;
-  for (int p = tb0 + static_cast<int>(threadIdx.x); p < tb1; p += blockDim.x) {
-    int c0[3];
-    float off0[3], v[3], s[3];
-    chordAt(P, p, false, G, c0, off0, v);
-    scaleOf(p, v, s);
-    bsum += fmax(fabs(static_cast<double>(s[0])), fmax(fabs(static_cast<double>(s[1])), fabs(static_cast<double>(s[2]))));
-    // a held entry's release kink splits its path: both legs stay below
-    // the chord plus the held leg
-    if (P.state[p] == kStateHeld) {
-      double leg = static_cast<double>(P.kq[2]) * (kC * S.dt * fabs(static_cast<double>(P.beta[p])) / P.h[2]);
-      if (!S.holdTransverse)  // ballistic: at most c dt along any axis
-        for (int a = 0; a < 3; ++a) leg = fmax(leg, static_cast<double>(P.kq[a]) * (kC * S.dt / P.h[a]));
-      bsum += fabs(static_cast<double>(P.q[p])) * leg;
-    }
-  }
-  bound[threadIdx.x] = bsum;
+  const unsigned long long tsc = tileScan[tile];
+  const int shift = tileFixedShift(Box<SHAPE>::kWeightBound, P.tileBound[tile], tileEnd[tile] - tileBegin[tile]);
+  if (threadIdx.x == 0 && static_cast<int>(blockIdx.x) == chunkField(tsc)) tileShift[tile] = shift;
   __syncthreads();
-  for (int w = kChunk / 2; w > 0; w >>= 1) {
-    if (static_cast<int>(threadIdx.x) < w) bound[threadIdx.x] += bound[threadIdx.x + w];
-    __syncthreads();
-  }
-  int e2 = 0;
-  (void)frexp(Box<SHAPE>::kWeightBound * bound[0], &e2);
-  const int shift = 62 - e2;  // bound * 2^shift < 2^62
-  if (threadIdx.x == 0 && static_cast<int>(blockIdx.x) == static_cast<int>(tileScan[tile] & 0xffffffffull))
-    tileShift[tile] = shift;
   unsigned err = 0;
   const int p = ch.y + static_cast<int>(threadIdx.x);
   if (p < ch.z) {
@@ -669,9 +672,9 @@ __global__ void __launch_bounds__(kChunk) kDepositSplit(PArgs32 P, GArgs32 G, St
     }
   }
   __syncthreads();
-  const int first = static_cast<int>(tileScan[tile] & 0xffffffffull);
-  const bool alone = static_cast<int>(tileScan[tile + 1] & 0xffffffffull) - first == 1;
-  unsigned long long* out = reinterpret_cast<unsigned long long*>(partial + static_cast<size_t>(first) * 3 * N);
+  const bool alone = chunkField(tileScan[tile + 1]) - chunkField(tsc) == 1;
+  unsigned long long* out =
+      reinterpret_cast<unsigned long long*>(partial + static_cast<size_t>(occupiedField(tsc)) * 3 * N);
   for (int t = threadIdx.x; t < 3 * N; t += blockDim.x) {
     const unsigned long long val = acc[t];
     if (alone) out[t] = val;
@@ -687,21 +690,23 @@ __global__ void __launch_bounds__(kChunk) kDepositSplit(PArgs32 P, GArgs32 G, St
 template <int SHAPE, bool WRITE_ESUM>
 __global__ void __launch_bounds__(256) kReduceCurrentSplit(float* jx, float* jy, float* jz, GArgs32 G,
                                                            const int* active, const unsigned long long* tileScan,
-                                                           const int* tileShift, long long* partial, int ntx,
-                                                           int nty, int ntz) {
+                                                           const int* tileShift, long long* partial,
+                                                           unsigned long long* tileBound, int ntx, int nty, int ntz) {
   constexpr int RZ = Box<SHAPE>::RZ, NB = 9 * (2 * RZ + 1);
   const int tile = active[xcdContiguous(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x))];
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
   __shared__ int nbFirst[NB], nbShift[NB];
+  // the next trial's push raises the bound from zero
+  if (threadIdx.x == 0) tileBound[tile] = 0ull;
   if (threadIdx.x < NB) {
     const int nb = static_cast<int>(threadIdx.x);
     const int x = tx + nb % 3 - 1, y = ty + (nb / 3) % 3 - 1, z = tz + nb / 9 - RZ;
     int first = -1, sh = 0;
     if (x >= 0 && x < ntx && y >= 0 && y < nty && z >= 0 && z < ntz) {
       const int s = (z * nty + y) * ntx + x;
-      const int f = static_cast<int>(tileScan[s] & 0xffffffffull);
-      if (static_cast<int>(tileScan[s + 1] & 0xffffffffull) > f) {
-        first = f;
+      const unsigned long long sc = tileScan[s];
+      if (chunkField(tileScan[s + 1]) > chunkField(sc)) {
+        first = occupiedField(sc);
         sh = tileShift[s];
       }
     }
@@ -978,7 +983,8 @@ void sortSplit(CouplingState& C, SplitStore& P, const GArgs32& G, const SortArgs
   const unsigned blocks = ceilDiv(n, 256);
   kTileKeys32<<<blocks, 256, 0, st>>>(P.cell[0].get(), P.cell[1].get(), P.cell[2].get(), P.s[2].get(),
                                       P.state.get(), n, G, ntx, nty, P.key.get(), P.perm.get(),
-                                      TileRangeArgs{C.tileBegin.get(), C.tileEnd.get(), ntiles}, K, F.g.dz);
+                                      TileRangeArgs{C.tileBegin.get(), C.tileEnd.get(), ntiles, C.tileBound.get()}, K,
+                                      F.g.dz);
   launchCheck("tile keys");
   const int bits = bitsFor32(static_cast<unsigned>(ntiles));  // keys 0..ntiles (ntiles: behind the live entries)
   std::size_t bytes = 0;
@@ -1013,7 +1019,7 @@ void sortSplit(CouplingState& C, SplitStore& P, const GArgs32& G, const SortArgs
   P.state.swap(P.stateT);
   P.beta.swap(P.betaT);
   const int rz = C.shape == kShapeQuintic ? Box<kShapeQuintic>::RZ : Box<kShapeBinomial3X>::RZ;
-  buildChunkLists(C, st, P.keySorted.get(), n, ntx, nty, ntiles, rz, occupied, nullptr, 0u, true);
+  buildChunkLists(C, st, ntx, nty, ntiles, rz);
 }
 
 }  // namespace
@@ -1126,8 +1132,10 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
   unsigned nChunks = 0, nActive = 0, occupiedTiles = 0;
   if (n) {
     hipCheck(hipEventSynchronize(C.sorted.get()), "hipEventSynchronize");
-    nChunks = pin[0];
-    nActive = pin[1];
+    unsigned long long total;
+    std::memcpy(&total, pin, sizeof(total));
+    nChunks = static_cast<unsigned>(chunkField(total));
+    nActive = static_cast<unsigned>(activeField(total));
     occupiedTiles = pin[121];
     // finishMovingWindowStep's checks of the entries, applied to this origin
     raiseParticleErrors(pin[120]);
@@ -1140,7 +1148,7 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
   }
   const bool quintic = C.shape == kShapeQuintic;
   const int boxN = quintic ? boxNodes<kShapeQuintic>() : boxNodes<kShapeBinomial3X>();
-  const std::size_t partialNeed = static_cast<std::size_t>(nChunks) * 3 * boxN;
+  const std::size_t partialNeed = static_cast<std::size_t>(occupiedTiles) * 3 * boxN;
   if (partialNeed > C.partial.size()) {
     C.partial.resize(partialNeed);
     hipCheck(hipMemsetAsync(C.partial.get(), 0, partialNeed * sizeof(long long), st), "hipMemsetAsync");
@@ -1150,6 +1158,8 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
 
   const StepArgs& S = X.S;
   PArgs32 A = particleArgs32(P, F);
+  A.tileOf = P.keySorted.get();
+  A.tileBound = C.tileBound.get();
   const bool verify = C.splitVerify;
   for (unsigned trial = 0; trial < trials; ++trial) {
     unsigned* errT = C.error.get() + kTrialWord + trial;
@@ -1192,7 +1202,8 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
       float* j[3] = {F.base(6), F.base(7), F.base(8)};
 #define APRL_REDUCE32(SH, W)                                                                                      \
   kReduceCurrentSplit<SH, W><<<nActive, 256, 0, st>>>(j[0], j[1], j[2], G, C.activeTiles.get(), C.tileScan.get(), \
-                                                       C.tileShift.get(), C.partial.get(), ntx, nty, ntz)
+                                                       C.tileShift.get(), C.partial.get(), C.tileBound.get(), ntx,  \
+                                                       nty, ntz)
       if (quintic && esum) APRL_REDUCE32(kShapeQuintic, true);
       else if (quintic) APRL_REDUCE32(kShapeQuintic, false);
       else if (esum) APRL_REDUCE32(kShapeBinomial3X, true);

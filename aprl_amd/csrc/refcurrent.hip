@@ -29,7 +29,7 @@ __global__ void kReferenceCurrent(R* jz, R* esum, const R* estar, const R* eold,
     const int i = pts[2 * p], j = pts[2 * p + 1];
     const size_t idx = (static_cast<size_t>(k) * (ny + 1) + j) * (nx + 1) + i;
     const int tx = min(i / kTileX, ntx - 1), ty = min(j / kTileY, nty - 1), tz = min(k / kTileZ, ntz - 1);
-    const bool active = anyActive && (tileCounts[(static_cast<size_t>(tz) * nty + ty) * ntx + tx] >> 32) != 0ull;
+    const bool active = anyActive && activeField(tileCounts[(static_cast<size_t>(tz) * nty + ty) * ntx + tx]) != 0;
     const double current = (active ? static_cast<double>(jz[idx]) : 0.0) + value;
     jz[idx] = static_cast<R>(current);
     if (active && writeEsum) {

@@ -9,6 +9,12 @@ linear-binomial3 (+x) source shape. The bunch is an active box-frame
 snapshot (its lab sample is boosted with the light-front transform and its
 length stretched to the box frame), the inlet reference is the zero field,
 and no particle reaches a boundary during the timed steps.
+
+The other configurations: ``DRIFT_C1`` (the CPU-runnable generated-Gaussian
+drift, 32 x 32 x 256), ``fel_segment`` (the planar-undulator FEL segment,
+128 x 128 x 1024, 4e6 quiet-start macro-particles of weight 355), the
+batched scan ``batch_workloads`` (48 instances) and ``CAPACITY_C5`` (6.4e7
+macro-particles in 256 x 256 x 2048 with a 16-cell CPML).
 """
 from __future__ import annotations
 
@@ -39,6 +45,8 @@ class Workload:
     seed: int
     z_cells: float          # bunch centre, cells above the window origin
     box_uz: float = 0.0     # nonzero: box-frame mean u_z of a bunch given directly in the box frame
+    undulator: tuple = ()   # (K, period [m], segments, periods per segment, break [m]) of a planar undulator line
+    substeps: int = 1       # particle substeps per field step
 
     @property
     def dt(self) -> float:
@@ -70,6 +78,31 @@ class Workload:
         u = bunch.lab_to_box_u(u, self.gamma_boost)
         return x, u, q, m, ids
 
+    def beam_parameters(self):
+        """(center, sigma_position, mean_u, sigma_u, boost gamma) of the
+        lab-frame draw that ``generate`` boosts into the box frame."""
+        nx, ny, nz = self.dims
+        dx, dy, dz = self.spacing
+        center = [0.5 * nx * dx, 0.5 * ny * dy, self.z_cells * dz]
+        if self.box_uz:
+            sig_u = [self.emittance / self.sigma[0], self.emittance / self.sigma[1],
+                     self.sigma_gamma * abs(self.box_uz)]
+            return center, list(self.sigma), [0.0, 0.0, self.box_uz], sig_u, 1.0
+        u_lab = np.sqrt(self.gamma_lab ** 2 - 1.0)
+        gb = bunch.gamma_from_u(np.array([[0.0, 0.0, u_lab]]))
+        g_box = bunch.gamma_from_u(bunch.lab_to_box_u(np.array([[0.0, 0.0, u_lab]]), self.gamma_boost))[0]
+        sig_pos = [self.sigma[0], self.sigma[1], self.sigma[2] * gb[0] / g_box]
+        sig_u = [self.emittance / self.sigma[0], self.emittance / self.sigma[1], self.sigma_gamma * self.gamma_lab]
+        return center, sig_pos, [0.0, 0.0, u_lab], sig_u, self.gamma_boost
+
+    def generate_on_device(self, store):
+        """The same beam as ``generate`` drawn on the device straight into
+        ``store`` (aprl_particles_generate_gaussian; the device's log and cos
+        instead of numpy's)."""
+        center, sig_pos, mean_u, sig_u, boost = self.beam_parameters()
+        store.generateGaussian(self.particles, center, sig_pos, mean_u, sig_u, self.charge_electrons, self.seed,
+                               boost_gamma=boost)
+
 
 BENDING = Workload(
     name="bending-radiation 64x64x512 dipole, 1e5 e-",
@@ -89,6 +122,195 @@ COMOVING = Workload(
     charge_electrons=6.0e5, dipole_tesla=0.0, dipole_length=0.0, seed=777, z_cells=128.0, box_uz=-30.0)
 
 
+# Plain FP64, CPU-runnable: the generated-Gaussian drift (one electron bunch
+# of 1e4 macro-particles, lab gamma 200, sigma_x = sigma_y = 30 um,
+# sigma_z = 10 um, normalized emittance 1 um, sigma_gamma/gamma = 1e-3,
+# 50 pC) in the gamma_b = 10 frame, 32 x 32 x 256 with 8 transverse CPML
+# cells, no magnets. The 20 um transverse cells put the CPML 5.3 sigma from
+# the axis; the box-frame bunch (sigma_z 200 um = 25 cells) is centred in
+# the window.
+DRIFT_C1 = Workload(
+    name="generated-Gaussian drift 32x32x256, 1e4 e-",
+    dims=(32, 32, 256), spacing=(20.0e-6, 20.0e-6, 8.0e-6), cpml=8, particles=10_000,
+    gamma_lab=200.0, gamma_boost=10.0, sigma=(30e-6, 30e-6, 10e-6), sigma_gamma=1e-3, emittance=1e-6,
+    charge_electrons=50e-12 / bunch.QE, dipole_tesla=0.0, dipole_length=0.0, seed=17, z_cells=128.0)
+
+
+# Single-GPU capacity: 6.4e7 macro-particles of a seeded 6D Gaussian (lab
+# gamma 500, sigma_x = sigma_y = 40 um, sigma_z = 30 um) in 256 x 256 x 2048
+# with 16 transverse CPML cells, the planar undulator line of the FEL
+# segment, quintic shape, FP32-split. gamma_b = 6 stretches the bunch to a
+# box-frame sigma_z of 0.36 mm, so that +-4 sigma fills 70% of the 4.1 mm
+# window; the 4 um transverse cells put the CPML 11 sigma from the axis.
+CAPACITY_C5 = Workload(
+    name="capacity 256x256x2048 undulator, 6.4e7 e-",
+    dims=(256, 256, 2048), spacing=(4.0e-6, 4.0e-6, 2.0e-6), cpml=16, particles=64_000_000,
+    gamma_lab=500.0, gamma_boost=6.0, sigma=(40e-6, 40e-6, 30e-6), sigma_gamma=1e-4, emittance=1e-6,
+    charge_electrons=64_000_000 * 355.0, dipole_tesla=0.0, dipole_length=0.0, seed=505, z_cells=1024.0,
+    undulator=(1.0, 0.02, 6, 100, 0.5))
+
+
+def undulator_line(K: float, period: float, segments: int, periods: int, gap: float, first_center: float = 0.0):
+    """Planar undulator segments of strength K (peak field K 2 pi m_e c /
+    (e lambda_u)) along the lab z axis: (center z, length, peak field)."""
+    peak = K * 2.0 * np.pi * bunch.ME * C_SI / (bunch.QE * period)
+    length = periods * period
+    return [(first_center + s * (length + gap), length, peak) for s in range(segments)]
+
+
+def quiet_start(electrons: float, length: float, wavelength: float, gamma: float, sigma_gamma: float,
+                sigma_x: float, sigma_y: float, emittance_x: float, emittance_y: float, harmonics: int,
+                beamlets: int, seed: int):
+    """Multi-beamlet, multiharmonic quiet start of utilities/
+    sase_bunch_generator.cpp (generate, noisyPhases, sampleTransverse,
+    sampleGamma; lines 262-302, 432-463, 604-680): a top-hat of whole
+    resonant-wavelength cells, `beamlets` Gaussian beamlets per cell, each
+    cloned into 2 (harmonics + 1) members at quiet-start phases displaced by
+    the shot-noise harmonics a_m cos(m phi) + b_m sin(m phi) (sigma =
+    sqrt(2 / (m^2 N_e))). One splitmix64 stream (seed) feeds every draw in
+    the generator's order; vectorised over beamlets, which is exact while
+    every gamma draw is accepted at its first attempt (checked).
+
+    Returns lab-frame positions relative to the bunch centre (n,3), proper
+    velocities (n,3) and the one-based source ids, beamlet-major and
+    member-major as the generator writes them."""
+    cells = int(np.floor(length / wavelength + 0.5))
+    if cells == 0:
+        raise ValueError("Bunch length must contain at least one resonant wavelength")
+    eff = cells * wavelength
+    members = 2 * (harmonics + 1)
+    nb = cells * beamlets
+    per_beamlet = (electrons / cells) / beamlets
+    draws = 2 * (4 + 1 + 2 * harmonics)  # uniforms per beamlet: 4 transverse, 1 gamma, 2 per harmonic normals
+    idx = np.arange(nb, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        state0 = np.uint64(seed) + idx * np.uint64(draws) * bunch._GOLDEN
+
+    def uniform(k):
+        # the k-th draw of each beamlet's stream: state advanced k + 1 times
+        with np.errstate(over="ignore"):
+            z = state0 + np.uint64(k + 1) * bunch._GOLDEN
+            z = (z ^ (z >> np.uint64(30))) * bunch._M1
+            z = (z ^ (z >> np.uint64(27))) * bunch._M2
+            z = z ^ (z >> np.uint64(31))
+        return ((z >> np.uint64(11)).astype(np.float64) + 0.5) / 9007199254740992.0
+
+    def normal(j):
+        return np.sqrt(-2.0 * np.log(uniform(2 * j))) * np.cos(2.0 * np.pi * uniform(2 * j + 1))
+
+    x1, x2, y1, y2 = normal(0), normal(1), normal(2), normal(3)
+    x = sigma_x * x1
+    y = sigma_y * y1
+    ux = emittance_x / sigma_x * x2 if emittance_x else np.zeros(nb)
+    uy = emittance_y / sigma_y * y2 if emittance_y else np.zeros(nb)
+    g = gamma + sigma_gamma * normal(4)
+    if np.any(g <= np.sqrt(1.0 + ux * ux + uy * uy)):
+        raise ValueError("a gamma draw needs a second attempt; the vectorised stream no longer applies")
+    uz = np.sqrt(g * g - 1.0 - ux * ux - uy * uy)
+    quiet_order = harmonics + 1
+    beamlet = (idx % np.uint64(beamlets)).astype(np.float64)
+    cell = (idx // np.uint64(beamlets)).astype(np.float64)
+    base = (beamlet + 0.5) * np.pi / (quiet_order * beamlets)
+    disp = np.zeros((nb, members))
+    quiet = base[:, None] + np.arange(members)[None, :] * np.pi / quiet_order
+    for m in range(1, harmonics + 1):
+        sig = np.sqrt(2.0 / (m * m * per_beamlet))
+        a = sig * normal(5 + 2 * (m - 1))
+        b = sig * normal(6 + 2 * (m - 1))
+        disp += a[:, None] * np.cos(m * quiet) + b[:, None] * np.sin(m * quiet)
+    phase = np.mod(quiet + disp, 2.0 * np.pi)
+    n = nb * members
+    pos = np.empty((n, 3))
+    u = np.empty((n, 3))
+    pos[:, 0] = np.repeat(x, members)
+    pos[:, 1] = np.repeat(y, members)
+    pos[:, 2] = (-0.5 * eff + (cell[:, None] + phase / (2.0 * np.pi)) * wavelength).ravel()
+    u[:, 0] = np.repeat(ux, members)
+    u[:, 1] = np.repeat(uy, members)
+    u[:, 2] = np.repeat(uz, members)
+    return pos, u, np.arange(1, n + 1, dtype=np.uint64)
+
+
+@dataclass
+class FelSegment:
+    """Planar-undulator FEL segment: 4e6 macro-particles of weight 355,
+    quiet-start loaded (8 phase-spaced clones per seeded 6D sample, shot
+    noise through harmonic 3), lab gamma 1000, sigma_gamma/gamma 1e-4,
+    K = 1, lambda_u = 2 cm, 6 segments of 100 periods, resonant boost
+    gamma_b = gamma / sqrt(1 + K^2 / 2), 128 x 128 x 1024, quintic,
+    FP32-split. The window holds a flat-top slice of the bunch whose box-frame
+    length fills 70% of it (48 resonant wavelengths, 10417 beamlets each);
+    the bunch is placed at the entrance of the first segment."""
+    name: str = "FEL undulator segment 128x128x1024, 4e6 quiet-start e-"
+    dims: tuple = (128, 128, 1024)
+    spacing: tuple = (4.0e-6, 4.0e-6, 1.0e-6)
+    cpml: int = 8
+    gamma_lab: float = 1000.0
+    sigma_gamma: float = 1e-4
+    sigma: tuple = (30e-6, 30e-6)
+    emittance: float = 0.5e-6
+    weight: float = 355.0
+    K: float = 1.0
+    period: float = 0.02
+    segments: int = 6
+    periods: int = 100
+    gap: float = 0.5
+    harmonics: int = 3
+    beamlets: int = 10417
+    wavelength_cells: int = 48
+    seed: int = 1
+    z_cells: float = 512.0
+    substeps: int = 1
+
+    @property
+    def gamma_boost(self) -> float:
+        return self.gamma_lab / np.sqrt(1.0 + 0.5 * self.K ** 2)
+
+    @property
+    def dt(self) -> float:
+        return self.spacing[2] / C_SI
+
+    @property
+    def cells(self) -> int:
+        return int(np.prod(self.dims))
+
+    @property
+    def particles(self) -> int:
+        return self.wavelength_cells * self.beamlets * 2 * (self.harmonics + 1)
+
+    @property
+    def resonant_wavelength(self) -> float:
+        return self.period * (1.0 + 0.5 * self.K ** 2) / (2.0 * self.gamma_lab ** 2)
+
+    def magnets(self):
+        return undulator_line(self.K, self.period, self.segments, self.periods, self.gap,
+                              first_center=0.5 * self.periods * self.period)
+
+    def generate(self):
+        """Box-frame particles: position (n,3), u (n,3), charge, mass, ids."""
+        nx, ny, nz = self.dims
+        dx, dy, dz = self.spacing
+        lam = self.resonant_wavelength
+        n = self.particles
+        pos, u_lab, ids = quiet_start(n * self.weight, self.wavelength_cells * lam, lam, self.gamma_lab,
+                                      self.sigma_gamma * self.gamma_lab, self.sigma[0], self.sigma[1],
+                                      self.emittance, self.emittance, self.harmonics, self.beamlets, self.seed)
+        u_mean = np.sqrt(self.gamma_lab ** 2 - 1.0)
+        g_box = bunch.gamma_from_u(bunch.lab_to_box_u(np.array([[0.0, 0.0, u_mean]]), self.gamma_boost))[0]
+        stretch = self.gamma_lab / g_box
+        x = np.empty_like(pos)
+        x[:, 0] = 0.5 * nx * dx + pos[:, 0]
+        x[:, 1] = 0.5 * ny * dy + pos[:, 1]
+        x[:, 2] = self.z_cells * dz + stretch * pos[:, 2]
+        u = bunch.lab_to_box_u(u_lab, self.gamma_boost)
+        q = np.full(n, -bunch.QE * self.weight)
+        m = np.full(n, bunch.ME * self.weight)
+        return x, u, q, m, ids
+
+
+FEL_C3 = FelSegment()
+
+
 def setup_gpu(w: Workload, shape: str = "linear-binomial3", precision: str = "fp64", data=None):
     """Field grid + window + coupling + particle store for workload w;
     shape is "linear-binomial3" (with x-binomial3 smoothing) or
@@ -101,27 +323,53 @@ def setup_gpu(w: Workload, shape: str = "linear-binomial3", precision: str = "fp
     coupling = A.ConservativeSourceCoupling(fields, shape)
     coupling.setFrame(w.gamma_boost)
     # lab dipole B_y over z_lab in [-L/2, L/2]
-    if w.dipole_tesla:
+    if getattr(w, "dipole_tesla", 0.0):
         coupling.addPrescribedLabMagnet("uniform-dipole", [0.0, 0.0, -0.5 * w.dipole_length], w.dipole_length,
                                         peak_field=w.dipole_tesla, polarization_angle=0.5 * np.pi)
+    for zc, length, peak in magnets_of(w):
+        coupling.addPrescribedLabMagnet("planar-undulator", [0.0, 0.0, zc], length, period=undulator_period(w),
+                                        peak_field=peak)
     coupling.setParticleBoundary((w.cpml, w.cpml, 0))
+    if data == "device":
+        store = A.ParticleStore(fields, w.particles)
+        w.generate_on_device(store)
+        return fields, window, coupling, store
     x, u, q, m, ids = w.generate() if data is None else data
     store = A.ParticleStore(fields, len(q))
     store.upload(x, u, q, m, None, ids)
     return fields, window, coupling, store
 
 
+def magnets_of(w):
+    """Planar undulator segments (center z, length, peak field) of a workload."""
+    if isinstance(w, FelSegment):
+        return w.magnets()
+    if getattr(w, "undulator", ()):
+        K, period, segments, periods, gap = w.undulator
+        return undulator_line(K, period, segments, periods, gap, first_center=0.5 * periods * period)
+    return []
+
+
+def undulator_period(w) -> float:
+    return w.period if isinstance(w, FelSegment) else (w.undulator[1] if getattr(w, "undulator", ()) else 0.0)
+
+
 class Stepper:
     """Drives Simulation::solve's per-step order (simulation.cpp:518-583):
     clear J, window begin, magnetic, coupled source step, shift."""
 
-    def __init__(self, w: Workload, fields, window, coupling, store, offset: int = 10_000):
+    def __init__(self, w: Workload, fields, window, coupling, store, offset: int = 10_000,
+                 continuity: bool = False):
         self.w, self.fields, self.window, self.coupling, self.store = w, fields, window, coupling, store
         self.step_index = 0
         self.offset = offset
         self.origin = np.zeros(3)
         self.time = 0.0
         self.trials = []
+        # (max |(rho1 - rho0)/dt + div J|, max |(rho1 - rho0)/dt|) of every
+        # step, read before the shift clears J
+        self.track_continuity = continuity
+        self.continuity = []
 
     def step(self):
         f, win = self.fields, self.window
@@ -130,8 +378,10 @@ class Stepper:
         win.beginStep(n, self.offset)
         f.advanceMagnetic()
         win.setInlet()
-        its, _, _ = self.coupling.step(self.store, self.origin, self.time)
+        its, _, _ = self.coupling.step(self.store, self.origin, self.time, getattr(self.w, "substeps", 1))
         self.trials.append(its)
+        if self.track_continuity:
+            self.continuity.append(self.coupling.continuity(self.store))
         self.offset -= 1
         win.shift(n + 1, self.offset)
         # Simulation::finishMovingWindowStep: origin = fma(offset, dz, ref)

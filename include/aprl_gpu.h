@@ -172,6 +172,33 @@ aprl_status aprl_particles_download_states(aprl_particles* particles, int* state
  * window it deposits only its shaped support until it crosses the inlet. */
 aprl_status aprl_particles_set_held(aprl_particles* particles, const int* frozen, const double* beta);
 
+/* The generated-gaussian beam of ParticleInitializer (generatedGaussian,
+ * particle_initializer.cpp:60-100), drawn on the device straight into the
+ * store, replacing its contents: entry k of `count` (global index
+ * g = first + k) gets position center + sigma_position * gaussian(seed, g,
+ * axis) and proper velocity mean_u + sigma_u * gaussian(seed ^
+ * 0xd6e8feb86659fd93, g, axis) (splitmix64 uniforms and a Box-Muller
+ * cosine, particle_initializer.cpp:24-50), charge -e * electrons / total,
+ * mass m_e * electrons / total, weight 1 and id g + 1, as an active
+ * particle. With boost_gamma > 1 every proper velocity is then mapped into
+ * the frame boosted by boost_gamma along z (the light-front form of
+ * BoostFrameTransform::properVelocityLabToBox, boostframe.cpp:110-153). The
+ * device's log and cos stand in for glibc's, so the draw is deterministic
+ * but not bitwise the reference's. */
+typedef struct aprl_gaussian_beam {
+  double center[3];         /* config.center + positionOffset, m */
+  double sigma_position[3]; /* m */
+  double mean_u[3];         /* mean proper velocity (gamma beta along the direction) */
+  double sigma_u[3];
+  double electrons;         /* represented electrons of the whole beam */
+  uint64_t seed;            /* beam.input.random_seed */
+  uint64_t first;           /* global index of the first entry drawn */
+  uint64_t total;           /* macroparticles of the whole beam */
+  double boost_gamma;       /* 1: no boost */
+} aprl_gaussian_beam;
+aprl_status aprl_particles_generate_gaussian(aprl_particles* particles, const aprl_gaussian_beam* beam,
+                                             size_t count);
+
 /* Longitudinal source shape (longitudinal_shape.h:16). */
 typedef enum aprl_shape {
   APRL_SHAPE_QUINTIC_COMPENSATED = 0,
@@ -198,7 +225,7 @@ typedef struct aprl_step_report {
   double force;        /* self-force defect of the last trial */
   double work;         /* relative fixed-point work defect */
   unsigned tiles;      /* occupied cell tiles (8x8x4 cells) of the step */
-  unsigned chunks;     /* particle work chunks (at most 256 particles of one tile) */
+  unsigned chunks;     /* particle work chunks (at most 128 particles of one tile) */
 } aprl_step_report;
 
 /* ConservativeSourceCoupling (source_coupling.h:12-41) bound to a field
