@@ -136,6 +136,10 @@ def lib() -> C.CDLL:
     L.aprl_particles_destroy.restype = None
     L.aprl_particles_upload.argtypes = [vp, _sz, _dp, _dp, _dp, _dp, _dp, u64p]
     L.aprl_particles_generate_gaussian.argtypes = [vp, C.POINTER(GaussianBeam), _sz]
+    L.aprl_coupling_particle_energy.argtypes = [vp, vp, C.c_double, _dp]
+    L.aprl_window_enable_energy_ledger.argtypes = [vp, C.c_int]
+    L.aprl_window_finish_energy_step.argtypes = [vp]
+    L.aprl_window_energy_ledger.argtypes = [vp, _dp]
     L.aprl_particles_download.argtypes = [vp, _dp, _dp, _dp, _dp, _dp, u64p]
     L.aprl_particles_count.argtypes = [vp]
     L.aprl_particles_count.restype = _sz
@@ -213,7 +217,8 @@ EXPORTED = [
     "aprl_coupling_set_reference_current",
     "aprl_profile_enable", "aprl_profile_query", "aprl_profile_reset",
     "aprl_particles_create", "aprl_particles_destroy", "aprl_particles_upload", "aprl_particles_download",
-    "aprl_particles_generate_gaussian",
+    "aprl_particles_generate_gaussian", "aprl_coupling_particle_energy", "aprl_window_enable_energy_ledger",
+    "aprl_window_finish_energy_step", "aprl_window_energy_ledger",
     "aprl_particles_count", "aprl_particles_download_states", "aprl_coupling_boundary_counters",
     "aprl_particles_set_held", "aprl_coupling_set_activation", "aprl_coupling_set_hold_transverse",
     "aprl_repeat_held_addition", "aprl_pending_create", "aprl_pending_destroy", "aprl_pending_extract",

@@ -182,6 +182,14 @@ class ConservativeSourceCoupling:
         self.last_tiles, self.last_chunks = r.tiles, r.chunks
         return r.iterations, r.force, r.work
 
+    def particleEnergy(self, particles: ParticleStore, time):
+        """Energy-ledger particle sums (simulation.cpp:3591-3625): kinetic,
+        total, represented electrons, represented-weighted lab gamma, box
+        gamma, lab z, lab kinetic energy, c p_z box, c p_z lab."""
+        out = np.zeros(9)
+        check(lib().aprl_coupling_particle_energy(self.handle, particles.handle, float(time), dptr(out)))
+        return out
+
     def continuity(self, particles: ParticleStore):
         """(max |(rho1 - rho0)/dt + div J|, max |(rho1 - rho0)/dt|) of the
         last step (ChargeConservingCurrentDepositor::maxContinuityResidual,

@@ -52,6 +52,7 @@ aprl_status aprl_window_step_batch(aprl_instance_step* steps, size_t count) {
       aprl_instance_step& s = steps[i];
       if (s.fields->fp32) splitStepMiddle(s.coupling->c, s.particles->sp);
       else couplingStep(s.coupling->c, s.particles->p, s.origin, s.time, s.substeps, &s.report);
+      rethrow(aprl_window_finish_energy_step(s.fields));
       rethrow(aprl_window_shift(s.fields, s.step + 1, s.offset - 1, nullptr));
     }
     // Phase 3: wait for every instance and check its errors.

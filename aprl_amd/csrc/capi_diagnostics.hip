@@ -72,6 +72,16 @@ aprl_status aprl_fields_face_power(aprl_fields* fields, const size_t* lo, const 
   });
 }
 
+aprl_status aprl_coupling_particle_energy(aprl_coupling* coupling, aprl_particles* particles, double time,
+                                          double* out) {
+  return guarded([&] {
+    checkCoupled(coupling, particles);
+    if (!out) throw std::invalid_argument("Null diagnostic argument");
+    if (coupling->fields->fp32) particleEnergySums(coupling->fields->s32, coupling->c, particles->sp, time, out);
+    else particleEnergySums(coupling->fields->s, coupling->c, particles->p, time, out);
+  });
+}
+
 aprl_status aprl_coupling_continuity(aprl_coupling* coupling, aprl_particles* particles, double* out) {
   return guarded([&] {
     checkCoupled(coupling, particles);

@@ -7,6 +7,7 @@
 #include <memory>
 #include <type_traits>
 
+#include "diagnostics.hpp"
 #include "handles.hpp"
 #include "inlet.hpp"
 
@@ -58,6 +59,18 @@ void uploadPlanes(FieldStateT<R>& s, DeviceArray<R>* dst, const double* const* h
       out[a] = c < 3 ? s.zeroPlane[c].get() : nullptr;
     }
   }
+}
+
+// The ledger's step start: a pending inserted-plane measurement (a
+// charge-matched window whose inlet completion did not follow the shift)
+// is taken first, then the start power.
+template <class R>
+void beginLedger(FieldStateT<R>& s) {
+  if (s.ledger.insertPending) {
+    ledgerAfterShift(s);
+    s.ledger.insertPending = false;
+  }
+  ledgerBegin(s);
 }
 
 // Run fn on the grid's state of its precision.
@@ -247,7 +260,7 @@ aprl_status aprl_fields_advance_electric(aprl_fields* fields) {
       if (s.window) {
         if (s.phase != 2) throw std::logic_error("Moving-window electric phase mismatch");
         if (!s.inletSet) throw std::logic_error("Moving-window inlet planes were not supplied");
-        launchElectric(s, f, 0, none, s.inletPlane[0], s.inletPlane[1]);
+        windowCommitElectric(s, f);
         s.phase = 3;
         return;
       }
@@ -304,6 +317,7 @@ aprl_status aprl_window_begin_step(aprl_fields* fields, size_t step, ptrdiff_t o
       if (s.phase != 0 || (s.established && (step != s.step || offset != s.offset)))
         throw std::logic_error("Moving-window beginStep phase/origin mismatch");
       uploadPlanes(s, s.haloLo, lower, 0, 3, s.loPlane);
+      beginLedger(s);
       s.step = step;
       s.offset = offset;
       s.established = true;
@@ -342,6 +356,7 @@ aprl_status aprl_window_begin_step_centered(aprl_fields* fields, size_t step, pt
       if (ez_at_offset) toDevice(s.stream, s.refEz.get(), ez_at_offset, s.plane(2));
       else hipCheck(hipMemsetAsync(s.refEz.get(), 0, s.plane(2) * sizeof(R), s.stream), "hipMemsetAsync");
       centerInletHalo(s, s.haloLo[2].get(), s.refEz.get());
+      beginLedger(s);
       s.step = step;
       s.offset = offset;
       s.established = true;
@@ -371,10 +386,40 @@ aprl_status aprl_window_shift(aprl_fields* fields, size_t newStep, ptrdiff_t new
         throw std::logic_error("Moving-window requires one left shift after each complete step");
       const R* dev[6];
       uploadPlanes(s, s.shiftPlanes, planes, 0, 6, dev);
+      ledgerBeforeShift(s);
       windowShift(s, dev);
+      // the charge-matched inlet completes the new plane first
+      if (s.chargeMatch && s.ledger.enabled) s.ledger.insertPending = true;
+      else ledgerAfterShift(s);
       s.step = newStep;
       s.offset = newOffset;
       s.phase = 0;
+    });
+  });
+}
+
+aprl_status aprl_window_enable_energy_ledger(aprl_fields* fields, int enable) {
+  return guarded([&] { withState(fields, [&](auto& s) { ledgerEnable(s, enable != 0); }); });
+}
+
+aprl_status aprl_window_finish_energy_step(aprl_fields* fields) {
+  return guarded([&] {
+    withState(fields, [&](auto& s) {
+      if (!s.window) throw std::logic_error("No moving window is installed");
+      ledgerFinish(s);
+    });
+  });
+}
+
+aprl_status aprl_window_energy_ledger(aprl_fields* fields, double* out) {
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("Null ledger output");
+    withState(fields, [&](auto& s) {
+      if (s.ledger.insertPending && s.phase == 0) {
+        ledgerAfterShift(s);
+        s.ledger.insertPending = false;
+      }
+      ledgerRead(s, out);
     });
   });
 }

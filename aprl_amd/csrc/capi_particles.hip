@@ -3,6 +3,7 @@
 #include <cmath>
 #include <memory>
 
+#include "diagnostics.hpp"
 #include "handles.hpp"
 #include "inlet.hpp"
 
@@ -295,6 +296,14 @@ aprl_status aprl_window_complete_inlet(aprl_fields* fields, aprl_coupling* coupl
     const bool xBinomial = coupling->c.smoothX;
     if (fields->fp32) completeInletSplit(fields->s32, particles->sp, origin, shape, xBinomial);
     else completeInletPlain(fields->s, particles->p, origin, shape, xBinomial);
+    // the shift's inserted-plane energy is measured on the completed plane
+    auto inserted = [](auto& s) {
+      if (!s.ledger.insertPending) return;
+      ledgerAfterShift(s);
+      s.ledger.insertPending = false;
+    };
+    if (fields->fp32) inserted(fields->s32);
+    else inserted(fields->s);
   });
 }
 

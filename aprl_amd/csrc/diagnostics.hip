@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <type_traits>
 
 #include "diagnostics.hpp"
 #include "charge_basis.hpp"
@@ -492,7 +493,329 @@ void gaussResidual(FieldStateT<R>& s, const DeviceArray<double>& rho, const std:
   std::memcpy(&out[3], &m[1], sizeof(double));
 }
 
+// ---------------------------------------------------------------------------
+// Moving-window energy ledger.
+
+namespace {
+
+// Ledger accumulator slots (compensated pairs).
+constexpr int kLedgerOverwrite = 0, kLedgerInserted = 1, kLedgerCropped = 2, kLedgerRemap = 3, kLedgerFace = 4;
+constexpr int kLedgerSlots = 10, kLedgerWork = 32;
+constexpr int kLedgerFaceBlocks = 64;
+
+__device__ void ledgerAdd(double* acc, int slot, double v) {
+  Acc a{acc[2 * slot], acc[2 * slot + 1]};
+  a.add(v);
+  acc[2 * slot] = a.s;
+  acc[2 * slot + 1] = a.c;
+}
+
+// op 0: begin (overwrite delta := 0); op 1: inlet overwrite measured
+// (moving_window_fields.cpp:283-287); op 2: trapezoid face flux of the
+// finished step (430-435); op 3: the shift's removed and inserted planes
+// (503-509).
+__global__ void kLedgerUpdate(double* acc, double* w, int op, double halfDt) {
+  if (threadIdx.x || blockIdx.x) return;
+  if (op == 0) {
+    for (int f = 0; f < 6; ++f) w[6 + f] = 0.0;
+  } else if (op == 1) {
+    ledgerAdd(acc, kLedgerOverwrite, w[25] - w[24]);
+    // the inlet-only power has no upper z face
+    for (int f = 0; f < 6; ++f) w[6 + f] = f == 5 ? 0.0 : w[18 + f] - w[12 + f];
+  } else if (op == 2) {
+    for (int f = 0; f < 6; ++f) ledgerAdd(acc, kLedgerFace + f, halfDt * (w[f] + w[18 + f] - w[6 + f]));
+  } else {
+    ledgerAdd(acc, kLedgerRemap, w[27] - w[26]);
+    ledgerAdd(acc, kLedgerInserted, w[27]);
+    ledgerAdd(acc, kLedgerCropped, w[26]);
+  }
+}
+
+template <class R>
+__global__ void kInletOverwrite(R* ex, R* ey, const R* inletX, const R* inletY, int nx, int ny) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int planeX = nx * (ny + 1), planeY = (nx + 1) * ny;
+  if (inletX && t < planeX) ex[t] = inletX[t];
+  if (inletY && t < planeY) ey[t] = inletY[t];
+}
+
+// planeEnergy of moving_window_fields.cpp:387-397 over cells [lo, hi) into
+// a device slot.
+template <class R>
+void energyTo(FieldStateT<R>& s, int3 lo, int3 hi, double* dst) {
+  DiagnosticScratch& D = s.diag;
+  ensure(D.partial, 2 * 6 * kRedBlocks);
+  kEnergyBox<<<kRedBlocks, kRedThreads, 0, s.stream>>>(s.ptrs(), lo, hi, s.g.dx * s.g.dy * s.g.dz,
+                                                          D.partial.get());
+  kFinish<<<1, kRedThreads, 0, s.stream>>>(D.partial.get(), kRedBlocks, dst);
+  launchCheck("ledger energy");
+}
+
+// boundaryPower of moving_window_fields.cpp:398-419 over the cell box
+// [lo, hi) into six device slots.
+template <class R>
+void powerTo(FieldStateT<R>& s, int3 lo, int3 hi, double* dst) {
+  DiagnosticScratch& D = s.diag;
+  ensure(D.partial, 2 * 6 * kRedBlocks);
+  const double3 area = make_double3(s.g.dy * s.g.dz, s.g.dx * s.g.dz, s.g.dx * s.g.dy);
+  kFacePower<<<dim3(kLedgerFaceBlocks, 6), kRedThreads, 0, s.stream>>>(s.ptrs(), lo, hi, area, D.partial.get());
+  kFinish<<<6, kRedThreads, 0, s.stream>>>(D.partial.get(), kLedgerFaceBlocks, dst);
+  launchCheck("ledger power");
+}
+
+// The physical aperture (transverse CPML excluded) at z cells [k0, k1).
+template <class R>
+void apertureBox(const FieldStateT<R>& s, int k0, int k1, int3& lo, int3& hi) {
+  lo = make_int3(static_cast<int>(s.cpml.cells[0]), static_cast<int>(s.cpml.cells[1]), k0);
+  hi = make_int3(static_cast<int>(s.g.nx - s.cpml.cells[0]), static_cast<int>(s.g.ny - s.cpml.cells[1]), k1);
+}
+
+template <class R>
+void ledgerOp(FieldStateT<R>& s, int op) {
+  kLedgerUpdate<<<1, 1, 0, s.stream>>>(s.ledger.acc.get(), s.ledger.work.get(), op, 0.5 * s.g.dt);
+  launchCheck("ledger update");
+}
+
+}  // namespace
+
+template <class R>
+void ledgerEnable(FieldStateT<R>& s, bool enable) {
+  if (!s.window) throw std::logic_error("No moving window is installed");
+  // MovingWindowFields::enableEnergyLedger, moving_window_fields.cpp:541-545
+  if (s.established) throw std::logic_error("Moving-window energy ledger must be configured before the first step");
+  EnergyLedgerDev& L = s.ledger;
+  if (enable && !L.enabled) {
+    L.acc.resize(2 * kLedgerSlots);
+    L.work.resize(kLedgerWork);
+    hipCheck(hipMemsetAsync(L.acc.get(), 0, 2 * kLedgerSlots * sizeof(double), s.stream), "hipMemsetAsync");
+    hipCheck(hipMemsetAsync(L.work.get(), 0, kLedgerWork * sizeof(double), s.stream), "hipMemsetAsync");
+    L.overwriteEvents = L.fluxSteps = L.remapSteps = 0;
+    L.stepFinished = true;
+  }
+  L.enabled = enable;
+}
+
+template <class R>
+void ledgerBegin(FieldStateT<R>& s) {
+  if (!s.ledger.enabled) return;
+  int3 lo, hi;
+  apertureBox(s, 0, static_cast<int>(s.g.nz), lo, hi);
+  powerTo(s, lo, hi, s.ledger.work.get());
+  ledgerOp(s, 0);
+  s.ledger.stepFinished = false;
+}
+
+template <class R>
+void ledgerCommitElectric(FieldStateT<R>& s, const FieldPtrsT<R>& f) {
+  R* const none[3] = {nullptr, nullptr, nullptr};
+  launchElectric(s, f, 0, none, static_cast<const R*>(nullptr), static_cast<const R*>(nullptr));
+  double* w = s.ledger.work.get();
+  int3 lo, hi;
+  apertureBox(s, 0, 1, lo, hi);
+  energyTo(s, lo, hi, w + 24);
+  powerTo(s, lo, hi, w + 12);
+  const int nx = static_cast<int>(s.g.nx), ny = static_cast<int>(s.g.ny);
+  kInletOverwrite<R><<<ceilDiv(static_cast<std::size_t>(nx + 1) * (ny + 1), 256), 256, 0, s.stream>>>(
+      f.e[0], f.e[1], s.inletPlane[0], s.inletPlane[1], nx, ny);
+  launchCheck("inlet overwrite");
+  energyTo(s, lo, hi, w + 25);
+  powerTo(s, lo, hi, w + 18);
+  ledgerOp(s, 1);
+  ++s.ledger.overwriteEvents;
+}
+
+template <class R>
+void windowCommitElectric(FieldStateT<R>& s, const FieldPtrsT<R>& f) {
+  if (s.ledger.enabled) {
+    ledgerCommitElectric(s, f);
+    return;
+  }
+  R* const none[3] = {nullptr, nullptr, nullptr};
+  launchElectric(s, f, 0, none, s.inletPlane[0], s.inletPlane[1]);
+}
+
+template <class R>
+void ledgerFinish(FieldStateT<R>& s) {
+  if (!s.ledger.enabled) return;
+  if (s.phase != 3 || s.ledger.stepFinished)
+    throw std::logic_error("Moving-window energy flux requires one completed MPI field step");
+  int3 lo, hi;
+  apertureBox(s, 0, static_cast<int>(s.g.nz), lo, hi);
+  powerTo(s, lo, hi, s.ledger.work.get() + 18);
+  ledgerOp(s, 2);
+  ++s.ledger.fluxSteps;
+  s.ledger.stepFinished = true;
+}
+
+template <class R>
+void ledgerBeforeShift(FieldStateT<R>& s) {
+  if (!s.ledger.enabled) return;
+  if (!s.ledger.stepFinished) throw std::logic_error("Moving-window energy flux must finish after MPI and before shift");
+  int3 lo, hi;
+  apertureBox(s, static_cast<int>(s.g.nz) - 1, static_cast<int>(s.g.nz), lo, hi);
+  energyTo(s, lo, hi, s.ledger.work.get() + 26);
+}
+
+template <class R>
+void ledgerAfterShift(FieldStateT<R>& s) {
+  if (!s.ledger.enabled) return;
+  int3 lo, hi;
+  apertureBox(s, 0, 1, lo, hi);
+  energyTo(s, lo, hi, s.ledger.work.get() + 27);
+  ledgerOp(s, 3);
+  ++s.ledger.remapSteps;
+}
+
+template <class R>
+void ledgerRead(FieldStateT<R>& s, double out[13]) {
+  if (!s.ledger.enabled || !s.ledger.acc.size()) throw std::logic_error("The moving-window energy ledger is not enabled");
+  double acc[2 * kLedgerSlots];
+  hipCheck(hipMemcpyAsync(acc, s.ledger.acc.get(), sizeof(acc), hipMemcpyDeviceToHost, s.stream), "hipMemcpyAsync");
+  hipCheck(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
+  for (int k = 0; k < kLedgerSlots; ++k) out[k] = acc[2 * k] + acc[2 * k + 1];
+  out[10] = static_cast<double>(s.ledger.overwriteEvents);
+  out[11] = static_cast<double>(s.ledger.fluxSteps);
+  out[12] = static_cast<double>(s.ledger.remapSteps);
+}
+
+// ---------------------------------------------------------------------------
+// Particle sums of the energy ledger (Simulation, simulation.cpp:3591-3625).
+
+namespace {
+
+constexpr int kEnergyChannels = 9;
+
+struct ParticleSumArgs {
+  const double* x[3];  // plain form: absolute positions, u, mass
+  const double* u[3];
+  const double* m;
+  const int* cell[3];  // FP32-split form: lattice cell + offset, u, mass
+  const float* off[3];
+  const float* uf[3];
+  const float* mf;
+  const int* state;
+  int n, split;
+  double h[3];
+  double time;
+  BoostDev frame;
+};
+
+__device__ __forceinline__ double gammaOfU(const double u[3]) {
+  const double scale = fmax(1.0, fmax(fabs(u[0]), fmax(fabs(u[1]), fabs(u[2]))));
+  const double inv = 1.0 / scale;
+  const double a = u[0] * inv, b = u[1] * inv, c = u[2] * inv;
+  return scale * sqrt(inv * inv + a * a + b * b + c * c);
+}
+
+// One entry's channels: rest u^2 / (gamma + 1), rest gamma, m / m_e, and
+// the represented-weighted lab gamma, box gamma and lab z (simulation.cpp:
+// 3593-3620), then ParticleEnergyMoment::sample's lab kinetic energy and
+// c p_z in the box and the lab (particle_energy_audit.h:20-40).
+__device__ void particleChannels(const ParticleSumArgs& A, int p, double out[kEnergyChannels]) {
+  double u[3], z, m;
+  if (A.split) {
+    for (int a = 0; a < 3; ++a) u[a] = static_cast<double>(A.uf[a][p]);
+    z = (static_cast<double>(A.cell[2][p]) + static_cast<double>(A.off[2][p])) * A.h[2];
+    m = static_cast<double>(A.mf[p]);
+  } else {
+    for (int a = 0; a < 3; ++a) u[a] = A.u[a][p];
+    z = A.x[2][p];
+    m = A.m[p];
+  }
+  const BoostDev& F = A.frame;
+  const double gammaBox = gammaOfU(u);
+  const double rest = m * kC * kC;
+  const double u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  const double represented = m / kMe;
+  // properVelocityBoxToLab through the positive light-front component
+  // (boostframe.cpp:110-164)
+  const double mt2 = fma(u[0], u[0], fma(u[1], u[1], 1.0));
+  const double positive = (u[2] >= 0.0 ? gammaBox + u[2] : mt2 / (gammaBox - u[2])) * (F.gamma + F.gammaBeta);
+  const double lab[3] = {u[0], u[1], 0.5 * (positive - mt2 / positive)};
+  const double gammaLab = gammaOfU(lab);
+  const double labZ = F.labZ0 + fma(F.gammaBeta * F.c0, A.time, F.gamma * (z - F.boxZ0));
+  const double u2Lab = lab[0] * lab[0] + lab[1] * lab[1] + lab[2] * lab[2];
+  out[0] = rest * u2 / (gammaBox + 1.0);
+  out[1] = rest * gammaBox;
+  out[2] = represented;
+  out[3] = represented * gammaLab;
+  out[4] = represented * gammaBox;
+  out[5] = represented * labZ;
+  out[6] = rest * u2Lab / (sqrt(1.0 + u2Lab) + 1.0);
+  out[7] = rest * u[2];
+  out[8] = rest * lab[2];
+}
+
+__global__ void __launch_bounds__(kRedThreads) kParticleEnergy(ParticleSumArgs A, double* partial) {
+  __shared__ double ss[kRedThreads], sc[kRedThreads];
+  Acc a[kEnergyChannels];
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < A.n; p += gridDim.x * blockDim.x) {
+    const int st = A.state[p];
+    if (st != kStateActive && st != kStateHeld) continue;  // particles_ holds active and held entries
+    double v[kEnergyChannels];
+    particleChannels(A, p, v);
+    for (int c = 0; c < kEnergyChannels; ++c) a[c].add(v[c]);
+  }
+  for (int c = 0; c < kEnergyChannels; ++c) {
+    const Acc b = blockReduce(a[c], ss, sc);
+    if (threadIdx.x == 0) {
+      partial[2 * (c * gridDim.x + blockIdx.x)] = b.s;
+      partial[2 * (c * gridDim.x + blockIdx.x) + 1] = b.c;
+    }
+  }
+}
+
+}  // namespace
+
+template <class R, class Store>
+void particleEnergySums(FieldStateT<R>& s, const CouplingState& C, const Store& P, double time,
+                        double out[kEnergyChannels]) {
+  ParticleSumArgs A{};
+  A.n = static_cast<int>(P.count);
+  A.state = P.state.get();
+  A.h[0] = s.g.dx;
+  A.h[1] = s.g.dy;
+  A.h[2] = s.g.dz;
+  A.time = time;
+  A.frame = C.frame;
+  if constexpr (std::is_same_v<Store, SplitStore>) {
+    A.split = 1;
+    for (int a = 0; a < 3; ++a) {
+      A.cell[a] = P.cell[a].get();
+      A.off[a] = P.s[a].get();
+      A.uf[a] = P.s[3 + a].get();
+    }
+    A.mf = P.s[7].get();
+  } else {
+    for (int a = 0; a < 3; ++a) {
+      A.x[a] = P.s[a].get();
+      A.u[a] = P.s[3 + a].get();
+    }
+    A.m = P.s[7].get();
+  }
+  DiagnosticScratch& D = s.diag;
+  ensure(D.partial, 2 * kEnergyChannels * kRedBlocks);
+  ensure(D.out, 16);
+  kParticleEnergy<<<kRedBlocks, kRedThreads, 0, s.stream>>>(A, D.partial.get());
+  kFinish<<<kEnergyChannels, kRedThreads, 0, s.stream>>>(D.partial.get(), kRedBlocks, D.out.get());
+  launchCheck("particle energy");
+  hipCheck(hipMemcpyAsync(out, D.out.get(), kEnergyChannels * sizeof(double), hipMemcpyDeviceToHost, s.stream),
+           "hipMemcpyAsync");
+  hipCheck(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
+}
+
+template void particleEnergySums(FieldState&, const CouplingState&, const ParticleStore&, double, double*);
+template void particleEnergySums(FieldState32&, const CouplingState&, const SplitStore&, double, double*);
+
 #define APRL_DIAG_INSTANTIATE(R)                                                                              \
+  template void ledgerEnable(FieldStateT<R>&, bool);                                                          \
+  template void ledgerBegin(FieldStateT<R>&);                                                                 \
+  template void ledgerCommitElectric(FieldStateT<R>&, const FieldPtrsT<R>&);                                  \
+  template void windowCommitElectric(FieldStateT<R>&, const FieldPtrsT<R>&);                                  \
+  template void ledgerFinish(FieldStateT<R>&);                                                                \
+  template void ledgerBeforeShift(FieldStateT<R>&);                                                           \
+  template void ledgerAfterShift(FieldStateT<R>&);                                                            \
+  template void ledgerRead(FieldStateT<R>&, double*);                                                         \
   template double interiorEnergy(FieldStateT<R>&, const std::size_t*, const std::size_t*);                    \
   template void facePower(FieldStateT<R>&, const std::size_t*, const std::size_t*, double*);                  \
   template void continuityResidual(FieldStateT<R>&, const DeviceArray<double>&, const DeviceArray<double>&,   \

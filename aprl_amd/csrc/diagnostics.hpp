@@ -42,4 +42,39 @@ template <class R>
 void gaussResidual(FieldStateT<R>& s, const DeviceArray<double>& rho, const std::size_t lo[3], const std::size_t hi[3],
                    double out[4]);
 
+// Moving-window energy ledger (moving_window_fields.cpp:207-218, 264-288,
+// 387-438, 443-517). Enable allocates and clears the accumulators; begin
+// records the step's start power; commitElectric replaces the window
+// commit's fused inlet overwrite by the E update, the overwrite and its
+// plane-energy / inlet-power measurement; finish adds the trapezoid face
+// flux; beforeShift / afterShift measure the removed and inserted planes.
+template <class R>
+void ledgerEnable(FieldStateT<R>& s, bool enable);
+template <class R>
+void ledgerBegin(FieldStateT<R>& s);
+template <class R>
+void ledgerCommitElectric(FieldStateT<R>& s, const FieldPtrsT<R>& f);
+template <class R>
+void ledgerFinish(FieldStateT<R>& s);
+template <class R>
+void ledgerBeforeShift(FieldStateT<R>& s);
+template <class R>
+void ledgerAfterShift(FieldStateT<R>& s);
+// out[0..3] inlet overwrite, inserted, cropped, remap energy; out[4..9]
+// outward energy per face; out[10..12] overwrite events, flux steps, remap
+// steps.
+template <class R>
+void ledgerRead(FieldStateT<R>& s, double out[13]);
+// Sums over the store's particles (active and held entries) at box time
+// `time`: out[0] kinetic, out[1] total energy, out[2] represented
+// electrons, out[3..5] represented-weighted lab gamma, box gamma and lab z,
+// out[6] lab kinetic energy, out[7] / out[8] c p_z in the box / lab.
+template <class R, class Store>
+void particleEnergySums(FieldStateT<R>& s, const struct CouplingState& C, const Store& P, double time,
+                        double out[9]);
+// The window commit: fused E update with the inlet overwrite, or with the
+// ledger on, ledgerCommitElectric.
+template <class R>
+void windowCommitElectric(FieldStateT<R>& s, const FieldPtrsT<R>& f);
+
 }  // namespace aprl_gpu

@@ -76,6 +76,25 @@ struct DiagnosticScratch {
   DeviceArray<unsigned long long> acc, maxima;
 };
 
+// Moving-window field energy ledger (MovingWindowFieldEnergyLedger,
+// moving_window_fields.h:39-58) on the device: every per-step measurement
+// and accumulation is stream-ordered, nothing is read back until asked.
+struct EnergyLedgerDev {
+  bool enabled = false;
+  bool stepFinished = true;
+  // a charge-matched window measures the inserted plane after the inlet
+  // completion that follows the shift
+  bool insertPending = false;
+  // compensated (sum, carry) pairs: kLedger* slots
+  DeviceArray<double> acc;
+  // per-step values: start power [0, 6), overwrite delta [6, 12), power
+  // before the inlet overwrite [12, 18), power after / at the step end
+  // [18, 24), plane energy before [24] and after [25] the overwrite,
+  // removed plane energy [26], inserted plane energy [27]
+  DeviceArray<double> work;
+  unsigned long long overwriteEvents = 0, fluxSteps = 0, remapSteps = 0;
+};
+
 template <class R>
 struct FieldStateT {
   aprl_grid_geometry g{};
@@ -120,6 +139,7 @@ struct FieldStateT {
   hipStream_t stream = nullptr;
   Profiler profiler;
   DiagnosticScratch diag;
+  EnergyLedgerDev ledger;
 
   std::size_t compSize(int a) const { return dims[a][0] * dims[a][1] * dims[a][2]; }
   std::size_t plane(int a) const { return dims[a][0] * dims[a][1]; }

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <limits>
 
+#include "diagnostics.hpp"
 #include "particle_device.hpp"
 #include "shapes.hpp"
 
@@ -1376,7 +1377,8 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
   // commitConservativeElectric: E^n + curl B - dt/eps0 J, then the committed
   // CPML-E (history advance), PEC and inlet overwrite.
   double* const none[3] = {nullptr, nullptr, nullptr};
-  launchElectric(F, F.ptrs(), 0, none, F.window ? F.inletPlane[0] : nullptr, F.window ? F.inletPlane[1] : nullptr);
+  if (F.window) windowCommitElectric(F, F.ptrs());
+  else launchElectric(F, F.ptrs(), 0, none, F.inletPlane[0], F.inletPlane[1]);
   if (F.window) F.phase = 3;
   // Boundary events of the converged trial: carriers, removals, counters.
   if (lastWord & kEvtAny)

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <limits>
 
+#include "diagnostics.hpp"
 #include "particle_device.hpp"
 #include "shapes.hpp"
 
@@ -1240,7 +1241,8 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
   }
   // commitConservativeElectric with the last trial's current
   float* const none[3] = {nullptr, nullptr, nullptr};
-  launchElectric(F, F.ptrs(), 0, none, F.window ? F.inletPlane[0] : nullptr, F.window ? F.inletPlane[1] : nullptr);
+  if (F.window) windowCommitElectric(F, F.ptrs());
+  else launchElectric(F, F.ptrs(), 0, none, F.inletPlane[0], F.inletPlane[1]);
   if (F.window) F.phase = 3;
   // one readback per step: the trial error words (and the verifying residual)
   hipCheck(hipMemcpyAsync(pin + 4, C.error.get() + kTrialWord, trials * sizeof(unsigned), hipMemcpyDeviceToHost, st),

@@ -225,6 +225,22 @@ class MovingWindowFields:
         ey = None if ey is None else np.ascontiguousarray(ey, dtype=np.float64)
         check(lib().aprl_window_set_inlet(self.fields.handle, dptr(ex), dptr(ey)))
 
+    def enableEnergyLedger(self, enabled: bool = True) -> None:
+        """MovingWindowFields::enableEnergyLedger (moving_window_fields.h:91)."""
+        check(lib().aprl_window_enable_energy_ledger(self.fields.handle, 1 if enabled else 0))
+
+    def finishEnergyLedgerStep(self) -> None:
+        """MovingWindowFields::finishEnergyLedgerStep (moving_window_fields.cpp:425-438)."""
+        check(lib().aprl_window_finish_energy_step(self.fields.handle))
+
+    def energyLedger(self) -> dict:
+        """MovingWindowFieldEnergyLedger (moving_window_fields.h:39-58)."""
+        out = np.zeros(13)
+        check(lib().aprl_window_energy_ledger(self.fields.handle, dptr(out)))
+        return {"inletOverwriteEnergyJ": out[0], "insertedEnergyJ": out[1], "croppedEnergyJ": out[2],
+                "remapEnergyJ": out[3], "outwardEnergyFace": out[4:10].copy(), "overwriteEvents": int(out[10]),
+                "fluxSteps": int(out[11]), "remapSteps": int(out[12])}
+
     def shift(self, newStep: int, newOffset: int, planes=None) -> None:
         arr, keep = plane_array(planes)
         check(lib().aprl_window_shift(self.fields.handle, newStep, newOffset, arr))

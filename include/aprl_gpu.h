@@ -413,6 +413,38 @@ aprl_status aprl_coupling_continuity(aprl_coupling* coupling, aprl_particles* pa
 aprl_status aprl_coupling_gauss(aprl_coupling* coupling, aprl_particles* particles, const double* origin,
                                 const size_t* lo, const size_t* hi, double* out);
 
+/* Particle sums of the energy ledger (Simulation, simulation.cpp:3591-3625)
+ * over the store's particles (active and held entries: the reference's
+ * particles_; carriers and removed entries are not in it) at box time
+ * `time`, with the coupling's frame: out[0] sum rest u^2 / (gamma + 1),
+ * out[1] sum rest gamma, out[2] represented electrons sum m / m_e, out[3]
+ * sum represented * lab gamma, out[4] sum represented * box gamma, out[5]
+ * sum represented * lab z (labZFromBoxZT), out[6..8] the lab kinetic energy
+ * and c p_z in the box and the lab of ParticleEnergyMoment::sample
+ * (particle_energy_audit.h:20-40). FP64 compensated sums in a fixed order
+ * (the reference accumulates in long double). */
+aprl_status aprl_coupling_particle_energy(aprl_coupling* coupling, aprl_particles* particles, double time,
+                                          double* out);
+
+/* The moving-window field energy ledger (MovingWindowFieldEnergyLedger,
+ * moving_window_fields.h:39-58; enableEnergyLedger, finishEnergyLedgerStep,
+ * energyLedger, moving_window_fields.cpp:207-218, 264-288, 387-438,
+ * 443-517), kept on the device. Enabled between steps; from then on every
+ * begin_step records the outward power through the faces of the physical
+ * aperture, the window commit measures the inlet overwrite's plane energy
+ * and inlet power change, finish_energy_step (after the commit, before the
+ * shift; APRL_LOGIC_ERROR otherwise) adds the trapezoid face flux
+ * dt/2 (start + end - overwrite delta), and every shift adds the removed
+ * and inserted plane energies (after aprl_window_complete_inlet on a
+ * charge-matched window). aprl_window_energy_ledger: out[0] inlet
+ * overwrite energy, out[1] inserted, out[2] cropped, out[3] remap energy,
+ * out[4..9] outward energy through x-, x+, y-, y+, z-, z+, out[10..12]
+ * overwrite events, flux steps, remap steps. FP64 compensated sums (the
+ * reference's are long double). */
+aprl_status aprl_window_enable_energy_ledger(aprl_fields* fields, int enable);
+aprl_status aprl_window_finish_energy_step(aprl_fields* fields);
+aprl_status aprl_window_energy_ledger(aprl_fields* fields, double* out);
+
 /* Per-kernel timing with HIP events on the instance stream. enable=1 starts
  * recording every launch; query returns launches and total milliseconds for
  * a kernel name (0 launches if unknown). */

@@ -147,6 +147,33 @@ int orc_window_commit(void* h) {
   });
 }
 
+int orc_window_enable_ledger(void* h, int enable) {
+  return guarded([&] {
+    if (!F(h)->window) throw std::logic_error("No moving window is installed");
+    if (F(h)->window->established)
+      throw std::logic_error("Moving-window energy ledger must be configured before the first step");
+    F(h)->window->ledger.enabled = enable != 0;
+  });
+}
+int orc_window_finish_ledger_step(void* h) { return guarded([&] { F(h)->finishEnergyLedgerStep(); }); }
+// out: inlet overwrite, inserted, cropped, remap, outward[6], overwrite
+// events, flux steps, remap steps.
+int orc_window_ledger(void* h, double* out) {
+  return guarded([&] {
+    orc::Window& w = *F(h)->window;
+    if (w.ledger.insertPending && w.phase == 0) F(h)->ledgerInserted();
+    const auto& L = w.ledger;
+    out[0] = static_cast<double>(L.inletOverwriteEnergyJ);
+    out[1] = static_cast<double>(L.insertedEnergyJ);
+    out[2] = static_cast<double>(L.croppedEnergyJ);
+    out[3] = static_cast<double>(L.remapEnergyJ);
+    for (int f = 0; f < 6; ++f) out[4 + f] = static_cast<double>(L.outwardEnergyFace[f]);
+    out[10] = static_cast<double>(L.overwriteEvents);
+    out[11] = static_cast<double>(L.fluxSteps);
+    out[12] = static_cast<double>(L.remapSteps);
+  });
+}
+
 double orc_fields_energy_density(void* h, std::size_t i, std::size_t j, std::size_t k) {
   return F(h)->energyDensityCell(i, j, k);
 }
@@ -345,6 +372,16 @@ int orc_coupling_complete_inlet(void* h, const double* newOrigin) {
       for (const orc::Particle& p : H.pending->near(f.window->step, 8)) sources.push_back(p);
     orc::chargeMatchedInlet(f, sources, vec(newOrigin), H.c.shape, H.c.smoothX, f.cpml->p.cells[0],
                             f.cpml->p.cells[1]);
+    if (f.window->ledger.insertPending) f.ledgerInserted();
+  });
+}
+
+// Energy-ledger particle sums at box time `time` (simulation.cpp:3591-3625).
+int orc_coupling_particle_energy(void* h, double time, double* out) {
+  return guarded([&] {
+    long double sums[9];
+    orc::particleEnergySums(CH(h)->particles, CH(h)->c.frame, time, sums);
+    for (int c = 0; c < 9; ++c) out[c] = static_cast<double>(sums[c]);
   });
 }
 

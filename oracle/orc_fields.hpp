@@ -553,6 +553,13 @@ class Fields {
   void setWindowInlet(const double* ex, const double* ey);
   void undoPreview();
   void shiftWindow(std::size_t n, std::ptrdiff_t k, const double* const planes[6]);
+  // MovingWindowFields energy ledger: planeEnergy, boundaryPower,
+  // finishEnergyLedgerStep (moving_window_fields.cpp:387-438) and the
+  // shift's inserted-plane measurement (503-509).
+  long double planeEnergy(std::size_t k) const;
+  void boundaryPower(bool inletOnly, long double power[6]) const;
+  void finishEnergyLedgerStep();
+  void ledgerInserted();
 
   void clearCurrent() {
     for (unsigned a = 6; a < 9; ++a) std::fill(f[a].v.begin(), f[a].v.end(), 0.0);
@@ -608,6 +615,21 @@ struct Window {
   // (moving_window_fields.cpp:551-555): the inlet Ez is completed from the
   // charge and its departure from the reference is reflected oddly.
   bool centered = false;
+  // MovingWindowFieldEnergyLedger (moving_window_fields.h:39-58) and the
+  // boundary's per-step ledger state (moving_window_fields.cpp:159-161).
+  struct Ledger {
+    bool enabled = false;
+    long double inletOverwriteEnergyJ = 0., insertedEnergyJ = 0., croppedEnergyJ = 0., remapEnergyJ = 0.;
+    long double outwardEnergyFace[6] = {0., 0., 0., 0., 0., 0.};
+    std::uint64_t overwriteEvents = 0, fluxSteps = 0, remapSteps = 0;
+    long double startPower[6] = {0., 0., 0., 0., 0., 0.};
+    long double overwritePowerDelta[6] = {0., 0., 0., 0., 0., 0.};
+    bool energyStepFinished = false;
+    // a charge-matched window measures the inserted plane after its inlet
+    // completion (the reference runs the completion inside the shift)
+    bool insertPending = false;
+    long double removedLocal = 0.;
+  } ledger;
 };
 
 inline Fields::Fields(const Grid& grid) : g(grid) {
@@ -671,12 +693,93 @@ inline void Fields::afterElectricBoundary() {
   }
   // Boundary::afterElectricUpdate, moving_window_fields.cpp:254-293.
   cpmlElectric();
+  long double energyBefore = 0.;
+  long double powerBefore[6];
+  const bool measure = w.ledger.enabled;
+  if (measure) {
+    energyBefore = planeEnergy(0);
+    boundaryPower(true, powerBefore);
+  }
   for (unsigned a = 0; a < 2; ++a) {
     Comp& c = f[a];
     for (std::size_t j = 0; j < c.ny; ++j)
       for (std::size_t i = 0; i < c.nx; ++i) c(i, j, 0) = w.inlet[a][j * c.nx + i];
   }
+  if (measure) {
+    w.ledger.inletOverwriteEnergyJ += planeEnergy(0) - energyBefore;
+    long double powerAfter[6];
+    boundaryPower(true, powerAfter);
+    for (unsigned face = 0; face < 6; ++face) w.ledger.overwritePowerDelta[face] = powerAfter[face] - powerBefore[face];
+    ++w.ledger.overwriteEvents;
+  }
   w.phase = 3;
+}
+
+// planeEnergy, moving_window_fields.cpp:387-397 (physical aperture).
+inline long double Fields::planeEnergy(std::size_t k) const {
+  const long double volume = static_cast<long double>(g.dx) * static_cast<long double>(g.dy) *
+                             static_cast<long double>(g.dz);
+  const std::size_t cx = cpml->p.cells[0], cy = cpml->p.cells[1];
+  long double value = 0.;
+  for (std::size_t j = cy; j < g.ny - cy; ++j)
+    for (std::size_t i = cx; i < g.nx - cx; ++i) value += static_cast<long double>(energyDensityCell(i, j, k)) * volume;
+  if (!std::isfinite(value)) throw std::runtime_error("Non-finite moving-window energy diagnostic");
+  return value;
+}
+
+// boundaryPower, moving_window_fields.cpp:398-419 (one rank: z- and z+).
+inline void Fields::boundaryPower(bool inletOnly, long double power[6]) const {
+  for (unsigned face = 0; face < 6; ++face) power[face] = 0.;
+  const std::size_t i0 = cpml->p.cells[0], i1 = g.nx - i0;
+  const std::size_t j0 = cpml->p.cells[1], j1 = g.ny - j0;
+  const long double yz = static_cast<long double>(g.dy) * g.dz;
+  const long double xz = static_cast<long double>(g.dx) * g.dz;
+  const long double xy = static_cast<long double>(g.dx) * g.dy;
+  for (std::size_t k = 0; k < (inletOnly ? 1 : g.nz); ++k) {
+    for (std::size_t j = j0; j < j1; ++j) {
+      power[0] -= static_cast<long double>(poyntingCell(i0, j, k, 0)) * yz;
+      power[1] += static_cast<long double>(poyntingCell(i1 - 1, j, k, 0)) * yz;
+    }
+    for (std::size_t i = i0; i < i1; ++i) {
+      power[2] -= static_cast<long double>(poyntingCell(i, j0, k, 1)) * xz;
+      power[3] += static_cast<long double>(poyntingCell(i, j1 - 1, k, 1)) * xz;
+    }
+  }
+  for (std::size_t j = j0; j < j1; ++j)
+    for (std::size_t i = i0; i < i1; ++i) {
+      power[4] -= static_cast<long double>(poyntingCell(i, j, 0, 2)) * xy;
+      if (!inletOnly) power[5] += static_cast<long double>(poyntingCell(i, j, g.nz - 1, 2)) * xy;
+    }
+  for (unsigned face = 0; face < 6; ++face)
+    if (!std::isfinite(power[face])) throw std::runtime_error("Non-finite moving-window boundary power");
+}
+
+// finishEnergyLedgerStep, moving_window_fields.cpp:425-438.
+inline void Fields::finishEnergyLedgerStep() {
+  Window& w = *window;
+  if (!w.ledger.enabled) return;
+  if (w.phase != 3 || w.ledger.energyStepFinished)
+    throw std::logic_error("Moving-window energy flux requires one completed MPI field step");
+  long double endPower[6];
+  boundaryPower(false, endPower);
+  const long double halfDt = .5L * static_cast<long double>(g.dt);
+  for (unsigned face = 0; face < 6; ++face)
+    w.ledger.outwardEnergyFace[face] +=
+        halfDt * (w.ledger.startPower[face] + endPower[face] - w.ledger.overwritePowerDelta[face]);
+  ++w.ledger.fluxSteps;
+  w.ledger.energyStepFinished = true;
+}
+
+// The shift's remap bookkeeping after the new plane is complete,
+// moving_window_fields.cpp:503-509.
+inline void Fields::ledgerInserted() {
+  Window& w = *window;
+  const long double insertedLocal = planeEnergy(0);
+  w.ledger.remapEnergyJ += insertedLocal - w.ledger.removedLocal;
+  w.ledger.insertedEnergyJ += insertedLocal;
+  w.ledger.croppedEnergyJ += w.ledger.removedLocal;
+  ++w.ledger.remapSteps;
+  w.ledger.insertPending = false;
 }
 
 // MovingWindowFields construction + installBoundary,
@@ -707,7 +810,13 @@ inline Halo Fields::beginWindowStep(std::size_t n, std::ptrdiff_t k, const doubl
   if (w.phase != 0 || (w.established && (n != w.step || k != w.offset)))
     throw std::logic_error("Moving-window beginStep phase/origin mismatch");
   if (w.centered && !ezAtOffset) throw std::logic_error("A centered inlet needs the reference Ez at the offset");
+  if (w.ledger.insertPending) ledgerInserted();
   w.step = n; w.offset = k; w.established = true;
+  if (w.ledger.enabled) {  // moving_window_fields.cpp:214-218
+    boundaryPower(false, w.ledger.startPower);
+    for (unsigned face = 0; face < 6; ++face) w.ledger.overwritePowerDelta[face] = 0.;
+    w.ledger.energyStepFinished = false;
+  }
   for (unsigned a = 0; a < 3; ++a)
     std::memcpy(w.lower[a].data(), lowerPlanes[a], w.lower[a].size() * sizeof(double));
   if (w.centered) {
@@ -746,6 +855,9 @@ inline void Fields::shiftWindow(std::size_t n, std::ptrdiff_t k, const double* c
   Window& w = *window;
   if (w.phase != 3 || n != w.step + 1 || k != w.offset - 1)
     throw std::logic_error("Moving-window requires one left shift after each complete step");
+  if (w.ledger.enabled && !w.ledger.energyStepFinished)
+    throw std::logic_error("Moving-window energy flux must finish after MPI and before shift");
+  if (w.ledger.enabled) w.ledger.removedLocal = planeEnergy(g.nz - 1);
   for (unsigned a = 0; a < 6; ++a) {
     Comp& c = f[a];
     const std::size_t plane = c.nx * c.ny;
@@ -769,6 +881,10 @@ inline void Fields::shiftWindow(std::size_t n, std::ptrdiff_t k, const double* c
     }
   }
   w.step = n; w.offset = k; w.phase = 0;
+  if (w.ledger.enabled) {
+    w.ledger.insertPending = true;
+    if (!w.centered) ledgerInserted();
+  }
 }
 
 }  // namespace orc

@@ -509,6 +509,51 @@ struct Particle {
   double betaHeld = 0.0;
 };
 
+// The energy ledger's particle sums, Simulation (simulation.cpp:3591-3625),
+// over particles_ (active and held), in long double as the reference sums
+// them, with ParticleEnergyMoment::sample (particle_energy_audit.h:20-40):
+// out = {kinetic, total, represented, rep * lab gamma, rep * box gamma,
+// rep * lab z, lab kinetic, c p_z box, c p_z lab}.
+inline V3 properVelocityBoxToLab(const Boost& frame, const V3& u) {
+  // positiveLightFrontComponent / properVelocityFromPositiveLightFront,
+  // boostframe.cpp:110-164
+  const double gamma = gammaFromU(u);
+  const double mt2 = std::fma(u[0], u[0], std::fma(u[1], u[1], 1.0));
+  const double positive = (u[2] >= 0.0 ? gamma + u[2] : mt2 / (gamma - u[2])) * (frame.gamma + frame.gammaBeta);
+  V3 lab = u;
+  lab[2] = 0.5 * (positive - mt2 / positive);
+  return lab;
+}
+
+inline void particleEnergySums(const std::vector<Particle>& particles, const Boost& frame, double timeBox,
+                               long double out[9]) {
+  for (int c = 0; c < 9; ++c) out[c] = 0.;
+  const long double c = static_cast<long double>(kC);
+  for (const Particle& p : particles) {
+    const double gammaBox = gammaFromU(p.u);
+    const long double mass = static_cast<long double>(p.mass);
+    const long double rest = mass * c * c;
+    const long double u2 = static_cast<long double>(p.u[0] * p.u[0] + p.u[1] * p.u[1] + p.u[2] * p.u[2]);
+    out[0] += rest * u2 / static_cast<long double>(gammaBox + 1.0);
+    out[1] += rest * static_cast<long double>(gammaBox);
+    const long double represented = mass / static_cast<long double>(kMe);
+    out[2] += represented;
+    const V3 lab = properVelocityBoxToLab(frame, p.u);
+    const double gammaLab = gammaFromU(lab);
+    out[3] += represented * static_cast<long double>(gammaLab);
+    out[4] += represented * static_cast<long double>(gammaBox);
+    out[5] += represented * static_cast<long double>(frame.labZFromBoxZT(p.position[2], timeBox));
+    // ParticleEnergyMoment::sample
+    const long double restM = static_cast<long double>(p.mass) * kC * kC;
+    long double ul2 = 0;
+    for (unsigned a = 0; a < 3; ++a) ul2 += static_cast<long double>(lab[a]) * lab[a];
+    const long double gammaL = std::sqrt(1 + ul2);
+    out[6] += restM * ul2 / (gammaL + 1);
+    out[7] += restM * p.u[2];
+    out[8] += restM * lab[2];
+  }
+}
+
 // ConservativeSourceCoupling (source_coupling.cpp) for one rank, with the
 // longitudinal_shape::Current of depth 8 (production, compact=false).
 class Coupling {

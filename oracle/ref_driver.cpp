@@ -224,6 +224,25 @@ int ref_window_preview(void* h, int enabled) {
 }
 int ref_window_undo_preview(void* h) { return guarded([&] { R(h)->window->undoPreviewElectricBoundary(*R(h)->f); }); }
 int ref_window_commit(void* h) { return guarded([&] { R(h)->window->commitPreparedElectricBoundary(*R(h)->f); }); }
+int ref_window_enable_ledger(void* h, int enable) {
+  return guarded([&] { R(h)->window->enableEnergyLedger(enable != 0); });
+}
+int ref_window_finish_ledger_step(void* h) {
+  return guarded([&] { R(h)->window->finishEnergyLedgerStep(*R(h)->f); });
+}
+int ref_window_ledger(void* h, double* out) {
+  return guarded([&] {
+    const MovingWindowFieldEnergyLedger& L = R(h)->window->energyLedger();
+    out[0] = static_cast<double>(L.inletOverwriteEnergyJ);
+    out[1] = static_cast<double>(L.insertedEnergyJ);
+    out[2] = static_cast<double>(L.croppedEnergyJ);
+    out[3] = static_cast<double>(L.remapEnergyJ);
+    for (int f = 0; f < 6; ++f) out[4 + f] = static_cast<double>(L.outwardEnergyFace[f]);
+    out[10] = static_cast<double>(L.overwriteEvents);
+    out[11] = static_cast<double>(L.fluxSteps);
+    out[12] = static_cast<double>(L.remapSteps);
+  });
+}
 
 double ref_fields_energy_density(void* h, std::size_t i, std::size_t j, std::size_t k) {
   return R(h)->f->fieldEnergyDensityCell(i, j, k);
@@ -660,6 +679,42 @@ void completeInlet(void* context, EBFieldGrid& f, std::size_t step, std::ptrdiff
   c.inlet->complete(f, false);
 }
 }  // namespace
+
+// The energy ledger's particle loop (simulation.cpp:3591-3625) over the
+// particle list with the reference's BoostFrameTransform; the audit moment
+// of ParticleEnergyMoment::sample (particle_energy_audit.h:20-40, a header
+// that pulls in the HDF5 detectors) is written out here.
+int ref_coupling_particle_energy(void* h, double timeBox, double* out) {
+  return guarded([&] {
+    RefCoupling& H = *RC(h);
+    long double local[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (const RelativisticParticleSI& particle : H.particles) {
+      const Double gammaBox = BoostFrameTransform::gammaFromProperVelocity(particle.properVelocity);
+      const long double mass = static_cast<long double>(particle.mass);
+      const long double c = static_cast<long double>(SI::c);
+      const long double rest = mass * c * c;
+      const long double u2 = static_cast<long double>(particle.properVelocity.norm2());
+      local[0] += rest * u2 / static_cast<long double>(gammaBox + 1.0);
+      local[1] += rest * static_cast<long double>(gammaBox);
+      const long double represented = mass / static_cast<long double>(SI::electronMass);
+      local[2] += represented;
+      FieldVector<Double> properVelocityLab(0.0);
+      H.frame.properVelocityBoxToLab(particle.properVelocity, properVelocityLab);
+      const Double gammaLab = BoostFrameTransform::gammaFromProperVelocity(properVelocityLab);
+      local[3] += represented * static_cast<long double>(gammaLab);
+      local[4] += represented * static_cast<long double>(gammaBox);
+      local[5] += represented * static_cast<long double>(H.frame.labZFromBoxZT(particle.position[2], timeBox));
+      const long double restM = static_cast<long double>(particle.mass) * SI::c * SI::c;
+      long double ul2 = 0;
+      for (unsigned a = 0; a < 3; ++a) ul2 += static_cast<long double>(properVelocityLab[a]) * properVelocityLab[a];
+      const long double gammaL = std::sqrt(1 + ul2);
+      local[6] += restM * ul2 / (gammaL + 1);
+      local[7] += restM * particle.properVelocity[2];
+      local[8] += restM * properVelocityLab[2];
+    }
+    for (int k = 0; k < 9; ++k) out[k] = static_cast<double>(local[k]);
+  });
+}
 
 int ref_window_set_charge_match(void* fields, void* coupling, const double* referenceOrigin) {
   return guarded([&] {

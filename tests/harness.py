@@ -193,6 +193,21 @@ class CpuFields:
         arr, keep = _ptr_array(planes6)
         self._check(self._fn("window_shift")(self.h, step, offset, arr))
 
+    def enable_ledger(self, enabled=True):
+        """MovingWindowFields::enableEnergyLedger."""
+        self._check(self._fn("window_enable_ledger")(self.h, int(enabled)))
+
+    def finish_ledger_step(self):
+        """MovingWindowFields::finishEnergyLedgerStep (after the commit)."""
+        self._check(self._fn("window_finish_ledger_step")(self.h))
+
+    def ledger(self):
+        """(inlet overwrite, inserted, cropped, remap, outward x6, overwrite
+        events, flux steps, remap steps) of the window's energy ledger."""
+        out = np.zeros(13)
+        self._check(self._fn("window_ledger")(self.h, _ptr(out)))
+        return out
+
     def window_preview(self, enabled):
         self._check(self._fn("window_preview")(self.h, int(enabled)))
 
@@ -488,6 +503,12 @@ class CpuCoupling:
         _call(self.prefix, "coupling_step", self.h, _ptr(_v(origin)), time, substeps, exit_lab_z, C.byref(it),
               C.byref(force), C.byref(work))
         return it.value, force.value, work.value
+
+    def particle_energy(self, time):
+        """Energy-ledger particle sums (simulation.cpp:3591-3625) at box time."""
+        out = np.zeros(9)
+        _call(self.prefix, "coupling_particle_energy", self.h, C.c_double(time), _ptr(out))
+        return out
 
     def prescribed_box(self, pos, t):
         e = np.zeros(3)
