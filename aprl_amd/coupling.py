@@ -191,12 +191,13 @@ class ConservativeSourceCoupling:
         return out
 
     def continuity(self, particles: ParticleStore):
-        """(max |(rho1 - rho0)/dt + div J|, max |(rho1 - rho0)/dt|) of the
-        last step (ChargeConservingCurrentDepositor::maxContinuityResidual,
+        """(max |(rho1 - rho0)/dt + div J|, max |(rho1 - rho0)/dt|,
+        max max(|rho0|, |rho1|)/dt) of the last step
+        (ChargeConservingCurrentDepositor::maxContinuityResidual,
         eb_deposition.cpp:195-226); call before the window shift."""
-        out = np.zeros(2)
+        out = np.zeros(3)
         check(lib().aprl_coupling_continuity(self.handle, particles.handle, dptr(out)))
-        return out[0], out[1]
+        return out[0], out[1], out[2]
 
     def gaussResidual(self, particles: ParticleStore, origin, lo, hi):
         """(sum err^2, sum src^2, max |err|, max |src|) of div E - rho/eps0

@@ -175,6 +175,7 @@ struct RhoArgs {
   int nx, ny, nz;
   int shape;  // 0 compensated quintic, 2 binomial3 with x-binomial3
   int shift;
+  int absolute;  // 1: accumulate |tap| (the bound pass)
   double invVolume;
   unsigned long long* acc;  // (nx+1)(ny+1)(nz+1) vertices
   ChargeWeights cw;
@@ -231,7 +232,7 @@ __global__ void kChargeDensity(RhoArgs R) {
         if (ii < 0 || ii >= vx) continue;
         const double w = qv * wx[a] * wy[b] * wz[c];
         atomicAdd(R.acc + (static_cast<size_t>(kk) * vy + jj) * vx + ii,
-                  static_cast<unsigned long long>(llrint(ldexp(w, R.shift))));
+                  static_cast<unsigned long long>(llrint(ldexp(R.absolute ? fabs(w) : w, R.shift))));
       }
     }
   }
@@ -243,6 +244,26 @@ __global__ void kFixedToDouble(unsigned long long* acc, double* out, size_t n, i
     out[t] = ldexp(static_cast<double>(static_cast<long long>(acc[t])), -shift);
     acc[t] = 0ull;
   }
+}
+
+// Largest entry of a non-negative fixed-point array into *out (zeroed
+// before); clears what it reads.
+__global__ void __launch_bounds__(kRedThreads) kMaxClearU64(unsigned long long* acc, size_t n,
+                                                             unsigned long long* out) {
+  __shared__ unsigned long long sm[kRedThreads];
+  unsigned long long m = 0ull;
+  for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    m = max(m, acc[t]);
+    acc[t] = 0ull;
+  }
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) sm[threadIdx.x] = max(sm[threadIdx.x], sm[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(out, sm[0]);
 }
 
 template <class T>
@@ -284,6 +305,7 @@ __global__ void kContinuity(FieldPtrsT<R> F, const double* rho0, const double* r
     const double residual = rate + (rightX - leftX) / h.x + (rightY - leftY) / h.y + (rightZ - leftZ) / h.z;
     atomicMaxAbs(out, residual);
     atomicMaxAbs(out + 1, rate);
+    atomicMaxAbs(out + 2, fmax(fabs(rho0[t]), fabs(rho1[t])) / dt);
   }
 }
 
@@ -407,16 +429,44 @@ void depositCharge(FieldStateT<Real>& s, RhoArgs R, const Q* q, DeviceArray<doub
     hipCheck(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
   }
   const double invVolume = 1.0 / (s.g.dx * s.g.dy * s.g.dz);
-  int e2 = 0;
-  (void)std::frexp(4.0 * total * invVolume, &e2);
-  const int shift = total > 0.0 ? 62 - e2 : 0;
   R.h[0] = s.g.dx; R.h[1] = s.g.dy; R.h[2] = s.g.dz;
   R.nx = static_cast<int>(s.g.nx);
   R.ny = static_cast<int>(s.g.ny);
   R.nz = static_cast<int>(s.g.nz);
-  R.shift = shift;
   R.invVolume = invVolume;
   R.acc = D.acc.get();
+  int shift = 0;
+  if (n > 0 && total > 0.0) {
+    // Bound pass: sum |tap| per vertex on the coarse exponent of the total
+    // |charge| (no vertex exceeds it), and take the largest vertex. The
+    // signed pass then uses the exponent of twice that bound, so its
+    // quantum is 2^-61 of the largest |rho| rather than of the whole
+    // bunch's charge (which for a diffuse bunch of many particles is
+    // orders of magnitude larger than any one vertex).
+    int e2 = 0;
+    (void)std::frexp(4.0 * total * invVolume, &e2);
+    const int coarse = 62 - e2;
+    R.shift = coarse;
+    R.absolute = 1;
+    ensure(D.maxima, 4);
+    kZeroU64<<<1, 64, 0, s.stream>>>(D.maxima.get(), 4);
+    kChargeDensity<<<ceilDiv(n, 256), 256, 0, s.stream>>>(R);
+    kMaxClearU64<<<kRedBlocks, kRedThreads, 0, s.stream>>>(D.acc.get(), nv, D.maxima.get());
+    launchCheck("charge bound");
+    unsigned long long peak = 0ull;
+    hipCheck(hipMemcpyAsync(&peak, D.maxima.get(), sizeof(peak), hipMemcpyDeviceToHost, s.stream),
+             "hipMemcpyAsync");
+    hipCheck(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
+    // Each tap rounds by at most half a coarse quantum: negligible beside
+    // the peak (the taps of a particle sum to at least its |charge|, so the
+    // peak holds at least 2^62 / (8 vertices) quanta), and covered by the
+    // factor 2 of headroom below.
+    const double bound = std::ldexp(static_cast<double>(peak) + 1.0, -coarse);
+    (void)std::frexp(2.0 * bound, &e2);
+    shift = 62 - e2;
+  }
+  R.shift = shift;
+  R.absolute = 0;
   if (n > 0) kChargeDensity<<<ceilDiv(n, 256), 256, 0, s.stream>>>(R);
   kFixedToDouble<<<2048, 256, 0, s.stream>>>(D.acc.get(), rho.get(), nv, shift);
   launchCheck("charge density");
@@ -453,18 +503,17 @@ void chargeDensitySplit(FieldState32& s, int shape, const int* const cell[3], co
 
 template <class R>
 void continuityResidual(FieldStateT<R>& s, const DeviceArray<double>& rho0, const DeviceArray<double>& rho1,
-                        double out[2]) {
+                        double out[3]) {
   DiagnosticScratch& D = s.diag;
   ensure(D.maxima, 4);
   kZeroU64<<<1, 64, 0, s.stream>>>(D.maxima.get(), 4);
   kContinuity<<<2048, 256, 0, s.stream>>>(s.ptrs(), rho0.get(), rho1.get(), s.g.dt,
                                           make_double3(s.g.dx, s.g.dy, s.g.dz), D.maxima.get());
   launchCheck("continuity");
-  unsigned long long m[2];
+  unsigned long long m[3];
   hipCheck(hipMemcpyAsync(m, D.maxima.get(), sizeof(m), hipMemcpyDeviceToHost, s.stream), "hipMemcpyAsync");
   hipCheck(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
-  std::memcpy(&out[0], &m[0], sizeof(double));
-  std::memcpy(&out[1], &m[1], sizeof(double));
+  std::memcpy(out, m, sizeof(m));
 }
 
 template <class R>

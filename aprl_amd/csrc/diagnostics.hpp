@@ -32,10 +32,11 @@ void chargeDensity(FieldState& s, int shape, const double* const x[3], const dou
                    const ChargeWeights& cw, DeviceArray<double>& rho);
 void chargeDensitySplit(FieldState32& s, int shape, const int* const cell[3], const float* const off[3],
                         const float* q, int n, const int oc[3], const ChargeWeights& cw, DeviceArray<double>& rho);
-// out = {max |(rho1 - rho0)/dt + div J|, max |(rho1 - rho0)/dt|}.
+// out = {max |(rho1 - rho0)/dt + div J|, max |(rho1 - rho0)/dt|,
+// max max(|rho0|, |rho1|)/dt}.
 template <class R>
 void continuityResidual(FieldStateT<R>& s, const DeviceArray<double>& rho0, const DeviceArray<double>& rho1,
-                        double out[2]);
+                        double out[3]);
 // out = {sum error^2, sum source^2, max |error|, max |source|} over the
 // vertices strictly inside [lo, hi], error = div E - rho / eps0.
 template <class R>

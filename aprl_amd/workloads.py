@@ -139,13 +139,15 @@ DRIFT_C1 = Workload(
 # Single-GPU capacity: 6.4e7 macro-particles of a seeded 6D Gaussian (lab
 # gamma 500, sigma_x = sigma_y = 40 um, sigma_z = 30 um) in 256 x 256 x 2048
 # with 16 transverse CPML cells, the planar undulator line of the FEL
-# segment, quintic shape, FP32-split. gamma_b = 6 stretches the bunch to a
-# box-frame sigma_z of 0.36 mm, so that +-4 sigma fills 70% of the 4.1 mm
-# window; the 4 um transverse cells put the CPML 11 sigma from the axis.
+# segment, quintic shape, FP32-split. gamma_b = 4 stretches the bunch to a
+# box-frame sigma_z of 0.24 mm (118 cells): the farthest of 6.4e7 Gaussian
+# draws (about 5.8 sigma, 685 cells) stays inside the 2048-cell window
+# centred on the bunch; the 4 um transverse cells put the CPML 11 sigma
+# from the axis.
 CAPACITY_C5 = Workload(
     name="capacity 256x256x2048 undulator, 6.4e7 e-",
     dims=(256, 256, 2048), spacing=(4.0e-6, 4.0e-6, 2.0e-6), cpml=16, particles=64_000_000,
-    gamma_lab=500.0, gamma_boost=6.0, sigma=(40e-6, 40e-6, 30e-6), sigma_gamma=1e-4, emittance=1e-6,
+    gamma_lab=500.0, gamma_boost=4.0, sigma=(40e-6, 40e-6, 30e-6), sigma_gamma=1e-4, emittance=1e-6,
     charge_electrons=64_000_000 * 355.0, dipole_tesla=0.0, dipole_length=0.0, seed=505, z_cells=1024.0,
     undulator=(1.0, 0.02, 6, 100, 0.5))
 
@@ -366,12 +368,15 @@ class Stepper:
         self.origin = np.zeros(3)
         self.time = 0.0
         self.trials = []
-        # (max |(rho1 - rho0)/dt + div J|, max |(rho1 - rho0)/dt|) of every
-        # step, read before the shift clears J
+        # (max |(rho1 - rho0)/dt + div J|, max |(rho1 - rho0)/dt|,
+        # max |rho|/dt) of every step, read before the shift clears J
         self.track_continuity = continuity
         self.continuity = []
 
-    def step(self):
+    def step(self, shift: bool = True):
+        """One step; shift=False stops after the coupled source step (J of
+        the step still on the grid, the window not yet moved) and leaves
+        the stepper there."""
         f, win = self.fields, self.window
         n = self.step_index
         f.clearCurrent()
@@ -382,6 +387,8 @@ class Stepper:
         self.trials.append(its)
         if self.track_continuity:
             self.continuity.append(self.coupling.continuity(self.store))
+        if not shift:
+            return
         self.offset -= 1
         win.shift(n + 1, self.offset)
         # Simulation::finishMovingWindowStep: origin = fma(offset, dz, ref)

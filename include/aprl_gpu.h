@@ -402,9 +402,13 @@ aprl_status aprl_fields_face_power(aprl_fields* fields, const size_t* lo, const 
 /* ChargeConservingCurrentDepositor::maxContinuityResidual (eb_deposition.cpp:
  * 195-226) of the last aprl_coupling_step of this store, before the window
  * shift clears J: out[0] = max over vertices of |(rho1 - rho0)/dt + div J|,
- * out[1] = max |(rho1 - rho0)/dt|, with rho the charge of the coupling's
- * shape at the step's start and end positions (binomial3-x: the shaped
- * charge of verification/transverse_binomial_checks.cpp:33-40). */
+ * out[1] = max |(rho1 - rho0)/dt|, out[2] = max max(|rho0|, |rho1|)/dt
+ * (the scale of the residual's round-off: rho and div J are each summed
+ * from many taps, so a bunch whose charge barely moves has a residual at
+ * round-off of rho/dt, far above round-off of the rate), with rho the
+ * charge of the coupling's shape at the step's start and end positions
+ * (binomial3-x: the shaped charge of
+ * verification/transverse_binomial_checks.cpp:33-40). */
 aprl_status aprl_coupling_continuity(aprl_coupling* coupling, aprl_particles* particles, double* out);
 /* Gauss residual of eb_initial_field.cpp:774-799 over the vertices strictly
  * inside [lo, hi] (hi <= (nx, ny, nz)) with the shaped charge of the particles at their current

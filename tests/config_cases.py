@@ -43,7 +43,7 @@ class OracleStepper:
         self.trials = []
         self.zero = [np.zeros(int(np.prod(H.component_shape(*w.dims, a)[1:]))) for a in range(6)]
 
-    def step(self):
+    def step(self, shift: bool = True):
         o, z = self.orc, self.zero
         o.clear_current()
         o.window_begin(self.n, self.offset, z[:3])
@@ -51,6 +51,8 @@ class OracleStepper:
         o.window_set_inlet(self.offset, self.n + 1, z[0], z[1])
         its, _, _ = self.co.step(self.origin, self.time, getattr(self.w, "substeps", 1))
         self.trials.append(its)
+        if not shift:
+            return
         self.offset -= 1
         o.window_shift(self.n + 1, self.offset, z)
         self.origin[2] = (self.offset - 10_000) * self.w.spacing[2]
@@ -65,6 +67,28 @@ def rel_l2(a, b):
 
 def field_errors(gpu, orc):
     return {a: rel_l2(gpu.component(a), orc.view(a)) for a in range(9)}
+
+
+def vector_errors(gpu, orc):
+    """Relative L2 error of E, B and J as vector fields: |test - ref| over
+    the three components together, divided by |ref| of the three (a
+    component that is small beside the others, such as Bz of a
+    relativistic bunch, cancels in the curl that forms it and carries
+    the absolute round-off of the larger ones)."""
+    out = {}
+    for name, comps in (("E", (0, 1, 2)), ("B", (3, 4, 5)), ("J", (6, 7, 8))):
+        num = sum(float(np.sum((gpu.component(a) - orc.view(a)) ** 2)) for a in comps)
+        den = sum(float(np.sum(orc.view(a) ** 2)) for a in comps)
+        out[name] = float(np.sqrt(num / den)) if den > 0 else float(np.sqrt(num))
+    return out
+
+
+def continuity_ok(record, tol):
+    """A step's continuity record (residual, rate, rho scale) within tol
+    of the charge-density scale max |rho|/dt: the residual's round-off is
+    that of rho and div J, each summed from many taps."""
+    residual, rate, scale = record
+    return rate > 0 and residual <= tol * scale
 
 
 def particle_errors(store, co):

@@ -14,10 +14,18 @@
   properties: charge continuity of every step at the form's round-off,
   run-to-run bitwise determinism, FP32-split against plain FP64.
 
-Tolerances: FP64 1e-12 (the tile-ordered current sum differs from the
-oracle's particle-ordered one by round-off); FP32-split fields 1e-4 and
-momenta 1e-5 relative L2 against FP64 after a few steps; continuity 1e-12
-(FP64) and 1e-5 (FP32-split) of the charge rate.
+The last step of an oracle comparison stops before the window shift, so
+that the step's current J is still on both grids and is compared too.
+
+Tolerances: FP64 1e-12 relative L2 of E, B and J as vector fields (the
+tile-ordered current sum differs from the oracle's particle-ordered one by
+round-off; every component's error is printed); FP32-split fields 1e-4 and
+momenta 1e-5 relative L2 against FP64 after a few steps. Charge
+continuity: max |(rho1 - rho0)/dt + div J| within 1e-13 (FP64) or 1e-6
+(FP32-split: J is stored in FP32) of max |rho|/dt, the scale of the
+residual's round-off (rho and div J are sums of many taps; a bunch whose
+density barely changes in one step, such as the flat-top FEL slice, has a
+rate far below that scale).
 """
 import hashlib
 
@@ -32,6 +40,18 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 FIELD_TOL32 = 1e-4
 U_TOL32 = 1e-5
+CONT64 = 1e-13
+CONT32 = 1e-6
+
+
+def cont_tol(precision):
+    return CONT32 if precision == "fp32-split" else CONT64
+
+
+def check_continuity(records, precision, label=""):
+    for r in records:
+        print(f"{label} continuity residual/rate {r[0] / r[1]:.2e} residual/(rho/dt) {r[0] / r[2]:.2e}", flush=True)
+        assert CC.continuity_ok(r, cont_tol(precision)), (label, r)
 
 
 def step_both(w, shape, steps, precision="fp64"):
@@ -44,24 +64,30 @@ def step_both(w, shape, steps, precision="fp64"):
     os_ = CC.OracleStepper(w, orc, co)
     for n in range(steps):
         print(f"{w.name} {shape} {precision} step {n}", flush=True)
-        os_.step()
-        gs.step()
+        last = n == steps - 1
+        os_.step(shift=not last)
+        gs.step(shift=not last)
     return orc, co, fields, store, gs, os_
+
+
+def print_errors(label, fields, orc):
+    errs = CC.field_errors(fields, orc)
+    vec = CC.vector_errors(fields, orc)
+    print(label, {a: f"{e:.2e}" for a, e in errs.items()}, {k: f"{e:.2e}" for k, e in vec.items()}, flush=True)
+    return errs, vec
 
 
 def test_c1_generated_gaussian_drift_matches_oracle():
     w = W.DRIFT_C1
     orc, co, fields, store, gs, os_ = step_both(w, "linear-binomial3", 6)
     assert gs.trials == os_.trials, (gs.trials, os_.trials)
-    errs = CC.field_errors(fields, orc)
-    print({a: f"{e:.2e}" for a, e in errs.items()})
-    assert max(errs.values()) <= TOL, errs
+    errs, vec = print_errors("C1", fields, orc)
+    assert max(vec.values()) <= TOL, vec
     for h in range(12):
         assert CC.rel_l2(fields.checkpointHistory(h), orc.history(h)) <= TOL, h
     ex, eu = CC.particle_errors(store, co)
     assert ex <= TOL and eu <= TOL, (ex, eu)
-    for residual, rate in gs.continuity:
-        assert rate > 0 and residual <= TOL * rate, (residual, rate)
+    check_continuity(gs.continuity, "fp64", "C1")
 
 
 @pytest.mark.parametrize("shape", ["linear-binomial3", "quintic-compensated"])
@@ -69,13 +95,13 @@ def test_c2_bending_radiation_fp64_matches_oracle(shape):
     w = W.BENDING
     orc, co, fields, store, gs, os_ = step_both(w, shape, 2)
     assert gs.trials == os_.trials, (gs.trials, os_.trials)
-    errs = CC.field_errors(fields, orc)
-    print(shape, {a: f"{e:.2e}" for a, e in errs.items()})
-    assert max(errs.values()) <= TOL, errs
+    errs, vec = print_errors(shape, fields, orc)
+    assert max(vec.values()) <= TOL, vec
+    for h in range(12):
+        assert CC.rel_l2(fields.checkpointHistory(h), orc.history(h)) <= TOL, h
     ex, eu = CC.particle_errors(store, co)
     assert ex <= TOL and eu <= TOL, (ex, eu)
-    for residual, rate in gs.continuity:
-        assert rate > 0 and residual <= TOL * rate, (residual, rate)
+    check_continuity(gs.continuity, "fp64", shape)
 
 
 @pytest.mark.parametrize("shape", ["linear-binomial3", "quintic-compensated"])
@@ -83,15 +109,12 @@ def test_c2_bending_radiation_fp32_split_tracks_fp64_oracle(shape):
     w = W.BENDING
     orc, co, fields, store, gs, os_ = step_both(w, shape, 2, precision="fp32-split")
     assert all(t == 2 for t in gs.trials), gs.trials
-    errs = CC.field_errors(fields, orc)
-    print(shape, {a: f"{e:.2e}" for a, e in errs.items()})
-    # components that carry the bunch's field (J, E, B) to FP32 tolerance
+    errs, vec = print_errors(shape, fields, orc)
     for a in range(9):
         assert errs[a] <= FIELD_TOL32, (a, errs)
     ex, eu = CC.particle_errors(store, co)
     assert eu <= U_TOL32 and ex <= 1e-6, (ex, eu)
-    for residual, rate in gs.continuity:
-        assert rate > 0 and residual <= 1e-5 * rate, (residual, rate)
+    check_continuity(gs.continuity, "fp32-split", shape)
 
 
 @pytest.mark.parametrize("precision,steps", [("fp32-split", 2), ("fp64", 1)])
@@ -116,9 +139,7 @@ def test_c4_batched_scan_all_instances_equal_single_steps(precision, steps):
         x1, u1, *_r1, id1 = st1.download()
         x2, u2, *_r2, id2 = st2.download()
         assert np.array_equal(id1, id2) and np.array_equal(x1, x2) and np.array_equal(u1, u2), (precision, i)
-        tol = 1e-5 if precision == "fp32-split" else TOL
-        for residual, rate in s.continuity:
-            assert rate > 0 and residual <= tol * rate, (i, residual, rate)
+        check_continuity(s.continuity, precision, f"instance {i}")
         del f2, win2, cg2, st2, s
 
 
@@ -147,9 +168,7 @@ def test_c3_fel_segment_full_size_properties():
         for n in range(2):
             print(f"FEL {precision} step {n}", flush=True)
             s.step()
-        tol = 1e-5 if precision == "fp32-split" else TOL
-        for residual, rate in s.continuity:
-            assert rate > 0 and residual <= tol * rate, (precision, residual, rate)
+        check_continuity(s.continuity, precision, f"FEL {precision}")
         assert len(st) == w.particles
         if precision in runs:
             assert _digest(f, st) == runs[precision][0], "FP32-split step is not repeatable"
@@ -182,9 +201,8 @@ def test_c5_capacity_full_size_properties():
         cg.setSplitTrials(2)
         s = W.Stepper(w, f, win, cg, st, continuity=True)
         s.step()
-        residual, rate = s.continuity[0]
-        print("continuity", residual, rate, "tiles", cg.last_tiles, "chunks", cg.last_chunks, flush=True)
-        assert rate > 0 and residual <= 1e-5 * rate, (residual, rate)
+        print("tiles", cg.last_tiles, "chunks", cg.last_chunks, flush=True)
+        check_continuity(s.continuity, "fp32-split", "capacity")
         assert len(st) == w.particles
         h = hashlib.sha256()
         for a in (2, 5, 6, 7, 8):  # Ez, Bz and J: the step's current and what it drove
