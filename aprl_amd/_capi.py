@@ -45,7 +45,7 @@ class Boost(C.Structure):
 
 class StepReport(C.Structure):
     _fields_ = [("iterations", C.c_uint), ("force", C.c_double), ("work", C.c_double), ("tiles", C.c_uint),
-                ("chunks", C.c_uint)]
+                ("chunks", C.c_uint), ("active_tiles", C.c_uint)]
 
 
 class InstanceStep(C.Structure):

@@ -179,7 +179,7 @@ class ConservativeSourceCoupling:
         o = np.ascontiguousarray(origin, dtype=np.float64)
         r = _capi.StepReport()
         check(lib().aprl_coupling_step(self.handle, particles.handle, dptr(o), time, substeps, C.byref(r)))
-        self.last_tiles, self.last_chunks = r.tiles, r.chunks
+        self.last_tiles, self.last_chunks, self.last_active_tiles = r.tiles, r.chunks, r.active_tiles
         return r.iterations, r.force, r.work
 
     def particleEnergy(self, particles: ParticleStore, time):

@@ -1397,6 +1397,7 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
     report->work = work;
     report->tiles = occupiedTiles;
     report->chunks = nChunks;
+    report->active_tiles = nActive;
   }
 }
 

@@ -1029,7 +1029,7 @@ struct SplitStepContext {
   bool pending = false, middleDone = false, verify = false;
   int n = 0, ntx = 0, nty = 0, ntz = 0, ntiles = 0;
   std::size_t chunkBound = 0;
-  unsigned trials = 2, nChunks = 0, occupied = 0;
+  unsigned trials = 2, nChunks = 0, occupied = 0, active = 0;
   GArgs32 G{};
   StepArgs S{};
   double o[3] = {0.0, 0.0, 0.0};
@@ -1253,6 +1253,7 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
   X.verify = verify;
   X.nChunks = nChunks;
   X.occupied = occupiedTiles;
+  X.active = nActive;
   X.middleDone = true;
   for (int a = 0; a < 3; ++a) P.cell[a].swap(P.endCell[a]);
   for (int a = 0; a < 6; ++a) P.s[a].swap(P.end[a]);
@@ -1292,6 +1293,7 @@ void splitStepEnd(CouplingState& C, SplitStore& P, aprl_step_report* report) {
     report->work = work;
     report->tiles = X.occupied;
     report->chunks = nChunks;
+    report->active_tiles = X.active;
   }
 }
 

@@ -43,15 +43,17 @@ MEASURED_TRAFFIC_FP64 = {
 }
 
 
-def kernel_bytes(w, name, tiles, chunks):
+def kernel_bytes(w, name, tiles, chunks, active=0):
     """Algorithmic HBM bytes of one launch: what the kernel must read or
     write at least once. Fields: 9 words/cell magnetic (read 3 E, read and
     write 3 B), 12 words/cell electric (read 3 B, 3 J, read and write 3 E),
-    plus 2 words per active CPML history entry. Particles: push 14
-    words/particle (read x, u, q, m, 6 fields; write x, u); gathers 14
-    words/particle plus 9 words per cell of each occupied 8x8x4 tile (E*,
-    E_old, B); deposit 7 words/particle (x0, x1, q) plus 3 words per node of
-    each occupied tile's 13x11x9 extended box."""
+    plus 2 words per active CPML history entry; the zero-current preview
+    12 words per node of each active 8x8x4 tile (read 3 B, 3 E; write E*
+    and E* + E_old). Particles: push 14 words/particle (read x, u, q, m, 6
+    fields; write x, u); gathers 14 words/particle plus 9 words per cell of
+    each occupied 8x8x4 tile (E*, E_old, B); deposit 7 words/particle (x0,
+    x1, q) plus 3 words per node of each occupied tile's 13x11x9 extended
+    box."""
     cells = w.cells
     n = w.particles
     cp = w.cpml
@@ -64,7 +66,7 @@ def kernel_bytes(w, name, tiles, chunks):
     if name == "electric":
         return WORD * (12 * cells + 2 * psi)
     if name == "electric_preview":
-        return WORD * (9 * cells + psi)
+        return WORD * 12 * tile_cells * active
     if name == "push":
         return WORD * 14 * n
     if name in ("gather", "residual"):
@@ -88,8 +90,9 @@ def run_gpu(args, shape="linear-binomial3", profile=True, precision="fp64"):
     fields.synchronize()
     elapsed = time.perf_counter() - t0
     trials = s.trials[args.warmup:]
+    counts = (coupling.last_tiles, coupling.last_chunks, coupling.last_active_tiles)
     if not profile:
-        return w, elapsed, trials, {}, (coupling.last_tiles, coupling.last_chunks)
+        return w, elapsed, trials, {}, counts
     # Per-kernel HIP-event durations (on the library's own stream) over a
     # further profiled window of the same steps.
     fields.profile(True)
@@ -104,8 +107,7 @@ def run_gpu(args, shape="linear-binomial3", profile=True, precision="fp64"):
         if launches:
             kernels[name] = {"launches": launches, "avg_us": 1e3 * ms / launches, "total_ms": ms}
     fields.profile(False)
-    x, u, q, m, wt, ids = store.download()
-    return w, elapsed, trials, kernels, (coupling.last_tiles, coupling.last_chunks)
+    return w, elapsed, trials, kernels, counts
 
 
 def run_batch(count=48, steps=20, warmup=2):
@@ -213,20 +215,75 @@ def error_study(horizons=(100, 1000, 10000), shape="linear-binomial3"):
     return {"workload": w.name, "shape": shape, "plane_k": k0, "rows": rows}
 
 
-def run_cpu_baseline(w, seconds_budget=25.0):
-    """The reference's own compiled hot path (oracle/_ref, OpenMP on the
-    host cores) on the same workload for a bounded number of steps."""
+def host_cores():
+    """Host CPU facts for the baseline record: physical cores and sockets
+    (lscpu -p=CORE,SOCKET), the CPUs this process may run on, the model
+    name, and the thread count used: every physical core this process may
+    use, capped by OMP_NUM_THREADS where the environment sets it."""
+    import subprocess
+    info = {"affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, timeout=20).stdout
+        pairs = {tuple(line.split(",")[:2]) for line in out.splitlines() if line and not line.startswith("#")}
+        info["physical_cores"] = len(pairs)
+        info["sockets"] = len({p[1] for p in pairs})
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout.splitlines():
+            if line.startswith("Model name:"):
+                info["model"] = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    threads = min(info.get("physical_cores", info["affinity_cpus"]), info["affinity_cpus"])
+    cap = os.environ.get("OMP_NUM_THREADS")
+    if cap and cap.isdigit() and int(cap) > 0:
+        info["omp_num_threads_env"] = int(cap)
+        threads = min(threads, int(cap))
+    info["threads"] = max(1, threads)
+    return info
+
+
+def run_cpu_baseline(warmup=20, max_timed=40, seconds_budget=20.0):
+    """The reference's own compiled hot path (oracle/_ref: its eb_field,
+    eb_cpml, moving_window_fields, source_coupling and longitudinal_shape
+    translation units with OpenMP) on the bending-radiation workload, in a
+    child process whose OpenMP runtime starts with OMP_NUM_THREADS = the
+    thread count of host_cores() and OMP_PROC_BIND=close, OMP_PLACES=cores.
+    Timed over steps `warmup` -> N (the reference's own convention,
+    verification/performance/cases/memory_scaling/README.md:43): N stops
+    at `max_timed` timed steps or when the timed steps pass the budget."""
+    import subprocess
+    info = host_cores()
+    env = dict(os.environ)
+    env.update({"OMP_NUM_THREADS": str(info["threads"]), "OMP_PROC_BIND": "close", "OMP_PLACES": "cores"})
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", str(warmup), str(max_timed),
+           str(seconds_budget)]
+    r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600)
+    if r.returncode != 0:
+        print(f"cpu baseline failed: {r.stderr[-2000:]}", file=sys.stderr, flush=True)
+        return None
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    if out is None:
+        return None
+    out["host"] = info
+    return out
+
+
+def cpu_baseline_child(warmup, max_timed, seconds_budget):
     import ctypes as C
+    import statistics
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import harness as H
+    from aprl_amd import workloads
     if not H.have_ref():
-        return None
+        print(json.dumps(None))
+        return
+    w = workloads.BENDING
+    threads = int(os.environ["OMP_NUM_THREADS"])
     lib = H.load_particles("ref")
-    threads = min(os.cpu_count() or 1, 16)
     lib.ref_set_threads.argtypes = [C.c_int]
     if lib.ref_set_threads(threads):
-        return None
+        print(json.dumps(None))
+        return
     nx, ny, nz = w.dims
     dx, dy, dz = w.spacing
     f = H.CpuFields("ref", nx, ny, nz, dx, dy, dz, w.dt, 1)
@@ -253,25 +310,78 @@ def run_cpu_baseline(w, seconds_budget=25.0):
         offset -= 1
         f.window_shift(step + 1, offset, zero6)
         origin[2] = (offset - 10_000) * dz
-        times.append(time.perf_counter() - t0)
+        dt_step = time.perf_counter() - t0
+        if step >= warmup:
+            times.append(dt_step)
         step += 1
-        if step >= 2 and (sum(times[1:]) > seconds_budget or step >= 40):
+        if len(times) >= 3 and (sum(times) > seconds_budget or len(times) >= max_timed):
             break
-    timed = times[1:]  # the first step is warm-up
-    per_step = sum(timed) / len(timed)
-    return {"value": w.particles / per_step, "unit": "particle-steps/s", "cores": threads, "kind": "reference",
-            "cell_updates_per_s": w.cells / per_step,
-            "sample": f"{len(timed)} steps of the same workload after 1 warm-up step, reference TUs "
-                      f"(eb_field, eb_cpml, moving_window_fields, source_coupling, longitudinal_shape) "
-                      f"with OpenMP on {threads} threads"}
+    per_step = sum(times) / len(times)
+    third = len(times) // 3
+    segments = [sum(times[i * third:(i + 1) * third]) / third for i in range(3)]
+    print(json.dumps({
+        "value": w.particles / per_step, "unit": "particle-steps/s", "cores": threads, "kind": "reference",
+        "cell_updates_per_s": w.cells / per_step, "ms_per_step": 1e3 * per_step,
+        "ms_per_step_median_of_3": 1e3 * statistics.median(segments),
+        "sample": f"steps {warmup} -> {warmup + len(times)} of the bending-radiation workload (64x64x512, 1e5 e-, "
+                  f"linear-binomial3 + x-binomial3, FP64), the first {warmup} untimed; reference TUs (eb_field, "
+                  f"eb_cpml, moving_window_fields, source_coupling, longitudinal_shape) with OpenMP on {threads} "
+                  f"threads, OMP_PROC_BIND=close, OMP_PLACES=cores"}))
+
+
+def run_large(w, steps=3, warmup=1, data=None):
+    """The FEL segment (128x128x1024, 4e6 quiet-start macro-particles) or the
+    capacity case (256x256x2048, 6.4e7 macro-particles drawn on the device)
+    in the FP32-split form with the quintic shape: `warmup` untimed steps,
+    then `steps` timed ones (a shortened run of the configurations' 20000
+    and 2000 steps)."""
+    from aprl_amd import workloads
+    fields, window, coupling, store = workloads.setup_gpu(w, "quintic-compensated", "fp32-split", data=data)
+    coupling.setSplitTrials(2)
+    s = workloads.Stepper(w, fields, window, coupling, store)
+    for _ in range(warmup):
+        s.step()
+    fields.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step()
+    fields.synchronize()
+    elapsed = time.perf_counter() - t0
+    out = {"workload": w.name, "particles": int(w.particles), "grid": list(w.dims), "form": "FP32-split",
+           "shape": "quintic-compensated", "steps": steps, "warmup": warmup,
+           "value": w.particles * steps / elapsed, "unit": "particle-steps/s",
+           "cell_updates_per_s": w.cells * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+           "occupied_tiles": coupling.last_tiles, "chunks": coupling.last_chunks}
+    print(f"{w.name}: {1e3 * elapsed / steps:.1f} ms per step", file=sys.stderr, flush=True)
+    del s, fields, window, coupling, store
+    return out
+
+
+def step_roofline(w, ms, word=WORD):
+    """Whole-step bytes against HBM peak: 21 words per cell-update (B: read
+    3 E, read and write 3 B; E: read 3 B, 3 J, read and write 3 E) plus 2
+    words per CPML history entry of each update, and 14 words per
+    particle-step (read x, u, q, m; write x, u)."""
+    nx, ny, nz = w.dims
+    psi = 2 * w.cpml * (ny + nx) * nz * 2
+    bytes_cells = word * (21 * w.cells + 2 * 2 * psi)
+    bytes_particles = word * 14 * w.particles
+    achieved = (bytes_cells + bytes_particles) / (ms * 1e-3) / 1e9
+    return {"bytes_per_step": bytes_cells + bytes_particles, "bytes_per_cell_update": 21 * word,
+            "bytes_per_particle_step": 14 * word, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4)}
 
 
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--cpu-baseline-child":
+        cpu_baseline_child(int(sys.argv[2]), int(sys.argv[3]), float(sys.argv[4]))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-large", action="store_true", help="skip the FEL segment and capacity legs")
     ap.add_argument("--leg", choices=("all", "fp64", "fp32-split"), default="all",
                     help="time only this form's binomial3-x leg (for profiling runs)")
     ap.add_argument("--error-study", action="store_true",
@@ -298,12 +408,12 @@ def main():
         print(json.dumps({"error_study": error_study()}))
         return
     if args.leg != "all":
-        w, elapsed, trials, _, (tiles, chunks) = run_gpu(args, profile=False, precision=args.leg)
+        w, elapsed, trials, _, (tiles, chunks, active) = run_gpu(args, profile=False, precision=args.leg)
         print(json.dumps({"leg": args.leg, "value": w.particles * args.steps / elapsed, "unit": "particle-steps/s",
                           "ms_per_step": 1e3 * elapsed / args.steps, "steps": args.steps, "warmup": args.warmup,
-                          "occupied_tiles": tiles, "chunks": chunks}))
+                          "occupied_tiles": tiles, "chunks": chunks, "active_tiles": active}))
         return
-    w, elapsed, trials, kernels, (tiles, chunks) = run_gpu(args)
+    w, elapsed, trials, kernels, (tiles, chunks, active) = run_gpu(args)
     # the other production shape of the same workload, timed the same way
     _, q_elapsed, q_trials, _, _ = run_gpu(args, "quintic-compensated", profile=False)
     # the FP32-split form of both shapes
@@ -311,21 +421,26 @@ def main():
     _, sq_elapsed, _, _, _ = run_gpu(args, "quintic-compensated", profile=False, precision="fp32-split")
     s_error = split_error()
     batch = run_batch()
+    large = None
+    if not args.no_large:
+        from aprl_amd import workloads
+        large = {"fel_segment": run_large(workloads.FEL_C3),
+                 "capacity": run_large(workloads.CAPACITY_C5, steps=2, data="device")}
     ms = 1e3 * elapsed / args.steps
     value = w.particles * args.steps / elapsed
-    modeled = {k: v for k, v in kernels.items() if kernel_bytes(w, k, tiles, chunks) is not None}
+    nbytes = {k: kernel_bytes(w, k, tiles, chunks, active) for k in kernels}
+    modeled = {k: v for k, v in kernels.items() if nbytes[k] is not None}
     dominant = max(modeled.items(), key=lambda kv: kv[1]["total_ms"])[0] if modeled else None
     roofline = None
     if dominant:
         k = kernels[dominant]
-        b = kernel_bytes(w, dominant, tiles, chunks)
-        if b is not None:
-            achieved = b / (k["avg_us"] * 1e-6) / 1e9
-            roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                        "traffic": MEASURED_TRAFFIC_FP64.get(dominant), "algorithmic_bytes": b,
-                        "avg_launch_us": round(k["avg_us"], 2)}
-    cpu = None if args.no_cpu_baseline else run_cpu_baseline(w)
+        b = nbytes[dominant]
+        achieved = b / (k["avg_us"] * 1e-6) / 1e9
+        roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": MEASURED_TRAFFIC_FP64.get(dominant), "algorithmic_bytes": b,
+                    "avg_launch_us": round(k["avg_us"], 2)}
+    cpu = None if args.no_cpu_baseline else run_cpu_baseline()
     line = {
         "metric": "particle-steps/s + cell-updates/s, FP32-split vs FP64, % of HBM peak",
         "value": value,
@@ -342,7 +457,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": w.name, "grid": list(w.dims), "particles": w.particles, "form": "plain FP64",
                    "shape": "linear-binomial3 + x-binomial3", "picard_trials_mean": sum(trials) / len(trials),
-                   "occupied_tiles": tiles, "chunks": chunks},
+                   "occupied_tiles": tiles, "active_tiles": active, "chunks": chunks},
+        "step_roofline": step_roofline(w, ms),
         "quintic": {"value": w.particles * args.steps / q_elapsed, "ms_per_step": 1e3 * q_elapsed / args.steps,
                     "picard_trials_mean": sum(q_trials) / len(q_trials)},
         "fp32_split": {"value": w.particles * args.steps / s_elapsed, "ms_per_step": 1e3 * s_elapsed / args.steps,
@@ -350,13 +466,14 @@ def main():
                        "quintic_value": w.particles * args.steps / sq_elapsed,
                        "quintic_vs_fp64": round(q_elapsed / sq_elapsed, 3),
                        "error_vs_fp64": s_error,
+                       "step_roofline": step_roofline(w, 1e3 * s_elapsed / args.steps, word=4),
                        "kernels_us": {k: round(v["avg_us"], 2) for k, v in s_kernels.items()}},
         "batch": batch,
+        "large": large,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels_us": {k: round(v["avg_us"], 2) for k, v in kernels.items()},
-        "kernels_gbs": {k: round(kernel_bytes(w, k, tiles, chunks) / (v["avg_us"] * 1e-6) / 1e9, 1)
-                        for k, v in modeled.items()},
+        "kernels_gbs": {k: round(nbytes[k] / (v["avg_us"] * 1e-6) / 1e9, 1) for k, v in modeled.items()},
         "kernels_traffic_gbs": {k: round(MEASURED_TRAFFIC_FP64[k] / (v["avg_us"] * 1e-6) / 1e9, 1)
                                 for k, v in kernels.items() if k in MEASURED_TRAFFIC_FP64},
     }

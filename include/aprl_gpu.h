@@ -226,6 +226,9 @@ typedef struct aprl_step_report {
   double work;         /* relative fixed-point work defect */
   unsigned tiles;      /* occupied cell tiles (8x8x4 cells) of the step */
   unsigned chunks;     /* particle work chunks (at most 128 particles of one tile) */
+  unsigned active_tiles; /* tiles whose nodes the preview and the current reduction
+                            touch: an occupied tile within one tile in x and y and
+                            the shape's reach in z */
 } aprl_step_report;
 
 /* ConservativeSourceCoupling (source_coupling.h:12-41) bound to a field
