@@ -174,6 +174,7 @@ def lib() -> C.CDLL:
     L.aprl_fields_interior_energy.argtypes = [vp, C.POINTER(_sz), C.POINTER(_sz), _dp]
     L.aprl_fields_face_power.argtypes = [vp, C.POINTER(_sz), C.POINTER(_sz), _dp]
     L.aprl_coupling_continuity.argtypes = [vp, vp, _dp]
+    L.aprl_coupling_continuity_noalloc.argtypes = [vp, vp, _dp]
     L.aprl_coupling_gauss.argtypes = [vp, vp, _dp, C.POINTER(_sz), C.POINTER(_sz), _dp]
     _lib = L
     return L
@@ -226,5 +227,6 @@ EXPORTED = [
     "aprl_coupling_create", "aprl_coupling_destroy", "aprl_coupling_set_frame",
     "aprl_coupling_add_magnet", "aprl_coupling_set_particle_boundary", "aprl_coupling_step",
     "aprl_coupling_set_split_trials", "aprl_window_step_batch",
-    "aprl_fields_interior_energy", "aprl_fields_face_power", "aprl_coupling_continuity", "aprl_coupling_gauss",
+    "aprl_fields_interior_energy", "aprl_fields_face_power", "aprl_coupling_continuity",
+    "aprl_coupling_continuity_noalloc", "aprl_coupling_gauss",
 ]

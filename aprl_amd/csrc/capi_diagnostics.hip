@@ -53,6 +53,50 @@ void latticeOrigin(const FieldState32& F, const double origin[3], int oc[3]) {
 }
 }  // namespace
 
+namespace {
+// Continuity of the last coupled step of this store in the workspace ws
+// with the charge densities rho0 / rho1 (one double per vertex each).
+void continuityOf(aprl_coupling* coupling, aprl_particles* particles, const RhoWorkspace& ws, double* rho0,
+                  double* rho1, double* out) {
+  CouplingState& C = coupling->c;
+  if (coupling->fields->fp32) {
+    SplitStore& P = particles->sp;
+    if (!C.lastStep.valid || C.lastStep.store != &P || C.lastStep.generation != P.generation)
+      throw std::logic_error("Continuity needs the particle store of the last coupled step");
+    FieldState32& F = *C.fields32;
+    int oc[3];
+    latticeOrigin(F, C.lastStep.origin, oc);
+    const int* cell[3];
+    const float* off[3];
+    const int n = static_cast<int>(P.count);
+    splitPositions(P, true, cell, off);
+    chargeDensitySplit(F, C.shape, cell, off, P.s[6].get(), n, oc, weightsOf(P, 2, P.s[8].get(), P.wend.get()), ws,
+                       rho0);
+    splitPositions(P, false, cell, off);
+    chargeDensitySplit(F, C.shape, cell, off, P.s[6].get(), n, oc, weightsOf(P, 1, P.s[8].get(), P.wend.get()), ws,
+                       rho1);
+    continuityResidual(F, rho0, rho1, ws, out);
+    return;
+  }
+  ParticleStore& P = particles->p;
+  if (!C.lastStep.valid || C.lastStep.store != &P || C.lastStep.generation != P.generation)
+    throw std::logic_error("Continuity needs the particle store of the last coupled step");
+  FieldState& F = *C.fields;
+  const double* x[3];
+  const int n = static_cast<int>(P.count);
+  chargeDensity(F, C.shape, positions(P, true, x), P.s[6].get(), n, C.lastStep.origin,
+                weightsOf(P, 2, P.s[8].get(), P.wend.get()), ws, rho0);
+  chargeDensity(F, C.shape, positions(P, false, x), P.s[6].get(), n, C.lastStep.origin,
+                weightsOf(P, 1, P.s[8].get(), P.wend.get()), ws, rho1);
+  continuityResidual(F, rho0, rho1, ws, out);
+}
+
+template <class R>
+std::size_t vertexCount(const FieldStateT<R>& F) {
+  return (F.g.nx + 1) * (F.g.ny + 1) * (F.g.nz + 1);
+}
+}  // namespace
+
 extern "C" {
 
 aprl_status aprl_fields_interior_energy(aprl_fields* fields, const size_t* lo, const size_t* hi, double* energy) {
@@ -82,41 +126,32 @@ aprl_status aprl_coupling_particle_energy(aprl_coupling* coupling, aprl_particle
   });
 }
 
+
 aprl_status aprl_coupling_continuity(aprl_coupling* coupling, aprl_particles* particles, double* out) {
   return guarded([&] {
     checkCoupled(coupling, particles);
     if (!out) throw std::invalid_argument("Null diagnostic argument");
     CouplingState& C = coupling->c;
-    if (coupling->fields->fp32) {
-      SplitStore& P = particles->sp;
-      if (!C.lastStep.valid || C.lastStep.store != &P || C.lastStep.generation != P.generation)
-        throw std::logic_error("Continuity needs the particle store of the last coupled step");
-      FieldState32& F = *C.fields32;
-      int oc[3];
-      latticeOrigin(F, C.lastStep.origin, oc);
-      const int* cell[3];
-      const float* off[3];
-      const int n = static_cast<int>(P.count);
-      splitPositions(P, true, cell, off);
-      chargeDensitySplit(F, C.shape, cell, off, P.s[6].get(), n, oc, weightsOf(P, 2, P.s[8].get(), P.wend.get()),
-                         C.rho[0]);
-      splitPositions(P, false, cell, off);
-      chargeDensitySplit(F, C.shape, cell, off, P.s[6].get(), n, oc, weightsOf(P, 1, P.s[8].get(), P.wend.get()),
-                         C.rho[1]);
-      continuityResidual(F, C.rho[0], C.rho[1], out);
-      return;
-    }
-    ParticleStore& P = particles->p;
-    if (!C.lastStep.valid || C.lastStep.store != &P || C.lastStep.generation != P.generation)
-      throw std::logic_error("Continuity needs the particle store of the last coupled step");
-    FieldState& F = *C.fields;
-    const double* x[3];
-    const int n = static_cast<int>(P.count);
-    chargeDensity(F, C.shape, positions(P, true, x), P.s[6].get(), n, C.lastStep.origin,
-                  weightsOf(P, 2, P.s[8].get(), P.wend.get()), C.rho[0]);
-    chargeDensity(F, C.shape, positions(P, false, x), P.s[6].get(), n, C.lastStep.origin,
-                  weightsOf(P, 1, P.s[8].get(), P.wend.get()), C.rho[1]);
-    continuityResidual(F, C.rho[0], C.rho[1], out);
+    const bool split = coupling->fields->fp32;
+    const std::size_t nv = split ? vertexCount(*C.fields32) : vertexCount(*C.fields);
+    for (auto& r : C.rho)
+      if (r.size() != nv) r.resize(nv);
+    const RhoWorkspace ws = split ? diagnosticWorkspace(*C.fields32) : diagnosticWorkspace(*C.fields);
+    continuityOf(coupling, particles, ws, C.rho[0].get(), C.rho[1].get(), out);
+  });
+}
+
+aprl_status aprl_coupling_continuity_noalloc(aprl_coupling* coupling, aprl_particles* particles, double* out) {
+  return guarded([&] {
+    checkCoupled(coupling, particles);
+    if (!out) throw std::invalid_argument("Null diagnostic argument");
+    CouplingState& C = coupling->c;
+    const bool split = coupling->fields->fp32;
+    double* rho0 = nullptr;
+    double* rho1 = nullptr;
+    const RhoWorkspace ws = split ? noAllocWorkspace(*C.fields32, vertexCount(*C.fields32), &rho0, &rho1)
+                                  : noAllocWorkspace(*C.fields, vertexCount(*C.fields), &rho0, &rho1);
+    continuityOf(coupling, particles, ws, rho0, rho1, out);
   });
 }
 
@@ -129,21 +164,24 @@ aprl_status aprl_coupling_gauss(aprl_coupling* coupling, aprl_particles* particl
     if (coupling->fields->fp32) {
       SplitStore& P = particles->sp;
       FieldState32& F = *C.fields32;
+      if (C.rho[0].size() != vertexCount(F)) C.rho[0].resize(vertexCount(F));
       int oc[3];
       latticeOrigin(F, origin, oc);
       const int* cell[3];
       const float* off[3];
       splitPositions(P, false, cell, off);
       chargeDensitySplit(F, C.shape, cell, off, P.s[6].get(), static_cast<int>(P.count), oc,
-                         weightsOf(P, 1, P.s[8].get(), P.wend.get()), C.rho[0]);
+                         weightsOf(P, 1, P.s[8].get(), P.wend.get()), diagnosticWorkspace(F), C.rho[0].get());
       gaussResidual(F, C.rho[0], lo, hi, out);
       return;
     }
     ParticleStore& P = particles->p;
+    FieldState& F = *C.fields;
+    if (C.rho[0].size() != vertexCount(F)) C.rho[0].resize(vertexCount(F));
     const double* x[3];
-    chargeDensity(*C.fields, C.shape, positions(P, false, x), P.s[6].get(), static_cast<int>(P.count), origin,
-                  weightsOf(P, 1, P.s[8].get(), P.wend.get()), C.rho[0]);
-    gaussResidual(*C.fields, C.rho[0], lo, hi, out);
+    chargeDensity(F, C.shape, positions(P, false, x), P.s[6].get(), static_cast<int>(P.count), origin,
+                  weightsOf(P, 1, P.s[8].get(), P.wend.get()), diagnosticWorkspace(F), C.rho[0].get());
+    gaussResidual(F, C.rho[0], lo, hi, out);
   });
 }
 

@@ -6,6 +6,9 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <mutex>
+#include <stdexcept>
+#include <string>
 #include <type_traits>
 
 #include "diagnostics.hpp"
@@ -407,25 +410,17 @@ void facePower(FieldStateT<R>& s, const std::size_t lo[3], const std::size_t hi[
 // Shaped charge density of a particle set on the vertices into `rho`; R
 // carries the positions and charges, q the charges for the bound.
 template <class Real, class Q>
-void depositCharge(FieldStateT<Real>& s, RhoArgs R, const Q* q, DeviceArray<double>& rho) {
-  DiagnosticScratch& D = s.diag;
+void depositCharge(FieldStateT<Real>& s, RhoArgs R, const Q* q, const RhoWorkspace& ws, double* rho) {
   const std::size_t nv = (s.g.nx + 1) * (s.g.ny + 1) * (s.g.nz + 1);
-  if (D.acc.size() != nv) {
-    D.acc.resize(nv);
-    hipCheck(hipMemsetAsync(D.acc.get(), 0, nv * sizeof(unsigned long long), s.stream), "hipMemsetAsync");
-  }
-  if (rho.size() != nv) rho.resize(nv);
-  ensure(D.partial, 2 * 6 * kRedBlocks);
-  ensure(D.out, 8);
   // global fixed-point exponent from the total |charge| density bound (each
   // shape's weights sum to at most 4 in absolute value)
   const int n = R.n;
   double total = 0.0;
   if (n > 0) {
-    kSumAbs<<<kRedBlocks, kRedThreads, 0, s.stream>>>(q, n, D.partial.get());
-    kFinish<<<1, kRedThreads, 0, s.stream>>>(D.partial.get(), kRedBlocks, D.out.get());
+    kSumAbs<<<kRedBlocks, kRedThreads, 0, s.stream>>>(q, n, ws.partial);
+    kFinish<<<1, kRedThreads, 0, s.stream>>>(ws.partial, kRedBlocks, ws.out);
     launchCheck("charge bound");
-    hipCheck(hipMemcpyAsync(&total, D.out.get(), sizeof(double), hipMemcpyDeviceToHost, s.stream), "hipMemcpyAsync");
+    hipCheck(hipMemcpyAsync(&total, ws.out, sizeof(double), hipMemcpyDeviceToHost, s.stream), "hipMemcpyAsync");
     hipCheck(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
   }
   const double invVolume = 1.0 / (s.g.dx * s.g.dy * s.g.dz);
@@ -434,7 +429,7 @@ void depositCharge(FieldStateT<Real>& s, RhoArgs R, const Q* q, DeviceArray<doub
   R.ny = static_cast<int>(s.g.ny);
   R.nz = static_cast<int>(s.g.nz);
   R.invVolume = invVolume;
-  R.acc = D.acc.get();
+  R.acc = ws.acc;
   int shift = 0;
   if (n > 0 && total > 0.0) {
     // Bound pass: sum |tap| per vertex on the coarse exponent of the total
@@ -448,14 +443,12 @@ void depositCharge(FieldStateT<Real>& s, RhoArgs R, const Q* q, DeviceArray<doub
     const int coarse = 62 - e2;
     R.shift = coarse;
     R.absolute = 1;
-    ensure(D.maxima, 4);
-    kZeroU64<<<1, 64, 0, s.stream>>>(D.maxima.get(), 4);
+    kZeroU64<<<1, 64, 0, s.stream>>>(ws.maxima, 4);
     kChargeDensity<<<ceilDiv(n, 256), 256, 0, s.stream>>>(R);
-    kMaxClearU64<<<kRedBlocks, kRedThreads, 0, s.stream>>>(D.acc.get(), nv, D.maxima.get());
+    kMaxClearU64<<<kRedBlocks, kRedThreads, 0, s.stream>>>(ws.acc, nv, ws.maxima);
     launchCheck("charge bound");
     unsigned long long peak = 0ull;
-    hipCheck(hipMemcpyAsync(&peak, D.maxima.get(), sizeof(peak), hipMemcpyDeviceToHost, s.stream),
-             "hipMemcpyAsync");
+    hipCheck(hipMemcpyAsync(&peak, ws.maxima, sizeof(peak), hipMemcpyDeviceToHost, s.stream), "hipMemcpyAsync");
     hipCheck(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
     // Each tap rounds by at most half a coarse quantum: negligible beside
     // the peak (the taps of a particle sum to at least its |charge|, so the
@@ -468,12 +461,31 @@ void depositCharge(FieldStateT<Real>& s, RhoArgs R, const Q* q, DeviceArray<doub
   R.shift = shift;
   R.absolute = 0;
   if (n > 0) kChargeDensity<<<ceilDiv(n, 256), 256, 0, s.stream>>>(R);
-  kFixedToDouble<<<2048, 256, 0, s.stream>>>(D.acc.get(), rho.get(), nv, shift);
+  kFixedToDouble<<<2048, 256, 0, s.stream>>>(ws.acc, rho, nv, shift);
   launchCheck("charge density");
 }
 
+template <class R>
+RhoWorkspace diagnosticWorkspace(FieldStateT<R>& s) {
+  DiagnosticScratch& D = s.diag;
+  const std::size_t nv = (s.g.nx + 1) * (s.g.ny + 1) * (s.g.nz + 1);
+  if (D.acc.size() != nv) {
+    D.acc.resize(nv);
+    hipCheck(hipMemsetAsync(D.acc.get(), 0, nv * sizeof(unsigned long long), s.stream), "hipMemsetAsync");
+  }
+  ensure(D.partial, 2 * 6 * kRedBlocks);
+  ensure(D.out, 8);
+  ensure(D.maxima, 4);
+  RhoWorkspace ws;
+  ws.acc = D.acc.get();
+  ws.partial = D.partial.get();
+  ws.out = D.out.get();
+  ws.maxima = D.maxima.get();
+  return ws;
+}
+
 void chargeDensity(FieldState& s, int shape, const double* const x[3], const double* q, int n, const double origin[3],
-                   const ChargeWeights& cw, DeviceArray<double>& rho) {
+                   const ChargeWeights& cw, const RhoWorkspace& ws, double* rho) {
   RhoArgs R{};
   R.cw = cw;
   for (int a = 0; a < 3; ++a) {
@@ -483,11 +495,12 @@ void chargeDensity(FieldState& s, int shape, const double* const x[3], const dou
   R.q = q;
   R.n = n;
   R.shape = shape;
-  depositCharge(s, R, q, rho);
+  depositCharge(s, R, q, ws, rho);
 }
 
 void chargeDensitySplit(FieldState32& s, int shape, const int* const cell[3], const float* const off[3],
-                        const float* q, int n, const int oc[3], const ChargeWeights& cw, DeviceArray<double>& rho) {
+                        const float* q, int n, const int oc[3], const ChargeWeights& cw, const RhoWorkspace& ws,
+                        double* rho) {
   RhoArgs R{};
   R.cw = cw;
   for (int a = 0; a < 3; ++a) {
@@ -498,22 +511,86 @@ void chargeDensitySplit(FieldState32& s, int shape, const int* const cell[3], co
   R.qf = q;
   R.n = n;
   R.shape = shape;
-  depositCharge(s, R, q, rho);
+  depositCharge(s, R, q, ws, rho);
 }
 
 template <class R>
-void continuityResidual(FieldStateT<R>& s, const DeviceArray<double>& rho0, const DeviceArray<double>& rho1,
+void continuityResidual(FieldStateT<R>& s, const double* rho0, const double* rho1, const RhoWorkspace& ws,
                         double out[3]) {
-  DiagnosticScratch& D = s.diag;
-  ensure(D.maxima, 4);
-  kZeroU64<<<1, 64, 0, s.stream>>>(D.maxima.get(), 4);
-  kContinuity<<<2048, 256, 0, s.stream>>>(s.ptrs(), rho0.get(), rho1.get(), s.g.dt,
-                                          make_double3(s.g.dx, s.g.dy, s.g.dz), D.maxima.get());
+  kZeroU64<<<1, 64, 0, s.stream>>>(ws.maxima, 4);
+  kContinuity<<<2048, 256, 0, s.stream>>>(s.ptrs(), rho0, rho1, s.g.dt, make_double3(s.g.dx, s.g.dy, s.g.dz),
+                                          ws.maxima);
   launchCheck("continuity");
   unsigned long long m[3];
-  hipCheck(hipMemcpyAsync(m, D.maxima.get(), sizeof(m), hipMemcpyDeviceToHost, s.stream), "hipMemcpyAsync");
+  hipCheck(hipMemcpyAsync(m, ws.maxima, sizeof(m), hipMemcpyDeviceToHost, s.stream), "hipMemcpyAsync");
   hipCheck(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
   std::memcpy(out, m, sizeof(m));
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free continuity workspace: one file-scope device array, laid
+// out as the fixed-point accumulator, the two charge densities (one word
+// per vertex each, up to kNoAllocVertices) and the reduction partials.
+// Every GPU the library runs on holds its own copy for the life of the
+// process; its device address comes from hipGetSymbolAddress, taken once
+// per device with that device current (the array's name is not a device
+// address in host code). Calls on one GPU share it: overlapping calls race.
+
+namespace {
+constexpr std::size_t kNoAllocVertices = 2400000;  // 64 x 64 x 512 cells have 2,167,425 vertices
+constexpr std::size_t kNoAllocPartial = 2 * kRedBlocks, kNoAllocOut = 8, kNoAllocMaxima = 4;
+constexpr std::size_t kNoAllocWords = 3 * kNoAllocVertices + kNoAllocPartial + kNoAllocOut + kNoAllocMaxima;
+}  // namespace
+
+static __device__ __align__(256) unsigned long long scratch[kNoAllocWords];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex gScratchMutex;
+unsigned long long* gScratch[kMaxDevices] = {};
+
+__global__ void kFillU64(unsigned long long* p, size_t n) {
+  for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<size_t>(gridDim.x) * blockDim.x)
+    p[t] = 0ull;
+}
+
+unsigned long long* scratchOnCurrentDevice() {
+  int device = 0;
+  hipCheck(hipGetDevice(&device), "hipGetDevice");
+  if (device < 0 || device >= kMaxDevices) throw std::runtime_error("Allocation-free workspace: device index out of range");
+  std::lock_guard<std::mutex> lock(gScratchMutex);
+  if (!gScratch[device]) {
+    void* address = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&address, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || !address)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(rc));
+    gScratch[device] = static_cast<unsigned long long*>(address);
+  }
+  return gScratch[device];
+}
+}  // namespace
+
+std::size_t noAllocVertexLimit() { return kNoAllocVertices; }
+
+template <class R>
+RhoWorkspace noAllocWorkspace(FieldStateT<R>& s, std::size_t vertices, double** rho0, double** rho1) {
+  if (vertices > kNoAllocVertices)
+    throw std::invalid_argument("Allocation-free continuity: the grid has " + std::to_string(vertices) +
+                            " vertices, the workspace holds at most " + std::to_string(kNoAllocVertices));
+  unsigned long long* base = scratchOnCurrentDevice();
+  RhoWorkspace ws;
+  ws.acc = base;
+  *rho0 = reinterpret_cast<double*>(base + kNoAllocVertices);
+  *rho1 = reinterpret_cast<double*>(base + 2 * kNoAllocVertices);
+  ws.partial = reinterpret_cast<double*>(base + 3 * kNoAllocVertices);
+  ws.out = ws.partial + kNoAllocPartial;
+  ws.maxima = base + 3 * kNoAllocVertices + kNoAllocPartial + kNoAllocOut;
+  // the accumulator keeps the last call's contents; clear the part in use
+  if (vertices) kFillU64<<<static_cast<unsigned>(std::min<std::size_t>(ceilDiv(vertices, 256), 2048)), 256, 0,
+                           s.stream>>>(ws.acc, vertices);
+  launchCheck("allocation-free workspace clear");
+  return ws;
 }
 
 template <class R>
@@ -867,8 +944,10 @@ template void particleEnergySums(FieldState32&, const CouplingState&, const Spli
   template void ledgerRead(FieldStateT<R>&, double*);                                                         \
   template double interiorEnergy(FieldStateT<R>&, const std::size_t*, const std::size_t*);                    \
   template void facePower(FieldStateT<R>&, const std::size_t*, const std::size_t*, double*);                  \
-  template void continuityResidual(FieldStateT<R>&, const DeviceArray<double>&, const DeviceArray<double>&,   \
+  template void continuityResidual(FieldStateT<R>&, const double*, const double*, const RhoWorkspace&,         \
                                    double*);                                                                  \
+  template RhoWorkspace diagnosticWorkspace(FieldStateT<R>&);                                                 \
+  template RhoWorkspace noAllocWorkspace(FieldStateT<R>&, std::size_t, double**, double**);                   \
   template void gaussResidual(FieldStateT<R>&, const DeviceArray<double>&, const std::size_t*,                \
                               const std::size_t*, double*);
 APRL_DIAG_INSTANTIATE(double)

@@ -25,18 +25,40 @@ struct ChargeWeights {
   const double* wd = nullptr;  // weights of the plain form, or
   const float* wf = nullptr;   // of the split form
 };
+// Device memory one shaped-charge deposit works in: acc holds one 64-bit
+// fixed-point word per vertex and must be zero on entry (every pass clears
+// what it reads); partial holds 2 x 512 doubles, out 8 doubles, maxima 4
+// words.
+struct RhoWorkspace {
+  unsigned long long* acc = nullptr;
+  double* partial = nullptr;
+  double* out = nullptr;
+  unsigned long long* maxima = nullptr;
+};
+// The workspace of s.diag, grown to the grid (allocates on first use).
+template <class R>
+RhoWorkspace diagnosticWorkspace(FieldStateT<R>& s);
 // Shaped vertex charge density (shape 0 quintic, 2 binomial3-x) of
 // particles at absolute positions x, or (split form) at lattice cell +
-// offset with the grid origin at lattice cell oc.
+// offset with the grid origin at lattice cell oc, into rho (one double per
+// vertex).
 void chargeDensity(FieldState& s, int shape, const double* const x[3], const double* q, int n, const double origin[3],
-                   const ChargeWeights& cw, DeviceArray<double>& rho);
+                   const ChargeWeights& cw, const RhoWorkspace& ws, double* rho);
 void chargeDensitySplit(FieldState32& s, int shape, const int* const cell[3], const float* const off[3],
-                        const float* q, int n, const int oc[3], const ChargeWeights& cw, DeviceArray<double>& rho);
+                        const float* q, int n, const int oc[3], const ChargeWeights& cw, const RhoWorkspace& ws,
+                        double* rho);
 // out = {max |(rho1 - rho0)/dt + div J|, max |(rho1 - rho0)/dt|,
 // max max(|rho0|, |rho1|)/dt}.
 template <class R>
-void continuityResidual(FieldStateT<R>& s, const DeviceArray<double>& rho0, const DeviceArray<double>& rho1,
+void continuityResidual(FieldStateT<R>& s, const double* rho0, const double* rho1, const RhoWorkspace& ws,
                         double out[3]);
+// Vertices of the allocation-free continuity workspace, and that workspace
+// with its two charge-density arrays (this device's copy of a file-scope
+// device array; throws std::invalid_argument beyond the limit). The
+// accumulator is cleared by a fill kernel on s's stream.
+std::size_t noAllocVertexLimit();
+template <class R>
+RhoWorkspace noAllocWorkspace(FieldStateT<R>& s, std::size_t vertices, double** rho0, double** rho1);
 // out = {sum error^2, sum source^2, max |error|, max |source|} over the
 // vertices strictly inside [lo, hi], error = div E - rho / eps0.
 template <class R>

@@ -413,6 +413,16 @@ aprl_status aprl_fields_face_power(aprl_fields* fields, const size_t* lo, const 
  * (binomial3-x: the shaped charge of
  * verification/transverse_binomial_checks.cpp:33-40). */
 aprl_status aprl_coupling_continuity(aprl_coupling* coupling, aprl_particles* particles, double* out);
+/* aprl_coupling_continuity without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc); it has no counterpart in the
+ * reference. Same kernels, same outputs. Its workspace is one file-scope
+ * device array of 24 B per vertex for at most 2,400,000 vertices (57.6 MB,
+ * held on every GPU the library runs on until the process ends): grids
+ * with more vertices ((nx+1)(ny+1)(nz+1)) are rejected with
+ * APRL_INVALID_ARGUMENT. Calls on one GPU share the workspace, so calls
+ * that overlap on one GPU race; each call clears what must start at zero. */
+aprl_status aprl_coupling_continuity_noalloc(aprl_coupling* coupling, aprl_particles* particles, double* out);
 /* Gauss residual of eb_initial_field.cpp:774-799 over the vertices strictly
  * inside [lo, hi] (hi <= (nx, ny, nz)) with the shaped charge of the particles at their current
  * positions (grid origin `origin`): out = {sum error^2, sum source^2,
