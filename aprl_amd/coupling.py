@@ -175,6 +175,12 @@ class ConservativeSourceCoupling:
         after the last trial whose force and work the step reports."""
         check(lib().aprl_coupling_set_split_trials(self.handle, int(trials), 1 if verify else 0))
 
+    def prepareStep(self, particles: ParticleStore, origin):
+        """Start the next step's sort phase on the coupling's own stream
+        (aprl_coupling_prepare_step); call before the step's field updates."""
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        check(lib().aprl_coupling_prepare_step(self.handle, particles.handle, dptr(o)))
+
     def step(self, particles: ParticleStore, origin, time, substeps=1):
         o = np.ascontiguousarray(origin, dtype=np.float64)
         r = _capi.StepReport()

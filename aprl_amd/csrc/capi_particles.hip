@@ -399,6 +399,16 @@ aprl_status aprl_coupling_step(aprl_coupling* coupling, aprl_particles* particle
   });
 }
 
+aprl_status aprl_coupling_prepare_step(aprl_coupling* coupling, aprl_particles* particles, const double* origin) {
+  return guarded([&] {
+    if (!coupling || !particles || !origin) throw std::invalid_argument("Null coupling, particles or origin");
+    if (particles->fields != coupling->fields)
+      throw std::invalid_argument("Particles and coupling belong to different field grids");
+    if (coupling->fields->fp32) prepareStepSplit(coupling->c, particles->sp, origin);
+    else prepareStep(coupling->c, particles->p, origin);
+  });
+}
+
 aprl_status aprl_coupling_set_split_trials(aprl_coupling* coupling, unsigned trials, int verify) {
   return guarded([&] {
     if (!coupling) throw std::invalid_argument("Null coupling");

@@ -187,6 +187,24 @@ class Event {
   hipEvent_t e_ = nullptr;
 };
 
+// A non-blocking HIP stream at the default priority, created on first use.
+class Stream {
+ public:
+  Stream() = default;
+  Stream(const Stream&) = delete;
+  Stream& operator=(const Stream&) = delete;
+  ~Stream() {
+    if (s_) (void)hipStreamDestroy(s_);
+  }
+  hipStream_t get() {
+    if (!s_) hipCheck(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking), "hipStreamCreate");
+    return s_;
+  }
+
+ private:
+  hipStream_t s_ = nullptr;
+};
+
 // Per-kernel HIP-event timing on one stream. Events are recorded around each
 // launch only while enabled; durations are collected lazily on query.
 class Profiler {

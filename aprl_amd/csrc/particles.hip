@@ -568,16 +568,16 @@ __global__ void __launch_bounds__(128) kPush(PArgs P, GArgs G, StepArgs S, unsig
 }
 
 // Deterministic tile deposition of Current::deposit (longitudinal_shape.h:
-// 175-187): one workgroup per chunk, one lane per particle, every tap of
-// the chord x0 -> x1 accumulated into an LDS copy of the tile box in 64-bit
-// fixed point. The binary exponent comes from a bound on the whole tile's
+// 175-187): one workgroup per chunk, depositLanes<SHAPE>() lanes per
+// particle (one tap slice each), every tap of the chord x0 -> x1
+// accumulated into an LDS copy of the tile box in 64-bit fixed point. The binary exponent comes from a bound on the whole tile's
 // current (the shape's weight bound times the tile's particle count times
 // the largest per-particle bound the push recorded), so every chunk of a
 // tile uses the same scale. Integer addition is associative, so the LDS
 // atomics and the chunk sums are reproducible in any order; each tap is
 // rounded once, to 2^-62 of the tile bound.
 template <int SHAPE>
-__global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, StepArgs S, const int4* chunks,
+__global__ void __launch_bounds__(kChunk * depositLanes<SHAPE>()) kDeposit(PArgs P, GArgs G, StepArgs S, const int4* chunks,
                                                    const int* tileBegin, const int* tileEnd,
                                                    const unsigned long long* tileScan, int ntx, int nty,
                                                    long long* partial, int* tileShift, unsigned* errOut) {
@@ -592,7 +592,9 @@ __global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, StepArgs S,
   if (threadIdx.x == 0 && static_cast<int>(blockIdx.x) == chunkField(tsc)) tileShift[tile] = shift;
   __syncthreads();
   unsigned err = 0;
-  const int p = ch.y + static_cast<int>(threadIdx.x);
+  // lanes [s * kChunk, (s + 1) * kChunk) take tap slice s of particles ch.y + lane
+  const int sub = static_cast<int>(threadIdx.x) / kChunk;
+  const int p = ch.y + static_cast<int>(threadIdx.x) % kChunk;
   if (p < ch.z) {
     double scale[3];
     double x[3], v[3], cuts[kMaxCuts];
@@ -670,7 +672,7 @@ __global__ void __launch_bounds__(kChunk) kDeposit(PArgs P, GArgs G, StepArgs S,
           const double b = part ? fmin(c1, f2) : fmin(c1, f1);
           if (!(b > a)) continue;
           mult = part ? fw : 1.0;
-          segmentTaps<SHAPE, double>(G.nx, G.ny, org, x, v, a, b, interior, fast, visit);
+          segmentTaps<SHAPE, double, depositLanes<SHAPE>()>(G.nx, G.ny, org, x, v, a, b, interior, fast, visit, sub);
         }
       }
     }
@@ -999,9 +1001,8 @@ static int bitsFor(unsigned v) {
 // entries last), then the tile ranges and the chunk list (tile order,
 // kChunk particles per chunk) in device memory.
 static void sortAndChunk(CouplingState& C, ParticleStore& P, const GArgs& G, const SortArgs& K, int ntx, int nty,
-                         int ntiles, unsigned* occupied, unsigned seq) {
+                         int ntiles, unsigned* occupied, unsigned seq, hipStream_t st) {
   FieldState& F = *C.fields;
-  hipStream_t st = F.stream;
   const int n = static_cast<int>(P.count);
   ScopedKernel sk(F.profiler, st, "sort");
   const unsigned blocks = ceilDiv(n, 256);
@@ -1181,6 +1182,62 @@ std::size_t liveEntries(std::size_t count, const DeviceArray<unsigned>& removed,
   return count - std::min<std::size_t>(count, r);
 }
 
+// The step's sort phase on stream ss: tile lists and readback words
+// sized, error words cleared, then (with particles) the stable tile sort,
+// the state permute and the chunk lists, whose totals are posted to the
+// mailbox under the returned sequence number.
+static unsigned sortPhase(CouplingState& C, ParticleStore& P, const double origin[3], hipStream_t ss) {
+  FieldState& F = *C.fields;
+  const int n = static_cast<int>(P.count);
+  const int ntx = static_cast<int>((F.g.nx + kTileX - 1) / kTileX);
+  const int nty = static_cast<int>((F.g.ny + kTileY - 1) / kTileY);
+  const int ntz = static_cast<int>((F.g.nz + kTileZ - 1) / kTileZ);
+  const int ntiles = ntx * nty * ntz;
+  reserveTileLists(C, ss, ntiles, n);
+  reserveCouplingWords(C, ss);
+  kFillInt<<<1, 128, 0, ss>>>(reinterpret_cast<int*>(C.error.get()), kErrorWords, 0);
+  launchCheck("fill");
+  if (!n) return 0;
+  SortArgs K{};
+  K.window = F.window ? 1 : 0;
+  K.nz = static_cast<int>(F.g.nz);
+  K.lowZ = origin[2];
+  K.upZ = std::fma(static_cast<double>(F.g.nz), F.g.dz, origin[2]);
+  K.dz = F.g.dz;
+  K.words = C.error.get();
+  K.removed = P.removed.get();
+  const unsigned seq = C.mailbox.next(0);
+  sortAndChunk(C, P, gatherArgs(C, F, origin), K, ntx, nty, ntiles, C.error.get() + 1, seq, ss);
+  return seq;
+}
+
+bool preparedFor(const CouplingState& C, const void* store, std::uint64_t generation, std::size_t count,
+                 const double origin[3]) {
+  const auto& R = C.prepared;
+  return R.valid && R.store == store && R.generation == generation && R.count == count && R.origin[0] == origin[0] &&
+         R.origin[1] == origin[1] && R.origin[2] == origin[2];
+}
+
+// Starts the next step's sort phase on the coupling's sort stream after
+// everything already enqueued on the grid's stream, so that it overlaps
+// the field work enqueued after it (clear, window begin, magnetic update).
+void prepareStep(CouplingState& C, ParticleStore& P, const double origin[3]) {
+  FieldState& F = *C.fields;
+  if (C.prepared.valid) throw std::logic_error("A prepared coupling step is already pending");
+  hipStream_t ss = C.sortStream.get();
+  hipCheck(hipEventRecord(C.prepareStart.get(), F.stream), "hipEventRecord");
+  hipCheck(hipStreamWaitEvent(ss, C.prepareStart.get(), 0), "hipStreamWaitEvent");
+  const unsigned seq = sortPhase(C, P, origin, ss);
+  hipCheck(hipEventRecord(C.sortReady.get(), ss), "hipEventRecord");
+  auto& R = C.prepared;
+  R.valid = true;
+  R.store = &P;
+  R.generation = P.generation;
+  R.count = P.count;
+  for (int a = 0; a < 3; ++a) R.origin[a] = origin[a];
+  R.seq = seq;
+}
+
 void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], double time, unsigned substeps,
                   aprl_step_report* report) {
   FieldState& F = *C.fields;
@@ -1203,28 +1260,20 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
   const int nty = static_cast<int>((F.g.ny + kTileY - 1) / kTileY);
   const int ntz = static_cast<int>((F.g.nz + kTileZ - 1) / kTileZ);
   const int ntiles = ntx * nty * ntz;
-  reserveTileLists(C, st, ntiles, n);
   const std::size_t chunkBound = chunkListBound(ntiles, n);
-  reserveCouplingWords(C, st);
   // Readbacks arrive in the mailbox: the scan total (chunks, active tiles)
   // and the sort-phase words at words 8..13 (flag 0); a trial's residual
   // sums (force, defect, scale, calls; doubles) and its error words from
   // word 16 (flag 1).
   unsigned sortSeq = 0;
-  kFillInt<<<1, 128, 0, st>>>(reinterpret_cast<int*>(C.error.get()), kErrorWords, 0);
-  launchCheck("fill");
-  if (n) {
-    SortArgs K{};
-    K.window = F.window ? 1 : 0;
-    K.nz = static_cast<int>(F.g.nz);
-    K.lowZ = origin[2];
-    K.upZ = std::fma(static_cast<double>(F.g.nz), F.g.dz, origin[2]);
-    K.dz = F.g.dz;
-    K.words = C.error.get();
-    K.removed = P.removed.get();
-    sortSeq = C.mailbox.next(0);
-    sortAndChunk(C, P, G, K, ntx, nty, ntiles, C.error.get() + 1, sortSeq);
+  if (preparedFor(C, &P, P.generation, P.count, origin)) {
+    hipCheck(hipStreamWaitEvent(st, C.sortReady.get(), 0), "hipStreamWaitEvent");
+    sortSeq = C.prepared.seq;
+  } else {
+    if (C.prepared.valid) hipCheck(hipStreamWaitEvent(st, C.sortReady.get(), 0), "hipStreamWaitEvent");
+    sortSeq = sortPhase(C, P, origin, st);
   }
+  C.prepared.valid = false;
   // Zero-current preview E* (MovingWindowFields::previewElectricBoundary);
   // it runs while the host waits for the chunk count.
   // E* and E* + E^n are read only on the nodes of active tiles (gathers
@@ -1295,11 +1344,11 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
       {
         ScopedKernel sk(F.profiler, st, "deposit");
         if (quintic)
-          kDeposit<kShapeQuintic><<<nChunks, kChunk, 0, st>>>(A, G, S, C.chunks.get(), C.tileBegin.get(),
+          kDeposit<kShapeQuintic><<<nChunks, kChunk * depositLanes<kShapeQuintic>(), 0, st>>>(A, G, S, C.chunks.get(), C.tileBegin.get(),
                                                               C.tileEnd.get(), C.tileScan.get(), ntx, nty,
                                                               C.partial.get(), C.tileShift.get(), errT);
         else
-          kDeposit<kShapeBinomial3X><<<nChunks, kChunk, 0, st>>>(A, G, S, C.chunks.get(), C.tileBegin.get(),
+          kDeposit<kShapeBinomial3X><<<nChunks, kChunk * depositLanes<kShapeBinomial3X>(), 0, st>>>(A, G, S, C.chunks.get(), C.tileBegin.get(),
                                                                  C.tileEnd.get(), C.tileScan.get(), ntx, nty,
                                                                  C.partial.get(), C.tileShift.get(), errT);
         launchCheck("deposit");

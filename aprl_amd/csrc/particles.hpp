@@ -190,6 +190,21 @@ struct CouplingState {
   HostMailbox mailbox;  // kernel-written readbacks of the plain form
   Event sorted, stepDone;
   std::shared_ptr<SplitStepContext> split;
+  // A sort phase started ahead of the step (prepareStep) on its own
+  // stream, so that it overlaps the field updates enqueued after it:
+  // sortStream waits for `prepareStart` (recorded on the grid's stream at
+  // the prepare) and `sortReady` marks its end. The step uses it only for
+  // the same store, generation, entry count and origin.
+  Stream sortStream;
+  Event prepareStart, sortReady;
+  struct {
+    bool valid = false;
+    const void* store = nullptr;
+    std::uint64_t generation = 0;
+    std::size_t count = 0;
+    double origin[3] = {0.0, 0.0, 0.0};
+    unsigned seq = 0;
+  } prepared;
   // reference surface current (TransverseReferenceCurrent): rows x points
   // table, target (i, j) pairs, translation beta
   bool refCurrent = false;
@@ -339,6 +354,14 @@ void downloadSplitParticles(const SplitStore& P, const FieldState32& F, double* 
                             double* w, uint64_t* id);
 void couplingStepSplit(CouplingState& C, SplitStore& P, const double origin[3], double time, unsigned substeps,
                        aprl_step_report* report);
+// The sort phase of the next coupled step started ahead of it (before the
+// field updates of the step are enqueued) on the coupling's sort stream;
+// the step uses it when called with the same store, unchanged since, and
+// the same origin, and otherwise sorts again.
+void prepareStep(CouplingState& C, ParticleStore& P, const double origin[3]);
+void prepareStepSplit(CouplingState& C, SplitStore& P, const double origin[3]);
+bool preparedFor(const CouplingState& C, const void* store, std::uint64_t generation, std::size_t count,
+                 const double origin[3]);
 // The same in three phases, so that steps of independent instances can be
 // interleaved: begin (sort, preview), middle (trials, commit, async
 // readback), end (wait, errors, report).

@@ -564,7 +564,7 @@ __global__ void __launch_bounds__(128) kPushSplit(PArgs32 P, StepArgs S, double 
 // scheme is the FP64 form's: one exponent per tile from the tile's current
 // bound, integer sums independent of order.
 template <int SHAPE>
-__global__ void __launch_bounds__(kChunk) kDepositSplit(PArgs32 P, GArgs32 G, StepArgs S, const int4* chunks,
+__global__ void __launch_bounds__(kChunk * depositLanes<SHAPE>()) kDepositSplit(PArgs32 P, GArgs32 G, StepArgs S, const int4* chunks,
                                                         const int* tileBegin,
                                                         const int* tileEnd, const unsigned long long* tileScan,
                                                         int ntx, int nty, long long* partial, int* tileShift,
@@ -585,7 +585,9 @@ __global__ void __launch_bounds__(kChunk) kDepositSplit(PArgs32 P, GArgs32 G, St
   if (threadIdx.x == 0 && static_cast<int>(blockIdx.x) == chunkField(tsc)) tileShift[tile] = shift;
   __syncthreads();
   unsigned err = 0;
-  const int p = ch.y + static_cast<int>(threadIdx.x);
+  // lanes [s * kChunk, (s + 1) * kChunk) take tap slice s of particles ch.y + lane
+  const int sub = static_cast<int>(threadIdx.x) / kChunk;
+  const int p = ch.y + static_cast<int>(threadIdx.x) % kChunk;
   if (p < ch.z) {
     int c0[3];
     float off0[3], v[3], scale[3];
@@ -667,7 +669,7 @@ __global__ void __launch_bounds__(kChunk) kDepositSplit(PArgs32 P, GArgs32 G, St
           const float b = part ? fminf(k1, f2) : fminf(k1, f1);
           if (!(b > a)) continue;
           mult = part ? fw : 1.f;
-          segmentTaps<SHAPE, float>(G.nx, G.ny, c.org, c.x, c.v, a, b, interior, fast, visit);
+          segmentTaps<SHAPE, float, depositLanes<SHAPE>()>(G.nx, G.ny, c.org, c.x, c.v, a, b, interior, fast, visit, sub);
         }
       }
     }
@@ -976,9 +978,8 @@ int bitsFor32(unsigned v) {
 }
 
 void sortSplit(CouplingState& C, SplitStore& P, const GArgs32& G, const SortArgs& K, int ntx, int nty, int ntiles,
-               unsigned* occupied) {
+               unsigned* occupied, hipStream_t st) {
   FieldState32& F = *C.fields32;
-  hipStream_t st = F.stream;
   const int n = static_cast<int>(P.count);
   ScopedKernel sk(F.profiler, st, "sort");
   const unsigned blocks = ceilDiv(n, 256);
@@ -1037,6 +1038,72 @@ struct SplitStepContext {
 
 // Phase 1 of a split step: checks, J clear, sort and chunk lists (their
 // totals copied back asynchronously), and the zero-current preview.
+// The split positions live on the lattice x = cell * h; the grid origin
+// must be one of its nodes: oc its lattice cell, o = oc * h.
+static void splitOrigin(const FieldState32& F, const double origin[3], int oc[3], double o[3]) {
+  const double h[3] = {F.g.dx, F.g.dy, F.g.dz};
+  for (int a = 0; a < 3; ++a) {
+    const double r = origin[a] / h[a];
+    if (!std::isfinite(r) || std::fabs(r) >= 1.0e9 || std::fabs(r - std::nearbyint(r)) > 1e-6)
+      throw std::invalid_argument("An FP32-split grid origin must lie on the cell lattice (origin = integer * spacing)");
+    oc[a] = static_cast<int>(std::nearbyint(r));
+    o[a] = static_cast<double>(oc[a]) * h[a];
+  }
+}
+
+// The step's sort phase on stream ss: tile lists and readback words sized,
+// error words cleared, then (with particles) the stable tile sort, the
+// state permute, the chunk lists and the asynchronous readback of their
+// totals and the sort-phase words (C.sorted marks its end).
+static void sortPhaseSplit(CouplingState& C, SplitStore& P, const double origin[3], hipStream_t ss) {
+  FieldState32& F = *C.fields32;
+  int oc[3];
+  double o[3];
+  splitOrigin(F, origin, oc, o);
+  const int n = static_cast<int>(P.count);
+  const int ntx = static_cast<int>((F.g.nx + kTileX - 1) / kTileX);
+  const int nty = static_cast<int>((F.g.ny + kTileY - 1) / kTileY);
+  const int ntz = static_cast<int>((F.g.nz + kTileZ - 1) / kTileZ);
+  const int ntiles = ntx * nty * ntz;
+  reserveTileLists(C, ss, ntiles, n);
+  reserveCouplingWords(C, ss);
+  unsigned* pin = static_cast<unsigned*>(C.pinned.get(1024));
+  fillInts(ss, reinterpret_cast<int*>(C.error.get()), kErrorWords, 0);
+  if (!n) return;
+  SortArgs K{};
+  K.window = F.window ? 1 : 0;
+  K.nz = static_cast<int>(F.g.nz);
+  K.lowZ = o[2];
+  K.upZ = std::fma(static_cast<double>(F.g.nz), F.g.dz, o[2]);
+  K.dz = F.g.dz;
+  K.words = C.error.get();
+  K.removed = P.removed.get();
+  sortSplit(C, P, gatherArgs32(C, F, oc), K, ntx, nty, ntiles, C.error.get() + 1, ss);
+  hipCheck(hipMemcpyAsync(pin, C.tileScan.get() + ntiles, sizeof(unsigned long long), hipMemcpyDeviceToHost, ss),
+           "hipMemcpyAsync");
+  hipCheck(hipMemcpyAsync(pin + 120, C.error.get(), 4 * sizeof(unsigned), hipMemcpyDeviceToHost, ss),
+           "hipMemcpyAsync");
+  hipCheck(hipEventRecord(C.sorted.get(), ss), "hipEventRecord");
+}
+
+void prepareStepSplit(CouplingState& C, SplitStore& P, const double origin[3]) {
+  FieldState32& F = *C.fields32;
+  if (C.prepared.valid) throw std::logic_error("A prepared coupling step is already pending");
+  if (C.split && C.split->pending) throw std::logic_error("The previous FP32-split coupling step was not finished");
+  hipStream_t ss = C.sortStream.get();
+  hipCheck(hipEventRecord(C.prepareStart.get(), F.stream), "hipEventRecord");
+  hipCheck(hipStreamWaitEvent(ss, C.prepareStart.get(), 0), "hipStreamWaitEvent");
+  sortPhaseSplit(C, P, origin, ss);
+  hipCheck(hipEventRecord(C.sortReady.get(), ss), "hipEventRecord");
+  auto& R = C.prepared;
+  R.valid = true;
+  R.store = &P;
+  R.generation = P.generation;
+  R.count = P.count;
+  for (int a = 0; a < 3; ++a) R.origin[a] = origin[a];
+  R.seq = 0;
+}
+
 void splitStepBegin(CouplingState& C, SplitStore& P, const double origin[3], double time, unsigned substeps) {
   FieldState32& F = *C.fields32;
   if (!C.split) C.split = std::make_shared<SplitStepContext>();
@@ -1045,18 +1112,9 @@ void splitStepBegin(CouplingState& C, SplitStore& P, const double origin[3], dou
   if (F.window && F.phase != 2) throw std::logic_error("Moving-window electric phase mismatch");
   if (F.window && !F.inletSet) throw std::logic_error("Moving-window inlet planes were not supplied");
   if (substeps == 0) throw std::invalid_argument("Particle substeps must be positive");
-  // The split positions live on the lattice x = cell * h; the grid origin
-  // must be one of its nodes.
-  const double h[3] = {F.g.dx, F.g.dy, F.g.dz};
   int oc[3];
   double o[3];
-  for (int a = 0; a < 3; ++a) {
-    const double r = origin[a] / h[a];
-    if (!std::isfinite(r) || std::fabs(r) >= 1.0e9 || std::fabs(r - std::nearbyint(r)) > 1e-6)
-      throw std::invalid_argument("An FP32-split grid origin must lie on the cell lattice (origin = integer * spacing)");
-    oc[a] = static_cast<int>(std::nearbyint(r));
-    o[a] = static_cast<double>(oc[a]) * h[a];
-  }
+  splitOrigin(F, origin, oc, o);
   const int n = static_cast<int>(P.count);
   hipStream_t st = F.stream;
   launchClearCurrent(F);
@@ -1070,28 +1128,15 @@ void splitStepBegin(CouplingState& C, SplitStore& P, const double origin[3], dou
   const int nty = static_cast<int>((F.g.ny + kTileY - 1) / kTileY);
   const int ntz = static_cast<int>((F.g.nz + kTileZ - 1) / kTileZ);
   const int ntiles = ntx * nty * ntz;
-  reserveTileLists(C, st, ntiles, n);
   const std::size_t chunkBound = chunkListBound(ntiles, n);
   const unsigned trials = std::max(2u, C.splitTrials);
-  reserveCouplingWords(C, st);
-  unsigned* pin = static_cast<unsigned*>(C.pinned.get(1024));
-  fillInts(st, reinterpret_cast<int*>(C.error.get()), kErrorWords, 0);
-  if (n) {
-    SortArgs K{};
-    K.window = F.window ? 1 : 0;
-    K.nz = static_cast<int>(F.g.nz);
-    K.lowZ = o[2];
-    K.upZ = std::fma(static_cast<double>(F.g.nz), F.g.dz, o[2]);
-    K.dz = F.g.dz;
-    K.words = C.error.get();
-    K.removed = P.removed.get();
-    sortSplit(C, P, G, K, ntx, nty, ntiles, C.error.get() + 1);
-    hipCheck(hipMemcpyAsync(pin, C.tileScan.get() + ntiles, sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
-             "hipMemcpyAsync");
-    hipCheck(hipMemcpyAsync(pin + 120, C.error.get(), 4 * sizeof(unsigned), hipMemcpyDeviceToHost, st),
-             "hipMemcpyAsync");
-    hipCheck(hipEventRecord(C.sorted.get(), st), "hipEventRecord");
+  if (preparedFor(C, &P, P.generation, P.count, origin)) {
+    hipCheck(hipStreamWaitEvent(st, C.sortReady.get(), 0), "hipStreamWaitEvent");
+  } else {
+    if (C.prepared.valid) hipCheck(hipStreamWaitEvent(st, C.sortReady.get(), 0), "hipStreamWaitEvent");
+    sortPhaseSplit(C, P, origin, st);
   }
+  C.prepared.valid = false;
   // E* and E* + E^n on the nodes of active tiles only (their only readers)
   float* const out[3] = {C.estar32[0].get(), C.estar32[1].get(), C.estar32[2].get()};
   float* const sums[3] = {C.esum32[0].get(), C.esum32[1].get(), C.esum32[2].get()};
@@ -1187,11 +1232,11 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
     {
       ScopedKernel sk(F.profiler, st, "deposit");
       if (quintic)
-        kDepositSplit<kShapeQuintic><<<nChunks, kChunk, 0, st>>>(A, G, S, C.chunks.get(), C.tileBegin.get(),
+        kDepositSplit<kShapeQuintic><<<nChunks, kChunk * depositLanes<kShapeQuintic>(), 0, st>>>(A, G, S, C.chunks.get(), C.tileBegin.get(),
                                                                  C.tileEnd.get(), C.tileScan.get(), ntx, nty,
                                                                  C.partial.get(), C.tileShift.get(), errT);
       else
-        kDepositSplit<kShapeBinomial3X><<<nChunks, kChunk, 0, st>>>(A, G, S, C.chunks.get(), C.tileBegin.get(),
+        kDepositSplit<kShapeBinomial3X><<<nChunks, kChunk * depositLanes<kShapeBinomial3X>(), 0, st>>>(A, G, S, C.chunks.get(), C.tileBegin.get(),
                                                                     C.tileEnd.get(), C.tileScan.get(), ntx, nty,
                                                                     C.partial.get(), C.tileShift.get(), errT);
       launchCheck("deposit");

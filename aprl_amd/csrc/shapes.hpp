@@ -15,6 +15,17 @@ namespace aprl_gpu {
 
 enum : int { kShapeQuintic = 0, kShapeBinomial3X = 2 };
 
+// Lanes per particle in the deposits: each lane of a particle visits one
+// slice of its chord's taps (segmentTaps), the lanes of one slice forming
+// whole waves. The quintic's 368 taps per segment are split over its four
+// Gauss points; binomial3-x's 64 taps cost less than the chord set-up each
+// lane would repeat, so it keeps one lane. The fixed-point LDS sums do not
+// depend on which lane adds a tap.
+template <int SHAPE>
+constexpr int depositLanes() {
+  return SHAPE == kShapeQuintic ? 4 : 1;
+}
+
 // Extended node box of a cell tile per shape: every tap of a chord that
 // starts in the tile and moves less than one cell per axis lies inside.
 // (X0, Y0, Z0) is the box origin below the tile origin; RZ is how many
@@ -169,9 +180,18 @@ __device__ __forceinline__ void quinticBasis(R z, bool edge, QBasisT<R>& r) {
 // is called instead, with offsets from the segment's lowest tap corner,
 // when the caller reports (through interior) that every tap of the segment
 // lies inside its domain.
-template <int SHAPE, class R, class Interior, class Fast, class Visit>
+//
+// With SPLIT > 1 only the taps of slice `sub` (0 <= sub < SPLIT) are
+// visited, so SPLIT lanes can share one segment: binomial3-x slices the
+// Jx / Jy taps by z offset c (c % SPLIT == sub) and the Jz taps by their
+// (y, x) pair ((4 b + a) % SPLIT == sub), 16 taps each for SPLIT = 4; the
+// quintic slices by Gauss point (t % SPLIT == sub). Each tap keeps the
+// weight it has without the split.
+template <int SHAPE, class R, int SPLIT = 1, class Interior, class Fast, class Visit>
 __device__ __forceinline__ void segmentTaps(int nx, int ny, const int org[3], const R x[3], const R v[3], R first,
-                                            R last, Interior interior, Fast fast, Visit visit) {
+                                            R last, Interior interior, Fast fast, Visit visit, int sub = 0) {
+  static_assert(SPLIT == 1 || SPLIT == 2 || SPLIT == 4, "tap slices: 1, 2 or 4");
+  auto mine = [sub](int slice) { return SPLIT == 1 || slice % SPLIT == sub; };
   const R width = last - first;
   if (SHAPE == kShapeBinomial3X) {
     const R mid = (first + last) * R(0.5);
@@ -197,6 +217,7 @@ __device__ __forceinline__ void segmentTaps(int nx, int ny, const int org[3], co
     // offsets are relative to (i-1, j, k-1)
     if (interior(i - 1, j, k - 1, 4, 2, 4)) {
       for (int c = 0; c < 4; ++c) {
+        if (!mine(c)) continue;
         for (int b = 0; b < 2; ++b) {
           const R weight = width * (wy[b] * wz[c] + yz * sy[b] * slope[c]);
           for (int aa = 0; aa < 3; ++aa) fast(0, aa, b, c, weight * edge[aa]);
@@ -205,12 +226,14 @@ __device__ __forceinline__ void segmentTaps(int nx, int ny, const int org[3], co
       }
       for (int b = 0; b < 2; ++b)
         for (int aa = 0; aa < 4; ++aa) {
+          if (!mine(4 * b + aa)) continue;
           const R weight = width * (wx[aa] * wy[b] + xy * slope[aa] * sy[b]);
           for (int c = 0; c < 3; ++c) fast(2, aa, b, c, weight * edge[c]);
         }
       return;
     }
     for (int c = 0; c < 4; ++c) {
+      if (!mine(c)) continue;
       for (int b = 0; b < 2; ++b) {
         const R weight = width * (wy[b] * wz[c] + yz * sy[b] * slope[c]);
         for (int aa = 0; aa < 3; ++aa) visit(0, i - 1 + aa, j + b, k - 1 + c, weight * edge[aa]);
@@ -219,11 +242,13 @@ __device__ __forceinline__ void segmentTaps(int nx, int ny, const int org[3], co
     }
     for (int b = 0; b < 2; ++b)
       for (int aa = 0; aa < 4; ++aa) {
+        if (!mine(4 * b + aa)) continue;
         const R weight = width * (wx[aa] * wy[b] + xy * slope[aa] * sy[b]);
         for (int c = 0; c < 3; ++c) visit(2, i - 1 + aa, j + b, k - 1 + c, weight * edge[c]);
       }
   } else {
     for (int t = 0; t < 4; ++t) {
+      if (!mine(t)) continue;
       const R node = static_cast<R>(gaussNode(t)), gw = static_cast<R>(gaussWeight(t));
       const R u = R(.5) * (last + first + width * node), factor = R(.5) * width * gw;
       const R xx = x[0] + u * v[0], yy = x[1] + u * v[1], zz = x[2] + u * v[2];

@@ -379,6 +379,8 @@ class Stepper:
         the stepper there."""
         f, win = self.fields, self.window
         n = self.step_index
+        # the sort phase runs on its own stream beside the field updates
+        self.coupling.prepareStep(self.store, self.origin)
         f.clearCurrent()
         win.beginStep(n, self.offset)
         f.advanceMagnetic()

@@ -345,6 +345,16 @@ aprl_status aprl_coupling_boundary_counters(aprl_coupling* coupling, uint64_t* c
  * pushDepositAndMigrate (simulation.cpp:4310-4917). */
 aprl_status aprl_coupling_step(aprl_coupling* coupling, aprl_particles* particles, const double* origin,
                                double time, unsigned substeps, aprl_step_report* report);
+/* Optional, no counterpart in the reference (whose particle loop follows
+ * the field updates on the same threads): starts the sort phase of the
+ * next aprl_coupling_step (tile keys, stable tile sort, state permute,
+ * chunk lists) on the coupling's own stream, ordered after everything
+ * already enqueued on the grid, so that it runs while the field work
+ * enqueued after it (clear_current, begin_step, advance_magnetic)
+ * proceeds. Call it before those; the step then uses the prepared sort if
+ * it is called with the same store, unwritten since, and the same origin,
+ * and otherwise waits for it and sorts again. */
+aprl_status aprl_coupling_prepare_step(aprl_coupling* coupling, aprl_particles* particles, const double* origin);
 /* FP32-split grids only (no counterpart in the reference, which has no FP32
  * form): the Picard loop of aprl_coupling_step runs exactly `trials` trials
  * (2..64, default 2, the reference's minimum with active particles) in
