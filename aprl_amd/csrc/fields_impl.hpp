@@ -412,12 +412,9 @@ __global__ void __launch_bounds__(256) kMagneticCowanEdges(FieldPtrsT<R> F, Step
   if (u < nx * nz) magneticBy<true>(F, c, P, u % nx, ny, u / nx);
 }
 
-// Tiled Cowan magnetic update for i < nx, j < ny, one workgroup per
-// (kMX x kMY) tile of one z plane k. E planes k-1, k, k+1 of the tile plus a
-// one-node halo are staged in LDS as electricValue() samples (reflection /
-// halo / PEC parity applied while loading; all loads of a lane are issued
-// before the first store); each smoothed sample is then the same nine
-// products in the same order as smoothedElectric().
+// Tile of the Cowan magnetic update: kMX x kMY nodes of a z plane, staged
+// with a one-node halo (kLX x kLY) for each of three planes and three E
+// components.
 constexpr int kMX = 32, kMY = 8;
 constexpr int kLX = kMX + 2, kLY = kMY + 2, kLP = kLX * kLY;
 constexpr int kStageN = 9 * kLP;  // 3 components x 3 planes
@@ -492,69 +489,106 @@ __device__ __forceinline__ const R* planeBaseE(const FieldPtrsT<R>& F, int k, in
   return F.e[A] + static_cast<size_t>(k) * n1 * n0;
 }
 
+// z-marching Cowan magnetic update for i < nx, j < ny: one workgroup per
+// (kMX x kMY) tile and run of kMZ z planes [kb, ke). A ring of three LDS
+// planes per E component holds planes k-1, k, k+1 of the tile plus a
+// one-node halo as electricValue() samples (reflection / halo / PEC parity
+// applied while loading); plane k+2 is loaded into registers before plane
+// k is computed and stored into the ring slot of plane k-1 afterwards, so
+// every E plane of the run is read from memory once and its load latency
+// overlaps the smoothing of the plane before. Each smoothed sample is the
+// same nine products in the same order as smoothedElectric().
+constexpr int kMZ = 8;
 template <class R>
-__global__ void __launch_bounds__(kMX * kMY) kMagneticCowanTiled(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P) {
-  __shared__ R buf[kStageN];  // [component][plane k-1, k, k+1][kLY][kLX]
+__global__ void __launch_bounds__(kMX * kMY) kMagneticCowanMarch(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P) {
+  __shared__ R buf[kStageN];  // [component][ring slot][kLY][kLX]
   const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
-  const int i0 = blockIdx.x * kMX, j0 = blockIdx.y * kMY, k = blockIdx.z;
+  const int i0 = blockIdx.x * kMX, j0 = blockIdx.y * kMY;
+  const int kb = blockIdx.z * kMZ, ke = min(kb + kMZ, nz + 1);
   const int tx = static_cast<int>(threadIdx.x) % kMX, ty = static_cast<int>(threadIdx.x) / kMX;
   const int i = i0 + tx, j = j0 + ty;
-  // Staging as electricValue() samples: the region's node (gi, gj) is the
-  // same for the nine (component, plane) pairs; each component maps it to
-  // the node evalE() reads (in-plane index and sign per lane, the plane
-  // base per block), and every load is issued before the first store.
+  // The region node (gi, gj) of a lane is the same on every plane: its
+  // in-plane index and sign per component are computed once.
   constexpr int kRegionR = (kLP + kMX * kMY - 1) / (kMX * kMY);
-  const R* zbase[9];
-  int zsign[9];
-#pragma unroll
-  for (int pz = 0; pz < 3; ++pz) {
-    zbase[pz] = planeBaseE<0>(F, k - 1 + pz, zsign[pz]);
-    zbase[3 + pz] = planeBaseE<1>(F, k - 1 + pz, zsign[3 + pz]);
-    zbase[6 + pz] = planeBaseE<2>(F, k - 1 + pz, zsign[6 + pz]);
-  }
-  R v[kRegionR][9];
-  int sxy[kRegionR][3];
+  int ix[kRegionR][3], sxy[kRegionR][3];
 #pragma unroll
   for (int r = 0; r < kRegionR; ++r) {
     const int q = static_cast<int>(threadIdx.x) + r * kMX * kMY;
     const int qc = q < kLP ? q : 0;
     const int gi = i0 - 1 + qc % kLX, gj = j0 - 1 + qc / kLX;
-    int ix[3];
-    ix[0] = planeIndexE<0>(F, gi, gj, sxy[r][0]);
-    ix[1] = planeIndexE<1>(F, gi, gj, sxy[r][1]);
-    ix[2] = planeIndexE<2>(F, gi, gj, sxy[r][2]);
-#pragma unroll
-    for (int cp = 0; cp < 9; ++cp) v[r][cp] = zbase[cp][ix[cp / 3]];
+    ix[r][0] = planeIndexE<0>(F, gi, gj, sxy[r][0]);
+    ix[r][1] = planeIndexE<1>(F, gi, gj, sxy[r][1]);
+    ix[r][2] = planeIndexE<2>(F, gi, gj, sxy[r][2]);
   }
+  // One plane's samples of this lane's region nodes, loads only.
+  auto load = [&](int k, R v[kRegionR][3], int zs[3]) {
+    const R* base[3];
+    base[0] = planeBaseE<0>(F, k, zs[0]);
+    base[1] = planeBaseE<1>(F, k, zs[1]);
+    base[2] = planeBaseE<2>(F, k, zs[2]);
 #pragma unroll
-  for (int r = 0; r < kRegionR; ++r) {
-    const int q = static_cast<int>(threadIdx.x) + r * kMX * kMY;
-    if (q < kLP)
+    for (int r = 0; r < kRegionR; ++r)
 #pragma unroll
-      for (int cp = 0; cp < 9; ++cp) {
-        const int sg = sxy[r][cp / 3] * zsign[cp];
-        buf[cp * kLP + q] = sg == 0 ? R(0) : (sg > 0 ? v[r][cp] : -v[r][cp]);
-      }
+      for (int cp = 0; cp < 3; ++cp) v[r][cp] = base[cp][ix[r][cp]];
+  };
+  auto store = [&](int slot, const R v[kRegionR][3], const int zs[3]) {
+#pragma unroll
+    for (int r = 0; r < kRegionR; ++r) {
+      const int q = static_cast<int>(threadIdx.x) + r * kMX * kMY;
+      if (q < kLP)
+#pragma unroll
+        for (int cp = 0; cp < 3; ++cp) {
+          const int sg = sxy[r][cp] * zs[cp];
+          buf[(cp * 3 + slot) * kLP + q] = sg == 0 ? R(0) : (sg > 0 ? v[r][cp] : -v[r][cp]);
+        }
+    }
+  };
+  auto slotOf = [](int k) { return ((k % 3) + 3) % 3; };
+  {
+    R v0[kRegionR][3], v1[kRegionR][3], v2[kRegionR][3];
+    int z0[3], z1[3], z2[3];
+    load(kb - 1, v0, z0);
+    load(kb, v1, z1);
+    load(kb + 1, v2, z2);
+    store(slotOf(kb - 1), v0, z0);
+    store(slotOf(kb), v1, z1);
+    store(slotOf(kb + 1), v2, z2);
   }
   __syncthreads();
-  if (i >= nx || j >= ny) return;
-  const R* ex[3] = {buf, buf + kLP, buf + 2 * kLP};
-  const R* ey[3] = {buf + 3 * kLP, buf + 4 * kLP, buf + 5 * kLP};
-  const R* ez[3] = {buf + 6 * kLP, buf + 7 * kLP, buf + 8 * kLP};
   const int li = tx + 1, lj = ty + 1;
-  if (k < nz) {
-    // S_z is x-y smoothing: only its own plane is read
-    const R* eyk[3] = {ey[1], ey[1], ey[1]};
-    const R* eyk1[3] = {ey[2], ey[2], ey[2]};
-    const R* exk[3] = {ex[1], ex[1], ex[1]};
-    const R* exk1[3] = {ex[2], ex[2], ex[2]};
-    finishBx(F, c, P, i, j, k, smoothL<1>(ez, c, li, lj + 1) - smoothL<1>(ez, c, li, lj),
-             smoothL<2>(eyk1, c, li, lj) - smoothL<2>(eyk, c, li, lj));
-    finishBy(F, c, P, i, j, k, smoothL<2>(exk1, c, li, lj) - smoothL<2>(exk, c, li, lj),
-             smoothL<0>(ez, c, li + 1, lj) - smoothL<0>(ez, c, li, lj));
+  const bool owner = i < nx && j < ny;
+  for (int k = kb; k < ke; ++k) {
+    R next[kRegionR][3];
+    int zn[3] = {0, 0, 0};
+    const bool more = k + 1 < ke;  // plane k + 2 feeds plane k + 1 of this run
+    if (more) load(k + 2, next, zn);
+    if (owner) {
+      const R* ex[3] = {buf + (0 * 3 + slotOf(k - 1)) * kLP, buf + (0 * 3 + slotOf(k)) * kLP,
+                        buf + (0 * 3 + slotOf(k + 1)) * kLP};
+      const R* ey[3] = {buf + (1 * 3 + slotOf(k - 1)) * kLP, buf + (1 * 3 + slotOf(k)) * kLP,
+                        buf + (1 * 3 + slotOf(k + 1)) * kLP};
+      const R* ez[3] = {buf + (2 * 3 + slotOf(k - 1)) * kLP, buf + (2 * 3 + slotOf(k)) * kLP,
+                        buf + (2 * 3 + slotOf(k + 1)) * kLP};
+      if (k < nz) {
+        // S_z is x-y smoothing: only its own plane is read
+        const R* eyk[3] = {ey[1], ey[1], ey[1]};
+        const R* eyk1[3] = {ey[2], ey[2], ey[2]};
+        const R* exk[3] = {ex[1], ex[1], ex[1]};
+        const R* exk1[3] = {ex[2], ex[2], ex[2]};
+        finishBx(F, c, P, i, j, k, smoothL<1>(ez, c, li, lj + 1) - smoothL<1>(ez, c, li, lj),
+                 smoothL<2>(eyk1, c, li, lj) - smoothL<2>(eyk, c, li, lj));
+        finishBy(F, c, P, i, j, k, smoothL<2>(exk1, c, li, lj) - smoothL<2>(exk, c, li, lj),
+                 smoothL<0>(ez, c, li + 1, lj) - smoothL<0>(ez, c, li, lj));
+      }
+      finishBz(F, c, P, i, j, k, smoothL<0>(ey, c, li + 1, lj) - smoothL<0>(ey, c, li, lj),
+               smoothL<1>(ex, c, li, lj + 1) - smoothL<1>(ex, c, li, lj));
+    }
+    if (more) {
+      __syncthreads();  // every lane is done with plane k - 1
+      store(slotOf(k + 2), next, zn);
+      __syncthreads();
+    }
   }
-  finishBz(F, c, P, i, j, k, smoothL<0>(ey, c, li + 1, lj) - smoothL<0>(ey, c, li, lj),
-           smoothL<1>(ex, c, li, lj + 1) - smoothL<1>(ex, c, li, lj));
 }
 
 template <class R>
@@ -707,8 +741,8 @@ void launchMagnetic(FieldStateT<R>& s, const FieldPtrsT<R>& f) {
   const CpmlArgsT<R> P = s.cpmlArgs();
   ScopedKernel sk(s.profiler, s.stream, "magnetic");
   if (s.g.solver == 1) {
-    dim3 tg(ceilDiv(s.g.nx, kMX), ceilDiv(s.g.ny, kMY), static_cast<unsigned>(s.g.nz + 1));
-    kMagneticCowanTiled<R><<<tg, kMX * kMY, 0, s.stream>>>(f, c, P);
+    dim3 tg(ceilDiv(s.g.nx, kMX), ceilDiv(s.g.ny, kMY), ceilDiv(s.g.nz + 1, kMZ));
+    kMagneticCowanMarch<R><<<tg, kMX * kMY, 0, s.stream>>>(f, c, P);
     kMagneticCowanEdges<R><<<ceilDiv((s.g.nx + s.g.ny) * s.g.nz, 256), 256, 0, s.stream>>>(f, c, P);
   } else {
     dim3 grid(ceilDiv((s.g.nx + 1) * (s.g.ny + 1), 256), static_cast<unsigned>(s.g.nz + 1));
