@@ -171,6 +171,11 @@ def lib() -> C.CDLL:
     L.aprl_coupling_step.argtypes = [vp, vp, _dp, C.c_double, C.c_uint, C.POINTER(StepReport)]
     L.aprl_coupling_set_split_trials.argtypes = [vp, C.c_uint, C.c_int]
     L.aprl_coupling_prepare_step.argtypes = [vp, vp, _dp]
+    L.aprl_lab_plane_create.argtypes = [vp, C.c_double, C.c_double, _sz, C.POINTER(vp)]
+    L.aprl_lab_plane_destroy.argtypes = [vp]
+    L.aprl_lab_plane_destroy.restype = None
+    L.aprl_lab_plane_sample.argtypes = [vp, C.c_double, _dp, C.POINTER(C.c_int)]
+    L.aprl_lab_plane_fetch.argtypes = [vp, _sz, _dp, _dp, C.POINTER(_sz)]
     L.aprl_window_step_batch.argtypes = [C.POINTER(InstanceStep), _sz]
     L.aprl_fields_interior_energy.argtypes = [vp, C.POINTER(_sz), C.POINTER(_sz), _dp]
     L.aprl_fields_face_power.argtypes = [vp, C.POINTER(_sz), C.POINTER(_sz), _dp]
@@ -228,6 +233,7 @@ EXPORTED = [
     "aprl_coupling_create", "aprl_coupling_destroy", "aprl_coupling_set_frame",
     "aprl_coupling_add_magnet", "aprl_coupling_set_particle_boundary", "aprl_coupling_step",
     "aprl_coupling_set_split_trials", "aprl_coupling_prepare_step", "aprl_window_step_batch",
+    "aprl_lab_plane_create", "aprl_lab_plane_destroy", "aprl_lab_plane_sample", "aprl_lab_plane_fetch",
     "aprl_fields_interior_energy", "aprl_fields_face_power", "aprl_coupling_continuity",
     "aprl_coupling_continuity_noalloc", "aprl_coupling_gauss",
 ]

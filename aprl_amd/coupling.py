@@ -214,3 +214,44 @@ class ConservativeSourceCoupling:
         check(lib().aprl_coupling_gauss(self.handle, particles.handle, dptr(o), _capi.size3(lo), _capi.size3(hi),
                                         dptr(out)))
         return tuple(out)
+
+
+class LabFieldPlane:
+    """One laboratory field plane (FieldDetectorPlaneConfig: lab z and lab-time
+    rhythm) of LabDetectorManager::sampleFieldPlanes (lab_detectors.cpp:
+    2426-2487), sampled on the device with the coupling's frame and grid."""
+
+    def __init__(self, coupling: ConservativeSourceCoupling, z_lab: float, rhythm: float, capacity: int = 64):
+        self.coupling = coupling
+        g = coupling.fields.geometry
+        self.points = g.nx * g.ny
+        self.shape = (g.ny, g.nx, 6)
+        self.capacity = capacity
+        h = C.c_void_p()
+        check(lib().aprl_lab_plane_create(coupling.handle, float(z_lab), float(rhythm), capacity, C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().aprl_lab_plane_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def sample(self, time_box: float, origin) -> bool:
+        """sampleFieldPlanes at the step's end box time and shifted origin."""
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        taken = C.c_int(0)
+        check(lib().aprl_lab_plane_sample(self.handle, float(time_box), dptr(o), C.byref(taken)))
+        return bool(taken.value)
+
+    def fetch(self):
+        """(lab times (n,), records (n, ny, nx, 6): E_lab then B_lab) of the
+        samples taken since the last fetch."""
+        times = np.zeros(self.capacity)
+        records = np.zeros((self.capacity, *self.shape))
+        n = _capi._sz(0)
+        check(lib().aprl_lab_plane_fetch(self.handle, self.capacity, dptr(times), dptr(records.reshape(-1)),
+                                         C.byref(n)))
+        return times[:n.value].copy(), records[:n.value].copy()

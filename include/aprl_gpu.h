@@ -472,6 +472,30 @@ aprl_status aprl_window_enable_energy_ledger(aprl_fields* fields, int enable);
 aprl_status aprl_window_finish_energy_step(aprl_fields* fields);
 aprl_status aprl_window_energy_ledger(aprl_fields* fields, double* out);
 
+/* Laboratory field planes: LabDetectorManager::sampleFieldPlanes
+ * (lab_detectors.cpp:2426-2487) for one field plane at fixed lab z, on one
+ * rank, with samplePlane (2836-2898) and the detector's clamped trilinear
+ * stencil (sampleDetectorComponent / interpolationAxis, 33-97); no
+ * external background, as the simulation constructs its manager
+ * (simulation.cpp:3291-3293). The plane uses the coupling's boost frame
+ * and field grid. aprl_lab_plane_sample is called where the simulation
+ * calls sampleFieldPlanes: after the window shift, with the step's end box
+ * time and the shifted box origin. When the plane lies on the grid
+ * (ownerRank) and its lab-time rhythm is due, it samples every cell centre
+ * (i, j) into a device ring of `capacity` samples (*taken = 1): records
+ * j-major, per point E_lab[3] then B_lab[3] (FieldPointRecord). A full
+ * ring is an APRL_LOGIC_ERROR. aprl_lab_plane_fetch synchronizes the grid's
+ * stream, copies the pending samples (count <= capacity of the caller's
+ * buffers: time_lab[count], records[count * nx * ny * 6]) and empties the
+ * ring. rhythm must be positive (config.cpp:1713-1717). */
+typedef struct aprl_lab_plane aprl_lab_plane;
+aprl_status aprl_lab_plane_create(aprl_coupling* coupling, double z_lab, double rhythm, size_t capacity,
+                                  aprl_lab_plane** out);
+void aprl_lab_plane_destroy(aprl_lab_plane* plane);
+aprl_status aprl_lab_plane_sample(aprl_lab_plane* plane, double time_box, const double* origin_box, int* taken);
+aprl_status aprl_lab_plane_fetch(aprl_lab_plane* plane, size_t capacity, double* time_lab, double* records,
+                                 size_t* count);
+
 /* Per-kernel timing with HIP events on the instance stream. enable=1 starts
  * recording every launch; query returns launches and total milliseconds for
  * a kernel name (0 launches if unknown). */

@@ -5,6 +5,7 @@
 #include <memory>
 #include <string>
 
+#include "orc_detectors.hpp"
 #include "orc_fields.hpp"
 #include "orc_particles.hpp"
 #include "orc_pending.hpp"
@@ -48,6 +49,40 @@ orc::CpmlParams params(const std::size_t cells[3], double order, double R, doubl
 extern "C" {
 
 const char* orc_last_error() { return lastError.c_str(); }
+
+// BoostFrameTransform (boostframe.cpp:199-203, 242-246, 267-285) for the
+// detector: out = {box z of the lab plane, its lab time, E_lab[3], B_lab[3]}.
+int orc_boost_lab(double gamma, double labT0, double labZ0, double boxZ0, double zLab, double tBox,
+                  const double* eBox, const double* bBox, double* out8) {
+  return guarded([&] {
+    const orc::LabFrame f = orc::LabFrame::fromGamma(gamma, labT0, labZ0, boxZ0);
+    out8[0] = f.boxZFromLabZAndBoxT(zLab, tBox);
+    out8[1] = f.labTFromLabZBoxT(zLab, tBox);
+    f.fieldsBoxToLab(eBox, bBox, out8 + 2, out8 + 5);
+  });
+}
+
+// One laboratory field plane (lab_detectors.cpp:2426-2487, one rank).
+int orc_lab_plane_new(double zLab, double rhythm, void** out) {
+  return guarded([&] {
+    // config.cpp:1713-1717
+    if (!(rhythm > 0.0) || !std::isfinite(rhythm)) throw std::invalid_argument("field detector rhythm must be positive");
+    auto* p = new orc::LabPlaneState;
+    p->z = zLab;
+    p->rhythm = rhythm;
+    *out = p;
+  });
+}
+void orc_lab_plane_free(void* h) { delete static_cast<orc::LabPlaneState*>(h); }
+// frame4 = (gamma, lab time origin, lab z origin, box z origin); records:
+// nx * ny * 6 doubles; *taken = 1 when this call sampled.
+int orc_lab_plane_sample(void* plane, void* fields, const double* frame4, double timeBox, const double* origin,
+                         double* records, double* timeLab, int* taken) {
+  return guarded([&] {
+    const orc::LabFrame f = orc::LabFrame::fromGamma(frame4[0], frame4[1], frame4[2], frame4[3]);
+    *taken = static_cast<orc::LabPlaneState*>(plane)->sample(*F(fields), f, timeBox, origin, records, *timeLab) ? 1 : 0;
+  });
+}
 
 int orc_cowan(double dx, double dy, double dz, double* out12) {
   return guarded([&] {

@@ -107,6 +107,7 @@ extern "C" {
 
 const char* ref_last_error() { return lastError.c_str(); }
 
+
 // configureComputeThreads (compute_threads.cpp:50-91) for the CPU baseline.
 int ref_set_threads(int threads) {
   return guarded([&] {
@@ -310,6 +311,28 @@ FieldVector<Double> fv(const double* p) {
 }  // namespace
 
 extern "C" {
+
+// The reference's BoostFrameTransform for the detector: out = {box z of the
+// lab plane, its lab time, E_lab[3], B_lab[3]}.
+int ref_boost_lab(double gamma, double labT0, double labZ0, double boxZ0, double zLab, double tBox,
+                  const double* eBox, const double* bBox, double* out8) {
+  return guarded([&] {
+    BoostFrameTransform f;
+    f.setOriginsFromGamma(gamma, SI::c, labT0, labZ0, boxZ0);
+    out8[0] = f.boxZFromLabZAndBoxT(zLab, tBox);
+    out8[1] = f.labTFromLabZBoxT(zLab, tBox);
+    FieldVector<Double> e(0.0), b(0.0), el(0.0), bl(0.0);
+    for (int a = 0; a < 3; ++a) {
+      e[a] = eBox[a];
+      b[a] = bBox[a];
+    }
+    f.fieldsBoxToLab(e, b, el, bl);
+    for (int a = 0; a < 3; ++a) {
+      out8[2 + a] = el[a];
+      out8[5 + a] = bl[a];
+    }
+  });
+}
 
 int ref_coupling_new(void* fields, int shape, int smoothX, void** out) {
   return guarded([&] {
