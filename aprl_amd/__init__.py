@@ -3,6 +3,7 @@ loop, driven through the C-ABI of include/aprl_gpu.h."""
 from ._capi import AprlError, InvalidArgument, LogicError, OutOfRange, DeviceError, lib  # noqa: F401
 from .fields import (EBCPMLParameters, EBFieldGrid, EBGridGeometry, MovingWindowFields,  # noqa: F401
                      component_shape)
-from .coupling import ConservativeSourceCoupling, LabFieldPlane, ParticleStore  # noqa: F401
+from .coupling import (ConservativeSourceCoupling, EBGaussFieldInitializer, LabFieldPlane,  # noqa: F401
+                       ParticleStore)
 
 __all__ = ["EBFieldGrid", "EBGridGeometry", "EBCPMLParameters", "MovingWindowFields", "AprlError", "lib"]

@@ -48,6 +48,15 @@ class StepReport(C.Structure):
                 ("chunks", C.c_uint), ("active_tiles", C.c_uint)]
 
 
+class GaussReport(C.Structure):
+    """aprl_gauss_report (EBGaussInitializationReport, eb_initial_field.h:17-35)."""
+    _fields_ = [("iterations", C.c_uint64), ("relative_residual", C.c_double), ("maximum_gauss_residual", C.c_double),
+                ("total_charge", C.c_double), ("mean_beta_z", C.c_double), ("mean_beta_transverse", C.c_double),
+                ("initial_electric_energy", C.c_double), ("initial_magnetic_energy", C.c_double),
+                ("rms_position_cells", C.c_double * 3), ("centre_padding_rms", C.c_double * 3),
+                ("static_field_guard_cells", C.c_uint64 * 3), ("transverse_poisson_padding_cells", C.c_uint64)]
+
+
 class InstanceStep(C.Structure):
     """aprl_instance_step: one instance of a batched window step."""
     _fields_ = [("fields", C.c_void_p), ("coupling", C.c_void_p), ("particles", C.c_void_p), ("step", C.c_size_t),
@@ -171,6 +180,8 @@ def lib() -> C.CDLL:
     L.aprl_coupling_step.argtypes = [vp, vp, _dp, C.c_double, C.c_uint, C.POINTER(StepReport)]
     L.aprl_coupling_set_split_trials.argtypes = [vp, C.c_uint, C.c_int]
     L.aprl_coupling_prepare_step.argtypes = [vp, vp, _dp]
+    L.aprl_fields_initialize_gauss.argtypes = [vp, vp, _dp, C.c_int, C.c_double, C.POINTER(_sz), C.c_double, _sz,
+                                               C.c_int, C.POINTER(GaussReport)]
     L.aprl_lab_plane_create.argtypes = [vp, C.c_double, C.c_double, _sz, C.POINTER(vp)]
     L.aprl_lab_plane_destroy.argtypes = [vp]
     L.aprl_lab_plane_destroy.restype = None
@@ -234,6 +245,7 @@ EXPORTED = [
     "aprl_coupling_add_magnet", "aprl_coupling_set_particle_boundary", "aprl_coupling_step",
     "aprl_coupling_set_split_trials", "aprl_coupling_prepare_step", "aprl_window_step_batch",
     "aprl_lab_plane_create", "aprl_lab_plane_destroy", "aprl_lab_plane_sample", "aprl_lab_plane_fetch",
+    "aprl_fields_initialize_gauss",
     "aprl_fields_interior_energy", "aprl_fields_face_power", "aprl_coupling_continuity",
     "aprl_coupling_continuity_noalloc", "aprl_coupling_gauss",
 ]

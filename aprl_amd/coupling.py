@@ -255,3 +255,31 @@ class LabFieldPlane:
         check(lib().aprl_lab_plane_fetch(self.handle, self.capacity, dptr(times), dptr(records.reshape(-1)),
                                          C.byref(n)))
         return times[:n.value].copy(), records[:n.value].copy()
+
+
+class EBGaussFieldInitializer:
+    """EBGaussFieldInitializer (eb_initial_field.h:37-70) on the device: the
+    Gauss-consistent initial E/B of the store's particles written into the
+    grid (aprl_fields_initialize_gauss)."""
+
+    MODELS = {"relativistic-poisson": 0, "electrostatic-poisson": 1, "cowan-relativistic-poisson": 2}
+
+    @staticmethod
+    def initialize(fields, particles: ParticleStore, origin, model: str = "relativistic-poisson",
+                   maximumMeanTransverseBeta: float = 1e-8, staticFieldGuardCells=(0, 0, 0),
+                   relativeTolerance: float = 1e-10, maximumIterations: int = 10000,
+                   includeTransverseCpmlParticles: bool = False) -> dict:
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        r = _capi.GaussReport()
+        check(lib().aprl_fields_initialize_gauss(fields.handle, particles.handle, dptr(o),
+                                                 EBGaussFieldInitializer.MODELS[model],
+                                                 float(maximumMeanTransverseBeta), _capi.size3(staticFieldGuardCells),
+                                                 float(relativeTolerance), int(maximumIterations),
+                                                 int(includeTransverseCpmlParticles), C.byref(r)))
+        return {"iterations": r.iterations, "relativeResidual": r.relative_residual,
+                "maximumGaussResidual": r.maximum_gauss_residual, "totalCharge": r.total_charge,
+                "meanBetaZ": r.mean_beta_z, "meanBetaTransverse": r.mean_beta_transverse,
+                "initialElectricEnergy": r.initial_electric_energy, "initialMagneticEnergy": r.initial_magnetic_energy,
+                "rmsPositionCells": list(r.rms_position_cells), "centrePaddingRms": list(r.centre_padding_rms),
+                "staticFieldGuardCells": list(r.static_field_guard_cells),
+                "transversePoissonPaddingCells": r.transverse_poisson_padding_cells}

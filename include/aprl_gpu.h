@@ -496,6 +496,44 @@ aprl_status aprl_lab_plane_sample(aprl_lab_plane* plane, double time_box, const 
 aprl_status aprl_lab_plane_fetch(aprl_lab_plane* plane, size_t capacity, double* time_lab, double* records,
                                  size_t* count);
 
+/* Gauss-consistent initial self-field: EBGaussFieldInitializer::initialize
+ * (eb_initial_field.cpp:829-1095) for one rank, with the store's active and
+ * held entries as the source (positions in m relative to the grid's box
+ * origin `origin`). The CIC charge drives a conjugate-gradient solve of the
+ * relativistic-Poisson (7-point, 1 - beta_z^2 along z), electrostatic or
+ * Cowan relativistic-Poisson (27-point) operator with zero potential on the
+ * static-field guard planes (guard_cells = the CPML cells per axis; with
+ * include_transverse_cpml_particles the transverse guard is replaced by a
+ * two-cell padded lattice and the fields outside the physical region are
+ * cleared); E is the (Cowan-smoothed) potential gradient and B the
+ * rigid-translation field at -dt/2, stored into the grid's E and B
+ * (replacing them). The post-solve discrete Gauss residual must stay within
+ * 4 x relative_tolerance. Errors carry the reference's messages: a charge
+ * cloud touching the guard, a mean transverse beta above
+ * max_mean_transverse_beta, an indefinite Cowan operator, no convergence
+ * within maximum_iterations (all APRL_RUNTIME_ERROR), positions outside
+ * the grid (APRL_OUT_OF_RANGE). Defaults of the reference's configuration:
+ * relativistic-poisson, tolerance 1e-10, max transverse beta 1e-8, 10000
+ * iterations (config.cpp:752-757); the moving window requires the Cowan
+ * model (config.cpp:2055-2060). No held-transverse mean velocity. */
+typedef enum aprl_initial_model {
+  APRL_INITIAL_RELATIVISTIC_POISSON = 0,
+  APRL_INITIAL_ELECTROSTATIC_POISSON = 1,
+  APRL_INITIAL_COWAN_RELATIVISTIC_POISSON = 2
+} aprl_initial_model;
+typedef struct aprl_gauss_report {
+  uint64_t iterations;
+  double relative_residual, maximum_gauss_residual, total_charge, mean_beta_z, mean_beta_transverse,
+      initial_electric_energy, initial_magnetic_energy;
+  double rms_position_cells[3], centre_padding_rms[3];
+  uint64_t static_field_guard_cells[3], transverse_poisson_padding_cells;
+} aprl_gauss_report;
+aprl_status aprl_fields_initialize_gauss(aprl_fields* fields, aprl_particles* particles, const double* origin,
+                                         aprl_initial_model model, double max_mean_transverse_beta,
+                                         const size_t* guard_cells, double relative_tolerance,
+                                         size_t maximum_iterations, int include_transverse_cpml_particles,
+                                         aprl_gauss_report* report);
+
 /* Per-kernel timing with HIP events on the instance stream. enable=1 starts
  * recording every launch; query returns launches and total milliseconds for
  * a kernel name (0 launches if unknown). */

@@ -6,6 +6,7 @@
 #include <string>
 
 #include "orc_detectors.hpp"
+#include "orc_initial_field.hpp"
 #include "orc_fields.hpp"
 #include "orc_particles.hpp"
 #include "orc_pending.hpp"
@@ -59,6 +60,33 @@ int orc_boost_lab(double gamma, double labT0, double labZ0, double boxZ0, double
     out8[0] = f.boxZFromLabZAndBoxT(zLab, tBox);
     out8[1] = f.labTFromLabZBoxT(zLab, tBox);
     f.fieldsBoxToLab(eBox, bBox, out8 + 2, out8 + 5);
+  });
+}
+
+// EBGaussFieldInitializer::initialize restated (orc_initial_field.hpp);
+// the same arguments and report layout as ref_gauss_initialize.
+int orc_gauss_initialize(void* h, const double* origin, std::size_t n, const double* x3, const double* u3,
+                         const double* q, const double* m, int model, double maxBetaT, const std::size_t* guard,
+                         double tolerance, std::size_t maxIterations, int includeCpml, double* report18) {
+  return guarded([&] {
+    orc::InitialParticles P;
+    P.n = n;
+    P.x = x3;
+    P.u = u3;
+    P.q = q;
+    P.m = m;
+    const orc::GaussReport r = orc::initializeGauss(*F(h), origin, P, static_cast<orc::InitialModel>(model),
+                                                    maxBetaT, guard, tolerance, maxIterations, includeCpml != 0);
+    const double head[8] = {static_cast<double>(r.iterations), r.relativeResidual, r.maximumGaussResidual,
+                            r.totalCharge, r.meanBetaZ, r.meanBetaTransverse, r.initialElectricEnergy,
+                            r.initialMagneticEnergy};
+    for (int a = 0; a < 8; ++a) report18[a] = head[a];
+    for (int a = 0; a < 3; ++a) {
+      report18[8 + a] = r.rmsPositionCells[a];
+      report18[11 + a] = r.centrePaddingRms[a];
+      report18[14 + a] = static_cast<double>(r.staticFieldGuardCells[a]);
+    }
+    report18[17] = static_cast<double>(r.transversePoissonPaddingCells);
   });
 }
 

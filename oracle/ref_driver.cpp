@@ -262,6 +262,7 @@ double ref_fields_poynting(void* h, std::size_t i, std::size_t j, std::size_t k,
 #include <unistd.h>
 
 #include "boostframe.h"
+#include "eb_initial_field.h"
 #include "compute_threads.h"
 #include "eb_deposition.h"
 #include "eb_particles.h"
@@ -314,6 +315,49 @@ extern "C" {
 
 // The reference's BoostFrameTransform for the detector: out = {box z of the
 // lab plane, its lab time, E_lab[3], B_lab[3]}.
+// EBGaussFieldInitializer::initialize (eb_initial_field.cpp:829-1095) on
+// one rank; report18 = iterations, relative residual, maximum Gauss
+// residual, total charge, mean beta_z, mean beta_transverse, E and B
+// energies, rms position (cells, 3), centre padding (rms, 3), guard cells
+// (3), transverse padding.
+int ref_gauss_initialize(void* h, const double* origin, std::size_t n, const double* x3, const double* u3,
+                         const double* q, const double* m, int model, double maxBetaT, const std::size_t* guard,
+                         double tolerance, std::size_t maxIterations, int includeCpml, double* report18) {
+  return guarded([&] {
+    ensureMpi();
+    EBFieldGrid& f = *R(h)->f;
+    std::vector<RelativisticParticleSI> particles(n);
+    for (std::size_t p = 0; p < n; ++p) {
+      for (int a = 0; a < 3; ++a) {
+        particles[p].position[a] = x3[3 * p + a];
+        particles[p].properVelocity[a] = u3[3 * p + a];
+      }
+      particles[p].charge = q[p];
+      particles[p].mass = m[p];
+      particles[p].weight = 1.0;
+      particles[p].id = p + 1;
+    }
+    FieldVector<Double> o(0.0);
+    for (int a = 0; a < 3; ++a) o[a] = origin[a];
+    const InitialSelfFieldModel models[3] = {InitialSelfFieldModel::RelativisticPoisson,
+                                             InitialSelfFieldModel::ElectrostaticPoisson,
+                                             InitialSelfFieldModel::CowanRelativisticPoisson};
+    const EBGaussInitializationReport r = EBGaussFieldInitializer::initialize(
+        f, f.geometry(), 0, o, particles, models[model], maxBetaT, guard, tolerance, maxIterations, MPI_COMM_WORLD,
+        includeCpml != 0, false, nullptr);
+    const double head[8] = {static_cast<double>(r.iterations), r.relativeResidual, r.maximumGaussResidual,
+                            r.totalCharge, r.meanBetaZ, r.meanBetaTransverse, r.initialElectricEnergy,
+                            r.initialMagneticEnergy};
+    for (int a = 0; a < 8; ++a) report18[a] = head[a];
+    for (int a = 0; a < 3; ++a) {
+      report18[8 + a] = r.rmsPositionCells[a];
+      report18[11 + a] = r.centrePaddingRms[a];
+      report18[14 + a] = static_cast<double>(r.staticFieldGuardCells[a]);
+    }
+    report18[17] = static_cast<double>(r.transversePoissonPaddingCells);
+  });
+}
+
 int ref_boost_lab(double gamma, double labT0, double labZ0, double boxZ0, double zLab, double tBox,
                   const double* eBox, const double* bBox, double* out8) {
   return guarded([&] {
