@@ -5,5 +5,6 @@ from .fields import (EBCPMLParameters, EBFieldGrid, EBGridGeometry, MovingWindow
                      component_shape)
 from .coupling import (ConservativeSourceCoupling, EBGaussFieldInitializer, LabFieldPlane,  # noqa: F401
                        ParticleStore)
+from .injection import InjectionPlanes, LabPlaneBunch, frozenLabPlaneInjection  # noqa: F401
 
 __all__ = ["EBFieldGrid", "EBGridGeometry", "EBCPMLParameters", "MovingWindowFields", "AprlError", "lib"]

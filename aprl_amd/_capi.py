@@ -57,6 +57,30 @@ class GaussReport(C.Structure):
                 ("static_field_guard_cells", C.c_uint64 * 3), ("transverse_poisson_padding_cells", C.c_uint64)]
 
 
+class LabPlaneProjection(C.Structure):
+    """aprl_lab_plane_projection (SILabPlaneProjectionReport, eb_bunch.h:36-47)."""
+    _fields_ = [("input_plane_lab", C.c_double), ("reference_position_lab", C.c_double),
+                ("minimum_forward_distance", C.c_double), ("maximum_forward_distance", C.c_double),
+                ("recommended_minimum_reference_position", C.c_double), ("reference_time_offset_lab", C.c_double),
+                ("mean_longitudinal_beta", C.c_double), ("particles", C.c_uint64)]
+
+
+class BunchPlacement(C.Structure):
+    """aprl_bunch_placement (SIBunchPlacementReport, eb_bunch.h:23-34)."""
+    _fields_ = [("relative_tail_lab", C.c_double), ("relative_head_lab", C.c_double),
+                ("reference_position_lab", C.c_double), ("head_after_lab", C.c_double),
+                ("actual_head_distance", C.c_double), ("recommended_maximum_reference_position", C.c_double),
+                ("particles", C.c_uint64)]
+
+
+class BunchBoostReport(C.Structure):
+    """aprl_bunch_boost_report (SIBunchBoostReport, eb_bunch.h:49-60)."""
+    _fields_ = [("earliest_lab_event_time", C.c_double), ("latest_lab_event_time", C.c_double),
+                ("maximum_absolute_drift_time", C.c_double), ("minimum_lab_gamma", C.c_double),
+                ("maximum_lab_gamma", C.c_double), ("maximum_relative_momentum_round_trip_error", C.c_double),
+                ("maximum_relative_gamma_round_trip_error", C.c_double)]
+
+
 class InstanceStep(C.Structure):
     """aprl_instance_step: one instance of a batched window step."""
     _fields_ = [("fields", C.c_void_p), ("coupling", C.c_void_p), ("particles", C.c_void_p), ("step", C.c_size_t),
@@ -188,6 +212,23 @@ def lib() -> C.CDLL:
     L.aprl_lab_plane_sample.argtypes = [vp, C.c_double, _dp, C.POINTER(C.c_int)]
     L.aprl_lab_plane_fetch.argtypes = [vp, _sz, _dp, _dp, C.POINTER(_sz)]
     L.aprl_window_step_batch.argtypes = [C.POINTER(InstanceStep), _sz]
+    L.aprl_bunch_create.argtypes = [_sz, C.POINTER(vp)]
+    L.aprl_bunch_destroy.argtypes = [vp]
+    L.aprl_bunch_destroy.restype = None
+    L.aprl_bunch_count.argtypes = [vp]
+    L.aprl_bunch_count.restype = _sz
+    L.aprl_bunch_upload.argtypes = [vp, _sz, _dp, _dp, _dp, _dp, _dp, u64p]
+    L.aprl_bunch_download.argtypes = [vp, _dp, _dp, _dp, _dp, _dp, u64p, _dp]
+    L.aprl_bunch_generate_plane_events.argtypes = [vp, C.POINTER(GaussianBeam), _sz]
+    L.aprl_bunch_project_lab_plane_events.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int,
+                                                      C.POINTER(LabPlaneProjection)]
+    L.aprl_bunch_place_relative.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double,
+                                            C.POINTER(BunchPlacement)]
+    L.aprl_bunch_frozen_frame_origin.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                                 C.POINTER(Boost)]
+    L.aprl_bunch_boost_to_box_time_zero.argtypes = [vp, C.POINTER(Boost), C.c_double, C.c_int, C.c_double,
+                                                    C.POINTER(BunchBoostReport)]
+    L.aprl_bunch_freeze.argtypes = [vp, C.POINTER(Boost), C.c_double, C.c_int]
     L.aprl_fields_interior_energy.argtypes = [vp, C.POINTER(_sz), C.POINTER(_sz), _dp]
     L.aprl_fields_face_power.argtypes = [vp, C.POINTER(_sz), C.POINTER(_sz), _dp]
     L.aprl_coupling_continuity.argtypes = [vp, vp, _dp]
@@ -246,6 +287,9 @@ EXPORTED = [
     "aprl_coupling_set_split_trials", "aprl_coupling_prepare_step", "aprl_window_step_batch",
     "aprl_lab_plane_create", "aprl_lab_plane_destroy", "aprl_lab_plane_sample", "aprl_lab_plane_fetch",
     "aprl_fields_initialize_gauss",
+    "aprl_bunch_create", "aprl_bunch_destroy", "aprl_bunch_count", "aprl_bunch_upload", "aprl_bunch_download",
+    "aprl_bunch_generate_plane_events", "aprl_bunch_project_lab_plane_events", "aprl_bunch_place_relative",
+    "aprl_bunch_frozen_frame_origin", "aprl_bunch_boost_to_box_time_zero", "aprl_bunch_freeze",
     "aprl_fields_interior_energy", "aprl_fields_face_power", "aprl_coupling_continuity",
     "aprl_coupling_continuity_noalloc", "aprl_coupling_gauss",
 ]

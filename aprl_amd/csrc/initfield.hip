@@ -20,6 +20,7 @@
 #include <limits>
 #include <sstream>
 
+#include "ddsum.hpp"
 #include "handles.hpp"
 #include "particle_device.hpp"
 
@@ -29,55 +30,7 @@ namespace {
 constexpr double kEpsilon = std::numeric_limits<double>::epsilon();
 constexpr int kThreads = 256, kBlocks = 1024;
 
-// double-double accumulator: value = hi + lo
-struct DD {
-  double hi = 0.0, lo = 0.0;
-};
-__device__ __forceinline__ DD ddAdd(DD a, double b) {
-  const double s = a.hi + b, bb = s - a.hi, e = (a.hi - (s - bb)) + (b - bb) + a.lo;
-  const double h = s + e;
-  return DD{h, e - (h - s)};
-}
-__device__ __forceinline__ DD ddAdd(DD a, DD b) {
-  const double s = a.hi + b.hi, bb = s - a.hi, e = (a.hi - (s - bb)) + (b.hi - bb) + a.lo + b.lo;
-  const double h = s + e;
-  return DD{h, e - (h - s)};
-}
-// exact product x * y added
-__device__ __forceinline__ DD ddAddProduct(DD a, double x, double y) {
-  const double p = x * y;
-  return ddAdd(ddAdd(a, p), fma(x, y, -p));
-}
-
-// Block reduction of K accumulators into out[K * blockIdx.x + c] (fixed tree).
-template <int K>
-__device__ void blockDD(DD v[K], DD* out) {
-  __shared__ double sh[K][kThreads], sl[K][kThreads];
-  for (int c = 0; c < K; ++c) {
-    sh[c][threadIdx.x] = v[c].hi;
-    sl[c][threadIdx.x] = v[c].lo;
-  }
-  __syncthreads();
-  for (int w = kThreads / 2; w > 0; w >>= 1) {
-    if (static_cast<int>(threadIdx.x) < w)
-      for (int c = 0; c < K; ++c) {
-        const DD r = ddAdd(DD{sh[c][threadIdx.x], sl[c][threadIdx.x]}, DD{sh[c][threadIdx.x + w], sl[c][threadIdx.x + w]});
-        sh[c][threadIdx.x] = r.hi;
-        sl[c][threadIdx.x] = r.lo;
-      }
-    __syncthreads();
-  }
-  if (threadIdx.x < static_cast<unsigned>(K)) out[K * blockIdx.x + threadIdx.x] = DD{sh[threadIdx.x][0], sl[threadIdx.x][0]};
-}
-
-// out[c] = sum over blocks of in[K * b + c], in block order (one thread each).
-__global__ void kFinishDD(const DD* in, int blocks, int K, DD* out) {
-  const int c = threadIdx.x;
-  if (c >= K) return;
-  DD s;
-  for (int b = 0; b < blocks; ++b) s = ddAdd(s, in[K * b + c]);
-  out[c] = s;
-}
+using namespace ddsum;
 
 __device__ __forceinline__ void atomicMaxAbs(unsigned long long* slot, double v) {
   atomicMax(slot, static_cast<unsigned long long>(__double_as_longlong(fabs(v))));

@@ -299,6 +299,10 @@ __global__ void kTileKeys(const double* x, const double* y, const double* z, int
   const int cy = clampCell((y[p] - G.o[1]) / G.h[1], G.ny);
   const int cz = clampCell((z[p] - G.o[2]) / G.h[2], G.nz);
   key[p] = (static_cast<unsigned>(cz / kTileZ) * nty + cy / kTileY) * ntx + cx / kTileX;
+  // live active entries, one atomic per wavefront
+  const unsigned long long act = __ballot(state[p] == kStateActive);
+  if (act && static_cast<int>(__lane_id()) == __ffsll(static_cast<long long>(act)) - 1)
+    atomicAdd(K.words + 4, static_cast<unsigned>(__popcll(act)));
 }
 
 struct PermuteArgs {
@@ -366,7 +370,7 @@ __global__ void kChunkList(const int* begin, const int* end, const unsigned long
     volatile unsigned* v = box;
     v[8] = static_cast<unsigned>(chunkField(total));
     v[9] = static_cast<unsigned>(activeField(total));
-    for (int w = 0; w < 4; ++w) v[10 + w] = words[w];
+    for (int w = 0; w < 5; ++w) v[10 + w] = words[w];
     __threadfence_system();
     v[0] = seq;
   }
@@ -1284,7 +1288,7 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
     launchElectricPreviewTiles(F, F.ptrs(), out, sums, F.window ? F.inletPlane[0] : nullptr,
                                F.window ? F.inletPlane[1] : nullptr, C.activeTiles.get(), C.tileScan.get() + ntiles,
                                TileGrid{ntx, nty, ntz, kTileX, kTileY, kTileZ});
-  unsigned nChunks = 0, nActive = 0, occupiedTiles = 0;
+  unsigned nChunks = 0, nActive = 0, occupiedTiles = 0, activeEntries = 0;
   if (n) {
     C.mailbox.wait(0, sortSeq, st);
     const unsigned* box = C.mailbox.host();
@@ -1293,6 +1297,7 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
     occupiedTiles = box[11];
     // finishMovingWindowStep's checks of the entries, applied to this origin
     raiseParticleErrors(box[10]);
+    activeEntries = box[14];
     const int behind = static_cast<int>(box[12]);
     if (box[13]) cropCounters(C, st, n - behind, n, P.state.get(), P.s[6].get(), P.s[8].get(), false);
     n -= behind;  // removed and cropped entries now trail the live ones
@@ -1380,9 +1385,10 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
     // (simulation_source_coupling.cpp:44)
     referenceCurrentTrial(C, F, C.esum[2].get(), C.estar[2].get(), true, nChunks > 0, ntx, nty, ntz);
     // With active particles the convergence test needs at least two trials
-    // (simulation_source_coupling.cpp:50), so trial 0 is never read back;
-    // its error word is checked, first, after trial 1.
-    const bool readback = !(n && trial == 0);
+    // (simulation_source_coupling.cpp:50), so trial 0 is not read back; its
+    // error word is checked, first, after trial 1. Held entries alone may
+    // converge in one trial (no coupling call unless one releases).
+    const bool readback = !(activeEntries && trial == 0);
     const unsigned trialSeq = readback ? C.mailbox.next(1) : 0u;
     {
       {

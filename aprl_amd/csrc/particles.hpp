@@ -63,8 +63,9 @@ constexpr int kCounterSlots = 20;
 
 // Error words of a coupling step: [0] sort-phase errors, [1] occupied
 // tiles, [2] entries keyed behind the live ones (removed or cropped),
-// [3] cropped carriers, [kTrialWord + t] the kernels of Picard trial t.
-constexpr int kTrialWord = 4;
+// [3] cropped carriers, [4] live active (not held) particles at the sort,
+// [kTrialWord + t] the kernels of Picard trial t.
+constexpr int kTrialWord = 5;
 constexpr int kErrorWords = kTrialWord + 64;
 
 struct MagnetDev {

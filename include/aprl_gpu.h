@@ -534,6 +534,82 @@ aprl_status aprl_fields_initialize_gauss(aprl_fields* fields, aprl_particles* pa
                                          size_t maximum_iterations, int include_transverse_cpml_particles,
                                          aprl_gauss_report* report);
 
+/* The laboratory-plane input route of a moving-window run: the startup
+ * sequence of Simulation for HDF5 version-4 plane-arrival records with
+ * frozen injection (simulation.cpp:1983-2245), on a device-resident bunch
+ * in laboratory coordinates (the reference's particles_ before the boost,
+ * RelativisticParticleSI: position[2] holds the arrival time offset of an
+ * event until the projection). One lane per macro-particle; the three sums
+ * the reference keeps in long double are double-double sums finished in
+ * long double on the host, extrema are exact. Errors carry the reference's
+ * types and messages. After aprl_bunch_freeze the bunch is the box-frame
+ * snapshot at box time zero with every entry held: download it, hand it to
+ * the pending queue (aprl_pending_extract) and upload the rest into the
+ * store with aprl_particles_set_held. */
+typedef struct aprl_bunch aprl_bunch;
+aprl_status aprl_bunch_create(size_t capacity, aprl_bunch** out);
+void aprl_bunch_destroy(aprl_bunch* bunch);
+size_t aprl_bunch_count(const aprl_bunch* bunch);
+/* weight may be NULL (unit weights); held speeds start at zero. */
+aprl_status aprl_bunch_upload(aprl_bunch* bunch, size_t n, const double* position3, const double* u3,
+                              const double* charge, const double* mass, const double* weight, const uint64_t* id);
+/* Any output pointer may be NULL; held_beta = frozenBetaZ after the freeze. */
+aprl_status aprl_bunch_download(aprl_bunch* bunch, double* position3, double* u3, double* charge, double* mass,
+                                double* weight, uint64_t* id, double* held_beta);
+/* Synthetic plane-arrival records (the fields of docs/PARTICLE_INPUT_HDF5.md
+ * version 4) of the generated-gaussian beam: the draws of
+ * aprl_particles_generate_gaussian with axis 2 read as the arrival time
+ * offset, so center[2] and sigma_position[2] are in seconds; proper velocity
+ * in the laboratory (boost_gamma must be 1); unit weight. Replaces the
+ * bunch's contents. */
+aprl_status aprl_bunch_generate_plane_events(aprl_bunch* bunch, const aprl_gaussian_beam* beam, size_t count);
+/* SILabPlaneProjectionReport (eb_bunch.h:36-47). */
+typedef struct aprl_lab_plane_projection {
+  double input_plane_lab, reference_position_lab, minimum_forward_distance, maximum_forward_distance,
+      recommended_minimum_reference_position, reference_time_offset_lab, mean_longitudinal_beta;
+  uint64_t particles;
+} aprl_lab_plane_projection;
+/* SIBunchPreprocessor::projectLabPlaneEventsToSnapshot (eb_bunch.h:76-88,
+ * eb_bunch.cpp:134-289): every event advanced along its own velocity to the
+ * common lab time that puts the weighted longitudinal centroid at
+ * reference_position_lab; position[2] becomes the lab z relative to it. */
+aprl_status aprl_bunch_project_lab_plane_events(aprl_bunch* bunch, double input_plane_lab,
+                                                double reference_position_lab, double longitudinal_offset_lab,
+                                                int hold_transverse, aprl_lab_plane_projection* report);
+/* SIBunchPlacementReport (eb_bunch.h:23-34). */
+typedef struct aprl_bunch_placement {
+  double relative_tail_lab, relative_head_lab, reference_position_lab, head_after_lab, actual_head_distance,
+      recommended_maximum_reference_position;
+  uint64_t particles;
+} aprl_bunch_placement;
+/* SIBunchPreprocessor::placeRelativeLabSnapshot (eb_bunch.h:90-97,
+ * eb_bunch.cpp:291-352) with SIBunchPlacement's fields as arguments. */
+aprl_status aprl_bunch_place_relative(aprl_bunch* bunch, int has_entrance, double entrance_lab,
+                                      double reference_position_lab, double margin_lab,
+                                      aprl_bunch_placement* report);
+/* The frame of frozen lab-plane injection (simulation.cpp:2126-2186): lab
+ * time origin two z-Courant steps before the earliest activation-plane
+ * crossing, lab z origin at the activation plane, box z origin 0. */
+aprl_status aprl_bunch_frozen_frame_origin(aprl_bunch* bunch, double gamma_boost, double input_plane_lab,
+                                           double activation_plane_lab, double dz, int hold_transverse,
+                                           aprl_boost* frame);
+/* SIBunchBoostReport (eb_bunch.h:49-60). */
+typedef struct aprl_bunch_boost_report {
+  double earliest_lab_event_time, latest_lab_event_time, maximum_absolute_drift_time, minimum_lab_gamma,
+      maximum_lab_gamma, maximum_relative_momentum_round_trip_error, maximum_relative_gamma_round_trip_error;
+} aprl_bunch_boost_report;
+/* SIBunchPreprocessor::boostLabSnapshotToBoxTimeZero (eb_bunch.h:99-108,
+ * eb_bunch.cpp:354-482). */
+aprl_status aprl_bunch_boost_to_box_time_zero(aprl_bunch* bunch, const aprl_boost* frame, double snapshot_time_lab,
+                                              int hold_transverse, double input_plane_lab,
+                                              aprl_bunch_boost_report* report);
+/* The frozen flags of frozen injection (simulation.cpp:2228-2245): the held
+ * speed frozenBetaZ = heldLongitudinalBetaZ(u) with the transverse hold (0
+ * without), and APRL_RUNTIME_ERROR when an entry is at or past the
+ * activation plane at box time zero. */
+aprl_status aprl_bunch_freeze(aprl_bunch* bunch, const aprl_boost* frame, double activation_plane_lab,
+                              int hold_transverse);
+
 /* Per-kernel timing with HIP events on the instance stream. enable=1 starts
  * recording every launch; query returns launches and total milliseconds for
  * a kernel name (0 launches if unknown). */

@@ -5,6 +5,7 @@
 #include <memory>
 #include <string>
 
+#include "orc_bunch.hpp"
 #include "orc_detectors.hpp"
 #include "orc_initial_field.hpp"
 #include "orc_fields.hpp"
@@ -36,6 +37,25 @@ int guarded(F&& f) {
   }
 }
 orc::Fields* F(void* h) { return static_cast<orc::Fields*>(h); }
+// n particles from interleaved position / proper-velocity arrays and weights
+std::vector<orc::Particle> particlesOf(std::size_t n, const double* x3, const double* u3, const double* w) {
+  std::vector<orc::Particle> P(n);
+  for (std::size_t p = 0; p < n; ++p) {
+    for (int a = 0; a < 3; ++a) {
+      P[p].position[a] = x3[3 * p + a];
+      P[p].u[a] = u3 ? u3[3 * p + a] : 0.0;
+    }
+    P[p].weight = w ? w[p] : 1.0;
+  }
+  return P;
+}
+void storeParticles(const std::vector<orc::Particle>& P, double* x3, double* u3) {
+  for (std::size_t p = 0; p < P.size(); ++p)
+    for (int a = 0; a < 3; ++a) {
+      x3[3 * p + a] = P[p].position[a];
+      if (u3) u3[3 * p + a] = P[p].u[a];
+    }
+}
 orc::CpmlParams params(const std::size_t cells[3], double order, double R, double kappa, double alpha) {
   orc::CpmlParams p;
   for (int a = 0; a < 3; ++a) p.cells[a] = cells[a];
@@ -60,6 +80,73 @@ int orc_boost_lab(double gamma, double labT0, double labZ0, double boxZ0, double
     out8[0] = f.boxZFromLabZAndBoxT(zLab, tBox);
     out8[1] = f.labTFromLabZBoxT(zLab, tBox);
     f.fieldsBoxToLab(eBox, bBox, out8 + 2, out8 + 5);
+  });
+}
+
+// The laboratory-plane input route (orc_bunch.hpp), in place on interleaved
+// arrays. report8 = input plane, reference position, minimum and maximum
+// forward distance, recommended minimum reference, reference time offset,
+// mean beta_z, particles.
+int orc_bunch_project(std::size_t n, double* x3, const double* u3, const double* w, double inputPlane,
+                      double reference, double offset, int hold, double* report8) {
+  return guarded([&] {
+    auto P = particlesOf(n, x3, u3, w);
+    const orc::bunch::Projection r = orc::bunch::projectLabPlaneEvents(P, inputPlane, reference, offset, hold != 0);
+    storeParticles(P, x3, nullptr);
+    const double v[8] = {r.inputPlaneLab, r.referencePositionLab, r.minimumForwardDistance, r.maximumForwardDistance,
+                         r.recommendedMinimumReferencePosition, r.referenceTimeOffsetLab, r.meanLongitudinalBeta,
+                         static_cast<double>(r.particles)};
+    for (int a = 0; a < 8; ++a) report8[a] = v[a];
+  });
+}
+// report7 = relative tail, relative head, reference, head after, actual head
+// distance, recommended maximum reference, particles.
+int orc_bunch_place(std::size_t n, double* x3, int hasEntrance, double entrance, double reference, double margin,
+                    double* report7) {
+  return guarded([&] {
+    auto P = particlesOf(n, x3, nullptr, nullptr);
+    const orc::bunch::Placement r = orc::bunch::placeRelative(P, hasEntrance != 0, entrance, reference, margin);
+    storeParticles(P, x3, nullptr);
+    const double v[7] = {r.relativeTailLab, r.relativeHeadLab, r.referencePositionLab, r.headAfterLab,
+                         r.actualHeadDistance, r.recommendedMaximumReferencePosition,
+                         static_cast<double>(r.particles)};
+    for (int a = 0; a < 7; ++a) report7[a] = v[a];
+  });
+}
+// out2 = lab time origin, lab z origin of the frozen-injection frame.
+int orc_bunch_frozen_origin(std::size_t n, const double* x3, const double* u3, double gammaBoost, double inputPlane,
+                            double activationPlane, double dz, int hold, double* out2) {
+  return guarded([&] {
+    const auto P = particlesOf(n, x3, u3, nullptr);
+    orc::bunch::frozenFrameOrigin(P, gammaBoost, inputPlane, activationPlane, dz, hold != 0, out2);
+  });
+}
+// frame4 = gamma, lab time origin, lab z origin, box z origin; report7 =
+// earliest and latest lab event time, maximum |drift time|, minimum and
+// maximum lab gamma, maximum relative momentum and gamma round-trip errors.
+int orc_bunch_boost(std::size_t n, double* x3, double* u3, const double* frame4, double snapshotTime, int hold,
+                    double inputPlane, double* report7) {
+  return guarded([&] {
+    auto P = particlesOf(n, x3, u3, nullptr);
+    orc::bunch::Frame f;
+    f.set(frame4[0], frame4[1], frame4[2], frame4[3]);
+    const orc::bunch::BoostReport r = orc::bunch::boostToBoxTimeZero(P, f, snapshotTime, hold != 0, inputPlane);
+    storeParticles(P, x3, u3);
+    const double v[7] = {r.earliestLabEventTime, r.latestLabEventTime, r.maximumAbsoluteDriftTime,
+                         r.minimumLabGamma, r.maximumLabGamma, r.maximumRelativeMomentumRoundTripError,
+                         r.maximumRelativeGammaRoundTripError};
+    for (int a = 0; a < 7; ++a) report7[a] = v[a];
+  });
+}
+// beta: the held speed per particle (0 without the transverse hold).
+int orc_bunch_freeze(std::size_t n, const double* x3, const double* u3, const double* frame4, double activationPlane,
+                     int hold, double* beta) {
+  return guarded([&] {
+    auto P = particlesOf(n, x3, u3, nullptr);
+    orc::bunch::Frame f;
+    f.set(frame4[0], frame4[1], frame4[2], frame4[3]);
+    orc::bunch::freeze(P, f, activationPlane, hold != 0);
+    for (std::size_t p = 0; p < n; ++p) beta[p] = P[p].betaHeld;
   });
 }
 

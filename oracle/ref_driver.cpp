@@ -262,6 +262,7 @@ double ref_fields_poynting(void* h, std::size_t i, std::size_t j, std::size_t k,
 #include <unistd.h>
 
 #include "boostframe.h"
+#include "eb_bunch.h"
 #include "eb_initial_field.h"
 #include "compute_threads.h"
 #include "eb_deposition.h"
@@ -312,6 +313,131 @@ FieldVector<Double> fv(const double* p) {
 }  // namespace
 
 extern "C" {
+
+// The laboratory-plane input route through the reference's own
+// SIBunchPreprocessor (eb_bunch.cpp) on MPI_COMM_SELF; the frozen-injection
+// frame origin and the frozen flags (simulation.cpp:2126-2186, 2228-2245,
+// not buildable here) are transcribed around the reference's own
+// BoostFrameTransform. Arguments and reports as the orc_bunch_* entry points.
+std::vector<RelativisticParticleSI> refParticles(std::size_t n, const double* x3, const double* u3, const double* w) {
+  std::vector<RelativisticParticleSI> P(n);
+  for (std::size_t p = 0; p < n; ++p) {
+    for (int a = 0; a < 3; ++a) {
+      P[p].position[a] = x3[3 * p + a];
+      P[p].properVelocity[a] = u3 ? u3[3 * p + a] : 0.0;
+    }
+    P[p].weight = w ? w[p] : 1.0;
+  }
+  return P;
+}
+void refStore(const std::vector<RelativisticParticleSI>& P, double* x3, double* u3) {
+  for (std::size_t p = 0; p < P.size(); ++p)
+    for (int a = 0; a < 3; ++a) {
+      x3[3 * p + a] = P[p].position[a];
+      if (u3) u3[3 * p + a] = P[p].properVelocity[a];
+    }
+}
+int ref_bunch_project(std::size_t n, double* x3, const double* u3, const double* w, double inputPlane,
+                      double reference, double offset, int hold, double* report8) {
+  return guarded([&] {
+    ensureMpi();
+    auto P = refParticles(n, x3, u3, w);
+    const SILabPlaneProjectionReport r = SIBunchPreprocessor::projectLabPlaneEventsToSnapshot(
+        P, inputPlane, reference, offset, MPI_COMM_SELF, hold != 0);
+    refStore(P, x3, nullptr);
+    const double v[8] = {r.inputPlaneLab, r.referencePositionLab, r.minimumForwardDistance, r.maximumForwardDistance,
+                         r.recommendedMinimumReferencePosition, r.referenceTimeOffsetLab, r.meanLongitudinalBeta,
+                         static_cast<double>(r.particles)};
+    for (int a = 0; a < 8; ++a) report8[a] = v[a];
+  });
+}
+int ref_bunch_place(std::size_t n, double* x3, int hasEntrance, double entrance, double reference, double margin,
+                    double* report7) {
+  return guarded([&] {
+    ensureMpi();
+    auto P = refParticles(n, x3, nullptr, nullptr);
+    SIBunchPlacement placement;
+    placement.hasFirstInteractionEntrance = hasEntrance != 0;
+    placement.firstInteractionEntranceLab = entrance;
+    placement.referencePositionLab = reference;
+    placement.recommendationMarginLab = margin;
+    const SIBunchPlacementReport r = SIBunchPreprocessor::placeRelativeLabSnapshot(P, placement, MPI_COMM_SELF);
+    refStore(P, x3, nullptr);
+    const double v[7] = {r.relativeTailLab, r.relativeHeadLab, r.referencePositionLab, r.headAfterLab,
+                         r.actualHeadDistance, r.recommendedMaximumReferencePosition,
+                         static_cast<double>(r.particles)};
+    for (int a = 0; a < 7; ++a) report7[a] = v[a];
+  });
+}
+int ref_bunch_frozen_origin(std::size_t n, const double* x3, const double* u3, double gammaBoost, double inputPlane,
+                            double activationPlane, double dz, int hold, double* out2) {
+  return guarded([&] {
+    const auto P = refParticles(n, x3, u3, nullptr);
+    Double earliest = std::numeric_limits<Double>::infinity();
+    int invalidHeld = 0;
+    BoostFrameTransform probe;
+    if (hold) probe.setOriginsFromGamma(gammaBoost, SI::c, 0.0, activationPlane, 0.0);
+    for (const RelativisticParticleSI& particle : P) {
+      const Double gamma = BoostFrameTransform::gammaFromProperVelocity(particle.properVelocity);
+      const Double betaZ = particle.properVelocity[2] / gamma;
+      const Double snapshotZ = particle.position[2];
+      if (hold) {
+        FieldVector<Double> uBox(0.0);
+        probe.properVelocityLabToBox(particle.properVelocity, uBox);
+        const Double betaHeld = probe.heldLongitudinalBetaZ(uBox);
+        const Double labAdvance = probe.beta() + betaHeld;
+        if (!(labAdvance > 0.0) || !std::isfinite(labAdvance)) {
+          invalidHeld = 1;
+          continue;
+        }
+        const Double inputTime = (inputPlane - snapshotZ) / (betaZ * SI::c);
+        const Double inputToActivation = inputPlane - activationPlane;
+        const Double crossing = inputTime - inputToActivation * (1.0 + probe.beta() * betaHeld) / (SI::c * labAdvance);
+        const Double margin = 2.0 * dz / (probe.gamma() * SI::c * labAdvance);
+        earliest = std::min(earliest, crossing - margin);
+      } else {
+        earliest = std::min(earliest, (activationPlane - snapshotZ) / (betaZ * SI::c));
+      }
+    }
+    if (invalidHeld)
+      throw std::invalid_argument("Held-transverse pseudo-trajectory must advance in laboratory z for every particle");
+    out2[0] = hold ? earliest : earliest - 2.0 * dz / (gammaBoost * SI::c);
+    out2[1] = activationPlane;
+  });
+}
+int ref_bunch_boost(std::size_t n, double* x3, double* u3, const double* frame4, double snapshotTime, int hold,
+                    double inputPlane, double* report7) {
+  return guarded([&] {
+    auto P = refParticles(n, x3, u3, nullptr);
+    BoostFrameTransform frame;
+    frame.setOriginsFromGamma(frame4[0], SI::c, frame4[1], frame4[2], frame4[3]);
+    const SIBunchBoostReport r =
+        SIBunchPreprocessor::boostLabSnapshotToBoxTimeZero(P, frame, snapshotTime, hold != 0, inputPlane);
+    refStore(P, x3, u3);
+    const double v[7] = {r.earliestLabEventTime, r.latestLabEventTime, r.maximumAbsoluteDriftTime,
+                         r.minimumLabGamma, r.maximumLabGamma, r.maximumRelativeMomentumRoundTripError,
+                         r.maximumRelativeGammaRoundTripError};
+    for (int a = 0; a < 7; ++a) report7[a] = v[a];
+  });
+}
+int ref_bunch_freeze(std::size_t n, const double* x3, const double* u3, const double* frame4, double activationPlane,
+                     int hold, double* beta) {
+  return guarded([&] {
+    auto P = refParticles(n, x3, u3, nullptr);
+    BoostFrameTransform frame;
+    frame.setOriginsFromGamma(frame4[0], SI::c, frame4[1], frame4[2], frame4[3]);
+    int alreadyPast = 0;
+    for (RelativisticParticleSI& particle : P) {
+      particle.frozen = true;
+      if (hold) particle.frozenBetaZ = frame.heldLongitudinalBetaZ(particle.properVelocity);
+      if (frame.labZFromBoxZT(particle.position[2], 0.0) >= activationPlane) alreadyPast = 1;
+    }
+    if (alreadyPast)
+      throw std::runtime_error(
+          "Frozen lab-plane injection requires every particle upstream of the input plane at box time zero");
+    for (std::size_t p = 0; p < n; ++p) beta[p] = hold ? P[p].frozenBetaZ : 0.0;
+  });
+}
 
 // The reference's BoostFrameTransform for the detector: out = {box z of the
 // lab plane, its lab time, E_lab[3], B_lab[3]}.
