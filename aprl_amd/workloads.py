@@ -400,6 +400,138 @@ class Stepper:
         self.step_index += 1
 
 
+@dataclass
+class InjectionSetup:
+    """Planes and window placement of a frozen lab-plane injection run: the
+    input (= activation) plane, the initial bunch centre and the first
+    magnetic interaction entrance in the lab (m); the box z (cells) of the
+    window's lower face at step 0, and of the lab exit plane and the lab
+    field plane at box time zero."""
+    input_plane_z: float
+    initial_center_z: float
+    entrance: float | None
+    window_bottom_cells: float
+    exit_box_cells: float
+    field_plane_box_cells: float
+
+
+# The bending-radiation card as a production run: the bunch's plane-arrival
+# events 1 mm upstream of the dipole entrance (z_lab = -0.1 m), projected to a
+# snapshot centred 0.1 mm further on, frozen and injected through the
+# pending queue; released particles cross the window (about two cells per
+# step relative to it) and leave through the lab exit plane, which the
+# window passes at 0.02 cells per step (it stays 32 cells below the upper
+# face over 2000 steps); the lab field plane sits inside the window for the
+# whole run.
+BENDING_INJECTION = InjectionSetup(input_plane_z=-0.101, initial_center_z=-0.1009, entrance=-0.1,
+                                   window_bottom_cells=-16.0, exit_box_cells=440.0, field_plane_box_cells=200.0)
+
+
+class ProductionRun:
+    """Simulation's moving-window run for workload w from plane-arrival
+    events (simulation.cpp:1983-2245, then solve's per-step order,
+    simulation.cpp:518-597): the events of the seeded Gaussian are drawn on
+    the device, pass the input route (aprl_amd.injection), the snapshot's
+    upstream particles wait in the library's pending queue and the rest
+    enter the store held; every step admits the due particles (8-step
+    look-ahead), runs the field half step, the coupled source step and the
+    energy-ledger flux, shifts the window and samples the lab field plane
+    at every step's lab time."""
+
+    def __init__(self, w: Workload, setup: InjectionSetup, shape: str = "linear-binomial3",
+                 precision: str = "fp64", max_steps: int = 2000, plane_capacity: int = 64, keep_patch=None):
+        import aprl_amd as A
+        from . import pending
+        from .injection import InjectionPlanes, LabPlaneBunch, frozenLabPlaneInjection
+        self.w, self.setup = w, setup
+        nx, ny, nz = w.dims
+        dx, dy, dz = w.spacing
+        g = A.EBGridGeometry(*w.dims, *w.spacing, w.dt)
+        self.fields = A.EBFieldGrid(g, precision=precision)
+        self.window = A.MovingWindowFields(self.fields, A.EBCPMLParameters((w.cpml, w.cpml, 0)))
+        self.coupling = A.ConservativeSourceCoupling(self.fields, shape)
+        if w.dipole_tesla:
+            self.coupling.addPrescribedLabMagnet("uniform-dipole", [0.0, 0.0, -0.5 * w.dipole_length],
+                                                 w.dipole_length, peak_field=w.dipole_tesla,
+                                                 polarization_angle=0.5 * np.pi)
+        # the plane-arrival events and the input route
+        self.bunch = LabPlaneBunch()
+        u_lab = np.sqrt(w.gamma_lab ** 2 - 1.0)
+        beta_lab = u_lab / w.gamma_lab
+        sig_u = (w.emittance / w.sigma[0], w.emittance / w.sigma[1], w.sigma_gamma * w.gamma_lab)
+        self.bunch.generatePlaneEvents(w.particles, (0.0, 0.0), w.sigma[:2], w.sigma[2] / (beta_lab * C_SI),
+                                       (0.0, 0.0, u_lab), sig_u, w.charge_electrons, w.seed)
+        planes = InjectionPlanes(setup.input_plane_z, setup.input_plane_z, setup.initial_center_z, setup.entrance)
+        self.frame, self.reports = frozenLabPlaneInjection(self.bunch, planes, w.gamma_boost, dz)
+        x, u, q, m, wt, ids, beta = self.bunch.download()
+        self.coupling.setFrame(*self.frame)
+        self.exit_lab = self.frame[2] + w.gamma_boost * setup.exit_box_cells * dz
+        self.coupling.setParticleBoundary((w.cpml, w.cpml, 0), self.exit_lab)
+        self.coupling.setActivationPlane(setup.input_plane_z)
+        self.coupling.setHoldTransverse(True)
+        # the window: lower face at window_bottom_cells at step 0
+        self.offset = 10_000
+        ref = setup.window_bottom_cells * dz - self.offset * dz
+        self.queue = pending.PendingParticleQueue(
+            pending.PendingWindowGeometry(ref, self.offset, dz, C_SI * w.dt, max_steps + 64))
+        xs, us, qs, ms, ws, idss, fs, bs = self.queue.extract(x, u, q, m, wt, ids, np.ones(len(q), np.int32), beta)
+        self.store = A.ParticleStore(self.fields, len(q))
+        self.store.upload(xs, us, qs, ms, ws, idss)
+        self.store.setHeld(fs, bs)
+        self.queued0 = self.queue.size()
+        self.origin = np.array([-0.5 * nx * dx, -0.5 * ny * dy, self.queue.origin(0)])
+        self.window.enableEnergyLedger(True)
+        # the lab field plane, sampled once per step (the lab time of a fixed
+        # lab z advances dt / gamma_b per step)
+        z_plane = self.frame[2] + w.gamma_boost * setup.field_plane_box_cells * dz
+        self.plane = A.LabFieldPlane(self.coupling, z_plane, 0.999 * w.dt / w.gamma_boost, plane_capacity)
+        self.plane_samples = 0
+        # (i0, i1, j0, j1): the part of every fetched plane record kept on the host
+        self.keep_patch = keep_patch
+        self.kept_times, self.kept = [], []
+        self.step_index = 0
+        self.time = 0.0
+        self.trials = []
+        self.particle_steps = 0
+
+    def step(self):
+        n = self.step_index
+        self.queue.admit(n, self.store, 8)
+        self.particle_steps += len(self.store)
+        f, win = self.fields, self.window
+        f.clearCurrent()
+        win.beginStep(n, self.offset)
+        f.advanceMagnetic()
+        win.setInlet()
+        its, _, _ = self.coupling.step(self.store, self.origin, self.time, self.w.substeps)
+        self.trials.append(its)
+        win.finishEnergyLedgerStep()
+        self.offset -= 1
+        win.shift(n + 1, self.offset)
+        self.step_index += 1
+        self.time += self.w.dt
+        self.origin[2] = self.queue.origin(self.step_index)
+        if self.plane.sample(self.time, self.origin):
+            self.plane_samples += 1
+            if self.plane_samples % self.plane.capacity == 0:
+                self._fetch()
+
+    def _fetch(self):
+        times, records = self.plane.fetch()
+        if self.keep_patch is not None and len(times):
+            i0, i1, j0, j1 = self.keep_patch
+            self.kept_times.append(times)
+            self.kept.append(records[:, j0:j1, i0:i1, :].copy())
+
+    def finish(self):
+        """Fetches the plane samples still in the device ring; returns the
+        kept (lab times, records) of the whole run."""
+        self._fetch()
+        if not self.kept:
+            return np.zeros(0), np.zeros((0,))
+        return np.concatenate(self.kept_times), np.concatenate(self.kept)
+
+
 class BatchStepper:
     """Steps several independent instances (each a workload with its own
     grid, window, coupling and store) with aprl_window_step_batch: the

@@ -357,6 +357,70 @@ def run_large(w, steps=3, warmup=1, data=None):
     return out
 
 
+def run_production(steps=2000, precision="fp64", shape="linear-binomial3", keep_patch=(16, 48, 16, 48)):
+    """The bending-radiation card run as specified: 2000 steps from
+    plane-arrival events through the input route and the pending queue,
+    with the dipole, the lab exit plane, the energy ledger and a lab field
+    plane sampled every step (workloads.ProductionRun). Timed over the whole
+    run (queue admissions and plane samples included); particle-steps count
+    every store entry (held ones too) at each step."""
+    import numpy as np
+    from aprl_amd import workloads
+    w = workloads.BENDING
+    run = workloads.ProductionRun(w, workloads.BENDING_INJECTION, shape, precision, max_steps=steps,
+                                  keep_patch=keep_patch)
+    run.fields.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run.step()
+    run.fields.synchronize()
+    elapsed = time.perf_counter() - t0
+    counts, _ = run.coupling.boundaryCounters()
+    ledger = run.window.energyLedger()
+    times, records = run.finish()
+    out = {"workload": w.name + ", frozen lab-plane injection", "form": "plain FP64" if precision == "fp64"
+           else "FP32-split", "shape": shape, "steps": steps, "seconds": round(elapsed, 3),
+           "ms_per_step": 1e3 * elapsed / steps, "particle_steps": int(run.particle_steps),
+           "value": run.particle_steps / elapsed, "unit": "particle-steps/s",
+           "cell_updates_per_s": w.cells * steps / elapsed, "queued_initially": int(run.queued0),
+           "admitted": int(run.queue.admittedCount()), "window_exits": int(counts[18]),
+           "cropped_carriers": int(counts[19]), "picard_trials_mean": sum(run.trials) / len(run.trials),
+           "lab_plane_samples": int(run.plane_samples),
+           "ledger_outward_energy_J": float(np.sum(ledger["outwardEnergyFace"])),
+           "ledger_cropped_energy_J": float(ledger["croppedEnergyJ"])}
+    print(f"production run ({out['form']}): {out['ms_per_step']:.3f} ms per step, {out['window_exits']} exits",
+          file=sys.stderr, flush=True)
+    del run
+    return out, (times, records)
+
+
+def plane_error(test, ref):
+    """FP32-split against FP64 on the lab field plane of the production run
+    (the central 32 x 32 nodes, every step): relative L2 of the E_lab and
+    B_lab records as vector fields over the whole run, and of the power
+    spectrum over lab time of E_x and B_y summed over the patch."""
+    import numpy as np
+    (tt, rt), (tr, rr) = test, ref
+    n = min(len(tt), len(tr))
+    rt, rr = rt[:n], rr[:n]
+    out = {"samples": int(n)}
+    for name, comps in (("e", (0, 1, 2)), ("b", (3, 4, 5))):
+        out[name] = float(np.linalg.norm(rt[..., comps] - rr[..., comps]) / np.linalg.norm(rr[..., comps]))
+    for name, c in (("ex_spectrum", 0), ("by_spectrum", 4)):
+        st = np.abs(np.fft.rfft(rt[..., c], axis=0)) ** 2
+        sr = np.abs(np.fft.rfft(rr[..., c], axis=0)) ** 2
+        out[name] = float(np.linalg.norm(st.sum(axis=(1, 2)) - sr.sum(axis=(1, 2))) /
+                          np.linalg.norm(sr.sum(axis=(1, 2))))
+    return out
+
+
+def production_legs():
+    p64, plane64 = run_production()
+    p32, plane32 = run_production(precision="fp32-split")
+    p32["plane_error_vs_fp64"] = plane_error(plane32, plane64)
+    return {"fp64": p64, "fp32_split": p32}
+
+
 def step_roofline(w, ms, word=WORD):
     """Whole-step bytes against HBM peak: 21 words per cell-update (B: read
     3 E, read and write 3 B; E: read 3 B, 3 J, read and write 3 E) plus 2
@@ -382,8 +446,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-large", action="store_true", help="skip the FEL segment and capacity legs")
-    ap.add_argument("--leg", choices=("all", "fp64", "fp32-split"), default="all",
-                    help="time only this form's binomial3-x leg (for profiling runs)")
+    ap.add_argument("--leg", choices=("all", "fp64", "fp32-split", "production"), default="all",
+                    help="time only this form's binomial3-x leg (for profiling runs), or only the 2000-step "
+                         "production runs")
     ap.add_argument("--error-study", action="store_true",
                     help="print the FP32-split vs FP64 errors at 100, 1000 and 10000 steps and exit")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -407,6 +472,9 @@ def main():
     if args.error_study:
         print(json.dumps({"error_study": error_study()}))
         return
+    if args.leg == "production":
+        print(json.dumps({"production": production_legs()}))
+        return
     if args.leg != "all":
         w, elapsed, trials, _, (tiles, chunks, active) = run_gpu(args, profile=False, precision=args.leg)
         print(json.dumps({"leg": args.leg, "value": w.particles * args.steps / elapsed, "unit": "particle-steps/s",
@@ -421,6 +489,7 @@ def main():
     _, sq_elapsed, _, _, _ = run_gpu(args, "quintic-compensated", profile=False, precision="fp32-split")
     s_error = split_error()
     batch = run_batch()
+    production = production_legs()
     large = None
     if not args.no_large:
         from aprl_amd import workloads
@@ -469,6 +538,7 @@ def main():
                        "step_roofline": step_roofline(w, 1e3 * s_elapsed / args.steps, word=4),
                        "kernels_us": {k: round(v["avg_us"], 2) for k, v in s_kernels.items()}},
         "batch": batch,
+        "production": production,
         "large": large,
         "roofline": roofline,
         "cpu_baseline": cpu,

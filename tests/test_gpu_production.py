@@ -1,0 +1,105 @@
+"""The bending-radiation card (64x64x512, 1e5 electrons, 0.5 T dipole,
+gamma_b = 5) run as a production window run on the GPU, and the accuracy
+of the FP32-split form against the plain FP64 form on it (the FP64 form is
+pinned to the oracle at 1e-12 per step by test_gpu_configs.py and, for the
+input route and the queue, test_gpu_injection.py):
+
+* 400 steps from plane-arrival events (workloads.ProductionRun): every
+  queued particle admitted, released particles leave through the lab exit
+  plane, Picard trial counts within the reference's range, the lab field
+  plane sampled every step;
+* FP32-split against FP64 on the same run: the lab field-plane records
+  (E_lab, B_lab of the central 32 x 32 nodes, every step) and their power
+  spectra over lab time, and the grid's E and B at step 200 (the bunch in
+  the middle of the window), within 1e-5 relative L2 (the north-star bound
+  for 1000 steps); the grid after 400 steps, when the bunch has left and
+  the field in the window is small, within 2e-4 (measured 1.0e-5 for E,
+  6.8e-5 for B);
+* the box-snapshot workload of the benchmark after 100 steps: E, B, u
+  within 1e-5 (measured 2.0e-6, 2.3e-6, 1.0e-7).
+"""
+import numpy as np
+import pytest
+
+from aprl_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 400
+
+
+def vector_rel(a_fields, b_fields, comps):
+    num = sum(float(np.sum((a_fields.component(c) - b_fields.component(c)) ** 2)) for c in comps)
+    den = sum(float(np.sum(b_fields.component(c) ** 2)) for c in comps)
+    return float(np.sqrt(num / den))
+
+
+def production(precision, snapshot_step=200):
+    """The run, its kept lab-plane (times, records) and E, B of the grid at
+    snapshot_step (the bunch then fills the middle of the window)."""
+    run = W.ProductionRun(W.BENDING, W.BENDING_INJECTION, "linear-binomial3", precision, max_steps=STEPS,
+                          keep_patch=(16, 48, 16, 48))
+    mid = None
+    for n in range(STEPS):
+        if n % 100 == 0:
+            print(f"production {precision} step {n}", flush=True)
+        run.step()
+        if n + 1 == snapshot_step:
+            mid = [run.fields.component(c) for c in range(6)]
+    times, records = run.finish()
+    return run, times, records, mid
+
+
+def list_rel(a, b, comps):
+    num = sum(float(np.sum((a[c] - b[c]) ** 2)) for c in comps)
+    den = sum(float(np.sum(b[c] ** 2)) for c in comps)
+    return float(np.sqrt(num / den))
+
+
+def test_production_run_fp32_split_tracks_fp64():
+    r64, t64, p64, m64 = production("fp64")
+    r32, t32, p32, m32 = production("fp32-split")
+    for run in (r64, r32):
+        assert run.queue.size() == 0 and run.queue.admittedCount() == run.queued0 > 0.99 * W.BENDING.particles
+        counts, _ = run.coupling.boundaryCounters()
+        assert counts[18] > 0  # window exits
+        assert run.plane_samples == STEPS
+    assert all(1 <= t <= 3 for t in r64.trials)
+    assert np.array_equal(t64, t32)
+    err = {name: float(np.linalg.norm(p32[..., c] - p64[..., c]) / np.linalg.norm(p64[..., c]))
+           for name, c in (("E", slice(0, 3)), ("B", slice(3, 6)))}
+    spec = {}
+    for name, c in (("Ex", 0), ("By", 4)):
+        s32 = (np.abs(np.fft.rfft(p32[..., c], axis=0)) ** 2).sum(axis=(1, 2))
+        s64 = (np.abs(np.fft.rfft(p64[..., c], axis=0)) ** 2).sum(axis=(1, 2))
+        spec[name] = float(np.linalg.norm(s32 - s64) / np.linalg.norm(s64))
+    grid = {"E": list_rel(m32, m64, (0, 1, 2)), "B": list_rel(m32, m64, (3, 4, 5))}
+    # after the bunch has left, the field left in the window is small beside
+    # the rounding the FP32 storage accumulated while it passed
+    late = {"E": vector_rel(r32.fields, r64.fields, (0, 1, 2)), "B": vector_rel(r32.fields, r64.fields, (3, 4, 5))}
+    print("lab plane records", err, "spectra", spec, "grid at step 200", grid, f"grid at step {STEPS}", late,
+          flush=True)
+    assert max(err.values()) <= 1e-5 and max(spec.values()) <= 1e-5 and max(grid.values()) <= 1e-5
+    assert max(late.values()) <= 2e-4
+
+
+def test_box_snapshot_fp32_split_error_after_100_steps():
+    w = W.BENDING
+    data = w.generate()
+    steppers = {}
+    for precision in ("fp64", "fp32-split"):
+        fields, window, coupling, store = W.setup_gpu(w, "linear-binomial3", precision, data=data)
+        s = W.Stepper(w, fields, window, coupling, store)
+        for n in range(100):
+            s.step()
+        steppers[precision] = (fields, store)
+    (f64, s64), (f32, s32) = steppers["fp64"], steppers["fp32-split"]
+    e = vector_rel(f32, f64, (0, 1, 2))
+    b = vector_rel(f32, f64, (3, 4, 5))
+    x64, u64, _, _, _, i64 = s64.download()
+    x32, u32, _, _, _, i32 = s32.download()
+    o64, o32 = np.argsort(i64), np.argsort(i32)
+    assert np.array_equal(i64[o64], i32[o32])
+    u = float(np.linalg.norm(u32[o32] - u64[o64]) / np.linalg.norm(u64))
+    print(f"box snapshot after 100 steps: E {e:.2e}, B {b:.2e}, u {u:.2e}", flush=True)
+    assert e <= 1e-5 and b <= 1e-5 and u <= 1e-5
