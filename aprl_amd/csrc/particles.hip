@@ -319,14 +319,24 @@ struct PermuteArgs {
 // State scatter into sorted order, and the tile ranges of the sorted keys.
 __global__ void kPermute(PermuteArgs A, const unsigned* perm, int n, TileBoundsArgs T) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const unsigned q = perm[p];
+  unsigned tile = ~0u;
+  double bound = 0.0;
+  if (p < n) {
+    const unsigned from = perm[p];
 #pragma unroll
-  for (int a = 0; a < 9; ++a) A.dst[a][p] = A.src[a][q];
-  A.idDst[p] = A.idSrc[q];
-  A.stateDst[p] = A.stateSrc[q];
-  A.betaDst[p] = A.betaSrc[q];
-  tileBoundsAt(T, p);
+    for (int a = 0; a < 9; ++a) A.dst[a][p] = A.src[a][from];
+    A.idDst[p] = A.idSrc[from];
+    const int state = A.stateSrc[from];
+    A.stateDst[p] = state;
+    A.betaDst[p] = A.betaSrc[from];
+    tileBoundsAt(T, p);
+    const unsigned k = T.key[p];
+    if (k < static_cast<unsigned>(T.ntiles)) {
+      tile = k;
+      bound = entryCurrentBound(A.src[6][from], state, T.perCharge);
+    }
+  }
+  raiseTileBound(T.bound, tile, bound);
 }
 
 // Per tile, packed (packTileCounts): chunk count, whether it is occupied,
@@ -530,45 +540,33 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks(PArgs P, GArgs G, St
 // Push of source_coupling.cpp:133-147 for one trial with the gathered
 // fields (the substeps with prescribed magnets), or the straight move of a
 // CPML carrier (position += dt u c / gamma, simulation.cpp:4407-4413), then
-// the segment's boundary events.
-// It also raises the entry's tile current bound: the largest |scale_a| of
-// its chord (scale_a = q (x1_a - x0_a) / (dx dy dz dt), eb_deposition.cpp:
-// 41-51), plus, for a held entry whose release kink splits the path, the
-// held leg's largest scale (both legs stay below the chord plus the leg).
+// the segment's boundary events. (The tile current bounds the deposit's
+// fixed-point exponents come from are set once per step by the sort.)
 __global__ void __launch_bounds__(128) kPush(PArgs P, GArgs G, StepArgs S, unsigned* errOut) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned tile = ~0u;
-  double bound = 0.0;
-  if (p < P.n) {
-    unsigned err = 0;
-    const double x0[3] = {P.x[0][p], P.x[1][p], P.x[2][p]};
-    const double u0[3] = {P.u[0][p], P.u[1][p], P.u[2][p]};
-    double x[3], u[3], absGrid, t1, t2, wDeposit, wEnd;
-    const double usedE[3] = {P.fin[0][p], P.fin[1][p], P.fin[2][p]};
-    const double usedB[3] = {P.fin[3][p], P.fin[4][p], P.fin[5][p]};
-    const int state = P.state[p];
-    const double q = P.q[p], beta = P.beta[p];
-    const int code = entryStep(S, G.o, state, x0, u0, q, P.m[p], P.w[p], beta, usedE, usedB, x, u, absGrid, t1, t2,
-                               wDeposit, wEnd, err);
-    for (int a = 0; a < 3; ++a) {
-      P.x1[a][p] = x[a];
-      P.u1[a][p] = u[a];
-      bound = fmax(bound, fabs(q * (x[a] - x0[a]) / P.scaleDen));
-    }
-    if (state == kStateHeld) bound += fabs(q) * (kC * S.dt * (S.holdTransverse ? fabs(beta) : 1.0)) / P.scaleDen;
-    tile = P.tileOf[p];
-    P.gridAbs[p] = absGrid;
-    P.evt[p] = code;
-    if (code) {
-      P.cut[0][p] = t1;
-      P.cut[1][p] = t2;
-      P.cut[2][p] = wDeposit;
-      P.wend[p] = wEnd;
-      err |= kEvtAny;
-    }
-    if (err) atomicOr(errOut, err);
+  if (p >= P.n) return;
+  unsigned err = 0;
+  const double x0[3] = {P.x[0][p], P.x[1][p], P.x[2][p]};
+  const double u0[3] = {P.u[0][p], P.u[1][p], P.u[2][p]};
+  double x[3], u[3], absGrid, t1, t2, wDeposit, wEnd;
+  const double usedE[3] = {P.fin[0][p], P.fin[1][p], P.fin[2][p]};
+  const double usedB[3] = {P.fin[3][p], P.fin[4][p], P.fin[5][p]};
+  const int code = entryStep(S, G.o, P.state[p], x0, u0, P.q[p], P.m[p], P.w[p], P.beta[p], usedE, usedB, x, u,
+                             absGrid, t1, t2, wDeposit, wEnd, err);
+  for (int a = 0; a < 3; ++a) {
+    P.x1[a][p] = x[a];
+    P.u1[a][p] = u[a];
   }
-  raiseTileBound(P.tileBound, tile, bound);
+  P.gridAbs[p] = absGrid;
+  P.evt[p] = code;
+  if (code) {
+    P.cut[0][p] = t1;
+    P.cut[1][p] = t2;
+    P.cut[2][p] = wDeposit;
+    P.wend[p] = wEnd;
+    err |= kEvtAny;
+  }
+  if (err) atomicOr(errOut, err);
 }
 
 // Deterministic tile deposition of Current::deposit (longitudinal_shape.h:
@@ -708,14 +706,11 @@ __global__ void __launch_bounds__(kChunk * depositLanes<SHAPE>()) kDeposit(PArgs
 template <int SHAPE>
 __global__ void __launch_bounds__(256) kReduceCurrent(double* jx, double* jy, double* jz, GArgs G, const int* active,
                                                       const unsigned long long* tileScan, const int* tileShift,
-                                                      long long* partial, unsigned long long* tileBound, int ntx,
-                                                      int nty, int ntz) {
+                                                      long long* partial, int ntx, int nty, int ntz) {
   constexpr int RZ = Box<SHAPE>::RZ, NB = 9 * (2 * RZ + 1);
   const int tile = active[xcdContiguous(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x))];
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
   __shared__ int nbFirst[NB], nbShift[NB];
-  // the next trial's push raises the bound from zero
-  if (threadIdx.x == 0) tileBound[tile] = 0ull;
   if (threadIdx.x < NB) {
     const int nb = static_cast<int>(threadIdx.x);
     const int x = tx + nb % 3 - 1, y = ty + (nb / 3) % 3 - 1, z = tz + nb / 9 - RZ;
@@ -1034,7 +1029,10 @@ static void sortAndChunk(CouplingState& C, ParticleStore& P, const GArgs& G, con
   A.stateDst = P.stateT.get();
   A.betaSrc = P.beta.get();
   A.betaDst = P.betaT.get();
-  const TileBoundsArgs T{P.keySorted.get(), n, ntiles, C.tileBegin.get(), C.tileEnd.get(), occupied};
+  // |scale| = |q| |dx_a| / (dx dy dz dt) < |q| c dt / (dx dy dz dt), with a margin for rounding
+  const double perCharge = kC * F.g.dt / (F.g.dx * F.g.dy * F.g.dz * F.g.dt) * (1.0 + 1e-9);
+  const TileBoundsArgs T{P.keySorted.get(), n,       ntiles, C.tileBegin.get(), C.tileEnd.get(), occupied,
+                         C.tileBound.get(),  perCharge};
   kPermute<<<blocks, 256, 0, st>>>(A, P.permSorted.get(), n, T);
   launchCheck("permute");
   for (int a = 0; a < 9; ++a) P.s[a].swap(P.t[a]);
@@ -1363,11 +1361,11 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
         if (quintic)
           kReduceCurrent<kShapeQuintic><<<nActive, 256, 0, st>>>(
               F.base(6), F.base(7), F.base(8), G, C.activeTiles.get(), C.tileScan.get(), C.tileShift.get(),
-              C.partial.get(), C.tileBound.get(), ntx, nty, ntz);
+              C.partial.get(), ntx, nty, ntz);
         else
           kReduceCurrent<kShapeBinomial3X><<<nActive, 256, 0, st>>>(
               F.base(6), F.base(7), F.base(8), G, C.activeTiles.get(), C.tileScan.get(), C.tileShift.get(),
-              C.partial.get(), C.tileBound.get(), ntx, nty, ntz);
+              C.partial.get(), ntx, nty, ntz);
         launchCheck("reduce current");
       }
       F.currentZero = false;

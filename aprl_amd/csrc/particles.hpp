@@ -266,7 +266,18 @@ struct TileBoundsArgs {
   int n, ntiles;
   int *begin, *end;
   unsigned* occupied;
+  unsigned long long* bound;  // per tile: largest entry current bound (cleared by the key kernels)
+  double perCharge;           // current bound per unit |charge| of a step's chord
 };
+
+// An entry's current bound for the step, known at the sort: every substep
+// moves a particle or a carrier less than c dt along any axis (|v| < c), so
+// a chord's current scale stays below |q| c dt / (dx dy dz dt) (perCharge,
+// with a rounding margin); a held entry deposits at most its held leg and
+// the released remainder, twice that.
+__device__ __forceinline__ double entryCurrentBound(double q, int state, double perCharge) {
+  return fabs(q) * perCharge * (state == kStateHeld ? 2.0 : 1.0);
+}
 
 __device__ __forceinline__ void tileBoundsAt(const TileBoundsArgs& T, int p) {
   const unsigned k = T.key[p];

@@ -320,16 +320,26 @@ struct PermuteArgs32 {
 // State scatter into sorted order, and the tile ranges of the sorted keys.
 __global__ void kPermute32(PermuteArgs32 A, const unsigned* perm, int n, TileBoundsArgs T) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const unsigned q = perm[p];
+  unsigned tile = ~0u;
+  double bound = 0.0;
+  if (p < n) {
+    const unsigned from = perm[p];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) A.cellDst[a][p] = A.cellSrc[a][q];
+    for (int a = 0; a < 3; ++a) A.cellDst[a][p] = A.cellSrc[a][from];
 #pragma unroll
-  for (int a = 0; a < 9; ++a) A.dst[a][p] = A.src[a][q];
-  A.idDst[p] = A.idSrc[q];
-  A.stateDst[p] = A.stateSrc[q];
-  A.betaDst[p] = A.betaSrc[q];
-  tileBoundsAt(T, p);
+    for (int a = 0; a < 9; ++a) A.dst[a][p] = A.src[a][from];
+    A.idDst[p] = A.idSrc[from];
+    const int state = A.stateSrc[from];
+    A.stateDst[p] = state;
+    A.betaDst[p] = A.betaSrc[from];
+    tileBoundsAt(T, p);
+    const unsigned k = T.key[p];
+    if (k < static_cast<unsigned>(T.ntiles)) {
+      tile = k;
+      bound = entryCurrentBound(static_cast<double>(A.src[6][from]), state, T.perCharge);
+    }
+  }
+  raiseTileBound(T.bound, tile, bound);
 }
 
 // ---------------------------------------------------------------------------
@@ -482,8 +492,8 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks32(PArgs32 P, GArgs32
 // |scale_a| of its FP32 chord as the deposit forms it (q kq_a v_a), plus,
 // for a held entry whose release kink splits the path, the held leg's
 // largest scale.
-__device__ __forceinline__ double pushSplitEntry(const PArgs32& P, const StepArgs& S, double o0, double o1, double o2,
-                                                 int p, unsigned* errOut) {
+__device__ __forceinline__ void pushSplitEntry(const PArgs32& P, const StepArgs& S, double o0, double o1, double o2,
+                                               int p, unsigned* errOut) {
   unsigned err = 0;
   const double o[3] = {o0, o1, o2};
   int c0[3];
@@ -527,19 +537,6 @@ __device__ __forceinline__ double pushSplitEntry(const PArgs32& P, const StepArg
     err |= kEvtAny;
   }
   if (err) atomicOr(errOut, err);
-  const float q = P.q[p];
-  double bound = 0.0;
-  for (int a = 0; a < 3; ++a) {
-    const float v = static_cast<float>((x[a] - x0[a]) / P.h[a]);
-    bound = fmax(bound, fabs(static_cast<double>(q * P.kq[a] * v)));
-  }
-  if (P.state[p] == kStateHeld) {
-    double leg = static_cast<double>(P.kq[2]) * (kC * S.dt * fabs(static_cast<double>(P.beta[p])) / P.h[2]);
-    if (!S.holdTransverse)  // ballistic: at most c dt along any axis
-      for (int a = 0; a < 3; ++a) leg = fmax(leg, static_cast<double>(P.kq[a]) * (kC * S.dt / P.h[a]));
-    bound += fabs(static_cast<double>(q)) * leg;
-  }
-  return bound;
 }
 
 // One trial's push: FP32 state and fields widened to FP64, the FP64 orbit
@@ -550,13 +547,7 @@ __device__ __forceinline__ double pushSplitEntry(const PArgs32& P, const StepArg
 __global__ void __launch_bounds__(128) kPushSplit(PArgs32 P, StepArgs S, double o0, double o1, double o2,
                                                   unsigned* errOut) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned tile = ~0u;
-  double bound = 0.0;
-  if (p < P.n) {
-    tile = P.tileOf[p];
-    bound = pushSplitEntry(P, S, o0, o1, o2, p, errOut);
-  }
-  raiseTileBound(P.tileBound, tile, bound);
+  if (p < P.n) pushSplitEntry(P, S, o0, o1, o2, p, errOut);
 }
 
 // Deterministic tile deposition (Current::deposit, longitudinal_shape.h:
@@ -693,14 +684,12 @@ __global__ void __launch_bounds__(kChunk * depositLanes<SHAPE>()) kDepositSplit(
 template <int SHAPE, bool WRITE_ESUM>
 __global__ void __launch_bounds__(256) kReduceCurrentSplit(float* jx, float* jy, float* jz, GArgs32 G,
                                                            const int* active, const unsigned long long* tileScan,
-                                                           const int* tileShift, long long* partial,
-                                                           unsigned long long* tileBound, int ntx, int nty, int ntz) {
+                                                           const int* tileShift, long long* partial, int ntx,
+                                                           int nty, int ntz) {
   constexpr int RZ = Box<SHAPE>::RZ, NB = 9 * (2 * RZ + 1);
   const int tile = active[xcdContiguous(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x))];
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
   __shared__ int nbFirst[NB], nbShift[NB];
-  // the next trial's push raises the bound from zero
-  if (threadIdx.x == 0) tileBound[tile] = 0ull;
   if (threadIdx.x < NB) {
     const int nb = static_cast<int>(threadIdx.x);
     const int x = tx + nb % 3 - 1, y = ty + (nb / 3) % 3 - 1, z = tz + nb / 9 - RZ;
@@ -1012,7 +1001,15 @@ void sortSplit(CouplingState& C, SplitStore& P, const GArgs32& G, const SortArgs
   A.stateDst = P.stateT.get();
   A.betaSrc = P.beta.get();
   A.betaDst = P.betaT.get();
-  const TileBoundsArgs T{P.keySorted.get(), n, ntiles, C.tileBegin.get(), C.tileEnd.get(), occupied};
+  // |q kq_a v_a| with |v_a| < c dt / h_a cells, with a margin for the FP32 rounding
+  const double h[3] = {F.g.dx, F.g.dy, F.g.dz};
+  double perCharge = 0.0;
+  for (int a = 0; a < 3; ++a)
+    perCharge = std::max(perCharge, static_cast<double>(static_cast<float>(h[a] / (F.g.dx * F.g.dy * F.g.dz * F.g.dt))) *
+                                        (kC * F.g.dt / h[a]));
+  perCharge *= 1.0 + 1e-6;
+  const TileBoundsArgs T{P.keySorted.get(), n,       ntiles, C.tileBegin.get(), C.tileEnd.get(), occupied,
+                         C.tileBound.get(),  perCharge};
   kPermute32<<<blocks, 256, 0, st>>>(A, P.permSorted.get(), n, T);
   launchCheck("permute");
   for (int a = 0; a < 3; ++a) P.cell[a].swap(P.cellT[a]);
@@ -1248,8 +1245,7 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
       float* j[3] = {F.base(6), F.base(7), F.base(8)};
 #define APRL_REDUCE32(SH, W)                                                                                      \
   kReduceCurrentSplit<SH, W><<<nActive, 256, 0, st>>>(j[0], j[1], j[2], G, C.activeTiles.get(), C.tileScan.get(), \
-                                                       C.tileShift.get(), C.partial.get(), C.tileBound.get(), ntx,  \
-                                                       nty, ntz)
+                                                       C.tileShift.get(), C.partial.get(), ntx, nty, ntz)
       if (quintic && esum) APRL_REDUCE32(kShapeQuintic, true);
       else if (quintic) APRL_REDUCE32(kShapeQuintic, false);
       else if (esum) APRL_REDUCE32(kShapeBinomial3X, true);
