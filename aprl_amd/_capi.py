@@ -178,6 +178,7 @@ def lib() -> C.CDLL:
     L.aprl_particles_count.restype = _sz
     L.aprl_particles_download_states.argtypes = [vp, C.POINTER(C.c_int)]
     L.aprl_particles_set_held.argtypes = [vp, C.POINTER(C.c_int), _dp]
+    L.aprl_particles_download_held.argtypes = [vp, _dp]
     L.aprl_coupling_set_activation.argtypes = [vp, C.c_double]
     L.aprl_coupling_set_hold_transverse.argtypes = [vp, C.c_int]
     L.aprl_repeat_held_addition.argtypes = [C.c_double, C.c_double, C.c_uint64, _dp]
@@ -279,7 +280,7 @@ EXPORTED = [
     "aprl_particles_generate_gaussian", "aprl_coupling_particle_energy", "aprl_window_enable_energy_ledger",
     "aprl_window_finish_energy_step", "aprl_window_energy_ledger",
     "aprl_particles_count", "aprl_particles_download_states", "aprl_coupling_boundary_counters",
-    "aprl_particles_set_held", "aprl_coupling_set_activation", "aprl_coupling_set_hold_transverse",
+    "aprl_particles_set_held", "aprl_particles_download_held", "aprl_coupling_set_activation", "aprl_coupling_set_hold_transverse",
     "aprl_repeat_held_addition", "aprl_pending_create", "aprl_pending_destroy", "aprl_pending_extract",
     "aprl_pending_admit", "aprl_pending_view", "aprl_pending_state", "aprl_pending_origin",
     "aprl_coupling_create", "aprl_coupling_destroy", "aprl_coupling_set_frame",

@@ -249,6 +249,26 @@ aprl_status aprl_particles_download_states(aprl_particles* particles, int* state
   });
 }
 
+aprl_status aprl_particles_download_held(aprl_particles* particles, double* beta) {
+  return guarded([&] {
+    if (!particles || !beta) throw std::invalid_argument("Null particle store or held-speed array");
+    hipCheck(hipStreamSynchronize(particles->stream()), "hipStreamSynchronize");
+    const size_t n = particles->fields->fp32 ? particles->sp.count : particles->p.count;
+    const std::vector<int> st = storeStates(particles, n);
+    std::vector<double> b(n);
+    if (particles->fields->fp32) {
+      std::vector<float> f(n);
+      if (n) hipCheck(hipMemcpy(f.data(), particles->sp.beta.get(), n * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+      for (size_t i = 0; i < n; ++i) b[i] = static_cast<double>(f[i]);
+    } else if (n) {
+      hipCheck(hipMemcpy(b.data(), particles->p.beta.get(), n * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+    }
+    size_t k = 0;
+    for (size_t i = 0; i < n; ++i)
+      if (st[i] >= 0) beta[k++] = st[i] == kStateHeld ? b[i] : 0.0;
+  });
+}
+
 aprl_status aprl_coupling_set_reference_current(aprl_coupling* coupling, const double* table, size_t rows,
                                                size_t count, const size_t* guards, double translation_beta) {
   return guarded([&] {

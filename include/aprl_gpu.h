@@ -171,6 +171,10 @@ aprl_status aprl_particles_download_states(aprl_particles* particles, int* state
  * hold-transverse branch, simulation.cpp:4951-5000); upstream of a moving
  * window it deposits only its shaped support until it crosses the inlet. */
 aprl_status aprl_particles_set_held(aprl_particles* particles, const int* frozen, const double* beta);
+/* RelativisticParticleSI::frozenBetaZ per live entry in download order (0
+ * for entries that are not held), the cached held speed of
+ * eb_particles.h:22. */
+aprl_status aprl_particles_download_held(aprl_particles* particles, double* beta);
 
 /* The generated-gaussian beam of ParticleInitializer (generatedGaussian,
  * particle_initializer.cpp:60-100), drawn on the device straight into the
