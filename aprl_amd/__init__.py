@@ -6,5 +6,6 @@ from .fields import (EBCPMLParameters, EBFieldGrid, EBGridGeometry, MovingWindow
 from .coupling import (ConservativeSourceCoupling, EBGaussFieldInitializer, LabFieldPlane,  # noqa: F401
                        ParticleStore)
 from .injection import InjectionPlanes, LabPlaneBunch, frozenLabPlaneInjection  # noqa: F401
+from .reference import BlockedCowanReference  # noqa: F401
 
 __all__ = ["EBFieldGrid", "EBGridGeometry", "EBCPMLParameters", "MovingWindowFields", "AprlError", "lib"]

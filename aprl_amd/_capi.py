@@ -81,6 +81,12 @@ class BunchBoostReport(C.Structure):
                 ("maximum_relative_gamma_round_trip_error", C.c_double)]
 
 
+class InletReferenceReport(C.Structure):
+    """aprl_inlet_reference_report (BlockedCowanReferenceReport, blocked_cowan_reference.h:16-56)."""
+    _fields_ = [("minimum_mode_eigenvalue", C.c_double), ("minimum_pivot", C.c_double),
+                ("source_charge", C.c_double), ("deposited_charge", C.c_double), ("interior_nodes", C.c_uint64 * 3)]
+
+
 class InstanceStep(C.Structure):
     """aprl_instance_step: one instance of a batched window step."""
     _fields_ = [("fields", C.c_void_p), ("coupling", C.c_void_p), ("particles", C.c_void_p), ("step", C.c_size_t),
@@ -213,6 +219,14 @@ def lib() -> C.CDLL:
     L.aprl_lab_plane_sample.argtypes = [vp, C.c_double, _dp, C.POINTER(C.c_int)]
     L.aprl_lab_plane_fetch.argtypes = [vp, _sz, _dp, _dp, C.POINTER(_sz)]
     L.aprl_window_step_batch.argtypes = [C.POINTER(InstanceStep), _sz]
+    L.aprl_inlet_reference_create.argtypes = [C.POINTER(GridGeometry), _dp, _sz, _dp, _dp, C.c_double,
+                                              C.POINTER(_sz), C.c_int, C.c_int, C.POINTER(vp),
+                                              C.POINTER(InletReferenceReport)]
+    L.aprl_inlet_reference_destroy.argtypes = [vp]
+    L.aprl_inlet_reference_destroy.restype = None
+    L.aprl_inlet_reference_download.argtypes = [vp, C.c_int, _dp, _sz]
+    L.aprl_inlet_reference_plane.argtypes = [vp, C.c_int, C.c_ssize_t, _sz, C.c_double, _dp, _sz]
+    L.aprl_window_set_reference.argtypes = [vp, vp, C.c_double]
     L.aprl_bunch_create.argtypes = [_sz, C.POINTER(vp)]
     L.aprl_bunch_destroy.argtypes = [vp]
     L.aprl_bunch_destroy.restype = None
@@ -288,6 +302,8 @@ EXPORTED = [
     "aprl_coupling_set_split_trials", "aprl_coupling_prepare_step", "aprl_window_step_batch",
     "aprl_lab_plane_create", "aprl_lab_plane_destroy", "aprl_lab_plane_sample", "aprl_lab_plane_fetch",
     "aprl_fields_initialize_gauss",
+    "aprl_inlet_reference_create", "aprl_inlet_reference_destroy", "aprl_inlet_reference_download",
+    "aprl_inlet_reference_plane", "aprl_window_set_reference",
     "aprl_bunch_create", "aprl_bunch_destroy", "aprl_bunch_count", "aprl_bunch_upload", "aprl_bunch_download",
     "aprl_bunch_generate_plane_events", "aprl_bunch_project_lab_plane_events", "aprl_bunch_place_relative",
     "aprl_bunch_frozen_frame_origin", "aprl_bunch_boost_to_box_time_zero", "aprl_bunch_freeze",

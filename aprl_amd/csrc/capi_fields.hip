@@ -3,6 +3,7 @@
 // FP64 form or the FP32 form; every entry point dispatches on it. Host
 // arrays are always double: the FP32 form rounds them once on upload and
 // widens them on download.
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <type_traits>
@@ -10,6 +11,7 @@
 #include "diagnostics.hpp"
 #include "handles.hpp"
 #include "inlet.hpp"
+#include "inletref.hpp"
 
 namespace aprl_gpu {
 namespace {
@@ -58,6 +60,23 @@ void uploadPlanes(FieldStateT<R>& s, DeviceArray<R>* dst, const double* const* h
     } else {
       out[a] = c < 3 ? s.zeroPlane[c].get() : nullptr;
     }
+  }
+}
+
+// The window's planes of components first.. at fixed-world plane k and
+// step: the caller's where given, else the device reference's
+// (movingWindowFieldValue), else the zero reference.
+template <class R>
+void windowPlanes(FieldStateT<R>& s, DeviceArray<R>* dst, const double* const* host, int first, int count,
+                  const R** out, std::ptrdiff_t k, std::size_t step) {
+  uploadPlanes(s, dst, host, first, count, out);
+  if (!s.reference) return;
+  for (int a = 0; a < count; ++a) {
+    if (host && host[a]) continue;
+    const int c = first + a;
+    if (dst[a].size() != s.plane(c)) dst[a].resize(s.plane(c));
+    referencePlane<R>(*s.reference, static_cast<unsigned>(c), k, step, s.referenceBeta, dst[a].get(), s.stream);
+    out[a] = dst[a].get();
   }
 }
 
@@ -316,7 +335,7 @@ aprl_status aprl_window_begin_step(aprl_fields* fields, size_t step, ptrdiff_t o
       if (s.chargeMatch) throw std::logic_error("A charge-matched inlet begins its steps with the centered reference");
       if (s.phase != 0 || (s.established && (step != s.step || offset != s.offset)))
         throw std::logic_error("Moving-window beginStep phase/origin mismatch");
-      uploadPlanes(s, s.haloLo, lower, 0, 3, s.loPlane);
+      windowPlanes(s, s.haloLo, lower, 0, 3, s.loPlane, offset - 1, step);
       beginLedger(s);
       s.step = step;
       s.offset = offset;
@@ -349,11 +368,16 @@ aprl_status aprl_window_begin_step_centered(aprl_fields* fields, size_t step, pt
       for (int a = 0; a < 3; ++a) {
         if (s.haloLo[a].size() != s.plane(a)) s.haloLo[a].resize(s.plane(a));
         if (lower && lower[a]) toDevice(s.stream, s.haloLo[a].get(), lower[a], s.plane(a));
+        else if (s.reference)
+          referencePlane<R>(*s.reference, static_cast<unsigned>(a), offset - 1, step, s.referenceBeta,
+                            s.haloLo[a].get(), s.stream);
         else hipCheck(hipMemsetAsync(s.haloLo[a].get(), 0, s.plane(a) * sizeof(R), s.stream), "hipMemsetAsync");
         s.loPlane[a] = s.haloLo[a].get();
       }
       if (s.refEz.size() != s.plane(2)) s.refEz.resize(s.plane(2));
       if (ez_at_offset) toDevice(s.stream, s.refEz.get(), ez_at_offset, s.plane(2));
+      else if (s.reference)
+        referencePlane<R>(*s.reference, 2u, offset, step, s.referenceBeta, s.refEz.get(), s.stream);
       else hipCheck(hipMemsetAsync(s.refEz.get(), 0, s.plane(2) * sizeof(R), s.stream), "hipMemsetAsync");
       centerInletHalo(s, s.haloLo[2].get(), s.refEz.get());
       beginLedger(s);
@@ -371,8 +395,25 @@ aprl_status aprl_window_set_inlet(aprl_fields* fields, const double* ex, const d
     withState(fields, [&](auto& s) {
       if (!s.window) throw std::logic_error("No moving window is installed");
       const double* const host[2] = {ex, ey};
-      uploadPlanes(s, s.inlet, host, 0, 2, s.inletPlane);
+      windowPlanes(s, s.inlet, host, 0, 2, s.inletPlane, s.offset, s.step + 1);
       s.inletSet = true;
+    });
+  });
+}
+
+aprl_status aprl_window_set_reference(aprl_fields* fields, aprl_inlet_reference* reference, double translation_beta) {
+  return guarded([&] {
+    withState(fields, [&](auto& s) {
+      if (!s.window) throw std::logic_error("No moving window is installed");
+      if (reference) {
+        const aprl_grid_geometry& r = reference->g;
+        if (r.nx != s.g.nx || r.ny != s.g.ny || r.dx != s.g.dx || r.dy != s.g.dy || r.dz != s.g.dz ||
+            r.dt != s.g.dt || r.solver != s.g.solver)
+          throw std::invalid_argument("Blocked reference fill geometry/offset mismatch");
+        if (!std::isfinite(translation_beta)) throw std::invalid_argument("Reference translation beta must be finite");
+      }
+      s.reference = reference;
+      s.referenceBeta = translation_beta;
     });
   });
 }
@@ -385,7 +426,7 @@ aprl_status aprl_window_shift(aprl_fields* fields, size_t newStep, ptrdiff_t new
       if (s.phase != 3 || newStep != s.step + 1 || newOffset != s.offset - 1)
         throw std::logic_error("Moving-window requires one left shift after each complete step");
       const R* dev[6];
-      uploadPlanes(s, s.shiftPlanes, planes, 0, 6, dev);
+      windowPlanes(s, s.shiftPlanes, planes, 0, 6, dev, newOffset, newStep);
       ledgerBeforeShift(s);
       windowShift(s, dev);
       // the charge-matched inlet completes the new plane first

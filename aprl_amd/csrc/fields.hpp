@@ -10,6 +10,8 @@
 
 #include "common.hpp"
 
+struct aprl_inlet_reference;
+
 namespace aprl_gpu {
 
 struct CowanCoef {
@@ -130,6 +132,10 @@ struct FieldStateT {
   DeviceArray<R> refEz;                      // reference Ez at the step's offset
   DeviceArray<unsigned long long> inletAcc;  // fixed-point inlet charge, (nx+1)(ny+1)
   DeviceArray<unsigned long long> inletWords;
+  // the inlet reference on the device (aprl_window_set_reference): planes
+  // not given by the caller come from it at the window's offset and step
+  const aprl_inlet_reference* reference = nullptr;
+  double referenceBeta = 0.0;
 
   // True while J is known to hold zeros on the device (set by clears and
   // shifts, dropped by any write of J), so repeated clears cost nothing.

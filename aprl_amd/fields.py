@@ -220,6 +220,15 @@ class MovingWindowFields:
         o = np.ascontiguousarray(origin, dtype=np.float64)
         check(lib().aprl_window_complete_inlet(self.fields.handle, coupling.handle, particles.handle, dptr(o)))
 
+    def setReference(self, reference, translationBeta: float) -> None:
+        """The reference callback on the device (aprl_window_set_reference):
+        planes not passed to beginStep / setInlet / shift come from the
+        BlockedCowanReference at the window's offset and step
+        (movingWindowFieldValue); None restores the zero reference."""
+        self._reference = reference  # kept alive while the window uses it
+        check(lib().aprl_window_set_reference(self.fields.handle, None if reference is None else reference.handle,
+                                              float(translationBeta)))
+
     def setInlet(self, ex=None, ey=None) -> None:
         ex = None if ex is None else np.ascontiguousarray(ex, dtype=np.float64)
         ey = None if ey is None else np.ascontiguousarray(ey, dtype=np.float64)

@@ -614,6 +614,48 @@ aprl_status aprl_bunch_boost_to_box_time_zero(aprl_bunch* bunch, const aprl_boos
 aprl_status aprl_bunch_freeze(aprl_bunch* bunch, const aprl_boost* frame, double activation_plane_lab,
                               int hold_transverse);
 
+/* The moving window's inlet reference (BlockedCowanReference,
+ * blocked_cowan_reference.h:68-129) prepared on the device: the source
+ * particles' charge on the interior vertices of the global reference grid
+ * (CIC, source_shape = -1, or the production longitudinal basis with
+ * linear or x-binomial3 transverse weights: APRL_SHAPE_QUINTIC_COMPENSATED,
+ * APRL_SHAPE_LINEAR_BINOMIAL3), the 27-point Cowan operator with the
+ * translation term at the mean held beta solved exactly by sine modes in
+ * x, y and a Thomas sweep per mode along z, and the native E(0), B(-dt/2)
+ * of the whole grid kept in device memory. The charge is summed in 64-bit
+ * fixed point (order-free), the transforms are direct sums: fields agree
+ * with the reference's FFTW-based construction to round-off, not bitwise.
+ * The open-wall transverse reference and the block files are not built.
+ * Errors carry the reference's types and messages. */
+typedef struct aprl_inlet_reference aprl_inlet_reference;
+typedef struct aprl_inlet_reference_report {
+  double minimum_mode_eigenvalue, minimum_pivot, source_charge, deposited_charge;
+  uint64_t interior_nodes[3];
+} aprl_inlet_reference_report;
+aprl_status aprl_inlet_reference_create(const aprl_grid_geometry* global, const double* global_origin, size_t n,
+                                        const double* position3, const double* charge, double beta,
+                                        const size_t* guards, int source_shape, int transverse_x_binomial3,
+                                        aprl_inlet_reference** out, aprl_inlet_reference_report* report);
+void aprl_inlet_reference_destroy(aprl_inlet_reference* reference);
+/* BlockedCowanReference::field for a whole native component (0..5 = Ex Ey Ez
+ * Bx By Bz, x fastest). */
+aprl_status aprl_inlet_reference_download(aprl_inlet_reference* reference, int component, double* host,
+                                          size_t count);
+/* Simulation::movingWindowFieldValue (simulation_moving_window.cpp:280-294)
+ * over component `component`'s x-y nodes at fixed-world plane k and step:
+ * the reference at z = k - translation_beta step, linear between planes;
+ * APRL_OUT_OF_RANGE outside the prepared planes. */
+aprl_status aprl_inlet_reference_plane(aprl_inlet_reference* reference, int component, ptrdiff_t k, size_t step,
+                                       double translation_beta, double* host, size_t count);
+/* The window's reference callback on the device: with a reference set, the
+ * plane arguments of aprl_window_begin_step, aprl_window_begin_step_centered,
+ * aprl_window_set_inlet and aprl_window_shift that are NULL are filled from
+ * it (movingWindowFieldValue at the window's offset and step, at
+ * translation_beta) by a kernel instead of taking the zero reference. The
+ * reference's x-y grid and spacings must be the window's. NULL restores the
+ * zero reference. The reference must outlive its use by the window. */
+aprl_status aprl_window_set_reference(aprl_fields* fields, aprl_inlet_reference* reference, double translation_beta);
+
 /* Per-kernel timing with HIP events on the instance stream. enable=1 starts
  * recording every launch; query returns launches and total milliseconds for
  * a kernel name (0 launches if unknown). */

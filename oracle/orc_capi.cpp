@@ -8,6 +8,7 @@
 #include "orc_bunch.hpp"
 #include "orc_detectors.hpp"
 #include "orc_initial_field.hpp"
+#include "orc_inlet_reference.hpp"
 #include "orc_fields.hpp"
 #include "orc_particles.hpp"
 #include "orc_pending.hpp"
@@ -147,6 +148,61 @@ int orc_bunch_freeze(std::size_t n, const double* x3, const double* u3, const do
     f.set(frame4[0], frame4[1], frame4[2], frame4[3]);
     orc::bunch::freeze(P, f, activationPlane, hold != 0);
     for (std::size_t p = 0; p < n; ++p) beta[p] = P[p].betaHeld;
+  });
+}
+
+// The moving window's inlet reference (orc_inlet_reference.hpp).
+// spacing4 = dx, dy, dz, dt; shape = -1 CIC, 0 quintic, 1 notch, 2
+// linear-binomial3; report4 = minimum mode eigenvalue, minimum pivot,
+// source charge, deposited charge.
+int orc_inlet_reference_new(const std::size_t* cells3, const double* spacing4, const double* origin3, std::size_t n,
+                            const double* x3, const double* q, double beta, const std::size_t* guards3,
+                            std::size_t blockLayers, int shape, int xBinomial, void** out, double* report4) {
+  return guarded([&] {
+    orc::inletref::Config c;
+    c.nx = cells3[0];
+    c.ny = cells3[1];
+    c.nz = cells3[2];
+    c.dx = spacing4[0];
+    c.dy = spacing4[1];
+    c.dz = spacing4[2];
+    c.dt = spacing4[3];
+    for (int a = 0; a < 3; ++a) {
+      c.origin[a] = origin3[a];
+      c.guards[a] = guards3[a];
+    }
+    c.beta = beta;
+    c.blockLayers = blockLayers;
+    c.shape = shape;
+    c.xBinomial = xBinomial != 0;
+    auto* r = new orc::inletref::Reference(orc::inletref::prepare(c, n, x3, q));
+    report4[0] = r->minimumModeEigenvalue;
+    report4[1] = r->minimumPivot;
+    report4[2] = r->sourceCharge;
+    report4[3] = r->depositedCharge;
+    *out = r;
+  });
+}
+void orc_inlet_reference_free(void* h) { delete static_cast<orc::inletref::Reference*>(h); }
+// which: 0..5 a native field component, 6 the potential (vertex lattice),
+// 7 the interior charge density; count must match.
+int orc_inlet_reference_array(void* h, int which, double* out, std::size_t count) {
+  return guarded([&] {
+    const auto* r = static_cast<const orc::inletref::Reference*>(h);
+    const std::vector<double>& v = which < 6 ? r->field[which] : (which == 6 ? r->phi : r->rho);
+    if (count != v.size()) throw std::invalid_argument("Inlet reference array size mismatch");
+    std::copy(v.begin(), v.end(), out);
+  });
+}
+// movingWindowFieldValue over component a's x-y nodes at fixed-world k.
+int orc_inlet_reference_moving(void* h, unsigned a, std::ptrdiff_t k, std::size_t step, double translationBeta,
+                               double* plane, std::size_t count) {
+  return guarded([&] {
+    const auto* r = static_cast<const orc::inletref::Reference*>(h);
+    if (a >= 6 || count != r->nx[a] * r->ny[a]) throw std::invalid_argument("Inlet reference plane size mismatch");
+    for (std::size_t j = 0; j < r->ny[a]; ++j)
+      for (std::size_t i = 0; i < r->nx[a]; ++i)
+        plane[j * r->nx[a] + i] = orc::inletref::movingValue(*r, a, i, j, k, step, translationBeta);
   });
 }
 
