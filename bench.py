@@ -32,14 +32,14 @@ WORD = 8  # FP64
 # profiles/bending_64x64x512_fp64_binomial3x_kernels_traffic.md. Traffic
 # above the algorithmic bytes is re-reading (staged halos, per-chunk boxes).
 MEASURED_TRAFFIC_FP64 = {
-    "magnetic": (2 * 89556 + 69040 + 2 * 10115 + 1792) * 1024,
-    "electric": (2 * 119401 + 70101) * 1024,
-    "electric_preview": (2 * 56568 + 27084) * 1024,
-    "gather": (2 * 13734 + 4690) * 1024,
-    "residual": (2 * 17274 + 4729) * 1024,
-    "deposit": (2 * 19942 + 26176) * 1024,
-    "push": (2 * 5841 + 5859) * 1024,
-    "reduce_current": (2 * 24287 + 46902) * 1024,
+    "magnetic": (2 * 100846 + 69040 + 2 * 10119 + 1792) * 1024,
+    "electric": (2 * 119785 + 70120) * 1024,
+    "electric_preview": (2 * 56596 + 27084) * 1024,
+    "gather": (2 * 13736 + 4690) * 1024,
+    "residual": (2 * 17270 + 4730) * 1024,
+    "deposit": (2 * 4224 + 26158) * 1024,
+    "push": (2 * 5840 + 5859) * 1024,
+    "reduce_current": (2 * 24387 + 47325) * 1024,
 }
 
 

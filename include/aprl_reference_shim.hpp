@@ -15,6 +15,8 @@
 //                                                 -> Coupling::step (one call per time step)
 //   SIBunchPreprocessor (eb_bunch.h:62-109)       -> projectLabPlaneEventsToSnapshot,
 //                                                    placeRelativeLabSnapshot, boostLabSnapshotToBoxTimeZero
+//   verification::BlockedCowanReference           -> InletReference (prepared and kept on the device; a
+//                                                    MovingWindow built on it fills its planes by a kernel)
 #ifndef APRL_REFERENCE_SHIM_HPP
 #define APRL_REFERENCE_SHIM_HPP
 
@@ -25,6 +27,7 @@
 #include <vector>
 
 #include "aprl_gpu.h"
+#include "blocked_cowan_reference.h"
 #include "boostframe.h"
 #include "eb_bunch.h"
 #include "eb_cpml.h"
@@ -134,6 +137,95 @@ class FieldGrid {
   aprl_fields* f_ = nullptr;
 };
 
+// verification::BlockedCowanReference (blocked_cowan_reference.h:68-129)
+// for one rank, prepared on the device at construction and kept in device
+// memory: no work directory, no block files, no communicator. field() reads
+// a host copy of the component made on first use; fill() writes E/B of a
+// grid of the same x-y geometry from it; a MovingWindow built on it takes
+// movingWindowFieldValue's planes from it without a host callback. The
+// open-wall transverse reference is not built (openTransverse rejected).
+class InletReference {
+ public:
+  InletReference(const EBGridGeometry& globalGeometry, const FieldVector<Double>& globalOrigin,
+                 const std::vector<RelativisticParticleSI>& particles, Double beta, const std::size_t guards[3],
+                 bool longitudinalQuintic = false, bool linearNotch = false, bool linearBinomial3 = false,
+                 bool transverseXBinomial3 = false, bool openTransverse = false)
+      : g_(globalGeometry) {
+    if (static_cast<unsigned>(longitudinalQuintic) + static_cast<unsigned>(linearNotch) +
+            static_cast<unsigned>(linearBinomial3) > 1)
+      throw std::invalid_argument("Ambiguous reference source shape");
+    if (linearNotch) throw std::invalid_argument("The retired linear-notch reference source is not built");
+    if (openTransverse) throw std::invalid_argument("The open-wall transverse reference is not built");
+    const std::size_t n = particles.size();
+    std::vector<double> x(3 * n), q(n);
+    for (std::size_t p = 0; p < n; ++p) {
+      for (int a = 0; a < 3; ++a) x[3 * p + a] = particles[p].position[a];
+      q[p] = particles[p].charge;
+    }
+    const aprl_grid_geometry c = geometryOf(globalGeometry);
+    const double o[3] = {globalOrigin[0], globalOrigin[1], globalOrigin[2]};
+    const int shape = longitudinalQuintic ? APRL_SHAPE_QUINTIC_COMPENSATED
+                                          : (linearBinomial3 ? APRL_SHAPE_LINEAR_BINOMIAL3 : -1);
+    aprl_inlet_reference_report r;
+    check(aprl_inlet_reference_create(&c, o, n, x.data(), q.data(), beta, guards, shape,
+                                      transverseXBinomial3 ? 1 : 0, &r_, &r));
+    report_.complete = true;
+    report_.particles = n;
+    for (int a = 0; a < 3; ++a) report_.interiorNodes[a] = r.interior_nodes[a];
+    report_.beta = beta;
+    report_.sourceCharge = r.source_charge;
+    report_.depositedCharge = r.deposited_charge;
+    report_.minimumPivot = r.minimum_pivot;
+    report_.minimumModeEigenvalue = r.minimum_mode_eigenvalue;
+  }
+  ~InletReference() { aprl_inlet_reference_destroy(r_); }
+  InletReference(const InletReference&) = delete;
+  InletReference& operator=(const InletReference&) = delete;
+
+  // native E(0), B(-dt/2) at global native indices (component order Ex..Bz)
+  Double field(unsigned component, std::size_t i, std::size_t j, std::size_t globalK) {
+    std::size_t n[3] = {0, 0, 0};
+    if (component < 6) componentShape(g_, static_cast<int>(component), n);
+    if (component >= 6 || i >= n[0] || j >= n[1] || globalK >= n[2])
+      throw std::out_of_range("Blocked field query outside native global Yee grid");
+    std::vector<double>& h = host_[component];
+    if (h.empty()) {
+      h.resize(n[0] * n[1] * n[2]);
+      check(aprl_inlet_reference_download(r_, static_cast<int>(component), h.data(), h.size()));
+    }
+    return h[(globalK * n[1] + j) * n[0] + i];
+  }
+  // BlockedCowanReference::fill: E/B of a grid with the same x-y geometry
+  // from global plane globalZOffset on; J is left alone.
+  void fill(FieldGrid& fields, std::size_t globalZOffset = 0) {
+    // blocked_cowan_reference.cpp:958-963
+    const EBGridGeometry& w = fields.geometry();
+    if (w.nx != g_.nx || w.ny != g_.ny || w.dx != g_.dx || w.dy != g_.dy || w.dz != g_.dz || w.dt != g_.dt ||
+        w.solver != g_.solver || globalZOffset > g_.nz || w.nz > g_.nz - globalZOffset)
+      throw std::invalid_argument("Blocked reference fill geometry/offset mismatch");
+    for (unsigned a = 0; a < 6; ++a) {
+      std::size_t n[3], m[3];
+      componentShape(g_, static_cast<int>(a), n);
+      componentShape(w, static_cast<int>(a), m);
+      field(a, 0, 0, 0);
+      const std::vector<double>& h = host_[a];
+      check(aprl_fields_upload(fields.handle(), static_cast<int>(a), h.data() + globalZOffset * n[0] * n[1],
+                               m[0] * m[1] * m[2]));
+    }
+  }
+  void clearCache() {
+    for (auto& h : host_) std::vector<double>().swap(h);
+  }
+  const verification::BlockedCowanReferenceReport& report() const { return report_; }
+  aprl_inlet_reference* handle() const { return r_; }
+
+ private:
+  EBGridGeometry g_;
+  aprl_inlet_reference* r_ = nullptr;
+  verification::BlockedCowanReferenceReport report_;
+  std::vector<double> host_[6];
+};
+
 // MovingWindowFields on the device. The reference callback stays on the host
 // (moving_window_fields.h:15-20): every step it is evaluated into the
 // planes the device needs, reference(a, i, j, k, step) over component a's
@@ -144,6 +236,15 @@ class MovingWindow {
       : grid_(grid), ref_(reference) {
     const aprl_cpml_parameters c = cpmlOf(cpml);
     check(aprl_window_install(grid.handle(), &c));
+  }
+  // the production reference (movingWindowFieldValue over a
+  // BlockedCowanReference translating at translationBeta,
+  // simulation_moving_window.cpp:280-294): planes filled on the device
+  MovingWindow(FieldGrid& grid, const EBCPMLParameters& cpml, InletReference& reference, Double translationBeta)
+      : grid_(grid), ref_{} {
+    const aprl_cpml_parameters c = cpmlOf(cpml);
+    check(aprl_window_install(grid.handle(), &c));
+    check(aprl_window_set_reference(grid.handle(), reference.handle(), translationBeta));
   }
   // MovingWindowFields::beginStep: lower E halo planes at offset - 1.
   void beginStep(std::size_t step, std::ptrdiff_t offset) {

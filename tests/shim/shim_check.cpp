@@ -49,6 +49,24 @@ int main() {
     aprl::gpu::componentShape(g, a, n);
     expect(n[0] == comp[a]->nx() && n[1] == comp[a]->ny() && n[2] == comp[a]->nz(), "componentShape");
   }
+  // the reference's ambiguous-shape rule and the unbuilt source forms are
+  // refused before any device work
+  const std::size_t guards[3] = {3, 3, 2};
+  const aprl::FieldVector<aprl::Double> origin;
+  const std::vector<aprl::RelativisticParticleSI> none;
+  const char* expected[3] = {"Ambiguous reference source shape", "The retired linear-notch reference source is not built",
+                             "The open-wall transverse reference is not built"};
+  const bool flags[3][4] = {{true, true, false, false}, {false, true, false, false}, {false, false, false, true}};
+  for (int c = 0; c < 3; ++c) {
+    std::string message;
+    try {
+      aprl::gpu::InletReference r(g, origin, none, 0.3, guards, flags[c][0], flags[c][1], flags[c][2], false,
+                                  flags[c][3]);
+    } catch (const std::invalid_argument& e) {
+      message = e.what();
+    }
+    expect(message == expected[c], expected[c]);
+  }
   std::printf(failures ? "shim check failed\n" : "shim check passed\n");
   return failures ? 1 : 0;
 }
