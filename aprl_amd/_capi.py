@@ -210,6 +210,7 @@ def lib() -> C.CDLL:
     L.aprl_coupling_set_particle_boundary.argtypes = [vp, C.POINTER(_sz), C.c_double]
     L.aprl_coupling_step.argtypes = [vp, vp, _dp, C.c_double, C.c_uint, C.POINTER(StepReport)]
     L.aprl_coupling_set_split_trials.argtypes = [vp, C.c_uint, C.c_int]
+    L.aprl_coupling_set_trial_batch.argtypes = [vp, C.c_uint]
     L.aprl_coupling_prepare_step.argtypes = [vp, vp, _dp]
     L.aprl_fields_initialize_gauss.argtypes = [vp, vp, _dp, C.c_int, C.c_double, C.POINTER(_sz), C.c_double, _sz,
                                                C.c_int, C.POINTER(GaussReport)]
@@ -299,7 +300,7 @@ EXPORTED = [
     "aprl_pending_admit", "aprl_pending_view", "aprl_pending_state", "aprl_pending_origin",
     "aprl_coupling_create", "aprl_coupling_destroy", "aprl_coupling_set_frame",
     "aprl_coupling_add_magnet", "aprl_coupling_set_particle_boundary", "aprl_coupling_step",
-    "aprl_coupling_set_split_trials", "aprl_coupling_prepare_step", "aprl_window_step_batch",
+    "aprl_coupling_set_split_trials", "aprl_coupling_set_trial_batch", "aprl_coupling_prepare_step", "aprl_window_step_batch",
     "aprl_lab_plane_create", "aprl_lab_plane_destroy", "aprl_lab_plane_sample", "aprl_lab_plane_fetch",
     "aprl_fields_initialize_gauss",
     "aprl_inlet_reference_create", "aprl_inlet_reference_destroy", "aprl_inlet_reference_download",

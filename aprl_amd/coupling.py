@@ -175,6 +175,11 @@ class ConservativeSourceCoupling:
         after the last trial whose force and work the step reports."""
         check(lib().aprl_coupling_set_split_trials(self.handle, int(trials), 1 if verify else 0))
 
+    def setTrialBatch(self, batch=0):
+        """Plain FP64 grids: Picard trials enqueued before the first wait for
+        the device's convergence decision (0: the last step's count)."""
+        check(lib().aprl_coupling_set_trial_batch(self.handle, int(batch)))
+
     def prepareStep(self, particles: ParticleStore, origin):
         """Start the next step's sort phase on the coupling's own stream
         (aprl_coupling_prepare_step); call before the step's field updates."""

@@ -2,11 +2,6 @@
 // (declared in include/aprl_gpu.h).
 #include "handles.hpp"
 
-namespace aprl_gpu {
-void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], double time, unsigned substeps,
-                  aprl_step_report* report);
-}  // namespace aprl_gpu
-
 using namespace aprl_gpu;
 
 namespace {
@@ -38,27 +33,42 @@ aprl_status aprl_window_step_batch(aprl_instance_step* steps, size_t count) {
         if (steps[j].fields == s.fields) throw std::invalid_argument("An instance appears twice in one batch");
     }
     // Phase 1: field half step and, for the FP32-split form, the sort and
-    // the preview, for every instance before any instance waits.
+    // the preview, for every instance before any instance waits. (A sort
+    // prepared on a second stream per instance was measured slower here:
+    // the instances' streams already keep the queues busy.)
     for (size_t i = 0; i < count; ++i) {
       aprl_instance_step& s = steps[i];
+      CouplingState& C = s.coupling->c;
       rethrow(aprl_fields_clear_current(s.fields));
       rethrow(aprl_window_begin_step(s.fields, s.step, s.offset, nullptr));
       rethrow(aprl_fields_advance_magnetic(s.fields, nullptr));
       rethrow(aprl_window_set_inlet(s.fields, nullptr, nullptr));
-      if (s.fields->fp32) splitStepBegin(s.coupling->c, s.particles->sp, s.origin, s.time, s.substeps);
+      if (s.fields->fp32) splitStepBegin(C, s.particles->sp, s.origin, s.time, s.substeps);
     }
-    // Phase 2: the trials, the commit and the window shift.
+    // Phase 2: the trials (the FP32-split form's fixed count with its
+    // commit and window shift; the plain form's first batch, whose
+    // convergence the device decides), with no wait on an instance's trials.
     for (size_t i = 0; i < count; ++i) {
       aprl_instance_step& s = steps[i];
-      if (s.fields->fp32) splitStepMiddle(s.coupling->c, s.particles->sp);
-      else couplingStep(s.coupling->c, s.particles->p, s.origin, s.time, s.substeps, &s.report);
-      rethrow(aprl_window_finish_energy_step(s.fields));
-      rethrow(aprl_window_shift(s.fields, s.step + 1, s.offset - 1, nullptr));
+      if (s.fields->fp32) {
+        splitStepMiddle(s.coupling->c, s.particles->sp);
+        rethrow(aprl_window_finish_energy_step(s.fields));
+        rethrow(aprl_window_shift(s.fields, s.step + 1, s.offset - 1, nullptr));
+      } else {
+        plainStepBegin(s.coupling->c, s.particles->p, s.origin, s.time, s.substeps);
+      }
     }
-    // Phase 3: wait for every instance and check its errors.
+    // Phase 3: wait for every instance and check its errors; the plain
+    // form's commit and window shift follow its convergence.
     for (size_t i = 0; i < count; ++i) {
       aprl_instance_step& s = steps[i];
-      if (s.fields->fp32) splitStepEnd(s.coupling->c, s.particles->sp, &s.report);
+      if (s.fields->fp32) {
+        splitStepEnd(s.coupling->c, s.particles->sp, &s.report);
+      } else {
+        plainStepFinish(s.coupling->c, s.particles->p, &s.report);
+        rethrow(aprl_window_finish_energy_step(s.fields));
+        rethrow(aprl_window_shift(s.fields, s.step + 1, s.offset - 1, nullptr));
+      }
     }
   });
 }

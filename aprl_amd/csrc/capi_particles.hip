@@ -439,4 +439,13 @@ aprl_status aprl_coupling_set_split_trials(aprl_coupling* coupling, unsigned tri
   });
 }
 
+aprl_status aprl_coupling_set_trial_batch(aprl_coupling* coupling, unsigned batch) {
+  return guarded([&] {
+    if (!coupling) throw std::invalid_argument("Null coupling");
+    if (coupling->fields->fp32) throw std::invalid_argument("Picard trial batches apply to plain FP64 grids only");
+    if (batch > static_cast<unsigned>(kMaxTrials)) throw std::invalid_argument("Picard trial batch must be in [0, 64]");
+    coupling->c.picardFixed = batch;
+  });
+}
+
 }  // extern "C"

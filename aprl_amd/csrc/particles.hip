@@ -32,6 +32,9 @@ struct GArgs {
   double o[3], h[3];
   double negS;             // (-1*dt)/eps0, applyCurrentSource(-1)
   int smoothX;
+  // the Picard stop word: a trial's kernels return at once when it is set
+  // (null for work that always runs)
+  const unsigned* stop;
 };
 
 struct PArgs {
@@ -411,6 +414,7 @@ __global__ void kFillInt(int* p, int n, int v) {
 template <int MODE, int SHAPE>
 __global__ void __launch_bounds__(2 * kChunk) kGatherChunks(PArgs P, GArgs G, StepArgs S, const int4* chunks, int ntx,
                                                             int nty, double dt, double* chunkOut, unsigned* errOut) {
+  if (G.stop && *G.stop) return;
   constexpr bool kStageB = SHAPE == kShapeBinomial3X;
   __shared__ double es[3 * boxNodes<SHAPE>()];
   __shared__ double bs[kStageB ? 3 * boxNodes<SHAPE>() : 4 * kChunk];
@@ -543,6 +547,7 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks(PArgs P, GArgs G, St
 // the segment's boundary events. (The tile current bounds the deposit's
 // fixed-point exponents come from are set once per step by the sort.)
 __global__ void __launch_bounds__(128) kPush(PArgs P, GArgs G, StepArgs S, unsigned* errOut) {
+  if (G.stop && *G.stop) return;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P.n) return;
   unsigned err = 0;
@@ -583,6 +588,7 @@ __global__ void __launch_bounds__(kChunk * depositLanes<SHAPE>()) kDeposit(PArgs
                                                    const int* tileBegin, const int* tileEnd,
                                                    const unsigned long long* tileScan, int ntx, int nty,
                                                    long long* partial, int* tileShift, unsigned* errOut) {
+  if (G.stop && *G.stop) return;
   constexpr int N = boxNodes<SHAPE>(), NX = Box<SHAPE>::NX, NY = Box<SHAPE>::NY;
   __shared__ unsigned long long acc[3 * N];
   const int4 ch = chunks[blockIdx.x];
@@ -707,6 +713,7 @@ template <int SHAPE>
 __global__ void __launch_bounds__(256) kReduceCurrent(double* jx, double* jy, double* jz, GArgs G, const int* active,
                                                       const unsigned long long* tileScan, const int* tileShift,
                                                       long long* partial, int ntx, int nty, int ntz) {
+  if (G.stop && *G.stop) return;
   constexpr int RZ = Box<SHAPE>::RZ, NB = 9 * (2 * RZ + 1);
   const int tile = active[xcdContiguous(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x))];
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
@@ -751,40 +758,66 @@ __global__ void __launch_bounds__(256) kReduceCurrent(double* jx, double* jy, do
   }
 }
 
-// With a mailbox, lane 0 also posts the four sums (doubles at byte 2048)
-// and nWords trial words (words 16..) to host memory, sequence number in
-// flag word 1.
+// One Picard trial's residual: the chunk partials reduced in a fixed tree
+// to (max force, defect sum, scale sum, coupling calls) in out. With a
+// Picard word (W.stop) thread 0 then takes the reference's convergence
+// decision on the device (simulation_source_coupling.cpp:46-51): at least
+// two trials when a particle made a coupling call, force < 1e-13 and
+// work < 1e-8, the same double arithmetic as the host test; a trial whose
+// error word holds a device error also stops the trials. Once the word is
+// set, later trials skip the reduction. With a mailbox, thread 0 posts the
+// sums (doubles at byte 2048), the stop word (word 15) and the trial words
+// firstWord..W.trial (words 16..) to host memory, sequence number in flag
+// word 1, whether or not the trial ran.
+struct PicardWords {
+  unsigned* stop;
+  unsigned trial;
+  const unsigned* trialWords;  // error words of trials 0..kMaxTrials-1
+  unsigned firstWord;
+};
+
 __global__ void __launch_bounds__(256) kReduceBlocks(const double* blockIn, int blocks, double* out, unsigned* box,
-                                                     unsigned seq, const unsigned* trialWords, int nWords) {
+                                                     unsigned seq, PicardWords W) {
   __shared__ double sf[256], sd[256], ss[256], sc[256];
-  double f = 0., d = 0., s = 0., c = 0.;
-  for (int b = threadIdx.x; b < blocks; b += 256) {
-    f = fmax(f, blockIn[4 * b]);
-    d += blockIn[4 * b + 1];
-    s += blockIn[4 * b + 2];
-    c += blockIn[4 * b + 3];
-  }
-  sf[threadIdx.x] = f; sd[threadIdx.x] = d; ss[threadIdx.x] = s; sc[threadIdx.x] = c;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
-      sf[threadIdx.x] = fmax(sf[threadIdx.x], sf[threadIdx.x + w]);
-      sd[threadIdx.x] += sd[threadIdx.x + w];
-      ss[threadIdx.x] += ss[threadIdx.x + w];
-      sc[threadIdx.x] += sc[threadIdx.x + w];
+  const bool skip = W.stop && *W.stop;
+  if (!skip) {
+    double f = 0., d = 0., s = 0., c = 0.;
+    for (int b = threadIdx.x; b < blocks; b += 256) {
+      f = fmax(f, blockIn[4 * b]);
+      d += blockIn[4 * b + 1];
+      s += blockIn[4 * b + 2];
+      c += blockIn[4 * b + 3];
     }
+    sf[threadIdx.x] = f; sd[threadIdx.x] = d; ss[threadIdx.x] = s; sc[threadIdx.x] = c;
     __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    out[0] = sf[0]; out[1] = sd[0]; out[2] = ss[0]; out[3] = sc[0];
-    if (box) {
-      volatile double* vd = reinterpret_cast<volatile double*>(reinterpret_cast<unsigned char*>(box) + 2048);
-      vd[0] = sf[0]; vd[1] = sd[0]; vd[2] = ss[0]; vd[3] = sc[0];
-      volatile unsigned* v = box;
-      for (int w = 0; w < nWords; ++w) v[16 + w] = trialWords[w];
-      __threadfence_system();
-      v[1] = seq;
+    for (int w = 128; w > 0; w >>= 1) {
+      if (threadIdx.x < w) {
+        sf[threadIdx.x] = fmax(sf[threadIdx.x], sf[threadIdx.x + w]);
+        sd[threadIdx.x] += sd[threadIdx.x + w];
+        ss[threadIdx.x] += ss[threadIdx.x + w];
+        sc[threadIdx.x] += sc[threadIdx.x + w];
+      }
+      __syncthreads();
     }
+    if (threadIdx.x == 0) {
+      out[0] = sf[0]; out[1] = sd[0]; out[2] = ss[0]; out[3] = sc[0];
+      if (W.stop) {
+        const double force = sf[0];
+        const double work = sc[0] == 0. ? 0. : (ss[0] > 0. ? sd[0] / ss[0] : sd[0]);
+        const unsigned iterations = W.trial + 1;
+        if (W.trialWords[W.trial] & ~kEvtAny) *W.stop = iterations | kStopError;
+        else if (iterations >= (sc[0] > 0. ? 2u : 1u) && force < 1e-13 && work < 1e-8) *W.stop = iterations;
+      }
+    }
+  }
+  if (threadIdx.x == 0 && box) {
+    volatile double* vd = reinterpret_cast<volatile double*>(reinterpret_cast<unsigned char*>(box) + 2048);
+    vd[0] = out[0]; vd[1] = out[1]; vd[2] = out[2]; vd[3] = out[3];
+    volatile unsigned* v = box;
+    v[15] = W.stop ? *W.stop : 0u;
+    for (unsigned w = W.firstWord; w <= W.trial; ++w) v[16 + w - W.firstWord] = W.trialWords[w];
+    __threadfence_system();
+    v[1] = seq;
   }
 }
 
@@ -1099,7 +1132,7 @@ void fillInts(hipStream_t st, int* p, int n, int v) {
 }
 
 void reduceChunkStats(hipStream_t st, const double* in, int chunks, double* out) {
-  kReduceBlocks<<<1, 256, 0, st>>>(in, chunks, out, nullptr, 0u, nullptr, 0);
+  kReduceBlocks<<<1, 256, 0, st>>>(in, chunks, out, nullptr, 0u, PicardWords{nullptr, 0u, nullptr, 0u});
   launchCheck("chunk stats");
 }
 
@@ -1240,9 +1273,128 @@ void prepareStep(CouplingState& C, ParticleStore& P, const double origin[3]) {
   R.seq = seq;
 }
 
-void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], double time, unsigned substeps,
-                  aprl_step_report* report) {
+// Per-step state of a plain FP64 step between its two phases.
+struct PlainStepContext {
+  bool pending = false;
+  int n = 0, ntx = 0, nty = 0, ntz = 0;
+  unsigned nChunks = 0, nActive = 0, occupied = 0, activeEntries = 0;
+  unsigned enqueued = 0;       // Picard trials enqueued so far
+  unsigned checkedTrials = 0;  // trial words read back so far
+  unsigned postSeq = 0;        // mailbox sequence of the last posted trial
+  bool quintic = false;
+  double origin[3] = {0.0, 0.0, 0.0};
+  PArgs A{};
+  GArgs G{};
+  StepArgs S{};
+};
+
+namespace {
+
+// Enqueue Picard trials [first, last) of the step; the last one posts its
+// residual, the stop word and the unchecked trial words to the mailbox.
+// Every trial runs behind the device stop word: once a trial has converged
+// (or failed), the kernels of the later ones return at once, so trials can
+// be enqueued ahead of the convergence decision.
+void enqueueTrials(CouplingState& C, PlainStepContext& X, unsigned first, unsigned last) {
   FieldState& F = *C.fields;
+  hipStream_t st = F.stream;
+  const int n = X.n, ntx = X.ntx, nty = X.nty, ntz = X.ntz;
+  const unsigned nChunks = X.nChunks, nActive = X.nActive;
+  const bool quintic = X.quintic;
+  PArgs& A = X.A;
+  const GArgs& G = X.G;
+  const StepArgs& S = X.S;
+  unsigned* const stop = C.error.get() + kStopWord;
+  for (unsigned trial = first; trial < last; ++trial) {
+    unsigned* errT = C.error.get() + kTrialWord + trial;
+    if (nChunks && trial == 0) {
+      // The first trial pushes against the preview with J cleared
+      // (simulation_source_coupling.cpp:43).
+      GArgs G0 = G;
+      for (int a = 0; a < 3; ++a) G0.j[a] = nullptr;
+      G0.stop = nullptr;
+      ScopedKernel sk(F.profiler, st, "gather");
+      if (quintic)
+        kGatherChunks<0, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G0, S, C.chunks.get(), ntx, nty, F.g.dt,
+                                                                         nullptr, errT);
+      else
+        kGatherChunks<0, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G0, S, C.chunks.get(), ntx, nty, F.g.dt,
+                                                                            nullptr, errT);
+      launchCheck("gather");
+    }
+    if (n) {
+      ScopedKernel sk(F.profiler, st, "push");
+      kPush<<<ceilDiv(n, 128), 128, 0, st>>>(A, G, S, errT);
+      launchCheck("push");
+    }
+    if (nChunks) {
+      {
+        ScopedKernel sk(F.profiler, st, "deposit");
+        if (quintic)
+          kDeposit<kShapeQuintic><<<nChunks, kChunk * depositLanes<kShapeQuintic>(), 0, st>>>(
+              A, G, S, C.chunks.get(), C.tileBegin.get(), C.tileEnd.get(), C.tileScan.get(), ntx, nty,
+              C.partial.get(), C.tileShift.get(), errT);
+        else
+          kDeposit<kShapeBinomial3X><<<nChunks, kChunk * depositLanes<kShapeBinomial3X>(), 0, st>>>(
+              A, G, S, C.chunks.get(), C.tileBegin.get(), C.tileEnd.get(), C.tileScan.get(), ntx, nty,
+              C.partial.get(), C.tileShift.get(), errT);
+        launchCheck("deposit");
+      }
+      {
+        ScopedKernel sk(F.profiler, st, "reduce_current");
+        if (quintic)
+          kReduceCurrent<kShapeQuintic><<<nActive, 256, 0, st>>>(F.base(6), F.base(7), F.base(8), G,
+                                                                 C.activeTiles.get(), C.tileScan.get(),
+                                                                 C.tileShift.get(), C.partial.get(), ntx, nty, ntz);
+        else
+          kReduceCurrent<kShapeBinomial3X><<<nActive, 256, 0, st>>>(F.base(6), F.base(7), F.base(8), G,
+                                                                    C.activeTiles.get(), C.tileScan.get(),
+                                                                    C.tileShift.get(), C.partial.get(), ntx, nty,
+                                                                    ntz);
+        launchCheck("reduce current");
+      }
+      F.currentZero = false;
+      F.footprint.valid = true;
+      F.footprint.active = C.activeTiles.get();
+      F.footprint.count = static_cast<int>(nActive);
+      F.footprint.ntx = ntx;
+      F.footprint.nty = nty;
+      F.footprint.ntz = ntz;
+      F.footprint.tx = kTileX;
+      F.footprint.ty = kTileY;
+      F.footprint.tz = kTileZ;
+    }
+    // TransverseReferenceCurrent::apply after the trial's deposit
+    // (simulation_source_coupling.cpp:44)
+    referenceCurrentTrial(C, F, C.esum[2].get(), C.estar[2].get(), true, nChunks > 0, ntx, nty, ntz, stop);
+    const bool post = trial + 1 == last;
+    if (post) X.postSeq = C.mailbox.next(1);
+    {
+      ScopedKernel sk(F.profiler, st, "residual");
+      if (nChunks && quintic)
+        kGatherChunks<1, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G, S, C.chunks.get(), ntx, nty, F.g.dt,
+                                                                         C.reduceBlocks.get(), errT);
+      else if (nChunks)
+        kGatherChunks<1, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G, S, C.chunks.get(), ntx, nty, F.g.dt,
+                                                                            C.reduceBlocks.get(), errT);
+    }
+    kReduceBlocks<<<1, 256, 0, st>>>(C.reduceBlocks.get(), static_cast<int>(nChunks), C.reduceOut.get(),
+                                     post ? C.mailbox.device() : nullptr, post ? X.postSeq : 0u,
+                                     PicardWords{stop, trial, C.error.get() + kTrialWord, X.checkedTrials});
+    launchCheck("residual");
+    // The next trial pushes with the fields this residual gathered.
+    for (int a = 0; a < 6; ++a) std::swap(A.fin[a], A.fout[a]);
+  }
+  X.enqueued = last;
+}
+
+}  // namespace
+
+void plainStepBegin(CouplingState& C, ParticleStore& P, const double origin[3], double time, unsigned substeps) {
+  FieldState& F = *C.fields;
+  if (!C.plain) C.plain = std::make_shared<PlainStepContext>();
+  PlainStepContext& X = *C.plain;
+  if (X.pending) throw std::logic_error("The previous coupling step was not finished");
   if (F.window && F.phase != 2) throw std::logic_error("Moving-window electric phase mismatch");
   if (F.window && !F.inletSet) throw std::logic_error("Moving-window inlet planes were not supplied");
   if (substeps == 0) throw std::invalid_argument("Particle substeps must be positive");
@@ -1257,16 +1409,15 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
       C.esum[a].resize(F.compSize(a));
     }
 
-  const GArgs G = gatherArgs(C, F, origin);
   const int ntx = static_cast<int>((F.g.nx + kTileX - 1) / kTileX);
   const int nty = static_cast<int>((F.g.ny + kTileY - 1) / kTileY);
   const int ntz = static_cast<int>((F.g.nz + kTileZ - 1) / kTileZ);
   const int ntiles = ntx * nty * ntz;
   const std::size_t chunkBound = chunkListBound(ntiles, n);
   // Readbacks arrive in the mailbox: the scan total (chunks, active tiles)
-  // and the sort-phase words at words 8..13 (flag 0); a trial's residual
-  // sums (force, defect, scale, calls; doubles) and its error words from
-  // word 16 (flag 1).
+  // and the sort-phase words at words 8..14 (flag 0); the posting trial's
+  // residual sums (doubles), the stop word and the trial error words from
+  // word 15 (flag 1).
   unsigned sortSeq = 0;
   if (preparedFor(C, &P, P.generation, P.count, origin)) {
     hipCheck(hipStreamWaitEvent(st, C.sortReady.get(), 0), "hipStreamWaitEvent");
@@ -1313,120 +1464,65 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
   if (C.reduceBlocks.size() < 4 * static_cast<std::size_t>(std::max(1u, nChunks)))
     C.reduceBlocks.resize(4 * static_cast<std::size_t>(std::max(1u, nChunks)));
 
-  const StepArgs S = stepArgsFor(C, F.g, F.hasCpml ? &F.cpml : nullptr, F.window, origin, time, substeps);
-  PArgs A = particleArgs(P);
-  A.scaleDen = F.g.dx * F.g.dy * F.g.dz * F.g.dt;
-  A.tileOf = P.keySorted.get();
-  A.tileBound = C.tileBound.get();
-  bool converged = false;
-  unsigned iterations = 0;
+  X.S = stepArgsFor(C, F.g, F.hasCpml ? &F.cpml : nullptr, F.window, origin, time, substeps);
+  X.A = particleArgs(P);
+  X.A.scaleDen = F.g.dx * F.g.dy * F.g.dz * F.g.dt;
+  X.A.tileOf = P.keySorted.get();
+  X.A.tileBound = C.tileBound.get();
+  X.G = gatherArgs(C, F, origin);
+  X.G.stop = C.error.get() + kStopWord;
+  X.n = n;
+  X.ntx = ntx;
+  X.nty = nty;
+  X.ntz = ntz;
+  X.nChunks = nChunks;
+  X.nActive = nActive;
+  X.occupied = occupiedTiles;
+  X.activeEntries = activeEntries;
+  X.quintic = quintic;
+  X.enqueued = 0;
+  X.checkedTrials = 0;
+  for (int a = 0; a < 3; ++a) X.origin[a] = origin[a];
+  X.pending = true;
+  // With active particles the convergence test needs at least two trials
+  // (simulation_source_coupling.cpp:50); the first batch is the last
+  // step's trial count.
+  const unsigned batch = C.picardFixed ? C.picardFixed
+                                       : std::min<unsigned>(kMaxTrials, std::max(C.picardBatch, activeEntries ? 2u : 1u));
+  enqueueTrials(C, X, 0, batch);
+}
+
+void plainStepFinish(CouplingState& C, ParticleStore& P, aprl_step_report* report) {
+  if (!C.plain || !C.plain->pending) throw std::logic_error("Coupling step phase mismatch");
+  PlainStepContext& X = *C.plain;
+  X.pending = false;
+  FieldState& F = *C.fields;
+  hipStream_t st = F.stream;
+  unsigned iterations = 0, lastWord = 0, stop = 0;
   double force = 0., work = 0.;
-  unsigned checkedTrials = 0, lastWord = 0;
-  for (unsigned trial = 0; trial < 64; ++trial) {
-    unsigned* errT = C.error.get() + kTrialWord + trial;
-    if (nChunks && trial == 0) {
-      // The first trial pushes against the preview with J cleared
-      // (simulation_source_coupling.cpp:43).
-      GArgs G0 = G;
-      for (int a = 0; a < 3; ++a) G0.j[a] = nullptr;
-      ScopedKernel sk(F.profiler, st, "gather");
-      if (quintic)
-        kGatherChunks<0, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G0, S, C.chunks.get(), ntx, nty, F.g.dt,
-                                                                         nullptr, errT);
-      else
-        kGatherChunks<0, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G0, S, C.chunks.get(), ntx, nty, F.g.dt,
-                                                                            nullptr, errT);
-      launchCheck("gather");
-    }
-    if (n) {
-      ScopedKernel sk(F.profiler, st, "push");
-      kPush<<<ceilDiv(n, 128), 128, 0, st>>>(A, G, S, errT);
-      launchCheck("push");
-    }
-    if (nChunks) {
-      {
-        ScopedKernel sk(F.profiler, st, "deposit");
-        if (quintic)
-          kDeposit<kShapeQuintic><<<nChunks, kChunk * depositLanes<kShapeQuintic>(), 0, st>>>(A, G, S, C.chunks.get(), C.tileBegin.get(),
-                                                              C.tileEnd.get(), C.tileScan.get(), ntx, nty,
-                                                              C.partial.get(), C.tileShift.get(), errT);
-        else
-          kDeposit<kShapeBinomial3X><<<nChunks, kChunk * depositLanes<kShapeBinomial3X>(), 0, st>>>(A, G, S, C.chunks.get(), C.tileBegin.get(),
-                                                                 C.tileEnd.get(), C.tileScan.get(), ntx, nty,
-                                                                 C.partial.get(), C.tileShift.get(), errT);
-        launchCheck("deposit");
-      }
-      {
-        ScopedKernel sk(F.profiler, st, "reduce_current");
-        if (quintic)
-          kReduceCurrent<kShapeQuintic><<<nActive, 256, 0, st>>>(
-              F.base(6), F.base(7), F.base(8), G, C.activeTiles.get(), C.tileScan.get(), C.tileShift.get(),
-              C.partial.get(), ntx, nty, ntz);
-        else
-          kReduceCurrent<kShapeBinomial3X><<<nActive, 256, 0, st>>>(
-              F.base(6), F.base(7), F.base(8), G, C.activeTiles.get(), C.tileScan.get(), C.tileShift.get(),
-              C.partial.get(), ntx, nty, ntz);
-        launchCheck("reduce current");
-      }
-      F.currentZero = false;
-      F.footprint.valid = true;
-      F.footprint.active = C.activeTiles.get();
-      F.footprint.count = static_cast<int>(nActive);
-      F.footprint.ntx = ntx;
-      F.footprint.nty = nty;
-      F.footprint.ntz = ntz;
-      F.footprint.tx = kTileX;
-      F.footprint.ty = kTileY;
-      F.footprint.tz = kTileZ;
-    }
-    // TransverseReferenceCurrent::apply after the trial's deposit
-    // (simulation_source_coupling.cpp:44)
-    referenceCurrentTrial(C, F, C.esum[2].get(), C.estar[2].get(), true, nChunks > 0, ntx, nty, ntz);
-    // With active particles the convergence test needs at least two trials
-    // (simulation_source_coupling.cpp:50), so trial 0 is not read back; its
-    // error word is checked, first, after trial 1. Held entries alone may
-    // converge in one trial (no coupling call unless one releases).
-    const bool readback = !(activeEntries && trial == 0);
-    const unsigned trialSeq = readback ? C.mailbox.next(1) : 0u;
-    {
-      {
-        ScopedKernel sk(F.profiler, st, "residual");
-        if (nChunks && quintic)
-          kGatherChunks<1, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G, S, C.chunks.get(), ntx, nty, F.g.dt,
-                                                                           C.reduceBlocks.get(), errT);
-        else if (nChunks)
-          kGatherChunks<1, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G, S, C.chunks.get(), ntx, nty,
-                                                                              F.g.dt, C.reduceBlocks.get(), errT);
-      }
-      kReduceBlocks<<<1, 256, 0, st>>>(C.reduceBlocks.get(), static_cast<int>(nChunks), C.reduceOut.get(),
-                                       readback ? C.mailbox.device() : nullptr, trialSeq,
-                                       C.error.get() + kTrialWord + checkedTrials,
-                                       static_cast<int>(trial + 1 - checkedTrials));
-      launchCheck("residual");
-    }
-    iterations = trial + 1;
-    // The next trial pushes with the fields this residual gathered.
-    for (int a = 0; a < 6; ++a) std::swap(A.fin[a], A.fout[a]);
-    if (!readback) continue;
-    C.mailbox.wait(1, trialSeq, st);
-    const unsigned* words = C.mailbox.host() + 16;
+  for (;;) {
+    C.mailbox.wait(1, X.postSeq, st);
+    const unsigned* box = C.mailbox.host();
     const double* sums = C.mailbox.hostDoubles();
-    for (unsigned t = checkedTrials; t <= trial; ++t) raiseParticleErrors(words[t - checkedTrials]);
-    lastWord = words[trial - checkedTrials];
-    checkedTrials = trial + 1;
-    // simulation_source_coupling.cpp:46-51.
+    stop = box[15];
+    const unsigned ran = stop ? (stop & ~kStopError) : X.enqueued;
+    // device errors in trial order, as the reference raises them
+    for (unsigned t = X.checkedTrials; t < ran; ++t) raiseParticleErrors(box[16 + t - X.checkedTrials]);
+    if (ran) lastWord = box[16 + ran - 1 - X.checkedTrials];
+    X.checkedTrials = X.enqueued;
+    iterations = ran;
     force = sums[0];
     work = sums[3] == 0. ? 0. : (sums[2] > 0. ? sums[1] / sums[2] : sums[1]);
-    if (iterations >= (sums[3] > 0. ? 2u : 1u) && force < 1e-13 && work < 1e-8) {
-      converged = true;
-      break;
-    }
+    if (stop || X.enqueued == static_cast<unsigned>(kMaxTrials)) break;
+    // not converged within the batch: one more trial at a time
+    enqueueTrials(C, X, X.enqueued, X.enqueued + 1);
   }
-  if (!converged) {
+  if (!stop) {
     std::string m = "Conservative source coupling failed: force=" + std::to_string(force) +
                     " work=" + std::to_string(work) + " iterations=" + std::to_string(iterations);
     throw std::runtime_error(m);
   }
+  C.picardBatch = iterations;
   // commitConservativeElectric: E^n + curl B - dt/eps0 J, then the committed
   // CPML-E (history advance), PEC and inlet overwrite.
   double* const none[3] = {nullptr, nullptr, nullptr};
@@ -1435,7 +1531,7 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
   if (F.window) F.phase = 3;
   // Boundary events of the converged trial: carriers, removals, counters.
   if (lastWord & kEvtAny)
-    commitEvents(C, st, n, P.evt.get(), P.state.get(), P.s[8].get(), P.s[6].get(), P.wend.get(), P.removed.get(),
+    commitEvents(C, st, X.n, P.evt.get(), P.state.get(), P.s[8].get(), P.s[6].get(), P.wend.get(), P.removed.get(),
                  C.error.get() + kTrialWord + (iterations - 1), false);
   // The trial end state becomes the particle state; the start positions
   // stay in the end arrays for the step diagnostics.
@@ -1443,15 +1539,21 @@ void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], do
   C.lastStep.valid = true;
   C.lastStep.store = &P;
   C.lastStep.generation = P.generation;
-  for (int a = 0; a < 3; ++a) C.lastStep.origin[a] = origin[a];
+  for (int a = 0; a < 3; ++a) C.lastStep.origin[a] = X.origin[a];
   if (report) {
     report->iterations = iterations;
     report->force = force;
     report->work = work;
-    report->tiles = occupiedTiles;
-    report->chunks = nChunks;
-    report->active_tiles = nActive;
+    report->tiles = X.occupied;
+    report->chunks = X.nChunks;
+    report->active_tiles = X.nActive;
   }
+}
+
+void couplingStep(CouplingState& C, ParticleStore& P, const double origin[3], double time, unsigned substeps,
+                  aprl_step_report* report) {
+  plainStepBegin(C, P, origin, time, substeps);
+  plainStepFinish(C, P, report);
 }
 
 }  // namespace aprl_gpu

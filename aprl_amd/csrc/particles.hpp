@@ -64,9 +64,15 @@ constexpr int kCounterSlots = 20;
 // Error words of a coupling step: [0] sort-phase errors, [1] occupied
 // tiles, [2] entries keyed behind the live ones (removed or cropped),
 // [3] cropped carriers, [4] live active (not held) particles at the sort,
-// [kTrialWord + t] the kernels of Picard trial t.
+// [kTrialWord + t] the kernels of Picard trial t, [kStopWord] the plain
+// form's Picard state: 0 while trials run, else the iteration count of the
+// trial that converged or failed (kStopError); the trials' kernels return
+// at once when it is set.
 constexpr int kTrialWord = 5;
-constexpr int kErrorWords = kTrialWord + 64;
+constexpr int kMaxTrials = 64;  // simulation_source_coupling.cpp:40
+constexpr int kStopWord = kTrialWord + kMaxTrials;
+constexpr int kErrorWords = kStopWord + 1;
+constexpr unsigned kStopError = 1u << 31;
 
 struct MagnetDev {
   int type;  // 0 undulator, 1 dipole, 2 quadrupole
@@ -134,6 +140,7 @@ struct SplitStore {
 };
 
 struct SplitStepContext;  // per-step state between the phases of a split step
+struct PlainStepContext;  // the same for a plain FP64 step
 
 // Particles per work chunk: one chunk is a contiguous run of tile-sorted
 // particles of one tile. The deposit gives each chunk one workgroup of
@@ -191,6 +198,11 @@ struct CouplingState {
   HostMailbox mailbox;  // kernel-written readbacks of the plain form
   Event sorted, stepDone;
   std::shared_ptr<SplitStepContext> split;
+  std::shared_ptr<PlainStepContext> plain;
+  // plain form: Picard trials enqueued before the host first waits (the
+  // last step's count; later trials of a converged step return at once)
+  unsigned picardBatch = 2;
+  unsigned picardFixed = 0;  // a fixed first batch (0: adaptive)
   // A sort phase started ahead of the step (prepareStep) on its own
   // stream, so that it overlaps the field updates enqueued after it:
   // sortStream waits for `prepareStart` (recorded on the grid's stream at
@@ -337,9 +349,11 @@ void cropCounters(CouplingState& C, hipStream_t st, int lo, int n, const int* st
 // trial's particle current on active tiles, else 0) + the interpolated
 // table value; with writeEsum the E_cand + E_old sums of the active ring
 // nodes follow. Checks the table range and leaves J for a full clear.
+// A non-null stop word skips the update once it is set (the plain form's
+// speculative trials).
 template <class R>
 void referenceCurrentTrial(CouplingState& C, FieldStateT<R>& F, R* esumZ, const R* estarZ, bool writeEsum,
-                           bool anyActive, int ntx, int nty, int ntz);
+                           bool anyActive, int ntx, int nty, int ntz, const unsigned* stop = nullptr);
 // Live entries of a store whose last commit may have removed some.
 std::size_t liveEntries(std::size_t count, const DeviceArray<unsigned>& removed, hipStream_t st);
 // Stable sort of (tile key, particle index) pairs on the low `bits` key
@@ -378,6 +392,11 @@ bool preparedFor(const CouplingState& C, const void* store, std::uint64_t genera
 // interleaved: begin (sort, preview), middle (trials, commit, async
 // readback), end (wait, errors, report).
 void splitStepBegin(CouplingState& C, SplitStore& P, const double origin[3], double time, unsigned substeps);
+// Plain FP64 step in two phases: begin (sort or the prepared sort, preview,
+// the first batch of Picard trials) and finish (wait for the device's
+// convergence decision, further trials if needed, commit, events, report).
+void plainStepBegin(CouplingState& C, ParticleStore& P, const double origin[3], double time, unsigned substeps);
+void plainStepFinish(CouplingState& C, ParticleStore& P, aprl_step_report* report);
 void splitStepMiddle(CouplingState& C, SplitStore& P);
 void splitStepEnd(CouplingState& C, SplitStore& P, aprl_step_report* report);
 

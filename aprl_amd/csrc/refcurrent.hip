@@ -16,7 +16,8 @@ template <class R>
 __global__ void kReferenceCurrent(R* jz, R* esum, const R* estar, const R* eold, double negS, int writeEsum,
                                   const unsigned long long* tileCounts, int anyActive, int ntx, int nty, int ntz,
                                   const double* table, const int* pts, int count, int rows, int nx, int ny, int nz,
-                                  double offset, double shiftZ) {
+                                  double offset, double shiftZ, const unsigned* stop) {
+  if (stop && *stop) return;
   const long long total = static_cast<long long>(count) * nz;
   for (long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; t < total;
        t += static_cast<long long>(gridDim.x) * blockDim.x) {
@@ -45,7 +46,7 @@ __global__ void kReferenceCurrent(R* jz, R* esum, const R* estar, const R* eold,
 
 template <class R>
 void referenceCurrentTrial(CouplingState& C, FieldStateT<R>& F, R* esumZ, const R* estarZ, bool writeEsum,
-                           bool anyActive, int ntx, int nty, int ntz) {
+                           bool anyActive, int ntx, int nty, int ntz, const unsigned* stop) {
   if (!C.refCurrent) return;
   const aprl_grid_geometry& g = F.g;
   const double offset = static_cast<double>(F.offset);
@@ -59,7 +60,7 @@ void referenceCurrentTrial(CouplingState& C, FieldStateT<R>& F, R* esumZ, const 
   kReferenceCurrent<R><<<std::max(blocks, 1), 256, 0, F.stream>>>(
       F.base(8), esumZ, estarZ, F.base(2), -1.0 * g.dt / kEps0, writeEsum ? 1 : 0, C.tileCounts.get(), anyActive ? 1 : 0, ntx, nty, ntz,
       C.refTable.get(), C.refPoints.get(), static_cast<int>(C.refCount), static_cast<int>(C.refRows),
-      static_cast<int>(g.nx), static_cast<int>(g.ny), static_cast<int>(g.nz), offset, shiftZ);
+      static_cast<int>(g.nx), static_cast<int>(g.ny), static_cast<int>(g.nz), offset, shiftZ, stop);
   launchCheck("reference current");
   // J now holds values outside the particles' tiles: the next clear is full
   F.footprint.valid = false;
@@ -67,8 +68,8 @@ void referenceCurrentTrial(CouplingState& C, FieldStateT<R>& F, R* esumZ, const 
 }
 
 template void referenceCurrentTrial<double>(CouplingState&, FieldStateT<double>&, double*, const double*, bool, bool,
-                                            int, int, int);
+                                            int, int, int, const unsigned*);
 template void referenceCurrentTrial<float>(CouplingState&, FieldStateT<float>&, float*, const float*, bool, bool, int,
-                                           int, int);
+                                           int, int, const unsigned*);
 
 }  // namespace aprl_gpu

@@ -446,9 +446,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-large", action="store_true", help="skip the FEL segment and capacity legs")
-    ap.add_argument("--leg", choices=("all", "fp64", "fp32-split", "production"), default="all",
-                    help="time only this form's binomial3-x leg (for profiling runs), or only the 2000-step "
-                         "production runs")
+    ap.add_argument("--leg", choices=("all", "fp64", "fp32-split", "production", "batch"), default="all",
+                    help="time only this form's binomial3-x leg (for profiling runs), only the 2000-step "
+                         "production runs, or only the batched scan")
     ap.add_argument("--error-study", action="store_true",
                     help="print the FP32-split vs FP64 errors at 100, 1000 and 10000 steps and exit")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -474,6 +474,9 @@ def main():
         return
     if args.leg == "production":
         print(json.dumps({"production": production_legs()}))
+        return
+    if args.leg == "batch":
+        print(json.dumps({"batch": run_batch()}))
         return
     if args.leg != "all":
         w, elapsed, trials, _, (tiles, chunks, active) = run_gpu(args, profile=False, precision=args.leg)

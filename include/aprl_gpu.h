@@ -367,6 +367,13 @@ aprl_status aprl_coupling_prepare_step(aprl_coupling* coupling, aprl_particles* 
  * trial so that the step report carries its force and work (otherwise 0).
  * On such a grid the step's origin must lie on the cell lattice. */
 aprl_status aprl_coupling_set_split_trials(aprl_coupling* coupling, unsigned trials, int verify);
+/* Plain FP64 grids only (no counterpart in the reference, whose loop tests
+ * convergence after every trial on the host): aprl_coupling_step enqueues
+ * its first `batch` Picard trials (1..64) before it waits for the device's
+ * convergence decision, then one trial at a time; a trial enqueued after
+ * the one that converged returns at once, so the result does not depend on
+ * the batch. 0 (the default) takes the last step's trial count. */
+aprl_status aprl_coupling_set_trial_batch(aprl_coupling* coupling, unsigned batch);
 
 /* ---------------------------------------------------------------------------
  * Batched steps of independent instances.

@@ -144,6 +144,28 @@ def test_dense_bunch_many_particles_per_tile(shape):
     compare(*args[:3], args[5])
 
 
+def test_speculative_trial_batches_do_not_change_the_step():
+    """The device decides convergence (simulation_source_coupling.cpp:46-51)
+    and later trials of a batch return at once: one trial per host wait,
+    eight enqueued ahead and the adaptive batch give the oracle's trial
+    counts and bitwise the same fields and particles."""
+    dims = (24, 24, 48)
+    runs = []
+    for batch in (1, 8, 0):
+        args = case(dims, (4, 4, 0), 4000, 23, spread=(0.8, 0.8, 2.0))
+        args[4].setTrialBatch(batch)
+        print(f"trial batch {batch}", flush=True)
+        logs = run_both(*args, dims, 3, 9)
+        assert all(a == b for a, b in logs), (batch, logs)
+        gpu, store = args[2], args[5]
+        runs.append(([gpu.component(a) for a in range(9)], store.download()))
+    for comps, parts in runs[1:]:
+        for a in range(9):
+            assert np.array_equal(comps[a], runs[0][0][a]), a
+        for x, y in zip(parts, runs[0][1]):
+            assert np.array_equal(x, y)
+
+
 def test_gpu_step_is_deterministic_and_sort_is_stable():
     import aprl_amd as A
     dims = (20, 20, 40)
