@@ -15,6 +15,16 @@
 #include <utility>
 #include <vector>
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2. Work items next to each other in a list (chunks of one tile box,
+// blocks of neighbouring z planes) share data, so block b is given item
+// (b mod 8)'s contiguous share of the list instead of item b: neighbouring
+// items then run on one XCD and re-read its L2. A bijection on [0, n).
+__device__ __forceinline__ int xcdContiguous(int b, int n) {
+  const int q = n >> 3, r = n & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 #include "aprl_gpu.h"
 
 namespace aprl_gpu {

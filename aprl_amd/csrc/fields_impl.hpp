@@ -503,8 +503,13 @@ template <class R>
 __global__ void __launch_bounds__(kMX * kMY) kMagneticCowanMarch(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P) {
   __shared__ R buf[kStageN];  // [component][ring slot][kLY][kLX]
   const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
-  const int i0 = blockIdx.x * kMX, j0 = blockIdx.y * kMY;
-  const int kb = blockIdx.z * kMZ, ke = min(kb + kMZ, nz + 1);
+  // tiles x fastest, then y, then the z runs; an XCD takes a contiguous
+  // share, so the halo rows and the runs' boundary planes its neighbours
+  // read stay in its L2
+  const int gx = (nx + kMX - 1) / kMX, gy = (ny + kMY - 1) / kMY;
+  const int w = xcdContiguous(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+  const int i0 = (w % gx) * kMX, j0 = ((w / gx) % gy) * kMY;
+  const int kb = (w / (gx * gy)) * kMZ, ke = min(kb + kMZ, nz + 1);
   const int tx = static_cast<int>(threadIdx.x) % kMX, ty = static_cast<int>(threadIdx.x) / kMX;
   const int i = i0 + tx, j = j0 + ty;
   // The region node (gi, gj) of a lane is the same on every plane: its
@@ -686,11 +691,15 @@ template <int MODE, class R>
 __global__ void __launch_bounds__(256) kElectric(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P, R* ox, R* oy,
                                                   R* oz, R* sx, R* sy, R* sz, const R* inletX,
                                                   const R* inletY) {
-  // one thread per node of the (nx+1) x (ny+1) plane k = blockIdx.y
+  // one thread per node of the (nx+1) x (ny+1) plane k; the blocks of a
+  // plane and of the planes next to it run on one XCD, whose L2 then holds
+  // the B plane k - 1 the curl reads
   const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int perPlane = (((nx + 1) * (ny + 1)) + blockDim.x - 1) / blockDim.x;
+  const int w = xcdContiguous(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+  const int t = (w % perPlane) * blockDim.x + threadIdx.x;
   const int i = t % (nx + 1), j = t / (nx + 1);
-  const int k = blockIdx.y;
+  const int k = w / perPlane;
   if (j > ny || k > nz) return;
   electricNode<MODE>(F, c, P, ox, oy, oz, sx, sy, sz, inletX, inletY, i, j, k);
 }
@@ -707,7 +716,8 @@ __global__ void __launch_bounds__(256) kElectricPreviewTiles(FieldPtrsT<R> F, St
   const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
   const int count = activeField(*total);
   for (int b = blockIdx.x; b < count; b += gridDim.x) {
-    const int tile = tiles[b];
+    // neighbouring tiles of the list on one XCD (when one pass covers it)
+    const int tile = tiles[count <= static_cast<int>(gridDim.x) ? xcdContiguous(b, count) : b];
     const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
     const int i0 = tx * tg.tx, i1 = tx == tg.ntx - 1 ? nx + 1 : min(nx + 1, i0 + tg.tx);
     const int j0 = ty * tg.ty, j1 = ty == tg.nty - 1 ? ny + 1 : min(ny + 1, j0 + tg.ty);
@@ -741,7 +751,7 @@ void launchMagnetic(FieldStateT<R>& s, const FieldPtrsT<R>& f) {
   const CpmlArgsT<R> P = s.cpmlArgs();
   ScopedKernel sk(s.profiler, s.stream, "magnetic");
   if (s.g.solver == 1) {
-    dim3 tg(ceilDiv(s.g.nx, kMX), ceilDiv(s.g.ny, kMY), ceilDiv(s.g.nz + 1, kMZ));
+    dim3 tg(ceilDiv(s.g.nx, kMX) * ceilDiv(s.g.ny, kMY) * ceilDiv(s.g.nz + 1, kMZ), 1, 1);
     kMagneticCowanMarch<R><<<tg, kMX * kMY, 0, s.stream>>>(f, c, P);
     kMagneticCowanEdges<R><<<ceilDiv((s.g.nx + s.g.ny) * s.g.nz, 256), 256, 0, s.stream>>>(f, c, P);
   } else {
@@ -761,7 +771,7 @@ void launchElectric(FieldStateT<R>& s, const FieldPtrsT<R>& f, int mode, R* cons
     P.upperZ = s.upperZ;
   }
   dim3 block(256, 1, 1);
-  dim3 grid(ceilDiv((s.g.nx + 1) * (s.g.ny + 1), 256), static_cast<unsigned>(s.g.nz + 1));
+  dim3 grid(ceilDiv((s.g.nx + 1) * (s.g.ny + 1), 256) * static_cast<unsigned>(s.g.nz + 1), 1, 1);
   if (mode == 0) {
     ScopedKernel sk(s.profiler, s.stream, "electric");
     kElectric<0, R><<<grid, block, 0, s.stream>>>(f, c, P, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, inletX,

@@ -6,16 +6,6 @@
 
 #include "particles.hpp"
 
-// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
-// L2. Chunks next to each other in the list share tile boxes, so block b is
-// given chunk (b mod 8)'s contiguous share of the list instead of chunk b:
-// neighbouring chunks then run on one XCD and re-read its L2. A bijection on
-// [0, n).
-__device__ __forceinline__ int xcdContiguous(int b, int n) {
-  const int q = n >> 3, r = n & 7, x = b & 7;
-  return x * q + (x < r ? x : r) + (b >> 3);
-}
-
 namespace aprl_gpu {
 
 struct StepArgs {

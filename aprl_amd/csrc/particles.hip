@@ -244,6 +244,55 @@ __device__ void gatherBBinomial(const GArgs& G, const double* bs, const TileBox<
   }
 }
 
+// The same sample read straight from B in HBM (through the caches: the
+// particles of a chunk share most of their 4x2x4 stencils), so that the
+// gather stages only E in LDS. The arithmetic and its order are those of
+// the staged form.
+__device__ void gatherBBinomialDirect(const GArgs& G, const double a[3], const double bpt[3], double res[3],
+                                      unsigned& err) {
+  res[0] = res[1] = res[2] = 0.0;
+  const Clip path = clipOrbit(G, a, bpt, err);
+  if (!path.valid) { err |= kErrMagneticOutside; return; }
+  double mid[3];
+  for (int d = 0; d < 3; ++d) { mid[d] = path.b[d]; mid[d] += path.e[d]; mid[d] *= .5; }
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp) {
+    long long cell[3];
+    double frac[3];
+    for (int d = 0; d < 3; ++d) {
+      const double pos = (mid[d] - G.o[d]) / G.h[d] - (comp == d ? 0. : 0.5);
+      cell[d] = static_cast<long long>(floor(pos));
+      frac[d] = pos - cell[d];
+    }
+    double wx[4], wz[4];
+    nodeWeights(frac[0], wx);
+    nodeWeights(frac[2], wz);
+    const double wy[2] = {1. - frac[1], frac[1]};
+    const int n0 = G.nb[comp][0], n1 = G.nb[comp][1], n2 = G.nb[comp][2];
+    const double* bc = G.b[comp];
+    if (cell[0] >= 1 && cell[0] + 2 <= n0 - 1 && cell[1] >= 0 && cell[1] + 1 <= n1 - 1 && cell[2] >= 1 &&
+        cell[2] + 2 <= n2 - 1) {
+      const double* base = bc + (static_cast<size_t>(cell[2] - 1) * n1 + cell[1]) * n0 + (cell[0] - 1);
+      for (int k = 0; k < 4; ++k)
+        for (int j = 0; j < 2; ++j) {
+          const double* r = base + (static_cast<size_t>(k) * n1 + j) * n0;
+          double row = 0.;
+          for (int i = 0; i < 4; ++i) row += wx[i] * r[i];
+          res[comp] += wz[k] * wy[j] * row;
+        }
+      continue;
+    }
+    for (int k = 0; k < 4; ++k)
+      for (int j = 0; j < 2; ++j) {
+        const int kk = clampIndex(cell[2] - 1 + k, n2), jj = clampIndex(cell[1] + j, n1);
+        const double* r = bc + (static_cast<size_t>(kk) * n1 + jj) * n0;
+        double row = 0.;
+        for (int i = 0; i < 4; ++i) row += wx[i] * r[clampIndex(cell[0] - 1 + i, n0)];
+        res[comp] += wz[k] * wy[j] * row;
+      }
+  }
+}
+
 // Impl::magnetic for the quintic shape: magneticSample at the clipped chord
 // midpoint with clamped indices (source_coupling.cpp:83-86;
 // source_coupling_math.h:122-138), read from HBM.
@@ -412,10 +461,10 @@ __global__ void kFillInt(int* p, int n, int v) {
 // (source_coupling.cpp:148-202) at (x0, x1): next fields, and the chunk's
 // force max and defect / scale / call sums in a fixed tree order.
 template <int MODE, int SHAPE>
-__global__ void __launch_bounds__(2 * kChunk) kGatherChunks(PArgs P, GArgs G, StepArgs S, const int4* chunks, int ntx,
+__global__ void __launch_bounds__(2 * kChunk) __attribute__((amdgpu_waves_per_eu(SHAPE == kShapeBinomial3X ? 3 : 1))) kGatherChunks(PArgs P, GArgs G, StepArgs S, const int4* chunks, int ntx,
                                                             int nty, double dt, double* chunkOut, unsigned* errOut) {
   if (G.stop && *G.stop) return;
-  constexpr bool kStageB = SHAPE == kShapeBinomial3X;
+  constexpr bool kStageB = false;
   __shared__ double es[3 * boxNodes<SHAPE>()];
   __shared__ double bs[kStageB ? 3 * boxNodes<SHAPE>() : 4 * kChunk];
   const int cb = xcdContiguous(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
@@ -473,8 +522,12 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks(PArgs P, GArgs G, St
       for (int d = 0; d < 3; ++d) b[d] = uv[d];
     }
     if (bWaves) {
-      if constexpr (SHAPE == kShapeBinomial3X) gatherBBinomial(G, bs, box, a, b, g, err);
-      else gatherBQuintic(G, a, b, g, err);
+      if constexpr (SHAPE == kShapeBinomial3X) {
+        if constexpr (kStageB) gatherBBinomial(G, bs, box, a, b, g, err);
+        else gatherBBinomialDirect(G, a, b, g, err);
+      } else {
+        gatherBQuintic(G, a, b, g, err);
+      }
     } else {
       gatherE<SHAPE>(G, es, box, a, b, g, err);
     }
