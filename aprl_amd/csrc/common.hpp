@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -44,6 +45,27 @@ void setLastError(const std::string& message);
 
 inline void hipCheck(hipError_t e, const char* what) {
   if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Workgroups of `kernel` (threads per block, no dynamic LDS) that are
+// resident on the current device at once: compute units x the occupancy
+// limit, queried once per kernel and device.
+inline int residentBlocks(const void* kernel, int threads) {
+  int dev = 0;
+  hipCheck(hipGetDevice(&dev), "hipGetDevice");
+  static std::mutex lock;
+  static std::map<std::pair<const void*, int>, int> cache;
+  std::lock_guard<std::mutex> guard(lock);
+  const auto key = std::make_pair(kernel, dev);
+  const auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int cus = 0, perCu = 0;
+  hipCheck(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+  hipCheck(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, threads, 0),
+           "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+  const int r = cus * perCu > 0 ? cus * perCu : 1;
+  cache[key] = r;
+  return r;
 }
 
 // Run f and map the reference's exception types onto aprl_status.

@@ -497,11 +497,20 @@ __device__ __forceinline__ const R* planeBaseE(const FieldPtrsT<R>& F, int k, in
 // k is computed and stored into the ring slot of plane k-1 afterwards, so
 // every E plane of the run is read from memory once and its load latency
 // overlaps the smoothing of the plane before. Each smoothed sample is the
-// same nine products in the same order as smoothedElectric().
+// same nine products in the same order as smoothedElectric(), and each is
+// computed once: a node's curl takes the difference of two samples at
+// neighbouring nodes, so the x-z and y-z smoothed samples of the plane are
+// formed once per node (plus one halo row or column) into LDS and the curl
+// reads both from there, and the x-y smoothed sample of plane k+1 is kept
+// in registers for the next plane. 6 samples per node instead of 12.
 constexpr int kMZ = 8;
+constexpr int kSX = kMX + 1, kSY = kMY + 1;  // sample grid: one halo column / row
 template <class R>
-__global__ void __launch_bounds__(kMX * kMY) kMagneticCowanMarch(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P) {
+__global__ void __launch_bounds__(kMX * kMY) kMagneticCowanMarch(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P,
+                                                                 int runZ) {
   __shared__ R buf[kStageN];  // [component][ring slot][kLY][kLX]
+  // smoothed samples of the plane: [S_y Ez, S_y Ex, S_x Ez, S_x Ey][kSY][kSX]
+  __shared__ double smp[4 * kSY * kSX];
   const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
   // tiles x fastest, then y, then the z runs; an XCD takes a contiguous
   // share, so the halo rows and the runs' boundary planes its neighbours
@@ -509,7 +518,7 @@ __global__ void __launch_bounds__(kMX * kMY) kMagneticCowanMarch(FieldPtrsT<R> F
   const int gx = (nx + kMX - 1) / kMX, gy = (ny + kMY - 1) / kMY;
   const int w = xcdContiguous(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
   const int i0 = (w % gx) * kMX, j0 = ((w / gx) % gy) * kMY;
-  const int kb = (w / (gx * gy)) * kMZ, ke = min(kb + kMZ, nz + 1);
+  const int kb = (w / (gx * gy)) * runZ, ke = min(kb + runZ, nz + 1);
   const int tx = static_cast<int>(threadIdx.x) % kMX, ty = static_cast<int>(threadIdx.x) / kMX;
   const int i = i0 + tx, j = j0 + ty;
   // The region node (gi, gj) of a lane is the same on every plane: its
@@ -562,31 +571,59 @@ __global__ void __launch_bounds__(kMX * kMY) kMagneticCowanMarch(FieldPtrsT<R> F
   __syncthreads();
   const int li = tx + 1, lj = ty + 1;
   const bool owner = i < nx && j < ny;
+  double* const syEz = smp;
+  double* const syEx = smp + kSY * kSX;
+  double* const sxEz = smp + 2 * kSY * kSX;
+  double* const sxEy = smp + 3 * kSY * kSX;
+  const int at = ty * kSX + tx;
+  // the x-y smoothed Ey, Ex of plane k + 1, kept for plane k + 1
+  double zEy = 0.0, zEx = 0.0;
+  bool zKept = false;
   for (int k = kb; k < ke; ++k) {
     R next[kRegionR][3];
     int zn[3] = {0, 0, 0};
     const bool more = k + 1 < ke;  // plane k + 2 feeds plane k + 1 of this run
     if (more) load(k + 2, next, zn);
+    const R* ex[3] = {buf + (0 * 3 + slotOf(k - 1)) * kLP, buf + (0 * 3 + slotOf(k)) * kLP,
+                      buf + (0 * 3 + slotOf(k + 1)) * kLP};
+    const R* ey[3] = {buf + (1 * 3 + slotOf(k - 1)) * kLP, buf + (1 * 3 + slotOf(k)) * kLP,
+                      buf + (1 * 3 + slotOf(k + 1)) * kLP};
+    const R* ez[3] = {buf + (2 * 3 + slotOf(k - 1)) * kLP, buf + (2 * 3 + slotOf(k)) * kLP,
+                      buf + (2 * 3 + slotOf(k + 1)) * kLP};
+    // Every lane's node of the tile, then the halo row (S_y samples at
+    // j0 + kMY) and the halo column (S_x samples at i0 + kMX); all read the
+    // staged region only.
+    syEz[at] = smoothL<1>(ez, c, li, lj);
+    syEx[at] = smoothL<1>(ex, c, li, lj);
+    sxEz[at] = smoothL<0>(ez, c, li, lj);
+    sxEy[at] = smoothL<0>(ey, c, li, lj);
+    if (threadIdx.x < kMX) {
+      const int h = kMY * kSX + tx;
+      syEz[h] = smoothL<1>(ez, c, li, kMY + 1);
+      syEx[h] = smoothL<1>(ex, c, li, kMY + 1);
+    } else if (threadIdx.x < kMX + kMY) {
+      const int r = static_cast<int>(threadIdx.x) - kMX;
+      const int h = r * kSX + kMX;
+      sxEz[h] = smoothL<0>(ez, c, kMX + 1, r + 1);
+      sxEy[h] = smoothL<0>(ey, c, kMX + 1, r + 1);
+    }
+    __syncthreads();
     if (owner) {
-      const R* ex[3] = {buf + (0 * 3 + slotOf(k - 1)) * kLP, buf + (0 * 3 + slotOf(k)) * kLP,
-                        buf + (0 * 3 + slotOf(k + 1)) * kLP};
-      const R* ey[3] = {buf + (1 * 3 + slotOf(k - 1)) * kLP, buf + (1 * 3 + slotOf(k)) * kLP,
-                        buf + (1 * 3 + slotOf(k + 1)) * kLP};
-      const R* ez[3] = {buf + (2 * 3 + slotOf(k - 1)) * kLP, buf + (2 * 3 + slotOf(k)) * kLP,
-                        buf + (2 * 3 + slotOf(k + 1)) * kLP};
       if (k < nz) {
         // S_z is x-y smoothing: only its own plane is read
         const R* eyk[3] = {ey[1], ey[1], ey[1]};
         const R* eyk1[3] = {ey[2], ey[2], ey[2]};
         const R* exk[3] = {ex[1], ex[1], ex[1]};
         const R* exk1[3] = {ex[2], ex[2], ex[2]};
-        finishBx(F, c, P, i, j, k, smoothL<1>(ez, c, li, lj + 1) - smoothL<1>(ez, c, li, lj),
-                 smoothL<2>(eyk1, c, li, lj) - smoothL<2>(eyk, c, li, lj));
-        finishBy(F, c, P, i, j, k, smoothL<2>(exk1, c, li, lj) - smoothL<2>(exk, c, li, lj),
-                 smoothL<0>(ez, c, li + 1, lj) - smoothL<0>(ez, c, li, lj));
+        const double ey0 = zKept ? zEy : smoothL<2>(eyk, c, li, lj);
+        const double ex0 = zKept ? zEx : smoothL<2>(exk, c, li, lj);
+        zEy = smoothL<2>(eyk1, c, li, lj);
+        zEx = smoothL<2>(exk1, c, li, lj);
+        zKept = true;
+        finishBx(F, c, P, i, j, k, syEz[at + kSX] - syEz[at], zEy - ey0);
+        finishBy(F, c, P, i, j, k, zEx - ex0, sxEz[at + 1] - sxEz[at]);
       }
-      finishBz(F, c, P, i, j, k, smoothL<0>(ey, c, li + 1, lj) - smoothL<0>(ey, c, li, lj),
-               smoothL<1>(ex, c, li, lj + 1) - smoothL<1>(ex, c, li, lj));
+      finishBz(F, c, P, i, j, k, sxEy[at + 1] - sxEy[at], syEx[at + kSX] - syEx[at]);
     }
     if (more) {
       __syncthreads();  // every lane is done with plane k - 1
@@ -751,8 +788,16 @@ void launchMagnetic(FieldStateT<R>& s, const FieldPtrsT<R>& f) {
   const CpmlArgsT<R> P = s.cpmlArgs();
   ScopedKernel sk(s.profiler, s.stream, "magnetic");
   if (s.g.solver == 1) {
-    dim3 tg(ceilDiv(s.g.nx, kMX) * ceilDiv(s.g.ny, kMY) * ceilDiv(s.g.nz + 1, kMZ), 1, 1);
-    kMagneticCowanMarch<R><<<tg, kMX * kMY, 0, s.stream>>>(f, c, P);
+    // z runs as long as lets every workgroup be resident in one round (a
+    // second partial round would double the sweep's time); at least kMZ
+    // planes per run, whose ends re-read two planes each
+    const int tiles = static_cast<int>(ceilDiv(s.g.nx, kMX) * ceilDiv(s.g.ny, kMY));
+    const int planes = static_cast<int>(s.g.nz + 1);
+    const int slots = residentBlocks(reinterpret_cast<const void*>(&kMagneticCowanMarch<R>), kMX * kMY);
+    const int runs = std::max(1, std::min(planes, slots / tiles));
+    const int runZ = std::max(kMZ, static_cast<int>(ceilDiv(planes, runs)));
+    dim3 tg(static_cast<unsigned>(tiles) * ceilDiv(planes, runZ), 1, 1);
+    kMagneticCowanMarch<R><<<tg, kMX * kMY, 0, s.stream>>>(f, c, P, runZ);
     kMagneticCowanEdges<R><<<ceilDiv((s.g.nx + s.g.ny) * s.g.nz, 256), 256, 0, s.stream>>>(f, c, P);
   } else {
     dim3 grid(ceilDiv((s.g.nx + 1) * (s.g.ny + 1), 256), static_cast<unsigned>(s.g.nz + 1));

@@ -372,6 +372,8 @@ class Stepper:
         # max |rho|/dt) of every step, read before the shift clears J
         self.track_continuity = continuity
         self.continuity = []
+        # start each step's sort beside the field updates (aprl_coupling_prepare_step)
+        self.prepare = True
 
     def step(self, shift: bool = True):
         """One step; shift=False stops after the coupled source step (J of
@@ -380,7 +382,8 @@ class Stepper:
         f, win = self.fields, self.window
         n = self.step_index
         # the sort phase runs on its own stream beside the field updates
-        self.coupling.prepareStep(self.store, self.origin)
+        if self.prepare:
+            self.coupling.prepareStep(self.store, self.origin)
         f.clearCurrent()
         win.beginStep(n, self.offset)
         f.advanceMagnetic()

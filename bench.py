@@ -81,6 +81,7 @@ def run_gpu(args, shape="linear-binomial3", profile=True, precision="fp64"):
     w = workloads.BENDING
     fields, window, coupling, store = workloads.setup_gpu(w, shape, precision)
     s = workloads.Stepper(w, fields, window, coupling, store)
+    s.prepare = not getattr(args, "sort_inline", False)
     for _ in range(args.warmup):
         s.step()
     fields.synchronize()
@@ -94,7 +95,11 @@ def run_gpu(args, shape="linear-binomial3", profile=True, precision="fp64"):
     if not profile:
         return w, elapsed, trials, {}, counts
     # Per-kernel HIP-event durations (on the library's own stream) over a
-    # further profiled window of the same steps.
+    # further profiled window of the same steps, with the sort enqueued in
+    # line (in the timed steps it overlaps the field updates on a second
+    # stream, which would fold the two kernels' sharing of the device into
+    # each one's duration).
+    s.prepare = False
     fields.profile(True)
     prof_steps = max(5, min(args.steps, 20))
     for _ in range(prof_steps):
@@ -449,6 +454,9 @@ def main():
     ap.add_argument("--leg", choices=("all", "fp64", "fp32-split", "production", "batch"), default="all",
                     help="time only this form's binomial3-x leg (for profiling runs), only the 2000-step "
                          "production runs, or only the batched scan")
+    ap.add_argument("--sort-inline", action="store_true",
+                    help="with --leg: enqueue each step's sort in line instead of beside the field updates "
+                         "(for per-kernel profiles without two kernels sharing the device)")
     ap.add_argument("--error-study", action="store_true",
                     help="print the FP32-split vs FP64 errors at 100, 1000 and 10000 steps and exit")
     ap.add_argument("--copy-calibration", action="store_true",

@@ -341,7 +341,9 @@ aprl_status aprl_pending_origin(const aprl_pending* queue, uint64_t step, double
  * origin as finishMovingWindowStep does after the shift. */
 aprl_status aprl_coupling_boundary_counters(aprl_coupling* coupling, uint64_t* counts, double* charges);
 /* One Simulation::solveConservativeSourceStep + commitConservativeElectric
- * (simulation_source_coupling.cpp:16-65) for every entry of the store,
+ * (simulation_source_coupling.cpp:16-65) for every entry of the store (the
+ * Picard convergence test evaluated on the device, see
+ * aprl_coupling_set_trial_batch),
  * after clear_current / begin_step / advance_magnetic / set_inlet. origin =
  * the grid's box origin (m); time = box time at the step start; substeps =
  * particle substeps per field step. Particle segments end at their first
@@ -398,12 +400,15 @@ typedef struct aprl_instance_step {
 /* No counterpart in the reference, which advances one bunch per process:
  * one whole window step of Simulation::solve (simulation.cpp:518-583) --
  * clear J, begin_step, advance_magnetic, set_inlet, the coupled source step,
- * shift -- for each of `count` independent instances with a zero inlet
- * reference, enqueued phase by phase across the instances so that their
- * kernels overlap on the device (every field grid owns a stream). The
+ * shift -- for each of `count` independent instances, with the planes of
+ * each grid's device inlet reference (aprl_window_set_reference) or the
+ * zero reference, enqueued phase by phase across the instances so that
+ * their kernels overlap on the device (every field grid owns a stream). The
  * FP32-split form runs its step in three phases (sort and preview; trials,
- * commit and shift; wait); the FP64 form runs its adaptive Picard loop in
- * the middle phase. Errors are the single-instance calls' errors. */
+ * commit and shift; wait); the FP64 form enqueues its first Picard trials
+ * for every instance before it waits for any instance's device
+ * convergence decision, then commits and shifts each. Errors are the
+ * single-instance calls' errors. */
 aprl_status aprl_window_step_batch(aprl_instance_step* steps, size_t count);
 
 /* ---------------------------------------------------------------------------
