@@ -94,6 +94,13 @@ class InstanceStep(C.Structure):
                 ("report", StepReport)]
 
 
+class InstanceRun(C.Structure):
+    """aprl_instance_run: one instance of a run of batched window steps."""
+    _fields_ = [("state", InstanceStep), ("reference_z", C.c_double), ("queue", C.c_void_p),
+                ("look_ahead", C.c_uint64), ("plane", C.c_void_p), ("iterations", C.c_uint64),
+                ("samples", C.c_uint64), ("entry_steps", C.c_uint64)]
+
+
 class GaussianBeam(C.Structure):
     """aprl_gaussian_beam (generated-gaussian beam, particle_initializer.cpp:60-100)."""
     _fields_ = [("center", C.c_double * 3), ("sigma_position", C.c_double * 3), ("mean_u", C.c_double * 3),
@@ -220,6 +227,7 @@ def lib() -> C.CDLL:
     L.aprl_lab_plane_sample.argtypes = [vp, C.c_double, _dp, C.POINTER(C.c_int)]
     L.aprl_lab_plane_fetch.argtypes = [vp, _sz, _dp, _dp, C.POINTER(_sz)]
     L.aprl_window_step_batch.argtypes = [C.POINTER(InstanceStep), _sz]
+    L.aprl_window_run_batch.argtypes = [C.POINTER(InstanceRun), _sz, _sz]
     L.aprl_inlet_reference_create.argtypes = [C.POINTER(GridGeometry), _dp, _sz, _dp, _dp, C.c_double,
                                               C.POINTER(_sz), C.c_int, C.c_int, C.POINTER(vp),
                                               C.POINTER(InletReferenceReport)]
@@ -300,7 +308,8 @@ EXPORTED = [
     "aprl_pending_admit", "aprl_pending_view", "aprl_pending_state", "aprl_pending_origin",
     "aprl_coupling_create", "aprl_coupling_destroy", "aprl_coupling_set_frame",
     "aprl_coupling_add_magnet", "aprl_coupling_set_particle_boundary", "aprl_coupling_step",
-    "aprl_coupling_set_split_trials", "aprl_coupling_set_trial_batch", "aprl_coupling_prepare_step", "aprl_window_step_batch",
+    "aprl_coupling_set_split_trials", "aprl_coupling_set_trial_batch", "aprl_coupling_prepare_step",
+    "aprl_window_step_batch", "aprl_window_run_batch",
     "aprl_lab_plane_create", "aprl_lab_plane_destroy", "aprl_lab_plane_sample", "aprl_lab_plane_fetch",
     "aprl_fields_initialize_gauss",
     "aprl_inlet_reference_create", "aprl_inlet_reference_destroy", "aprl_inlet_reference_download",

@@ -1168,8 +1168,10 @@ void reserveTileLists(CouplingState& C, hipStream_t st, int ntiles, int n) {
   if (chunkBound > kMaxChunks)
     throw std::invalid_argument("More than " + std::to_string(kMaxChunks) + " particle chunks in one step");
   if (chunkBound > C.chunkCapacity) {
-    C.chunks.resize(std::max<std::size_t>(chunkBound, 1));
-    C.chunkCapacity = chunkBound;
+    // headroom: admissions raise the bound a little in many steps
+    const std::size_t capacity = std::min<std::size_t>(chunkBound + chunkBound / 4 + 64, kMaxChunks);
+    C.chunks.resize(capacity);
+    C.chunkCapacity = capacity;
   }
 }
 
@@ -1187,6 +1189,17 @@ void fillInts(hipStream_t st, int* p, int n, int v) {
 void reduceChunkStats(hipStream_t st, const double* in, int chunks, double* out) {
   kReduceBlocks<<<1, 256, 0, st>>>(in, chunks, out, nullptr, 0u, PicardWords{nullptr, 0u, nullptr, 0u});
   launchCheck("chunk stats");
+}
+
+void reserveStepBuffers(CouplingState& C, std::size_t partialNeed, unsigned nChunks, hipStream_t st) {
+  if (partialNeed > C.partial.size()) {
+    // tile partials must start at zero; the reduction keeps them so
+    const std::size_t n = partialNeed + partialNeed / 2;
+    C.partial.resize(n);
+    hipCheck(hipMemsetAsync(C.partial.get(), 0, n * sizeof(long long), st), "hipMemsetAsync");
+  }
+  const std::size_t blocks = 4 * static_cast<std::size_t>(std::max(1u, nChunks));
+  if (C.reduceBlocks.size() < blocks) C.reduceBlocks.resize(blocks + blocks / 2);
 }
 
 void reserveCouplingWords(CouplingState& C, hipStream_t st) {
@@ -1221,8 +1234,8 @@ __global__ void __launch_bounds__(64) kCounterTotals(const double* blockQ, const
 void reserveBlockCounters(CouplingState& C, int blocks) {
   const std::size_t need = static_cast<std::size_t>(std::max(blocks, 1)) * kCounterSlots;
   if (C.blockQ.size() < need) {
-    C.blockQ.resize(need);
-    C.blockN.resize(need);
+    C.blockQ.resize(need + need / 2);
+    C.blockN.resize(need + need / 2);
   }
 }
 }  // namespace
@@ -1509,13 +1522,7 @@ void plainStepBegin(CouplingState& C, ParticleStore& P, const double origin[3], 
   const bool quintic = C.shape == kShapeQuintic;
   const int boxN = quintic ? boxNodes<kShapeQuintic>() : boxNodes<kShapeBinomial3X>();
   const std::size_t partialNeed = static_cast<std::size_t>(occupiedTiles) * 3 * boxN;
-  if (partialNeed > C.partial.size()) {
-    // tile partials must start at zero; the reduction keeps them so
-    C.partial.resize(partialNeed);
-    hipCheck(hipMemsetAsync(C.partial.get(), 0, partialNeed * sizeof(long long), st), "hipMemsetAsync");
-  }
-  if (C.reduceBlocks.size() < 4 * static_cast<std::size_t>(std::max(1u, nChunks)))
-    C.reduceBlocks.resize(4 * static_cast<std::size_t>(std::max(1u, nChunks)));
+  reserveStepBuffers(C, partialNeed, nChunks, st);
 
   X.S = stepArgsFor(C, F.g, F.hasCpml ? &F.cpml : nullptr, F.window, origin, time, substeps);
   X.A = particleArgs(P);

@@ -337,6 +337,11 @@ void appendParticles(ParticleStore& P, std::size_t k, const double* x3, const do
 // Error words and boundary counters of a coupling (allocated and cleared
 // once).
 void reserveCouplingWords(CouplingState& C, hipStream_t st);
+// The step's tile partials (zeroed) and residual block slots, grown with
+// half again as much headroom: a bunch that spreads over new tiles raises
+// the need a little in many steps, and every reallocation frees device
+// memory, which waits for the device.
+void reserveStepBuffers(CouplingState& C, std::size_t partialNeed, unsigned nChunks, hipStream_t st);
 // Boundary-event commit after the converged trial: entry states and
 // weights, removed-entry count and the counters, skipped on the device when
 // the trial's error word carries no kEvtAny. R is the store's real type.

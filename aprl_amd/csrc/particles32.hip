@@ -1192,12 +1192,7 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
   const bool quintic = C.shape == kShapeQuintic;
   const int boxN = quintic ? boxNodes<kShapeQuintic>() : boxNodes<kShapeBinomial3X>();
   const std::size_t partialNeed = static_cast<std::size_t>(occupiedTiles) * 3 * boxN;
-  if (partialNeed > C.partial.size()) {
-    C.partial.resize(partialNeed);
-    hipCheck(hipMemsetAsync(C.partial.get(), 0, partialNeed * sizeof(long long), st), "hipMemsetAsync");
-  }
-  if (C.reduceBlocks.size() < 4 * static_cast<std::size_t>(std::max(1u, nChunks)))
-    C.reduceBlocks.resize(4 * static_cast<std::size_t>(std::max(1u, nChunks)));
+  reserveStepBuffers(C, partialNeed, nChunks, st);
 
   const StepArgs& S = X.S;
   PArgs32 A = particleArgs32(P, F);

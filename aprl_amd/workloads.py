@@ -540,14 +540,16 @@ class BatchStepper:
     grid, window, coupling and store) with aprl_window_step_batch: the
     per-instance bookkeeping of Stepper, one C-ABI call per step."""
 
-    def __init__(self, instances, offset: int = 10_000):
+    def __init__(self, instances):
         import ctypes as C
         from . import _capi
         self.instances = instances  # (workload, fields, window, coupling, store)
         self.steps = (_capi.InstanceStep * len(instances))()
         for s, (w, f, win, cg, st) in zip(self.steps, instances):
             s.fields, s.coupling, s.particles = f.handle, cg.handle, st.handle
-            s.step, s.offset, s.time, s.substeps = 0, offset, 0.0, 1
+            # the window's offset counts from 0 at the start, with the
+            # reference origin at 0: origin z = fma(offset, dz, 0)
+            s.step, s.offset, s.time, s.substeps = 0, 0, 0.0, 1
             for a in range(3):
                 s.origin[a] = 0.0
         self._C = C
@@ -561,8 +563,24 @@ class BatchStepper:
             s.step += 1
             s.offset -= 1
             # Simulation::finishMovingWindowStep: origin = fma(offset, dz, ref)
-            s.origin[2] = (s.offset - 10_000) * w.spacing[2]
+            s.origin[2] = s.offset * w.spacing[2]
             s.time += w.dt
+
+    def run(self, steps: int):
+        """`steps` steps with one aprl_window_run_batch call (the library
+        advances step, offset, origin and time; an instance's next step is
+        enqueued as soon as its own is committed). Returns the Picard trials
+        of each instance summed over the steps."""
+        from . import _capi
+        from ._capi import check, lib
+        runs = (_capi.InstanceRun * len(self.instances))()
+        for r, s in zip(runs, self.steps):
+            r.state = s
+            r.reference_z = 0.0
+        check(lib().aprl_window_run_batch(runs, len(self.instances), int(steps)))
+        for r, s in zip(runs, self.steps):
+            self._C.memmove(self._C.addressof(s), self._C.addressof(r.state), self._C.sizeof(s))
+        return [int(r.iterations) for r in runs]
 
 
 def batch_workloads(count: int, seed: int = 4242):

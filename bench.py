@@ -117,8 +117,9 @@ def run_gpu(args, shape="linear-binomial3", profile=True, precision="fp64"):
 
 def run_batch(count=48, steps=20, warmup=2):
     """The batched scan: `count` independent instances (workloads.
-    batch_workloads) advanced together by aprl_window_step_batch, in both
-    forms from the same generated bunches; whole-batch particle-steps/s."""
+    batch_workloads) advanced together by one aprl_window_run_batch call per
+    run of steps, in both forms from the same generated bunches;
+    whole-batch particle-steps/s."""
     from aprl_amd import workloads
     ws = workloads.batch_workloads(count)
     data = [w.generate() for w in ws]
@@ -126,13 +127,11 @@ def run_batch(count=48, steps=20, warmup=2):
     for precision in ("fp32-split", "fp64"):
         inst = [(w, *workloads.setup_gpu(w, "linear-binomial3", precision, d)) for w, d in zip(ws, data)]
         bs = workloads.BatchStepper(inst)
-        for _ in range(warmup):
-            bs.step()
+        bs.run(warmup)
         for _, f, *_r in inst:
             f.synchronize()
         t0 = time.perf_counter()
-        for _ in range(steps):
-            bs.step()
+        bs.run(steps)
         for _, f, *_r in inst:
             f.synchronize()
         elapsed = time.perf_counter() - t0

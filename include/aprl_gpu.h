@@ -410,6 +410,27 @@ typedef struct aprl_instance_step {
  * convergence decision, then commits and shifts each. Errors are the
  * single-instance calls' errors. */
 aprl_status aprl_window_step_batch(aprl_instance_step* steps, size_t count);
+/* No counterpart in the reference: `steps` consecutive window steps of
+ * every instance, each as aprl_window_step_batch, then the driver's end of
+ * step done in the library: step + 1, offset - 1, time + dt, and the box
+ * origin's z from the queue's inlet (aprl_pending_origin) or, without a
+ * queue, fma(offset, dz, reference_z) (Simulation::finishMovingWindowStep,
+ * simulation_moving_window.cpp:351); before each step the queue admits its
+ * due particles (aprl_pending_admit with look_ahead) and after it the lab
+ * plane is sampled (aprl_lab_plane_sample; its ring must hold the samples
+ * of the call). An instance's next step is enqueued as soon as its own
+ * step is committed, so instances do not wait for each other between
+ * steps. state is advanced in place; iterations, samples and entry_steps
+ * (store entries summed over the steps) accumulate. */
+typedef struct aprl_instance_run {
+  aprl_instance_step state;
+  double reference_z;
+  aprl_pending* queue;          /* optional */
+  uint64_t look_ahead;
+  struct aprl_lab_plane* plane; /* optional */
+  uint64_t iterations, samples, entry_steps;
+} aprl_instance_run;
+aprl_status aprl_window_run_batch(aprl_instance_run* runs, size_t count, size_t steps);
 
 /* ---------------------------------------------------------------------------
  * FP64 diagnostics (compensated, fixed-order device reductions).
