@@ -50,8 +50,11 @@ aprl_status aprl_pending_admit(aprl_pending* queue, uint64_t step, uint64_t look
                                size_t* admitted) {
   return guarded([&] {
     if (!queue || !particles || !admitted) throw std::invalid_argument("Null pending queue, store or output");
-    hipCheck(hipStreamSynchronize(particles->stream()), "hipStreamSynchronize");
     *admitted = queue->q.take(step, look_ahead, [&](const std::vector<PendingParticle>& due) {
+      if (due.empty()) return;
+      // the store may grow (a reallocation copies what the last step's
+      // kernels may still write): wait for them, only when entries come in
+      hipCheck(hipStreamSynchronize(particles->stream()), "hipStreamSynchronize");
       const std::size_t k = due.size();
       std::vector<double> x(3 * k), u(3 * k), q(k), m(k), w(k), beta(k);
       std::vector<uint64_t> id(k);

@@ -302,6 +302,10 @@ __global__ void kTileKeys32(const int* cx, const int* cy, const int* cz, const f
   const int y = clampIndex32(cy[p] - G.oc[1], G.ny);
   const int zc = clampIndex32(cz[p] - G.oc[2], G.nz);
   key[p] = (static_cast<unsigned>(zc / kTileZ) * nty + y / kTileY) * ntx + x / kTileX;
+  // live active entries, one atomic per wavefront
+  const unsigned long long act = __ballot(state[p] == kStateActive);
+  if (act && static_cast<int>(__lane_id()) == __ffsll(static_cast<long long>(act)) - 1)
+    atomicAdd(K.words + 4, static_cast<unsigned>(__popcll(act)));
 }
 
 struct PermuteArgs32 {
@@ -1078,7 +1082,7 @@ static void sortPhaseSplit(CouplingState& C, SplitStore& P, const double origin[
   sortSplit(C, P, gatherArgs32(C, F, oc), K, ntx, nty, ntiles, C.error.get() + 1, ss);
   hipCheck(hipMemcpyAsync(pin, C.tileScan.get() + ntiles, sizeof(unsigned long long), hipMemcpyDeviceToHost, ss),
            "hipMemcpyAsync");
-  hipCheck(hipMemcpyAsync(pin + 120, C.error.get(), 4 * sizeof(unsigned), hipMemcpyDeviceToHost, ss),
+  hipCheck(hipMemcpyAsync(pin + 120, C.error.get(), 5 * sizeof(unsigned), hipMemcpyDeviceToHost, ss),
            "hipMemcpyAsync");
   hipCheck(hipEventRecord(C.sorted.get(), ss), "hipEventRecord");
 }
@@ -1165,7 +1169,7 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
   int n = X.n;
   const int ntx = X.ntx, nty = X.nty, ntz = X.ntz, ntiles = X.ntiles;
   const std::size_t chunkBound = X.chunkBound;
-  const unsigned trials = X.trials;
+  unsigned trials = X.trials;
   const GArgs32& G = X.G;
   const double* o = X.o;
   hipStream_t st = F.stream;
@@ -1187,7 +1191,13 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
     n -= behind;  // removed and cropped entries now trail the live ones
     P.count = static_cast<std::size_t>(n);
     X.n = n;
+    // with no active entry (held ones and carriers only) no particle makes a
+    // coupling call, and the reference converges in one trial
+    // (simulation_source_coupling.cpp:50)
+    if (!pin[124]) trials = 1;
     if (nChunks > chunkBound) throw std::logic_error("Chunk list exceeds its bound");
+  } else {
+    trials = 1;
   }
   const bool quintic = C.shape == kShapeQuintic;
   const int boxN = quintic ? boxNodes<kShapeQuintic>() : boxNodes<kShapeBinomial3X>();
@@ -1287,6 +1297,7 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
     hipCheck(hipMemcpyAsync(pinD, C.reduceOut.get(), 4 * sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
   hipCheck(hipEventRecord(C.stepDone.get(), st), "hipEventRecord");
   X.verify = verify;
+  X.trials = trials;
   X.nChunks = nChunks;
   X.occupied = occupiedTiles;
   X.active = nActive;

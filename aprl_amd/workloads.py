@@ -583,6 +583,60 @@ class BatchStepper:
         return [int(r.iterations) for r in runs]
 
 
+def batch_injection(w: Workload) -> InjectionSetup:
+    """The bending card's planes for a batched-scan instance: events 1 mm
+    upstream of its dipole, the window's lower face 16 cells below the box
+    origin, the lab exit plane 72 cells below the upper face and the lab
+    field plane at 40% of the window at box time zero."""
+    nz = w.dims[2]
+    return InjectionSetup(input_plane_z=-0.101, initial_center_z=-0.1009, entrance=-0.1, window_bottom_cells=-16.0,
+                          exit_box_cells=float(nz - 72), field_plane_box_cells=0.4 * nz)
+
+
+class BatchProduction:
+    """Several production runs (ProductionRun, each its own card) advanced
+    together by aprl_window_run_batch: in every step each instance admits
+    its due particles, takes the window step and samples its lab field
+    plane, and an instance's next step is enqueued as soon as its own is
+    committed. Runs are cut at the plane rings' capacity, whose samples are
+    then fetched (the fetch waits for the device)."""
+
+    def __init__(self, runs):
+        from . import _capi
+        self.runs = runs
+        self.c = (_capi.InstanceRun * len(runs))()
+        for r, run in zip(self.c, runs):
+            s = r.state
+            s.fields, s.coupling, s.particles = run.fields.handle, run.coupling.handle, run.store.handle
+            s.step, s.offset, s.time, s.substeps = run.step_index, run.offset, run.time, run.w.substeps
+            for a in range(3):
+                s.origin[a] = run.origin[a]
+            r.queue = run.queue.handle
+            r.look_ahead = 8
+            r.plane = run.plane.handle
+
+    def run(self, steps: int):
+        from ._capi import check, lib
+        done = 0
+        cap = min(run.plane.capacity for run in self.runs)
+        while done < steps:
+            k = min(steps - done, cap)
+            check(lib().aprl_window_run_batch(self.c, len(self.runs), k))
+            for r, run in zip(self.c, self.runs):
+                run.plane_samples = int(r.samples)
+                run._fetch()
+            done += k
+        for r, run in zip(self.c, self.runs):
+            s = r.state
+            run.step_index, run.offset, run.time = int(s.step), int(s.offset), float(s.time)
+            run.origin[:] = [s.origin[0], s.origin[1], s.origin[2]]
+            run.particle_steps = int(r.entry_steps)
+
+    @property
+    def iterations(self):
+        return [int(r.iterations) for r in self.c]
+
+
 def batch_workloads(count: int, seed: int = 4242):
     """Independent instances of the batched scan: particle counts
     log-uniform in [2e4, 1e6], grids drawn from 32x32x256, 48x48x384 and

@@ -143,6 +143,47 @@ def run_batch(count=48, steps=20, warmup=2):
     return out
 
 
+def run_batch_production(count=48, steps=1000, capacity=100):
+    """The batched scan as production runs: the same `count` instances
+    (workloads.batch_workloads), each from plane-arrival events through the
+    input route and its pending queue (workloads.batch_injection: the
+    bending card's planes on its own grid), advanced together for `steps`
+    steps by aprl_window_run_batch (queue admission, window step, lab field
+    plane per step), in both forms. particle-steps count the store's
+    entries at every step."""
+    import numpy as np
+    from aprl_amd import workloads
+    ws = workloads.batch_workloads(count)
+    out = {"workload": f"batched scan of {count} production runs (grids 32x32x256 to 64x64x512, "
+                       "2e4-1e6 e-, dipole 0.1-1 T, gamma 100-1000)",
+           "instances": count, "particles": int(sum(w.particles for w in ws)), "steps": steps}
+    for precision in ("fp32-split", "fp64"):
+        runs = [workloads.ProductionRun(w, workloads.batch_injection(w), "linear-binomial3", precision,
+                                        max_steps=steps, plane_capacity=capacity) for w in ws]
+        bp = workloads.BatchProduction(runs)
+        for run in runs:
+            run.fields.synchronize()
+        t0 = time.perf_counter()
+        bp.run(steps)
+        for run in runs:
+            run.fields.synchronize()
+        elapsed = time.perf_counter() - t0
+        ps = sum(run.particle_steps for run in runs)
+        exits = sum(int(run.coupling.boundaryCounters()[0][18]) for run in runs)
+        admitted = sum(int(run.queue.admittedCount()) for run in runs)
+        trials = float(np.mean(bp.iterations)) / steps
+        out[precision.replace("-", "_")] = {"ms_per_step": 1e3 * elapsed / steps, "seconds": round(elapsed, 3),
+                                            "particle_steps": int(ps), "value": ps / elapsed,
+                                            "unit": "particle-steps/s", "admitted": admitted,
+                                            "window_exits": exits, "picard_trials_mean": round(trials, 3),
+                                            "lab_plane_samples": int(sum(run.plane_samples for run in runs))}
+        print(f"batched production {precision}: {1e3 * elapsed / steps:.2f} ms per step", file=sys.stderr,
+              flush=True)
+        del bp, runs
+    out["fp32_split_vs_fp64"] = round(out["fp32_split"]["value"] / out["fp64"]["value"], 3)
+    return out
+
+
 def split_error(steps=100, shape="linear-binomial3"):
     """Relative L2 error of the FP32-split form against the plain FP64 form
     (itself within 1e-12 of the oracle per step) after `steps` steps of the
@@ -450,9 +491,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-large", action="store_true", help="skip the FEL segment and capacity legs")
-    ap.add_argument("--leg", choices=("all", "fp64", "fp32-split", "production", "batch"), default="all",
+    ap.add_argument("--leg", choices=("all", "fp64", "fp32-split", "production", "batch", "batch-production"),
+                    default="all",
                     help="time only this form's binomial3-x leg (for profiling runs), only the 2000-step "
-                         "production runs, or only the batched scan")
+                         "production runs, only the batched scan, or only its 1000-step production form")
     ap.add_argument("--sort-inline", action="store_true",
                     help="with --leg: enqueue each step's sort in line instead of beside the field updates "
                          "(for per-kernel profiles without two kernels sharing the device)")
@@ -485,6 +527,9 @@ def main():
     if args.leg == "batch":
         print(json.dumps({"batch": run_batch()}))
         return
+    if args.leg == "batch-production":
+        print(json.dumps({"batch_production": run_batch_production()}))
+        return
     if args.leg != "all":
         w, elapsed, trials, _, (tiles, chunks, active) = run_gpu(args, profile=False, precision=args.leg)
         print(json.dumps({"leg": args.leg, "value": w.particles * args.steps / elapsed, "unit": "particle-steps/s",
@@ -499,6 +544,7 @@ def main():
     _, sq_elapsed, _, _, _ = run_gpu(args, "quintic-compensated", profile=False, precision="fp32-split")
     s_error = split_error()
     batch = run_batch()
+    batch_production = run_batch_production()
     production = production_legs()
     large = None
     if not args.no_large:
@@ -548,6 +594,7 @@ def main():
                        "step_roofline": step_roofline(w, 1e3 * s_elapsed / args.steps, word=4),
                        "kernels_us": {k: round(v["avg_us"], 2) for k, v in s_kernels.items()}},
         "batch": batch,
+        "batch_production": batch_production,
         "production": production,
         "large": large,
         "roofline": roofline,

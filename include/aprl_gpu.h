@@ -365,7 +365,9 @@ aprl_status aprl_coupling_prepare_step(aprl_coupling* coupling, aprl_particles* 
  * form): the Picard loop of aprl_coupling_step runs exactly `trials` trials
  * (2..64, default 2, the reference's minimum with active particles) in
  * place of the FP64 convergence test, whose tolerances (force < 1e-13) lie
- * below FP32 resolution. verify = 1 adds one residual gather after the last
+ * below FP32 resolution; a step with no active particle (held entries and
+ * carriers only) runs one, as the reference converges in one trial when no
+ * particle makes a coupling call. verify = 1 adds one residual gather after the last
  * trial so that the step report carries its force and work (otherwise 0).
  * On such a grid the step's origin must lie on the cell lattice. */
 aprl_status aprl_coupling_set_split_trials(aprl_coupling* coupling, unsigned trials, int verify);

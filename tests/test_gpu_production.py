@@ -103,3 +103,41 @@ def test_box_snapshot_fp32_split_error_after_100_steps():
     u = float(np.linalg.norm(u32[o32] - u64[o64]) / np.linalg.norm(u64))
     print(f"box snapshot after 100 steps: E {e:.2e}, B {b:.2e}, u {u:.2e}", flush=True)
     assert e <= 1e-5 and b <= 1e-5 and u <= 1e-5
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32-split"])
+def test_batched_production_runs_equal_single_runs(precision):
+    """aprl_window_run_batch over production instances (queue admission,
+    window step, lab plane sample per step, the library advancing each
+    instance): two batched-scan cards for 80 steps equal the same cards run
+    one by one with ProductionRun.step bitwise (fields, particles, counters,
+    plane records) with the same trial counts."""
+    ws = W.batch_workloads(6)
+    chosen = [w for w in ws if w.dims[2] == 256][:2]
+    assert len(chosen) == 2
+    steps = 80
+    batched = [W.ProductionRun(w, W.batch_injection(w), "linear-binomial3", precision, max_steps=steps,
+                               plane_capacity=32, keep_patch=(0, 16, 0, 16)) for w in chosen]
+    single = [W.ProductionRun(w, W.batch_injection(w), "linear-binomial3", precision, max_steps=steps,
+                              plane_capacity=32, keep_patch=(0, 16, 0, 16)) for w in chosen]
+    bp = W.BatchProduction(batched)
+    bp.run(steps)
+    for i, (a, b) in enumerate(zip(batched, single)):
+        print(f"single production run {i}: {chosen[i].particles} particles", flush=True)
+        for _ in range(steps):
+            b.step()
+        assert bp.iterations[i] == sum(b.trials), (i, bp.iterations[i], sum(b.trials))
+        assert a.step_index == b.step_index and a.offset == b.offset and a.time == b.time
+        assert np.array_equal(a.origin, b.origin)
+        assert a.particle_steps == b.particle_steps and a.plane_samples == b.plane_samples
+        for c in range(9):
+            assert np.array_equal(a.fields.component(c), b.fields.component(c)), (precision, i, c)
+        ca, qa = a.coupling.boundaryCounters()
+        cb, qb = b.coupling.boundaryCounters()
+        assert np.array_equal(ca, cb) and np.array_equal(qa, qb)
+        ta, ra = a.finish()
+        tb, rb = b.finish()
+        assert np.array_equal(ta, tb) and np.array_equal(ra, rb)
+        xa, ua, *_r, ida = a.store.download()
+        xb, ub, *_s, idb = b.store.download()
+        assert np.array_equal(ida, idb) and np.array_equal(xa, xb) and np.array_equal(ua, ub)
