@@ -7,5 +7,6 @@ from .coupling import (ConservativeSourceCoupling, EBGaussFieldInitializer, LabF
                        ParticleStore)
 from .injection import InjectionPlanes, LabPlaneBunch, frozenLabPlaneInjection  # noqa: F401
 from .reference import BlockedCowanReference  # noqa: F401
+from .tuning import autotuneDecisions, setAutotune  # noqa: F401
 
 __all__ = ["EBFieldGrid", "EBGridGeometry", "EBCPMLParameters", "MovingWindowFields", "AprlError", "lib"]

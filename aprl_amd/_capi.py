@@ -228,6 +228,8 @@ def lib() -> C.CDLL:
     L.aprl_lab_plane_fetch.argtypes = [vp, _sz, _dp, _dp, C.POINTER(_sz)]
     L.aprl_window_step_batch.argtypes = [C.POINTER(InstanceStep), _sz]
     L.aprl_window_run_batch.argtypes = [C.POINTER(InstanceRun), _sz, _sz]
+    L.aprl_autotune.argtypes = [C.c_int]
+    L.aprl_autotune_report.argtypes = [C.c_char_p, _sz, C.POINTER(_sz)]
     L.aprl_inlet_reference_create.argtypes = [C.POINTER(GridGeometry), _dp, _sz, _dp, _dp, C.c_double,
                                               C.POINTER(_sz), C.c_int, C.c_int, C.POINTER(vp),
                                               C.POINTER(InletReferenceReport)]
@@ -309,7 +311,7 @@ EXPORTED = [
     "aprl_coupling_create", "aprl_coupling_destroy", "aprl_coupling_set_frame",
     "aprl_coupling_add_magnet", "aprl_coupling_set_particle_boundary", "aprl_coupling_step",
     "aprl_coupling_set_split_trials", "aprl_coupling_set_trial_batch", "aprl_coupling_prepare_step",
-    "aprl_window_step_batch", "aprl_window_run_batch",
+    "aprl_window_step_batch", "aprl_window_run_batch", "aprl_autotune", "aprl_autotune_report",
     "aprl_lab_plane_create", "aprl_lab_plane_destroy", "aprl_lab_plane_sample", "aprl_lab_plane_fetch",
     "aprl_fields_initialize_gauss",
     "aprl_inlet_reference_create", "aprl_inlet_reference_destroy", "aprl_inlet_reference_download",

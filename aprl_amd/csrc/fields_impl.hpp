@@ -9,9 +9,11 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <vector>
 
 #include "fields.hpp"
 #include "tilecounts.hpp"
+#include "tuner.hpp"
 
 namespace aprl_gpu {
 
@@ -795,7 +797,13 @@ void launchMagnetic(FieldStateT<R>& s, const FieldPtrsT<R>& f) {
     const int planes = static_cast<int>(s.g.nz + 1);
     const int slots = residentBlocks(reinterpret_cast<const void*>(&kMagneticCowanMarch<R>), kMX * kMY);
     const int runs = std::max(1, std::min(planes, slots / tiles));
-    const int runZ = std::max(kMZ, static_cast<int>(ceilDiv(planes, runs)));
+    const int oneRound = std::max(kMZ, static_cast<int>(ceilDiv(planes, runs)));
+    // the one-round length, and longer and shorter runs, are tuned per grid
+    std::vector<int> lengths{oneRound};
+    for (int z : {kMZ, 2 * kMZ, 4 * kMZ, 2 * oneRound})
+      if (z <= planes && std::find(lengths.begin(), lengths.end(), z) == lengths.end()) lengths.push_back(z);
+    TuneScope tune(shapeKey("cowan_march", s.g, sizeof(R)), lengths, s.stream);
+    const int runZ = tune.value();
     dim3 tg(static_cast<unsigned>(tiles) * ceilDiv(planes, runZ), 1, 1);
     kMagneticCowanMarch<R><<<tg, kMX * kMY, 0, s.stream>>>(f, c, P, runZ);
     kMagneticCowanEdges<R><<<ceilDiv((s.g.nx + s.g.ny) * s.g.nz, 256), 256, 0, s.stream>>>(f, c, P);
@@ -815,8 +823,11 @@ void launchElectric(FieldStateT<R>& s, const FieldPtrsT<R>& f, int mode, R* cons
     P.lowerZ = s.lowerZ;
     P.upperZ = s.upperZ;
   }
-  dim3 block(256, 1, 1);
-  dim3 grid(ceilDiv((s.g.nx + 1) * (s.g.ny + 1), 256) * static_cast<unsigned>(s.g.nz + 1), 1, 1);
+  // workgroup size tuned per grid (one thread per node either way)
+  TuneScope tune(shapeKey(mode == 0 ? "electric" : "electric_preview_grid", s.g, sizeof(R)), {256, 128, 64},
+                 s.stream);
+  dim3 block(static_cast<unsigned>(tune.value()), 1, 1);
+  dim3 grid(ceilDiv((s.g.nx + 1) * (s.g.ny + 1), block.x) * static_cast<unsigned>(s.g.nz + 1), 1, 1);
   if (mode == 0) {
     ScopedKernel sk(s.profiler, s.stream, "electric");
     kElectric<0, R><<<grid, block, 0, s.stream>>>(f, c, P, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, inletX,

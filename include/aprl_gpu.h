@@ -691,6 +691,17 @@ aprl_status aprl_inlet_reference_plane(aprl_inlet_reference* reference, int comp
  * zero reference. The reference must outlive its use by the window. */
 aprl_status aprl_window_set_reference(aprl_fields* fields, aprl_inlet_reference* reference, double translation_beta);
 
+/* Launch-shape autotuner (no counterpart in the reference): launch shapes
+ * that give bitwise the same results (the Cowan sweep's z-run length, the E
+ * update's workgroup size) are timed per kernel and grid shape on their
+ * first uses and the fastest is kept; with the environment variable
+ * APRL_TUNE_CACHE naming a file, decisions are read from it and appended
+ * to it. enable = 0 launches the default shapes. The report lists the
+ * decisions taken, one "key value" line each (*length = its size; the
+ * buffer receives at most capacity - 1 characters and a terminating 0). */
+aprl_status aprl_autotune(int enable);
+aprl_status aprl_autotune_report(char* buffer, size_t capacity, size_t* length);
+
 /* Per-kernel timing with HIP events on the instance stream. enable=1 starts
  * recording every launch; query returns launches and total milliseconds for
  * a kernel name (0 launches if unknown). */
