@@ -338,23 +338,32 @@ __device__ __forceinline__ int clampCell(double lattice, int n) {
 
 __global__ void kTileKeys(const double* x, const double* y, const double* z, int* state, int n, GArgs G, int ntx,
                           int nty, unsigned* key, unsigned* perm, TileRangeArgs R, SortArgs K) {
+  __shared__ unsigned blockActive;
+  if (threadIdx.x == 0) blockActive = 0u;
+  __syncthreads();
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   clearTileRanges(R, p);
   if (p == 0) *K.removed = 0u;
-  if (p >= n) return;
-  perm[p] = p;
-  if (!liveEntry(K, state, p, z[p])) {
-    key[p] = static_cast<unsigned>(R.ntiles);
-    return;
+  bool active = false;
+  if (p < n) {
+    perm[p] = p;
+    if (!liveEntry(K, state, p, z[p])) {
+      key[p] = static_cast<unsigned>(R.ntiles);
+    } else {
+      const int cx = clampCell((x[p] - G.o[0]) / G.h[0], G.nx);
+      const int cy = clampCell((y[p] - G.o[1]) / G.h[1], G.ny);
+      const int cz = clampCell((z[p] - G.o[2]) / G.h[2], G.nz);
+      key[p] = (static_cast<unsigned>(cz / kTileZ) * nty + cy / kTileY) * ntx + cx / kTileX;
+      active = state[p] == kStateActive;
+    }
   }
-  const int cx = clampCell((x[p] - G.o[0]) / G.h[0], G.nx);
-  const int cy = clampCell((y[p] - G.o[1]) / G.h[1], G.ny);
-  const int cz = clampCell((z[p] - G.o[2]) / G.h[2], G.nz);
-  key[p] = (static_cast<unsigned>(cz / kTileZ) * nty + cy / kTileY) * ntx + cx / kTileX;
-  // live active entries, one atomic per wavefront
-  const unsigned long long act = __ballot(state[p] == kStateActive);
+  // live active entries: wave ballots summed in LDS, one global atomic per
+  // block (a global atomic per wave would serialize on the one word)
+  const unsigned long long act = __ballot(active);
   if (act && static_cast<int>(__lane_id()) == __ffsll(static_cast<long long>(act)) - 1)
-    atomicAdd(K.words + 4, static_cast<unsigned>(__popcll(act)));
+    atomicAdd(&blockActive, static_cast<unsigned>(__popcll(act)));
+  __syncthreads();
+  if (threadIdx.x == 0 && blockActive) atomicAdd(K.words + 4, blockActive);
 }
 
 struct PermuteArgs {
@@ -452,6 +461,84 @@ __global__ void kFillInt(int* p, int n, int v) {
 // ---------------------------------------------------------------------------
 // Gathers, push, deposit, residual.
 
+// One Picard trial's residual: the chunk partials reduced in a fixed tree
+// to (max force, defect sum, scale sum, coupling calls) in out. With a
+// Picard word (W.stop) thread 0 then takes the reference's convergence
+// decision on the device (simulation_source_coupling.cpp:46-51): at least
+// two trials when a particle made a coupling call, force < 1e-13 and
+// work < 1e-8, the same double arithmetic as the host test; a trial whose
+// error word holds a device error also stops the trials. Once the word is
+// set, later trials skip the reduction. With a mailbox, thread 0 posts the
+// sums (doubles at byte 2048), the stop word (word 15) and the trial words
+// firstWord..W.trial (words 16..) to host memory, sequence number in flag
+// word 1, whether or not the trial ran. Called by all 256 threads of one
+// workgroup; sh holds 4 x 256 doubles.
+struct PicardWords {
+  unsigned* stop;
+  unsigned trial;
+  const unsigned* trialWords;  // error words of trials 0..kMaxTrials-1
+  unsigned firstWord;
+};
+
+__device__ void residualTail(const double* blockIn, int blocks, double* out, unsigned* box, unsigned seq,
+                             const PicardWords& W, double* sh) {
+  double* sf = sh;
+  double* sd = sh + 256;
+  double* ss = sh + 512;
+  double* sc = sh + 768;
+  const bool skip = W.stop && *W.stop;
+  if (!skip) {
+    double f = 0., d = 0., s = 0., c = 0.;
+    for (int b = threadIdx.x; b < blocks; b += 256) {
+      f = fmax(f, blockIn[4 * b]);
+      d += blockIn[4 * b + 1];
+      s += blockIn[4 * b + 2];
+      c += blockIn[4 * b + 3];
+    }
+    sf[threadIdx.x] = f; sd[threadIdx.x] = d; ss[threadIdx.x] = s; sc[threadIdx.x] = c;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (threadIdx.x < w) {
+        sf[threadIdx.x] = fmax(sf[threadIdx.x], sf[threadIdx.x + w]);
+        sd[threadIdx.x] += sd[threadIdx.x + w];
+        ss[threadIdx.x] += ss[threadIdx.x + w];
+        sc[threadIdx.x] += sc[threadIdx.x + w];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      out[0] = sf[0]; out[1] = sd[0]; out[2] = ss[0]; out[3] = sc[0];
+      if (W.stop) {
+        const double force = sf[0];
+        const double work = sc[0] == 0. ? 0. : (ss[0] > 0. ? sd[0] / ss[0] : sd[0]);
+        const unsigned iterations = W.trial + 1;
+        if (W.trialWords[W.trial] & ~kEvtAny) *W.stop = iterations | kStopError;
+        else if (iterations >= (sc[0] > 0. ? 2u : 1u) && force < 1e-13 && work < 1e-8) *W.stop = iterations;
+      }
+    }
+  }
+  if (threadIdx.x == 0 && box) {
+    volatile double* vd = reinterpret_cast<volatile double*>(reinterpret_cast<unsigned char*>(box) + 2048);
+    vd[0] = out[0]; vd[1] = out[1]; vd[2] = out[2]; vd[3] = out[3];
+    volatile unsigned* v = box;
+    v[15] = W.stop ? *W.stop : 0u;
+    for (unsigned w = W.firstWord; w <= W.trial; ++w) v[16 + w - W.firstWord] = W.trialWords[w];
+    __threadfence_system();
+    v[1] = seq;
+  }
+}
+
+// The residual gather's own reduction: the last workgroup to finish (a
+// ticket counter, back to 0 after it) reduces every chunk's partial and
+// takes the convergence decision, so no separate launch follows the gather.
+struct ResidualTailArgs {
+  unsigned* counter;  // null: no tail (trial-0 gather)
+  double* out;
+  unsigned* box;
+  unsigned seq;
+  PicardWords W;
+};
+
 // Chunked orbit gathers from LDS-staged tile fields. A workgroup of
 // 2*kChunk lanes takes one chunk: the first kChunk lanes (whole waves)
 // gather E, the other kChunk lanes gather B, one lane per particle each.
@@ -462,11 +549,17 @@ __global__ void kFillInt(int* p, int n, int v) {
 // force max and defect / scale / call sums in a fixed tree order.
 template <int MODE, int SHAPE>
 __global__ void __launch_bounds__(2 * kChunk) __attribute__((amdgpu_waves_per_eu(SHAPE == kShapeBinomial3X ? 3 : 1))) kGatherChunks(PArgs P, GArgs G, StepArgs S, const int4* chunks, int ntx,
-                                                            int nty, double dt, double* chunkOut, unsigned* errOut) {
-  if (G.stop && *G.stop) return;
+                                                            int nty, double dt, double* chunkOut, unsigned* errOut,
+                                                            ResidualTailArgs T) {
   constexpr bool kStageB = false;
   __shared__ double es[3 * boxNodes<SHAPE>()];
   __shared__ double bs[kStageB ? 3 * boxNodes<SHAPE>() : 4 * kChunk];
+  __shared__ unsigned ticket;
+  if (G.stop && *G.stop) {
+    // a trial after convergence: only the mailbox post, if asked for
+    if (MODE == 1 && T.counter && blockIdx.x == 0) residualTail(chunkOut, 0, T.out, T.box, T.seq, T.W, es);
+    return;
+  }
   const int cb = xcdContiguous(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
   const int4 ch = chunks[cb];
   const TileBox<SHAPE> box = tileBox<SHAPE>(ch.x, ntx, nty);
@@ -591,6 +684,19 @@ __global__ void __launch_bounds__(2 * kChunk) __attribute__((amdgpu_waves_per_eu
     }
   }
   if (err) atomicOr(errOut, err);
+  if (MODE == 1 && T.counter) {
+    // every chunk partial and error bit of this workgroup is visible before
+    // its ticket; the last ticket reduces them all in chunk order
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) ticket = atomicAdd(T.counter, 1u);
+    __syncthreads();
+    if (ticket == gridDim.x - 1) {
+      __threadfence();
+      residualTail(chunkOut, static_cast<int>(gridDim.x), T.out, T.box, T.seq, T.W, es);
+      if (threadIdx.x == 0) *T.counter = 0u;
+    }
+  }
 }
 
 
@@ -811,67 +917,10 @@ __global__ void __launch_bounds__(256) kReduceCurrent(double* jx, double* jy, do
   }
 }
 
-// One Picard trial's residual: the chunk partials reduced in a fixed tree
-// to (max force, defect sum, scale sum, coupling calls) in out. With a
-// Picard word (W.stop) thread 0 then takes the reference's convergence
-// decision on the device (simulation_source_coupling.cpp:46-51): at least
-// two trials when a particle made a coupling call, force < 1e-13 and
-// work < 1e-8, the same double arithmetic as the host test; a trial whose
-// error word holds a device error also stops the trials. Once the word is
-// set, later trials skip the reduction. With a mailbox, thread 0 posts the
-// sums (doubles at byte 2048), the stop word (word 15) and the trial words
-// firstWord..W.trial (words 16..) to host memory, sequence number in flag
-// word 1, whether or not the trial ran.
-struct PicardWords {
-  unsigned* stop;
-  unsigned trial;
-  const unsigned* trialWords;  // error words of trials 0..kMaxTrials-1
-  unsigned firstWord;
-};
-
 __global__ void __launch_bounds__(256) kReduceBlocks(const double* blockIn, int blocks, double* out, unsigned* box,
                                                      unsigned seq, PicardWords W) {
-  __shared__ double sf[256], sd[256], ss[256], sc[256];
-  const bool skip = W.stop && *W.stop;
-  if (!skip) {
-    double f = 0., d = 0., s = 0., c = 0.;
-    for (int b = threadIdx.x; b < blocks; b += 256) {
-      f = fmax(f, blockIn[4 * b]);
-      d += blockIn[4 * b + 1];
-      s += blockIn[4 * b + 2];
-      c += blockIn[4 * b + 3];
-    }
-    sf[threadIdx.x] = f; sd[threadIdx.x] = d; ss[threadIdx.x] = s; sc[threadIdx.x] = c;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if (threadIdx.x < w) {
-        sf[threadIdx.x] = fmax(sf[threadIdx.x], sf[threadIdx.x + w]);
-        sd[threadIdx.x] += sd[threadIdx.x + w];
-        ss[threadIdx.x] += ss[threadIdx.x + w];
-        sc[threadIdx.x] += sc[threadIdx.x + w];
-      }
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      out[0] = sf[0]; out[1] = sd[0]; out[2] = ss[0]; out[3] = sc[0];
-      if (W.stop) {
-        const double force = sf[0];
-        const double work = sc[0] == 0. ? 0. : (ss[0] > 0. ? sd[0] / ss[0] : sd[0]);
-        const unsigned iterations = W.trial + 1;
-        if (W.trialWords[W.trial] & ~kEvtAny) *W.stop = iterations | kStopError;
-        else if (iterations >= (sc[0] > 0. ? 2u : 1u) && force < 1e-13 && work < 1e-8) *W.stop = iterations;
-      }
-    }
-  }
-  if (threadIdx.x == 0 && box) {
-    volatile double* vd = reinterpret_cast<volatile double*>(reinterpret_cast<unsigned char*>(box) + 2048);
-    vd[0] = out[0]; vd[1] = out[1]; vd[2] = out[2]; vd[3] = out[3];
-    volatile unsigned* v = box;
-    v[15] = W.stop ? *W.stop : 0u;
-    for (unsigned w = W.firstWord; w <= W.trial; ++w) v[16 + w - W.firstWord] = W.trialWords[w];
-    __threadfence_system();
-    v[1] = seq;
-  }
+  __shared__ double sh[4 * 256];
+  residualTail(blockIn, blocks, out, box, seq, W, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -1205,6 +1254,10 @@ void reserveStepBuffers(CouplingState& C, std::size_t partialNeed, unsigned nChu
 void reserveCouplingWords(CouplingState& C, hipStream_t st) {
   if (C.error.size() < static_cast<std::size_t>(kErrorWords)) C.error.resize(kErrorWords);
   if (C.reduceOut.size() < 4) C.reduceOut.resize(4);
+  if (!C.residualTicket.size()) {
+    C.residualTicket.resize(1);
+    hipCheck(hipMemsetAsync(C.residualTicket.get(), 0, sizeof(unsigned), st), "hipMemsetAsync");
+  }
   if (!C.counterQ.size()) {
     C.counterQ.resize(kCounterSlots);
     C.counterN.resize(kCounterSlots);
@@ -1380,12 +1433,13 @@ void enqueueTrials(CouplingState& C, PlainStepContext& X, unsigned first, unsign
       for (int a = 0; a < 3; ++a) G0.j[a] = nullptr;
       G0.stop = nullptr;
       ScopedKernel sk(F.profiler, st, "gather");
+      const ResidualTailArgs none{};
       if (quintic)
         kGatherChunks<0, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G0, S, C.chunks.get(), ntx, nty, F.g.dt,
-                                                                         nullptr, errT);
+                                                                         nullptr, errT, none);
       else
         kGatherChunks<0, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G0, S, C.chunks.get(), ntx, nty, F.g.dt,
-                                                                            nullptr, errT);
+                                                                            nullptr, errT, none);
       launchCheck("gather");
     }
     if (n) {
@@ -1435,18 +1489,22 @@ void enqueueTrials(CouplingState& C, PlainStepContext& X, unsigned first, unsign
     referenceCurrentTrial(C, F, C.esum[2].get(), C.estar[2].get(), true, nChunks > 0, ntx, nty, ntz, stop);
     const bool post = trial + 1 == last;
     if (post) X.postSeq = C.mailbox.next(1);
-    {
+    const PicardWords W{stop, trial, C.error.get() + kTrialWord, X.checkedTrials};
+    unsigned* const box = post ? C.mailbox.device() : nullptr;
+    const unsigned seq = post ? X.postSeq : 0u;
+    if (nChunks) {
+      // the gather's last workgroup reduces the residual and decides
+      const ResidualTailArgs T{C.residualTicket.get(), C.reduceOut.get(), box, seq, W};
       ScopedKernel sk(F.profiler, st, "residual");
-      if (nChunks && quintic)
+      if (quintic)
         kGatherChunks<1, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G, S, C.chunks.get(), ntx, nty, F.g.dt,
-                                                                         C.reduceBlocks.get(), errT);
-      else if (nChunks)
+                                                                         C.reduceBlocks.get(), errT, T);
+      else
         kGatherChunks<1, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G, S, C.chunks.get(), ntx, nty, F.g.dt,
-                                                                            C.reduceBlocks.get(), errT);
+                                                                            C.reduceBlocks.get(), errT, T);
+    } else {
+      kReduceBlocks<<<1, 256, 0, st>>>(C.reduceBlocks.get(), 0, C.reduceOut.get(), box, seq, W);
     }
-    kReduceBlocks<<<1, 256, 0, st>>>(C.reduceBlocks.get(), static_cast<int>(nChunks), C.reduceOut.get(),
-                                     post ? C.mailbox.device() : nullptr, post ? X.postSeq : 0u,
-                                     PicardWords{stop, trial, C.error.get() + kTrialWord, X.checkedTrials});
     launchCheck("residual");
     // The next trial pushes with the fields this residual gathered.
     for (int a = 0; a < 6; ++a) std::swap(A.fin[a], A.fout[a]);

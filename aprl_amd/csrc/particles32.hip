@@ -288,24 +288,33 @@ __device__ void gatherBQuintic32(const GArgs32& G, Chord32 c, float res[3], unsi
 __global__ void kTileKeys32(const int* cx, const int* cy, const int* cz, const float* oz, int* state, int n,
                             GArgs32 G, int ntx, int nty, unsigned* key, unsigned* perm, TileRangeArgs R, SortArgs K,
                             double hz) {
+  __shared__ unsigned blockActive;
+  if (threadIdx.x == 0) blockActive = 0u;
+  __syncthreads();
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   clearTileRanges(R, p);
   if (p == 0) *K.removed = 0u;
-  if (p >= n) return;
-  perm[p] = p;
-  const double z = (static_cast<double>(cz[p]) + static_cast<double>(oz[p])) * hz;
-  if (!liveEntry(K, state, p, z)) {
-    key[p] = static_cast<unsigned>(R.ntiles);
-    return;
+  bool active = false;
+  if (p < n) {
+    perm[p] = p;
+    const double z = (static_cast<double>(cz[p]) + static_cast<double>(oz[p])) * hz;
+    if (!liveEntry(K, state, p, z)) {
+      key[p] = static_cast<unsigned>(R.ntiles);
+    } else {
+      const int x = clampIndex32(cx[p] - G.oc[0], G.nx);
+      const int y = clampIndex32(cy[p] - G.oc[1], G.ny);
+      const int zc = clampIndex32(cz[p] - G.oc[2], G.nz);
+      key[p] = (static_cast<unsigned>(zc / kTileZ) * nty + y / kTileY) * ntx + x / kTileX;
+      active = state[p] == kStateActive;
+    }
   }
-  const int x = clampIndex32(cx[p] - G.oc[0], G.nx);
-  const int y = clampIndex32(cy[p] - G.oc[1], G.ny);
-  const int zc = clampIndex32(cz[p] - G.oc[2], G.nz);
-  key[p] = (static_cast<unsigned>(zc / kTileZ) * nty + y / kTileY) * ntx + x / kTileX;
-  // live active entries, one atomic per wavefront
-  const unsigned long long act = __ballot(state[p] == kStateActive);
+  // live active entries: wave ballots summed in LDS, one global atomic per
+  // block
+  const unsigned long long act = __ballot(active);
   if (act && static_cast<int>(__lane_id()) == __ffsll(static_cast<long long>(act)) - 1)
-    atomicAdd(K.words + 4, static_cast<unsigned>(__popcll(act)));
+    atomicAdd(&blockActive, static_cast<unsigned>(__popcll(act)));
+  __syncthreads();
+  if (threadIdx.x == 0 && blockActive) atomicAdd(K.words + 4, blockActive);
 }
 
 struct PermuteArgs32 {
