@@ -528,17 +528,6 @@ __device__ void residualTail(const double* blockIn, int blocks, double* out, uns
   }
 }
 
-// The residual gather's own reduction: the last workgroup to finish (a
-// ticket counter, back to 0 after it) reduces every chunk's partial and
-// takes the convergence decision, so no separate launch follows the gather.
-struct ResidualTailArgs {
-  unsigned* counter;  // null: no tail (trial-0 gather)
-  double* out;
-  unsigned* box;
-  unsigned seq;
-  PicardWords W;
-};
-
 // Chunked orbit gathers from LDS-staged tile fields. A workgroup of
 // 2*kChunk lanes takes one chunk: the first kChunk lanes (whole waves)
 // gather E, the other kChunk lanes gather B, one lane per particle each.
@@ -548,18 +537,15 @@ struct ResidualTailArgs {
 // (source_coupling.cpp:148-202) at (x0, x1): next fields, and the chunk's
 // force max and defect / scale / call sums in a fixed tree order.
 template <int MODE, int SHAPE>
-__global__ void __launch_bounds__(2 * kChunk) __attribute__((amdgpu_waves_per_eu(SHAPE == kShapeBinomial3X ? 3 : 1))) kGatherChunks(PArgs P, GArgs G, StepArgs S, const int4* chunks, int ntx,
-                                                            int nty, double dt, double* chunkOut, unsigned* errOut,
-                                                            ResidualTailArgs T) {
+// Binomial3-x stages only E (31 KB of LDS): three waves per SIMD fit once
+// the residual gather keeps its registers within 168.
+__global__ void __launch_bounds__(2 * kChunk) __attribute__((amdgpu_waves_per_eu(SHAPE == kShapeBinomial3X ? 3 : 1)))
+kGatherChunks(PArgs P, GArgs G, StepArgs S, const int4* chunks, int ntx, int nty, double dt, double* chunkOut,
+              unsigned* errOut) {
+  if (G.stop && *G.stop) return;
   constexpr bool kStageB = false;
   __shared__ double es[3 * boxNodes<SHAPE>()];
   __shared__ double bs[kStageB ? 3 * boxNodes<SHAPE>() : 4 * kChunk];
-  __shared__ unsigned ticket;
-  if (G.stop && *G.stop) {
-    // a trial after convergence: only the mailbox post, if asked for
-    if (MODE == 1 && T.counter && blockIdx.x == 0) residualTail(chunkOut, 0, T.out, T.box, T.seq, T.W, es);
-    return;
-  }
   const int cb = xcdContiguous(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
   const int4 ch = chunks[cb];
   const TileBox<SHAPE> box = tileBox<SHAPE>(ch.x, ntx, nty);
@@ -684,19 +670,6 @@ __global__ void __launch_bounds__(2 * kChunk) __attribute__((amdgpu_waves_per_eu
     }
   }
   if (err) atomicOr(errOut, err);
-  if (MODE == 1 && T.counter) {
-    // every chunk partial and error bit of this workgroup is visible before
-    // its ticket; the last ticket reduces them all in chunk order
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) ticket = atomicAdd(T.counter, 1u);
-    __syncthreads();
-    if (ticket == gridDim.x - 1) {
-      __threadfence();
-      residualTail(chunkOut, static_cast<int>(gridDim.x), T.out, T.box, T.seq, T.W, es);
-      if (threadIdx.x == 0) *T.counter = 0u;
-    }
-  }
 }
 
 
@@ -1254,10 +1227,7 @@ void reserveStepBuffers(CouplingState& C, std::size_t partialNeed, unsigned nChu
 void reserveCouplingWords(CouplingState& C, hipStream_t st) {
   if (C.error.size() < static_cast<std::size_t>(kErrorWords)) C.error.resize(kErrorWords);
   if (C.reduceOut.size() < 4) C.reduceOut.resize(4);
-  if (!C.residualTicket.size()) {
-    C.residualTicket.resize(1);
-    hipCheck(hipMemsetAsync(C.residualTicket.get(), 0, sizeof(unsigned), st), "hipMemsetAsync");
-  }
+
   if (!C.counterQ.size()) {
     C.counterQ.resize(kCounterSlots);
     C.counterN.resize(kCounterSlots);
@@ -1433,13 +1403,12 @@ void enqueueTrials(CouplingState& C, PlainStepContext& X, unsigned first, unsign
       for (int a = 0; a < 3; ++a) G0.j[a] = nullptr;
       G0.stop = nullptr;
       ScopedKernel sk(F.profiler, st, "gather");
-      const ResidualTailArgs none{};
       if (quintic)
         kGatherChunks<0, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G0, S, C.chunks.get(), ntx, nty, F.g.dt,
-                                                                         nullptr, errT, none);
+                                                                         nullptr, errT);
       else
         kGatherChunks<0, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G0, S, C.chunks.get(), ntx, nty, F.g.dt,
-                                                                            nullptr, errT, none);
+                                                                            nullptr, errT);
       launchCheck("gather");
     }
     if (n) {
@@ -1493,18 +1462,18 @@ void enqueueTrials(CouplingState& C, PlainStepContext& X, unsigned first, unsign
     unsigned* const box = post ? C.mailbox.device() : nullptr;
     const unsigned seq = post ? X.postSeq : 0u;
     if (nChunks) {
-      // the gather's last workgroup reduces the residual and decides
-      const ResidualTailArgs T{C.residualTicket.get(), C.reduceOut.get(), box, seq, W};
       ScopedKernel sk(F.profiler, st, "residual");
       if (quintic)
         kGatherChunks<1, kShapeQuintic><<<nChunks, 2 * kChunk, 0, st>>>(A, G, S, C.chunks.get(), ntx, nty, F.g.dt,
-                                                                         C.reduceBlocks.get(), errT, T);
+                                                                         C.reduceBlocks.get(), errT);
       else
         kGatherChunks<1, kShapeBinomial3X><<<nChunks, 2 * kChunk, 0, st>>>(A, G, S, C.chunks.get(), ntx, nty, F.g.dt,
-                                                                            C.reduceBlocks.get(), errT, T);
-    } else {
-      kReduceBlocks<<<1, 256, 0, st>>>(C.reduceBlocks.get(), 0, C.reduceOut.get(), box, seq, W);
+                                                                            C.reduceBlocks.get(), errT);
     }
+    // a separate launch: folding it into the gather's last workgroup needs
+    // a device-scope fence in every workgroup (the XCDs' L2s write back),
+    // which was measured to make the gather three times slower
+    kReduceBlocks<<<1, 256, 0, st>>>(C.reduceBlocks.get(), static_cast<int>(nChunks), C.reduceOut.get(), box, seq, W);
     launchCheck("residual");
     // The next trial pushes with the fields this residual gathered.
     for (int a = 0; a < 6; ++a) std::swap(A.fin[a], A.fout[a]);

@@ -189,9 +189,6 @@ struct CouplingState {
   // fixed-point current
   DeviceArray<long long> partial;
   DeviceArray<double> reduceBlocks, reduceOut;
-  // ticket counter of the residual gather's last-workgroup reduction (0
-  // between launches)
-  DeviceArray<unsigned> residualTicket;
   DeviceArray<unsigned> error;
   // cumulative boundary counters and their per-block partials
   DeviceArray<double> counterQ, blockQ;

@@ -546,6 +546,9 @@ def main():
     ap.add_argument("--sort-inline", action="store_true",
                     help="with --leg: enqueue each step's sort in line instead of beside the field updates "
                          "(for per-kernel profiles without two kernels sharing the device)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="launch every kernel's default shape (for profiles whose averages should not include "
+                         "the tuner's trial launches)")
     ap.add_argument("--error-study", action="store_true",
                     help="print the FP32-split vs FP64 errors at 100, 1000 and 10000 steps and exit")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -562,6 +565,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": n * 8, "copies": 3}))
         return
+    if args.no_autotune:
+        import aprl_amd
+        aprl_amd.setAutotune(False)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the whole window lives on one device); --gpus {args.gpus} "
               "is not supported", file=sys.stderr)
