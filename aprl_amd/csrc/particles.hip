@@ -536,9 +536,9 @@ __device__ void residualTail(const double* blockIn, int blocks, double* out, uns
 // the first push uses. MODE 1: the residual gather of localResidual
 // (source_coupling.cpp:148-202) at (x0, x1): next fields, and the chunk's
 // force max and defect / scale / call sums in a fixed tree order.
-template <int MODE, int SHAPE>
 // Binomial3-x stages only E (31 KB of LDS): three waves per SIMD fit once
 // the residual gather keeps its registers within 168.
+template <int MODE, int SHAPE>
 __global__ void __launch_bounds__(2 * kChunk) __attribute__((amdgpu_waves_per_eu(SHAPE == kShapeBinomial3X ? 3 : 1)))
 kGatherChunks(PArgs P, GArgs G, StepArgs S, const int4* chunks, int ntx, int nty, double dt, double* chunkOut,
               unsigned* errOut) {
