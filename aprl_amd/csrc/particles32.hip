@@ -249,6 +249,54 @@ __device__ void gatherBBinomial32(const GArgs32& G, const float* bs, const TileB
   }
 }
 
+// The same sample read straight from B in HBM (the particles of a chunk
+// share most of their 4x2x4 stencils in the caches), so that the gather
+// stages only E; the arithmetic and its order are the staged form's.
+__device__ void gatherBBinomial32Direct(const GArgs32& G, Chord32 c, float res[3], unsigned& err) {
+  res[0] = res[1] = res[2] = 0.f;
+  float kept;
+  if (!clipChord(G, c, kept, err)) { err |= kErrMagneticOutside; return; }
+  float mid[3];
+  for (int d = 0; d < 3; ++d) mid[d] = c.x[d] + 0.5f * c.v[d];
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp) {
+    int cell[3];
+    float frac[3];
+    for (int d = 0; d < 3; ++d) {
+      const float pos = mid[d] - (comp == d ? 0.f : 0.5f);
+      const float fl = floorf(pos);
+      cell[d] = static_cast<int>(fl) + c.org[d];
+      frac[d] = pos - fl;
+    }
+    float wx[4], wz[4];
+    nodeWeights(frac[0], wx);
+    nodeWeights(frac[2], wz);
+    const float wy[2] = {1.f - frac[1], frac[1]};
+    const int n0 = G.nb[comp][0], n1 = G.nb[comp][1], n2 = G.nb[comp][2];
+    const float* bc = G.b[comp];
+    if (cell[0] >= 1 && cell[0] + 2 <= n0 - 1 && cell[1] >= 0 && cell[1] + 1 <= n1 - 1 && cell[2] >= 1 &&
+        cell[2] + 2 <= n2 - 1) {
+      const float* base = bc + (static_cast<size_t>(cell[2] - 1) * n1 + cell[1]) * n0 + (cell[0] - 1);
+      for (int k = 0; k < 4; ++k)
+        for (int j = 0; j < 2; ++j) {
+          const float* r = base + (static_cast<size_t>(k) * n1 + j) * n0;
+          float row = 0.f;
+          for (int i = 0; i < 4; ++i) row += wx[i] * r[i];
+          res[comp] += wz[k] * wy[j] * row;
+        }
+      continue;
+    }
+    for (int k = 0; k < 4; ++k)
+      for (int j = 0; j < 2; ++j) {
+        const int kk = clampIndex32(cell[2] - 1 + k, n2), jj = clampIndex32(cell[1] + j, n1);
+        const float* r = bc + (static_cast<size_t>(kk) * n1 + jj) * n0;
+        float row = 0.f;
+        for (int i = 0; i < 4; ++i) row += wx[i] * r[clampIndex32(cell[0] - 1 + i, n0)];
+        res[comp] += wz[k] * wy[j] * row;
+      }
+  }
+}
+
 // Impl::magnetic for the quintic shape (source_coupling_math.h:122-138):
 // trilinear sample at the clipped chord midpoint, clamped indices, from HBM.
 __device__ void gatherBQuintic32(const GArgs32& G, Chord32 c, float res[3], unsigned& err) {
@@ -387,7 +435,7 @@ template <int MODE, int SHAPE>
 __global__ void __launch_bounds__(2 * kChunk) kGatherChunks32(PArgs32 P, GArgs32 G, StepArgs S, const int4* chunks,
                                                               int ntx, int nty, double dt, double* chunkOut,
                                                               unsigned* errOut) {
-  constexpr bool kStageB = SHAPE == kShapeBinomial3X;
+  constexpr bool kStageB = false;
   __shared__ float es[3 * boxNodes<SHAPE>()];
   __shared__ float bs[kStageB ? 3 * boxNodes<SHAPE>() : 1];
   __shared__ double red[MODE == 1 ? 4 * kChunk : 1];
@@ -435,8 +483,12 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks32(PArgs32 P, GArgs32
     }
     const Chord32 c = chordOf(G, c0[0], c0[1], c0[2], off0, v);
     if (bWaves) {
-      if constexpr (SHAPE == kShapeBinomial3X) gatherBBinomial32(G, bs, box, c, g, err);
-      else gatherBQuintic32(G, c, g, err);
+      if constexpr (SHAPE == kShapeBinomial3X) {
+        if constexpr (kStageB) gatherBBinomial32(G, bs, box, c, g, err);
+        else gatherBBinomial32Direct(G, c, g, err);
+      } else {
+        gatherBQuintic32(G, c, g, err);
+      }
     } else {
       gatherE32<SHAPE>(G, es, box, c, g, err);
     }
