@@ -378,26 +378,34 @@ struct PermuteArgs {
 };
 
 // State scatter into sorted order, and the tile ranges of the sorted keys.
+// The gathered reads of a permute are latency-bound: blockIdx.y splits the
+// arrays over four times as many waves (y = 0: positions, tile ranges and
+// bounds; 1: momenta; 2: q, m, w; 3: ids, states, held speeds).
 __global__ void kPermute(PermuteArgs A, const unsigned* perm, int n, TileBoundsArgs T) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int part = static_cast<int>(blockIdx.y);
   unsigned tile = ~0u;
   double bound = 0.0;
   if (p < n) {
     const unsigned from = perm[p];
+    if (part < 3) {
 #pragma unroll
-    for (int a = 0; a < 9; ++a) A.dst[a][p] = A.src[a][from];
-    A.idDst[p] = A.idSrc[from];
-    const int state = A.stateSrc[from];
-    A.stateDst[p] = state;
-    A.betaDst[p] = A.betaSrc[from];
-    tileBoundsAt(T, p);
-    const unsigned k = T.key[p];
-    if (k < static_cast<unsigned>(T.ntiles)) {
-      tile = k;
-      bound = entryCurrentBound(A.src[6][from], state, T.perCharge);
+      for (int a = 0; a < 3; ++a) A.dst[3 * part + a][p] = A.src[3 * part + a][from];
+    } else {
+      A.idDst[p] = A.idSrc[from];
+      A.stateDst[p] = A.stateSrc[from];
+      A.betaDst[p] = A.betaSrc[from];
+    }
+    if (part == 0) {
+      tileBoundsAt(T, p);
+      const unsigned k = T.key[p];
+      if (k < static_cast<unsigned>(T.ntiles)) {
+        tile = k;
+        bound = entryCurrentBound(A.src[6][from], A.stateSrc[from], T.perCharge);
+      }
     }
   }
-  raiseTileBound(T.bound, tile, bound);
+  if (part == 0) raiseTileBound(T.bound, tile, bound);
 }
 
 // Per tile, packed (packTileCounts): chunk count, whether it is occupied,
@@ -429,9 +437,9 @@ __global__ void kTileCounts(const int* begin, const int* end, int ntx, int nty, 
 
 // Chunk list in tile order and the list of active tiles (tiles with an
 // occupied neighbour), both from the scanned counts.
-// With a mailbox, lane 0 also posts the scan total (chunks, active tiles)
-// and the four sort-phase words to host memory (values at words 8..13,
-// sequence number in flag word 0).
+// With a mailbox, lane 0 also posts the scan total (chunks, active tiles,
+// occupied tiles) and the sort-phase words to host memory (values at words
+// 8..14, sequence number in flag word 0).
 __global__ void kChunkList(const int* begin, const int* end, const unsigned long long* counts,
                            const unsigned long long* scan, int ntiles, int4* chunks, int* active, unsigned* box,
                            unsigned seq, const unsigned* words) {
@@ -442,6 +450,7 @@ __global__ void kChunkList(const int* begin, const int* end, const unsigned long
     v[8] = static_cast<unsigned>(chunkField(total));
     v[9] = static_cast<unsigned>(activeField(total));
     for (int w = 0; w < 5; ++w) v[10 + w] = words[w];
+    v[11] = static_cast<unsigned>(occupiedField(total));  // in place of word 1
     __threadfence_system();
     v[0] = seq;
   }
@@ -1108,7 +1117,7 @@ static int bitsFor(unsigned v) {
 // entries last), then the tile ranges and the chunk list (tile order,
 // kChunk particles per chunk) in device memory.
 static void sortAndChunk(CouplingState& C, ParticleStore& P, const GArgs& G, const SortArgs& K, int ntx, int nty,
-                         int ntiles, unsigned* occupied, unsigned seq, hipStream_t st) {
+                         int ntiles, unsigned seq, hipStream_t st) {
   FieldState& F = *C.fields;
   const int n = static_cast<int>(P.count);
   ScopedKernel sk(F.profiler, st, "sort");
@@ -1139,9 +1148,9 @@ static void sortAndChunk(CouplingState& C, ParticleStore& P, const GArgs& G, con
   A.betaDst = P.betaT.get();
   // |scale| = |q| |dx_a| / (dx dy dz dt) < |q| c dt / (dx dy dz dt), with a margin for rounding
   const double perCharge = kC * F.g.dt / (F.g.dx * F.g.dy * F.g.dz * F.g.dt) * (1.0 + 1e-9);
-  const TileBoundsArgs T{P.keySorted.get(), n,       ntiles, C.tileBegin.get(), C.tileEnd.get(), occupied,
-                         C.tileBound.get(),  perCharge};
-  kPermute<<<blocks, 256, 0, st>>>(A, P.permSorted.get(), n, T);
+  const TileBoundsArgs T{P.keySorted.get(), n, ntiles, C.tileBegin.get(), C.tileEnd.get(), C.tileBound.get(),
+                         perCharge};
+  kPermute<<<dim3(blocks, 4), 256, 0, st>>>(A, P.permSorted.get(), n, T);
   launchCheck("permute");
   for (int a = 0; a < 9; ++a) P.s[a].swap(P.t[a]);
   P.id.swap(P.idT);
@@ -1331,7 +1340,7 @@ static unsigned sortPhase(CouplingState& C, ParticleStore& P, const double origi
   K.words = C.error.get();
   K.removed = P.removed.get();
   const unsigned seq = C.mailbox.next(0);
-  sortAndChunk(C, P, gatherArgs(C, F, origin), K, ntx, nty, ntiles, C.error.get() + 1, seq, ss);
+  sortAndChunk(C, P, gatherArgs(C, F, origin), K, ntx, nty, ntiles, seq, ss);
   return seq;
 }
 

@@ -61,8 +61,9 @@ enum EntryState : int { kStateActive = 0, kStateCarrier = 1, kStateHeld = 2, kSt
 // the window state's exits / croppedCarriers, with their charges.
 constexpr int kCounterSlots = 20;
 
-// Error words of a coupling step: [0] sort-phase errors, [1] occupied
-// tiles, [2] entries keyed behind the live ones (removed or cropped),
+// Error words of a coupling step: [0] sort-phase errors, [1] unused (the
+// occupied tiles come with the tile scan's total), [2] entries keyed
+// behind the live ones (removed or cropped),
 // [3] cropped carriers, [4] live active (not held) particles at the sort,
 // [kTrialWord + t] the kernels of Picard trial t, [kStopWord] the plain
 // form's Picard state: 0 while trials run, else the iteration count of the
@@ -277,7 +278,6 @@ struct TileBoundsArgs {
   const unsigned* key;  // sorted keys
   int n, ntiles;
   int *begin, *end;
-  unsigned* occupied;
   unsigned long long* bound;  // per tile: largest entry current bound (cleared by the key kernels)
   double perCharge;           // current bound per unit |charge| of a step's chord
 };
@@ -294,10 +294,7 @@ __device__ __forceinline__ double entryCurrentBound(double q, int state, double 
 __device__ __forceinline__ void tileBoundsAt(const TileBoundsArgs& T, int p) {
   const unsigned k = T.key[p];
   if (k >= static_cast<unsigned>(T.ntiles)) return;  // removed or cropped entries sort last
-  if (p == 0 || T.key[p - 1] != k) {
-    T.begin[k] = p;
-    atomicAdd(T.occupied, 1u);
-  }
+  if (p == 0 || T.key[p - 1] != k) T.begin[k] = p;
   if (p == T.n - 1 || T.key[p + 1] != k) T.end[k] = p + 1;
 }
 

@@ -379,28 +379,36 @@ struct PermuteArgs32 {
 };
 
 // State scatter into sorted order, and the tile ranges of the sorted keys.
+// blockIdx.y splits the gathered arrays over four times as many waves
+// (y = 0: cells and offsets, tile ranges and bounds; 1: momenta; 2: q, m,
+// w; 3: ids, states, held speeds).
 __global__ void kPermute32(PermuteArgs32 A, const unsigned* perm, int n, TileBoundsArgs T) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int part = static_cast<int>(blockIdx.y);
   unsigned tile = ~0u;
   double bound = 0.0;
   if (p < n) {
     const unsigned from = perm[p];
+    if (part < 3) {
 #pragma unroll
-    for (int a = 0; a < 3; ++a) A.cellDst[a][p] = A.cellSrc[a][from];
+      for (int a = 0; a < 3; ++a) A.dst[3 * part + a][p] = A.src[3 * part + a][from];
+    } else {
+      A.idDst[p] = A.idSrc[from];
+      A.stateDst[p] = A.stateSrc[from];
+      A.betaDst[p] = A.betaSrc[from];
+    }
+    if (part == 0) {
 #pragma unroll
-    for (int a = 0; a < 9; ++a) A.dst[a][p] = A.src[a][from];
-    A.idDst[p] = A.idSrc[from];
-    const int state = A.stateSrc[from];
-    A.stateDst[p] = state;
-    A.betaDst[p] = A.betaSrc[from];
-    tileBoundsAt(T, p);
-    const unsigned k = T.key[p];
-    if (k < static_cast<unsigned>(T.ntiles)) {
-      tile = k;
-      bound = entryCurrentBound(static_cast<double>(A.src[6][from]), state, T.perCharge);
+      for (int a = 0; a < 3; ++a) A.cellDst[a][p] = A.cellSrc[a][from];
+      tileBoundsAt(T, p);
+      const unsigned k = T.key[p];
+      if (k < static_cast<unsigned>(T.ntiles)) {
+        tile = k;
+        bound = entryCurrentBound(static_cast<double>(A.src[6][from]), A.stateSrc[from], T.perCharge);
+      }
     }
   }
-  raiseTileBound(T.bound, tile, bound);
+  if (part == 0) raiseTileBound(T.bound, tile, bound);
 }
 
 // ---------------------------------------------------------------------------
@@ -1032,7 +1040,7 @@ int bitsFor32(unsigned v) {
 }
 
 void sortSplit(CouplingState& C, SplitStore& P, const GArgs32& G, const SortArgs& K, int ntx, int nty, int ntiles,
-               unsigned* occupied, hipStream_t st) {
+               hipStream_t st) {
   FieldState32& F = *C.fields32;
   const int n = static_cast<int>(P.count);
   ScopedKernel sk(F.profiler, st, "sort");
@@ -1073,9 +1081,9 @@ void sortSplit(CouplingState& C, SplitStore& P, const GArgs32& G, const SortArgs
     perCharge = std::max(perCharge, static_cast<double>(static_cast<float>(h[a] / (F.g.dx * F.g.dy * F.g.dz * F.g.dt))) *
                                         (kC * F.g.dt / h[a]));
   perCharge *= 1.0 + 1e-6;
-  const TileBoundsArgs T{P.keySorted.get(), n,       ntiles, C.tileBegin.get(), C.tileEnd.get(), occupied,
-                         C.tileBound.get(),  perCharge};
-  kPermute32<<<blocks, 256, 0, st>>>(A, P.permSorted.get(), n, T);
+  const TileBoundsArgs T{P.keySorted.get(), n, ntiles, C.tileBegin.get(), C.tileEnd.get(), C.tileBound.get(),
+                         perCharge};
+  kPermute32<<<dim3(blocks, 4), 256, 0, st>>>(A, P.permSorted.get(), n, T);
   launchCheck("permute");
   for (int a = 0; a < 3; ++a) P.cell[a].swap(P.cellT[a]);
   for (int a = 0; a < 9; ++a) P.s[a].swap(P.t[a]);
@@ -1140,7 +1148,7 @@ static void sortPhaseSplit(CouplingState& C, SplitStore& P, const double origin[
   K.dz = F.g.dz;
   K.words = C.error.get();
   K.removed = P.removed.get();
-  sortSplit(C, P, gatherArgs32(C, F, oc), K, ntx, nty, ntiles, C.error.get() + 1, ss);
+  sortSplit(C, P, gatherArgs32(C, F, oc), K, ntx, nty, ntiles, ss);
   hipCheck(hipMemcpyAsync(pin, C.tileScan.get() + ntiles, sizeof(unsigned long long), hipMemcpyDeviceToHost, ss),
            "hipMemcpyAsync");
   hipCheck(hipMemcpyAsync(pin + 120, C.error.get(), 5 * sizeof(unsigned), hipMemcpyDeviceToHost, ss),
@@ -1244,7 +1252,7 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
     std::memcpy(&total, pin, sizeof(total));
     nChunks = static_cast<unsigned>(chunkField(total));
     nActive = static_cast<unsigned>(activeField(total));
-    occupiedTiles = pin[121];
+    occupiedTiles = static_cast<unsigned>(occupiedField(total));
     // finishMovingWindowStep's checks of the entries, applied to this origin
     raiseParticleErrors(pin[120]);
     const int behind = static_cast<int>(pin[122]);
