@@ -100,45 +100,6 @@ __device__ Clip clipOrbit(const GArgs& G, const double begin[3], const double en
   return r;
 }
 
-// Copy a tile box of E_cand + E_old (and, when bs is given, of B) into LDS,
-// zero outside the component ranges (such nodes are never read). All loads
-// of a lane are issued before the first LDS store.
-template <int SHAPE, bool WITH_B>
-__device__ void stageTile(const GArgs& G, const TileBox<SHAPE>& box, double* es, double* bs) {
-  constexpr int N = boxNodes<SHAPE>();
-  constexpr int kPer = (N + 2 * kChunk - 1) / (2 * kChunk);
-  // The component loop is unrolled so that its extents and base pointers
-  // are kernel-argument constants (a runtime component index would turn
-  // each of them into a dependent memory load).
-  double ev[kPer][3], bv[kPer][3];
-#pragma unroll
-  for (int r = 0; r < kPer; ++r) {
-    const int q0 = static_cast<int>(threadIdx.x) + r * 2 * kChunk;
-    const int li = q0 % Box<SHAPE>::NX, q = q0 / Box<SHAPE>::NX, lj = q % Box<SHAPE>::NY, lk = q / Box<SHAPE>::NY;
-    const int i = box.x0 + li, j = box.y0 + lj, k = box.z0 + lk;
-    const bool inBox = q0 < N && i >= 0 && j >= 0 && k >= 0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      ev[r][d] = double(0);
-      bv[r][d] = double(0);
-      if (inBox && i < G.ne[d][0] && j < G.ne[d][1] && k < G.ne[d][2])
-        ev[r][d] = G.esum[d][(static_cast<size_t>(k) * G.ne[d][1] + j) * G.ne[d][0] + i];
-      if (WITH_B && inBox && i < G.nb[d][0] && j < G.nb[d][1] && k < G.nb[d][2])
-        bv[r][d] = G.b[d][(static_cast<size_t>(k) * G.nb[d][1] + j) * G.nb[d][0] + i];
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < kPer; ++r) {
-    const int q0 = static_cast<int>(threadIdx.x) + r * 2 * kChunk;
-    if (q0 < N)
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        es[d * N + q0] = ev[r][d];
-        if (WITH_B) bs[d * N + q0] = bv[r][d];
-      }
-  }
-}
-
 // Impl::electric wide branch (source_coupling.cpp:74-82): orbit average of
 // 1/2 (E_cand + E_old) over the clipped chord with the exact adjoint weights
 // of the deposit, read from the staged box es.
@@ -196,59 +157,10 @@ __device__ __forceinline__ int clampIndex(long long v, int n) {
 
 // Impl::magnetic for binomial3-x: binomialXZMagneticSample at the clipped
 // chord midpoint with clamped ("open") indices (source_coupling.cpp:83-90;
-// source_coupling_math.h:142-167; NativeHalo::openMagneticValue), read from
-// the staged box bs.
-__device__ void gatherBBinomial(const GArgs& G, const double* bs, const TileBox<kShapeBinomial3X>& box,
-                                const double a[3], const double bpt[3], double res[3], unsigned& err) {
-  constexpr int N = boxNodes<kShapeBinomial3X>(), NX = Box<kShapeBinomial3X>::NX, NY = Box<kShapeBinomial3X>::NY;
-  res[0] = res[1] = res[2] = 0.0;
-  const Clip path = clipOrbit(G, a, bpt, err);
-  if (!path.valid) { err |= kErrMagneticOutside; return; }
-  double mid[3];
-  for (int d = 0; d < 3; ++d) { mid[d] = path.b[d]; mid[d] += path.e[d]; mid[d] *= .5; }
-  for (int comp = 0; comp < 3; ++comp) {
-    long long cell[3];
-    double frac[3];
-    for (int d = 0; d < 3; ++d) {
-      const double pos = (mid[d] - G.o[d]) / G.h[d] - (comp == d ? 0. : 0.5);
-      cell[d] = static_cast<long long>(floor(pos));
-      frac[d] = pos - cell[d];
-    }
-    double wx[4], wz[4];
-    nodeWeights(frac[0], wx);
-    nodeWeights(frac[2], wz);
-    const double wy[2] = {1. - frac[1], frac[1]};
-    const int* n = G.nb[comp];
-    if (cell[0] >= 1 && cell[0] + 2 <= n[0] - 1 && cell[1] >= 0 && cell[1] + 1 <= n[1] - 1 && cell[2] >= 1 &&
-        cell[2] + 2 <= n[2] - 1 && box.holds(cell[0] - 1, cell[1], cell[2] - 1, 4, 2, 4)) {
-      // interior: no clamping, fixed offsets from one base
-      const double* bc = bs + comp * N + box.base(cell[0] - 1, cell[1], cell[2] - 1);
-      for (int k = 0; k < 4; ++k)
-        for (int j = 0; j < 2; ++j) {
-          double row = 0.;
-          for (int i = 0; i < 4; ++i) row += wx[i] * bc[(k * NY + j) * NX + i];
-          res[comp] += wz[k] * wy[j] * row;
-        }
-      continue;
-    }
-    for (int k = 0; k < 4; ++k)
-      for (int j = 0; j < 2; ++j) {
-        const int kk = clampIndex(cell[2] - 1 + k, n[2]), jj = clampIndex(cell[1] + j, n[1]);
-        double row = 0.;
-        for (int i = 0; i < 4; ++i) {
-          const int l = box.local(clampIndex(cell[0] - 1 + i, n[0]), jj, kk, err);
-          row += wx[i] * (l < 0 ? 0.0 : bs[comp * N + l]);
-        }
-        res[comp] += wz[k] * wy[j] * row;
-      }
-  }
-}
-
-// The same sample read straight from B in HBM (through the caches: the
-// particles of a chunk share most of their 4x2x4 stencils), so that the
-// gather stages only E in LDS. The arithmetic and its order are those of
-// the staged form.
-__device__ void gatherBBinomialDirect(const GArgs& G, const double a[3], const double bpt[3], double res[3],
+// source_coupling_math.h:142-167; NativeHalo::openMagneticValue), read
+// straight from B in HBM through the caches (the particles of a chunk
+// share most of their 4x2x4 stencils), so that the gather stages only E.
+__device__ void gatherBBinomial(const GArgs& G, const double a[3], const double bpt[3], double res[3],
                                       unsigned& err) {
   res[0] = res[1] = res[2] = 0.0;
   const Clip path = clipOrbit(G, a, bpt, err);
@@ -552,9 +464,8 @@ __global__ void __launch_bounds__(2 * kChunk) __attribute__((amdgpu_waves_per_eu
 kGatherChunks(PArgs P, GArgs G, StepArgs S, const int4* chunks, int ntx, int nty, double dt, double* chunkOut,
               unsigned* errOut) {
   if (G.stop && *G.stop) return;
-  constexpr bool kStageB = false;
   __shared__ double es[3 * boxNodes<SHAPE>()];
-  __shared__ double bs[kStageB ? 3 * boxNodes<SHAPE>() : 4 * kChunk];
+  __shared__ double bs[4 * kChunk];
   const int cb = xcdContiguous(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
   const int4 ch = chunks[cb];
   const TileBox<SHAPE> box = tileBox<SHAPE>(ch.x, ntx, nty);
@@ -578,7 +489,7 @@ kGatherChunks(PArgs P, GArgs G, StepArgs S, const int4* chunks, int ntx, int nty
     qm[1] = P.m[pv];
     qm[2] = P.gridAbs[pv];
   }
-  stageTile<SHAPE, kStageB>(G, box, es, bs);
+  stageTileE<SHAPE>(G, box, es);
   __syncthreads();
   // carriers move without fields and take no part in the residual; a held
   // entry takes part from its release point with the rest of the step
@@ -610,12 +521,8 @@ kGatherChunks(PArgs P, GArgs G, StepArgs S, const int4* chunks, int ntx, int nty
       for (int d = 0; d < 3; ++d) b[d] = uv[d];
     }
     if (bWaves) {
-      if constexpr (SHAPE == kShapeBinomial3X) {
-        if constexpr (kStageB) gatherBBinomial(G, bs, box, a, b, g, err);
-        else gatherBBinomialDirect(G, a, b, g, err);
-      } else {
-        gatherBQuintic(G, a, b, g, err);
-      }
+      if constexpr (SHAPE == kShapeBinomial3X) gatherBBinomial(G, a, b, g, err);
+      else gatherBQuintic(G, a, b, g, err);
     } else {
       gatherE<SHAPE>(G, es, box, a, b, g, err);
     }
@@ -651,32 +558,7 @@ kGatherChunks(PArgs P, GArgs G, StepArgs S, const int4* chunks, int ntx, int nty
       scale = qm[2] + fabs(w2);
       calls = 1.;
     }
-    double* sf = bs;
-    double* sd = bs + kChunk;
-    double* ss = bs + 2 * kChunk;
-    double* sc = bs + 3 * kChunk;
-    if (!bWaves) {
-      sf[lane] = force;
-      sd[lane] = defect;
-      ss[lane] = scale;
-      sc[lane] = calls;
-    }
-    __syncthreads();
-    for (int w = kChunk / 2; w > 0; w >>= 1) {
-      if (static_cast<int>(threadIdx.x) < w) {
-        sf[threadIdx.x] = fmax(sf[threadIdx.x], sf[threadIdx.x + w]);
-        sd[threadIdx.x] += sd[threadIdx.x + w];
-        ss[threadIdx.x] += ss[threadIdx.x + w];
-        sc[threadIdx.x] += sc[threadIdx.x + w];
-      }
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      chunkOut[4 * cb + 0] = sf[0];
-      chunkOut[4 * cb + 1] = sd[0];
-      chunkOut[4 * cb + 2] = ss[0];
-      chunkOut[4 * cb + 3] = sc[0];
-    }
+    chunkStatsTree(bs, !bWaves, lane, force, defect, scale, calls, chunkOut + 4 * cb);
   }
   if (err) atomicOr(errOut, err);
 }
@@ -827,18 +709,7 @@ __global__ void __launch_bounds__(kChunk * depositLanes<SHAPE>()) kDeposit(PArgs
     }
   }
   __syncthreads();
-  // The tile's partial sits at its occupied-tile slot and holds zeros on
-  // entry (the reduction clears what it reads). A tile of one chunk stores
-  // its box; chunks of a larger tile add their nonzero nodes with integer
-  // atomics, whose sum is independent of their order.
-  const bool alone = chunkField(tileScan[tile + 1]) - chunkField(tsc) == 1;
-  unsigned long long* out =
-      reinterpret_cast<unsigned long long*>(partial + static_cast<size_t>(occupiedField(tsc)) * 3 * N);
-  for (int t = threadIdx.x; t < 3 * N; t += blockDim.x) {
-    const unsigned long long val = acc[t];
-    if (alone) out[t] = val;
-    else if (val) atomicAdd(out + t, val);
-  }
+  flushTileBox<SHAPE>(acc, partial, tileScan, tile);
   if (err) atomicOr(errOut, err);
 }
 
@@ -859,21 +730,7 @@ __global__ void __launch_bounds__(256) kReduceCurrent(double* jx, double* jy, do
   const int tile = active[xcdContiguous(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x))];
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
   __shared__ int nbFirst[NB], nbShift[NB];
-  if (threadIdx.x < NB) {
-    const int nb = static_cast<int>(threadIdx.x);
-    const int x = tx + nb % 3 - 1, y = ty + (nb / 3) % 3 - 1, z = tz + nb / 9 - RZ;
-    int first = -1, sh = 0;
-    if (x >= 0 && x < ntx && y >= 0 && y < nty && z >= 0 && z < ntz) {
-      const int s = (z * nty + y) * ntx + x;
-      const unsigned long long sc = tileScan[s];
-      if (chunkField(tileScan[s + 1]) > chunkField(sc)) {
-        first = occupiedField(sc);
-        sh = tileShift[s];
-      }
-    }
-    nbFirst[nb] = first;
-    nbShift[nb] = sh;
-  }
+  coveringTiles<SHAPE>(tileScan, tileShift, tx, ty, tz, ntx, nty, ntz, nbFirst, nbShift);
   __syncthreads();
 #pragma unroll
   for (int d = 0; d < 3; ++d) {

@@ -121,42 +121,6 @@ __device__ bool clipChord(const GArgs32& G, Chord32& c, float& kept, unsigned& e
 // ---------------------------------------------------------------------------
 // Gathers from the staged tile box.
 
-template <int SHAPE, bool WITH_B>
-__device__ void stageTile32(const GArgs32& G, const TileBox<SHAPE>& box, float* es, float* bs) {
-  constexpr int N = boxNodes<SHAPE>();
-  constexpr int kPer = (N + 2 * kChunk - 1) / (2 * kChunk);
-  // The component loop is unrolled so that its extents and base pointers
-  // are kernel-argument constants (a runtime component index would turn
-  // each of them into a dependent memory load).
-  float ev[kPer][3], bv[kPer][3];
-#pragma unroll
-  for (int r = 0; r < kPer; ++r) {
-    const int q0 = static_cast<int>(threadIdx.x) + r * 2 * kChunk;
-    const int li = q0 % Box<SHAPE>::NX, q = q0 / Box<SHAPE>::NX, lj = q % Box<SHAPE>::NY, lk = q / Box<SHAPE>::NY;
-    const int i = box.x0 + li, j = box.y0 + lj, k = box.z0 + lk;
-    const bool inBox = q0 < N && i >= 0 && j >= 0 && k >= 0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      ev[r][d] = float(0);
-      bv[r][d] = float(0);
-      if (inBox && i < G.ne[d][0] && j < G.ne[d][1] && k < G.ne[d][2])
-        ev[r][d] = G.esum[d][(static_cast<size_t>(k) * G.ne[d][1] + j) * G.ne[d][0] + i];
-      if (WITH_B && inBox && i < G.nb[d][0] && j < G.nb[d][1] && k < G.nb[d][2])
-        bv[r][d] = G.b[d][(static_cast<size_t>(k) * G.nb[d][1] + j) * G.nb[d][0] + i];
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < kPer; ++r) {
-    const int q0 = static_cast<int>(threadIdx.x) + r * 2 * kChunk;
-    if (q0 < N)
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        es[d * N + q0] = ev[r][d];
-        if (WITH_B) bs[d * N + q0] = bv[r][d];
-      }
-  }
-}
-
 // Impl::electric (source_coupling.cpp:31-39): orbit average of
 // 1/2 (E_cand + E_old) over the clipped chord with the deposit's weights.
 template <int SHAPE>
@@ -202,57 +166,10 @@ __device__ void gatherE32(const GArgs32& G, const float* es, const TileBox<SHAPE
 __device__ __forceinline__ int clampIndex32(int v, int n) { return v < 0 ? 0 : (v > n - 1 ? n - 1 : v); }
 
 // Impl::magnetic for binomial3-x (source_coupling_math.h:142-167) at the
-// clipped chord midpoint, clamped indices, from the staged box.
-__device__ void gatherBBinomial32(const GArgs32& G, const float* bs, const TileBox<kShapeBinomial3X>& box, Chord32 c,
-                                  float res[3], unsigned& err) {
-  constexpr int N = boxNodes<kShapeBinomial3X>(), NX = Box<kShapeBinomial3X>::NX, NY = Box<kShapeBinomial3X>::NY;
-  res[0] = res[1] = res[2] = 0.f;
-  float kept;
-  if (!clipChord(G, c, kept, err)) { err |= kErrMagneticOutside; return; }
-  float mid[3];
-  for (int d = 0; d < 3; ++d) mid[d] = c.x[d] + 0.5f * c.v[d];
-  for (int comp = 0; comp < 3; ++comp) {
-    int cell[3];
-    float frac[3];
-    for (int d = 0; d < 3; ++d) {
-      const float pos = mid[d] - (comp == d ? 0.f : 0.5f);
-      const float fl = floorf(pos);
-      cell[d] = static_cast<int>(fl) + c.org[d];
-      frac[d] = pos - fl;
-    }
-    float wx[4], wz[4];
-    nodeWeights(frac[0], wx);
-    nodeWeights(frac[2], wz);
-    const float wy[2] = {1.f - frac[1], frac[1]};
-    const int* n = G.nb[comp];
-    if (cell[0] >= 1 && cell[0] + 2 <= n[0] - 1 && cell[1] >= 0 && cell[1] + 1 <= n[1] - 1 && cell[2] >= 1 &&
-        cell[2] + 2 <= n[2] - 1 && box.holds(cell[0] - 1, cell[1], cell[2] - 1, 4, 2, 4)) {
-      const float* bc = bs + comp * N + box.base(cell[0] - 1, cell[1], cell[2] - 1);
-      for (int k = 0; k < 4; ++k)
-        for (int j = 0; j < 2; ++j) {
-          float row = 0.f;
-          for (int i = 0; i < 4; ++i) row += wx[i] * bc[(k * NY + j) * NX + i];
-          res[comp] += wz[k] * wy[j] * row;
-        }
-      continue;
-    }
-    for (int k = 0; k < 4; ++k)
-      for (int j = 0; j < 2; ++j) {
-        const int kk = clampIndex32(cell[2] - 1 + k, n[2]), jj = clampIndex32(cell[1] + j, n[1]);
-        float row = 0.f;
-        for (int i = 0; i < 4; ++i) {
-          const int l = box.local(clampIndex32(cell[0] - 1 + i, n[0]), jj, kk, err);
-          row += wx[i] * (l < 0 ? 0.f : bs[comp * N + l]);
-        }
-        res[comp] += wz[k] * wy[j] * row;
-      }
-  }
-}
-
-// The same sample read straight from B in HBM (the particles of a chunk
-// share most of their 4x2x4 stencils in the caches), so that the gather
-// stages only E; the arithmetic and its order are the staged form's.
-__device__ void gatherBBinomial32Direct(const GArgs32& G, Chord32 c, float res[3], unsigned& err) {
+// clipped chord midpoint, clamped indices, read straight from B in HBM
+// (the particles of a chunk share most of their 4x2x4 stencils in the
+// caches), so that the gather stages only E.
+__device__ void gatherBBinomial32(const GArgs32& G, Chord32 c, float res[3], unsigned& err) {
   res[0] = res[1] = res[2] = 0.f;
   float kept;
   if (!clipChord(G, c, kept, err)) { err |= kErrMagneticOutside; return; }
@@ -443,14 +360,12 @@ template <int MODE, int SHAPE>
 __global__ void __launch_bounds__(2 * kChunk) kGatherChunks32(PArgs32 P, GArgs32 G, StepArgs S, const int4* chunks,
                                                               int ntx, int nty, double dt, double* chunkOut,
                                                               unsigned* errOut) {
-  constexpr bool kStageB = false;
   __shared__ float es[3 * boxNodes<SHAPE>()];
-  __shared__ float bs[kStageB ? 3 * boxNodes<SHAPE>() : 1];
   __shared__ double red[MODE == 1 ? 4 * kChunk : 1];
   const int cb = xcdContiguous(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
   const int4 ch = chunks[cb];
   const TileBox<SHAPE> box = tileBox<SHAPE>(ch.x, ntx, nty);
-  stageTile32<SHAPE, kStageB>(G, box, es, bs);
+  stageTileE<SHAPE>(G, box, es);
   __syncthreads();
   const bool bWaves = threadIdx.x >= kChunk;
   const int lane = static_cast<int>(threadIdx.x) - (bWaves ? kChunk : 0);
@@ -491,12 +406,8 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks32(PArgs32 P, GArgs32
     }
     const Chord32 c = chordOf(G, c0[0], c0[1], c0[2], off0, v);
     if (bWaves) {
-      if constexpr (SHAPE == kShapeBinomial3X) {
-        if constexpr (kStageB) gatherBBinomial32(G, bs, box, c, g, err);
-        else gatherBBinomial32Direct(G, c, g, err);
-      } else {
-        gatherBQuintic32(G, c, g, err);
-      }
+      if constexpr (SHAPE == kShapeBinomial3X) gatherBBinomial32(G, c, g, err);
+      else gatherBQuintic32(G, c, g, err);
     } else {
       gatherE32<SHAPE>(G, es, box, c, g, err);
     }
@@ -531,32 +442,7 @@ __global__ void __launch_bounds__(2 * kChunk) kGatherChunks32(PArgs32 P, GArgs32
       scale = static_cast<double>(P.gridAbs[p]) + fabs(w2);
       calls = 1.;
     }
-    double* sf = red;
-    double* sd = red + kChunk;
-    double* ss = red + 2 * kChunk;
-    double* sc = red + 3 * kChunk;
-    if (!bWaves) {
-      sf[lane] = force;
-      sd[lane] = defect;
-      ss[lane] = scale;
-      sc[lane] = calls;
-    }
-    __syncthreads();
-    for (int w = kChunk / 2; w > 0; w >>= 1) {
-      if (static_cast<int>(threadIdx.x) < w) {
-        sf[threadIdx.x] = fmax(sf[threadIdx.x], sf[threadIdx.x + w]);
-        sd[threadIdx.x] += sd[threadIdx.x + w];
-        ss[threadIdx.x] += ss[threadIdx.x + w];
-        sc[threadIdx.x] += sc[threadIdx.x + w];
-      }
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      chunkOut[4 * cb + 0] = sf[0];
-      chunkOut[4 * cb + 1] = sd[0];
-      chunkOut[4 * cb + 2] = ss[0];
-      chunkOut[4 * cb + 3] = sc[0];
-    }
+    chunkStatsTree(red, !bWaves, lane, force, defect, scale, calls, chunkOut + 4 * cb);
   }
   if (err) atomicOr(errOut, err);
 }
@@ -739,14 +625,7 @@ __global__ void __launch_bounds__(kChunk * depositLanes<SHAPE>()) kDepositSplit(
     }
   }
   __syncthreads();
-  const bool alone = chunkField(tileScan[tile + 1]) - chunkField(tsc) == 1;
-  unsigned long long* out =
-      reinterpret_cast<unsigned long long*>(partial + static_cast<size_t>(occupiedField(tsc)) * 3 * N);
-  for (int t = threadIdx.x; t < 3 * N; t += blockDim.x) {
-    const unsigned long long val = acc[t];
-    if (alone) out[t] = val;
-    else if (val) atomicAdd(out + t, val);
-  }
+  flushTileBox<SHAPE>(acc, partial, tileScan, tile);
   if (err) atomicOr(errOut, err);
 }
 
@@ -763,21 +642,7 @@ __global__ void __launch_bounds__(256) kReduceCurrentSplit(float* jx, float* jy,
   const int tile = active[xcdContiguous(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x))];
   const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
   __shared__ int nbFirst[NB], nbShift[NB];
-  if (threadIdx.x < NB) {
-    const int nb = static_cast<int>(threadIdx.x);
-    const int x = tx + nb % 3 - 1, y = ty + (nb / 3) % 3 - 1, z = tz + nb / 9 - RZ;
-    int first = -1, sh = 0;
-    if (x >= 0 && x < ntx && y >= 0 && y < nty && z >= 0 && z < ntz) {
-      const int s = (z * nty + y) * ntx + x;
-      const unsigned long long sc = tileScan[s];
-      if (chunkField(tileScan[s + 1]) > chunkField(sc)) {
-        first = occupiedField(sc);
-        sh = tileShift[s];
-      }
-    }
-    nbFirst[nb] = first;
-    nbShift[nb] = sh;
-  }
+  coveringTiles<SHAPE>(tileScan, tileShift, tx, ty, tz, ntx, nty, ntz, nbFirst, nbShift);
   __syncthreads();
 #pragma unroll
   for (int d = 0; d < 3; ++d) {

@@ -9,6 +9,8 @@
 // the particle's own start cell.
 #pragma once
 
+#include "tilecounts.hpp"
+
 #include "particles.hpp"
 
 namespace aprl_gpu {
@@ -331,6 +333,119 @@ __device__ __forceinline__ double ownedNodeSum(long long* partial, const int* nb
         *at = 0;
       }
   return sum;
+}
+
+// The owner side's table of the covering tiles of tile (tx, ty, tz):
+// threads 0..9(2RZ+1)-1 fill, per neighbour offset, the first-chunk slot
+// (-1: unoccupied) and the exponent from the tile scan (ownedNodeSum's
+// nbFirst / nbShift). The caller synchronizes before reading it.
+template <int SHAPE>
+__device__ __forceinline__ void coveringTiles(const unsigned long long* tileScan, const int* tileShift, int tx,
+                                              int ty, int tz, int ntx, int nty, int ntz, int* nbFirst,
+                                              int* nbShift) {
+  constexpr int RZ = Box<SHAPE>::RZ, NB = 9 * (2 * RZ + 1);
+  if (threadIdx.x >= NB) return;
+  const int nb = static_cast<int>(threadIdx.x);
+  const int x = tx + nb % 3 - 1, y = ty + (nb / 3) % 3 - 1, z = tz + nb / 9 - RZ;
+  int first = -1, sh = 0;
+  if (x >= 0 && x < ntx && y >= 0 && y < nty && z >= 0 && z < ntz) {
+    const int s = (z * nty + y) * ntx + x;
+    const unsigned long long sc = tileScan[s];
+    if (chunkField(tileScan[s + 1]) > chunkField(sc)) {
+      first = occupiedField(sc);
+      sh = tileShift[s];
+    }
+  }
+  nbFirst[nb] = first;
+  nbShift[nb] = sh;
+}
+
+// The deposit side's write-out of one chunk's LDS box acc (3 components):
+// the tile's partial sits at its occupied-tile slot and holds zeros on
+// entry (the reduction clears what it reads). A tile of one chunk stores
+// its box; chunks of a larger tile add their nonzero nodes with integer
+// atomics, whose sum is independent of their order.
+template <int SHAPE>
+__device__ __forceinline__ void flushTileBox(const unsigned long long* acc, long long* partial,
+                                             const unsigned long long* tileScan, int tile) {
+  constexpr int N = boxNodes<SHAPE>();
+  const unsigned long long tsc = tileScan[tile];
+  const bool alone = chunkField(tileScan[tile + 1]) - chunkField(tsc) == 1;
+  unsigned long long* out =
+      reinterpret_cast<unsigned long long*>(partial + static_cast<size_t>(occupiedField(tsc)) * 3 * N);
+  for (int t = threadIdx.x; t < 3 * N; t += blockDim.x) {
+    const unsigned long long val = acc[t];
+    if (alone) out[t] = val;
+    else if (val) atomicAdd(out + t, val);
+  }
+}
+
+// Copy the tile box of E_cand + E_old into LDS (es, 3 components of
+// boxNodes), zero outside the component ranges (such nodes are never read),
+// with the workgroup's 2 kChunk lanes; every load of a lane is issued before
+// its first LDS store. The component loop is unrolled so that its extents
+// and base pointers are kernel-argument constants (a runtime component
+// index would turn each of them into a dependent memory load). GA is the
+// form's gather arguments (esum, ne), R its real type.
+template <int SHAPE, class GA, class R>
+__device__ void stageTileE(const GA& G, const TileBox<SHAPE>& box, R* es) {
+  constexpr int N = boxNodes<SHAPE>();
+  constexpr int kPer = (N + 2 * kChunk - 1) / (2 * kChunk);
+  R ev[kPer][3];
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {
+    const int q0 = static_cast<int>(threadIdx.x) + r * 2 * kChunk;
+    const int li = q0 % Box<SHAPE>::NX, q = q0 / Box<SHAPE>::NX, lj = q % Box<SHAPE>::NY, lk = q / Box<SHAPE>::NY;
+    const int i = box.x0 + li, j = box.y0 + lj, k = box.z0 + lk;
+    const bool inBox = q0 < N && i >= 0 && j >= 0 && k >= 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      ev[r][d] = R(0);
+      if (inBox && i < G.ne[d][0] && j < G.ne[d][1] && k < G.ne[d][2])
+        ev[r][d] = G.esum[d][(static_cast<size_t>(k) * G.ne[d][1] + j) * G.ne[d][0] + i];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {
+    const int q0 = static_cast<int>(threadIdx.x) + r * 2 * kChunk;
+    if (q0 < N)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) es[d * N + q0] = ev[r][d];
+  }
+}
+
+// A chunk's residual statistics in a fixed tree: the E lanes (threads
+// 0..kChunk-1) enter (force, defect, scale, calls) of their particle; the
+// chunk's max force and defect / scale / call sums go to out[0..3]. sh
+// holds 4 kChunk doubles; every thread of the workgroup calls it.
+__device__ __forceinline__ void chunkStatsTree(double* sh, bool enter, int lane, double force, double defect,
+                                               double scale, double calls, double* out) {
+  double* sf = sh;
+  double* sd = sh + kChunk;
+  double* ss = sh + 2 * kChunk;
+  double* sc = sh + 3 * kChunk;
+  if (enter) {
+    sf[lane] = force;
+    sd[lane] = defect;
+    ss[lane] = scale;
+    sc[lane] = calls;
+  }
+  __syncthreads();
+  for (int w = kChunk / 2; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) {
+      sf[threadIdx.x] = fmax(sf[threadIdx.x], sf[threadIdx.x + w]);
+      sd[threadIdx.x] += sd[threadIdx.x + w];
+      ss[threadIdx.x] += ss[threadIdx.x + w];
+      sc[threadIdx.x] += sc[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = sf[0];
+    out[1] = sd[0];
+    out[2] = ss[0];
+    out[3] = sc[0];
+  }
 }
 
 }  // namespace aprl_gpu
