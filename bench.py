@@ -490,6 +490,8 @@ def production_accuracy():
     s64 = production_snapshots("fp64")
     s32 = production_snapshots("fp32-split")
     out = {}
+    f0 = s64[min(s64)][0]
+    first_norm = float(np.sqrt(sum(float(np.sum(f0[c] ** 2)) for c in range(3))))
     for n in sorted(s64):
         (f64, id64, u64), (f32, id32, u32) = s64[n], s32[n]
         err = {}
@@ -497,6 +499,11 @@ def production_accuracy():
             num = sum(float(np.sum((f32[c] - f64[c]) ** 2)) for c in comps)
             den = sum(float(np.sum(f64[c] ** 2)) for c in comps)
             err[name] = float(np.sqrt(num / den)) if den > 0 else None
+        # the field's size at this step beside its size at the first
+        # snapshot (after the bunch has left, what stays in the window is
+        # small, and its relative error larger)
+        err["field_norm_vs_first"] = float(np.sqrt(sum(float(np.sum(f64[c] ** 2)) for c in range(3))) /
+                                           first_norm) if first_norm else None
         common, i64, i32 = np.intersect1d(id64, id32, return_indices=True)
         if len(common):
             du = u32[i32] - u64[i64]
