@@ -33,14 +33,14 @@ WORD = 8  # FP64
 # profiles/bending_64x64x512_fp64_binomial3x_kernels_traffic.md. Traffic
 # above the algorithmic bytes is re-reading (staged halos, per-chunk boxes).
 MEASURED_TRAFFIC_FP64 = {
-    "magnetic": (2 * 100846 + 69040 + 2 * 10119 + 1792) * 1024,
-    "electric": (2 * 119785 + 70120) * 1024,
-    "electric_preview": (2 * 56596 + 27084) * 1024,
-    "gather": (2 * 13736 + 4690) * 1024,
-    "residual": (2 * 17270 + 4730) * 1024,
-    "deposit": (2 * 4224 + 26158) * 1024,
+    "magnetic": (2 * 64300 + 67189 + 2 * 10116 + 1792) * 1024,
+    "electric": (2 * 81846 + 64815) * 1024,
+    "electric_preview": (2 * 24048 + 23102) * 1024,
+    "gather": (2 * 10400 + 4689) * 1024,
+    "residual": (2 * 13926 + 4731) * 1024,
+    "deposit": (2 * 4202 + 25641) * 1024,
     "push": (2 * 5840 + 5859) * 1024,
-    "reduce_current": (2 * 24387 + 47325) * 1024,
+    "reduce_current": (2 * 24164 + 46499) * 1024,
 }
 
 
