@@ -607,7 +607,7 @@ __global__ void __launch_bounds__(128) kPush(PArgs P, GArgs G, StepArgs S, unsig
 // atomics and the chunk sums are reproducible in any order; each tap is
 // rounded once, to 2^-62 of the tile bound.
 template <int SHAPE>
-__global__ void __launch_bounds__(kChunk * depositLanes<SHAPE>()) kDeposit(PArgs P, GArgs G, StepArgs S, const int4* chunks,
+__global__ void __launch_bounds__(kChunk * depositLanes<SHAPE>()) __attribute__((amdgpu_waves_per_eu(SHAPE == kShapeBinomial3X ? 3 : 1))) kDeposit(PArgs P, GArgs G, StepArgs S, const int4* chunks,
                                                    const int* tileBegin, const int* tileEnd,
                                                    const unsigned long long* tileScan, int ntx, int nty,
                                                    long long* partial, int* tileShift, unsigned* errOut) {
