@@ -301,10 +301,14 @@ __device__ __forceinline__ R* histAt(const CpmlArgsT<R>& P, int o, int i, int j,
 // transformedCorrection, eb_cpml.cpp:131-138. WRITE selects whether the
 // advanced history is stored (preview keeps the old one).
 template <bool WRITE, class R>
-__device__ __forceinline__ double cpmlCorrection(const CpmlPointT<R>& c, double raw, R* hist) {
-  const double h = c.decay * static_cast<double>(*hist) + c.drive * raw;
+__device__ __forceinline__ double cpmlCorrection(const CpmlPointT<R>& c, double raw, R* hist, R old) {
+  const double h = c.decay * static_cast<double>(old) + c.drive * raw;
   if (WRITE) *hist = static_cast<R>(h);
   return c.ikm1 * raw + h;
+}
+template <bool WRITE, class R>
+__device__ __forceinline__ double cpmlCorrection(const CpmlPointT<R>& c, double raw, R* hist) {
+  return cpmlCorrection<WRITE>(c, raw, hist, *hist);
 }
 
 // Per-component magnetic updates at one node: advanceMagneticCowan
@@ -315,50 +319,78 @@ template <class R>
 __device__ __forceinline__ size_t bIndex(const FieldPtrsT<R>& F, int a, int i, int j, int k) {
   return (static_cast<size_t>(k) * F.n[3 + a][1] + j) * F.n[3 + a][0] + i;
 }
+// The loads of component A's update at node (i, j, k): its old value and
+// the CPML histories of its two terms (null pointer: term inactive). oI, oJ,
+// oK are the magnetic CPML ordinals of i, j, k (-1 outside the layers or
+// without CPML). Term order per component (afterMagneticUpdate): Bx (y, z),
+// By (z, x), Bz (x, y).
+template <class R>
+struct BLoads {
+  R old;
+  R h[2];
+  R* hp[2];
+};
+template <int A, class R>
+__device__ __forceinline__ BLoads<R> loadB(const FieldPtrsT<R>& F, const CpmlArgsT<R>& P, int i, int j, int k, int oI,
+                                           int oJ, int oK) {
+  const int nx = F.n[0][0];
+  BLoads<R> L;
+  L.old = F.b[A][bIndex(F, A, i, j, k)];
+  const int o1 = A == 0 ? oJ : (A == 1 ? oK : oI), o2 = A == 0 ? oK : (A == 1 ? oI : oJ);
+  L.hp[0] = nullptr;
+  L.hp[1] = nullptr;
+  L.h[0] = 0;
+  L.h[1] = 0;
+  if (o1 >= 0) {
+    L.hp[0] = A == 0 ? histAt<0, 0, 1>(P, o1, i, j, k, nx + 1)
+                     : (A == 1 ? histAt<0, 1, 2>(P, o1, i, j, k, nx) : histAt<0, 2, 0>(P, o1, i, j, k, nx));
+    L.h[0] = *L.hp[0];
+  }
+  if (o2 >= 0) {
+    L.hp[1] = A == 0 ? histAt<0, 0, 2>(P, o2, i, j, k, nx + 1)
+                     : (A == 1 ? histAt<0, 1, 0>(P, o2, i, j, k, nx) : histAt<0, 2, 1>(P, o2, i, j, k, nx));
+    L.h[1] = *L.hp[1];
+  }
+  return L;
+}
+// The update itself: advanceMagneticCowan / advanceMagneticYee's curl step
+// with the two curl differences d1, d2 (Bx: dEz/dy, dEy/dz; By: dEx/dz,
+// dEz/dx; Bz: dEy/dx, dEx/dy), then the CPML terms, then the stores.
+template <int A, class R>
+__device__ __forceinline__ void finishB(FieldPtrsT<R>& F, const StepCoefsT<R>& c, const CpmlArgsT<R>& P, int i, int j,
+                                        int k, double d1, double d2, const BLoads<R>& L) {
+  constexpr int S1 = A == 0 ? 1 : (A == 1 ? 2 : 0), S2 = A == 0 ? 2 : (A == 1 ? 0 : 1);
+  const double t1 = A == 0 ? c.ty : (A == 1 ? c.tz : c.tx), t2 = A == 0 ? c.tz : (A == 1 ? c.tx : c.ty);
+  const int ax1 = S1, ax2 = S2;
+  double b = static_cast<double>(L.old);
+  b -= t1 * d1 - t2 * d2;
+  if (L.hp[0]) {
+    const int at1 = ax1 == 0 ? i : (ax1 == 1 ? j : k);
+    b += (-1.0 * P.magScale[S1]) * cpmlCorrection<true>(P.prof[0][ax1][at1], d1, L.hp[0], L.h[0]);
+  }
+  if (L.hp[1]) {
+    const int at2 = ax2 == 0 ? i : (ax2 == 1 ? j : k);
+    b += (1.0 * P.magScale[S2]) * cpmlCorrection<true>(P.prof[0][ax2][at2], d2, L.hp[1], L.h[1]);
+  }
+  F.b[A][bIndex(F, A, i, j, k)] = static_cast<R>(b);
+}
 template <class R>
 __device__ __forceinline__ void finishBx(FieldPtrsT<R>& F, const StepCoefsT<R>& c, const CpmlArgsT<R>& P, int i, int j, int k,
-                                         double dEzDy, double dEyDz, const R* old = nullptr) {
-  const int nx = F.n[0][0], ny = F.n[1][1];
-  const size_t idx = (static_cast<size_t>(k) * ny + j) * (nx + 1) + i;
-  double b = old ? static_cast<double>(*old) : static_cast<double>(F.b[0][idx]);
-  b -= c.ty * dEzDy - c.tz * dEyDz;
-  if (P.enabled) {
-    int o = P.ord[0][1][j];
-    if (o >= 0) b += (-1.0 * P.magScale[1]) * cpmlCorrection<true>(P.prof[0][1][j], dEzDy, histAt<0, 0, 1>(P, o, i, j, k, nx + 1));
-    o = P.ord[0][2][k];
-    if (o >= 0) b += (1.0 * P.magScale[2]) * cpmlCorrection<true>(P.prof[0][2][k], dEyDz, histAt<0, 0, 2>(P, o, i, j, k, nx + 1));
-  }
-  F.b[0][idx] = static_cast<R>(b);
+                                         double dEzDy, double dEyDz) {
+  const int oJ = P.enabled ? P.ord[0][1][j] : -1, oK = P.enabled ? P.ord[0][2][k] : -1;
+  finishB<0>(F, c, P, i, j, k, dEzDy, dEyDz, loadB<0>(F, P, i, j, k, -1, oJ, oK));
 }
 template <class R>
 __device__ __forceinline__ void finishBy(FieldPtrsT<R>& F, const StepCoefsT<R>& c, const CpmlArgsT<R>& P, int i, int j, int k,
-                                         double dExDz, double dEzDx, const R* old = nullptr) {
-  const int nx = F.n[0][0], ny = F.n[1][1];
-  const size_t idx = (static_cast<size_t>(k) * (ny + 1) + j) * nx + i;
-  double b = old ? static_cast<double>(*old) : static_cast<double>(F.b[1][idx]);
-  b -= c.tz * dExDz - c.tx * dEzDx;
-  if (P.enabled) {
-    int o = P.ord[0][2][k];
-    if (o >= 0) b += (-1.0 * P.magScale[2]) * cpmlCorrection<true>(P.prof[0][2][k], dExDz, histAt<0, 1, 2>(P, o, i, j, k, nx));
-    o = P.ord[0][0][i];
-    if (o >= 0) b += (1.0 * P.magScale[0]) * cpmlCorrection<true>(P.prof[0][0][i], dEzDx, histAt<0, 1, 0>(P, o, i, j, k, nx));
-  }
-  F.b[1][idx] = static_cast<R>(b);
+                                         double dExDz, double dEzDx) {
+  const int oK = P.enabled ? P.ord[0][2][k] : -1, oI = P.enabled ? P.ord[0][0][i] : -1;
+  finishB<1>(F, c, P, i, j, k, dExDz, dEzDx, loadB<1>(F, P, i, j, k, oI, -1, oK));
 }
 template <class R>
 __device__ __forceinline__ void finishBz(FieldPtrsT<R>& F, const StepCoefsT<R>& c, const CpmlArgsT<R>& P, int i, int j, int k,
-                                         double dEyDx, double dExDy, const R* old = nullptr) {
-  const int nx = F.n[0][0], ny = F.n[1][1];
-  const size_t idx = (static_cast<size_t>(k) * ny + j) * nx + i;
-  double b = old ? static_cast<double>(*old) : static_cast<double>(F.b[2][idx]);
-  b -= c.tx * dEyDx - c.ty * dExDy;
-  if (P.enabled) {
-    int o = P.ord[0][0][i];
-    if (o >= 0) b += (-1.0 * P.magScale[0]) * cpmlCorrection<true>(P.prof[0][0][i], dEyDx, histAt<0, 2, 0>(P, o, i, j, k, nx));
-    o = P.ord[0][1][j];
-    if (o >= 0) b += (1.0 * P.magScale[1]) * cpmlCorrection<true>(P.prof[0][1][j], dExDy, histAt<0, 2, 1>(P, o, i, j, k, nx));
-  }
-  F.b[2][idx] = static_cast<R>(b);
+                                         double dEyDx, double dExDy) {
+  const int oI = P.enabled ? P.ord[0][0][i] : -1, oJ = P.enabled ? P.ord[0][1][j] : -1;
+  finishB<2>(F, c, P, i, j, k, dEyDx, dExDy, loadB<2>(F, P, i, j, k, oI, oJ, -1));
 }
 
 // One thread per node over the union of the three B extents (Yee; and the
@@ -508,7 +540,7 @@ __device__ __forceinline__ const R* planeBaseE(const FieldPtrsT<R>& F, int k, in
 constexpr int kMZ = 8;
 constexpr int kSX = kMX + 1, kSY = kMY + 1;  // sample grid: one halo column / row
 template <class R>
-__global__ void __launch_bounds__(kMX * kMY) kMagneticCowanMarch(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P,
+__global__ void __launch_bounds__(kMX * kMY) __attribute__((amdgpu_waves_per_eu(4))) kMagneticCowanMarch(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P,
                                                                  int runZ) {
   __shared__ R buf[kStageN];  // [component][ring slot][kLY][kLX]
   // smoothed samples of the plane: [S_y Ez, S_y Ex, S_x Ez, S_x Ey][kSY][kSX]
@@ -581,7 +613,24 @@ __global__ void __launch_bounds__(kMX * kMY) kMagneticCowanMarch(FieldPtrsT<R> F
   // the x-y smoothed Ey, Ex of plane k + 1, kept for plane k + 1
   double zEy = 0.0, zEx = 0.0;
   bool zKept = false;
+  // magnetic CPML ordinals: of the lane's i and j once, of each plane's k
+  // one plane ahead
+  const int mI = owner && P.enabled ? P.ord[0][0][i] : -1, mJ = owner && P.enabled ? P.ord[0][1][j] : -1;
+  int mKn = owner && P.enabled ? P.ord[0][2][kb] : -1;
   for (int k = kb; k < ke; ++k) {
+    // the plane's B loads (old values, CPML histories) first, so that the
+    // update waits for them only and their latency hides behind the
+    // smoothing; then plane k + 2's E loads
+    const int mK = mKn;
+    if (owner && P.enabled && k + 1 < ke) mKn = P.ord[0][2][k + 1];
+    BLoads<R> lx{}, ly{}, lz{};
+    if (owner) {
+      if (k < nz) {
+        lx = loadB<0>(F, P, i, j, k, -1, mJ, mK);
+        ly = loadB<1>(F, P, i, j, k, mI, -1, mK);
+      }
+      lz = loadB<2>(F, P, i, j, k, mI, mJ, -1);
+    }
     R next[kRegionR][3];
     int zn[3] = {0, 0, 0};
     const bool more = k + 1 < ke;  // plane k + 2 feeds plane k + 1 of this run
@@ -622,10 +671,10 @@ __global__ void __launch_bounds__(kMX * kMY) kMagneticCowanMarch(FieldPtrsT<R> F
         zEy = smoothL<2>(eyk1, c, li, lj);
         zEx = smoothL<2>(exk1, c, li, lj);
         zKept = true;
-        finishBx(F, c, P, i, j, k, syEz[at + kSX] - syEz[at], zEy - ey0);
-        finishBy(F, c, P, i, j, k, zEx - ex0, sxEz[at + 1] - sxEz[at]);
+        finishB<0>(F, c, P, i, j, k, syEz[at + kSX] - syEz[at], zEy - ey0, lx);
+        finishB<1>(F, c, P, i, j, k, zEx - ex0, sxEz[at + 1] - sxEz[at], ly);
       }
-      finishBz(F, c, P, i, j, k, sxEy[at + 1] - sxEy[at], syEx[at + kSX] - syEx[at]);
+      finishB<2>(F, c, P, i, j, k, sxEy[at + 1] - sxEy[at], syEx[at + kSX] - syEx[at], lz);
     }
     if (more) {
       __syncthreads();  // every lane is done with plane k - 1
