@@ -540,7 +540,7 @@ __device__ __forceinline__ const R* planeBaseE(const FieldPtrsT<R>& F, int k, in
 constexpr int kMZ = 8;
 constexpr int kSX = kMX + 1, kSY = kMY + 1;  // sample grid: one halo column / row
 template <class R>
-__global__ void __launch_bounds__(kMX * kMY) __attribute__((amdgpu_waves_per_eu(4))) kMagneticCowanMarch(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P,
+__global__ void __launch_bounds__(kMX * kMY) kMagneticCowanMarch(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P,
                                                                  int runZ) {
   __shared__ R buf[kStageN];  // [component][ring slot][kLY][kLX]
   // smoothed samples of the plane: [S_y Ez, S_y Ex, S_x Ez, S_x Ey][kSY][kSX]
