@@ -434,16 +434,48 @@ __global__ void __launch_bounds__(256) kMagneticYee(FieldPtrsT<R> F, StepCoefsT<
 }
 
 // Cowan edge nodes the tiles do not cover: Bx at i = nx and By at j = ny.
+constexpr int kEdgeT = 32, kEdgeZ = 4;  // edge nodes per workgroup: 32 along the edge's free axis x 4 planes
 template <class R>
 __global__ void __launch_bounds__(256) kMagneticCowanEdges(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P) {
+  // Workgroups [0, nbX) take tiles of the Bx(i = nx) edge over (j, k), the
+  // rest tiles of the By(j = ny) edge over (i, k). Each smoothed sample a
+  // tile's curls read is computed once into LDS (the difference of two
+  // neighbouring samples is each node's curl term), then each node is
+  // finished from there: per tile, (T+1) x Z samples along the free axis
+  // and T x (Z+1) along z.
+  constexpr int T = kEdgeT, Z = kEdgeZ, NA = (T + 1) * Z, NB = T * (Z + 1);
+  __shared__ double sa[NA], sb[NB];
   const int nx = F.n[0][0], ny = F.n[1][1], nz = F.n[2][2];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < ny * nz) {
-    magneticBx<true>(F, c, P, nx, t % ny, t / ny);
-    return;
+  const int gz = (nz + Z - 1) / Z;
+  const int nbX = ((ny + T - 1) / T) * gz;
+  const bool edgeX = static_cast<int>(blockIdx.x) < nbX;
+  const int w = edgeX ? static_cast<int>(blockIdx.x) : static_cast<int>(blockIdx.x) - nbX;
+  const int n = edgeX ? ny : nx;  // extent of the free axis (nodes 0..n-1)
+  const int gu = (n + T - 1) / T;
+  const int t0 = (w % gu) * T, k0 = (w / gu) * Z;
+  for (int q = static_cast<int>(threadIdx.x); q < NA + NB; q += static_cast<int>(blockDim.x)) {
+    if (q < NA) {
+      // S_y Ez (x edge) / S_x Ez (y edge) at free index t0 + q % (T+1) <= n, plane k0 + q / (T+1) < nz
+      const int u = t0 + q % (T + 1), k = k0 + q / (T + 1);
+      if (u <= n && k < nz) sa[q] = edgeX ? smoothE<2, 1>(F, c, nx, u, k) : smoothE<2, 0>(F, c, u, ny, k);
+    } else {
+      // S_z Ey (x edge) / S_z Ex (y edge) at free index t0 + r % T < n, plane k0 + r / T <= nz
+      const int r = q - NA;
+      const int u = t0 + r % T, k = k0 + r / T;
+      if (u < n && k <= nz) sb[r] = edgeX ? smoothE<1, 2>(F, c, nx, u, k) : smoothE<0, 2>(F, c, u, ny, k);
+    }
   }
-  const int u = t - ny * nz;
-  if (u < nx * nz) magneticBy<true>(F, c, P, u % nx, ny, u / nx);
+  __syncthreads();
+  if (threadIdx.x >= T * Z) return;
+  const int du = static_cast<int>(threadIdx.x) % T, dk = static_cast<int>(threadIdx.x) / T;
+  const int u = t0 + du, k = k0 + dk;
+  if (u >= n || k >= nz) return;
+  const double across = sa[dk * (T + 1) + du + 1] - sa[dk * (T + 1) + du];  // dEz/dy (x edge), dEz/dx (y edge)
+  const double along = sb[(dk + 1) * T + du] - sb[dk * T + du];              // dEy/dz (x edge), dEx/dz (y edge)
+  if (edgeX)
+    finishBx(F, c, P, nx, u, k, across, along);
+  else
+    finishBy(F, c, P, u, ny, k, along, across);
 }
 
 // Tile of the Cowan magnetic update: kMX x kMY nodes of a z plane, staged
@@ -997,7 +1029,8 @@ void launchMagnetic(FieldStateT<R>& s, const FieldPtrsT<R>& f) {
     const int runZ = tune.value();
     dim3 tg(static_cast<unsigned>(tiles) * ceilDiv(planes, runZ), 1, 1);
     kMagneticCowanMarch<R><<<tg, kMX * kMY, 0, s.stream>>>(f, c, P, runZ);
-    kMagneticCowanEdges<R><<<ceilDiv((s.g.nx + s.g.ny) * s.g.nz, 256), 256, 0, s.stream>>>(f, c, P);
+    kMagneticCowanEdges<R><<<(ceilDiv(s.g.ny, kEdgeT) + ceilDiv(s.g.nx, kEdgeT)) * ceilDiv(s.g.nz, kEdgeZ), 256, 0,
+                              s.stream>>>(f, c, P);
   } else {
     dim3 grid(ceilDiv((s.g.nx + 1) * (s.g.ny + 1), 256), static_cast<unsigned>(s.g.nz + 1));
     kMagneticYee<R><<<grid, 256, 0, s.stream>>>(f, c, P);
