@@ -15,8 +15,10 @@
 // on timing. Keys of `bits` bits take ceil(bits / 8) passes.
 //
 // Scans: reduce-then-scan over tiles of kScanTile values: per-tile sums,
-// one workgroup scanning the sums, then every tile rescanned with its
-// offset (a single launch when the input fits one tile). Inside a
+// then every tile rescanned with its offset, which the tile's workgroup
+// sums from the tile sums before it (above 1024 tiles one workgroup scans
+// the sums in a launch of its own; a single launch when the input fits one
+// tile). Inside a
 // workgroup, wavefront inclusive scans by lane shuffles and one LDS step
 // over the four waves.
 #include <algorithm>
@@ -31,6 +33,9 @@ constexpr int kScanThreads = 256;
 constexpr int kScanWaves = kScanThreads / 64;
 constexpr int kScanPer = 8;
 constexpr int kScanTile = kScanThreads * kScanPer;
+// Up to this many tiles, a tile's offset is summed by its own workgroup
+// (at most 4 loads per lane) instead of by a separate one-workgroup scan.
+constexpr std::size_t kSelfOffsetTiles = 4 * kScanThreads;
 
 template <class T>
 __device__ __forceinline__ T waveInclusiveScan(T v, int lane) {
@@ -98,15 +103,24 @@ __global__ void __launch_bounds__(kScanThreads) kScanSums(T* sums, int count) {
   }
 }
 
-// Exclusive scan of one tile plus its offset (offsets == nullptr: 0). The
-// tile passes through LDS so that global loads and stores stay coalesced
-// while every lane scans kScanPer consecutive values.
+// Exclusive scan of one tile plus its offset: offsets[tile] (scanned tile
+// sums), or with sums the sum of the tile sums before this tile, formed by
+// the workgroup itself (exact in any order: integers), or 0. The tile passes
+// through LDS so that global loads and stores stay coalesced while every
+// lane scans kScanPer consecutive values.
 template <class T>
-__global__ void __launch_bounds__(kScanThreads) kScanTiles(const T* in, std::size_t n, const T* offsets, T* out) {
+__global__ void __launch_bounds__(kScanThreads) kScanTiles(const T* in, std::size_t n, const T* offsets,
+                                                           const T* sums, T* out) {
   __shared__ T tile[kScanTile];
   __shared__ T ws[kScanWaves + 1];
   const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kScanTile;
   const int t = static_cast<int>(threadIdx.x);
+  T before = T(0);
+  if (sums) {
+    T part = T(0);
+    for (int q = t; q < static_cast<int>(blockIdx.x); q += kScanThreads) part += sums[q];
+    (void)blockExclusiveScan(part, ws, before);
+  }
 #pragma unroll
   for (int r = 0; r < kScanPer; ++r) {
     const int q = r * kScanThreads + t;
@@ -123,6 +137,7 @@ __global__ void __launch_bounds__(kScanThreads) kScanTiles(const T* in, std::siz
   T total;
   T ex = blockExclusiveScan(s, ws, total);
   if (offsets) ex += offsets[blockIdx.x];
+  ex += before;
 #pragma unroll
   for (int r = 0; r < kScanPer; ++r) {
     tile[t * kScanPer + r] = ex;
@@ -146,15 +161,20 @@ void exclusiveScanImpl(const T* in, T* out, std::size_t n, void* temp, std::size
   if (!n) return;
   const std::size_t tiles = scanTiles<T>(n);
   if (tiles == 1) {
-    kScanTiles<T><<<1, kScanThreads, 0, st>>>(in, n, nullptr, out);
+    kScanTiles<T><<<1, kScanThreads, 0, st>>>(in, n, nullptr, nullptr, out);
     launchCheck("scan");
     return;
   }
   if (!temp || bytes < tiles * sizeof(T)) throw std::logic_error("Scan scratch too small");
   T* sums = static_cast<T*>(temp);
   kScanTileSums<T><<<static_cast<unsigned>(tiles), kScanThreads, 0, st>>>(in, n, sums);
-  kScanSums<T><<<1, kScanThreads, 0, st>>>(sums, static_cast<int>(tiles));
-  kScanTiles<T><<<static_cast<unsigned>(tiles), kScanThreads, 0, st>>>(in, n, sums, out);
+  if (tiles <= kSelfOffsetTiles) {
+    // each tile sums the tile sums before it: no separate scan of the sums
+    kScanTiles<T><<<static_cast<unsigned>(tiles), kScanThreads, 0, st>>>(in, n, nullptr, sums, out);
+  } else {
+    kScanSums<T><<<1, kScanThreads, 0, st>>>(sums, static_cast<int>(tiles));
+    kScanTiles<T><<<static_cast<unsigned>(tiles), kScanThreads, 0, st>>>(in, n, sums, nullptr, out);
+  }
   launchCheck("scan");
 }
 
