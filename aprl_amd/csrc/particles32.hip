@@ -1219,17 +1219,19 @@ void splitStepMiddle(CouplingState& C, SplitStore& P) {
     if (last) reduceChunkStats(st, C.reduceBlocks.get(), static_cast<int>(nChunks), C.reduceOut.get());
     for (int a = 0; a < 6; ++a) std::swap(A.fin[a], A.fout[a]);
   }
-  // commitConservativeElectric with the last trial's current
-  float* const none[3] = {nullptr, nullptr, nullptr};
-  if (F.window) windowCommitElectric(F, F.ptrs());
-  else launchElectric(F, F.ptrs(), 0, none, F.inletPlane[0], F.inletPlane[1]);
-  if (F.window) F.phase = 3;
-  // one readback per step: the trial error words (and the verifying residual)
+  // one readback per step: the trial error words (and the verifying
+  // residual), final once the last trial has run; the step's end waits for
+  // them only, so the commit below runs while the host enqueues the next step
   hipCheck(hipMemcpyAsync(pin + 4, C.error.get() + kTrialWord, trials * sizeof(unsigned), hipMemcpyDeviceToHost, st),
            "hipMemcpyAsync");
   if (verify && nChunks)
     hipCheck(hipMemcpyAsync(pinD, C.reduceOut.get(), 4 * sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
   hipCheck(hipEventRecord(C.stepDone.get(), st), "hipEventRecord");
+  // commitConservativeElectric with the last trial's current
+  float* const none[3] = {nullptr, nullptr, nullptr};
+  if (F.window) windowCommitElectric(F, F.ptrs());
+  else launchElectric(F, F.ptrs(), 0, none, F.inletPlane[0], F.inletPlane[1]);
+  if (F.window) F.phase = 3;
   X.verify = verify;
   X.trials = trials;
   X.nChunks = nChunks;
