@@ -177,10 +177,12 @@ template <class R>
 void launchMagnetic(FieldStateT<R>& s, const FieldPtrsT<R>& f);
 // mode 0: in place with history advance; mode 1: preview into `out` without
 // touching histories (and, when sum != nullptr, sum = preview + old E); the
-// window inlet overwrite applies in both modes.
+// window inlet overwrite applies in both modes. With a gate the launch does
+// nothing unless the device word *gate is nonzero with its top bit clear
+// when it runs (a commit enqueued ahead of the decision that enables it).
 template <class R>
 void launchElectric(FieldStateT<R>& s, const FieldPtrsT<R>& f, int mode, R* const out[3], const R* inletX,
-                    const R* inletY, R* const sum[3] = nullptr);
+                    const R* inletY, R* const sum[3] = nullptr, const unsigned* gate = nullptr);
 // The MODE 1 preview of launchElectric (with sums) restricted to the nodes
 // owned by the first count tiles of `tiles`, count = high word of *total
 // (read on the device).

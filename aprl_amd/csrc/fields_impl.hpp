@@ -949,7 +949,11 @@ __device__ __forceinline__ void electricNodeOnePass(const FieldPtrsT<R>& F, cons
 template <int MODE, class R>
 __global__ void __launch_bounds__(256) kElectric(FieldPtrsT<R> F, StepCoefsT<R> c, CpmlArgsT<R> P, R* ox, R* oy,
                                                   R* oz, R* sx, R* sy, R* sz, const R* inletX,
-                                                  const R* inletY) {
+                                                  const R* inletY, const unsigned* gate) {
+  if (gate) {
+    const unsigned g = *gate;
+    if (g == 0u || (g >> 31)) return;
+  }
   // one thread per node of the (nx+1) x (ny+1) plane k; the blocks of a
   // plane and of the planes next to it run on one XCD, whose L2 then holds
   // the B plane k - 1 the curl reads
@@ -1040,7 +1044,7 @@ void launchMagnetic(FieldStateT<R>& s, const FieldPtrsT<R>& f) {
 
 template <class R>
 void launchElectric(FieldStateT<R>& s, const FieldPtrsT<R>& f, int mode, R* const out[3], const R* inletX,
-                    const R* inletY, R* const sum[3]) {
+                    const R* inletY, R* const sum[3], const unsigned* gate) {
   const StepCoefsT<R> c = s.coefs();
   CpmlArgsT<R> P = s.cpmlArgs();
   if (!s.hasCpml) {  // PerfectElectricConductorBoundary(lowerZ, upperZ)
@@ -1048,18 +1052,19 @@ void launchElectric(FieldStateT<R>& s, const FieldPtrsT<R>& f, int mode, R* cons
     P.upperZ = s.upperZ;
   }
   // workgroup size tuned per grid (one thread per node either way)
+  // (a gated launch may skip its work, so it is never timed)
   TuneScope tune(shapeKey(mode == 0 ? "electric" : "electric_preview_grid", s.g, sizeof(R)), {256, 128, 64},
-                 s.stream);
+                 s.stream, gate == nullptr);
   dim3 block(static_cast<unsigned>(tune.value()), 1, 1);
   dim3 grid(ceilDiv((s.g.nx + 1) * (s.g.ny + 1), block.x) * static_cast<unsigned>(s.g.nz + 1), 1, 1);
   if (mode == 0) {
     ScopedKernel sk(s.profiler, s.stream, "electric");
     kElectric<0, R><<<grid, block, 0, s.stream>>>(f, c, P, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, inletX,
-                                               inletY);
+                                               inletY, gate);
   } else {
     ScopedKernel sk(s.profiler, s.stream, "electric_preview");
     kElectric<1, R><<<grid, block, 0, s.stream>>>(f, c, P, out[0], out[1], out[2], sum ? sum[0] : nullptr,
-                                               sum ? sum[1] : nullptr, sum ? sum[2] : nullptr, inletX, inletY);
+                                               sum ? sum[1] : nullptr, sum ? sum[2] : nullptr, inletX, inletY, gate);
   }
   launchCheck("electric");
 }
@@ -1219,7 +1224,7 @@ void windowShift(FieldStateT<R>& s, const R* const planes[6]) {
   template struct FieldStateT<R>;                                                                            \
   template void launchMagnetic<R>(FieldStateT<R>&, const FieldPtrsT<R>&);                                     \
   template void launchElectric<R>(FieldStateT<R>&, const FieldPtrsT<R>&, int, R* const*, const R*, const R*,  \
-                                  R* const*);                                                                 \
+                                  R* const*, const unsigned*);                                                \
   template void launchClearCurrent<R>(FieldStateT<R>&);                                                       \
   template void launchElectricPreviewTiles<R>(FieldStateT<R>&, const FieldPtrsT<R>&, R* const*, R* const*,     \
                                               const R*, const R*, const int*, const unsigned long long*,       \

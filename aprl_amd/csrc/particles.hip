@@ -15,6 +15,7 @@
 #include "diagnostics.hpp"
 #include "particle_device.hpp"
 #include "shapes.hpp"
+#include "tuner.hpp"
 
 namespace aprl_gpu {
 
@@ -1236,6 +1237,7 @@ struct PlainStepContext {
   unsigned enqueued = 0;       // Picard trials enqueued so far
   unsigned checkedTrials = 0;  // trial words read back so far
   unsigned postSeq = 0;        // mailbox sequence of the last posted trial
+  unsigned commitAfter = 0;    // trials ahead of the gated E commit (0: none enqueued)
   bool quintic = false;
   double origin[3] = {0.0, 0.0, 0.0};
   PArgs A{};
@@ -1443,6 +1445,19 @@ void plainStepBegin(CouplingState& C, ParticleStore& P, const double origin[3], 
   const unsigned batch = C.picardFixed ? C.picardFixed
                                        : std::min<unsigned>(kMaxTrials, std::max(C.picardBatch, activeEntries ? 2u : 1u));
   enqueueTrials(C, X, 0, batch);
+  // commitConservativeElectric enqueued behind the batch, gated on the stop
+  // word: it runs when a trial of the batch has converged, so the device
+  // does not wait for the host to read the decision. (The energy ledger's
+  // commit, several launches, stays after the decision, and so does the
+  // commit while its workgroup size is being tuned: a skipped launch would
+  // be a false sample.)
+  X.commitAfter = 0;
+  if (!(F.window && F.ledger.enabled) && autotuneSettled(shapeKey("electric", F.g, sizeof(double)))) {
+    double* const none[3] = {nullptr, nullptr, nullptr};
+    double* const* noSum = nullptr;
+    launchElectric(F, F.ptrs(), 0, none, F.inletPlane[0], F.inletPlane[1], noSum, C.error.get() + kStopWord);
+    X.commitAfter = X.enqueued;
+  }
 }
 
 void plainStepFinish(CouplingState& C, ParticleStore& P, aprl_step_report* report) {
@@ -1477,10 +1492,14 @@ void plainStepFinish(CouplingState& C, ParticleStore& P, aprl_step_report* repor
   }
   C.picardBatch = iterations;
   // commitConservativeElectric: E^n + curl B - dt/eps0 J, then the committed
-  // CPML-E (history advance), PEC and inlet overwrite.
-  double* const none[3] = {nullptr, nullptr, nullptr};
-  if (F.window) windowCommitElectric(F, F.ptrs());
-  else launchElectric(F, F.ptrs(), 0, none, F.inletPlane[0], F.inletPlane[1]);
+  // CPML-E (history advance), PEC and inlet overwrite; done already by the
+  // gated commit when the converged trial was ahead of it
+  const bool gatedRan = X.commitAfter && iterations <= X.commitAfter && !(stop & kStopError);
+  if (!gatedRan) {
+    double* const none[3] = {nullptr, nullptr, nullptr};
+    if (F.window) windowCommitElectric(F, F.ptrs());
+    else launchElectric(F, F.ptrs(), 0, none, F.inletPlane[0], F.inletPlane[1]);
+  }
   if (F.window) F.phase = 3;
   // Boundary events of the converged trial: carriers, removals, counters.
   if (lastWord & kEvtAny)

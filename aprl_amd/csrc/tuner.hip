@@ -108,7 +108,7 @@ Registry& registry() {
 
 }  // namespace
 
-TuneScope::TuneScope(const std::string& key, const std::vector<int>& candidates, hipStream_t stream) {
+TuneScope::TuneScope(const std::string& key, const std::vector<int>& candidates, hipStream_t stream, bool timed) {
   if (candidates.empty()) throw std::logic_error("Autotuner key without candidates");
   value_ = candidates.front();
   Registry& r = registry();
@@ -131,6 +131,7 @@ TuneScope::TuneScope(const std::string& key, const std::vector<int>& candidates,
     value_ = e.candidates[static_cast<std::size_t>(e.chosen)];
     return;
   }
+  if (!timed) return;  // the default, untimed
   // the candidate with the fewest timed or pending launches
   std::vector<std::size_t> uses(candidates.size());
   for (std::size_t c = 0; c < candidates.size(); ++c) uses[c] = e.times[c].size();
@@ -164,6 +165,17 @@ void setAutotune(bool enabled) {
   Registry& r = registry();
   std::lock_guard<std::mutex> guard(r.lock);
   r.enabled = enabled;
+}
+
+bool autotuneSettled(const std::string& key) {
+  Registry& r = registry();
+  std::lock_guard<std::mutex> guard(r.lock);
+  if (!r.enabled) return true;
+  r.load();
+  const auto it = r.entries.find(key);
+  if (it == r.entries.end()) return false;  // not used yet: its first launches are timed
+  r.harvest(key, it->second);
+  return it->second.chosen >= 0;
 }
 
 std::string autotuneReport() {

@@ -19,8 +19,10 @@ class TuneScope {
  public:
   // Picks the launch shape for `key` among `candidates` (the first is the
   // default, used when tuning is off); while the key is being tuned the
-  // launch enqueued on `stream` within this scope is timed.
-  TuneScope(const std::string& key, const std::vector<int>& candidates, hipStream_t stream);
+  // launch enqueued on `stream` within this scope is timed, unless `timed`
+  // is false (a launch that may skip its work): it then takes the decision
+  // if one is taken, else the default.
+  TuneScope(const std::string& key, const std::vector<int>& candidates, hipStream_t stream, bool timed = true);
   ~TuneScope();
   TuneScope(const TuneScope&) = delete;
   TuneScope& operator=(const TuneScope&) = delete;
@@ -43,6 +45,9 @@ inline std::string shapeKey(const char* kernel, const aprl_grid_geometry& g, std
 // Tuning on or off for the process (on by default); off launches every
 // key's first candidate.
 void setAutotune(bool enabled);
+// Whether `key` launches a settled shape (tuning off, or a decision taken
+// or read from the cache), so that an untimed launch loses no sample.
+bool autotuneSettled(const std::string& key);
 // The decisions taken so far, one "key value" line each.
 std::string autotuneReport();
 
