@@ -1216,6 +1216,10 @@ void prepareStep(CouplingState& C, ParticleStore& P, const double origin[3]) {
   FieldState& F = *C.fields;
   if (C.prepared.valid) throw std::logic_error("A prepared coupling step is already pending");
   hipStream_t ss = C.sortStream.get();
+  // J of the last deposit is cleared through that step's active-tile list,
+  // which this sort rewrites: clear it now, ahead of the sort (every step
+  // starts from J = 0, and the step's own clear then finds nothing to do)
+  if (F.footprint.valid && !F.currentZero) launchClearCurrent(F);
   hipCheck(hipEventRecord(C.prepareStart.get(), F.stream), "hipEventRecord");
   hipCheck(hipStreamWaitEvent(ss, C.prepareStart.get(), 0), "hipStreamWaitEvent");
   const unsigned seq = sortPhase(C, P, origin, ss);

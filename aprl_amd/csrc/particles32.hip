@@ -1026,6 +1026,10 @@ void prepareStepSplit(CouplingState& C, SplitStore& P, const double origin[3]) {
   if (C.prepared.valid) throw std::logic_error("A prepared coupling step is already pending");
   if (C.split && C.split->pending) throw std::logic_error("The previous FP32-split coupling step was not finished");
   hipStream_t ss = C.sortStream.get();
+  // J of the last deposit is cleared through that step's active-tile list,
+  // which this sort rewrites: clear it now, ahead of the sort (every step
+  // starts from J = 0, and the step's own clear then finds nothing to do)
+  if (F.footprint.valid && !F.currentZero) launchClearCurrent(F);
   hipCheck(hipEventRecord(C.prepareStart.get(), F.stream), "hipEventRecord");
   hipCheck(hipStreamWaitEvent(ss, C.prepareStart.get(), 0), "hipStreamWaitEvent");
   sortPhaseSplit(C, P, origin, ss);
