@@ -33,14 +33,14 @@ WORD = 8  # FP64
 # profiles/bending_64x64x512_fp64_binomial3x_kernels_traffic.md. Traffic
 # above the algorithmic bytes is re-reading (staged halos, per-chunk boxes).
 MEASURED_TRAFFIC_FP64 = {
-    "magnetic": (2 * 63378 + 67049 + 2 * 10528 + 1792) * 1024,
-    "electric": (2 * 81839 + 64872) * 1024,
-    "electric_preview": (2 * 22207 + 23289) * 1024,
+    "magnetic": (2 * 63341 + 66930 + 2 * 10527 + 1792) * 1024,
+    "electric": (2 * 81838 + 64850) * 1024,
+    "electric_preview": (2 * 22206 + 23402) * 1024,
     "gather": (2 * 10400 + 4689) * 1024,
-    "residual": (2 * 13926 + 4731) * 1024,
-    "deposit": (2 * 4316 + 26442) * 1024,
+    "residual": (2 * 13926 + 4733) * 1024,
+    "deposit": (2 * 4317 + 26442) * 1024,
     "push": (2 * 5840 + 5859) * 1024,
-    "reduce_current": (2 * 24169 + 46488) * 1024,
+    "reduce_current": (2 * 24171 + 46501) * 1024,
 }
 
 
